@@ -1,0 +1,354 @@
+/*
+ * sw_gpu.h -- C ABI of the MI355X surface builder, span rasteriser and HDR tonemap.
+ *
+ * This is the drop-in boundary for the per-frame pixel path of Square-Wheel's software renderer.
+ * The caller (the renderer front-end: visibility, per-polygon setup, draw ordering) stays on the
+ * CPU and hands over two command buffers per view:
+ *
+ *   - an array of surface descriptors  (what square_wheel/partial_renderer.rs:1682-1820 passes to
+ *     build_surface_{simple,directional}_lightmap and mix_surface_with_texture), and
+ *   - an ordered draw list              (what partial_renderer.rs:2311-2792 / 4011-4142 passes to
+ *     Rasterizer::fill_polygon and Rasterizer::fill_triangle, rasterizer.rs:35, 731).
+ *
+ * All structs are plain C PODs (#[repr(C)]-compatible), all functions return 0 on success and a
+ * negative sw_status on failure (the message is kept in sw_last_error), nothing throws across the
+ * ABI. One context per host thread; work is queued on the context's HIP stream and completes at
+ * sw_sync / sw_frame_end / sw_tonemap.
+ *
+ * Colour layout is the reference's (common/color.rs:2-168): Color32 = A<<24|R<<16|G<<8|B,
+ * Color64 = A<<48|R<<32|G<<16|B. Framebuffer pitch is given in bytes here (the reference's
+ * SurfaceInfo::pitch is in pixels, common/system_window.rs:11-16).
+ */
+#ifndef SW_GPU_H
+#define SW_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------------------------
+ * Status codes
+ * ------------------------------------------------------------------------------------------- */
+typedef enum sw_status
+{
+	SW_OK = 0,
+	SW_ERR_INVALID_ARGUMENT = -1,
+	SW_ERR_OUT_OF_MEMORY = -2,
+	SW_ERR_DEVICE = -3,
+	SW_ERR_STATE = -4,
+	SW_ERR_BUDGET = -5,
+	SW_ERR_NOT_LOADED = -6,
+} sw_status;
+
+/* BlendingMode, common/material.rs:102-116. */
+enum
+{
+	SW_BLEND_NONE = 0,
+	SW_BLEND_AVERAGE = 1,
+	SW_BLEND_ADDITIVE = 2,
+	SW_BLEND_ALPHA_TEST = 3,
+	SW_BLEND_ALPHA_BLEND = 4,
+};
+
+/* TetureCoordinatesInterpolationMode, rasterizer.rs:5-14. */
+enum
+{
+	SW_TC_PERSPECTIVE = 0,
+	SW_TC_LINE_Z = 1,
+	SW_TC_AFFINE = 2,
+};
+
+/* Draw kinds. */
+enum
+{
+	SW_DRAW_POLYGON = 0,  /* Rasterizer::fill_polygon, texture = built surface or skybox side */
+	SW_DRAW_TRIANGLE = 1, /* Rasterizer::fill_triangle, texture = TextureLite (Color32) */
+};
+
+/* Polygon texture sources. */
+enum
+{
+	SW_TEXSRC_SURFACE = 0, /* offset into the frame's surface arena (ColorT texels) */
+	SW_TEXSRC_SKYBOX = 1,  /* offset into the map's skybox texel buffer (ColorT texels) */
+};
+
+/* Texture flags (textures.rs:11-18). */
+enum
+{
+	SW_TEX_HAS_NORMAL_MAP = 1u << 0,
+	SW_TEX_HAS_NON_ONE_ROUGHNESS = 1u << 1,
+	SW_TEX_IS_METAL = 1u << 2,
+	SW_TEX_LITE_LEVEL = 1u << 8, /* the level indexes lite_texels (Color32), not texels */
+};
+
+/* Surface flags. */
+enum
+{
+	SW_SURF_DIFFUSE_LAYER = 1u << 0, /* material.light && material.diffuse (partial_renderer.rs:1701) */
+	SW_SURF_DIRECTIONAL = 1u << 1,   /* use DirectionalLightmapElement data (partial_renderer.rs:1677) */
+	SW_SURF_EMISSIVE = 1u << 2,      /* apply mix_surface_with_texture (partial_renderer.rs:1801-1819) */
+};
+
+#define SW_MAX_POLYGON_LIGHTS 6       /* partial_renderer.rs:1666 */
+#define SW_MAX_POLYGON_VERTICES 24    /* renderer_utils.rs:4 */
+#define SW_MAX_LIGHTMAP_SIZE 17       /* common/lightmap.rs:9; line buffer holds +2 */
+#define SW_NO_TEXTURE 0xFFFFFFFFu
+
+/* ---------------------------------------------------------------------------------------------
+ * Map-resident data (uploaded once, kept in HBM)
+ * ------------------------------------------------------------------------------------------- */
+
+/* TextureElement, textures.rs:20-26: diffuse Color32 + PackedNormalRoughness (i8 x4, scale 127). */
+typedef struct sw_texel
+{
+	uint32_t diffuse;
+	int32_t packed_normal_roughness;
+} sw_texel;
+
+/* One mip level of a Texture (textures.rs:11-18) or of a TextureLite (textures.rs:89-91). */
+typedef struct sw_texture_level
+{
+	uint64_t offset; /* element offset into the texel buffer */
+	uint32_t width;
+	uint32_t height;
+	uint32_t flags; /* SW_TEX_* (unused for lite textures) */
+	uint32_t reserved;
+} sw_texture_level;
+
+/* DirectionalLightmapElement, common/bsp_map_compact.rs:144-156 (40 bytes). */
+typedef struct sw_directional_lightmap_element
+{
+	float ambient_light[3];
+	float light_direction_vector_scaled[3];
+	float directional_light_deviation;
+	float directional_light_color[3];
+} sw_directional_lightmap_element;
+
+typedef struct sw_map_desc
+{
+	const sw_texel* texels; /* all Texture mip levels */
+	uint64_t num_texels;
+	const uint32_t* lite_texels; /* all TextureLite mip levels (Color32) */
+	uint64_t num_lite_texels;
+	const float* lightmaps_simple; /* LightmapElement = [f32;3], bsp_map_compact.rs:142 */
+	uint64_t num_lightmap_elements;
+	const sw_directional_lightmap_element* lightmaps_directional;
+	uint64_t num_directional_lightmap_elements;
+	const sw_texture_level* levels; /* texture table (textures and lite textures, all mips) */
+	uint32_t num_levels;
+	uint32_t reserved;
+	const uint32_t* skybox_texels32; /* skybox side textures, Color32 (may be NULL) */
+	uint64_t num_skybox_texels32;
+	const uint64_t* skybox_texels64; /* skybox side textures, Color64 (may be NULL) */
+	uint64_t num_skybox_texels64;
+} sw_map_desc;
+
+/* ---------------------------------------------------------------------------------------------
+ * Per-frame command buffers
+ * ------------------------------------------------------------------------------------------- */
+
+/* DynamicLightWithShadow with ShadowMap::None (light.rs:4-11, renderer_structs.rs:228-264). */
+typedef struct sw_light
+{
+	float position[3];
+	float radius;
+	float inv_square_radius;
+	float color[3];
+} sw_light;
+
+/*
+ * One visible polygon's surface, as prepared by PartialRenderer::build_polygons_surfaces
+ * (partial_renderer.rs:1682-1820): basis already scaled for the mip and corrected by the
+ * texture-scroll shift (:1736-1740), tc_start = surface_tc_min + shift (:1690-1693).
+ */
+typedef struct sw_surface_desc
+{
+	float basis_u[3];
+	float basis_v[3];
+	float basis_start[3];
+	float basis_normal[3];
+	uint32_t size[2];
+	int32_t tc_start[2];
+	uint32_t texture_level; /* index into sw_map_desc::levels (the mip level to sample) */
+	uint32_t lightmap_offset; /* element offset; 0 = no lightmap (partial_renderer.rs:1757-1765) */
+	uint32_t lightmap_size[2];
+	uint32_t lightmap_scale_log2; /* LIGHTMAP_SCALE_LOG2 - mip, in [0, 4] */
+	uint32_t lightmap_tc_shift[2];
+	uint32_t first_light; /* index into the frame's light-index list */
+	uint32_t num_lights; /* <= SW_MAX_POLYGON_LIGHTS */
+	uint32_t emissive_level; /* lite texture level, SW_NO_TEXTURE if none */
+	float emissive_light[3];
+	uint32_t emissive_blend; /* SW_BLEND_* */
+	uint32_t flags; /* SW_SURF_* */
+	uint64_t pixels_offset; /* element offset into the surface arena (ColorT elements) */
+} sw_surface_desc;
+
+/*
+ * One draw of the ordered draw list. Polygon draws are exactly the arguments of
+ * Rasterizer::fill_polygon after draw_polygon's setup (partial_renderer.rs:4011-4142):
+ * clipped convex clockwise vertices in Fixed16 (vertex_data: x0,y0,x1,y1,...), the (possibly
+ * z-scaled) depth and texture-coordinate equations, the interpolation mode and blending mode.
+ * Triangle draws are the arguments of Rasterizer::fill_triangle: three TrianglePointProjected
+ * (rasterizer.rs:1491-1498), 7 ints each {x, y, tc0, tc1, light0, light1, light2}.
+ */
+typedef struct sw_draw
+{
+	uint32_t kind; /* SW_DRAW_* */
+	uint32_t blend; /* SW_BLEND_* */
+	uint32_t mode; /* SW_TC_* (polygons) */
+	uint32_t texture_source; /* SW_TEXSRC_* (polygons) */
+	uint32_t first_vertex_int; /* offset into vertex_data (ints) */
+	uint32_t num_vertices; /* polygon: 3..24; triangle: 3 */
+	float depth_equation[3]; /* d_inv_z_dx, d_inv_z_dy, k (equations.rs:5-10) */
+	float tc_equation[6]; /* d_tc_dx[2], d_tc_dy[2], k[2] (equations.rs:14-19) */
+	uint64_t texture_offset; /* element offset: surface arena / skybox buffer / lite texels */
+	int32_t texture_size[2];
+	int32_t clip_rect[4]; /* min_x, min_y, max_x, max_y (rasterizer.rs:1476-1482) */
+} sw_draw;
+
+/* One view (frame) of a batch. */
+typedef struct sw_view
+{
+	uint32_t width;
+	uint32_t height;
+	float cam_pos[3];
+	uint32_t clear_background; /* fill with background_color before drawing */
+	uint64_t background_color; /* raw ColorT value */
+	uint32_t first_surface, num_surfaces;
+	uint32_t first_draw, num_draws;
+	uint64_t fb_offset; /* element offset of this view's framebuffer in the batch output (pitch = width) */
+	uint64_t surface_arena_offset; /* added to every surface pixels_offset / polygon texture_offset */
+} sw_view;
+
+/*
+ * A batch of views in one call. Pointers are host pointers (inputs_on_device = 0) or device
+ * pointers (inputs_on_device = 1, e.g. command buffers kept resident in HBM by the caller).
+ */
+typedef struct sw_batch
+{
+	const sw_view* views;
+	uint32_t num_views;
+	uint32_t color_bits; /* 32 or 64 */
+	const sw_surface_desc* surfaces;
+	uint32_t num_surfaces;
+	uint32_t num_lights;
+	const sw_light* lights;
+	const uint32_t* light_indices;
+	uint32_t num_light_indices;
+	uint32_t num_draws;
+	const sw_draw* draws;
+	const int32_t* vertex_data;
+	uint64_t num_vertex_ints;
+	uint64_t surface_arena_texels; /* total arena size in ColorT elements */
+	uint64_t framebuffer_texels; /* total output size in ColorT elements */
+	uint32_t inputs_on_device;
+	uint32_t reserved;
+} sw_batch;
+
+/* Work counters of the last batch (used for the algorithmic-bytes roofline). */
+typedef struct sw_stats
+{
+	uint64_t surface_texels; /* P_surf */
+	uint64_t lightmap_elements; /* L: lightmap elements read (two rows per surface row) */
+	uint64_t emissive_texels; /* P_emis */
+	uint64_t polygon_pixels; /* P_poly */
+	uint64_t triangle_pixels; /* P_tri */
+	uint64_t blend_pixels; /* P_blend: writes whose blending mode reads the destination */
+	uint64_t spans; /* span records produced */
+	uint64_t parts; /* polygon/triangle parts produced by the edge walk */
+} sw_stats;
+
+typedef struct sw_ctx sw_ctx;
+typedef struct sw_map sw_map;
+
+/* ---------------------------------------------------------------------------------------------
+ * Context and map.  (Replaces Renderer::new, renderer.rs:24-59, and the device-side half of the
+ * resources manager: textures/mips and lightmaps become HBM-resident.)
+ * ------------------------------------------------------------------------------------------- */
+int sw_ctx_create(int device, sw_ctx** out_ctx);
+void sw_ctx_destroy(sw_ctx* ctx);
+const char* sw_last_error(const sw_ctx* ctx);
+int sw_sync(sw_ctx* ctx);
+
+int sw_map_upload(sw_ctx* ctx, const sw_map_desc* desc, sw_map** out_map);
+void sw_map_release(sw_ctx* ctx, sw_map* map);
+
+/* ---------------------------------------------------------------------------------------------
+ * Plain drop-in path (one view per frame).
+ *   sw_frame_begin      <- Renderer::prepare_frame<ColorT>   (renderer.rs:61-104)
+ *   sw_build_surfaces   <- PartialRenderer::build_polygons_surfaces (partial_renderer.rs:1640-1832)
+ *   sw_raster_draws     <- PartialRenderer::perform_rasterization   (partial_renderer.rs:1192-1325)
+ *   sw_frame_end        <- Renderer::draw_frame's write into &mut [ColorT] (renderer.rs:106-124)
+ *   sw_tonemap          <- Postprocessor::perform_postprocessing, bloom off (postprocessor.rs:68-204)
+ * ------------------------------------------------------------------------------------------- */
+int sw_frame_begin(
+	sw_ctx* ctx, sw_map* map, uint32_t color_bits, uint32_t width, uint32_t height, const float cam_pos[3],
+	uint32_t clear_background, uint64_t background_color);
+int sw_build_surfaces(
+	sw_ctx* ctx, const sw_surface_desc* surfaces, uint32_t num_surfaces, const sw_light* lights, uint32_t num_lights,
+	const uint32_t* light_indices, uint32_t num_light_indices);
+int sw_raster_draws(
+	sw_ctx* ctx, const sw_draw* draws, uint32_t num_draws, const int32_t* vertex_data, uint64_t num_vertex_ints);
+int sw_frame_end(sw_ctx* ctx, void* dst, size_t dst_pitch_bytes, uint32_t dst_on_device);
+int sw_tonemap(
+	sw_ctx* ctx, float exposure, const float color_modulate[3], void* dst32, size_t dst_pitch_bytes,
+	uint32_t dst_on_device, float out_colors_sum[3]);
+
+/* ---------------------------------------------------------------------------------------------
+ * Batched path: many views of uneven size in one set of launches (surfaces of all views in one
+ * flattened-texel launch, all draws of all views in one setup launch, one raster launch).
+ * out_framebuffer is a device pointer of batch->framebuffer_texels ColorT elements.
+ * No counterpart in the reference (its renderer draws one view per call); it exists for callers
+ * that render many independent views (multi-view serving, cube maps, portals).
+ * ------------------------------------------------------------------------------------------- */
+int sw_render_batch(sw_ctx* ctx, sw_map* map, const sw_batch* batch, void* out_framebuffer_device);
+
+/* Captures the last sw_render_batch call's launches (same shapes, same device buffers) into a
+ * HIP graph and replays it; inputs must be device-resident and unchanged in size. */
+int sw_render_batch_graph(sw_ctx* ctx, sw_map* map, const sw_batch* batch, void* out_framebuffer_device);
+
+/* Tonemap a device Color64 buffer (pitch = width) into a device Color32 buffer. */
+int sw_tonemap_device(
+	sw_ctx* ctx, const void* hdr_device, uint32_t width, uint32_t height, float exposure,
+	const float color_modulate[3], void* dst32_device, size_t dst_pitch_bytes, float out_colors_sum[3]);
+
+int sw_get_stats(sw_ctx* ctx, sw_stats* out_stats);
+
+/* Copies the first num_texels ColorT elements of the surface arena built by the last
+ * sw_build_surfaces / sw_render_batch into host memory (synchronous). The arena is the renderer's
+ * surfaces_pixels buffer (partial_renderer.rs:26-30); callers use it to inspect or cache built
+ * surfaces. */
+int sw_read_surfaces(sw_ctx* ctx, void* dst, uint64_t num_texels);
+
+/* Device pointer of the plain path's current framebuffer (valid after sw_raster_draws). */
+void* sw_frame_device_framebuffer(sw_ctx* ctx);
+
+/* Device memory helpers for callers that keep command buffers resident in HBM (all on the
+ * context's device; copies are synchronous). */
+int sw_device_alloc(sw_ctx* ctx, uint64_t bytes, void** out_device_ptr);
+void sw_device_free(sw_ctx* ctx, void* device_ptr);
+int sw_copy_to_device(sw_ctx* ctx, void* device_dst, const void* host_src, uint64_t bytes);
+int sw_copy_from_device(sw_ctx* ctx, void* host_dst, const void* device_src, uint64_t bytes);
+
+/* Kernel timing with HIP events recorded on the context's stream around every launch.
+ * Stages: 0 prep_surfaces, 1 build_surfaces, 2 setup_draws, 3 bin_bands, 4 raster_tiles,
+ * 5 tonemap. sw_profile_read returns the summed milliseconds and launch counts since the last
+ * reset (and resets when reset != 0). Not recorded inside captured graphs. */
+#define SW_PROFILE_STAGES 6
+int sw_profile_enable(sw_ctx* ctx, uint32_t enable);
+int sw_profile_read(sw_ctx* ctx, double out_ms[SW_PROFILE_STAGES], uint64_t out_counts[SW_PROFILE_STAGES], uint32_t reset);
+
+/* Raw HIP stream handle (hipStream_t) the context queues its work on. */
+void* sw_ctx_stream(sw_ctx* ctx);
+
+/* Bytes of HBM currently held by the context (arenas, command buffers, maps). */
+uint64_t sw_ctx_device_bytes(const sw_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SW_GPU_H */

@@ -1,0 +1,1721 @@
+// ORACLE -- CPU restatement of Square-Wheel's per-frame pixel path. TEST INFRASTRUCTURE ONLY.
+//
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this library, and
+// only as the checker (or as the timed CPU baseline). The product path never calls it.
+//
+// It restates, loop for loop, the reference's Rust code:
+//   - surface builder      square_wheel/surfaces.rs:433-852 (build_surface_impl_6_static_params),
+//                          :854-971 (mix_surface_with_texture), :973-1086 (lightmap element ops),
+//                          :1088-1154 (specular / Fresnel helpers)
+//   - span rasteriser      square_wheel/rasterizer.rs:137-271 (edge walk), :273-454 (perspective),
+//                          :456-581 (line z corrected), :583-729 (affine), :765-1292 (triangles),
+//                          :1511-1536 (write_into_framebuffer)
+//   - colour model         square_wheel/fast_math.rs:6-368 (the SSE4.1 branch, with x86 intrinsics),
+//                          square_wheel/abstract_color.rs:4-131, common/color.rs:2-168
+//   - fixed point          common/fixed_math.rs:1-71
+//   - tonemap              square_wheel/postprocessor.rs:226-256, :996-1024
+//   - textures / mips      square_wheel/textures.rs:31-62, :93-161, :288-415
+//   - screen split         square_wheel/rect_splitting.rs:23-78 (CPU baseline only)
+//
+// Two numeric modes: ORACLE_MODE_SSE uses rsqrtss / rcpss exactly as fast_math.rs:13-22 (host
+// dependent, +-1 LSB vs. other hosts); ORACLE_MODE_EXACT uses 1/sqrt(x) and 1/x, the reference's
+// own non-SSE fallback (fast_math.rs:376-384), and is host independent.
+//
+// Built with -msse4.1 -mfma -ffp-contract=off -fwrapv: FMA only where the reference calls
+// f32_mul_add / _mm_fmadd_ps, every other float op unfused (Rust never contracts), integer
+// overflow wraps as in a Rust release build.
+
+#include "../include/sw_gpu.h"
+
+#include <immintrin.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace
+{
+
+enum
+{
+	ORACLE_MODE_SSE = 0,
+	ORACLE_MODE_EXACT = 1,
+};
+
+// ---------------------------------------------------------------------------------------------
+// fast_math.rs
+// ---------------------------------------------------------------------------------------------
+
+struct Numerics
+{
+	int mode;
+
+	// fast_math.rs:13-16 / :376-379
+	float inv_sqrt_fast(float x) const
+	{
+		if(mode == ORACLE_MODE_SSE)
+			return _mm_cvtss_f32(_mm_rsqrt_ss(_mm_set1_ps(x)));
+		return 1.0f / std::sqrt(x);
+	}
+
+	// fast_math.rs:19-22 / :381-384
+	float inv_fast(float x) const
+	{
+		if(mode == ORACLE_MODE_SSE)
+			return _mm_cvtss_f32(_mm_rcp_ss(_mm_set1_ps(x)));
+		return 1.0f / x;
+	}
+};
+
+// fast_math.rs:730-734 (release build has +fma)
+inline float f32_mul_add(float x, float y, float z) { return std::fma(x, y, z); }
+
+const float MIN_POSITIVE_VALUE = 1.0f / float(1 << 30); // fast_math.rs:744
+
+// Rust f32::max / f32::min: IEEE maxNum / minNum (the non-NaN operand wins).
+inline float fmax_r(float a, float b) { return std::fmax(a, b); }
+inline float fmin_r(float a, float b) { return std::fmin(a, b); }
+
+struct Vec3
+{
+	float x, y, z;
+};
+
+inline Vec3 v3(float x, float y, float z) { return Vec3{x, y, z}; }
+inline Vec3 operator+(Vec3 a, Vec3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline Vec3 operator-(Vec3 a, Vec3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline Vec3 operator*(Vec3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+
+// fast_math.rs:780-783
+inline float vec3_dot(Vec3 a, Vec3 b) { return f32_mul_add(a.x, b.x, f32_mul_add(a.y, b.y, a.z * b.z)); }
+// fast_math.rs:786-793
+inline Vec3 vec3_scalar_mul_add(Vec3 a, float s, Vec3 b)
+{
+	return v3(f32_mul_add(a.x, s, b.x), f32_mul_add(a.y, s, b.y), f32_mul_add(a.z, s, b.z));
+}
+inline float vec3_len2(Vec3 v) { return vec3_dot(v, v); }
+
+// ColorVec (fast_math.rs:98-248): lanes [B, G, R, A] of an __m128.
+inline __m128 cv_from_color32(uint32_t c) { return _mm_cvtepi32_ps(_mm_cvtepu8_epi32(_mm_cvtsi32_si128(int(c)))); }
+inline __m128 cv_from_color64(uint64_t c)
+{
+	return _mm_cvtepi32_ps(_mm_cvtepu16_epi32(_mm_cvtsi64_si128((long long)c)));
+}
+// fast_math.rs:131-144: min(x, 255) -> cvtps2dq (round half even) -> low byte per lane.
+inline uint32_t cv_into_color32(__m128 v)
+{
+	const __m128 clamped = _mm_min_ps(v, _mm_set1_ps(255.0f));
+	const __m128i i = _mm_cvtps_epi32(clamped);
+	const __m128i mask = _mm_set_epi8(0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 12, 8, 4, 0);
+	return uint32_t(_mm_cvtsi128_si32(_mm_shuffle_epi8(i, mask)));
+}
+// fast_math.rs:146-160
+inline uint64_t cv_into_color64(__m128 v)
+{
+	const __m128 clamped = _mm_min_ps(v, _mm_set1_ps(65535.0f));
+	const __m128i i = _mm_cvtps_epi32(clamped);
+	const __m128i mask = _mm_set_epi8(0, 0, 0, 0, 0, 0, 0, 0, 13, 12, 9, 8, 5, 4, 1, 0);
+	return uint64_t(_mm_cvtsi128_si64(_mm_shuffle_epi8(i, mask)));
+}
+inline __m128 cv_from_f32x3(const float c[3]) { return _mm_set_ps(0.0f, c[0], c[1], c[2]); }
+inline __m128 cv_from_f32x3_with_one(const float c[3]) { return _mm_set_ps(1.0f, c[0], c[1], c[2]); }
+inline __m128 cv_scalar_mul(__m128 a, float s) { return _mm_mul_ps(a, _mm_set1_ps(s)); }
+inline __m128 cv_mul_scalar_add(__m128 a, float s, __m128 b) { return _mm_fmadd_ps(a, _mm_set1_ps(s), b); }
+inline __m128 cv_insert3(__m128 a, float s) { return _mm_blend_ps(a, _mm_set1_ps(s), 1 << 3); }
+
+// ColorVecI (fast_math.rs:258-367)
+inline __m128i ci_from_color32(uint32_t c) { return _mm_cvtepu8_epi32(_mm_cvtsi32_si128(int(c))); }
+inline __m128i ci_from_color64(uint64_t c) { return _mm_cvtepu16_epi32(_mm_cvtsi64_si128((long long)c)); }
+inline __m128i ci_from_f32x3(const float c[3]) { return _mm_cvtps_epi32(_mm_set_ps(0.0f, c[0], c[1], c[2])); }
+inline __m128i ci_from_i32x3(const int32_t c[3]) { return _mm_set_epi32(0, c[0], c[1], c[2]); }
+inline uint32_t ci_into_color32(__m128i v)
+{
+	const __m128i zero = _mm_setzero_si128();
+	return uint32_t(_mm_cvtsi128_si32(_mm_packus_epi16(_mm_packus_epi32(v, zero), zero)));
+}
+inline uint64_t ci_into_color64(__m128i v)
+{
+	return uint64_t(_mm_cvtsi128_si64(_mm_packus_epi32(v, _mm_setzero_si128())));
+}
+
+// AbstractColor (abstract_color.rs) for Color32 / Color64.
+struct C32
+{
+	using T = uint32_t;
+	static __m128 to_vec(T c) { return cv_from_color32(c); }
+	static T from_vec(__m128 v) { return cv_into_color32(v); }
+	static __m128i to_veci(T c) { return ci_from_color32(c); }
+	static T from_veci(__m128i v) { return ci_into_color32(v); }
+	// fast_math.rs:44-52 (pavgb)
+	static T average(T a, T b)
+	{
+		return uint32_t(_mm_cvtsi128_si32(_mm_avg_epu8(_mm_cvtsi32_si128(int(a)), _mm_cvtsi32_si128(int(b)))));
+	}
+	// fast_math.rs:24-32 (paddusb)
+	static T saturated_sum(T a, T b)
+	{
+		return uint32_t(_mm_cvtsi128_si32(_mm_adds_epu8(_mm_cvtsi32_si128(int(a)), _mm_cvtsi32_si128(int(b)))));
+	}
+	static int32_t get_alpha(T c) { return int32_t(c >> 24); }
+	static bool test_alpha(T c) { return (c & 0xFF000000u) > 0x7F000000u; }
+};
+
+struct C64
+{
+	using T = uint64_t;
+	static __m128 to_vec(T c) { return cv_from_color64(c); }
+	static T from_vec(__m128 v) { return cv_into_color64(v); }
+	static __m128i to_veci(T c) { return ci_from_color64(c); }
+	static T from_veci(__m128i v) { return ci_into_color64(v); }
+	static T average(T a, T b)
+	{
+		return uint64_t(
+			_mm_cvtsi128_si64(_mm_avg_epu16(_mm_cvtsi64_si128((long long)a), _mm_cvtsi64_si128((long long)b))));
+	}
+	static T saturated_sum(T a, T b)
+	{
+		return uint64_t(
+			_mm_cvtsi128_si64(_mm_adds_epu16(_mm_cvtsi64_si128((long long)a), _mm_cvtsi64_si128((long long)b))));
+	}
+	static int32_t get_alpha(T c) { return int32_t(c >> 48); }
+	static bool test_alpha(T c) { return (c & 0xFFFF000000000000ull) > 0x007F000000000000ull; }
+};
+
+// abstract_color.rs:30-40 / :90-100
+template <class CT>
+typename CT::T alpha_blend(typename CT::T dst, typename CT::T src)
+{
+	const int32_t alpha = CT::get_alpha(src);
+	const __m128i d = _mm_mullo_epi32(CT::to_veci(dst), _mm_set1_epi32(255 - alpha));
+	const __m128i s = _mm_mullo_epi32(CT::to_veci(src), _mm_set1_epi32(alpha));
+	return CT::from_veci(_mm_srai_epi32(_mm_add_epi32(d, s), 8));
+}
+
+// rasterizer.rs:1511-1536
+template <class CT>
+inline void write_into_framebuffer(uint32_t blend, typename CT::T& dst, typename CT::T texel)
+{
+	switch(blend)
+	{
+	case SW_BLEND_NONE: dst = texel; break;
+	case SW_BLEND_AVERAGE: dst = CT::average(dst, texel); break;
+	case SW_BLEND_ADDITIVE: dst = CT::saturated_sum(dst, texel); break;
+	case SW_BLEND_ALPHA_TEST:
+		if(CT::test_alpha(texel))
+			dst = texel;
+		break;
+	case SW_BLEND_ALPHA_BLEND: dst = alpha_blend<CT>(dst, texel); break;
+	default: break;
+	}
+}
+
+// fast_math.rs:80-96 + textures.rs:41-61
+inline void unpack_normal_roughness(int32_t packed, Vec3& n, float& r)
+{
+	const float s = 1.0f / 127.0f;
+	const __m128 f = _mm_mul_ps(_mm_cvtepi32_ps(_mm_cvtepi8_epi32(_mm_cvtsi32_si128(packed))), _mm_set1_ps(s));
+	alignas(16) float out[4];
+	_mm_store_ps(out, f);
+	n = v3(out[0], out[1], out[2]);
+	r = out[3];
+}
+inline float unpack_roughness(int32_t packed) { return float(packed >> 24) * (1.0f / 127.0f); }
+
+// fast_math.rs:64-78 + textures.rs:33-39
+inline int32_t pack_normal_roughness(Vec3 n, float roughness)
+{
+	const __m128 v = _mm_set_ps(roughness, n.z, n.y, n.x);
+	const __m128 scaled = _mm_mul_ps(v, _mm_set1_ps(127.0f));
+	const __m128i i32v = _mm_cvtps_epi32(scaled);
+	const __m128i zero = _mm_setzero_si128();
+	return _mm_cvtsi128_si32(_mm_packs_epi16(_mm_packs_epi32(i32v, zero), zero));
+}
+
+// ---------------------------------------------------------------------------------------------
+// fixed_math.rs
+// ---------------------------------------------------------------------------------------------
+typedef int32_t Fixed16;
+const Fixed16 FIXED16_ONE = 1 << 16;
+const Fixed16 FIXED16_HALF = FIXED16_ONE >> 1;
+
+inline Fixed16 int_to_fixed16(int32_t x) { return int32_t(uint32_t(x) << 16); }
+inline int32_t fixed16_round_to_int(Fixed16 x) { return (x + FIXED16_HALF) >> 16; }
+inline int32_t fixed16_floor_to_int(Fixed16 x) { return x >> 16; }
+inline float fixed16_to_f32(Fixed16 x) { return float(x) / float(FIXED16_ONE); }
+inline Fixed16 f32_to_fixed16_unchecked(float x) { return _mm_cvttss_si32(_mm_set_ss(x * float(FIXED16_ONE))); }
+inline Fixed16 fixed16_mul(Fixed16 x, Fixed16 y) { return int32_t((int64_t(x) * int64_t(y)) >> 16); }
+inline Fixed16 fixed16_div(Fixed16 x, Fixed16 y) { return int32_t((int64_t(x) * 65536) / int64_t(y)); }
+inline Fixed16 fixed16_mul_div(Fixed16 x, Fixed16 y, Fixed16 z) { return int32_t(int64_t(x) * int64_t(y) / int64_t(z)); }
+
+// rasterizer.rs:1552-1556 (cvttss2si with 64-bit destination)
+inline int64_t unchecked_to_int64(float x) { return _mm_cvttss_si64(_mm_set_ss(x)); }
+
+const int TC_SHIFT = 24;
+const int INV_Z_SHIFT = 29;
+const int INV_Z_PRE_SHIFT = 8;
+const int Z_CALC_SHIFT = 31;
+const int TC_FINAL_SHIFT = TC_SHIFT + Z_CALC_SHIFT - INV_Z_SHIFT + INV_Z_PRE_SHIFT;
+const float TC_SCALE = float(1 << TC_SHIFT);
+const float INV_Z_SCALE = float(1 << INV_Z_SHIFT);
+
+inline int32_t rem_euclid(int32_t a, int32_t b)
+{
+	const int32_t r = a % b;
+	return r < 0 ? r + b : r;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Instrumentation (algorithmic work counts)
+// ---------------------------------------------------------------------------------------------
+struct Counters
+{
+	std::atomic<uint64_t> surface_texels{0}, lightmap_elements{0}, emissive_texels{0}, polygon_pixels{0},
+		triangle_pixels{0}, blend_pixels{0};
+	// Texel fetches outside the texture. The reference debug_assert!s against them
+	// (rasterizer.rs:436-437, :561-564, :709-712, :1231-1234); here they are skipped and counted.
+	std::atomic<uint64_t> out_of_bounds{0};
+};
+
+inline bool blend_reads_dst(uint32_t blend)
+{
+	return blend == SW_BLEND_AVERAGE || blend == SW_BLEND_ADDITIVE || blend == SW_BLEND_ALPHA_BLEND ||
+		blend == SW_BLEND_ALPHA_TEST;
+}
+
+// ---------------------------------------------------------------------------------------------
+// surfaces.rs
+// ---------------------------------------------------------------------------------------------
+struct MapData
+{
+	const sw_texel* texels;
+	const uint32_t* lite_texels;
+	const float* lightmaps_simple; // [f32; 3] per element
+	const sw_directional_lightmap_element* lightmaps_directional;
+	const sw_texture_level* levels;
+	const void* skybox;
+};
+
+struct LmElem
+{
+	float ambient[3];
+	Vec3 dir;
+	float deviation;
+	float color[3];
+};
+
+// surfaces.rs:996-1004 / :1022-1070: mix(a, b, ratio) = fma(a, ratio, b * (1 - ratio))
+inline LmElem lm_mix(const LmElem& a, const LmElem& b, float ratio, bool directional)
+{
+	const float omr = 1.0f - ratio;
+	LmElem r;
+	for(int i = 0; i < 3; ++i)
+		r.ambient[i] = f32_mul_add(a.ambient[i], ratio, b.ambient[i] * omr);
+	if(directional)
+	{
+		r.dir = v3(
+			f32_mul_add(a.dir.x, ratio, b.dir.x * omr), f32_mul_add(a.dir.y, ratio, b.dir.y * omr),
+			f32_mul_add(a.dir.z, ratio, b.dir.z * omr));
+		r.deviation = f32_mul_add(a.deviation, ratio, b.deviation * omr);
+		for(int i = 0; i < 3; ++i)
+			r.color[i] = f32_mul_add(a.color[i], ratio, b.color[i] * omr);
+	}
+	else
+	{
+		r.dir = v3(0, 0, 0);
+		r.deviation = 0.0f;
+		r.color[0] = r.color[1] = r.color[2] = 0.0f;
+	}
+	return r;
+}
+
+inline LmElem lm_load(const MapData& m, bool directional, uint64_t index)
+{
+	LmElem e;
+	if(directional)
+	{
+		const sw_directional_lightmap_element& d = m.lightmaps_directional[index];
+		for(int i = 0; i < 3; ++i)
+		{
+			e.ambient[i] = d.ambient_light[i];
+			e.color[i] = d.directional_light_color[i];
+		}
+		e.dir = v3(d.light_direction_vector_scaled[0], d.light_direction_vector_scaled[1], d.light_direction_vector_scaled[2]);
+		e.deviation = d.directional_light_deviation;
+	}
+	else
+	{
+		for(int i = 0; i < 3; ++i)
+			e.ambient[i] = m.lightmaps_simple[index * 3 + i];
+		e.dir = v3(0, 0, 0);
+		e.deviation = 0.0f;
+		e.color[0] = e.color[1] = e.color[2] = 0.0f;
+	}
+	return e;
+}
+
+const uint32_t SPECULAR_TYPE_NONE = 0, SPECULAR_TYPE_DIELECTRIC = 1, SPECULAR_TYPE_METAL = 2;
+const float DIELECTRIC_ZERO_REFLECTIVITY = 0.04f;
+const float DIELECTRIC_AVERAGE_REFLECTIVITY = DIELECTRIC_ZERO_REFLECTIVITY * 3.0f;
+const float METAL_AVERAGE_SCHLICK_FACTOR = 0.5f;
+
+// surfaces.rs:1088-1114 (the "new formula" branch)
+inline float get_specular_intensity(const Numerics& nm, float angle_cos, float inv_roughness)
+{
+	const float x = (angle_cos - 1.0f) * inv_roughness;
+	const float SQRT_7 = 2.645751311f;
+	const float A = 2.0f * float(M_PI) / SQRT_7;
+	const float B = float(M_PI) * SQRT_7 / 2.0f;
+	return nm.inv_fast(f32_mul_add(x * x, B, A)) * inv_roughness;
+}
+// surfaces.rs:1116-1123
+inline float get_fresnel_factor_base(float angle_cos)
+{
+	const float a = fmax_r(1.0f - angle_cos, 0.0f);
+	const float a2 = a * a;
+	return a2 * a2 * a;
+}
+// surfaces.rs:1125-1141
+inline float get_specular_k_dielectric(float f_base, float roughness)
+{
+	const float f = f32_mul_add(f_base, 1.0f - DIELECTRIC_ZERO_REFLECTIVITY, DIELECTRIC_ZERO_REFLECTIVITY);
+	return f32_mul_add(f, 1.0f - roughness, DIELECTRIC_AVERAGE_REFLECTIVITY * roughness);
+}
+// surfaces.rs:1143-1150
+inline float get_specular_k_metal(float f_base, float roughness)
+{
+	return f32_mul_add(f_base, 1.0f - roughness, METAL_AVERAGE_SCHLICK_FACTOR * roughness);
+}
+
+// surfaces.rs:433-852 (all static specialisations folded into runtime flags).
+template <class CT>
+void build_surface(
+	const Numerics& nm, const MapData& m, const sw_surface_desc& s, const sw_light* lights,
+	const uint32_t* light_indices, Vec3 cam_pos, typename CT::T* out, Counters* counters)
+{
+	const sw_texture_level& tex = m.levels[s.texture_level];
+	const sw_texel* tex_pixels = m.texels + tex.offset;
+	const bool use_normal_map = (tex.flags & SW_TEX_HAS_NORMAL_MAP) != 0;
+	const uint32_t specular_type = (tex.flags & SW_TEX_HAS_NON_ONE_ROUGHNESS) ?
+		((tex.flags & SW_TEX_IS_METAL) ? SPECULAR_TYPE_METAL : SPECULAR_TYPE_DIELECTRIC) :
+		SPECULAR_TYPE_NONE;
+	const bool has_lightmap = s.lightmap_offset != 0;
+	const bool directional = (s.flags & SW_SURF_DIRECTIONAL) != 0;
+	const uint32_t num_lights = std::min<uint32_t>(s.num_lights, SW_MAX_POLYGON_LIGHTS);
+	const bool use_dynamic_lights = num_lights > 0;
+	const uint32_t scale_log2 = s.lightmap_scale_log2;
+
+	const Vec3 u_vec = v3(s.basis_u[0], s.basis_u[1], s.basis_u[2]);
+	const Vec3 v_vec = v3(s.basis_v[0], s.basis_v[1], s.basis_v[2]);
+	const Vec3 basis_start = v3(s.basis_start[0], s.basis_start[1], s.basis_start[2]);
+	const Vec3 basis_normal = v3(s.basis_normal[0], s.basis_normal[1], s.basis_normal[2]);
+
+	// :456-457
+	const Vec3 start_pos =
+		basis_start + u_vec * (float(s.tc_start[0]) + 0.5f) + v_vec * (float(s.tc_start[1]) + 0.5f);
+	// :461-462
+	const Vec3 u_vec_normalized = u_vec * nm.inv_sqrt_fast(fmax_r(vec3_len2(u_vec), MIN_POSITIVE_VALUE));
+	const Vec3 v_vec_normalized = v_vec * nm.inv_sqrt_fast(fmax_r(vec3_len2(v_vec), MIN_POSITIVE_VALUE));
+
+	LmElem line_lightmap[SW_MAX_LIGHTMAP_SIZE + 2];
+	std::memset(line_lightmap, 0, sizeof(line_lightmap));
+	const float lightmap_scale_f = float(1u << (has_lightmap ? scale_log2 : 0));
+	const float inv_lightmap_scale_f = 1.0f / lightmap_scale_f;
+	const float k_shift = 0.5f * inv_lightmap_scale_f;
+	const uint32_t lightmap_fetch_mask = (1u << (has_lightmap ? scale_log2 : 0)) - 1u;
+
+	const uint32_t w = s.size[0], h = s.size[1];
+	for(uint32_t dst_v = 0; dst_v < h; ++dst_v)
+	{
+		if(has_lightmap)
+		{
+			// :477-502
+			const uint32_t lightmap_base_v = dst_v + s.lightmap_tc_shift[1];
+			const uint32_t lightmap_v = lightmap_base_v >> scale_log2;
+			const float k = f32_mul_add(float(lightmap_base_v & lightmap_fetch_mask), inv_lightmap_scale_f, k_shift);
+			const uint64_t base = uint64_t(s.lightmap_offset) + uint64_t(lightmap_v) * s.lightmap_size[0];
+			const uint32_t n = std::min<uint32_t>(s.lightmap_size[0], SW_MAX_LIGHTMAP_SIZE + 2);
+			for(uint32_t i = 0; i < n; ++i)
+			{
+				const LmElem l0 = lm_load(m, directional, base + i);
+				const LmElem l1 = lm_load(m, directional, base + s.lightmap_size[0] + i);
+				line_lightmap[i] = lm_mix(l1, l0, k, directional);
+			}
+			if(counters)
+				counters->lightmap_elements += 2 * uint64_t(s.lightmap_size[0]);
+		}
+
+		typename CT::T* dst_line = out + uint64_t(dst_v) * w;
+		const int32_t src_v = rem_euclid(s.tc_start[1] + int32_t(dst_v), int32_t(tex.height));
+		const sw_texel* src_line = tex_pixels + uint64_t(uint32_t(src_v)) * tex.width;
+		int32_t src_u = rem_euclid(s.tc_start[0], int32_t(tex.width));
+		const Vec3 start_pos_v = vec3_scalar_mul_add(v_vec, float(dst_v), start_pos);
+
+		for(uint32_t dst_u = 0; dst_u < w; ++dst_u)
+		{
+			const Vec3 pos = vec3_scalar_mul_add(u_vec, float(dst_u), start_pos_v);
+			const sw_texel texel_value = src_line[src_u];
+
+			Vec3 texel_normal;
+			float texel_roughness;
+			if(use_normal_map)
+				unpack_normal_roughness(texel_value.packed_normal_roughness, texel_normal, texel_roughness);
+			else if(specular_type == SPECULAR_TYPE_NONE)
+			{
+				texel_normal = v3(0, 0, 1);
+				texel_roughness = 0.0f;
+			}
+			else
+			{
+				texel_normal = v3(0, 0, 1);
+				texel_roughness = unpack_roughness(texel_value.packed_normal_roughness);
+			}
+
+			__m128 total_light_albedo_modulated = _mm_setzero_ps();
+			__m128 total_light_direct = _mm_setzero_ps();
+			if(has_lightmap)
+			{
+				// :538-549
+				const uint32_t lightmap_base_u = dst_u + s.lightmap_tc_shift[0];
+				const uint32_t lightmap_u = lightmap_base_u >> scale_log2;
+				const LmElem& l0 = line_lightmap[lightmap_u];
+				const LmElem& l1 = line_lightmap[lightmap_u + 1];
+				const float k = f32_mul_add(float(lightmap_base_u & lightmap_fetch_mask), inv_lightmap_scale_f, k_shift);
+				const LmElem l_mixed = lm_mix(l1, l0, k, directional);
+
+				if(specular_type == SPECULAR_TYPE_NONE)
+				{
+					// :551-572
+					total_light_albedo_modulated = cv_from_f32x3_with_one(l_mixed.ambient);
+					if(directional)
+					{
+						const float dot =
+							use_normal_map ? fmax_r(vec3_dot(l_mixed.dir, texel_normal), 0.0f) : l_mixed.dir.z;
+						total_light_albedo_modulated =
+							cv_mul_scalar_add(cv_from_f32x3(l_mixed.color), dot, total_light_albedo_modulated);
+					}
+				}
+				else
+				{
+					// :575-677
+					const Vec3 normal = use_normal_map ? texel_normal : v3(0, 0, 1);
+					const Vec3 vec_to_camera = cam_pos - pos;
+					const Vec3 vec_to_camera_ts = v3(
+						vec3_dot(vec_to_camera, u_vec_normalized), vec3_dot(vec_to_camera, v_vec_normalized),
+						vec3_dot(vec_to_camera, basis_normal));
+					const float vec_to_camera_normal_dot = vec3_dot(vec_to_camera_ts, normal);
+					const Vec3 vec_to_camera_reflected = normal * (2.0f * vec_to_camera_normal_dot) - vec_to_camera_ts;
+					const float vec_to_camera_len2 = fmax_r(vec3_len2(vec_to_camera_reflected), MIN_POSITIVE_VALUE);
+					const float angle_cos =
+						fmax_r(vec_to_camera_normal_dot * nm.inv_sqrt_fast(vec_to_camera_len2), 0.0f);
+					const float fresnel_factor_base = get_fresnel_factor_base(angle_cos);
+					const float specular_k = specular_type == SPECULAR_TYPE_DIELECTRIC ?
+						get_specular_k_dielectric(fresnel_factor_base, texel_roughness) :
+						get_specular_k_metal(fresnel_factor_base, texel_roughness);
+					const float one_minus_specular_k = 1.0f - specular_k;
+					const __m128 constant_component = cv_from_f32x3(l_mixed.ambient);
+					total_light_albedo_modulated = cv_scalar_mul(constant_component, one_minus_specular_k);
+					total_light_direct = cv_scalar_mul(constant_component, specular_k);
+					total_light_albedo_modulated = cv_insert3(total_light_albedo_modulated, 1.0f);
+
+					if(directional)
+					{
+						const float direction_vec_len2 = vec3_len2(l_mixed.dir);
+						const float direction_vec_len = direction_vec_len2 * nm.inv_sqrt_fast(direction_vec_len2);
+						const float reflected_light_angle_cos = vec3_dot(vec_to_camera_reflected, l_mixed.dir) *
+							nm.inv_sqrt_fast(vec_to_camera_len2 * direction_vec_len2);
+						const float inv_roughness_corrected =
+							fmax_r(nm.inv_fast(texel_roughness + l_mixed.deviation), 0.75f);
+						const float specular_intensity =
+							get_specular_intensity(nm, reflected_light_angle_cos, inv_roughness_corrected);
+						const __m128 dir_color = cv_from_f32x3(l_mixed.color);
+						if(specular_type == SPECULAR_TYPE_DIELECTRIC)
+						{
+							const float diffuse_intensity = fmax_r(vec3_dot(l_mixed.dir, texel_normal), 0.0f);
+							const float light_intensity_diffuse = diffuse_intensity * one_minus_specular_k;
+							total_light_albedo_modulated =
+								cv_mul_scalar_add(dir_color, light_intensity_diffuse, total_light_albedo_modulated);
+							const float light_intensity_specular = specular_intensity * specular_k * direction_vec_len;
+							total_light_direct = cv_mul_scalar_add(dir_color, light_intensity_specular, total_light_direct);
+						}
+						else
+						{
+							const float scale_factor = specular_intensity * direction_vec_len;
+							const float light_intensity_modulated = one_minus_specular_k * scale_factor;
+							total_light_albedo_modulated =
+								cv_mul_scalar_add(dir_color, light_intensity_modulated, total_light_albedo_modulated);
+							const float light_intensity_direct = specular_k * scale_factor;
+							total_light_direct = cv_mul_scalar_add(dir_color, light_intensity_direct, total_light_direct);
+						}
+					}
+				}
+			}
+
+			if(use_dynamic_lights)
+			{
+				// :683-707
+				Vec3 normal;
+				if(use_normal_map)
+				{
+					normal = v3(
+						f32_mul_add(
+							texel_normal.x, u_vec_normalized.x,
+							f32_mul_add(texel_normal.y, v_vec_normalized.x, texel_normal.z * basis_normal.x)),
+						f32_mul_add(
+							texel_normal.x, u_vec_normalized.y,
+							f32_mul_add(texel_normal.y, v_vec_normalized.y, texel_normal.z * basis_normal.y)),
+						f32_mul_add(
+							texel_normal.x, u_vec_normalized.z,
+							f32_mul_add(texel_normal.y, v_vec_normalized.z, texel_normal.z * basis_normal.z)));
+				}
+				else
+					normal = basis_normal;
+
+				Vec3 vec_to_camera_reflected;
+				float vec_to_camera_len2, specular_k, inv_roughness;
+				if(specular_type != SPECULAR_TYPE_NONE)
+				{
+					// :714-741
+					const Vec3 vec_to_camera = cam_pos - pos;
+					const float vec_to_camera_normal_dot = vec3_dot(vec_to_camera, normal);
+					vec_to_camera_reflected = normal * (2.0f * vec_to_camera_normal_dot) - vec_to_camera;
+					vec_to_camera_len2 = fmax_r(vec3_len2(vec_to_camera_reflected), MIN_POSITIVE_VALUE);
+					const float angle_cos =
+						fmax_r(vec_to_camera_normal_dot * nm.inv_sqrt_fast(vec_to_camera_len2), 0.0f);
+					const float fresnel_factor_base = get_fresnel_factor_base(angle_cos);
+					specular_k = specular_type == SPECULAR_TYPE_DIELECTRIC ?
+						get_specular_k_dielectric(fresnel_factor_base, texel_roughness) :
+						get_specular_k_metal(fresnel_factor_base, texel_roughness);
+					inv_roughness = nm.inv_fast(texel_roughness);
+				}
+				else
+				{
+					vec_to_camera_reflected = v3(0, 0, 0);
+					vec_to_camera_len2 = MIN_POSITIVE_VALUE;
+					specular_k = 0.0f;
+					inv_roughness = 0.0f;
+				}
+
+				for(uint32_t li = 0; li < num_lights; ++li)
+				{
+					// :750-830
+					const sw_light& light = lights[light_indices[s.first_light + li]];
+					const Vec3 light_pos = v3(light.position[0], light.position[1], light.position[2]);
+					const Vec3 vec_to_light = light_pos - pos;
+					const float shadow_factor = 1.0f; // ShadowMap::None (light.rs:185-294)
+					const float vec_to_light_len2 = fmax_r(vec3_len2(vec_to_light), MIN_POSITIVE_VALUE);
+					const float distance_factor = fmax_r(nm.inv_fast(vec_to_light_len2) - light.inv_square_radius, 0.0f);
+					const float shadow_distance_factor = shadow_factor * distance_factor;
+					const float diffuse_intensity = specular_type == SPECULAR_TYPE_METAL ?
+						0.0f :
+						fmax_r(vec3_dot(normal, vec_to_light) * nm.inv_sqrt_fast(vec_to_light_len2), 0.0f);
+					float specular_intensity = 0.0f;
+					if(specular_type != SPECULAR_TYPE_NONE)
+					{
+						const float c = vec3_dot(vec_to_camera_reflected, vec_to_light) *
+							nm.inv_sqrt_fast(vec_to_camera_len2 * vec_to_light_len2);
+						specular_intensity = get_specular_intensity(nm, c, inv_roughness);
+					}
+					const __m128 light_color = cv_from_f32x3(light.color);
+					if(specular_type == SPECULAR_TYPE_NONE)
+					{
+						total_light_albedo_modulated = cv_mul_scalar_add(
+							light_color, diffuse_intensity * shadow_distance_factor, total_light_albedo_modulated);
+					}
+					else if(specular_type == SPECULAR_TYPE_DIELECTRIC)
+					{
+						const float light_intensity_diffuse =
+							diffuse_intensity * (1.0f - specular_k) * shadow_distance_factor;
+						total_light_albedo_modulated =
+							cv_mul_scalar_add(light_color, light_intensity_diffuse, total_light_albedo_modulated);
+						const float light_intensity_specular = specular_intensity * specular_k * shadow_distance_factor;
+						total_light_direct = cv_mul_scalar_add(light_color, light_intensity_specular, total_light_direct);
+					}
+					else
+					{
+						const float sisdf = specular_intensity * shadow_distance_factor;
+						const float light_intensity_modulated = (1.0f - specular_k) * sisdf;
+						total_light_albedo_modulated =
+							cv_mul_scalar_add(light_color, light_intensity_modulated, total_light_albedo_modulated);
+						const float light_intensity_direct = specular_k * sisdf;
+						total_light_direct = cv_mul_scalar_add(light_color, light_intensity_direct, total_light_direct);
+					}
+				}
+			}
+
+			// :833-842
+			__m128 result_color = _mm_mul_ps(cv_from_color32(texel_value.diffuse), total_light_albedo_modulated);
+			if(specular_type != SPECULAR_TYPE_NONE)
+				result_color = cv_mul_scalar_add(total_light_direct, 255.0f, result_color);
+			dst_line[dst_u] = CT::from_vec(result_color);
+
+			++src_u;
+			if(src_u == int32_t(tex.width))
+				src_u = 0;
+		}
+	}
+	if(counters)
+		counters->surface_texels += uint64_t(w) * h;
+}
+
+// surfaces.rs:903-971
+template <class CT>
+void mix_surface_with_texture(
+	const MapData& m, const sw_surface_desc& s, uint32_t blend, typename CT::T* data, Counters* counters)
+{
+	const sw_texture_level& tex = m.levels[s.emissive_level];
+	const uint32_t* tex_pixels = m.lite_texels + tex.offset;
+	const float light_scale = float(1 << 8);
+	const float lv[3] = {s.emissive_light[0] * light_scale, s.emissive_light[1] * light_scale, s.emissive_light[2] * light_scale};
+	const __m128i light_vec = ci_from_f32x3(lv);
+	for(uint32_t dst_v = 0; dst_v < s.size[1]; ++dst_v)
+	{
+		typename CT::T* dst_line = data + uint64_t(dst_v) * s.size[0];
+		const int32_t src_v = rem_euclid(s.tc_start[1] + int32_t(dst_v), int32_t(tex.height));
+		const uint32_t* src_line = tex_pixels + uint64_t(uint32_t(src_v)) * tex.width;
+		int32_t src_u = rem_euclid(s.tc_start[0], int32_t(tex.width));
+		for(uint32_t x = 0; x < s.size[0]; ++x)
+		{
+			const uint32_t texel_value = src_line[src_u];
+			const __m128i modulated = _mm_srai_epi32(_mm_mullo_epi32(ci_from_color32(texel_value), light_vec), 8);
+			typename CT::T& dst = dst_line[x];
+			switch(blend)
+			{
+			case SW_BLEND_NONE: dst = CT::from_veci(modulated); break;
+			case SW_BLEND_AVERAGE:
+				dst = CT::from_veci(_mm_srai_epi32(_mm_add_epi32(modulated, CT::to_veci(dst)), 1));
+				break;
+			case SW_BLEND_ADDITIVE: dst = CT::from_veci(_mm_add_epi32(modulated, CT::to_veci(dst))); break;
+			case SW_BLEND_ALPHA_TEST:
+				if(C32::test_alpha(texel_value))
+					dst = CT::from_veci(modulated);
+				break;
+			case SW_BLEND_ALPHA_BLEND:
+			{
+				const int32_t alpha = C32::get_alpha(texel_value);
+				dst = CT::from_veci(_mm_srai_epi32(
+					_mm_add_epi32(
+						_mm_mullo_epi32(modulated, _mm_set1_epi32(alpha)),
+						_mm_mullo_epi32(CT::to_veci(dst), _mm_set1_epi32(255 - alpha))),
+					8));
+				break;
+			}
+			default: break;
+			}
+			++src_u;
+			if(src_u == int32_t(tex.width))
+				src_u = 0;
+		}
+	}
+	if(counters)
+		counters->emissive_texels += uint64_t(s.size[0]) * s.size[1];
+}
+
+// ---------------------------------------------------------------------------------------------
+// rasterizer.rs
+// ---------------------------------------------------------------------------------------------
+struct ClipRect
+{
+	int32_t min_x, min_y, max_x, max_y;
+};
+
+struct PolygonSide
+{
+	Fixed16 x_start, dx_dy;
+};
+
+template <class CT>
+struct Rasterizer
+{
+	typename CT::T* color_buffer;
+	int32_t row_size;
+	ClipRect clip_rect;
+	Counters* counters;
+
+	bool in_texture(uint32_t u, uint32_t v, const int32_t* tsize)
+	{
+		if(u < uint32_t(tsize[0]) && v < uint32_t(tsize[1]))
+			return true;
+		if(counters)
+			counters->out_of_bounds++;
+		return false;
+	}
+
+	void count(uint32_t blend, int64_t n, bool triangle)
+	{
+		if(!counters || n <= 0)
+			return;
+		if(triangle)
+			counters->triangle_pixels += uint64_t(n);
+		else
+			counters->polygon_pixels += uint64_t(n);
+		if(blend_reads_dst(blend))
+			counters->blend_pixels += uint64_t(n);
+	}
+
+	// rasterizer.rs:273-454
+	void fill_polygon_part(
+		uint32_t blend, Fixed16 y_start, Fixed16 y_end, PolygonSide left_side, PolygonSide right_side, const float* de,
+		const float* te, const int32_t* tsize, const typename CT::T* texture_data)
+	{
+		const int64_t d_inv_z_dx = unchecked_to_int64(INV_Z_SCALE * de[0]);
+		const int64_t d_inv_z_dy = unchecked_to_int64(INV_Z_SCALE * de[1]);
+		const int64_t inv_z_k = unchecked_to_int64(INV_Z_SCALE * (de[2] + (de[0] + de[1]) * 0.5f));
+		const int64_t d_tc_dx[2] = {unchecked_to_int64(TC_SCALE * te[0]), unchecked_to_int64(TC_SCALE * te[1])};
+		const int64_t d_tc_dy[2] = {unchecked_to_int64(TC_SCALE * te[2]), unchecked_to_int64(TC_SCALE * te[3])};
+		const int64_t tc_k[2] = {
+			unchecked_to_int64(TC_SCALE * (te[4] + (te[0] + te[2]) * 0.5f)),
+			unchecked_to_int64(TC_SCALE * (te[5] + (te[1] + te[3]) * 0.5f))};
+		const uint32_t texture_width = uint32_t(tsize[0]);
+
+		const int32_t y_start_int = std::max(fixed16_round_to_int(y_start), clip_rect.min_y);
+		const int32_t y_end_int = std::min(fixed16_round_to_int(y_end), clip_rect.max_y);
+		const Fixed16 y_start_delta = int_to_fixed16(y_start_int) + FIXED16_HALF - y_start;
+		Fixed16 x_left = left_side.x_start + fixed16_mul(y_start_delta, left_side.dx_dy) + FIXED16_HALF;
+		Fixed16 x_right = right_side.x_start + fixed16_mul(y_start_delta, right_side.dx_dy) + FIXED16_HALF;
+		int64_t line_inv_z = int64_t(y_start_int) * d_inv_z_dy + inv_z_k;
+		int64_t line_tc[2] = {int64_t(y_start_int) * d_tc_dy[0] + tc_k[0], int64_t(y_start_int) * d_tc_dy[1] + tc_k[1]};
+
+		for(int32_t y_int = y_start_int; y_int < y_end_int; ++y_int)
+		{
+			const int32_t x_start_int = std::max(fixed16_floor_to_int(x_left), clip_rect.min_x);
+			const int32_t x_end_int = std::min(fixed16_floor_to_int(x_right), clip_rect.max_x);
+			if(x_start_int < x_end_int)
+			{
+				typename CT::T* line_dst = color_buffer + int64_t(y_int) * row_size;
+				const int64_t span_start_x = x_start_int;
+				const int64_t span_end_x = x_end_int - 1;
+				const int32_t span_length_minus_one = (x_end_int - 1) - x_start_int;
+
+				const int64_t inv_z_min = 1 << INV_Z_PRE_SHIFT;
+				const int64_t inv_z_max = 1 << 29;
+				const int32_t span_inv_z_start =
+					int32_t(std::max(inv_z_min, std::min(span_start_x * d_inv_z_dx + line_inv_z, inv_z_max)));
+				const int32_t span_inv_z_end =
+					int32_t(std::max(inv_z_min, std::min(span_end_x * d_inv_z_dx + line_inv_z, inv_z_max)));
+				const int32_t span_d_inv_z =
+					span_length_minus_one > 0 ? (span_inv_z_end - span_inv_z_start) / span_length_minus_one : 0;
+				int32_t span_inv_z = span_inv_z_start;
+
+				int32_t span_tc[2] = {0, 0}, span_d_tc[2] = {0, 0};
+				for(int i = 0; i < 2; ++i)
+				{
+					const int tc_max_shift = INV_Z_SHIFT - TC_SHIFT;
+					const int64_t tc_max_max = 1 << 29;
+					const int64_t span_inv_z_corrected = int64_t(span_inv_z - (1 << INV_Z_PRE_SHIFT));
+					const int64_t tc_max_start = std::max<int64_t>(
+						0, std::min(((span_inv_z_corrected * int64_t(tsize[i])) >> tc_max_shift) - 1, tc_max_max));
+					const int32_t span_tc_start =
+						int32_t(std::max<int64_t>(0, std::min(span_start_x * d_tc_dx[i] + line_tc[i], tc_max_start)));
+					if(span_length_minus_one > 0)
+					{
+						const int64_t tc_max_end = std::max<int64_t>(
+							0,
+							std::min(
+								(((span_inv_z_corrected + int64_t(int32_t(span_d_inv_z * span_length_minus_one))) *
+								  int64_t(tsize[i])) >>
+								 tc_max_shift) -
+									1,
+								tc_max_max));
+						const int32_t span_tc_end =
+							int32_t(std::max<int64_t>(0, std::min(span_end_x * d_tc_dx[i] + line_tc[i], tc_max_end)));
+						span_d_tc[i] = (span_tc_end - span_tc_start) / span_length_minus_one;
+						if(span_tc_end > span_tc_start)
+							span_tc[i] = span_tc_start;
+						else
+							span_tc[i] = span_tc_end - span_d_tc[i] * span_length_minus_one;
+					}
+					else
+						span_tc[i] = span_tc_start;
+				}
+
+				for(int32_t x = x_start_int; x < x_end_int; ++x)
+				{
+					const uint32_t z = (1u << Z_CALC_SHIFT) / (uint32_t(span_inv_z) >> INV_Z_PRE_SHIFT);
+					const uint32_t pix_tc0 = uint32_t((uint64_t(z) * uint64_t(int64_t(span_tc[0]))) >> TC_FINAL_SHIFT);
+					const uint32_t pix_tc1 = uint32_t((uint64_t(z) * uint64_t(int64_t(span_tc[1]))) >> TC_FINAL_SHIFT);
+					if(in_texture(pix_tc0, pix_tc1, tsize))
+						write_into_framebuffer<CT>(blend, line_dst[x], texture_data[pix_tc0 + pix_tc1 * texture_width]);
+					span_inv_z += span_d_inv_z;
+					span_tc[0] += span_d_tc[0];
+					span_tc[1] += span_d_tc[1];
+				}
+				count(blend, x_end_int - x_start_int, false);
+			}
+			x_left += left_side.dx_dy;
+			x_right += right_side.dx_dy;
+			line_inv_z += d_inv_z_dy;
+			line_tc[0] += d_tc_dy[0];
+			line_tc[1] += d_tc_dy[1];
+		}
+	}
+
+	// rasterizer.rs:456-581
+	void fill_polygon_part_line_z_corrected(
+		uint32_t blend, Fixed16 y_start, Fixed16 y_end, PolygonSide left_side, PolygonSide right_side, const float* de,
+		const float* te, const int32_t* tsize, const typename CT::T* texture_data)
+	{
+		const int LINE_TC_SHIFT = 16;
+		const int64_t d_inv_z_dx = unchecked_to_int64(INV_Z_SCALE * de[0]);
+		const int64_t d_inv_z_dy = unchecked_to_int64(INV_Z_SCALE * de[1]);
+		const int64_t inv_z_k = unchecked_to_int64(INV_Z_SCALE * (de[2] + (de[0] + de[1]) * 0.5f));
+		const int64_t d_tc_dx[2] = {unchecked_to_int64(TC_SCALE * te[0]), unchecked_to_int64(TC_SCALE * te[1])};
+		const int64_t d_tc_dy[2] = {unchecked_to_int64(TC_SCALE * te[2]), unchecked_to_int64(TC_SCALE * te[3])};
+		const int64_t tc_k[2] = {
+			unchecked_to_int64(TC_SCALE * (te[4] + (te[0] + te[2]) * 0.5f)),
+			unchecked_to_int64(TC_SCALE * (te[5] + (te[1] + te[3]) * 0.5f))};
+		const uint32_t texture_width = uint32_t(tsize[0]);
+
+		const int32_t y_start_int = std::max(fixed16_round_to_int(y_start), clip_rect.min_y);
+		const int32_t y_end_int = std::min(fixed16_round_to_int(y_end), clip_rect.max_y);
+		const Fixed16 y_start_delta = int_to_fixed16(y_start_int) + FIXED16_HALF - y_start;
+		Fixed16 x_left = left_side.x_start + fixed16_mul(y_start_delta, left_side.dx_dy) + FIXED16_HALF;
+		Fixed16 x_right = right_side.x_start + fixed16_mul(y_start_delta, right_side.dx_dy) + FIXED16_HALF;
+		int64_t line_inv_z = int64_t(y_start_int) * d_inv_z_dy + inv_z_k;
+		int64_t line_tc[2] = {int64_t(y_start_int) * d_tc_dy[0] + tc_k[0], int64_t(y_start_int) * d_tc_dy[1] + tc_k[1]};
+
+		for(int32_t y_int = y_start_int; y_int < y_end_int; ++y_int)
+		{
+			const int32_t x_start_int = std::max(fixed16_floor_to_int(x_left), clip_rect.min_x);
+			const int32_t x_end_int = std::min(fixed16_floor_to_int(x_right), clip_rect.max_x);
+			if(x_start_int < x_end_int)
+			{
+				typename CT::T* line_dst = color_buffer + int64_t(y_int) * row_size;
+				const int64_t span_start_x = x_start_int;
+				const int64_t span_end_x = x_end_int - 1;
+				const int32_t span_length_minus_one = (x_end_int - 1) - x_start_int;
+				const int64_t inv_z_min = 1 << INV_Z_PRE_SHIFT;
+				const int64_t inv_z_max = 1 << 29;
+				const uint32_t span_inv_z_start =
+					uint32_t(std::min(std::max(span_start_x * d_inv_z_dx + line_inv_z, inv_z_min), inv_z_max));
+				const uint32_t span_inv_z_end =
+					uint32_t(std::min(std::max(span_end_x * d_inv_z_dx + line_inv_z, inv_z_min), inv_z_max));
+				const uint64_t span_z_start = (1u << Z_CALC_SHIFT) / (span_inv_z_start >> INV_Z_PRE_SHIFT);
+				const uint64_t span_z_end = (1u << Z_CALC_SHIFT) / (span_inv_z_end >> INV_Z_PRE_SHIFT);
+
+				int32_t span_tc[2] = {0, 0}, span_d_tc[2] = {0, 0};
+				for(int i = 0; i < 2; ++i)
+				{
+					const uint64_t tc_start = uint64_t(std::max<int64_t>(line_tc[i] + span_start_x * d_tc_dx[i], 0));
+					const uint64_t tc_end = uint64_t(std::max<int64_t>(line_tc[i] + span_end_x * d_tc_dx[i], 0));
+					const uint64_t max_tc = (uint64_t(uint32_t(tsize[i])) << LINE_TC_SHIFT) - 1;
+					const int32_t tc_mul_z_start =
+						int32_t(std::min((span_z_start * tc_start) >> (TC_FINAL_SHIFT - LINE_TC_SHIFT), max_tc));
+					const int32_t tc_mul_z_end =
+						int32_t(std::min((span_z_end * tc_end) >> (TC_FINAL_SHIFT - LINE_TC_SHIFT), max_tc));
+					span_tc[i] = tc_mul_z_start;
+					if(span_length_minus_one > 0)
+						span_d_tc[i] = (tc_mul_z_end - tc_mul_z_start) / span_length_minus_one;
+				}
+
+				for(int32_t x = x_start_int; x < x_end_int; ++x)
+				{
+					const int32_t tc0 = span_tc[0] >> LINE_TC_SHIFT;
+					const int32_t tc1 = span_tc[1] >> LINE_TC_SHIFT;
+					if(in_texture(uint32_t(tc0), uint32_t(tc1), tsize))
+						write_into_framebuffer<CT>(
+							blend, line_dst[x], texture_data[uint32_t(tc0) + uint32_t(tc1) * texture_width]);
+					span_tc[0] += span_d_tc[0];
+					span_tc[1] += span_d_tc[1];
+				}
+				count(blend, x_end_int - x_start_int, false);
+			}
+			x_left += left_side.dx_dy;
+			x_right += right_side.dx_dy;
+			line_inv_z += d_inv_z_dy;
+			line_tc[0] += d_tc_dy[0];
+			line_tc[1] += d_tc_dy[1];
+		}
+	}
+
+	// rasterizer.rs:583-729
+	void fill_polygon_part_affine(
+		uint32_t blend, Fixed16 y_start, Fixed16 y_end, PolygonSide left_side, PolygonSide right_side, const float* de,
+		const float* te, const int32_t* tsize, const typename CT::T* texture_data)
+	{
+		const float y_start_f32 = fixed16_to_f32(y_start);
+		const float y_end_f32 = fixed16_to_f32(y_end);
+		const Fixed16 y_delta = y_end - y_start;
+		const float x_start_left = fixed16_to_f32(left_side.x_start);
+		const float x_end_left = fixed16_to_f32(left_side.x_start + fixed16_mul(y_delta, left_side.dx_dy));
+		const float x_start_right = fixed16_to_f32(right_side.x_start);
+		const float x_end_right = fixed16_to_f32(right_side.x_start + fixed16_mul(y_delta, right_side.dx_dy));
+		const float z_start_left = 1.0f / (de[0] * x_start_left + de[1] * y_start_f32 + de[2]);
+		const float z_end_left = 1.0f / (de[0] * x_end_left + de[1] * y_end_f32 + de[2]);
+		const float z_start_right = 1.0f / (de[0] * x_start_right + de[1] * y_start_f32 + de[2]);
+		const float z_end_right = 1.0f / (de[0] * x_end_right + de[1] * y_end_f32 + de[2]);
+		const Fixed16 y_delta_for_tc_interpolation = std::max(y_delta, FIXED16_ONE);
+
+		Fixed16 tc_left[2], tc_right[2], d_tc_left[2], d_tc_right[2];
+		for(int i = 0; i < 2; ++i)
+		{
+			const Fixed16 max_tc = int_to_fixed16(tsize[i]) - 1;
+			auto conv = [&](float z, float x, float y) {
+				return std::min(
+					std::max(f32_to_fixed16_unchecked(z * (x * te[0 + i] + y * te[2 + i] + te[4 + i])), 0), max_tc);
+			};
+			const Fixed16 tc_start_left = conv(z_start_left, x_start_left, y_start_f32);
+			const Fixed16 tc_end_left = conv(z_end_left, x_end_left, y_end_f32);
+			const Fixed16 tc_start_right = conv(z_start_right, x_start_right, y_start_f32);
+			const Fixed16 tc_end_right = conv(z_end_right, x_end_right, y_end_f32);
+			d_tc_left[i] = fixed16_div(tc_end_left - tc_start_left, y_delta_for_tc_interpolation);
+			d_tc_right[i] = fixed16_div(tc_end_right - tc_start_right, y_delta_for_tc_interpolation);
+			tc_left[i] = tc_start_left;
+			tc_right[i] = tc_start_right;
+		}
+
+		const int32_t y_start_int = std::max(fixed16_round_to_int(y_start), clip_rect.min_y);
+		const int32_t y_end_int = std::min(fixed16_round_to_int(y_end), clip_rect.max_y);
+		const Fixed16 y_start_delta = int_to_fixed16(y_start_int) + FIXED16_HALF - y_start;
+		Fixed16 x_left = left_side.x_start + fixed16_mul(y_start_delta, left_side.dx_dy) + FIXED16_HALF;
+		Fixed16 x_right = right_side.x_start + fixed16_mul(y_start_delta, right_side.dx_dy) + FIXED16_HALF;
+		for(int i = 0; i < 2; ++i)
+		{
+			tc_left[i] += fixed16_mul(y_start_delta, d_tc_left[i]);
+			tc_right[i] += fixed16_mul(y_start_delta, d_tc_right[i]);
+		}
+
+		for(int32_t y_int = y_start_int; y_int < y_end_int; ++y_int)
+		{
+			const int32_t x_start_int = std::max(fixed16_floor_to_int(x_left), clip_rect.min_x);
+			const int32_t x_end_int = std::min(fixed16_floor_to_int(x_right), clip_rect.max_x);
+			if(x_start_int < x_end_int)
+			{
+				typename CT::T* line_dst = color_buffer + int64_t(y_int) * row_size;
+				Fixed16 tc[2] = {tc_left[0], tc_left[1]};
+				const Fixed16 x_delta = std::max(x_right - x_left, FIXED16_ONE);
+				const Fixed16 d_tc[2] = {
+					fixed16_div(tc_right[0] - tc_left[0], x_delta), fixed16_div(tc_right[1] - tc_left[1], x_delta)};
+				for(int32_t x = x_start_int; x < x_end_int; ++x)
+				{
+					const int32_t u = fixed16_floor_to_int(tc[0]);
+					const int32_t v = fixed16_floor_to_int(tc[1]);
+					if(in_texture(uint32_t(u), uint32_t(v), tsize))
+						write_into_framebuffer<CT>(blend, line_dst[x], texture_data[u + v * tsize[0]]);
+					tc[0] += d_tc[0];
+					tc[1] += d_tc[1];
+				}
+				count(blend, x_end_int - x_start_int, false);
+			}
+			x_left += left_side.dx_dy;
+			x_right += right_side.dx_dy;
+			tc_left[0] += d_tc_left[0];
+			tc_left[1] += d_tc_left[1];
+			tc_right[0] += d_tc_right[0];
+			tc_right[1] += d_tc_right[1];
+		}
+	}
+
+	void draw_part(
+		uint32_t mode, uint32_t blend, Fixed16 y0, Fixed16 y1, PolygonSide l, PolygonSide r, const float* de,
+		const float* te, const int32_t* ts, const typename CT::T* tex)
+	{
+		if(mode == SW_TC_LINE_Z)
+			fill_polygon_part_line_z_corrected(blend, y0, y1, l, r, de, te, ts, tex);
+		else if(mode == SW_TC_AFFINE)
+			fill_polygon_part_affine(blend, y0, y1, l, r, de, te, ts, tex);
+		else
+			fill_polygon_part(blend, y0, y1, l, r, de, te, ts, tex);
+	}
+
+	// rasterizer.rs:137-271
+	void fill_polygon(
+		const int32_t* vertices, uint32_t n, uint32_t mode, uint32_t blend, const float* de, const float* te,
+		const int32_t* tsize, const typename CT::T* texture_data)
+	{
+		auto vx = [&](uint32_t i) { return vertices[i * 2 + 0]; };
+		auto vy = [&](uint32_t i) { return vertices[i * 2 + 1]; };
+		uint32_t lower_vertex_index = 0;
+		Fixed16 min_y = vy(0);
+		for(uint32_t i = 0; i < n; ++i)
+			if(vy(i) < min_y)
+			{
+				min_y = vy(i);
+				lower_vertex_index = i;
+			}
+
+		uint32_t left_index = lower_vertex_index, right_index = lower_vertex_index;
+		Fixed16 cur_y = min_y;
+		for(;;)
+		{
+			uint32_t next_left_index = left_index + n - 1;
+			if(next_left_index >= n)
+				next_left_index -= n;
+			uint32_t next_right_index = right_index + 1;
+			if(next_right_index >= n)
+				next_right_index -= n;
+
+			const Fixed16 dy_left = vy(next_left_index) - vy(left_index);
+			const Fixed16 dy_right = vy(next_right_index) - vy(right_index);
+			const Fixed16 next_y = std::min(vy(next_left_index), vy(next_right_index));
+			if(dy_left > FIXED16_HALF && dy_right > FIXED16_HALF)
+			{
+				const Fixed16 dx_dy_left = fixed16_div(vx(next_left_index) - vx(left_index), dy_left);
+				const Fixed16 dx_dy_right = fixed16_div(vx(next_right_index) - vx(right_index), dy_right);
+				draw_part(
+					mode, blend, cur_y, next_y,
+					PolygonSide{vx(left_index) + fixed16_mul(dx_dy_left, cur_y - vy(left_index)), dx_dy_left},
+					PolygonSide{vx(right_index) + fixed16_mul(dx_dy_right, cur_y - vy(right_index)), dx_dy_right}, de,
+					te, tsize, texture_data);
+			}
+			else if(dy_left > 0 && dy_right > 0)
+			{
+				const int32_t cur_y_int = fixed16_round_to_int(cur_y);
+				const int32_t next_y_int = fixed16_round_to_int(next_y);
+				if(cur_y_int < next_y_int)
+				{
+					const Fixed16 thin_line_y = int_to_fixed16(cur_y_int) + FIXED16_HALF;
+					const Fixed16 x_start_left = vx(left_index) +
+						fixed16_mul_div(thin_line_y - vy(left_index), vx(next_left_index) - vx(left_index), dy_left);
+					const Fixed16 x_start_right = vx(right_index) +
+						fixed16_mul_div(thin_line_y - vy(right_index), vx(next_right_index) - vx(right_index), dy_right);
+					draw_part(
+						mode, blend, cur_y, next_y, PolygonSide{x_start_left, 0}, PolygonSide{x_start_right, 0}, de, te,
+						tsize, texture_data);
+				}
+			}
+
+			if(next_left_index == next_right_index)
+				break;
+			if(vy(next_right_index) < vy(next_left_index))
+				right_index = next_right_index;
+			else
+				left_index = next_left_index;
+			cur_y = next_y;
+		}
+	}
+
+	// rasterizer.rs:984-1292
+	void fill_triangle_part(
+		uint32_t blend, Fixed16 y_start, Fixed16 y_end, PolygonSide left_side, PolygonSide right_side,
+		const Fixed16 tc_start_left[2], Fixed16 d_tc_dy_left[2], Fixed16 d_tc_dx[2], const Fixed16 light_start_left[3],
+		Fixed16 d_light_left[3], Fixed16 d_light_dx[3], const int32_t* tsize, const uint32_t* texture_data)
+	{
+		const int32_t y_start_int = std::max(fixed16_round_to_int(y_start), clip_rect.min_y);
+		const int32_t y_end_int = std::min(fixed16_round_to_int(y_end), clip_rect.max_y);
+		if(y_start_int >= y_end_int)
+			return;
+
+		const Fixed16 y_start_delta = int_to_fixed16(y_start_int) + FIXED16_HALF - y_start;
+		Fixed16 x_left = left_side.x_start + fixed16_mul(y_start_delta, left_side.dx_dy) + FIXED16_HALF;
+		Fixed16 x_right = right_side.x_start + fixed16_mul(y_start_delta, right_side.dx_dy) + FIXED16_HALF;
+
+		const Fixed16 y_end_delta = y_end - (y_start + y_start_delta);
+		const Fixed16 d_x_lower = std::max(x_right - x_left, 0);
+		const Fixed16 x_upper_left = x_left + fixed16_mul(y_end_delta, left_side.dx_dy);
+		const Fixed16 x_upper_right = x_right + fixed16_mul(y_end_delta, right_side.dx_dy);
+		const Fixed16 d_x_upper = std::max(x_upper_right - x_upper_left, 0);
+
+		Fixed16 tc_left[2] = {0, 0};
+		for(int i = 0; i < 2; ++i)
+		{
+			const Fixed16 max_tc = int_to_fixed16(tsize[i]) - 1;
+			{
+				tc_left[i] = std::min(std::max(tc_start_left[i] + fixed16_mul(y_start_delta, d_tc_dy_left[i]), 0), max_tc);
+				const Fixed16 tc_lower_delta = fixed16_mul(d_tc_dx[i], d_x_lower);
+				if(tc_lower_delta > max_tc)
+				{
+					tc_left[i] = 0;
+					d_tc_dx[i] = fixed16_div(max_tc, d_x_lower);
+				}
+				else if(tc_lower_delta < -max_tc)
+				{
+					tc_left[i] = max_tc;
+					d_tc_dx[i] = fixed16_div(-max_tc, d_x_lower);
+				}
+				else
+				{
+					const Fixed16 tc_lower_right = tc_left[i] + tc_lower_delta;
+					if(tc_lower_right > max_tc)
+						tc_left[i] -= tc_lower_right - max_tc;
+					else if(tc_lower_right < 0)
+						tc_left[i] -= tc_lower_right;
+				}
+			}
+			{
+				Fixed16 tc_upper_left =
+					std::min(std::max(tc_left[i] + fixed16_mul(y_end_delta, d_tc_dy_left[i]), 0), max_tc);
+				const Fixed16 tc_upper_delta = fixed16_mul(d_tc_dx[i], d_x_upper);
+				if(tc_upper_delta > max_tc)
+				{
+					tc_upper_left = 0;
+					d_tc_dx[i] = fixed16_div(max_tc, d_x_upper);
+				}
+				else if(tc_upper_delta < -max_tc)
+				{
+					tc_upper_left = max_tc;
+					d_tc_dx[i] = fixed16_div(-max_tc, d_x_upper);
+				}
+				else
+				{
+					const Fixed16 tc_upper_right = tc_upper_left + tc_upper_delta;
+					if(tc_upper_right > max_tc)
+						tc_upper_left -= tc_upper_right - max_tc;
+					else if(tc_upper_right < 0)
+						tc_upper_left -= tc_upper_right;
+				}
+				if(y_end_delta > 0)
+					d_tc_dy_left[i] = fixed16_div(tc_upper_left - tc_left[i], y_end_delta);
+				else
+					d_tc_dy_left[i] = 0;
+
+				const Fixed16 tc_upper_left2 = tc_left[i] + fixed16_mul(y_end_delta, d_tc_dy_left[i]);
+				const Fixed16 tc_upper_delta2 = fixed16_mul(d_tc_dx[i], d_x_upper);
+				const Fixed16 tc_upper_right2 = tc_upper_left2 + tc_upper_delta2;
+				if(tc_upper_right2 < 0)
+					d_tc_dx[i] = d_x_upper > 0 ? fixed16_div(-tc_upper_left2, d_x_upper) : 0;
+				else if(tc_upper_right2 > max_tc)
+					d_tc_dx[i] = d_x_upper > 0 ? fixed16_div(max_tc - tc_upper_left2, d_x_upper) : 0;
+			}
+		}
+
+		Fixed16 light_left[3] = {0, 0, 0};
+		for(int i = 0; i < 3; ++i)
+		{
+			light_left[i] = std::max(light_start_left[i] + fixed16_mul(y_start_delta, d_light_left[i]), 0);
+			const Fixed16 light_lower_right = light_left[i] + fixed16_mul(d_x_lower, d_light_dx[i]);
+			if(light_lower_right < 0)
+				light_left[i] -= light_lower_right;
+			{
+				Fixed16 light_upper_left = std::max(light_left[i] + fixed16_mul(y_end_delta, d_light_left[i]), 0);
+				const Fixed16 light_upper_right = light_upper_left + fixed16_mul(d_x_upper, d_light_dx[i]);
+				if(light_upper_right < 0)
+					light_upper_left -= light_upper_right;
+				if(y_end_delta > 0)
+					d_light_left[i] = fixed16_div(light_upper_left - light_left[i], y_end_delta);
+				else
+					d_light_left[i] = 0;
+				const Fixed16 light_upper_left2 = light_left[i] + fixed16_mul(y_end_delta, d_light_left[i]);
+				const Fixed16 light_upper_right2 = light_upper_left2 + fixed16_mul(d_x_upper, d_light_dx[i]);
+				if(light_upper_right2 < 0)
+					d_light_dx[i] = d_x_upper > 0 ? fixed16_div(-light_upper_left2, d_x_upper) : 0;
+			}
+		}
+
+		const __m128i d_light_dx_vec = ci_from_i32x3(d_light_dx);
+		for(int32_t y_int = y_start_int; y_int < y_end_int; ++y_int)
+		{
+			const int32_t x_start_int = std::max(fixed16_floor_to_int(x_left), clip_rect.min_x);
+			const int32_t x_end_int = std::min(fixed16_floor_to_int(x_right), clip_rect.max_x);
+			if(x_start_int < x_end_int)
+			{
+				const Fixed16 x_start_delta = int_to_fixed16(x_start_int) + FIXED16_ONE - x_left;
+				Fixed16 line_tc[2] = {
+					tc_left[0] + fixed16_mul(x_start_delta, d_tc_dx[0]), tc_left[1] + fixed16_mul(x_start_delta, d_tc_dx[1])};
+				const int32_t ll[3] = {
+					light_left[0] + fixed16_mul(x_start_delta, d_light_dx[0]),
+					light_left[1] + fixed16_mul(x_start_delta, d_light_dx[1]),
+					light_left[2] + fixed16_mul(x_start_delta, d_light_dx[2])};
+				__m128i line_light = ci_from_i32x3(ll);
+				typename CT::T* line_dst = color_buffer + int64_t(y_int) * row_size;
+				for(int32_t x = x_start_int; x < x_end_int; ++x)
+				{
+					const int32_t u = fixed16_floor_to_int(line_tc[0]);
+					const int32_t v = fixed16_floor_to_int(line_tc[1]);
+					if(!in_texture(uint32_t(u), uint32_t(v), tsize))
+					{
+						line_tc[0] += d_tc_dx[0];
+						line_tc[1] += d_tc_dx[1];
+						line_light = _mm_add_epi32(line_light, d_light_dx_vec);
+						continue;
+					}
+					const uint32_t texel = texture_data[u + v * tsize[0]];
+					const __m128i lighted = _mm_srai_epi32(_mm_mullo_epi32(ci_from_color32(texel), line_light), 16);
+					typename CT::T& dst = line_dst[x];
+					switch(blend)
+					{
+					case SW_BLEND_NONE: dst = CT::from_veci(lighted); break;
+					case SW_BLEND_AVERAGE: dst = CT::average(dst, CT::from_veci(lighted)); break;
+					case SW_BLEND_ADDITIVE: dst = CT::saturated_sum(dst, CT::from_veci(lighted)); break;
+					case SW_BLEND_ALPHA_TEST:
+						if(C32::test_alpha(texel))
+							dst = CT::from_veci(lighted);
+						break;
+					case SW_BLEND_ALPHA_BLEND:
+					{
+						const int32_t alpha = C32::get_alpha(texel);
+						dst = CT::from_veci(_mm_srai_epi32(
+							_mm_add_epi32(
+								_mm_mullo_epi32(CT::to_veci(dst), _mm_set1_epi32(255 - alpha)),
+								_mm_mullo_epi32(lighted, _mm_set1_epi32(alpha))),
+							8));
+						break;
+					}
+					default: break;
+					}
+					line_tc[0] += d_tc_dx[0];
+					line_tc[1] += d_tc_dx[1];
+					line_light = _mm_add_epi32(line_light, d_light_dx_vec);
+				}
+				count(blend, x_end_int - x_start_int, true);
+			}
+			x_left += left_side.dx_dy;
+			x_right += right_side.dx_dy;
+			tc_left[0] += d_tc_dy_left[0];
+			tc_left[1] += d_tc_dy_left[1];
+			light_left[0] += d_light_left[0];
+			light_left[1] += d_light_left[1];
+			light_left[2] += d_light_left[2];
+		}
+	}
+
+	// rasterizer.rs:765-982
+	void fill_triangle(const int32_t* vdata, uint32_t blend, const int32_t* tsize, const uint32_t* texture_data)
+	{
+		struct TV
+		{
+			Fixed16 x, y, tc[2], light[3];
+		};
+		TV v[3];
+		for(int i = 0; i < 3; ++i)
+		{
+			const int32_t* p = vdata + i * 7;
+			v[i] = TV{p[0], p[1], {p[2], p[3]}, {p[4], p[5], p[6]}};
+		}
+		int upper_index, lower_index;
+		if(v[0].y >= v[1].y && v[0].y >= v[2].y)
+		{
+			upper_index = 0;
+			lower_index = v[1].y < v[2].y ? 1 : 2;
+		}
+		else if(v[1].y >= v[0].y && v[1].y >= v[2].y)
+		{
+			upper_index = 1;
+			lower_index = v[0].y < v[2].y ? 0 : 2;
+		}
+		else
+		{
+			upper_index = 2;
+			lower_index = v[0].y < v[1].y ? 0 : 1;
+		}
+		const int middle_index = 3 - upper_index - lower_index;
+		const TV& up = v[upper_index];
+		const TV& lo = v[lower_index];
+		const TV& mi = v[middle_index];
+
+		const Fixed16 long_edge_dy = std::max(up.y - lo.y, FIXED16_HALF);
+		const Fixed16 lower_part_dy = std::max(mi.y - lo.y, FIXED16_HALF);
+		const Fixed16 upper_part_dy = std::max(up.y - mi.y, FIXED16_HALF);
+		const Fixed16 long_edge_dx_dy = fixed16_div(up.x - lo.x, long_edge_dy);
+		const Fixed16 long_edge_x_in_middle = lo.x + fixed16_mul(long_edge_dx_dy, mi.y - lo.y);
+		const Fixed16 lower_part_dx_dy = fixed16_div(mi.x - lo.x, lower_part_dy);
+		const Fixed16 upper_part_dx_dy = fixed16_div(up.x - mi.x, upper_part_dy);
+
+		Fixed16 long_edge_d_tc_dy[2], long_edge_tc_in_middle[2], d_tc_dy_lower[2], d_tc_dy_upper[2];
+		for(int i = 0; i < 2; ++i)
+		{
+			long_edge_d_tc_dy[i] = fixed16_div(up.tc[i] - lo.tc[i], long_edge_dy);
+			long_edge_tc_in_middle[i] = lo.tc[i] + fixed16_mul(long_edge_d_tc_dy[i], lower_part_dy);
+			d_tc_dy_lower[i] = fixed16_div(mi.tc[i] - lo.tc[i], lower_part_dy);
+			d_tc_dy_upper[i] = fixed16_div(up.tc[i] - mi.tc[i], upper_part_dy);
+		}
+		Fixed16 long_edge_d_light_dy[3], long_edge_light_in_middle[3], d_light_dy_lower[3], d_light_dy_upper[3];
+		for(int i = 0; i < 3; ++i)
+		{
+			long_edge_d_light_dy[i] = fixed16_div(up.light[i] - lo.light[i], long_edge_dy);
+			long_edge_light_in_middle[i] = lo.light[i] + fixed16_mul(long_edge_d_light_dy[i], lower_part_dy);
+			d_light_dy_lower[i] = fixed16_div(mi.light[i] - lo.light[i], lower_part_dy);
+			d_light_dy_upper[i] = fixed16_div(up.light[i] - mi.light[i], upper_part_dy);
+		}
+
+		if(long_edge_x_in_middle >= v[middle_index].x)
+		{
+			const Fixed16 middle_dx = std::max(long_edge_x_in_middle - mi.x, FIXED16_HALF);
+			Fixed16 d_tc_dx[2], d_light_dx[3];
+			for(int i = 0; i < 2; ++i)
+				d_tc_dx[i] = fixed16_div(long_edge_tc_in_middle[i] - mi.tc[i], middle_dx);
+			for(int i = 0; i < 3; ++i)
+				d_light_dx[i] = fixed16_div(long_edge_light_in_middle[i] - mi.light[i], middle_dx);
+			{
+				Fixed16 dtcy[2] = {d_tc_dy_lower[0], d_tc_dy_lower[1]}, dtcx[2] = {d_tc_dx[0], d_tc_dx[1]};
+				Fixed16 dly[3] = {d_light_dy_lower[0], d_light_dy_lower[1], d_light_dy_lower[2]};
+				Fixed16 dlx[3] = {d_light_dx[0], d_light_dx[1], d_light_dx[2]};
+				fill_triangle_part(
+					blend, lo.y, mi.y, PolygonSide{lo.x, lower_part_dx_dy}, PolygonSide{lo.x, long_edge_dx_dy}, lo.tc, dtcy,
+					dtcx, lo.light, dly, dlx, tsize, texture_data);
+			}
+			{
+				Fixed16 dtcy[2] = {d_tc_dy_upper[0], d_tc_dy_upper[1]}, dtcx[2] = {d_tc_dx[0], d_tc_dx[1]};
+				Fixed16 dly[3] = {d_light_dy_upper[0], d_light_dy_upper[1], d_light_dy_upper[2]};
+				Fixed16 dlx[3] = {d_light_dx[0], d_light_dx[1], d_light_dx[2]};
+				fill_triangle_part(
+					blend, mi.y, up.y, PolygonSide{mi.x, upper_part_dx_dy}, PolygonSide{long_edge_x_in_middle, long_edge_dx_dy},
+					mi.tc, dtcy, dtcx, mi.light, dly, dlx, tsize, texture_data);
+			}
+		}
+		else
+		{
+			const Fixed16 middle_dx = std::max(mi.x - long_edge_x_in_middle, FIXED16_HALF);
+			Fixed16 d_tc_dx[2], d_light_dx[3];
+			for(int i = 0; i < 2; ++i)
+				d_tc_dx[i] = fixed16_div(mi.tc[i] - long_edge_tc_in_middle[i], middle_dx);
+			for(int i = 0; i < 3; ++i)
+				d_light_dx[i] = fixed16_div(mi.light[i] - long_edge_light_in_middle[i], middle_dx);
+			{
+				Fixed16 dtcy[2] = {long_edge_d_tc_dy[0], long_edge_d_tc_dy[1]}, dtcx[2] = {d_tc_dx[0], d_tc_dx[1]};
+				Fixed16 dly[3] = {long_edge_d_light_dy[0], long_edge_d_light_dy[1], long_edge_d_light_dy[2]};
+				Fixed16 dlx[3] = {d_light_dx[0], d_light_dx[1], d_light_dx[2]};
+				fill_triangle_part(
+					blend, lo.y, mi.y, PolygonSide{lo.x, long_edge_dx_dy}, PolygonSide{lo.x, lower_part_dx_dy}, lo.tc, dtcy,
+					dtcx, lo.light, dly, dlx, tsize, texture_data);
+			}
+			{
+				Fixed16 dtcy[2] = {long_edge_d_tc_dy[0], long_edge_d_tc_dy[1]}, dtcx[2] = {d_tc_dx[0], d_tc_dx[1]};
+				Fixed16 dly[3] = {long_edge_d_light_dy[0], long_edge_d_light_dy[1], long_edge_d_light_dy[2]};
+				Fixed16 dlx[3] = {d_light_dx[0], d_light_dx[1], d_light_dx[2]};
+				fill_triangle_part(
+					blend, mi.y, up.y, PolygonSide{long_edge_x_in_middle, long_edge_dx_dy}, PolygonSide{mi.x, upper_part_dx_dy},
+					long_edge_tc_in_middle, dtcy, dtcx, long_edge_light_in_middle, dly, dlx, tsize, texture_data);
+			}
+		}
+	}
+};
+
+// ---------------------------------------------------------------------------------------------
+// Frame driver over a batch of views
+// ---------------------------------------------------------------------------------------------
+template <class CT>
+void build_view_surfaces(
+	const Numerics& nm, const MapData& m, const sw_batch& b, const sw_view& view, typename CT::T* arena,
+	uint32_t first, uint32_t end, Counters* counters)
+{
+	const Vec3 cam = v3(view.cam_pos[0], view.cam_pos[1], view.cam_pos[2]);
+	for(uint32_t si = first; si < end; ++si)
+	{
+		const sw_surface_desc& s = b.surfaces[si];
+		typename CT::T* out = arena + view.surface_arena_offset + s.pixels_offset;
+		if(s.flags & SW_SURF_DIFFUSE_LAYER)
+			build_surface<CT>(nm, m, s, b.lights, b.light_indices, cam, out, counters);
+		if((s.flags & SW_SURF_EMISSIVE) && s.emissive_level != SW_NO_TEXTURE)
+			mix_surface_with_texture<CT>(m, s, s.emissive_blend, out, counters);
+	}
+}
+
+template <class CT>
+void raster_view(
+	const MapData& m, const sw_batch& b, const sw_view& view, const typename CT::T* arena, typename CT::T* fb,
+	const ClipRect* override_clip, Counters* counters)
+{
+	Rasterizer<CT> r;
+	r.color_buffer = fb;
+	r.row_size = int32_t(view.width);
+	r.counters = counters;
+	for(uint32_t di = view.first_draw; di < view.first_draw + view.num_draws; ++di)
+	{
+		const sw_draw& d = b.draws[di];
+		r.clip_rect = override_clip ? *override_clip : ClipRect{d.clip_rect[0], d.clip_rect[1], d.clip_rect[2], d.clip_rect[3]};
+		const int32_t* vd = b.vertex_data + d.first_vertex_int;
+		if(d.kind == SW_DRAW_POLYGON)
+		{
+			if(d.num_vertices < 3)
+				continue;
+			const typename CT::T* tex = d.texture_source == SW_TEXSRC_SKYBOX ?
+				static_cast<const typename CT::T*>(m.skybox) + d.texture_offset :
+				arena + view.surface_arena_offset + d.texture_offset;
+			r.fill_polygon(vd, d.num_vertices, d.mode, d.blend, d.depth_equation, d.tc_equation, d.texture_size, tex);
+		}
+		else
+			r.fill_triangle(vd, d.blend, d.texture_size, m.lite_texels + d.texture_offset);
+	}
+}
+
+// rect_splitting.rs:29-78
+struct RectF
+{
+	float min_x, min_y, max_x, max_y;
+};
+void split_rect_r(const RectF& r, uint32_t num_parts, std::vector<RectF>& out)
+{
+	if(num_parts <= 1)
+	{
+		out.push_back(r);
+		return;
+	}
+	const uint32_t n0 = num_parts / 2, n1 = num_parts - n0;
+	const float ratio0 = float(n0) / float(num_parts), ratio1 = float(n1) / float(num_parts);
+	RectF r0, r1;
+	if(r.max_x - r.min_x >= r.max_y - r.min_y)
+	{
+		const float middle = r.max_x * ratio0 + r.min_x * ratio1;
+		r0 = RectF{r.min_x, r.min_y, middle, r.max_y};
+		r1 = RectF{middle, r.min_y, r.max_x, r.max_y};
+	}
+	else
+	{
+		const float middle = r.max_y * ratio0 + r.min_y * ratio1;
+		r0 = RectF{r.min_x, r.min_y, r.max_x, middle};
+		r1 = RectF{r.min_x, middle, r.max_x, r.max_y};
+	}
+	split_rect_r(r0, n0, out);
+	split_rect_r(r1, n1, out);
+}
+
+template <class CT>
+int render_batch(
+	int mode, const sw_map_desc* map, const sw_batch* b, void* arena_v, void* fb_v, uint32_t num_threads,
+	sw_stats* stats, uint64_t* out_of_bounds)
+{
+	Numerics nm{mode};
+	MapData m;
+	m.texels = map->texels;
+	m.lite_texels = map->lite_texels;
+	m.lightmaps_simple = map->lightmaps_simple;
+	m.lightmaps_directional = map->lightmaps_directional;
+	m.levels = map->levels;
+	m.skybox = sizeof(typename CT::T) == 4 ? static_cast<const void*>(map->skybox_texels32) :
+											static_cast<const void*>(map->skybox_texels64);
+	typename CT::T* arena = static_cast<typename CT::T*>(arena_v);
+	typename CT::T* fb = static_cast<typename CT::T*>(fb_v);
+	Counters counters;
+	Counters* cptr = &counters;
+
+	for(uint32_t vi = 0; vi < b->num_views; ++vi)
+	{
+		const sw_view& view = b->views[vi];
+		typename CT::T* vfb = fb + view.fb_offset;
+		if(num_threads <= 1)
+		{
+			build_view_surfaces<CT>(nm, m, *b, view, arena, view.first_surface, view.first_surface + view.num_surfaces, cptr);
+			if(view.clear_background)
+				std::fill(vfb, vfb + uint64_t(view.width) * view.height, typename CT::T(view.background_color));
+			raster_view<CT>(m, *b, view, arena, vfb, nullptr, cptr);
+		}
+		else
+		{
+			// Reference decomposition: surfaces in parallel (partial_renderer.rs:1822-1831), screen split
+			// into num_threads rects each rasterised with its own clip rect (:1235-1282).
+			std::atomic<uint32_t> next{view.first_surface};
+			const uint32_t end = view.first_surface + view.num_surfaces;
+			std::vector<std::thread> pool;
+			for(uint32_t t = 0; t < num_threads; ++t)
+				pool.emplace_back([&]() {
+					for(;;)
+					{
+						const uint32_t i = next.fetch_add(4);
+						if(i >= end)
+							break;
+						build_view_surfaces<CT>(nm, m, *b, view, arena, i, std::min(i + 4, end), cptr);
+					}
+				});
+			for(auto& t : pool)
+				t.join();
+			pool.clear();
+			if(view.clear_background)
+				std::fill(vfb, vfb + uint64_t(view.width) * view.height, typename CT::T(view.background_color));
+			std::vector<RectF> rects;
+			split_rect_r(RectF{0.0f, 0.0f, float(view.width), float(view.height)}, std::min<uint32_t>(num_threads, 64), rects);
+			for(const RectF& rf : rects)
+				pool.emplace_back([&, rf]() {
+					const ClipRect c{int32_t(rf.min_x), int32_t(rf.min_y), int32_t(rf.max_x), int32_t(rf.max_y)};
+					raster_view<CT>(m, *b, view, arena, vfb, &c, cptr);
+				});
+			for(auto& t : pool)
+				t.join();
+		}
+	}
+	if(stats)
+	{
+		stats->surface_texels = counters.surface_texels;
+		stats->lightmap_elements = counters.lightmap_elements;
+		stats->emissive_texels = counters.emissive_texels;
+		stats->polygon_pixels = counters.polygon_pixels;
+		stats->triangle_pixels = counters.triangle_pixels;
+		stats->blend_pixels = counters.blend_pixels;
+	}
+	if(out_of_bounds)
+		*out_of_bounds = counters.out_of_bounds;
+	return 0;
+}
+
+// postprocessor.rs:226-256 + :1003-1024
+void tonemap_rows(
+	const uint64_t* hdr, uint32_t width, uint32_t y0, uint32_t y1, float exposure, const float modulate[3], uint32_t* dst,
+	uint32_t dst_pitch, __m128* sum_out)
+{
+	const float inv_255 = 1.0f / 255.0f;
+	const float inv_scale[3] = {1.0f / (exposure * modulate[0]), 1.0f / (exposure * modulate[1]), 1.0f / (exposure * modulate[2])};
+	const float i255[3] = {inv_255, inv_255, inv_255};
+	const __m128 inv_scale_vec = cv_from_f32x3(inv_scale);
+	const __m128 inv_255_vec = cv_from_f32x3(i255);
+	__m128 colors_sum = _mm_setzero_ps();
+	for(uint32_t y = y0; y < y1; ++y)
+	{
+		__m128 line_sum = _mm_setzero_ps();
+		for(uint32_t x = 0; x < width; ++x)
+		{
+			const __m128 c = cv_from_color64(hdr[uint64_t(y) * width + x]);
+			line_sum = _mm_add_ps(line_sum, c);
+			const __m128 mapped = _mm_div_ps(c, _mm_fmadd_ps(c, inv_255_vec, inv_scale_vec));
+			dst[uint64_t(y) * dst_pitch + x] = cv_into_color32(mapped);
+		}
+		colors_sum = _mm_add_ps(colors_sum, line_sum);
+	}
+	*sum_out = colors_sum;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------------------------
+// C ABI of the oracle (loaded with ctypes by the tests and bench.py)
+// ---------------------------------------------------------------------------------------------
+extern "C" {
+
+int oracle_render_batch(
+	int mode, const sw_map_desc* map, const sw_batch* batch, void* arena, void* framebuffer, uint32_t num_threads,
+	sw_stats* stats, uint64_t* out_of_bounds)
+{
+	if(!map || !batch || !arena || !framebuffer)
+		return -1;
+	if(batch->color_bits == 32)
+		return render_batch<C32>(mode, map, batch, arena, framebuffer, num_threads, stats, out_of_bounds);
+	if(batch->color_bits == 64)
+		return render_batch<C64>(mode, map, batch, arena, framebuffer, num_threads, stats, out_of_bounds);
+	return -1;
+}
+
+// Surfaces only (arena output), for the surface-builder parity tests.
+int oracle_build_surfaces(int mode, const sw_map_desc* map, const sw_batch* b, void* arena_v, sw_stats* stats)
+{
+	Numerics nm{mode};
+	MapData m{map->texels, map->lite_texels, map->lightmaps_simple, map->lightmaps_directional, map->levels, nullptr};
+	Counters counters;
+	for(uint32_t vi = 0; vi < b->num_views; ++vi)
+	{
+		const sw_view& view = b->views[vi];
+		if(b->color_bits == 32)
+			build_view_surfaces<C32>(
+				nm, m, *b, view, static_cast<uint32_t*>(arena_v), view.first_surface, view.first_surface + view.num_surfaces,
+				&counters);
+		else
+			build_view_surfaces<C64>(
+				nm, m, *b, view, static_cast<uint64_t*>(arena_v), view.first_surface, view.first_surface + view.num_surfaces,
+				&counters);
+	}
+	if(stats)
+	{
+		std::memset(stats, 0, sizeof(*stats));
+		stats->surface_texels = counters.surface_texels;
+		stats->lightmap_elements = counters.lightmap_elements;
+		stats->emissive_texels = counters.emissive_texels;
+	}
+	return 0;
+}
+
+int oracle_tonemap(
+	const uint64_t* hdr, uint32_t width, uint32_t height, float exposure, const float modulate[3], uint32_t* dst,
+	uint32_t dst_pitch_pixels, float out_sum[3])
+{
+	__m128 s;
+	tonemap_rows(hdr, width, 0, height, exposure, modulate, dst, dst_pitch_pixels, &s);
+	alignas(16) float v[4];
+	_mm_store_ps(v, s);
+	// lanes [B, G, R, A] -> RGB
+	out_sum[0] = v[2];
+	out_sum[1] = v[1];
+	out_sum[2] = v[0];
+	return 0;
+}
+
+// Elementary operations, exposed for the closed-form / exhaustive tests.
+uint32_t oracle_color32_from_f32x4(const float bgra[4])
+{
+	return cv_into_color32(_mm_set_ps(bgra[3], bgra[2], bgra[1], bgra[0]));
+}
+uint64_t oracle_color64_from_f32x4(const float bgra[4])
+{
+	return cv_into_color64(_mm_set_ps(bgra[3], bgra[2], bgra[1], bgra[0]));
+}
+uint32_t oracle_blend32(uint32_t blend, uint32_t dst, uint32_t src)
+{
+	write_into_framebuffer<C32>(blend, dst, src);
+	return dst;
+}
+uint64_t oracle_blend64(uint32_t blend, uint64_t dst, uint64_t src)
+{
+	write_into_framebuffer<C64>(blend, dst, src);
+	return dst;
+}
+int32_t oracle_pack_normal_roughness(const float n[3], float roughness)
+{
+	return pack_normal_roughness(v3(n[0], n[1], n[2]), roughness);
+}
+void oracle_unpack_normal_roughness(int32_t packed, float out[4])
+{
+	Vec3 n;
+	float r;
+	unpack_normal_roughness(packed, n, r);
+	out[0] = n.x;
+	out[1] = n.y;
+	out[2] = n.z;
+	out[3] = r;
+}
+int32_t oracle_fixed16_mul(int32_t a, int32_t b) { return fixed16_mul(a, b); }
+int32_t oracle_fixed16_div(int32_t a, int32_t b) { return fixed16_div(a, b); }
+int32_t oracle_fixed16_mul_div(int32_t a, int32_t b, int32_t c) { return fixed16_mul_div(a, b, c); }
+float oracle_specular_intensity(int mode, float angle_cos, float inv_roughness)
+{
+	return get_specular_intensity(Numerics{mode}, angle_cos, inv_roughness);
+}
+float oracle_fresnel_factor_base(float c) { return get_fresnel_factor_base(c); }
+float oracle_specular_k_dielectric(float f, float r) { return get_specular_k_dielectric(f, r); }
+float oracle_specular_k_metal(float f, float r) { return get_specular_k_metal(f, r); }
+
+// Mip chain of a Texture (textures.rs:288-362): src is mip 0 (w x h), out receives mips 1..4
+// back to back. Returns the number of texels written.
+uint64_t oracle_build_texture_mips(const sw_texel* src, uint32_t w, uint32_t h, sw_texel* out)
+{
+	std::vector<sw_texel> prev(src, src + uint64_t(w) * h);
+	uint32_t pw = w, ph = h;
+	uint64_t written = 0;
+	for(int mip = 1; mip <= 4; ++mip)
+	{
+		const uint32_t mw = pw >> 1, mh = ph >> 1;
+		std::vector<sw_texel> cur(uint64_t(mw) * mh);
+		for(uint32_t y = 0; y < mh; ++y)
+			for(uint32_t x = 0; x < mw; ++x)
+			{
+				const sw_texel p00 = prev[(y * 2) * pw + x * 2];
+				const sw_texel p01 = prev[(y * 2 + 1) * pw + x * 2];
+				const sw_texel p10 = prev[(y * 2) * pw + x * 2 + 1];
+				const sw_texel p11 = prev[(y * 2 + 1) * pw + x * 2 + 1];
+				// color.rs:51-71
+				uint32_t sum[4] = {0, 0, 0, 0};
+				for(uint32_t c : {p00.diffuse, p01.diffuse, p10.diffuse, p11.diffuse})
+				{
+					sum[0] += c & 0x000000FFu;
+					sum[1] += c & 0x0000FF00u;
+					sum[2] += c & 0x00FF0000u;
+					sum[3] += (c & 0xFF000000u) >> 2;
+				}
+				sw_texel d;
+				d.diffuse = (sum[0] >> 2) | ((sum[1] >> 2) & 0x0000FF00u) | ((sum[2] >> 2) & 0x00FF0000u) | (sum[3] & 0xFF000000u);
+				Vec3 n[4];
+				float r[4];
+				unpack_normal_roughness(p00.packed_normal_roughness, n[0], r[0]);
+				unpack_normal_roughness(p01.packed_normal_roughness, n[1], r[1]);
+				unpack_normal_roughness(p10.packed_normal_roughness, n[2], r[2]);
+				unpack_normal_roughness(p11.packed_normal_roughness, n[3], r[3]);
+				const Vec3 ns = ((n[0] + n[1]) + n[2]) + n[3];
+				const float len = fmax_r(std::sqrt((ns.x * ns.x + ns.y * ns.y) + ns.z * ns.z), 0.000001f);
+				const Vec3 dn = v3(ns.x / len, ns.y / len, ns.z / len);
+				const float half_cos = len * 0.25f;
+				const float deviation = (1.0f - half_cos) / (1.0f - 0.5f);
+				const float rs = ((r[0] + r[1]) + r[2]) + r[3];
+				const float min_valid = 1.0f / (127.0f - 1.0f);
+				const float dr = fmin_r(fmax_r(f32_mul_add(rs, 0.25f, deviation), min_valid), 1.0f);
+				d.packed_normal_roughness = pack_normal_roughness(dn, dr);
+				cur[uint64_t(y) * mw + x] = d;
+			}
+		std::memcpy(out + written, cur.data(), cur.size() * sizeof(sw_texel));
+		written += cur.size();
+		prev.swap(cur);
+		pw = mw;
+		ph = mh;
+	}
+	return written;
+}
+
+} // extern "C"

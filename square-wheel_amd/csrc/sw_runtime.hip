@@ -1,0 +1,1092 @@
+// Host runtime of the C ABI declared in include/sw_gpu.h.
+//
+// Device memory model: the map (textures with their mip chains, lite textures, both lightmap
+// kinds, skybox sides) is uploaded once and stays in HBM. Per batch the context owns grow-only
+// device buffers for the command buffers, the surface arena, span/part records and band lists,
+// so steady-state frames allocate nothing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "sw_internal.h"
+
+namespace
+{
+
+__global__ void k_reset_raster_counters(uint32_t* counters)
+{
+	if(threadIdx.x == 0)
+	{
+		for(int i = COUNTER_SPANS_LO; i <= COUNTER_PARTS; ++i)
+			counters[i] = 0;
+		counters[COUNTER_ERROR] &= ~uint32_t(ERROR_SPAN_OVERFLOW);
+	}
+}
+
+__global__ void k_fill_color(void* fb, uint64_t n, uint64_t value, uint32_t bytes)
+{
+	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if(i >= n)
+		return;
+	if(bytes == 8)
+		static_cast<uint64_t*>(fb)[i] = value;
+	else
+		static_cast<uint32_t*>(fb)[i] = uint32_t(value);
+}
+
+struct DevBuf
+{
+	void* p = nullptr;
+	size_t cap = 0;
+};
+
+} // namespace
+
+struct sw_map
+{
+	DevBuf texels, lite, lm_simple, lm_dir, levels, sky32, sky64;
+	MapPtrs ptrs;
+	uint64_t num_sky32 = 0, num_sky64 = 0;
+	std::vector<sw_texture_level> host_levels;
+};
+
+struct sw_ctx
+{
+	int device = 0;
+	hipStream_t stream = nullptr;
+	std::string error;
+	uint64_t device_bytes = 0;
+	uint64_t alloc_epoch = 0; // bumped whenever a device buffer moves (invalidates captured graphs)
+	uint32_t last_color_bits = 32; // colour size of the arena's current contents
+
+	// command buffers / work buffers
+	DevBuf views, surfaces, lights, light_indices, draws, vdata, surf_end, parts, spans, drec, band_lists, band_counts,
+		counters, arena, fb, tonemap_out, tonemap_sums;
+	uint64_t span_capacity = 1u << 20;
+
+	// plain-path frame state
+	sw_map* frame_map = nullptr;
+	uint32_t frame_color_bits = 32;
+	sw_view frame_view{};
+	uint64_t frame_arena_texels = 0;
+	bool frame_active = false;
+	bool frame_rastered = false;
+	uint32_t frame_num_surfaces = 0, frame_num_lights = 0, frame_num_light_indices = 0;
+	uint64_t frame_num_vertex_ints = 0;
+
+	// last batch launch description (for re-runs after span overflow and graph capture)
+	struct Launch
+	{
+		sw_map* map = nullptr;
+		uint32_t color_bits = 32;
+		uint32_t num_views = 0;
+		const sw_surface_desc* surfaces = nullptr;
+		uint32_t num_surfaces = 0;
+		const sw_light* lights = nullptr;
+		uint32_t num_lights = 0;
+		const uint32_t* light_indices = nullptr;
+		uint32_t num_light_indices = 0;
+		const sw_draw* draws = nullptr;
+		uint32_t num_draws = 0;
+		const int32_t* vdata = nullptr;
+		uint64_t num_vertex_ints = 0;
+		uint64_t arena_texels = 0;
+		void* arena = nullptr;
+		void* fb = nullptr;
+		uint32_t total_bands = 0, total_tiles = 0;
+		uint64_t band_list_entries = 0;
+	} last;
+
+	// graph replay
+	hipGraphExec_t graph_exec = nullptr;
+	hipGraph_t graph = nullptr;
+	uint64_t graph_key = 0;
+	uint64_t graph_epoch = 0;
+
+	sw_stats stats{};
+
+	// kernel timing
+	bool profiling = false;
+	bool capturing = false;
+	struct Pending
+	{
+		int stage;
+		hipEvent_t a, b;
+	};
+	std::vector<Pending> pending;
+	std::vector<hipEvent_t> event_pool;
+	double prof_ms[SW_PROFILE_STAGES] = {};
+	uint64_t prof_count[SW_PROFILE_STAGES] = {};
+};
+
+namespace
+{
+
+int set_error(sw_ctx* ctx, int code, const std::string& msg)
+{
+	if(ctx)
+		ctx->error = msg;
+	return code;
+}
+
+int hip_check(sw_ctx* ctx, hipError_t e, const char* what)
+{
+	if(e == hipSuccess)
+		return SW_OK;
+	return set_error(ctx, SW_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int ensure(sw_ctx* ctx, DevBuf& b, size_t bytes)
+{
+	if(bytes <= b.cap && b.p)
+		return SW_OK;
+	size_t want = std::max<size_t>(bytes + bytes / 4, 256);
+	if(b.p)
+	{
+		(void)hipStreamSynchronize(ctx->stream);
+		(void)hipFree(b.p);
+		ctx->device_bytes -= b.cap;
+		b.p = nullptr;
+		b.cap = 0;
+	}
+	void* p = nullptr;
+	hipError_t e = hipMalloc(&p, want);
+	if(e != hipSuccess)
+	{
+		(void)hipGetLastError();
+		want = std::max<size_t>(bytes, 256);
+		e = hipMalloc(&p, want);
+		if(e != hipSuccess)
+		{
+			(void)hipGetLastError();
+			return set_error(ctx, SW_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(want) + " bytes failed");
+		}
+	}
+	b.p = p;
+	b.cap = want;
+	ctx->device_bytes += want;
+	ctx->alloc_epoch++;
+	return SW_OK;
+}
+
+void release(sw_ctx* ctx, DevBuf& b)
+{
+	if(b.p)
+	{
+		(void)hipFree(b.p);
+		if(ctx)
+			ctx->device_bytes -= b.cap;
+	}
+	b.p = nullptr;
+	b.cap = 0;
+}
+
+int upload(sw_ctx* ctx, DevBuf& b, const void* src, size_t bytes)
+{
+	int r = ensure(ctx, b, bytes);
+	if(r)
+		return r;
+	if(bytes == 0)
+		return SW_OK;
+	return hip_check(ctx, hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync H2D");
+}
+
+uint64_t color_bytes(uint32_t bits) { return bits == 64 ? 8 : 4; }
+
+enum
+{
+	STAGE_PREP = 0,
+	STAGE_SURF = 1,
+	STAGE_SETUP = 2,
+	STAGE_BIN = 3,
+	STAGE_RASTER = 4,
+	STAGE_TONEMAP = 5,
+};
+
+hipEvent_t take_event(sw_ctx* ctx)
+{
+	if(!ctx->event_pool.empty())
+	{
+		hipEvent_t e = ctx->event_pool.back();
+		ctx->event_pool.pop_back();
+		return e;
+	}
+	hipEvent_t e = nullptr;
+	if(hipEventCreate(&e) != hipSuccess)
+	{
+		(void)hipGetLastError();
+		return nullptr;
+	}
+	return e;
+}
+
+// Brackets one launch with events on the context's stream when profiling is on.
+struct StageTimer
+{
+	sw_ctx* ctx;
+	int stage;
+	hipEvent_t a = nullptr;
+	StageTimer(sw_ctx* c, int s) : ctx(c), stage(s)
+	{
+		if(ctx->profiling && !ctx->capturing)
+		{
+			a = take_event(ctx);
+			if(a)
+				(void)hipEventRecord(a, ctx->stream);
+		}
+	}
+	~StageTimer()
+	{
+		if(!a)
+			return;
+		hipEvent_t b = take_event(ctx);
+		if(!b)
+		{
+			ctx->event_pool.push_back(a);
+			return;
+		}
+		(void)hipEventRecord(b, ctx->stream);
+		ctx->pending.push_back(sw_ctx::Pending{stage, a, b});
+	}
+};
+
+// Accumulates the elapsed times of completed launches (call after a stream synchronisation).
+void collect_timings(sw_ctx* ctx)
+{
+	for(const sw_ctx::Pending& p : ctx->pending)
+	{
+		float ms = 0.0f;
+		if(hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess)
+		{
+			ctx->prof_ms[p.stage] += ms;
+			ctx->prof_count[p.stage]++;
+		}
+		else
+			(void)hipGetLastError();
+		ctx->event_pool.push_back(p.a);
+		ctx->event_pool.push_back(p.b);
+	}
+	ctx->pending.clear();
+}
+
+// Per-view tiling of a batch (views are always host memory).
+int build_view_ext(sw_ctx* ctx, const sw_view* views, uint32_t n, std::vector<ViewExt>& out, uint32_t& total_bands,
+	uint32_t& total_tiles, uint64_t& band_list_entries)
+{
+	out.resize(n);
+	uint64_t bands = 0, tiles = 0, entries = 0;
+	uint32_t next_surface = 0, next_draw = 0;
+	for(uint32_t i = 0; i < n; ++i)
+	{
+		const sw_view& v = views[i];
+		if(v.width == 0 || v.height == 0 || v.width > 16384 || v.height > 16384)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "view " + std::to_string(i) + ": size must be in [1, 16384]");
+		if(v.first_surface != next_surface || v.first_draw != next_draw)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT,
+				"view " + std::to_string(i) + ": surfaces and draws of views must be consecutive ranges in view order");
+		next_surface += v.num_surfaces;
+		next_draw += v.num_draws;
+		ViewExt& e = out[i];
+		e.v = v;
+		e.first_band = uint32_t(bands);
+		e.num_bands = (v.height + BAND_H - 1) / BAND_H;
+		e.cols = (v.width + 63) / 64;
+		e.first_tile = uint32_t(tiles);
+		e.band_list_base = entries;
+		bands += e.num_bands;
+		tiles += uint64_t(e.num_bands) * e.cols;
+		entries += uint64_t(e.num_bands) * v.num_draws;
+	}
+	if(bands >= (1ull << 31) || tiles >= (1ull << 31))
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "batch too large");
+	total_bands = uint32_t(bands);
+	total_tiles = uint32_t(tiles);
+	band_list_entries = entries;
+	return SW_OK;
+}
+
+int validate_views_fit(sw_ctx* ctx, const std::vector<ViewExt>& ve, uint64_t fb_texels, uint32_t num_surfaces, uint32_t num_draws)
+{
+	uint64_t ns = 0, nd = 0;
+	for(size_t i = 0; i < ve.size(); ++i)
+	{
+		const sw_view& v = ve[i].v;
+		if(v.fb_offset + uint64_t(v.width) * v.height > fb_texels)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "view " + std::to_string(i) + ": framebuffer range outside the output");
+		ns += v.num_surfaces;
+		nd += v.num_draws;
+	}
+	if(ns != num_surfaces || nd != num_draws)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "views do not cover the batch's surfaces and draws exactly");
+	return SW_OK;
+}
+
+// Raster stages of a batch: reset counters, draw setup, band binning, tiles.
+int launch_raster_stages(sw_ctx* ctx, const sw_ctx::Launch& L, const ViewExt* views_dev)
+{
+	sw_map* map = L.map;
+	int r;
+	if((r = ensure(ctx, ctx->parts, sizeof(PartRec) * uint64_t(std::max<uint32_t>(L.num_draws, 1)) * MAX_PARTS_PER_DRAW)))
+		return r;
+	if((r = ensure(ctx, ctx->spans, sizeof(SpanRec) * ctx->span_capacity)))
+		return r;
+	if((r = ensure(ctx, ctx->drec, sizeof(DrawRec) * std::max<uint32_t>(L.num_draws, 1))))
+		return r;
+	if((r = ensure(ctx, ctx->band_lists, sizeof(uint32_t) * std::max<uint64_t>(L.band_list_entries, 1))))
+		return r;
+	if((r = ensure(ctx, ctx->band_counts, sizeof(uint32_t) * std::max<uint32_t>(L.total_bands, 1))))
+		return r;
+	uint32_t* counters = static_cast<uint32_t*>(ctx->counters.p);
+	hipLaunchKernelGGL(k_reset_raster_counters, dim3(1), dim3(64), 0, ctx->stream, counters);
+	const uint64_t sky_texels = L.color_bits == 64 ? map->num_sky64 : map->num_sky32;
+	{
+		StageTimer t(ctx, STAGE_SETUP);
+		swk::launch_setup_draws(ctx->stream, L.draws, L.num_draws, L.vdata, L.num_vertex_ints, views_dev, L.num_views,
+			static_cast<PartRec*>(ctx->parts.p), static_cast<SpanRec*>(ctx->spans.p), static_cast<DrawRec*>(ctx->drec.p),
+			counters, ctx->span_capacity, L.arena_texels, sky_texels, map->ptrs.num_lite);
+	}
+	{
+		StageTimer t(ctx, STAGE_BIN);
+		swk::launch_bin_bands(ctx->stream, static_cast<DrawRec*>(ctx->drec.p), views_dev, L.num_views, L.total_bands,
+			static_cast<uint32_t*>(ctx->band_lists.p), static_cast<uint32_t*>(ctx->band_counts.p), counters);
+	}
+	{
+		StageTimer t(ctx, STAGE_RASTER);
+		swk::launch_raster_tiles(ctx->stream, L.color_bits, map->ptrs, L.draws, static_cast<DrawRec*>(ctx->drec.p),
+			static_cast<PartRec*>(ctx->parts.p), static_cast<SpanRec*>(ctx->spans.p), views_dev, L.num_views,
+			static_cast<uint32_t*>(ctx->band_lists.p), static_cast<uint32_t*>(ctx->band_counts.p), L.arena,
+			L.color_bits == 64 ? map->sky64.p : map->sky32.p, L.fb, L.total_tiles, counters);
+	}
+	return hip_check(ctx, hipGetLastError(), "raster launch");
+}
+
+int launch_surface_stages(sw_ctx* ctx, const sw_ctx::Launch& L, const ViewExt* views_dev)
+{
+	int r;
+	if((r = ensure(ctx, ctx->surf_end, sizeof(uint64_t) * std::max<uint32_t>(L.num_surfaces, 1))))
+		return r;
+	uint32_t* counters = static_cast<uint32_t*>(ctx->counters.p);
+	{
+		StageTimer t(ctx, STAGE_PREP);
+		swk::launch_prep_surfaces(ctx->stream, L.surfaces, L.num_surfaces, views_dev, L.num_views,
+			static_cast<uint64_t*>(ctx->surf_end.p), L.arena_texels, L.map->ptrs, counters);
+	}
+	{
+		StageTimer t(ctx, STAGE_SURF);
+		swk::launch_build_surfaces(ctx->stream, L.color_bits, L.map->ptrs, L.surfaces, L.num_surfaces,
+			static_cast<uint64_t*>(ctx->surf_end.p), views_dev, L.num_views, L.lights, L.num_lights, L.light_indices,
+			L.num_light_indices, L.arena, L.arena_texels, counters);
+	}
+	return hip_check(ctx, hipGetLastError(), "surface launch");
+}
+
+// Waits for the batch, reads the counters; re-runs the raster stages if span storage overflowed.
+int finish(sw_ctx* ctx, const ViewExt* views_dev)
+{
+	for(int attempt = 0; attempt < 4; ++attempt)
+	{
+		uint32_t host_counters[COUNTER_COUNT];
+		int r = hip_check(ctx,
+			hipMemcpyAsync(host_counters, ctx->counters.p, sizeof(host_counters), hipMemcpyDeviceToHost, ctx->stream),
+			"hipMemcpyAsync counters");
+		if(r)
+			return r;
+		if((r = hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize")))
+			return r;
+		collect_timings(ctx);
+		auto u64 = [&](int lo) { return uint64_t(host_counters[lo]) | (uint64_t(host_counters[lo + 1]) << 32); };
+		if(host_counters[COUNTER_ERROR] & ERROR_BAD_SURFACE)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT,
+				"invalid surface descriptor (arena order/overlap, texture level, or lightmap range out of bounds)");
+		const uint64_t needed = u64(COUNTER_SPANS_LO);
+		if(host_counters[COUNTER_ERROR] & ERROR_SPAN_OVERFLOW)
+		{
+			if(needed >= (1ull << 31))
+				return set_error(ctx, SW_ERR_BUDGET, "span storage above 2^31 records");
+			ctx->span_capacity = std::max<uint64_t>(needed + needed / 4, ctx->span_capacity * 2);
+			if((r = launch_raster_stages(ctx, ctx->last, views_dev)))
+				return r;
+			continue;
+		}
+		ctx->stats.polygon_pixels = u64(COUNTER_POLY_PIXELS_LO);
+		ctx->stats.triangle_pixels = u64(COUNTER_TRI_PIXELS_LO);
+		ctx->stats.blend_pixels = u64(COUNTER_BLEND_PIXELS_LO);
+		ctx->stats.spans = needed;
+		ctx->stats.parts = host_counters[COUNTER_PARTS];
+		return SW_OK;
+	}
+	return set_error(ctx, SW_ERR_BUDGET, "span storage overflow persisted");
+}
+
+void fill_surface_stats(sw_ctx* ctx, const sw_surface_desc* s, uint32_t n, bool host)
+{
+	ctx->stats.surface_texels = 0;
+	ctx->stats.lightmap_elements = 0;
+	ctx->stats.emissive_texels = 0;
+	if(!host)
+		return;
+	for(uint32_t i = 0; i < n; ++i)
+	{
+		const uint64_t t = uint64_t(s[i].size[0]) * s[i].size[1];
+		if(s[i].flags & SW_SURF_DIFFUSE_LAYER)
+		{
+			ctx->stats.surface_texels += t;
+			if(s[i].lightmap_offset)
+				ctx->stats.lightmap_elements += 2ull * s[i].lightmap_size[0] * s[i].size[1];
+		}
+		if(s[i].flags & SW_SURF_EMISSIVE)
+			ctx->stats.emissive_texels += t;
+	}
+}
+
+} // namespace
+
+// =============================================================================================
+// C ABI
+// =============================================================================================
+extern "C" {
+
+int sw_ctx_create(int device, sw_ctx** out_ctx)
+{
+	if(!out_ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	*out_ctx = nullptr;
+	int count = 0;
+	if(hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+	{
+		(void)hipGetLastError();
+		return SW_ERR_DEVICE;
+	}
+	sw_ctx* ctx = new(std::nothrow) sw_ctx();
+	if(!ctx)
+		return SW_ERR_OUT_OF_MEMORY;
+	ctx->device = device;
+	if(hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+	{
+		(void)hipGetLastError();
+		delete ctx;
+		return SW_ERR_DEVICE;
+	}
+	if(ensure(ctx, ctx->counters, sizeof(uint32_t) * COUNTER_COUNT) ||
+		hipMemsetAsync(ctx->counters.p, 0, sizeof(uint32_t) * COUNTER_COUNT, ctx->stream) != hipSuccess ||
+		hipStreamSynchronize(ctx->stream) != hipSuccess)
+	{
+		sw_ctx_destroy(ctx);
+		return SW_ERR_DEVICE;
+	}
+	*out_ctx = ctx;
+	return SW_OK;
+}
+
+void sw_ctx_destroy(sw_ctx* ctx)
+{
+	if(!ctx)
+		return;
+	(void)hipSetDevice(ctx->device);
+	if(ctx->stream)
+		(void)hipStreamSynchronize(ctx->stream);
+	if(ctx->graph_exec)
+		(void)hipGraphExecDestroy(ctx->graph_exec);
+	if(ctx->graph)
+		(void)hipGraphDestroy(ctx->graph);
+	DevBuf* bufs[] = {&ctx->views, &ctx->surfaces, &ctx->lights, &ctx->light_indices, &ctx->draws, &ctx->vdata, &ctx->surf_end,
+		&ctx->parts, &ctx->spans, &ctx->drec, &ctx->band_lists, &ctx->band_counts, &ctx->counters, &ctx->arena, &ctx->fb,
+		&ctx->tonemap_out, &ctx->tonemap_sums};
+	for(DevBuf* b : bufs)
+		release(ctx, *b);
+	collect_timings(ctx);
+	for(hipEvent_t e : ctx->event_pool)
+		(void)hipEventDestroy(e);
+	if(ctx->stream)
+		(void)hipStreamDestroy(ctx->stream);
+	delete ctx;
+}
+
+const char* sw_last_error(const sw_ctx* ctx) { return ctx ? ctx->error.c_str() : "null context"; }
+
+int sw_sync(sw_ctx* ctx)
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	return hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+
+void* sw_ctx_stream(sw_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+uint64_t sw_ctx_device_bytes(const sw_ctx* ctx) { return ctx ? ctx->device_bytes : 0; }
+
+int sw_map_upload(sw_ctx* ctx, const sw_map_desc* d, sw_map** out_map)
+{
+	if(!ctx || !d || !out_map)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	*out_map = nullptr;
+	(void)hipSetDevice(ctx->device);
+	// Validate the texture table against its buffers.
+	for(uint32_t i = 0; i < d->num_levels; ++i)
+	{
+		const sw_texture_level& l = d->levels[i];
+		const uint64_t n = (l.flags & SW_TEX_LITE_LEVEL) ? d->num_lite_texels : d->num_texels;
+		if(l.width == 0 || l.height == 0 || l.width > 65536 || l.height > 65536 || l.offset + uint64_t(l.width) * l.height > n)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "texture level " + std::to_string(i) + " outside its texel buffer");
+	}
+	sw_map* m = new(std::nothrow) sw_map();
+	if(!m)
+		return set_error(ctx, SW_ERR_OUT_OF_MEMORY, "map allocation");
+	int r = SW_OK;
+	r = r ? r : upload(ctx, m->texels, d->texels, sizeof(sw_texel) * d->num_texels);
+	r = r ? r : upload(ctx, m->lite, d->lite_texels, sizeof(uint32_t) * d->num_lite_texels);
+	r = r ? r : upload(ctx, m->lm_simple, d->lightmaps_simple, sizeof(float) * 3 * d->num_lightmap_elements);
+	r = r ? r : upload(ctx, m->lm_dir, d->lightmaps_directional, sizeof(sw_directional_lightmap_element) * d->num_directional_lightmap_elements);
+	r = r ? r : upload(ctx, m->levels, d->levels, sizeof(sw_texture_level) * d->num_levels);
+	r = r ? r : upload(ctx, m->sky32, d->skybox_texels32, sizeof(uint32_t) * (d->skybox_texels32 ? d->num_skybox_texels32 : 0));
+	r = r ? r : upload(ctx, m->sky64, d->skybox_texels64, sizeof(uint64_t) * (d->skybox_texels64 ? d->num_skybox_texels64 : 0));
+	if(!r)
+		r = hip_check(ctx, hipStreamSynchronize(ctx->stream), "map upload");
+	if(r)
+	{
+		sw_map_release(ctx, m);
+		return r;
+	}
+	m->num_sky32 = d->skybox_texels32 ? d->num_skybox_texels32 : 0;
+	m->num_sky64 = d->skybox_texels64 ? d->num_skybox_texels64 : 0;
+	m->host_levels.assign(d->levels, d->levels + d->num_levels);
+	m->ptrs.texels = static_cast<const sw_texel*>(m->texels.p);
+	m->ptrs.lite = static_cast<const uint32_t*>(m->lite.p);
+	m->ptrs.lm_simple = static_cast<const float*>(m->lm_simple.p);
+	m->ptrs.lm_dir = static_cast<const sw_directional_lightmap_element*>(m->lm_dir.p);
+	m->ptrs.levels = static_cast<const sw_texture_level*>(m->levels.p);
+	m->ptrs.num_levels = d->num_levels;
+	m->ptrs.num_texels = d->num_texels;
+	m->ptrs.num_lite = d->num_lite_texels;
+	m->ptrs.num_lm_simple = d->num_lightmap_elements;
+	m->ptrs.num_lm_dir = d->num_directional_lightmap_elements;
+	*out_map = m;
+	return SW_OK;
+}
+
+void sw_map_release(sw_ctx* ctx, sw_map* m)
+{
+	if(!m)
+		return;
+	if(ctx)
+		(void)hipStreamSynchronize(ctx->stream);
+	DevBuf* bufs[] = {&m->texels, &m->lite, &m->lm_simple, &m->lm_dir, &m->levels, &m->sky32, &m->sky64};
+	for(DevBuf* b : bufs)
+		release(ctx, *b);
+	delete m;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Plain path
+// ---------------------------------------------------------------------------------------------
+int sw_frame_begin(sw_ctx* ctx, sw_map* map, uint32_t color_bits, uint32_t width, uint32_t height, const float cam_pos[3],
+	uint32_t clear_background, uint64_t background_color)
+{
+	if(!ctx || !map || !cam_pos)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	if(color_bits != 32 && color_bits != 64)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "color_bits must be 32 or 64");
+	if(width == 0 || height == 0 || width > 16384 || height > 16384)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "frame size must be in [1, 16384]");
+	(void)hipSetDevice(ctx->device);
+	int r = ensure(ctx, ctx->fb, uint64_t(width) * height * color_bytes(color_bits));
+	if(r)
+		return r;
+	ctx->graph_key = 0;
+	ctx->frame_map = map;
+	ctx->frame_color_bits = color_bits;
+	sw_view& v = ctx->frame_view;
+	v = sw_view{};
+	v.width = width;
+	v.height = height;
+	v.cam_pos[0] = cam_pos[0];
+	v.cam_pos[1] = cam_pos[1];
+	v.cam_pos[2] = cam_pos[2];
+	v.clear_background = clear_background;
+	v.background_color = background_color;
+	ctx->frame_arena_texels = 0;
+	ctx->frame_num_surfaces = 0;
+	ctx->frame_active = true;
+	ctx->frame_rastered = false;
+	return SW_OK;
+}
+
+int sw_build_surfaces(sw_ctx* ctx, const sw_surface_desc* surfaces, uint32_t num_surfaces, const sw_light* lights,
+	uint32_t num_lights, const uint32_t* light_indices, uint32_t num_light_indices)
+{
+	if(!ctx || !ctx->frame_active)
+		return set_error(ctx, SW_ERR_STATE, "sw_build_surfaces outside sw_frame_begin");
+	if(num_surfaces && !surfaces)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null surfaces");
+	(void)hipSetDevice(ctx->device);
+	uint64_t arena = 0;
+	for(uint32_t i = 0; i < num_surfaces; ++i)
+		arena = std::max<uint64_t>(arena, surfaces[i].pixels_offset + uint64_t(surfaces[i].size[0]) * surfaces[i].size[1]);
+	int r;
+	if((r = ensure(ctx, ctx->arena, std::max<uint64_t>(arena, 1) * color_bytes(ctx->frame_color_bits))))
+		return r;
+	if((r = upload(ctx, ctx->surfaces, surfaces, sizeof(sw_surface_desc) * num_surfaces)))
+		return r;
+	if((r = upload(ctx, ctx->lights, lights, sizeof(sw_light) * (lights ? num_lights : 0))))
+		return r;
+	if((r = upload(ctx, ctx->light_indices, light_indices, sizeof(uint32_t) * (light_indices ? num_light_indices : 0))))
+		return r;
+	ctx->frame_arena_texels = arena;
+	ctx->last_color_bits = ctx->frame_color_bits;
+	ctx->frame_num_surfaces = num_surfaces;
+	ctx->frame_num_lights = lights ? num_lights : 0;
+	ctx->frame_num_light_indices = light_indices ? num_light_indices : 0;
+	fill_surface_stats(ctx, surfaces, num_surfaces, true);
+
+	std::vector<ViewExt> ve;
+	uint32_t tb, tt;
+	uint64_t be;
+	sw_view v = ctx->frame_view;
+	v.first_surface = 0;
+	v.num_surfaces = num_surfaces;
+	v.first_draw = 0;
+	v.num_draws = 0;
+	v.fb_offset = 0;
+	v.surface_arena_offset = 0;
+	if((r = build_view_ext(ctx, &v, 1, ve, tb, tt, be)))
+		return r;
+	if((r = upload(ctx, ctx->views, ve.data(), sizeof(ViewExt))))
+		return r;
+	sw_ctx::Launch L;
+	L.map = ctx->frame_map;
+	L.color_bits = ctx->frame_color_bits;
+	L.num_views = 1;
+	L.surfaces = static_cast<const sw_surface_desc*>(ctx->surfaces.p);
+	L.num_surfaces = num_surfaces;
+	L.lights = static_cast<const sw_light*>(ctx->lights.p);
+	L.num_lights = ctx->frame_num_lights;
+	L.light_indices = static_cast<const uint32_t*>(ctx->light_indices.p);
+	L.num_light_indices = ctx->frame_num_light_indices;
+	L.arena_texels = arena;
+	L.arena = ctx->arena.p;
+	return launch_surface_stages(ctx, L, static_cast<const ViewExt*>(ctx->views.p));
+}
+
+int sw_raster_draws(sw_ctx* ctx, const sw_draw* draws, uint32_t num_draws, const int32_t* vertex_data, uint64_t num_vertex_ints)
+{
+	if(!ctx || !ctx->frame_active)
+		return set_error(ctx, SW_ERR_STATE, "sw_raster_draws outside sw_frame_begin");
+	if((num_draws && !draws) || (num_vertex_ints && !vertex_data))
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null draws / vertex data");
+	(void)hipSetDevice(ctx->device);
+	int r;
+	if((r = upload(ctx, ctx->draws, draws, sizeof(sw_draw) * num_draws)))
+		return r;
+	if((r = upload(ctx, ctx->vdata, vertex_data, sizeof(int32_t) * num_vertex_ints)))
+		return r;
+	if(ctx->arena.p == nullptr && (r = ensure(ctx, ctx->arena, 8)))
+		return r;
+	sw_view v = ctx->frame_view;
+	v.first_surface = 0;
+	v.num_surfaces = ctx->frame_num_surfaces;
+	v.first_draw = 0;
+	v.num_draws = num_draws;
+	v.fb_offset = 0;
+	v.surface_arena_offset = 0;
+	std::vector<ViewExt> ve;
+	sw_ctx::Launch& L = ctx->last;
+	L = sw_ctx::Launch{};
+	if((r = build_view_ext(ctx, &v, 1, ve, L.total_bands, L.total_tiles, L.band_list_entries)))
+		return r;
+	// The views buffer may still be read by the surface kernels: write it after them on the stream.
+	if((r = upload(ctx, ctx->views, ve.data(), sizeof(ViewExt))))
+		return r;
+	L.map = ctx->frame_map;
+	L.color_bits = ctx->frame_color_bits;
+	L.num_views = 1;
+	L.draws = static_cast<const sw_draw*>(ctx->draws.p);
+	L.num_draws = num_draws;
+	L.vdata = static_cast<const int32_t*>(ctx->vdata.p);
+	L.num_vertex_ints = num_vertex_ints;
+	L.arena_texels = ctx->frame_arena_texels;
+	L.arena = ctx->arena.p;
+	L.fb = ctx->fb.p;
+	L.num_surfaces = ctx->frame_num_surfaces;
+	if((r = launch_raster_stages(ctx, L, static_cast<const ViewExt*>(ctx->views.p))))
+		return r;
+	ctx->frame_rastered = true;
+	return SW_OK;
+}
+
+int sw_frame_end(sw_ctx* ctx, void* dst, size_t dst_pitch_bytes, uint32_t dst_on_device)
+{
+	if(!ctx || !ctx->frame_active)
+		return set_error(ctx, SW_ERR_STATE, "sw_frame_end outside sw_frame_begin");
+	(void)hipSetDevice(ctx->device);
+	int r;
+	if(!ctx->frame_rastered)
+	{
+		// no draws: the framebuffer is the cleared background (or untouched)
+		if((r = sw_raster_draws(ctx, nullptr, 0, nullptr, 0)))
+			return r;
+	}
+	if((r = finish(ctx, static_cast<const ViewExt*>(ctx->views.p))))
+		return r;
+	if(dst)
+	{
+		const size_t row = size_t(ctx->frame_view.width) * color_bytes(ctx->frame_color_bits);
+		if(dst_pitch_bytes < row)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "dst pitch smaller than a row");
+		if((r = hip_check(ctx,
+				hipMemcpy2DAsync(dst, dst_pitch_bytes, ctx->fb.p, row, row, ctx->frame_view.height,
+					dst_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx->stream),
+				"hipMemcpy2DAsync framebuffer")))
+			return r;
+		if((r = hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize")))
+			return r;
+	}
+	ctx->frame_active = false;
+	return SW_OK;
+}
+
+int sw_tonemap(sw_ctx* ctx, float exposure, const float color_modulate[3], void* dst32, size_t dst_pitch_bytes,
+	uint32_t dst_on_device, float out_colors_sum[3])
+{
+	if(!ctx || !color_modulate)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	if(ctx->frame_color_bits != 64 || !ctx->fb.p)
+		return set_error(ctx, SW_ERR_STATE, "sw_tonemap needs a 64-bit (HDR) frame");
+	(void)hipSetDevice(ctx->device);
+	int r;
+	if(ctx->frame_active)
+	{
+		if(!ctx->frame_rastered && (r = sw_raster_draws(ctx, nullptr, 0, nullptr, 0)))
+			return r;
+		if((r = finish(ctx, static_cast<const ViewExt*>(ctx->views.p))))
+			return r;
+		ctx->frame_active = false;
+	}
+	const uint32_t w = ctx->frame_view.width, h = ctx->frame_view.height;
+	void* out = dst32;
+	uint32_t pitch_px = uint32_t(dst_pitch_bytes / 4);
+	if(!dst_on_device)
+	{
+		if((r = ensure(ctx, ctx->tonemap_out, uint64_t(w) * h * 4)))
+			return r;
+		out = ctx->tonemap_out.p;
+		pitch_px = w;
+	}
+	else if(dst_pitch_bytes < size_t(w) * 4 || dst_pitch_bytes % 4)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "bad dst pitch");
+	if((r = sw_tonemap_device(ctx, ctx->fb.p, w, h, exposure, color_modulate, out, size_t(pitch_px) * 4, out_colors_sum)))
+		return r;
+	if(!dst_on_device && dst32)
+	{
+		if(dst_pitch_bytes < size_t(w) * 4)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "bad dst pitch");
+		if((r = hip_check(ctx,
+				hipMemcpy2DAsync(dst32, dst_pitch_bytes, out, size_t(w) * 4, size_t(w) * 4, h, hipMemcpyDeviceToHost, ctx->stream),
+				"hipMemcpy2DAsync tonemap")))
+			return r;
+		return hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+	}
+	return SW_OK;
+}
+
+int sw_tonemap_device(sw_ctx* ctx, const void* hdr_device, uint32_t width, uint32_t height, float exposure,
+	const float color_modulate[3], void* dst32_device, size_t dst_pitch_bytes, float out_colors_sum[3])
+{
+	if(!ctx || !hdr_device || !dst32_device || !color_modulate)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	(void)hipSetDevice(ctx->device);
+	int r;
+	if((r = ensure(ctx, ctx->tonemap_sums, sizeof(float) * 4)))
+		return r;
+	// TonemappingFunction::new (postprocessor.rs:1005-1017)
+	const float isr = 1.0f / (exposure * color_modulate[0]);
+	const float isg = 1.0f / (exposure * color_modulate[1]);
+	const float isb = 1.0f / (exposure * color_modulate[2]);
+	{
+		StageTimer t(ctx, STAGE_TONEMAP);
+		swk::launch_tonemap(ctx->stream, static_cast<const uint64_t*>(hdr_device), width, height, isr, isg, isb,
+			static_cast<uint32_t*>(dst32_device), uint32_t(dst_pitch_bytes / 4), static_cast<float*>(ctx->tonemap_sums.p));
+	}
+	if((r = hip_check(ctx, hipGetLastError(), "tonemap launch")))
+		return r;
+	if(out_colors_sum)
+	{
+		float sums[4] = {0, 0, 0, 0};
+		if((r = hip_check(ctx,
+				hipMemcpyAsync(sums, ctx->tonemap_sums.p, sizeof(float) * 3, hipMemcpyDeviceToHost, ctx->stream), "sums")))
+			return r;
+		if((r = hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize")))
+			return r;
+		out_colors_sum[0] = sums[0];
+		out_colors_sum[1] = sums[1];
+		out_colors_sum[2] = sums[2];
+	}
+	return SW_OK;
+}
+
+void* sw_frame_device_framebuffer(sw_ctx* ctx) { return ctx ? ctx->fb.p : nullptr; }
+
+int sw_device_alloc(sw_ctx* ctx, uint64_t bytes, void** out)
+{
+	if(!ctx || !out)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	(void)hipSetDevice(ctx->device);
+	*out = nullptr;
+	if(hipMalloc(out, std::max<uint64_t>(bytes, 16)) != hipSuccess)
+	{
+		(void)hipGetLastError();
+		*out = nullptr;
+		return set_error(ctx, SW_ERR_OUT_OF_MEMORY, "hipMalloc failed");
+	}
+	return SW_OK;
+}
+
+void sw_device_free(sw_ctx* ctx, void* p)
+{
+	if(!ctx || !p)
+		return;
+	(void)hipSetDevice(ctx->device);
+	(void)hipStreamSynchronize(ctx->stream);
+	(void)hipFree(p);
+}
+
+int sw_copy_to_device(sw_ctx* ctx, void* dst, const void* src, uint64_t bytes)
+{
+	if(!ctx || ((!dst || !src) && bytes))
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	(void)hipSetDevice(ctx->device);
+	if(!bytes)
+		return SW_OK;
+	int r = hip_check(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "copy to device");
+	return r ? r : hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+
+int sw_copy_from_device(sw_ctx* ctx, void* dst, const void* src, uint64_t bytes)
+{
+	if(!ctx || ((!dst || !src) && bytes))
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	(void)hipSetDevice(ctx->device);
+	if(!bytes)
+		return SW_OK;
+	int r = hip_check(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "copy from device");
+	return r ? r : hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+
+int sw_profile_enable(sw_ctx* ctx, uint32_t enable)
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	ctx->profiling = enable != 0;
+	return SW_OK;
+}
+
+int sw_profile_read(sw_ctx* ctx, double out_ms[SW_PROFILE_STAGES], uint64_t out_counts[SW_PROFILE_STAGES], uint32_t reset)
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	int r = hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+	if(r)
+		return r;
+	collect_timings(ctx);
+	for(int i = 0; i < SW_PROFILE_STAGES; ++i)
+	{
+		if(out_ms)
+			out_ms[i] = ctx->prof_ms[i];
+		if(out_counts)
+			out_counts[i] = ctx->prof_count[i];
+		if(reset)
+		{
+			ctx->prof_ms[i] = 0.0;
+			ctx->prof_count[i] = 0;
+		}
+	}
+	return SW_OK;
+}
+
+int sw_read_surfaces(sw_ctx* ctx, void* dst, uint64_t num_texels)
+{
+	if(!ctx || (!dst && num_texels))
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	(void)hipSetDevice(ctx->device);
+	const uint64_t bytes = num_texels * color_bytes(ctx->last_color_bits);
+	if(bytes > ctx->arena.cap)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "num_texels beyond the surface arena");
+	if(bytes == 0)
+		return SW_OK;
+	int r = hip_check(ctx, hipMemcpyAsync(dst, ctx->arena.p, bytes, hipMemcpyDeviceToHost, ctx->stream), "read surfaces");
+	return r ? r : hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+
+int sw_get_stats(sw_ctx* ctx, sw_stats* out)
+{
+	if(!ctx || !out)
+		return SW_ERR_INVALID_ARGUMENT;
+	*out = ctx->stats;
+	return SW_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Batched path
+// ---------------------------------------------------------------------------------------------
+static int prepare_batch(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_fb, sw_ctx::Launch& L, std::vector<ViewExt>& ve)
+{
+	if(!ctx || !map || !b || !out_fb || (b->num_views && !b->views))
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	if(b->color_bits != 32 && b->color_bits != 64)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "color_bits must be 32 or 64");
+	int r;
+	if((r = build_view_ext(ctx, b->views, b->num_views, ve, L.total_bands, L.total_tiles, L.band_list_entries)))
+		return r;
+	if((r = validate_views_fit(ctx, ve, b->framebuffer_texels, b->num_surfaces, b->num_draws)))
+		return r;
+	for(const ViewExt& e : ve)
+		if(e.v.surface_arena_offset > b->surface_arena_texels)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "view arena offset outside the arena");
+	if((r = upload(ctx, ctx->views, ve.data(), sizeof(ViewExt) * ve.size())))
+		return r;
+	if((r = ensure(ctx, ctx->arena, std::max<uint64_t>(b->surface_arena_texels, 1) * color_bytes(b->color_bits))))
+		return r;
+	L.map = map;
+	L.color_bits = b->color_bits;
+	L.num_views = b->num_views;
+	L.num_surfaces = b->num_surfaces;
+	L.num_lights = b->lights ? b->num_lights : 0;
+	L.num_light_indices = b->light_indices ? b->num_light_indices : 0;
+	L.num_draws = b->num_draws;
+	L.num_vertex_ints = b->num_vertex_ints;
+	L.arena_texels = b->surface_arena_texels;
+	L.arena = ctx->arena.p;
+	L.fb = out_fb;
+	if(b->inputs_on_device)
+	{
+		L.surfaces = b->surfaces;
+		L.lights = b->lights;
+		L.light_indices = b->light_indices;
+		L.draws = b->draws;
+		L.vdata = b->vertex_data;
+	}
+	else
+	{
+		if((r = upload(ctx, ctx->surfaces, b->surfaces, sizeof(sw_surface_desc) * b->num_surfaces)))
+			return r;
+		if((r = upload(ctx, ctx->lights, b->lights, sizeof(sw_light) * L.num_lights)))
+			return r;
+		if((r = upload(ctx, ctx->light_indices, b->light_indices, sizeof(uint32_t) * L.num_light_indices)))
+			return r;
+		if((r = upload(ctx, ctx->draws, b->draws, sizeof(sw_draw) * b->num_draws)))
+			return r;
+		if((r = upload(ctx, ctx->vdata, b->vertex_data, sizeof(int32_t) * b->num_vertex_ints)))
+			return r;
+		L.surfaces = static_cast<const sw_surface_desc*>(ctx->surfaces.p);
+		L.lights = static_cast<const sw_light*>(ctx->lights.p);
+		L.light_indices = static_cast<const uint32_t*>(ctx->light_indices.p);
+		L.draws = static_cast<const sw_draw*>(ctx->draws.p);
+		L.vdata = static_cast<const int32_t*>(ctx->vdata.p);
+		fill_surface_stats(ctx, b->surfaces, b->num_surfaces, true);
+	}
+	return SW_OK;
+}
+
+int sw_render_batch(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_fb)
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	(void)hipSetDevice(ctx->device);
+	ctx->graph_key = 0; // the views / command buffers a captured graph reads are about to change
+	std::vector<ViewExt> ve;
+	sw_ctx::Launch L;
+	int r;
+	if((r = prepare_batch(ctx, map, b, out_fb, L, ve)))
+		return r;
+	ctx->last = L;
+	ctx->last_color_bits = L.color_bits;
+	if((r = launch_surface_stages(ctx, L, static_cast<const ViewExt*>(ctx->views.p))))
+		return r;
+	// views that ask for a cleared background get it in the raster kernel's tile init
+	if((r = launch_raster_stages(ctx, L, static_cast<const ViewExt*>(ctx->views.p))))
+		return r;
+	return finish(ctx, static_cast<const ViewExt*>(ctx->views.p));
+}
+
+int sw_render_batch_graph(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_fb)
+{
+	if(!ctx || !b)
+		return SW_ERR_INVALID_ARGUMENT;
+	if(!b->inputs_on_device)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "graph replay needs device-resident inputs");
+	(void)hipSetDevice(ctx->device);
+	// key: pointers and sizes that the captured launches bake in
+	uint64_t key = 1469598103934665603ull;
+	auto mix = [&](uint64_t v) { key = (key ^ v) * 1099511628211ull; };
+	mix(uint64_t(uintptr_t(map)));
+	mix(uint64_t(uintptr_t(out_fb)));
+	mix(uint64_t(uintptr_t(b->surfaces)));
+	mix(uint64_t(uintptr_t(b->draws)));
+	mix(uint64_t(uintptr_t(b->vertex_data)));
+	mix(b->num_views);
+	mix(b->num_surfaces);
+	mix(b->num_draws);
+	mix(b->color_bits);
+	mix(b->surface_arena_texels);
+	for(uint32_t i = 0; i < b->num_views; ++i)
+	{
+		mix(b->views[i].width);
+		mix(b->views[i].height);
+		mix(b->views[i].num_draws);
+		mix(b->views[i].num_surfaces);
+	}
+	int r;
+	if(!ctx->graph_exec || key != ctx->graph_key || ctx->graph_epoch != ctx->alloc_epoch)
+	{
+		// An eager run sizes every buffer (span capacity included) before capture.
+		if((r = sw_render_batch(ctx, map, b, out_fb)))
+			return r;
+		ctx->span_capacity = std::max<uint64_t>(ctx->span_capacity, ctx->stats.spans + ctx->stats.spans / 2 + 1024);
+		std::vector<ViewExt> ve;
+		sw_ctx::Launch L;
+		if((r = prepare_batch(ctx, map, b, out_fb, L, ve)))
+			return r;
+		// allocate at the new span capacity outside the capture
+		if((r = ensure(ctx, ctx->spans, sizeof(SpanRec) * ctx->span_capacity)))
+			return r;
+		if((r = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync")))
+			return r;
+		if(ctx->graph_exec)
+		{
+			(void)hipGraphExecDestroy(ctx->graph_exec);
+			ctx->graph_exec = nullptr;
+		}
+		if(ctx->graph)
+		{
+			(void)hipGraphDestroy(ctx->graph);
+			ctx->graph = nullptr;
+		}
+		if((r = hip_check(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture")))
+			return r;
+		ctx->last = L;
+		ctx->capturing = true;
+		int r1 = launch_surface_stages(ctx, L, static_cast<const ViewExt*>(ctx->views.p));
+		int r2 = r1 ? r1 : launch_raster_stages(ctx, L, static_cast<const ViewExt*>(ctx->views.p));
+		hipGraph_t g = nullptr;
+		const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+		ctx->capturing = false;
+		if(r2)
+			return r2;
+		if((r = hip_check(ctx, ec, "end capture")))
+			return r;
+		ctx->graph = g;
+		if((r = hip_check(ctx, hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0), "graph instantiate")))
+			return r;
+		ctx->graph_key = key;
+		ctx->graph_epoch = ctx->alloc_epoch;
+	}
+	if((r = hip_check(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream), "graph launch")))
+		return r;
+	return finish(ctx, static_cast<const ViewExt*>(ctx->views.p));
+}
+
+} // extern "C"

@@ -1,0 +1,387 @@
+"""ctypes / numpy mirror of include/sw_gpu.h and loaders for the in-tree native libraries.
+
+The product path is libsw_gpu.so (HIP kernels behind the C ABI). `load_gpu()` raises if it is
+missing: there is no CPU fallback anywhere in the product path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+GPU_LIB_PATH = os.path.join(HERE, "libsw_gpu.so")
+HOST_LIB_PATH = os.path.join(HERE, "libsw_host.so")
+
+# ---------------------------------------------------------------------------------------------
+# Constants (sw_gpu.h)
+# ---------------------------------------------------------------------------------------------
+SW_OK = 0
+BLEND_NONE, BLEND_AVERAGE, BLEND_ADDITIVE, BLEND_ALPHA_TEST, BLEND_ALPHA_BLEND = range(5)
+TC_PERSPECTIVE, TC_LINE_Z, TC_AFFINE = range(3)
+DRAW_POLYGON, DRAW_TRIANGLE = 0, 1
+TEXSRC_SURFACE, TEXSRC_SKYBOX = 0, 1
+TEX_HAS_NORMAL_MAP, TEX_HAS_NON_ONE_ROUGHNESS, TEX_IS_METAL = 1, 2, 4
+TEX_LITE_LEVEL = 1 << 8
+SURF_DIFFUSE_LAYER, SURF_DIRECTIONAL, SURF_EMISSIVE = 1, 2, 4
+NO_TEXTURE = 0xFFFFFFFF
+MAX_POLYGON_LIGHTS = 6
+MAX_POLYGON_VERTICES = 24
+
+# ---------------------------------------------------------------------------------------------
+# numpy dtypes of the POD structs (aligned like the C structs)
+# ---------------------------------------------------------------------------------------------
+TEXEL_DTYPE = np.dtype([("diffuse", "<u4"), ("packed_normal_roughness", "<i4")], align=True)
+LEVEL_DTYPE = np.dtype([("offset", "<u8"), ("width", "<u4"), ("height", "<u4"), ("flags", "<u4"),
+                        ("reserved", "<u4")], align=True)
+DIRLM_DTYPE = np.dtype([("ambient_light", "<f4", 3), ("light_direction_vector_scaled", "<f4", 3),
+                        ("directional_light_deviation", "<f4"), ("directional_light_color", "<f4", 3)],
+                       align=True)
+LIGHT_DTYPE = np.dtype([("position", "<f4", 3), ("radius", "<f4"), ("inv_square_radius", "<f4"),
+                        ("color", "<f4", 3)], align=True)
+SURFACE_DTYPE = np.dtype([
+    ("basis_u", "<f4", 3), ("basis_v", "<f4", 3), ("basis_start", "<f4", 3), ("basis_normal", "<f4", 3),
+    ("size", "<u4", 2), ("tc_start", "<i4", 2), ("texture_level", "<u4"), ("lightmap_offset", "<u4"),
+    ("lightmap_size", "<u4", 2), ("lightmap_scale_log2", "<u4"), ("lightmap_tc_shift", "<u4", 2),
+    ("first_light", "<u4"), ("num_lights", "<u4"), ("emissive_level", "<u4"), ("emissive_light", "<f4", 3),
+    ("emissive_blend", "<u4"), ("flags", "<u4"), ("pixels_offset", "<u8")], align=True)
+DRAW_DTYPE = np.dtype([
+    ("kind", "<u4"), ("blend", "<u4"), ("mode", "<u4"), ("texture_source", "<u4"), ("first_vertex_int", "<u4"),
+    ("num_vertices", "<u4"), ("depth_equation", "<f4", 3), ("tc_equation", "<f4", 6), ("texture_offset", "<u8"),
+    ("texture_size", "<i4", 2), ("clip_rect", "<i4", 4)], align=True)
+VIEW_DTYPE = np.dtype([
+    ("width", "<u4"), ("height", "<u4"), ("cam_pos", "<f4", 3), ("clear_background", "<u4"),
+    ("background_color", "<u8"), ("first_surface", "<u4"), ("num_surfaces", "<u4"), ("first_draw", "<u4"),
+    ("num_draws", "<u4"), ("fb_offset", "<u8"), ("surface_arena_offset", "<u8")], align=True)
+
+EXPECTED_SIZES = {"TEXEL": 8, "LEVEL": 24, "DIRLM": 40, "LIGHT": 32, "SURFACE": 136, "DRAW": 96, "VIEW": 64}
+for _name, _size in EXPECTED_SIZES.items():
+    assert globals()[_name + "_DTYPE"].itemsize == _size, (_name, globals()[_name + "_DTYPE"].itemsize)
+
+
+def _ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+class MapDesc(C.Structure):
+    _fields_ = [
+        ("texels", C.c_void_p), ("num_texels", C.c_uint64),
+        ("lite_texels", C.c_void_p), ("num_lite_texels", C.c_uint64),
+        ("lightmaps_simple", C.c_void_p), ("num_lightmap_elements", C.c_uint64),
+        ("lightmaps_directional", C.c_void_p), ("num_directional_lightmap_elements", C.c_uint64),
+        ("levels", C.c_void_p), ("num_levels", C.c_uint32), ("reserved", C.c_uint32),
+        ("skybox_texels32", C.c_void_p), ("num_skybox_texels32", C.c_uint64),
+        ("skybox_texels64", C.c_void_p), ("num_skybox_texels64", C.c_uint64),
+    ]
+
+
+class Batch(C.Structure):
+    _fields_ = [
+        ("views", C.c_void_p), ("num_views", C.c_uint32), ("color_bits", C.c_uint32),
+        ("surfaces", C.c_void_p), ("num_surfaces", C.c_uint32), ("num_lights", C.c_uint32),
+        ("lights", C.c_void_p), ("light_indices", C.c_void_p), ("num_light_indices", C.c_uint32),
+        ("num_draws", C.c_uint32), ("draws", C.c_void_p), ("vertex_data", C.c_void_p),
+        ("num_vertex_ints", C.c_uint64), ("surface_arena_texels", C.c_uint64), ("framebuffer_texels", C.c_uint64),
+        ("inputs_on_device", C.c_uint32), ("reserved", C.c_uint32),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("surface_texels", "lightmap_elements", "emissive_texels", "polygon_pixels",
+                                          "triangle_pixels", "blend_pixels", "spans", "parts")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+# ---------------------------------------------------------------------------------------------
+# Scene containers (numpy arrays that back a map / a batch)
+# ---------------------------------------------------------------------------------------------
+class MapData:
+    """Map-resident data: textures + mips, lite textures, lightmaps, skybox sides."""
+
+    def __init__(self, texels, lite, lm_simple, lm_dir, levels, sky32=None, sky64=None):
+        self.texels = np.ascontiguousarray(texels, dtype=TEXEL_DTYPE)
+        self.lite = np.ascontiguousarray(lite, dtype=np.uint32)
+        self.lm_simple = np.ascontiguousarray(lm_simple, dtype=np.float32).reshape(-1, 3)
+        self.lm_dir = np.ascontiguousarray(lm_dir, dtype=DIRLM_DTYPE)
+        self.levels = np.ascontiguousarray(levels, dtype=LEVEL_DTYPE)
+        self.sky32 = None if sky32 is None else np.ascontiguousarray(sky32, dtype=np.uint32)
+        self.sky64 = None if sky64 is None else np.ascontiguousarray(sky64, dtype=np.uint64)
+
+    def desc(self):
+        d = MapDesc()
+        d.texels, d.num_texels = _ptr(self.texels), len(self.texels)
+        d.lite_texels, d.num_lite_texels = _ptr(self.lite), len(self.lite)
+        d.lightmaps_simple, d.num_lightmap_elements = _ptr(self.lm_simple), len(self.lm_simple)
+        d.lightmaps_directional, d.num_directional_lightmap_elements = _ptr(self.lm_dir), len(self.lm_dir)
+        d.levels, d.num_levels = _ptr(self.levels), len(self.levels)
+        if self.sky32 is not None:
+            d.skybox_texels32, d.num_skybox_texels32 = _ptr(self.sky32), len(self.sky32)
+        if self.sky64 is not None:
+            d.skybox_texels64, d.num_skybox_texels64 = _ptr(self.sky64), len(self.sky64)
+        return d
+
+
+class BatchData:
+    """Command buffers of a batch of views (host numpy arrays)."""
+
+    def __init__(self, views, surfaces, lights, light_indices, draws, vertex_data, color_bits, arena_texels=None):
+        self.views = np.ascontiguousarray(views, dtype=VIEW_DTYPE)
+        self.surfaces = np.ascontiguousarray(surfaces, dtype=SURFACE_DTYPE)
+        self.lights = np.ascontiguousarray(lights, dtype=LIGHT_DTYPE)
+        self.light_indices = np.ascontiguousarray(light_indices, dtype=np.uint32)
+        self.draws = np.ascontiguousarray(draws, dtype=DRAW_DTYPE)
+        self.vertex_data = np.ascontiguousarray(vertex_data, dtype=np.int32)
+        self.color_bits = int(color_bits)
+        if arena_texels is None:
+            arena_texels = 0
+            for v in self.views:
+                s0, n = int(v["first_surface"]), int(v["num_surfaces"])
+                for s in self.surfaces[s0:s0 + n]:
+                    end = int(v["surface_arena_offset"]) + int(s["pixels_offset"]) + int(s["size"][0]) * int(s["size"][1])
+                    arena_texels = max(arena_texels, end)
+        self.arena_texels = int(arena_texels)
+        self.fb_texels = int(max((int(v["fb_offset"]) + int(v["width"]) * int(v["height"]) for v in self.views),
+                                 default=0))
+
+    @property
+    def color_dtype(self):
+        return np.uint64 if self.color_bits == 64 else np.uint32
+
+    def ctypes_batch(self, inputs_on_device=False, device_ptrs=None):
+        b = Batch()
+        b.views, b.num_views = _ptr(self.views), len(self.views)
+        b.color_bits = self.color_bits
+        b.num_surfaces = len(self.surfaces)
+        b.num_lights = len(self.lights)
+        b.num_light_indices = len(self.light_indices)
+        b.num_draws = len(self.draws)
+        b.num_vertex_ints = len(self.vertex_data)
+        b.surface_arena_texels = self.arena_texels
+        b.framebuffer_texels = self.fb_texels
+        if inputs_on_device:
+            b.inputs_on_device = 1
+            b.surfaces = device_ptrs["surfaces"]
+            b.lights = device_ptrs["lights"]
+            b.light_indices = device_ptrs["light_indices"]
+            b.draws = device_ptrs["draws"]
+            b.vertex_data = device_ptrs["vertex_data"]
+        else:
+            b.surfaces = _ptr(self.surfaces)
+            b.lights = _ptr(self.lights)
+            b.light_indices = _ptr(self.light_indices)
+            b.draws = _ptr(self.draws)
+            b.vertex_data = _ptr(self.vertex_data)
+        return b
+
+
+# ---------------------------------------------------------------------------------------------
+# Library loading
+# ---------------------------------------------------------------------------------------------
+GPU_SYMBOLS = {
+    "sw_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "sw_ctx_destroy": (None, [C.c_void_p]),
+    "sw_last_error": (C.c_char_p, [C.c_void_p]),
+    "sw_sync": (C.c_int, [C.c_void_p]),
+    "sw_map_upload": (C.c_int, [C.c_void_p, C.POINTER(MapDesc), C.POINTER(C.c_void_p)]),
+    "sw_map_release": (None, [C.c_void_p, C.c_void_p]),
+    "sw_frame_begin": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_float),
+                                 C.c_uint32, C.c_uint64]),
+    "sw_build_surfaces": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                    C.c_uint32]),
+    "sw_raster_draws": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]),
+    "sw_frame_end": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint32]),
+    "sw_tonemap": (C.c_int, [C.c_void_p, C.c_float, C.POINTER(C.c_float), C.c_void_p, C.c_size_t, C.c_uint32,
+                             C.POINTER(C.c_float)]),
+    "sw_render_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Batch), C.c_void_p]),
+    "sw_render_batch_graph": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Batch), C.c_void_p]),
+    "sw_tonemap_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float,
+                                    C.POINTER(C.c_float), C.c_void_p, C.c_size_t, C.POINTER(C.c_float)]),
+    "sw_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "sw_read_surfaces": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "sw_frame_device_framebuffer": (C.c_void_p, [C.c_void_p]),
+    "sw_ctx_stream": (C.c_void_p, [C.c_void_p]),
+    "sw_device_alloc": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "sw_device_free": (None, [C.c_void_p, C.c_void_p]),
+    "sw_copy_to_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    "sw_copy_from_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    "sw_profile_enable": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "sw_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_uint32]),
+    "sw_ctx_device_bytes": (C.c_uint64, [C.c_void_p]),
+}
+
+_gpu_lib = None
+
+
+def load_gpu():
+    """Loads libsw_gpu.so (the HIP product path). Raises when it is not built."""
+    global _gpu_lib
+    if _gpu_lib is None:
+        if not os.path.exists(GPU_LIB_PATH):
+            raise RuntimeError(f"{GPU_LIB_PATH} is not built: run square-wheel_amd/build.py (no CPU fallback exists)")
+        lib = C.CDLL(GPU_LIB_PATH)
+        for name, (res, args) in GPU_SYMBOLS.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _gpu_lib = lib
+    return _gpu_lib
+
+
+class SwError(RuntimeError):
+    pass
+
+
+class Context:
+    """Owns an sw_ctx (one HIP stream on one device)."""
+
+    def __init__(self, device=0):
+        self.lib = load_gpu()
+        h = C.c_void_p()
+        r = self.lib.sw_ctx_create(device, C.byref(h))
+        if r != SW_OK:
+            raise SwError(f"sw_ctx_create failed: {r}")
+        self.h = h
+
+    def check(self, r, what):
+        if r != SW_OK:
+            raise SwError(f"{what} failed ({r}): {self.lib.sw_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            self.lib.sw_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload_map(self, m: MapData):
+        d = m.desc()
+        h = C.c_void_p()
+        self.check(self.lib.sw_map_upload(self.h, C.byref(d), C.byref(h)), "sw_map_upload")
+        return h
+
+    def release_map(self, h):
+        self.lib.sw_map_release(self.h, h)
+
+    def read_surfaces(self, num_texels, color_bits):
+        out = np.zeros(max(int(num_texels), 1), dtype=np.uint64 if color_bits == 64 else np.uint32)
+        self.check(self.lib.sw_read_surfaces(self.h, _ptr(out), int(num_texels)), "sw_read_surfaces")
+        return out[:num_texels]
+
+    def stats(self):
+        s = Stats()
+        self.check(self.lib.sw_get_stats(self.h, C.byref(s)), "sw_get_stats")
+        return s.as_dict()
+
+    # plain path -------------------------------------------------------------------------
+    def render_plain(self, map_h, b: BatchData, view_index=0, tonemap=None):
+        """Renders one view through sw_frame_begin / sw_build_surfaces / sw_raster_draws / sw_frame_end.
+        Returns the framebuffer (H, W) as uint32 / uint64 (and the tonemapped uint32 image if asked)."""
+        v = b.views[view_index]
+        w, h = int(v["width"]), int(v["height"])
+        cam = (C.c_float * 3)(*[float(x) for x in v["cam_pos"]])
+        self.check(self.lib.sw_frame_begin(self.h, map_h, b.color_bits, w, h, cam, int(v["clear_background"]),
+                                           int(v["background_color"])), "sw_frame_begin")
+        s0, ns = int(v["first_surface"]), int(v["num_surfaces"])
+        surfaces = np.ascontiguousarray(b.surfaces[s0:s0 + ns])
+        self.check(self.lib.sw_build_surfaces(self.h, _ptr(surfaces), ns, _ptr(b.lights), len(b.lights),
+                                              _ptr(b.light_indices), len(b.light_indices)), "sw_build_surfaces")
+        d0, nd = int(v["first_draw"]), int(v["num_draws"])
+        draws = np.ascontiguousarray(b.draws[d0:d0 + nd])
+        self.check(self.lib.sw_raster_draws(self.h, _ptr(draws), nd, _ptr(b.vertex_data), len(b.vertex_data)),
+                   "sw_raster_draws")
+        fb = np.zeros((h, w), dtype=b.color_dtype)
+        if tonemap is None:
+            self.check(self.lib.sw_frame_end(self.h, _ptr(fb), w * fb.itemsize, 0), "sw_frame_end")
+            return fb
+        exposure, modulate = tonemap
+        self.check(self.lib.sw_frame_end(self.h, _ptr(fb), w * fb.itemsize, 0), "sw_frame_end")
+        out = np.zeros((h, w), dtype=np.uint32)
+        sums = (C.c_float * 3)()
+        mod = (C.c_float * 3)(*modulate)
+        self.check(self.lib.sw_tonemap(self.h, float(exposure), mod, _ptr(out), w * 4, 0, sums), "sw_tonemap")
+        return fb, out, np.array(sums[:], dtype=np.float32)
+
+    # device memory -----------------------------------------------------------------------
+    def alloc(self, nbytes):
+        p = C.c_void_p()
+        self.check(self.lib.sw_device_alloc(self.h, int(nbytes), C.byref(p)), "sw_device_alloc")
+        return DeviceBuffer(self, p, int(nbytes))
+
+    def to_device(self, arr):
+        a = np.ascontiguousarray(arr)
+        buf = self.alloc(max(a.nbytes, 16))
+        if a.nbytes:
+            self.check(self.lib.sw_copy_to_device(self.h, buf.ptr, _ptr(a), a.nbytes), "sw_copy_to_device")
+        return buf
+
+    def profile_enable(self, on=True):
+        self.check(self.lib.sw_profile_enable(self.h, 1 if on else 0), "sw_profile_enable")
+
+    PROFILE_STAGES = ("prep_surfaces", "build_surfaces", "setup_draws", "bin_bands", "raster_tiles", "tonemap")
+
+    def profile_read(self, reset=True):
+        ms = (C.c_double * 6)()
+        cnt = (C.c_uint64 * 6)()
+        self.check(self.lib.sw_profile_read(self.h, ms, cnt, 1 if reset else 0), "sw_profile_read")
+        return {n: (float(ms[i]), int(cnt[i])) for i, n in enumerate(self.PROFILE_STAGES)}
+
+    # batched path -----------------------------------------------------------------------
+    def render_batch(self, map_h, b: BatchData, graph=False, device_inputs=None, out=None, copy_back=True):
+        """Renders all views of `b` in one set of launches; returns the flat framebuffer (uint32/64).
+        device_inputs: DeviceCommandBuffers already in HBM (inputs stay resident across calls)."""
+        nbytes = max(b.fb_texels, 1) * (8 if b.color_bits == 64 else 4)
+        own = out is None
+        if own:
+            out = self.alloc(nbytes)
+        try:
+            if device_inputs is not None:
+                cb = b.ctypes_batch(inputs_on_device=True, device_ptrs=device_inputs.ptrs())
+            else:
+                cb = b.ctypes_batch()
+            fn = self.lib.sw_render_batch_graph if graph else self.lib.sw_render_batch
+            self.check(fn(self.h, map_h, C.byref(cb), out.ptr), fn.__name__)
+            if not copy_back:
+                return None
+            return out.read(b.color_dtype, b.fb_texels)
+        finally:
+            if own:
+                out.free()
+
+
+class DeviceBuffer:
+    def __init__(self, ctx, ptr, nbytes):
+        self.ctx, self.ptr, self.nbytes = ctx, ptr, nbytes
+
+    def read(self, dtype, count):
+        a = np.zeros(max(int(count), 1), dtype=dtype)
+        if count:
+            self.ctx.check(self.ctx.lib.sw_copy_from_device(self.ctx.h, _ptr(a), self.ptr, a.itemsize * int(count)),
+                           "sw_copy_from_device")
+        return a[:count]
+
+    def free(self):
+        if self.ptr:
+            self.ctx.lib.sw_device_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+
+class DeviceCommandBuffers:
+    """A batch's command buffers copied into HBM once (for resident-input runs)."""
+
+    def __init__(self, ctx, b: BatchData):
+        self.bufs = {"surfaces": ctx.to_device(b.surfaces), "lights": ctx.to_device(b.lights),
+                     "light_indices": ctx.to_device(b.light_indices), "draws": ctx.to_device(b.draws),
+                     "vertex_data": ctx.to_device(b.vertex_data)}
+
+    def ptrs(self):
+        return {k: v.ptr for k, v in self.bufs.items()}
+
+    def free(self):
+        for v in self.bufs.values():
+            v.free()

@@ -1,0 +1,278 @@
+"""Seeded synthetic command buffers for the kernel parity tests.
+
+They exercise every specialisation of the surface builder (lightmap none / scale 0..4, simple and
+directional, normal map on/off, specular none / dielectric / metal, 0..6 dynamic lights, emissive
+layer with every blending mode, Color32 / Color64) and every rasteriser path (perspective, line-z,
+affine polygons incl. thin and partly clipped ones, triangles with light, every blending mode,
+several views of different sizes).
+"""
+import os
+import sys
+
+import numpy as np
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "square-wheel_amd"))
+import sw_abi as A  # noqa: E402
+
+
+def _rand_normals(rng, n, z_min=0.2):
+    v = rng.normal(size=(n, 3))
+    v[:, 2] = np.abs(v[:, 2]) + z_min
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    return v
+
+
+def pack_normal_roughness(normals, roughness):
+    """PackedNormalRoughness::pack (textures.rs:33-39): cvtps2dq (half-even) then signed saturation."""
+    vals = np.concatenate([normals * 127.0, (roughness * 127.0)[:, None]], axis=1).astype(np.float32)
+    q = np.clip(np.rint(vals), -128, 127).astype(np.int64) & 0xFF
+    return (q[:, 0] | (q[:, 1] << 8) | (q[:, 2] << 16) | (q[:, 3] << 24)).astype(np.uint32).view(np.int32)
+
+
+def make_map(rng, num_textures=6, num_lite=4, num_lm=4000):
+    texels, levels, lite = [], [], []
+    off = 0
+    tex_levels = []
+    sizes = [(16, 16), (32, 32), (64, 32), (24, 40), (8, 8), (64, 64), (5, 7), (128, 16)]
+    for t in range(num_textures):
+        w, h = sizes[t % len(sizes)]
+        n = w * h
+        diffuse = rng.integers(0, 2**32, size=n, dtype=np.uint64).astype(np.uint32)
+        flags = 0
+        kind = t % 4
+        normals = np.tile(np.array([0.0, 0.0, 1.0]), (n, 1))
+        rough = np.ones(n)
+        if kind in (1, 3):
+            flags |= A.TEX_HAS_NORMAL_MAP
+            normals = _rand_normals(rng, n)
+        if kind in (2, 3):
+            flags |= A.TEX_HAS_NON_ONE_ROUGHNESS
+            rough = rng.uniform(1.0 / 126.0, 1.0, size=n)
+            if t % 3 == 0:
+                flags |= A.TEX_IS_METAL
+        tx = np.zeros(n, dtype=A.TEXEL_DTYPE)
+        tx["diffuse"] = diffuse
+        tx["packed_normal_roughness"] = pack_normal_roughness(normals, rough)
+        texels.append(tx)
+        levels.append((off, w, h, flags, 0))
+        tex_levels.append(len(levels) - 1)
+        off += n
+    lite_levels = []
+    loff = 0
+    for t in range(num_lite):
+        w, h = [(16, 16), (32, 8), (13, 9), (32, 32)][t % 4]
+        lite.append(rng.integers(0, 2**32, size=w * h, dtype=np.uint64).astype(np.uint32))
+        levels.append((loff, w, h, A.TEX_LITE_LEVEL, 0))
+        lite_levels.append(len(levels) - 1)
+        loff += w * h
+    lm_simple = rng.uniform(0.0, 2.5, size=(num_lm, 3)).astype(np.float32)
+    lm_dir = np.zeros(num_lm, dtype=A.DIRLM_DTYPE)
+    lm_dir["ambient_light"] = rng.uniform(0.0, 0.6, size=(num_lm, 3))
+    d = rng.normal(size=(num_lm, 3))
+    d[:, 2] = np.abs(d[:, 2])
+    lm_dir["light_direction_vector_scaled"] = d * rng.uniform(0.0, 2.0, size=(num_lm, 1))
+    lm_dir["directional_light_deviation"] = rng.uniform(0.0, 1.0, size=num_lm)
+    lm_dir["directional_light_color"] = rng.uniform(0.0, 1.0, size=(num_lm, 3))
+    lv = np.array(levels, dtype=A.LEVEL_DTYPE)
+    m = A.MapData(np.concatenate(texels), np.concatenate(lite), lm_simple, lm_dir, lv)
+    m.tex_levels = tex_levels
+    m.lite_levels = lite_levels
+    return m
+
+
+def make_surfaces(rng, m, n, lights_count, max_size=40, lm_total=4000):
+    surfs = np.zeros(n, dtype=A.SURFACE_DTYPE)
+    light_indices = []
+    off = 0
+    for i in range(n):
+        s = surfs[i]
+        w = int(rng.integers(1, max_size + 1))
+        h = int(rng.integers(1, max_size + 1))
+        s["size"] = (w, h)
+        s["tc_start"] = rng.integers(-300, 300, size=2)
+        s["texture_level"] = m.tex_levels[i % len(m.tex_levels)]
+        # world-space basis of a texel (the scale stays sensible: 0.5..4 units per texel)
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        v = np.cross(u, rng.normal(size=3))
+        v /= np.linalg.norm(v)
+        nrm = np.cross(u, v)
+        s["basis_u"] = u * rng.uniform(0.5, 4.0)
+        s["basis_v"] = v * rng.uniform(0.5, 4.0)
+        s["basis_start"] = rng.uniform(-200, 200, size=3)
+        s["basis_normal"] = nrm / np.linalg.norm(nrm)
+        flags = A.SURF_DIFFUSE_LAYER
+        lm_kind = i % 6
+        if lm_kind == 0:
+            s["lightmap_offset"] = 0
+        else:
+            sl = int(rng.integers(0, 5))
+            shift = rng.integers(0, 1 << sl, size=2)
+            lw = ((w - 1 + int(shift[0])) >> sl) + 2 + int(rng.integers(0, 2))
+            lh = ((h - 1 + int(shift[1])) >> sl) + 2 + int(rng.integers(0, 2))
+            lw = min(lw, 19)
+            if ((w - 1 + int(shift[0])) >> sl) + 1 >= lw:
+                w = max(1, ((lw - 2) << sl) - int(shift[0]))
+                s["size"] = (w, h)
+            s["lightmap_scale_log2"] = sl
+            s["lightmap_tc_shift"] = shift
+            s["lightmap_size"] = (lw, lh)
+            s["lightmap_offset"] = int(rng.integers(1, lm_total - lw * lh - 1))
+            if lm_kind % 2 == 0:
+                flags |= A.SURF_DIRECTIONAL
+        nl = int(rng.integers(0, 7)) if (i % 3 == 1 and lights_count) else 0
+        s["first_light"] = len(light_indices)
+        s["num_lights"] = nl
+        for _ in range(nl):
+            light_indices.append(int(rng.integers(0, lights_count)))
+        if i % 7 == 3:
+            flags |= A.SURF_EMISSIVE
+            s["emissive_level"] = m.lite_levels[i % len(m.lite_levels)]
+            s["emissive_light"] = rng.uniform(0.0, 3.0, size=3)
+            s["emissive_blend"] = int(rng.integers(0, 5))
+            if i % 14 == 3:
+                # without a diffuse layer the caller mixes with BlendingMode::None
+                # (partial_renderer.rs:1808-1815): the surface memory holds nothing to blend with
+                flags &= ~A.SURF_DIFFUSE_LAYER
+                s["emissive_blend"] = A.BLEND_NONE
+        else:
+            s["emissive_level"] = A.NO_TEXTURE
+        s["flags"] = flags
+        s["pixels_offset"] = off
+        off += int(s["size"][0]) * int(s["size"][1])
+    return surfs, np.array(light_indices, dtype=np.uint32), off
+
+
+def make_lights(rng, n):
+    lights = np.zeros(n, dtype=A.LIGHT_DTYPE)
+    lights["position"] = rng.uniform(-250, 250, size=(n, 3))
+    r = rng.uniform(50, 600, size=n)
+    lights["radius"] = r
+    lights["inv_square_radius"] = (1.0 / (r * r)).astype(np.float32)
+    lights["color"] = rng.uniform(0, 20000, size=(n, 3))
+    return lights
+
+
+def _solve_plane(pts, vals):
+    """Coefficients (a, b, c) with a*x + b*y + c = val at three points."""
+    M = np.array([[p[0], p[1], 1.0] for p in pts])
+    return np.linalg.solve(M, np.array(vals))
+
+
+def make_polygon(rng, W, H, tex_w, tex_h, mode, thin=False):
+    cx, cy = rng.uniform(-0.2 * W, 1.2 * W), rng.uniform(-0.2 * H, 1.2 * H)
+    rx = rng.uniform(0.3, 0.6 * W) if not thin else rng.uniform(1, 20)
+    ry = rng.uniform(0.3, 0.6 * H) if not thin else rng.uniform(0.05, 0.7)
+    n = int(rng.integers(3, 9))
+    ang = np.sort(rng.uniform(-np.pi, np.pi, size=n))
+    ang = np.unique(np.round(ang, 3))
+    if len(ang) < 3:
+        ang = np.array([-2.0, 0.0, 2.0])
+    xs = cx + rx * np.cos(ang)
+    ys = cy + ry * np.sin(ang)
+    verts = np.stack([xs, ys], axis=1)
+    # depth / texture equations: inv_z linear and positive over the screen, tc * inv_z linear
+    pts = [(0.0, 0.0), (float(W), 0.0), (0.0, float(H))]
+    zs = rng.uniform(2.0, 300.0, size=3)
+    inv_z = 1.0 / zs
+    de = _solve_plane(pts, inv_z)
+    tcs_u = rng.uniform(0.0, tex_w, size=3)
+    tcs_v = rng.uniform(0.0, tex_h, size=3)
+    tu = _solve_plane(pts, tcs_u * inv_z)
+    tv = _solve_plane(pts, tcs_v * inv_z)
+    if mode != A.TC_AFFINE:
+        # the rasteriser's own power-of-two z scale (partial_renderer.rs:4106-4109)
+        max_inv_z = max(float(inv_z.max()), 1.0 / (1 << 20))
+        z_scale = np.float32(2.0) ** np.float32(-5.0 - np.ceil(np.log2(np.float32(max_inv_z))))
+        de = de * z_scale
+        tu = tu * z_scale
+        tv = tv * z_scale
+    fx = (verts.astype(np.float32) * np.float32(65536.0)).astype(np.int64).astype(np.int32)
+    return fx.reshape(-1), de.astype(np.float32), np.array([tu[0], tv[0], tu[1], tv[1], tu[2], tv[2]], np.float32)
+
+
+def make_triangle(rng, W, H, tex_w, tex_h):
+    pts = np.stack([rng.uniform(-0.1 * W, 1.1 * W, size=3), rng.uniform(-0.1 * H, 1.1 * H, size=3)], axis=1)
+    if rng.uniform() < 0.3:
+        c = pts.mean(axis=0)
+        pts = c + (pts - c) * rng.uniform(0.001, 0.05)
+    out = []
+    for p in pts:
+        x, y = (np.float32(p) * np.float32(65536.0)).astype(np.int64).astype(np.int32)
+        tc = (rng.uniform(0, 1, size=2) * np.array([tex_w, tex_h]) * 65536).astype(np.int32)
+        light = (rng.uniform(0, 2.0, size=3) * 65536).astype(np.int32)
+        out += [x, y, tc[0], tc[1], light[0], light[1], light[2]]
+    return np.array(out, dtype=np.int32)
+
+
+def make_draws(rng, m, W, H, surfs, n, triangles=True):
+    draws = np.zeros(n, dtype=A.DRAW_DTYPE)
+    vdata = []
+    vi = 0
+    lite_levels = [m.levels[i] for i in m.lite_levels]
+    for i in range(n):
+        d = draws[i]
+        d["clip_rect"] = (0, 0, W, H)
+        r = rng.uniform()
+        d["blend"] = 0 if r < 0.6 else int(rng.integers(1, 5))
+        if triangles and i % 5 == 4:
+            lv = m.lite_levels[i % len(m.lite_levels)]
+            L = m.levels[lv]
+            d["kind"] = A.DRAW_TRIANGLE
+            d["texture_offset"] = int(L["offset"])
+            d["texture_size"] = (int(L["width"]), int(L["height"]))
+            d["num_vertices"] = 3
+            v = make_triangle(rng, W, H, int(L["width"]), int(L["height"]))
+        else:
+            s = surfs[i % len(surfs)]
+            tw, th = int(s["size"][0]), int(s["size"][1])
+            d["kind"] = A.DRAW_POLYGON
+            d["mode"] = i % 3
+            d["texture_source"] = A.TEXSRC_SURFACE
+            d["texture_offset"] = int(s["pixels_offset"])
+            d["texture_size"] = (tw, th)
+            v, de, te = make_polygon(rng, W, H, tw, th, d["mode"], thin=(i % 11 == 7))
+            d["num_vertices"] = len(v) // 2
+            d["depth_equation"] = de
+            d["tc_equation"] = te
+        d["first_vertex_int"] = vi
+        vdata.append(v)
+        vi += len(v)
+    return draws, np.concatenate(vdata) if vdata else np.zeros(0, np.int32)
+
+
+def make_batch(seed, color_bits=32, view_sizes=((160, 120),), surfaces_per_view=24, draws_per_view=40,
+               num_lights=8, clear=True, triangles=True):
+    rng = np.random.default_rng(seed)
+    m = make_map(rng)
+    lights = make_lights(rng, num_lights)
+    views = np.zeros(len(view_sizes), dtype=A.VIEW_DTYPE)
+    all_s, all_li, all_d, all_v = [], [], [], []
+    arena_off = fb_off = 0
+    li_base = vi_base = 0
+    for k, (W, H) in enumerate(view_sizes):
+        s, li, texels = make_surfaces(rng, m, surfaces_per_view, num_lights)
+        s["first_light"] += li_base
+        d, vd = make_draws(rng, m, W, H, s, draws_per_view, triangles=triangles)
+        d["first_vertex_int"] += vi_base
+        views[k]["width"], views[k]["height"] = W, H
+        views[k]["cam_pos"] = rng.uniform(-100, 100, size=3)
+        views[k]["clear_background"] = 1 if clear else 0
+        views[k]["background_color"] = 0x00201008 if color_bits == 32 else 0x0000002000100008
+        views[k]["first_surface"] = k * surfaces_per_view
+        views[k]["num_surfaces"] = surfaces_per_view
+        views[k]["first_draw"] = k * draws_per_view
+        views[k]["num_draws"] = draws_per_view
+        views[k]["fb_offset"] = fb_off
+        views[k]["surface_arena_offset"] = arena_off
+        arena_off += texels
+        fb_off += W * H
+        all_s.append(s)
+        all_li.append(li)
+        all_d.append(d)
+        all_v.append(vd)
+        li_base += len(li)
+        vi_base += len(vd)
+    b = A.BatchData(views, np.concatenate(all_s), lights, np.concatenate(all_li) if li_base else np.zeros(0, np.uint32),
+                    np.concatenate(all_d), np.concatenate(all_v), color_bits)
+    return m, b
