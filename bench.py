@@ -1,0 +1,267 @@
+"""Benchmark of the MI355X surface builder + span rasteriser (Square-Wheel's per-frame pixel path).
+
+Default workload (one GPU): the plain drop-in path at 1920x1080, Color32, directional lightmaps,
+on a seeded 32x32x4 grid map (p_solid = 0.35, 32 point lights + sun, 256x256 textures with normal
+maps and mips, a third of the materials specular). A step renders one frame of a 64-pose seeded
+camera path: surface build + draw setup + binning + raster, with the frame's command buffers already
+resident in HBM (the CPU front-end that produces them is not timed, as on the CPU side).
+
+Also measured in the same run:
+  - p50 / p99 frame latency of the plain path,
+  - the batched form: 256 views of log-uniform size 160x120 .. 1920x1080 (seeded poses) rendered by
+    one captured HIP graph per batch, reported as aggregate Gpixel/s,
+  - per-kernel durations with HIP events on the context's stream, for the roofline of the dominant
+    kernel (algorithmic bytes / average launch duration vs 8 TB/s HBM peak),
+  - the CPU baseline: the C++ restatement of the reference's CPU renderer (oracle/), multithreaded
+    with the reference's decomposition, timed on this host's cores on a bounded sample.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "square-wheel_amd"))
+
+import numpy as np  # noqa: E402
+
+import sw_abi  # noqa: E402
+import sw_host  # noqa: E402
+
+METRIC = "1080p frames/s + batched Gpixel/s vs plain (HBM GB/s); p99 frame latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(stats, color_bits, e_bytes):
+    """Bytes the path must move per frame (SURVEY d-formula), split per kernel."""
+    c = 8 if color_bits == 64 else 4
+    surf = stats["surface_texels"] * (8 + c) + stats["lightmap_elements"] * e_bytes + stats["emissive_texels"] * (4 + c)
+    raster = stats["polygon_pixels"] * (c + c) + stats["triangle_pixels"] * (4 + c) + stats["blend_pixels"] * c
+    return {"build_surfaces": surf, "raster_tiles": raster}
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary, if present."""
+    path = os.path.join(ROOT, "profiles", "traffic.json.txt")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        v = data.get(kernel)
+        return None if v is None else float(v)
+    except Exception:
+        return None
+
+
+def build_frames(m, poses, width, height, color_bits):
+    fb = sw_host.FrameBuilder()
+    frames = []
+    for p in poses:
+        frames.append(fb.build(m, p, width=width, height=height, color_bits=color_bits, use_directional_lightmaps=1,
+                               clear_background=1, force_clear=1))
+    return frames
+
+
+def cpu_baseline(m, batches, seconds, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # the CPU restatement; timed as the baseline only
+
+    oracle.load()
+    n = 0
+    t0 = time.perf_counter()
+    pixels = 0
+    while True:
+        b = batches[n % len(batches)]
+        oracle.render(m.data, b, mode=oracle.MODE_SSE, num_threads=threads)
+        n += 1
+        pixels += int(b.views[0]["width"]) * int(b.views[0]["height"])
+        el = time.perf_counter() - t0
+        if el >= seconds or n >= 4096:
+            break
+    return n / el, n, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=384)
+    ap.add_argument("--warmup", type=int, default=64)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--poses", type=int, default=64)
+    ap.add_argument("--batch-views", type=int, default=256)
+    ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); views shard across GPUs with no "
+              "collective, see DESIGN.md", flush=True)
+        sys.exit(2)
+
+    color_bits = 32
+    t = time.perf_counter()
+    m = sw_host.config1_map(args.seed)
+    log(f"map built in {time.perf_counter() - t:.1f}s: {m.info}")
+    poses = m.camera_path(args.seed + 1, args.poses)
+    frames = build_frames(m, poses, args.width, args.height, color_bits)
+    batches = [sw_host.batch_from_frames([f]) for f in frames]
+    log(f"{len(frames)} frames built; draws/frame {np.mean([len(f['draws']) for f in frames]):.0f}, "
+        f"surface texels/frame {np.mean([f['arena_texels'] for f in frames]):.0f}")
+
+    ctx = sw_abi.Context(0)
+    mh = ctx.upload_map(m.data)
+    dev = [sw_abi.DeviceCommandBuffers(ctx, b) for b in batches]
+    out = ctx.alloc(args.width * args.height * 4)
+    lib = ctx.lib
+    import ctypes as C
+    cbs = [b.ctypes_batch(inputs_on_device=True, device_ptrs=d.ptrs()) for b, d in zip(batches, dev)]
+
+    def render(i):
+        r = lib.sw_render_batch(ctx.h, mh, C.byref(cbs[i % len(cbs)]), out.ptr)
+        if r != 0:
+            raise RuntimeError(lib.sw_last_error(ctx.h).decode())
+
+    # per-frame work counters (identical every pass of the path)
+    per_frame_stats = []
+    for i in range(len(frames)):
+        render(i)
+        st = ctx.stats()
+        st["surface_texels"] = int(frames[i]["arena_texels"])
+        s = frames[i]["surfaces"]
+        st["lightmap_elements"] = int(np.sum(2 * s["lightmap_size"][:, 0].astype(np.int64) * s["size"][:, 1] *
+                                             (s["lightmap_offset"] != 0)))
+        st["emissive_texels"] = 0
+        per_frame_stats.append(st)
+
+    for i in range(args.warmup):
+        render(i)
+    ctx.profile_read(reset=True)
+    ctx.profile_enable(True)
+    lat = np.zeros(args.steps)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ts = time.perf_counter()
+        render(i)
+        lat[i] = time.perf_counter() - ts
+    wall = time.perf_counter() - t0
+    ctx.profile_enable(False)
+    prof = ctx.profile_read(reset=True)
+    fps = args.steps / wall
+    log(f"plain: {fps:.1f} frames/s, p50 {np.percentile(lat, 50) * 1e3:.3f} ms, p99 {np.percentile(lat, 99) * 1e3:.3f} ms")
+    log(f"kernels: { {k: (round(v[0] / max(v[1], 1) * 1e3, 2), v[1]) for k, v in prof.items()} } (us/launch, launches)")
+
+    # roofline of the dominant kernel over the timed region
+    steps_stats = [per_frame_stats[i % len(per_frame_stats)] for i in range(args.steps)]
+    bytes_sum = {"build_surfaces": 0, "raster_tiles": 0}
+    for st in steps_stats:
+        for k, v in algorithmic_bytes(st, color_bits, 40).items():
+            bytes_sum[k] += v
+    kernel_ms = {k: prof[k][0] for k in ("build_surfaces", "raster_tiles")}
+    dominant = max(kernel_ms, key=kernel_ms.get)
+    launches = max(prof[dominant][1], 1)
+    avg_s = prof[dominant][0] / 1e3 / launches
+    achieved = bytes_sum[dominant] / launches / avg_s / 1e9
+    traffic = read_traffic(dominant)
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dominant,
+                "algorithmic_bytes_per_launch": int(bytes_sum[dominant] / launches),
+                "avg_launch_us": round(avg_s * 1e6, 3)}
+
+    result = {
+        "metric": METRIC,
+        "value": round(fps, 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 surfaces / int32 spans, Color32 out",
+        "data": "synthetic (seeded grid map, value-noise textures, generated lightmaps)",
+        "config": {"workload": f"configs[1] plain drop-in: {args.width}x{args.height} Color32, directional lightmaps, "
+                               f"32x32x4 grid p=0.35, 32 lights + sun, 256^2 textures + normal maps, "
+                               f"{args.poses}-pose camera path, command buffers resident in HBM"},
+        "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 4),
+        "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 4),
+        "plain_gpixel_s": round(fps * args.width * args.height / 1e9, 4),
+        "kernel_us_per_launch": {k: round(v[0] / max(v[1], 1) * 1e3, 3) for k, v in prof.items()},
+        "roofline": roofline,
+    }
+
+    if not args.no_batch:
+        rng = np.random.default_rng(args.seed + 17)
+        bposes = m.camera_path(args.seed + 2, args.batch_views)
+        fbld = sw_host.FrameBuilder()
+        bframes = []
+        total_px = 0
+        for i in range(args.batch_views):
+            wid = int(round(math.exp(rng.uniform(math.log(160), math.log(1920)))))
+            hei = max(1, int(round(wid * (9 / 16 if rng.uniform() < 0.5 else 3 / 4))))
+            wid, hei = min(wid, 1920), min(hei, 1080)
+            bframes.append(fbld.build(m, bposes[i], width=wid, height=hei, color_bits=color_bits, use_directional_lightmaps=1,
+                                      clear_background=1, force_clear=1))
+            total_px += wid * hei
+        bb = sw_host.batch_from_frames(bframes)
+        bdev = sw_abi.DeviceCommandBuffers(ctx, bb)
+        bout = ctx.alloc(bb.fb_texels * 4)
+        bcb = bb.ctypes_batch(inputs_on_device=True, device_ptrs=bdev.ptrs())
+        ctx.check(lib.sw_render_batch_graph(ctx.h, mh, C.byref(bcb), bout.ptr), "sw_render_batch_graph")
+        ctx.profile_enable(True)
+        ctx.profile_read(reset=True)
+        tb = time.perf_counter()
+        for _ in range(args.batch_reps):
+            ctx.check(lib.sw_render_batch_graph(ctx.h, mh, C.byref(bcb), bout.ptr), "sw_render_batch_graph")
+        bwall = (time.perf_counter() - tb) / args.batch_reps
+        ctx.profile_enable(False)
+        bst = ctx.stats()
+        surf_texels = int(bb.arena_texels)
+        s = bb.surfaces
+        lm = int(np.sum(2 * s["lightmap_size"][:, 0].astype(np.int64) * s["size"][:, 1] * (s["lightmap_offset"] != 0)))
+        bst.update(surface_texels=surf_texels, lightmap_elements=lm, emissive_texels=0)
+        bbytes = sum(algorithmic_bytes(bst, color_bits, 40).values())
+        result["batched"] = {"views": args.batch_views, "pixels": total_px, "ms_per_batch": round(bwall * 1e3, 3),
+                             "gpixel_s": round(total_px / bwall / 1e9, 4),
+                             "vs_plain_gpixel_s": round((total_px / bwall) / (fps * args.width * args.height), 3),
+                             "achieved_gb_s": round(bbytes / bwall / 1e9, 2),
+                             "hbm_frac": round(bbytes / bwall / 1e9 / HBM_PEAK_GBS, 5),
+                             "surfaces": int(len(bb.surfaces)), "draws": int(len(bb.draws))}
+        log(f"batched: {result['batched']}")
+        bdev.free()
+        bout.free()
+
+    if not args.no_cpu:
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        threads = max(1, min(threads, 64))
+        cfps, n, el = cpu_baseline(m, batches, args.cpu_seconds, threads)
+        result["cpu_baseline"] = {"value": round(cfps, 3), "unit": "frames/s", "cores": threads, "kind": "port",
+                                  "sample": f"{n} frames of the same {args.poses}-pose 1080p path in {el:.1f}s "
+                                            f"(surface build + raster, reference decomposition over {threads} threads, "
+                                            "x86 rsqrtss/rcpss numerics)"}
+        result["speedup_vs_cpu"] = round(fps / cfps, 2)
+        log(f"cpu baseline: {cfps:.2f} frames/s on {threads} threads")
+
+    for d in dev:
+        d.free()
+    out.free()
+    ctx.release_map(mh)
+    ctx.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

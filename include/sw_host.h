@@ -1,0 +1,119 @@
+/*
+ * sw_host.h -- C API of the CPU-side renderer front-end (libsw_host.so).
+ *
+ * This is the caller of the drop-in boundary in sw_gpu.h: the part of Square-Wheel's renderer that
+ * stays on the CPU (visibility, per-polygon setup, draw ordering), restated in C++ because the
+ * reference's Rust toolchain is not available here. Per view it emits exactly the command buffers
+ * sw_gpu.h consumes:
+ *
+ *   swh_frame_build  <- PartialRenderer::prepare_frame + draw_frame up to the pixel loops
+ *                       (partial_renderer.rs:156-261): MapVisibilityCalculator::update_visibility
+ *                       (map_visibility_calculator.rs:52-97), prepare_polygon_surface (:1464-1638),
+ *                       build_polygons_surfaces' per-surface parameters (:1682-1820), draw_tree /
+ *                       draw_leaf / draw_polygon (:2450-2576, :4011-4142), decals and triangle models.
+ *
+ * It also generates the seeded content the benchmark configurations name (the reference's
+ * map_compiler / lightmapper cannot run here): a voxel-grid BSP map with portals, value-noise
+ * textures with normal maps and mip chains (textures.rs:93-362), and directional / simple
+ * lightmaps from point lights and a sun.
+ */
+#ifndef SW_HOST_H
+#define SW_HOST_H
+
+#include <stdint.h>
+
+#include "sw_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct swh_map swh_map;
+typedef struct swh_frame swh_frame;
+
+typedef struct swh_map_config
+{
+	uint32_t seed;
+	uint32_t grid[3]; /* cells along x, y, z (128 units each) */
+	float solid_probability;
+	uint32_t num_lights; /* white point lights */
+	float light_intensity_min, light_intensity_max;
+	uint32_t sun; /* 1: sun shining through sky openings in the top layer */
+	float sky_fraction; /* fraction of top-layer ceilings that are sky */
+	uint32_t texture_size; /* 64 / 256 ... (power of two >= 16) */
+	uint32_t num_materials;
+	uint32_t normal_maps; /* 1: tangent-space normal maps from a Gaussian-bumped height field */
+	float specular_fraction; /* share of materials with roughness U[0.1, 0.6] (half of them metal) */
+	float alpha_blend_fraction; /* share of materials drawn with BlendingMode::AlphaBlend */
+	uint32_t directional_lightmaps; /* 1: build DirectionalLightmapElement data too */
+	uint32_t num_decal_textures; /* lite 32x32 decal textures */
+	uint32_t num_model_textures; /* lite 64x64 model skins */
+} swh_map_config;
+
+typedef struct swh_map_info
+{
+	uint32_t num_polygons, num_leafs, num_nodes, num_portals, num_empty_cells;
+	uint64_t num_texels, num_lightmap_elements;
+} swh_map_info;
+
+typedef struct swh_frame_config
+{
+	uint32_t width, height;
+	uint32_t color_bits; /* 32 or 64 */
+	float fov; /* radians, horizontal (matrix.rs:31-50) */
+	float textures_mip_bias;
+	uint32_t use_directional_lightmaps;
+	uint32_t clear_background; /* RendererConfig::clear_background */
+	uint32_t force_clear; /* clear even when the camera is inside its leaf volume */
+	uint32_t num_dynamic_lights; /* seeded DynamicLight (shadow None) */
+	uint32_t num_models; /* seeded animated triangle models */
+	uint32_t num_decals; /* seeded decals */
+	uint32_t seed; /* dynamic objects seed */
+	float time; /* animation time, seconds */
+	uint32_t virtual_threads; /* 1: full-screen clip rect (the parity configuration) */
+} swh_frame_config;
+
+/* A view's command buffers, valid until the frame is rebuilt or destroyed. */
+typedef struct swh_frame_data
+{
+	sw_view view; /* offsets zero; first_* = 0 */
+	const sw_surface_desc* surfaces;
+	uint32_t num_surfaces;
+	const sw_light* lights;
+	uint32_t num_lights;
+	const uint32_t* light_indices;
+	uint32_t num_light_indices;
+	const sw_draw* draws;
+	uint32_t num_draws;
+	const int32_t* vertex_data;
+	uint64_t num_vertex_ints;
+	uint64_t surface_arena_texels;
+	uint32_t num_visible_leafs;
+	uint32_t num_triangles;
+} swh_frame_data;
+
+int swh_map_create(const swh_map_config* config, swh_map** out_map);
+void swh_map_destroy(swh_map* map);
+/* Pointers into host arrays owned by the map (upload them with sw_map_upload). */
+int swh_map_desc(const swh_map* map, sw_map_desc* out_desc);
+int swh_map_info_get(const swh_map* map, swh_map_info* out_info);
+/* n seeded camera poses at empty-cell centres: out = n x {x, y, z, yaw, pitch}. */
+int swh_camera_path(const swh_map* map, uint32_t seed, uint32_t n, float* out_poses);
+
+int swh_frame_create(swh_frame** out_frame);
+void swh_frame_destroy(swh_frame* frame);
+int swh_frame_build(swh_frame* frame, const swh_map* map, const swh_frame_config* config, const float cam_pos[3],
+	float yaw, float pitch);
+int swh_frame_get(const swh_frame* frame, swh_frame_data* out_data);
+
+/* Map-side building blocks, exposed for the tests. */
+/* make_texture (textures.rs:93-161) of a single texel: normal map RGB + roughness -> packed. */
+int32_t swh_pack_normal_roughness_from_rgb(uint32_t normal_rgb, float roughness);
+/* build_texture_mips (textures.rs:288-362): mip0 (w x h) -> mips 1..4 back to back. */
+uint64_t swh_build_texture_mips(const sw_texel* mip0, uint32_t w, uint32_t h, sw_texel* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SW_HOST_H */

@@ -1,0 +1,979 @@
+// Per-frame CPU front-end: camera, portal visibility, per-polygon surface setup, draw ordering and
+// draw setup. It emits the command buffers of include/sw_gpu.h for one view.
+//
+// Restates, in order of execution:
+//   matrix.rs:13-109                               camera matrices
+//   map_visibility_calculator.rs:52-326            portal flood with octagon bounds
+//   partial_renderer.rs:1327-1638                  prepare_polygons_surfaces / prepare_polygon_surface
+//   renderer_utils.rs:10-58, :60-242               projection + clipping, interpolation-mode tests, mip
+//   partial_renderer.rs:1640-1820                  per-surface parameters of build_polygons_surfaces
+//   partial_renderer.rs:3877-3985                  polygon_is_affected_by_light
+//   partial_renderer.rs:2311-2448                  draw_skybox
+//   partial_renderer.rs:2450-2576, :4011-4142      draw_tree / draw_leaf / draw_polygon
+//   common/clipping.rs, common/clipping_polygon.rs 3D / 2D clipping, ClippingPolygon
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <random>
+
+#include "host_internal.h"
+#include "host_frame_state.h"
+
+using namespace swh;
+
+namespace swh
+{
+
+
+// ---------------------------------------------------------------------------------------------
+// clipping.rs
+// ---------------------------------------------------------------------------------------------
+V3 line_z_intersection(V3 v0, V3 v1, float z)
+{
+	const float dist0 = v0.z - z, dist1 = v1.z - z, dist_sum = v1.z - v0.z;
+	const float k0 = dist0 / dist_sum, k1 = dist1 / dist_sum;
+	return v0 * k1 - v1 * k0;
+}
+
+uint32_t clip_3d_polygon_by_z_plane(const V3* poly, uint32_t n, float z_dist, V3* out, uint32_t cap)
+{
+	if(n == 0)
+		return 0;
+	const V3* prev = &poly[n - 1];
+	uint32_t count = 0;
+	for(uint32_t i = 0; i < n; ++i)
+	{
+		const V3& v = poly[i];
+		if(v.z > z_dist)
+		{
+			if(prev->z < z_dist)
+			{
+				out[count++] = line_z_intersection(*prev, v, z_dist);
+				if(count == cap)
+					break;
+			}
+			out[count++] = v;
+			if(count == cap)
+				break;
+		}
+		else if(v.z == z_dist)
+		{
+			out[count++] = v;
+			if(count == cap)
+				break;
+		}
+		else if(prev->z > z_dist)
+		{
+			out[count++] = line_z_intersection(*prev, v, z_dist);
+			if(count == cap)
+				break;
+		}
+		prev = &v;
+	}
+	return count;
+}
+
+V2 line_line_intersection(V2 v0, V2 v1, V3 line)
+{
+	const float dist0 = dot(v0, v2(line.x, line.y)) - line.z, dist1 = dot(v1, v2(line.x, line.y)) - line.z;
+	const float dist_sum = dist1 - dist0;
+	const float k0 = dist0 / dist_sum, k1 = dist1 / dist_sum;
+	return v0 * k1 - v1 * k0;
+}
+
+uint32_t clip_2d_polygon(const V2* poly, uint32_t n, V3 line, V2* out, uint32_t cap)
+{
+	if(n == 0)
+		return 0;
+	const V2* prev = &poly[n - 1];
+	float prev_dist = dot(*prev, v2(line.x, line.y)) - line.z;
+	uint32_t count = 0;
+	for(uint32_t i = 0; i < n; ++i)
+	{
+		const V2& v = poly[i];
+		const float dist = dot(v, v2(line.x, line.y)) - line.z;
+		if(dist > 0.0f)
+		{
+			if(prev_dist < 0.0f)
+			{
+				out[count++] = line_line_intersection(*prev, v, line);
+				if(count == cap)
+					break;
+			}
+			out[count++] = v;
+			if(count == cap)
+				break;
+		}
+		else if(dist == 0.0f)
+		{
+			out[count++] = v;
+			if(count == cap)
+				break;
+		}
+		else if(prev_dist > 0.0f)
+		{
+			out[count++] = line_line_intersection(*prev, v, line);
+			if(count == cap)
+				break;
+		}
+		prev = &v;
+		prev_dist = dist;
+	}
+	return count;
+}
+
+// renderer_utils.rs:10-58
+uint32_t project_and_clip_polygon(const V3 clip_planes[8], const V3* verts, uint32_t n, V2* out)
+{
+	uint32_t count = std::min(n, MAX_VERTICES);
+	V3 zc[MAX_VERTICES];
+	count = clip_3d_polygon_by_z_plane(verts, count, Z_NEAR, zc, MAX_VERTICES);
+	if(count < 3)
+		return count;
+	for(uint32_t i = 0; i < count; ++i)
+		out[i] = v2(zc[i].x, zc[i].y) / zc[i].z;
+	V2 tmp[MAX_VERTICES];
+	for(int i = 0; i < 4; ++i)
+	{
+		count = clip_2d_polygon(out, count, clip_planes[i * 2], tmp, MAX_VERTICES);
+		if(count < 3)
+			return count;
+		count = clip_2d_polygon(tmp, count, clip_planes[i * 2 + 1], out, MAX_VERTICES);
+		if(count < 3)
+			return count;
+	}
+	return count;
+}
+
+// ---------------------------------------------------------------------------------------------
+// equations.rs
+// ---------------------------------------------------------------------------------------------
+DepthEq depth_from_plane(V4 p)
+{
+	const float w = -p.w;
+	return DepthEq{p.x / w, p.y / w, p.z / w};
+}
+
+TcEq tc_from_depth_and_planes(const DepthEq& d, const V4 t[2])
+{
+	TcEq e;
+	for(int i = 0; i < 2; ++i)
+	{
+		e.dx[i] = t[i].x + t[i].w * d.dx;
+		e.dy[i] = t[i].y + t[i].w * d.dy;
+		e.k[i] = t[i].z + t[i].w * d.k;
+	}
+	return e;
+}
+
+DepthEq scale(const DepthEq& d, float s) { return DepthEq{d.dx * s, d.dy * s, d.k * s}; }
+TcEq scale(const TcEq& e, float s)
+{
+	TcEq r;
+	for(int i = 0; i < 2; ++i)
+	{
+		r.dx[i] = e.dx[i] * s;
+		r.dy[i] = e.dy[i] * s;
+		r.k[i] = e.k[i] * s;
+	}
+	return r;
+}
+
+// ---------------------------------------------------------------------------------------------
+// renderer_utils.rs:60-242
+// ---------------------------------------------------------------------------------------------
+const float TC_ERROR_THRESHOLD = 0.75f;
+
+bool affine_may_be_used(const DepthEq& de, const TcEq& te, V2 min_p, V2 max_p)
+{
+	const V2 edge = max_p - min_p;
+	const float edge_square_len = dot(edge, edge);
+	if(edge_square_len == 0.0f)
+		return true;
+	const float edge_len = std::sqrt(edge_square_len);
+	const V2 en = edge / edge_len;
+	const float inv_z_clamp = 1.0f / float(1 << 20);
+	const float min_point_inv_z = std::fmax(de.sample(min_p), inv_z_clamp);
+	const float max_point_inv_z = std::fmax(de.sample(max_p), inv_z_clamp);
+	(void)max_point_inv_z;
+	const float a = dot(v2(de.dx, de.dy), en);
+	const float b = min_point_inv_z;
+	if(std::fabs(a) < 1.0e-10f)
+		return true;
+	const float b_div_a = b / a;
+	const float max_diff_point = std::sqrt((0.0f + b_div_a) * (edge_len + b_div_a)) - b_div_a;
+	const float max_diff_point_inv_z = a * max_diff_point + b;
+	for(int i = 0; i < 2; ++i)
+	{
+		const float min_point_tc = te.dx[i] * min_p.x + te.dy[i] * min_p.y + te.k[i];
+		const float max_point_tc = te.dx[i] * max_p.x + te.dy[i] * max_p.y + te.k[i];
+		const float tc_a = dot(v2(te.dx[i], te.dy[i]), en);
+		const float tc_b = min_point_tc;
+		const float min_mul = min_point_tc / min_point_inv_z;
+		const float max_mul = max_point_tc / std::fmax(de.sample(max_p), inv_z_clamp);
+		const float real = (tc_a * max_diff_point + tc_b) / max_diff_point_inv_z;
+		const float approx = min_mul + (max_mul - min_mul) * (max_diff_point - 0.0f) / (edge_len - 0.0f);
+		if(std::fabs(real - approx) > TC_ERROR_THRESHOLD)
+			return false;
+	}
+	return true;
+}
+
+bool line_z_may_be_used(const DepthEq& de, const TcEq& te, V2 max_p, float min_x, float max_x)
+{
+	if(max_x - min_x < 1.0f)
+		return true;
+	const float a = de.dx;
+	const float b = de.dy * max_p.y + de.k;
+	if(std::fabs(a) < 1.0e-10f)
+		return true;
+	const float b_div_a = b / a;
+	const float max_diff_x = std::sqrt((min_x + b_div_a) * (max_x + b_div_a)) - b_div_a;
+	const float max_diff_point_inv_z = a * max_diff_x + b;
+	const float inv_z_at_min_x = a * min_x + b;
+	const float inv_z_at_max_x = a * max_x + b;
+	const float almost_zero = 1e-20f;
+	if(inv_z_at_min_x <= almost_zero || inv_z_at_max_x <= almost_zero || max_diff_point_inv_z <= almost_zero)
+		return false;
+	for(int i = 0; i < 2; ++i)
+	{
+		const float ta = te.dx[i];
+		const float tb = te.dy[i] * max_p.y + te.k[i];
+		const float tc_at_min_x = (ta * min_x + tb) / inv_z_at_min_x;
+		const float tc_at_max_x = (ta * max_x + tb) / inv_z_at_max_x;
+		const float real = (ta * max_diff_x + tb) / max_diff_point_inv_z;
+		const float approx = tc_at_min_x + (tc_at_max_x - tc_at_min_x) * (max_diff_x - min_x) / (max_x - min_x);
+		if(std::fabs(real - approx) > TC_ERROR_THRESHOLD)
+			return false;
+	}
+	return true;
+}
+
+uint32_t calculate_mip(const V2* points, uint32_t n, const DepthEq& de, const TcEq& te, float mip_bias)
+{
+	V2 mip_point = points[0];
+	float mip_point_inv_z = 0.0f;
+	for(uint32_t i = 0; i < n; ++i)
+	{
+		const float inv_z = de.sample(points[i]);
+		if(inv_z > mip_point_inv_z)
+		{
+			mip_point_inv_z = inv_z;
+			mip_point = points[i];
+		}
+	}
+	const float z_2 = 1.0f / (mip_point_inv_z * mip_point_inv_z);
+	const float z_4 = z_2 * z_2;
+	float d_tc_2[2];
+	for(int i = 0; i < 2; ++i)
+	{
+		const float d_tc_dx = te.dx[i] * (de.k + de.dy * mip_point.y) - (te.k[i] + te.dy[i] * mip_point.y) * de.dx;
+		const float d_tc_dy = te.dy[i] * (de.k + de.dx * mip_point.x) - (te.k[i] + te.dx[i] * mip_point.x) * de.dy;
+		d_tc_2[i] = (d_tc_dx * d_tc_dx + d_tc_dy * d_tc_dy) * z_4;
+	}
+	const float max_d_tc_2 = std::fmax(d_tc_2[0], d_tc_2[1]);
+	const float mip_f = std::log2(max_d_tc_2) * 0.5f + mip_bias;
+	const float c = std::ceil(mip_f);
+	// Rust `as i32`: saturating, NaN -> 0
+	int32_t mi = std::isnan(c) ? 0 : (c >= 2147483647.0f ? INT32_MAX : (c <= -2147483648.0f ? INT32_MIN : int32_t(c)));
+	return uint32_t(std::max(0, std::min(mi, int32_t(MAX_MIP))));
+}
+
+// f32 -> Fixed16 with x86 cvttss2si semantics (fixed_math.rs:36-39)
+int32_t f32_to_fixed16_unchecked(float x)
+{
+	const float s = x * 65536.0f;
+	if(!(s >= -2147483648.0f && s < 2147483648.0f))
+		return INT32_MIN;
+	return int32_t(s);
+}
+
+int32_t saturating_f32_to_i32(float x)
+{
+	if(std::isnan(x))
+		return 0;
+	if(x >= 2147483647.0f)
+		return INT32_MAX;
+	if(x <= -2147483648.0f)
+		return INT32_MIN;
+	return int32_t(x);
+}
+
+V3 view_transform(const M4& m, V3 v) { return truncate(mul(m, extend(v, 1.0f))); }
+
+// Quaternion (cgmath) just far enough for the camera and the skybox
+struct Quat
+{
+	float s;
+	V3 v;
+};
+Quat quat_mul(Quat a, Quat b)
+{
+	return Quat{a.s * b.s - a.v.x * b.v.x - a.v.y * b.v.y - a.v.z * b.v.z,
+		v3(a.s * b.v.x + a.v.x * b.s + a.v.y * b.v.z - a.v.z * b.v.y, a.s * b.v.y + a.v.y * b.s + a.v.z * b.v.x - a.v.x * b.v.z,
+			a.s * b.v.z + a.v.z * b.s + a.v.x * b.v.y - a.v.y * b.v.x)};
+}
+Quat from_angle_z(float a) { return Quat{std::cos(a * 0.5f), v3(0.0f, 0.0f, std::sin(a * 0.5f))}; }
+Quat from_angle_y(float a) { return Quat{std::cos(a * 0.5f), v3(0.0f, std::sin(a * 0.5f), 0.0f)}; }
+M4 quat_to_m4(Quat q)
+{
+	const float x2 = q.v.x + q.v.x, y2 = q.v.y + q.v.y, z2 = q.v.z + q.v.z;
+	const float xx2 = x2 * q.v.x, xy2 = x2 * q.v.y, xz2 = x2 * q.v.z, yy2 = y2 * q.v.y, yz2 = y2 * q.v.z, zz2 = z2 * q.v.z;
+	const float sy2 = y2 * q.s, sz2 = z2 * q.s, sx2 = x2 * q.s;
+	M4 m = identity();
+	m.c[0] = v4(1.0f - yy2 - zz2, xy2 + sz2, xz2 - sy2, 0.0f);
+	m.c[1] = v4(xy2 - sz2, 1.0f - xx2 - zz2, yz2 + sx2, 0.0f);
+	m.c[2] = v4(xz2 + sy2, yz2 - sx2, 1.0f - xx2 - yy2, 0.0f);
+	return m;
+}
+
+struct Camera
+{
+	V3 position;
+	M4 view_matrix;
+	M4 planes_matrix;
+};
+
+// matrix.rs:13-109
+Camera build_camera(V3 position, float azimuth, float elevation, float fov, float width, float height)
+{
+	const Quat rotation = quat_mul(from_angle_z(azimuth + 3.14159265f * 0.5f), from_angle_y(-elevation));
+	const float inv_half_fov_tan = 1.0f / std::tan(fov * 0.5f);
+	const float aspect = width / height;
+	const V2 fov_scale = v2(inv_half_fov_tan / aspect, inv_half_fov_tan);
+	M4 basis_change = identity();
+	basis_change.c[0].x = 0.0f;
+	basis_change.c[1].y = 0.0f;
+	basis_change.c[2].z = 0.0f;
+	basis_change.c[0].z = 1.0f;
+	basis_change.c[1].x = -1.0f;
+	basis_change.c[2].y = -1.0f;
+	const Quat conj = Quat{rotation.s, v3(-rotation.v.x, -rotation.v.y, -rotation.v.z)};
+	const M4 rotation_matrix = mul(basis_change, quat_to_m4(conj));
+	const M4 translate = from_translation(v3(-position.x, -position.y, -position.z));
+	const M4 perspective = from_nonuniform_scale(fov_scale.x, fov_scale.y, 1.0f);
+	const M4 resize = from_nonuniform_scale(width * 0.5f, height * 0.5f, 1.0f);
+	M4 shift = identity();
+	shift.c[2].x = width * 0.5f;
+	shift.c[2].y = height * 0.5f;
+	Camera c;
+	c.position = position;
+	c.view_matrix = mul(mul(mul(mul(shift, resize), perspective), rotation_matrix), translate);
+	if(!invert(transpose(c.view_matrix), c.planes_matrix))
+		c.planes_matrix = identity();
+	return c;
+}
+
+} // namespace swh
+
+namespace
+{
+
+void ensure_basis(swh_frame& f, const swh_map& m)
+{
+	if(f.basis_map == &m && f.basis.size() == m.polygons.size())
+		return;
+	f.basis_map = &m;
+	f.basis.resize(m.polygons.size());
+	f.poly.assign(m.polygons.size(), swh_frame::PolyData{});
+	f.vertices_transformed.assign(m.vertices.size(), v3(0, 0, 0));
+	f.leaf_visible_frame.assign(m.leafs.size(), 0);
+	f.leaf_bounds.assign(m.leafs.size(), ClippingPolygon{});
+	f.portal_visible_frame.assign(m.portals.size(), 0);
+	f.portal_has_projection.assign(m.portals.size(), 0);
+	f.portal_projection.assign(m.portals.size(), ClippingPolygon{});
+	f.frame_number = 0;
+	for(size_t i = 0; i < m.polygons.size(); ++i)
+	{
+		// PolygonBasisVecs::form_plane_and_tex_coord_equation (surfaces.rs:18-47)
+		const Polygon& p = m.polygons[i];
+		M4 b;
+		b.c[0] = extend(p.tex_coord_equation[0].vec, p.tex_coord_equation[0].dist);
+		b.c[1] = extend(p.tex_coord_equation[1].vec, p.tex_coord_equation[1].dist);
+		b.c[2] = extend(p.plane.vec, -p.plane.dist);
+		b.c[3] = v4(0, 0, 0, 1);
+		M4 inv;
+		swh_frame::Basis& out = f.basis[i];
+		if(!invert(transpose(b), inv))
+		{
+			out = swh_frame::Basis{v3(1, 0, 0), v3(0, 1, 0), v3(0, 0, 0), v3(0, 0, 1)};
+			continue;
+		}
+		out.u = truncate(inv.c[0]);
+		out.v = truncate(inv.c[1]);
+		out.start = truncate(inv.c[3]);
+		out.normal = p.plane.vec / std::fmax(length(p.plane.vec), 1.0f / float(1 << 30));
+	}
+}
+
+// map_visibility_calculator.rs:286-326
+bool project_portal(const swh_map& m, const Portal& portal, const M4& view, ClippingPolygon& out)
+{
+	uint32_t n = std::min(portal.num_vertices, MAX_VERTICES);
+	V3 t[MAX_VERTICES], zc[MAX_VERTICES];
+	for(uint32_t i = 0; i < n; ++i)
+		t[i] = view_transform(view, m.vertices[portal.first_vertex + i]);
+	n = clip_3d_polygon_by_z_plane(t, n, 1.0f / 4096.0f, zc, MAX_VERTICES);
+	if(n < 3)
+		return false;
+	out = ClippingPolygon::from_point(v2(zc[0].x, zc[0].y) / zc[0].z);
+	for(uint32_t i = 1; i < n; ++i)
+		out.extend_point(v2(zc[i].x, zc[i].y) / zc[i].z);
+	return true;
+}
+
+// map_visibility_calculator.rs:52-284
+void update_visibility(swh_frame& f, const swh_map& m, const Camera& cam, const ClippingPolygon& frame_bounds)
+{
+	const uint32_t current_leaf = find_leaf(m, cam.position);
+	const uint32_t frame = f.frame_number;
+	uint32_t start_leafs[32];
+	uint32_t num_start = 1;
+	start_leafs[0] = current_leaf;
+	const Leaf& leaf_value = m.leafs[current_leaf];
+	for(uint32_t k = 0; k < leaf_value.num_leaf_portals; ++k)
+	{
+		const Portal& p = m.portals[m.leafs_portals[leaf_value.first_leaf_portal + k]];
+		const float scaled_dist = dot(p.plane.vec, cam.position) - p.plane.dist;
+		const float eps = Z_NEAR * 2.0f;
+		if(std::fabs(scaled_dist) <= eps * length(p.plane.vec))
+		{
+			start_leafs[num_start++] = p.leafs[0] == current_leaf ? p.leafs[1] : p.leafs[0];
+			if(num_start == 32)
+				break;
+		}
+	}
+	std::vector<uint32_t> cur, next;
+	for(uint32_t i = 0; i < num_start; ++i)
+	{
+		cur.push_back(start_leafs[i]);
+		f.leaf_bounds[start_leafs[i]] = frame_bounds;
+		f.leaf_visible_frame[start_leafs[i]] = frame;
+	}
+	int depth = 0;
+	while(!cur.empty())
+	{
+		for(uint32_t leaf : cur)
+		{
+			const ClippingPolygon leaf_bounds = f.leaf_bounds[leaf];
+			const Leaf& lv = m.leafs[leaf];
+			for(uint32_t k = 0; k < lv.num_leaf_portals; ++k)
+			{
+				const uint32_t pi = m.leafs_portals[lv.first_leaf_portal + k];
+				const Portal& portal = m.portals[pi];
+				const float portal_plane_pos = mul(cam.planes_matrix, extend(portal.plane.vec, -portal.plane.dist)).w;
+				uint32_t next_leaf;
+				if(portal.leafs[0] == leaf)
+				{
+					if(portal_plane_pos <= 0.0f)
+						continue;
+					next_leaf = portal.leafs[1];
+				}
+				else
+				{
+					if(portal_plane_pos >= 0.0f)
+						continue;
+					next_leaf = portal.leafs[0];
+				}
+				if(f.portal_visible_frame[pi] != frame)
+				{
+					f.portal_visible_frame[pi] = frame;
+					f.portal_has_projection[pi] = project_portal(m, portal, cam.view_matrix, f.portal_projection[pi]) ? 1 : 0;
+				}
+				if(!f.portal_has_projection[pi])
+					continue;
+				ClippingPolygon bi = f.portal_projection[pi];
+				bi.intersect(leaf_bounds);
+				if(!bi.valid())
+					continue;
+				if(f.leaf_visible_frame[next_leaf] != frame)
+				{
+					f.leaf_visible_frame[next_leaf] = frame;
+					f.leaf_bounds[next_leaf] = bi;
+				}
+				else
+				{
+					if(f.leaf_bounds[next_leaf].contains(bi))
+						continue;
+					f.leaf_bounds[next_leaf].extend(bi);
+				}
+				next.push_back(next_leaf);
+			}
+		}
+		cur.clear();
+		std::swap(cur, next);
+		if(++depth > 1024)
+			break;
+	}
+	// is_inside_leaf_volume (:269-283)
+	f.inside_leaf_volume = true;
+	for(uint32_t k = 0; k < leaf_value.num_polygons; ++k)
+	{
+		const Polygon& p = m.polygons[leaf_value.first_polygon + k];
+		if(mul(cam.planes_matrix, extend(p.plane.vec, -p.plane.dist)).w < 0.0f)
+		{
+			f.inside_leaf_volume = false;
+			break;
+		}
+	}
+}
+
+// partial_renderer.rs:1464-1638
+void prepare_polygon_surface(swh_frame& f, const swh_map& m, const Camera& cam, const V3 clip_planes[8], uint64_t& offset,
+	uint32_t pi, float mip_bias)
+{
+	swh_frame::PolyData& pd = f.poly[pi];
+	const Polygon& polygon = m.polygons[pi];
+	const V4 plane_transformed = mul(cam.planes_matrix, extend(polygon.plane.vec, -polygon.plane.dist));
+	if(plane_transformed.w <= 0.0f)
+		return;
+	const Material& material = m.materials[polygon.texture];
+	for(uint32_t k = 0; k < polygon.num_vertices; ++k)
+		f.vertices_transformed[polygon.first_vertex + k] = view_transform(cam.view_matrix, m.vertices[polygon.first_vertex + k]);
+	V2 v2d[MAX_VERTICES];
+	const uint32_t n = project_and_clip_polygon(clip_planes, &f.vertices_transformed[polygon.first_vertex], polygon.num_vertices, v2d);
+	if(n < 3)
+		return;
+	if(material.sky)
+	{
+		ClippingPolygon b = ClippingPolygon::from_point(v2d[0]);
+		for(uint32_t i = 1; i < n; ++i)
+			b.extend_point(v2d[i]);
+		if(f.have_sky)
+			f.sky_bounds.extend(b);
+		else
+		{
+			f.have_sky = true;
+			f.sky_material = polygon.texture;
+			f.sky_bounds = b;
+		}
+		return;
+	}
+	const DepthEq de = depth_from_plane(plane_transformed);
+	const V4 tcb[2] = {mul(cam.planes_matrix, extend(polygon.tex_coord_equation[0].vec, polygon.tex_coord_equation[0].dist)),
+		mul(cam.planes_matrix, extend(polygon.tex_coord_equation[1].vec, polygon.tex_coord_equation[1].dist))};
+	const TcEq te = tc_from_depth_and_planes(de, tcb);
+	const uint32_t mip = calculate_mip(v2d, n, de, te, mip_bias);
+	const TcEq tes = scale(te, 1.0f / float(1 << mip));
+
+	const float inf = float(1 << 29);
+	const float max_z = float(1 << 16);
+	float tc_min[2] = {inf, inf}, tc_max[2] = {-inf, -inf};
+	for(uint32_t k = 0; k < n; ++k)
+	{
+		const float inv_z = std::fmax(de.sample(v2d[k]), 1.0f / max_z);
+		const float z = 1.0f / inv_z;
+		for(int i = 0; i < 2; ++i)
+		{
+			const float tc = z * (tes.dx[i] * v2d[k].x + tes.dy[i] * v2d[k].y + tes.k[i]);
+			if(tc < tc_min[i])
+				tc_min[i] = tc;
+			if(tc > tc_max[i])
+				tc_max[i] = tc;
+		}
+	}
+	int32_t surface_tc_min[2], surface_size[2];
+	for(int i = 0; i < 2; ++i)
+	{
+		const float eps = 1.0f / 32.0f;
+		tc_min[i] += eps;
+		tc_max[i] -= eps;
+		const int32_t round_mask = ~(int32_t(LIGHTMAP_SCALE) - 1);
+		const int32_t tc_min_round_down = (polygon.tex_coord_min[i] & round_mask) >> mip;
+		const int32_t tc_max_round_up = ((polygon.tex_coord_max[i] + int32_t(LIGHTMAP_SCALE) - 1) & round_mask) >> mip;
+		int32_t tc_min_int = std::max(saturating_f32_to_i32(std::floor(std::fmax(tc_min[i], -inf))), tc_min_round_down);
+		int32_t tc_max_int = std::min(saturating_f32_to_i32(std::ceil(std::fmin(tc_max[i], inf))), tc_max_round_up);
+		if(tc_min_int >= tc_max_int)
+		{
+			tc_min_int = std::min(tc_min_int, tc_max_round_up - 1);
+			tc_max_int = tc_min_int + 1;
+		}
+		surface_tc_min[i] = tc_min_int;
+		surface_size[i] = tc_max_int - tc_min_int;
+	}
+	pd.visible_frame = f.frame_number;
+	pd.de = de;
+	pd.te = tes;
+	pd.surface_offset = offset;
+	pd.size[0] = uint32_t(surface_size[0]);
+	pd.size[1] = uint32_t(surface_size[1]);
+	pd.mip = mip;
+	pd.tc_min[0] = surface_tc_min[0];
+	pd.tc_min[1] = surface_tc_min[1];
+	offset += uint64_t(pd.size[0]) * pd.size[1];
+	for(int i = 0; i < 2; ++i)
+	{
+		const float t = float(surface_tc_min[i]);
+		pd.te.dx[i] -= t * de.dx;
+		pd.te.dy[i] -= t * de.dy;
+		pd.te.k[i] -= t * de.k;
+	}
+	f.visible_polygons.push_back(pi);
+}
+
+// partial_renderer.rs:3877-3985 (ShadowMap::None branch)
+bool polygon_is_affected_by_light(const Polygon& polygon, const swh_frame::Basis& b, const sw_light& light)
+{
+	const V3 lp = v3(light.position[0], light.position[1], light.position[2]);
+	const V3 d = lp - b.start;
+	const float sd = dot(d, b.normal);
+	if(sd <= 0.0f)
+		return false;
+	const float sq = light.radius * light.radius - sd * sd;
+	if(sq <= 0.0f)
+		return false;
+	// invert the 3x3 (u, v, normal) and transpose
+	M4 m3 = identity();
+	m3.c[0] = extend(b.u, 0.0f);
+	m3.c[1] = extend(b.v, 0.0f);
+	m3.c[2] = extend(b.normal, 0.0f);
+	M4 inv;
+	if(!invert(m3, inv))
+		return false;
+	const M4 t = transpose(inv);
+	const float tc0 = dot(d, truncate(t.c[0])), tc1 = dot(d, truncate(t.c[1]));
+	const float r = std::sqrt(sq);
+	const float u_r = r * (1.0f / std::sqrt(dot(b.u, b.u)));
+	const float v_r = r * (1.0f / std::sqrt(dot(b.v, b.v)));
+	return !(tc0 + u_r < float(polygon.tex_coord_min[0]) || tc1 + v_r < float(polygon.tex_coord_min[1]) ||
+		tc0 - u_r > float(polygon.tex_coord_max[0]) || tc1 - v_r > float(polygon.tex_coord_max[1]));
+}
+
+// partial_renderer.rs:1682-1799 (per-surface parameters)
+void emit_surface(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, uint32_t pi)
+{
+	const Polygon& polygon = m.polygons[pi];
+	const swh_frame::PolyData& pd = f.poly[pi];
+	const Material& material = m.materials[polygon.texture];
+	const swh_frame::Basis& basis = f.basis[pi];
+	sw_surface_desc s;
+	std::memset(&s, 0, sizeof(s));
+	const float scale = float(1u << pd.mip);
+	const V3 u = pd.mip ? basis.u * scale : basis.u, v = pd.mip ? basis.v * scale : basis.v;
+	s.basis_u[0] = u.x, s.basis_u[1] = u.y, s.basis_u[2] = u.z;
+	s.basis_v[0] = v.x, s.basis_v[1] = v.y, s.basis_v[2] = v.z;
+	s.basis_start[0] = basis.start.x, s.basis_start[1] = basis.start.y, s.basis_start[2] = basis.start.z;
+	s.basis_normal[0] = basis.normal.x, s.basis_normal[1] = basis.normal.y, s.basis_normal[2] = basis.normal.z;
+	s.size[0] = pd.size[0];
+	s.size[1] = pd.size[1];
+	s.tc_start[0] = pd.tc_min[0];
+	s.tc_start[1] = pd.tc_min[1];
+	s.texture_level = material.first_level + pd.mip;
+	for(int i = 0; i < 2; ++i)
+	{
+		const int32_t round_mask = ~(int32_t(LIGHTMAP_SCALE) - 1);
+		s.lightmap_tc_shift[i] = uint32_t(pd.tc_min[i] - ((polygon.tex_coord_min[i] & round_mask) >> pd.mip));
+	}
+	s.lightmap_size[0] = uint32_t(((polygon.tex_coord_max[0] + int32_t(LIGHTMAP_SCALE) - 1) >> LIGHTMAP_SCALE_LOG2) -
+		(polygon.tex_coord_min[0] >> LIGHTMAP_SCALE_LOG2) + 1);
+	s.lightmap_size[1] = uint32_t(((polygon.tex_coord_max[1] + int32_t(LIGHTMAP_SCALE) - 1) >> LIGHTMAP_SCALE_LOG2) -
+		(polygon.tex_coord_min[1] >> LIGHTMAP_SCALE_LOG2) + 1);
+	s.lightmap_scale_log2 = LIGHTMAP_SCALE_LOG2 - pd.mip;
+	s.lightmap_offset = polygon.lightmap_data_offset;
+	s.flags = SW_SURF_DIFFUSE_LAYER;
+	if(cfg.use_directional_lightmaps && !m.lm_dir.empty())
+		s.flags |= SW_SURF_DIRECTIONAL;
+	s.emissive_level = SW_NO_TEXTURE;
+	s.pixels_offset = pd.surface_offset;
+	// dynamic lights of the polygon's leaf, at most 6
+	s.first_light = uint32_t(f.light_indices.size());
+	if(!f.leaf_lights.empty())
+	{
+		for(uint32_t li : f.leaf_lights[m.polygon_leaf[pi]])
+		{
+			if(polygon_is_affected_by_light(polygon, basis, f.lights[li]))
+			{
+				f.light_indices.push_back(li);
+				if(++s.num_lights == SW_MAX_POLYGON_LIGHTS)
+					break;
+			}
+		}
+	}
+	f.surfaces.push_back(s);
+}
+
+} // namespace
+
+namespace swh
+{
+
+// partial_renderer.rs:4011-4142
+void draw_polygon(swh_frame& f, const V3 clip_planes[8], const V3* verts, uint32_t nverts, const DepthEq& de, const TcEq& te,
+	const uint32_t tsize[2], uint64_t texture_offset, uint32_t texture_source, uint32_t blend, const int32_t clip_rect[4])
+{
+	if(nverts < 3)
+		return;
+	V2 v2d[MAX_VERTICES];
+	const uint32_t n = project_and_clip_polygon(clip_planes, verts, nverts, v2d);
+	if(n < 3)
+		return;
+	float min_inv_z = 1e24f, max_inv_z = -1e24f, min_x = 1e24f, max_x = -1e24f;
+	V2 min_p = v2d[0], max_p = v2d[0];
+	for(uint32_t i = 0; i < n; ++i)
+	{
+		const float inv_z = de.sample(v2d[i]);
+		if(inv_z < min_inv_z)
+		{
+			min_inv_z = inv_z;
+			min_p = v2d[i];
+		}
+		if(inv_z > max_inv_z)
+		{
+			max_inv_z = inv_z;
+			max_p = v2d[i];
+		}
+		if(v2d[i].x < min_x)
+			min_x = v2d[i].x;
+		if(v2d[i].x > max_x)
+			max_x = v2d[i].x;
+	}
+	sw_draw d;
+	std::memset(&d, 0, sizeof(d));
+	d.kind = SW_DRAW_POLYGON;
+	d.blend = blend;
+	d.texture_source = texture_source;
+	d.first_vertex_int = uint32_t(f.vdata.size());
+	d.num_vertices = n;
+	for(uint32_t i = 0; i < n; ++i)
+	{
+		f.vdata.push_back(f32_to_fixed16_unchecked(v2d[i].x));
+		f.vdata.push_back(f32_to_fixed16_unchecked(v2d[i].y));
+	}
+	d.texture_offset = texture_offset;
+	d.texture_size[0] = int32_t(tsize[0]);
+	d.texture_size[1] = int32_t(tsize[1]);
+	for(int i = 0; i < 4; ++i)
+		d.clip_rect[i] = clip_rect[i];
+	DepthEq des = de;
+	TcEq tes = te;
+	if(affine_may_be_used(de, te, min_p, max_p))
+		d.mode = SW_TC_AFFINE;
+	else
+	{
+		const float z_scale = std::exp2(-5.0f - std::ceil(std::log2(std::fmax(max_inv_z, 1.0f / float(1 << 20)))));
+		des = scale(de, z_scale);
+		tes = scale(te, z_scale);
+		d.mode = line_z_may_be_used(de, te, max_p, min_x, max_x) ? SW_TC_LINE_Z : SW_TC_PERSPECTIVE;
+	}
+	d.depth_equation[0] = des.dx;
+	d.depth_equation[1] = des.dy;
+	d.depth_equation[2] = des.k;
+	for(int i = 0; i < 2; ++i)
+	{
+		d.tc_equation[i] = tes.dx[i];
+		d.tc_equation[2 + i] = tes.dy[i];
+		d.tc_equation[4 + i] = tes.k[i];
+	}
+	f.draws.push_back(d);
+}
+
+} // namespace swh
+
+namespace
+{
+
+// partial_renderer.rs:2311-2448
+void draw_skybox(swh_frame& f, const swh_map& m, const Camera& cam, const ClippingPolygon& viewport, float mip_bias,
+	const int32_t clip_rect[4])
+{
+	if(!f.have_sky || m.sky_side_size == 0)
+		return;
+	ClippingPolygon bounds = f.sky_bounds;
+	bounds.intersect(viewport);
+	if(!bounds.valid())
+		return;
+	static const float BOX[8][3] = {{-1, -1, -1}, {-1, -1, 1}, {-1, 1, -1}, {-1, 1, 1}, {1, -1, -1}, {1, -1, 1}, {1, 1, -1}, {1, 1, 1}};
+	struct Side
+	{
+		int idx[4];
+		V3 n, u, v;
+	};
+	const Side sides[6] = {{{0, 1, 3, 2}, v3(1, 0, 0), v3(0, 1, 0), v3(0, 0, -1)}, {{4, 6, 7, 5}, v3(-1, 0, 0), v3(0, -1, 0), v3(0, 0, -1)},
+		{{0, 4, 5, 1}, v3(0, 1, 0), v3(-1, 0, 0), v3(0, 0, -1)}, {{2, 3, 7, 6}, v3(0, -1, 0), v3(1, 0, 0), v3(0, 0, -1)},
+		{{0, 2, 6, 4}, v3(0, 0, 1), v3(1, 0, 0), v3(0, -1, 0)}, {{1, 5, 7, 3}, v3(0, 0, -1), v3(1, 0, 0), v3(0, 1, 0)}};
+	const float side_plane_dist = -1.0f, side_tc_shift = 1.0f;
+	const M4 skybox_matrix = from_translation(cam.position); // identity rotation
+	M4 skybox_matrix_inverse;
+	if(!invert(transpose(skybox_matrix), skybox_matrix_inverse))
+		return;
+	const M4 sview = mul(cam.view_matrix, skybox_matrix);
+	const M4 splanes = mul(cam.planes_matrix, skybox_matrix_inverse);
+	const float SKY_SCALE = 16384.0f;
+	V3 bt[8];
+	for(int i = 0; i < 8; ++i)
+		bt[i] = view_transform(sview, v3(BOX[i][0], BOX[i][1], BOX[i][2]) * SKY_SCALE);
+	V3 clip_planes[8];
+	bounds.clip_planes(clip_planes);
+	uint64_t side_base = 0;
+	uint64_t side_texels = 0;
+	for(uint32_t mip = 0, s = m.sky_side_size; mip <= MAX_MIP; ++mip, s >>= 1)
+		side_texels += uint64_t(s) * s;
+	for(int side = 0; side < 6; ++side, side_base += side_texels)
+	{
+		const V3 vt[4] = {bt[sides[side].idx[0]], bt[sides[side].idx[1]], bt[sides[side].idx[2]], bt[sides[side].idx[3]]};
+		const DepthEq de = depth_from_plane(mul(splanes, extend(sides[side].n, -side_plane_dist)));
+		const float tc_scale = float(m.sky_side_size) * 0.5f;
+		const V4 u4 = extend(sides[side].u, side_tc_shift), v4_ = extend(sides[side].v, side_tc_shift);
+		const V4 tcb[2] = {mul(splanes, v4(u4.x * tc_scale, u4.y * tc_scale, u4.z * tc_scale, u4.w * tc_scale)),
+			mul(splanes, v4(v4_.x * tc_scale, v4_.y * tc_scale, v4_.z * tc_scale, v4_.w * tc_scale))};
+		const TcEq te = tc_from_depth_and_planes(de, tcb);
+		V2 v2d[MAX_VERTICES];
+		const uint32_t n = project_and_clip_polygon(clip_planes, vt, 4, v2d);
+		if(n < 3)
+			continue;
+		const uint32_t mip = calculate_mip(v2d, n, de, te, mip_bias);
+		const TcEq tes = scale(te, 1.0f / float(1 << mip));
+		uint64_t off = side_base;
+		uint32_t size = m.sky_side_size;
+		for(uint32_t k = 0; k < mip; ++k)
+		{
+			off += uint64_t(size) * size;
+			size >>= 1;
+		}
+		const uint32_t ts[2] = {size, size};
+		draw_polygon(f, clip_planes, vt, 4, de, tes, ts, off, SW_TEXSRC_SKYBOX, SW_BLEND_NONE, clip_rect);
+	}
+}
+
+} // namespace
+
+// =============================================================================================
+// C API
+// =============================================================================================
+extern "C" {
+
+int swh_frame_create(swh_frame** out)
+{
+	if(!out)
+		return -1;
+	*out = new(std::nothrow) swh_frame();
+	return *out ? 0 : -2;
+}
+
+void swh_frame_destroy(swh_frame* f) { delete f; }
+
+int swh_frame_build(swh_frame* f, const swh_map* m, const swh_frame_config* cfg, const float cam_pos[3], float yaw, float pitch)
+{
+	if(!f || !m || !cfg || !cam_pos || cfg->width == 0 || cfg->height == 0 || (cfg->color_bits != 32 && cfg->color_bits != 64))
+		return -1;
+	ensure_basis(*f, *m);
+	f->frame_number++;
+	f->visible_polygons.clear();
+	f->surfaces.clear();
+	f->lights.clear();
+	f->light_indices.clear();
+	f->draws.clear();
+	f->vdata.clear();
+	f->have_sky = false;
+	f->num_triangles = 0;
+	const float W = float(cfg->width), H = float(cfg->height);
+	const Camera cam = build_camera(v3(cam_pos[0], cam_pos[1], cam_pos[2]), yaw, pitch, cfg->fov, W, H);
+	const ClippingPolygon frame_bounds = ClippingPolygon::from_box(0.0f, 0.0f, W, H);
+	update_visibility(*f, *m, cam, frame_bounds);
+
+	// dynamic lights and the leaves they touch
+	build_dynamic_lights(*f, *m, *cfg);
+
+	// surfaces (partial_renderer.rs:1327-1452), leaf order
+	uint64_t offset = 0;
+	f->num_visible_leafs = 0;
+	for(uint32_t li = 0; li < m->leafs.size(); ++li)
+	{
+		if(f->leaf_visible_frame[li] != f->frame_number)
+			continue;
+		f->num_visible_leafs++;
+		V3 cp[8];
+		f->leaf_bounds[li].clip_planes(cp);
+		const Leaf& leaf = m->leafs[li];
+		for(uint32_t k = 0; k < leaf.num_polygons; ++k)
+			prepare_polygon_surface(*f, *m, cam, cp, offset, leaf.first_polygon + k, cfg->textures_mip_bias);
+	}
+	f->arena_texels = offset;
+	for(uint32_t pi : f->visible_polygons)
+		emit_surface(*f, *m, *cfg, pi);
+
+	// models: animation, lighting, projection (per frame, before the tree walk)
+	prepare_models(*f, *m, *cfg, cam.position, cam.view_matrix, cam.planes_matrix);
+
+	// draw list (partial_renderer.rs:1285-1325 with one full-screen rect)
+	const int32_t clip_rect[4] = {0, 0, int32_t(cfg->width), int32_t(cfg->height)};
+	draw_skybox(*f, *m, cam, frame_bounds, cfg->textures_mip_bias, clip_rect);
+	// draw_tree (:2450-2517): back to front
+	std::vector<uint32_t> stack;
+	stack.push_back(uint32_t(m->nodes.size() - 1));
+	const V4 w_row = cam.planes_matrix.row(3);
+	while(!stack.empty())
+	{
+		const uint32_t node_index = stack.back();
+		if(node_index >= FIRST_LEAF_INDEX)
+		{
+			stack.pop_back();
+			const uint32_t leaf_index = node_index - FIRST_LEAF_INDEX;
+			if(f->leaf_visible_frame[leaf_index] != f->frame_number)
+				continue;
+			ClippingPolygon lb = f->leaf_bounds[leaf_index];
+			lb.intersect(frame_bounds);
+			if(!lb.valid())
+				continue;
+			V3 cp[8];
+			lb.clip_planes(cp);
+			const Leaf& leaf = m->leafs[leaf_index];
+			for(uint32_t k = 0; k < leaf.num_polygons; ++k)
+			{
+				const uint32_t pi = leaf.first_polygon + k;
+				const swh_frame::PolyData& pd = f->poly[pi];
+				if(pd.visible_frame != f->frame_number)
+					continue;
+				const Polygon& p = m->polygons[pi];
+				draw_polygon(*f, cp, &f->vertices_transformed[p.first_vertex], p.num_vertices, pd.de, pd.te, pd.size, pd.surface_offset,
+					SW_TEXSRC_SURFACE, m->materials[p.texture].blending_mode, clip_rect);
+			}
+			draw_leaf_objects(*f, *m, *cfg, leaf_index, cp, clip_rect);
+		}
+		else
+		{
+			const Node& node = m->nodes[node_index];
+			const float w = dot(w_row, extend(node.plane.vec, -node.plane.dist));
+			const uint32_t mask = w >= 0.0f ? 1u : 0u;
+			stack.back() = node.children[1 ^ mask];
+			stack.push_back(node.children[mask]);
+		}
+	}
+
+	sw_view& v = f->view;
+	std::memset(&v, 0, sizeof(v));
+	v.width = cfg->width;
+	v.height = cfg->height;
+	v.cam_pos[0] = cam_pos[0];
+	v.cam_pos[1] = cam_pos[1];
+	v.cam_pos[2] = cam_pos[2];
+	v.clear_background = (cfg->force_clear || (cfg->clear_background && !f->inside_leaf_volume)) ? 1u : 0u;
+	// ColorVec::from_color_f32x3(&[32.0, 16.0, 8.0]).into() (partial_renderer.rs:255)
+	v.background_color = cfg->color_bits == 64 ? 0x0000002000100008ull : 0x00201008ull;
+	v.num_surfaces = uint32_t(f->surfaces.size());
+	v.num_draws = uint32_t(f->draws.size());
+	return 0;
+}
+
+int swh_frame_get(const swh_frame* f, swh_frame_data* out)
+{
+	if(!f || !out)
+		return -1;
+	out->view = f->view;
+	out->surfaces = f->surfaces.data();
+	out->num_surfaces = uint32_t(f->surfaces.size());
+	out->lights = f->lights.data();
+	out->num_lights = uint32_t(f->lights.size());
+	out->light_indices = f->light_indices.data();
+	out->num_light_indices = uint32_t(f->light_indices.size());
+	out->draws = f->draws.data();
+	out->num_draws = uint32_t(f->draws.size());
+	out->vertex_data = f->vdata.data();
+	out->num_vertex_ints = f->vdata.size();
+	out->surface_arena_texels = f->arena_texels;
+	out->num_visible_leafs = f->num_visible_leafs;
+	out->num_triangles = f->num_triangles;
+	return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,175 @@
+// Frame state shared by the front-end's translation units.
+#pragma once
+
+#include "host_internal.h"
+#include "host_models.h"
+
+namespace swh
+{
+
+const uint32_t MAX_VERTICES = 24; // renderer_utils.rs:4
+const float Z_NEAR = 1.0f; // renderer_utils.rs:7
+
+// ---------------------------------------------------------------------------------------------
+// ClippingPolygon (clipping_polygon.rs)
+// ---------------------------------------------------------------------------------------------
+struct ClipAxis
+{
+	float min, max;
+	bool valid() const { return min < max; }
+	bool contains(const ClipAxis& o) const { return o.min >= min && o.max <= max; }
+	void extend(const ClipAxis& o)
+	{
+		if(o.min < min)
+			min = o.min;
+		if(o.max > max)
+			max = o.max;
+	}
+	void extend_point(float p)
+	{
+		if(p < min)
+			min = p;
+		if(p > max)
+			max = p;
+	}
+	void intersect(const ClipAxis& o)
+	{
+		if(o.min > min)
+			min = o.min;
+		if(o.max < max)
+			max = o.max;
+	}
+};
+
+struct ClippingPolygon
+{
+	ClipAxis x, y, xpy, xmy;
+	static ClippingPolygon from_box(float min_x, float min_y, float max_x, float max_y)
+	{
+		return ClippingPolygon{{min_x, max_x}, {min_y, max_y}, {min_x + min_y, max_x + max_y}, {min_x - max_y, max_x - min_y}};
+	}
+	static ClippingPolygon from_point(V2 p)
+	{
+		return ClippingPolygon{{p.x, p.x}, {p.y, p.y}, {p.x + p.y, p.x + p.y}, {p.x - p.y, p.x - p.y}};
+	}
+	bool valid() const { return x.valid() && y.valid() && xpy.valid() && xmy.valid(); }
+	bool contains(const ClippingPolygon& o) const
+	{
+		return x.contains(o.x) && y.contains(o.y) && xpy.contains(o.xpy) && xmy.contains(o.xmy);
+	}
+	void extend(const ClippingPolygon& o)
+	{
+		x.extend(o.x);
+		y.extend(o.y);
+		xpy.extend(o.xpy);
+		xmy.extend(o.xmy);
+	}
+	void extend_point(V2 p)
+	{
+		x.extend_point(p.x);
+		y.extend_point(p.y);
+		xpy.extend_point(p.x + p.y);
+		xmy.extend_point(p.x - p.y);
+	}
+	void intersect(const ClippingPolygon& o)
+	{
+		x.intersect(o.x);
+		y.intersect(o.y);
+		xpy.intersect(o.xpy);
+		xmy.intersect(o.xmy);
+	}
+	void clip_planes(V3 out[8]) const
+	{
+		out[0] = v3(-1.0f, 0.0f, -x.max);
+		out[1] = v3(1.0f, 0.0f, x.min);
+		out[2] = v3(0.0f, -1.0f, -y.max);
+		out[3] = v3(0.0f, 1.0f, y.min);
+		out[4] = v3(-1.0f, -1.0f, -xpy.max);
+		out[5] = v3(1.0f, 1.0f, xpy.min);
+		out[6] = v3(-1.0f, 1.0f, -xmy.max);
+		out[7] = v3(1.0f, -1.0f, xmy.min);
+	}
+};
+
+struct DepthEq
+{
+	float dx, dy, k;
+	float sample(V2 p) const { return dx * p.x + dy * p.y + k; }
+};
+struct TcEq
+{
+	float dx[2], dy[2], k[2];
+};
+
+
+uint32_t clip_3d_polygon_by_z_plane(const V3* poly, uint32_t n, float z_dist, V3* out, uint32_t cap);
+uint32_t clip_2d_polygon(const V2* poly, uint32_t n, V3 line, V2* out, uint32_t cap);
+uint32_t project_and_clip_polygon(const V3 clip_planes[8], const V3* verts, uint32_t n, V2* out);
+V3 view_transform(const M4& m, V3 v);
+int32_t f32_to_fixed16_unchecked(float x);
+void draw_polygon(swh_frame& f, const V3 clip_planes[8], const V3* verts, uint32_t nverts, const DepthEq& de, const TcEq& te,
+	const uint32_t tsize[2], uint64_t texture_offset, uint32_t texture_source, uint32_t blend, const int32_t clip_rect[4]);
+
+} // namespace swh
+
+using swh::ClippingPolygon;
+using swh::DepthEq;
+using swh::ModelFrameData;
+using swh::TcEq;
+using swh::V3;
+
+// =============================================================================================
+// Frame state
+// =============================================================================================
+struct swh_frame
+{
+	const swh_map* basis_map = nullptr;
+	struct Basis
+	{
+		V3 u, v, start, normal;
+	};
+	std::vector<Basis> basis;
+
+	struct PolyData
+	{
+		uint32_t visible_frame = 0;
+		DepthEq de;
+		TcEq te;
+		uint64_t surface_offset;
+		uint32_t size[2];
+		uint32_t mip;
+		int32_t tc_min[2];
+	};
+	std::vector<PolyData> poly;
+	uint32_t frame_number = 0;
+	std::vector<V3> vertices_transformed;
+
+	// visibility
+	std::vector<uint32_t> leaf_visible_frame;
+	std::vector<ClippingPolygon> leaf_bounds;
+	std::vector<uint32_t> portal_visible_frame;
+	std::vector<uint8_t> portal_has_projection;
+	std::vector<ClippingPolygon> portal_projection;
+	bool inside_leaf_volume = true;
+
+	bool have_sky = false;
+	uint32_t sky_material = 0;
+	ClippingPolygon sky_bounds;
+
+	// outputs
+	std::vector<uint32_t> visible_polygons;
+	std::vector<sw_surface_desc> surfaces;
+	std::vector<sw_light> lights;
+	std::vector<uint32_t> light_indices;
+	std::vector<sw_draw> draws;
+	std::vector<int32_t> vdata;
+	uint64_t arena_texels = 0;
+	sw_view view{};
+	uint32_t num_visible_leafs = 0;
+	uint32_t num_triangles = 0;
+
+	// dynamic objects
+	std::vector<std::vector<uint32_t>> leaf_lights;
+	ModelFrameData models;
+};
+

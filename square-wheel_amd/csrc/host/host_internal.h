@@ -1,0 +1,167 @@
+// Internal data model of the CPU front-end (libsw_host.so).
+//
+// Map layout follows common/bsp_map_compact.rs:7-199 (BSPNode / BSPLeaf / Polygon / Portal,
+// FIRST_LEAF_INDEX = 1 << 31, root node last, lightmap offset 0 = "no lightmap"); math follows
+// cgmath 0.18's conventions as used by the reference (column-major Matrix4, Vector*Matrix via row
+// dot products).
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "../../../include/sw_host.h"
+
+namespace swh
+{
+
+struct V2
+{
+	float x, y;
+};
+struct V3
+{
+	float x, y, z;
+};
+struct V4
+{
+	float x, y, z, w;
+};
+
+inline V2 v2(float x, float y) { return V2{x, y}; }
+inline V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+inline V4 v4(float x, float y, float z, float w) { return V4{x, y, z, w}; }
+inline V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline V3 operator*(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+inline V3 operator/(V3 a, float s) { return v3(a.x / s, a.y / s, a.z / s); }
+inline V2 operator+(V2 a, V2 b) { return v2(a.x + b.x, a.y + b.y); }
+inline V2 operator-(V2 a, V2 b) { return v2(a.x - b.x, a.y - b.y); }
+inline V2 operator*(V2 a, float s) { return v2(a.x * s, a.y * s); }
+inline V2 operator/(V2 a, float s) { return v2(a.x / s, a.y / s); }
+// cgmath dot: element-wise product, then sum left to right
+inline float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+inline float dot(V2 a, V2 b) { return a.x * b.x + a.y * b.y; }
+inline float dot(V4 a, V4 b) { return ((a.x * b.x + a.y * b.y) + a.z * b.z) + a.w * b.w; }
+inline V3 cross(V3 a, V3 b) { return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+inline float length(V3 a) { return std::sqrt(dot(a, a)); }
+inline V4 extend(V3 a, float w) { return v4(a.x, a.y, a.z, w); }
+inline V3 truncate(V4 a) { return v3(a.x, a.y, a.z); }
+
+// Column-major 4x4 (cgmath Matrix4): c[i] is column i.
+struct M4
+{
+	V4 c[4];
+	float& at(int row, int col) { return (&c[col].x)[row]; }
+	float at(int row, int col) const { return (&c[col].x)[row]; }
+	V4 row(int r) const { return v4(at(r, 0), at(r, 1), at(r, 2), at(r, 3)); }
+};
+
+M4 identity();
+M4 mul(const M4& a, const M4& b);
+V4 mul(const M4& m, V4 v);
+M4 transpose(const M4& m);
+bool invert(const M4& m, M4& out);
+M4 from_nonuniform_scale(float x, float y, float z);
+M4 from_translation(V3 t);
+
+struct Plane
+{
+	V3 vec;
+	float dist;
+};
+
+struct Polygon
+{
+	uint32_t first_vertex, num_vertices;
+	Plane plane;
+	Plane tex_coord_equation[2];
+	int32_t tex_coord_min[2], tex_coord_max[2];
+	uint32_t lightmap_data_offset;
+	uint32_t texture; // material index
+};
+
+struct Node
+{
+	uint32_t children[2]; // >= FIRST_LEAF_INDEX: leaf
+	Plane plane;
+};
+
+struct Leaf
+{
+	uint32_t first_polygon, num_polygons, first_leaf_portal, num_leaf_portals;
+};
+
+struct Portal
+{
+	uint32_t leafs[2];
+	Plane plane;
+	uint32_t first_vertex, num_vertices;
+};
+
+const uint32_t FIRST_LEAF_INDEX = 1u << 31;
+const uint32_t LIGHTMAP_SCALE_LOG2 = 4;
+const uint32_t LIGHTMAP_SCALE = 1u << LIGHTMAP_SCALE_LOG2;
+const uint32_t MAX_MIP = 4;
+
+struct Material
+{
+	uint32_t first_level; // index of mip 0 in the level table; mips follow
+	float roughness;
+	bool is_metal;
+	uint32_t blending_mode;
+	bool sky;
+};
+
+struct BBox
+{
+	V3 min, max;
+};
+
+struct PointLight
+{
+	V3 pos;
+	float intensity;
+	V3 color;
+};
+
+} // namespace swh
+
+struct swh_map
+{
+	swh_map_config config;
+	uint32_t gx, gy, gz;
+	std::vector<uint8_t> cell; // 0 empty, 1 solid, 2 sky
+	std::vector<uint32_t> cell_leaf;
+	std::vector<swh::BBox> leaf_bbox;
+	std::vector<uint32_t> empty_cells;
+
+	std::vector<swh::Node> nodes;
+	std::vector<swh::Leaf> leafs;
+	std::vector<swh::Polygon> polygons;
+	std::vector<swh::Portal> portals;
+	std::vector<uint32_t> leafs_portals;
+	std::vector<swh::V3> vertices;
+	std::vector<swh::Material> materials;
+	std::vector<uint32_t> polygon_leaf;
+
+	std::vector<float> lm_simple; // 3 floats per element
+	std::vector<sw_directional_lightmap_element> lm_dir;
+	std::vector<sw_texel> texels;
+	std::vector<uint32_t> lite;
+	std::vector<sw_texture_level> levels;
+	std::vector<uint32_t> decal_levels; // mip-0 level of each decal texture
+	std::vector<uint32_t> model_levels; // mip-0 level of each model skin
+	std::vector<uint32_t> sky32;
+	std::vector<uint64_t> sky64;
+	uint32_t sky_side_size = 0;
+	std::vector<swh::PointLight> lights;
+	swh::V3 sun_dir{0, 0, -1};
+	float sun_intensity = 0.0f;
+};
+
+namespace swh
+{
+inline uint32_t cell_index(const swh_map& m, uint32_t x, uint32_t y, uint32_t z) { return (z * m.gy + y) * m.gx + x; }
+uint32_t find_leaf(const swh_map& m, V3 p);
+} // namespace swh

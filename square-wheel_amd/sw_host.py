@@ -1,0 +1,250 @@
+"""ctypes binding of libsw_host.so (the CPU renderer front-end, include/sw_host.h) and the seeded
+workload recipes of the benchmark configurations."""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+import sw_abi as A
+
+_lib = None
+
+
+class MapConfig(C.Structure):
+    _fields_ = [("seed", C.c_uint32), ("grid", C.c_uint32 * 3), ("solid_probability", C.c_float),
+                ("num_lights", C.c_uint32), ("light_intensity_min", C.c_float), ("light_intensity_max", C.c_float),
+                ("sun", C.c_uint32), ("sky_fraction", C.c_float), ("texture_size", C.c_uint32),
+                ("num_materials", C.c_uint32), ("normal_maps", C.c_uint32), ("specular_fraction", C.c_float),
+                ("alpha_blend_fraction", C.c_float), ("directional_lightmaps", C.c_uint32),
+                ("num_decal_textures", C.c_uint32), ("num_model_textures", C.c_uint32)]
+
+
+class MapInfo(C.Structure):
+    _fields_ = [("num_polygons", C.c_uint32), ("num_leafs", C.c_uint32), ("num_nodes", C.c_uint32),
+                ("num_portals", C.c_uint32), ("num_empty_cells", C.c_uint32), ("num_texels", C.c_uint64),
+                ("num_lightmap_elements", C.c_uint64)]
+
+
+class FrameConfig(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("color_bits", C.c_uint32), ("fov", C.c_float),
+                ("textures_mip_bias", C.c_float), ("use_directional_lightmaps", C.c_uint32),
+                ("clear_background", C.c_uint32), ("force_clear", C.c_uint32), ("num_dynamic_lights", C.c_uint32),
+                ("num_models", C.c_uint32), ("num_decals", C.c_uint32), ("seed", C.c_uint32), ("time", C.c_float),
+                ("virtual_threads", C.c_uint32)]
+
+
+class ViewC(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("cam_pos", C.c_float * 3),
+                ("clear_background", C.c_uint32), ("background_color", C.c_uint64), ("first_surface", C.c_uint32),
+                ("num_surfaces", C.c_uint32), ("first_draw", C.c_uint32), ("num_draws", C.c_uint32),
+                ("fb_offset", C.c_uint64), ("surface_arena_offset", C.c_uint64)]
+
+
+class FrameData(C.Structure):
+    _fields_ = [("view", ViewC), ("surfaces", C.c_void_p), ("num_surfaces", C.c_uint32), ("lights", C.c_void_p),
+                ("num_lights", C.c_uint32), ("light_indices", C.c_void_p), ("num_light_indices", C.c_uint32),
+                ("draws", C.c_void_p), ("num_draws", C.c_uint32), ("vertex_data", C.c_void_p),
+                ("num_vertex_ints", C.c_uint64), ("surface_arena_texels", C.c_uint64),
+                ("num_visible_leafs", C.c_uint32), ("num_triangles", C.c_uint32)]
+
+
+assert C.sizeof(ViewC) == A.VIEW_DTYPE.itemsize
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(A.HOST_LIB_PATH):
+            raise RuntimeError(f"{A.HOST_LIB_PATH} is not built: run square-wheel_amd/build.py")
+        lib = C.CDLL(A.HOST_LIB_PATH)
+        lib.swh_map_create.argtypes = [C.POINTER(MapConfig), C.POINTER(C.c_void_p)]
+        lib.swh_map_create.restype = C.c_int
+        lib.swh_map_destroy.argtypes = [C.c_void_p]
+        lib.swh_map_destroy.restype = None
+        lib.swh_map_desc.argtypes = [C.c_void_p, C.POINTER(A.MapDesc)]
+        lib.swh_map_desc.restype = C.c_int
+        lib.swh_map_info_get.argtypes = [C.c_void_p, C.POINTER(MapInfo)]
+        lib.swh_map_info_get.restype = C.c_int
+        lib.swh_camera_path.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
+        lib.swh_camera_path.restype = C.c_int
+        lib.swh_frame_create.argtypes = [C.POINTER(C.c_void_p)]
+        lib.swh_frame_create.restype = C.c_int
+        lib.swh_frame_destroy.argtypes = [C.c_void_p]
+        lib.swh_frame_destroy.restype = None
+        lib.swh_frame_build.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(FrameConfig), C.POINTER(C.c_float), C.c_float,
+                                        C.c_float]
+        lib.swh_frame_build.restype = C.c_int
+        lib.swh_frame_get.argtypes = [C.c_void_p, C.POINTER(FrameData)]
+        lib.swh_frame_get.restype = C.c_int
+        lib.swh_pack_normal_roughness_from_rgb.argtypes = [C.c_uint32, C.c_float]
+        lib.swh_pack_normal_roughness_from_rgb.restype = C.c_int32
+        lib.swh_build_texture_mips.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
+        lib.swh_build_texture_mips.restype = C.c_uint64
+        _lib = lib
+    return _lib
+
+
+def _copy(ptr, n, dtype):
+    dtype = np.dtype(dtype)
+    if n == 0 or not ptr:
+        return np.zeros(0, dtype=dtype)
+    buf = (C.c_char * (n * dtype.itemsize)).from_address(ptr)
+    return np.frombuffer(bytes(buf), dtype=dtype).copy()
+
+
+class Map:
+    """A generated map (host arrays) and its upload description."""
+
+    def __init__(self, **kw):
+        lib = load()
+        cfg = MapConfig()
+        defaults = dict(seed=0, grid=(8, 8, 2), solid_probability=0.3, num_lights=4, light_intensity_min=0.5,
+                        light_intensity_max=2.0, sun=0, sky_fraction=0.1, texture_size=64, num_materials=8,
+                        normal_maps=0, specular_fraction=0.0, alpha_blend_fraction=0.0, directional_lightmaps=0,
+                        num_decal_textures=0, num_model_textures=0)
+        defaults.update(kw)
+        for k, v in defaults.items():
+            if k == "grid":
+                cfg.grid[:] = list(v)
+            else:
+                setattr(cfg, k, v)
+        self.config = defaults
+        h = C.c_void_p()
+        if lib.swh_map_create(C.byref(cfg), C.byref(h)) != 0:
+            raise RuntimeError("swh_map_create failed")
+        self.h = h
+        d = A.MapDesc()
+        lib.swh_map_desc(self.h, C.byref(d))
+        self.desc_ = d
+        info = MapInfo()
+        lib.swh_map_info_get(self.h, C.byref(info))
+        self.info = {n: int(getattr(info, n)) for n, _ in MapInfo._fields_}
+        # numpy copies of the map data (for the oracle and the GPU upload)
+        self.data = A.MapData(
+            _copy(d.texels, d.num_texels, A.TEXEL_DTYPE), _copy(d.lite_texels, d.num_lite_texels, np.uint32),
+            _copy(d.lightmaps_simple, d.num_lightmap_elements * 3, np.float32),
+            _copy(d.lightmaps_directional, d.num_directional_lightmap_elements, A.DIRLM_DTYPE),
+            _copy(d.levels, d.num_levels, A.LEVEL_DTYPE), _copy(d.skybox_texels32, d.num_skybox_texels32, np.uint32),
+            _copy(d.skybox_texels64, d.num_skybox_texels64, np.uint64))
+
+    def camera_path(self, seed, n):
+        out = np.zeros((n, 5), dtype=np.float32)
+        if load().swh_camera_path(self.h, seed, n, C.c_void_p(out.ctypes.data)) != 0:
+            raise RuntimeError("swh_camera_path failed")
+        return out
+
+    def close(self):
+        if self.h:
+            load().swh_map_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FrameBuilder:
+    def __init__(self):
+        h = C.c_void_p()
+        if load().swh_frame_create(C.byref(h)) != 0:
+            raise RuntimeError("swh_frame_create failed")
+        self.h = h
+
+    def build(self, m: Map, pose, **kw):
+        cfg = FrameConfig()
+        defaults = dict(width=320, height=200, color_bits=32, fov=math.pi / 2, textures_mip_bias=0.0,
+                        use_directional_lightmaps=0, clear_background=1, force_clear=1, num_dynamic_lights=0,
+                        num_models=0, num_decals=0, seed=0, time=0.0, virtual_threads=1)
+        defaults.update(kw)
+        for k, v in defaults.items():
+            setattr(cfg, k, v)
+        cam = (C.c_float * 3)(float(pose[0]), float(pose[1]), float(pose[2]))
+        lib = load()
+        if lib.swh_frame_build(self.h, m.h, C.byref(cfg), cam, float(pose[3]), float(pose[4])) != 0:
+            raise RuntimeError("swh_frame_build failed")
+        fd = FrameData()
+        lib.swh_frame_get(self.h, C.byref(fd))
+        view = np.frombuffer(bytes(fd.view), dtype=A.VIEW_DTYPE).copy()
+        return {
+            "view": view,
+            "surfaces": _copy(fd.surfaces, fd.num_surfaces, A.SURFACE_DTYPE),
+            "lights": _copy(fd.lights, fd.num_lights, A.LIGHT_DTYPE),
+            "light_indices": _copy(fd.light_indices, fd.num_light_indices, np.uint32),
+            "draws": _copy(fd.draws, fd.num_draws, A.DRAW_DTYPE),
+            "vertex_data": _copy(fd.vertex_data, fd.num_vertex_ints, np.int32),
+            "arena_texels": int(fd.surface_arena_texels),
+            "num_visible_leafs": int(fd.num_visible_leafs),
+            "num_triangles": int(fd.num_triangles),
+            "color_bits": int(cfg.color_bits),
+        }
+
+    def close(self):
+        if self.h:
+            load().swh_frame_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_from_frames(frames):
+    """Concatenates built frames into one BatchData (views get consecutive ranges)."""
+    views = np.concatenate([f["view"] for f in frames]).copy()
+    surf, lights, lidx, draws, vdata = [], [], [], [], []
+    s_off = d_off = l_off = li_off = v_off = 0
+    arena_off = fb_off = 0
+    for k, f in enumerate(frames):
+        v = views[k]
+        v["first_surface"], v["num_surfaces"] = s_off, len(f["surfaces"])
+        v["first_draw"], v["num_draws"] = d_off, len(f["draws"])
+        v["fb_offset"], v["surface_arena_offset"] = fb_off, arena_off
+        s = f["surfaces"].copy()
+        s["first_light"] += li_off
+        li = f["light_indices"] + l_off
+        d = f["draws"].copy()
+        d["first_vertex_int"] += v_off
+        surf.append(s)
+        lights.append(f["lights"])
+        lidx.append(li.astype(np.uint32))
+        draws.append(d)
+        vdata.append(f["vertex_data"])
+        s_off += len(s)
+        d_off += len(d)
+        l_off += len(f["lights"])
+        li_off += len(li)
+        v_off += len(f["vertex_data"])
+        arena_off += f["arena_texels"]
+        fb_off += int(v["width"]) * int(v["height"])
+    cat = lambda xs, dt: np.concatenate(xs) if xs else np.zeros(0, dtype=dt)  # noqa: E731
+    return A.BatchData(views, cat(surf, A.SURFACE_DTYPE), cat(lights, A.LIGHT_DTYPE), cat(lidx, np.uint32),
+                       cat(draws, A.DRAW_DTYPE), cat(vdata, np.int32), frames[0]["color_bits"], arena_texels=arena_off)
+
+
+# ---------------------------------------------------------------------------------------------
+# Workload recipes of the benchmark configurations
+# ---------------------------------------------------------------------------------------------
+def config0_map(seed=0):
+    """320x200 plain CPU reference frame: 8x8x2 grid, p=0.3, 4 white lights, 64x64 textures, simple lightmaps."""
+    return Map(seed=seed, grid=(8, 8, 4), solid_probability=0.3, num_lights=4, texture_size=64, num_materials=6,
+               normal_maps=0, specular_fraction=0.0, directional_lightmaps=0)
+
+
+def config1_map(seed=0, grid=(32, 32, 4), **kw):
+    """1080p plain GPU drop-in: 32x32x4 grid, p=0.35, 32 lights + sun, 256x256 textures with normal maps,
+    a third specular (half metal), directional lightmaps."""
+    args = dict(seed=seed, grid=grid, solid_probability=0.35, num_lights=32, sun=1, sky_fraction=0.1,
+                texture_size=256, num_materials=12, normal_maps=1, specular_fraction=1.0 / 3.0,
+                directional_lightmaps=1, num_decal_textures=4, num_model_textures=4)
+    args.update(kw)
+    return Map(**args)
+
+
+def config2_map(seed=0):
+    """configs[1] map with 10% alpha-blended materials, decal and model textures (HDR + models)."""
+    return config1_map(seed, alpha_blend_fraction=0.1)
