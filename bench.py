@@ -41,7 +41,12 @@ def log(*a):
 
 
 def algorithmic_bytes(stats, color_bits, e_bytes):
-    """Bytes the path must move per frame (SURVEY d-formula), split per kernel."""
+    """Algorithmic bytes the path must move per frame, split per kernel: a surface texel reads its
+    8-byte texture element and writes a c-byte surface texel, each lightmap element of the rows a
+    surface touches is read once (40 B directional, 12 B simple), an emissive texel reads 4 B and
+    read-modify-writes the surface; a polygon pixel reads a c-byte surface texel and writes the
+    framebuffer, a triangle pixel reads a 4-byte lite texel and writes, a blending write also reads
+    the destination (c = 4 for Color32, 8 for Color64)."""
     c = 8 if color_bits == 64 else 4
     surf = stats["surface_texels"] * (8 + c) + stats["lightmap_elements"] * e_bytes + stats["emissive_texels"] * (4 + c)
     raster = stats["polygon_pixels"] * (c + c) + stats["triangle_pixels"] * (4 + c) + stats["blend_pixels"] * c
@@ -107,13 +112,13 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); views shard across GPUs with no "
-              "collective, see DESIGN.md", flush=True)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): this build runs on a single GPU; "
+              "views and requests would shard across GPUs with no collective", flush=True)
         sys.exit(2)
 
     color_bits = 32
     t = time.perf_counter()
-    m = sw_host.config1_map(args.seed)
+    m = sw_host.large_map(args.seed)
     log(f"map built in {time.perf_counter() - t:.1f}s: {m.info}")
     poses = m.camera_path(args.seed + 1, args.poses)
     frames = build_frames(m, poses, args.width, args.height, color_bits)
@@ -193,7 +198,7 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32 surfaces / int32 spans, Color32 out",
         "data": "synthetic (seeded grid map, value-noise textures, generated lightmaps)",
-        "config": {"workload": f"configs[1] plain drop-in: {args.width}x{args.height} Color32, directional lightmaps, "
+        "config": {"workload": f"plain drop-in frame: {args.width}x{args.height} Color32, directional lightmaps, "
                                f"32x32x4 grid p=0.35, 32 lights + sun, 256^2 textures + normal maps, "
                                f"{args.poses}-pose camera path, command buffers resident in HBM"},
         "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 4),

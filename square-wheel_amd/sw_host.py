@@ -229,15 +229,16 @@ def batch_from_frames(frames):
 # ---------------------------------------------------------------------------------------------
 # Workload recipes of the benchmark configurations
 # ---------------------------------------------------------------------------------------------
-def config0_map(seed=0):
-    """320x200 plain CPU reference frame: 8x8x2 grid, p=0.3, 4 white lights, 64x64 textures, simple lightmaps."""
+def small_map(seed=0):
+    """The 320x200 CPU-scale scene: 8x8 grid (4 layers, so that two interior layers remain inside the
+    always-solid perimeter), p=0.3, 4 white lights, 64x64 textures, simple lightmaps."""
     return Map(seed=seed, grid=(8, 8, 4), solid_probability=0.3, num_lights=4, texture_size=64, num_materials=6,
                normal_maps=0, specular_fraction=0.0, directional_lightmaps=0)
 
 
-def config1_map(seed=0, grid=(32, 32, 4), **kw):
-    """1080p plain GPU drop-in: 32x32x4 grid, p=0.35, 32 lights + sun, 256x256 textures with normal maps,
-    a third specular (half metal), directional lightmaps."""
+def large_map(seed=0, grid=(32, 32, 4), **kw):
+    """The 1080p scene: 32x32x4 grid, p=0.35, 32 lights + sun through sky faces, 256x256 textures with
+    normal maps, a third of the materials specular (half of those metal), directional lightmaps."""
     args = dict(seed=seed, grid=grid, solid_probability=0.35, num_lights=32, sun=1, sky_fraction=0.1,
                 texture_size=256, num_materials=12, normal_maps=1, specular_fraction=1.0 / 3.0,
                 directional_lightmaps=1, num_decal_textures=4, num_model_textures=4)
@@ -245,6 +246,6 @@ def config1_map(seed=0, grid=(32, 32, 4), **kw):
     return Map(**args)
 
 
-def config2_map(seed=0):
-    """configs[1] map with 10% alpha-blended materials, decal and model textures (HDR + models)."""
-    return config1_map(seed, alpha_blend_fraction=0.1)
+def large_map_alpha(seed=0, **kw):
+    """The 1080p scene with 10% alpha-blended materials (the HDR + models scene)."""
+    return large_map(seed, alpha_blend_fraction=0.1, **kw)

@@ -16,20 +16,20 @@ def _say(msg):
 
 @pytest.fixture(scope="module")
 def map0():
-    return sw_host.config0_map(3)
+    return sw_host.small_map(3)
 
 
 @pytest.fixture(scope="module")
 def map1():
-    return sw_host.config1_map(5, grid=(12, 12, 4))
+    return sw_host.large_map(5, grid=(12, 12, 4))
 
 
-def test_config0_frames_bit_exact(gpu_ctx, map0):
+def test_small_map_frames_bit_exact(gpu_ctx, map0):
     fb = sw_host.FrameBuilder()
     mh = gpu_ctx.upload_map(map0.data)
     try:
         for i, pose in enumerate(map0.camera_path(2, 6)):
-            _say(f"case config0 pose {i}")
+            _say(f"case small map pose {i}")
             f = fb.build(map0, pose, width=320, height=200)
             b = sw_host.batch_from_frames([f])
             img = gpu_ctx.render_plain(mh, b).reshape(-1)
@@ -41,13 +41,13 @@ def test_config0_frames_bit_exact(gpu_ctx, map0):
 
 
 @pytest.mark.parametrize("color_bits", [32, 64])
-def test_config1_frames_bit_exact(gpu_ctx, map1, color_bits):
+def test_large_map_frames_bit_exact(gpu_ctx, map1, color_bits):
     fb = sw_host.FrameBuilder()
     mh = gpu_ctx.upload_map(map1.data)
     try:
         frames = []
         for i, pose in enumerate(map1.camera_path(3, 4)):
-            _say(f"case config1 pose {i} bits {color_bits}")
+            _say(f"case large map pose {i} bits {color_bits}")
             f = fb.build(map1, pose, width=640, height=360, color_bits=color_bits, use_directional_lightmaps=1,
                          num_dynamic_lights=6, seed=i)
             frames.append(f)
@@ -57,7 +57,7 @@ def test_config1_frames_bit_exact(gpu_ctx, map1, color_bits):
             assert oob == 0
             assert np.array_equal(img, ref), f"pose {i}: {np.count_nonzero(img != ref)} pixels differ"
         # the same views batched into one launch give the same pixels
-        _say("case config1 batch")
+        _say("case large map batch")
         b = sw_host.batch_from_frames(frames)
         out = gpu_ctx.render_batch(mh, b)
         ref, _, _, _ = oracle.render(map1.data, b)

@@ -344,6 +344,73 @@ def test_emissive_layer_integer_formulas(blend):
         assert np.array_equal(((arena[:64] >> (8 * lane)) & 0xFF).astype(np.int64), e), lane
 
 
+def _dir_lightmap(ambient, vec, color, deviation=0.25, n=1 + 9):
+    lm = np.zeros(n, dtype=A.DIRLM_DTYPE)
+    lm["ambient_light"] = ambient
+    lm["light_direction_vector_scaled"] = vec
+    lm["directional_light_deviation"] = deviation
+    lm["directional_light_color"] = color
+    return lm
+
+
+def _wrap_u8(x):
+    """cvtps2dq (half-even) of min(x, 255), low byte (fast_math.rs:131-144)."""
+    return (np.rint(np.minimum(x, 255.0)).astype(np.int64)) & 0xFF
+
+
+@pytest.mark.parametrize("dz", [0.8, 0.0, -0.35])
+def test_directional_lightmap_without_normal_map(dz):
+    """Constant directional element, no normal map, no specular (surfaces.rs:551-572): light =
+    ambient + color * vector_scaled.z, with z NOT clamped, so a negative z darkens below ambient and
+    a negative total wraps to the low byte. Distinct R, G, B values pin the lane order
+    (fast_math.rs:167-175: [R, G, B] arrays land in lanes R, G, B of the B, G, R, A vector)."""
+    rng = np.random.default_rng(11)
+    diffuse = rng.integers(0, 2**32, size=16 * 16, dtype=np.uint64).astype(np.uint32)
+    ambient = np.array([0.30, 0.55, 0.10], np.float32)
+    color = np.array([1.40, 0.35, 0.90], np.float32)
+    vec = np.array([0.3, -0.6, dz], np.float32)
+    lm = _dir_lightmap(ambient, vec, color)
+    arena = _single_surface_scene(_texels(diffuse), 16, 16, lm_dir=lm,
+                                  surf_kw={"lightmap_offset": 1, "lightmap_size": (3, 3), "lightmap_scale_log2": 4,
+                                           "flags": A.SURF_DIFFUSE_LAYER | A.SURF_DIRECTIONAL})
+    light_rgb = ambient.astype(np.float64) + color.astype(np.float64) * float(dz)
+    for lane, li in ((0, light_rgb[2]), (1, light_rgb[1]), (2, light_rgb[0])):
+        ch = ((diffuse >> (8 * lane)) & 0xFF).astype(np.float64)
+        expect = _wrap_u8(ch * li)
+        got = ((arena[:256] >> (8 * lane)) & 0xFF).astype(np.int64)
+        diff = np.minimum((got - expect) % 256, (expect - got) % 256)
+        assert diff.max() <= 1, (lane, dz)
+    assert np.array_equal(arena[:256] >> 24, diffuse >> 24)  # alpha * the lane-3 1.0
+
+
+def test_directional_lightmap_with_normal_map():
+    """With a normal map the directional term is color * max(dot(vector_scaled, n), 0), n = the
+    texel's packed signed bytes / 127 (textures.rs:41-55), not renormalised."""
+    rng = np.random.default_rng(12)
+    n = 16 * 16
+    diffuse = rng.integers(0, 2**32, size=n, dtype=np.uint64).astype(np.uint32)
+    normals = scenes._rand_normals(rng, n, z_min=-0.5)
+    tex = np.zeros(n, dtype=A.TEXEL_DTYPE)
+    tex["diffuse"] = diffuse
+    tex["packed_normal_roughness"] = scenes.pack_normal_roughness(normals, np.ones(n))
+    ambient = np.array([0.05, 0.10, 0.20], np.float32)
+    color = np.array([0.9, 0.6, 1.3], np.float32)
+    vec = np.array([0.5, 0.2, 0.7], np.float32)
+    lm = _dir_lightmap(ambient, vec, color)
+    arena = _single_surface_scene(tex, 16, 16, lm_dir=lm, tex_flags=A.TEX_HAS_NORMAL_MAP,
+                                  surf_kw={"lightmap_offset": 1, "lightmap_size": (3, 3), "lightmap_scale_log2": 4,
+                                           "flags": A.SURF_DIFFUSE_LAYER | A.SURF_DIRECTIONAL})
+    packed = tex["packed_normal_roughness"].view(np.uint32)
+    nb = np.stack([((packed >> (8 * i)) & 0xFF).astype(np.int8).astype(np.float64) for i in range(3)], -1) / 127.0
+    dot = np.maximum(nb @ vec.astype(np.float64), 0.0)
+    assert np.any(dot == 0.0) and np.any(dot > 0.5)  # both sides of the clamp are exercised
+    for lane, c in ((0, 2), (1, 1), (2, 0)):
+        ch = ((diffuse >> (8 * lane)) & 0xFF).astype(np.float64)
+        expect = _wrap_u8(ch * (ambient[c] + color[c] * dot))
+        got = ((arena[:256] >> (8 * lane)) & 0xFF).astype(np.int64)
+        assert np.max(np.abs(got - expect)) <= 1, lane
+
+
 # ---------------------------------------------------------------------------------------------
 # Rasteriser invariants (rasterizer.rs)
 # ---------------------------------------------------------------------------------------------
@@ -487,6 +554,72 @@ def test_painter_order_last_writer_wins():
             alone, _, _ = _raster([specs[i]], W, H, texs)
             expect = np.where(alone != 0, alone, expect)
         assert np.array_equal(fb, expect)
+
+
+# ---------------------------------------------------------------------------------------------
+# Triangles (rasterizer.rs:731-1292): textured Gouraud over lite textures
+# ---------------------------------------------------------------------------------------------
+def _raster_tris(tris, W, H, lite, blend=A.BLEND_NONE, color_bits=32, bg=0):
+    """tris: list of 3-vertex lists of (x, y, tc_u, tc_v, light_r, light_g, light_b) in floats."""
+    draws = np.zeros(len(tris), dtype=A.DRAW_DTYPE)
+    vdata = []
+    for i, tri in enumerate(tris):
+        d = draws[i]
+        d["kind"], d["blend"] = A.DRAW_TRIANGLE, blend
+        d["first_vertex_int"], d["num_vertices"] = 21 * i, 3
+        d["texture_offset"] = 0
+        d["texture_size"] = (lite.shape[1], lite.shape[0])
+        d["clip_rect"] = (0, 0, W, H)
+        vdata.append(np.array(_fx(tri), np.int32))
+    v = np.zeros(1, dtype=A.VIEW_DTYPE)
+    v["width"], v["height"], v["num_draws"] = W, H, len(draws)
+    v["clear_background"], v["background_color"] = 1, bg
+    b = A.BatchData(v, np.zeros(0, A.SURFACE_DTYPE), np.zeros(0, A.LIGHT_DTYPE), np.zeros(0, np.uint32), draws,
+                    np.concatenate(vdata), color_bits, arena_texels=1)
+    m = A.MapData(np.zeros(1, A.TEXEL_DTYPE), lite.ravel().astype(np.uint32), np.zeros((1, 3), np.float32),
+                  np.zeros(1, A.DIRLM_DTYPE), np.zeros(0, A.LEVEL_DTYPE))
+    fb, _, st, oob = oracle.render(m, b)
+    return fb.reshape(H, W), st, oob
+
+
+@pytest.mark.parametrize("light", [(1.0, 1.0, 1.0), (1.5, 0.25, 0.75), (3.0, 2.0, 0.0)])
+def test_triangle_constant_light_closed_form(light):
+    """Constant texel T and constant light L: every covered pixel is min((T_c * L_c) >> 16, 255)
+    per channel with alpha 0 (rasterizer.rs:1239-1240, ColorVecI saturating pack)."""
+    T = 0x80C864F0
+    lite = np.full((8, 8), T, dtype=np.uint32)
+    lr, lg, lb = light
+    quad = [(3.5, 2.5), (20.5, 2.5), (20.5, 11.5), (3.5, 11.5)]
+    v = [(x, y, 4.0, 4.0, lr, lg, lb) for x, y in quad]
+    fb, st, oob = _raster_tris([[v[0], v[1], v[2]], [v[0], v[2], v[3]]], 32, 16, lite, bg=0x11111111)
+    assert oob == 0
+    exp = 0
+    for lane, lc in ((0, lb), (1, lg), (2, lr)):
+        c = (T >> (8 * lane)) & 0xFF
+        exp |= min((c * int(lc * 65536)) >> 16, 255) << (8 * lane)
+    expect = np.full((16, 32), 0x11111111, np.uint32)
+    expect[3:12, 4:21] = exp
+    assert np.array_equal(fb, expect)
+
+
+def test_triangle_pair_is_watertight():
+    """Two triangles sharing a random edge, together covering the screen: with additive blending of
+    a texel 1 at light 1.0 every pixel is written exactly once."""
+    rng = np.random.default_rng(13)
+    lite = np.full((4, 4), 1, dtype=np.uint32)
+    W, H = 48, 40
+    for _ in range(200):
+        a = (rng.uniform(-5, W + 5), rng.uniform(-5, H + 5))
+        big = [(-20.0, -20.0), (W + 20.0, -20.0), (W + 20.0, H + 20.0), (-20.0, H + 20.0)]
+        # split the big quad into a fan around `a`: 4 triangles sharing edges
+        tris = []
+        for j in range(4):
+            q0, q1 = big[j], big[(j + 1) % 4]
+            tris.append([(q0[0], q0[1], 1.0, 1.0, 1.0, 1.0, 1.0), (q1[0], q1[1], 1.0, 1.0, 1.0, 1.0, 1.0),
+                         (a[0], a[1], 1.0, 1.0, 1.0, 1.0, 1.0)])
+        fb, st, oob = _raster_tris(tris, W, H, lite, blend=A.BLEND_ADDITIVE)
+        assert oob == 0
+        assert np.all(fb == 1), np.unique(fb, return_counts=True)
 
 
 # ---------------------------------------------------------------------------------------------
