@@ -151,6 +151,9 @@ def main():
         st["emissive_texels"] = 0
         per_frame_stats.append(st)
 
+    # tile height for this frame size, chosen by the autotuner on the first pose
+    plain_rows, plain_tune = ctx.autotune(mh, batches[0], dev[0], out)
+    log(f"autotune (plain): {plain_rows} rows; ms per frame by rows {plain_tune}")
     for i in range(args.warmup):
         render(i)
     ctx.profile_read(reset=True)
@@ -165,6 +168,11 @@ def main():
     ctx.profile_enable(False)
     prof = ctx.profile_read(reset=True)
     fps = args.steps / wall
+    # the same loop without the per-launch timing events (host-side cost of the events)
+    t1 = time.perf_counter()
+    for i in range(args.steps):
+        render(i)
+    fps_no_events = args.steps / (time.perf_counter() - t1)
     log(f"plain: {fps:.1f} frames/s, p50 {np.percentile(lat, 50) * 1e3:.3f} ms, p99 {np.percentile(lat, 99) * 1e3:.3f} ms")
     log(f"kernels: { {k: (round(v[0] / max(v[1], 1) * 1e3, 2), v[1]) for k, v in prof.items()} } (us/launch, launches)")
 
@@ -205,6 +213,8 @@ def main():
         "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 4),
         "plain_gpixel_s": round(fps * args.width * args.height / 1e9, 4),
         "kernel_us_per_launch": {k: round(v[0] / max(v[1], 1) * 1e3, 3) for k, v in prof.items()},
+        "tile_rows": plain_rows,
+        "frames_s_without_event_timing": round(fps_no_events, 2),
         "roofline": roofline,
     }
 
@@ -225,6 +235,8 @@ def main():
         bdev = sw_abi.DeviceCommandBuffers(ctx, bb)
         bout = ctx.alloc(bb.fb_texels * 4)
         bcb = bb.ctypes_batch(inputs_on_device=True, device_ptrs=bdev.ptrs())
+        batch_rows, batch_tune = ctx.autotune(mh, bb, bdev, bout)
+        log(f"autotune (batch): {batch_rows} rows; ms per batch by rows {batch_tune}")
         ctx.check(lib.sw_render_batch_graph(ctx.h, mh, C.byref(bcb), bout.ptr), "sw_render_batch_graph")
         ctx.profile_enable(True)
         ctx.profile_read(reset=True)
@@ -232,7 +244,13 @@ def main():
         for _ in range(args.batch_reps):
             ctx.check(lib.sw_render_batch_graph(ctx.h, mh, C.byref(bcb), bout.ptr), "sw_render_batch_graph")
         bwall = (time.perf_counter() - tb) / args.batch_reps
+        # per-kernel durations of the batch: HIP events around the launches of eager runs (a
+        # captured graph's launches carry no events)
+        ctx.profile_read(reset=True)
+        for _ in range(2):
+            ctx.check(lib.sw_render_batch(ctx.h, mh, C.byref(bcb), bout.ptr), "sw_render_batch")
         ctx.profile_enable(False)
+        bprof = ctx.profile_read(reset=True)
         bst = ctx.stats()
         surf_texels = int(bb.arena_texels)
         s = bb.surfaces
@@ -244,7 +262,8 @@ def main():
                              "vs_plain_gpixel_s": round((total_px / bwall) / (fps * args.width * args.height), 3),
                              "achieved_gb_s": round(bbytes / bwall / 1e9, 2),
                              "hbm_frac": round(bbytes / bwall / 1e9 / HBM_PEAK_GBS, 5),
-                             "surfaces": int(len(bb.surfaces)), "draws": int(len(bb.draws))}
+                             "surfaces": int(len(bb.surfaces)), "draws": int(len(bb.draws)), "tile_rows": batch_rows,
+                             "kernel_us_per_batch": {k: round(v[0] / max(v[1], 1) * 1e3, 2) for k, v in bprof.items()}}
         log(f"batched: {result['batched']}")
         bdev.free()
         bout.free()

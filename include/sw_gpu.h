@@ -283,6 +283,9 @@ void sw_map_release(sw_ctx* ctx, sw_map* map);
  *   sw_raster_draws     <- PartialRenderer::perform_rasterization   (partial_renderer.rs:1192-1325)
  *   sw_frame_end        <- Renderer::draw_frame's write into &mut [ColorT] (renderer.rs:106-124)
  *   sw_tonemap          <- Postprocessor::perform_postprocessing, bloom off (postprocessor.rs:68-204)
+ * sw_build_surfaces copies the descriptors and returns; the surfaces are built on the device by
+ * the launch sequence of the next sw_raster_draws (or sw_frame_end / sw_read_surfaces), ahead of
+ * the draws that sample them, so a frame costs three kernel launches.
  * ------------------------------------------------------------------------------------------- */
 int sw_frame_begin(
 	sw_ctx* ctx, sw_map* map, uint32_t color_bits, uint32_t width, uint32_t height, const float cam_pos[3],
@@ -310,6 +313,20 @@ int sw_render_batch(sw_ctx* ctx, sw_map* map, const sw_batch* batch, void* out_f
  * HIP graph and replays it; inputs must be device-resident and unchanged in size. */
 int sw_render_batch_graph(sw_ctx* ctx, sw_map* map, const sw_batch* batch, void* out_framebuffer_device);
 
+/* Raster tile height. The raster kernel gives each 64-pixel-wide screen tile of 4, 8 or 16 rows
+ * to one wave (default 8). Small tiles give more waves (latency hiding on small frames), tall tiles
+ * amortise the per-tile draw binning (large batches). No counterpart in the reference (its
+ * rasteriser walks the rows of one screen rect per thread, rect_splitting.rs:23-78). */
+#define SW_TILE_ROWS_COUNT 3 /* candidates: 4, 8, 16 */
+int sw_set_tile_rows(sw_ctx* ctx, uint32_t rows);
+uint32_t sw_get_tile_rows(const sw_ctx* ctx);
+
+/* Autotuner: renders `batch` with every tile height, keeps the fastest for this context and
+ * reports the milliseconds per batch of each candidate (4, 8, 16 rows) in out_ms (may be NULL). */
+int sw_autotune_batch(
+	sw_ctx* ctx, sw_map* map, const sw_batch* batch, void* out_framebuffer_device, uint32_t* out_rows,
+	double out_ms[SW_TILE_ROWS_COUNT]);
+
 /* Tonemap a device Color64 buffer (pitch = width) into a device Color32 buffer. */
 int sw_tonemap_device(
 	sw_ctx* ctx, const void* hdr_device, uint32_t width, uint32_t height, float exposure,
@@ -334,10 +351,10 @@ int sw_copy_to_device(sw_ctx* ctx, void* device_dst, const void* host_src, uint6
 int sw_copy_from_device(sw_ctx* ctx, void* host_dst, const void* device_src, uint64_t bytes);
 
 /* Kernel timing with HIP events recorded on the context's stream around every launch.
- * Stages: 0 prep_surfaces, 1 build_surfaces, 2 setup_draws, 3 bin_bands, 4 raster_tiles,
- * 5 tonemap. sw_profile_read returns the summed milliseconds and launch counts since the last
- * reset (and resets when reset != 0). Not recorded inside captured graphs. */
-#define SW_PROFILE_STAGES 6
+ * Stages: 0 setup_draws (edge walk, span records and surface extents), 1 build_surfaces,
+ * 2 raster_tiles, 3 tonemap. sw_profile_read returns the summed milliseconds and launch counts
+ * since the last reset (and resets when reset != 0). Not recorded inside captured graphs. */
+#define SW_PROFILE_STAGES 4
 int sw_profile_enable(sw_ctx* ctx, uint32_t enable);
 int sw_profile_read(sw_ctx* ctx, double out_ms[SW_PROFILE_STAGES], uint64_t out_counts[SW_PROFILE_STAGES], uint32_t reset);
 

@@ -1,20 +1,25 @@
 // CDNA4 kernels of the surface builder, span rasteriser and tonemap.
 //
-// Pipeline for a batch of views (the plain path is a batch of one view):
-//   k_prep_surfaces   one thread per surface: global arena start/end, ordering check
-//   k_build_surfaces  one thread per arena texel (flattened over all surfaces of all views):
+// A batch of views (the plain path is a batch of one view) takes four launches:
+//   k_setup_draws     64 or 256 threads per draw: the edge walk of rasterizer.rs:137-271 (or the
+//                     triangle split of :765-982) executed identically by the draw's threads; each
+//                     part's rows are spread over them, and each writes one 32-byte span record per
+//                     (part, row) holding the span's clipped x range and the start / step of its
+//                     interpolants (:316-422, :500-556, :673-703, :1000-1214). Extra blocks compute
+//                     every surface's arena extent, validate the surface descriptors and fill the
+//                     block -> first surface table of the next kernel.
+//   k_build_surfaces  one thread per surface-arena texel, flattened over all surfaces of all views:
 //                     surfaces.rs:433-852 + :854-971 for that texel
-//   k_setup_draws     one wave per draw: the edge walk of rasterizer.rs:137-271 (or the triangle
-//                     split of :765-982) executed wave-uniformly; each part's rows are spread over
-//                     the lanes, which write one 32-byte span record per (part, row) holding the
-//                     span's clipped x range and the start/step of its interpolants
-//                     (:316-422, :500-556, :673-703, :1000-1214)
-//   k_bin_bands       one workgroup per (view, band of BAND_H rows): ordered compaction of the
-//                     draws overlapping the band (ballot + popcount prefix per wave, LDS across waves)
-//   k_raster_tiles    one wave per 64 x BAND_H screen tile: tile colour in LDS, walks the band's
-//                     draw list in draw order, per pixel: texel address from the span record, fetch,
+//   k_raster_tiles    one wave per 64 x ROWS screen tile (ROWS = 4, 8 or 16): ballots over the
+//                     view's draws pick the ones overlapping the tile, in draw order; the tile's
+//                     colour lives in LDS; a part's span records for the tile rows are loaded
+//                     across lanes at once and broadcast with readlane; the texel addresses of all
+//                     rows are computed and fetched together, then blended in row order with
 //                     write_into_framebuffer (:424-445, :558-572, :705-719, :1223-1279)
+//   k_finish          one block: per-draw statistics -> result counters, live counters cleared
 //   k_tonemap         one thread per pixel: postprocessor.rs:237-253
+#include <algorithm>
+
 #include "sw_device.h"
 #include "sw_internal.h"
 
@@ -24,22 +29,9 @@ using namespace swd;
 // Surfaces
 // =============================================================================================
 
-__global__ void k_prep_surfaces(
-	const sw_surface_desc* __restrict__ surfaces, uint32_t num_surfaces, const ViewExt* __restrict__ views,
-	uint32_t num_views, uint64_t* __restrict__ surf_end, uint64_t arena_texels, uint32_t num_levels,
-	const sw_texture_level* __restrict__ levels, uint64_t m_num_lm_simple, uint64_t m_num_lm_dir,
-	uint32_t* __restrict__ counters)
+// Arena extent and validity of surface s (run by the extra blocks of k_setup_draws).
+SW_DEV uint32_t view_of_surface(const ViewExt* __restrict__ views, uint32_t num_views, uint32_t s)
 {
-	const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-	if(s == 0)
-	{
-		// reset per-batch counters (no memset nodes: this runs inside captured graphs too)
-		for(int i = 0; i < COUNTER_COUNT; ++i)
-			counters[i] = 0;
-	}
-	if(s >= num_surfaces)
-		return;
-	// view owning surface s (views are sorted by first_surface)
 	uint32_t lo = 0, hi = num_views;
 	while(hi - lo > 1)
 	{
@@ -49,48 +41,62 @@ __global__ void k_prep_surfaces(
 		else
 			hi = mid;
 	}
+	return lo;
+}
+
+// Arena extent, owning view and validation of surface s; also fills the block -> first surface
+// table for the surface blocks whose first texel lies in [end of s - 1, end of s).
+SW_DEV void prep_surface(const sw_surface_desc* __restrict__ surfaces, uint32_t s, uint32_t num_surfaces,
+	const ViewExt* __restrict__ views, uint32_t num_views, SurfExt* __restrict__ surf_ext, uint32_t* __restrict__ blk_surf,
+	uint64_t arena_texels, const MapPtrs& m, uint32_t* __restrict__ counters)
+{
+	const uint32_t view = view_of_surface(views, num_views, s);
 	const sw_surface_desc& d = surfaces[s];
-	const uint64_t start = views[lo].v.surface_arena_offset + d.pixels_offset;
+	const uint64_t start = views[view].v.surface_arena_offset + d.pixels_offset;
 	const uint64_t end = start + uint64_t(d.size[0]) * d.size[1];
-	surf_end[s] = end;
+	surf_ext[s] = SurfExt{end, view, 0u};
 	bool bad = end > arena_texels || d.size[0] > 65536u || d.size[1] > 65536u;
-	if((d.flags & SW_SURF_DIFFUSE_LAYER) && (d.texture_level >= num_levels || (levels[d.texture_level].flags & SW_TEX_LITE_LEVEL)))
+	if((d.flags & SW_SURF_DIFFUSE_LAYER) &&
+		(d.texture_level >= m.num_levels || (m.levels[d.texture_level].flags & SW_TEX_LITE_LEVEL)))
 		bad = true;
 	if((d.flags & SW_SURF_EMISSIVE) && d.emissive_level != SW_NO_TEXTURE &&
-		(d.emissive_level >= num_levels || !(levels[d.emissive_level].flags & SW_TEX_LITE_LEVEL)))
+		(d.emissive_level >= m.num_levels || !(m.levels[d.emissive_level].flags & SW_TEX_LITE_LEVEL)))
 		bad = true;
 	if((d.flags & SW_SURF_DIFFUSE_LAYER) && d.lightmap_offset != 0 && d.size[0] > 0 && d.size[1] > 0)
 	{
 		// Every lightmap fetch (rows lv, lv + 1; columns lu, lu + 1) must stay inside the polygon's
 		// lightmap (the reference's debug_assert!s at surfaces.rs:482, :541) and inside the buffer.
 		const uint32_t sl = d.lightmap_scale_log2;
-		const uint64_t num_lm = (d.flags & SW_SURF_DIRECTIONAL) ? m_num_lm_dir : m_num_lm_simple;
+		const uint64_t num_lm = (d.flags & SW_SURF_DIRECTIONAL) ? m.num_lm_dir : m.num_lm_simple;
 		if(sl > 4 || d.lightmap_size[0] < 2 || d.lightmap_size[1] < 2 ||
 			((uint64_t(d.size[0]) - 1 + d.lightmap_tc_shift[0]) >> sl) + 1 >= d.lightmap_size[0] ||
 			((uint64_t(d.size[1]) - 1 + d.lightmap_tc_shift[1]) >> sl) + 1 >= d.lightmap_size[1] ||
 			uint64_t(d.lightmap_offset) + uint64_t(d.lightmap_size[0]) * d.lightmap_size[1] > num_lm)
 			bad = true;
 	}
+	uint64_t prev_end = 0;
 	if(s > 0)
 	{
 		// Arena ranges must follow descriptor order (the reference's bump allocator,
-		// partial_renderer.rs:1617-1618). Recompute the predecessor's end to check.
-		uint32_t plo = 0, phi = num_views;
-		while(phi - plo > 1)
-		{
-			const uint32_t mid = (plo + phi) >> 1;
-			if(views[mid].v.first_surface <= s - 1)
-				plo = mid;
-			else
-				phi = mid;
-		}
+		// partial_renderer.rs:1617-1618): compare with the predecessor's end.
+		const uint32_t pv = view_of_surface(views, num_views, s - 1);
 		const sw_surface_desc& p = surfaces[s - 1];
-		const uint64_t pend = views[plo].v.surface_arena_offset + p.pixels_offset + uint64_t(p.size[0]) * p.size[1];
-		if(pend > start)
+		prev_end = views[pv].v.surface_arena_offset + p.pixels_offset + uint64_t(p.size[0]) * p.size[1];
+		if(prev_end > start)
 			bad = true;
 	}
 	if(bad)
+	{
 		atomicOr(&counters[COUNTER_ERROR], ERROR_BAD_SURFACE);
+		return;
+	}
+	// Blocks whose first texel lies in [prev_end, end) start their search at s; the last surface
+	// also takes the blocks past its end (their texels belong to no surface).
+	const uint64_t num_blocks = (arena_texels + SURF_BLOCK - 1) / SURF_BLOCK;
+	const uint64_t b0 = (prev_end + SURF_BLOCK - 1) / SURF_BLOCK;
+	const uint64_t b1 = s + 1 == num_surfaces ? num_blocks : std::min<uint64_t>((end + SURF_BLOCK - 1) / SURF_BLOCK, num_blocks);
+	for(uint64_t b = b0; b < b1; ++b)
+		blk_surf[b] = s;
 }
 
 struct LmE
@@ -409,61 +415,45 @@ SW_DEV typename CT::T emissive_texel(const MapPtrs& m, const sw_surface_desc& s,
 }
 
 template <class CT>
-__global__ __launch_bounds__(256) void k_build_surfaces(
-	MapPtrs m, const sw_surface_desc* __restrict__ surfaces, uint32_t num_surfaces, const uint64_t* __restrict__ surf_end,
-	const ViewExt* __restrict__ views, uint32_t num_views, const sw_light* __restrict__ lights, uint32_t num_lights,
+__global__ __launch_bounds__(SURF_BLOCK) void k_build_surfaces(MapPtrs m, const sw_surface_desc* __restrict__ surfaces,
+	uint32_t num_surfaces, const SurfExt* __restrict__ surf_ext, const uint32_t* __restrict__ blk_surf,
+	const ViewExt* __restrict__ views, const sw_light* __restrict__ lights, uint32_t num_lights,
 	const uint32_t* __restrict__ light_indices, uint32_t num_light_indices, typename CT::T* __restrict__ arena,
 	uint64_t arena_texels, const uint32_t* __restrict__ counters)
 {
 	if(counters[COUNTER_ERROR] & ERROR_BAD_SURFACE)
-		return;
-	const uint64_t idx = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-	// Wave-level search for the first surface whose end lies past the wave's first texel.
-	const uint64_t wave_first = idx & ~uint64_t(63);
-	uint32_t s0 = 0;
-	if((threadIdx.x & 63) == 0)
-	{
-		uint32_t lo = 0, hi = num_surfaces;
-		while(lo < hi)
-		{
-			const uint32_t mid = (lo + hi) >> 1;
-			if(surf_end[mid] <= wave_first)
-				lo = mid + 1;
-			else
-				hi = mid;
-		}
-		s0 = lo;
-	}
-	s0 = __shfl(s0, 0);
+		return; // the host reports the invalid descriptor; nothing is written
+	const uint64_t idx = uint64_t(blockIdx.x) * SURF_BLOCK + threadIdx.x;
 	if(idx >= arena_texels)
 		return;
-	uint32_t s = s0;
-	while(s < num_surfaces && surf_end[s] <= idx)
+	// The block's first surface comes from the table; a block spans few surfaces (most surfaces are
+	// larger than a block), so a short forward scan finds each texel's.
+	uint32_t s = blk_surf[blockIdx.x];
+	while(s < num_surfaces && surf_ext[s].end <= idx)
 		++s;
 	if(s >= num_surfaces)
 		return;
 	const sw_surface_desc& d = surfaces[s];
+	const SurfExt ext = surf_ext[s];
 	const uint64_t size = uint64_t(d.size[0]) * d.size[1];
-	const uint64_t start = surf_end[s] - size;
+	const uint64_t start = ext.end - size;
 	if(idx < start)
 		return; // gap between surfaces
-	uint32_t lo = 0, hi = num_views;
-	while(hi - lo > 1)
-	{
-		const uint32_t mid = (lo + hi) >> 1;
-		if(views[mid].v.first_surface <= s)
-			lo = mid;
-		else
-			hi = mid;
-	}
-	const V3 cam = v3(views[lo].v.cam_pos[0], views[lo].v.cam_pos[1], views[lo].v.cam_pos[2]);
+	const sw_view& view = views[ext.view].v;
+	const V3 cam = v3(view.cam_pos[0], view.cam_pos[1], view.cam_pos[2]);
 	const uint32_t local = uint32_t(idx - start);
 	const uint32_t du = local % d.size[0];
 	const uint32_t dv = local / d.size[0];
-	typename CT::T out = arena[idx];
-	if(d.flags & SW_SURF_DIFFUSE_LAYER)
+	const bool diffuse = (d.flags & SW_SURF_DIFFUSE_LAYER) != 0;
+	const bool emissive = (d.flags & SW_SURF_EMISSIVE) && d.emissive_level < m.num_levels;
+	if(!diffuse && !emissive)
+		return; // the reference leaves such a surface's texels untouched
+	typename CT::T out = 0;
+	if(diffuse)
 		out = surface_texel<CT>(m, d, lights, light_indices, num_light_indices, num_lights, cam, du, dv);
-	if((d.flags & SW_SURF_EMISSIVE) && d.emissive_level < m.num_levels)
+	else if(d.emissive_blend != SW_BLEND_NONE)
+		out = arena[idx]; // an emissive layer without a diffuse layer blends over what the arena holds
+	if(emissive)
 		out = emissive_texel<CT>(m, d, d.emissive_blend, out, du, dv);
 	arena[idx] = out;
 }
@@ -711,20 +701,33 @@ SW_DEV void tri_part_setup(const int32_t* tsize, int32_t y_start, int32_t y_end,
 	}
 }
 
+// Per-thread state of one draw's setup; the draw's `lanes` threads walk its edges identically and
+// split the rows of each part among themselves.
 struct SetupState
 {
-	uint32_t draw;
+	uint32_t lane, lanes;
 	uint32_t num_parts;
+	uint64_t next_span; // next free span record of this draw
+	uint64_t span_end;
+	bool overflow;
 	int32_t row_min, row_max, x_min, x_max;
-	uint64_t pixels, blend_pixels;
+	uint64_t pixels;
 };
 
-// Emits one part: rows [y_start_int, y_end_int) spread over the wave's lanes.
+// Rows a part covers after clipping (rasterizer.rs:316-317, :1000-1005).
+SW_DEV uint32_t part_rows(const int32_t clip[4], int32_t y_start, int32_t y_end)
+{
+	const int32_t a = imax32(fixed16_round_to_int(y_start), clip[1]);
+	const int32_t b = imin32(fixed16_round_to_int(y_end), clip[3]);
+	return a < b ? uint32_t(b - a) : 0u;
+}
+
+// Emits one part: rows [y_start_int, y_end_int) spread over the draw's lanes. Bounds and pixel
+// counts accumulate per thread (reduced once per draw).
 // kind: 0 perspective, 1 line z, 2 affine, 3 triangle.
 SW_DEV void emit_part(const sw_draw& d, const PolyEq& e, uint32_t kind, const int32_t clip[4], int32_t y_start,
 	int32_t y_end, Side ls, Side rs, const AffinePart* ap, const TriPart* tp, PartRec* __restrict__ parts,
-	SpanRec* __restrict__ spans, uint32_t* __restrict__ counters, uint64_t span_capacity, uint32_t part_slot_base,
-	SetupState& st)
+	SpanRec* __restrict__ spans, uint32_t part_slot_base, SetupState& st)
 {
 	const int32_t y_start_int = imax32(fixed16_round_to_int(y_start), clip[1]);
 	const int32_t y_end_int = imin32(fixed16_round_to_int(y_end), clip[3]);
@@ -736,13 +739,9 @@ SW_DEV void emit_part(const sw_draw& d, const PolyEq& e, uint32_t kind, const in
 	const int32_t x_left0 = ls.x_start + fixed16_mul(y_start_delta, ls.dx_dy) + FIXED16_HALF;
 	const int32_t x_right0 = rs.x_start + fixed16_mul(y_start_delta, rs.dx_dy) + FIXED16_HALF;
 	const uint32_t rows = uint32_t(y_end_int - y_start_int);
-	const uint32_t lane = threadIdx.x & 63;
-
-	uint64_t base = 0;
-	if(lane == 0)
-		base = atomicAdd(reinterpret_cast<unsigned long long*>(&counters[COUNTER_SPANS_LO]), (unsigned long long)rows);
-	base = __shfl(base, 0);
-	const bool fits = base + rows <= span_capacity;
+	const uint64_t base = st.next_span;
+	const bool fits = !st.overflow && base + rows <= st.span_end;
+	st.next_span += rows;
 
 	PartRec pr;
 	pr.row0 = y_start_int;
@@ -759,19 +758,16 @@ SW_DEV void emit_part(const sw_draw& d, const PolyEq& e, uint32_t kind, const in
 		pr.d[3] = tp->d_light_dx[1];
 		pr.d[4] = tp->d_light_dx[2];
 	}
-	if(lane == 0)
+	if(st.lane == 0)
 		parts[part_slot_base + st.num_parts] = pr;
 	st.num_parts++;
 	if(!fits)
 	{
-		if(lane == 0)
-			atomicOr(&counters[COUNTER_ERROR], ERROR_SPAN_OVERFLOW);
+		st.overflow = true;
 		return;
 	}
 
-	int32_t lx_min = INT32_MAX, lx_max = INT32_MIN, ly_min = INT32_MAX, ly_max = INT32_MIN;
-	uint64_t lpix = 0;
-	for(uint32_t k = lane; k < rows; k += 64)
+	for(uint32_t k = st.lane; k < rows; k += st.lanes)
 	{
 		const int32_t y = y_start_int + int32_t(k);
 		const int32_t x_left = wadd(x_left0, wmul(int32_t(k), ls.dx_dy));
@@ -817,397 +813,659 @@ SW_DEV void emit_part(const sw_draw& d, const PolyEq& e, uint32_t kind, const in
 					r.v[2 + i] = ll + fixed16_mul(x_start_delta, tp->d_light_dx[i]);
 				}
 			}
-			lx_min = imin32(lx_min, x_start_int);
-			lx_max = imax32(lx_max, x_end_int);
-			ly_min = imin32(ly_min, y);
-			ly_max = imax32(ly_max, y + 1);
-			lpix += uint64_t(x_end_int - x_start_int);
+			st.x_min = imin32(st.x_min, x_start_int);
+			st.x_max = imax32(st.x_max, x_end_int);
+			st.row_min = imin32(st.row_min, y);
+			st.row_max = imax32(st.row_max, y + 1);
+			st.pixels += uint64_t(x_end_int - x_start_int);
 		}
 		spans[base + k] = r;
 	}
-	// wave reductions
-	for(int off = 32; off > 0; off >>= 1)
-	{
-		lx_min = imin32(lx_min, __shfl_xor(lx_min, off));
-		lx_max = imax32(lx_max, __shfl_xor(lx_max, off));
-		ly_min = imin32(ly_min, __shfl_xor(ly_min, off));
-		ly_max = imax32(ly_max, __shfl_xor(ly_max, off));
-		lpix += __shfl_xor(lpix, off);
-	}
-	st.x_min = imin32(st.x_min, lx_min);
-	st.x_max = imax32(st.x_max, lx_max);
-	st.row_min = imin32(st.row_min, ly_min);
-	st.row_max = imax32(st.row_max, ly_max);
-	st.pixels += lpix;
 }
 
-__global__ __launch_bounds__(256) void k_setup_draws(const sw_draw* __restrict__ draws, uint32_t num_draws,
-	const int32_t* __restrict__ vdata, uint64_t num_vertex_ints, const ViewExt* __restrict__ views, uint32_t num_views,
-	PartRec* __restrict__ parts, SpanRec* __restrict__ spans, DrawRec* __restrict__ drec, uint32_t* __restrict__ counters,
-	uint64_t span_capacity, uint64_t arena_texels, uint64_t skybox_texels, uint64_t lite_texels)
+// The polygon edge walk of rasterizer.rs:160-270, executed identically by every lane of the wave.
+// EMIT = false only counts the rows of the parts (no divisions); EMIT = true writes parts / spans.
+template <bool EMIT>
+SW_DEV uint32_t walk_polygon(const sw_draw& d, const int32_t* __restrict__ v, const int32_t clip[4],
+	PartRec* __restrict__ parts, SpanRec* __restrict__ spans, uint32_t part_slot_base, SetupState& st)
 {
-	const uint32_t di = blockIdx.x * (blockDim.x / 64) + (threadIdx.x / 64);
-	if(di >= num_draws)
-		return;
-	const uint32_t lane = threadIdx.x & 63;
-	const sw_draw d = draws[di];
-	uint32_t lo = 0, hi = num_views;
-	while(hi - lo > 1)
-	{
-		const uint32_t mid = (lo + hi) >> 1;
-		if(views[mid].v.first_draw <= di)
-			lo = mid;
-		else
-			hi = mid;
-	}
-	const sw_view& view = views[lo].v;
-	// the draw's clip rect, kept inside the view (memory safety; the reference's rects always are)
-	const int32_t clip[4] = {imax32(d.clip_rect[0], 0), imax32(d.clip_rect[1], 0), imin32(d.clip_rect[2], int32_t(view.width)),
-		imin32(d.clip_rect[3], int32_t(view.height))};
-
-	SetupState st;
-	st.draw = di;
-	st.num_parts = 0;
-	st.row_min = INT32_MAX;
-	st.row_max = INT32_MIN;
-	st.x_min = INT32_MAX;
-	st.x_max = INT32_MIN;
-	st.pixels = 0;
-	st.blend_pixels = 0;
-	const uint32_t part_slot_base = di * MAX_PARTS_PER_DRAW;
-
-	// Texture and vertex ranges must lie inside their buffers (the reference indexes them unchecked
-	// in release builds, fast_math.rs:750-777; here an invalid draw is skipped instead).
-	const uint64_t tex_texels = uint64_t(uint32_t(d.texture_size[0])) * uint32_t(d.texture_size[1]);
-	bool valid = d.texture_size[0] > 0 && d.texture_size[1] > 0 && d.texture_size[0] <= 65536 && d.texture_size[1] <= 65536;
-	if(d.kind == SW_DRAW_POLYGON)
-	{
-		const uint64_t limit = d.texture_source == SW_TEXSRC_SKYBOX ? skybox_texels : arena_texels;
-		const uint64_t base = d.texture_source == SW_TEXSRC_SKYBOX ? 0 : view.surface_arena_offset;
-		valid = valid && d.num_vertices >= 3 && d.num_vertices <= SW_MAX_POLYGON_VERTICES &&
-			uint64_t(d.first_vertex_int) + 2ull * d.num_vertices <= num_vertex_ints &&
-			base + d.texture_offset + tex_texels <= limit && d.blend <= SW_BLEND_ALPHA_BLEND;
-	}
-	else
-		valid = valid && d.kind == SW_DRAW_TRIANGLE && uint64_t(d.first_vertex_int) + 21ull <= num_vertex_ints &&
-			d.texture_offset + tex_texels <= lite_texels && d.blend <= SW_BLEND_ALPHA_BLEND;
-
-	if(valid && d.kind == SW_DRAW_POLYGON)
-	{
-		// rasterizer.rs:137-271, executed identically by every lane
-		const uint32_t n = d.num_vertices;
-		const int32_t* v = vdata + d.first_vertex_int;
-		const PolyEq e = make_poly_eq(d);
-		const uint32_t kind = d.mode == SW_TC_LINE_Z ? 1u : (d.mode == SW_TC_AFFINE ? 2u : 0u);
-		uint32_t lower = 0;
-		int32_t min_y = v[1];
-		for(uint32_t i = 0; i < n; ++i)
-			if(v[i * 2 + 1] < min_y)
-			{
-				min_y = v[i * 2 + 1];
-				lower = i;
-			}
-		uint32_t left = lower, right = lower;
-		int32_t cur_y = min_y;
-		for(uint32_t iter = 0; iter < n; ++iter)
+	const uint32_t n = d.num_vertices;
+	const uint32_t kind = d.mode == SW_TC_LINE_Z ? 1u : (d.mode == SW_TC_AFFINE ? 2u : 0u);
+	PolyEq e;
+	if(EMIT)
+		e = make_poly_eq(d);
+	uint32_t lower = 0;
+	int32_t min_y = v[1];
+	for(uint32_t i = 0; i < n; ++i)
+		if(v[i * 2 + 1] < min_y)
 		{
-			uint32_t nl = left + n - 1;
-			if(nl >= n)
-				nl -= n;
-			uint32_t nr = right + 1;
-			if(nr >= n)
-				nr -= n;
-			const int32_t lx = v[left * 2], ly = v[left * 2 + 1], nlx = v[nl * 2], nly = v[nl * 2 + 1];
-			const int32_t rx = v[right * 2], ry = v[right * 2 + 1], nrx = v[nr * 2], nry = v[nr * 2 + 1];
-			const int32_t dy_left = nly - ly;
-			const int32_t dy_right = nry - ry;
-			const int32_t next_y = imin32(nly, nry);
-			bool have = false;
-			Side ls, rs;
-			if(dy_left > FIXED16_HALF && dy_right > FIXED16_HALF)
+			min_y = v[i * 2 + 1];
+			lower = i;
+		}
+	uint32_t left = lower, right = lower;
+	int32_t cur_y = min_y;
+	uint32_t rows = 0;
+	for(uint32_t iter = 0; iter < n; ++iter)
+	{
+		uint32_t nl = left + n - 1;
+		if(nl >= n)
+			nl -= n;
+		uint32_t nr = right + 1;
+		if(nr >= n)
+			nr -= n;
+		const int32_t lx = v[left * 2], ly = v[left * 2 + 1], nlx = v[nl * 2], nly = v[nl * 2 + 1];
+		const int32_t rx = v[right * 2], ry = v[right * 2 + 1], nrx = v[nr * 2], nry = v[nr * 2 + 1];
+		const int32_t dy_left = nly - ly;
+		const int32_t dy_right = nry - ry;
+		const int32_t next_y = imin32(nly, nry);
+		const bool wide = dy_left > FIXED16_HALF && dy_right > FIXED16_HALF;
+		const bool thin = !wide && dy_left > 0 && dy_right > 0 && fixed16_round_to_int(cur_y) < fixed16_round_to_int(next_y);
+		if(wide || thin)
+		{
+			if(!EMIT)
+				rows += part_rows(clip, cur_y, next_y);
+			else
 			{
-				const int32_t dxl = fixed16_div(nlx - lx, dy_left);
-				const int32_t dxr = fixed16_div(nrx - rx, dy_right);
-				ls = Side{lx + fixed16_mul(dxl, cur_y - ly), dxl};
-				rs = Side{rx + fixed16_mul(dxr, cur_y - ry), dxr};
-				have = true;
-			}
-			else if(dy_left > 0 && dy_right > 0)
-			{
-				const int32_t cy = fixed16_round_to_int(cur_y);
-				const int32_t ny = fixed16_round_to_int(next_y);
-				if(cy < ny)
+				Side ls, rs;
+				if(wide)
 				{
-					const int32_t thin = int_to_fixed16(cy) + FIXED16_HALF;
-					ls = Side{lx + fixed16_mul_div(thin - ly, nlx - lx, dy_left), 0};
-					rs = Side{rx + fixed16_mul_div(thin - ry, nrx - rx, dy_right), 0};
-					have = true;
+					const int32_t dxl = fixed16_div(nlx - lx, dy_left);
+					const int32_t dxr = fixed16_div(nrx - rx, dy_right);
+					ls = Side{lx + fixed16_mul(dxl, cur_y - ly), dxl};
+					rs = Side{rx + fixed16_mul(dxr, cur_y - ry), dxr};
 				}
-			}
-			if(have)
-			{
+				else
+				{
+					const int32_t thin_y = int_to_fixed16(fixed16_round_to_int(cur_y)) + FIXED16_HALF;
+					ls = Side{lx + fixed16_mul_div(thin_y - ly, nlx - lx, dy_left), 0};
+					rs = Side{rx + fixed16_mul_div(thin_y - ry, nrx - rx, dy_right), 0};
+				}
 				if(kind == 2)
 				{
 					const int32_t y_start_int = imax32(fixed16_round_to_int(cur_y), clip[1]);
 					const int32_t y_start_delta = int_to_fixed16(y_start_int) + FIXED16_HALF - cur_y;
 					const AffinePart ap = affine_part_setup(d, cur_y, next_y, ls, rs, y_start_delta);
-					emit_part(d, e, kind, clip, cur_y, next_y, ls, rs, &ap, nullptr, parts, spans, counters, span_capacity,
-						part_slot_base, st);
+					emit_part(d, e, kind, clip, cur_y, next_y, ls, rs, &ap, nullptr, parts, spans, part_slot_base, st);
 				}
 				else
-					emit_part(d, e, kind, clip, cur_y, next_y, ls, rs, nullptr, nullptr, parts, spans, counters,
-						span_capacity, part_slot_base, st);
+					emit_part(d, e, kind, clip, cur_y, next_y, ls, rs, nullptr, nullptr, parts, spans, part_slot_base, st);
 			}
-			if(nl == nr)
-				break;
-			if(nry < nly)
-				right = nr;
-			else
-				left = nl;
-			cur_y = next_y;
+		}
+		if(nl == nr)
+			break;
+		if(nry < nly)
+			right = nr;
+		else
+			left = nl;
+		cur_y = next_y;
+	}
+	return rows;
+}
+
+// The triangle split of rasterizer.rs:765-982 (sorting, long edge, middle vertex).
+struct TriSplit
+{
+	int32_t y0[2], y1[2];
+	Side ls[2], rs[2];
+	int32_t tc_start[2][2], dtc_dy[2][2], light_start[2][3], dlight_dy[2][3];
+	int32_t d_tc_dx[2], d_light_dx[3];
+};
+
+SW_DEV void split_triangle(const int32_t* __restrict__ p, TriSplit& t)
+{
+	int32_t vx[3], vy[3], vtc[3][2], vl[3][3];
+#pragma unroll
+	for(int i = 0; i < 3; ++i)
+	{
+		vx[i] = p[i * 7 + 0];
+		vy[i] = p[i * 7 + 1];
+		vtc[i][0] = p[i * 7 + 2];
+		vtc[i][1] = p[i * 7 + 3];
+		vl[i][0] = p[i * 7 + 4];
+		vl[i][1] = p[i * 7 + 5];
+		vl[i][2] = p[i * 7 + 6];
+	}
+	int up, lw;
+	if(vy[0] >= vy[1] && vy[0] >= vy[2])
+	{
+		up = 0;
+		lw = vy[1] < vy[2] ? 1 : 2;
+	}
+	else if(vy[1] >= vy[0] && vy[1] >= vy[2])
+	{
+		up = 1;
+		lw = vy[0] < vy[2] ? 0 : 2;
+	}
+	else
+	{
+		up = 2;
+		lw = vy[0] < vy[1] ? 0 : 1;
+	}
+	const int mi = 3 - up - lw;
+	// select the three vertices without dynamic register indexing
+	int32_t ux = 0, uy = 0, lx = 0, ly = 0, mx = 0, my = 0, utc[2] = {0, 0}, ltc[2] = {0, 0}, mtc[2] = {0, 0};
+	int32_t ul[3] = {0, 0, 0}, ll[3] = {0, 0, 0}, ml[3] = {0, 0, 0};
+#pragma unroll
+	for(int i = 0; i < 3; ++i)
+	{
+		if(i == up)
+		{
+			ux = vx[i], uy = vy[i], utc[0] = vtc[i][0], utc[1] = vtc[i][1], ul[0] = vl[i][0], ul[1] = vl[i][1], ul[2] = vl[i][2];
+		}
+		if(i == lw)
+		{
+			lx = vx[i], ly = vy[i], ltc[0] = vtc[i][0], ltc[1] = vtc[i][1], ll[0] = vl[i][0], ll[1] = vl[i][1], ll[2] = vl[i][2];
+		}
+		if(i == mi)
+		{
+			mx = vx[i], my = vy[i], mtc[0] = vtc[i][0], mtc[1] = vtc[i][1], ml[0] = vl[i][0], ml[1] = vl[i][1], ml[2] = vl[i][2];
 		}
 	}
-	else if(valid)
+	const int32_t long_edge_dy = imax32(uy - ly, FIXED16_HALF);
+	const int32_t lower_part_dy = imax32(my - ly, FIXED16_HALF);
+	const int32_t upper_part_dy = imax32(uy - my, FIXED16_HALF);
+	const int32_t long_edge_dx_dy = fixed16_div(ux - lx, long_edge_dy);
+	const int32_t long_edge_x_in_middle = lx + fixed16_mul(long_edge_dx_dy, my - ly);
+	const int32_t lower_part_dx_dy = fixed16_div(mx - lx, lower_part_dy);
+	const int32_t upper_part_dx_dy = fixed16_div(ux - mx, upper_part_dy);
+	int32_t le_dtc[2], le_tc_mid[2], dtc_lower[2], dtc_upper[2];
+#pragma unroll
+	for(int i = 0; i < 2; ++i)
 	{
-		// rasterizer.rs:765-982
-		const int32_t* p = vdata + d.first_vertex_int;
-		int32_t vx[3], vy[3], vtc[3][2], vl[3][3];
+		le_dtc[i] = fixed16_div(utc[i] - ltc[i], long_edge_dy);
+		le_tc_mid[i] = ltc[i] + fixed16_mul(le_dtc[i], lower_part_dy);
+		dtc_lower[i] = fixed16_div(mtc[i] - ltc[i], lower_part_dy);
+		dtc_upper[i] = fixed16_div(utc[i] - mtc[i], upper_part_dy);
+	}
+	int32_t le_dl[3], le_l_mid[3], dl_lower[3], dl_upper[3];
+#pragma unroll
+	for(int i = 0; i < 3; ++i)
+	{
+		le_dl[i] = fixed16_div(ul[i] - ll[i], long_edge_dy);
+		le_l_mid[i] = ll[i] + fixed16_mul(le_dl[i], lower_part_dy);
+		dl_lower[i] = fixed16_div(ml[i] - ll[i], lower_part_dy);
+		dl_upper[i] = fixed16_div(ul[i] - ml[i], upper_part_dy);
+	}
+	t.y0[0] = ly;
+	t.y1[0] = my;
+	t.y0[1] = my;
+	t.y1[1] = uy;
+	if(long_edge_x_in_middle >= mx)
+	{
+		const int32_t middle_dx = imax32(long_edge_x_in_middle - mx, FIXED16_HALF);
+#pragma unroll
+		for(int i = 0; i < 2; ++i)
+			t.d_tc_dx[i] = fixed16_div(le_tc_mid[i] - mtc[i], middle_dx);
+#pragma unroll
 		for(int i = 0; i < 3; ++i)
-		{
-			vx[i] = p[i * 7 + 0];
-			vy[i] = p[i * 7 + 1];
-			vtc[i][0] = p[i * 7 + 2];
-			vtc[i][1] = p[i * 7 + 3];
-			vl[i][0] = p[i * 7 + 4];
-			vl[i][1] = p[i * 7 + 5];
-			vl[i][2] = p[i * 7 + 6];
-		}
-		int up, lw;
-		if(vy[0] >= vy[1] && vy[0] >= vy[2])
-		{
-			up = 0;
-			lw = vy[1] < vy[2] ? 1 : 2;
-		}
-		else if(vy[1] >= vy[0] && vy[1] >= vy[2])
-		{
-			up = 1;
-			lw = vy[0] < vy[2] ? 0 : 2;
-		}
-		else
-		{
-			up = 2;
-			lw = vy[0] < vy[1] ? 0 : 1;
-		}
-		const int mi = 3 - up - lw;
-		const int32_t long_edge_dy = imax32(vy[up] - vy[lw], FIXED16_HALF);
-		const int32_t lower_part_dy = imax32(vy[mi] - vy[lw], FIXED16_HALF);
-		const int32_t upper_part_dy = imax32(vy[up] - vy[mi], FIXED16_HALF);
-		const int32_t long_edge_dx_dy = fixed16_div(vx[up] - vx[lw], long_edge_dy);
-		const int32_t long_edge_x_in_middle = vx[lw] + fixed16_mul(long_edge_dx_dy, vy[mi] - vy[lw]);
-		const int32_t lower_part_dx_dy = fixed16_div(vx[mi] - vx[lw], lower_part_dy);
-		const int32_t upper_part_dx_dy = fixed16_div(vx[up] - vx[mi], upper_part_dy);
-		int32_t le_dtc[2], le_tc_mid[2], dtc_lower[2], dtc_upper[2];
+			t.d_light_dx[i] = fixed16_div(le_l_mid[i] - ml[i], middle_dx);
+		t.ls[0] = Side{lx, lower_part_dx_dy};
+		t.rs[0] = Side{lx, long_edge_dx_dy};
+		t.ls[1] = Side{mx, upper_part_dx_dy};
+		t.rs[1] = Side{long_edge_x_in_middle, long_edge_dx_dy};
+#pragma unroll
 		for(int i = 0; i < 2; ++i)
 		{
-			le_dtc[i] = fixed16_div(vtc[up][i] - vtc[lw][i], long_edge_dy);
-			le_tc_mid[i] = vtc[lw][i] + fixed16_mul(le_dtc[i], lower_part_dy);
-			dtc_lower[i] = fixed16_div(vtc[mi][i] - vtc[lw][i], lower_part_dy);
-			dtc_upper[i] = fixed16_div(vtc[up][i] - vtc[mi][i], upper_part_dy);
+			t.tc_start[0][i] = ltc[i];
+			t.dtc_dy[0][i] = dtc_lower[i];
+			t.tc_start[1][i] = mtc[i];
+			t.dtc_dy[1][i] = dtc_upper[i];
 		}
-		int32_t le_dl[3], le_l_mid[3], dl_lower[3], dl_upper[3];
+#pragma unroll
 		for(int i = 0; i < 3; ++i)
 		{
-			le_dl[i] = fixed16_div(vl[up][i] - vl[lw][i], long_edge_dy);
-			le_l_mid[i] = vl[lw][i] + fixed16_mul(le_dl[i], lower_part_dy);
-			dl_lower[i] = fixed16_div(vl[mi][i] - vl[lw][i], lower_part_dy);
-			dl_upper[i] = fixed16_div(vl[up][i] - vl[mi][i], upper_part_dy);
-		}
-		const PolyEq e = PolyEq{};
-		int32_t y0[2], y1[2], tcs[2][2], dtcy[2][2], ls_[3][2], dly[2][3];
-		Side lsd[2], rsd[2];
-		int32_t d_tc_dx[2], d_light_dx[3];
-		if(long_edge_x_in_middle >= vx[mi])
-		{
-			const int32_t middle_dx = imax32(long_edge_x_in_middle - vx[mi], FIXED16_HALF);
-			for(int i = 0; i < 2; ++i)
-				d_tc_dx[i] = fixed16_div(le_tc_mid[i] - vtc[mi][i], middle_dx);
-			for(int i = 0; i < 3; ++i)
-				d_light_dx[i] = fixed16_div(le_l_mid[i] - vl[mi][i], middle_dx);
-			y0[0] = vy[lw];
-			y1[0] = vy[mi];
-			lsd[0] = Side{vx[lw], lower_part_dx_dy};
-			rsd[0] = Side{vx[lw], long_edge_dx_dy};
-			y0[1] = vy[mi];
-			y1[1] = vy[up];
-			lsd[1] = Side{vx[mi], upper_part_dx_dy};
-			rsd[1] = Side{long_edge_x_in_middle, long_edge_dx_dy};
-			for(int i = 0; i < 2; ++i)
-			{
-				tcs[0][i] = vtc[lw][i];
-				dtcy[0][i] = dtc_lower[i];
-				tcs[1][i] = vtc[mi][i];
-				dtcy[1][i] = dtc_upper[i];
-			}
-			for(int i = 0; i < 3; ++i)
-			{
-				ls_[i][0] = vl[lw][i];
-				dly[0][i] = dl_lower[i];
-				ls_[i][1] = vl[mi][i];
-				dly[1][i] = dl_upper[i];
-			}
-		}
-		else
-		{
-			const int32_t middle_dx = imax32(vx[mi] - long_edge_x_in_middle, FIXED16_HALF);
-			for(int i = 0; i < 2; ++i)
-				d_tc_dx[i] = fixed16_div(vtc[mi][i] - le_tc_mid[i], middle_dx);
-			for(int i = 0; i < 3; ++i)
-				d_light_dx[i] = fixed16_div(vl[mi][i] - le_l_mid[i], middle_dx);
-			y0[0] = vy[lw];
-			y1[0] = vy[mi];
-			lsd[0] = Side{vx[lw], long_edge_dx_dy};
-			rsd[0] = Side{vx[lw], lower_part_dx_dy};
-			y0[1] = vy[mi];
-			y1[1] = vy[up];
-			lsd[1] = Side{long_edge_x_in_middle, long_edge_dx_dy};
-			rsd[1] = Side{vx[mi], upper_part_dx_dy};
-			for(int i = 0; i < 2; ++i)
-			{
-				tcs[0][i] = vtc[lw][i];
-				dtcy[0][i] = le_dtc[i];
-				tcs[1][i] = le_tc_mid[i];
-				dtcy[1][i] = le_dtc[i];
-			}
-			for(int i = 0; i < 3; ++i)
-			{
-				ls_[i][0] = vl[lw][i];
-				dly[0][i] = le_dl[i];
-				ls_[i][1] = le_l_mid[i];
-				dly[1][i] = le_dl[i];
-			}
-		}
-		for(int pi = 0; pi < 2; ++pi)
-		{
-			const int32_t y_start_int = imax32(fixed16_round_to_int(y0[pi]), clip[1]);
-			const int32_t y_end_int = imin32(fixed16_round_to_int(y1[pi]), clip[3]);
-			if(y_start_int >= y_end_int)
-				continue;
-			const int32_t y_start_delta = int_to_fixed16(y_start_int) + FIXED16_HALF - y0[pi];
-			int32_t tc_start[2] = {tcs[pi][0], tcs[pi][1]};
-			int32_t dty[2] = {dtcy[pi][0], dtcy[pi][1]};
-			int32_t dtx[2] = {d_tc_dx[0], d_tc_dx[1]};
-			int32_t l_start[3] = {ls_[0][pi], ls_[1][pi], ls_[2][pi]};
-			int32_t dl[3] = {dly[pi][0], dly[pi][1], dly[pi][2]};
-			int32_t dlx[3] = {d_light_dx[0], d_light_dx[1], d_light_dx[2]};
-			TriPart tp;
-			tri_part_setup(d.texture_size, y0[pi], y1[pi], lsd[pi], rsd[pi], y_start_delta, tc_start, dty, dtx, l_start, dl, dlx, tp);
-			emit_part(d, e, 3u, clip, y0[pi], y1[pi], lsd[pi], rsd[pi], nullptr, &tp, parts, spans, counters, span_capacity,
-				part_slot_base, st);
+			t.light_start[0][i] = ll[i];
+			t.dlight_dy[0][i] = dl_lower[i];
+			t.light_start[1][i] = ml[i];
+			t.dlight_dy[1][i] = dl_upper[i];
 		}
 	}
-
-	if(lane == 0)
+	else
 	{
-		DrawRec r;
-		r.num_parts = st.num_parts;
-		r.row_min = st.row_min;
-		r.row_max = st.row_max;
-		r.x_min = st.x_min;
-		r.x_max = st.x_max;
-		drec[di] = r;
-		if(st.pixels)
+		const int32_t middle_dx = imax32(mx - long_edge_x_in_middle, FIXED16_HALF);
+#pragma unroll
+		for(int i = 0; i < 2; ++i)
+			t.d_tc_dx[i] = fixed16_div(mtc[i] - le_tc_mid[i], middle_dx);
+#pragma unroll
+		for(int i = 0; i < 3; ++i)
+			t.d_light_dx[i] = fixed16_div(ml[i] - le_l_mid[i], middle_dx);
+		t.ls[0] = Side{lx, long_edge_dx_dy};
+		t.rs[0] = Side{lx, lower_part_dx_dy};
+		t.ls[1] = Side{long_edge_x_in_middle, long_edge_dx_dy};
+		t.rs[1] = Side{mx, upper_part_dx_dy};
+#pragma unroll
+		for(int i = 0; i < 2; ++i)
 		{
-			const bool reads = d.blend == SW_BLEND_AVERAGE || d.blend == SW_BLEND_ADDITIVE || d.blend == SW_BLEND_ALPHA_TEST ||
-				d.blend == SW_BLEND_ALPHA_BLEND;
-			atomicAdd(reinterpret_cast<unsigned long long*>(
-						  &counters[d.kind == SW_DRAW_POLYGON ? COUNTER_POLY_PIXELS_LO : COUNTER_TRI_PIXELS_LO]),
-				(unsigned long long)st.pixels);
-			if(reads)
-				atomicAdd(reinterpret_cast<unsigned long long*>(&counters[COUNTER_BLEND_PIXELS_LO]), (unsigned long long)st.pixels);
+			t.tc_start[0][i] = ltc[i];
+			t.dtc_dy[0][i] = le_dtc[i];
+			t.tc_start[1][i] = le_tc_mid[i];
+			t.dtc_dy[1][i] = le_dtc[i];
 		}
-		atomicAdd(&counters[COUNTER_PARTS], st.num_parts);
+#pragma unroll
+		for(int i = 0; i < 3; ++i)
+		{
+			t.light_start[0][i] = ll[i];
+			t.dlight_dy[0][i] = le_dl[i];
+			t.light_start[1][i] = le_l_mid[i];
+			t.dlight_dy[1][i] = le_dl[i];
+		}
 	}
 }
 
-// =============================================================================================
-// Band binning: ordered list of draws per (view, band)
-// =============================================================================================
-__global__ __launch_bounds__(256) void k_bin_bands(const DrawRec* __restrict__ drec, const ViewExt* __restrict__ views,
-	uint32_t num_views, uint32_t* __restrict__ band_lists, uint32_t* __restrict__ band_counts,
-	const uint32_t* __restrict__ counters)
+SW_DEV uint32_t triangle_rows(const int32_t* __restrict__ p, const int32_t clip[4])
 {
-	__shared__ uint32_t wave_counts[4];
-	__shared__ uint32_t running;
-	const uint32_t band_global = blockIdx.x;
-	if(counters[COUNTER_ERROR] & ERROR_SPAN_OVERFLOW)
+	const int32_t y0 = p[1], y1 = p[8], y2 = p[15];
+	// lower -> middle and middle -> upper, sorted like rasterizer.rs:775-790
+	int32_t lo, mid, up;
+	if(y0 >= y1 && y0 >= y2)
+	{
+		up = y0;
+		lo = y1 < y2 ? y1 : y2;
+		mid = y1 < y2 ? y2 : y1;
+	}
+	else if(y1 >= y0 && y1 >= y2)
+	{
+		up = y1;
+		lo = y0 < y2 ? y0 : y2;
+		mid = y0 < y2 ? y2 : y0;
+	}
+	else
+	{
+		up = y2;
+		lo = y0 < y1 ? y0 : y1;
+		mid = y0 < y1 ? y1 : y0;
+	}
+	return part_rows(clip, lo, mid) + part_rows(clip, mid, up);
+}
+
+SW_DEV void emit_triangle(const sw_draw& d, const int32_t* __restrict__ p, const int32_t clip[4], PartRec* __restrict__ parts,
+	SpanRec* __restrict__ spans, uint32_t part_slot_base, SetupState& st)
+{
+	TriSplit t;
+	split_triangle(p, t);
+	const PolyEq e = PolyEq{};
+#pragma unroll
+	for(int pi = 0; pi < 2; ++pi)
+	{
+		const int32_t y_start_int = imax32(fixed16_round_to_int(t.y0[pi]), clip[1]);
+		const int32_t y_end_int = imin32(fixed16_round_to_int(t.y1[pi]), clip[3]);
+		if(y_start_int >= y_end_int)
+			continue;
+		const int32_t y_start_delta = int_to_fixed16(y_start_int) + FIXED16_HALF - t.y0[pi];
+		int32_t tc_start[2] = {t.tc_start[pi][0], t.tc_start[pi][1]};
+		int32_t dty[2] = {t.dtc_dy[pi][0], t.dtc_dy[pi][1]};
+		int32_t dtx[2] = {t.d_tc_dx[0], t.d_tc_dx[1]};
+		int32_t l_start[3] = {t.light_start[pi][0], t.light_start[pi][1], t.light_start[pi][2]};
+		int32_t dl[3] = {t.dlight_dy[pi][0], t.dlight_dy[pi][1], t.dlight_dy[pi][2]};
+		int32_t dlx[3] = {t.d_light_dx[0], t.d_light_dx[1], t.d_light_dx[2]};
+		TriPart tp;
+		tri_part_setup(d.texture_size, t.y0[pi], t.y1[pi], t.ls[pi], t.rs[pi], y_start_delta, tc_start, dty, dtx, l_start, dl,
+			dlx, tp);
+		emit_part(d, e, 3u, clip, t.y0[pi], t.y1[pi], t.ls[pi], t.rs[pi], nullptr, &tp, parts, spans, part_slot_base, st);
+	}
+}
+
+SW_DEV int32_t wave_min(int32_t v)
+{
+	for(int off = 32; off > 0; off >>= 1)
+		v = imin32(v, __shfl_xor(v, off));
+	return v;
+}
+SW_DEV int32_t wave_max(int32_t v)
+{
+	for(int off = 32; off > 0; off >>= 1)
+		v = imax32(v, __shfl_xor(v, off));
+	return v;
+}
+SW_DEV uint64_t wave_sum(uint64_t v)
+{
+	for(int off = 32; off > 0; off >>= 1)
+		v += __shfl_xor(v, off);
+	return v;
+}
+
+// LANES threads per draw (64: a wave per draw, for batches of many draws; 256: a block per draw,
+// for frames of few, large draws). Blocks past `setup_blocks` prepare the surfaces instead.
+template <int LANES>
+__global__ __launch_bounds__(256) void k_setup_draws(const sw_draw* __restrict__ draws, uint32_t num_draws,
+	const int32_t* __restrict__ vdata, uint64_t num_vertex_ints, const ViewExt* __restrict__ views, uint32_t num_views,
+	PartRec* __restrict__ parts, SpanRec* __restrict__ spans, DrawRec* __restrict__ drec, uint32_t* __restrict__ counters,
+	uint64_t span_capacity, uint64_t arena_texels, uint64_t skybox_texels, uint64_t lite_texels,
+	const sw_surface_desc* __restrict__ surfaces, uint32_t num_surfaces, SurfExt* __restrict__ surf_ext,
+	uint32_t* __restrict__ blk_surf, MapPtrs m, uint32_t setup_blocks)
+{
+	if(blockIdx.x >= setup_blocks)
+	{
+		const uint32_t s = (blockIdx.x - setup_blocks) * blockDim.x + threadIdx.x;
+		if(s < num_surfaces)
+			prep_surface(surfaces, s, num_surfaces, views, num_views, surf_ext, blk_surf, arena_texels, m, counters);
 		return;
-	uint32_t lo = 0, hi = num_views;
-	while(hi - lo > 1)
-	{
-		const uint32_t mid = (lo + hi) >> 1;
-		if(views[mid].first_band <= band_global)
-			lo = mid;
-		else
-			hi = mid;
 	}
-	const ViewExt& ve = views[lo];
-	const uint32_t band = band_global - ve.first_band;
-	const int32_t y0 = int32_t(band * BAND_H);
-	const int32_t y1 = y0 + BAND_H;
-	uint32_t* list = band_lists + ve.band_list_base + uint64_t(band) * ve.v.num_draws;
-	if(threadIdx.x == 0)
-		running = 0;
-	__syncthreads();
-	const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-	for(uint32_t base = 0; base < ve.v.num_draws; base += 256)
+	constexpr uint32_t GROUPS = 256 / LANES; // draws per block
+	__shared__ uint32_t group_rows[GROUPS];
+	__shared__ unsigned long long block_base;
+	__shared__ int32_t red_i[4][4];
+	__shared__ unsigned long long red_p[4];
+	const uint32_t group = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+	const uint32_t di = blockIdx.x * GROUPS + group;
+	const bool active = di < num_draws;
+
+	sw_draw d;
+	int32_t clip[4] = {0, 0, 0, 0};
+	bool valid = false;
+	const int32_t* v = vdata;
+	if(active)
 	{
-		const uint32_t i = base + threadIdx.x;
-		bool flag = false;
-		if(i < ve.v.num_draws)
+		d = draws[di];
+		uint32_t lo = 0, hi = num_views;
+		while(hi - lo > 1)
 		{
-			const DrawRec r = drec[ve.v.first_draw + i];
-			flag = r.num_parts > 0 && r.row_min < y1 && r.row_max > y0 && r.x_min < r.x_max;
+			const uint32_t mid = (lo + hi) >> 1;
+			if(views[mid].v.first_draw <= di)
+				lo = mid;
+			else
+				hi = mid;
 		}
-		const uint64_t ballot = __ballot(flag);
-		const uint32_t prefix = __popcll(ballot & ((1ull << lane) - 1ull));
-		if(lane == 0)
-			wave_counts[wave] = __popcll(ballot);
-		__syncthreads();
-		uint32_t off = running;
-		for(uint32_t w = 0; w < wave; ++w)
-			off += wave_counts[w];
-		if(flag)
-			list[off + prefix] = ve.v.first_draw + i;
-		__syncthreads();
-		if(threadIdx.x == 0)
-			running += wave_counts[0] + wave_counts[1] + wave_counts[2] + wave_counts[3];
-		__syncthreads();
+		const sw_view& view = views[lo].v;
+		// the draw's clip rect, kept inside the view (memory safety; the reference's rects always are)
+		clip[0] = imax32(d.clip_rect[0], 0);
+		clip[1] = imax32(d.clip_rect[1], 0);
+		clip[2] = imin32(d.clip_rect[2], int32_t(view.width));
+		clip[3] = imin32(d.clip_rect[3], int32_t(view.height));
+		// Texture and vertex ranges must lie inside their buffers (the reference indexes them
+		// unchecked in release builds, fast_math.rs:750-777; here an invalid draw is skipped).
+		const uint64_t tex_texels = uint64_t(uint32_t(d.texture_size[0])) * uint32_t(d.texture_size[1]);
+		valid = d.texture_size[0] > 0 && d.texture_size[1] > 0 && d.texture_size[0] <= 65536 && d.texture_size[1] <= 65536 &&
+			tex_texels < (1ull << 31) && d.blend <= SW_BLEND_ALPHA_BLEND;
+		if(d.kind == SW_DRAW_POLYGON)
+		{
+			const uint64_t limit = d.texture_source == SW_TEXSRC_SKYBOX ? skybox_texels : arena_texels;
+			const uint64_t tbase = d.texture_source == SW_TEXSRC_SKYBOX ? 0 : view.surface_arena_offset;
+			valid = valid && d.num_vertices >= 3 && d.num_vertices <= SW_MAX_POLYGON_VERTICES &&
+				uint64_t(d.first_vertex_int) + 2ull * d.num_vertices <= num_vertex_ints && tbase + d.texture_offset + tex_texels <= limit;
+		}
+		else
+			valid = valid && d.kind == SW_DRAW_TRIANGLE && uint64_t(d.first_vertex_int) + 21ull <= num_vertex_ints &&
+				d.texture_offset + tex_texels <= lite_texels;
+		v = vdata + d.first_vertex_int;
 	}
+
+	SetupState st;
+	st.lane = lane;
+	st.lanes = LANES;
+	st.num_parts = 0;
+	st.overflow = false;
+	st.row_min = INT32_MAX;
+	st.row_max = INT32_MIN;
+	st.x_min = INT32_MAX;
+	st.x_max = INT32_MIN;
+	st.pixels = 0;
+	const uint32_t part_slot_base = di * MAX_PARTS_PER_DRAW;
+
+	// pass 1: rows of every part (no divisions), one span allocation per block
+	uint32_t rows = 0;
+	if(active && valid)
+		rows = d.kind == SW_DRAW_POLYGON ? walk_polygon<false>(d, v, clip, parts, spans, part_slot_base, st) : triangle_rows(v, clip);
+	if(lane == 0)
+		group_rows[group] = rows;
+	__syncthreads();
 	if(threadIdx.x == 0)
-		band_counts[band_global] = running;
+	{
+		unsigned long long total = 0;
+		for(uint32_t g = 0; g < GROUPS; ++g)
+			total += group_rows[g];
+		block_base = total ? atomicAdd(reinterpret_cast<unsigned long long*>(&counters[COUNTER_SPANS_LO]), total) : 0ull;
+	}
+	__syncthreads();
+	uint64_t base = block_base;
+	for(uint32_t g = 0; g < group; ++g)
+		base += group_rows[g];
+	st.next_span = base;
+	st.overflow = base + rows > span_capacity;
+	st.span_end = st.overflow ? 0 : base + rows;
+	if(active && st.overflow && lane == 0 && rows)
+		atomicOr(&counters[COUNTER_ERROR], ERROR_SPAN_OVERFLOW);
+
+	// pass 2: parts and spans
+	if(active && valid && !st.overflow)
+	{
+		if(d.kind == SW_DRAW_POLYGON)
+			walk_polygon<true>(d, v, clip, parts, spans, part_slot_base, st);
+		else
+			emit_triangle(d, v, clip, parts, spans, part_slot_base, st);
+	}
+
+	// reduce the draw's bounds and pixel count over its lanes
+	int32_t x_min = wave_min(st.x_min), x_max = wave_max(st.x_max);
+	int32_t row_min = wave_min(st.row_min), row_max = wave_max(st.row_max);
+	uint64_t pixels = wave_sum(st.pixels);
+	if(LANES > 64)
+	{
+		const uint32_t wave = threadIdx.x >> 6;
+		if((threadIdx.x & 63) == 0)
+		{
+			red_i[wave][0] = x_min;
+			red_i[wave][1] = x_max;
+			red_i[wave][2] = row_min;
+			red_i[wave][3] = row_max;
+			red_p[wave] = pixels;
+		}
+		__syncthreads();
+		for(uint32_t w = 0; w < 4; ++w)
+		{
+			x_min = imin32(x_min, red_i[w][0]);
+			x_max = imax32(x_max, red_i[w][1]);
+			row_min = imin32(row_min, red_i[w][2]);
+			row_max = imax32(row_max, red_i[w][3]);
+		}
+		pixels = red_p[0] + red_p[1] + red_p[2] + red_p[3];
+	}
+	if(active && lane == 0)
+	{
+		const bool reads = d.blend == SW_BLEND_AVERAGE || d.blend == SW_BLEND_ADDITIVE || d.blend == SW_BLEND_ALPHA_TEST ||
+			d.blend == SW_BLEND_ALPHA_BLEND;
+		DrawRec r;
+		r.num_parts = st.overflow ? 0 : st.num_parts;
+		r.row_min = row_min;
+		r.row_max = row_max;
+		r.x_min = x_min;
+		r.x_max = x_max;
+		r.pixels = uint32_t(pixels);
+		r.stat = (d.kind == SW_DRAW_TRIANGLE ? DRAW_STAT_TRIANGLE : 0u) | (reads ? DRAW_STAT_READS_DST : 0u);
+		r.pad = 0;
+		drec[di] = r;
+	}
 }
 
-// =============================================================================================
-// Tiled raster: one wave per 64 x BAND_H tile, colour kept in LDS
-// =============================================================================================
-template <class CT>
-SW_DEV typename CT::T fetch_checked(const typename CT::T* __restrict__ tex, uint32_t u, uint32_t v, const sw_draw& d, bool& ok)
+// One block after the raster kernel: sums the per-draw statistics into the result block and clears
+// the live block for the next batch (a kernel boundary orders it after every other kernel's writes).
+__global__ __launch_bounds__(1024) void k_finish(const DrawRec* __restrict__ drec, uint32_t num_draws, uint32_t* __restrict__ counters)
 {
-	ok = u < uint32_t(d.texture_size[0]) && v < uint32_t(d.texture_size[1]);
-	return ok ? tex[u + v * uint32_t(d.texture_size[0])] : typename CT::T(0);
+	unsigned long long poly = 0, tri = 0, blend = 0, parts = 0;
+	for(uint32_t i = threadIdx.x; i < num_draws; i += blockDim.x)
+	{
+		const DrawRec r = drec[i];
+		if(r.stat & DRAW_STAT_TRIANGLE)
+			tri += r.pixels;
+		else
+			poly += r.pixels;
+		if(r.stat & DRAW_STAT_READS_DST)
+			blend += r.pixels;
+		parts += r.num_parts;
+	}
+	poly = wave_sum(poly);
+	tri = wave_sum(tri);
+	blend = wave_sum(blend);
+	parts = wave_sum(parts);
+	__shared__ unsigned long long red[16][4];
+	const uint32_t wave = threadIdx.x >> 6;
+	if((threadIdx.x & 63) == 0)
+	{
+		red[wave][0] = poly;
+		red[wave][1] = tri;
+		red[wave][2] = blend;
+		red[wave][3] = parts;
+	}
+	__syncthreads();
+	if(threadIdx.x == 0)
+	{
+		unsigned long long sum[4] = {0, 0, 0, 0};
+		for(uint32_t w = 0; w < (blockDim.x + 63) / 64; ++w)
+			for(int k = 0; k < 4; ++k)
+				sum[k] += red[w][k];
+		uint32_t* res = counters + COUNTER_RESULT;
+		for(int k = 0; k < COUNTER_COUNT; ++k)
+			res[k] = 0;
+		res[COUNTER_SPANS_LO] = counters[COUNTER_SPANS_LO];
+		res[COUNTER_SPANS_HI] = counters[COUNTER_SPANS_HI];
+		res[COUNTER_POLY_PIXELS_LO] = uint32_t(sum[0]);
+		res[COUNTER_POLY_PIXELS_HI] = uint32_t(sum[0] >> 32);
+		res[COUNTER_TRI_PIXELS_LO] = uint32_t(sum[1]);
+		res[COUNTER_TRI_PIXELS_HI] = uint32_t(sum[1] >> 32);
+		res[COUNTER_BLEND_PIXELS_LO] = uint32_t(sum[2]);
+		res[COUNTER_BLEND_PIXELS_HI] = uint32_t(sum[2] >> 32);
+		res[COUNTER_PARTS] = uint32_t(sum[3]);
+		res[COUNTER_ERROR] = counters[COUNTER_ERROR];
+		for(int k = 0; k < COUNTER_COUNT; ++k)
+			counters[k] = 0;
+	}
 }
 
-template <class CT>
+// =============================================================================================
+// Tiled raster: one wave per 64 x ROWS tile, colour kept in LDS
+// =============================================================================================
+
+// Texel address of one pixel of a polygon part: perspective (rasterizer.rs:425-452) or linear
+// tc (line-z :559-579, affine :705-727); -1 if the texel lies outside the texture (skipped, like the
+// reference's unchecked fetch never going out of range after its clamps).
+template <int KIND>
+SW_DEV int32_t polygon_texel_index(const int32_t* sv, int32_t j, uint32_t tw, uint32_t th)
+{
+	uint32_t u, v;
+	if(KIND == 0)
+	{
+		const int32_t inv_z = wadd(sv[0], wmul(j, sv[1]));
+		uint32_t div = uint32_t(inv_z) >> INV_Z_PRE_SHIFT;
+		div = div ? div : 1u;
+		const uint32_t z = (1u << Z_CALC_SHIFT) / div;
+		const int32_t t0 = wadd(sv[2], wmul(j, sv[3]));
+		const int32_t t1 = wadd(sv[4], wmul(j, sv[5]));
+		u = uint32_t((uint64_t(z) * uint64_t(int64_t(t0))) >> TC_FINAL_SHIFT);
+		v = uint32_t((uint64_t(z) * uint64_t(int64_t(t1))) >> TC_FINAL_SHIFT);
+	}
+	else
+	{
+		u = uint32_t(wadd(sv[0], wmul(j, sv[1])) >> 16);
+		v = uint32_t(wadd(sv[2], wmul(j, sv[3])) >> 16);
+	}
+	return (u < tw && v < th) ? int32_t(u + v * tw) : -1;
+}
+
+// One polygon part over the tile's rows. The texel addresses of all rows are computed first and
+// all fetches issued together, then blended into the LDS tile in row order.
+template <class CT, int ROWS, int KIND>
+SW_DEV void raster_polygon_part(const SpanRec& mine, int32_t ra, int32_t rb, int32_t y0, int32_t x, uint32_t blend,
+	uint32_t tw, uint32_t th, const typename CT::T* __restrict__ tex, typename CT::T (*tile)[64])
+{
+	typedef typename CT::T C;
+	int32_t tix[ROWS];
+#pragma unroll
+	for(int r = 0; r < ROWS; ++r)
+	{
+		tix[r] = -1;
+		const int32_t y = y0 + r;
+		if(y < ra || y >= rb)
+			continue;
+		const int sl = y - ra;
+		const int32_t sx0 = __builtin_amdgcn_readlane(mine.x0, sl);
+		const int32_t sx1 = __builtin_amdgcn_readlane(mine.x1, sl);
+		int32_t sv[6];
+#pragma unroll
+		for(int k = 0; k < (KIND == 0 ? 6 : 4); ++k)
+			sv[k] = __builtin_amdgcn_readlane(mine.v[k], sl);
+		if(x >= sx0 && x < sx1)
+			tix[r] = polygon_texel_index<KIND>(sv, x - sx0, tw, th);
+	}
+	C texel[ROWS];
+#pragma unroll
+	for(int r = 0; r < ROWS; ++r)
+		texel[r] = tix[r] >= 0 ? tex[tix[r]] : C(0);
+#pragma unroll
+	for(int r = 0; r < ROWS; ++r)
+		if(tix[r] >= 0)
+			tile[r][threadIdx.x & 63] = write_into_framebuffer<CT>(blend, tile[r][threadIdx.x & 63], texel[r]);
+}
+
+// One triangle part (rasterizer.rs:1207-1290): fixed16 tc and light, (texel * light) >> 16, blend.
+template <class CT, int ROWS>
+SW_DEV void raster_triangle_part(const SpanRec& mine, const PartRec& pr, int32_t ra, int32_t rb, int32_t y0, int32_t x,
+	uint32_t blend, uint32_t tw, uint32_t th, const uint32_t* __restrict__ tex, typename CT::T (*tile)[64])
+{
+	int32_t tix[ROWS];
+#pragma unroll
+	for(int r = 0; r < ROWS; ++r)
+	{
+		tix[r] = -1;
+		const int32_t y = y0 + r;
+		if(y < ra || y >= rb)
+			continue;
+		const int sl = y - ra;
+		const int32_t sx0 = __builtin_amdgcn_readlane(mine.x0, sl);
+		const int32_t sx1 = __builtin_amdgcn_readlane(mine.x1, sl);
+		const int32_t s0 = __builtin_amdgcn_readlane(mine.v[0], sl);
+		const int32_t s1 = __builtin_amdgcn_readlane(mine.v[1], sl);
+		if(x >= sx0 && x < sx1)
+		{
+			const int32_t j = x - sx0;
+			const uint32_t u = uint32_t(fixed16_floor_to_int(wadd(s0, wmul(j, pr.d[0]))));
+			const uint32_t v = uint32_t(fixed16_floor_to_int(wadd(s1, wmul(j, pr.d[1]))));
+			if(u < tw && v < th)
+				tix[r] = int32_t(u + v * tw);
+		}
+	}
+	uint32_t texel[ROWS];
+#pragma unroll
+	for(int r = 0; r < ROWS; ++r)
+		texel[r] = tix[r] >= 0 ? tex[tix[r]] : 0u;
+#pragma unroll
+	for(int r = 0; r < ROWS; ++r)
+	{
+		if(tix[r] < 0)
+			continue;
+		const int sl = y0 + r - ra;
+		const int32_t j = x - __builtin_amdgcn_readlane(mine.x0, sl);
+		const int32_t lr = wadd(__builtin_amdgcn_readlane(mine.v[2], sl), wmul(j, pr.d[2]));
+		const int32_t lg = wadd(__builtin_amdgcn_readlane(mine.v[3], sl), wmul(j, pr.d[3]));
+		const int32_t lb = wadd(__builtin_amdgcn_readlane(mine.v[4], sl), wmul(j, pr.d[4]));
+		const CI tc = Color32T::to_ci(texel[r]);
+		const CI lit = CI{wmul(tc.b, lb) >> 16, wmul(tc.g, lg) >> 16, wmul(tc.r, lr) >> 16, 0};
+		typename CT::T& dst = tile[r][threadIdx.x & 63];
+		switch(blend)
+		{
+		case SW_BLEND_NONE: dst = CT::from_ci(lit); break;
+		case SW_BLEND_AVERAGE: dst = CT::average(dst, CT::from_ci(lit)); break;
+		case SW_BLEND_ADDITIVE: dst = CT::saturated_sum(dst, CT::from_ci(lit)); break;
+		case SW_BLEND_ALPHA_TEST:
+			if(Color32T::test_alpha(texel[r]))
+				dst = CT::from_ci(lit);
+			break;
+		case SW_BLEND_ALPHA_BLEND: dst = alpha_blend_ci<CT>(dst, lit, Color32T::get_alpha(texel[r])); break;
+		default: break;
+		}
+	}
+}
+
+template <class CT, int ROWS>
 __global__ __launch_bounds__(256) void k_raster_tiles(MapPtrs m, const sw_draw* __restrict__ draws,
 	const DrawRec* __restrict__ drec, const PartRec* __restrict__ parts, const SpanRec* __restrict__ spans,
-	const ViewExt* __restrict__ views, uint32_t num_views, const uint32_t* __restrict__ band_lists,
-	const uint32_t* __restrict__ band_counts, const typename CT::T* __restrict__ arena,
+	const ViewExt* __restrict__ views, uint32_t num_views, const typename CT::T* __restrict__ arena,
 	const typename CT::T* __restrict__ skybox, typename CT::T* __restrict__ fb, uint32_t total_tiles,
 	const uint32_t* __restrict__ counters)
 {
 	typedef typename CT::T C;
-	__shared__ C tile[4][BAND_H][64];
+	__shared__ C tiles[4][ROWS][64];
 	const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	C(*tile)[64] = tiles[wave];
 	const uint32_t t = blockIdx.x * 4 + wave;
 	if(t >= total_tiles || (counters[COUNTER_ERROR] & ERROR_SPAN_OVERFLOW))
 		return;
@@ -1225,101 +1483,62 @@ __global__ __launch_bounds__(256) void k_raster_tiles(MapPtrs m, const sw_draw* 
 	const uint32_t band = local / ve.cols;
 	const uint32_t col = local - band * ve.cols;
 	const int32_t x0 = int32_t(col * 64);
-	const int32_t y0 = int32_t(band * BAND_H);
+	const int32_t y0 = int32_t(band * ROWS);
 	const int32_t W = int32_t(ve.v.width), H = int32_t(ve.v.height);
 	const int32_t x = x0 + int32_t(lane);
 	C* vfb = fb + ve.v.fb_offset;
 	const C bg = C(ve.v.background_color);
 
 #pragma unroll
-	for(int r = 0; r < BAND_H; ++r)
+	for(int r = 0; r < ROWS; ++r)
 	{
 		const int32_t y = y0 + r;
 		C c = bg;
 		if(!ve.v.clear_background && x < W && y < H)
 			c = vfb[uint64_t(y) * uint32_t(W) + uint32_t(x)];
-		tile[wave][r][lane] = c;
+		tile[r][lane] = c;
 	}
 
-	const uint32_t band_global = ve.first_band + band;
-	const uint32_t n = band_counts[band_global];
-	const uint32_t* list = band_lists + ve.band_list_base + uint64_t(band) * ve.v.num_draws;
 	const C* view_arena = arena + ve.v.surface_arena_offset;
-	for(uint32_t e = 0; e < n; ++e)
+	const uint32_t first_draw = ve.v.first_draw, num_draws = ve.v.num_draws;
+	for(uint32_t base = 0; base < num_draws; base += 64)
 	{
-		const uint32_t di = list[e];
-		const DrawRec dr = drec[di];
-		if(dr.x_max <= x0 || dr.x_min >= x0 + 64)
-			continue;
-		const sw_draw& d = draws[di];
-		const uint32_t blend = d.blend;
-		const C* tex_c = d.texture_source == SW_TEXSRC_SKYBOX ? skybox + d.texture_offset : view_arena + d.texture_offset;
-		const uint32_t* tex_lite = m.lite + d.texture_offset;
-		for(uint32_t p = 0; p < dr.num_parts; ++p)
+		// ordered binning: lane i tests draw base + i against this tile
+		bool hit = false;
+		if(base + lane < num_draws)
 		{
-			const PartRec pr = parts[di * MAX_PARTS_PER_DRAW + p];
-			const int32_t ra = imax32(pr.row0, y0), rb = imin32(pr.row1, y0 + BAND_H);
-			for(int32_t y = ra; y < rb; ++y)
+			const DrawRec r = drec[first_draw + base + lane];
+			hit = r.num_parts > 0 && r.row_min < y0 + ROWS && r.row_max > y0 && r.x_min < x0 + 64 && r.x_max > x0;
+		}
+		uint64_t mask = __ballot(hit);
+		while(mask)
+		{
+			const uint32_t j = uint32_t(__builtin_ctzll(mask));
+			mask &= mask - 1;
+			const uint32_t di = first_draw + base + j;
+			const uint32_t num_parts = drec[di].num_parts;
+			const sw_draw& d = draws[di];
+			const uint32_t blend = d.blend;
+			const uint32_t tw = uint32_t(d.texture_size[0]), th = uint32_t(d.texture_size[1]);
+			for(uint32_t p = 0; p < num_parts; ++p)
 			{
-				const SpanRec s = spans[pr.span_base + uint32_t(y - pr.row0)];
-				if(s.x1 <= x0 || s.x0 >= x0 + 64 || s.x0 >= s.x1)
+				const PartRec pr = parts[di * MAX_PARTS_PER_DRAW + p];
+				const int32_t ra = imax32(pr.row0, y0), rb = imin32(pr.row1, y0 + ROWS);
+				if(ra >= rb)
 					continue;
-				if(x < s.x0 || x >= s.x1)
-					continue;
-				const int32_t j = x - s.x0;
-				C& dst = tile[wave][y - y0][lane];
-				if(pr.kind == 0)
-				{
-					const int32_t inv_z = wadd(s.v[0], wmul(j, s.v[1]));
-					uint32_t div = uint32_t(inv_z) >> INV_Z_PRE_SHIFT;
-					div = div ? div : 1u;
-					const uint32_t z = (1u << Z_CALC_SHIFT) / div;
-					const int32_t t0 = wadd(s.v[2], wmul(j, s.v[3]));
-					const int32_t t1 = wadd(s.v[4], wmul(j, s.v[5]));
-					const uint32_t u = uint32_t((uint64_t(z) * uint64_t(int64_t(t0))) >> TC_FINAL_SHIFT);
-					const uint32_t v = uint32_t((uint64_t(z) * uint64_t(int64_t(t1))) >> TC_FINAL_SHIFT);
-					bool ok;
-					const C texel = fetch_checked<CT>(tex_c, u, v, d, ok);
-					if(ok)
-						dst = write_into_framebuffer<CT>(blend, dst, texel);
-				}
-				else if(pr.kind == 1 || pr.kind == 2)
-				{
-					const int32_t t0 = wadd(s.v[0], wmul(j, s.v[1]));
-					const int32_t t1 = wadd(s.v[2], wmul(j, s.v[3]));
-					const uint32_t u = uint32_t(pr.kind == 1 ? (t0 >> 16) : fixed16_floor_to_int(t0));
-					const uint32_t v = uint32_t(pr.kind == 1 ? (t1 >> 16) : fixed16_floor_to_int(t1));
-					bool ok;
-					const C texel = fetch_checked<CT>(tex_c, u, v, d, ok);
-					if(ok)
-						dst = write_into_framebuffer<CT>(blend, dst, texel);
-				}
+				// the part's span records for the tile's rows, one per lane, broadcast with readlane
+				SpanRec mine = SpanRec{0, 0, {0, 0, 0, 0, 0, 0}};
+				if(int32_t(lane) < rb - ra)
+					mine = spans[pr.span_base + uint32_t(ra - pr.row0) + lane];
+				if(pr.kind == 3)
+					raster_triangle_part<CT, ROWS>(mine, pr, ra, rb, y0, x, blend, tw, th, m.lite + d.texture_offset, tile);
 				else
 				{
-					const int32_t t0 = wadd(s.v[0], wmul(j, pr.d[0]));
-					const int32_t t1 = wadd(s.v[1], wmul(j, pr.d[1]));
-					const int32_t u = fixed16_floor_to_int(t0), v = fixed16_floor_to_int(t1);
-					if(uint32_t(u) < uint32_t(d.texture_size[0]) && uint32_t(v) < uint32_t(d.texture_size[1]))
-					{
-						const uint32_t texel = tex_lite[uint32_t(u) + uint32_t(v) * uint32_t(d.texture_size[0])];
-						const int32_t lr = wadd(s.v[2], wmul(j, pr.d[2]));
-						const int32_t lg = wadd(s.v[3], wmul(j, pr.d[3]));
-						const int32_t lb = wadd(s.v[4], wmul(j, pr.d[4]));
-						const CI tc = Color32T::to_ci(texel);
-						const CI lit = CI{wmul(tc.b, lb) >> 16, wmul(tc.g, lg) >> 16, wmul(tc.r, lr) >> 16, 0};
-						switch(blend)
-						{
-						case SW_BLEND_NONE: dst = CT::from_ci(lit); break;
-						case SW_BLEND_AVERAGE: dst = CT::average(dst, CT::from_ci(lit)); break;
-						case SW_BLEND_ADDITIVE: dst = CT::saturated_sum(dst, CT::from_ci(lit)); break;
-						case SW_BLEND_ALPHA_TEST:
-							if(Color32T::test_alpha(texel))
-								dst = CT::from_ci(lit);
-							break;
-						case SW_BLEND_ALPHA_BLEND: dst = alpha_blend_ci<CT>(dst, lit, Color32T::get_alpha(texel)); break;
-						default: break;
-						}
-					}
+					const C* tex = d.texture_source == SW_TEXSRC_SKYBOX ? skybox + d.texture_offset : view_arena + d.texture_offset;
+					if(pr.kind == 0)
+						raster_polygon_part<CT, ROWS, 0>(mine, ra, rb, y0, x, blend, tw, th, tex, tile);
+					else
+						raster_polygon_part<CT, ROWS, 1>(mine, ra, rb, y0, x, blend, tw, th, tex, tile);
 				}
 			}
 		}
@@ -1328,11 +1547,11 @@ __global__ __launch_bounds__(256) void k_raster_tiles(MapPtrs m, const sw_draw* 
 	if(x < W)
 	{
 #pragma unroll
-		for(int r = 0; r < BAND_H; ++r)
+		for(int r = 0; r < ROWS; ++r)
 		{
 			const int32_t y = y0 + r;
 			if(y < H)
-				vfb[uint64_t(y) * uint32_t(W) + uint32_t(x)] = tile[wave][r][lane];
+				vfb[uint64_t(y) * uint32_t(W) + uint32_t(x)] = tile[r][lane];
 		}
 	}
 }
@@ -1402,63 +1621,78 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t value)
 namespace swk
 {
 
-void launch_prep_surfaces(hipStream_t st, const sw_surface_desc* s, uint32_t n, const ViewExt* views, uint32_t nv,
-	uint64_t* surf_end, uint64_t arena_texels, const MapPtrs& m, uint32_t* counters)
+void launch_setup(hipStream_t st, uint32_t lanes, const sw_draw* draws, uint32_t nd, const int32_t* vdata, uint64_t nvi,
+	const ViewExt* views, uint32_t nv, PartRec* parts, SpanRec* spans, DrawRec* drec, uint32_t* counters,
+	uint64_t span_capacity, uint64_t arena_texels, uint64_t skybox_texels, uint64_t lite_texels,
+	const sw_surface_desc* surfaces, uint32_t ns, SurfExt* surf_ext, uint32_t* blk_surf, const MapPtrs& m)
 {
-	const uint32_t blocks = n ? (n + 255) / 256 : 1;
-	hipLaunchKernelGGL(k_prep_surfaces, dim3(blocks), dim3(256), 0, st, s, n, views, nv, surf_end, arena_texels, m.num_levels,
-		m.levels, m.num_lm_simple, m.num_lm_dir, counters);
+	const uint32_t groups = lanes == 256 ? 1u : 4u;
+	const uint32_t setup_blocks = (nd + groups - 1) / groups;
+	const uint32_t prep_blocks = (ns + 255) / 256;
+	const uint32_t blocks = setup_blocks + prep_blocks;
+	if(blocks == 0)
+		return;
+	if(lanes == 256)
+		hipLaunchKernelGGL(k_setup_draws<256>, dim3(blocks), dim3(256), 0, st, draws, nd, vdata, nvi, views, nv, parts, spans,
+			drec, counters, span_capacity, arena_texels, skybox_texels, lite_texels, surfaces, ns, surf_ext, blk_surf, m,
+			setup_blocks);
+	else
+		hipLaunchKernelGGL(k_setup_draws<64>, dim3(blocks), dim3(256), 0, st, draws, nd, vdata, nvi, views, nv, parts, spans,
+			drec, counters, span_capacity, arena_texels, skybox_texels, lite_texels, surfaces, ns, surf_ext, blk_surf, m,
+			setup_blocks);
 }
 
 void launch_build_surfaces(hipStream_t st, uint32_t color_bits, const MapPtrs& m, const sw_surface_desc* s, uint32_t n,
-	const uint64_t* surf_end, const ViewExt* views, uint32_t nv, const sw_light* lights, uint32_t nl,
+	const SurfExt* surf_ext, const uint32_t* blk_surf, const ViewExt* views, const sw_light* lights, uint32_t nl,
 	const uint32_t* lidx, uint32_t nli, void* arena, uint64_t arena_texels, const uint32_t* counters)
 {
 	if(arena_texels == 0 || n == 0)
 		return;
-	const uint64_t blocks = (arena_texels + 255) / 256;
+	const uint64_t blocks = (arena_texels + SURF_BLOCK - 1) / SURF_BLOCK;
 	if(color_bits == 64)
-		hipLaunchKernelGGL(k_build_surfaces<Color64T>, dim3(uint32_t(blocks)), dim3(256), 0, st, m, s, n, surf_end, views, nv,
-			lights, nl, lidx, nli, static_cast<uint64_t*>(arena), arena_texels, counters);
+		hipLaunchKernelGGL(k_build_surfaces<Color64T>, dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, s, n, surf_ext,
+			blk_surf, views, lights, nl, lidx, nli, static_cast<uint64_t*>(arena), arena_texels, counters);
 	else
-		hipLaunchKernelGGL(k_build_surfaces<Color32T>, dim3(uint32_t(blocks)), dim3(256), 0, st, m, s, n, surf_end, views, nv,
-			lights, nl, lidx, nli, static_cast<uint32_t*>(arena), arena_texels, counters);
+		hipLaunchKernelGGL(k_build_surfaces<Color32T>, dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, s, n, surf_ext,
+			blk_surf, views, lights, nl, lidx, nli, static_cast<uint32_t*>(arena), arena_texels, counters);
 }
 
-void launch_setup_draws(hipStream_t st, const sw_draw* draws, uint32_t nd, const int32_t* vdata, uint64_t nvi,
-	const ViewExt* views, uint32_t nv, PartRec* parts, SpanRec* spans, DrawRec* drec, uint32_t* counters, uint64_t span_capacity,
-	uint64_t arena_texels, uint64_t skybox_texels, uint64_t lite_texels)
+template <class CT>
+static void launch_raster_ct(hipStream_t st, uint32_t tile_rows, const MapPtrs& m, const sw_draw* draws, const DrawRec* drec,
+	const PartRec* parts, const SpanRec* spans, const ViewExt* views, uint32_t nv, const void* arena, const void* skybox,
+	void* fb, uint32_t total_tiles, const uint32_t* counters)
 {
-	if(nd == 0)
-		return;
-	hipLaunchKernelGGL(k_setup_draws, dim3((nd + 3) / 4), dim3(256), 0, st, draws, nd, vdata, nvi, views, nv, parts, spans, drec,
-		counters, span_capacity, arena_texels, skybox_texels, lite_texels);
-}
-
-void launch_bin_bands(hipStream_t st, const DrawRec* drec, const ViewExt* views, uint32_t nv, uint32_t total_bands,
-	uint32_t* band_lists, uint32_t* band_counts, const uint32_t* counters)
-{
-	if(total_bands == 0)
-		return;
-	hipLaunchKernelGGL(k_bin_bands, dim3(total_bands), dim3(256), 0, st, drec, views, nv, band_lists, band_counts, counters);
-}
-
-void launch_raster_tiles(hipStream_t st, uint32_t color_bits, const MapPtrs& m, const sw_draw* draws, const DrawRec* drec,
-	const PartRec* parts, const SpanRec* spans, const ViewExt* views, uint32_t nv, const uint32_t* band_lists,
-	const uint32_t* band_counts, const void* arena, const void* skybox, void* fb, uint32_t total_tiles,
-	const uint32_t* counters)
-{
-	if(total_tiles == 0)
-		return;
+	typedef typename CT::T C;
 	const uint32_t blocks = (total_tiles + 3) / 4;
-	if(color_bits == 64)
-		hipLaunchKernelGGL(k_raster_tiles<Color64T>, dim3(blocks), dim3(256), 0, st, m, draws, drec, parts, spans, views, nv,
-			band_lists, band_counts, static_cast<const uint64_t*>(arena), static_cast<const uint64_t*>(skybox),
-			static_cast<uint64_t*>(fb), total_tiles, counters);
+	if(blocks == 0)
+		return;
+	const C* a = static_cast<const C*>(arena);
+	const C* sky = static_cast<const C*>(skybox);
+	C* out = static_cast<C*>(fb);
+	if(tile_rows == 4)
+		hipLaunchKernelGGL((k_raster_tiles<CT, 4>), dim3(blocks), dim3(256), 0, st, m, draws, drec, parts, spans, views, nv, a,
+			sky, out, total_tiles, counters);
+	else if(tile_rows == 8)
+		hipLaunchKernelGGL((k_raster_tiles<CT, 8>), dim3(blocks), dim3(256), 0, st, m, draws, drec, parts, spans, views, nv, a,
+			sky, out, total_tiles, counters);
 	else
-		hipLaunchKernelGGL(k_raster_tiles<Color32T>, dim3(blocks), dim3(256), 0, st, m, draws, drec, parts, spans, views, nv,
-			band_lists, band_counts, static_cast<const uint32_t*>(arena), static_cast<const uint32_t*>(skybox),
-			static_cast<uint32_t*>(fb), total_tiles, counters);
+		hipLaunchKernelGGL((k_raster_tiles<CT, 16>), dim3(blocks), dim3(256), 0, st, m, draws, drec, parts, spans, views, nv, a,
+			sky, out, total_tiles, counters);
+}
+
+void launch_raster_tiles(hipStream_t st, uint32_t color_bits, uint32_t tile_rows, const MapPtrs& m, const sw_draw* draws,
+	const DrawRec* drec, const PartRec* parts, const SpanRec* spans, const ViewExt* views, uint32_t nv, const void* arena,
+	const void* skybox, void* fb, uint32_t total_tiles, const uint32_t* counters)
+{
+	if(color_bits == 64)
+		launch_raster_ct<Color64T>(st, tile_rows, m, draws, drec, parts, spans, views, nv, arena, skybox, fb, total_tiles, counters);
+	else
+		launch_raster_ct<Color32T>(st, tile_rows, m, draws, drec, parts, spans, views, nv, arena, skybox, fb, total_tiles, counters);
+}
+
+void launch_finish(hipStream_t st, const DrawRec* drec, uint32_t nd, uint32_t* counters)
+{
+	hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, st, drec, nd, counters);
 }
 
 void launch_tonemap(hipStream_t st, const uint64_t* hdr, uint32_t w, uint32_t h, float isr, float isg, float isb,

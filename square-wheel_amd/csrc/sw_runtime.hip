@@ -2,8 +2,9 @@
 //
 // Device memory model: the map (textures with their mip chains, lite textures, both lightmap
 // kinds, skybox sides) is uploaded once and stays in HBM. Per batch the context owns grow-only
-// device buffers for the command buffers, the surface arena, span/part records and band lists,
-// so steady-state frames allocate nothing.
+// device buffers for the command buffers, the surface arena and the span/part records, so
+// steady-state frames allocate nothing. A batch is three launches on one stream: draw setup
+// (plus surface extents), surface building, tiled raster.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -18,14 +19,12 @@
 namespace
 {
 
-__global__ void k_reset_raster_counters(uint32_t* counters)
+// Clears the live counter block after a launch sequence without a raster kernel (which otherwise
+// hands the counters over and clears them itself).
+__global__ void k_reset_counters(uint32_t* counters)
 {
-	if(threadIdx.x == 0)
-	{
-		for(int i = COUNTER_SPANS_LO; i <= COUNTER_PARTS; ++i)
-			counters[i] = 0;
-		counters[COUNTER_ERROR] &= ~uint32_t(ERROR_SPAN_OVERFLOW);
-	}
+	if(threadIdx.x < COUNTER_COUNT)
+		counters[threadIdx.x] = 0;
 }
 
 __global__ void k_fill_color(void* fb, uint64_t n, uint64_t value, uint32_t bytes)
@@ -65,8 +64,11 @@ struct sw_ctx
 	uint32_t last_color_bits = 32; // colour size of the arena's current contents
 
 	// command buffers / work buffers
-	DevBuf views, surfaces, lights, light_indices, draws, vdata, surf_end, parts, spans, drec, band_lists, band_counts,
-		counters, arena, fb, tonemap_out, tonemap_sums;
+	DevBuf views, surfaces, lights, light_indices, draws, vdata, surf_ext, blk_surf, parts, spans, drec, counters, arena, fb,
+		tonemap_out, tonemap_sums;
+	uint32_t tile_rows = 8; // raster tile height (SW_TILE_ROWS_*), set by sw_set_tile_rows / the autotuner
+	uint32_t* counters_host = nullptr; // pinned copy of the result block
+	bool surface_error = false; // an invalid surface seen by a surface-only launch sequence
 	uint64_t span_capacity = 1u << 20;
 
 	// plain-path frame state
@@ -76,6 +78,7 @@ struct sw_ctx
 	uint64_t frame_arena_texels = 0;
 	bool frame_active = false;
 	bool frame_rastered = false;
+	bool frame_surfaces_pending = false; // surfaces are built by the launch sequence of sw_raster_draws
 	uint32_t frame_num_surfaces = 0, frame_num_lights = 0, frame_num_light_indices = 0;
 	uint64_t frame_num_vertex_ints = 0;
 
@@ -98,8 +101,7 @@ struct sw_ctx
 		uint64_t arena_texels = 0;
 		void* arena = nullptr;
 		void* fb = nullptr;
-		uint32_t total_bands = 0, total_tiles = 0;
-		uint64_t band_list_entries = 0;
+		uint32_t total_tiles = 0;
 	} last;
 
 	// graph replay
@@ -200,12 +202,10 @@ uint64_t color_bytes(uint32_t bits) { return bits == 64 ? 8 : 4; }
 
 enum
 {
-	STAGE_PREP = 0,
+	STAGE_SETUP = 0,
 	STAGE_SURF = 1,
-	STAGE_SETUP = 2,
-	STAGE_BIN = 3,
-	STAGE_RASTER = 4,
-	STAGE_TONEMAP = 5,
+	STAGE_RASTER = 2,
+	STAGE_TONEMAP = 3,
 };
 
 hipEvent_t take_event(sw_ctx* ctx)
@@ -275,11 +275,11 @@ void collect_timings(sw_ctx* ctx)
 }
 
 // Per-view tiling of a batch (views are always host memory).
-int build_view_ext(sw_ctx* ctx, const sw_view* views, uint32_t n, std::vector<ViewExt>& out, uint32_t& total_bands,
-	uint32_t& total_tiles, uint64_t& band_list_entries)
+int build_view_ext(sw_ctx* ctx, const sw_view* views, uint32_t n, std::vector<ViewExt>& out, uint32_t& total_tiles)
 {
+	const uint32_t tile_rows = ctx->tile_rows;
 	out.resize(n);
-	uint64_t bands = 0, tiles = 0, entries = 0;
+	uint64_t tiles = 0;
 	uint32_t next_surface = 0, next_draw = 0;
 	for(uint32_t i = 0; i < n; ++i)
 	{
@@ -293,20 +293,15 @@ int build_view_ext(sw_ctx* ctx, const sw_view* views, uint32_t n, std::vector<Vi
 		next_draw += v.num_draws;
 		ViewExt& e = out[i];
 		e.v = v;
-		e.first_band = uint32_t(bands);
-		e.num_bands = (v.height + BAND_H - 1) / BAND_H;
+		e.num_bands = (v.height + tile_rows - 1) / tile_rows;
 		e.cols = (v.width + 63) / 64;
 		e.first_tile = uint32_t(tiles);
-		e.band_list_base = entries;
-		bands += e.num_bands;
+		e.pad = 0;
 		tiles += uint64_t(e.num_bands) * e.cols;
-		entries += uint64_t(e.num_bands) * v.num_draws;
+		if(tiles >= (1ull << 31))
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "batch too large");
 	}
-	if(bands >= (1ull << 31) || tiles >= (1ull << 31))
-		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "batch too large");
-	total_bands = uint32_t(bands);
 	total_tiles = uint32_t(tiles);
-	band_list_entries = entries;
 	return SW_OK;
 }
 
@@ -326,63 +321,74 @@ int validate_views_fit(sw_ctx* ctx, const std::vector<ViewExt>& ve, uint64_t fb_
 	return SW_OK;
 }
 
-// Raster stages of a batch: reset counters, draw setup, band binning, tiles.
-int launch_raster_stages(sw_ctx* ctx, const sw_ctx::Launch& L, const ViewExt* views_dev)
+// Launch sequence of a batch: draw setup (with the surface extents when `surfaces`), surface
+// building (when `surfaces`), tiled raster (when `raster`). The raster kernel's last block hands
+// the counters over to the result block; a sequence without it must be followed by
+// flush_surface_errors.
+int launch_stages(sw_ctx* ctx, const sw_ctx::Launch& L, const ViewExt* views_dev, bool surfaces, bool raster)
 {
 	sw_map* map = L.map;
+	const uint32_t nd = raster ? L.num_draws : 0;
+	const uint32_t ns = surfaces ? L.num_surfaces : 0;
+	const uint64_t surf_blocks = (L.arena_texels + SURF_BLOCK - 1) / SURF_BLOCK;
 	int r;
-	if((r = ensure(ctx, ctx->parts, sizeof(PartRec) * uint64_t(std::max<uint32_t>(L.num_draws, 1)) * MAX_PARTS_PER_DRAW)))
+	if((r = ensure(ctx, ctx->parts, sizeof(PartRec) * uint64_t(std::max<uint32_t>(nd, 1)) * MAX_PARTS_PER_DRAW)))
 		return r;
 	if((r = ensure(ctx, ctx->spans, sizeof(SpanRec) * ctx->span_capacity)))
 		return r;
-	if((r = ensure(ctx, ctx->drec, sizeof(DrawRec) * std::max<uint32_t>(L.num_draws, 1))))
+	if((r = ensure(ctx, ctx->drec, sizeof(DrawRec) * std::max<uint32_t>(nd, 1))))
 		return r;
-	if((r = ensure(ctx, ctx->band_lists, sizeof(uint32_t) * std::max<uint64_t>(L.band_list_entries, 1))))
+	if((r = ensure(ctx, ctx->surf_ext, sizeof(SurfExt) * std::max<uint32_t>(ns, 1))))
 		return r;
-	if((r = ensure(ctx, ctx->band_counts, sizeof(uint32_t) * std::max<uint32_t>(L.total_bands, 1))))
+	if((r = ensure(ctx, ctx->blk_surf, sizeof(uint32_t) * std::max<uint64_t>(surf_blocks, 1))))
 		return r;
 	uint32_t* counters = static_cast<uint32_t*>(ctx->counters.p);
-	hipLaunchKernelGGL(k_reset_raster_counters, dim3(1), dim3(64), 0, ctx->stream, counters);
 	const uint64_t sky_texels = L.color_bits == 64 ? map->num_sky64 : map->num_sky32;
+	// a block per draw while the draws alone cannot fill the chip, a wave per draw beyond
+	const uint32_t lanes = nd <= 4096 ? 256u : 64u;
 	{
 		StageTimer t(ctx, STAGE_SETUP);
-		swk::launch_setup_draws(ctx->stream, L.draws, L.num_draws, L.vdata, L.num_vertex_ints, views_dev, L.num_views,
+		swk::launch_setup(ctx->stream, lanes, L.draws, nd, L.vdata, L.num_vertex_ints, views_dev, L.num_views,
 			static_cast<PartRec*>(ctx->parts.p), static_cast<SpanRec*>(ctx->spans.p), static_cast<DrawRec*>(ctx->drec.p),
-			counters, ctx->span_capacity, L.arena_texels, sky_texels, map->ptrs.num_lite);
+			counters, ctx->span_capacity, L.arena_texels, sky_texels, map->ptrs.num_lite, L.surfaces, ns,
+			static_cast<SurfExt*>(ctx->surf_ext.p), static_cast<uint32_t*>(ctx->blk_surf.p), map->ptrs);
 	}
-	{
-		StageTimer t(ctx, STAGE_BIN);
-		swk::launch_bin_bands(ctx->stream, static_cast<DrawRec*>(ctx->drec.p), views_dev, L.num_views, L.total_bands,
-			static_cast<uint32_t*>(ctx->band_lists.p), static_cast<uint32_t*>(ctx->band_counts.p), counters);
-	}
-	{
-		StageTimer t(ctx, STAGE_RASTER);
-		swk::launch_raster_tiles(ctx->stream, L.color_bits, map->ptrs, L.draws, static_cast<DrawRec*>(ctx->drec.p),
-			static_cast<PartRec*>(ctx->parts.p), static_cast<SpanRec*>(ctx->spans.p), views_dev, L.num_views,
-			static_cast<uint32_t*>(ctx->band_lists.p), static_cast<uint32_t*>(ctx->band_counts.p), L.arena,
-			L.color_bits == 64 ? map->sky64.p : map->sky32.p, L.fb, L.total_tiles, counters);
-	}
-	return hip_check(ctx, hipGetLastError(), "raster launch");
-}
-
-int launch_surface_stages(sw_ctx* ctx, const sw_ctx::Launch& L, const ViewExt* views_dev)
-{
-	int r;
-	if((r = ensure(ctx, ctx->surf_end, sizeof(uint64_t) * std::max<uint32_t>(L.num_surfaces, 1))))
-		return r;
-	uint32_t* counters = static_cast<uint32_t*>(ctx->counters.p);
-	{
-		StageTimer t(ctx, STAGE_PREP);
-		swk::launch_prep_surfaces(ctx->stream, L.surfaces, L.num_surfaces, views_dev, L.num_views,
-			static_cast<uint64_t*>(ctx->surf_end.p), L.arena_texels, L.map->ptrs, counters);
-	}
+	if(surfaces)
 	{
 		StageTimer t(ctx, STAGE_SURF);
-		swk::launch_build_surfaces(ctx->stream, L.color_bits, L.map->ptrs, L.surfaces, L.num_surfaces,
-			static_cast<uint64_t*>(ctx->surf_end.p), views_dev, L.num_views, L.lights, L.num_lights, L.light_indices,
-			L.num_light_indices, L.arena, L.arena_texels, counters);
+		swk::launch_build_surfaces(ctx->stream, L.color_bits, map->ptrs, L.surfaces, L.num_surfaces,
+			static_cast<const SurfExt*>(ctx->surf_ext.p), static_cast<const uint32_t*>(ctx->blk_surf.p), views_dev, L.lights,
+			L.num_lights, L.light_indices, L.num_light_indices, L.arena, L.arena_texels, counters);
 	}
-	return hip_check(ctx, hipGetLastError(), "surface launch");
+	if(raster)
+	{
+		{
+			StageTimer t(ctx, STAGE_RASTER);
+			swk::launch_raster_tiles(ctx->stream, L.color_bits, ctx->tile_rows, map->ptrs, L.draws,
+				static_cast<DrawRec*>(ctx->drec.p), static_cast<PartRec*>(ctx->parts.p), static_cast<SpanRec*>(ctx->spans.p),
+				views_dev, L.num_views, L.arena, L.color_bits == 64 ? map->sky64.p : map->sky32.p, L.fb, L.total_tiles, counters);
+		}
+		swk::launch_finish(ctx->stream, static_cast<DrawRec*>(ctx->drec.p), nd, counters);
+	}
+	return hip_check(ctx, hipGetLastError(), "kernel launch");
+}
+
+// After a surface-only launch sequence: waits, keeps its error flag, clears the live counters.
+int flush_surface_errors(sw_ctx* ctx)
+{
+	uint32_t* counters = static_cast<uint32_t*>(ctx->counters.p);
+	int r = hip_check(ctx,
+		hipMemcpyAsync(ctx->counters_host, counters, sizeof(uint32_t) * COUNTER_COUNT, hipMemcpyDeviceToHost, ctx->stream),
+		"hipMemcpyAsync counters");
+	if(r)
+		return r;
+	hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, ctx->stream, counters);
+	if((r = hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize")))
+		return r;
+	collect_timings(ctx);
+	if(ctx->counters_host[COUNTER_ERROR] & ERROR_BAD_SURFACE)
+		ctx->surface_error = true;
+	return SW_OK;
 }
 
 // Waits for the batch, reads the counters; re-runs the raster stages if span storage overflowed.
@@ -390,9 +396,10 @@ int finish(sw_ctx* ctx, const ViewExt* views_dev)
 {
 	for(int attempt = 0; attempt < 4; ++attempt)
 	{
-		uint32_t host_counters[COUNTER_COUNT];
+		uint32_t* host_counters = ctx->counters_host;
 		int r = hip_check(ctx,
-			hipMemcpyAsync(host_counters, ctx->counters.p, sizeof(host_counters), hipMemcpyDeviceToHost, ctx->stream),
+			hipMemcpyAsync(host_counters, static_cast<uint32_t*>(ctx->counters.p) + COUNTER_RESULT,
+				sizeof(uint32_t) * COUNTER_COUNT, hipMemcpyDeviceToHost, ctx->stream),
 			"hipMemcpyAsync counters");
 		if(r)
 			return r;
@@ -400,7 +407,9 @@ int finish(sw_ctx* ctx, const ViewExt* views_dev)
 			return r;
 		collect_timings(ctx);
 		auto u64 = [&](int lo) { return uint64_t(host_counters[lo]) | (uint64_t(host_counters[lo + 1]) << 32); };
-		if(host_counters[COUNTER_ERROR] & ERROR_BAD_SURFACE)
+		const bool surface_error = ctx->surface_error;
+		ctx->surface_error = false;
+		if(surface_error || (host_counters[COUNTER_ERROR] & ERROR_BAD_SURFACE))
 			return set_error(ctx, SW_ERR_INVALID_ARGUMENT,
 				"invalid surface descriptor (arena order/overlap, texture level, or lightmap range out of bounds)");
 		const uint64_t needed = u64(COUNTER_SPANS_LO);
@@ -409,7 +418,7 @@ int finish(sw_ctx* ctx, const ViewExt* views_dev)
 			if(needed >= (1ull << 31))
 				return set_error(ctx, SW_ERR_BUDGET, "span storage above 2^31 records");
 			ctx->span_capacity = std::max<uint64_t>(needed + needed / 4, ctx->span_capacity * 2);
-			if((r = launch_raster_stages(ctx, ctx->last, views_dev)))
+			if((r = launch_stages(ctx, ctx->last, views_dev, false, true)))
 				return r;
 			continue;
 		}
@@ -444,6 +453,53 @@ void fill_surface_stats(sw_ctx* ctx, const sw_surface_desc* s, uint32_t n, bool 
 	}
 }
 
+// Launch sequence of the plain path's frame: the pending surfaces (if any), then the draws
+// already uploaded to ctx->draws / ctx->vdata when `raster`.
+int launch_frame(sw_ctx* ctx, uint32_t num_draws, uint64_t num_vertex_ints, bool raster)
+{
+	int r;
+	if(ctx->arena.p == nullptr && (r = ensure(ctx, ctx->arena, 8)))
+		return r;
+	sw_view v = ctx->frame_view;
+	v.first_surface = 0;
+	v.num_surfaces = ctx->frame_num_surfaces;
+	v.first_draw = 0;
+	v.num_draws = raster ? num_draws : 0;
+	v.fb_offset = 0;
+	v.surface_arena_offset = 0;
+	std::vector<ViewExt> ve;
+	sw_ctx::Launch& L = ctx->last;
+	L = sw_ctx::Launch{};
+	if((r = build_view_ext(ctx, &v, 1, ve, L.total_tiles)))
+		return r;
+	// written after the previous launches on the stream
+	if((r = upload(ctx, ctx->views, ve.data(), sizeof(ViewExt))))
+		return r;
+	L.map = ctx->frame_map;
+	L.color_bits = ctx->frame_color_bits;
+	L.num_views = 1;
+	L.surfaces = static_cast<const sw_surface_desc*>(ctx->surfaces.p);
+	L.num_surfaces = ctx->frame_num_surfaces;
+	L.lights = static_cast<const sw_light*>(ctx->lights.p);
+	L.num_lights = ctx->frame_num_lights;
+	L.light_indices = static_cast<const uint32_t*>(ctx->light_indices.p);
+	L.num_light_indices = ctx->frame_num_light_indices;
+	L.draws = static_cast<const sw_draw*>(ctx->draws.p);
+	L.num_draws = raster ? num_draws : 0;
+	L.vdata = static_cast<const int32_t*>(ctx->vdata.p);
+	L.num_vertex_ints = raster ? num_vertex_ints : 0;
+	L.arena_texels = ctx->frame_arena_texels;
+	L.arena = ctx->arena.p;
+	L.fb = ctx->fb.p;
+	const bool surfaces = ctx->frame_surfaces_pending;
+	ctx->frame_surfaces_pending = false;
+	if(!raster && !surfaces)
+		return SW_OK;
+	if((r = launch_stages(ctx, L, static_cast<const ViewExt*>(ctx->views.p), surfaces, raster)))
+		return r;
+	return raster ? SW_OK : flush_surface_errors(ctx);
+}
+
 } // namespace
 
 // =============================================================================================
@@ -472,9 +528,11 @@ int sw_ctx_create(int device, sw_ctx** out_ctx)
 		delete ctx;
 		return SW_ERR_DEVICE;
 	}
-	if(ensure(ctx, ctx->counters, sizeof(uint32_t) * COUNTER_COUNT) ||
-		hipMemsetAsync(ctx->counters.p, 0, sizeof(uint32_t) * COUNTER_COUNT, ctx->stream) != hipSuccess ||
-		hipStreamSynchronize(ctx->stream) != hipSuccess)
+	if(ensure(ctx, ctx->counters, sizeof(uint32_t) * COUNTER_TOTAL) ||
+		hipMemsetAsync(ctx->counters.p, 0, sizeof(uint32_t) * COUNTER_TOTAL, ctx->stream) != hipSuccess ||
+		hipStreamSynchronize(ctx->stream) != hipSuccess ||
+		hipHostMalloc(reinterpret_cast<void**>(&ctx->counters_host), sizeof(uint32_t) * COUNTER_COUNT, hipHostMallocDefault) !=
+			hipSuccess)
 	{
 		sw_ctx_destroy(ctx);
 		return SW_ERR_DEVICE;
@@ -494,11 +552,12 @@ void sw_ctx_destroy(sw_ctx* ctx)
 		(void)hipGraphExecDestroy(ctx->graph_exec);
 	if(ctx->graph)
 		(void)hipGraphDestroy(ctx->graph);
-	DevBuf* bufs[] = {&ctx->views, &ctx->surfaces, &ctx->lights, &ctx->light_indices, &ctx->draws, &ctx->vdata, &ctx->surf_end,
-		&ctx->parts, &ctx->spans, &ctx->drec, &ctx->band_lists, &ctx->band_counts, &ctx->counters, &ctx->arena, &ctx->fb,
-		&ctx->tonemap_out, &ctx->tonemap_sums};
+	DevBuf* bufs[] = {&ctx->views, &ctx->surfaces, &ctx->lights, &ctx->light_indices, &ctx->draws, &ctx->vdata, &ctx->surf_ext, &ctx->blk_surf,
+		&ctx->parts, &ctx->spans, &ctx->drec, &ctx->counters, &ctx->arena, &ctx->fb, &ctx->tonemap_out, &ctx->tonemap_sums};
 	for(DevBuf* b : bufs)
 		release(ctx, *b);
+	if(ctx->counters_host)
+		(void)hipHostFree(ctx->counters_host);
 	collect_timings(ctx);
 	for(hipEvent_t e : ctx->event_pool)
 		(void)hipEventDestroy(e);
@@ -611,6 +670,10 @@ int sw_frame_begin(sw_ctx* ctx, sw_map* map, uint32_t color_bits, uint32_t width
 	v.background_color = background_color;
 	ctx->frame_arena_texels = 0;
 	ctx->frame_num_surfaces = 0;
+	ctx->frame_num_lights = 0;
+	ctx->frame_num_light_indices = 0;
+	ctx->frame_surfaces_pending = false;
+	ctx->surface_error = false;
 	ctx->frame_active = true;
 	ctx->frame_rastered = false;
 	return SW_OK;
@@ -642,34 +705,9 @@ int sw_build_surfaces(sw_ctx* ctx, const sw_surface_desc* surfaces, uint32_t num
 	ctx->frame_num_lights = lights ? num_lights : 0;
 	ctx->frame_num_light_indices = light_indices ? num_light_indices : 0;
 	fill_surface_stats(ctx, surfaces, num_surfaces, true);
-
-	std::vector<ViewExt> ve;
-	uint32_t tb, tt;
-	uint64_t be;
-	sw_view v = ctx->frame_view;
-	v.first_surface = 0;
-	v.num_surfaces = num_surfaces;
-	v.first_draw = 0;
-	v.num_draws = 0;
-	v.fb_offset = 0;
-	v.surface_arena_offset = 0;
-	if((r = build_view_ext(ctx, &v, 1, ve, tb, tt, be)))
-		return r;
-	if((r = upload(ctx, ctx->views, ve.data(), sizeof(ViewExt))))
-		return r;
-	sw_ctx::Launch L;
-	L.map = ctx->frame_map;
-	L.color_bits = ctx->frame_color_bits;
-	L.num_views = 1;
-	L.surfaces = static_cast<const sw_surface_desc*>(ctx->surfaces.p);
-	L.num_surfaces = num_surfaces;
-	L.lights = static_cast<const sw_light*>(ctx->lights.p);
-	L.num_lights = ctx->frame_num_lights;
-	L.light_indices = static_cast<const uint32_t*>(ctx->light_indices.p);
-	L.num_light_indices = ctx->frame_num_light_indices;
-	L.arena_texels = arena;
-	L.arena = ctx->arena.p;
-	return launch_surface_stages(ctx, L, static_cast<const ViewExt*>(ctx->views.p));
+	// built by the launch sequence of sw_raster_draws / sw_frame_end (or sw_read_surfaces)
+	ctx->frame_surfaces_pending = true;
+	return SW_OK;
 }
 
 int sw_raster_draws(sw_ctx* ctx, const sw_draw* draws, uint32_t num_draws, const int32_t* vertex_data, uint64_t num_vertex_ints)
@@ -684,35 +722,7 @@ int sw_raster_draws(sw_ctx* ctx, const sw_draw* draws, uint32_t num_draws, const
 		return r;
 	if((r = upload(ctx, ctx->vdata, vertex_data, sizeof(int32_t) * num_vertex_ints)))
 		return r;
-	if(ctx->arena.p == nullptr && (r = ensure(ctx, ctx->arena, 8)))
-		return r;
-	sw_view v = ctx->frame_view;
-	v.first_surface = 0;
-	v.num_surfaces = ctx->frame_num_surfaces;
-	v.first_draw = 0;
-	v.num_draws = num_draws;
-	v.fb_offset = 0;
-	v.surface_arena_offset = 0;
-	std::vector<ViewExt> ve;
-	sw_ctx::Launch& L = ctx->last;
-	L = sw_ctx::Launch{};
-	if((r = build_view_ext(ctx, &v, 1, ve, L.total_bands, L.total_tiles, L.band_list_entries)))
-		return r;
-	// The views buffer may still be read by the surface kernels: write it after them on the stream.
-	if((r = upload(ctx, ctx->views, ve.data(), sizeof(ViewExt))))
-		return r;
-	L.map = ctx->frame_map;
-	L.color_bits = ctx->frame_color_bits;
-	L.num_views = 1;
-	L.draws = static_cast<const sw_draw*>(ctx->draws.p);
-	L.num_draws = num_draws;
-	L.vdata = static_cast<const int32_t*>(ctx->vdata.p);
-	L.num_vertex_ints = num_vertex_ints;
-	L.arena_texels = ctx->frame_arena_texels;
-	L.arena = ctx->arena.p;
-	L.fb = ctx->fb.p;
-	L.num_surfaces = ctx->frame_num_surfaces;
-	if((r = launch_raster_stages(ctx, L, static_cast<const ViewExt*>(ctx->views.p))))
+	if((r = launch_frame(ctx, num_draws, num_vertex_ints, true)))
 		return r;
 	ctx->frame_rastered = true;
 	return SW_OK;
@@ -912,12 +922,15 @@ int sw_read_surfaces(sw_ctx* ctx, void* dst, uint64_t num_texels)
 	if(!ctx || (!dst && num_texels))
 		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
 	(void)hipSetDevice(ctx->device);
+	int r;
+	if(ctx->frame_active && ctx->frame_surfaces_pending && (r = launch_frame(ctx, 0, 0, false)))
+		return r;
 	const uint64_t bytes = num_texels * color_bytes(ctx->last_color_bits);
 	if(bytes > ctx->arena.cap)
 		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "num_texels beyond the surface arena");
 	if(bytes == 0)
 		return SW_OK;
-	int r = hip_check(ctx, hipMemcpyAsync(dst, ctx->arena.p, bytes, hipMemcpyDeviceToHost, ctx->stream), "read surfaces");
+	r = hip_check(ctx, hipMemcpyAsync(dst, ctx->arena.p, bytes, hipMemcpyDeviceToHost, ctx->stream), "read surfaces");
 	return r ? r : hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
 }
 
@@ -939,7 +952,7 @@ static int prepare_batch(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_
 	if(b->color_bits != 32 && b->color_bits != 64)
 		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "color_bits must be 32 or 64");
 	int r;
-	if((r = build_view_ext(ctx, b->views, b->num_views, ve, L.total_bands, L.total_tiles, L.band_list_entries)))
+	if((r = build_view_ext(ctx, b->views, b->num_views, ve, L.total_tiles)))
 		return r;
 	if((r = validate_views_fit(ctx, ve, b->framebuffer_texels, b->num_surfaces, b->num_draws)))
 		return r;
@@ -1004,10 +1017,8 @@ int sw_render_batch(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_fb)
 		return r;
 	ctx->last = L;
 	ctx->last_color_bits = L.color_bits;
-	if((r = launch_surface_stages(ctx, L, static_cast<const ViewExt*>(ctx->views.p))))
-		return r;
 	// views that ask for a cleared background get it in the raster kernel's tile init
-	if((r = launch_raster_stages(ctx, L, static_cast<const ViewExt*>(ctx->views.p))))
+	if((r = launch_stages(ctx, L, static_cast<const ViewExt*>(ctx->views.p), true, true)))
 		return r;
 	return finish(ctx, static_cast<const ViewExt*>(ctx->views.p));
 }
@@ -1032,6 +1043,7 @@ int sw_render_batch_graph(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out
 	mix(b->num_draws);
 	mix(b->color_bits);
 	mix(b->surface_arena_texels);
+	mix(ctx->tile_rows);
 	for(uint32_t i = 0; i < b->num_views; ++i)
 	{
 		mix(b->views[i].width);
@@ -1069,8 +1081,7 @@ int sw_render_batch_graph(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out
 			return r;
 		ctx->last = L;
 		ctx->capturing = true;
-		int r1 = launch_surface_stages(ctx, L, static_cast<const ViewExt*>(ctx->views.p));
-		int r2 = r1 ? r1 : launch_raster_stages(ctx, L, static_cast<const ViewExt*>(ctx->views.p));
+		const int r2 = launch_stages(ctx, L, static_cast<const ViewExt*>(ctx->views.p), true, true);
 		hipGraph_t g = nullptr;
 		const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
 		ctx->capturing = false;
@@ -1087,6 +1098,79 @@ int sw_render_batch_graph(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out
 	if((r = hip_check(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream), "graph launch")))
 		return r;
 	return finish(ctx, static_cast<const ViewExt*>(ctx->views.p));
+}
+
+int sw_set_tile_rows(sw_ctx* ctx, uint32_t rows)
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	if(rows != 4 && rows != 8 && rows != 16)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "tile rows must be 4, 8 or 16");
+	if(rows != ctx->tile_rows)
+	{
+		ctx->tile_rows = rows;
+		ctx->graph_key = 0; // captured launches bake the tiling in
+	}
+	return SW_OK;
+}
+
+uint32_t sw_get_tile_rows(const sw_ctx* ctx) { return ctx ? ctx->tile_rows : 0; }
+
+int sw_autotune_batch(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_fb, uint32_t* out_rows,
+	double out_ms[SW_TILE_ROWS_COUNT])
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	(void)hipSetDevice(ctx->device);
+	static const uint32_t candidates[SW_TILE_ROWS_COUNT] = {4, 8, 16};
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	int r = hip_check(ctx, hipEventCreate(&e0), "hipEventCreate");
+	if(!r)
+		r = hip_check(ctx, hipEventCreate(&e1), "hipEventCreate");
+	double best_ms = 0.0;
+	uint32_t best = ctx->tile_rows;
+	const bool profiling = ctx->profiling;
+	ctx->profiling = false;
+	for(uint32_t c = 0; c < SW_TILE_ROWS_COUNT && !r; ++c)
+	{
+		if((r = sw_set_tile_rows(ctx, candidates[c])))
+			break;
+		// one untimed run sizes the buffers, then the median of three timed runs
+		if((r = sw_render_batch(ctx, map, b, out_fb)))
+			break;
+		float t[3] = {0.0f, 0.0f, 0.0f};
+		for(int k = 0; k < 3 && !r; ++k)
+		{
+			r = hip_check(ctx, hipEventRecord(e0, ctx->stream), "hipEventRecord");
+			if(!r)
+				r = sw_render_batch(ctx, map, b, out_fb);
+			if(!r)
+				r = hip_check(ctx, hipEventRecord(e1, ctx->stream), "hipEventRecord");
+			if(!r)
+				r = hip_check(ctx, hipEventSynchronize(e1), "hipEventSynchronize");
+			if(!r)
+				r = hip_check(ctx, hipEventElapsedTime(&t[k], e0, e1), "hipEventElapsedTime");
+		}
+		if(r)
+			break;
+		std::sort(t, t + 3);
+		if(out_ms)
+			out_ms[c] = t[1];
+		if(c == 0 || t[1] < best_ms)
+		{
+			best_ms = t[1];
+			best = candidates[c];
+		}
+	}
+	ctx->profiling = profiling;
+	if(e0)
+		(void)hipEventDestroy(e0);
+	if(e1)
+		(void)hipEventDestroy(e1);
+	const int r2 = sw_set_tile_rows(ctx, best);
+	if(out_rows)
+		*out_rows = best;
+	return r ? r : r2;
 }
 
 } // extern "C"

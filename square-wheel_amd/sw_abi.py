@@ -209,7 +209,13 @@ GPU_SYMBOLS = {
     "sw_profile_enable": (C.c_int, [C.c_void_p, C.c_uint32]),
     "sw_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_uint32]),
     "sw_ctx_device_bytes": (C.c_uint64, [C.c_void_p]),
+    "sw_set_tile_rows": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "sw_get_tile_rows": (C.c_uint32, [C.c_void_p]),
+    "sw_autotune_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Batch), C.c_void_p, C.POINTER(C.c_uint32),
+                                    C.POINTER(C.c_double)]),
 }
+
+TILE_ROWS = (4, 8, 16)
 
 _gpu_lib = None
 
@@ -279,9 +285,11 @@ class Context:
         return s.as_dict()
 
     # plain path -------------------------------------------------------------------------
-    def render_plain(self, map_h, b: BatchData, view_index=0, tonemap=None):
+    def render_plain(self, map_h, b: BatchData, view_index=0, tonemap=None, peek_surfaces=False):
         """Renders one view through sw_frame_begin / sw_build_surfaces / sw_raster_draws / sw_frame_end.
-        Returns the framebuffer (H, W) as uint32 / uint64 (and the tonemapped uint32 image if asked)."""
+        Returns the framebuffer (H, W) as uint32 / uint64 (and the tonemapped uint32 image if asked).
+        peek_surfaces: read the surface arena between sw_build_surfaces and sw_raster_draws and
+        return (framebuffer, arena)."""
         v = b.views[view_index]
         w, h = int(v["width"]), int(v["height"])
         cam = (C.c_float * 3)(*[float(x) for x in v["cam_pos"]])
@@ -291,6 +299,10 @@ class Context:
         surfaces = np.ascontiguousarray(b.surfaces[s0:s0 + ns])
         self.check(self.lib.sw_build_surfaces(self.h, _ptr(surfaces), ns, _ptr(b.lights), len(b.lights),
                                               _ptr(b.light_indices), len(b.light_indices)), "sw_build_surfaces")
+        arena = None
+        if peek_surfaces:
+            n = int(max(int(sv["pixels_offset"]) + int(sv["size"][0]) * int(sv["size"][1]) for sv in surfaces)) if ns else 0
+            arena = self.read_surfaces(n, b.color_bits)
         d0, nd = int(v["first_draw"]), int(v["num_draws"])
         draws = np.ascontiguousarray(b.draws[d0:d0 + nd])
         self.check(self.lib.sw_raster_draws(self.h, _ptr(draws), nd, _ptr(b.vertex_data), len(b.vertex_data)),
@@ -298,7 +310,7 @@ class Context:
         fb = np.zeros((h, w), dtype=b.color_dtype)
         if tonemap is None:
             self.check(self.lib.sw_frame_end(self.h, _ptr(fb), w * fb.itemsize, 0), "sw_frame_end")
-            return fb
+            return (fb, arena) if peek_surfaces else fb
         exposure, modulate = tonemap
         self.check(self.lib.sw_frame_end(self.h, _ptr(fb), w * fb.itemsize, 0), "sw_frame_end")
         out = np.zeros((h, w), dtype=np.uint32)
@@ -323,15 +335,30 @@ class Context:
     def profile_enable(self, on=True):
         self.check(self.lib.sw_profile_enable(self.h, 1 if on else 0), "sw_profile_enable")
 
-    PROFILE_STAGES = ("prep_surfaces", "build_surfaces", "setup_draws", "bin_bands", "raster_tiles", "tonemap")
+    PROFILE_STAGES = ("setup_draws", "build_surfaces", "raster_tiles", "tonemap")
 
     def profile_read(self, reset=True):
-        ms = (C.c_double * 6)()
-        cnt = (C.c_uint64 * 6)()
+        ms = (C.c_double * len(self.PROFILE_STAGES))()
+        cnt = (C.c_uint64 * len(self.PROFILE_STAGES))()
         self.check(self.lib.sw_profile_read(self.h, ms, cnt, 1 if reset else 0), "sw_profile_read")
         return {n: (float(ms[i]), int(cnt[i])) for i, n in enumerate(self.PROFILE_STAGES)}
 
     # batched path -----------------------------------------------------------------------
+    def set_tile_rows(self, rows):
+        self.check(self.lib.sw_set_tile_rows(self.h, rows), "sw_set_tile_rows")
+
+    def tile_rows(self):
+        return int(self.lib.sw_get_tile_rows(self.h))
+
+    def autotune(self, map_h, b: BatchData, device_inputs, out):
+        """Times every raster tile height on batch `b` (device-resident inputs) and keeps the fastest.
+        Returns (rows, {rows: ms per batch})."""
+        cb = b.ctypes_batch(inputs_on_device=True, device_ptrs=device_inputs.ptrs())
+        rows = C.c_uint32()
+        ms = (C.c_double * len(TILE_ROWS))()
+        self.check(self.lib.sw_autotune_batch(self.h, map_h, C.byref(cb), out.ptr, C.byref(rows), ms), "sw_autotune_batch")
+        return int(rows.value), {r: float(ms[i]) for i, r in enumerate(TILE_ROWS)}
+
     def render_batch(self, map_h, b: BatchData, graph=False, device_inputs=None, out=None, copy_back=True):
         """Renders all views of `b` in one set of launches; returns the flat framebuffer (uint32/64).
         device_inputs: DeviceCommandBuffers already in HBM (inputs stay resident across calls)."""

@@ -125,3 +125,68 @@ def test_tonemap_bit_exact(gpu_ctx, exposure):
     assert np.array_equal(out, ref)
     assert np.all((out >> 24) == 255)
     np.testing.assert_allclose(sums, rsums, rtol=1e-5)
+
+
+@pytest.mark.parametrize("color_bits", [32, 64])
+@pytest.mark.parametrize("rows", [4, 8, 16])
+def test_every_tile_height_bit_exact(gpu_ctx, rows, color_bits):
+    """The raster tile height changes the launch shape only, never a pixel."""
+    _say(f"case tile rows={rows} bits={color_bits}")
+    m, b = scenes.make_batch(30 + rows, color_bits=color_bits, view_sizes=((160, 120), (97, 61), (64, 200)))
+    prev = gpu_ctx.tile_rows()
+    mh = gpu_ctx.upload_map(m)
+    try:
+        gpu_ctx.set_tile_rows(rows)
+        assert gpu_ctx.tile_rows() == rows
+        fb = gpu_ctx.render_batch(mh, b)
+        st = gpu_ctx.stats()
+    finally:
+        gpu_ctx.set_tile_rows(prev)
+        gpu_ctx.release_map(mh)
+    ofb, _, ost, _ = oracle.render(m, b, mode=oracle.MODE_EXACT)
+    assert np.array_equal(fb, ofb)
+    for k in ("polygon_pixels", "triangle_pixels", "blend_pixels"):
+        assert st[k] == ost[k], (k, st[k], ost[k])
+
+
+def test_autotune_keeps_output(gpu_ctx):
+    """The autotuner times every tile height, keeps one of them, and the output stays exact."""
+    import sw_abi
+    _say("case autotune")
+    m, b = scenes.make_batch(41, color_bits=32, view_sizes=((320, 200), (160, 120)))
+    prev = gpu_ctx.tile_rows()
+    mh = gpu_ctx.upload_map(m)
+    dev = sw_abi.DeviceCommandBuffers(gpu_ctx, b)
+    out = gpu_ctx.alloc(b.fb_texels * 4)
+    try:
+        rows, ms = gpu_ctx.autotune(mh, b, dev, out)
+        assert rows in sw_abi.TILE_ROWS and gpu_ctx.tile_rows() == rows
+        assert all(v > 0 for v in ms.values()), ms
+        assert ms[rows] == min(ms.values())
+        fb = gpu_ctx.render_batch(mh, b, device_inputs=dev, out=out)
+        g = gpu_ctx.render_batch(mh, b, graph=True, device_inputs=dev, out=out)
+    finally:
+        gpu_ctx.set_tile_rows(prev)
+        out.free()
+        dev.free()
+        gpu_ctx.release_map(mh)
+    ofb, _, _, _ = oracle.render(m, b)
+    assert np.array_equal(fb, ofb)
+    assert np.array_equal(g, ofb)
+
+
+@pytest.mark.parametrize("color_bits", [32, 64])
+def test_surfaces_read_before_raster(gpu_ctx, color_bits):
+    """sw_read_surfaces between sw_build_surfaces and sw_raster_draws builds the surfaces on the
+    spot (they are otherwise built by the raster call's launch sequence); both the arena and the
+    frame equal the oracle's."""
+    _say(f"case peek surfaces bits={color_bits}")
+    m, b = scenes.make_batch(43, color_bits=color_bits, view_sizes=((200, 150),), draws_per_view=50)
+    mh = gpu_ctx.upload_map(m)
+    try:
+        fb, arena = gpu_ctx.render_plain(mh, b, peek_surfaces=True)
+    finally:
+        gpu_ctx.release_map(mh)
+    ofb, oarena, _, _ = oracle.render(m, b, mode=oracle.MODE_EXACT)
+    assert np.array_equal(arena, oarena[:len(arena)])
+    assert np.array_equal(fb.reshape(-1), ofb)
