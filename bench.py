@@ -53,17 +53,18 @@ def algorithmic_bytes(stats, color_bits, e_bytes):
     return {"build_surfaces": surf, "raster_tiles": raster}
 
 
-def read_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary, if present."""
+def read_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` in `workload` from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE, each scaled by its reading for a copy of known size), or None."""
     path = os.path.join(ROOT, "profiles", "traffic.json.txt")
     if not os.path.exists(path):
         return None
     try:
         with open(path) as f:
             data = json.load(f)
-        v = data.get(kernel)
-        return None if v is None else float(v)
-    except Exception:
+        v = data.get(workload, {}).get(kernel)
+        return None if v is None else int(v["traffic_bytes"])
+    except (OSError, ValueError, KeyError, TypeError):
         return None
 
 
@@ -109,7 +110,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="only run device-to-device copies of this size (HBM counter calibration) and exit")
     args = ap.parse_args()
+
+    if args.copy_calibration_mb:
+        # A plain copy kernel of known size: rocprofv3's FETCH_SIZE / WRITE_SIZE of it against its
+        # byte count gives the scale of those counters on this device.
+        ctx = sw_abi.Context(0)
+        n = args.copy_calibration_mb << 20
+        a, b = ctx.alloc(n), ctx.alloc(n)
+        for _ in range(3):
+            ctx.copy_device(b, a, n)
+        a.free()
+        b.free()
+        ctx.close()
+        print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): this build runs on a single GPU; "
@@ -187,7 +204,7 @@ def main():
     launches = max(prof[dominant][1], 1)
     avg_s = prof[dominant][0] / 1e3 / launches
     achieved = bytes_sum[dominant] / launches / avg_s / 1e9
-    traffic = read_traffic(dominant)
+    traffic = read_traffic("plain1080p", f"k_raster_tiles_rows{plain_rows}" if dominant == "raster_tiles" else "k_build_surfaces")
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dominant,
                 "algorithmic_bytes_per_launch": int(bytes_sum[dominant] / launches),

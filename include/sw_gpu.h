@@ -344,7 +344,8 @@ int sw_read_surfaces(sw_ctx* ctx, void* dst, uint64_t num_texels);
 void* sw_frame_device_framebuffer(sw_ctx* ctx);
 
 /* Device memory helpers for callers that keep command buffers resident in HBM (all on the
- * context's device; copies are synchronous). */
+ * context's device; copies are synchronous; the copy direction follows the pointers, so a
+ * device-to-device copy works through either function). */
 int sw_device_alloc(sw_ctx* ctx, uint64_t bytes, void** out_device_ptr);
 void sw_device_free(sw_ctx* ctx, void* device_ptr);
 int sw_copy_to_device(sw_ctx* ctx, void* device_dst, const void* host_src, uint64_t bytes);

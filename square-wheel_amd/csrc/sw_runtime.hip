@@ -871,7 +871,7 @@ int sw_copy_to_device(sw_ctx* ctx, void* dst, const void* src, uint64_t bytes)
 	(void)hipSetDevice(ctx->device);
 	if(!bytes)
 		return SW_OK;
-	int r = hip_check(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "copy to device");
+	int r = hip_check(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, ctx->stream), "copy to device");
 	return r ? r : hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
 }
 
@@ -882,7 +882,7 @@ int sw_copy_from_device(sw_ctx* ctx, void* dst, const void* src, uint64_t bytes)
 	(void)hipSetDevice(ctx->device);
 	if(!bytes)
 		return SW_OK;
-	int r = hip_check(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "copy from device");
+	int r = hip_check(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, ctx->stream), "copy from device");
 	return r ? r : hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
 }
 

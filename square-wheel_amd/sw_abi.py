@@ -332,6 +332,10 @@ class Context:
             self.check(self.lib.sw_copy_to_device(self.h, buf.ptr, _ptr(a), a.nbytes), "sw_copy_to_device")
         return buf
 
+    def copy_device(self, dst, src, nbytes):
+        """Device-to-device copy (a blit kernel on the context's stream; synchronous)."""
+        self.check(self.lib.sw_copy_to_device(self.h, dst.ptr, src.ptr, int(nbytes)), "sw_copy_to_device")
+
     def profile_enable(self, on=True):
         self.check(self.lib.sw_profile_enable(self.h, 1 if on else 0), "sw_profile_enable")
 
