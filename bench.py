@@ -50,7 +50,7 @@ def algorithmic_bytes(stats, color_bits, e_bytes):
     c = 8 if color_bits == 64 else 4
     surf = stats["surface_texels"] * (8 + c) + stats["lightmap_elements"] * e_bytes + stats["emissive_texels"] * (4 + c)
     raster = stats["polygon_pixels"] * (c + c) + stats["triangle_pixels"] * (4 + c) + stats["blend_pixels"] * c
-    return {"build_surfaces": surf, "raster_tiles": raster}
+    return {"build_surfaces": surf, "raster_tiles": raster, "setup_and_surfaces": surf}
 
 
 def read_traffic(workload, kernel):
@@ -96,6 +96,125 @@ def cpu_baseline(m, batches, seconds, threads):
     return n / el, n, el
 
 
+def hdr_leg(ctx, args):
+    """The HDR + models frame: 2560x1440 Color64 with 16 dynamic lights, 64 animated triangle
+    models, 256 decals and 10% alpha-blended materials, rendered from HBM-resident command buffers
+    and tonemapped on the device into a Color32 frame (sw_render_batch + sw_tonemap_device)."""
+    import ctypes as C
+    W, H = 2560, 1440
+    m = sw_host.large_map_alpha(args.seed)
+    mh = ctx.upload_map(m.data)
+    fbld = sw_host.FrameBuilder()
+    poses = m.camera_path(args.seed + 3, args.hdr_poses)
+    frames = [fbld.build(m, p, width=W, height=H, color_bits=64, use_directional_lightmaps=1, num_dynamic_lights=16,
+                         num_models=64, num_decals=256, seed=args.seed + 5, time=0.1 * i, clear_background=1, force_clear=1)
+              for i, p in enumerate(poses)]
+    batches = [sw_host.batch_from_frames([f]) for f in frames]
+    dev = [sw_abi.DeviceCommandBuffers(ctx, b) for b in batches]
+    hdr = ctx.alloc(W * H * 8)
+    ldr = ctx.alloc(W * H * 4)
+    cbs = [b.ctypes_batch(inputs_on_device=True, device_ptrs=d.ptrs()) for b, d in zip(batches, dev)]
+    mod = (C.c_float * 3)(1.0, 1.0, 1.0)
+    sums = (C.c_float * 3)()
+    lib = ctx.lib
+
+    def step(i):
+        ctx.check(lib.sw_render_batch(ctx.h, mh, C.byref(cbs[i % len(cbs)]), hdr.ptr), "sw_render_batch")
+        ctx.check(lib.sw_tonemap_device(ctx.h, hdr.ptr, W, H, C.c_float(1.0), mod, ldr.ptr, W * 4, sums), "sw_tonemap_device")
+
+    tris = [int(np.sum(f["draws"]["kind"] == sw_abi.DRAW_TRIANGLE)) for f in frames]
+    for i in range(len(cbs)):
+        step(i)
+    ctx.profile_read(reset=True)
+    lat = np.zeros(args.hdr_steps)
+    t0 = time.perf_counter()
+    for i in range(args.hdr_steps):
+        sample = i % 8 == 0
+        if sample:
+            ctx.profile_enable(True)
+        ts = time.perf_counter()
+        step(i)
+        lat[i] = time.perf_counter() - ts
+        if sample:
+            ctx.profile_enable(False)
+    wall = time.perf_counter() - t0
+    prof = ctx.profile_read(reset=True)
+    for d in dev:
+        d.free()
+    hdr.free()
+    ldr.free()
+    ctx.release_map(mh)
+    out = {"workload": f"{W}x{H} Color64 + tonemap, 16 dynamic lights, 64 models, 256 decals, 10% alpha materials, "
+                       f"{len(frames)}-pose path",
+           "frames_s": round(args.hdr_steps / wall, 2), "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 4),
+           "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 4),
+           "triangle_draws_per_frame": round(float(np.mean(tris)), 1),
+           "kernel_us_per_launch": {k: round(v[0] / max(v[1], 1) * 1e3, 3) for k, v in prof.items() if v[1]}}
+    log(f"hdr + models: {out}")
+    return out
+
+
+def serving_leg(args):
+    """Open-loop serving: four resident maps (16x16x4 .. 48x48x4 grids), Poisson arrivals at fixed
+    rates, each request a random map, pose, resolution (log-uniform 320x200 .. 2560x1440, 16:10) and
+    colour depth (Color32, or Color64 tonemapped on the device). The server batches requests on the
+    fly onto three concurrent slots under an HBM budget; latency is arrival -> result in host memory."""
+    rng = np.random.default_rng(args.seed + 77)
+    grids = [(16, 16, 4), (24, 24, 4), (32, 32, 4), (48, 48, 4)]
+    maps = [sw_host.large_map(args.seed + i, grid=g) for i, g in enumerate(grids)]
+    fbld = sw_host.FrameBuilder()
+    pool = []
+    for i, m in enumerate(maps):
+        for p in m.camera_path(args.seed + 40 + i, 12):
+            w = int(round(math.exp(rng.uniform(math.log(320), math.log(2560)))))
+            h = max(1, int(round(w * 10 / 16)))
+            bits = 64 if rng.uniform() < 0.5 else 32
+            pool.append((i, fbld.build(m, p, width=w, height=h, color_bits=bits, use_directional_lightmaps=1,
+                                       num_dynamic_lights=4, clear_background=1, force_clear=1)))
+    mean_px = float(np.mean([int(f["view"]["width"]) * int(f["view"]["height"]) for _, f in pool]))
+    srv = sw_abi.Server(slots=3, hbm_budget=16 << 30, max_views=32)
+    ids = [srv.add_map(m.data) for m in maps]
+    out = {"workload": f"4 maps {grids}, {len(pool)} distinct requests, mean {mean_px / 1e6:.2f} Mpixel, "
+                       "log-uniform 320x200..2560x1440, half Color64 + tonemap", "rates": []}
+    # warm-up: every request once
+    for mi, f in pool:
+        srv.submit(ids[mi], f, tonemap_exposure=1.0 if f["color_bits"] == 64 else None)
+    srv.drain()
+    for rate in (200, 500, 1000, 2000, 5000):
+        n = max(int(rate * args.serve_seconds), 50)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n))
+        picks = rng.integers(0, len(pool), size=n)
+        lat = []
+        views = []
+        submitted = {}
+        t0 = time.perf_counter()
+        k = 0
+        while k < n or srv.pending():
+            now = time.perf_counter() - t0
+            while k < n and arrivals[k] <= now:
+                mi, f = pool[int(picks[k])]
+                t = srv.submit(ids[mi], f, tonemap_exposure=1.0 if f["color_bits"] == 64 else None)
+                submitted[t] = arrivals[k]
+                k += 1
+            for c in srv.poll():
+                done_at = time.perf_counter() - t0
+                lat.append((done_at - submitted.pop(c["ticket"])) * 1e3)
+                views.append(c["batch_views"])
+                if c["status"] != 0:
+                    raise RuntimeError(f"request failed: {c}")
+        wall = time.perf_counter() - t0
+        lat = np.array(lat)
+        row = {"offered_req_s": rate, "requests": n, "achieved_req_s": round(n / wall, 1),
+               "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+               "mean_batch_views": round(float(np.mean(views)), 2),
+               "gpixel_s": round(n * mean_px / wall / 1e9, 3)}
+        out["rates"].append(row)
+        log(f"serving {row}")
+    out["server_stats"] = srv.stats()
+    srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +228,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-hdr", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--hdr-poses", type=int, default=16)
+    ap.add_argument("--hdr-steps", type=int, default=64)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run device-to-device copies of this size (HBM counter calibration) and exit")
@@ -174,32 +298,33 @@ def main():
     for i in range(args.warmup):
         render(i)
     ctx.profile_read(reset=True)
-    ctx.profile_enable(True)
+    # Timed region. Every 8th frame carries HIP events around its launches (on the context's
+    # stream), which give the per-kernel durations of the roofline; recording events costs host
+    # time, so the other frames go without.
     lat = np.zeros(args.steps)
     t0 = time.perf_counter()
     for i in range(args.steps):
+        sample = i % 8 == 0
+        if sample:
+            ctx.profile_enable(True)
         ts = time.perf_counter()
         render(i)
         lat[i] = time.perf_counter() - ts
+        if sample:
+            ctx.profile_enable(False)
     wall = time.perf_counter() - t0
-    ctx.profile_enable(False)
     prof = ctx.profile_read(reset=True)
     fps = args.steps / wall
-    # the same loop without the per-launch timing events (host-side cost of the events)
-    t1 = time.perf_counter()
-    for i in range(args.steps):
-        render(i)
-    fps_no_events = args.steps / (time.perf_counter() - t1)
     log(f"plain: {fps:.1f} frames/s, p50 {np.percentile(lat, 50) * 1e3:.3f} ms, p99 {np.percentile(lat, 99) * 1e3:.3f} ms")
     log(f"kernels: { {k: (round(v[0] / max(v[1], 1) * 1e3, 2), v[1]) for k, v in prof.items()} } (us/launch, launches)")
 
     # roofline of the dominant kernel over the timed region
-    steps_stats = [per_frame_stats[i % len(per_frame_stats)] for i in range(args.steps)]
-    bytes_sum = {"build_surfaces": 0, "raster_tiles": 0}
+    steps_stats = [per_frame_stats[i % len(per_frame_stats)] for i in range(args.steps) if i % 8 == 0]
+    bytes_sum = {"build_surfaces": 0, "raster_tiles": 0, "setup_and_surfaces": 0}
     for st in steps_stats:
         for k, v in algorithmic_bytes(st, color_bits, 40).items():
             bytes_sum[k] += v
-    kernel_ms = {k: prof[k][0] for k in ("build_surfaces", "raster_tiles")}
+    kernel_ms = {k: prof[k][0] for k in ("build_surfaces", "raster_tiles", "setup_and_surfaces")}
     dominant = max(kernel_ms, key=kernel_ms.get)
     launches = max(prof[dominant][1], 1)
     avg_s = prof[dominant][0] / 1e3 / launches
@@ -231,7 +356,7 @@ def main():
         "plain_gpixel_s": round(fps * args.width * args.height / 1e9, 4),
         "kernel_us_per_launch": {k: round(v[0] / max(v[1], 1) * 1e3, 3) for k, v in prof.items()},
         "tile_rows": plain_rows,
-        "frames_s_without_event_timing": round(fps_no_events, 2),
+        "event_sampled_frames": (args.steps + 7) // 8,
         "roofline": roofline,
     }
 
@@ -273,7 +398,7 @@ def main():
         s = bb.surfaces
         lm = int(np.sum(2 * s["lightmap_size"][:, 0].astype(np.int64) * s["size"][:, 1] * (s["lightmap_offset"] != 0)))
         bst.update(surface_texels=surf_texels, lightmap_elements=lm, emissive_texels=0)
-        bbytes = sum(algorithmic_bytes(bst, color_bits, 40).values())
+        bbytes = sum(v for k, v in algorithmic_bytes(bst, color_bits, 40).items() if k != "setup_and_surfaces")
         result["batched"] = {"views": args.batch_views, "pixels": total_px, "ms_per_batch": round(bwall * 1e3, 3),
                              "gpixel_s": round(total_px / bwall / 1e9, 4),
                              "vs_plain_gpixel_s": round((total_px / bwall) / (fps * args.width * args.height), 3),
@@ -284,6 +409,12 @@ def main():
         log(f"batched: {result['batched']}")
         bdev.free()
         bout.free()
+
+    if not args.no_hdr:
+        result["hdr_models"] = hdr_leg(ctx, args)
+
+    if not args.no_serve:
+        result["serving"] = serving_leg(args)
 
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()

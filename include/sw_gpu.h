@@ -309,6 +309,17 @@ int sw_tonemap(
  * ------------------------------------------------------------------------------------------- */
 int sw_render_batch(sw_ctx* ctx, sw_map* map, const sw_batch* batch, void* out_framebuffer_device);
 
+/* Asynchronous form of sw_render_batch, for schedulers that keep several batches in flight on
+ * several contexts: validates, uploads host-resident inputs (copied before returning) and queues
+ * the launches on the context's stream, then returns. sw_batch_poll returns 1 once the batch has
+ * finished on the device (0 while it runs); sw_batch_wait waits, checks the work counters (re-running
+ * the raster stage if span storage overflowed) and reports the batch's status; stats are valid
+ * after it. One batch in flight per context. No counterpart in the reference (one synchronous
+ * frame per call). */
+int sw_render_batch_async(sw_ctx* ctx, sw_map* map, const sw_batch* batch, void* out_framebuffer_device);
+int sw_batch_poll(sw_ctx* ctx);
+int sw_batch_wait(sw_ctx* ctx);
+
 /* Captures the last sw_render_batch call's launches (same shapes, same device buffers) into a
  * HIP graph and replays it; inputs must be device-resident and unchanged in size. */
 int sw_render_batch_graph(sw_ctx* ctx, sw_map* map, const sw_batch* batch, void* out_framebuffer_device);
@@ -352,10 +363,12 @@ int sw_copy_to_device(sw_ctx* ctx, void* device_dst, const void* host_src, uint6
 int sw_copy_from_device(sw_ctx* ctx, void* host_dst, const void* device_src, uint64_t bytes);
 
 /* Kernel timing with HIP events recorded on the context's stream around every launch.
- * Stages: 0 setup_draws (edge walk, span records and surface extents), 1 build_surfaces,
- * 2 raster_tiles, 3 tonemap. sw_profile_read returns the summed milliseconds and launch counts
- * since the last reset (and resets when reset != 0). Not recorded inside captured graphs. */
-#define SW_PROFILE_STAGES 4
+ * Stages: 0 setup_draws (edge walk, span records and, for device-resident descriptors, surface
+ * extents), 1 build_surfaces, 2 raster_tiles, 3 tonemap, 4 setup_and_surfaces (draw setup and
+ * surface texels in one launch, for host-resident descriptors). sw_profile_read returns the summed
+ * milliseconds and launch counts since the last reset (and resets when reset != 0). Not recorded
+ * inside captured graphs. */
+#define SW_PROFILE_STAGES 5
 int sw_profile_enable(sw_ctx* ctx, uint32_t enable);
 int sw_profile_read(sw_ctx* ctx, double out_ms[SW_PROFILE_STAGES], uint64_t out_counts[SW_PROFILE_STAGES], uint32_t reset);
 

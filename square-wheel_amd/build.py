@@ -1,6 +1,7 @@
 """Builds the native libraries of the package, in-tree.
 
-  libsw_gpu.so   HIP kernels + C ABI (include/sw_gpu.h), gfx950 code objects, built by hipcc
+  libsw_gpu.so   HIP kernels + C ABI (include/sw_gpu.h) + frame-serving scheduler
+                 (include/sw_server.h), gfx950 code objects, built by hipcc
   libsw_host.so  C++ renderer front-end (map/texture/lightmap generation, visibility,
                  per-polygon surface and draw setup) that emits the command buffers
 
@@ -49,7 +50,7 @@ def _run(cmd):
 
 
 def build_gpu():
-    srcs = [os.path.join(CSRC, "sw_kernels.hip"), os.path.join(CSRC, "sw_runtime.hip")]
+    srcs = [os.path.join(CSRC, "sw_kernels.hip"), os.path.join(CSRC, "sw_runtime.hip"), os.path.join(CSRC, "sw_server.cpp")]
     tmp = GPU_LIB + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-o", tmp, *srcs])
     os.replace(tmp, GPU_LIB)

@@ -327,13 +327,6 @@ M4 quat_to_m4(Quat q)
 	return m;
 }
 
-struct Camera
-{
-	V3 position;
-	M4 view_matrix;
-	M4 planes_matrix;
-};
-
 // matrix.rs:13-109
 Camera build_camera(V3 position, float azimuth, float elevation, float fov, float width, float height)
 {
@@ -892,8 +885,9 @@ int swh_frame_build(swh_frame* f, const swh_map* m, const swh_frame_config* cfg,
 	for(uint32_t pi : f->visible_polygons)
 		emit_surface(*f, *m, *cfg, pi);
 
-	// models: animation, lighting, projection (per frame, before the tree walk)
-	prepare_models(*f, *m, *cfg, cam.position, cam.view_matrix, cam.planes_matrix);
+	// models and decals: animation, lighting, projection (per frame, before the tree walk)
+	prepare_models(*f, *m, *cfg, cam.view_matrix, cam.planes_matrix);
+	prepare_decals(*f, *m, *cfg, cam.planes_matrix);
 
 	// draw list (partial_renderer.rs:1285-1325 with one full-screen rect)
 	const int32_t clip_rect[4] = {0, 0, int32_t(cfg->width), int32_t(cfg->height)};
@@ -928,7 +922,7 @@ int swh_frame_build(swh_frame* f, const swh_map* m, const swh_frame_config* cfg,
 				draw_polygon(*f, cp, &f->vertices_transformed[p.first_vertex], p.num_vertices, pd.de, pd.te, pd.size, pd.surface_offset,
 					SW_TEXSRC_SURFACE, m->materials[p.texture].blending_mode, clip_rect);
 			}
-			draw_leaf_objects(*f, *m, *cfg, leaf_index, cp, clip_rect);
+			draw_leaf_objects(*f, *m, *cfg, leaf_index, lb, clip_rect, cam.planes_matrix);
 		}
 		else
 		{

@@ -102,13 +102,31 @@ struct TcEq
 };
 
 
+struct Camera
+{
+	V3 position;
+	M4 view_matrix;
+	M4 planes_matrix;
+};
+
 uint32_t clip_3d_polygon_by_z_plane(const V3* poly, uint32_t n, float z_dist, V3* out, uint32_t cap);
+V3 line_z_intersection(V3 v0, V3 v1, float z);
+DepthEq depth_from_plane(V4 p);
+TcEq tc_from_depth_and_planes(const DepthEq& d, const V4 t[2]);
+TcEq scale(const TcEq& e, float s);
+bool affine_may_be_used(const DepthEq& de, const TcEq& te, V2 min_p, V2 max_p);
+uint32_t calculate_mip(const V2* points, uint32_t n, const DepthEq& de, const TcEq& te, float mip_bias);
+int32_t saturating_f32_to_i32(float x);
 uint32_t clip_2d_polygon(const V2* poly, uint32_t n, V3 line, V2* out, uint32_t cap);
 uint32_t project_and_clip_polygon(const V3 clip_planes[8], const V3* verts, uint32_t n, V2* out);
 V3 view_transform(const M4& m, V3 v);
 int32_t f32_to_fixed16_unchecked(float x);
 void draw_polygon(swh_frame& f, const V3 clip_planes[8], const V3* verts, uint32_t nverts, const DepthEq& de, const TcEq& te,
 	const uint32_t tsize[2], uint64_t texture_offset, uint32_t texture_source, uint32_t blend, const int32_t clip_rect[4]);
+// Decals of the leaf's polygons, then the leaf's models, after the leaf's polygons (draw_leaf,
+// partial_renderer.rs:2519-2792). `bounds` is the leaf's screen ClippingPolygon.
+void draw_leaf_objects(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, uint32_t leaf, const ClippingPolygon& bounds,
+	const int32_t clip_rect[4], const M4& planes);
 
 } // namespace swh
 

@@ -125,6 +125,90 @@ struct PointLight
 	V3 color;
 };
 
+// LightGridElement (bsp_map_compact.rs): a light cube (+x, -x... order of light_cube.rs:24-64) plus
+// one dominant directional light.
+struct LightGridElement
+{
+	float light_cube[6][3];
+	V3 light_direction_vector_scaled;
+	float directional_light_color[3];
+};
+
+// LightCube (common/light_cube.rs:3-160): accumulates light samples by direction into six
+// signed axis bins plus a per-channel sum of scaled directions, then splits off one dominant
+// directional light.
+struct LightCubeAccum
+{
+	float cube[6][3] = {};
+	V3 sum_dirs[3] = {v3(0, 0, 0), v3(0, 0, 0), v3(0, 0, 0)};
+
+	void add_sample(V3 direction, const float light[3])
+	{
+		const float len = std::max(length(direction), 0.00000001f);
+		const V3 d = direction / len;
+		const float comps[3] = {d.x, d.y, d.z};
+		for(int axis = 0; axis < 3; ++axis)
+		{
+			const int side = comps[axis] <= 0.0f ? axis * 2 : axis * 2 + 1;
+			const float k = comps[axis] <= 0.0f ? -comps[axis] : comps[axis];
+			for(int i = 0; i < 3; ++i)
+				cube[side][i] += light[i] * k;
+		}
+		for(int i = 0; i < 3; ++i)
+			sum_dirs[i] = sum_dirs[i] + d * light[i];
+	}
+	void add_constant(const float light[3])
+	{
+		for(int side = 0; side < 6; ++side)
+			for(int i = 0; i < 3; ++i)
+				cube[side][i] += light[i];
+	}
+	void scale(float k)
+	{
+		for(int side = 0; side < 6; ++side)
+			for(int i = 0; i < 3; ++i)
+				cube[side][i] *= k;
+		for(int i = 0; i < 3; ++i)
+			sum_dirs[i] = sum_dirs[i] * k;
+	}
+	LightGridElement convert() const
+	{
+		LightGridElement e{};
+		const V3 dir = (sum_dirs[0] + sum_dirs[1] + sum_dirs[2]) * 0.33f;
+		const float len2 = std::max(dot(dir, dir), 0.00000001f);
+		float color[3];
+		for(int i = 0; i < 3; ++i)
+			color[i] = std::max(dot(sum_dirs[i], dir), 0.0f) / len2;
+		for(int side = 0; side < 6; ++side)
+			for(int i = 0; i < 3; ++i)
+				e.light_cube[side][i] = cube[side][i];
+		const float comps[3] = {dir.x, dir.y, dir.z};
+		for(int i = 0; i < 3; ++i)
+			for(int axis = 0; axis < 3; ++axis)
+			{
+				const int side = comps[axis] <= 0.0f ? axis * 2 : axis * 2 + 1;
+				const float k = comps[axis] <= 0.0f ? -comps[axis] : comps[axis];
+				e.light_cube[side][i] = std::max(e.light_cube[side][i] - color[i] * k, 0.0f);
+			}
+		e.light_direction_vector_scaled = dir;
+		for(int i = 0; i < 3; ++i)
+			e.directional_light_color[i] = color[i];
+		return e;
+	}
+};
+
+// A vertex-animated triangle model (triangle_model.rs VertexData::VertexAnimated): positions and
+// normals per frame, texture coordinates shared, in model space.
+struct ModelMesh
+{
+	uint32_t num_vertices = 0, num_frames = 0;
+	std::vector<V3> pos; // [frame][vertex]
+	std::vector<V3> normal; // [frame][vertex]
+	std::vector<float> tc; // 2 per vertex, in [0, 1]
+	std::vector<uint32_t> tris; // 3 per triangle
+	std::vector<BBox> frame_bbox;
+};
+
 } // namespace swh
 
 struct swh_map
@@ -158,6 +242,9 @@ struct swh_map
 	std::vector<swh::PointLight> lights;
 	swh::V3 sun_dir{0, 0, -1};
 	float sun_intensity = 0.0f;
+	// light grid: one sample at the centre of every cell (zero for solid cells)
+	std::vector<swh::LightGridElement> light_grid;
+	swh::ModelMesh model_mesh;
 };
 
 namespace swh

@@ -598,6 +598,132 @@ void build_skybox(swh_map& m, uint32_t seed)
 	}
 }
 
+// Light grid samples at cell centres from the static lights (the lightmapper's light grid pass).
+void build_light_grid(swh_map& m)
+{
+	const float CELL = 128.0f;
+	m.light_grid.assign(m.cell.size(), LightGridElement{});
+	for(uint32_t c : m.empty_cells)
+	{
+		const uint32_t x = c % m.gx, y = (c / m.gx) % m.gy, z = c / (m.gx * m.gy);
+		const V3 pos = v3(CELL * (float(x) + 0.5f), CELL * (float(y) + 0.5f), CELL * (float(z) + 0.5f));
+		LightCubeAccum acc;
+		for(const PointLight& l : m.lights)
+		{
+			const V3 d = l.pos - pos;
+			const float d2 = std::max(dot(d, d), 64.0f * 64.0f);
+			if(!segment_clear(m, pos, l.pos))
+				continue;
+			const float k = l.intensity / d2;
+			const float light[3] = {l.color.x * k, l.color.y * k, l.color.z * k};
+			acc.add_sample(d, light);
+		}
+		if(m.sun_intensity > 0.0f)
+		{
+			const V3 to_sun = m.sun_dir * -1.0f;
+			if(sun_visible(m, pos, to_sun))
+			{
+				const float light[3] = {m.sun_intensity, m.sun_intensity * 0.95f, m.sun_intensity * 0.85f};
+				acc.add_sample(to_sun, light);
+			}
+		}
+		const float ambient[3] = {0.06f, 0.06f, 0.06f};
+		acc.add_constant(ambient);
+		m.light_grid[c] = acc.convert();
+	}
+}
+
+// A geodesic sphere (icosahedron faces split into freq^2 triangles, 20 * 7^2 = 980 triangles) with
+// `frames` frames of a seeded radial wobble: the procedural vertex-animated model.
+void build_model_mesh(swh_map& m, uint32_t seed, uint32_t freq, uint32_t frames, float radius)
+{
+	const float t = (1.0f + std::sqrt(5.0f)) * 0.5f;
+	const V3 ico[12] = {v3(-1, t, 0), v3(1, t, 0), v3(-1, -t, 0), v3(1, -t, 0), v3(0, -1, t), v3(0, 1, t), v3(0, -1, -t),
+		v3(0, 1, -t), v3(t, 0, -1), v3(t, 0, 1), v3(-t, 0, -1), v3(-t, 0, 1)};
+	const uint32_t faces[20][3] = {{0, 11, 5}, {0, 5, 1}, {0, 1, 7}, {0, 7, 10}, {0, 10, 11}, {1, 5, 9}, {5, 11, 4}, {11, 10, 2},
+		{10, 7, 6}, {7, 1, 8}, {3, 9, 4}, {3, 4, 2}, {3, 2, 6}, {3, 6, 8}, {3, 8, 9}, {4, 9, 5}, {2, 4, 11}, {6, 2, 10},
+		{8, 6, 7}, {9, 8, 1}};
+	ModelMesh& mm = m.model_mesh;
+	std::vector<V3> dirs;
+	std::vector<uint32_t>& tris = mm.tris;
+	tris.clear();
+	auto vertex_id = [&](V3 p) -> uint32_t {
+		const V3 d = p / length(p);
+		for(uint32_t i = 0; i < dirs.size(); ++i)
+		{
+			const V3 e = dirs[i] - d;
+			if(dot(e, e) < 1e-10f)
+				return i;
+		}
+		dirs.push_back(d);
+		return uint32_t(dirs.size() - 1);
+	};
+	for(const auto& f : faces)
+	{
+		const V3 a = ico[f[0]], b = ico[f[1]], c = ico[f[2]];
+		std::vector<uint32_t> grid;
+		// row i has freq - i + 1 points: a + (b - a) * j / freq + (c - a) * i / freq
+		for(uint32_t i = 0; i <= freq; ++i)
+			for(uint32_t j = 0; j + i <= freq; ++j)
+				grid.push_back(vertex_id(a + (b - a) * (float(j) / float(freq)) + (c - a) * (float(i) / float(freq))));
+		auto at = [&](uint32_t i, uint32_t j) {
+			uint32_t idx = 0;
+			for(uint32_t r = 0; r < i; ++r)
+				idx += freq - r + 1;
+			return grid[idx + j];
+		};
+		for(uint32_t i = 0; i < freq; ++i)
+			for(uint32_t j = 0; j + i < freq; ++j)
+			{
+				tris.insert(tris.end(), {at(i, j), at(i, j + 1), at(i + 1, j)});
+				if(j + i + 1 < freq)
+					tris.insert(tris.end(), {at(i, j + 1), at(i + 1, j + 1), at(i + 1, j)});
+			}
+	}
+	mm.num_vertices = uint32_t(dirs.size());
+	mm.num_frames = frames;
+	mm.tc.resize(size_t(mm.num_vertices) * 2);
+	for(uint32_t v = 0; v < mm.num_vertices; ++v)
+	{
+		const V3 d = dirs[v];
+		mm.tc[v * 2 + 0] = 0.5f + std::atan2(d.y, d.x) * (0.5f / 3.14159265f);
+		mm.tc[v * 2 + 1] = std::acos(std::max(-1.0f, std::min(1.0f, d.z))) * (1.0f / 3.14159265f);
+	}
+	Rng rng(seed * 31u + 5u);
+	const float ka = rng.uniform(2.0f, 4.0f), kb = rng.uniform(2.0f, 4.0f), kc = rng.uniform(1.0f, 3.0f);
+	mm.pos.resize(size_t(frames) * mm.num_vertices);
+	mm.normal.assign(size_t(frames) * mm.num_vertices, v3(0, 0, 0));
+	mm.frame_bbox.resize(frames);
+	for(uint32_t f = 0; f < frames; ++f)
+	{
+		const float phase = 6.2831853f * float(f) / float(frames);
+		V3* pos = &mm.pos[size_t(f) * mm.num_vertices];
+		V3* nrm = &mm.normal[size_t(f) * mm.num_vertices];
+		for(uint32_t v = 0; v < mm.num_vertices; ++v)
+		{
+			const V3 d = dirs[v];
+			const float wob = 1.0f + 0.18f * std::sin(ka * d.x + kb * d.y + phase) * std::cos(kc * d.z - phase);
+			pos[v] = d * (radius * wob);
+		}
+		for(size_t k = 0; k < tris.size(); k += 3)
+		{
+			const V3 a = pos[tris[k]], b = pos[tris[k + 1]], c = pos[tris[k + 2]];
+			const V3 n = cross(b - a, c - a);
+			nrm[tris[k]] = nrm[tris[k]] + n;
+			nrm[tris[k + 1]] = nrm[tris[k + 1]] + n;
+			nrm[tris[k + 2]] = nrm[tris[k + 2]] + n;
+		}
+		BBox bb{pos[0], pos[0]};
+		for(uint32_t v = 0; v < mm.num_vertices; ++v)
+		{
+			nrm[v] = nrm[v] / std::max(length(nrm[v]), 1e-20f);
+			bb.min = v3(std::min(bb.min.x, pos[v].x), std::min(bb.min.y, pos[v].y), std::min(bb.min.z, pos[v].z));
+			bb.max = v3(std::max(bb.max.x, pos[v].x), std::max(bb.max.y, pos[v].y), std::max(bb.max.z, pos[v].z));
+		}
+		mm.frame_bbox[f] = bb;
+	}
+}
+
 void light_polygons(swh_map& m)
 {
 	const bool directional = m.config.directional_lightmaps != 0;
@@ -961,6 +1087,8 @@ int swh_map_create(const swh_map_config* cfg, swh_map** out)
 		m->sun_intensity = 0.8f;
 	}
 	light_polygons(*m);
+	build_light_grid(*m);
+	build_model_mesh(*m, cfg->seed, 7, 8, 28.0f);
 	*out = m;
 	return 0;
 }

@@ -8,50 +8,72 @@ struct swh_frame;
 namespace swh
 {
 
-struct ModelTriangle
-{
-	uint32_t v[3];
-};
-
+// ModelVertex3d (triangle_models_rendering.rs): camera-space position, texture coordinates in
+// texels, light.
 struct ModelVertex3d
 {
-	V3 pos; // camera space (view_matrix applied, not projected)
+	V3 pos;
 	float tc[2];
 	float light[3];
 };
 
-// One visible model of the frame, with its transformed vertices and its back-to-front sorted
-// front-facing triangles (triangle_models_rendering.rs:324-391).
+// One model entity of the frame (frame_info.rs ModelEntity, with VertexAnimated data and
+// ModelLighting::Default).
+struct ModelEntity
+{
+	V3 pos;
+	float yaw;
+	uint32_t frames[2];
+	float lerp; // weight of frames[0]
+	uint32_t skin; // mip-0 lite level of its texture
+	uint32_t blend;
+};
+
+// One visible model with its transformed vertices and its back-to-front sorted front-facing
+// triangles (partial_renderer.rs VisibleDynamicMeshInfo).
 struct VisibleModel
 {
-	uint32_t model;
+	uint32_t entity;
 	uint32_t first_vertex, num_vertices;
 	uint32_t first_triangle, num_triangles;
-	uint32_t skin_level;
-	BBox bbox;
-	std::vector<uint32_t> leafs;
+	uint32_t level; // lite texture level of the selected mip
+	V3 bbox_vertices[8]; // camera space
+	float clip[4][2]; // screen ClippingPolygon axes (x, y, x + y, x - y) as min/max pairs
+	float depth; // for ordering within a leaf
+};
+
+// Decal (frame_info.rs:113-123): a unit cube (half-size 1) scaled and rotated, projected onto
+// the polygons it touches.
+struct DecalEntity
+{
+	V3 pos;
+	V3 axis[3]; // rotation columns (unit), decal-space x, y, z in world space
+	V3 scale;
+	uint32_t level; // mip-0 lite level
+	uint32_t blend;
+	float lightmap_light_scale;
+	float light_add[3];
 };
 
 struct ModelFrameData
 {
+	std::vector<ModelEntity> entities;
 	std::vector<VisibleModel> visible;
 	std::vector<ModelVertex3d> vertices;
-	std::vector<ModelTriangle> triangles;
+	std::vector<uint32_t> triangles; // 3 per triangle
 	std::vector<std::vector<uint32_t>> leaf_models; // per leaf: indices into `visible`
-	std::vector<std::vector<uint32_t>> leaf_decals; // per leaf: decal indices
-	struct Decal
+	std::vector<DecalEntity> decals;
+	struct DecalInfo
 	{
-		V3 pos;
-		V3 axis[3]; // scaled box axes (half extents)
-		uint32_t level;
-		uint32_t blend;
+		M4 camera_planes_matrix;
+		LightGridElement dynamic_light;
 	};
-	std::vector<Decal> decals;
+	std::vector<DecalInfo> decal_info;
+	std::vector<std::vector<uint32_t>> leaf_decals; // per leaf: decal indices
 };
 
 void build_dynamic_lights(swh_frame& f, const swh_map& m, const swh_frame_config& cfg);
-void prepare_models(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, V3 cam_pos, const M4& view, const M4& planes);
-void draw_leaf_objects(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, uint32_t leaf, const V3 clip_planes[8],
-	const int32_t clip_rect[4]);
+void prepare_models(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, const M4& view, const M4& planes);
+void prepare_decals(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, const M4& planes);
 
 } // namespace swh

@@ -1,6 +1,7 @@
 // Internal records shared by the kernels and the host runtime (not part of the C ABI).
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "../../include/sw_gpu.h"
@@ -36,6 +37,7 @@ enum
 {
 	ERROR_BAD_SURFACE = 1u << 0,
 	ERROR_SPAN_OVERFLOW = 1u << 1,
+	ERROR_PART_GAP = 1u << 2, // the parts of a draw did not cover consecutive rows (never expected)
 };
 
 // Surface texels are built by blocks of SURF_BLOCK threads over the flattened arena.
@@ -57,6 +59,17 @@ struct ViewExt
 	uint32_t cols; // 64-pixel columns
 	uint32_t first_tile; // global tile (wave) index of tile 0
 	uint32_t pad;
+};
+
+// The views of a batch: a device array, or (a single view) the view itself, passed by value in
+// the kernel arguments so that a one-view batch needs no upload before its launches.
+struct ViewsArg
+{
+	const ViewExt* ptr; // device array (when !inline_one)
+	uint32_t count;
+	uint32_t inline_one; // the single view is `one`
+	ViewExt one;
+	__host__ __device__ const ViewExt& operator[](uint32_t i) const { return inline_one ? one : ptr[i]; }
 };
 
 // One part of a draw: a y range with its left/right edges (the draw_func calls of
@@ -83,7 +96,11 @@ struct DrawRec
 	int32_t row_min, row_max; // rows with non-empty spans
 	int32_t x_min, x_max; // union of the non-empty spans
 	uint32_t pixels; // pixels covered by the draw's spans (at most 2^28)
-	uint32_t stat; // DRAW_STAT_*
+	uint32_t stat; // DRAW_STAT_* | kind << DRAW_STAT_KIND_SHIFT
+	uint32_t rows; // span records the draw needs (written unless storage overflowed)
+	uint32_t span_base; // span record of row_first; row y uses span_base + (y - row_first)
+	int32_t row_first; // first row of the draw's parts
+	uint32_t rows0; // rows of the first part (a triangle's lower part)
 	uint32_t pad;
 };
 
@@ -91,6 +108,7 @@ enum
 {
 	DRAW_STAT_TRIANGLE = 1u << 0,
 	DRAW_STAT_READS_DST = 1u << 1, // the blending mode reads the destination
+	DRAW_STAT_KIND_SHIFT = 4, // 0 perspective, 1 line z, 2 affine, 3 triangle
 };
 
 struct MapPtrs
@@ -105,25 +123,59 @@ struct MapPtrs
 	uint64_t num_texels, num_lite, num_lm_simple, num_lm_dir;
 };
 
-#include <hip/hip_runtime.h>
+// Arguments of the draw setup blocks.
+struct SetupArgs
+{
+	const sw_draw* draws;
+	uint32_t num_draws;
+	uint32_t setup_blocks; // blocks of draws; the launch's later blocks do other work
+	const int32_t* vdata;
+	uint64_t num_vertex_ints;
+	PartRec* parts;
+	SpanRec* spans;
+	DrawRec* drec;
+	uint32_t* counters;
+	uint64_t span_capacity;
+	uint64_t span_stride; // > 0: draw i owns span records [i * stride, (i + 1) * stride), no allocation
+	uint64_t arena_texels, skybox_texels, lite_texels;
+};
+
+// Arguments of the surface texel blocks.
+struct SurfaceArgs
+{
+	const sw_surface_desc* surfaces;
+	uint32_t num_surfaces;
+	uint32_t num_lights;
+	const SurfExt* surf_ext;
+	const uint32_t* blk_surf; // block -> first surface table; null: one view, searched in the kernel
+	const sw_light* lights;
+	const uint32_t* light_indices;
+	uint32_t num_light_indices;
+	uint32_t pad;
+	void* arena;
+	uint64_t arena_texels;
+	uint32_t* counters; // error flags (validation of descriptors without a prepared table)
+};
+
 namespace swk
 {
 // Kernel 1: draw setup (edge walk -> parts -> span records) with `lanes` (64 or 256) threads per
-// draw and, in extra blocks, the surfaces' arena extents, validation and block -> surface table.
-void launch_setup(hipStream_t st, uint32_t lanes, const sw_draw* draws, uint32_t nd, const int32_t* vdata, uint64_t nvi,
-	const ViewExt* views, uint32_t nv, PartRec* parts, SpanRec* spans, DrawRec* drec, uint32_t* counters,
-	uint64_t span_capacity, uint64_t arena_texels, uint64_t skybox_texels, uint64_t lite_texels,
-	const sw_surface_desc* surfaces, uint32_t ns, SurfExt* surf_ext, uint32_t* blk_surf, const MapPtrs& m);
+// draw and, in extra blocks, the surfaces' arena extents, validation and block -> surface table
+// (device-resident descriptors).
+void launch_setup(hipStream_t st, uint32_t lanes, SetupArgs a, const ViewsArg& views, const sw_surface_desc* surfaces,
+	uint32_t ns, SurfExt* surf_ext, uint32_t* blk_surf, const MapPtrs& m);
 // Kernel 2: one thread per surface-arena texel.
-void launch_build_surfaces(hipStream_t st, uint32_t color_bits, const MapPtrs& m, const sw_surface_desc* s, uint32_t n,
-	const SurfExt* surf_ext, const uint32_t* blk_surf, const ViewExt* views, const sw_light* lights, uint32_t nl,
-	const uint32_t* lidx, uint32_t nli, void* arena, uint64_t arena_texels, const uint32_t* counters);
+void launch_build_surfaces(hipStream_t st, uint32_t color_bits, const MapPtrs& m, const SurfaceArgs& sa, const ViewsArg& views,
+	const uint32_t* counters);
+// Kernels 1 + 2 in one launch when the host prepared the surfaces.
+void launch_setup_and_surfaces(hipStream_t st, uint32_t lanes, uint32_t color_bits, SetupArgs a, const ViewsArg& views,
+	const SurfaceArgs& sa, const MapPtrs& m);
 // Kernel 3: one wave per 64 x tile_rows tile (binning fused).
 void launch_raster_tiles(hipStream_t st, uint32_t color_bits, uint32_t tile_rows, const MapPtrs& m, const sw_draw* draws,
-	const DrawRec* drec, const PartRec* parts, const SpanRec* spans, const ViewExt* views, uint32_t nv, const void* arena,
+	const DrawRec* drec, const PartRec* parts, const SpanRec* spans, const ViewsArg& views, const void* arena,
 	const void* skybox, void* fb, uint32_t total_tiles, const uint32_t* counters);
-// Kernel 4 (one block): per-draw statistics -> result block, live block cleared.
-void launch_finish(hipStream_t st, const DrawRec* drec, uint32_t nd, uint32_t* counters);
+// Kernel 4 (one block): per-draw statistics -> result block (host-mapped), live block cleared.
+void launch_finish(hipStream_t st, const DrawRec* drec, uint32_t nd, uint32_t* counters, uint32_t* result);
 void launch_tonemap(hipStream_t st, const uint64_t* hdr, uint32_t w, uint32_t h, float isr, float isg, float isb,
 	uint32_t* dst, uint32_t dst_pitch, float* sums);
 } // namespace swk

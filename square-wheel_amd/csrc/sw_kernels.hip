@@ -30,9 +30,9 @@ using namespace swd;
 // =============================================================================================
 
 // Arena extent and validity of surface s (run by the extra blocks of k_setup_draws).
-SW_DEV uint32_t view_of_surface(const ViewExt* __restrict__ views, uint32_t num_views, uint32_t s)
+SW_DEV uint32_t view_of_surface(const ViewsArg& views, uint32_t s)
 {
-	uint32_t lo = 0, hi = num_views;
+	uint32_t lo = 0, hi = views.count;
 	while(hi - lo > 1)
 	{
 		const uint32_t mid = (lo + hi) >> 1;
@@ -44,24 +44,16 @@ SW_DEV uint32_t view_of_surface(const ViewExt* __restrict__ views, uint32_t num_
 	return lo;
 }
 
-// Arena extent, owning view and validation of surface s; also fills the block -> first surface
-// table for the surface blocks whose first texel lies in [end of s - 1, end of s).
-SW_DEV void prep_surface(const sw_surface_desc* __restrict__ surfaces, uint32_t s, uint32_t num_surfaces,
-	const ViewExt* __restrict__ views, uint32_t num_views, SurfExt* __restrict__ surf_ext, uint32_t* __restrict__ blk_surf,
-	uint64_t arena_texels, const MapPtrs& m, uint32_t* __restrict__ counters)
+// Whether a surface descriptor only reads inside the map's buffers and writes inside the arena.
+SW_DEV bool surface_desc_ok(const sw_surface_desc& d, uint64_t end, uint64_t arena_texels, const MapPtrs& m)
 {
-	const uint32_t view = view_of_surface(views, num_views, s);
-	const sw_surface_desc& d = surfaces[s];
-	const uint64_t start = views[view].v.surface_arena_offset + d.pixels_offset;
-	const uint64_t end = start + uint64_t(d.size[0]) * d.size[1];
-	surf_ext[s] = SurfExt{end, view, 0u};
-	bool bad = end > arena_texels || d.size[0] > 65536u || d.size[1] > 65536u;
+	bool ok = end <= arena_texels && d.size[0] <= 65536u && d.size[1] <= 65536u;
 	if((d.flags & SW_SURF_DIFFUSE_LAYER) &&
 		(d.texture_level >= m.num_levels || (m.levels[d.texture_level].flags & SW_TEX_LITE_LEVEL)))
-		bad = true;
+		ok = false;
 	if((d.flags & SW_SURF_EMISSIVE) && d.emissive_level != SW_NO_TEXTURE &&
 		(d.emissive_level >= m.num_levels || !(m.levels[d.emissive_level].flags & SW_TEX_LITE_LEVEL)))
-		bad = true;
+		ok = false;
 	if((d.flags & SW_SURF_DIFFUSE_LAYER) && d.lightmap_offset != 0 && d.size[0] > 0 && d.size[1] > 0)
 	{
 		// Every lightmap fetch (rows lv, lv + 1; columns lu, lu + 1) must stay inside the polygon's
@@ -72,14 +64,29 @@ SW_DEV void prep_surface(const sw_surface_desc* __restrict__ surfaces, uint32_t 
 			((uint64_t(d.size[0]) - 1 + d.lightmap_tc_shift[0]) >> sl) + 1 >= d.lightmap_size[0] ||
 			((uint64_t(d.size[1]) - 1 + d.lightmap_tc_shift[1]) >> sl) + 1 >= d.lightmap_size[1] ||
 			uint64_t(d.lightmap_offset) + uint64_t(d.lightmap_size[0]) * d.lightmap_size[1] > num_lm)
-			bad = true;
+			ok = false;
 	}
+	return ok;
+}
+
+// Arena extent, owning view and validation of surface s; also fills the block -> first surface
+// table for the surface blocks whose first texel lies in [end of s - 1, end of s).
+SW_DEV void prep_surface(const sw_surface_desc* __restrict__ surfaces, uint32_t s, uint32_t num_surfaces,
+	const ViewsArg& views, SurfExt* __restrict__ surf_ext, uint32_t* __restrict__ blk_surf,
+	uint64_t arena_texels, const MapPtrs& m, uint32_t* __restrict__ counters)
+{
+	const uint32_t view = view_of_surface(views, s);
+	const sw_surface_desc& d = surfaces[s];
+	const uint64_t start = views[view].v.surface_arena_offset + d.pixels_offset;
+	const uint64_t end = start + uint64_t(d.size[0]) * d.size[1];
+	surf_ext[s] = SurfExt{end, view, 0u};
+	bool bad = !surface_desc_ok(d, end, arena_texels, m);
 	uint64_t prev_end = 0;
 	if(s > 0)
 	{
 		// Arena ranges must follow descriptor order (the reference's bump allocator,
 		// partial_renderer.rs:1617-1618): compare with the predecessor's end.
-		const uint32_t pv = view_of_surface(views, num_views, s - 1);
+		const uint32_t pv = view_of_surface(views, s - 1);
 		const sw_surface_desc& p = surfaces[s - 1];
 		prev_end = views[pv].v.surface_arena_offset + p.pixels_offset + uint64_t(p.size[0]) * p.size[1];
 		if(prev_end > start)
@@ -414,32 +421,89 @@ SW_DEV typename CT::T emissive_texel(const MapPtrs& m, const sw_surface_desc& s,
 	}
 }
 
+
+// One block of surface texels (SURF_BLOCK consecutive arena texels).
 template <class CT>
-__global__ __launch_bounds__(SURF_BLOCK) void k_build_surfaces(MapPtrs m, const sw_surface_desc* __restrict__ surfaces,
-	uint32_t num_surfaces, const SurfExt* __restrict__ surf_ext, const uint32_t* __restrict__ blk_surf,
-	const ViewExt* __restrict__ views, const sw_light* __restrict__ lights, uint32_t num_lights,
-	const uint32_t* __restrict__ light_indices, uint32_t num_light_indices, typename CT::T* __restrict__ arena,
-	uint64_t arena_texels, const uint32_t* __restrict__ counters)
+SW_DEV void build_surfaces_block(uint32_t block, const MapPtrs& m, const SurfaceArgs& a, const ViewsArg& views)
 {
-	if(counters[COUNTER_ERROR] & ERROR_BAD_SURFACE)
-		return; // the host reports the invalid descriptor; nothing is written
-	const uint64_t idx = uint64_t(blockIdx.x) * SURF_BLOCK + threadIdx.x;
-	if(idx >= arena_texels)
-		return;
-	// The block's first surface comes from the table; a block spans few surfaces (most surfaces are
-	// larger than a block), so a short forward scan finds each texel's.
-	uint32_t s = blk_surf[blockIdx.x];
-	while(s < num_surfaces && surf_ext[s].end <= idx)
-		++s;
-	if(s >= num_surfaces)
-		return;
+	const sw_surface_desc* __restrict__ surfaces = a.surfaces;
+	const uint32_t num_surfaces = a.num_surfaces;
+	const SurfExt* __restrict__ surf_ext = a.surf_ext;
+	const sw_light* __restrict__ lights = a.lights;
+	const uint32_t* __restrict__ light_indices = a.light_indices;
+	const uint32_t num_lights = a.num_lights, num_light_indices = a.num_light_indices;
+	typename CT::T* __restrict__ arena = static_cast<typename CT::T*>(a.arena);
+	const uint64_t idx = uint64_t(block) * SURF_BLOCK + threadIdx.x;
+	uint32_t s;
+	uint64_t start;
+	uint32_t view_index = 0;
+	if(a.blk_surf)
+	{
+		if(idx >= a.arena_texels)
+			return;
+		// The block's first surface comes from the table; a block spans few surfaces (most surfaces
+		// are larger than a block), so a short forward scan finds each texel's.
+		s = a.blk_surf[block];
+		while(s < num_surfaces && surf_ext[s].end <= idx)
+			++s;
+		if(s >= num_surfaces)
+			return;
+		const sw_surface_desc& d = surfaces[s];
+		const SurfExt ext = surf_ext[s];
+		start = ext.end - uint64_t(d.size[0]) * d.size[1];
+		view_index = ext.view;
+	}
+	else
+	{
+		// One view, no table: a 64-ary search by the wave (one probe per lane and level) finds the
+		// last surface starting at or before the wave's first texel, then each lane scans forward.
+		// The descriptors were not validated ahead, so each lane validates its own surface.
+		const uint64_t base = views[0].v.surface_arena_offset;
+		const uint64_t wave_first = idx & ~uint64_t(63);
+		const uint32_t lane = threadIdx.x & 63;
+		uint32_t lo = 0, hi = num_surfaces;
+		while(hi - lo > 1)
+		{
+			const uint32_t step = (hi - lo + 63) / 64;
+			const uint32_t probe = lo + lane * step;
+			const bool le = probe < hi && base + surfaces[probe].pixels_offset <= wave_first;
+			const uint64_t mask = __ballot(le);
+			const uint32_t last = mask ? 63u - uint32_t(__builtin_clzll(mask)) : 0u;
+			lo = lo + last * step;
+			hi = std::min(lo + step, hi);
+		}
+		if(idx >= a.arena_texels)
+			return;
+		s = lo;
+		uint64_t end = 0;
+		for(;;)
+		{
+			if(s >= num_surfaces)
+				return;
+			const sw_surface_desc& d = surfaces[s];
+			start = base + d.pixels_offset;
+			end = start + uint64_t(d.size[0]) * d.size[1];
+			if(end > idx)
+				break;
+			++s;
+		}
+		if(idx < start)
+			return; // gap between surfaces
+		const sw_surface_desc& d = surfaces[s];
+		bool ok = surface_desc_ok(d, end, a.arena_texels, m);
+		if(s > 0)
+		{
+			const sw_surface_desc& p = surfaces[s - 1];
+			ok = ok && base + p.pixels_offset + uint64_t(p.size[0]) * p.size[1] <= start;
+		}
+		if(!ok)
+		{
+			atomicOr(&a.counters[COUNTER_ERROR], ERROR_BAD_SURFACE);
+			return;
+		}
+	}
 	const sw_surface_desc& d = surfaces[s];
-	const SurfExt ext = surf_ext[s];
-	const uint64_t size = uint64_t(d.size[0]) * d.size[1];
-	const uint64_t start = ext.end - size;
-	if(idx < start)
-		return; // gap between surfaces
-	const sw_view& view = views[ext.view].v;
+	const sw_view& view = views[view_index].v;
 	const V3 cam = v3(view.cam_pos[0], view.cam_pos[1], view.cam_pos[2]);
 	const uint32_t local = uint32_t(idx - start);
 	const uint32_t du = local % d.size[0];
@@ -456,6 +520,15 @@ __global__ __launch_bounds__(SURF_BLOCK) void k_build_surfaces(MapPtrs m, const 
 	if(emissive)
 		out = emissive_texel<CT>(m, d, d.emissive_blend, out, du, dv);
 	arena[idx] = out;
+}
+
+template <class CT>
+__global__ __launch_bounds__(SURF_BLOCK) void k_build_surfaces(MapPtrs m, SurfaceArgs a, const ViewsArg views,
+	const uint32_t* __restrict__ counters)
+{
+	if(counters[COUNTER_ERROR] & ERROR_BAD_SURFACE)
+		return; // the host reports the invalid descriptor; nothing is written
+	build_surfaces_block<CT>(blockIdx.x, m, a, views);
 }
 
 // =============================================================================================
@@ -710,8 +783,11 @@ struct SetupState
 	uint64_t next_span; // next free span record of this draw
 	uint64_t span_end;
 	bool overflow;
+	bool gap; // a part did not start on the row after the previous part's last
 	int32_t row_min, row_max, x_min, x_max;
 	uint64_t pixels;
+	int32_t row_first, row_next; // first emitted row, row after the last emitted one
+	uint32_t rows0; // rows of the first emitted part
 };
 
 // Rows a part covers after clipping (rasterizer.rs:316-317, :1000-1005).
@@ -735,6 +811,16 @@ SW_DEV void emit_part(const sw_draw& d, const PolyEq& e, uint32_t kind, const in
 		return; // no rows: the reference draws nothing for this part
 	if(st.num_parts >= MAX_PARTS_PER_DRAW)
 		return;
+	// The parts of a draw cover consecutive rows, so the draw's span records are one run indexed
+	// by row (the raster kernel relies on it; a violation is reported, never drawn).
+	if(st.num_parts == 0)
+	{
+		st.row_first = y_start_int;
+		st.rows0 = uint32_t(y_end_int - y_start_int);
+	}
+	else if(y_start_int != st.row_next)
+		st.gap = true;
+	st.row_next = y_end_int;
 	const int32_t y_start_delta = int_to_fixed16(y_start_int) + FIXED16_HALF - y_start;
 	const int32_t x_left0 = ls.x_start + fixed16_mul(y_start_delta, ls.dx_dy) + FIXED16_HALF;
 	const int32_t x_right0 = rs.x_start + fixed16_mul(y_start_delta, rs.dx_dy) + FIXED16_HALF;
@@ -1124,30 +1210,29 @@ SW_DEV uint64_t wave_sum(uint64_t v)
 	return v;
 }
 
-// LANES threads per draw (64: a wave per draw, for batches of many draws; 256: a block per draw,
-// for frames of few, large draws). Blocks past `setup_blocks` prepare the surfaces instead.
+
+// One block of draw setup. LANES threads per draw (64: a wave per draw, for batches of many draws;
+// 256: a block per draw, for frames of few, large draws).
 template <int LANES>
-__global__ __launch_bounds__(256) void k_setup_draws(const sw_draw* __restrict__ draws, uint32_t num_draws,
-	const int32_t* __restrict__ vdata, uint64_t num_vertex_ints, const ViewExt* __restrict__ views, uint32_t num_views,
-	PartRec* __restrict__ parts, SpanRec* __restrict__ spans, DrawRec* __restrict__ drec, uint32_t* __restrict__ counters,
-	uint64_t span_capacity, uint64_t arena_texels, uint64_t skybox_texels, uint64_t lite_texels,
-	const sw_surface_desc* __restrict__ surfaces, uint32_t num_surfaces, SurfExt* __restrict__ surf_ext,
-	uint32_t* __restrict__ blk_surf, MapPtrs m, uint32_t setup_blocks)
+SW_DEV void setup_draws_block(uint32_t block, const SetupArgs& a, const ViewsArg& views)
 {
-	if(blockIdx.x >= setup_blocks)
-	{
-		const uint32_t s = (blockIdx.x - setup_blocks) * blockDim.x + threadIdx.x;
-		if(s < num_surfaces)
-			prep_surface(surfaces, s, num_surfaces, views, num_views, surf_ext, blk_surf, arena_texels, m, counters);
-		return;
-	}
+	const sw_draw* __restrict__ draws = a.draws;
+	const uint32_t num_draws = a.num_draws;
+	const int32_t* __restrict__ vdata = a.vdata;
+	const uint64_t num_vertex_ints = a.num_vertex_ints;
+	PartRec* __restrict__ parts = a.parts;
+	SpanRec* __restrict__ spans = a.spans;
+	DrawRec* __restrict__ drec = a.drec;
+	uint32_t* __restrict__ counters = a.counters;
+	const uint64_t span_capacity = a.span_capacity, arena_texels = a.arena_texels, skybox_texels = a.skybox_texels,
+				   lite_texels = a.lite_texels;
 	constexpr uint32_t GROUPS = 256 / LANES; // draws per block
 	__shared__ uint32_t group_rows[GROUPS];
 	__shared__ unsigned long long block_base;
 	__shared__ int32_t red_i[4][4];
 	__shared__ unsigned long long red_p[4];
 	const uint32_t group = threadIdx.x / LANES, lane = threadIdx.x % LANES;
-	const uint32_t di = blockIdx.x * GROUPS + group;
+	const uint32_t di = block * GROUPS + group;
 	const bool active = di < num_draws;
 
 	sw_draw d;
@@ -1157,7 +1242,7 @@ __global__ __launch_bounds__(256) void k_setup_draws(const sw_draw* __restrict__
 	if(active)
 	{
 		d = draws[di];
-		uint32_t lo = 0, hi = num_views;
+		uint32_t lo = 0, hi = views.count;
 		while(hi - lo > 1)
 		{
 			const uint32_t mid = (lo + hi) >> 1;
@@ -1195,6 +1280,10 @@ __global__ __launch_bounds__(256) void k_setup_draws(const sw_draw* __restrict__
 	st.lanes = LANES;
 	st.num_parts = 0;
 	st.overflow = false;
+	st.gap = false;
+	st.row_first = 0;
+	st.row_next = 0;
+	st.rows0 = 0;
 	st.row_min = INT32_MAX;
 	st.row_max = INT32_MIN;
 	st.x_min = INT32_MAX;
@@ -1202,29 +1291,42 @@ __global__ __launch_bounds__(256) void k_setup_draws(const sw_draw* __restrict__
 	st.pixels = 0;
 	const uint32_t part_slot_base = di * MAX_PARTS_PER_DRAW;
 
-	// pass 1: rows of every part (no divisions), one span allocation per block
 	uint32_t rows = 0;
-	if(active && valid)
-		rows = d.kind == SW_DRAW_POLYGON ? walk_polygon<false>(d, v, clip, parts, spans, part_slot_base, st) : triangle_rows(v, clip);
-	if(lane == 0)
-		group_rows[group] = rows;
-	__syncthreads();
-	if(threadIdx.x == 0)
+	uint64_t base = 0;
+	if(a.span_stride)
 	{
-		unsigned long long total = 0;
-		for(uint32_t g = 0; g < GROUPS; ++g)
-			total += group_rows[g];
-		block_base = total ? atomicAdd(reinterpret_cast<unsigned long long*>(&counters[COUNTER_SPANS_LO]), total) : 0ull;
+		// static span storage: a draw's rows never exceed its clip rect's height <= the stride
+		base = uint64_t(di) * a.span_stride;
+		st.next_span = base;
+		st.overflow = false;
+		st.span_end = base + a.span_stride;
 	}
-	__syncthreads();
-	uint64_t base = block_base;
-	for(uint32_t g = 0; g < group; ++g)
-		base += group_rows[g];
-	st.next_span = base;
-	st.overflow = base + rows > span_capacity;
-	st.span_end = st.overflow ? 0 : base + rows;
-	if(active && st.overflow && lane == 0 && rows)
-		atomicOr(&counters[COUNTER_ERROR], ERROR_SPAN_OVERFLOW);
+	else
+	{
+		// pass 1: rows of every part (no divisions), one span allocation per block
+		if(active && valid)
+			rows = d.kind == SW_DRAW_POLYGON ? walk_polygon<false>(d, v, clip, parts, spans, part_slot_base, st)
+											 : triangle_rows(v, clip);
+		if(lane == 0)
+			group_rows[group] = rows;
+		__syncthreads();
+		if(threadIdx.x == 0)
+		{
+			unsigned long long total = 0;
+			for(uint32_t g = 0; g < GROUPS; ++g)
+				total += group_rows[g];
+			block_base = total ? atomicAdd(reinterpret_cast<unsigned long long*>(&counters[COUNTER_SPANS_LO]), total) : 0ull;
+		}
+		__syncthreads();
+		base = block_base;
+		for(uint32_t g = 0; g < group; ++g)
+			base += group_rows[g];
+		st.next_span = base;
+		st.overflow = base + rows > span_capacity;
+		st.span_end = st.overflow ? 0 : base + rows;
+		if(active && st.overflow && lane == 0 && rows)
+			atomicOr(&counters[COUNTER_ERROR], ERROR_SPAN_OVERFLOW);
+	}
 
 	// pass 2: parts and spans
 	if(active && valid && !st.overflow)
@@ -1264,24 +1366,60 @@ __global__ __launch_bounds__(256) void k_setup_draws(const sw_draw* __restrict__
 	{
 		const bool reads = d.blend == SW_BLEND_AVERAGE || d.blend == SW_BLEND_ADDITIVE || d.blend == SW_BLEND_ALPHA_TEST ||
 			d.blend == SW_BLEND_ALPHA_BLEND;
+		if(st.gap)
+			atomicOr(&counters[COUNTER_ERROR], ERROR_PART_GAP);
+		const uint32_t kind = d.kind == SW_DRAW_TRIANGLE ? 3u : (d.mode == SW_TC_LINE_Z ? 1u : (d.mode == SW_TC_AFFINE ? 2u : 0u));
 		DrawRec r;
-		r.num_parts = st.overflow ? 0 : st.num_parts;
+		r.num_parts = (st.overflow || st.gap) ? 0 : st.num_parts;
+		r.span_base = uint32_t(base);
+		r.row_first = st.row_first;
+		r.rows0 = st.rows0;
+		r.pad = 0;
 		r.row_min = row_min;
 		r.row_max = row_max;
 		r.x_min = x_min;
 		r.x_max = x_max;
 		r.pixels = uint32_t(pixels);
-		r.stat = (d.kind == SW_DRAW_TRIANGLE ? DRAW_STAT_TRIANGLE : 0u) | (reads ? DRAW_STAT_READS_DST : 0u);
-		r.pad = 0;
+		r.stat = (d.kind == SW_DRAW_TRIANGLE ? DRAW_STAT_TRIANGLE : 0u) | (reads ? DRAW_STAT_READS_DST : 0u) |
+			(kind << DRAW_STAT_KIND_SHIFT);
+		r.rows = a.span_stride ? uint32_t(st.next_span - base) : rows;
 		drec[di] = r;
 	}
 }
 
+// Draw setup; blocks past the draws' validate the surfaces and fill the surface block table
+// (when the descriptors are device-resident, so the host cannot).
+template <int LANES>
+__global__ __launch_bounds__(256) void k_setup_draws(SetupArgs a, const ViewsArg views, const sw_surface_desc* __restrict__ surfaces,
+	uint32_t num_surfaces, SurfExt* __restrict__ surf_ext, uint32_t* __restrict__ blk_surf, MapPtrs m)
+{
+	if(blockIdx.x >= a.setup_blocks)
+	{
+		const uint32_t s = (blockIdx.x - a.setup_blocks) * blockDim.x + threadIdx.x;
+		if(s < num_surfaces)
+			prep_surface(surfaces, s, num_surfaces, views, surf_ext, blk_surf, a.arena_texels, m, a.counters);
+		return;
+	}
+	setup_draws_block<LANES>(blockIdx.x, a, views);
+}
+
+// Draw setup and surface texels in one launch (the host prepared and validated the surfaces): the
+// two are independent, so they run side by side.
+template <int LANES, class CT>
+__global__ __launch_bounds__(256) void k_setup_and_surfaces(SetupArgs a, const ViewsArg views, SurfaceArgs sa, MapPtrs m)
+{
+	if(blockIdx.x < a.setup_blocks)
+		setup_draws_block<LANES>(blockIdx.x, a, views);
+	else
+		build_surfaces_block<CT>(blockIdx.x - a.setup_blocks, m, sa, views);
+}
+
 // One block after the raster kernel: sums the per-draw statistics into the result block and clears
 // the live block for the next batch (a kernel boundary orders it after every other kernel's writes).
-__global__ __launch_bounds__(1024) void k_finish(const DrawRec* __restrict__ drec, uint32_t num_draws, uint32_t* __restrict__ counters)
+__global__ __launch_bounds__(1024) void k_finish(const DrawRec* __restrict__ drec, uint32_t num_draws, uint32_t* __restrict__ counters,
+	uint32_t* __restrict__ result)
 {
-	unsigned long long poly = 0, tri = 0, blend = 0, parts = 0;
+	unsigned long long poly = 0, tri = 0, blend = 0, parts = 0, spans = 0;
 	for(uint32_t i = threadIdx.x; i < num_draws; i += blockDim.x)
 	{
 		const DrawRec r = drec[i];
@@ -1292,12 +1430,14 @@ __global__ __launch_bounds__(1024) void k_finish(const DrawRec* __restrict__ dre
 		if(r.stat & DRAW_STAT_READS_DST)
 			blend += r.pixels;
 		parts += r.num_parts;
+		spans += r.rows;
 	}
 	poly = wave_sum(poly);
 	tri = wave_sum(tri);
 	blend = wave_sum(blend);
 	parts = wave_sum(parts);
-	__shared__ unsigned long long red[16][4];
+	spans = wave_sum(spans);
+	__shared__ unsigned long long red[16][5];
 	const uint32_t wave = threadIdx.x >> 6;
 	if((threadIdx.x & 63) == 0)
 	{
@@ -1305,19 +1445,20 @@ __global__ __launch_bounds__(1024) void k_finish(const DrawRec* __restrict__ dre
 		red[wave][1] = tri;
 		red[wave][2] = blend;
 		red[wave][3] = parts;
+		red[wave][4] = spans;
 	}
 	__syncthreads();
 	if(threadIdx.x == 0)
 	{
-		unsigned long long sum[4] = {0, 0, 0, 0};
+		unsigned long long sum[5] = {0, 0, 0, 0, 0};
 		for(uint32_t w = 0; w < (blockDim.x + 63) / 64; ++w)
-			for(int k = 0; k < 4; ++k)
+			for(int k = 0; k < 5; ++k)
 				sum[k] += red[w][k];
-		uint32_t* res = counters + COUNTER_RESULT;
+		uint32_t res[COUNTER_COUNT];
 		for(int k = 0; k < COUNTER_COUNT; ++k)
 			res[k] = 0;
-		res[COUNTER_SPANS_LO] = counters[COUNTER_SPANS_LO];
-		res[COUNTER_SPANS_HI] = counters[COUNTER_SPANS_HI];
+		res[COUNTER_SPANS_LO] = uint32_t(sum[4]);
+		res[COUNTER_SPANS_HI] = uint32_t(sum[4] >> 32);
 		res[COUNTER_POLY_PIXELS_LO] = uint32_t(sum[0]);
 		res[COUNTER_POLY_PIXELS_HI] = uint32_t(sum[0] >> 32);
 		res[COUNTER_TRI_PIXELS_LO] = uint32_t(sum[1]);
@@ -1327,17 +1468,20 @@ __global__ __launch_bounds__(1024) void k_finish(const DrawRec* __restrict__ dre
 		res[COUNTER_PARTS] = uint32_t(sum[3]);
 		res[COUNTER_ERROR] = counters[COUNTER_ERROR];
 		for(int k = 0; k < COUNTER_COUNT; ++k)
+		{
+			result[k] = res[k];
 			counters[k] = 0;
+		}
 	}
 }
 
 // =============================================================================================
-// Tiled raster: one wave per 64 x ROWS tile, colour kept in LDS
+// Tiled raster: one wave per 64 x ROWS tile, one pixel column per lane, colour in registers
 // =============================================================================================
 
-// Texel address of one pixel of a polygon part: perspective (rasterizer.rs:425-452) or linear
-// tc (line-z :559-579, affine :705-727); -1 if the texel lies outside the texture (skipped, like the
-// reference's unchecked fetch never going out of range after its clamps).
+// Texel address of one pixel of a polygon span: perspective (rasterizer.rs:425-452) or linear tc
+// (line-z :559-579, affine :705-727); -1 if the texel lies outside the texture (skipped; the
+// reference's clamps keep its unchecked fetches inside).
 template <int KIND>
 SW_DEV int32_t polygon_texel_index(const int32_t* sv, int32_t j, uint32_t tw, uint32_t th)
 {
@@ -1361,115 +1505,29 @@ SW_DEV int32_t polygon_texel_index(const int32_t* sv, int32_t j, uint32_t tw, ui
 	return (u < tw && v < th) ? int32_t(u + v * tw) : -1;
 }
 
-// One polygon part over the tile's rows. The texel addresses of all rows are computed first and
-// all fetches issued together, then blended into the LDS tile in row order.
-template <class CT, int ROWS, int KIND>
-SW_DEV void raster_polygon_part(const SpanRec& mine, int32_t ra, int32_t rb, int32_t y0, int32_t x, uint32_t blend,
-	uint32_t tw, uint32_t th, const typename CT::T* __restrict__ tex, typename CT::T (*tile)[64])
-{
-	typedef typename CT::T C;
-	int32_t tix[ROWS];
-#pragma unroll
-	for(int r = 0; r < ROWS; ++r)
-	{
-		tix[r] = -1;
-		const int32_t y = y0 + r;
-		if(y < ra || y >= rb)
-			continue;
-		const int sl = y - ra;
-		const int32_t sx0 = __builtin_amdgcn_readlane(mine.x0, sl);
-		const int32_t sx1 = __builtin_amdgcn_readlane(mine.x1, sl);
-		int32_t sv[6];
-#pragma unroll
-		for(int k = 0; k < (KIND == 0 ? 6 : 4); ++k)
-			sv[k] = __builtin_amdgcn_readlane(mine.v[k], sl);
-		if(x >= sx0 && x < sx1)
-			tix[r] = polygon_texel_index<KIND>(sv, x - sx0, tw, th);
-	}
-	C texel[ROWS];
-#pragma unroll
-	for(int r = 0; r < ROWS; ++r)
-		texel[r] = tix[r] >= 0 ? tex[tix[r]] : C(0);
-#pragma unroll
-	for(int r = 0; r < ROWS; ++r)
-		if(tix[r] >= 0)
-			tile[r][threadIdx.x & 63] = write_into_framebuffer<CT>(blend, tile[r][threadIdx.x & 63], texel[r]);
-}
+// Wave lane exchange: pull `v` from lane `src` / push `v` to lane `dst` (ds_bpermute / ds_permute).
+SW_DEV int32_t lane_pull(int32_t v, uint32_t src) { return __builtin_amdgcn_ds_bpermute(int32_t(src << 2), v); }
+SW_DEV int32_t lane_push(int32_t v, uint32_t dst) { return __builtin_amdgcn_ds_permute(int32_t(dst << 2), v); }
+SW_DEV int32_t rl(int32_t v, uint32_t lane) { return __builtin_amdgcn_readlane(v, lane); }
 
-// One triangle part (rasterizer.rs:1207-1290): fixed16 tc and light, (texel * light) >> 16, blend.
-template <class CT, int ROWS>
-SW_DEV void raster_triangle_part(const SpanRec& mine, const PartRec& pr, int32_t ra, int32_t rb, int32_t y0, int32_t x,
-	uint32_t blend, uint32_t tw, uint32_t th, const uint32_t* __restrict__ tex, typename CT::T (*tile)[64])
-{
-	int32_t tix[ROWS];
-#pragma unroll
-	for(int r = 0; r < ROWS; ++r)
-	{
-		tix[r] = -1;
-		const int32_t y = y0 + r;
-		if(y < ra || y >= rb)
-			continue;
-		const int sl = y - ra;
-		const int32_t sx0 = __builtin_amdgcn_readlane(mine.x0, sl);
-		const int32_t sx1 = __builtin_amdgcn_readlane(mine.x1, sl);
-		const int32_t s0 = __builtin_amdgcn_readlane(mine.v[0], sl);
-		const int32_t s1 = __builtin_amdgcn_readlane(mine.v[1], sl);
-		if(x >= sx0 && x < sx1)
-		{
-			const int32_t j = x - sx0;
-			const uint32_t u = uint32_t(fixed16_floor_to_int(wadd(s0, wmul(j, pr.d[0]))));
-			const uint32_t v = uint32_t(fixed16_floor_to_int(wadd(s1, wmul(j, pr.d[1]))));
-			if(u < tw && v < th)
-				tix[r] = int32_t(u + v * tw);
-		}
-	}
-	uint32_t texel[ROWS];
-#pragma unroll
-	for(int r = 0; r < ROWS; ++r)
-		texel[r] = tix[r] >= 0 ? tex[tix[r]] : 0u;
-#pragma unroll
-	for(int r = 0; r < ROWS; ++r)
-	{
-		if(tix[r] < 0)
-			continue;
-		const int sl = y0 + r - ra;
-		const int32_t j = x - __builtin_amdgcn_readlane(mine.x0, sl);
-		const int32_t lr = wadd(__builtin_amdgcn_readlane(mine.v[2], sl), wmul(j, pr.d[2]));
-		const int32_t lg = wadd(__builtin_amdgcn_readlane(mine.v[3], sl), wmul(j, pr.d[3]));
-		const int32_t lb = wadd(__builtin_amdgcn_readlane(mine.v[4], sl), wmul(j, pr.d[4]));
-		const CI tc = Color32T::to_ci(texel[r]);
-		const CI lit = CI{wmul(tc.b, lb) >> 16, wmul(tc.g, lg) >> 16, wmul(tc.r, lr) >> 16, 0};
-		typename CT::T& dst = tile[r][threadIdx.x & 63];
-		switch(blend)
-		{
-		case SW_BLEND_NONE: dst = CT::from_ci(lit); break;
-		case SW_BLEND_AVERAGE: dst = CT::average(dst, CT::from_ci(lit)); break;
-		case SW_BLEND_ADDITIVE: dst = CT::saturated_sum(dst, CT::from_ci(lit)); break;
-		case SW_BLEND_ALPHA_TEST:
-			if(Color32T::test_alpha(texel[r]))
-				dst = CT::from_ci(lit);
-			break;
-		case SW_BLEND_ALPHA_BLEND: dst = alpha_blend_ci<CT>(dst, lit, Color32T::get_alpha(texel[r])); break;
-		default: break;
-		}
-	}
-}
-
+// The tile's draws are taken 64 at a time in draw order: a ballot bins them, the hits are
+// compacted into the low lanes, which load their draw's parameters at once. Then GD draws at a
+// time (GD * ROWS = 16 (draw, row) items): one lane per item loads its span record, every lane
+// computes the texel address of its pixel for all 16 items, issues the 16 fetches together and
+// blends them in painter's order into its ROWS colours.
 template <class CT, int ROWS>
 __global__ __launch_bounds__(256) void k_raster_tiles(MapPtrs m, const sw_draw* __restrict__ draws,
 	const DrawRec* __restrict__ drec, const PartRec* __restrict__ parts, const SpanRec* __restrict__ spans,
-	const ViewExt* __restrict__ views, uint32_t num_views, const typename CT::T* __restrict__ arena,
-	const typename CT::T* __restrict__ skybox, typename CT::T* __restrict__ fb, uint32_t total_tiles,
-	const uint32_t* __restrict__ counters)
+	const ViewsArg views, const typename CT::T* __restrict__ arena, const typename CT::T* __restrict__ skybox,
+	typename CT::T* __restrict__ fb, uint32_t total_tiles, const uint32_t* __restrict__ counters)
 {
 	typedef typename CT::T C;
-	__shared__ C tiles[4][ROWS][64];
+	constexpr int GD = ROWS >= 16 ? 1 : 16 / ROWS;
 	const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-	C(*tile)[64] = tiles[wave];
 	const uint32_t t = blockIdx.x * 4 + wave;
 	if(t >= total_tiles || (counters[COUNTER_ERROR] & ERROR_SPAN_OVERFLOW))
 		return;
-	uint32_t lo = 0, hi = num_views;
+	uint32_t lo = 0, hi = views.count;
 	while(hi - lo > 1)
 	{
 		const uint32_t mid = (lo + hi) >> 1;
@@ -1489,6 +1547,7 @@ __global__ __launch_bounds__(256) void k_raster_tiles(MapPtrs m, const sw_draw* 
 	C* vfb = fb + ve.v.fb_offset;
 	const C bg = C(ve.v.background_color);
 
+	C acc[ROWS];
 #pragma unroll
 	for(int r = 0; r < ROWS; ++r)
 	{
@@ -1496,49 +1555,181 @@ __global__ __launch_bounds__(256) void k_raster_tiles(MapPtrs m, const sw_draw* 
 		C c = bg;
 		if(!ve.v.clear_background && x < W && y < H)
 			c = vfb[uint64_t(y) * uint32_t(W) + uint32_t(x)];
-		tile[r][lane] = c;
+		acc[r] = c;
 	}
 
 	const C* view_arena = arena + ve.v.surface_arena_offset;
 	const uint32_t first_draw = ve.v.first_draw, num_draws = ve.v.num_draws;
+	const uint64_t lanes_below = (1ull << lane) - 1ull;
 	for(uint32_t base = 0; base < num_draws; base += 64)
 	{
 		// ordered binning: lane i tests draw base + i against this tile
+		const uint32_t di = first_draw + base + lane;
+		DrawRec r = DrawRec{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 		bool hit = false;
 		if(base + lane < num_draws)
 		{
-			const DrawRec r = drec[first_draw + base + lane];
+			r = drec[di];
 			hit = r.num_parts > 0 && r.row_min < y0 + ROWS && r.row_max > y0 && r.x_min < x0 + 64 && r.x_max > x0;
 		}
-		uint64_t mask = __ballot(hit);
-		while(mask)
+		const uint64_t mask = __ballot(hit);
+		if(!mask)
+			continue;
+		const uint32_t nh = uint32_t(__popcll(mask));
+		// compact the hits into lanes 0 .. nh - 1, in draw order (a permutation of the lanes)
+		const uint32_t dst = hit ? uint32_t(__popcll(mask & lanes_below)) : nh + uint32_t(__popcll(~mask & lanes_below));
+		const uint32_t h_di = uint32_t(lane_push(int32_t(di), dst));
+		const int32_t h_row_first = lane_push(r.row_first, dst);
+		const int32_t h_rows = lane_push(int32_t(r.rows), dst);
+		const int32_t h_span_base = lane_push(int32_t(r.span_base), dst);
+		const int32_t h_rows0 = lane_push(int32_t(r.rows0), dst);
+		const int32_t h_kind = lane_push(int32_t(r.stat >> DRAW_STAT_KIND_SHIFT), dst);
+		// per-draw parameters, one draw per lane
+		int32_t h_blend = 0, h_tw = 1, h_th = 1;
+		uint64_t h_tex = 0;
+		int32_t pd[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+		if(lane < nh)
 		{
-			const uint32_t j = uint32_t(__builtin_ctzll(mask));
-			mask &= mask - 1;
-			const uint32_t di = first_draw + base + j;
-			const uint32_t num_parts = drec[di].num_parts;
-			const sw_draw& d = draws[di];
-			const uint32_t blend = d.blend;
-			const uint32_t tw = uint32_t(d.texture_size[0]), th = uint32_t(d.texture_size[1]);
-			for(uint32_t p = 0; p < num_parts; ++p)
+			const sw_draw& d = draws[h_di];
+			h_blend = int32_t(d.blend);
+			h_tw = d.texture_size[0];
+			h_th = d.texture_size[1];
+			if(h_kind == 3)
 			{
-				const PartRec pr = parts[di * MAX_PARTS_PER_DRAW + p];
-				const int32_t ra = imax32(pr.row0, y0), rb = imin32(pr.row1, y0 + ROWS);
-				if(ra >= rb)
-					continue;
-				// the part's span records for the tile's rows, one per lane, broadcast with readlane
-				SpanRec mine = SpanRec{0, 0, {0, 0, 0, 0, 0, 0}};
-				if(int32_t(lane) < rb - ra)
-					mine = spans[pr.span_base + uint32_t(ra - pr.row0) + lane];
-				if(pr.kind == 3)
-					raster_triangle_part<CT, ROWS>(mine, pr, ra, rb, y0, x, blend, tw, th, m.lite + d.texture_offset, tile);
-				else
+				h_tex = reinterpret_cast<uint64_t>(m.lite + d.texture_offset);
+				const PartRec& p0 = parts[h_di * MAX_PARTS_PER_DRAW];
+				const PartRec& p1 = parts[h_di * MAX_PARTS_PER_DRAW + 1];
+#pragma unroll
+				for(int i = 0; i < 5; ++i)
 				{
-					const C* tex = d.texture_source == SW_TEXSRC_SKYBOX ? skybox + d.texture_offset : view_arena + d.texture_offset;
-					if(pr.kind == 0)
-						raster_polygon_part<CT, ROWS, 0>(mine, ra, rb, y0, x, blend, tw, th, tex, tile);
+					pd[i] = p0.d[i];
+					pd[5 + i] = p1.d[i];
+				}
+			}
+			else
+				h_tex = reinterpret_cast<uint64_t>(
+					d.texture_source == SW_TEXSRC_SKYBOX ? skybox + d.texture_offset : view_arena + d.texture_offset);
+		}
+		const int32_t h_tex_lo = int32_t(uint32_t(h_tex)), h_tex_hi = int32_t(uint32_t(h_tex >> 32));
+
+		for(uint32_t kg = 0; kg < nh; kg += GD)
+		{
+			// lane L < GD * ROWS loads the span record of draw kg + L / ROWS at tile row L % ROWS
+			const uint32_t kq = (kg + lane / ROWS) & 63u;
+			const int32_t q_row = y0 + int32_t(lane % ROWS) - lane_pull(h_row_first, kq);
+			const int32_t q_rows = lane_pull(h_rows, kq);
+			const int32_t q_base = lane_pull(h_span_base, kq);
+			SpanRec mine = SpanRec{0, 0, {0, 0, 0, 0, 0, 0}};
+			if(lane < uint32_t(GD * ROWS) && kg + lane / ROWS < nh && q_row >= 0 && q_row < q_rows)
+				mine = spans[uint32_t(q_base) + uint32_t(q_row)];
+
+			int32_t tix[GD][ROWS];
+#pragma unroll
+			for(int g = 0; g < GD; ++g)
+			{
+				const uint32_t k = kg + uint32_t(g);
+				const uint32_t kk = k < nh ? k : 0u;
+				const int32_t kind = rl(h_kind, kk);
+				const uint32_t tw = uint32_t(rl(h_tw, kk)), th = uint32_t(rl(h_th, kk));
+				const int32_t part_row0 = rl(h_row_first, kk), rows0 = rl(h_rows0, kk);
+#pragma unroll
+				for(int rr = 0; rr < ROWS; ++rr)
+				{
+					tix[g][rr] = -1;
+					if(k >= nh)
+						continue;
+					const int sl = g * ROWS + rr;
+					const int32_t sx0 = rl(mine.x0, sl), sx1 = rl(mine.x1, sl);
+					if(sx0 >= sx1 || sx1 <= x0 || sx0 >= x0 + 64)
+						continue;
+					int32_t sv[6];
+#pragma unroll
+					for(int i = 0; i < 6; ++i)
+						sv[i] = rl(mine.v[i], sl);
+					if(x < sx0 || x >= sx1)
+						continue;
+					const int32_t j = x - sx0;
+					if(kind == 3)
+					{
+						const bool upper = y0 + rr - part_row0 >= rows0;
+						const int32_t d0 = upper ? rl(pd[5], kk) : rl(pd[0], kk);
+						const int32_t d1 = upper ? rl(pd[6], kk) : rl(pd[1], kk);
+						const uint32_t u = uint32_t(fixed16_floor_to_int(wadd(sv[0], wmul(j, d0))));
+						const uint32_t v = uint32_t(fixed16_floor_to_int(wadd(sv[1], wmul(j, d1))));
+						if(u < tw && v < th)
+							tix[g][rr] = int32_t(u + v * tw);
+					}
+					else if(kind == 0)
+						tix[g][rr] = polygon_texel_index<0>(sv, j, tw, th);
 					else
-						raster_polygon_part<CT, ROWS, 1>(mine, ra, rb, y0, x, blend, tw, th, tex, tile);
+						tix[g][rr] = polygon_texel_index<1>(sv, j, tw, th);
+				}
+			}
+
+			C texel[GD][ROWS];
+#pragma unroll
+			for(int g = 0; g < GD; ++g)
+			{
+				const uint32_t k = kg + uint32_t(g);
+				const uint32_t kk = k < nh ? k : 0u;
+				const bool tri = rl(h_kind, kk) == 3;
+				const uint64_t tex = uint64_t(uint32_t(rl(h_tex_lo, kk))) | (uint64_t(uint32_t(rl(h_tex_hi, kk))) << 32);
+#pragma unroll
+				for(int rr = 0; rr < ROWS; ++rr)
+				{
+					C tv = 0;
+					if(tix[g][rr] >= 0)
+						tv = tri ? C(reinterpret_cast<const uint32_t*>(tex)[tix[g][rr]]) : reinterpret_cast<const C*>(tex)[tix[g][rr]];
+					texel[g][rr] = tv;
+				}
+			}
+
+#pragma unroll
+			for(int g = 0; g < GD; ++g)
+			{
+				const uint32_t k = kg + uint32_t(g);
+				if(k >= nh)
+					continue;
+				const uint32_t blend = uint32_t(rl(h_blend, k));
+				if(rl(h_kind, k) != 3)
+				{
+#pragma unroll
+					for(int rr = 0; rr < ROWS; ++rr)
+						if(tix[g][rr] >= 0)
+							acc[rr] = write_into_framebuffer<CT>(blend, acc[rr], texel[g][rr]);
+					continue;
+				}
+				const int32_t part_row0 = rl(h_row_first, k), rows0 = rl(h_rows0, k);
+#pragma unroll
+				for(int rr = 0; rr < ROWS; ++rr)
+				{
+					if(tix[g][rr] < 0)
+						continue;
+					const int sl = g * ROWS + rr;
+					const bool upper = y0 + rr - part_row0 >= rows0;
+					const int32_t j = x - rl(mine.x0, sl);
+					const int32_t dr = upper ? rl(pd[7], k) : rl(pd[2], k);
+					const int32_t dg = upper ? rl(pd[8], k) : rl(pd[3], k);
+					const int32_t db = upper ? rl(pd[9], k) : rl(pd[4], k);
+					const int32_t lr = wadd(rl(mine.v[2], sl), wmul(j, dr));
+					const int32_t lg = wadd(rl(mine.v[3], sl), wmul(j, dg));
+					const int32_t lb = wadd(rl(mine.v[4], sl), wmul(j, db));
+					const uint32_t tv = uint32_t(texel[g][rr]);
+					const CI tc = Color32T::to_ci(tv);
+					const CI lit = CI{wmul(tc.b, lb) >> 16, wmul(tc.g, lg) >> 16, wmul(tc.r, lr) >> 16, 0};
+					C& dst_c = acc[rr];
+					switch(blend)
+					{
+					case SW_BLEND_NONE: dst_c = CT::from_ci(lit); break;
+					case SW_BLEND_AVERAGE: dst_c = CT::average(dst_c, CT::from_ci(lit)); break;
+					case SW_BLEND_ADDITIVE: dst_c = CT::saturated_sum(dst_c, CT::from_ci(lit)); break;
+					case SW_BLEND_ALPHA_TEST:
+						if(Color32T::test_alpha(tv))
+							dst_c = CT::from_ci(lit);
+						break;
+					case SW_BLEND_ALPHA_BLEND: dst_c = alpha_blend_ci<CT>(dst_c, lit, Color32T::get_alpha(tv)); break;
+					default: break;
+					}
 				}
 			}
 		}
@@ -1551,7 +1742,7 @@ __global__ __launch_bounds__(256) void k_raster_tiles(MapPtrs m, const sw_draw* 
 		{
 			const int32_t y = y0 + r;
 			if(y < H)
-				vfb[uint64_t(y) * uint32_t(W) + uint32_t(x)] = tile[r][lane];
+				vfb[uint64_t(y) * uint32_t(W) + uint32_t(x)] = acc[r];
 		}
 	}
 }
@@ -1621,45 +1812,61 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t value)
 namespace swk
 {
 
-void launch_setup(hipStream_t st, uint32_t lanes, const sw_draw* draws, uint32_t nd, const int32_t* vdata, uint64_t nvi,
-	const ViewExt* views, uint32_t nv, PartRec* parts, SpanRec* spans, DrawRec* drec, uint32_t* counters,
-	uint64_t span_capacity, uint64_t arena_texels, uint64_t skybox_texels, uint64_t lite_texels,
-	const sw_surface_desc* surfaces, uint32_t ns, SurfExt* surf_ext, uint32_t* blk_surf, const MapPtrs& m)
+void launch_setup(hipStream_t st, uint32_t lanes, SetupArgs a, const ViewsArg& views, const sw_surface_desc* surfaces,
+	uint32_t ns, SurfExt* surf_ext, uint32_t* blk_surf, const MapPtrs& m)
 {
 	const uint32_t groups = lanes == 256 ? 1u : 4u;
-	const uint32_t setup_blocks = (nd + groups - 1) / groups;
+	a.setup_blocks = (a.num_draws + groups - 1) / groups;
 	const uint32_t prep_blocks = (ns + 255) / 256;
-	const uint32_t blocks = setup_blocks + prep_blocks;
+	const uint32_t blocks = a.setup_blocks + prep_blocks;
 	if(blocks == 0)
 		return;
 	if(lanes == 256)
-		hipLaunchKernelGGL(k_setup_draws<256>, dim3(blocks), dim3(256), 0, st, draws, nd, vdata, nvi, views, nv, parts, spans,
-			drec, counters, span_capacity, arena_texels, skybox_texels, lite_texels, surfaces, ns, surf_ext, blk_surf, m,
-			setup_blocks);
+		hipLaunchKernelGGL(k_setup_draws<256>, dim3(blocks), dim3(256), 0, st, a, views, surfaces, ns, surf_ext, blk_surf, m);
 	else
-		hipLaunchKernelGGL(k_setup_draws<64>, dim3(blocks), dim3(256), 0, st, draws, nd, vdata, nvi, views, nv, parts, spans,
-			drec, counters, span_capacity, arena_texels, skybox_texels, lite_texels, surfaces, ns, surf_ext, blk_surf, m,
-			setup_blocks);
+		hipLaunchKernelGGL(k_setup_draws<64>, dim3(blocks), dim3(256), 0, st, a, views, surfaces, ns, surf_ext, blk_surf, m);
 }
 
-void launch_build_surfaces(hipStream_t st, uint32_t color_bits, const MapPtrs& m, const sw_surface_desc* s, uint32_t n,
-	const SurfExt* surf_ext, const uint32_t* blk_surf, const ViewExt* views, const sw_light* lights, uint32_t nl,
-	const uint32_t* lidx, uint32_t nli, void* arena, uint64_t arena_texels, const uint32_t* counters)
+template <int LANES>
+static void launch_fused_lanes(hipStream_t st, uint32_t color_bits, uint32_t blocks, const SetupArgs& a, const ViewsArg& views,
+	const SurfaceArgs& sa, const MapPtrs& m)
 {
-	if(arena_texels == 0 || n == 0)
-		return;
-	const uint64_t blocks = (arena_texels + SURF_BLOCK - 1) / SURF_BLOCK;
 	if(color_bits == 64)
-		hipLaunchKernelGGL(k_build_surfaces<Color64T>, dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, s, n, surf_ext,
-			blk_surf, views, lights, nl, lidx, nli, static_cast<uint64_t*>(arena), arena_texels, counters);
+		hipLaunchKernelGGL((k_setup_and_surfaces<LANES, Color64T>), dim3(blocks), dim3(256), 0, st, a, views, sa, m);
 	else
-		hipLaunchKernelGGL(k_build_surfaces<Color32T>, dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, s, n, surf_ext,
-			blk_surf, views, lights, nl, lidx, nli, static_cast<uint32_t*>(arena), arena_texels, counters);
+		hipLaunchKernelGGL((k_setup_and_surfaces<LANES, Color32T>), dim3(blocks), dim3(256), 0, st, a, views, sa, m);
+}
+
+void launch_setup_and_surfaces(hipStream_t st, uint32_t lanes, uint32_t color_bits, SetupArgs a, const ViewsArg& views,
+	const SurfaceArgs& sa, const MapPtrs& m)
+{
+	const uint32_t groups = lanes == 256 ? 1u : 4u;
+	a.setup_blocks = (a.num_draws + groups - 1) / groups;
+	const uint64_t surf_blocks = sa.num_surfaces ? (sa.arena_texels + SURF_BLOCK - 1) / SURF_BLOCK : 0;
+	const uint64_t blocks = a.setup_blocks + surf_blocks;
+	if(blocks == 0)
+		return;
+	if(lanes == 256)
+		launch_fused_lanes<256>(st, color_bits, uint32_t(blocks), a, views, sa, m);
+	else
+		launch_fused_lanes<64>(st, color_bits, uint32_t(blocks), a, views, sa, m);
+}
+
+void launch_build_surfaces(hipStream_t st, uint32_t color_bits, const MapPtrs& m, const SurfaceArgs& sa, const ViewsArg& views,
+	const uint32_t* counters)
+{
+	if(sa.arena_texels == 0 || sa.num_surfaces == 0)
+		return;
+	const uint64_t blocks = (sa.arena_texels + SURF_BLOCK - 1) / SURF_BLOCK;
+	if(color_bits == 64)
+		hipLaunchKernelGGL(k_build_surfaces<Color64T>, dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, sa, views, counters);
+	else
+		hipLaunchKernelGGL(k_build_surfaces<Color32T>, dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, sa, views, counters);
 }
 
 template <class CT>
 static void launch_raster_ct(hipStream_t st, uint32_t tile_rows, const MapPtrs& m, const sw_draw* draws, const DrawRec* drec,
-	const PartRec* parts, const SpanRec* spans, const ViewExt* views, uint32_t nv, const void* arena, const void* skybox,
+	const PartRec* parts, const SpanRec* spans, const ViewsArg& views, const void* arena, const void* skybox,
 	void* fb, uint32_t total_tiles, const uint32_t* counters)
 {
 	typedef typename CT::T C;
@@ -1670,29 +1877,29 @@ static void launch_raster_ct(hipStream_t st, uint32_t tile_rows, const MapPtrs& 
 	const C* sky = static_cast<const C*>(skybox);
 	C* out = static_cast<C*>(fb);
 	if(tile_rows == 4)
-		hipLaunchKernelGGL((k_raster_tiles<CT, 4>), dim3(blocks), dim3(256), 0, st, m, draws, drec, parts, spans, views, nv, a,
+		hipLaunchKernelGGL((k_raster_tiles<CT, 4>), dim3(blocks), dim3(256), 0, st, m, draws, drec, parts, spans, views, a,
 			sky, out, total_tiles, counters);
 	else if(tile_rows == 8)
-		hipLaunchKernelGGL((k_raster_tiles<CT, 8>), dim3(blocks), dim3(256), 0, st, m, draws, drec, parts, spans, views, nv, a,
+		hipLaunchKernelGGL((k_raster_tiles<CT, 8>), dim3(blocks), dim3(256), 0, st, m, draws, drec, parts, spans, views, a,
 			sky, out, total_tiles, counters);
 	else
-		hipLaunchKernelGGL((k_raster_tiles<CT, 16>), dim3(blocks), dim3(256), 0, st, m, draws, drec, parts, spans, views, nv, a,
+		hipLaunchKernelGGL((k_raster_tiles<CT, 16>), dim3(blocks), dim3(256), 0, st, m, draws, drec, parts, spans, views, a,
 			sky, out, total_tiles, counters);
 }
 
 void launch_raster_tiles(hipStream_t st, uint32_t color_bits, uint32_t tile_rows, const MapPtrs& m, const sw_draw* draws,
-	const DrawRec* drec, const PartRec* parts, const SpanRec* spans, const ViewExt* views, uint32_t nv, const void* arena,
+	const DrawRec* drec, const PartRec* parts, const SpanRec* spans, const ViewsArg& views, const void* arena,
 	const void* skybox, void* fb, uint32_t total_tiles, const uint32_t* counters)
 {
 	if(color_bits == 64)
-		launch_raster_ct<Color64T>(st, tile_rows, m, draws, drec, parts, spans, views, nv, arena, skybox, fb, total_tiles, counters);
+		launch_raster_ct<Color64T>(st, tile_rows, m, draws, drec, parts, spans, views, arena, skybox, fb, total_tiles, counters);
 	else
-		launch_raster_ct<Color32T>(st, tile_rows, m, draws, drec, parts, spans, views, nv, arena, skybox, fb, total_tiles, counters);
+		launch_raster_ct<Color32T>(st, tile_rows, m, draws, drec, parts, spans, views, arena, skybox, fb, total_tiles, counters);
 }
 
-void launch_finish(hipStream_t st, const DrawRec* drec, uint32_t nd, uint32_t* counters)
+void launch_finish(hipStream_t st, const DrawRec* drec, uint32_t nd, uint32_t* counters, uint32_t* result)
 {
-	hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, st, drec, nd, counters);
+	hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, st, drec, nd, counters, result);
 }
 
 void launch_tonemap(hipStream_t st, const uint64_t* hdr, uint32_t w, uint32_t h, float isr, float isg, float isb,

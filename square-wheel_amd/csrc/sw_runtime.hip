@@ -67,8 +67,11 @@ struct sw_ctx
 	DevBuf views, surfaces, lights, light_indices, draws, vdata, surf_ext, blk_surf, parts, spans, drec, counters, arena, fb,
 		tonemap_out, tonemap_sums;
 	uint32_t tile_rows = 8; // raster tile height (SW_TILE_ROWS_*), set by sw_set_tile_rows / the autotuner
-	uint32_t* counters_host = nullptr; // pinned copy of the result block
+	uint32_t* counters_host = nullptr; // pinned, device-mapped result block (written by k_finish)
+	uint32_t* counters_host_dev = nullptr; // its device address
+	ViewsArg views_arg{}; // views of the batch being launched
 	bool surface_error = false; // an invalid surface seen by a surface-only launch sequence
+	bool batch_in_flight = false; // sw_render_batch_async launched, sw_batch_wait not yet called
 	uint64_t span_capacity = 1u << 20;
 
 	// plain-path frame state
@@ -102,7 +105,14 @@ struct sw_ctx
 		void* arena = nullptr;
 		void* fb = nullptr;
 		uint32_t total_tiles = 0;
+		uint32_t max_view_height = 0;
+		// how the surface kernel finds each texel's surface: SURF_TABLE_DEVICE (the setup launch
+		// prepares the table), SURF_TABLE_HOST (the host did), SURF_SEARCH (one view, searched in
+		// the surface blocks, which then run in the setup launch)
+		uint32_t surface_mode = 0;
 	} last;
+	std::vector<SurfExt> host_ext;
+	std::vector<uint32_t> host_blk;
 
 	// graph replay
 	hipGraphExec_t graph_exec = nullptr;
@@ -206,6 +216,7 @@ enum
 	STAGE_SURF = 1,
 	STAGE_RASTER = 2,
 	STAGE_TONEMAP = 3,
+	STAGE_SETUP_SURF = 4,
 };
 
 hipEvent_t take_event(sw_ctx* ctx)
@@ -275,8 +286,11 @@ void collect_timings(sw_ctx* ctx)
 }
 
 // Per-view tiling of a batch (views are always host memory).
-int build_view_ext(sw_ctx* ctx, const sw_view* views, uint32_t n, std::vector<ViewExt>& out, uint32_t& total_tiles)
+int build_view_ext(sw_ctx* ctx, const sw_view* views, uint32_t n, std::vector<ViewExt>& out, uint32_t& total_tiles,
+	uint32_t* max_height = nullptr)
 {
+	if(max_height)
+		*max_height = 0;
 	const uint32_t tile_rows = ctx->tile_rows;
 	out.resize(n);
 	uint64_t tiles = 0;
@@ -293,6 +307,8 @@ int build_view_ext(sw_ctx* ctx, const sw_view* views, uint32_t n, std::vector<Vi
 		next_draw += v.num_draws;
 		ViewExt& e = out[i];
 		e.v = v;
+		if(max_height)
+			*max_height = std::max(*max_height, v.height);
 		e.num_bands = (v.height + tile_rows - 1) / tile_rows;
 		e.cols = (v.width + 63) / 64;
 		e.first_tile = uint32_t(tiles);
@@ -321,44 +337,114 @@ int validate_views_fit(sw_ctx* ctx, const std::vector<ViewExt>& ve, uint64_t fb_
 	return SW_OK;
 }
 
+// Makes `ve` the views of the next launches: a single view travels in the kernel arguments
+// (when allowed), several views are uploaded to the device array.
+int set_views(sw_ctx* ctx, const std::vector<ViewExt>& ve, bool allow_inline)
+{
+	ViewsArg& va = ctx->views_arg;
+	va.count = uint32_t(ve.size());
+	if(allow_inline && ve.size() == 1)
+	{
+		va.inline_one = 1;
+		va.one = ve[0];
+		va.ptr = nullptr;
+		return SW_OK;
+	}
+	int r = upload(ctx, ctx->views, ve.data(), sizeof(ViewExt) * ve.size());
+	if(r)
+		return r;
+	va.inline_one = 0;
+	va.one = ViewExt{};
+	va.ptr = static_cast<const ViewExt*>(ctx->views.p);
+	return SW_OK;
+}
+
 // Launch sequence of a batch: draw setup (with the surface extents when `surfaces`), surface
 // building (when `surfaces`), tiled raster (when `raster`). The raster kernel's last block hands
 // the counters over to the result block; a sequence without it must be followed by
 // flush_surface_errors.
-int launch_stages(sw_ctx* ctx, const sw_ctx::Launch& L, const ViewExt* views_dev, bool surfaces, bool raster)
+enum
+{
+	SURF_TABLE_DEVICE = 0,
+	SURF_TABLE_HOST = 1,
+	SURF_SEARCH = 2,
+};
+
+// Span records a batch may use with static per-draw storage (1 GiB of records); above it the
+// draws allocate their records with atomics.
+const uint64_t STATIC_SPAN_LIMIT = 1ull << 25;
+
+int launch_stages(sw_ctx* ctx, const sw_ctx::Launch& L, bool surfaces, bool raster)
 {
 	sw_map* map = L.map;
 	const uint32_t nd = raster ? L.num_draws : 0;
 	const uint32_t ns = surfaces ? L.num_surfaces : 0;
 	const uint64_t surf_blocks = (L.arena_texels + SURF_BLOCK - 1) / SURF_BLOCK;
+	// a draw's span records never exceed its clip rect's height, at most the tallest view
+	const uint64_t stride = L.max_view_height;
+	const bool static_spans = nd > 0 && uint64_t(nd) * stride <= STATIC_SPAN_LIMIT;
 	int r;
+	if(static_spans && !ctx->capturing)
+		ctx->span_capacity = std::max<uint64_t>(ctx->span_capacity, uint64_t(nd) * stride);
 	if((r = ensure(ctx, ctx->parts, sizeof(PartRec) * uint64_t(std::max<uint32_t>(nd, 1)) * MAX_PARTS_PER_DRAW)))
 		return r;
 	if((r = ensure(ctx, ctx->spans, sizeof(SpanRec) * ctx->span_capacity)))
 		return r;
 	if((r = ensure(ctx, ctx->drec, sizeof(DrawRec) * std::max<uint32_t>(nd, 1))))
 		return r;
-	if((r = ensure(ctx, ctx->surf_ext, sizeof(SurfExt) * std::max<uint32_t>(ns, 1))))
-		return r;
-	if((r = ensure(ctx, ctx->blk_surf, sizeof(uint32_t) * std::max<uint64_t>(surf_blocks, 1))))
-		return r;
+	if(L.surface_mode == SURF_TABLE_DEVICE)
+	{
+		if((r = ensure(ctx, ctx->surf_ext, sizeof(SurfExt) * std::max<uint32_t>(ns, 1))))
+			return r;
+		if((r = ensure(ctx, ctx->blk_surf, sizeof(uint32_t) * std::max<uint64_t>(surf_blocks, 1))))
+			return r;
+	}
 	uint32_t* counters = static_cast<uint32_t*>(ctx->counters.p);
-	const uint64_t sky_texels = L.color_bits == 64 ? map->num_sky64 : map->num_sky32;
+	SetupArgs a{};
+	a.draws = L.draws;
+	a.num_draws = nd;
+	a.vdata = L.vdata;
+	a.num_vertex_ints = L.num_vertex_ints;
+	a.parts = static_cast<PartRec*>(ctx->parts.p);
+	a.spans = static_cast<SpanRec*>(ctx->spans.p);
+	a.drec = static_cast<DrawRec*>(ctx->drec.p);
+	a.counters = counters;
+	a.span_capacity = ctx->span_capacity;
+	a.span_stride = static_spans ? stride : 0;
+	a.arena_texels = L.arena_texels;
+	a.skybox_texels = L.color_bits == 64 ? map->num_sky64 : map->num_sky32;
+	a.lite_texels = map->ptrs.num_lite;
+	SurfaceArgs sa{};
+	sa.surfaces = L.surfaces;
+	sa.num_surfaces = ns;
+	sa.num_lights = L.num_lights;
+	sa.surf_ext = static_cast<const SurfExt*>(ctx->surf_ext.p);
+	sa.blk_surf = L.surface_mode == SURF_SEARCH ? nullptr : static_cast<const uint32_t*>(ctx->blk_surf.p);
+	sa.counters = counters;
+	sa.lights = L.lights;
+	sa.light_indices = L.light_indices;
+	sa.num_light_indices = L.num_light_indices;
+	sa.arena = L.arena;
+	sa.arena_texels = L.arena_texels;
 	// a block per draw while the draws alone cannot fill the chip, a wave per draw beyond
 	const uint32_t lanes = nd <= 4096 ? 256u : 64u;
+	if(surfaces && L.surface_mode != SURF_TABLE_DEVICE)
 	{
-		StageTimer t(ctx, STAGE_SETUP);
-		swk::launch_setup(ctx->stream, lanes, L.draws, nd, L.vdata, L.num_vertex_ints, views_dev, L.num_views,
-			static_cast<PartRec*>(ctx->parts.p), static_cast<SpanRec*>(ctx->spans.p), static_cast<DrawRec*>(ctx->drec.p),
-			counters, ctx->span_capacity, L.arena_texels, sky_texels, map->ptrs.num_lite, L.surfaces, ns,
-			static_cast<SurfExt*>(ctx->surf_ext.p), static_cast<uint32_t*>(ctx->blk_surf.p), map->ptrs);
+		StageTimer t(ctx, STAGE_SETUP_SURF);
+		swk::launch_setup_and_surfaces(ctx->stream, lanes, L.color_bits, a, ctx->views_arg, sa, map->ptrs);
 	}
-	if(surfaces)
+	else
 	{
-		StageTimer t(ctx, STAGE_SURF);
-		swk::launch_build_surfaces(ctx->stream, L.color_bits, map->ptrs, L.surfaces, L.num_surfaces,
-			static_cast<const SurfExt*>(ctx->surf_ext.p), static_cast<const uint32_t*>(ctx->blk_surf.p), views_dev, L.lights,
-			L.num_lights, L.light_indices, L.num_light_indices, L.arena, L.arena_texels, counters);
+		{
+			StageTimer t(ctx, STAGE_SETUP);
+			swk::launch_setup(ctx->stream, lanes, a, ctx->views_arg, L.surfaces, ns, static_cast<SurfExt*>(ctx->surf_ext.p),
+				static_cast<uint32_t*>(ctx->blk_surf.p), map->ptrs);
+		}
+		if(surfaces)
+		{
+			StageTimer t(ctx, STAGE_SURF);
+			swk::launch_build_surfaces(ctx->stream, L.color_bits, map->ptrs, sa, ctx->views_arg, counters);
+		}
 	}
 	if(raster)
 	{
@@ -366,11 +452,63 @@ int launch_stages(sw_ctx* ctx, const sw_ctx::Launch& L, const ViewExt* views_dev
 			StageTimer t(ctx, STAGE_RASTER);
 			swk::launch_raster_tiles(ctx->stream, L.color_bits, ctx->tile_rows, map->ptrs, L.draws,
 				static_cast<DrawRec*>(ctx->drec.p), static_cast<PartRec*>(ctx->parts.p), static_cast<SpanRec*>(ctx->spans.p),
-				views_dev, L.num_views, L.arena, L.color_bits == 64 ? map->sky64.p : map->sky32.p, L.fb, L.total_tiles, counters);
+				ctx->views_arg, L.arena, L.color_bits == 64 ? map->sky64.p : map->sky32.p, L.fb, L.total_tiles, counters);
 		}
-		swk::launch_finish(ctx->stream, static_cast<DrawRec*>(ctx->drec.p), nd, counters);
+		swk::launch_finish(ctx->stream, static_cast<DrawRec*>(ctx->drec.p), nd, counters, ctx->counters_host_dev);
 	}
 	return hip_check(ctx, hipGetLastError(), "kernel launch");
+}
+
+// Host-side preparation of host-resident surface descriptors (what the setup launch's extra blocks
+// do for device-resident ones): arena extents, validation and the block -> first surface table,
+// uploaded for the surface kernel. Mirrors prep_surface in sw_kernels.hip.
+int host_prepare_surfaces(sw_ctx* ctx, const sw_map* map, const std::vector<ViewExt>& ve, const sw_surface_desc* s, uint32_t ns,
+	uint64_t arena_texels)
+{
+	std::vector<SurfExt>& ext = ctx->host_ext;
+	std::vector<uint32_t>& blk = ctx->host_blk;
+	const uint64_t num_blocks = (arena_texels + SURF_BLOCK - 1) / SURF_BLOCK;
+	ext.resize(std::max<uint32_t>(ns, 1));
+	blk.resize(std::max<uint64_t>(num_blocks, 1));
+	const std::vector<sw_texture_level>& lv = map->host_levels;
+	uint32_t view = 0;
+	uint64_t prev_end = 0;
+	for(uint32_t i = 0; i < ns; ++i)
+	{
+		while(view + 1 < ve.size() && ve[view + 1].v.first_surface <= i)
+			++view;
+		const sw_surface_desc& d = s[i];
+		const uint64_t start = (ve.empty() ? 0 : ve[view].v.surface_arena_offset) + d.pixels_offset;
+		const uint64_t end = start + uint64_t(d.size[0]) * d.size[1];
+		bool bad = end > arena_texels || d.size[0] > 65536u || d.size[1] > 65536u || start < prev_end;
+		if((d.flags & SW_SURF_DIFFUSE_LAYER) && (d.texture_level >= lv.size() || (lv[d.texture_level].flags & SW_TEX_LITE_LEVEL)))
+			bad = true;
+		if((d.flags & SW_SURF_EMISSIVE) && d.emissive_level != SW_NO_TEXTURE &&
+			(d.emissive_level >= lv.size() || !(lv[d.emissive_level].flags & SW_TEX_LITE_LEVEL)))
+			bad = true;
+		if((d.flags & SW_SURF_DIFFUSE_LAYER) && d.lightmap_offset != 0 && d.size[0] > 0 && d.size[1] > 0)
+		{
+			const uint32_t sl = d.lightmap_scale_log2;
+			const uint64_t num_lm = (d.flags & SW_SURF_DIRECTIONAL) ? map->ptrs.num_lm_dir : map->ptrs.num_lm_simple;
+			if(sl > 4 || d.lightmap_size[0] < 2 || d.lightmap_size[1] < 2 ||
+				((uint64_t(d.size[0]) - 1 + d.lightmap_tc_shift[0]) >> sl) + 1 >= d.lightmap_size[0] ||
+				((uint64_t(d.size[1]) - 1 + d.lightmap_tc_shift[1]) >> sl) + 1 >= d.lightmap_size[1] ||
+				uint64_t(d.lightmap_offset) + uint64_t(d.lightmap_size[0]) * d.lightmap_size[1] > num_lm)
+				bad = true;
+		}
+		if(bad)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT,
+				"surface " + std::to_string(i) +
+					": invalid descriptor (arena order/overlap, texture level, or lightmap range out of bounds)");
+		ext[i] = SurfExt{end, view, 0u};
+		const uint64_t b0 = (prev_end + SURF_BLOCK - 1) / SURF_BLOCK;
+		const uint64_t b1 = i + 1 == ns ? num_blocks : std::min<uint64_t>((end + SURF_BLOCK - 1) / SURF_BLOCK, num_blocks);
+		for(uint64_t b = b0; b < b1; ++b)
+			blk[b] = i;
+		prev_end = end;
+	}
+	int r = upload(ctx, ctx->surf_ext, ext.data(), sizeof(SurfExt) * ext.size());
+	return r ? r : upload(ctx, ctx->blk_surf, blk.data(), sizeof(uint32_t) * blk.size());
 }
 
 // After a surface-only launch sequence: waits, keeps its error flag, clears the live counters.
@@ -392,19 +530,17 @@ int flush_surface_errors(sw_ctx* ctx)
 }
 
 // Waits for the batch, reads the counters; re-runs the raster stages if span storage overflowed.
-int finish(sw_ctx* ctx, const ViewExt* views_dev)
+int finish(sw_ctx* ctx)
 {
 	for(int attempt = 0; attempt < 4; ++attempt)
 	{
-		uint32_t* host_counters = ctx->counters_host;
-		int r = hip_check(ctx,
-			hipMemcpyAsync(host_counters, static_cast<uint32_t*>(ctx->counters.p) + COUNTER_RESULT,
-				sizeof(uint32_t) * COUNTER_COUNT, hipMemcpyDeviceToHost, ctx->stream),
-			"hipMemcpyAsync counters");
+		// k_finish wrote the result block straight into pinned host memory
+		int r = hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
 		if(r)
 			return r;
-		if((r = hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize")))
-			return r;
+		uint32_t host_counters[COUNTER_COUNT];
+		for(int k = 0; k < COUNTER_COUNT; ++k)
+			host_counters[k] = static_cast<volatile const uint32_t*>(ctx->counters_host)[k];
 		collect_timings(ctx);
 		auto u64 = [&](int lo) { return uint64_t(host_counters[lo]) | (uint64_t(host_counters[lo + 1]) << 32); };
 		const bool surface_error = ctx->surface_error;
@@ -412,13 +548,15 @@ int finish(sw_ctx* ctx, const ViewExt* views_dev)
 		if(surface_error || (host_counters[COUNTER_ERROR] & ERROR_BAD_SURFACE))
 			return set_error(ctx, SW_ERR_INVALID_ARGUMENT,
 				"invalid surface descriptor (arena order/overlap, texture level, or lightmap range out of bounds)");
+		if(host_counters[COUNTER_ERROR] & ERROR_PART_GAP)
+			return set_error(ctx, SW_ERR_DEVICE, "internal: a draw's parts did not cover consecutive rows");
 		const uint64_t needed = u64(COUNTER_SPANS_LO);
 		if(host_counters[COUNTER_ERROR] & ERROR_SPAN_OVERFLOW)
 		{
 			if(needed >= (1ull << 31))
 				return set_error(ctx, SW_ERR_BUDGET, "span storage above 2^31 records");
 			ctx->span_capacity = std::max<uint64_t>(needed + needed / 4, ctx->span_capacity * 2);
-			if((r = launch_stages(ctx, ctx->last, views_dev, false, true)))
+			if((r = launch_stages(ctx, ctx->last, false, true)))
 				return r;
 			continue;
 		}
@@ -470,10 +608,10 @@ int launch_frame(sw_ctx* ctx, uint32_t num_draws, uint64_t num_vertex_ints, bool
 	std::vector<ViewExt> ve;
 	sw_ctx::Launch& L = ctx->last;
 	L = sw_ctx::Launch{};
-	if((r = build_view_ext(ctx, &v, 1, ve, L.total_tiles)))
+	if((r = build_view_ext(ctx, &v, 1, ve, L.total_tiles, &L.max_view_height)))
 		return r;
-	// written after the previous launches on the stream
-	if((r = upload(ctx, ctx->views, ve.data(), sizeof(ViewExt))))
+	L.surface_mode = SURF_SEARCH;
+	if((r = set_views(ctx, ve, true)))
 		return r;
 	L.map = ctx->frame_map;
 	L.color_bits = ctx->frame_color_bits;
@@ -495,7 +633,7 @@ int launch_frame(sw_ctx* ctx, uint32_t num_draws, uint64_t num_vertex_ints, bool
 	ctx->frame_surfaces_pending = false;
 	if(!raster && !surfaces)
 		return SW_OK;
-	if((r = launch_stages(ctx, L, static_cast<const ViewExt*>(ctx->views.p), surfaces, raster)))
+	if((r = launch_stages(ctx, L, surfaces, raster)))
 		return r;
 	return raster ? SW_OK : flush_surface_errors(ctx);
 }
@@ -531,8 +669,9 @@ int sw_ctx_create(int device, sw_ctx** out_ctx)
 	if(ensure(ctx, ctx->counters, sizeof(uint32_t) * COUNTER_TOTAL) ||
 		hipMemsetAsync(ctx->counters.p, 0, sizeof(uint32_t) * COUNTER_TOTAL, ctx->stream) != hipSuccess ||
 		hipStreamSynchronize(ctx->stream) != hipSuccess ||
-		hipHostMalloc(reinterpret_cast<void**>(&ctx->counters_host), sizeof(uint32_t) * COUNTER_COUNT, hipHostMallocDefault) !=
-			hipSuccess)
+		hipHostMalloc(reinterpret_cast<void**>(&ctx->counters_host), sizeof(uint32_t) * COUNTER_COUNT,
+			hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+		hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->counters_host_dev), ctx->counters_host, 0) != hipSuccess)
 	{
 		sw_ctx_destroy(ctx);
 		return SW_ERR_DEVICE;
@@ -740,7 +879,7 @@ int sw_frame_end(sw_ctx* ctx, void* dst, size_t dst_pitch_bytes, uint32_t dst_on
 		if((r = sw_raster_draws(ctx, nullptr, 0, nullptr, 0)))
 			return r;
 	}
-	if((r = finish(ctx, static_cast<const ViewExt*>(ctx->views.p))))
+	if((r = finish(ctx)))
 		return r;
 	if(dst)
 	{
@@ -772,7 +911,7 @@ int sw_tonemap(sw_ctx* ctx, float exposure, const float color_modulate[3], void*
 	{
 		if(!ctx->frame_rastered && (r = sw_raster_draws(ctx, nullptr, 0, nullptr, 0)))
 			return r;
-		if((r = finish(ctx, static_cast<const ViewExt*>(ctx->views.p))))
+		if((r = finish(ctx)))
 			return r;
 		ctx->frame_active = false;
 	}
@@ -945,21 +1084,24 @@ int sw_get_stats(sw_ctx* ctx, sw_stats* out)
 // ---------------------------------------------------------------------------------------------
 // Batched path
 // ---------------------------------------------------------------------------------------------
-static int prepare_batch(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_fb, sw_ctx::Launch& L, std::vector<ViewExt>& ve)
+// Validates a batch, makes its views current (inline for one view unless `for_graph`: a captured
+// graph reads the views from the device array, which every replay refreshes) and fills L.
+static int prepare_batch(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_fb, sw_ctx::Launch& L, std::vector<ViewExt>& ve,
+	bool for_graph = false)
 {
 	if(!ctx || !map || !b || !out_fb || (b->num_views && !b->views))
 		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
 	if(b->color_bits != 32 && b->color_bits != 64)
 		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "color_bits must be 32 or 64");
 	int r;
-	if((r = build_view_ext(ctx, b->views, b->num_views, ve, L.total_tiles)))
+	if((r = build_view_ext(ctx, b->views, b->num_views, ve, L.total_tiles, &L.max_view_height)))
 		return r;
 	if((r = validate_views_fit(ctx, ve, b->framebuffer_texels, b->num_surfaces, b->num_draws)))
 		return r;
 	for(const ViewExt& e : ve)
 		if(e.v.surface_arena_offset > b->surface_arena_texels)
 			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "view arena offset outside the arena");
-	if((r = upload(ctx, ctx->views, ve.data(), sizeof(ViewExt) * ve.size())))
+	if((r = set_views(ctx, ve, !for_graph)))
 		return r;
 	if((r = ensure(ctx, ctx->arena, std::max<uint64_t>(b->surface_arena_texels, 1) * color_bytes(b->color_bits))))
 		return r;
@@ -1000,7 +1142,15 @@ static int prepare_batch(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_
 		L.draws = static_cast<const sw_draw*>(ctx->draws.p);
 		L.vdata = static_cast<const int32_t*>(ctx->vdata.p);
 		fill_surface_stats(ctx, b->surfaces, b->num_surfaces, true);
+		if(b->num_views > 1)
+		{
+			if((r = host_prepare_surfaces(ctx, map, ve, b->surfaces, b->num_surfaces, b->surface_arena_texels)))
+				return r;
+			L.surface_mode = SURF_TABLE_HOST;
+		}
 	}
+	if(b->num_views == 1)
+		L.surface_mode = SURF_SEARCH;
 	return SW_OK;
 }
 
@@ -1018,9 +1168,55 @@ int sw_render_batch(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_fb)
 	ctx->last = L;
 	ctx->last_color_bits = L.color_bits;
 	// views that ask for a cleared background get it in the raster kernel's tile init
-	if((r = launch_stages(ctx, L, static_cast<const ViewExt*>(ctx->views.p), true, true)))
+	if((r = launch_stages(ctx, L, true, true)))
 		return r;
-	return finish(ctx, static_cast<const ViewExt*>(ctx->views.p));
+	return finish(ctx);
+}
+
+int sw_render_batch_async(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_fb)
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	if(ctx->batch_in_flight)
+		return set_error(ctx, SW_ERR_STATE, "a batch is already in flight on this context");
+	(void)hipSetDevice(ctx->device);
+	ctx->graph_key = 0;
+	std::vector<ViewExt> ve;
+	sw_ctx::Launch L;
+	int r;
+	if((r = prepare_batch(ctx, map, b, out_fb, L, ve)))
+		return r;
+	ctx->last = L;
+	ctx->last_color_bits = L.color_bits;
+	if((r = launch_stages(ctx, L, true, true)))
+		return r;
+	ctx->batch_in_flight = true;
+	return SW_OK;
+}
+
+int sw_batch_poll(sw_ctx* ctx)
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	if(!ctx->batch_in_flight)
+		return 1;
+	const hipError_t e = hipStreamQuery(ctx->stream);
+	if(e == hipSuccess)
+		return 1;
+	if(e == hipErrorNotReady)
+		return 0;
+	return hip_check(ctx, e, "hipStreamQuery");
+}
+
+int sw_batch_wait(sw_ctx* ctx)
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	if(!ctx->batch_in_flight)
+		return SW_OK;
+	(void)hipSetDevice(ctx->device);
+	ctx->batch_in_flight = false;
+	return finish(ctx);
 }
 
 int sw_render_batch_graph(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_fb)
@@ -1060,7 +1256,7 @@ int sw_render_batch_graph(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out
 		ctx->span_capacity = std::max<uint64_t>(ctx->span_capacity, ctx->stats.spans + ctx->stats.spans / 2 + 1024);
 		std::vector<ViewExt> ve;
 		sw_ctx::Launch L;
-		if((r = prepare_batch(ctx, map, b, out_fb, L, ve)))
+		if((r = prepare_batch(ctx, map, b, out_fb, L, ve, true)))
 			return r;
 		// allocate at the new span capacity outside the capture
 		if((r = ensure(ctx, ctx->spans, sizeof(SpanRec) * ctx->span_capacity)))
@@ -1081,7 +1277,7 @@ int sw_render_batch_graph(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out
 			return r;
 		ctx->last = L;
 		ctx->capturing = true;
-		const int r2 = launch_stages(ctx, L, static_cast<const ViewExt*>(ctx->views.p), true, true);
+		const int r2 = launch_stages(ctx, L, true, true);
 		hipGraph_t g = nullptr;
 		const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
 		ctx->capturing = false;
@@ -1095,9 +1291,18 @@ int sw_render_batch_graph(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out
 		ctx->graph_key = key;
 		ctx->graph_epoch = ctx->alloc_epoch;
 	}
+	else
+	{
+		// the captured kernels read the views from the device array: refresh it (camera
+		// positions, framebuffer offsets and clear flags may change between replays)
+		std::vector<ViewExt> ve;
+		uint32_t tiles = 0;
+		if((r = build_view_ext(ctx, b->views, b->num_views, ve, tiles)) || (r = set_views(ctx, ve, false)))
+			return r;
+	}
 	if((r = hip_check(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream), "graph launch")))
 		return r;
-	return finish(ctx, static_cast<const ViewExt*>(ctx->views.p));
+	return finish(ctx);
 }
 
 int sw_set_tile_rows(sw_ctx* ctx, uint32_t rows)

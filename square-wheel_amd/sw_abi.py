@@ -17,6 +17,7 @@ HOST_LIB_PATH = os.path.join(HERE, "libsw_host.so")
 # Constants (sw_gpu.h)
 # ---------------------------------------------------------------------------------------------
 SW_OK = 0
+SW_ERR_INVALID_ARGUMENT, SW_ERR_OUT_OF_MEMORY, SW_ERR_DEVICE, SW_ERR_STATE, SW_ERR_BUDGET, SW_ERR_NOT_LOADED = -1, -2, -3, -4, -5, -6
 BLEND_NONE, BLEND_AVERAGE, BLEND_ADDITIVE, BLEND_ALPHA_TEST, BLEND_ALPHA_BLEND = range(5)
 TC_PERSPECTIVE, TC_LINE_Z, TC_AFFINE = range(3)
 DRAW_POLYGON, DRAW_TRIANGLE = 0, 1
@@ -209,6 +210,9 @@ GPU_SYMBOLS = {
     "sw_profile_enable": (C.c_int, [C.c_void_p, C.c_uint32]),
     "sw_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_uint32]),
     "sw_ctx_device_bytes": (C.c_uint64, [C.c_void_p]),
+    "sw_render_batch_async": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Batch), C.c_void_p]),
+    "sw_batch_poll": (C.c_int, [C.c_void_p]),
+    "sw_batch_wait": (C.c_int, [C.c_void_p]),
     "sw_set_tile_rows": (C.c_int, [C.c_void_p, C.c_uint32]),
     "sw_get_tile_rows": (C.c_uint32, [C.c_void_p]),
     "sw_autotune_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Batch), C.c_void_p, C.POINTER(C.c_uint32),
@@ -339,7 +343,7 @@ class Context:
     def profile_enable(self, on=True):
         self.check(self.lib.sw_profile_enable(self.h, 1 if on else 0), "sw_profile_enable")
 
-    PROFILE_STAGES = ("setup_draws", "build_surfaces", "raster_tiles", "tonemap")
+    PROFILE_STAGES = ("setup_draws", "build_surfaces", "raster_tiles", "tonemap", "setup_and_surfaces")
 
     def profile_read(self, reset=True):
         ms = (C.c_double * len(self.PROFILE_STAGES))()
@@ -416,3 +420,140 @@ class DeviceCommandBuffers:
     def free(self):
         for v in self.bufs.values():
             v.free()
+
+
+# ---------------------------------------------------------------------------------------------
+# Frame-serving scheduler (include/sw_server.h)
+# ---------------------------------------------------------------------------------------------
+class ViewC(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("cam_pos", C.c_float * 3),
+                ("clear_background", C.c_uint32), ("background_color", C.c_uint64),
+                ("first_surface", C.c_uint32), ("num_surfaces", C.c_uint32),
+                ("first_draw", C.c_uint32), ("num_draws", C.c_uint32),
+                ("fb_offset", C.c_uint64), ("surface_arena_offset", C.c_uint64)]
+
+
+class FrameRequest(C.Structure):
+    _fields_ = [
+        ("map", C.c_uint32), ("color_bits", C.c_uint32), ("view", ViewC),
+        ("surfaces", C.c_void_p), ("num_surfaces", C.c_uint32), ("num_lights", C.c_uint32),
+        ("lights", C.c_void_p), ("light_indices", C.c_void_p), ("num_light_indices", C.c_uint32),
+        ("num_draws", C.c_uint32), ("draws", C.c_void_p), ("vertex_data", C.c_void_p),
+        ("num_vertex_ints", C.c_uint64), ("surface_arena_texels", C.c_uint64),
+        ("framebuffer", C.c_void_p), ("tonemapped", C.c_void_p), ("exposure", C.c_float), ("reserved", C.c_uint32),
+    ]
+
+
+class Completion(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("status", C.c_int32), ("batch_views", C.c_uint32),
+                ("queue_ms", C.c_double), ("total_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
+                ("batches", C.c_uint64), ("views", C.c_uint64), ("peak_bytes_in_flight", C.c_uint64),
+                ("peak_slots_busy", C.c_uint32), ("reserved", C.c_uint32)]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_ if n != "reserved"}
+
+
+GPU_SYMBOLS.update({
+    "sw_server_create": (C.c_int, [C.c_int, C.c_uint32, C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "sw_server_destroy": (None, [C.c_void_p]),
+    "sw_server_last_error": (C.c_char_p, [C.c_void_p]),
+    "sw_server_add_map": (C.c_int, [C.c_void_p, C.POINTER(MapDesc), C.POINTER(C.c_uint32)]),
+    "sw_server_submit": (C.c_int, [C.c_void_p, C.POINTER(FrameRequest), C.POINTER(C.c_uint64)]),
+    "sw_server_poll": (C.c_int, [C.c_void_p, C.POINTER(Completion), C.c_uint32, C.POINTER(C.c_uint32)]),
+    "sw_server_drain": (C.c_int, [C.c_void_p, C.POINTER(Completion), C.c_uint32, C.POINTER(C.c_uint32)]),
+    "sw_server_pending": (C.c_uint32, [C.c_void_p]),
+    "sw_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+})
+
+
+class Server:
+    """Continuous-batching frame server on one device (slots = concurrent batches)."""
+
+    def __init__(self, device=0, slots=3, hbm_budget=8 << 30, max_views=64):
+        self.lib = load_gpu()
+        h = C.c_void_p()
+        r = self.lib.sw_server_create(device, slots, hbm_budget, max_views, C.byref(h))
+        if r != SW_OK:
+            raise SwError(f"sw_server_create failed ({r})")
+        self.h = h
+        self._keep = {}  # ticket -> result arrays (kept alive until completion)
+
+    def check(self, r, what):
+        if r != SW_OK:
+            raise SwError(f"{what} failed ({r}): {self.lib.sw_server_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            self.lib.sw_server_destroy(self.h)
+            self.h = None
+
+    def add_map(self, m: MapData):
+        idx = C.c_uint32()
+        self.check(self.lib.sw_server_add_map(self.h, C.byref(m.desc()), C.byref(idx)), "sw_server_add_map")
+        return int(idx.value)
+
+    def submit(self, map_index, frame, tonemap_exposure=None):
+        """frame: a front-end frame dict. Returns the ticket; results via poll()/drain()."""
+        v = frame["view"]
+        w, h = int(v["width"]), int(v["height"])
+        bits = int(frame["color_bits"])
+        fb = np.zeros((h, w), dtype=np.uint64 if bits == 64 else np.uint32)
+        tm = np.zeros((h, w), dtype=np.uint32) if (tonemap_exposure is not None and bits == 64) else None
+        req = FrameRequest()
+        req.map = map_index
+        req.color_bits = bits
+        C.memmove(C.byref(req.view), np.ascontiguousarray(v).ctypes.data, C.sizeof(ViewC))
+        arrays = {k: np.ascontiguousarray(frame[k]) for k in ("surfaces", "lights", "light_indices", "draws", "vertex_data")}
+        req.surfaces, req.num_surfaces = arrays["surfaces"].ctypes.data, len(arrays["surfaces"])
+        req.lights, req.num_lights = arrays["lights"].ctypes.data, len(arrays["lights"])
+        req.light_indices, req.num_light_indices = arrays["light_indices"].ctypes.data, len(arrays["light_indices"])
+        req.draws, req.num_draws = arrays["draws"].ctypes.data, len(arrays["draws"])
+        req.vertex_data, req.num_vertex_ints = arrays["vertex_data"].ctypes.data, len(arrays["vertex_data"])
+        req.surface_arena_texels = int(frame["arena_texels"])
+        req.framebuffer = fb.ctypes.data
+        req.tonemapped = tm.ctypes.data if tm is not None else None
+        req.exposure = float(tonemap_exposure or 1.0)
+        t = C.c_uint64()
+        self.check(self.lib.sw_server_submit(self.h, C.byref(req), C.byref(t)), "sw_server_submit")
+        self._keep[int(t.value)] = (fb, tm)
+        return int(t.value)
+
+    def _collect(self, fn, max_n=1024):
+        if getattr(self, "_out", None) is None:
+            self._out = (Completion * max_n)()
+            self._n = C.c_uint32()
+        out, n = self._out, self._n
+        self.check(fn(self.h, out, max_n, C.byref(n)), fn.__name__)
+        res = []
+        for i in range(n.value):
+            c = out[i]
+            fb, tm = self._keep.pop(int(c.ticket))
+            res.append({"ticket": int(c.ticket), "status": int(c.status), "batch_views": int(c.batch_views),
+                        "queue_ms": float(c.queue_ms), "total_ms": float(c.total_ms), "framebuffer": fb,
+                        "tonemapped": tm})
+        return res
+
+    def poll(self):
+        return self._collect(self.lib.sw_server_poll)
+
+    def drain(self):
+        res = []
+        while self.pending():
+            got = self._collect(self.lib.sw_server_drain)
+            res += got
+            if not got:
+                break
+        return res
+
+    def pending(self):
+        return int(self.lib.sw_server_pending(self.h))
+
+    def stats(self):
+        s = ServerStats()
+        self.check(self.lib.sw_server_get_stats(self.h, C.byref(s)), "sw_server_get_stats")
+        return s.as_dict()

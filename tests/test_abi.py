@@ -25,10 +25,11 @@ def declared_functions(header):
     return sorted(set(names))
 
 
-@pytest.mark.parametrize("header,lib", [("sw_gpu.h", "libsw_gpu.so"), ("sw_host.h", "libsw_host.so")])
+@pytest.mark.parametrize("header,lib", [("sw_gpu.h", "libsw_gpu.so"), ("sw_server.h", "libsw_gpu.so"),
+                                        ("sw_host.h", "libsw_host.so")])
 def test_library_exports_every_declared_function(header, lib):
     names = declared_functions(header)
-    assert len(names) >= 10, names
+    assert len(names) >= 8, names
     path = os.path.join(PKG, lib)
     assert os.path.exists(path), "build the libraries first (python square-wheel_amd/build.py)"
     so = ctypes.CDLL(path)
@@ -55,15 +56,20 @@ STRUCTS = {
 }
 
 
+CTYPES_STRUCTS = [(sw_abi.MapDesc, "sw_map_desc"), (sw_abi.Batch, "sw_batch"), (sw_abi.Stats, "sw_stats"),
+                  (sw_abi.ViewC, "sw_view"), (sw_abi.FrameRequest, "sw_frame_request"), (sw_abi.Completion, "sw_completion"),
+                  (sw_abi.ServerStats, "sw_server_stats")]
+
+
 @pytest.fixture(scope="module")
 def c_layout():
     """sizeof / offsetof of every struct field, printed by a C program compiled from the header."""
-    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "sw_gpu.h"', "int main(void) {"]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "sw_server.h"', "int main(void) {"]
     for name, dt in STRUCTS.items():
         lines.append(f'printf("{name} @sizeof %zu\\n", sizeof({name}));')
         for field in dt.names:
             lines.append(f'printf("{name} {field} %zu\\n", offsetof({name}, {field}));')
-    for cls, cname in ((sw_abi.MapDesc, "sw_map_desc"), (sw_abi.Batch, "sw_batch"), (sw_abi.Stats, "sw_stats")):
+    for cls, cname in CTYPES_STRUCTS:
         lines.append(f'printf("{cname} @sizeof %zu\\n", sizeof({cname}));')
         for field, _ in cls._fields_:
             lines.append(f'printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
@@ -89,7 +95,7 @@ def test_numpy_dtype_matches_c_layout(c_layout, name):
         assert dt.fields[field][1] == c_layout[(name, field)], field
 
 
-@pytest.mark.parametrize("cls,cname", [(sw_abi.MapDesc, "sw_map_desc"), (sw_abi.Batch, "sw_batch"), (sw_abi.Stats, "sw_stats")])
+@pytest.mark.parametrize("cls,cname", CTYPES_STRUCTS)
 def test_ctypes_struct_matches_c_layout(c_layout, cls, cname):
     assert ctypes.sizeof(cls) == c_layout[(cname, "@sizeof")]
     for field, _ in cls._fields_:
@@ -123,4 +129,4 @@ def test_product_path_fails_loudly_without_library(tmp_path, monkeypatch):
 
 def test_gpu_symbol_table_covers_header():
     # every declared entry point has a typed ctypes binding
-    assert sorted(sw_abi.GPU_SYMBOLS) == declared_functions("sw_gpu.h")
+    assert sorted(sw_abi.GPU_SYMBOLS) == sorted(declared_functions("sw_gpu.h") + declared_functions("sw_server.h"))

@@ -64,3 +64,38 @@ def test_large_map_frames_bit_exact(gpu_ctx, map1, color_bits):
         assert np.array_equal(out, ref)
     finally:
         gpu_ctx.release_map(mh)
+
+
+@pytest.fixture(scope="module")
+def map2():
+    return sw_host.large_map_alpha(7, grid=(12, 12, 4))
+
+
+@pytest.mark.parametrize("color_bits", [64, 32])
+def test_models_decals_hdr_frames(gpu_ctx, map2, color_bits):
+    """The HDR + models scene at reduced size: 16 dynamic lights, animated triangle models,
+    decals, alpha-blended materials; 64-bit frames are tonemapped. Framebuffers are bit-exact
+    against the oracle, the tonemapped image too."""
+    fb = sw_host.FrameBuilder()
+    mh = gpu_ctx.upload_map(map2.data)
+    tri_pixels = 0
+    try:
+        for i, pose in enumerate(map2.camera_path(4, 6)):
+            _say(f"case models pose {i} bits {color_bits}")
+            f = fb.build(map2, pose, width=640, height=360, color_bits=color_bits, use_directional_lightmaps=1,
+                         num_dynamic_lights=16, num_models=24, num_decals=96, seed=11, time=0.37 * i)
+            b = sw_host.batch_from_frames([f])
+            ref, _, st, oob = oracle.render(map2.data, b)
+            assert oob == 0
+            tri_pixels += st["triangle_pixels"]
+            if color_bits == 64:
+                img, out, sums = gpu_ctx.render_plain(mh, b, tonemap=(1.0, (1.0, 1.0, 1.0)))
+                tref, tsums = oracle.tonemap(ref.reshape(img.shape), 1.0, (1.0, 1.0, 1.0))
+                assert np.array_equal(out, tref)
+            else:
+                img = gpu_ctx.render_plain(mh, b)
+            img = img.reshape(-1)
+            assert np.array_equal(img, ref), f"pose {i}: {np.count_nonzero(img != ref)} pixels differ"
+    finally:
+        gpu_ctx.release_map(mh)
+    assert tri_pixels > 0, "the path drew no model or decal pixels"

@@ -190,3 +190,66 @@ def test_surfaces_read_before_raster(gpu_ctx, color_bits):
     ofb, oarena, _, _ = oracle.render(m, b, mode=oracle.MODE_EXACT)
     assert np.array_equal(arena, oarena[:len(arena)])
     assert np.array_equal(fb.reshape(-1), ofb)
+
+
+@pytest.mark.parametrize("color_bits", [32, 64])
+def test_single_view_device_inputs(gpu_ctx, color_bits):
+    """A one-view batch with device-resident command buffers: the surface blocks search their
+    surfaces themselves and run in the setup launch; eager and graph replay equal the oracle."""
+    import sw_abi
+    _say(f"case single view device inputs bits={color_bits}")
+    m, b = scenes.make_batch(47, color_bits=color_bits, view_sizes=((320, 240),), draws_per_view=90)
+    mh = gpu_ctx.upload_map(m)
+    dev = sw_abi.DeviceCommandBuffers(gpu_ctx, b)
+    out = gpu_ctx.alloc(b.fb_texels * (8 if color_bits == 64 else 4))
+    try:
+        eager = gpu_ctx.render_batch(mh, b, device_inputs=dev, out=out)
+        arena = gpu_ctx.read_surfaces(b.arena_texels, color_bits)
+        graphs = [gpu_ctx.render_batch(mh, b, graph=True, device_inputs=dev, out=out) for _ in range(3)]
+    finally:
+        out.free()
+        dev.free()
+        gpu_ctx.release_map(mh)
+    ofb, oarena, _, _ = oracle.render(m, b, mode=oracle.MODE_EXACT)
+    assert np.array_equal(arena, oarena[:b.arena_texels])
+    assert np.array_equal(eager, ofb)
+    for g in graphs:
+        assert np.array_equal(g, ofb)
+
+
+@pytest.mark.parametrize("views", [1, 3])
+@pytest.mark.parametrize("device_inputs", [False, True])
+def test_invalid_surface_is_an_error_and_recovers(gpu_ctx, views, device_inputs):
+    """A descriptor naming a texture level past the table is reported (status code and message),
+    nothing reads outside the map; the next valid batch renders exactly."""
+    import sw_abi
+    _say(f"case invalid surface views={views} device={device_inputs}")
+    sizes = ((160, 120), (97, 61), (64, 200))[:views]
+    m, b = scenes.make_batch(49, color_bits=32, view_sizes=sizes)
+    bad_surfaces = b.surfaces.copy()
+    bad_surfaces[len(bad_surfaces) // 2]["texture_level"] = 10 ** 6
+    bad_surfaces[len(bad_surfaces) // 2]["flags"] |= sw_abi.SURF_DIFFUSE_LAYER
+    bad = sw_abi.BatchData(b.views, bad_surfaces, b.lights, b.light_indices, b.draws, b.vertex_data, 32,
+                           arena_texels=b.arena_texels)
+    mh = gpu_ctx.upload_map(m)
+    try:
+        if device_inputs:
+            dev_bad = sw_abi.DeviceCommandBuffers(gpu_ctx, bad)
+            dev_ok = sw_abi.DeviceCommandBuffers(gpu_ctx, b)
+            out = gpu_ctx.alloc(b.fb_texels * 4)
+            try:
+                with pytest.raises(sw_abi.SwError, match="surface"):
+                    gpu_ctx.render_batch(mh, bad, device_inputs=dev_bad, out=out)
+                fb = gpu_ctx.render_batch(mh, b, device_inputs=dev_ok, out=out)
+            finally:
+                out.free()
+                dev_bad.free()
+                dev_ok.free()
+        else:
+            with pytest.raises(sw_abi.SwError, match="surface"):
+                gpu_ctx.render_batch(mh, bad)
+            fb = gpu_ctx.render_batch(mh, b)
+    finally:
+        gpu_ctx.release_map(mh)
+    ofb, _, _, _ = oracle.render(m, b)
+    assert np.array_equal(fb, ofb)
