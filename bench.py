@@ -171,7 +171,7 @@ def serving_leg(args):
             bits = 64 if rng.uniform() < 0.5 else 32
             pool.append((i, fbld.build(m, p, width=w, height=h, color_bits=bits, use_directional_lightmaps=1,
                                        num_dynamic_lights=4, clear_background=1, force_clear=1)))
-    mean_px = float(np.mean([int(f["view"]["width"]) * int(f["view"]["height"]) for _, f in pool]))
+    mean_px = float(np.mean([int(np.asarray(f["view"]["width"]).item()) * int(np.asarray(f["view"]["height"]).item()) for _, f in pool]))
     srv = sw_abi.Server(slots=3, hbm_budget=16 << 30, max_views=32)
     ids = [srv.add_map(m.data) for m in maps]
     out = {"workload": f"4 maps {grids}, {len(pool)} distinct requests, mean {mean_px / 1e6:.2f} Mpixel, "

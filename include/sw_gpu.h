@@ -259,6 +259,7 @@ typedef struct sw_stats
 	uint64_t blend_pixels; /* P_blend: writes whose blending mode reads the destination */
 	uint64_t spans; /* span records produced */
 	uint64_t parts; /* polygon/triangle parts produced by the edge walk */
+	uint64_t reruns; /* raster re-runs after span storage overflowed (work queued behind the batch saw the first run) */
 } sw_stats;
 
 typedef struct sw_ctx sw_ctx;
@@ -361,6 +362,12 @@ int sw_device_alloc(sw_ctx* ctx, uint64_t bytes, void** out_device_ptr);
 void sw_device_free(sw_ctx* ctx, void* device_ptr);
 int sw_copy_to_device(sw_ctx* ctx, void* device_dst, const void* host_src, uint64_t bytes);
 int sw_copy_from_device(sw_ctx* ctx, void* host_dst, const void* device_src, uint64_t bytes);
+/* Queues a copy on the context's stream and returns (complete at the next wait / sync; truly
+ * asynchronous for pinned host memory from sw_host_alloc). */
+int sw_copy_from_device_async(sw_ctx* ctx, void* host_dst, const void* device_src, uint64_t bytes);
+/* Pinned host memory (for asynchronous copies). */
+int sw_host_alloc(sw_ctx* ctx, uint64_t bytes, void** out_host_ptr);
+void sw_host_free(sw_ctx* ctx, void* host_ptr);
 
 /* Kernel timing with HIP events recorded on the context's stream around every launch.
  * Stages: 0 setup_draws (edge walk, span records and, for device-resident descriptors, surface

@@ -176,6 +176,7 @@ void launch_raster_tiles(hipStream_t st, uint32_t color_bits, uint32_t tile_rows
 	const void* skybox, void* fb, uint32_t total_tiles, const uint32_t* counters);
 // Kernel 4 (one block): per-draw statistics -> result block (host-mapped), live block cleared.
 void launch_finish(hipStream_t st, const DrawRec* drec, uint32_t nd, uint32_t* counters, uint32_t* result);
+// Tonemap; `sums` holds 4 + 3 * 1024 floats (totals first, then per-block partials).
 void launch_tonemap(hipStream_t st, const uint64_t* hdr, uint32_t w, uint32_t h, float isr, float isg, float isb,
 	uint32_t* dst, uint32_t dst_pitch, float* sums);
 } // namespace swk

@@ -1750,46 +1750,67 @@ __global__ __launch_bounds__(256) void k_raster_tiles(MapPtrs m, const sw_draw* 
 // =============================================================================================
 // Tonemap (postprocessor.rs:237-253, :1019-1023)
 // =============================================================================================
+// Tonemap blocks stride over the frame; each block leaves the partial colour sums of its pixels
+// (no atomics on three shared addresses), k_tonemap_sums adds them up.
+#define TONEMAP_BLOCKS 1024u
+
 __global__ __launch_bounds__(256) void k_tonemap(const uint64_t* __restrict__ hdr, uint32_t width, uint32_t height,
 	float inv_scale_r, float inv_scale_g, float inv_scale_b, uint32_t* __restrict__ dst, uint32_t dst_pitch,
-	float* __restrict__ sums)
+	float* __restrict__ partials)
 {
-	__shared__ float red[3][256];
-	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	__shared__ float red[3][4];
+	const uint64_t n = uint64_t(width) * height;
+	const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
 	float sb = 0.0f, sg = 0.0f, sr = 0.0f;
-	if(i < uint64_t(width) * height)
+	const float inv_255 = 1.0f / 255.0f;
+	for(uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
 	{
 		const uint32_t y = uint32_t(i / width), x = uint32_t(i - uint64_t(y) * width);
 		const CV c = cv_from_color64(hdr[i]);
-		sb = c.b;
-		sg = c.g;
-		sr = c.r;
-		const float inv_255 = 1.0f / 255.0f;
+		sb += c.b;
+		sg += c.g;
+		sr += c.r;
 		// div(c, mul_add(c, inv_255_vec, inv_scale_vec)); the alpha lane is c.a / fma(c.a, 0, 0)
 		const CV mapped = CV{c.b / mul_add(c.b, inv_255, inv_scale_b), c.g / mul_add(c.g, inv_255, inv_scale_g),
 			c.r / mul_add(c.r, inv_255, inv_scale_r), c.a / mul_add(c.a, 0.0f, 0.0f)};
 		dst[uint64_t(y) * dst_pitch + x] = Color32T::from_cv(mapped);
 	}
-	red[0][threadIdx.x] = sr;
-	red[1][threadIdx.x] = sg;
-	red[2][threadIdx.x] = sb;
+	for(int off = 32; off > 0; off >>= 1)
+	{
+		sr += __shfl_xor(sr, off);
+		sg += __shfl_xor(sg, off);
+		sb += __shfl_xor(sb, off);
+	}
+	const uint32_t wave = threadIdx.x >> 6;
+	if((threadIdx.x & 63) == 0)
+	{
+		red[0][wave] = sr;
+		red[1][wave] = sg;
+		red[2][wave] = sb;
+	}
 	__syncthreads();
-	for(uint32_t s = 128; s > 0; s >>= 1)
-	{
-		if(threadIdx.x < s)
-		{
-			red[0][threadIdx.x] += red[0][threadIdx.x + s];
-			red[1][threadIdx.x] += red[1][threadIdx.x + s];
-			red[2][threadIdx.x] += red[2][threadIdx.x + s];
-		}
-		__syncthreads();
-	}
-	if(threadIdx.x == 0)
-	{
-		atomicAdd(&sums[0], red[0][0]);
-		atomicAdd(&sums[1], red[1][0]);
-		atomicAdd(&sums[2], red[2][0]);
-	}
+	if(threadIdx.x < 3)
+		partials[blockIdx.x * 3 + threadIdx.x] =
+			red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
+}
+
+__global__ __launch_bounds__(256) void k_tonemap_sums(const float* __restrict__ partials, uint32_t blocks, float* __restrict__ sums)
+{
+	__shared__ float red[3][4];
+	float s[3] = {0.0f, 0.0f, 0.0f};
+	for(uint32_t b = threadIdx.x; b < blocks; b += blockDim.x)
+		for(int c = 0; c < 3; ++c)
+			s[c] += partials[b * 3 + c];
+	for(int c = 0; c < 3; ++c)
+		for(int off = 32; off > 0; off >>= 1)
+			s[c] += __shfl_xor(s[c], off);
+	const uint32_t wave = threadIdx.x >> 6;
+	if((threadIdx.x & 63) == 0)
+		for(int c = 0; c < 3; ++c)
+			red[c][wave] = s[c];
+	__syncthreads();
+	if(threadIdx.x < 3)
+		sums[threadIdx.x] = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
 }
 
 __global__ void k_zero_floats(float* p, uint32_t n)
@@ -1905,11 +1926,11 @@ void launch_finish(hipStream_t st, const DrawRec* drec, uint32_t nd, uint32_t* c
 void launch_tonemap(hipStream_t st, const uint64_t* hdr, uint32_t w, uint32_t h, float isr, float isg, float isb,
 	uint32_t* dst, uint32_t dst_pitch, float* sums)
 {
-	hipLaunchKernelGGL(k_zero_floats, dim3(1), dim3(64), 0, st, sums, 3u);
+	// sums[0..2]: totals; sums[4 ..]: 3 partials per block
 	const uint64_t n = uint64_t(w) * h;
-	if(n == 0)
-		return;
-	hipLaunchKernelGGL(k_tonemap, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, st, hdr, w, h, isr, isg, isb, dst, dst_pitch, sums);
+	const uint32_t blocks = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, TONEMAP_BLOCKS)));
+	hipLaunchKernelGGL(k_tonemap, dim3(blocks), dim3(256), 0, st, hdr, w, h, isr, isg, isb, dst, dst_pitch, sums + 4);
+	hipLaunchKernelGGL(k_tonemap_sums, dim3(1), dim3(256), 0, st, sums + 4, blocks, sums);
 }
 
 } // namespace swk

@@ -532,6 +532,7 @@ int flush_surface_errors(sw_ctx* ctx)
 // Waits for the batch, reads the counters; re-runs the raster stages if span storage overflowed.
 int finish(sw_ctx* ctx)
 {
+	ctx->stats.reruns = 0;
 	for(int attempt = 0; attempt < 4; ++attempt)
 	{
 		// k_finish wrote the result block straight into pinned host memory
@@ -558,6 +559,7 @@ int finish(sw_ctx* ctx)
 			ctx->span_capacity = std::max<uint64_t>(needed + needed / 4, ctx->span_capacity * 2);
 			if((r = launch_stages(ctx, ctx->last, false, true)))
 				return r;
+			ctx->stats.reruns++;
 			continue;
 		}
 		ctx->stats.polygon_pixels = u64(COUNTER_POLY_PIXELS_LO);
@@ -949,7 +951,7 @@ int sw_tonemap_device(sw_ctx* ctx, const void* hdr_device, uint32_t width, uint3
 		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
 	(void)hipSetDevice(ctx->device);
 	int r;
-	if((r = ensure(ctx, ctx->tonemap_sums, sizeof(float) * 4)))
+	if((r = ensure(ctx, ctx->tonemap_sums, sizeof(float) * (4 + 3 * 1024))))
 		return r;
 	// TonemappingFunction::new (postprocessor.rs:1005-1017)
 	const float isr = 1.0f / (exposure * color_modulate[0]);
@@ -1023,6 +1025,40 @@ int sw_copy_from_device(sw_ctx* ctx, void* dst, const void* src, uint64_t bytes)
 		return SW_OK;
 	int r = hip_check(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, ctx->stream), "copy from device");
 	return r ? r : hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+
+int sw_copy_from_device_async(sw_ctx* ctx, void* dst, const void* src, uint64_t bytes)
+{
+	if(!ctx || ((!dst || !src) && bytes))
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	(void)hipSetDevice(ctx->device);
+	if(!bytes)
+		return SW_OK;
+	return hip_check(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, ctx->stream), "copy from device");
+}
+
+int sw_host_alloc(sw_ctx* ctx, uint64_t bytes, void** out)
+{
+	if(!ctx || !out)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	(void)hipSetDevice(ctx->device);
+	*out = nullptr;
+	if(hipHostMalloc(out, std::max<uint64_t>(bytes, 16), hipHostMallocDefault) != hipSuccess)
+	{
+		(void)hipGetLastError();
+		*out = nullptr;
+		return set_error(ctx, SW_ERR_OUT_OF_MEMORY, "hipHostMalloc failed");
+	}
+	return SW_OK;
+}
+
+void sw_host_free(sw_ctx* ctx, void* p)
+{
+	if(!ctx || !p)
+		return;
+	(void)hipSetDevice(ctx->device);
+	(void)hipStreamSynchronize(ctx->stream);
+	(void)hipHostFree(p);
 }
 
 int sw_profile_enable(sw_ctx* ctx, uint32_t enable)

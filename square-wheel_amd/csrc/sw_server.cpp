@@ -31,6 +31,8 @@ struct Request
 	Clock::time_point t_submit;
 	Clock::time_point t_launch;
 	uint64_t fb_offset = 0; // in the batch output, ColorT elements
+	uint64_t stage_fb = 0, stage_ldr = 0; // byte offsets of its results in the slot's pinned staging
+	uint64_t ldr_offset = 0; // in the slot's tonemap output, Color32 elements
 };
 
 struct Slot
@@ -42,6 +44,8 @@ struct Slot
 	uint64_t out_cap = 0;
 	void* ldr_dev = nullptr;
 	uint64_t ldr_cap = 0;
+	void* staging = nullptr; // pinned host memory the results are copied into behind the batch
+	uint64_t staging_cap = 0;
 	uint64_t bytes = 0;
 };
 
@@ -115,34 +119,77 @@ int ensure_dev(sw_ctx* ctx, void*& p, uint64_t& cap, uint64_t bytes)
 	return r;
 }
 
+// Queues behind the batch: the tonemap of each 64-bit request that asks for one and the copies of
+// every result into the slot's pinned staging memory.
+int queue_results(Slot& slot)
+{
+	const float mod[3] = {1.0f, 1.0f, 1.0f};
+	int st = SW_OK;
+	for(Request& q : slot.reqs)
+	{
+		const uint64_t c = q.r.color_bits == 64 ? 8 : 4;
+		const uint32_t w = q.r.view.width, h = q.r.view.height;
+		const char* src = static_cast<const char*>(slot.out_dev) + q.fb_offset * c;
+		if(st == SW_OK && q.r.framebuffer)
+			st = sw_copy_from_device_async(slot.ctx, static_cast<char*>(slot.staging) + q.stage_fb, src, uint64_t(w) * h * c);
+		if(st == SW_OK && q.r.tonemapped && q.r.color_bits == 64)
+		{
+			uint32_t* ldr = static_cast<uint32_t*>(slot.ldr_dev) + q.ldr_offset;
+			st = sw_tonemap_device(slot.ctx, src, w, h, q.r.exposure, mod, ldr, size_t(w) * 4, nullptr);
+			if(st == SW_OK)
+				st = sw_copy_from_device_async(slot.ctx, static_cast<char*>(slot.staging) + q.stage_ldr, ldr, uint64_t(w) * h * 4);
+		}
+	}
+	return st;
+}
+
 // A finished batch: status, results into the callers' buffers, budget back.
 void collect(sw_server* s, Slot& slot)
 {
 	int status = sw_batch_wait(slot.ctx);
+	if(status == SW_OK)
+	{
+		sw_stats st{};
+		(void)sw_get_stats(slot.ctx, &st);
+		if(st.reruns)
+		{
+			// the raster stage ran again after the copies were queued: copy again
+			status = queue_results(slot);
+			if(status == SW_OK)
+				status = sw_sync(slot.ctx);
+		}
+	}
 	const uint32_t views = uint32_t(slot.reqs.size());
+	const Clock::time_point now = Clock::now();
 	for(Request& q : slot.reqs)
 	{
-		int st = status;
 		const uint64_t c = q.r.color_bits == 64 ? 8 : 4;
-		const uint32_t w = q.r.view.width, h = q.r.view.height;
-		char* src = static_cast<char*>(slot.out_dev) + q.fb_offset * c;
-		if(st == SW_OK && q.r.framebuffer)
-			st = sw_copy_from_device(slot.ctx, q.r.framebuffer, src, uint64_t(w) * h * c);
-		if(st == SW_OK && q.r.tonemapped && q.r.color_bits == 64)
-		{
-			const float mod[3] = {1.0f, 1.0f, 1.0f};
-			st = ensure_dev(slot.ctx, slot.ldr_dev, slot.ldr_cap, uint64_t(w) * h * 4);
-			if(st == SW_OK)
-				st = sw_tonemap_device(slot.ctx, src, w, h, q.r.exposure, mod, slot.ldr_dev, size_t(w) * 4, nullptr);
-			if(st == SW_OK)
-				st = sw_copy_from_device(slot.ctx, q.r.tonemapped, slot.ldr_dev, uint64_t(w) * h * 4);
-		}
-		complete(s, q, st, views, Clock::now());
+		const uint64_t px = uint64_t(q.r.view.width) * q.r.view.height;
+		if(status == SW_OK && q.r.framebuffer)
+			std::memcpy(q.r.framebuffer, static_cast<const char*>(slot.staging) + q.stage_fb, px * c);
+		if(status == SW_OK && q.r.tonemapped && q.r.color_bits == 64)
+			std::memcpy(q.r.tonemapped, static_cast<const char*>(slot.staging) + q.stage_ldr, px * 4);
+		complete(s, q, status, views, now);
 	}
 	slot.reqs.clear();
 	s->in_flight_bytes -= slot.bytes;
 	slot.bytes = 0;
 	slot.busy = false;
+}
+
+int ensure_host(sw_ctx* ctx, void*& p, uint64_t& cap, uint64_t bytes)
+{
+	if(bytes <= cap && p)
+		return SW_OK;
+	if(p)
+		sw_host_free(ctx, p);
+	p = nullptr;
+	cap = 0;
+	const uint64_t want = std::max<uint64_t>(bytes + bytes / 4, 1 << 20);
+	int r = sw_host_alloc(ctx, want, &p);
+	if(r == SW_OK)
+		cap = want;
+	return r;
 }
 
 // Forms a batch from the queue head for an idle slot and launches it. Returns false when nothing
@@ -191,13 +238,23 @@ bool launch_next(sw_server* s, Slot& slot)
 	std::vector<uint32_t> light_indices;
 	std::vector<sw_draw> draws;
 	std::vector<int32_t> vdata;
-	uint64_t fb = 0, arena = 0;
+	uint64_t fb = 0, arena = 0, stage = 0, ldr = 0;
 	const Clock::time_point now = Clock::now();
 	for(size_t i = 0; i < take.size(); ++i)
 	{
 		Request& q = take[i];
 		q.t_launch = now;
 		q.fb_offset = fb;
+		const uint64_t px = uint64_t(q.r.view.width) * q.r.view.height;
+		q.stage_fb = stage;
+		stage += px * (bits == 64 ? 8 : 4);
+		q.stage_ldr = stage;
+		q.ldr_offset = ldr;
+		if(q.r.tonemapped && bits == 64)
+		{
+			stage += px * 4;
+			ldr += px;
+		}
 		sw_view& v = views[i];
 		v = q.r.view;
 		v.first_surface = uint32_t(surfaces.size());
@@ -228,6 +285,10 @@ bool launch_next(sw_server* s, Slot& slot)
 	const uint64_t c = bits == 64 ? 8 : 4;
 	int r = ensure_dev(slot.ctx, slot.out_dev, slot.out_cap, std::max<uint64_t>(fb, 1) * c);
 	if(r == SW_OK)
+		r = ensure_dev(slot.ctx, slot.ldr_dev, slot.ldr_cap, std::max<uint64_t>(ldr, 1) * 4);
+	if(r == SW_OK)
+		r = ensure_host(slot.ctx, slot.staging, slot.staging_cap, std::max<uint64_t>(stage, 1));
+	if(r == SW_OK)
 	{
 		sw_batch b{};
 		b.views = views.data();
@@ -247,6 +308,10 @@ bool launch_next(sw_server* s, Slot& slot)
 		b.framebuffer_texels = fb;
 		b.inputs_on_device = 0;
 		r = sw_render_batch_async(slot.ctx, s->maps[map], &b, slot.out_dev);
+		if(r == SW_OK)
+			r = queue_results(slot);
+		if(r != SW_OK)
+			(void)sw_batch_wait(slot.ctx); // leave the context idle
 	}
 	if(r != SW_OK)
 	{
@@ -335,6 +400,8 @@ void sw_server_destroy(sw_server* s)
 				sw_device_free(slot.ctx, slot.out_dev);
 			if(slot.ldr_dev)
 				sw_device_free(slot.ctx, slot.ldr_dev);
+			if(slot.staging)
+				sw_host_free(slot.ctx, slot.staging);
 			sw_ctx_destroy(slot.ctx);
 		}
 	delete s;

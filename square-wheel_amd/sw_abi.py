@@ -89,7 +89,7 @@ class Batch(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("surface_texels", "lightmap_elements", "emissive_texels", "polygon_pixels",
-                                          "triangle_pixels", "blend_pixels", "spans", "parts")]
+                                          "triangle_pixels", "blend_pixels", "spans", "parts", "reruns")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -211,6 +211,9 @@ GPU_SYMBOLS = {
     "sw_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_uint32]),
     "sw_ctx_device_bytes": (C.c_uint64, [C.c_void_p]),
     "sw_render_batch_async": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Batch), C.c_void_p]),
+    "sw_copy_from_device_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    "sw_host_alloc": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "sw_host_free": (None, [C.c_void_p, C.c_void_p]),
     "sw_batch_poll": (C.c_int, [C.c_void_p]),
     "sw_batch_wait": (C.c_int, [C.c_void_p]),
     "sw_set_tile_rows": (C.c_int, [C.c_void_p, C.c_uint32]),
