@@ -40,8 +40,17 @@ enum
 	ERROR_PART_GAP = 1u << 2, // the parts of a draw did not cover consecutive rows (never expected)
 };
 
-// Surface texels are built by blocks of SURF_BLOCK threads over the flattened arena.
+// Surface texels are built by blocks of SURF_BLOCK threads over the flattened arena, each block
+// surf_block_texels() consecutive texels: up to SURF_MAX_TEXELS_PER_THREAD per thread (the block's
+// per-surface set-up is shared by more texels), fewer when the arena is too small to fill the
+// device with several rounds of blocks.
 #define SURF_BLOCK 256
+#define SURF_MAX_TEXELS_PER_THREAD 4
+__host__ __device__ inline uint32_t surf_block_texels(uint64_t arena_texels)
+{
+	const uint64_t rounds = arena_texels / (uint64_t(SURF_BLOCK) * 5120u);
+	return SURF_BLOCK * (rounds >= 4 ? 4u : rounds >= 2 ? 2u : 1u);
+}
 
 // Arena extent of one surface and the view it belongs to.
 struct SurfExt

@@ -99,9 +99,10 @@ SW_DEV void prep_surface(const sw_surface_desc* __restrict__ surfaces, uint32_t 
 	}
 	// Blocks whose first texel lies in [prev_end, end) start their search at s; the last surface
 	// also takes the blocks past its end (their texels belong to no surface).
-	const uint64_t num_blocks = (arena_texels + SURF_BLOCK - 1) / SURF_BLOCK;
-	const uint64_t b0 = (prev_end + SURF_BLOCK - 1) / SURF_BLOCK;
-	const uint64_t b1 = s + 1 == num_surfaces ? num_blocks : std::min<uint64_t>((end + SURF_BLOCK - 1) / SURF_BLOCK, num_blocks);
+	const uint64_t bt = surf_block_texels(arena_texels);
+	const uint64_t num_blocks = (arena_texels + bt - 1) / bt;
+	const uint64_t b0 = (prev_end + bt - 1) / bt;
+	const uint64_t b1 = s + 1 == num_surfaces ? num_blocks : std::min<uint64_t>((end + bt - 1) / bt, num_blocks);
 	for(uint64_t b = b0; b < b1; ++b)
 		blk_surf[b] = s;
 }
@@ -192,38 +193,222 @@ SW_DEV float specular_k(uint32_t spec, float f_base, float roughness)
 	return mul_add(f_base, 1.0f - roughness, 0.5f * roughness);
 }
 
-// One texel of build_surface_impl_6_static_params (surfaces.rs:433-852), specialisations as
-// runtime flags that are uniform per surface.
-template <class CT>
-SW_DEV typename CT::T surface_texel(
-	const MapPtrs& m, const sw_surface_desc& s, const sw_light* __restrict__ lights,
-	const uint32_t* __restrict__ light_indices, uint32_t num_light_indices, uint32_t num_lights_total, V3 cam_pos,
-	uint32_t dst_u, uint32_t dst_v)
+// Per-surface constants of the texel loop: what surfaces.rs:433-470 and :903-915 compute once per
+// surface (basis normalisation, start position, texture wrap, lightmap steps, emissive light
+// vector), computed once per surface and block by the surface's lane of the block's first wave and
+// kept in LDS for every texel of the block.
+struct SurfRec
 {
-	const sw_texture_level tex = m.levels[s.texture_level];
-	const bool use_normal_map = (tex.flags & SW_TEX_HAS_NORMAL_MAP) != 0;
-	const uint32_t spec = (tex.flags & SW_TEX_HAS_NON_ONE_ROUGHNESS) ? ((tex.flags & SW_TEX_IS_METAL) ? 2u : 1u) : 0u;
-	const bool directional = (s.flags & SW_SURF_DIRECTIONAL) != 0;
-	const bool has_lightmap = s.lightmap_offset != 0;
-	const uint32_t num_lights = s.num_lights < SW_MAX_POLYGON_LIGHTS ? s.num_lights : SW_MAX_POLYGON_LIGHTS;
+	uint64_t start, end; // arena texel range [start, end)
+	uint64_t tex_base; // m.texels index of the diffuse level's texel (0, 0)
+	uint64_t etex_base; // m.lite index of the emissive level's texel (0, 0)
+	double inv_w, inv_tw, inv_th, inv_ew, inv_eh; // double reciprocals of the divisors below
+	uint32_t w; // size[0]
+	uint32_t flags; // REC_*
+	uint32_t tw, th, ew, eh; // diffuse and emissive level sizes
+	uint32_t tw_mask, th_mask, ew_mask, eh_mask; // size - 1 for a power of two, else NO_MASK
+	int32_t tcu, tcv, etcu, etcv; // tc_start reduced modulo the level sizes (as given with *_SLOW_WRAP)
+	uint32_t first_light;
+	uint32_t lm_offset, lm_stride, lm_sl, lm_su, lm_sv;
+	int32_t elight[3]; // cvtps2dq(emissive_light * 256), lanes B, G, R
+	float inv_scale, k_shift;
+	V3 start_pos, u_vec, v_vec, u_n, v_n, normal, cam;
+};
 
-	const V3 u_vec = v3(s.basis_u[0], s.basis_u[1], s.basis_u[2]);
-	const V3 v_vec = v3(s.basis_v[0], s.basis_v[1], s.basis_v[2]);
-	const V3 basis_normal = v3(s.basis_normal[0], s.basis_normal[1], s.basis_normal[2]);
-	// :456-457
-	const V3 start_pos = add(add(v3(s.basis_start[0], s.basis_start[1], s.basis_start[2]),
-								 scale(u_vec, float(s.tc_start[0]) + 0.5f)),
-		scale(v_vec, float(s.tc_start[1]) + 0.5f));
-	const V3 u_n = scale(u_vec, inv_sqrt_fast(fmax_r(len2(u_vec), MIN_POSITIVE_VALUE)));
-	const V3 v_n = scale(v_vec, inv_sqrt_fast(fmax_r(len2(v_vec), MIN_POSITIVE_VALUE)));
+enum : uint32_t
+{
+	REC_DIFFUSE = 1u,
+	REC_EMISSIVE = 2u,
+	REC_NORMAL_MAP = 4u,
+	REC_DIRECTIONAL = 8u,
+	REC_LIGHTMAP = 16u,
+	REC_SKIP = 32u, // invalid descriptor, or neither layer: the texels are left untouched
+	REC_SLOW_WRAP = 64u, // sizes or coordinates outside the fast wrap's range
+	REC_EMISSIVE_SLOW_WRAP = 128u,
+	REC_SPEC_SHIFT = 8, // 2 bits: 0 none, 1 dielectric, 2 metal
+	REC_BLEND_SHIFT = 12, // 4 bits: the emissive layer's blend
+	REC_LIGHTS_SHIFT = 16, // clamped light count
+};
+#define NO_MASK 0xFFFFFFFFu
+#define SURF_CHUNK 32 // surface records per pass of a block
 
-	// texel fetch with rem_euclid wrap (:507-517, :843-847)
-	const int32_t tw = int32_t(tex.width), th = int32_t(tex.height);
-	const int32_t src_v = rem_euclid(s.tc_start[1] + int32_t(dst_v), th);
-	const int32_t src_u = rem_euclid(rem_euclid(s.tc_start[0], tw) + int32_t(dst_u % uint32_t(tw)), tw);
-	const sw_texel texel = m.texels[tex.offset + uint64_t(uint32_t(src_v)) * uint32_t(tw) + uint32_t(src_u)];
+// n mod d for 1 <= d <= 2^30: a power of two by its mask, otherwise through the double reciprocal,
+// whose truncated quotient is within one of n / d for every 32-bit n (relative error below 2^-51,
+// quotient below 2^32), so one correction either way makes it exact.
+SW_DEV uint32_t mod_rcp(uint32_t n, uint32_t d, uint32_t mask, double inv)
+{
+	if(mask != NO_MASK)
+		return n & mask;
+	const uint32_t q = uint32_t(double(n) * inv);
+	int32_t r = int32_t(n - q * d);
+	if(r < 0)
+		r += int32_t(d);
+	else if(r >= int32_t(d))
+		r -= int32_t(d);
+	return uint32_t(r);
+}
 
-	const V3 start_pos_v = scalar_mul_add(v_vec, float(dst_v), start_pos);
+// n / d and n mod d for 1 <= d <= 2^30, exact as mod_rcp.
+SW_DEV uint32_t div_rcp(uint32_t n, uint32_t d, double inv, uint32_t& rem)
+{
+	uint32_t q = uint32_t(double(n) * inv);
+	int32_t r = int32_t(n - q * d);
+	if(r < 0)
+	{
+		r += int32_t(d);
+		--q;
+	}
+	else if(r >= int32_t(d))
+	{
+		r -= int32_t(d);
+		++q;
+	}
+	rem = uint32_t(r);
+	return q;
+}
+
+SW_DEV uint32_t pow2_mask(uint32_t d) { return (d & (d - 1u)) == 0u ? d - 1u : NO_MASK; }
+
+// Wrap set-up of one texture level: with tc_start[1] + dst_v free of i32 overflow and sizes below
+// 2^30, rem_euclid(tc_start + d, size) == (rem_euclid(tc_start, size) + d) mod size, which the
+// texel evaluates with mod_rcp; otherwise the texel keeps the reference's expressions.
+SW_DEV bool wrap_setup(const sw_surface_desc& d, uint32_t tw, uint32_t th, int32_t& tcu, int32_t& tcv, uint32_t& tw_mask,
+	uint32_t& th_mask, double& inv_tw, double& inv_th)
+{
+	const bool fast = tw >= 1u && th >= 1u && tw <= (1u << 30) && th <= (1u << 30) &&
+		int64_t(d.tc_start[1]) + int64_t(d.size[1]) <= int64_t(INT32_MAX) + 1;
+	if(fast)
+	{
+		tcu = rem_euclid(d.tc_start[0], int32_t(tw));
+		tcv = rem_euclid(d.tc_start[1], int32_t(th));
+	}
+	else
+	{
+		tcu = d.tc_start[0];
+		tcv = d.tc_start[1];
+	}
+	tw_mask = pow2_mask(tw);
+	th_mask = pow2_mask(th);
+	inv_tw = 1.0 / double(tw);
+	inv_th = 1.0 / double(th);
+	return fast;
+}
+
+// Fills the record of a surface (ok: the descriptor was validated).
+SW_DEV void make_surf_rec(SurfRec& r, const sw_surface_desc& d, uint64_t start, uint64_t end, const sw_view& view,
+	const MapPtrs& m, bool ok)
+{
+	r.start = start;
+	r.end = end;
+	r.w = d.size[0];
+	r.inv_w = 1.0 / double(d.size[0] ? d.size[0] : 1u);
+	const bool diffuse = (d.flags & SW_SURF_DIFFUSE_LAYER) != 0;
+	const bool emissive = (d.flags & SW_SURF_EMISSIVE) && d.emissive_level < m.num_levels;
+	uint32_t fl = 0;
+	if(!ok || (!diffuse && !emissive))
+	{
+		r.flags = REC_SKIP;
+		return;
+	}
+	if(diffuse)
+	{
+		const sw_texture_level tex = m.levels[d.texture_level];
+		fl |= REC_DIFFUSE;
+		if(tex.flags & SW_TEX_HAS_NORMAL_MAP)
+			fl |= REC_NORMAL_MAP;
+		const uint32_t spec = (tex.flags & SW_TEX_HAS_NON_ONE_ROUGHNESS) ? ((tex.flags & SW_TEX_IS_METAL) ? 2u : 1u) : 0u;
+		fl |= spec << REC_SPEC_SHIFT;
+		if(d.flags & SW_SURF_DIRECTIONAL)
+			fl |= REC_DIRECTIONAL;
+		if(d.lightmap_offset != 0)
+			fl |= REC_LIGHTMAP;
+		const uint32_t nl = d.num_lights < SW_MAX_POLYGON_LIGHTS ? d.num_lights : SW_MAX_POLYGON_LIGHTS;
+		fl |= nl << REC_LIGHTS_SHIFT;
+		r.tex_base = tex.offset;
+		r.tw = tex.width;
+		r.th = tex.height;
+		if(!wrap_setup(d, tex.width, tex.height, r.tcu, r.tcv, r.tw_mask, r.th_mask, r.inv_tw, r.inv_th))
+			fl |= REC_SLOW_WRAP;
+
+		const V3 u_vec = v3(d.basis_u[0], d.basis_u[1], d.basis_u[2]);
+		const V3 v_vec = v3(d.basis_v[0], d.basis_v[1], d.basis_v[2]);
+		r.u_vec = u_vec;
+		r.v_vec = v_vec;
+		r.normal = v3(d.basis_normal[0], d.basis_normal[1], d.basis_normal[2]);
+		// :456-457
+		r.start_pos = add(add(v3(d.basis_start[0], d.basis_start[1], d.basis_start[2]), scale(u_vec, float(d.tc_start[0]) + 0.5f)),
+			scale(v_vec, float(d.tc_start[1]) + 0.5f));
+		r.u_n = scale(u_vec, inv_sqrt_fast(fmax_r(len2(u_vec), MIN_POSITIVE_VALUE)));
+		r.v_n = scale(v_vec, inv_sqrt_fast(fmax_r(len2(v_vec), MIN_POSITIVE_VALUE)));
+		r.cam = v3(view.cam_pos[0], view.cam_pos[1], view.cam_pos[2]);
+
+		const uint32_t sl = d.lightmap_scale_log2;
+		r.lm_sl = sl;
+		r.inv_scale = 1.0f / float(1u << sl);
+		r.k_shift = 0.5f * r.inv_scale;
+		r.lm_offset = d.lightmap_offset;
+		r.lm_stride = d.lightmap_size[0];
+		r.lm_su = d.lightmap_tc_shift[0];
+		r.lm_sv = d.lightmap_tc_shift[1];
+		r.first_light = d.first_light;
+	}
+	if(emissive)
+	{
+		const sw_texture_level tex = m.levels[d.emissive_level];
+		// blends past the known ones keep the destination (the switch's default), as 15 does
+		const uint32_t blend = d.emissive_blend < 15u ? d.emissive_blend : 15u;
+		fl |= REC_EMISSIVE | (blend << REC_BLEND_SHIFT);
+		r.etex_base = tex.offset;
+		r.ew = tex.width;
+		r.eh = tex.height;
+		if(!wrap_setup(d, tex.width, tex.height, r.etcu, r.etcv, r.ew_mask, r.eh_mask, r.inv_ew, r.inv_eh))
+			fl |= REC_EMISSIVE_SLOW_WRAP;
+		// light_vec = cvtps2dq(light * 256): lanes [B, G, R]
+		r.elight[0] = f32_to_i32_rne_x86(d.emissive_light[2] * 256.0f);
+		r.elight[1] = f32_to_i32_rne_x86(d.emissive_light[1] * 256.0f);
+		r.elight[2] = f32_to_i32_rne_x86(d.emissive_light[0] * 256.0f);
+	}
+	r.flags = fl;
+}
+
+// Texel coordinates of (dst_u, dst_v) in a wrapped level (:507-517, :843-847): rem_euclid of
+// tc_start + dst per axis.
+SW_DEV void wrap_texel(bool slow, int32_t tcu, int32_t tcv, uint32_t tw, uint32_t th, uint32_t tw_mask, uint32_t th_mask,
+	double inv_tw, double inv_th, uint32_t dst_u, uint32_t dst_v, uint32_t& src_u, uint32_t& src_v)
+{
+	if(slow)
+	{
+		src_v = uint32_t(rem_euclid(tcv + int32_t(dst_v), int32_t(th)));
+		src_u = uint32_t(rem_euclid(rem_euclid(tcu, int32_t(tw)) + int32_t(dst_u % tw), int32_t(tw)));
+	}
+	else
+	{
+		src_v = mod_rcp(uint32_t(tcv) + dst_v, th, th_mask, inv_th);
+		src_u = mod_rcp(uint32_t(tcu) + dst_u, tw, tw_mask, inv_tw);
+	}
+}
+
+// One texel of build_surface_impl_6_static_params (surfaces.rs:433-852), specialisations as flags
+// that are uniform per surface.
+template <class CT>
+SW_DEV typename CT::T surface_texel(const MapPtrs& m, const SurfRec& R, const sw_light* __restrict__ lights,
+	const uint32_t* __restrict__ light_indices, uint32_t num_light_indices, uint32_t num_lights_total, uint32_t dst_u,
+	uint32_t dst_v)
+{
+	const uint32_t fl = R.flags;
+	const bool use_normal_map = (fl & REC_NORMAL_MAP) != 0;
+	const uint32_t spec = (fl >> REC_SPEC_SHIFT) & 3u;
+	const bool directional = (fl & REC_DIRECTIONAL) != 0;
+	const bool has_lightmap = (fl & REC_LIGHTMAP) != 0;
+	const uint32_t num_lights = fl >> REC_LIGHTS_SHIFT;
+
+	uint32_t src_u, src_v;
+	wrap_texel((fl & REC_SLOW_WRAP) != 0, R.tcu, R.tcv, R.tw, R.th, R.tw_mask, R.th_mask, R.inv_tw, R.inv_th, dst_u, dst_v,
+		src_u, src_v);
+	const sw_texel texel = m.texels[R.tex_base + uint64_t(src_v) * R.tw + src_u];
+
+	const V3 u_vec = R.u_vec, v_vec = R.v_vec;
+	const V3 start_pos_v = scalar_mul_add(v_vec, float(dst_v), R.start_pos);
 	const V3 pos = scalar_mul_add(u_vec, float(dst_u), start_pos_v);
 
 	V3 tn;
@@ -246,19 +431,19 @@ SW_DEV typename CT::T surface_texel(
 	CV direct = cv_zero();
 	if(has_lightmap)
 	{
-		const uint32_t sl = s.lightmap_scale_log2;
-		const float inv_scale = 1.0f / float(1u << sl);
-		const float k_shift = 0.5f * inv_scale;
+		const uint32_t sl = R.lm_sl;
+		const float inv_scale = R.inv_scale;
+		const float k_shift = R.k_shift;
 		const uint32_t mask = (1u << sl) - 1u;
 		// vertical mix of the two lightmap rows (:477-502) for the two columns this texel needs
-		const uint32_t base_v = dst_v + s.lightmap_tc_shift[1];
+		const uint32_t base_v = dst_v + R.lm_sv;
 		const uint32_t lv = base_v >> sl;
 		const float kv = mul_add(float(base_v & mask), inv_scale, k_shift);
-		const uint32_t base_u = dst_u + s.lightmap_tc_shift[0];
+		const uint32_t base_u = dst_u + R.lm_su;
 		const uint32_t lu = base_u >> sl;
 		const float ku = mul_add(float(base_u & mask), inv_scale, k_shift);
-		const uint64_t row0 = uint64_t(s.lightmap_offset) + uint64_t(lv) * s.lightmap_size[0];
-		const uint64_t row1 = row0 + s.lightmap_size[0];
+		const uint64_t row0 = uint64_t(R.lm_offset) + uint64_t(lv) * R.lm_stride;
+		const uint64_t row1 = row0 + R.lm_stride;
 		const LmE l00 = lm_load(m, directional, row0 + lu);
 		const LmE l01 = lm_load(m, directional, row0 + lu + 1);
 		const LmE l10 = lm_load(m, directional, row1 + lu);
@@ -279,8 +464,8 @@ SW_DEV typename CT::T surface_texel(
 		else
 		{
 			const V3 normal = use_normal_map ? tn : v3(0.0f, 0.0f, 1.0f);
-			const V3 to_cam = sub(cam_pos, pos);
-			const V3 to_cam_ts = v3(dot(to_cam, u_n), dot(to_cam, v_n), dot(to_cam, basis_normal));
+			const V3 to_cam = sub(R.cam, pos);
+			const V3 to_cam_ts = v3(dot(to_cam, R.u_n), dot(to_cam, R.v_n), dot(to_cam, R.normal));
 			const float n_dot = dot(to_cam_ts, normal);
 			const V3 refl = sub(scale(normal, 2.0f * n_dot), to_cam_ts);
 			const float refl_len2 = fmax_r(len2(refl), MIN_POSITIVE_VALUE);
@@ -318,6 +503,7 @@ SW_DEV typename CT::T surface_texel(
 	if(num_lights > 0)
 	{
 		// :683-831
+		const V3 u_n = R.u_n, v_n = R.v_n, basis_normal = R.normal;
 		V3 normal;
 		if(use_normal_map)
 			normal = v3(mul_add(tn.x, u_n.x, mul_add(tn.y, v_n.x, tn.z * basis_normal.x)),
@@ -330,7 +516,7 @@ SW_DEV typename CT::T surface_texel(
 		float refl_len2 = MIN_POSITIVE_VALUE, sk = 0.0f, inv_r = 0.0f;
 		if(spec != 0)
 		{
-			const V3 to_cam = sub(cam_pos, pos);
+			const V3 to_cam = sub(R.cam, pos);
 			const float n_dot = dot(to_cam, normal);
 			refl = sub(scale(normal, 2.0f * n_dot), to_cam);
 			refl_len2 = fmax_r(len2(refl), MIN_POSITIVE_VALUE);
@@ -338,9 +524,10 @@ SW_DEV typename CT::T surface_texel(
 			sk = specular_k(spec, fresnel_factor_base(cos_a), tr);
 			inv_r = inv_fast(tr);
 		}
+		const uint32_t first_light = R.first_light;
 		for(uint32_t li = 0; li < num_lights; ++li)
 		{
-			const uint32_t idx_pos = s.first_light + li;
+			const uint32_t idx_pos = first_light + li;
 			if(idx_pos >= num_light_indices)
 				break;
 			const uint32_t lidx = light_indices[idx_pos];
@@ -381,21 +568,16 @@ SW_DEV typename CT::T surface_texel(
 
 // surfaces.rs:903-971, one texel
 template <class CT>
-SW_DEV typename CT::T emissive_texel(const MapPtrs& m, const sw_surface_desc& s, uint32_t blend, typename CT::T dst,
-	uint32_t dst_u, uint32_t dst_v)
+SW_DEV typename CT::T emissive_texel(const MapPtrs& m, const SurfRec& R, typename CT::T dst, uint32_t dst_u, uint32_t dst_v)
 {
-	const sw_texture_level tex = m.levels[s.emissive_level];
-	const int32_t tw = int32_t(tex.width), th = int32_t(tex.height);
-	const int32_t src_v = rem_euclid(s.tc_start[1] + int32_t(dst_v), th);
-	const int32_t src_u = rem_euclid(rem_euclid(s.tc_start[0], tw) + int32_t(dst_u % uint32_t(tw)), tw);
-	const uint32_t t = m.lite[tex.offset + uint64_t(uint32_t(src_v)) * uint32_t(tw) + uint32_t(src_u)];
-	// light_vec = cvtps2dq(light * 256): lanes [B, G, R]
-	const int32_t lb = f32_to_i32_rne_x86(s.emissive_light[2] * 256.0f);
-	const int32_t lg = f32_to_i32_rne_x86(s.emissive_light[1] * 256.0f);
-	const int32_t lr = f32_to_i32_rne_x86(s.emissive_light[0] * 256.0f);
+	uint32_t src_u, src_v;
+	wrap_texel((R.flags & REC_EMISSIVE_SLOW_WRAP) != 0, R.etcu, R.etcv, R.ew, R.eh, R.ew_mask, R.eh_mask, R.inv_ew, R.inv_eh,
+		dst_u, dst_v, src_u, src_v);
+	const uint32_t t = m.lite[R.etex_base + uint64_t(src_v) * R.ew + src_u];
+	const int32_t lb = R.elight[0], lg = R.elight[1], lr = R.elight[2];
 	const CI tc = Color32T::to_ci(t);
 	const CI mod = CI{wmul(tc.b, lb) >> 8, wmul(tc.g, lg) >> 8, wmul(tc.r, lr) >> 8, wmul(tc.a, 0) >> 8};
-	switch(blend)
+	switch((R.flags >> REC_BLEND_SHIFT) & 15u)
 	{
 	case SW_BLEND_NONE: return CT::from_ci(mod);
 	case SW_BLEND_AVERAGE:
@@ -421,105 +603,136 @@ SW_DEV typename CT::T emissive_texel(const MapPtrs& m, const sw_surface_desc& s,
 	}
 }
 
-
-// One block of surface texels (SURF_BLOCK consecutive arena texels).
+// One block of surface texels: surf_block_texels() consecutive arena texels, up to
+// SURF_MAX_TEXELS_PER_THREAD per thread (strided by the block size, so each pass is coalesced). The block's first wave builds
+// the records of up to SURF_CHUNK consecutive surfaces overlapping the block (one lane each), then
+// every thread finds its texels' surfaces among them by a forward scan; a block overlapping more
+// surfaces takes further passes.
 template <class CT>
 SW_DEV void build_surfaces_block(uint32_t block, const MapPtrs& m, const SurfaceArgs& a, const ViewsArg& views)
 {
+	__shared__ SurfRec recs[SURF_CHUNK];
+	__shared__ uint32_t sh_base, sh_count;
 	const sw_surface_desc* __restrict__ surfaces = a.surfaces;
 	const uint32_t num_surfaces = a.num_surfaces;
 	const SurfExt* __restrict__ surf_ext = a.surf_ext;
-	const sw_light* __restrict__ lights = a.lights;
-	const uint32_t* __restrict__ light_indices = a.light_indices;
-	const uint32_t num_lights = a.num_lights, num_light_indices = a.num_light_indices;
 	typename CT::T* __restrict__ arena = static_cast<typename CT::T*>(a.arena);
-	const uint64_t idx = uint64_t(block) * SURF_BLOCK + threadIdx.x;
-	uint32_t s;
-	uint64_t start;
-	uint32_t view_index = 0;
-	if(a.blk_surf)
+	const uint32_t bt = surf_block_texels(a.arena_texels);
+	const uint64_t b0 = uint64_t(block) * bt;
+	if(b0 >= a.arena_texels || num_surfaces == 0)
+		return; // uniform over the block
+	const uint64_t b1 = std::min<uint64_t>(b0 + bt, a.arena_texels);
+	const uint32_t tid = threadIdx.x;
+	const bool search = a.blk_surf == nullptr;
+	const uint64_t search_base = search ? views[0].v.surface_arena_offset : 0;
+	if(tid < 64)
 	{
-		if(idx >= a.arena_texels)
-			return;
-		// The block's first surface comes from the table; a block spans few surfaces (most surfaces
-		// are larger than a block), so a short forward scan finds each texel's.
-		s = a.blk_surf[block];
-		while(s < num_surfaces && surf_ext[s].end <= idx)
-			++s;
-		if(s >= num_surfaces)
-			return;
-		const sw_surface_desc& d = surfaces[s];
-		const SurfExt ext = surf_ext[s];
-		start = ext.end - uint64_t(d.size[0]) * d.size[1];
-		view_index = ext.view;
-	}
-	else
-	{
-		// One view, no table: a 64-ary search by the wave (one probe per lane and level) finds the
-		// last surface starting at or before the wave's first texel, then each lane scans forward.
-		// The descriptors were not validated ahead, so each lane validates its own surface.
-		const uint64_t base = views[0].v.surface_arena_offset;
-		const uint64_t wave_first = idx & ~uint64_t(63);
-		const uint32_t lane = threadIdx.x & 63;
-		uint32_t lo = 0, hi = num_surfaces;
-		while(hi - lo > 1)
+		uint32_t first;
+		if(!search)
+			first = a.blk_surf[block]; // the surface holding b0, or the next one after a gap
+		else
 		{
-			const uint32_t step = (hi - lo + 63) / 64;
-			const uint32_t probe = lo + lane * step;
-			const bool le = probe < hi && base + surfaces[probe].pixels_offset <= wave_first;
-			const uint64_t mask = __ballot(le);
-			const uint32_t last = mask ? 63u - uint32_t(__builtin_clzll(mask)) : 0u;
-			lo = lo + last * step;
-			hi = std::min(lo + step, hi);
+			// One view, no table: a 64-ary search by the wave (one probe per lane and level) finds
+			// the last surface starting at or before b0. The descriptors were not validated ahead,
+			// so each record's lane validates its surface below.
+			const uint32_t lane = tid;
+			uint32_t lo = 0, hi = num_surfaces;
+			while(hi - lo > 1)
+			{
+				const uint32_t step = (hi - lo + 63) / 64;
+				const uint32_t probe = lo + lane * step;
+				const bool le = probe < hi && search_base + surfaces[probe].pixels_offset <= b0;
+				const uint64_t mask = __ballot(le);
+				const uint32_t last = mask ? 63u - uint32_t(__builtin_clzll(mask)) : 0u;
+				lo = lo + last * step;
+				hi = std::min(lo + step, hi);
+			}
+			first = lo;
 		}
-		if(idx >= a.arena_texels)
-			return;
-		s = lo;
-		uint64_t end = 0;
-		for(;;)
+		if(tid == 0)
+			sh_base = first;
+	}
+	__syncthreads();
+	uint32_t base = sh_base;
+	for(;;)
+	{
+		if(tid < 64)
 		{
-			if(s >= num_surfaces)
-				return;
-			const sw_surface_desc& d = surfaces[s];
-			start = base + d.pixels_offset;
-			end = start + uint64_t(d.size[0]) * d.size[1];
-			if(end > idx)
+			const uint32_t s = base + tid;
+			bool overlaps = false;
+			if(tid < SURF_CHUNK && s < num_surfaces)
+			{
+				const sw_surface_desc& d = surfaces[s];
+				const uint64_t size = uint64_t(d.size[0]) * d.size[1];
+				uint64_t start, end;
+				uint32_t view = 0;
+				bool ok = true;
+				if(!search)
+				{
+					const SurfExt e = surf_ext[s];
+					end = e.end;
+					start = end - size;
+					view = e.view;
+				}
+				else
+				{
+					start = search_base + d.pixels_offset;
+					end = start + size;
+				}
+				overlaps = start < b1;
+				if(overlaps)
+				{
+					if(search)
+					{
+						ok = surface_desc_ok(d, end, a.arena_texels, m);
+						if(s > 0)
+						{
+							const sw_surface_desc& p = surfaces[s - 1];
+							ok = ok && search_base + p.pixels_offset + uint64_t(p.size[0]) * p.size[1] <= start;
+						}
+						if(!ok)
+							atomicOr(&a.counters[COUNTER_ERROR], ERROR_BAD_SURFACE);
+					}
+					make_surf_rec(recs[tid], d, start, end, views[view].v, m, ok);
+				}
+			}
+			// surfaces follow arena order: the overlapping ones are a prefix of the chunk
+			const uint64_t mask = __ballot(overlaps);
+			if(tid == 0)
+				sh_count = uint32_t(__builtin_ctzll(~mask));
+		}
+		__syncthreads();
+		const uint32_t count = sh_count;
+		uint32_t r = 0;
+#pragma unroll 1
+		for(uint32_t k = 0; k < bt / SURF_BLOCK; ++k)
+		{
+			const uint64_t idx = b0 + uint64_t(k) * SURF_BLOCK + tid;
+			if(idx >= b1)
 				break;
-			++s;
+			while(r < count && recs[r].end <= idx)
+				++r;
+			if(r == count)
+				break; // this and later texels lie past the pass's surfaces
+			const SurfRec& R = recs[r];
+			if(idx < R.start || (R.flags & REC_SKIP))
+				continue; // gap between surfaces, or a surface left untouched
+			uint32_t du;
+			const uint32_t dv = div_rcp(uint32_t(idx - R.start), R.w, R.inv_w, du);
+			typename CT::T out = 0;
+			if(R.flags & REC_DIFFUSE)
+				out = surface_texel<CT>(m, R, a.lights, a.light_indices, a.num_light_indices, a.num_lights, du, dv);
+			else if(((R.flags >> REC_BLEND_SHIFT) & 15u) != SW_BLEND_NONE)
+				out = arena[idx]; // an emissive layer without a diffuse layer blends over what the arena holds
+			if(R.flags & REC_EMISSIVE)
+				out = emissive_texel<CT>(m, R, out, du, dv);
+			arena[idx] = out;
 		}
-		if(idx < start)
-			return; // gap between surfaces
-		const sw_surface_desc& d = surfaces[s];
-		bool ok = surface_desc_ok(d, end, a.arena_texels, m);
-		if(s > 0)
-		{
-			const sw_surface_desc& p = surfaces[s - 1];
-			ok = ok && base + p.pixels_offset + uint64_t(p.size[0]) * p.size[1] <= start;
-		}
-		if(!ok)
-		{
-			atomicOr(&a.counters[COUNTER_ERROR], ERROR_BAD_SURFACE);
-			return;
-		}
+		if(count < SURF_CHUNK)
+			break;
+		__syncthreads(); // the next pass rewrites the records
+		base += SURF_CHUNK;
 	}
-	const sw_surface_desc& d = surfaces[s];
-	const sw_view& view = views[view_index].v;
-	const V3 cam = v3(view.cam_pos[0], view.cam_pos[1], view.cam_pos[2]);
-	const uint32_t local = uint32_t(idx - start);
-	const uint32_t du = local % d.size[0];
-	const uint32_t dv = local / d.size[0];
-	const bool diffuse = (d.flags & SW_SURF_DIFFUSE_LAYER) != 0;
-	const bool emissive = (d.flags & SW_SURF_EMISSIVE) && d.emissive_level < m.num_levels;
-	if(!diffuse && !emissive)
-		return; // the reference leaves such a surface's texels untouched
-	typename CT::T out = 0;
-	if(diffuse)
-		out = surface_texel<CT>(m, d, lights, light_indices, num_light_indices, num_lights, cam, du, dv);
-	else if(d.emissive_blend != SW_BLEND_NONE)
-		out = arena[idx]; // an emissive layer without a diffuse layer blends over what the arena holds
-	if(emissive)
-		out = emissive_texel<CT>(m, d, d.emissive_blend, out, du, dv);
-	arena[idx] = out;
 }
 
 template <class CT>
@@ -1404,9 +1617,9 @@ __global__ __launch_bounds__(256) void k_setup_draws(SetupArgs a, const ViewsArg
 }
 
 // Draw setup and surface texels in one launch (the host prepared and validated the surfaces): the
-// two are independent, so they run side by side.
+// two are independent, so they run side by side. Held to 128 VGPRs: four waves per SIMD.
 template <int LANES, class CT>
-__global__ __launch_bounds__(256) void k_setup_and_surfaces(SetupArgs a, const ViewsArg views, SurfaceArgs sa, MapPtrs m)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_setup_and_surfaces(SetupArgs a, const ViewsArg views, SurfaceArgs sa, MapPtrs m)
 {
 	if(blockIdx.x < a.setup_blocks)
 		setup_draws_block<LANES>(blockIdx.x, a, views);
@@ -1863,7 +2076,8 @@ void launch_setup_and_surfaces(hipStream_t st, uint32_t lanes, uint32_t color_bi
 {
 	const uint32_t groups = lanes == 256 ? 1u : 4u;
 	a.setup_blocks = (a.num_draws + groups - 1) / groups;
-	const uint64_t surf_blocks = sa.num_surfaces ? (sa.arena_texels + SURF_BLOCK - 1) / SURF_BLOCK : 0;
+	const uint64_t bt = surf_block_texels(sa.arena_texels);
+	const uint64_t surf_blocks = sa.num_surfaces ? (sa.arena_texels + bt - 1) / bt : 0;
 	const uint64_t blocks = a.setup_blocks + surf_blocks;
 	if(blocks == 0)
 		return;
@@ -1878,7 +2092,8 @@ void launch_build_surfaces(hipStream_t st, uint32_t color_bits, const MapPtrs& m
 {
 	if(sa.arena_texels == 0 || sa.num_surfaces == 0)
 		return;
-	const uint64_t blocks = (sa.arena_texels + SURF_BLOCK - 1) / SURF_BLOCK;
+	const uint64_t bt = surf_block_texels(sa.arena_texels);
+	const uint64_t blocks = (sa.arena_texels + bt - 1) / bt;
 	if(color_bits == 64)
 		hipLaunchKernelGGL(k_build_surfaces<Color64T>, dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, sa, views, counters);
 	else

@@ -379,7 +379,8 @@ int launch_stages(sw_ctx* ctx, const sw_ctx::Launch& L, bool surfaces, bool rast
 	sw_map* map = L.map;
 	const uint32_t nd = raster ? L.num_draws : 0;
 	const uint32_t ns = surfaces ? L.num_surfaces : 0;
-	const uint64_t surf_blocks = (L.arena_texels + SURF_BLOCK - 1) / SURF_BLOCK;
+	const uint64_t surf_bt = surf_block_texels(L.arena_texels);
+	const uint64_t surf_blocks = (L.arena_texels + surf_bt - 1) / surf_bt;
 	// a draw's span records never exceed its clip rect's height, at most the tallest view
 	const uint64_t stride = L.max_view_height;
 	const bool static_spans = nd > 0 && uint64_t(nd) * stride <= STATIC_SPAN_LIMIT;
@@ -467,7 +468,8 @@ int host_prepare_surfaces(sw_ctx* ctx, const sw_map* map, const std::vector<View
 {
 	std::vector<SurfExt>& ext = ctx->host_ext;
 	std::vector<uint32_t>& blk = ctx->host_blk;
-	const uint64_t num_blocks = (arena_texels + SURF_BLOCK - 1) / SURF_BLOCK;
+	const uint64_t bt = surf_block_texels(arena_texels);
+	const uint64_t num_blocks = (arena_texels + bt - 1) / bt;
 	ext.resize(std::max<uint32_t>(ns, 1));
 	blk.resize(std::max<uint64_t>(num_blocks, 1));
 	const std::vector<sw_texture_level>& lv = map->host_levels;
@@ -501,8 +503,8 @@ int host_prepare_surfaces(sw_ctx* ctx, const sw_map* map, const std::vector<View
 				"surface " + std::to_string(i) +
 					": invalid descriptor (arena order/overlap, texture level, or lightmap range out of bounds)");
 		ext[i] = SurfExt{end, view, 0u};
-		const uint64_t b0 = (prev_end + SURF_BLOCK - 1) / SURF_BLOCK;
-		const uint64_t b1 = i + 1 == ns ? num_blocks : std::min<uint64_t>((end + SURF_BLOCK - 1) / SURF_BLOCK, num_blocks);
+		const uint64_t b0 = (prev_end + bt - 1) / bt;
+		const uint64_t b1 = i + 1 == ns ? num_blocks : std::min<uint64_t>((end + bt - 1) / bt, num_blocks);
 		for(uint64_t b = b0; b < b1; ++b)
 			blk[b] = i;
 		prev_end = end;

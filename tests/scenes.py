@@ -80,7 +80,7 @@ def make_map(rng, num_textures=6, num_lite=4, num_lm=4000):
     return m
 
 
-def make_surfaces(rng, m, n, lights_count, max_size=40, lm_total=4000):
+def make_surfaces(rng, m, n, lights_count, max_size=40, lm_total=4000, tc_range=300):
     surfs = np.zeros(n, dtype=A.SURFACE_DTYPE)
     light_indices = []
     off = 0
@@ -89,7 +89,7 @@ def make_surfaces(rng, m, n, lights_count, max_size=40, lm_total=4000):
         w = int(rng.integers(1, max_size + 1))
         h = int(rng.integers(1, max_size + 1))
         s["size"] = (w, h)
-        s["tc_start"] = rng.integers(-300, 300, size=2)
+        s["tc_start"] = rng.integers(-tc_range, tc_range, size=2)
         s["texture_level"] = m.tex_levels[i % len(m.tex_levels)]
         # world-space basis of a texel (the scale stays sensible: 0.5..4 units per texel)
         u = rng.normal(size=3)
@@ -242,16 +242,17 @@ def make_draws(rng, m, W, H, surfs, n, triangles=True):
 
 
 def make_batch(seed, color_bits=32, view_sizes=((160, 120),), surfaces_per_view=24, draws_per_view=40,
-               num_lights=8, clear=True, triangles=True):
+               num_lights=8, clear=True, triangles=True, max_surface_size=40, tc_range=300, num_lm=4000):
     rng = np.random.default_rng(seed)
-    m = make_map(rng)
+    m = make_map(rng, num_lm=num_lm)
     lights = make_lights(rng, num_lights)
     views = np.zeros(len(view_sizes), dtype=A.VIEW_DTYPE)
     all_s, all_li, all_d, all_v = [], [], [], []
     arena_off = fb_off = 0
     li_base = vi_base = 0
     for k, (W, H) in enumerate(view_sizes):
-        s, li, texels = make_surfaces(rng, m, surfaces_per_view, num_lights)
+        s, li, texels = make_surfaces(rng, m, surfaces_per_view, num_lights, max_size=max_surface_size, tc_range=tc_range,
+                                      lm_total=num_lm)
         s["first_light"] += li_base
         d, vd = make_draws(rng, m, W, H, s, draws_per_view, triangles=triangles)
         d["first_vertex_int"] += vi_base

@@ -253,3 +253,35 @@ def test_invalid_surface_is_an_error_and_recovers(gpu_ctx, views, device_inputs)
         gpu_ctx.release_map(mh)
     ofb, _, _, _ = oracle.render(m, b)
     assert np.array_equal(fb, ofb)
+
+
+@pytest.mark.parametrize("case", ["tiny", "large", "wrap_overflow"])
+def test_surface_block_layouts(gpu_ctx, case):
+    """The surface kernel's block structure: blocks overlapping more surfaces than one pass of
+    per-surface records holds (many 1..3-texel surfaces), arenas large enough for several texels
+    per thread (> 5.2 Mtexels), and texture coordinates whose tc_start + v overflows i32 (the
+    reference's wrapping expression is kept); arena and frame equal the oracle's."""
+    _say(f"case surface blocks {case}")
+    if case == "tiny":
+        m, b = scenes.make_batch(61, view_sizes=((96, 64), (80, 60)), surfaces_per_view=3000, draws_per_view=20,
+                                 max_surface_size=3)
+    elif case == "large":
+        m, b = scenes.make_batch(62, view_sizes=((128, 96), (96, 64), (64, 64)), surfaces_per_view=170, draws_per_view=20,
+                                 max_surface_size=300, num_lm=8000)
+        assert b.arena_texels > 5_300_000, b.arena_texels
+    else:
+        m, b = scenes.make_batch(63, view_sizes=((160, 120),), surfaces_per_view=40, draws_per_view=30)
+        s = b.surfaces
+        s["tc_start"][::3, 1] = 2**31 - 7
+        s["tc_start"][1::3, 0] = -2**31 + 2
+        s["tc_start"][2::3, 1] = -2**31 + 1
+    mh = gpu_ctx.upload_map(m)
+    try:
+        fb = gpu_ctx.render_batch(mh, b)
+        arena = gpu_ctx.read_surfaces(b.arena_texels, 32)
+    finally:
+        gpu_ctx.release_map(mh)
+    ofb, oarena, _, _ = oracle.render(m, b, mode=oracle.MODE_EXACT)
+    assert np.array_equal(arena, oarena[:b.arena_texels]), \
+        f"surface arena differs in {np.count_nonzero(arena != oarena[:b.arena_texels])} texels"
+    assert np.array_equal(fb, ofb)
