@@ -1704,7 +1704,10 @@ SW_DEV int32_t polygon_texel_index(const int32_t* sv, int32_t j, uint32_t tw, ui
 		const int32_t inv_z = wadd(sv[0], wmul(j, sv[1]));
 		uint32_t div = uint32_t(inv_z) >> INV_Z_PRE_SHIFT;
 		div = div ? div : 1u;
-		const uint32_t z = (1u << Z_CALC_SHIFT) / div;
+		// (1 << Z_CALC_SHIFT) / div: the correctly rounded double quotient truncates to the integer
+		// quotient exactly (its fraction is 0 or at least 1 / div, far above the double's rounding
+		// error of at most 2^-53 relative), at a fraction of the integer division's cost.
+		const uint32_t z = uint32_t(double(1u << Z_CALC_SHIFT) / double(div));
 		const int32_t t0 = wadd(sv[2], wmul(j, sv[3]));
 		const int32_t t1 = wadd(sv[4], wmul(j, sv[5]));
 		u = uint32_t((uint64_t(z) * uint64_t(int64_t(t0))) >> TC_FINAL_SHIFT);
@@ -1848,34 +1851,41 @@ __global__ __launch_bounds__(256) void k_raster_tiles(MapPtrs m, const sw_draw* 
 #pragma unroll
 				for(int rr = 0; rr < ROWS; ++rr)
 				{
-					tix[g][rr] = -1;
-					if(k >= nh)
-						continue;
+					// every branch below is uniform over the wave (item values are read with
+					// readlane); the lane's own span test only masks the result
 					const int sl = g * ROWS + rr;
 					const int32_t sx0 = rl(mine.x0, sl), sx1 = rl(mine.x1, sl);
-					if(sx0 >= sx1 || sx1 <= x0 || sx0 >= x0 + 64)
-						continue;
-					int32_t sv[6];
-#pragma unroll
-					for(int i = 0; i < 6; ++i)
-						sv[i] = rl(mine.v[i], sl);
-					if(x < sx0 || x >= sx1)
-						continue;
-					const int32_t j = x - sx0;
-					if(kind == 3)
+					int32_t ti = -1;
+					if(k < nh && sx0 < sx1 && sx1 > x0 && sx0 < x0 + 64)
 					{
-						const bool upper = y0 + rr - part_row0 >= rows0;
-						const int32_t d0 = upper ? rl(pd[5], kk) : rl(pd[0], kk);
-						const int32_t d1 = upper ? rl(pd[6], kk) : rl(pd[1], kk);
-						const uint32_t u = uint32_t(fixed16_floor_to_int(wadd(sv[0], wmul(j, d0))));
-						const uint32_t v = uint32_t(fixed16_floor_to_int(wadd(sv[1], wmul(j, d1))));
-						if(u < tw && v < th)
-							tix[g][rr] = int32_t(u + v * tw);
+						int32_t sv[6];
+#pragma unroll
+						for(int i = 0; i < 6; ++i)
+							sv[i] = rl(mine.v[i], sl);
+						const int32_t j = x - sx0;
+						const bool in = x >= sx0 && x < sx1;
+						if(kind == 3)
+						{
+							const bool upper = y0 + rr - part_row0 >= rows0;
+							const int32_t d0u = rl(pd[5], kk), d0l = rl(pd[0], kk);
+							const int32_t d1u = rl(pd[6], kk), d1l = rl(pd[1], kk);
+							const int32_t d0 = upper ? d0u : d0l, d1 = upper ? d1u : d1l;
+							const uint32_t u = uint32_t(fixed16_floor_to_int(wadd(sv[0], wmul(j, d0))));
+							const uint32_t v = uint32_t(fixed16_floor_to_int(wadd(sv[1], wmul(j, d1))));
+							ti = in && u < tw && v < th ? int32_t(u + v * tw) : -1;
+						}
+						else if(kind == 0)
+						{
+							const int32_t t = polygon_texel_index<0>(sv, j, tw, th);
+							ti = in ? t : -1;
+						}
+						else
+						{
+							const int32_t t = polygon_texel_index<1>(sv, j, tw, th);
+							ti = in ? t : -1;
+						}
 					}
-					else if(kind == 0)
-						tix[g][rr] = polygon_texel_index<0>(sv, j, tw, th);
-					else
-						tix[g][rr] = polygon_texel_index<1>(sv, j, tw, th);
+					tix[g][rr] = ti;
 				}
 			}
 
@@ -1913,20 +1923,25 @@ __global__ __launch_bounds__(256) void k_raster_tiles(MapPtrs m, const sw_draw* 
 					continue;
 				}
 				const int32_t part_row0 = rl(h_row_first, k), rows0 = rl(h_rows0, k);
+				const int32_t dr_u = rl(pd[7], k), dr_l = rl(pd[2], k);
+				const int32_t dg_u = rl(pd[8], k), dg_l = rl(pd[3], k);
+				const int32_t db_u = rl(pd[9], k), db_l = rl(pd[4], k);
 #pragma unroll
 				for(int rr = 0; rr < ROWS; ++rr)
 				{
-					if(tix[g][rr] < 0)
-						continue;
 					const int sl = g * ROWS + rr;
 					const bool upper = y0 + rr - part_row0 >= rows0;
-					const int32_t j = x - rl(mine.x0, sl);
-					const int32_t dr = upper ? rl(pd[7], k) : rl(pd[2], k);
-					const int32_t dg = upper ? rl(pd[8], k) : rl(pd[3], k);
-					const int32_t db = upper ? rl(pd[9], k) : rl(pd[4], k);
-					const int32_t lr = wadd(rl(mine.v[2], sl), wmul(j, dr));
-					const int32_t lg = wadd(rl(mine.v[3], sl), wmul(j, dg));
-					const int32_t lb = wadd(rl(mine.v[4], sl), wmul(j, db));
+					const int32_t sx0 = rl(mine.x0, sl);
+					const int32_t l0 = rl(mine.v[2], sl), l1 = rl(mine.v[3], sl), l2 = rl(mine.v[4], sl);
+					if(tix[g][rr] < 0)
+						continue;
+					const int32_t j = x - sx0;
+					const int32_t dr = upper ? dr_u : dr_l;
+					const int32_t dg = upper ? dg_u : dg_l;
+					const int32_t db = upper ? db_u : db_l;
+					const int32_t lr = wadd(l0, wmul(j, dr));
+					const int32_t lg = wadd(l1, wmul(j, dg));
+					const int32_t lb = wadd(l2, wmul(j, db));
 					const uint32_t tv = uint32_t(texel[g][rr]);
 					const CI tc = Color32T::to_ci(tv);
 					const CI lit = CI{wmul(tc.b, lb) >> 16, wmul(tc.g, lg) >> 16, wmul(tc.r, lr) >> 16, 0};
