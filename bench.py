@@ -157,8 +157,10 @@ def hdr_leg(ctx, args):
 def serving_leg(args):
     """Open-loop serving: four resident maps (16x16x4 .. 48x48x4 grids), Poisson arrivals at fixed
     rates, each request a random map, pose, resolution (log-uniform 320x200 .. 2560x1440, 16:10) and
-    colour depth (Color32, or Color64 tonemapped on the device). The server batches requests on the
-    fly onto three concurrent slots under an HBM budget; latency is arrival -> result in host memory."""
+    colour depth (Color32, or Color64 tonemapped on the device and returned as the tonemapped image
+    only). The server batches requests on the fly onto three concurrent slots under an HBM budget;
+    results are copied by the device into pinned host buffers; latency is arrival -> result in host
+    memory."""
     rng = np.random.default_rng(args.seed + 77)
     grids = [(16, 16, 4), (24, 24, 4), (32, 32, 4), (48, 48, 4)]
     maps = [sw_host.large_map(args.seed + i, grid=g) for i, g in enumerate(grids)]
@@ -175,12 +177,17 @@ def serving_leg(args):
     srv = sw_abi.Server(slots=3, hbm_budget=16 << 30, max_views=32)
     ids = [srv.add_map(m.data) for m in maps]
     out = {"workload": f"4 maps {grids}, {len(pool)} distinct requests, mean {mean_px / 1e6:.2f} Mpixel, "
-                       "log-uniform 320x200..2560x1440, half Color64 + tonemap", "rates": []}
+                       "log-uniform 320x200..2560x1440, half Color64 + tonemap (tonemapped image returned), "
+                       "results in pinned host memory", "rates": []}
     # warm-up: every request once
+    def submit(mi, f):
+        return srv.submit(ids[mi], f, tonemap_exposure=1.0 if f["color_bits"] == 64 else None, hdr=False, pinned=True)
+
     for mi, f in pool:
-        srv.submit(ids[mi], f, tonemap_exposure=1.0 if f["color_bits"] == 64 else None)
-    srv.drain()
-    for rate in (200, 500, 1000, 2000, 5000):
+        submit(mi, f)
+    for c in srv.drain():
+        srv.release(c)
+    for rate in (200, 1000, 5000, 10000, 20000):
         n = max(int(rate * args.serve_seconds), 50)
         arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n))
         picks = rng.integers(0, len(pool), size=n)
@@ -193,7 +200,7 @@ def serving_leg(args):
             now = time.perf_counter() - t0
             while k < n and arrivals[k] <= now:
                 mi, f = pool[int(picks[k])]
-                t = srv.submit(ids[mi], f, tonemap_exposure=1.0 if f["color_bits"] == 64 else None)
+                t = submit(mi, f)
                 submitted[t] = arrivals[k]
                 k += 1
             for c in srv.poll():
@@ -202,6 +209,7 @@ def serving_leg(args):
                 views.append(c["batch_views"])
                 if c["status"] != 0:
                     raise RuntimeError(f"request failed: {c}")
+                srv.release(c)
         wall = time.perf_counter() - t0
         lat = np.array(lat)
         row = {"offered_req_s": rate, "requests": n, "achieved_req_s": round(n / wall, 1),

@@ -42,11 +42,16 @@ typedef struct sw_frame_request
 	const int32_t* vertex_data;
 	uint64_t num_vertex_ints;
 	uint64_t surface_arena_texels;
-	void* framebuffer; /* host destination: width * height ColorT elements, pitch = width */
-	uint32_t* tonemapped; /* optional host destination (64-bit requests): width * height Color32 */
+	void* framebuffer; /* optional destination: width * height ColorT elements, pitch = width */
+	uint32_t* tonemapped; /* optional destination (64-bit requests): width * height Color32 */
 	float exposure; /* tonemap exposure (postprocessor.rs:1005-1017), modulation 1 */
-	uint32_t reserved;
+	uint32_t flags; /* SW_REQ_* */
 } sw_frame_request;
+
+/* The result buffers are pinned host memory (sw_server_host_alloc / sw_host_alloc) or device
+ * memory: the DMA engine copies the results into them directly, behind the batch. Without the flag
+ * they are ordinary host memory and the results pass through the slot's pinned staging memory. */
+#define SW_REQ_PINNED_OUTPUT 1u
 
 typedef struct sw_completion
 {
@@ -89,6 +94,11 @@ int sw_server_drain(sw_server* server, sw_completion* out, uint32_t max, uint32_
 uint32_t sw_server_pending(const sw_server* server);
 
 int sw_server_get_stats(const sw_server* server, sw_server_stats* out);
+
+/* Pinned host memory for SW_REQ_PINNED_OUTPUT result buffers (freed with sw_server_host_free, before
+ * the server is destroyed). */
+int sw_server_host_alloc(sw_server* server, uint64_t bytes, void** out);
+void sw_server_host_free(sw_server* server, void* p);
 
 #ifdef __cplusplus
 }

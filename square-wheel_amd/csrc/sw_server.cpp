@@ -129,15 +129,19 @@ int queue_results(Slot& slot)
 	{
 		const uint64_t c = q.r.color_bits == 64 ? 8 : 4;
 		const uint32_t w = q.r.view.width, h = q.r.view.height;
+		const bool direct = (q.r.flags & SW_REQ_PINNED_OUTPUT) != 0;
 		const char* src = static_cast<const char*>(slot.out_dev) + q.fb_offset * c;
 		if(st == SW_OK && q.r.framebuffer)
-			st = sw_copy_from_device_async(slot.ctx, static_cast<char*>(slot.staging) + q.stage_fb, src, uint64_t(w) * h * c);
+			st = sw_copy_from_device_async(slot.ctx,
+				direct ? q.r.framebuffer : static_cast<char*>(slot.staging) + q.stage_fb, src, uint64_t(w) * h * c);
 		if(st == SW_OK && q.r.tonemapped && q.r.color_bits == 64)
 		{
 			uint32_t* ldr = static_cast<uint32_t*>(slot.ldr_dev) + q.ldr_offset;
 			st = sw_tonemap_device(slot.ctx, src, w, h, q.r.exposure, mod, ldr, size_t(w) * 4, nullptr);
 			if(st == SW_OK)
-				st = sw_copy_from_device_async(slot.ctx, static_cast<char*>(slot.staging) + q.stage_ldr, ldr, uint64_t(w) * h * 4);
+				st = sw_copy_from_device_async(slot.ctx,
+					direct ? static_cast<void*>(q.r.tonemapped) : static_cast<char*>(slot.staging) + q.stage_ldr, ldr,
+					uint64_t(w) * h * 4);
 		}
 	}
 	return st;
@@ -165,9 +169,10 @@ void collect(sw_server* s, Slot& slot)
 	{
 		const uint64_t c = q.r.color_bits == 64 ? 8 : 4;
 		const uint64_t px = uint64_t(q.r.view.width) * q.r.view.height;
-		if(status == SW_OK && q.r.framebuffer)
+		const bool staged = status == SW_OK && !(q.r.flags & SW_REQ_PINNED_OUTPUT);
+		if(staged && q.r.framebuffer)
 			std::memcpy(q.r.framebuffer, static_cast<const char*>(slot.staging) + q.stage_fb, px * c);
-		if(status == SW_OK && q.r.tonemapped && q.r.color_bits == 64)
+		if(staged && q.r.tonemapped && q.r.color_bits == 64)
 			std::memcpy(q.r.tonemapped, static_cast<const char*>(slot.staging) + q.stage_ldr, px * 4);
 		complete(s, q, status, views, now);
 	}
@@ -246,13 +251,16 @@ bool launch_next(sw_server* s, Slot& slot)
 		q.t_launch = now;
 		q.fb_offset = fb;
 		const uint64_t px = uint64_t(q.r.view.width) * q.r.view.height;
+		const bool staged = !(q.r.flags & SW_REQ_PINNED_OUTPUT);
 		q.stage_fb = stage;
-		stage += px * (bits == 64 ? 8 : 4);
+		if(staged && q.r.framebuffer)
+			stage += px * (bits == 64 ? 8 : 4);
 		q.stage_ldr = stage;
 		q.ldr_offset = ldr;
 		if(q.r.tonemapped && bits == 64)
 		{
-			stage += px * 4;
+			if(staged)
+				stage += px * 4;
 			ldr += px;
 		}
 		sw_view& v = views[i];
@@ -409,6 +417,22 @@ void sw_server_destroy(sw_server* s)
 
 const char* sw_server_last_error(const sw_server* s) { return s ? s->error.c_str() : "null server"; }
 
+int sw_server_host_alloc(sw_server* s, uint64_t bytes, void** out)
+{
+	if(!s || !out)
+		return SW_ERR_INVALID_ARGUMENT;
+	const int r = sw_host_alloc(s->slots[0].ctx, bytes, out);
+	if(r != SW_OK)
+		return fail(s, r, sw_last_error(s->slots[0].ctx));
+	return SW_OK;
+}
+
+void sw_server_host_free(sw_server* s, void* p)
+{
+	if(s && p)
+		sw_host_free(s->slots[0].ctx, p);
+}
+
 int sw_server_add_map(sw_server* s, const sw_map_desc* desc, uint32_t* out_map)
 {
 	if(!s || !desc || !out_map)
@@ -433,6 +457,8 @@ int sw_server_submit(sw_server* s, const sw_frame_request* r, uint64_t* out_tick
 	if((r->num_surfaces && !r->surfaces) || (r->num_draws && !r->draws) || (r->num_vertex_ints && !r->vertex_data) ||
 		(r->num_lights && !r->lights) || (r->num_light_indices && !r->light_indices))
 		return fail(s, SW_ERR_INVALID_ARGUMENT, "null command buffer");
+	if(r->flags & ~SW_REQ_PINNED_OUTPUT)
+		return fail(s, SW_ERR_INVALID_ARGUMENT, "unknown request flags");
 	Request q;
 	q.ticket = s->next_ticket++;
 	q.r = *r;

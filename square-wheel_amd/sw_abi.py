@@ -443,7 +443,7 @@ class FrameRequest(C.Structure):
         ("lights", C.c_void_p), ("light_indices", C.c_void_p), ("num_light_indices", C.c_uint32),
         ("num_draws", C.c_uint32), ("draws", C.c_void_p), ("vertex_data", C.c_void_p),
         ("num_vertex_ints", C.c_uint64), ("surface_arena_texels", C.c_uint64),
-        ("framebuffer", C.c_void_p), ("tonemapped", C.c_void_p), ("exposure", C.c_float), ("reserved", C.c_uint32),
+        ("framebuffer", C.c_void_p), ("tonemapped", C.c_void_p), ("exposure", C.c_float), ("flags", C.c_uint32),
     ]
 
 
@@ -471,7 +471,11 @@ GPU_SYMBOLS.update({
     "sw_server_drain": (C.c_int, [C.c_void_p, C.POINTER(Completion), C.c_uint32, C.POINTER(C.c_uint32)]),
     "sw_server_pending": (C.c_uint32, [C.c_void_p]),
     "sw_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    "sw_server_host_alloc": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "sw_server_host_free": (None, [C.c_void_p, C.c_void_p]),
 })
+
+REQ_PINNED_OUTPUT = 1
 
 
 class Server:
@@ -485,6 +489,8 @@ class Server:
             raise SwError(f"sw_server_create failed ({r})")
         self.h = h
         self._keep = {}  # ticket -> result arrays (kept alive until completion)
+        self._pool = {}  # bytes -> free pinned buffers (pointers)
+        self._pinned = {}  # pointer -> bytes, every pinned buffer allocated
 
     def check(self, r, what):
         if r != SW_OK:
@@ -492,21 +498,56 @@ class Server:
 
     def close(self):
         if self.h:
+            if self._pinned and self.pending():
+                self.drain()  # no copy may still target a pinned buffer when it is freed
+            for ptr in list(self._pinned):
+                self.lib.sw_server_host_free(self.h, C.c_void_p(ptr))
+            self._pinned.clear()
+            self._pool.clear()
             self.lib.sw_server_destroy(self.h)
             self.h = None
+
+    def _pinned_array(self, shape, dtype):
+        """A numpy view of a pinned host buffer from the pool (SW_REQ_PINNED_OUTPUT results)."""
+        n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        free = self._pool.setdefault(n, [])
+        if free:
+            ptr = free.pop()
+        else:
+            p = C.c_void_p()
+            self.check(self.lib.sw_server_host_alloc(self.h, n, C.byref(p)), "sw_server_host_alloc")
+            ptr = int(p.value)
+            self._pinned[ptr] = n
+        buf = (C.c_uint8 * n).from_address(ptr)
+        return np.frombuffer(buf, dtype=dtype).reshape(shape)
+
+    def release(self, completion):
+        """Returns a completion's pinned result buffers to the pool (their contents become invalid)."""
+        for k in ("framebuffer", "tonemapped"):
+            a = completion.get(k)
+            if a is not None:
+                ptr = a.ctypes.data
+                if ptr in self._pinned:
+                    self._pool[self._pinned[ptr]].append(ptr)
+            completion[k] = None
 
     def add_map(self, m: MapData):
         idx = C.c_uint32()
         self.check(self.lib.sw_server_add_map(self.h, C.byref(m.desc()), C.byref(idx)), "sw_server_add_map")
         return int(idx.value)
 
-    def submit(self, map_index, frame, tonemap_exposure=None):
-        """frame: a front-end frame dict. Returns the ticket; results via poll()/drain()."""
+    def submit(self, map_index, frame, tonemap_exposure=None, hdr=True, pinned=False):
+        """frame: a front-end frame dict. Returns the ticket; results via poll()/drain().
+        hdr=False with a tonemap returns only the tonemapped image of a 64-bit request; pinned=True
+        takes the result buffers from a pool of pinned host memory the device copies into directly
+        (hand them back with release())."""
         v = frame["view"]
         w, h = int(v["width"]), int(v["height"])
         bits = int(frame["color_bits"])
-        fb = np.zeros((h, w), dtype=np.uint64 if bits == 64 else np.uint32)
-        tm = np.zeros((h, w), dtype=np.uint32) if (tonemap_exposure is not None and bits == 64) else None
+        want_tm = tonemap_exposure is not None and bits == 64
+        alloc = self._pinned_array if pinned else (lambda shape, dtype: np.zeros(shape, dtype=dtype))
+        fb = alloc((h, w), np.uint64 if bits == 64 else np.uint32) if (hdr or not want_tm) else None
+        tm = alloc((h, w), np.uint32) if want_tm else None
         req = FrameRequest()
         req.map = map_index
         req.color_bits = bits
@@ -518,9 +559,10 @@ class Server:
         req.draws, req.num_draws = arrays["draws"].ctypes.data, len(arrays["draws"])
         req.vertex_data, req.num_vertex_ints = arrays["vertex_data"].ctypes.data, len(arrays["vertex_data"])
         req.surface_arena_texels = int(frame["arena_texels"])
-        req.framebuffer = fb.ctypes.data
+        req.framebuffer = fb.ctypes.data if fb is not None else None
         req.tonemapped = tm.ctypes.data if tm is not None else None
         req.exposure = float(tonemap_exposure or 1.0)
+        req.flags = REQ_PINNED_OUTPUT if pinned else 0
         t = C.c_uint64()
         self.check(self.lib.sw_server_submit(self.h, C.byref(req), C.byref(t)), "sw_server_submit")
         self._keep[int(t.value)] = (fb, tm)

@@ -38,15 +38,19 @@ def _requests(maps, n, seed):
     return reqs
 
 
-def test_mixed_requests_equal_alone(gpu_ctx, maps):
-    _say("case server mix")
+@pytest.mark.parametrize("pinned", [False, True])
+def test_mixed_requests_equal_alone(gpu_ctx, maps, pinned):
+    """pinned: result buffers in pinned host memory (copied into directly by the device), and every
+    third 64-bit request returns only its tonemapped image."""
+    _say(f"case server mix pinned={pinned}")
     reqs = _requests(maps, 28, 5)
     srv = sw_abi.Server(slots=3, hbm_budget=2 << 30, max_views=6)
     try:
         ids = [srv.add_map(m.data) for m in maps]
         tickets = {}
         for k, (mi, f) in enumerate(reqs):
-            t = srv.submit(ids[mi], f, tonemap_exposure=1.0 if f["color_bits"] == 64 else None)
+            t = srv.submit(ids[mi], f, tonemap_exposure=1.0 if f["color_bits"] == 64 else None,
+                           pinned=pinned, hdr=not (pinned and k % 3 == 0))
             tickets[t] = k
             if k % 5 == 4:
                 # requests keep arriving while earlier batches run
@@ -57,30 +61,35 @@ def test_mixed_requests_equal_alone(gpu_ctx, maps):
             if isinstance(key, tuple):
                 done.append(tickets.pop(key))
         st = srv.stats()
+        assert len(done) == len(reqs), (len(done), len(reqs))
+        assert st["batches"] < len(reqs), st  # requests were batched
+        # pinned results stay valid until released: compare before closing the server
+        handles = [gpu_ctx.upload_map(m.data) for m in maps]
+        try:
+            checked = 0
+            for c in done:
+                assert c["status"] == 0, c
+                mi, f = reqs[tickets[c["ticket"]]]
+                b = sw_host.batch_from_frames([f])
+                alone = gpu_ctx.render_plain(handles[mi], b)
+                if c["framebuffer"] is not None:
+                    assert np.array_equal(c["framebuffer"], alone), f"request {tickets[c['ticket']]} differs from its solo render"
+                else:
+                    assert f["color_bits"] == 64
+                if f["color_bits"] == 64:
+                    ref, _ = oracle.tonemap(alone, 1.0, (1.0, 1.0, 1.0))
+                    assert np.array_equal(c["tonemapped"], ref)
+                # a few against the oracle as well
+                if checked < 4 and c["framebuffer"] is not None:
+                    ref, _, _, _ = oracle.render(maps[mi].data, b)
+                    assert np.array_equal(c["framebuffer"].reshape(-1), ref)
+                    checked += 1
+                srv.release(c)
+        finally:
+            for h in handles:
+                gpu_ctx.release_map(h)
     finally:
         srv.close()
-    assert len(done) == len(reqs), (len(done), len(reqs))
-    assert st["batches"] < len(reqs), st  # requests were batched
-    handles = [gpu_ctx.upload_map(m.data) for m in maps]
-    try:
-        for c in done:
-            assert c["status"] == 0, c
-            mi, f = reqs[tickets[c["ticket"]]]
-            b = sw_host.batch_from_frames([f])
-            alone = gpu_ctx.render_plain(handles[mi], b)
-            assert np.array_equal(c["framebuffer"], alone), f"request {tickets[c['ticket']]} differs from its solo render"
-            if f["color_bits"] == 64:
-                ref, _ = oracle.tonemap(alone, 1.0, (1.0, 1.0, 1.0))
-                assert np.array_equal(c["tonemapped"], ref)
-        # a few against the oracle as well
-        for c in done[:4]:
-            mi, f = reqs[tickets[c["ticket"]]]
-            b = sw_host.batch_from_frames([f])
-            ref, _, _, _ = oracle.render(maps[mi].data, b)
-            assert np.array_equal(c["framebuffer"].reshape(-1), ref)
-    finally:
-        for h in handles:
-            gpu_ctx.release_map(h)
 
 
 def test_budget_gates_admission(maps):
