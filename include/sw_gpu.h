@@ -222,6 +222,10 @@ typedef struct sw_view
 	uint64_t fb_offset; /* element offset of this view's framebuffer in the batch output (pitch = width) */
 	uint64_t surface_arena_offset; /* added to every surface pixels_offset / polygon texture_offset */
 } sw_view;
+/* The views' surface_arena_offset values are non-decreasing. A view's arena region runs from its
+ * offset to the next view's (the last view's to the arena's end): its surfaces lie inside it (an
+ * error otherwise) and its polygon draws read their surface texels inside it (a draw that does not
+ * is skipped). */
 
 /*
  * A batch of views in one call. Pointers are host pointers (inputs_on_device = 0) or device

@@ -68,6 +68,7 @@ struct ViewExt
 	uint32_t cols; // 64-pixel columns
 	uint32_t first_tile; // global tile (wave) index of tile 0
 	uint32_t pad;
+	uint64_t arena_end; // end of the view's arena region: the next view's offset, or the arena's end
 };
 
 // The views of a batch: a device array, or (a single view) the view itself, passed by value in

@@ -80,7 +80,7 @@ SW_DEV void prep_surface(const sw_surface_desc* __restrict__ surfaces, uint32_t 
 	const uint64_t start = views[view].v.surface_arena_offset + d.pixels_offset;
 	const uint64_t end = start + uint64_t(d.size[0]) * d.size[1];
 	surf_ext[s] = SurfExt{end, view, 0u};
-	bool bad = !surface_desc_ok(d, end, arena_texels, m);
+	bool bad = !surface_desc_ok(d, end, views[view].arena_end, m);
 	uint64_t prev_end = 0;
 	if(s > 0)
 	{
@@ -684,7 +684,7 @@ SW_DEV void build_surfaces_block(uint32_t block, const MapPtrs& m, const Surface
 				{
 					if(search)
 					{
-						ok = surface_desc_ok(d, end, a.arena_texels, m);
+						ok = surface_desc_ok(d, end, views[0].arena_end, m);
 						if(s > 0)
 						{
 							const sw_surface_desc& p = surfaces[s - 1];
@@ -1477,7 +1477,7 @@ SW_DEV void setup_draws_block(uint32_t block, const SetupArgs& a, const ViewsArg
 			tex_texels < (1ull << 31) && d.blend <= SW_BLEND_ALPHA_BLEND;
 		if(d.kind == SW_DRAW_POLYGON)
 		{
-			const uint64_t limit = d.texture_source == SW_TEXSRC_SKYBOX ? skybox_texels : arena_texels;
+			const uint64_t limit = d.texture_source == SW_TEXSRC_SKYBOX ? skybox_texels : views[lo].arena_end;
 			const uint64_t tbase = d.texture_source == SW_TEXSRC_SKYBOX ? 0 : view.surface_arena_offset;
 			valid = valid && d.num_vertices >= 3 && d.num_vertices <= SW_MAX_POLYGON_VERTICES &&
 				uint64_t(d.first_vertex_int) + 2ull * d.num_vertices <= num_vertex_ints && tbase + d.texture_offset + tex_texels <= limit;

@@ -286,8 +286,8 @@ void collect_timings(sw_ctx* ctx)
 }
 
 // Per-view tiling of a batch (views are always host memory).
-int build_view_ext(sw_ctx* ctx, const sw_view* views, uint32_t n, std::vector<ViewExt>& out, uint32_t& total_tiles,
-	uint32_t* max_height = nullptr)
+int build_view_ext(sw_ctx* ctx, const sw_view* views, uint32_t n, uint64_t arena_texels, std::vector<ViewExt>& out,
+	uint32_t& total_tiles, uint32_t* max_height = nullptr)
 {
 	if(max_height)
 		*max_height = 0;
@@ -313,6 +313,12 @@ int build_view_ext(sw_ctx* ctx, const sw_view* views, uint32_t n, std::vector<Vi
 		e.cols = (v.width + 63) / 64;
 		e.first_tile = uint32_t(tiles);
 		e.pad = 0;
+		// a view's surfaces, and the arena texels its draws read, lie in [its offset, the next
+		// view's offset) (the last view: to the arena's end)
+		if(v.surface_arena_offset > arena_texels || (i > 0 && v.surface_arena_offset < views[i - 1].surface_arena_offset))
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT,
+				"view " + std::to_string(i) + ": surface arena offsets must be non-decreasing and inside the arena");
+		e.arena_end = i + 1 < n ? std::max(views[i + 1].surface_arena_offset, v.surface_arena_offset) : arena_texels;
 		tiles += uint64_t(e.num_bands) * e.cols;
 		if(tiles >= (1ull << 31))
 			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "batch too large");
@@ -482,7 +488,8 @@ int host_prepare_surfaces(sw_ctx* ctx, const sw_map* map, const std::vector<View
 		const sw_surface_desc& d = s[i];
 		const uint64_t start = (ve.empty() ? 0 : ve[view].v.surface_arena_offset) + d.pixels_offset;
 		const uint64_t end = start + uint64_t(d.size[0]) * d.size[1];
-		bool bad = end > arena_texels || d.size[0] > 65536u || d.size[1] > 65536u || start < prev_end;
+		const uint64_t limit = ve.empty() ? arena_texels : ve[view].arena_end;
+		bool bad = end > limit || d.size[0] > 65536u || d.size[1] > 65536u || start < prev_end;
 		if((d.flags & SW_SURF_DIFFUSE_LAYER) && (d.texture_level >= lv.size() || (lv[d.texture_level].flags & SW_TEX_LITE_LEVEL)))
 			bad = true;
 		if((d.flags & SW_SURF_EMISSIVE) && d.emissive_level != SW_NO_TEXTURE &&
@@ -612,7 +619,7 @@ int launch_frame(sw_ctx* ctx, uint32_t num_draws, uint64_t num_vertex_ints, bool
 	std::vector<ViewExt> ve;
 	sw_ctx::Launch& L = ctx->last;
 	L = sw_ctx::Launch{};
-	if((r = build_view_ext(ctx, &v, 1, ve, L.total_tiles, &L.max_view_height)))
+	if((r = build_view_ext(ctx, &v, 1, ctx->frame_arena_texels, ve, L.total_tiles, &L.max_view_height)))
 		return r;
 	L.surface_mode = SURF_SEARCH;
 	if((r = set_views(ctx, ve, true)))
@@ -1132,13 +1139,10 @@ static int prepare_batch(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_
 	if(b->color_bits != 32 && b->color_bits != 64)
 		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "color_bits must be 32 or 64");
 	int r;
-	if((r = build_view_ext(ctx, b->views, b->num_views, ve, L.total_tiles, &L.max_view_height)))
+	if((r = build_view_ext(ctx, b->views, b->num_views, b->surface_arena_texels, ve, L.total_tiles, &L.max_view_height)))
 		return r;
 	if((r = validate_views_fit(ctx, ve, b->framebuffer_texels, b->num_surfaces, b->num_draws)))
 		return r;
-	for(const ViewExt& e : ve)
-		if(e.v.surface_arena_offset > b->surface_arena_texels)
-			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "view arena offset outside the arena");
 	if((r = set_views(ctx, ve, !for_graph)))
 		return r;
 	if((r = ensure(ctx, ctx->arena, std::max<uint64_t>(b->surface_arena_texels, 1) * color_bytes(b->color_bits))))
@@ -1335,7 +1339,9 @@ int sw_render_batch_graph(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out
 		// positions, framebuffer offsets and clear flags may change between replays)
 		std::vector<ViewExt> ve;
 		uint32_t tiles = 0;
-		if((r = build_view_ext(ctx, b->views, b->num_views, ve, tiles)) || (r = set_views(ctx, ve, false)))
+		if((r = build_view_ext(ctx, b->views, b->num_views, b->surface_arena_texels, ve, tiles)) ||
+			(r = validate_views_fit(ctx, ve, b->framebuffer_texels, b->num_surfaces, b->num_draws)) ||
+			(r = set_views(ctx, ve, false)))
 			return r;
 	}
 	if((r = hip_check(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream), "graph launch")))

@@ -285,3 +285,55 @@ def test_surface_block_layouts(gpu_ctx, case):
     assert np.array_equal(arena, oarena[:b.arena_texels]), \
         f"surface arena differs in {np.count_nonzero(arena != oarena[:b.arena_texels])} texels"
     assert np.array_equal(fb, ofb)
+
+
+@pytest.mark.parametrize("device_inputs", [False, True])
+def test_many_views_large_arena(gpu_ctx, device_inputs):
+    """Nine views over 4.6 Mtexels of surfaces (two texels per thread in the surface blocks, the
+    block table from the host or from the setup launch); arena, frame and graph replays equal the
+    oracle's."""
+    import sw_abi
+    _say(f"case many views large arena device={device_inputs}")
+    sizes = ((96, 64), (80, 60), (128, 72), (64, 64), (100, 50), (72, 96), (90, 90), (120, 64), (60, 40))
+    m, b = scenes.make_batch(71, view_sizes=sizes, surfaces_per_view=40, draws_per_view=30, max_surface_size=300,
+                             num_lm=8000)
+    assert b.arena_texels > (1 << 22), b.arena_texels
+    mh = gpu_ctx.upload_map(m)
+    try:
+        if device_inputs:
+            dev = sw_abi.DeviceCommandBuffers(gpu_ctx, b)
+            out = gpu_ctx.alloc(b.fb_texels * 4)
+            try:
+                fb = gpu_ctx.render_batch(mh, b, device_inputs=dev, out=out)
+                arena = gpu_ctx.read_surfaces(b.arena_texels, 32)
+                graphs = [gpu_ctx.render_batch(mh, b, graph=True, device_inputs=dev, out=out) for _ in range(3)]
+            finally:
+                out.free()
+                dev.free()
+        else:
+            fb = gpu_ctx.render_batch(mh, b)
+            arena = gpu_ctx.read_surfaces(b.arena_texels, 32)
+            graphs = []
+    finally:
+        gpu_ctx.release_map(mh)
+    ofb, oarena, _, _ = oracle.render(m, b, mode=oracle.MODE_EXACT)
+    assert np.array_equal(arena, oarena[:b.arena_texels])
+    assert np.array_equal(fb, ofb)
+    for g in graphs:
+        assert np.array_equal(g, ofb)
+
+
+def test_decreasing_arena_offsets_are_an_error(gpu_ctx):
+    import sw_abi
+    _say("case decreasing arena offsets")
+    m, b = scenes.make_batch(73, view_sizes=((64, 48), (64, 48)), surfaces_per_view=4, draws_per_view=4)
+    views = b.views.copy()
+    views["surface_arena_offset"] = views["surface_arena_offset"][::-1]
+    bad = sw_abi.BatchData(views, b.surfaces, b.lights, b.light_indices, b.draws, b.vertex_data, 32,
+                           arena_texels=b.arena_texels)
+    mh = gpu_ctx.upload_map(m)
+    try:
+        with pytest.raises(sw_abi.SwError, match="non-decreasing"):
+            gpu_ctx.render_batch(mh, bad)
+    finally:
+        gpu_ctx.release_map(mh)
