@@ -111,14 +111,27 @@ struct DrawRec
 	uint32_t span_base; // span record of row_first; row y uses span_base + (y - row_first)
 	int32_t row_first; // first row of the draw's parts
 	uint32_t rows0; // rows of the first part (a triangle's lower part)
+	uint32_t tex_w, tex_h; // texture size
 	uint32_t pad;
+	uint64_t tex_offset; // texture's first element: in the arena (view offset included), skybox or lite texels
+};
+
+// A draw's screen bounds for the raster kernel's binning (8 bytes, so that testing 64 draws
+// against a tile moves little data; the full record is loaded for the draws that hit). An empty
+// draw has x_min > x_max.
+struct DrawBounds
+{
+	int16_t x_min, x_max; // columns [x_min, x_max)
+	int16_t row_min, row_max; // rows [row_min, row_max)
 };
 
 enum
 {
 	DRAW_STAT_TRIANGLE = 1u << 0,
 	DRAW_STAT_READS_DST = 1u << 1, // the blending mode reads the destination
-	DRAW_STAT_KIND_SHIFT = 4, // 0 perspective, 1 line z, 2 affine, 3 triangle
+	DRAW_STAT_SKYBOX = 1u << 2, // a polygon textured from the skybox (else the surface arena)
+	DRAW_STAT_KIND_SHIFT = 4, // 2 bits: 0 perspective, 1 line z, 2 affine, 3 triangle
+	DRAW_STAT_BLEND_SHIFT = 8, // 4 bits: SW_BLEND_*
 };
 
 struct MapPtrs
@@ -144,6 +157,7 @@ struct SetupArgs
 	PartRec* parts;
 	SpanRec* spans;
 	DrawRec* drec;
+	DrawBounds* bounds;
 	uint32_t* counters;
 	uint64_t span_capacity;
 	uint64_t span_stride; // > 0: draw i owns span records [i * stride, (i + 1) * stride), no allocation
@@ -181,7 +195,7 @@ void launch_build_surfaces(hipStream_t st, uint32_t color_bits, const MapPtrs& m
 void launch_setup_and_surfaces(hipStream_t st, uint32_t lanes, uint32_t color_bits, SetupArgs a, const ViewsArg& views,
 	const SurfaceArgs& sa, const MapPtrs& m);
 // Kernel 3: one wave per 64 x tile_rows tile (binning fused).
-void launch_raster_tiles(hipStream_t st, uint32_t color_bits, uint32_t tile_rows, const MapPtrs& m, const sw_draw* draws,
+void launch_raster_tiles(hipStream_t st, uint32_t color_bits, uint32_t tile_rows, const MapPtrs& m, const DrawBounds* bounds,
 	const DrawRec* drec, const PartRec* parts, const SpanRec* spans, const ViewsArg& views, const void* arena,
 	const void* skybox, void* fb, uint32_t total_tiles, const uint32_t* counters);
 // Kernel 4 (one block): per-draw statistics -> result block (host-mapped), live block cleared.

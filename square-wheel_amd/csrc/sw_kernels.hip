@@ -1444,7 +1444,9 @@ SW_DEV void setup_draws_block(uint32_t block, const SetupArgs& a, const ViewsArg
 	__shared__ unsigned long long block_base;
 	__shared__ int32_t red_i[4][4];
 	__shared__ unsigned long long red_p[4];
-	const uint32_t group = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+	// the group is uniform over each wave (LANES >= 64): readfirstlane tells the compiler, so the
+	// draw's parameters load through the scalar unit and its uniform values stay in SGPRs
+	const uint32_t group = __builtin_amdgcn_readfirstlane(threadIdx.x / LANES), lane = threadIdx.x % LANES;
 	const uint32_t di = block * GROUPS + group;
 	const bool active = di < num_draws;
 
@@ -1485,6 +1487,17 @@ SW_DEV void setup_draws_block(uint32_t block, const SetupArgs& a, const ViewsArg
 		else
 			valid = valid && d.kind == SW_DRAW_TRIANGLE && uint64_t(d.first_vertex_int) + 21ull <= num_vertex_ints &&
 				d.texture_offset + tex_texels <= lite_texels;
+		if(lane == 0)
+		{
+			// the texture part of the draw's record, written now so that nothing of it stays live
+			// through the edge walk (the rest is written at the end)
+			DrawRec& w = drec[di];
+			w.tex_w = uint32_t(d.texture_size[0]);
+			w.tex_h = uint32_t(d.texture_size[1]);
+			w.pad = 0;
+			w.tex_offset = (d.kind == SW_DRAW_POLYGON && d.texture_source != SW_TEXSRC_SKYBOX ? view.surface_arena_offset : 0) +
+				d.texture_offset;
+		}
 		v = vdata + d.first_vertex_int;
 	}
 
@@ -1582,21 +1595,23 @@ SW_DEV void setup_draws_block(uint32_t block, const SetupArgs& a, const ViewsArg
 		if(st.gap)
 			atomicOr(&counters[COUNTER_ERROR], ERROR_PART_GAP);
 		const uint32_t kind = d.kind == SW_DRAW_TRIANGLE ? 3u : (d.mode == SW_TC_LINE_Z ? 1u : (d.mode == SW_TC_AFFINE ? 2u : 0u));
-		DrawRec r;
+		DrawRec& r = drec[di];
 		r.num_parts = (st.overflow || st.gap) ? 0 : st.num_parts;
 		r.span_base = uint32_t(base);
 		r.row_first = st.row_first;
 		r.rows0 = st.rows0;
-		r.pad = 0;
 		r.row_min = row_min;
 		r.row_max = row_max;
 		r.x_min = x_min;
 		r.x_max = x_max;
 		r.pixels = uint32_t(pixels);
 		r.stat = (d.kind == SW_DRAW_TRIANGLE ? DRAW_STAT_TRIANGLE : 0u) | (reads ? DRAW_STAT_READS_DST : 0u) |
-			(kind << DRAW_STAT_KIND_SHIFT);
+			(d.kind == SW_DRAW_POLYGON && d.texture_source == SW_TEXSRC_SKYBOX ? DRAW_STAT_SKYBOX : 0u) |
+			(kind << DRAW_STAT_KIND_SHIFT) | ((d.blend & 15u) << DRAW_STAT_BLEND_SHIFT);
 		r.rows = a.span_stride ? uint32_t(st.next_span - base) : rows;
-		drec[di] = r;
+		const bool empty = r.num_parts == 0 || row_min >= row_max || x_min >= x_max;
+		a.bounds[di] = empty ? DrawBounds{1, 0, 1, 0}
+							 : DrawBounds{int16_t(x_min), int16_t(x_max), int16_t(row_min), int16_t(row_max)};
 	}
 }
 
@@ -1721,6 +1736,16 @@ SW_DEV int32_t polygon_texel_index(const int32_t* sv, int32_t j, uint32_t tw, ui
 	return (u < tw && v < th) ? int32_t(u + v * tw) : -1;
 }
 
+// Element i of a global-memory array whose address travels as an integer (a readlane result): the
+// global address space keeps the load a global_load (a generic pointer would make it a flat load,
+// which also waits on, and occupies, the LDS counter).
+template <class T>
+SW_DEV T global_at(uint64_t addr, int32_t i)
+{
+	typedef __attribute__((address_space(1))) const T* GlobalPtr;
+	return ((GlobalPtr)addr)[i];
+}
+
 // Wave lane exchange: pull `v` from lane `src` / push `v` to lane `dst` (ds_bpermute / ds_permute).
 SW_DEV int32_t lane_pull(int32_t v, uint32_t src) { return __builtin_amdgcn_ds_bpermute(int32_t(src << 2), v); }
 SW_DEV int32_t lane_push(int32_t v, uint32_t dst) { return __builtin_amdgcn_ds_permute(int32_t(dst << 2), v); }
@@ -1732,17 +1757,22 @@ SW_DEV int32_t rl(int32_t v, uint32_t lane) { return __builtin_amdgcn_readlane(v
 // computes the texel address of its pixel for all 16 items, issues the 16 fetches together and
 // blends them in painter's order into its ROWS colours.
 template <class CT, int ROWS>
-__global__ __launch_bounds__(256) void k_raster_tiles(MapPtrs m, const sw_draw* __restrict__ draws,
+__global__ __launch_bounds__(256) void k_raster_tiles(MapPtrs m, const DrawBounds* __restrict__ bounds,
 	const DrawRec* __restrict__ drec, const PartRec* __restrict__ parts, const SpanRec* __restrict__ spans,
 	const ViewsArg views, const typename CT::T* __restrict__ arena, const typename CT::T* __restrict__ skybox,
 	typename CT::T* __restrict__ fb, uint32_t total_tiles, const uint32_t* __restrict__ counters)
 {
 	typedef typename CT::T C;
 	constexpr int GD = ROWS >= 16 ? 1 : 16 / ROWS;
-	const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	// wave-uniform (readfirstlane): the tile, its view and every per-item value derived from them
+	// stay scalar, so the uniform tests below are scalar branches
+	const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
 	const uint32_t t = blockIdx.x * 4 + wave;
-	if(t >= total_tiles || (counters[COUNTER_ERROR] & ERROR_SPAN_OVERFLOW))
+	if(t >= total_tiles)
 		return;
+	// the error flag is read now and tested before the first span record is read (span records may
+	// be incomplete after an overflow), so its latency overlaps the tile's first loads
+	const uint32_t err = counters[COUNTER_ERROR];
 	uint32_t lo = 0, hi = views.count;
 	while(hi - lo > 1)
 	{
@@ -1774,60 +1804,59 @@ __global__ __launch_bounds__(256) void k_raster_tiles(MapPtrs m, const sw_draw* 
 		acc[r] = c;
 	}
 
-	const C* view_arena = arena + ve.v.surface_arena_offset;
 	const uint32_t first_draw = ve.v.first_draw, num_draws = ve.v.num_draws;
 	const uint64_t lanes_below = (1ull << lane) - 1ull;
 	for(uint32_t base = 0; base < num_draws; base += 64)
 	{
 		// ordered binning: lane i tests draw base + i against this tile
 		const uint32_t di = first_draw + base + lane;
-		DrawRec r = DrawRec{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 		bool hit = false;
 		if(base + lane < num_draws)
 		{
-			r = drec[di];
-			hit = r.num_parts > 0 && r.row_min < y0 + ROWS && r.row_max > y0 && r.x_min < x0 + 64 && r.x_max > x0;
+			const DrawBounds b = bounds[di];
+			hit = b.row_min < y0 + ROWS && b.row_max > y0 && b.x_min < x0 + 64 && b.x_max > x0;
 		}
 		const uint64_t mask = __ballot(hit);
 		if(!mask)
 			continue;
 		const uint32_t nh = uint32_t(__popcll(mask));
-		// compact the hits into lanes 0 .. nh - 1, in draw order (a permutation of the lanes)
+		// compact the hits into lanes 0 .. nh - 1, in draw order (a permutation of the lanes); the
+		// hits' full records are loaded by those lanes
 		const uint32_t dst = hit ? uint32_t(__popcll(mask & lanes_below)) : nh + uint32_t(__popcll(~mask & lanes_below));
 		const uint32_t h_di = uint32_t(lane_push(int32_t(di), dst));
-		const int32_t h_row_first = lane_push(r.row_first, dst);
-		const int32_t h_rows = lane_push(int32_t(r.rows), dst);
-		const int32_t h_span_base = lane_push(int32_t(r.span_base), dst);
-		const int32_t h_rows0 = lane_push(int32_t(r.rows0), dst);
-		const int32_t h_kind = lane_push(int32_t(r.stat >> DRAW_STAT_KIND_SHIFT), dst);
-		// per-draw parameters, one draw per lane
-		int32_t h_blend = 0, h_tw = 1, h_th = 1;
-		uint64_t h_tex = 0;
-		int32_t pd[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+		DrawRec r{};
 		if(lane < nh)
+			r = drec[h_di];
+		// per-draw parameters, one hit per lane: all in the record, except a triangle's two parts'
+		// steps
+		const int32_t h_row_first = r.row_first;
+		const int32_t h_rows = int32_t(r.rows);
+		const int32_t h_span_base = int32_t(r.span_base);
+		const int32_t h_rows0 = int32_t(r.rows0);
+		const int32_t h_kind = int32_t((r.stat >> DRAW_STAT_KIND_SHIFT) & 3u);
+		const int32_t h_blend = int32_t((r.stat >> DRAW_STAT_BLEND_SHIFT) & 15u);
+		const int32_t h_tw = int32_t(r.tex_w);
+		const int32_t h_th = int32_t(r.tex_h);
+		const uint64_t tex = (r.stat & DRAW_STAT_TRIANGLE) ? reinterpret_cast<uint64_t>(m.lite + r.tex_offset)
+			: (r.stat & DRAW_STAT_SKYBOX)                  ? reinterpret_cast<uint64_t>(skybox + r.tex_offset)
+														   : reinterpret_cast<uint64_t>(arena + r.tex_offset);
+		const int32_t h_tex_lo = int32_t(uint32_t(tex));
+		const int32_t h_tex_hi = int32_t(uint32_t(tex >> 32));
+		int32_t pd[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+		if(lane < nh && h_kind == 3)
 		{
-			const sw_draw& d = draws[h_di];
-			h_blend = int32_t(d.blend);
-			h_tw = d.texture_size[0];
-			h_th = d.texture_size[1];
-			if(h_kind == 3)
-			{
-				h_tex = reinterpret_cast<uint64_t>(m.lite + d.texture_offset);
-				const PartRec& p0 = parts[h_di * MAX_PARTS_PER_DRAW];
-				const PartRec& p1 = parts[h_di * MAX_PARTS_PER_DRAW + 1];
+			const PartRec& p0 = parts[h_di * MAX_PARTS_PER_DRAW];
+			const PartRec& p1 = parts[h_di * MAX_PARTS_PER_DRAW + 1];
 #pragma unroll
-				for(int i = 0; i < 5; ++i)
-				{
-					pd[i] = p0.d[i];
-					pd[5 + i] = p1.d[i];
-				}
+			for(int i = 0; i < 5; ++i)
+			{
+				pd[i] = p0.d[i];
+				pd[5 + i] = p1.d[i];
 			}
-			else
-				h_tex = reinterpret_cast<uint64_t>(
-					d.texture_source == SW_TEXSRC_SKYBOX ? skybox + d.texture_offset : view_arena + d.texture_offset);
 		}
-		const int32_t h_tex_lo = int32_t(uint32_t(h_tex)), h_tex_hi = int32_t(uint32_t(h_tex >> 32));
 
+		if(err & ERROR_SPAN_OVERFLOW)
+			return; // the host runs the batch again with room for every span record
 		for(uint32_t kg = 0; kg < nh; kg += GD)
 		{
 			// lane L < GD * ROWS loads the span record of draw kg + L / ROWS at tile row L % ROWS
@@ -1902,7 +1931,7 @@ __global__ __launch_bounds__(256) void k_raster_tiles(MapPtrs m, const sw_draw* 
 				{
 					C tv = 0;
 					if(tix[g][rr] >= 0)
-						tv = tri ? C(reinterpret_cast<const uint32_t*>(tex)[tix[g][rr]]) : reinterpret_cast<const C*>(tex)[tix[g][rr]];
+						tv = tri ? C(global_at<uint32_t>(tex, tix[g][rr])) : global_at<C>(tex, tix[g][rr]);
 					texel[g][rr] = tv;
 				}
 			}
@@ -2116,7 +2145,7 @@ void launch_build_surfaces(hipStream_t st, uint32_t color_bits, const MapPtrs& m
 }
 
 template <class CT>
-static void launch_raster_ct(hipStream_t st, uint32_t tile_rows, const MapPtrs& m, const sw_draw* draws, const DrawRec* drec,
+static void launch_raster_ct(hipStream_t st, uint32_t tile_rows, const MapPtrs& m, const DrawBounds* bounds, const DrawRec* drec,
 	const PartRec* parts, const SpanRec* spans, const ViewsArg& views, const void* arena, const void* skybox,
 	void* fb, uint32_t total_tiles, const uint32_t* counters)
 {
@@ -2128,24 +2157,24 @@ static void launch_raster_ct(hipStream_t st, uint32_t tile_rows, const MapPtrs& 
 	const C* sky = static_cast<const C*>(skybox);
 	C* out = static_cast<C*>(fb);
 	if(tile_rows == 4)
-		hipLaunchKernelGGL((k_raster_tiles<CT, 4>), dim3(blocks), dim3(256), 0, st, m, draws, drec, parts, spans, views, a,
+		hipLaunchKernelGGL((k_raster_tiles<CT, 4>), dim3(blocks), dim3(256), 0, st, m, bounds, drec, parts, spans, views, a,
 			sky, out, total_tiles, counters);
 	else if(tile_rows == 8)
-		hipLaunchKernelGGL((k_raster_tiles<CT, 8>), dim3(blocks), dim3(256), 0, st, m, draws, drec, parts, spans, views, a,
+		hipLaunchKernelGGL((k_raster_tiles<CT, 8>), dim3(blocks), dim3(256), 0, st, m, bounds, drec, parts, spans, views, a,
 			sky, out, total_tiles, counters);
 	else
-		hipLaunchKernelGGL((k_raster_tiles<CT, 16>), dim3(blocks), dim3(256), 0, st, m, draws, drec, parts, spans, views, a,
+		hipLaunchKernelGGL((k_raster_tiles<CT, 16>), dim3(blocks), dim3(256), 0, st, m, bounds, drec, parts, spans, views, a,
 			sky, out, total_tiles, counters);
 }
 
-void launch_raster_tiles(hipStream_t st, uint32_t color_bits, uint32_t tile_rows, const MapPtrs& m, const sw_draw* draws,
+void launch_raster_tiles(hipStream_t st, uint32_t color_bits, uint32_t tile_rows, const MapPtrs& m, const DrawBounds* bounds,
 	const DrawRec* drec, const PartRec* parts, const SpanRec* spans, const ViewsArg& views, const void* arena,
 	const void* skybox, void* fb, uint32_t total_tiles, const uint32_t* counters)
 {
 	if(color_bits == 64)
-		launch_raster_ct<Color64T>(st, tile_rows, m, draws, drec, parts, spans, views, arena, skybox, fb, total_tiles, counters);
+		launch_raster_ct<Color64T>(st, tile_rows, m, bounds, drec, parts, spans, views, arena, skybox, fb, total_tiles, counters);
 	else
-		launch_raster_ct<Color32T>(st, tile_rows, m, draws, drec, parts, spans, views, arena, skybox, fb, total_tiles, counters);
+		launch_raster_ct<Color32T>(st, tile_rows, m, bounds, drec, parts, spans, views, arena, skybox, fb, total_tiles, counters);
 }
 
 void launch_finish(hipStream_t st, const DrawRec* drec, uint32_t nd, uint32_t* counters, uint32_t* result)

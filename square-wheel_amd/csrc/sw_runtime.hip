@@ -64,7 +64,7 @@ struct sw_ctx
 	uint32_t last_color_bits = 32; // colour size of the arena's current contents
 
 	// command buffers / work buffers
-	DevBuf views, surfaces, lights, light_indices, draws, vdata, surf_ext, blk_surf, parts, spans, drec, counters, arena, fb,
+	DevBuf views, surfaces, lights, light_indices, draws, vdata, surf_ext, blk_surf, parts, spans, drec, dbounds, counters, arena, fb,
 		tonemap_out, tonemap_sums;
 	uint32_t tile_rows = 8; // raster tile height (SW_TILE_ROWS_*), set by sw_set_tile_rows / the autotuner
 	uint32_t* counters_host = nullptr; // pinned, device-mapped result block (written by k_finish)
@@ -399,6 +399,8 @@ int launch_stages(sw_ctx* ctx, const sw_ctx::Launch& L, bool surfaces, bool rast
 		return r;
 	if((r = ensure(ctx, ctx->drec, sizeof(DrawRec) * std::max<uint32_t>(nd, 1))))
 		return r;
+	if((r = ensure(ctx, ctx->dbounds, sizeof(DrawBounds) * std::max<uint32_t>(nd, 1))))
+		return r;
 	if(L.surface_mode == SURF_TABLE_DEVICE)
 	{
 		if((r = ensure(ctx, ctx->surf_ext, sizeof(SurfExt) * std::max<uint32_t>(ns, 1))))
@@ -415,6 +417,7 @@ int launch_stages(sw_ctx* ctx, const sw_ctx::Launch& L, bool surfaces, bool rast
 	a.parts = static_cast<PartRec*>(ctx->parts.p);
 	a.spans = static_cast<SpanRec*>(ctx->spans.p);
 	a.drec = static_cast<DrawRec*>(ctx->drec.p);
+	a.bounds = static_cast<DrawBounds*>(ctx->dbounds.p);
 	a.counters = counters;
 	a.span_capacity = ctx->span_capacity;
 	a.span_stride = static_spans ? stride : 0;
@@ -457,7 +460,7 @@ int launch_stages(sw_ctx* ctx, const sw_ctx::Launch& L, bool surfaces, bool rast
 	{
 		{
 			StageTimer t(ctx, STAGE_RASTER);
-			swk::launch_raster_tiles(ctx->stream, L.color_bits, ctx->tile_rows, map->ptrs, L.draws,
+			swk::launch_raster_tiles(ctx->stream, L.color_bits, ctx->tile_rows, map->ptrs, static_cast<DrawBounds*>(ctx->dbounds.p),
 				static_cast<DrawRec*>(ctx->drec.p), static_cast<PartRec*>(ctx->parts.p), static_cast<SpanRec*>(ctx->spans.p),
 				ctx->views_arg, L.arena, L.color_bits == 64 ? map->sky64.p : map->sky32.p, L.fb, L.total_tiles, counters);
 		}
@@ -703,7 +706,7 @@ void sw_ctx_destroy(sw_ctx* ctx)
 	if(ctx->graph)
 		(void)hipGraphDestroy(ctx->graph);
 	DevBuf* bufs[] = {&ctx->views, &ctx->surfaces, &ctx->lights, &ctx->light_indices, &ctx->draws, &ctx->vdata, &ctx->surf_ext, &ctx->blk_surf,
-		&ctx->parts, &ctx->spans, &ctx->drec, &ctx->counters, &ctx->arena, &ctx->fb, &ctx->tonemap_out, &ctx->tonemap_sums};
+		&ctx->parts, &ctx->spans, &ctx->drec, &ctx->dbounds, &ctx->counters, &ctx->arena, &ctx->fb, &ctx->tonemap_out, &ctx->tonemap_sums};
 	for(DevBuf* b : bufs)
 		release(ctx, *b);
 	if(ctx->counters_host)
