@@ -363,7 +363,9 @@ def main():
     launches = max(prof[dominant][1], 1)
     avg_s = prof[dominant][0] / 1e3 / launches
     achieved = bytes_sum[dominant] / launches / avg_s / 1e9
-    traffic = read_traffic("plain1080p", f"k_raster_tiles_rows{plain_rows}" if dominant == "raster_tiles" else "k_build_surfaces")
+    traffic_key = {"raster_tiles": f"k_raster_tiles_rows{plain_rows}", "setup_and_surfaces": "k_setup_and_surfaces_256",
+                   "build_surfaces": "k_build_surfaces"}.get(dominant)
+    traffic = read_traffic("plain1080p", traffic_key) if traffic_key else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dominant,
                 "algorithmic_bytes_per_launch": int(bytes_sum[dominant] / launches),

@@ -735,8 +735,10 @@ SW_DEV void build_surfaces_block(uint32_t block, const MapPtrs& m, const Surface
 	}
 }
 
+// Held to 80 VGPRs (six waves per SIMD): the kernel is bound by arithmetic and load latency, and
+// the extra waves hide more of the latter than the few spilled values cost.
 template <class CT>
-__global__ __launch_bounds__(SURF_BLOCK) void k_build_surfaces(MapPtrs m, SurfaceArgs a, const ViewsArg views,
+__global__ __launch_bounds__(SURF_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_build_surfaces(MapPtrs m, SurfaceArgs a, const ViewsArg views,
 	const uint32_t* __restrict__ counters)
 {
 	if(counters[COUNTER_ERROR] & ERROR_BAD_SURFACE)
@@ -1632,9 +1634,9 @@ __global__ __launch_bounds__(256) void k_setup_draws(SetupArgs a, const ViewsArg
 }
 
 // Draw setup and surface texels in one launch (the host prepared and validated the surfaces): the
-// two are independent, so they run side by side. Held to 128 VGPRs: four waves per SIMD.
+// two are independent, so they run side by side. Held to 96 VGPRs: five waves per SIMD.
 template <int LANES, class CT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_setup_and_surfaces(SetupArgs a, const ViewsArg views, SurfaceArgs sa, MapPtrs m)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_setup_and_surfaces(SetupArgs a, const ViewsArg views, SurfaceArgs sa, MapPtrs m)
 {
 	if(blockIdx.x < a.setup_blocks)
 		setup_draws_block<LANES>(blockIdx.x, a, views);
