@@ -348,7 +348,15 @@ int sw_tonemap_device(
 	sw_ctx* ctx, const void* hdr_device, uint32_t width, uint32_t height, float exposure,
 	const float color_modulate[3], void* dst32_device, size_t dst_pitch_bytes, float out_colors_sum[3]);
 
+/* Work counters of the last batch. The pixel counters of a batch are summed on the first call
+ * after it (one small kernel and a wait), so batches that nobody asks about pay nothing for them.
+ * No counterpart in the reference (its renderer keeps no such counters). */
 int sw_get_stats(sw_ctx* ctx, sw_stats* out_stats);
+
+/* Raster re-runs of the last batch after its span storage overflowed (sw_stats::reruns, without
+ * summing the counters): work queued behind the batch before the wait saw the first run. No
+ * counterpart in the reference. */
+uint32_t sw_last_reruns(const sw_ctx* ctx);
 
 /* Copies the first num_texels ColorT elements of the surface arena built by the last
  * sw_build_surfaces / sw_render_batch into host memory (synchronous). The arena is the renderer's

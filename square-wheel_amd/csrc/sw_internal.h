@@ -40,6 +40,13 @@ enum
 	ERROR_PART_GAP = 1u << 2, // the parts of a draw did not cover consecutive rows (never expected)
 };
 
+// Error words in pinned, device-mapped host memory, one per error kind (word k for the flag 1 << k):
+// kernels set them with plain stores (several writers of one kind store the same value, so no
+// atomic read-modify-write crosses the bus), the host reads and clears them after the sequence.
+// The live counter block keeps the same flags for the later kernels of the sequence.
+#define HOST_ERR_WORDS 4
+#define HOST_ERR_OFFSET COUNTER_COUNT // after the result block in the host memory
+
 // Surface texels are built by blocks of SURF_BLOCK threads over the flattened arena, each block
 // surf_block_texels() consecutive texels: up to SURF_MAX_TEXELS_PER_THREAD per thread (the block's
 // per-surface set-up is shared by more texels), fewer when the arena is too small to fill the
@@ -159,6 +166,7 @@ struct SetupArgs
 	DrawRec* drec;
 	DrawBounds* bounds;
 	uint32_t* counters;
+	uint32_t* host_err; // HOST_ERR_WORDS error words in host memory
 	uint64_t span_capacity;
 	uint64_t span_stride; // > 0: draw i owns span records [i * stride, (i + 1) * stride), no allocation
 	uint64_t arena_texels, skybox_texels, lite_texels;
@@ -179,6 +187,7 @@ struct SurfaceArgs
 	void* arena;
 	uint64_t arena_texels;
 	uint32_t* counters; // error flags (validation of descriptors without a prepared table)
+	uint32_t* host_err; // HOST_ERR_WORDS error words in host memory
 };
 
 namespace swk

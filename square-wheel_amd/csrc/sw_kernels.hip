@@ -29,6 +29,14 @@ using namespace swd;
 // Surfaces
 // =============================================================================================
 
+// Raises an error flag: in the live counters (for the sequence's later kernels) and in its word in
+// host memory (for the host).
+SW_DEV void raise_error(uint32_t* counters, uint32_t* host_err, uint32_t flag)
+{
+	atomicOr(&counters[COUNTER_ERROR], flag);
+	host_err[__builtin_ctz(flag)] = 1u;
+}
+
 // Arena extent and validity of surface s (run by the extra blocks of k_setup_draws).
 SW_DEV uint32_t view_of_surface(const ViewsArg& views, uint32_t s)
 {
@@ -73,7 +81,7 @@ SW_DEV bool surface_desc_ok(const sw_surface_desc& d, uint64_t end, uint64_t are
 // table for the surface blocks whose first texel lies in [end of s - 1, end of s).
 SW_DEV void prep_surface(const sw_surface_desc* __restrict__ surfaces, uint32_t s, uint32_t num_surfaces,
 	const ViewsArg& views, SurfExt* __restrict__ surf_ext, uint32_t* __restrict__ blk_surf,
-	uint64_t arena_texels, const MapPtrs& m, uint32_t* __restrict__ counters)
+	uint64_t arena_texels, const MapPtrs& m, uint32_t* __restrict__ counters, uint32_t* __restrict__ host_err)
 {
 	const uint32_t view = view_of_surface(views, s);
 	const sw_surface_desc& d = surfaces[s];
@@ -94,7 +102,7 @@ SW_DEV void prep_surface(const sw_surface_desc* __restrict__ surfaces, uint32_t 
 	}
 	if(bad)
 	{
-		atomicOr(&counters[COUNTER_ERROR], ERROR_BAD_SURFACE);
+		raise_error(counters, host_err, ERROR_BAD_SURFACE);
 		return;
 	}
 	// Blocks whose first texel lies in [prev_end, end) start their search at s; the last surface
@@ -691,7 +699,7 @@ SW_DEV void build_surfaces_block(uint32_t block, const MapPtrs& m, const Surface
 							ok = ok && search_base + p.pixels_offset + uint64_t(p.size[0]) * p.size[1] <= start;
 						}
 						if(!ok)
-							atomicOr(&a.counters[COUNTER_ERROR], ERROR_BAD_SURFACE);
+							raise_error(a.counters, a.host_err, ERROR_BAD_SURFACE);
 					}
 					make_surf_rec(recs[tid], d, start, end, views[view].v, m, ok);
 				}
@@ -1553,7 +1561,7 @@ SW_DEV void setup_draws_block(uint32_t block, const SetupArgs& a, const ViewsArg
 		st.overflow = base + rows > span_capacity;
 		st.span_end = st.overflow ? 0 : base + rows;
 		if(active && st.overflow && lane == 0 && rows)
-			atomicOr(&counters[COUNTER_ERROR], ERROR_SPAN_OVERFLOW);
+			raise_error(counters, a.host_err, ERROR_SPAN_OVERFLOW);
 	}
 
 	// pass 2: parts and spans
@@ -1595,7 +1603,7 @@ SW_DEV void setup_draws_block(uint32_t block, const SetupArgs& a, const ViewsArg
 		const bool reads = d.blend == SW_BLEND_AVERAGE || d.blend == SW_BLEND_ADDITIVE || d.blend == SW_BLEND_ALPHA_TEST ||
 			d.blend == SW_BLEND_ALPHA_BLEND;
 		if(st.gap)
-			atomicOr(&counters[COUNTER_ERROR], ERROR_PART_GAP);
+			raise_error(counters, a.host_err, ERROR_PART_GAP);
 		const uint32_t kind = d.kind == SW_DRAW_TRIANGLE ? 3u : (d.mode == SW_TC_LINE_Z ? 1u : (d.mode == SW_TC_AFFINE ? 2u : 0u));
 		DrawRec& r = drec[di];
 		r.num_parts = (st.overflow || st.gap) ? 0 : st.num_parts;
@@ -1627,7 +1635,7 @@ __global__ __launch_bounds__(256) void k_setup_draws(SetupArgs a, const ViewsArg
 	{
 		const uint32_t s = (blockIdx.x - a.setup_blocks) * blockDim.x + threadIdx.x;
 		if(s < num_surfaces)
-			prep_surface(surfaces, s, num_surfaces, views, surf_ext, blk_surf, a.arena_texels, m, a.counters);
+			prep_surface(surfaces, s, num_surfaces, views, surf_ext, blk_surf, a.arena_texels, m, a.counters, a.host_err);
 		return;
 	}
 	setup_draws_block<LANES>(blockIdx.x, a, views);

@@ -67,7 +67,10 @@ struct sw_ctx
 	DevBuf views, surfaces, lights, light_indices, draws, vdata, surf_ext, blk_surf, parts, spans, drec, dbounds, counters, arena, fb,
 		tonemap_out, tonemap_sums;
 	uint32_t tile_rows = 8; // raster tile height (SW_TILE_ROWS_*), set by sw_set_tile_rows / the autotuner
-	uint32_t* counters_host = nullptr; // pinned, device-mapped result block (written by k_finish)
+	uint32_t* counters_host = nullptr; // pinned, device-mapped: result block (k_finish), then the error words
+	bool finish_launched = false; // the last launch sequence ended with k_finish (span allocation mode)
+	bool stats_pending = false; // the last raster batch's statistics are computed when asked for
+	uint32_t stats_draws = 0; // draws of that batch (its DrawRecs stay in `drec` until the next raster)
 	uint32_t* counters_host_dev = nullptr; // its device address
 	ViewsArg views_arg{}; // views of the batch being launched
 	bool surface_error = false; // an invalid surface seen by a surface-only launch sequence
@@ -418,6 +421,7 @@ int launch_stages(sw_ctx* ctx, const sw_ctx::Launch& L, bool surfaces, bool rast
 	a.spans = static_cast<SpanRec*>(ctx->spans.p);
 	a.drec = static_cast<DrawRec*>(ctx->drec.p);
 	a.bounds = static_cast<DrawBounds*>(ctx->dbounds.p);
+	a.host_err = ctx->counters_host_dev + HOST_ERR_OFFSET;
 	a.counters = counters;
 	a.span_capacity = ctx->span_capacity;
 	a.span_stride = static_spans ? stride : 0;
@@ -431,6 +435,7 @@ int launch_stages(sw_ctx* ctx, const sw_ctx::Launch& L, bool surfaces, bool rast
 	sa.surf_ext = static_cast<const SurfExt*>(ctx->surf_ext.p);
 	sa.blk_surf = L.surface_mode == SURF_SEARCH ? nullptr : static_cast<const uint32_t*>(ctx->blk_surf.p);
 	sa.counters = counters;
+	sa.host_err = ctx->counters_host_dev + HOST_ERR_OFFSET;
 	sa.lights = L.lights;
 	sa.light_indices = L.light_indices;
 	sa.num_light_indices = L.num_light_indices;
@@ -464,7 +469,14 @@ int launch_stages(sw_ctx* ctx, const sw_ctx::Launch& L, bool surfaces, bool rast
 				static_cast<DrawRec*>(ctx->drec.p), static_cast<PartRec*>(ctx->parts.p), static_cast<SpanRec*>(ctx->spans.p),
 				ctx->views_arg, L.arena, L.color_bits == 64 ? map->sky64.p : map->sky32.p, L.fb, L.total_tiles, counters);
 		}
-		swk::launch_finish(ctx->stream, static_cast<DrawRec*>(ctx->drec.p), nd, counters, ctx->counters_host_dev);
+		// With static span storage nothing in the live counters needs clearing after a sequence
+		// (errors also reach the host words), so the statistics wait until they are asked for; the
+		// span allocation counter needs k_finish to hand it over and clear it.
+		ctx->finish_launched = !static_spans;
+		ctx->stats_pending = static_spans;
+		ctx->stats_draws = nd;
+		if(!static_spans)
+			swk::launch_finish(ctx->stream, static_cast<DrawRec*>(ctx->drec.p), nd, counters, ctx->counters_host_dev);
 	}
 	return hip_check(ctx, hipGetLastError(), "kernel launch");
 }
@@ -523,6 +535,55 @@ int host_prepare_surfaces(sw_ctx* ctx, const sw_map* map, const std::vector<View
 	return r ? r : upload(ctx, ctx->blk_surf, blk.data(), sizeof(uint32_t) * blk.size());
 }
 
+// The result block k_finish wrote into host memory -> the pixel statistics.
+void read_result_block(sw_ctx* ctx)
+{
+	uint32_t c[COUNTER_COUNT];
+	for(int k = 0; k < COUNTER_COUNT; ++k)
+		c[k] = static_cast<volatile const uint32_t*>(ctx->counters_host)[k];
+	auto u64 = [&](int lo) { return uint64_t(c[lo]) | (uint64_t(c[lo + 1]) << 32); };
+	ctx->stats.polygon_pixels = u64(COUNTER_POLY_PIXELS_LO);
+	ctx->stats.triangle_pixels = u64(COUNTER_TRI_PIXELS_LO);
+	ctx->stats.blend_pixels = u64(COUNTER_BLEND_PIXELS_LO);
+	ctx->stats.spans = u64(COUNTER_SPANS_LO);
+	ctx->stats.parts = c[COUNTER_PARTS];
+}
+
+// Computes the last raster batch's statistics if they are still pending (one k_finish over its
+// draw records, which also clears the live counters).
+int ensure_stats(sw_ctx* ctx)
+{
+	if(!ctx->stats_pending)
+		return SW_OK;
+	ctx->stats_pending = false;
+	swk::launch_finish(ctx->stream, static_cast<DrawRec*>(ctx->drec.p), ctx->stats_draws,
+		static_cast<uint32_t*>(ctx->counters.p), ctx->counters_host_dev);
+	int r = hip_check(ctx, hipGetLastError(), "kernel launch");
+	if(!r)
+		r = hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+	if(r)
+		return r;
+	read_result_block(ctx);
+	return SW_OK;
+}
+
+// Reads and clears the error words the kernels of the last sequence set in host memory; after an
+// error the live counters are cleared as well (a sequence without errors leaves them clear).
+uint32_t take_errors(sw_ctx* ctx)
+{
+	volatile uint32_t* w = ctx->counters_host + HOST_ERR_OFFSET;
+	uint32_t flags = 0;
+	for(int k = 0; k < HOST_ERR_WORDS; ++k)
+		if(w[k])
+		{
+			flags |= 1u << k;
+			w[k] = 0;
+		}
+	if(flags)
+		hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, ctx->stream, static_cast<uint32_t*>(ctx->counters.p));
+	return flags;
+}
+
 // After a surface-only launch sequence: waits, keeps its error flag, clears the live counters.
 int flush_surface_errors(sw_ctx* ctx)
 {
@@ -536,7 +597,7 @@ int flush_surface_errors(sw_ctx* ctx)
 	if((r = hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize")))
 		return r;
 	collect_timings(ctx);
-	if(ctx->counters_host[COUNTER_ERROR] & ERROR_BAD_SURFACE)
+	if((ctx->counters_host[COUNTER_ERROR] & ERROR_BAD_SURFACE) | (take_errors(ctx) & ERROR_BAD_SURFACE))
 		ctx->surface_error = true;
 	return SW_OK;
 }
@@ -547,25 +608,30 @@ int finish(sw_ctx* ctx)
 	ctx->stats.reruns = 0;
 	for(int attempt = 0; attempt < 4; ++attempt)
 	{
-		// k_finish wrote the result block straight into pinned host memory
 		int r = hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
 		if(r)
 			return r;
-		uint32_t host_counters[COUNTER_COUNT];
-		for(int k = 0; k < COUNTER_COUNT; ++k)
-			host_counters[k] = static_cast<volatile const uint32_t*>(ctx->counters_host)[k];
 		collect_timings(ctx);
-		auto u64 = [&](int lo) { return uint64_t(host_counters[lo]) | (uint64_t(host_counters[lo + 1]) << 32); };
+		const uint32_t errors = take_errors(ctx);
 		const bool surface_error = ctx->surface_error;
 		ctx->surface_error = false;
-		if(surface_error || (host_counters[COUNTER_ERROR] & ERROR_BAD_SURFACE))
+		if(surface_error || (errors & ERROR_BAD_SURFACE))
+		{
+			ctx->stats_pending = false;
 			return set_error(ctx, SW_ERR_INVALID_ARGUMENT,
 				"invalid surface descriptor (arena order/overlap, texture level, or lightmap range out of bounds)");
-		if(host_counters[COUNTER_ERROR] & ERROR_PART_GAP)
-			return set_error(ctx, SW_ERR_DEVICE, "internal: a draw's parts did not cover consecutive rows");
-		const uint64_t needed = u64(COUNTER_SPANS_LO);
-		if(host_counters[COUNTER_ERROR] & ERROR_SPAN_OVERFLOW)
+		}
+		if(errors & ERROR_PART_GAP)
 		{
+			ctx->stats_pending = false;
+			return set_error(ctx, SW_ERR_DEVICE, "internal: a draw's parts did not cover consecutive rows");
+		}
+		if(ctx->finish_launched)
+			read_result_block(ctx); // k_finish ran: the statistics (and the span count) are in
+		if(errors & ERROR_SPAN_OVERFLOW)
+		{
+			// only span allocation overflows, and it always ends with k_finish
+			const uint64_t needed = ctx->stats.spans;
 			if(needed >= (1ull << 31))
 				return set_error(ctx, SW_ERR_BUDGET, "span storage above 2^31 records");
 			ctx->span_capacity = std::max<uint64_t>(needed + needed / 4, ctx->span_capacity * 2);
@@ -574,11 +640,6 @@ int finish(sw_ctx* ctx)
 			ctx->stats.reruns++;
 			continue;
 		}
-		ctx->stats.polygon_pixels = u64(COUNTER_POLY_PIXELS_LO);
-		ctx->stats.triangle_pixels = u64(COUNTER_TRI_PIXELS_LO);
-		ctx->stats.blend_pixels = u64(COUNTER_BLEND_PIXELS_LO);
-		ctx->stats.spans = needed;
-		ctx->stats.parts = host_counters[COUNTER_PARTS];
 		return SW_OK;
 	}
 	return set_error(ctx, SW_ERR_BUDGET, "span storage overflow persisted");
@@ -683,13 +744,14 @@ int sw_ctx_create(int device, sw_ctx** out_ctx)
 	if(ensure(ctx, ctx->counters, sizeof(uint32_t) * COUNTER_TOTAL) ||
 		hipMemsetAsync(ctx->counters.p, 0, sizeof(uint32_t) * COUNTER_TOTAL, ctx->stream) != hipSuccess ||
 		hipStreamSynchronize(ctx->stream) != hipSuccess ||
-		hipHostMalloc(reinterpret_cast<void**>(&ctx->counters_host), sizeof(uint32_t) * COUNTER_COUNT,
+		hipHostMalloc(reinterpret_cast<void**>(&ctx->counters_host), sizeof(uint32_t) * (COUNTER_COUNT + HOST_ERR_WORDS),
 			hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
 		hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->counters_host_dev), ctx->counters_host, 0) != hipSuccess)
 	{
 		sw_ctx_destroy(ctx);
 		return SW_ERR_DEVICE;
 	}
+	std::memset(ctx->counters_host, 0, sizeof(uint32_t) * (COUNTER_COUNT + HOST_ERR_WORDS));
 	*out_ctx = ctx;
 	return SW_OK;
 }
@@ -1125,9 +1187,15 @@ int sw_get_stats(sw_ctx* ctx, sw_stats* out)
 {
 	if(!ctx || !out)
 		return SW_ERR_INVALID_ARGUMENT;
+	(void)hipSetDevice(ctx->device);
+	int r = ensure_stats(ctx);
+	if(r)
+		return r;
 	*out = ctx->stats;
 	return SW_OK;
 }
+
+uint32_t sw_last_reruns(const sw_ctx* ctx) { return ctx ? uint32_t(ctx->stats.reruns) : 0u; }
 
 // ---------------------------------------------------------------------------------------------
 // Batched path
@@ -1297,6 +1365,8 @@ int sw_render_batch_graph(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out
 	{
 		// An eager run sizes every buffer (span capacity included) before capture.
 		if((r = sw_render_batch(ctx, map, b, out_fb)))
+			return r;
+		if((r = ensure_stats(ctx)))
 			return r;
 		ctx->span_capacity = std::max<uint64_t>(ctx->span_capacity, ctx->stats.spans + ctx->stats.spans / 2 + 1024);
 		std::vector<ViewExt> ve;

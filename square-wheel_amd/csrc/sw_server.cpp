@@ -153,9 +153,7 @@ void collect(sw_server* s, Slot& slot)
 	int status = sw_batch_wait(slot.ctx);
 	if(status == SW_OK)
 	{
-		sw_stats st{};
-		(void)sw_get_stats(slot.ctx, &st);
-		if(st.reruns)
+		if(sw_last_reruns(slot.ctx))
 		{
 			// the raster stage ran again after the copies were queued: copy again
 			status = queue_results(slot);

@@ -200,6 +200,7 @@ GPU_SYMBOLS = {
     "sw_tonemap_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float,
                                     C.POINTER(C.c_float), C.c_void_p, C.c_size_t, C.POINTER(C.c_float)]),
     "sw_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "sw_last_reruns": (C.c_uint32, [C.c_void_p]),
     "sw_read_surfaces": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     "sw_frame_device_framebuffer": (C.c_void_p, [C.c_void_p]),
     "sw_ctx_stream": (C.c_void_p, [C.c_void_p]),
