@@ -123,6 +123,8 @@ def hdr_leg(ctx, args):
         ctx.check(lib.sw_tonemap_device(ctx.h, hdr.ptr, W, H, C.c_float(1.0), mod, ldr.ptr, W * 4, sums), "sw_tonemap_device")
 
     tris = [int(np.sum(f["draws"]["kind"] == sw_abi.DRAW_TRIANGLE)) for f in frames]
+    rows, tune = ctx.autotune(mh, batches[0], dev[0], hdr)
+    log(f"autotune (hdr): {rows} rows, scalar spans {ctx.scalar_spans()}; ms per frame by rows {tune}")
     for i in range(len(cbs)):
         step(i)
     ctx.profile_read(reset=True)
@@ -149,7 +151,8 @@ def hdr_leg(ctx, args):
            "frames_s": round(args.hdr_steps / wall, 2), "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 4),
            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 4),
            "triangle_draws_per_frame": round(float(np.mean(tris)), 1),
-           "kernel_us_per_launch": {k: round(v[0] / max(v[1], 1) * 1e3, 3) for k, v in prof.items() if v[1]}}
+           "kernel_us_per_launch": {k: round(v[0] / max(v[1], 1) * 1e3, 3) for k, v in prof.items() if v[1]},
+           "tile_rows": rows, "scalar_spans": ctx.scalar_spans()}
     log(f"hdr + models: {out}")
     return out
 

@@ -337,8 +337,15 @@ int sw_render_batch_graph(sw_ctx* ctx, sw_map* map, const sw_batch* batch, void*
 int sw_set_tile_rows(sw_ctx* ctx, uint32_t rows);
 uint32_t sw_get_tile_rows(const sw_ctx* ctx);
 
-/* Autotuner: renders `batch` with every tile height, keeps the fastest for this context and
- * reports the milliseconds per batch of each candidate (4, 8, 16 rows) in out_ms (may be NULL). */
+/* How the raster kernel loads the span records of its (draw, row) items: 1 = one scalar load per
+ * item into SGPRs, 0 = one vector load per 16 items shared with readlane. Both give identical
+ * output; which is faster depends on the workload (the autotuner picks). */
+int sw_set_scalar_spans(sw_ctx* ctx, uint32_t enable);
+uint32_t sw_get_scalar_spans(const sw_ctx* ctx);
+
+/* Autotuner: renders `batch` with every tile height and both span load modes, keeps the fastest
+ * for this context and reports the milliseconds per batch of each tile height (4, 8, 16 rows; the
+ * better mode) in out_ms (may be NULL). */
 int sw_autotune_batch(
 	sw_ctx* ctx, sw_map* map, const sw_batch* batch, void* out_framebuffer_device, uint32_t* out_rows,
 	double out_ms[SW_TILE_ROWS_COUNT]);

@@ -203,9 +203,10 @@ void launch_build_surfaces(hipStream_t st, uint32_t color_bits, const MapPtrs& m
 // Kernels 1 + 2 in one launch when the host prepared the surfaces.
 void launch_setup_and_surfaces(hipStream_t st, uint32_t lanes, uint32_t color_bits, SetupArgs a, const ViewsArg& views,
 	const SurfaceArgs& sa, const MapPtrs& m);
-// Kernel 3: one wave per 64 x tile_rows tile (binning fused).
-void launch_raster_tiles(hipStream_t st, uint32_t color_bits, uint32_t tile_rows, const MapPtrs& m, const DrawBounds* bounds,
-	const DrawRec* drec, const PartRec* parts, const SpanRec* spans, const ViewsArg& views, const void* arena,
+// Kernel 3: one wave per 64 x tile_rows tile (binning fused); scalar_spans: span records through
+// the scalar unit (else vector loads shared with readlane).
+void launch_raster_tiles(hipStream_t st, uint32_t color_bits, uint32_t tile_rows, bool scalar_spans, const MapPtrs& m,
+	const DrawBounds* bounds, const DrawRec* drec, const PartRec* parts, const SpanRec* spans, const ViewsArg& views, const void* arena,
 	const void* skybox, void* fb, uint32_t total_tiles, const uint32_t* counters);
 // Kernel 4 (one block): per-draw statistics -> result block (host-mapped), live block cleared.
 void launch_finish(hipStream_t st, const DrawRec* drec, uint32_t nd, uint32_t* counters, uint32_t* result);

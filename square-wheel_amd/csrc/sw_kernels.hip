@@ -1766,8 +1766,9 @@ SW_DEV int32_t rl(int32_t v, uint32_t lane) { return __builtin_amdgcn_readlane(v
 // time (GD * ROWS = 16 (draw, row) items): one lane per item loads its span record, every lane
 // computes the texel address of its pixel for all 16 items, issues the 16 fetches together and
 // blends them in painter's order into its ROWS colours.
-template <class CT, int ROWS>
-__global__ __launch_bounds__(256) void k_raster_tiles(MapPtrs m, const DrawBounds* __restrict__ bounds,
+template <class CT, int ROWS, bool SSPAN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(typename CT::T) == 4 && ROWS <= 8 ? 5 : 1)))
+void k_raster_tiles(MapPtrs m, const DrawBounds* __restrict__ bounds,
 	const DrawRec* __restrict__ drec, const PartRec* __restrict__ parts, const SpanRec* __restrict__ spans,
 	const ViewsArg views, const typename CT::T* __restrict__ arena, const typename CT::T* __restrict__ skybox,
 	typename CT::T* __restrict__ fb, uint32_t total_tiles, const uint32_t* __restrict__ counters)
@@ -1869,15 +1870,20 @@ __global__ __launch_bounds__(256) void k_raster_tiles(MapPtrs m, const DrawBound
 			return; // the host runs the batch again with room for every span record
 		for(uint32_t kg = 0; kg < nh; kg += GD)
 		{
-			// lane L < GD * ROWS loads the span record of draw kg + L / ROWS at tile row L % ROWS
-			const uint32_t kq = (kg + lane / ROWS) & 63u;
-			const int32_t q_row = y0 + int32_t(lane % ROWS) - lane_pull(h_row_first, kq);
-			const int32_t q_rows = lane_pull(h_rows, kq);
-			const int32_t q_base = lane_pull(h_span_base, kq);
+			// Span records of the group's 16 items. SSPAN: each item's record is loaded through the
+			// scalar unit into SGPRs when the item is processed. Otherwise lane L < GD * ROWS loads
+			// the record of draw kg + L / ROWS at tile row L % ROWS and items read them with
+			// readlane (fewer, wider loads; better where many distinct records are in flight).
 			SpanRec mine = SpanRec{0, 0, {0, 0, 0, 0, 0, 0}};
-			if(lane < uint32_t(GD * ROWS) && kg + lane / ROWS < nh && q_row >= 0 && q_row < q_rows)
-				mine = spans[uint32_t(q_base) + uint32_t(q_row)];
-
+			if constexpr(!SSPAN)
+			{
+				const uint32_t kq = (kg + lane / ROWS) & 63u;
+				const int32_t q_row = y0 + int32_t(lane % ROWS) - lane_pull(h_row_first, kq);
+				const int32_t q_rows = lane_pull(h_rows, kq);
+				const int32_t q_base = lane_pull(h_span_base, kq);
+				if(lane < uint32_t(GD * ROWS) && kg + lane / ROWS < nh && q_row >= 0 && q_row < q_rows)
+					mine = spans[uint32_t(q_base) + uint32_t(q_row)];
+			}
 			int32_t tix[GD][ROWS];
 #pragma unroll
 			for(int g = 0; g < GD; ++g)
@@ -1887,41 +1893,64 @@ __global__ __launch_bounds__(256) void k_raster_tiles(MapPtrs m, const DrawBound
 				const int32_t kind = rl(h_kind, kk);
 				const uint32_t tw = uint32_t(rl(h_tw, kk)), th = uint32_t(rl(h_th, kk));
 				const int32_t part_row0 = rl(h_row_first, kk), rows0 = rl(h_rows0, kk);
+				const int32_t nrows = rl(h_rows, kk), span_base = rl(h_span_base, kk);
 #pragma unroll
 				for(int rr = 0; rr < ROWS; ++rr)
 				{
-					// every branch below is uniform over the wave (item values are read with
-					// readlane); the lane's own span test only masks the result
+					// every value of the item is uniform over the wave, its span record is loaded
+					// through the scalar unit straight into SGPRs, and every branch below is a
+					// scalar one; the lane's own span test only masks the result
+					const int32_t q = y0 + rr - part_row0;
 					const int sl = g * ROWS + rr;
-					const int32_t sx0 = rl(mine.x0, sl), sx1 = rl(mine.x1, sl);
 					int32_t ti = -1;
-					if(k < nh && sx0 < sx1 && sx1 > x0 && sx0 < x0 + 64)
+					SpanRec sp = SpanRec{0, 0, {0, 0, 0, 0, 0, 0}};
+					bool row_ok;
+					if constexpr(SSPAN)
 					{
-						int32_t sv[6];
+						row_ok = k < nh && q >= 0 && q < nrows;
+						if(row_ok)
+							sp = spans[uint32_t(span_base) + uint32_t(q)];
+					}
+					else
+					{
+						row_ok = k < nh;
+						sp.x0 = rl(mine.x0, sl);
+						sp.x1 = rl(mine.x1, sl);
+					}
+					if(row_ok)
+					{
+						const int32_t sx0 = sp.x0, sx1 = sp.x1;
+						if(sx0 < sx1 && sx1 > x0 && sx0 < x0 + 64)
+						{
+							if constexpr(!SSPAN)
+							{
 #pragma unroll
-						for(int i = 0; i < 6; ++i)
-							sv[i] = rl(mine.v[i], sl);
-						const int32_t j = x - sx0;
-						const bool in = x >= sx0 && x < sx1;
-						if(kind == 3)
-						{
-							const bool upper = y0 + rr - part_row0 >= rows0;
-							const int32_t d0u = rl(pd[5], kk), d0l = rl(pd[0], kk);
-							const int32_t d1u = rl(pd[6], kk), d1l = rl(pd[1], kk);
-							const int32_t d0 = upper ? d0u : d0l, d1 = upper ? d1u : d1l;
-							const uint32_t u = uint32_t(fixed16_floor_to_int(wadd(sv[0], wmul(j, d0))));
-							const uint32_t v = uint32_t(fixed16_floor_to_int(wadd(sv[1], wmul(j, d1))));
-							ti = in && u < tw && v < th ? int32_t(u + v * tw) : -1;
-						}
-						else if(kind == 0)
-						{
-							const int32_t t = polygon_texel_index<0>(sv, j, tw, th);
-							ti = in ? t : -1;
-						}
-						else
-						{
-							const int32_t t = polygon_texel_index<1>(sv, j, tw, th);
-							ti = in ? t : -1;
+								for(int i = 0; i < 6; ++i)
+									sp.v[i] = rl(mine.v[i], sl);
+							}
+							const int32_t* sv = sp.v;
+							const int32_t j = x - sx0;
+							const bool in = x >= sx0 && x < sx1;
+							if(kind == 3)
+							{
+								const bool upper = q >= rows0;
+								const int32_t d0u = rl(pd[5], kk), d0l = rl(pd[0], kk);
+								const int32_t d1u = rl(pd[6], kk), d1l = rl(pd[1], kk);
+								const int32_t d0 = upper ? d0u : d0l, d1 = upper ? d1u : d1l;
+								const uint32_t u = uint32_t(fixed16_floor_to_int(wadd(sv[0], wmul(j, d0))));
+								const uint32_t v = uint32_t(fixed16_floor_to_int(wadd(sv[1], wmul(j, d1))));
+								ti = in && u < tw && v < th ? int32_t(u + v * tw) : -1;
+							}
+							else if(kind == 0)
+							{
+								const int32_t t = polygon_texel_index<0>(sv, j, tw, th);
+								ti = in ? t : -1;
+							}
+							else
+							{
+								const int32_t t = polygon_texel_index<1>(sv, j, tw, th);
+								ti = in ? t : -1;
+							}
 						}
 					}
 					tix[g][rr] = ti;
@@ -1962,16 +1991,34 @@ __global__ __launch_bounds__(256) void k_raster_tiles(MapPtrs m, const DrawBound
 					continue;
 				}
 				const int32_t part_row0 = rl(h_row_first, k), rows0 = rl(h_rows0, k);
+				const int32_t nrows = rl(h_rows, k), span_base = rl(h_span_base, k);
 				const int32_t dr_u = rl(pd[7], k), dr_l = rl(pd[2], k);
 				const int32_t dg_u = rl(pd[8], k), dg_l = rl(pd[3], k);
 				const int32_t db_u = rl(pd[9], k), db_l = rl(pd[4], k);
 #pragma unroll
 				for(int rr = 0; rr < ROWS; ++rr)
 				{
-					const int sl = g * ROWS + rr;
-					const bool upper = y0 + rr - part_row0 >= rows0;
-					const int32_t sx0 = rl(mine.x0, sl);
-					const int32_t l0 = rl(mine.v[2], sl), l1 = rl(mine.v[3], sl), l2 = rl(mine.v[4], sl);
+					const int32_t q = y0 + rr - part_row0;
+					if(q < 0 || q >= nrows)
+						continue; // no span in this row (so no texel either)
+					const bool upper = q >= rows0;
+					int32_t sx0, l0, l1, l2;
+					if constexpr(SSPAN)
+					{
+						const SpanRec sp = spans[uint32_t(span_base) + uint32_t(q)]; // scalar load, cached
+						sx0 = sp.x0;
+						l0 = sp.v[2];
+						l1 = sp.v[3];
+						l2 = sp.v[4];
+					}
+					else
+					{
+						const int sl = g * ROWS + rr;
+						sx0 = rl(mine.x0, sl);
+						l0 = rl(mine.v[2], sl);
+						l1 = rl(mine.v[3], sl);
+						l2 = rl(mine.v[4], sl);
+					}
 					if(tix[g][rr] < 0)
 						continue;
 					const int32_t j = x - sx0;
@@ -2154,7 +2201,7 @@ void launch_build_surfaces(hipStream_t st, uint32_t color_bits, const MapPtrs& m
 		hipLaunchKernelGGL(k_build_surfaces<Color32T>, dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, sa, views, counters);
 }
 
-template <class CT>
+template <class CT, bool SSPAN>
 static void launch_raster_ct(hipStream_t st, uint32_t tile_rows, const MapPtrs& m, const DrawBounds* bounds, const DrawRec* drec,
 	const PartRec* parts, const SpanRec* spans, const ViewsArg& views, const void* arena, const void* skybox,
 	void* fb, uint32_t total_tiles, const uint32_t* counters)
@@ -2167,24 +2214,35 @@ static void launch_raster_ct(hipStream_t st, uint32_t tile_rows, const MapPtrs& 
 	const C* sky = static_cast<const C*>(skybox);
 	C* out = static_cast<C*>(fb);
 	if(tile_rows == 4)
-		hipLaunchKernelGGL((k_raster_tiles<CT, 4>), dim3(blocks), dim3(256), 0, st, m, bounds, drec, parts, spans, views, a,
-			sky, out, total_tiles, counters);
+		hipLaunchKernelGGL((k_raster_tiles<CT, 4, SSPAN>), dim3(blocks), dim3(256), 0, st, m, bounds, drec, parts, spans, views,
+			a, sky, out, total_tiles, counters);
 	else if(tile_rows == 8)
-		hipLaunchKernelGGL((k_raster_tiles<CT, 8>), dim3(blocks), dim3(256), 0, st, m, bounds, drec, parts, spans, views, a,
-			sky, out, total_tiles, counters);
+		hipLaunchKernelGGL((k_raster_tiles<CT, 8, SSPAN>), dim3(blocks), dim3(256), 0, st, m, bounds, drec, parts, spans, views,
+			a, sky, out, total_tiles, counters);
 	else
-		hipLaunchKernelGGL((k_raster_tiles<CT, 16>), dim3(blocks), dim3(256), 0, st, m, bounds, drec, parts, spans, views, a,
-			sky, out, total_tiles, counters);
+		hipLaunchKernelGGL((k_raster_tiles<CT, 16, SSPAN>), dim3(blocks), dim3(256), 0, st, m, bounds, drec, parts, spans, views,
+			a, sky, out, total_tiles, counters);
 }
 
-void launch_raster_tiles(hipStream_t st, uint32_t color_bits, uint32_t tile_rows, const MapPtrs& m, const DrawBounds* bounds,
-	const DrawRec* drec, const PartRec* parts, const SpanRec* spans, const ViewsArg& views, const void* arena,
-	const void* skybox, void* fb, uint32_t total_tiles, const uint32_t* counters)
+void launch_raster_tiles(hipStream_t st, uint32_t color_bits, uint32_t tile_rows, bool scalar_spans, const MapPtrs& m,
+	const DrawBounds* bounds, const DrawRec* drec, const PartRec* parts, const SpanRec* spans, const ViewsArg& views,
+	const void* arena, const void* skybox, void* fb, uint32_t total_tiles, const uint32_t* counters)
 {
 	if(color_bits == 64)
-		launch_raster_ct<Color64T>(st, tile_rows, m, bounds, drec, parts, spans, views, arena, skybox, fb, total_tiles, counters);
+	{
+		if(scalar_spans)
+			launch_raster_ct<Color64T, true>(st, tile_rows, m, bounds, drec, parts, spans, views, arena, skybox, fb, total_tiles,
+				counters);
+		else
+			launch_raster_ct<Color64T, false>(st, tile_rows, m, bounds, drec, parts, spans, views, arena, skybox, fb, total_tiles,
+				counters);
+	}
+	else if(scalar_spans)
+		launch_raster_ct<Color32T, true>(st, tile_rows, m, bounds, drec, parts, spans, views, arena, skybox, fb, total_tiles,
+			counters);
 	else
-		launch_raster_ct<Color32T>(st, tile_rows, m, bounds, drec, parts, spans, views, arena, skybox, fb, total_tiles, counters);
+		launch_raster_ct<Color32T, false>(st, tile_rows, m, bounds, drec, parts, spans, views, arena, skybox, fb, total_tiles,
+			counters);
 }
 
 void launch_finish(hipStream_t st, const DrawRec* drec, uint32_t nd, uint32_t* counters, uint32_t* result)

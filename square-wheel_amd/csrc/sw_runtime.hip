@@ -67,6 +67,7 @@ struct sw_ctx
 	DevBuf views, surfaces, lights, light_indices, draws, vdata, surf_ext, blk_surf, parts, spans, drec, dbounds, counters, arena, fb,
 		tonemap_out, tonemap_sums;
 	uint32_t tile_rows = 8; // raster tile height (SW_TILE_ROWS_*), set by sw_set_tile_rows / the autotuner
+	bool scalar_spans = true; // raster span records through the scalar unit (chosen by the autotuner)
 	uint32_t* counters_host = nullptr; // pinned, device-mapped: result block (k_finish), then the error words
 	bool finish_launched = false; // the last launch sequence ended with k_finish (span allocation mode)
 	bool stats_pending = false; // the last raster batch's statistics are computed when asked for
@@ -465,7 +466,8 @@ int launch_stages(sw_ctx* ctx, const sw_ctx::Launch& L, bool surfaces, bool rast
 	{
 		{
 			StageTimer t(ctx, STAGE_RASTER);
-			swk::launch_raster_tiles(ctx->stream, L.color_bits, ctx->tile_rows, map->ptrs, static_cast<DrawBounds*>(ctx->dbounds.p),
+			swk::launch_raster_tiles(ctx->stream, L.color_bits, ctx->tile_rows, ctx->scalar_spans, map->ptrs,
+				static_cast<DrawBounds*>(ctx->dbounds.p),
 				static_cast<DrawRec*>(ctx->drec.p), static_cast<PartRec*>(ctx->parts.p), static_cast<SpanRec*>(ctx->spans.p),
 				ctx->views_arg, L.arena, L.color_bits == 64 ? map->sky64.p : map->sky32.p, L.fb, L.total_tiles, counters);
 		}
@@ -1353,6 +1355,7 @@ int sw_render_batch_graph(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out
 	mix(b->color_bits);
 	mix(b->surface_arena_texels);
 	mix(ctx->tile_rows);
+	mix(ctx->scalar_spans ? 1 : 0);
 	for(uint32_t i = 0; i < b->num_views; ++i)
 	{
 		mix(b->views[i].width);
@@ -1438,6 +1441,20 @@ int sw_set_tile_rows(sw_ctx* ctx, uint32_t rows)
 
 uint32_t sw_get_tile_rows(const sw_ctx* ctx) { return ctx ? ctx->tile_rows : 0; }
 
+int sw_set_scalar_spans(sw_ctx* ctx, uint32_t enable)
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	if((enable != 0) != ctx->scalar_spans)
+	{
+		ctx->scalar_spans = enable != 0;
+		ctx->graph_key = 0;
+	}
+	return SW_OK;
+}
+
+uint32_t sw_get_scalar_spans(const sw_ctx* ctx) { return ctx && ctx->scalar_spans ? 1u : 0u; }
+
 int sw_autotune_batch(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_fb, uint32_t* out_rows,
 	double out_ms[SW_TILE_ROWS_COUNT])
 {
@@ -1451,12 +1468,16 @@ int sw_autotune_batch(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_fb,
 		r = hip_check(ctx, hipEventCreate(&e1), "hipEventCreate");
 	double best_ms = 0.0;
 	uint32_t best = ctx->tile_rows;
+	bool best_scalar = ctx->scalar_spans;
 	const bool profiling = ctx->profiling;
 	ctx->profiling = false;
-	for(uint32_t c = 0; c < SW_TILE_ROWS_COUNT && !r; ++c)
+	// every tile height with both ways of loading span records; out_ms keeps each height's better
+	for(uint32_t c = 0; c < 2 * SW_TILE_ROWS_COUNT && !r; ++c)
 	{
-		if((r = sw_set_tile_rows(ctx, candidates[c])))
+		const uint32_t rows = candidates[c / 2];
+		if((r = sw_set_tile_rows(ctx, rows)))
 			break;
+		ctx->scalar_spans = (c & 1) != 0;
 		// one untimed run sizes the buffers, then the median of three timed runs
 		if((r = sw_render_batch(ctx, map, b, out_fb)))
 			break;
@@ -1476,12 +1497,13 @@ int sw_autotune_batch(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_fb,
 		if(r)
 			break;
 		std::sort(t, t + 3);
-		if(out_ms)
-			out_ms[c] = t[1];
+		if(out_ms && ((c & 1) == 0 || t[1] < out_ms[c / 2]))
+			out_ms[c / 2] = t[1];
 		if(c == 0 || t[1] < best_ms)
 		{
 			best_ms = t[1];
-			best = candidates[c];
+			best = rows;
+			best_scalar = ctx->scalar_spans;
 		}
 	}
 	ctx->profiling = profiling;
@@ -1490,6 +1512,11 @@ int sw_autotune_batch(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_fb,
 	if(e1)
 		(void)hipEventDestroy(e1);
 	const int r2 = sw_set_tile_rows(ctx, best);
+	if(ctx->scalar_spans != best_scalar)
+	{
+		ctx->scalar_spans = best_scalar;
+		ctx->graph_key = 0;
+	}
 	if(out_rows)
 		*out_rows = best;
 	return r ? r : r2;

@@ -201,6 +201,8 @@ GPU_SYMBOLS = {
                                     C.POINTER(C.c_float), C.c_void_p, C.c_size_t, C.POINTER(C.c_float)]),
     "sw_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "sw_last_reruns": (C.c_uint32, [C.c_void_p]),
+    "sw_set_scalar_spans": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "sw_get_scalar_spans": (C.c_uint32, [C.c_void_p]),
     "sw_read_surfaces": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     "sw_frame_device_framebuffer": (C.c_void_p, [C.c_void_p]),
     "sw_ctx_stream": (C.c_void_p, [C.c_void_p]),
@@ -361,6 +363,12 @@ class Context:
 
     def tile_rows(self):
         return int(self.lib.sw_get_tile_rows(self.h))
+
+    def set_scalar_spans(self, enable):
+        self.check(self.lib.sw_set_scalar_spans(self.h, 1 if enable else 0), "sw_set_scalar_spans")
+
+    def scalar_spans(self):
+        return bool(self.lib.sw_get_scalar_spans(self.h))
 
     def autotune(self, map_h, b: BatchData, device_inputs, out):
         """Times every raster tile height on batch `b` (device-resident inputs) and keeps the fastest.

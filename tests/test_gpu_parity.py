@@ -129,19 +129,23 @@ def test_tonemap_bit_exact(gpu_ctx, exposure):
 
 @pytest.mark.parametrize("color_bits", [32, 64])
 @pytest.mark.parametrize("rows", [4, 8, 16])
-def test_every_tile_height_bit_exact(gpu_ctx, rows, color_bits):
-    """The raster tile height changes the launch shape only, never a pixel."""
-    _say(f"case tile rows={rows} bits={color_bits}")
+@pytest.mark.parametrize("scalar_spans", [False, True])
+def test_every_tile_height_bit_exact(gpu_ctx, rows, color_bits, scalar_spans):
+    """The raster tile height and the way span records are loaded change the launch only, never a
+    pixel."""
+    _say(f"case tile rows={rows} bits={color_bits} scalar_spans={scalar_spans}")
     m, b = scenes.make_batch(30 + rows, color_bits=color_bits, view_sizes=((160, 120), (97, 61), (64, 200)))
-    prev = gpu_ctx.tile_rows()
+    prev = gpu_ctx.tile_rows(), gpu_ctx.scalar_spans()
     mh = gpu_ctx.upload_map(m)
     try:
         gpu_ctx.set_tile_rows(rows)
-        assert gpu_ctx.tile_rows() == rows
+        gpu_ctx.set_scalar_spans(scalar_spans)
+        assert gpu_ctx.tile_rows() == rows and gpu_ctx.scalar_spans() == scalar_spans
         fb = gpu_ctx.render_batch(mh, b)
         st = gpu_ctx.stats()
     finally:
-        gpu_ctx.set_tile_rows(prev)
+        gpu_ctx.set_tile_rows(prev[0])
+        gpu_ctx.set_scalar_spans(prev[1])
         gpu_ctx.release_map(mh)
     ofb, _, ost, _ = oracle.render(m, b, mode=oracle.MODE_EXACT)
     assert np.array_equal(fb, ofb)
