@@ -141,6 +141,40 @@ def hdr_leg(ctx, args):
             ctx.profile_enable(False)
     wall = time.perf_counter() - t0
     prof = ctx.profile_read(reset=True)
+
+    # the same frames postprocessed with bloom (sw_postprocess_device): the reference's default
+    # settings (sigma 1, half-resolution bloom buffer, scale 0.25, nearest upsampling) and a wide
+    # linear-filtered bloom (sigma 8)
+    bloom_out = {}
+    avg = (C.c_float * 3)()
+    for name, bp in (("default", sw_abi.BloomParams(1.0, 1, 0.25, 0)), ("sigma8_linear", sw_abi.BloomParams(8.0, 1, 0.25, 1))):
+        def bstep(i):
+            ctx.check(lib.sw_render_batch(ctx.h, mh, C.byref(cbs[i % len(cbs)]), hdr.ptr), "sw_render_batch")
+            ctx.check(lib.sw_postprocess_device(ctx.h, hdr.ptr, W, H, C.c_float(1.0), mod, C.byref(bp), ldr.ptr, W * 4, avg),
+                      "sw_postprocess_device")
+        for i in range(len(cbs)):
+            bstep(i)
+        ctx.profile_read(reset=True)
+        blat = np.zeros(args.hdr_steps)
+        tb = time.perf_counter()
+        for i in range(args.hdr_steps):
+            sample = i % 8 == 0
+            if sample:
+                ctx.profile_enable(True)
+            ts = time.perf_counter()
+            bstep(i)
+            blat[i] = time.perf_counter() - ts
+            if sample:
+                ctx.profile_enable(False)
+        bwall = time.perf_counter() - tb
+        bprof = ctx.profile_read(reset=True)
+        bus = bprof["bloom"][0] / max(bprof["bloom"][1], 1) * 1e3
+        # the postprocess's algorithmic bytes: the HDR frame read, the Color32 frame written
+        bloom_out[name] = {"frames_s": round(args.hdr_steps / bwall, 2),
+                           "p50_ms": round(float(np.percentile(blat, 50)) * 1e3, 4),
+                           "p99_ms": round(float(np.percentile(blat, 99)) * 1e3, 4),
+                           "postprocess_us": round(bus, 3),
+                           "postprocess_gb_s": round(W * H * 12 / (bus * 1e-6) / 1e9, 1)}
     for d in dev:
         d.free()
     hdr.free()
@@ -152,7 +186,7 @@ def hdr_leg(ctx, args):
            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 4),
            "triangle_draws_per_frame": round(float(np.mean(tris)), 1),
            "kernel_us_per_launch": {k: round(v[0] / max(v[1], 1) * 1e3, 3) for k, v in prof.items() if v[1]},
-           "tile_rows": rows, "scalar_spans": ctx.scalar_spans()}
+           "tile_rows": rows, "scalar_spans": ctx.scalar_spans(), "with_bloom": bloom_out}
     log(f"hdr + models: {out}")
     return out
 

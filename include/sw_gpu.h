@@ -355,6 +355,43 @@ int sw_tonemap_device(
 	sw_ctx* ctx, const void* hdr_device, uint32_t width, uint32_t height, float exposure,
 	const float color_modulate[3], void* dst32_device, size_t dst_pitch_bytes, float out_colors_sum[3]);
 
+/* The bloom settings of PostprocessorConfig (postprocessor_config.rs), as the reference clamps them
+ * (postprocessor.rs:928-940): sigma in [0, 40] (0 = no bloom), scale_log2 in [1, 3], scale in
+ * [1/16, 1/2]. */
+typedef struct sw_bloom_params
+{
+	float sigma; /* bloom_sigma, in frame pixels */
+	uint32_t scale_log2; /* bloom_buffer_scale_log2 */
+	float scale; /* bloom_scale: the blurred buffer's weight */
+	uint32_t linear_filter; /* linear_bloom_filter: bilinear upsampling (else nearest) */
+} sw_bloom_params;
+
+/* Postprocessor::perform_postprocessing (postprocessor.rs:68-204) on a device Color64 frame:
+ * with bloom (bloom != NULL and sigma > 0): the frame is downscaled by 2^scale_log2 (integer box
+ * average), blurred by a separable integer Gaussian and added, upsampled, to every pixel before the
+ * tonemap (:258-956); without, the plain tonemap. out_average_rgb: the average colour the reference
+ * hands to update_exposure (the downscaled buffer's with bloom, the frame's without). The frame
+ * must be at least one bloom texel in each direction. */
+int sw_postprocess_device(
+	sw_ctx* ctx, const void* hdr_device, uint32_t width, uint32_t height, float exposure, const float color_modulate[3],
+	const sw_bloom_params* bloom, void* dst32_device, size_t dst_pitch_bytes, float out_average_rgb[3]);
+
+/* The auto-exposure settings of PostprocessorConfig, clamped as the reference does. */
+typedef struct sw_exposure_params
+{
+	float exposure_update_speed;
+	float base_brightness;
+	float zero_level_brightness;
+	float brightness_scale_power;
+	float min_exposure;
+	float max_exposure;
+} sw_exposure_params;
+
+/* Postprocessor::update_exposure (postprocessor.rs:206-223): the next frame's exposure from this
+ * frame's average colour (sw_postprocess_device) and duration. Host-side arithmetic. */
+float sw_update_exposure(
+	const sw_exposure_params* params, float current_exposure, const float average_rgb[3], float frame_duration_s);
+
 /* Work counters of the last batch. The pixel counters of a batch are summed on the first call
  * after it (one small kernel and a wait), so batches that nobody asks about pay nothing for them.
  * No counterpart in the reference (its renderer keeps no such counters). */
@@ -391,10 +428,10 @@ void sw_host_free(sw_ctx* ctx, void* host_ptr);
 /* Kernel timing with HIP events recorded on the context's stream around every launch.
  * Stages: 0 setup_draws (edge walk, span records and, for device-resident descriptors, surface
  * extents), 1 build_surfaces, 2 raster_tiles, 3 tonemap, 4 setup_and_surfaces (draw setup and
- * surface texels in one launch, for host-resident descriptors). sw_profile_read returns the summed
- * milliseconds and launch counts since the last reset (and resets when reset != 0). Not recorded
- * inside captured graphs. */
-#define SW_PROFILE_STAGES 5
+ * surface texels in one launch, for host-resident descriptors), 5 bloom (downscale, blur and the
+ * tonemap with bloom). sw_profile_read returns the summed milliseconds and launch counts since the
+ * last reset (and resets when reset != 0). Not recorded inside captured graphs. */
+#define SW_PROFILE_STAGES 6
 int sw_profile_enable(sw_ctx* ctx, uint32_t enable);
 int sw_profile_read(sw_ctx* ctx, double out_ms[SW_PROFILE_STAGES], uint64_t out_counts[SW_PROFILE_STAGES], uint32_t reset);
 

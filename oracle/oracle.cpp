@@ -1556,6 +1556,238 @@ void tonemap_rows(
 	*sum_out = colors_sum;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Bloom postprocessing (postprocessor.rs:258-956), restated loop by loop, single-threaded (the
+// reference's row ranges partition the same per-pixel work).
+// ---------------------------------------------------------------------------------------------
+const int MAX_BLOOM_RADIUS = 16; // postprocessor.rs:941
+const int MAX_BLOOM_KERNEL = 1 + 2 * MAX_BLOOM_RADIUS;
+
+// compute_gaussian_kernel (postprocessor.rs:1026-1045)
+void gaussian_kernel(float sigma, int radius, float out[MAX_BLOOM_KERNEL])
+{
+	for(int i = 0; i < MAX_BLOOM_KERNEL; ++i)
+		out[i] = 0.0f;
+	const float samples[4] = {-0.375f, -0.125f, 0.125f, 0.375f};
+	const float sqrt_tau = std::sqrt(6.28318530717958647692f);
+	for(int x = -radius; x <= radius; ++x)
+	{
+		float val = 0.0f;
+		for(float sample : samples)
+		{
+			const float coord = float(x) + sample;
+			val += std::exp(-0.5f * (coord / sigma) * (coord / sigma)) / (sigma * sqrt_tau);
+		}
+		out[x + radius] = val / 4.0f;
+	}
+}
+
+struct Bloom
+{
+	uint32_t w = 0, h = 0; // hdr size
+	uint32_t bw = 0, bh = 0, scale = 1;
+	std::vector<uint64_t> buf[2];
+	__m128 average = _mm_setzero_ps();
+};
+
+// downscale_hdr_buffer (postprocessor.rs:805-853): integer box average into buffer 0, and the
+// average colour (integer line sums, float sum of lines)
+void bloom_downscale(const uint64_t* hdr, Bloom& b)
+{
+	const int color_shift = 8;
+	const int32_t average_scaler = (1 << color_shift) / int32_t(b.scale * b.scale);
+	__m128 colors_sum = _mm_setzero_ps();
+	for(uint32_t dy = 0; dy < b.bh; ++dy)
+	{
+		__m128i line_sum = _mm_setzero_si128();
+		for(uint32_t dx = 0; dx < b.bw; ++dx)
+		{
+			__m128i sum = _mm_setzero_si128();
+			for(uint32_t y = 0; y < b.scale; ++y)
+				for(uint32_t x = 0; x < b.scale; ++x)
+					sum = _mm_add_epi32(sum, ci_from_color64(hdr[uint64_t(dy * b.scale + y) * b.w + dx * b.scale + x]));
+			const __m128i average = _mm_srai_epi32(_mm_mullo_epi32(sum, _mm_set1_epi32(average_scaler)), color_shift);
+			b.buf[0][uint64_t(dy) * b.bw + dx] = ci_into_color64(average);
+			line_sum = _mm_add_epi32(line_sum, average);
+		}
+		colors_sum = _mm_add_ps(colors_sum, _mm_cvtepi32_ps(line_sum));
+	}
+	b.average = _mm_mul_ps(colors_sum, _mm_set1_ps(1.0f / float(b.bw * b.bh)));
+}
+
+// perform_blur_impl (postprocessor.rs:855-937): horizontal pass 0 -> 1, vertical pass 1 -> 0 with
+// the weights scaled by bloom_scale
+void bloom_blur(Bloom& b, const float kernel[MAX_BLOOM_KERNEL], int radius, float bloom_scale)
+{
+	const int color_shift = 14;
+	int32_t ki[MAX_BLOOM_KERNEL];
+	for(int i = 0; i < MAX_BLOOM_KERNEL; ++i)
+		ki[i] = int32_t(kernel[i] * float(1 << color_shift));
+	const int32_t last_x = int32_t(b.bw) - 1, last_y = int32_t(b.bh) - 1;
+	for(uint32_t y = 0; y < b.bh; ++y)
+		for(uint32_t x = 0; x < b.bw; ++x)
+		{
+			const uint64_t* line = &b.buf[0][uint64_t(y) * b.bw];
+			__m128i sum = _mm_mullo_epi32(ci_from_color64(line[x]), _mm_set1_epi32(ki[radius]));
+			for(int d = 1; d <= radius; ++d)
+			{
+				const int32_t x0 = std::max(int32_t(x) - d, 0), x1 = std::min(int32_t(x) + d, last_x);
+				const __m128i two = _mm_add_epi32(ci_from_color64(line[x0]), ci_from_color64(line[x1]));
+				sum = _mm_add_epi32(sum, _mm_mullo_epi32(two, _mm_set1_epi32(ki[d + radius])));
+			}
+			b.buf[1][uint64_t(y) * b.bw + x] = ci_into_color64(_mm_srai_epi32(sum, color_shift));
+		}
+	for(int i = 0; i < MAX_BLOOM_KERNEL; ++i)
+		ki[i] = int32_t(kernel[i] * bloom_scale * float(1 << color_shift));
+	for(uint32_t y = 0; y < b.bh; ++y)
+		for(uint32_t x = 0; x < b.bw; ++x)
+		{
+			const uint64_t* col = &b.buf[1][x];
+			__m128i sum = _mm_mullo_epi32(ci_from_color64(col[uint64_t(y) * b.bw]), _mm_set1_epi32(ki[radius]));
+			for(int d = 1; d <= radius; ++d)
+			{
+				const int32_t y0 = std::max(int32_t(y) - d, 0), y1 = std::min(int32_t(y) + d, last_y);
+				const __m128i two =
+					_mm_add_epi32(ci_from_color64(col[uint64_t(y0) * b.bw]), ci_from_color64(col[uint64_t(y1) * b.bw]));
+				sum = _mm_add_epi32(sum, _mm_mullo_epi32(two, _mm_set1_epi32(ki[d + radius])));
+			}
+			b.buf[0][uint64_t(y) * b.bw + x] = ci_into_color64(_mm_srai_epi32(sum, color_shift));
+		}
+}
+
+struct Tonemapper
+{
+	__m128 inv_scale, inv_255;
+	Tonemapper(float exposure, const float modulate[3])
+	{
+		const float inv_255_s = 1.0f / 255.0f;
+		const float is[3] = {1.0f / (exposure * modulate[0]), 1.0f / (exposure * modulate[1]), 1.0f / (exposure * modulate[2])};
+		const float i255[3] = {inv_255_s, inv_255_s, inv_255_s};
+		inv_scale = cv_from_f32x3(is);
+		inv_255 = cv_from_f32x3(i255);
+	}
+	uint32_t map(__m128 c) const { return cv_into_color32(_mm_div_ps(c, _mm_fmadd_ps(c, inv_255, inv_scale))); }
+};
+
+// perform_tonemapping_with_bloom_nearest (postprocessor.rs:346-470)
+void bloom_tonemap_nearest(const uint64_t* hdr, const Bloom& b, const Tonemapper& t, uint32_t* dst, uint32_t pitch)
+{
+	const uint32_t S = b.scale;
+	const uint32_t columns_left = b.w - b.bw * S, lines_left = b.h - b.bh * S;
+	auto put = [&](uint32_t x, uint32_t y, __m128 bloom) {
+		dst[uint64_t(y) * pitch + x] = t.map(_mm_add_ps(bloom, cv_from_color64(hdr[uint64_t(y) * b.w + x])));
+	};
+	for(uint32_t sy = 0; sy < b.bh; ++sy)
+	{
+		for(uint32_t sx = 0; sx < b.bw; ++sx)
+		{
+			const __m128 bc = cv_from_color64(b.buf[0][uint64_t(sy) * b.bw + sx]);
+			for(uint32_t dy = 0; dy < S; ++dy)
+				for(uint32_t dx = 0; dx < S; ++dx)
+					put(sx * S + dx, sy * S + dy, bc);
+		}
+		if(columns_left > 0)
+		{
+			const __m128 bc = cv_from_color64(b.buf[0][uint64_t(sy) * b.bw + b.bw - 1]);
+			for(uint32_t dy = 0; dy < S; ++dy)
+				for(uint32_t dx = 0; dx < columns_left; ++dx)
+					put(b.bw * S + dx, sy * S + dy, bc);
+		}
+	}
+	if(lines_left > 0)
+	{
+		const uint64_t* last = &b.buf[0][uint64_t(b.bh - 1) * b.bw];
+		for(uint32_t sx = 0; sx < b.bw; ++sx)
+		{
+			const __m128 bc = cv_from_color64(last[sx]);
+			for(uint32_t dy = 0; dy < lines_left; ++dy)
+				for(uint32_t dx = 0; dx < S; ++dx)
+					put(sx * S + dx, b.bh * S + dy, bc);
+		}
+		if(columns_left > 0)
+		{
+			const __m128 bc = cv_from_color64(last[b.bw - 1]);
+			for(uint32_t dy = 0; dy < lines_left; ++dy)
+				for(uint32_t dx = 0; dx < columns_left; ++dx)
+					put(b.bw * S + dx, b.bh * S + dy, bc);
+		}
+	}
+}
+
+// perform_tonemapping_with_bloom_linear (postprocessor.rs:472-726)
+void bloom_tonemap_linear(const uint64_t* hdr, const Bloom& b, const Tonemapper& t, uint32_t* dst, uint32_t pitch)
+{
+	const uint32_t S = b.scale;
+	float mix[8];
+	const float sf = float(S);
+	for(uint32_t i = 0; i < S; ++i)
+		mix[i] = float((S + 1) & 1) / (2.0f * sf) + float(i) / sf;
+	auto bl = [&](uint32_t x, uint32_t y) { return cv_from_color64(b.buf[0][uint64_t(y) * b.bw + x]); };
+	auto hdr_at = [&](uint32_t x, uint32_t y) { return cv_from_color64(hdr[uint64_t(y) * b.w + x]); };
+	auto fms = [](__m128 a, float s, __m128 c) { return _mm_fmadd_ps(a, _mm_set1_ps(s), c); };
+	auto out = [&](uint32_t x, uint32_t y, __m128 sum) { dst[uint64_t(y) * pitch + x] = t.map(sum); };
+	const uint32_t right_x = (b.bw - 1) * S + S / 2, right_n = b.w - right_x;
+	const uint32_t upper_y = (b.bh - 1) * S + S / 2, upper_n = b.h - upper_y;
+	// lower border
+	for(uint32_t sx = 0; sx + 1 < b.bw; ++sx)
+	{
+		const __m128 b00 = bl(sx, 0), b10 = bl(sx + 1, 0);
+		for(uint32_t dy = 0; dy < S / 2; ++dy)
+			for(uint32_t dx = 0; dx < S; ++dx)
+			{
+				const uint32_t x = sx * S + S / 2 + dx;
+				out(x, dy, fms(b00, 1.0f - mix[dx], fms(b10, mix[dx], hdr_at(x, dy))));
+			}
+	}
+	// main rows
+	for(uint32_t sy = 0; sy + 1 < b.bh; ++sy)
+	{
+		const uint32_t y0 = sy * S + S / 2;
+		for(uint32_t dy = 0; dy < S; ++dy)
+		{
+			const float my = mix[dy], omy = 1.0f - my;
+			const uint32_t y = y0 + dy;
+			const __m128 left = fms(bl(0, sy), omy, _mm_mul_ps(bl(0, sy + 1), _mm_set1_ps(my)));
+			for(uint32_t x = 0; x < S / 2; ++x)
+				out(x, y, _mm_add_ps(left, hdr_at(x, y)));
+			for(uint32_t sx = 0; sx + 1 < b.bw; ++sx)
+			{
+				const __m128 l = fms(bl(sx, sy), omy, _mm_mul_ps(bl(sx, sy + 1), _mm_set1_ps(my)));
+				const __m128 r = fms(bl(sx + 1, sy), omy, _mm_mul_ps(bl(sx + 1, sy + 1), _mm_set1_ps(my)));
+				for(uint32_t dx = 0; dx < S; ++dx)
+				{
+					const uint32_t x = sx * S + S / 2 + dx;
+					out(x, y, fms(l, 1.0f - mix[dx], fms(r, mix[dx], hdr_at(x, y))));
+				}
+			}
+			const __m128 right = fms(bl(b.bw - 1, sy), omy, _mm_mul_ps(bl(b.bw - 1, sy + 1), _mm_set1_ps(my)));
+			for(uint32_t dx = 0; dx < right_n; ++dx)
+				out(right_x + dx, y, _mm_add_ps(right, hdr_at(right_x + dx, y)));
+		}
+	}
+	// upper border
+	for(uint32_t sx = 0; sx + 1 < b.bw; ++sx)
+	{
+		const __m128 b00 = bl(sx, b.bh - 1), b10 = bl(sx + 1, b.bh - 1);
+		for(uint32_t dy = 0; dy < upper_n; ++dy)
+			for(uint32_t dx = 0; dx < S; ++dx)
+			{
+				const uint32_t x = sx * S + S / 2 + dx, y = upper_y + dy;
+				out(x, y, fms(b00, 1.0f - mix[dx], fms(b10, mix[dx], hdr_at(x, y))));
+			}
+	}
+	// corners
+	auto corner = [&](uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1, __m128 v) {
+		for(uint32_t y = y0; y < y1; ++y)
+			for(uint32_t x = x0; x < x1; ++x)
+				out(x, y, _mm_add_ps(v, hdr_at(x, y)));
+	};
+	corner(0, S / 2, 0, S / 2, bl(0, 0));
+	corner(right_x, b.w, 0, S / 2, bl(b.bw - 1, 0));
+	corner(0, S / 2, upper_y, b.h, bl(0, b.bh - 1));
+	corner(right_x, b.w, upper_y, b.h, bl(b.bw - 1, b.bh - 1));
+}
+
 } // namespace
 
 // ---------------------------------------------------------------------------------------------
@@ -1617,6 +1849,50 @@ int oracle_tonemap(
 	out_sum[1] = v[1];
 	out_sum[2] = v[0];
 	return 0;
+}
+
+// Postprocessing with bloom (postprocessor.rs:68-204 with bloom_sigma > 0): returns the blur
+// radius (or -1 on a frame smaller than one bloom texel); out_average_rgb: the downscaled image's
+// average colour, which the reference passes to update_exposure.
+int oracle_bloom(
+	const uint64_t* hdr, uint32_t width, uint32_t height, float exposure, const float modulate[3], float sigma,
+	uint32_t scale_log2, float bloom_scale, uint32_t linear, uint32_t* dst, uint32_t dst_pitch_pixels, float out_average_rgb[3],
+	int32_t out_kernel[2 * MAX_BLOOM_KERNEL])
+{
+	Bloom b;
+	b.w = width;
+	b.h = height;
+	b.scale = 1u << std::min<uint32_t>(std::max<uint32_t>(scale_log2, 1), 3);
+	b.bw = width / b.scale;
+	b.bh = height / b.scale;
+	if(b.bw == 0 || b.bh == 0)
+		return -1;
+	b.buf[0].assign(uint64_t(b.bw) * b.bh, 0);
+	b.buf[1].assign(uint64_t(b.bw) * b.bh, 0);
+	bloom_downscale(hdr, b);
+	const float sigma_c = sigma / float(b.scale);
+	const int radius = std::min(int(std::max(std::ceil(3.0f * sigma_c - 0.5f), 0.0f)), MAX_BLOOM_RADIUS);
+	float kernel[MAX_BLOOM_KERNEL];
+	gaussian_kernel(sigma_c, radius, kernel);
+	if(out_kernel)
+		for(int i = 0; i < MAX_BLOOM_KERNEL; ++i)
+		{
+			out_kernel[i] = int32_t(kernel[i] * float(1 << 14));
+			out_kernel[MAX_BLOOM_KERNEL + i] = int32_t(kernel[i] * bloom_scale * float(1 << 14));
+		}
+	if(radius > 0)
+		bloom_blur(b, kernel, radius, bloom_scale);
+	const Tonemapper t(exposure, modulate);
+	if(linear)
+		bloom_tonemap_linear(hdr, b, t, dst, dst_pitch_pixels);
+	else
+		bloom_tonemap_nearest(hdr, b, t, dst, dst_pitch_pixels);
+	alignas(16) float v[4];
+	_mm_store_ps(v, b.average);
+	out_average_rgb[0] = v[2];
+	out_average_rgb[1] = v[1];
+	out_average_rgb[2] = v[0];
+	return radius;
 }
 
 // Elementary operations, exposed for the closed-form / exhaustive tests.

@@ -37,6 +37,10 @@ def load():
         lib.oracle_build_surfaces.restype = C.c_int
         lib.oracle_build_surfaces.argtypes = [C.c_int, C.POINTER(sw_abi.MapDesc), C.POINTER(sw_abi.Batch), C.c_void_p,
                                               C.POINTER(sw_abi.Stats)]
+        lib.oracle_bloom.restype = C.c_int
+        lib.oracle_bloom.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.POINTER(C.c_float), C.c_float,
+                                     C.c_uint32, C.c_float, C.c_uint32, C.c_void_p, C.c_uint32, C.POINTER(C.c_float),
+                                     C.POINTER(C.c_int32)]
         lib.oracle_tonemap.restype = C.c_int
         lib.oracle_tonemap.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.POINTER(C.c_float), C.c_void_p,
                                        C.c_uint32, C.POINTER(C.c_float)]
@@ -94,6 +98,24 @@ def build_surfaces(map_data, batch, mode=MODE_EXACT):
     st = sw_abi.Stats()
     assert lib.oracle_build_surfaces(mode, C.byref(md), C.byref(cb), C.c_void_p(arena.ctypes.data), C.byref(st)) == 0
     return arena, st.as_dict()
+
+
+def bloom(hdr, exposure, modulate, sigma, scale_log2, bloom_scale, linear):
+    """Postprocessing with bloom (postprocessor.rs:68-204, 258-956). Returns (ldr, average rgb,
+    blur radius, integer kernels of the two passes)."""
+    lib = load()
+    hdr = np.ascontiguousarray(hdr, dtype=np.uint64)
+    h, w = hdr.shape
+    out = np.zeros((h, w), dtype=np.uint32)
+    avg = (C.c_float * 3)()
+    mod = (C.c_float * 3)(*modulate)
+    ker = (C.c_int32 * 66)()
+    r = lib.oracle_bloom(C.c_void_p(hdr.ctypes.data), w, h, C.c_float(exposure), mod, C.c_float(sigma), scale_log2,
+                         C.c_float(bloom_scale), 1 if linear else 0, C.c_void_p(out.ctypes.data), w, avg, ker)
+    if r < 0:
+        raise ValueError("frame smaller than one bloom texel")
+    k = np.array(ker[:], dtype=np.int32)
+    return out, np.array(avg[:], dtype=np.float32), int(r), (k[:33], k[33:])
 
 
 def tonemap(hdr, exposure, modulate):

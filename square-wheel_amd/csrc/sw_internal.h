@@ -190,6 +190,27 @@ struct SurfaceArgs
 	uint32_t* host_err; // HOST_ERR_WORDS error words in host memory
 };
 
+// Bloom postprocessing (postprocessor.rs:258-956): the frame's downscaled, blurred copy added to it
+// before the tonemap.
+#define BLOOM_MAX_RADIUS 16
+#define BLOOM_KERNEL (1 + 2 * BLOOM_MAX_RADIUS)
+struct BloomArgs
+{
+	const uint64_t* hdr; // Color64 frame, pitch = w
+	uint32_t w, h;
+	uint32_t bw, bh, scale_log2; // bloom buffer size, scale 2^scale_log2 (1..3)
+	int32_t radius; // blur radius (0: no blur)
+	uint64_t* buf0; // bloom buffer (downscaled, then blurred)
+	uint64_t* buf1; // horizontal pass output
+	int32_t* line_sums; // 4 per bloom row: the row's sum of averages, lanes B, G, R, A
+	float* average; // 4 floats: the average colour, lanes B, G, R, A
+	int32_t k1[BLOOM_KERNEL], k2[BLOOM_KERNEL]; // integer weights of the two passes (scale 2^14)
+	float inv_scale[3]; // tonemap 1 / (exposure * modulate), lanes B, G, R
+	uint32_t linear; // bilinear upsampling of the bloom buffer (else nearest)
+	uint32_t* dst;
+	uint32_t dst_pitch; // in pixels
+};
+
 namespace swk
 {
 // Kernel 1: draw setup (edge walk -> parts -> span records) with `lanes` (64 or 256) threads per
@@ -210,6 +231,8 @@ void launch_raster_tiles(hipStream_t st, uint32_t color_bits, uint32_t tile_rows
 	const void* skybox, void* fb, uint32_t total_tiles, const uint32_t* counters);
 // Kernel 4 (one block): per-draw statistics -> result block (host-mapped), live block cleared.
 void launch_finish(hipStream_t st, const DrawRec* drec, uint32_t nd, uint32_t* counters, uint32_t* result);
+// Bloom: downscale + average colour, two blur passes, tonemap with the upsampled bloom.
+void launch_bloom(hipStream_t st, const BloomArgs& a);
 // Tonemap; `sums` holds 4 + 3 * 1024 floats (totals first, then per-block partials).
 void launch_tonemap(hipStream_t st, const uint64_t* hdr, uint32_t w, uint32_t h, float isr, float isg, float isb,
 	uint32_t* dst, uint32_t dst_pitch, float* sums);

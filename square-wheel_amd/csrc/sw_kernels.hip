@@ -2062,6 +2062,167 @@ void k_raster_tiles(MapPtrs m, const DrawBounds* __restrict__ bounds,
 }
 
 // =============================================================================================
+// Bloom (postprocessor.rs:258-956)
+// =============================================================================================
+
+// TonemappingFunction::do_it (postprocessor.rs:1019-1023): c / fma(c, 1/255, 1/(exposure*modulate))
+// per lane; the alpha lane's constants are 0, so it is c.a / 0 (255 after the pack).
+SW_DEV uint32_t tonemap_cv(CV c, const float* inv_scale)
+{
+	const float inv_255 = 1.0f / 255.0f;
+	const CV mapped = CV{c.b / mul_add(c.b, inv_255, inv_scale[0]), c.g / mul_add(c.g, inv_255, inv_scale[1]),
+		c.r / mul_add(c.r, inv_255, inv_scale[2]), c.a / mul_add(c.a, 0.0f, 0.0f)};
+	return Color32T::from_cv(mapped);
+}
+
+SW_DEV CV cv_add(CV a, CV b) { return CV{a.b + b.b, a.g + b.g, a.r + b.r, a.a + b.a}; }
+SW_DEV CI ci_add(CI a, CI b) { return CI{wadd(a.b, b.b), wadd(a.g, b.g), wadd(a.r, b.r), wadd(a.a, b.a)}; }
+SW_DEV CI ci_mul(CI a, int32_t s) { return CI{wmul(a.b, s), wmul(a.g, s), wmul(a.r, s), wmul(a.a, s)}; }
+SW_DEV CI ci_shr(CI a, int k) { return CI{a.b >> k, a.g >> k, a.r >> k, a.a >> k}; }
+
+// downscale_hdr_buffer (:805-853): one block per bloom row. Each bloom texel is the integer average
+// of its scale x scale source pixels ((sum * (256 / scale^2)) >> 8, packed with unsigned
+// saturation); the block also sums the row's averages (integers, so any order is exact).
+__global__ __launch_bounds__(256) void k_bloom_downscale(BloomArgs a)
+{
+	const uint32_t dy = blockIdx.x;
+	const uint32_t S = 1u << a.scale_log2;
+	const int32_t scaler = (1 << 8) / int32_t(S * S);
+	CI line = CI{0, 0, 0, 0};
+	for(uint32_t dx = threadIdx.x; dx < a.bw; dx += blockDim.x)
+	{
+		CI sum = CI{0, 0, 0, 0};
+		for(uint32_t y = 0; y < S; ++y)
+		{
+			const uint64_t* row = a.hdr + uint64_t(dy * S + y) * a.w + dx * S;
+			for(uint32_t x = 0; x < S; ++x)
+				sum = ci_add(sum, Color64T::to_ci(row[x]));
+		}
+		const CI average = ci_shr(ci_mul(sum, scaler), 8);
+		a.buf0[uint64_t(dy) * a.bw + dx] = Color64T::from_ci(average);
+		line = ci_add(line, average);
+	}
+	__shared__ int32_t red[4][4];
+	int32_t v[4] = {line.b, line.g, line.r, line.a};
+#pragma unroll
+	for(int k = 0; k < 4; ++k)
+		for(int off = 32; off > 0; off >>= 1)
+			v[k] = wadd(v[k], __shfl_xor(v[k], off));
+	const uint32_t wave = threadIdx.x >> 6;
+	if((threadIdx.x & 63) == 0)
+		for(int k = 0; k < 4; ++k)
+			red[wave][k] = v[k];
+	__syncthreads();
+	if(threadIdx.x < 4)
+	{
+		int32_t t = 0;
+		for(uint32_t w = 0; w < (blockDim.x + 63) / 64; ++w)
+			t = wadd(t, red[w][threadIdx.x]);
+		a.line_sums[dy * 4 + threadIdx.x] = t;
+	}
+}
+
+// The average colour (:838-852): the rows' integer sums converted and added in row order, then
+// scaled by 1 / (bloom texels) — sequential, like the reference, so the float sum is the same.
+__global__ void k_bloom_average(BloomArgs a)
+{
+	if(threadIdx.x != 0)
+		return;
+	CV sum = CV{0.0f, 0.0f, 0.0f, 0.0f};
+	for(uint32_t y = 0; y < a.bh; ++y)
+	{
+		const int32_t* l = a.line_sums + y * 4;
+		sum = cv_add(sum, CV{float(l[0]), float(l[1]), float(l[2]), float(l[3])});
+	}
+	const CV avg = cv_scalar_mul(sum, 1.0f / float(a.bw * a.bh));
+	a.average[0] = avg.b;
+	a.average[1] = avg.g;
+	a.average[2] = avg.r;
+	a.average[3] = avg.a;
+}
+
+// perform_blur_impl (:855-937), one pass: pass 0 horizontal (buf0 -> buf1, weights k1), pass 1
+// vertical (buf1 -> buf0, weights k2 = k1 scaled by bloom_scale); edge texels repeat (clamped
+// coordinates); integer weights at scale 2^14, arithmetic shift, unsigned-saturating pack.
+__global__ __launch_bounds__(256) void k_bloom_blur(BloomArgs a, uint32_t pass)
+{
+	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if(i >= uint64_t(a.bw) * a.bh)
+		return;
+	const uint32_t y = uint32_t(i / a.bw), x = uint32_t(i - uint64_t(y) * a.bw);
+	const int32_t R = a.radius;
+	const int32_t* k = pass == 0 ? a.k1 : a.k2;
+	const uint64_t* src = pass == 0 ? a.buf0 : a.buf1;
+	uint64_t* dst = pass == 0 ? a.buf1 : a.buf0;
+	const int32_t last = pass == 0 ? int32_t(a.bw) - 1 : int32_t(a.bh) - 1;
+	const int32_t c = pass == 0 ? int32_t(x) : int32_t(y);
+	auto at = [&](int32_t p) { return pass == 0 ? src[uint64_t(y) * a.bw + uint32_t(p)] : src[uint64_t(p) * a.bw + x]; };
+	CI sum = ci_mul(Color64T::to_ci(at(c)), k[R]);
+	for(int32_t d = 1; d <= R; ++d)
+	{
+		const int32_t p0 = imax32(c - d, 0), p1 = imin32(c + d, last);
+		const CI two = ci_add(Color64T::to_ci(at(p0)), Color64T::to_ci(at(p1)));
+		sum = ci_add(sum, ci_mul(two, k[d + R]));
+	}
+	dst[i] = Color64T::from_ci(ci_shr(sum, 14));
+}
+
+// perform_tonemapping_with_bloom_{nearest,linear} (:346-726), one thread per pixel: the bloom value
+// of the pixel's region (block, border or corner of the reference's loops), added to the pixel,
+// tonemapped.
+template <bool LINEAR>
+__global__ __launch_bounds__(256) void k_bloom_tonemap(BloomArgs a)
+{
+	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if(i >= uint64_t(a.w) * a.h)
+		return;
+	const uint32_t y = uint32_t(i / a.w), x = uint32_t(i - uint64_t(y) * a.w);
+	const uint32_t sl = a.scale_log2, S = 1u << sl;
+	const CV c = cv_from_color64(a.hdr[i]);
+	auto bl = [&](uint32_t bx, uint32_t by) { return cv_from_color64(a.buf0[uint64_t(by) * a.bw + bx]); };
+	CV sum;
+	if(!LINEAR)
+		sum = cv_add(bl(std::min(x >> sl, a.bw - 1), std::min(y >> sl, a.bh - 1)), c);
+	else
+	{
+		const uint32_t half = S / 2;
+		const float sf = float(S);
+		auto mix = [&](uint32_t k) { return float((S + 1) & 1) / (2.0f * sf) + float(k) / sf; };
+		const uint32_t right_x = (a.bw - 1) * S + half, upper_y = (a.bh - 1) * S + half;
+		const bool lower = y < half, upper = y >= upper_y, left = x < half, right = x >= right_x;
+		if((lower || upper) && (left || right))
+			sum = cv_add(bl(left ? 0 : a.bw - 1, lower ? 0 : a.bh - 1), c); // corners
+		else if(lower || upper)
+		{
+			// lower / upper border: horizontal mix of the first / last bloom row
+			const uint32_t by = lower ? 0 : a.bh - 1;
+			const uint32_t sx = (x - half) >> sl, dx = (x - half) & (S - 1);
+			const float mx = mix(dx);
+			sum = cv_mul_scalar_add(bl(sx, by), 1.0f - mx, cv_mul_scalar_add(bl(sx + 1, by), mx, c));
+		}
+		else
+		{
+			const uint32_t sy = (y - half) >> sl, dy = (y - half) & (S - 1);
+			const float my = mix(dy), omy = 1.0f - my;
+			if(left || right)
+			{
+				const uint32_t bx = left ? 0 : a.bw - 1;
+				sum = cv_add(cv_mul_scalar_add(bl(bx, sy), omy, cv_scalar_mul(bl(bx, sy + 1), my)), c);
+			}
+			else
+			{
+				const uint32_t sx = (x - half) >> sl, dx = (x - half) & (S - 1);
+				const float mx = mix(dx);
+				const CV l = cv_mul_scalar_add(bl(sx, sy), omy, cv_scalar_mul(bl(sx, sy + 1), my));
+				const CV r = cv_mul_scalar_add(bl(sx + 1, sy), omy, cv_scalar_mul(bl(sx + 1, sy + 1), my));
+				sum = cv_mul_scalar_add(l, 1.0f - mx, cv_mul_scalar_add(r, mx, c));
+			}
+		}
+	}
+	a.dst[uint64_t(y) * a.dst_pitch + x] = tonemap_cv(sum, a.inv_scale);
+}
+
+// =============================================================================================
 // Tonemap (postprocessor.rs:237-253, :1019-1023)
 // =============================================================================================
 // Tonemap blocks stride over the frame; each block leaves the partial colour sums of its pixels
@@ -2076,7 +2237,7 @@ __global__ __launch_bounds__(256) void k_tonemap(const uint64_t* __restrict__ hd
 	const uint64_t n = uint64_t(width) * height;
 	const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
 	float sb = 0.0f, sg = 0.0f, sr = 0.0f;
-	const float inv_255 = 1.0f / 255.0f;
+	const float inv_scale[3] = {inv_scale_b, inv_scale_g, inv_scale_r};
 	for(uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
 	{
 		const uint32_t y = uint32_t(i / width), x = uint32_t(i - uint64_t(y) * width);
@@ -2084,10 +2245,7 @@ __global__ __launch_bounds__(256) void k_tonemap(const uint64_t* __restrict__ hd
 		sb += c.b;
 		sg += c.g;
 		sr += c.r;
-		// div(c, mul_add(c, inv_255_vec, inv_scale_vec)); the alpha lane is c.a / fma(c.a, 0, 0)
-		const CV mapped = CV{c.b / mul_add(c.b, inv_255, inv_scale_b), c.g / mul_add(c.g, inv_255, inv_scale_g),
-			c.r / mul_add(c.r, inv_255, inv_scale_r), c.a / mul_add(c.a, 0.0f, 0.0f)};
-		dst[uint64_t(y) * dst_pitch + x] = Color32T::from_cv(mapped);
+		dst[uint64_t(y) * dst_pitch + x] = tonemap_cv(c, inv_scale);
 	}
 	for(int off = 32; off > 0; off >>= 1)
 	{
@@ -2248,6 +2406,23 @@ void launch_raster_tiles(hipStream_t st, uint32_t color_bits, uint32_t tile_rows
 void launch_finish(hipStream_t st, const DrawRec* drec, uint32_t nd, uint32_t* counters, uint32_t* result)
 {
 	hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, st, drec, nd, counters, result);
+}
+
+void launch_bloom(hipStream_t st, const BloomArgs& a)
+{
+	hipLaunchKernelGGL(k_bloom_downscale, dim3(a.bh), dim3(256), 0, st, a);
+	hipLaunchKernelGGL(k_bloom_average, dim3(1), dim3(64), 0, st, a);
+	const uint32_t blur_blocks = uint32_t((uint64_t(a.bw) * a.bh + 255) / 256);
+	if(a.radius > 0)
+	{
+		hipLaunchKernelGGL(k_bloom_blur, dim3(blur_blocks), dim3(256), 0, st, a, 0u);
+		hipLaunchKernelGGL(k_bloom_blur, dim3(blur_blocks), dim3(256), 0, st, a, 1u);
+	}
+	const uint32_t blocks = uint32_t((uint64_t(a.w) * a.h + 255) / 256);
+	if(a.linear)
+		hipLaunchKernelGGL(k_bloom_tonemap<true>, dim3(blocks), dim3(256), 0, st, a);
+	else
+		hipLaunchKernelGGL(k_bloom_tonemap<false>, dim3(blocks), dim3(256), 0, st, a);
 }
 
 void launch_tonemap(hipStream_t st, const uint64_t* hdr, uint32_t w, uint32_t h, float isr, float isg, float isb,

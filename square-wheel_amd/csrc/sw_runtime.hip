@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -65,7 +66,7 @@ struct sw_ctx
 
 	// command buffers / work buffers
 	DevBuf views, surfaces, lights, light_indices, draws, vdata, surf_ext, blk_surf, parts, spans, drec, dbounds, counters, arena, fb,
-		tonemap_out, tonemap_sums;
+		tonemap_out, tonemap_sums, bloom0, bloom1, bloom_lines, bloom_avg;
 	uint32_t tile_rows = 8; // raster tile height (SW_TILE_ROWS_*), set by sw_set_tile_rows / the autotuner
 	bool scalar_spans = true; // raster span records through the scalar unit (chosen by the autotuner)
 	uint32_t* counters_host = nullptr; // pinned, device-mapped: result block (k_finish), then the error words
@@ -221,6 +222,7 @@ enum
 	STAGE_RASTER = 2,
 	STAGE_TONEMAP = 3,
 	STAGE_SETUP_SURF = 4,
+	STAGE_BLOOM = 5,
 };
 
 hipEvent_t take_event(sw_ctx* ctx)
@@ -770,7 +772,8 @@ void sw_ctx_destroy(sw_ctx* ctx)
 	if(ctx->graph)
 		(void)hipGraphDestroy(ctx->graph);
 	DevBuf* bufs[] = {&ctx->views, &ctx->surfaces, &ctx->lights, &ctx->light_indices, &ctx->draws, &ctx->vdata, &ctx->surf_ext, &ctx->blk_surf,
-		&ctx->parts, &ctx->spans, &ctx->drec, &ctx->dbounds, &ctx->counters, &ctx->arena, &ctx->fb, &ctx->tonemap_out, &ctx->tonemap_sums};
+		&ctx->parts, &ctx->spans, &ctx->drec, &ctx->dbounds, &ctx->counters, &ctx->arena, &ctx->fb, &ctx->tonemap_out, &ctx->tonemap_sums,
+		&ctx->bloom0, &ctx->bloom1, &ctx->bloom_lines, &ctx->bloom_avg};
 	for(DevBuf* b : bufs)
 		release(ctx, *b);
 	if(ctx->counters_host)
@@ -1053,6 +1056,129 @@ int sw_tonemap_device(sw_ctx* ctx, const void* hdr_device, uint32_t width, uint3
 		out_colors_sum[2] = sums[2];
 	}
 	return SW_OK;
+}
+
+// compute_gaussian_kernel (postprocessor.rs:1026-1045): four samples per tap, f32, libm exp
+static void bloom_kernel(float sigma, int radius, float out[BLOOM_KERNEL])
+{
+	for(int i = 0; i < BLOOM_KERNEL; ++i)
+		out[i] = 0.0f;
+	const float samples[4] = {-0.375f, -0.125f, 0.125f, 0.375f};
+	const float sqrt_tau = std::sqrt(6.28318530717958647692f);
+	for(int x = -radius; x <= radius; ++x)
+	{
+		float val = 0.0f;
+		for(float sample : samples)
+		{
+			const float coord = float(x) + sample;
+			val += std::exp(-0.5f * (coord / sigma) * (coord / sigma)) / (sigma * sqrt_tau);
+		}
+		out[x + radius] = val / 4.0f;
+	}
+}
+
+int sw_postprocess_device(sw_ctx* ctx, const void* hdr_device, uint32_t width, uint32_t height, float exposure,
+	const float color_modulate[3], const sw_bloom_params* bloom, void* dst32_device, size_t dst_pitch_bytes,
+	float out_average_rgb[3])
+{
+	if(!ctx || !hdr_device || !dst32_device || !color_modulate)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	if(dst_pitch_bytes < size_t(width) * 4 || dst_pitch_bytes % 4)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "bad dst pitch");
+	(void)hipSetDevice(ctx->device);
+	// postprocessor.rs:928-940 clamps the settings
+	const float sigma = bloom ? std::min(std::max(bloom->sigma, 0.0f), 40.0f) : 0.0f;
+	if(!(sigma > 0.0f))
+	{
+		float sums[3] = {0.0f, 0.0f, 0.0f};
+		const int r = sw_tonemap_device(ctx, hdr_device, width, height, exposure, color_modulate, dst32_device, dst_pitch_bytes,
+			out_average_rgb ? sums : nullptr);
+		if(r == SW_OK && out_average_rgb)
+		{
+			// postprocessor.rs:181-184
+			const float inv = 1.0f / float(uint64_t(width) * height);
+			for(int k = 0; k < 3; ++k)
+				out_average_rgb[k] = sums[k] * inv;
+		}
+		return r;
+	}
+	BloomArgs a{};
+	a.scale_log2 = std::min<uint32_t>(std::max<uint32_t>(bloom->scale_log2, 1u), 3u);
+	const float bloom_scale = std::min(std::max(bloom->scale, 1.0f / 16.0f), 0.5f);
+	const uint32_t S = 1u << a.scale_log2;
+	a.hdr = static_cast<const uint64_t*>(hdr_device);
+	a.w = width;
+	a.h = height;
+	a.bw = width / S;
+	a.bh = height / S;
+	if(a.bw == 0 || a.bh == 0)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "frame smaller than one bloom texel");
+	// perform_bloom (postprocessor.rs:728-803): radius and integer weights of both passes
+	const float sigma_c = sigma / float(S);
+	a.radius = std::min(int32_t(std::max(std::ceil(3.0f * sigma_c - 0.5f), 0.0f)), int32_t(BLOOM_MAX_RADIUS));
+	float kernel[BLOOM_KERNEL];
+	bloom_kernel(sigma_c, a.radius, kernel);
+	for(int i = 0; i < BLOOM_KERNEL; ++i)
+	{
+		a.k1[i] = int32_t(kernel[i] * float(1 << 14));
+		a.k2[i] = int32_t(kernel[i] * bloom_scale * float(1 << 14));
+	}
+	// TonemappingFunction::new (postprocessor.rs:1005-1017), lanes B, G, R
+	a.inv_scale[0] = 1.0f / (exposure * color_modulate[2]);
+	a.inv_scale[1] = 1.0f / (exposure * color_modulate[1]);
+	a.inv_scale[2] = 1.0f / (exposure * color_modulate[0]);
+	a.linear = bloom->linear_filter ? 1u : 0u;
+	a.dst = static_cast<uint32_t*>(dst32_device);
+	a.dst_pitch = uint32_t(dst_pitch_bytes / 4);
+	const uint64_t texels = uint64_t(a.bw) * a.bh;
+	int r;
+	if((r = ensure(ctx, ctx->bloom0, texels * 8)) || (r = ensure(ctx, ctx->bloom1, texels * 8)) ||
+		(r = ensure(ctx, ctx->bloom_lines, uint64_t(a.bh) * 16)) || (r = ensure(ctx, ctx->bloom_avg, 16)))
+		return r;
+	a.buf0 = static_cast<uint64_t*>(ctx->bloom0.p);
+	a.buf1 = static_cast<uint64_t*>(ctx->bloom1.p);
+	a.line_sums = static_cast<int32_t*>(ctx->bloom_lines.p);
+	a.average = static_cast<float*>(ctx->bloom_avg.p);
+	{
+		StageTimer t(ctx, STAGE_BLOOM);
+		swk::launch_bloom(ctx->stream, a);
+	}
+	if((r = hip_check(ctx, hipGetLastError(), "bloom launch")))
+		return r;
+	if(out_average_rgb)
+	{
+		float avg[4] = {0, 0, 0, 0};
+		if((r = hip_check(ctx, hipMemcpyAsync(avg, ctx->bloom_avg.p, sizeof(avg), hipMemcpyDeviceToHost, ctx->stream),
+				"average colour")))
+			return r;
+		if((r = hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize")))
+			return r;
+		// lanes B, G, R -> RGB
+		out_average_rgb[0] = avg[2];
+		out_average_rgb[1] = avg[1];
+		out_average_rgb[2] = avg[0];
+	}
+	return SW_OK;
+}
+
+float sw_update_exposure(const sw_exposure_params* p, float current_exposure, const float average_rgb[3], float frame_duration_s)
+{
+	if(!p || !average_rgb)
+		return current_exposure;
+	// synchronize_config (postprocessor.rs:928-940) clamps
+	const float min_exposure = std::min(std::max(p->min_exposure, 1.0f / 65536.0f), 16.0f);
+	const float max_exposure = std::min(std::max(p->max_exposure, 1.0f), 65536.0f);
+	const float speed = std::min(std::max(p->exposure_update_speed, 0.5f), 16.0f);
+	const float base = std::min(std::max(p->base_brightness, 0.5f), 2.0f);
+	const float zero_level = std::min(std::max(p->zero_level_brightness, 1.0f / 16.0f), 16.0f);
+	const float power = std::min(std::max(p->brightness_scale_power, 0.01f), 0.5f);
+	// get_color_brightness (:1047-1051), update_exposure (:206-223)
+	const float brightness =
+		std::min(std::max((average_rgb[0] + average_rgb[1] + average_rgb[2]) * 0.33f, 1.0f / 1024.0f), 65536.0f) / 255.0f;
+	const float target = std::pow(brightness / zero_level, power) * 0.5f * base / brightness;
+	const float clamped = std::min(std::max(target, min_exposure), max_exposure);
+	const float mix = std::exp(-speed * frame_duration_s);
+	return std::exp(std::log(current_exposure) * mix + std::log(clamped) * (1.0f - mix));
 }
 
 void* sw_frame_device_framebuffer(sw_ctx* ctx) { return ctx ? ctx->fb.p : nullptr; }

@@ -87,6 +87,16 @@ class Batch(C.Structure):
     ]
 
 
+class BloomParams(C.Structure):
+    _fields_ = [("sigma", C.c_float), ("scale_log2", C.c_uint32), ("scale", C.c_float), ("linear_filter", C.c_uint32)]
+
+
+class ExposureParams(C.Structure):
+    _fields_ = [("exposure_update_speed", C.c_float), ("base_brightness", C.c_float),
+                ("zero_level_brightness", C.c_float), ("brightness_scale_power", C.c_float),
+                ("min_exposure", C.c_float), ("max_exposure", C.c_float)]
+
+
 class Stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("surface_texels", "lightmap_elements", "emissive_texels", "polygon_pixels",
                                           "triangle_pixels", "blend_pixels", "spans", "parts", "reruns")]
@@ -199,6 +209,10 @@ GPU_SYMBOLS = {
     "sw_render_batch_graph": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Batch), C.c_void_p]),
     "sw_tonemap_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float,
                                     C.POINTER(C.c_float), C.c_void_p, C.c_size_t, C.POINTER(C.c_float)]),
+    "sw_postprocess_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float,
+                                        C.POINTER(C.c_float), C.c_void_p, C.c_void_p, C.c_size_t,
+                                        C.POINTER(C.c_float)]),
+    "sw_update_exposure": (C.c_float, [C.c_void_p, C.c_float, C.POINTER(C.c_float), C.c_float]),
     "sw_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "sw_last_reruns": (C.c_uint32, [C.c_void_p]),
     "sw_set_scalar_spans": (C.c_int, [C.c_void_p, C.c_uint32]),
@@ -329,6 +343,28 @@ class Context:
         self.check(self.lib.sw_tonemap(self.h, float(exposure), mod, _ptr(out), w * 4, 0, sums), "sw_tonemap")
         return fb, out, np.array(sums[:], dtype=np.float32)
 
+    def postprocess(self, hdr, exposure, modulate=(1.0, 1.0, 1.0), bloom=None):
+        """sw_postprocess_device on a host Color64 image (copied to the device and back). bloom:
+        None or (sigma, scale_log2, scale, linear). Returns (ldr, average rgb)."""
+        hdr = np.ascontiguousarray(hdr, dtype=np.uint64)
+        h, w = hdr.shape
+        src = self.to_device(hdr)
+        dst = self.alloc(w * h * 4)
+        try:
+            avg = (C.c_float * 3)()
+            mod = (C.c_float * 3)(*modulate)
+            bp = None
+            if bloom is not None:
+                sigma, sl, scale, linear = bloom
+                bp = C.byref(BloomParams(float(sigma), int(sl), float(scale), 1 if linear else 0))
+            self.check(self.lib.sw_postprocess_device(self.h, src.ptr, w, h, float(exposure), mod, bp, dst.ptr, w * 4, avg),
+                       "sw_postprocess_device")
+            out = dst.read(np.uint32, w * h).reshape(h, w)
+        finally:
+            src.free()
+            dst.free()
+        return out, np.array(avg[:], dtype=np.float32)
+
     # device memory -----------------------------------------------------------------------
     def alloc(self, nbytes):
         p = C.c_void_p()
@@ -349,7 +385,7 @@ class Context:
     def profile_enable(self, on=True):
         self.check(self.lib.sw_profile_enable(self.h, 1 if on else 0), "sw_profile_enable")
 
-    PROFILE_STAGES = ("setup_draws", "build_surfaces", "raster_tiles", "tonemap", "setup_and_surfaces")
+    PROFILE_STAGES = ("setup_draws", "build_surfaces", "raster_tiles", "tonemap", "setup_and_surfaces", "bloom")
 
     def profile_read(self, reset=True):
         ms = (C.c_double * len(self.PROFILE_STAGES))()
@@ -626,3 +662,10 @@ class Server:
         s = ServerStats()
         self.check(self.lib.sw_server_get_stats(self.h, C.byref(s)), "sw_server_get_stats")
         return s.as_dict()
+
+
+def update_exposure(params: ExposureParams, current, average_rgb, frame_duration_s):
+    """sw_update_exposure (Postprocessor::update_exposure): the next frame's exposure."""
+    lib = load_gpu()
+    avg = (C.c_float * 3)(*[float(v) for v in average_rgb])
+    return float(lib.sw_update_exposure(C.byref(params), float(current), avg, float(frame_duration_s)))
