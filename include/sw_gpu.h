@@ -371,7 +371,9 @@ typedef struct sw_bloom_params
  * average), blurred by a separable integer Gaussian and added, upsampled, to every pixel before the
  * tonemap (:258-956); without, the plain tonemap. out_average_rgb: the average colour the reference
  * hands to update_exposure (the downscaled buffer's with bloom, the frame's without). The frame
- * must be at least one bloom texel in each direction. */
+ * must be at least one bloom texel in each direction and at most 65535 rows tall; dst_pitch_bytes
+ * a multiple of 4, at least width * 4. Synchronous when out_average_rgb is given (it waits for the
+ * average), else queued on the context's stream. */
 int sw_postprocess_device(
 	sw_ctx* ctx, const void* hdr_device, uint32_t width, uint32_t height, float exposure, const float color_modulate[3],
 	const sw_bloom_params* bloom, void* dst32_device, size_t dst_pitch_bytes, float out_average_rgb[3]);

@@ -46,6 +46,8 @@ enum
 // The live counter block keeps the same flags for the later kernels of the sequence.
 #define HOST_ERR_WORDS 4
 #define HOST_ERR_OFFSET COUNTER_COUNT // after the result block in the host memory
+#define HOST_POST_WORDS 8 // postprocess results: tonemap colour sums [0, 3), bloom average [4, 8)
+#define HOST_POST_OFFSET (HOST_ERR_OFFSET + HOST_ERR_WORDS)
 
 // Surface texels are built by blocks of SURF_BLOCK threads over the flattened arena, each block
 // surf_block_texels() consecutive texels: up to SURF_MAX_TEXELS_PER_THREAD per thread (the block's
@@ -202,8 +204,8 @@ struct BloomArgs
 	int32_t radius; // blur radius (0: no blur)
 	uint64_t* buf0; // bloom buffer (downscaled, then blurred)
 	uint64_t* buf1; // horizontal pass output
-	int32_t* line_sums; // 4 per bloom row: the row's sum of averages, lanes B, G, R, A
-	float* average; // 4 floats: the average colour, lanes B, G, R, A
+	int32_t* line_sums; // 4 per bloom row: the row's sum of averages, lanes B, G, R, A (16-byte aligned)
+	float* average; // 4 floats: the average colour, lanes B, G, R, A (host memory)
 	int32_t k1[BLOOM_KERNEL], k2[BLOOM_KERNEL]; // integer weights of the two passes (scale 2^14)
 	float inv_scale[3]; // tonemap 1 / (exposure * modulate), lanes B, G, R
 	uint32_t linear; // bilinear upsampling of the bloom buffer (else nearest)
@@ -231,9 +233,11 @@ void launch_raster_tiles(hipStream_t st, uint32_t color_bits, uint32_t tile_rows
 	const void* skybox, void* fb, uint32_t total_tiles, const uint32_t* counters);
 // Kernel 4 (one block): per-draw statistics -> result block (host-mapped), live block cleared.
 void launch_finish(hipStream_t st, const DrawRec* drec, uint32_t nd, uint32_t* counters, uint32_t* result);
-// Bloom: downscale + average colour, two blur passes, tonemap with the upsampled bloom.
-void launch_bloom(hipStream_t st, const BloomArgs& a);
-// Tonemap; `sums` holds 4 + 3 * 1024 floats (totals first, then per-block partials).
+// Bloom: downscale (+ the average colour when `average`), two blur passes, tonemap with the
+// upsampled bloom. Grids take a frame row per block row: height <= 65535.
+void launch_bloom(hipStream_t st, const BloomArgs& a, bool average);
+// Tonemap; `partials` holds 3 * 2048 floats (per-block colour sums); `totals` (3 floats, may be
+// host memory, null: not wanted) receives their sum.
 void launch_tonemap(hipStream_t st, const uint64_t* hdr, uint32_t w, uint32_t h, float isr, float isg, float isb,
-	uint32_t* dst, uint32_t dst_pitch, float* sums);
+	uint32_t* dst, uint32_t dst_pitch, float* partials, float* totals);
 } // namespace swk

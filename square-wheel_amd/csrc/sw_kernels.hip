@@ -2066,13 +2066,14 @@ void k_raster_tiles(MapPtrs m, const DrawBounds* __restrict__ bounds,
 // =============================================================================================
 
 // TonemappingFunction::do_it (postprocessor.rs:1019-1023): c / fma(c, 1/255, 1/(exposure*modulate))
-// per lane; the alpha lane's constants are 0, so it is c.a / 0 (255 after the pack).
+// per lane. The alpha lane's constants are 0, so it is c.a / +0: +inf for c.a > 0 and NaN for
+// c.a == 0, and the pack's min(x, 255) (minps: the second operand when either is NaN) makes both
+// 255 — the alpha byte is the constant 0xFF.
 SW_DEV uint32_t tonemap_cv(CV c, const float* inv_scale)
 {
 	const float inv_255 = 1.0f / 255.0f;
-	const CV mapped = CV{c.b / mul_add(c.b, inv_255, inv_scale[0]), c.g / mul_add(c.g, inv_255, inv_scale[1]),
-		c.r / mul_add(c.r, inv_255, inv_scale[2]), c.a / mul_add(c.a, 0.0f, 0.0f)};
-	return Color32T::from_cv(mapped);
+	return lane_to_u8(c.b / mul_add(c.b, inv_255, inv_scale[0])) | (lane_to_u8(c.g / mul_add(c.g, inv_255, inv_scale[1])) << 8) |
+		(lane_to_u8(c.r / mul_add(c.r, inv_255, inv_scale[2])) << 16) | 0xFF000000u;
 }
 
 SW_DEV CV cv_add(CV a, CV b) { return CV{a.b + b.b, a.g + b.g, a.r + b.r, a.a + b.a}; }
@@ -2083,23 +2084,25 @@ SW_DEV CI ci_shr(CI a, int k) { return CI{a.b >> k, a.g >> k, a.r >> k, a.a >> k
 // downscale_hdr_buffer (:805-853): one block per bloom row. Each bloom texel is the integer average
 // of its scale x scale source pixels ((sum * (256 / scale^2)) >> 8, packed with unsigned
 // saturation); the block also sums the row's averages (integers, so any order is exact).
+template <uint32_t S>
 __global__ __launch_bounds__(256) void k_bloom_downscale(BloomArgs a)
 {
 	const uint32_t dy = blockIdx.x;
-	const uint32_t S = 1u << a.scale_log2;
 	const int32_t scaler = (1 << 8) / int32_t(S * S);
+	const uint64_t* __restrict__ src = a.hdr + uint64_t(dy * S) * a.w;
+	uint64_t* __restrict__ dst = a.buf0 + uint64_t(dy) * a.bw;
 	CI line = CI{0, 0, 0, 0};
+#pragma unroll 2
 	for(uint32_t dx = threadIdx.x; dx < a.bw; dx += blockDim.x)
 	{
 		CI sum = CI{0, 0, 0, 0};
+#pragma unroll
 		for(uint32_t y = 0; y < S; ++y)
-		{
-			const uint64_t* row = a.hdr + uint64_t(dy * S + y) * a.w + dx * S;
+#pragma unroll
 			for(uint32_t x = 0; x < S; ++x)
-				sum = ci_add(sum, Color64T::to_ci(row[x]));
-		}
+				sum = ci_add(sum, Color64T::to_ci(src[uint64_t(y) * a.w + dx * S + x]));
 		const CI average = ci_shr(ci_mul(sum, scaler), 8);
-		a.buf0[uint64_t(dy) * a.bw + dx] = Color64T::from_ci(average);
+		dst[dx] = Color64T::from_ci(average);
 		line = ci_add(line, average);
 	}
 	__shared__ int32_t red[4][4];
@@ -2123,62 +2126,109 @@ __global__ __launch_bounds__(256) void k_bloom_downscale(BloomArgs a)
 }
 
 // The average colour (:838-852): the rows' integer sums converted and added in row order, then
-// scaled by 1 / (bloom texels) — sequential, like the reference, so the float sum is the same.
-__global__ void k_bloom_average(BloomArgs a)
+// scaled by 1 / (bloom texels). Float addition is not associative, so each lane is one sequential
+// sum, like the reference's; the block converts chunks of rows into LDS in parallel (one 16-byte
+// load per row) and four threads (one per lane) add them up in order. `a.average` is host memory.
+#define BLOOM_AVG_CHUNK 512u
+SW_DEV void bloom_average_block(const BloomArgs& a)
 {
-	if(threadIdx.x != 0)
-		return;
-	CV sum = CV{0.0f, 0.0f, 0.0f, 0.0f};
-	for(uint32_t y = 0; y < a.bh; ++y)
+	__shared__ float4 rows[BLOOM_AVG_CHUNK];
+	float sum = 0.0f;
+	const uint32_t lane = threadIdx.x;
+	const int4* line_sums = reinterpret_cast<const int4*>(a.line_sums);
+	for(uint32_t y0 = 0; y0 < a.bh; y0 += BLOOM_AVG_CHUNK)
 	{
-		const int32_t* l = a.line_sums + y * 4;
-		sum = cv_add(sum, CV{float(l[0]), float(l[1]), float(l[2]), float(l[3])});
+		const uint32_t n = min(BLOOM_AVG_CHUNK, a.bh - y0);
+		for(uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+		{
+			const int4 l = line_sums[y0 + i];
+			rows[i] = make_float4(float(l.x), float(l.y), float(l.z), float(l.w));
+		}
+		__syncthreads();
+		if(lane < 4)
+		{
+			const float* r = reinterpret_cast<const float*>(rows) + lane;
+#pragma unroll 8
+			for(uint32_t y = 0; y < n; ++y)
+				sum += r[y * 4];
+		}
+		__syncthreads();
 	}
-	const CV avg = cv_scalar_mul(sum, 1.0f / float(a.bw * a.bh));
-	a.average[0] = avg.b;
-	a.average[1] = avg.g;
-	a.average[2] = avg.r;
-	a.average[3] = avg.a;
+	if(lane < 4)
+		a.average[lane] = sum * (1.0f / float(a.bw * a.bh));
 }
+
+__global__ __launch_bounds__(256) void k_bloom_average(BloomArgs a) { bloom_average_block(a); }
 
 // perform_blur_impl (:855-937), one pass: pass 0 horizontal (buf0 -> buf1, weights k1), pass 1
 // vertical (buf1 -> buf0, weights k2 = k1 scaled by bloom_scale); edge texels repeat (clamped
 // coordinates); integer weights at scale 2^14, arithmetic shift, unsigned-saturating pack.
-__global__ __launch_bounds__(256) void k_bloom_blur(BloomArgs a, uint32_t pass)
+// Grid: x over the row in blocks of BLUR_PX * 256 texels (BLUR_PX per thread, their taps loaded
+// together), y = bloom row. with_average: block (0, 0), dispatched first, also computes the
+// average colour (pass 0 only; it reads the line sums the downscale left).
+#define BLUR_PX 2
+__global__ __launch_bounds__(256) void k_bloom_blur(BloomArgs a, uint32_t pass, uint32_t with_average)
 {
-	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-	if(i >= uint64_t(a.bw) * a.bh)
-		return;
-	const uint32_t y = uint32_t(i / a.bw), x = uint32_t(i - uint64_t(y) * a.bw);
+	if(with_average && blockIdx.x == 0 && blockIdx.y == 0)
+		bloom_average_block(a);
+	const uint32_t y = blockIdx.y;
 	const int32_t R = a.radius;
 	const int32_t* k = pass == 0 ? a.k1 : a.k2;
-	const uint64_t* src = pass == 0 ? a.buf0 : a.buf1;
-	uint64_t* dst = pass == 0 ? a.buf1 : a.buf0;
+	const uint64_t* __restrict__ src = pass == 0 ? a.buf0 : a.buf1;
+	uint64_t* __restrict__ dst = pass == 0 ? a.buf1 : a.buf0;
 	const int32_t last = pass == 0 ? int32_t(a.bw) - 1 : int32_t(a.bh) - 1;
-	const int32_t c = pass == 0 ? int32_t(x) : int32_t(y);
-	auto at = [&](int32_t p) { return pass == 0 ? src[uint64_t(y) * a.bw + uint32_t(p)] : src[uint64_t(p) * a.bw + x]; };
-	CI sum = ci_mul(Color64T::to_ci(at(c)), k[R]);
+	CI sum[BLUR_PX];
+	uint32_t xs[BLUR_PX];
+#pragma unroll
+	for(int j = 0; j < BLUR_PX; ++j)
+	{
+		// out-of-row threads work on the row's last texel and store nothing
+		xs[j] = min(blockIdx.x * (256u * BLUR_PX) + uint32_t(j) * 256u + threadIdx.x, a.bw - 1);
+		const int32_t c = pass == 0 ? int32_t(xs[j]) : int32_t(y);
+		const uint64_t v = pass == 0 ? src[uint64_t(y) * a.bw + uint32_t(c)] : src[uint64_t(c) * a.bw + xs[j]];
+		sum[j] = ci_mul(Color64T::to_ci(v), k[R]);
+	}
 	for(int32_t d = 1; d <= R; ++d)
 	{
-		const int32_t p0 = imax32(c - d, 0), p1 = imin32(c + d, last);
-		const CI two = ci_add(Color64T::to_ci(at(p0)), Color64T::to_ci(at(p1)));
-		sum = ci_add(sum, ci_mul(two, k[d + R]));
+		const int32_t w = k[d + R];
+#pragma unroll
+		for(int j = 0; j < BLUR_PX; ++j)
+		{
+			const int32_t c = pass == 0 ? int32_t(xs[j]) : int32_t(y);
+			const int32_t p0 = imax32(c - d, 0), p1 = imin32(c + d, last);
+			uint64_t v0, v1;
+			if(pass == 0)
+			{
+				v0 = src[uint64_t(y) * a.bw + uint32_t(p0)];
+				v1 = src[uint64_t(y) * a.bw + uint32_t(p1)];
+			}
+			else
+			{
+				v0 = src[uint64_t(p0) * a.bw + xs[j]];
+				v1 = src[uint64_t(p1) * a.bw + xs[j]];
+			}
+			sum[j] = ci_add(sum[j], ci_mul(ci_add(Color64T::to_ci(v0), Color64T::to_ci(v1)), w));
+		}
 	}
-	dst[i] = Color64T::from_ci(ci_shr(sum, 14));
+#pragma unroll
+	for(int j = 0; j < BLUR_PX; ++j)
+	{
+		const uint32_t x = blockIdx.x * (256u * BLUR_PX) + uint32_t(j) * 256u + threadIdx.x;
+		if(x < a.bw)
+			dst[uint64_t(y) * a.bw + x] = Color64T::from_ci(ci_shr(sum[j], 14));
+	}
 }
 
 // perform_tonemapping_with_bloom_{nearest,linear} (:346-726), one thread per pixel: the bloom value
 // of the pixel's region (block, border or corner of the reference's loops), added to the pixel,
 // tonemapped.
+// Grid: x over the row in blocks of TM_PX * 256 pixels (TM_PX per thread, their frame loads issued
+// together), y = frame row.
+#define TM_PX 4
 template <bool LINEAR>
-__global__ __launch_bounds__(256) void k_bloom_tonemap(BloomArgs a)
+SW_DEV uint32_t bloom_pixel(const BloomArgs& a, uint32_t x, uint32_t y, CV c)
 {
-	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-	if(i >= uint64_t(a.w) * a.h)
-		return;
-	const uint32_t y = uint32_t(i / a.w), x = uint32_t(i - uint64_t(y) * a.w);
 	const uint32_t sl = a.scale_log2, S = 1u << sl;
-	const CV c = cv_from_color64(a.hdr[i]);
 	auto bl = [&](uint32_t bx, uint32_t by) { return cv_from_color64(a.buf0[uint64_t(by) * a.bw + bx]); };
 	CV sum;
 	if(!LINEAR)
@@ -2219,33 +2269,68 @@ __global__ __launch_bounds__(256) void k_bloom_tonemap(BloomArgs a)
 			}
 		}
 	}
-	a.dst[uint64_t(y) * a.dst_pitch + x] = tonemap_cv(sum, a.inv_scale);
+	return tonemap_cv(sum, a.inv_scale);
+}
+
+template <bool LINEAR>
+__global__ __launch_bounds__(256) void k_bloom_tonemap(BloomArgs a)
+{
+	const uint32_t y = blockIdx.y;
+	const uint32_t x0 = blockIdx.x * (256u * TM_PX) + threadIdx.x;
+	const uint64_t* __restrict__ src = a.hdr + uint64_t(y) * a.w;
+	uint64_t v[TM_PX];
+#pragma unroll
+	for(int j = 0; j < TM_PX; ++j)
+		v[j] = src[min(x0 + uint32_t(j) * 256u, a.w - 1)];
+#pragma unroll
+	for(int j = 0; j < TM_PX; ++j)
+	{
+		const uint32_t x = x0 + uint32_t(j) * 256u;
+		if(x < a.w)
+			a.dst[uint64_t(y) * a.dst_pitch + x] = bloom_pixel<LINEAR>(a, x, y, cv_from_color64(v[j]));
+	}
 }
 
 // =============================================================================================
 // Tonemap (postprocessor.rs:237-253, :1019-1023)
 // =============================================================================================
-// Tonemap blocks stride over the frame; each block leaves the partial colour sums of its pixels
-// (no atomics on three shared addresses), k_tonemap_sums adds them up.
-#define TONEMAP_BLOCKS 1024u
+// Tonemap blocks walk the frame's rows (block b: rows b, b + grid, ...; the threads over the row);
+// each block leaves the partial colour sums of its pixels (no atomics on three shared addresses),
+// k_tonemap_sums adds them up.
+#define TONEMAP_BLOCKS 2048u
 
 __global__ __launch_bounds__(256) void k_tonemap(const uint64_t* __restrict__ hdr, uint32_t width, uint32_t height,
 	float inv_scale_r, float inv_scale_g, float inv_scale_b, uint32_t* __restrict__ dst, uint32_t dst_pitch,
 	float* __restrict__ partials)
 {
 	__shared__ float red[3][4];
-	const uint64_t n = uint64_t(width) * height;
-	const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
 	float sb = 0.0f, sg = 0.0f, sr = 0.0f;
 	const float inv_scale[3] = {inv_scale_b, inv_scale_g, inv_scale_r};
-	for(uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+	for(uint32_t y = blockIdx.x; y < height; y += gridDim.x)
 	{
-		const uint32_t y = uint32_t(i / width), x = uint32_t(i - uint64_t(y) * width);
-		const CV c = cv_from_color64(hdr[i]);
-		sb += c.b;
-		sg += c.g;
-		sr += c.r;
-		dst[uint64_t(y) * dst_pitch + x] = tonemap_cv(c, inv_scale);
+		const uint64_t* __restrict__ src = hdr + uint64_t(y) * width;
+		uint32_t* __restrict__ out = dst + uint64_t(y) * dst_pitch;
+		for(uint32_t x0 = threadIdx.x; x0 < width; x0 += 256u * TM_PX)
+		{
+			uint64_t v[TM_PX];
+#pragma unroll
+			for(int j = 0; j < TM_PX; ++j)
+			{
+				const uint32_t x = x0 + uint32_t(j) * 256u;
+				v[j] = x < width ? src[x] : 0; // zero: adds nothing to the sums
+			}
+#pragma unroll
+			for(int j = 0; j < TM_PX; ++j)
+			{
+				const uint32_t x = x0 + uint32_t(j) * 256u;
+				const CV c = cv_from_color64(v[j]);
+				sb += c.b;
+				sg += c.g;
+				sr += c.r;
+				if(x < width)
+					out[x] = tonemap_cv(c, inv_scale);
+			}
+		}
 	}
 	for(int off = 32; off > 0; off >>= 1)
 	{
@@ -2408,31 +2493,36 @@ void launch_finish(hipStream_t st, const DrawRec* drec, uint32_t nd, uint32_t* c
 	hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, st, drec, nd, counters, result);
 }
 
-void launch_bloom(hipStream_t st, const BloomArgs& a)
+void launch_bloom(hipStream_t st, const BloomArgs& a, bool average)
 {
-	hipLaunchKernelGGL(k_bloom_downscale, dim3(a.bh), dim3(256), 0, st, a);
-	hipLaunchKernelGGL(k_bloom_average, dim3(1), dim3(64), 0, st, a);
-	const uint32_t blur_blocks = uint32_t((uint64_t(a.bw) * a.bh + 255) / 256);
+	switch(a.scale_log2)
+	{
+	case 1: hipLaunchKernelGGL(k_bloom_downscale<2>, dim3(a.bh), dim3(256), 0, st, a); break;
+	case 2: hipLaunchKernelGGL(k_bloom_downscale<4>, dim3(a.bh), dim3(256), 0, st, a); break;
+	default: hipLaunchKernelGGL(k_bloom_downscale<8>, dim3(a.bh), dim3(256), 0, st, a); break;
+	}
 	if(a.radius > 0)
 	{
-		hipLaunchKernelGGL(k_bloom_blur, dim3(blur_blocks), dim3(256), 0, st, a, 0u);
-		hipLaunchKernelGGL(k_bloom_blur, dim3(blur_blocks), dim3(256), 0, st, a, 1u);
+		const dim3 blur_grid((a.bw + 256 * BLUR_PX - 1) / (256 * BLUR_PX), a.bh);
+		hipLaunchKernelGGL(k_bloom_blur, blur_grid, dim3(256), 0, st, a, 0u, average ? 1u : 0u);
+		hipLaunchKernelGGL(k_bloom_blur, blur_grid, dim3(256), 0, st, a, 1u, 0u);
 	}
-	const uint32_t blocks = uint32_t((uint64_t(a.w) * a.h + 255) / 256);
+	else if(average)
+		hipLaunchKernelGGL(k_bloom_average, dim3(1), dim3(256), 0, st, a);
+	const dim3 grid((a.w + 256 * TM_PX - 1) / (256 * TM_PX), a.h);
 	if(a.linear)
-		hipLaunchKernelGGL(k_bloom_tonemap<true>, dim3(blocks), dim3(256), 0, st, a);
+		hipLaunchKernelGGL(k_bloom_tonemap<true>, grid, dim3(256), 0, st, a);
 	else
-		hipLaunchKernelGGL(k_bloom_tonemap<false>, dim3(blocks), dim3(256), 0, st, a);
+		hipLaunchKernelGGL(k_bloom_tonemap<false>, grid, dim3(256), 0, st, a);
 }
 
 void launch_tonemap(hipStream_t st, const uint64_t* hdr, uint32_t w, uint32_t h, float isr, float isg, float isb,
-	uint32_t* dst, uint32_t dst_pitch, float* sums)
+	uint32_t* dst, uint32_t dst_pitch, float* partials, float* totals)
 {
-	// sums[0..2]: totals; sums[4 ..]: 3 partials per block
-	const uint64_t n = uint64_t(w) * h;
-	const uint32_t blocks = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, TONEMAP_BLOCKS)));
-	hipLaunchKernelGGL(k_tonemap, dim3(blocks), dim3(256), 0, st, hdr, w, h, isr, isg, isb, dst, dst_pitch, sums + 4);
-	hipLaunchKernelGGL(k_tonemap_sums, dim3(1), dim3(256), 0, st, sums + 4, blocks, sums);
+	const uint32_t blocks = std::max(1u, std::min(h, TONEMAP_BLOCKS));
+	hipLaunchKernelGGL(k_tonemap, dim3(blocks), dim3(256), 0, st, hdr, w, h, isr, isg, isb, dst, dst_pitch, partials);
+	if(totals)
+		hipLaunchKernelGGL(k_tonemap_sums, dim3(1), dim3(256), 0, st, partials, blocks, totals);
 }
 
 } // namespace swk

@@ -66,7 +66,7 @@ struct sw_ctx
 
 	// command buffers / work buffers
 	DevBuf views, surfaces, lights, light_indices, draws, vdata, surf_ext, blk_surf, parts, spans, drec, dbounds, counters, arena, fb,
-		tonemap_out, tonemap_sums, bloom0, bloom1, bloom_lines, bloom_avg;
+		tonemap_out, tonemap_sums, bloom0, bloom1, bloom_lines;
 	uint32_t tile_rows = 8; // raster tile height (SW_TILE_ROWS_*), set by sw_set_tile_rows / the autotuner
 	bool scalar_spans = true; // raster span records through the scalar unit (chosen by the autotuner)
 	uint32_t* counters_host = nullptr; // pinned, device-mapped: result block (k_finish), then the error words
@@ -748,14 +748,14 @@ int sw_ctx_create(int device, sw_ctx** out_ctx)
 	if(ensure(ctx, ctx->counters, sizeof(uint32_t) * COUNTER_TOTAL) ||
 		hipMemsetAsync(ctx->counters.p, 0, sizeof(uint32_t) * COUNTER_TOTAL, ctx->stream) != hipSuccess ||
 		hipStreamSynchronize(ctx->stream) != hipSuccess ||
-		hipHostMalloc(reinterpret_cast<void**>(&ctx->counters_host), sizeof(uint32_t) * (COUNTER_COUNT + HOST_ERR_WORDS),
+		hipHostMalloc(reinterpret_cast<void**>(&ctx->counters_host), sizeof(uint32_t) * (COUNTER_COUNT + HOST_ERR_WORDS + HOST_POST_WORDS),
 			hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
 		hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->counters_host_dev), ctx->counters_host, 0) != hipSuccess)
 	{
 		sw_ctx_destroy(ctx);
 		return SW_ERR_DEVICE;
 	}
-	std::memset(ctx->counters_host, 0, sizeof(uint32_t) * (COUNTER_COUNT + HOST_ERR_WORDS));
+	std::memset(ctx->counters_host, 0, sizeof(uint32_t) * (COUNTER_COUNT + HOST_ERR_WORDS + HOST_POST_WORDS));
 	*out_ctx = ctx;
 	return SW_OK;
 }
@@ -773,7 +773,7 @@ void sw_ctx_destroy(sw_ctx* ctx)
 		(void)hipGraphDestroy(ctx->graph);
 	DevBuf* bufs[] = {&ctx->views, &ctx->surfaces, &ctx->lights, &ctx->light_indices, &ctx->draws, &ctx->vdata, &ctx->surf_ext, &ctx->blk_surf,
 		&ctx->parts, &ctx->spans, &ctx->drec, &ctx->dbounds, &ctx->counters, &ctx->arena, &ctx->fb, &ctx->tonemap_out, &ctx->tonemap_sums,
-		&ctx->bloom0, &ctx->bloom1, &ctx->bloom_lines, &ctx->bloom_avg};
+		&ctx->bloom0, &ctx->bloom1, &ctx->bloom_lines};
 	for(DevBuf* b : bufs)
 		release(ctx, *b);
 	if(ctx->counters_host)
@@ -1028,10 +1028,14 @@ int sw_tonemap_device(sw_ctx* ctx, const void* hdr_device, uint32_t width, uint3
 {
 	if(!ctx || !hdr_device || !dst32_device || !color_modulate)
 		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	if(dst_pitch_bytes < size_t(width) * 4 || dst_pitch_bytes % 4)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "bad dst pitch");
 	(void)hipSetDevice(ctx->device);
 	int r;
-	if((r = ensure(ctx, ctx->tonemap_sums, sizeof(float) * (4 + 3 * 1024))))
+	if((r = ensure(ctx, ctx->tonemap_sums, sizeof(float) * 3 * 2048)))
 		return r;
+	float* host_sums = reinterpret_cast<float*>(ctx->counters_host + HOST_POST_OFFSET);
+	float* host_sums_dev = reinterpret_cast<float*>(ctx->counters_host_dev + HOST_POST_OFFSET);
 	// TonemappingFunction::new (postprocessor.rs:1005-1017)
 	const float isr = 1.0f / (exposure * color_modulate[0]);
 	const float isg = 1.0f / (exposure * color_modulate[1]);
@@ -1039,21 +1043,18 @@ int sw_tonemap_device(sw_ctx* ctx, const void* hdr_device, uint32_t width, uint3
 	{
 		StageTimer t(ctx, STAGE_TONEMAP);
 		swk::launch_tonemap(ctx->stream, static_cast<const uint64_t*>(hdr_device), width, height, isr, isg, isb,
-			static_cast<uint32_t*>(dst32_device), uint32_t(dst_pitch_bytes / 4), static_cast<float*>(ctx->tonemap_sums.p));
+			static_cast<uint32_t*>(dst32_device), uint32_t(dst_pitch_bytes / 4), static_cast<float*>(ctx->tonemap_sums.p),
+			out_colors_sum ? host_sums_dev : nullptr);
 	}
 	if((r = hip_check(ctx, hipGetLastError(), "tonemap launch")))
 		return r;
 	if(out_colors_sum)
 	{
-		float sums[4] = {0, 0, 0, 0};
-		if((r = hip_check(ctx,
-				hipMemcpyAsync(sums, ctx->tonemap_sums.p, sizeof(float) * 3, hipMemcpyDeviceToHost, ctx->stream), "sums")))
-			return r;
+		// the sums kernel stores the totals straight into the pinned host words
 		if((r = hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize")))
 			return r;
-		out_colors_sum[0] = sums[0];
-		out_colors_sum[1] = sums[1];
-		out_colors_sum[2] = sums[2];
+		for(int k = 0; k < 3; ++k)
+			out_colors_sum[k] = static_cast<volatile const float*>(host_sums)[k];
 	}
 	return SW_OK;
 }
@@ -1085,6 +1086,8 @@ int sw_postprocess_device(sw_ctx* ctx, const void* hdr_device, uint32_t width, u
 		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
 	if(dst_pitch_bytes < size_t(width) * 4 || dst_pitch_bytes % 4)
 		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "bad dst pitch");
+	if(height > 65535)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "frame taller than 65535 rows");
 	(void)hipSetDevice(ctx->device);
 	// postprocessor.rs:928-940 clamps the settings
 	const float sigma = bloom ? std::min(std::max(bloom->sigma, 0.0f), 40.0f) : 0.0f;
@@ -1133,26 +1136,24 @@ int sw_postprocess_device(sw_ctx* ctx, const void* hdr_device, uint32_t width, u
 	const uint64_t texels = uint64_t(a.bw) * a.bh;
 	int r;
 	if((r = ensure(ctx, ctx->bloom0, texels * 8)) || (r = ensure(ctx, ctx->bloom1, texels * 8)) ||
-		(r = ensure(ctx, ctx->bloom_lines, uint64_t(a.bh) * 16)) || (r = ensure(ctx, ctx->bloom_avg, 16)))
+		(r = ensure(ctx, ctx->bloom_lines, uint64_t(a.bh) * 16)))
 		return r;
 	a.buf0 = static_cast<uint64_t*>(ctx->bloom0.p);
 	a.buf1 = static_cast<uint64_t*>(ctx->bloom1.p);
 	a.line_sums = static_cast<int32_t*>(ctx->bloom_lines.p);
-	a.average = static_cast<float*>(ctx->bloom_avg.p);
+	a.average = reinterpret_cast<float*>(ctx->counters_host_dev + HOST_POST_OFFSET + 4);
 	{
 		StageTimer t(ctx, STAGE_BLOOM);
-		swk::launch_bloom(ctx->stream, a);
+		swk::launch_bloom(ctx->stream, a, out_average_rgb != nullptr);
 	}
 	if((r = hip_check(ctx, hipGetLastError(), "bloom launch")))
 		return r;
 	if(out_average_rgb)
 	{
-		float avg[4] = {0, 0, 0, 0};
-		if((r = hip_check(ctx, hipMemcpyAsync(avg, ctx->bloom_avg.p, sizeof(avg), hipMemcpyDeviceToHost, ctx->stream),
-				"average colour")))
-			return r;
+		// the average kernel stores it straight into the pinned host words
 		if((r = hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize")))
 			return r;
+		volatile const float* avg = reinterpret_cast<volatile const float*>(ctx->counters_host + HOST_POST_OFFSET + 4);
 		// lanes B, G, R -> RGB
 		out_average_rgb[0] = avg[2];
 		out_average_rgb[1] = avg[1];
