@@ -191,6 +191,59 @@ def hdr_leg(ctx, args):
     return out
 
 
+def models_leg(ctx, args):
+    """Triangle-model preparation (sw_prepare_models): 64 instances of a 1280-triangle, 642-vertex
+    vertex-animated mesh (8 frames), meshes resident in HBM, outputs in HBM; the kernel's duration
+    from HIP events, the oracle's single-thread CPU time on the same instances beside it."""
+    import ctypes as C
+    import sw_model_scenes as MS
+    rng = np.random.default_rng(args.seed + 31)
+    meshes = [MS.vertex_animated_mesh(rng, subdiv=3, frames=8)]
+    inst = np.ascontiguousarray(MS.instances(rng, meshes, 64))
+    arr, bufs = ctx.upload_meshes(meshes)
+    nv = len(inst) * meshes[0].num_vertices
+    nt = len(inst) * len(meshes[0].triangles)
+    vbuf, tbuf, cbuf = ctx.alloc(nv * 32), ctx.alloc(nt * 6), ctx.alloc(len(inst) * 4)
+    lib = ctx.lib
+
+    def call():
+        ctx.check(lib.sw_prepare_models(ctx.h, arr, len(meshes), C.c_void_p(inst.ctypes.data), len(inst), vbuf.ptr, nv,
+                                        tbuf.ptr, nt, cbuf.ptr), "sw_prepare_models")
+
+    for _ in range(5):
+        call()
+    ctx.check(lib.sw_sync(ctx.h), "sw_sync")
+    reps = 200
+    ctx.profile_enable(True)
+    ctx.profile_read(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    ctx.check(lib.sw_sync(ctx.h), "sw_sync")
+    wall = (time.perf_counter() - t0) / reps
+    ctx.profile_enable(False)
+    prof = ctx.profile_read(reset=True)
+    kus = prof["models"][0] / max(prof["models"][1], 1) * 1e3
+    visible = int(cbuf.read(np.uint32, len(inst)).sum())
+    for b in bufs + [vbuf, tbuf, cbuf]:
+        b.free()
+    out = {"workload": f"{len(inst)} instances x {len(meshes[0].triangles)}-triangle vertex-animated mesh (lerped frames, "
+                       "per-vertex light cube + 2 directional lights, back faces rejected, back-to-front sort)",
+           "kernel_us": round(kus, 2), "us_per_call": round(wall * 1e6, 2), "visible_triangles": visible,
+           "mtriangles_s": round(nt / (kus * 1e-6) / 1e6, 1)}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 2.0:
+            oracle.prepare_models(meshes, inst)
+            n += 1
+        cms = (time.perf_counter() - t0) / n * 1e3
+        out.update(cpu_oracle_ms=round(cms, 3), cpu_cores=1, speedup_vs_cpu=round(cms * 1e3 / kus, 1))
+    log(f"models: {out}")
+    return out
+
+
 def serving_leg(args):
     """Open-loop serving: four resident maps (16x16x4 .. 48x48x4 grids), Poisson arrivals at fixed
     rates, each request a random map, pose, resolution (log-uniform 320x200 .. 2560x1440, 16:10) and
@@ -300,6 +353,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-hdr", action="store_true")
+    ap.add_argument("--no-models", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--hdr-poses", type=int, default=16)
@@ -489,6 +543,9 @@ def main():
 
     if not args.no_hdr:
         result["hdr_models"] = hdr_leg(ctx, args)
+
+    if not args.no_models:
+        result["models"] = models_leg(ctx, args)
 
     if not args.no_serve:
         result["serving"] = serving_leg(args)

@@ -394,6 +394,85 @@ typedef struct sw_exposure_params
 float sw_update_exposure(
 	const sw_exposure_params* params, float current_exposure, const float average_rgb[3], float frame_duration_s);
 
+/* ---------------------------------------------------------------------------------------------
+ * Triangle-model preparation (triangle_models_rendering.rs:7-200, 324-391, 702-731): per model
+ * instance, the mesh's vertices animated, transformed into camera space and lit, then its
+ * front-facing triangles sorted back to front, for a batch of instances in one launch.
+ * ------------------------------------------------------------------------------------------- */
+
+/* VertexData's three kinds (triangle_model.rs:63-75). */
+#define SW_MODEL_NON_ANIMATED 0u
+#define SW_MODEL_VERTEX_ANIMATED 1u
+#define SW_MODEL_SKELETON_ANIMATED 2u
+#define SW_MODEL_MAX_BONES 255u /* MAX_TRIANGLE_MODEL_BONES */
+#define SW_MODEL_MAX_TRIANGLES 65536u
+
+/* A TriangleModelMesh and the parts of its TriangleModel the preparation reads, in device memory
+ * (the reference's in-memory layouts):
+ *   NON_ANIMATED:      vertices = VertexNonAnimated[num_vertices] {position[3], normal[3], tex_coord[2]}
+ *   VERTEX_ANIMATED:   vertices = VertexAnimatedVertexConstant[num_vertices] {tex_coord[2]},
+ *                      frame_vertices = VertexAnimatedVertexVariable[num_frames * num_vertices]
+ *                      {position[3], normal[3]}
+ *   SKELETON_ANIMATED: vertices = SkeletonAnimatedVertex[num_vertices] {tex_coord[2], position[3],
+ *                      normal[3], {bone_index u8, weight u8}[4]} (40 bytes), bone_parents[num_bones]
+ *                      (an index >= num_bones: no parent; a parent precedes its children),
+ *                      frame_bones = column-major 4x4 matrices [num_frames * num_bones] (num_bones
+ *                      0: the vertices are used as they are)
+ * triangles: 3 u16 vertex indices each (Triangle, triangle_model.rs:59-61). */
+typedef struct sw_model_mesh
+{
+	uint32_t kind;
+	uint32_t num_vertices;
+	uint32_t num_triangles; /* <= SW_MODEL_MAX_TRIANGLES */
+	uint32_t num_frames;
+	uint32_t num_bones; /* <= SW_MODEL_MAX_BONES */
+	uint32_t reserved;
+	const void* vertices;
+	const float* frame_vertices;
+	const uint16_t* triangles;
+	const uint32_t* bone_parents;
+	const float* frame_bones;
+} sw_model_mesh;
+
+/* One model of the frame: the arguments of animate_and_transform_triangle_mesh_vertices
+ * (triangle_models_rendering.rs:7-17) and where its results go. */
+typedef struct sw_model_instance
+{
+	uint32_t mesh; /* index into the meshes array */
+	uint32_t frames[2]; /* AnimationPoint::frames (< num_frames of the mesh when it is animated) */
+	float lerp; /* AnimationPoint::lerp: the weight of frames[0] */
+	float model_matrix[16]; /* column-major */
+	float model_view_matrix[16]; /* column-major: camera-space projection times model matrix */
+	float tc_scale[2];
+	float tc_shift[2];
+	float light_cube[6][3]; /* ModelLightData (:398-460): the summed light cube */
+	float light_vector_scaled[2][3]; /* its two directional components: static, dynamic */
+	float light_color[2][3];
+	uint32_t first_vertex; /* output: num_vertices sw_model_vertex from here */
+	uint32_t first_triangle; /* output: up to num_triangles triangles from here */
+} sw_model_instance;
+
+/* ModelVertex3d (clipping.rs:196-201). */
+typedef struct sw_model_vertex
+{
+	float pos[3];
+	float tc[2];
+	float light[3];
+} sw_model_vertex;
+
+/* animate_and_transform_triangle_mesh_vertices, then reject_triangle_model_back_faces and
+ * sort_model_triangles (back to front by the largest camera-space z of the three vertices; equal
+ * depths keep the mesh's triangle order — the reference's unstable sort leaves their order
+ * open), for every instance. meshes and instances are host arrays (the meshes' data pointers are
+ * device pointers); out_vertices (16-byte aligned, vertex_capacity entries), out_triangles (3 u16
+ * per triangle, triangle_capacity triangles) and out_counts (visible triangles per instance) are
+ * device memory. Queued on the context's stream; sw_sync reports an out-of-range vertex, bone or
+ * parent index found on the device (the kernel clamps it). */
+int sw_prepare_models(
+	sw_ctx* ctx, const sw_model_mesh* meshes, uint32_t num_meshes, const sw_model_instance* instances, uint32_t num_instances,
+	sw_model_vertex* out_vertices, uint64_t vertex_capacity, uint16_t* out_triangles, uint64_t triangle_capacity,
+	uint32_t* out_counts);
+
 /* Work counters of the last batch. The pixel counters of a batch are summed on the first call
  * after it (one small kernel and a wait), so batches that nobody asks about pay nothing for them.
  * No counterpart in the reference (its renderer keeps no such counters). */
@@ -431,9 +510,10 @@ void sw_host_free(sw_ctx* ctx, void* host_ptr);
  * Stages: 0 setup_draws (edge walk, span records and, for device-resident descriptors, surface
  * extents), 1 build_surfaces, 2 raster_tiles, 3 tonemap, 4 setup_and_surfaces (draw setup and
  * surface texels in one launch, for host-resident descriptors), 5 bloom (downscale, blur and the
- * tonemap with bloom). sw_profile_read returns the summed milliseconds and launch counts since the
- * last reset (and resets when reset != 0). Not recorded inside captured graphs. */
-#define SW_PROFILE_STAGES 6
+ * tonemap with bloom), 6 models (sw_prepare_models). sw_profile_read returns the summed
+ * milliseconds and launch counts since the last reset (and resets when reset != 0). Not recorded
+ * inside captured graphs. */
+#define SW_PROFILE_STAGES 7
 int sw_profile_enable(sw_ctx* ctx, uint32_t enable);
 int sw_profile_read(sw_ctx* ctx, double out_ms[SW_PROFILE_STAGES], uint64_t out_counts[SW_PROFILE_STAGES], uint32_t reset);
 

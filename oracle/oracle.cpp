@@ -1788,6 +1788,319 @@ void bloom_tonemap_linear(const uint64_t* hdr, const Bloom& b, const Tonemapper&
 	corner(right_x, b.w, upper_y, b.h, bl(b.bw - 1, b.bh - 1));
 }
 
+// ---------------------------------------------------------------------------------------------
+// Triangle-model preparation (triangle_models_rendering.rs:7-200, 324-401, 702-731; light.rs:296-340),
+// with cgmath 0.18's f32 matrix arithmetic as the reference calls it: Matrix4 * Vector4 is
+// a * v.x + b * v.y + c * v.z + d * v.w over the columns a..d, Matrix * Matrix takes that per column
+// of the right operand, dot is x + y + z of the products, all unfused.
+// ---------------------------------------------------------------------------------------------
+struct CgMat4
+{
+	float c[4][4]; // c[col][row]
+};
+struct CgMat3
+{
+	float c[3][3];
+};
+
+inline float cg_dot(Vec3 a, Vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline Vec3 cg_cross(Vec3 a, Vec3 b) { return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+
+CgMat4 cg_mat4(const float* m)
+{
+	CgMat4 r;
+	for(int c = 0; c < 4; ++c)
+		for(int k = 0; k < 4; ++k)
+			r.c[c][k] = m[c * 4 + k];
+	return r;
+}
+
+// Matrix4 * Vector4, then truncate (matrix.rs:125-128 with w = 1)
+Vec3 cg_transform(const CgMat4& m, Vec3 v, float w)
+{
+	float out[3];
+	for(int k = 0; k < 3; ++k)
+		out[k] = m.c[0][k] * v.x + m.c[1][k] * v.y + m.c[2][k] * v.z + m.c[3][k] * w;
+	return v3(out[0], out[1], out[2]);
+}
+Vec3 cg_mul3(const CgMat3& m, Vec3 v)
+{
+	float out[3];
+	for(int k = 0; k < 3; ++k)
+		out[k] = m.c[0][k] * v.x + m.c[1][k] * v.y + m.c[2][k] * v.z;
+	return v3(out[0], out[1], out[2]);
+}
+CgMat4 cg_mul44(const CgMat4& a, const CgMat4& b)
+{
+	CgMat4 r;
+	for(int col = 0; col < 4; ++col)
+		for(int k = 0; k < 4; ++k)
+			r.c[col][k] = a.c[0][k] * b.c[col][0] + a.c[1][k] * b.c[col][1] + a.c[2][k] * b.c[col][2] + a.c[3][k] * b.c[col][3];
+	return r;
+}
+CgMat3 cg_mul33(const CgMat3& a, const CgMat3& b)
+{
+	CgMat3 r;
+	for(int col = 0; col < 3; ++col)
+		for(int k = 0; k < 3; ++k)
+			r.c[col][k] = a.c[0][k] * b.c[col][0] + a.c[1][k] * b.c[col][1] + a.c[2][k] * b.c[col][2];
+	return r;
+}
+
+// get_normals_matrix (:722-731): Matrix3::from_cols of the model matrix's first three columns
+// (truncated), transposed, inverted — cgmath's Matrix3::invert: det = a0.x (a1.y a2.z - a2.y a1.z)
+// - a1.x (a0.y a2.z - a2.y a0.z) + a2.x (a0.y a1.z - a1.y a0.z); the inverse is
+// from_cols(a1 x a2 / det, a2 x a0 / det, a0 x a1 / det) transposed; None (-> identity) at det 0.
+CgMat3 get_normals_matrix(const CgMat4& m)
+{
+	CgMat3 axis;
+	for(int c = 0; c < 3; ++c)
+		for(int k = 0; k < 3; ++k)
+			axis.c[c][k] = m.c[c][k];
+	CgMat3 a; // transpose
+	for(int c = 0; c < 3; ++c)
+		for(int k = 0; k < 3; ++k)
+			a.c[c][k] = axis.c[k][c];
+	const Vec3 a0 = v3(a.c[0][0], a.c[0][1], a.c[0][2]), a1 = v3(a.c[1][0], a.c[1][1], a.c[1][2]), a2 = v3(a.c[2][0], a.c[2][1], a.c[2][2]);
+	const float det = a0.x * (a1.y * a2.z - a2.y * a1.z) - a1.x * (a0.y * a2.z - a2.y * a0.z) + a2.x * (a0.y * a1.z - a1.y * a0.z);
+	CgMat3 r;
+	if(det == 0.0f)
+	{
+		for(int c = 0; c < 3; ++c)
+			for(int k = 0; k < 3; ++k)
+				r.c[c][k] = c == k ? 1.0f : 0.0f;
+		return r;
+	}
+	const Vec3 cols[3] = {cg_cross(a1, a2), cg_cross(a2, a0), cg_cross(a0, a1)};
+	CgMat3 f;
+	for(int c = 0; c < 3; ++c)
+	{
+		f.c[c][0] = cols[c].x / det;
+		f.c[c][1] = cols[c].y / det;
+		f.c[c][2] = cols[c].z / det;
+	}
+	for(int c = 0; c < 3; ++c)
+		for(int k = 0; k < 3; ++k)
+			r.c[c][k] = f.c[k][c];
+	return r;
+}
+
+// get_light_cube_light (light.rs:296-340)
+void get_light_cube_light(const float cube[6][3], Vec3 n, float out[3])
+{
+	float t[3] = {0.0f, 0.0f, 0.0f};
+	const float comps[3] = {n.x, n.y, n.z};
+	for(int axis = 0; axis < 3; ++axis)
+	{
+		if(comps[axis] <= 0.0f)
+			for(int i = 0; i < 3; ++i)
+				t[i] += cube[axis * 2][i] * (-comps[axis]);
+		else
+			for(int i = 0; i < 3; ++i)
+				t[i] += cube[axis * 2 + 1][i] * comps[axis];
+	}
+	out[0] = t[0];
+	out[1] = t[1];
+	out[2] = t[2];
+}
+
+// get_vertex_light (:702-720)
+void get_vertex_light(const Numerics& nm, const sw_model_instance& in, Vec3 n, float out[3])
+{
+	const Vec3 nn = n * nm.inv_sqrt_fast(fmax_r(cg_dot(n, n), 0.00000001f));
+	get_light_cube_light(in.light_cube, nn, out);
+	for(int d = 0; d < 2; ++d)
+	{
+		const Vec3 v = v3(in.light_vector_scaled[d][0], in.light_vector_scaled[d][1], in.light_vector_scaled[d][2]);
+		const float dot = fmax_r(cg_dot(nn, v), 0.0f);
+		for(int i = 0; i < 3; ++i)
+			out[i] += in.light_color[d][i] * dot;
+	}
+}
+
+void put_vertex(sw_model_vertex& dst, Vec3 pos, const float tc[2], const sw_model_instance& in, const float light[3])
+{
+	dst.pos[0] = pos.x;
+	dst.pos[1] = pos.y;
+	dst.pos[2] = pos.z;
+	dst.tc[0] = tc[0] * in.tc_scale[0] + in.tc_shift[0];
+	dst.tc[1] = tc[1] * in.tc_scale[1] + in.tc_shift[1];
+	for(int i = 0; i < 3; ++i)
+		dst.light[i] = light[i];
+}
+
+// animate_and_transform_triangle_mesh_vertices (:7-200) for one instance; false on an index out
+// of range (clamped, as the device kernel does).
+bool animate_and_transform(const Numerics& nm, const sw_model_mesh& mesh, const sw_model_instance& in, sw_model_vertex* dst)
+{
+	bool ok = true;
+	const CgMat4 model = cg_mat4(in.model_matrix), mv = cg_mat4(in.model_view_matrix);
+	const CgMat3 normals = get_normals_matrix(model);
+	const uint32_t f0 = in.frames[0], f1 = in.frames[1];
+	const float lerp0 = fmin_r(fmax_r(in.lerp, 0.0f), 1.0f);
+	const float lerp1 = 1.0f - lerp0;
+	const bool perform_lerp = lerp0 > 0.01f && lerp0 < 0.99f;
+	const uint32_t nv = mesh.num_vertices;
+	float light[3];
+	if(mesh.kind == SW_MODEL_NON_ANIMATED)
+	{
+		const float* v = static_cast<const float*>(mesh.vertices);
+		for(uint32_t i = 0; i < nv; ++i, v += 8)
+		{
+			get_vertex_light(nm, in, cg_mul3(normals, v3(v[3], v[4], v[5])), light);
+			put_vertex(dst[i], cg_transform(mv, v3(v[0], v[1], v[2]), 1.0f), v + 6, in, light);
+		}
+	}
+	else if(mesh.kind == SW_MODEL_VERTEX_ANIMATED)
+	{
+		const float* tc = static_cast<const float*>(mesh.vertices);
+		const float* d0 = mesh.frame_vertices + uint64_t(f0) * nv * 6;
+		const float* d1 = mesh.frame_vertices + uint64_t(f1) * nv * 6;
+		for(uint32_t i = 0; i < nv; ++i)
+		{
+			const float* a = d0 + uint64_t(i) * 6;
+			const float* b = d1 + uint64_t(i) * 6;
+			Vec3 pos, normal;
+			if(perform_lerp)
+			{
+				pos = v3(a[0], a[1], a[2]) * lerp0 + v3(b[0], b[1], b[2]) * lerp1;
+				normal = v3(a[3], a[4], a[5]) * lerp0 + v3(b[3], b[4], b[5]) * lerp1;
+			}
+			else
+			{
+				const float* s = lerp0 > lerp1 ? a : b;
+				pos = v3(s[0], s[1], s[2]);
+				normal = v3(s[3], s[4], s[5]);
+			}
+			get_vertex_light(nm, in, cg_mul3(normals, normal), light);
+			put_vertex(dst[i], cg_transform(mv, pos, 1.0f), tc + uint64_t(i) * 2, in, light);
+		}
+	}
+	else if(mesh.num_bones == 0)
+	{
+		// SkeletonAnimated without frame bones: the source vertices (:96-107)
+		const float* v = static_cast<const float*>(mesh.vertices);
+		for(uint32_t i = 0; i < nv; ++i, v += 10)
+		{
+			get_vertex_light(nm, in, cg_mul3(normals, v3(v[5], v[6], v[7])), light);
+			put_vertex(dst[i], cg_transform(mv, v3(v[2], v[3], v[4]), 1.0f), v, in, light);
+		}
+	}
+	else
+	{
+		const uint32_t nb = mesh.num_bones;
+		std::vector<CgMat4> matrices(nb);
+		std::vector<CgMat3> normals_matrices(nb);
+		const float* fb0 = mesh.frame_bones + uint64_t(f0) * nb * 16;
+		const float* fb1 = mesh.frame_bones + uint64_t(f1) * nb * 16;
+		for(uint32_t b = 0; b < nb; ++b)
+		{
+			CgMat4 mat;
+			const CgMat4 m0 = cg_mat4(fb0 + uint64_t(b) * 16), m1 = cg_mat4(fb1 + uint64_t(b) * 16);
+			if(perform_lerp)
+			{
+				for(int c = 0; c < 4; ++c)
+					for(int k = 0; k < 4; ++k)
+						mat.c[c][k] = m0.c[c][k] * lerp0 + m1.c[c][k] * lerp1;
+			}
+			else
+				mat = lerp0 > lerp1 ? m0 : m1;
+			uint32_t parent = mesh.bone_parents[b];
+			if(parent < nb && parent >= b)
+			{
+				ok = false;
+				parent = nb;
+			}
+			matrices[b] = parent < nb ? cg_mul44(matrices[parent], mat) : mat;
+		}
+		const float weight_scale = 1.0f / 255.0f;
+		CgMat3 nws;
+		CgMat4 mvws;
+		for(int c = 0; c < 3; ++c)
+			for(int k = 0; k < 3; ++k)
+				nws.c[c][k] = normals.c[c][k] * weight_scale;
+		for(int c = 0; c < 4; ++c)
+			for(int k = 0; k < 4; ++k)
+				mvws.c[c][k] = mv.c[c][k] * weight_scale;
+		for(uint32_t b = 0; b < nb; ++b)
+		{
+			normals_matrices[b] = cg_mul33(nws, get_normals_matrix(matrices[b]));
+			matrices[b] = cg_mul44(mvws, matrices[b]);
+		}
+		const uint8_t* base = static_cast<const uint8_t*>(mesh.vertices);
+		for(uint32_t i = 0; i < nv; ++i)
+		{
+			const float* v = reinterpret_cast<const float*>(base + uint64_t(i) * 40);
+			const uint8_t* bones = base + uint64_t(i) * 40 + 32;
+			auto bone = [&](int k) {
+				uint32_t bi = bones[k * 2];
+				if(bi >= nb)
+				{
+					ok = false;
+					bi = nb - 1;
+				}
+				return bi;
+			};
+			const uint32_t i0 = bone(0);
+			const float w0 = float(bones[1]);
+			CgMat4 mat;
+			CgMat3 nmat;
+			for(int c = 0; c < 4; ++c)
+				for(int k = 0; k < 4; ++k)
+					mat.c[c][k] = matrices[i0].c[c][k] * w0;
+			for(int c = 0; c < 3; ++c)
+				for(int k = 0; k < 3; ++k)
+					nmat.c[c][k] = normals_matrices[i0].c[c][k] * w0;
+			for(int j = 1; j < 4; ++j)
+				if(bones[j * 2 + 1] > 0)
+				{
+					const uint32_t ii = bone(j);
+					const float wi = float(bones[j * 2 + 1]);
+					for(int c = 0; c < 4; ++c)
+						for(int k = 0; k < 4; ++k)
+							mat.c[c][k] += matrices[ii].c[c][k] * wi;
+					for(int c = 0; c < 3; ++c)
+						for(int k = 0; k < 3; ++k)
+							nmat.c[c][k] += normals_matrices[ii].c[c][k] * wi;
+				}
+			get_vertex_light(nm, in, cg_mul3(nmat, v3(v[5], v[6], v[7])), light);
+			put_vertex(dst[i], cg_transform(mat, v3(v[2], v[3], v[4]), 1.0f), v, in, light);
+		}
+	}
+	return ok;
+}
+
+// reject_triangle_model_back_faces (:324-341) and sort_model_triangles (:343-391): the stable
+// variant of the reference's comparator (greater max z first; equal depths in mesh order).
+uint32_t reject_and_sort(const sw_model_mesh& mesh, const sw_model_vertex* verts, uint16_t* out, bool& ok)
+{
+	std::vector<uint32_t> tris; // indices of visible triangles
+	auto vidx = [&](uint32_t t, int k) {
+		uint32_t i = mesh.triangles[uint64_t(t) * 3 + k];
+		if(i >= mesh.num_vertices)
+		{
+			ok = false;
+			i = mesh.num_vertices - 1;
+		}
+		return i;
+	};
+	auto pos = [&](uint32_t i) { return v3(verts[i].pos[0], verts[i].pos[1], verts[i].pos[2]); };
+	for(uint32_t t = 0; t < mesh.num_triangles; ++t)
+	{
+		// get_triangle_plane (:393-401)
+		const Vec3 v0 = pos(vidx(t, 0)), v1 = pos(vidx(t, 1)), v2 = pos(vidx(t, 2));
+		const Vec3 vec = cg_cross(v1 - v0, v2 - v1);
+		if(cg_dot(vec, v0) > 0.0f)
+			tris.push_back(t);
+	}
+	auto max_z = [&](uint32_t t) { return fmax_r(fmax_r(pos(vidx(t, 0)).z, pos(vidx(t, 1)).z), pos(vidx(t, 2)).z); };
+	std::stable_sort(tris.begin(), tris.end(), [&](uint32_t a, uint32_t b) { return max_z(a) > max_z(b); });
+	for(size_t i = 0; i < tris.size(); ++i)
+		for(int k = 0; k < 3; ++k)
+			out[i * 3 + k] = uint16_t(vidx(tris[i], k));
+	return uint32_t(tris.size());
+}
+
 } // namespace
 
 // ---------------------------------------------------------------------------------------------
@@ -1893,6 +2206,26 @@ int oracle_bloom(
 	out_average_rgb[1] = v[1];
 	out_average_rgb[2] = v[0];
 	return radius;
+}
+
+// Triangle-model preparation for a batch of instances (all pointers host memory). Returns 0, or
+// 1 when an index was out of range (clamped).
+int oracle_prepare_models(int mode, const sw_model_mesh* meshes, uint32_t num_meshes, const sw_model_instance* instances,
+	uint32_t num_instances, sw_model_vertex* out_vertices, uint16_t* out_triangles, uint32_t* out_counts)
+{
+	const Numerics nm{mode};
+	bool ok = true;
+	for(uint32_t i = 0; i < num_instances; ++i)
+	{
+		const sw_model_instance& in = instances[i];
+		if(in.mesh >= num_meshes)
+			return -1;
+		const sw_model_mesh& mesh = meshes[in.mesh];
+		sw_model_vertex* verts = out_vertices + in.first_vertex;
+		ok &= animate_and_transform(nm, mesh, in, verts);
+		out_counts[i] = reject_and_sort(mesh, verts, out_triangles + uint64_t(in.first_triangle) * 3, ok);
+	}
+	return ok ? 0 : 1;
 }
 
 // Elementary operations, exposed for the closed-form / exhaustive tests.

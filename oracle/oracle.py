@@ -37,6 +37,9 @@ def load():
         lib.oracle_build_surfaces.restype = C.c_int
         lib.oracle_build_surfaces.argtypes = [C.c_int, C.POINTER(sw_abi.MapDesc), C.POINTER(sw_abi.Batch), C.c_void_p,
                                               C.POINTER(sw_abi.Stats)]
+        lib.oracle_prepare_models.restype = C.c_int
+        lib.oracle_prepare_models.argtypes = [C.c_int, C.POINTER(sw_abi.ModelMesh), C.c_uint32, C.c_void_p, C.c_uint32,
+                                              C.c_void_p, C.c_void_p, C.c_void_p]
         lib.oracle_bloom.restype = C.c_int
         lib.oracle_bloom.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.POINTER(C.c_float), C.c_float,
                                      C.c_uint32, C.c_float, C.c_uint32, C.c_void_p, C.c_uint32, C.POINTER(C.c_float),
@@ -127,3 +130,24 @@ def tonemap(hdr, exposure, modulate):
     mod = (C.c_float * 3)(*modulate)
     lib.oracle_tonemap(C.c_void_p(hdr.ctypes.data), w, h, float(exposure), mod, C.c_void_p(out.ctypes.data), w, sums)
     return out, np.array(sums[:], dtype=np.float32)
+
+
+def prepare_models(meshes, instances, mode=MODE_EXACT):
+    """Triangle-model preparation (triangle_models_rendering.rs:7-200, 324-401) over host
+    ModelMeshData and a MODEL_INSTANCE_DTYPE array. Returns (vertices, triangles, counts, ok)."""
+    lib = load()
+    inst = np.ascontiguousarray(instances, dtype=sw_abi.MODEL_INSTANCE_DTYPE)
+    arr = (sw_abi.ModelMesh * max(len(meshes), 1))()
+    for i, m in enumerate(meshes):
+        arr[i] = m.desc()
+    nv = max([int(i["first_vertex"]) + meshes[int(i["mesh"])].num_vertices for i in inst] + [1])
+    nt = max([int(i["first_triangle"]) + len(meshes[int(i["mesh"])].triangles) for i in inst] + [1])
+    verts = np.zeros(nv, dtype=sw_abi.MODEL_VERTEX_DTYPE)
+    tris = np.zeros((nt, 3), dtype=np.uint16)
+    counts = np.zeros(max(len(inst), 1), dtype=np.uint32)
+    r = lib.oracle_prepare_models(mode, arr, len(meshes), C.c_void_p(inst.ctypes.data), len(inst),
+                                  C.c_void_p(verts.ctypes.data), C.c_void_p(tris.ctypes.data),
+                                  C.c_void_p(counts.ctypes.data))
+    if r < 0:
+        raise ValueError("mesh index out of range")
+    return verts, tris, counts[:len(inst)], r == 0

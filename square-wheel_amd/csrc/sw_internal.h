@@ -48,6 +48,8 @@ enum
 #define HOST_ERR_OFFSET COUNTER_COUNT // after the result block in the host memory
 #define HOST_POST_WORDS 8 // postprocess results: tonemap colour sums [0, 3), bloom average [4, 8)
 #define HOST_POST_OFFSET (HOST_ERR_OFFSET + HOST_ERR_WORDS)
+#define HOST_MODEL_ERR_OFFSET (HOST_POST_OFFSET + HOST_POST_WORDS) // model preparation's error word
+#define HOST_WORDS_TOTAL (HOST_MODEL_ERR_OFFSET + 4)
 
 // Surface texels are built by blocks of SURF_BLOCK threads over the flattened arena, each block
 // surf_block_texels() consecutive texels: up to SURF_MAX_TEXELS_PER_THREAD per thread (the block's
@@ -240,4 +242,11 @@ void launch_bloom(hipStream_t st, const BloomArgs& a, bool average);
 // host memory, null: not wanted) receives their sum.
 void launch_tonemap(hipStream_t st, const uint64_t* hdr, uint32_t w, uint32_t h, float isr, float isg, float isb,
 	uint32_t* dst, uint32_t dst_pitch, float* partials, float* totals);
+// Triangle-model preparation: one block per instance. meshes / instances / sort_offsets are device
+// copies; scratch holds the sort keys of instances above model_sort_lds_limit() triangles
+// (sort_offsets[i] entries into it, a power of two of them each).
+void launch_prepare_models(hipStream_t st, uint32_t num_instances, const sw_model_mesh* meshes, const sw_model_instance* instances,
+	const uint64_t* sort_offsets, uint64_t* scratch, sw_model_vertex* out_vertices, uint16_t* out_triangles, uint32_t* out_counts,
+	uint32_t* host_err);
+uint32_t model_sort_lds_limit();
 } // namespace swk

@@ -66,7 +66,7 @@ struct sw_ctx
 
 	// command buffers / work buffers
 	DevBuf views, surfaces, lights, light_indices, draws, vdata, surf_ext, blk_surf, parts, spans, drec, dbounds, counters, arena, fb,
-		tonemap_out, tonemap_sums, bloom0, bloom1, bloom_lines;
+		tonemap_out, tonemap_sums, bloom0, bloom1, bloom_lines, model_args, model_scratch;
 	uint32_t tile_rows = 8; // raster tile height (SW_TILE_ROWS_*), set by sw_set_tile_rows / the autotuner
 	bool scalar_spans = true; // raster span records through the scalar unit (chosen by the autotuner)
 	uint32_t* counters_host = nullptr; // pinned, device-mapped: result block (k_finish), then the error words
@@ -116,6 +116,9 @@ struct sw_ctx
 		// the surface blocks, which then run in the setup launch)
 		uint32_t surface_mode = 0;
 	} last;
+	char* model_staging = nullptr; // pinned: packed sw_prepare_models arguments (copied to model_args)
+	size_t model_staging_cap = 0;
+	hipEvent_t model_staging_event = nullptr; // recorded after the copy out of model_staging
 	std::vector<SurfExt> host_ext;
 	std::vector<uint32_t> host_blk;
 
@@ -223,6 +226,7 @@ enum
 	STAGE_TONEMAP = 3,
 	STAGE_SETUP_SURF = 4,
 	STAGE_BLOOM = 5,
+	STAGE_MODELS = 6,
 };
 
 hipEvent_t take_event(sw_ctx* ctx)
@@ -748,14 +752,14 @@ int sw_ctx_create(int device, sw_ctx** out_ctx)
 	if(ensure(ctx, ctx->counters, sizeof(uint32_t) * COUNTER_TOTAL) ||
 		hipMemsetAsync(ctx->counters.p, 0, sizeof(uint32_t) * COUNTER_TOTAL, ctx->stream) != hipSuccess ||
 		hipStreamSynchronize(ctx->stream) != hipSuccess ||
-		hipHostMalloc(reinterpret_cast<void**>(&ctx->counters_host), sizeof(uint32_t) * (COUNTER_COUNT + HOST_ERR_WORDS + HOST_POST_WORDS),
+		hipHostMalloc(reinterpret_cast<void**>(&ctx->counters_host), sizeof(uint32_t) * HOST_WORDS_TOTAL,
 			hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
 		hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->counters_host_dev), ctx->counters_host, 0) != hipSuccess)
 	{
 		sw_ctx_destroy(ctx);
 		return SW_ERR_DEVICE;
 	}
-	std::memset(ctx->counters_host, 0, sizeof(uint32_t) * (COUNTER_COUNT + HOST_ERR_WORDS + HOST_POST_WORDS));
+	std::memset(ctx->counters_host, 0, sizeof(uint32_t) * HOST_WORDS_TOTAL);
 	*out_ctx = ctx;
 	return SW_OK;
 }
@@ -773,11 +777,15 @@ void sw_ctx_destroy(sw_ctx* ctx)
 		(void)hipGraphDestroy(ctx->graph);
 	DevBuf* bufs[] = {&ctx->views, &ctx->surfaces, &ctx->lights, &ctx->light_indices, &ctx->draws, &ctx->vdata, &ctx->surf_ext, &ctx->blk_surf,
 		&ctx->parts, &ctx->spans, &ctx->drec, &ctx->dbounds, &ctx->counters, &ctx->arena, &ctx->fb, &ctx->tonemap_out, &ctx->tonemap_sums,
-		&ctx->bloom0, &ctx->bloom1, &ctx->bloom_lines};
+		&ctx->bloom0, &ctx->bloom1, &ctx->bloom_lines, &ctx->model_args, &ctx->model_scratch};
 	for(DevBuf* b : bufs)
 		release(ctx, *b);
 	if(ctx->counters_host)
 		(void)hipHostFree(ctx->counters_host);
+	if(ctx->model_staging)
+		(void)hipHostFree(ctx->model_staging);
+	if(ctx->model_staging_event)
+		(void)hipEventDestroy(ctx->model_staging_event);
 	collect_timings(ctx);
 	for(hipEvent_t e : ctx->event_pool)
 		(void)hipEventDestroy(e);
@@ -792,7 +800,17 @@ int sw_sync(sw_ctx* ctx)
 {
 	if(!ctx)
 		return SW_ERR_INVALID_ARGUMENT;
-	return hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+	const int r = hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+	if(r)
+		return r;
+	volatile uint32_t* model_err = ctx->counters_host + HOST_MODEL_ERR_OFFSET;
+	if(*model_err)
+	{
+		*model_err = 0;
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT,
+			"model preparation: a triangle's vertex index, a vertex's bone index or a bone's parent index was out of range");
+	}
+	return SW_OK;
 }
 
 void* sw_ctx_stream(sw_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
@@ -1160,6 +1178,105 @@ int sw_postprocess_device(sw_ctx* ctx, const void* hdr_device, uint32_t width, u
 		out_average_rgb[2] = avg[0];
 	}
 	return SW_OK;
+}
+
+int sw_prepare_models(sw_ctx* ctx, const sw_model_mesh* meshes, uint32_t num_meshes, const sw_model_instance* instances,
+	uint32_t num_instances, sw_model_vertex* out_vertices, uint64_t vertex_capacity, uint16_t* out_triangles,
+	uint64_t triangle_capacity, uint32_t* out_counts)
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	if(num_instances == 0)
+		return SW_OK;
+	if(!meshes || !instances || !out_vertices || !out_triangles || !out_counts)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	if(reinterpret_cast<uintptr_t>(out_vertices) % 16)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "out_vertices must be 16-byte aligned");
+	for(uint32_t i = 0; i < num_meshes; ++i)
+	{
+		const sw_model_mesh& m = meshes[i];
+		const std::string which = "model mesh " + std::to_string(i) + ": ";
+		if(m.kind > SW_MODEL_SKELETON_ANIMATED)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, which + "unknown kind");
+		if(m.num_triangles > SW_MODEL_MAX_TRIANGLES || m.num_vertices > 65536)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, which + "more than 65536 triangles or vertices");
+		if((m.num_triangles && (!m.triangles || m.num_vertices == 0)) || (m.num_vertices && !m.vertices))
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, which + "missing vertex or triangle data");
+		if(m.kind == SW_MODEL_VERTEX_ANIMATED && m.num_vertices && (m.num_frames == 0 || !m.frame_vertices))
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, which + "vertex animation without frames");
+		if(m.kind == SW_MODEL_SKELETON_ANIMATED &&
+			(m.num_bones > SW_MODEL_MAX_BONES || (m.num_bones && (m.num_frames == 0 || !m.frame_bones || !m.bone_parents))))
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, which + "bad skeleton (more than 255 bones, or no frames)");
+	}
+	// per-instance checks, output ranges, and the scratch regions of sorts too large for LDS
+	const uint32_t lds_limit = swk::model_sort_lds_limit();
+	std::vector<uint64_t> sort_offsets(num_instances, 0);
+	uint64_t scratch_entries = 0;
+	for(uint32_t i = 0; i < num_instances; ++i)
+	{
+		const sw_model_instance& in = instances[i];
+		const std::string which = "model instance " + std::to_string(i) + ": ";
+		if(in.mesh >= num_meshes)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, which + "mesh index out of range");
+		const sw_model_mesh& m = meshes[in.mesh];
+		const bool animated = m.kind == SW_MODEL_VERTEX_ANIMATED || (m.kind == SW_MODEL_SKELETON_ANIMATED && m.num_bones);
+		if(animated && m.num_vertices && (in.frames[0] >= m.num_frames || in.frames[1] >= m.num_frames))
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, which + "animation frame out of range");
+		if(uint64_t(in.first_vertex) + m.num_vertices > vertex_capacity ||
+			uint64_t(in.first_triangle) + m.num_triangles > triangle_capacity)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, which + "output range beyond the buffers' capacity");
+		uint64_t n_pad = 1;
+		while(n_pad < m.num_triangles)
+			n_pad <<= 1;
+		if(n_pad > lds_limit)
+		{
+			sort_offsets[i] = scratch_entries;
+			scratch_entries += n_pad;
+		}
+	}
+	(void)hipSetDevice(ctx->device);
+	const size_t mesh_bytes = sizeof(sw_model_mesh) * num_meshes, inst_bytes = sizeof(sw_model_instance) * num_instances,
+				 off_bytes = sizeof(uint64_t) * num_instances;
+	const size_t inst_at = (mesh_bytes + 255) & ~size_t(255), off_at = (inst_at + inst_bytes + 255) & ~size_t(255);
+	int r;
+	if((r = ensure(ctx, ctx->model_args, off_at + off_bytes)) ||
+		(scratch_entries && (r = ensure(ctx, ctx->model_scratch, scratch_entries * sizeof(uint64_t)))))
+		return r;
+	// the packed arguments go through pinned staging memory in one copy; the staging is reused once
+	// the previous call's copy has completed (its event)
+	const size_t total = off_at + off_bytes;
+	if(ctx->model_staging_event && (r = hip_check(ctx, hipEventSynchronize(ctx->model_staging_event), "model staging wait")))
+		return r;
+	if(total > ctx->model_staging_cap)
+	{
+		if(ctx->model_staging)
+			(void)hipHostFree(ctx->model_staging);
+		ctx->model_staging = nullptr;
+		ctx->model_staging_cap = 0;
+		const size_t cap = std::max<size_t>(total * 2, 64 << 10);
+		if((r = hip_check(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->model_staging), cap, hipHostMallocDefault),
+				"model staging allocation")))
+			return r;
+		ctx->model_staging_cap = cap;
+	}
+	if(!ctx->model_staging_event &&
+		(r = hip_check(ctx, hipEventCreateWithFlags(&ctx->model_staging_event, hipEventDisableTiming), "model staging event")))
+		return r;
+	std::memcpy(ctx->model_staging, meshes, mesh_bytes);
+	std::memcpy(ctx->model_staging + inst_at, instances, inst_bytes);
+	std::memcpy(ctx->model_staging + off_at, sort_offsets.data(), off_bytes);
+	char* args = static_cast<char*>(ctx->model_args.p);
+	if((r = hip_check(ctx, hipMemcpyAsync(args, ctx->model_staging, total, hipMemcpyHostToDevice, ctx->stream), "model arguments")) ||
+		(r = hip_check(ctx, hipEventRecord(ctx->model_staging_event, ctx->stream), "model staging event")))
+		return r;
+	{
+		StageTimer t(ctx, STAGE_MODELS);
+		swk::launch_prepare_models(ctx->stream, num_instances, reinterpret_cast<const sw_model_mesh*>(args),
+			reinterpret_cast<const sw_model_instance*>(args + inst_at), reinterpret_cast<const uint64_t*>(args + off_at),
+			static_cast<uint64_t*>(ctx->model_scratch.p), out_vertices, out_triangles, out_counts,
+			ctx->counters_host_dev + HOST_MODEL_ERR_OFFSET);
+	}
+	return hip_check(ctx, hipGetLastError(), "model preparation launch");
 }
 
 float sw_update_exposure(const sw_exposure_params* p, float current_exposure, const float average_rgb[3], float frame_duration_s)

@@ -55,7 +55,19 @@ VIEW_DTYPE = np.dtype([
     ("background_color", "<u8"), ("first_surface", "<u4"), ("num_surfaces", "<u4"), ("first_draw", "<u4"),
     ("num_draws", "<u4"), ("fb_offset", "<u8"), ("surface_arena_offset", "<u8")], align=True)
 
-EXPECTED_SIZES = {"TEXEL": 8, "LEVEL": 24, "DIRLM": 40, "LIGHT": 32, "SURFACE": 136, "DRAW": 96, "VIEW": 64}
+# triangle-model preparation (sw_prepare_models)
+MODEL_NON_ANIMATED, MODEL_VERTEX_ANIMATED, MODEL_SKELETON_ANIMATED = 0, 1, 2
+MODEL_INSTANCE_DTYPE = np.dtype([
+    ("mesh", "<u4"), ("frames", "<u4", 2), ("lerp", "<f4"), ("model_matrix", "<f4", 16), ("model_view_matrix", "<f4", 16),
+    ("tc_scale", "<f4", 2), ("tc_shift", "<f4", 2), ("light_cube", "<f4", (6, 3)), ("light_vector_scaled", "<f4", (2, 3)),
+    ("light_color", "<f4", (2, 3)), ("first_vertex", "<u4"), ("first_triangle", "<u4")], align=True)
+MODEL_VERTEX_DTYPE = np.dtype([("pos", "<f4", 3), ("tc", "<f4", 2), ("light", "<f4", 3)], align=True)
+# SkeletonAnimatedVertex as the C ABI lays it out (40 bytes)
+SKELETON_VERTEX_DTYPE = np.dtype([("tex_coord", "<f4", 2), ("position", "<f4", 3), ("normal", "<f4", 3),
+                                  ("bones", "u1", (4, 2))], align=True)
+
+EXPECTED_SIZES = {"TEXEL": 8, "LEVEL": 24, "DIRLM": 40, "LIGHT": 32, "SURFACE": 136, "DRAW": 96, "VIEW": 64,
+                  "MODEL_INSTANCE": 288, "MODEL_VERTEX": 32, "SKELETON_VERTEX": 40}
 for _name, _size in EXPECTED_SIZES.items():
     assert globals()[_name + "_DTYPE"].itemsize == _size, (_name, globals()[_name + "_DTYPE"].itemsize)
 
@@ -74,6 +86,58 @@ class MapDesc(C.Structure):
         ("skybox_texels32", C.c_void_p), ("num_skybox_texels32", C.c_uint64),
         ("skybox_texels64", C.c_void_p), ("num_skybox_texels64", C.c_uint64),
     ]
+
+
+class ModelMesh(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32),
+                ("num_frames", C.c_uint32), ("num_bones", C.c_uint32), ("reserved", C.c_uint32),
+                ("vertices", C.c_void_p), ("frame_vertices", C.c_void_p), ("triangles", C.c_void_p),
+                ("bone_parents", C.c_void_p), ("frame_bones", C.c_void_p)]
+
+
+class ModelMeshData:
+    """A TriangleModelMesh on the host (numpy arrays in the C ABI's layouts): `vertices` is
+    float32 [n, 8] (non-animated), float32 [n, 2] tex coords (vertex-animated, with `frame_vertices`
+    float32 [frames, n, 6]) or SKELETON_VERTEX_DTYPE [n] (with `bone_parents` u32 [bones] and
+    `frame_bones` float32 [frames, bones, 16]); `triangles` u16 [t, 3]."""
+
+    _ARRAYS = ("vertices", "frame_vertices", "triangles", "bone_parents", "frame_bones")
+
+    def __init__(self, kind, vertices, triangles, frame_vertices=None, bone_parents=None, frame_bones=None):
+        self.kind = kind
+        self.vertices = np.ascontiguousarray(vertices)
+        self.triangles = np.ascontiguousarray(triangles, dtype=np.uint16).reshape(-1, 3)
+        self.frame_vertices = None if frame_vertices is None else np.ascontiguousarray(frame_vertices, dtype=np.float32)
+        self.bone_parents = None if bone_parents is None else np.ascontiguousarray(bone_parents, dtype=np.uint32)
+        self.frame_bones = None if frame_bones is None else np.ascontiguousarray(frame_bones, dtype=np.float32)
+
+    @property
+    def num_vertices(self):
+        return len(self.vertices)
+
+    @property
+    def num_frames(self):
+        if self.kind == MODEL_VERTEX_ANIMATED:
+            return len(self.frame_vertices)
+        if self.kind == MODEL_SKELETON_ANIMATED and self.frame_bones is not None:
+            return len(self.frame_bones)
+        return 0
+
+    def desc(self, ptrs=None):
+        """sw_model_mesh over host arrays, or over the device pointers in `ptrs` (name -> c_void_p)."""
+        d = ModelMesh()
+        d.kind = self.kind
+        d.num_vertices = self.num_vertices
+        d.num_triangles = len(self.triangles)
+        d.num_frames = self.num_frames
+        d.num_bones = 0 if self.bone_parents is None else len(self.bone_parents)
+        for name in self._ARRAYS:
+            a = getattr(self, name)
+            if ptrs is not None:
+                setattr(d, name, ptrs.get(name))
+            else:
+                setattr(d, name, None if a is None or a.size == 0 else a.ctypes.data)
+        return d
 
 
 class Batch(C.Structure):
@@ -213,6 +277,8 @@ GPU_SYMBOLS = {
                                         C.POINTER(C.c_float), C.c_void_p, C.c_void_p, C.c_size_t,
                                         C.POINTER(C.c_float)]),
     "sw_update_exposure": (C.c_float, [C.c_void_p, C.c_float, C.POINTER(C.c_float), C.c_float]),
+    "sw_prepare_models": (C.c_int, [C.c_void_p, C.POINTER(ModelMesh), C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                    C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p]),
     "sw_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "sw_last_reruns": (C.c_uint32, [C.c_void_p]),
     "sw_set_scalar_spans": (C.c_int, [C.c_void_p, C.c_uint32]),
@@ -365,6 +431,48 @@ class Context:
             dst.free()
         return out, np.array(avg[:], dtype=np.float32)
 
+    # triangle-model preparation ------------------------------------------------------------
+    def upload_meshes(self, meshes):
+        """Copies ModelMeshData arrays to the device; returns (sw_model_mesh array, buffers)."""
+        arr = (ModelMesh * max(len(meshes), 1))()
+        bufs = []
+        for i, m in enumerate(meshes):
+            ptrs = {}
+            for name in ModelMeshData._ARRAYS:
+                a = getattr(m, name)
+                if a is not None and a.size:
+                    b = self.to_device(a)
+                    bufs.append(b)
+                    ptrs[name] = b.ptr
+            arr[i] = m.desc(ptrs)
+        return arr, bufs
+
+    def prepare_models(self, meshes, instances, check=True):
+        """sw_prepare_models over host meshes (uploaded here) and a MODEL_INSTANCE_DTYPE array.
+        Returns (vertices, triangles, counts) read back from the device."""
+        inst = np.ascontiguousarray(instances, dtype=MODEL_INSTANCE_DTYPE)
+        arr, bufs = self.upload_meshes(meshes)
+        def size(i, what):
+            mi = int(i["mesh"])
+            return 0 if mi >= len(meshes) else (meshes[mi].num_vertices if what == "v" else len(meshes[mi].triangles))
+        nv = max([int(i["first_vertex"]) + size(i, "v") for i in inst] + [1])
+        nt = max([int(i["first_triangle"]) + size(i, "t") for i in inst] + [1])
+        vbuf = self.alloc(nv * MODEL_VERTEX_DTYPE.itemsize)
+        tbuf = self.alloc(nt * 6)
+        cbuf = self.alloc(max(len(inst), 1) * 4)
+        try:
+            self.check(self.lib.sw_prepare_models(self.h, arr, len(meshes), _ptr(inst), len(inst), vbuf.ptr, nv, tbuf.ptr, nt,
+                                                  cbuf.ptr), "sw_prepare_models")
+            if check:
+                self.check(self.lib.sw_sync(self.h), "sw_sync")
+            verts = vbuf.read(np.uint8, nv * MODEL_VERTEX_DTYPE.itemsize).view(MODEL_VERTEX_DTYPE)
+            tris = tbuf.read(np.uint16, nt * 3).reshape(nt, 3)
+            counts = cbuf.read(np.uint32, len(inst))
+        finally:
+            for b in bufs + [vbuf, tbuf, cbuf]:
+                b.free()
+        return verts, tris, counts
+
     # device memory -----------------------------------------------------------------------
     def alloc(self, nbytes):
         p = C.c_void_p()
@@ -385,7 +493,7 @@ class Context:
     def profile_enable(self, on=True):
         self.check(self.lib.sw_profile_enable(self.h, 1 if on else 0), "sw_profile_enable")
 
-    PROFILE_STAGES = ("setup_draws", "build_surfaces", "raster_tiles", "tonemap", "setup_and_surfaces", "bloom")
+    PROFILE_STAGES = ("setup_draws", "build_surfaces", "raster_tiles", "tonemap", "setup_and_surfaces", "bloom", "models")
 
     def profile_read(self, reset=True):
         ms = (C.c_double * len(self.PROFILE_STAGES))()
