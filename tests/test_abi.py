@@ -53,12 +53,15 @@ STRUCTS = {
     "sw_surface_desc": sw_abi.SURFACE_DTYPE,
     "sw_draw": sw_abi.DRAW_DTYPE,
     "sw_view": sw_abi.VIEW_DTYPE,
+    "sw_model_instance": sw_abi.MODEL_INSTANCE_DTYPE,
+    "sw_model_vertex": sw_abi.MODEL_VERTEX_DTYPE,
 }
 
 
 CTYPES_STRUCTS = [(sw_abi.MapDesc, "sw_map_desc"), (sw_abi.Batch, "sw_batch"), (sw_abi.Stats, "sw_stats"),
                   (sw_abi.ViewC, "sw_view"), (sw_abi.FrameRequest, "sw_frame_request"), (sw_abi.Completion, "sw_completion"),
-                  (sw_abi.ServerStats, "sw_server_stats")]
+                  (sw_abi.ServerStats, "sw_server_stats"), (sw_abi.ModelMesh, "sw_model_mesh"),
+                  (sw_abi.BloomParams, "sw_bloom_params"), (sw_abi.ExposureParams, "sw_exposure_params")]
 
 
 @pytest.fixture(scope="module")
