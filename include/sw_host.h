@@ -106,6 +106,30 @@ int swh_frame_build(swh_frame* frame, const swh_map* map, const swh_frame_config
 	float yaw, float pitch);
 int swh_frame_get(const swh_frame* frame, swh_frame_data* out_data);
 
+/* Compiled-map files (common/bsp_map_save_load.rs:23-362): the map as a Square-Wheel .sqwm file,
+ * format version 11 — the compact BSP map's lumps (nodes, leafs, polygons, portals, leaf portals,
+ * vertices, texture names, entities with their key/value strings, lightmaps and the light grid,
+ * both compressed as common/lightmap_compression.rs does). The worldspawn entity carries this
+ * generator's configuration as "sw_*" keys, and the static lights are "light" entities. */
+int swh_map_save_sqwm(const swh_map* map, const char* path);
+/* Loads a .sqwm file written by swh_map_save_sqwm: header id and version, every lump's element
+ * size and range and every index a frame follows are checked (an error, not a crash, on a bad
+ * file). The textures and the voxel grid the frame builder needs are regenerated from the
+ * worldspawn's generator keys; the BSP, polygons, portals, lightmaps (decompressed) and light grid
+ * come from the file. */
+int swh_map_load_sqwm(const char* path, swh_map** out_map);
+/* The map's geometry as a Quake-format MAP file (common/map_file_q1.rs:24-115) for the reference's
+ * map_compiler: one axis-aligned box brush per run of solid cells along x (sky cells with the sky
+ * texture), the static lights as "light" entities, an info_player_start. */
+int swh_map_save_quake_map(const swh_map* map, const char* path);
+/* Message of the last failed save / load call. */
+const char* swh_last_error(void);
+/* CompressedColor / CompressedVector (lightmap_compression.rs), exposed for the tests:
+ * out = {components[3], scale}. */
+void swh_compress_color(const float color[3], uint8_t out[4]);
+void swh_decompress_color(const uint8_t compressed[4], float out[3]);
+void swh_compress_vector(const float v[3], int8_t out_components[3], uint8_t* out_scale);
+
 /* Map-side building blocks, exposed for the tests. */
 /* make_texture (textures.rs:93-161) of a single texel: normal map RGB + roughness -> packed. */
 int32_t swh_pack_normal_roughness_from_rgb(uint32_t normal_rgb, float roughness);

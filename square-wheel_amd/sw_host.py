@@ -81,6 +81,20 @@ def load():
         lib.swh_pack_normal_roughness_from_rgb.restype = C.c_int32
         lib.swh_build_texture_mips.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
         lib.swh_build_texture_mips.restype = C.c_uint64
+        lib.swh_map_save_sqwm.argtypes = [C.c_void_p, C.c_char_p]
+        lib.swh_map_save_sqwm.restype = C.c_int
+        lib.swh_map_load_sqwm.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+        lib.swh_map_load_sqwm.restype = C.c_int
+        lib.swh_map_save_quake_map.argtypes = [C.c_void_p, C.c_char_p]
+        lib.swh_map_save_quake_map.restype = C.c_int
+        lib.swh_last_error.argtypes = []
+        lib.swh_last_error.restype = C.c_char_p
+        lib.swh_compress_color.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_uint8)]
+        lib.swh_compress_color.restype = None
+        lib.swh_decompress_color.argtypes = [C.POINTER(C.c_uint8), C.POINTER(C.c_float)]
+        lib.swh_decompress_color.restype = None
+        lib.swh_compress_vector.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_int8), C.POINTER(C.c_uint8)]
+        lib.swh_compress_vector.restype = None
         _lib = lib
     return _lib
 
@@ -113,6 +127,32 @@ class Map:
         h = C.c_void_p()
         if lib.swh_map_create(C.byref(cfg), C.byref(h)) != 0:
             raise RuntimeError("swh_map_create failed")
+        self._attach(h)
+
+    @classmethod
+    def load_sqwm(cls, path):
+        """A map loaded from a .sqwm file written by save_sqwm (swh_map_load_sqwm)."""
+        lib = load()
+        h = C.c_void_p()
+        if lib.swh_map_load_sqwm(os.fsencode(path), C.byref(h)) != 0:
+            raise RuntimeError("swh_map_load_sqwm: " + lib.swh_last_error().decode())
+        m = cls.__new__(cls)
+        m.config = None
+        m._attach(h)
+        return m
+
+    def save_sqwm(self, path):
+        lib = load()
+        if lib.swh_map_save_sqwm(self.h, os.fsencode(path)) != 0:
+            raise RuntimeError("swh_map_save_sqwm: " + lib.swh_last_error().decode())
+
+    def save_quake_map(self, path):
+        lib = load()
+        if lib.swh_map_save_quake_map(self.h, os.fsencode(path)) != 0:
+            raise RuntimeError("swh_map_save_quake_map: " + lib.swh_last_error().decode())
+
+    def _attach(self, h):
+        lib = load()
         self.h = h
         d = A.MapDesc()
         lib.swh_map_desc(self.h, C.byref(d))

@@ -99,3 +99,24 @@ def test_models_decals_hdr_frames(gpu_ctx, map2, color_bits):
     finally:
         gpu_ctx.release_map(mh)
     assert tri_pixels > 0, "the path drew no model or decal pixels"
+
+
+def test_frames_of_a_loaded_sqwm_map(gpu_ctx, map1, tmp_path):
+    """A map saved as .sqwm and loaded back (lightmaps and light grid through the format's
+    compression) renders bit-exact against the oracle on the loaded data."""
+    path = str(tmp_path / "map1.sqwm")
+    map1.save_sqwm(path)
+    loaded = sw_host.Map.load_sqwm(path)
+    fb = sw_host.FrameBuilder()
+    mh = gpu_ctx.upload_map(loaded.data)
+    try:
+        for i, pose in enumerate(loaded.camera_path(4, 3)):
+            _say(f"case loaded sqwm map pose {i}")
+            f = fb.build(loaded, pose, width=480, height=270, use_directional_lightmaps=1, num_models=8, seed=i)
+            b = sw_host.batch_from_frames([f])
+            img = gpu_ctx.render_plain(mh, b).reshape(-1)
+            ref, _, _, oob = oracle.render(loaded.data, b)
+            assert oob == 0
+            assert np.array_equal(img, ref), f"pose {i}: {np.count_nonzero(img != ref)} pixels differ"
+    finally:
+        gpu_ctx.release_map(mh)
