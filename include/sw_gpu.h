@@ -389,6 +389,40 @@ typedef struct sw_exposure_params
 	float max_exposure;
 } sw_exposure_params;
 
+/* ---------------------------------------------------------------------------------------------
+ * Per-frame texture animation (the map materials processor's generative effects) on the map's
+ * resident textures: the caller (the materials processor) evaluates the effect's time functions
+ * and hands over one operation per mip level.
+ * ------------------------------------------------------------------------------------------- */
+#define SW_TEXOP_TURB 0u /* make_turb_distortion (generative_texture_effect_turb.rs:89-138) */
+#define SW_TEXOP_LAYER 1u /* apply_texture_layer (generative_texture_effect_layered.rs:162-381) */
+#define SW_TEXOP_LAYER_LITE 2u /* an emissive layer: mix_surface_with_texture (surfaces.rs:854-971) */
+
+typedef struct sw_texture_op
+{
+	uint32_t kind; /* SW_TEXOP_* */
+	uint32_t dst_level; /* the level written (index into the map's level table) */
+	uint32_t src_level; /* the level read; never a destination in the same call */
+	uint32_t blend; /* layers: the layer material's SW_BLEND_* */
+	int32_t shift[2]; /* layers: the texture-coordinate shift at this mip ((f(t) as i32) >> mip) */
+	float light[3]; /* layers: the modulation colour (1, 1, 1: none) */
+	uint32_t turb_shifts; /* TURB: index of the level's first value in turb_shifts; max(w, h) values */
+} sw_texture_op;
+
+/* Applies the operations to the map's device textures in order: operations with the same
+ * destination must be consecutive (they are applied texel by texel, layer after layer); TURB and
+ * TURB destinations have the source's power-of-two size, Texture levels for SW_TEXOP_LAYER,
+ * TextureLite levels for SW_TEXOP_LAYER_LITE (TURB takes either, both alike). turb_shifts: the
+ * per-coordinate shifts make_turb_distortion precomputes (round(sin(x * 2pi / (wave_length / 2^mip)
+ * + t * frequency * 2pi) * amplitude / 2^mip) for x < width; the values past the width are the
+ * reference's leftovers, see swh_turb_shifts). Queued on the context's stream. */
+int sw_apply_texture_ops(
+	sw_ctx* ctx, sw_map* map, const sw_texture_op* ops, uint32_t num_ops, const int32_t* turb_shifts, uint32_t num_turb_shifts);
+
+/* Copies map texels back to host memory (waits): which = 0 the Texture texels (sw_texel), 1 the
+ * TextureLite texels (Color32); `count` elements from element `first`. */
+int sw_map_read(sw_ctx* ctx, const sw_map* map, uint32_t which, uint64_t first, uint64_t count, void* dst);
+
 /* Postprocessor::update_exposure (postprocessor.rs:206-223): the next frame's exposure from this
  * frame's average colour (sw_postprocess_device) and duration. Host-side arithmetic. */
 float sw_update_exposure(
@@ -510,10 +544,10 @@ void sw_host_free(sw_ctx* ctx, void* host_ptr);
  * Stages: 0 setup_draws (edge walk, span records and, for device-resident descriptors, surface
  * extents), 1 build_surfaces, 2 raster_tiles, 3 tonemap, 4 setup_and_surfaces (draw setup and
  * surface texels in one launch, for host-resident descriptors), 5 bloom (downscale, blur and the
- * tonemap with bloom), 6 models (sw_prepare_models). sw_profile_read returns the summed
- * milliseconds and launch counts since the last reset (and resets when reset != 0). Not recorded
- * inside captured graphs. */
-#define SW_PROFILE_STAGES 7
+ * tonemap with bloom), 6 models (sw_prepare_models), 7 textures (sw_apply_texture_ops).
+ * sw_profile_read returns the summed milliseconds and launch counts since the last reset (and
+ * resets when reset != 0). Not recorded inside captured graphs. */
+#define SW_PROFILE_STAGES 8
 int sw_profile_enable(sw_ctx* ctx, uint32_t enable);
 int sw_profile_read(sw_ctx* ctx, double out_ms[SW_PROFILE_STAGES], uint64_t out_counts[SW_PROFILE_STAGES], uint32_t reset);
 

@@ -130,6 +130,22 @@ void swh_compress_color(const float color[3], uint8_t out[4]);
 void swh_decompress_color(const uint8_t compressed[4], float out[3]);
 void swh_compress_vector(const float v[3], int8_t out_components[3], uint8_t* out_scale);
 
+/* Per-frame texture animation parameters (the map materials processor's host side, for
+ * sw_apply_texture_ops). swh_turb_shifts: make_turb_distortion's shift table for one mip
+ * (generative_texture_effect_turb.rs:98-112) in a buffer kept between calls like the reference's
+ * temp buffer: it becomes max(width, height) long (entries past previous_len zero, the others
+ * kept) and its first `width` entries are recomputed; returns the new length. The buffer holds at
+ * least max(width, height) entries. */
+uint32_t swh_turb_shifts(
+	float amplitude, float wave_length, float frequency, float time_s, uint32_t width, uint32_t height, uint32_t mip, int32_t* buffer,
+	uint32_t previous_len);
+/* SingleArgumentFunction::evaluate (common/material_function.rs:90-156): kind 0 constant (p[0]),
+ * 1 linear (scale, offset), 2 sine, 3 triangle, 4 saw-tooth, 5 step (frequency, phase, amplitude,
+ * offset). */
+float swh_evaluate_function(uint32_t kind, const float p[4], float x);
+/* A layer's texture-coordinate shift at a mip: (value as i32) >> mip. */
+int32_t swh_layer_shift(float value, uint32_t mip);
+
 /* Map-side building blocks, exposed for the tests. */
 /* make_texture (textures.rs:93-161) of a single texel: normal map RGB + roughness -> packed. */
 int32_t swh_pack_normal_roughness_from_rgb(uint32_t normal_rgb, float roughness);

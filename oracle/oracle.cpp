@@ -2101,6 +2101,160 @@ uint32_t reject_and_sort(const sw_model_mesh& mesh, const sw_model_vertex* verts
 	return uint32_t(tris.size());
 }
 
+// ---------------------------------------------------------------------------------------------
+// Per-frame texture animation: make_turb_distortion (generative_texture_effect_turb.rs:89-138),
+// apply_texture_layer (generative_texture_effect_layered.rs:162-381) and the layered effect's
+// emissive layers (mix_surface_with_texture, surfaces.rs:903-971), level by level as the reference
+// loops.
+// ---------------------------------------------------------------------------------------------
+template <class T>
+void turb_level(const T* src, T* dst, int32_t w, int32_t h, const int32_t* shifts)
+{
+	const int32_t mask[2] = {w - 1, h - 1};
+	for(int32_t y = 0; y < h; ++y)
+	{
+		T* dst_row = dst + int64_t(y) * w;
+		const int32_t row_shift = shifts[y];
+		for(int32_t x = 0; x < w; ++x)
+		{
+			const int32_t column_shift = shifts[x];
+			const int32_t u = x + row_shift;
+			const int32_t v = y + column_shift;
+			dst_row[x] = src[(u & mask[0]) + (v & mask[1]) * w];
+		}
+	}
+}
+
+void texture_layer(sw_texel* dst, const sw_texture_level& dl, const sw_texel* src, const sw_texture_level& sl, const int32_t shift[2],
+	const float light[3], uint32_t blend)
+{
+	// fast path (:174-183): a plain copy
+	if(blend == SW_BLEND_NONE && dl.width == sl.width && dl.height == sl.height && shift[0] == 0 && shift[1] == 0 && light[0] == 1.0f &&
+		light[1] == 1.0f && light[2] == 1.0f)
+	{
+		std::memcpy(dst, src, sizeof(sw_texel) * dl.width * dl.height);
+		return;
+	}
+	bool modulate = false; // impl_1 (:231-256)
+	for(int c = 0; c < 3; ++c)
+		modulate |= light[c] < 0.98f || light[c] > 1.02f;
+	const float light_scale = float(1 << 8);
+	const float lv[3] = {light[0] * light_scale, light[1] * light_scale, light[2] * light_scale};
+	const __m128i light_vec = ci_from_f32x3(lv);
+	for(uint32_t dst_v = 0; dst_v < dl.height; ++dst_v)
+	{
+		sw_texel* dst_line = dst + uint64_t(dst_v) * dl.width;
+		const int32_t src_v = rem_euclid(shift[1] + int32_t(dst_v), int32_t(sl.height));
+		const sw_texel* src_line = src + uint64_t(uint32_t(src_v)) * sl.width;
+		int32_t src_u = rem_euclid(shift[0], int32_t(sl.width));
+		for(uint32_t x = 0; x < dl.width; ++x)
+		{
+			const sw_texel texel_value = src_line[src_u];
+			sw_texel& d = dst_line[x];
+			if(modulate)
+			{
+				const __m128i m = _mm_srai_epi32(_mm_mullo_epi32(ci_from_color32(texel_value.diffuse), light_vec), 8);
+				switch(blend)
+				{
+				case SW_BLEND_NONE:
+					d.diffuse = ci_into_color32(m);
+					d.packed_normal_roughness = texel_value.packed_normal_roughness;
+					break;
+				case SW_BLEND_AVERAGE: d.diffuse = ci_into_color32(_mm_srai_epi32(_mm_add_epi32(m, ci_from_color32(d.diffuse)), 1)); break;
+				case SW_BLEND_ADDITIVE: d.diffuse = ci_into_color32(_mm_add_epi32(m, ci_from_color32(d.diffuse))); break;
+				case SW_BLEND_ALPHA_TEST:
+					if(C32::test_alpha(texel_value.diffuse))
+					{
+						d.diffuse = ci_into_color32(m);
+						d.packed_normal_roughness = texel_value.packed_normal_roughness;
+					}
+					break;
+				case SW_BLEND_ALPHA_BLEND:
+				{
+					const int32_t alpha = C32::get_alpha(texel_value.diffuse);
+					d.diffuse = ci_into_color32(_mm_srai_epi32(_mm_add_epi32(_mm_mullo_epi32(m, _mm_set1_epi32(alpha)),
+																  _mm_mullo_epi32(ci_from_color32(d.diffuse), _mm_set1_epi32(255 - alpha))),
+						8));
+					break;
+				}
+				default: break;
+				}
+			}
+			else
+			{
+				switch(blend)
+				{
+				case SW_BLEND_NONE: d = texel_value; break;
+				case SW_BLEND_AVERAGE: d.diffuse = C32::average(d.diffuse, texel_value.diffuse); break;
+				case SW_BLEND_ADDITIVE:
+					d.diffuse = ci_into_color32(_mm_add_epi32(ci_from_color32(texel_value.diffuse), ci_from_color32(d.diffuse)));
+					break;
+				case SW_BLEND_ALPHA_TEST:
+					if(C32::test_alpha(texel_value.diffuse))
+						d = texel_value;
+					break;
+				case SW_BLEND_ALPHA_BLEND:
+				{
+					const int32_t alpha = C32::get_alpha(texel_value.diffuse);
+					d.diffuse = ci_into_color32(
+						_mm_srai_epi32(_mm_add_epi32(_mm_mullo_epi32(ci_from_color32(texel_value.diffuse), _mm_set1_epi32(alpha)),
+										   _mm_mullo_epi32(ci_from_color32(d.diffuse), _mm_set1_epi32(255 - alpha))),
+							8));
+					break;
+				}
+				default: break;
+				}
+			}
+			++src_u;
+			if(src_u == int32_t(sl.width))
+				src_u = 0;
+		}
+	}
+}
+
+void lite_layer(uint32_t* dst, const sw_texture_level& dl, const uint32_t* src, const sw_texture_level& sl, const int32_t shift[2],
+	const float light[3], uint32_t blend)
+{
+	const float light_scale = float(1 << 8);
+	const float lv[3] = {light[0] * light_scale, light[1] * light_scale, light[2] * light_scale};
+	const __m128i light_vec = ci_from_f32x3(lv);
+	for(uint32_t dst_v = 0; dst_v < dl.height; ++dst_v)
+	{
+		uint32_t* dst_line = dst + uint64_t(dst_v) * dl.width;
+		const int32_t src_v = rem_euclid(shift[1] + int32_t(dst_v), int32_t(sl.height));
+		const uint32_t* src_line = src + uint64_t(uint32_t(src_v)) * sl.width;
+		int32_t src_u = rem_euclid(shift[0], int32_t(sl.width));
+		for(uint32_t x = 0; x < dl.width; ++x)
+		{
+			const uint32_t texel_value = src_line[src_u];
+			const __m128i m = _mm_srai_epi32(_mm_mullo_epi32(ci_from_color32(texel_value), light_vec), 8);
+			uint32_t& d = dst_line[x];
+			switch(blend)
+			{
+			case SW_BLEND_NONE: d = ci_into_color32(m); break;
+			case SW_BLEND_AVERAGE: d = ci_into_color32(_mm_srai_epi32(_mm_add_epi32(m, ci_from_color32(d)), 1)); break;
+			case SW_BLEND_ADDITIVE: d = ci_into_color32(_mm_add_epi32(m, ci_from_color32(d))); break;
+			case SW_BLEND_ALPHA_TEST:
+				if(C32::test_alpha(texel_value))
+					d = ci_into_color32(m);
+				break;
+			case SW_BLEND_ALPHA_BLEND:
+			{
+				const int32_t alpha = C32::get_alpha(texel_value);
+				d = ci_into_color32(_mm_srai_epi32(
+					_mm_add_epi32(_mm_mullo_epi32(m, _mm_set1_epi32(alpha)), _mm_mullo_epi32(ci_from_color32(d), _mm_set1_epi32(255 - alpha))),
+					8));
+				break;
+			}
+			default: break;
+			}
+			++src_u;
+			if(src_u == int32_t(sl.width))
+				src_u = 0;
+		}
+	}
+}
+
 } // namespace
 
 // ---------------------------------------------------------------------------------------------
@@ -2226,6 +2380,34 @@ int oracle_prepare_models(int mode, const sw_model_mesh* meshes, uint32_t num_me
 		out_counts[i] = reject_and_sort(mesh, verts, out_triangles + uint64_t(in.first_triangle) * 3, ok);
 	}
 	return ok ? 0 : 1;
+}
+
+// Texture operations on host copies of the map's texel buffers, in order (the levels as in
+// sw_map_desc::levels).
+int oracle_apply_texture_ops(const sw_texture_level* levels, uint32_t num_levels, sw_texel* texels, uint32_t* lite,
+	const sw_texture_op* ops, uint32_t num_ops, const int32_t* turb_shifts)
+{
+	for(uint32_t i = 0; i < num_ops; ++i)
+	{
+		const sw_texture_op& op = ops[i];
+		if(op.dst_level >= num_levels || op.src_level >= num_levels)
+			return -1;
+		const sw_texture_level& dl = levels[op.dst_level];
+		const sw_texture_level& sl = levels[op.src_level];
+		const bool lite_level = (dl.flags & SW_TEX_LITE_LEVEL) != 0;
+		if(op.kind == SW_TEXOP_TURB)
+		{
+			if(lite_level)
+				turb_level(lite + sl.offset, lite + dl.offset, int32_t(dl.width), int32_t(dl.height), turb_shifts + op.turb_shifts);
+			else
+				turb_level(texels + sl.offset, texels + dl.offset, int32_t(dl.width), int32_t(dl.height), turb_shifts + op.turb_shifts);
+		}
+		else if(op.kind == SW_TEXOP_LAYER)
+			texture_layer(texels + dl.offset, dl, texels + sl.offset, sl, op.shift, op.light, op.blend);
+		else
+			lite_layer(lite + dl.offset, dl, lite + sl.offset, sl, op.shift, op.light, op.blend);
+	}
+	return 0;
 }
 
 // Elementary operations, exposed for the closed-form / exhaustive tests.

@@ -37,6 +37,8 @@ def load():
         lib.oracle_build_surfaces.restype = C.c_int
         lib.oracle_build_surfaces.argtypes = [C.c_int, C.POINTER(sw_abi.MapDesc), C.POINTER(sw_abi.Batch), C.c_void_p,
                                               C.POINTER(sw_abi.Stats)]
+        lib.oracle_apply_texture_ops.restype = C.c_int
+        lib.oracle_apply_texture_ops.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
         lib.oracle_prepare_models.restype = C.c_int
         lib.oracle_prepare_models.argtypes = [C.c_int, C.POINTER(sw_abi.ModelMesh), C.c_uint32, C.c_void_p, C.c_uint32,
                                               C.c_void_p, C.c_void_p, C.c_void_p]
@@ -151,3 +153,18 @@ def prepare_models(meshes, instances, mode=MODE_EXACT):
     if r < 0:
         raise ValueError("mesh index out of range")
     return verts, tris, counts[:len(inst)], r == 0
+
+
+def apply_texture_ops(map_data, ops, turb_shifts=None):
+    """Texture animation (generative_texture_effect_turb.rs / _layered.rs) on copies of the map's
+    texel buffers. Returns (texels, lite)."""
+    lib = load()
+    texels = map_data.texels.copy()
+    lite = map_data.lite.copy()
+    ops = np.ascontiguousarray(ops, dtype=sw_abi.TEXTURE_OP_DTYPE)
+    sh = np.ascontiguousarray(np.zeros(1, np.int32) if turb_shifts is None or len(turb_shifts) == 0 else turb_shifts, dtype=np.int32)
+    r = lib.oracle_apply_texture_ops(C.c_void_p(map_data.levels.ctypes.data), len(map_data.levels), C.c_void_p(texels.ctypes.data),
+                                     C.c_void_p(lite.ctypes.data), C.c_void_p(ops.ctypes.data), len(ops), C.c_void_p(sh.ctypes.data))
+    if r != 0:
+        raise ValueError("level out of range")
+    return texels, lite

@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+#include <vector>
+
 #include "../../include/sw_gpu.h"
 
 // Raster tiles are 64 pixels wide (one pixel column per lane) and TILE_ROWS rows high, chosen per
@@ -249,4 +252,16 @@ void launch_prepare_models(hipStream_t st, uint32_t num_instances, const sw_mode
 	const uint64_t* sort_offsets, uint64_t* scratch, sw_model_vertex* out_vertices, uint16_t* out_triangles, uint32_t* out_counts,
 	uint32_t* host_err);
 uint32_t model_sort_lds_limit();
+
+// Texture operations: pack_texture_ops validates them against the level table and packs the
+// launch arguments; the caller copies `packed` to the device and launches.
+struct TexOpPlan
+{
+	size_t bytes = 0, ops_at = 0;
+	uint32_t blocks = 0, num_runs = 0;
+};
+int pack_texture_ops(const sw_texture_op* ops, uint32_t num_ops, const sw_texture_level* levels, uint32_t num_levels,
+	uint32_t num_turb_shifts, std::vector<char>& packed, TexOpPlan& plan, std::string& error);
+void launch_texture_ops(hipStream_t st, const char* args_dev, const TexOpPlan& plan, const int32_t* turb_shifts_dev, sw_texel* texels,
+	uint32_t* lite);
 } // namespace swk

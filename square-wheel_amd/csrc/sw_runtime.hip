@@ -66,7 +66,7 @@ struct sw_ctx
 
 	// command buffers / work buffers
 	DevBuf views, surfaces, lights, light_indices, draws, vdata, surf_ext, blk_surf, parts, spans, drec, dbounds, counters, arena, fb,
-		tonemap_out, tonemap_sums, bloom0, bloom1, bloom_lines, model_args, model_scratch;
+		tonemap_out, tonemap_sums, bloom0, bloom1, bloom_lines, model_args, model_scratch, texop_args;
 	uint32_t tile_rows = 8; // raster tile height (SW_TILE_ROWS_*), set by sw_set_tile_rows / the autotuner
 	bool scalar_spans = true; // raster span records through the scalar unit (chosen by the autotuner)
 	uint32_t* counters_host = nullptr; // pinned, device-mapped: result block (k_finish), then the error words
@@ -116,9 +116,10 @@ struct sw_ctx
 		// the surface blocks, which then run in the setup launch)
 		uint32_t surface_mode = 0;
 	} last;
-	char* model_staging = nullptr; // pinned: packed sw_prepare_models arguments (copied to model_args)
-	size_t model_staging_cap = 0;
-	hipEvent_t model_staging_event = nullptr; // recorded after the copy out of model_staging
+	char* arg_staging = nullptr; // pinned: packed launch arguments (model preparation, texture ops)
+	size_t arg_staging_cap = 0;
+	hipEvent_t arg_staging_event = nullptr; // recorded after the last copy out of arg_staging
+	std::vector<char> texop_packed;
 	std::vector<SurfExt> host_ext;
 	std::vector<uint32_t> host_blk;
 
@@ -227,6 +228,7 @@ enum
 	STAGE_SETUP_SURF = 4,
 	STAGE_BLOOM = 5,
 	STAGE_MODELS = 6,
+	STAGE_TEXTURES = 7,
 };
 
 hipEvent_t take_event(sw_ctx* ctx)
@@ -777,15 +779,15 @@ void sw_ctx_destroy(sw_ctx* ctx)
 		(void)hipGraphDestroy(ctx->graph);
 	DevBuf* bufs[] = {&ctx->views, &ctx->surfaces, &ctx->lights, &ctx->light_indices, &ctx->draws, &ctx->vdata, &ctx->surf_ext, &ctx->blk_surf,
 		&ctx->parts, &ctx->spans, &ctx->drec, &ctx->dbounds, &ctx->counters, &ctx->arena, &ctx->fb, &ctx->tonemap_out, &ctx->tonemap_sums,
-		&ctx->bloom0, &ctx->bloom1, &ctx->bloom_lines, &ctx->model_args, &ctx->model_scratch};
+		&ctx->bloom0, &ctx->bloom1, &ctx->bloom_lines, &ctx->model_args, &ctx->model_scratch, &ctx->texop_args};
 	for(DevBuf* b : bufs)
 		release(ctx, *b);
 	if(ctx->counters_host)
 		(void)hipHostFree(ctx->counters_host);
-	if(ctx->model_staging)
-		(void)hipHostFree(ctx->model_staging);
-	if(ctx->model_staging_event)
-		(void)hipEventDestroy(ctx->model_staging_event);
+	if(ctx->arg_staging)
+		(void)hipHostFree(ctx->arg_staging);
+	if(ctx->arg_staging_event)
+		(void)hipEventDestroy(ctx->arg_staging_event);
 	collect_timings(ctx);
 	for(hipEvent_t e : ctx->event_pool)
 		(void)hipEventDestroy(e);
@@ -1180,6 +1182,41 @@ int sw_postprocess_device(sw_ctx* ctx, const void* hdr_device, uint32_t width, u
 	return SW_OK;
 }
 
+// Small launch arguments travel from pinned staging memory in one copy. stage_begin waits until
+// the previous copy out of the staging memory has completed (its event), grows it if needed and
+// returns it; stage_copy queues the copy and records the event.
+static int stage_begin(sw_ctx* ctx, size_t bytes, char** out)
+{
+	int r;
+	if(ctx->arg_staging_event && (r = hip_check(ctx, hipEventSynchronize(ctx->arg_staging_event), "argument staging wait")))
+		return r;
+	if(bytes > ctx->arg_staging_cap)
+	{
+		if(ctx->arg_staging)
+			(void)hipHostFree(ctx->arg_staging);
+		ctx->arg_staging = nullptr;
+		ctx->arg_staging_cap = 0;
+		const size_t cap = std::max<size_t>(bytes * 2, 64 << 10);
+		if((r = hip_check(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->arg_staging), cap, hipHostMallocDefault),
+				"argument staging allocation")))
+			return r;
+		ctx->arg_staging_cap = cap;
+	}
+	if(!ctx->arg_staging_event &&
+		(r = hip_check(ctx, hipEventCreateWithFlags(&ctx->arg_staging_event, hipEventDisableTiming), "argument staging event")))
+		return r;
+	*out = ctx->arg_staging;
+	return SW_OK;
+}
+
+static int stage_copy(sw_ctx* ctx, void* dst_device, size_t bytes)
+{
+	int r;
+	if((r = hip_check(ctx, hipMemcpyAsync(dst_device, ctx->arg_staging, bytes, hipMemcpyHostToDevice, ctx->stream), "argument copy")))
+		return r;
+	return hip_check(ctx, hipEventRecord(ctx->arg_staging_event, ctx->stream), "argument staging event");
+}
+
 int sw_prepare_models(sw_ctx* ctx, const sw_model_mesh* meshes, uint32_t num_meshes, const sw_model_instance* instances,
 	uint32_t num_instances, sw_model_vertex* out_vertices, uint64_t vertex_capacity, uint16_t* out_triangles,
 	uint64_t triangle_capacity, uint32_t* out_counts)
@@ -1242,32 +1279,14 @@ int sw_prepare_models(sw_ctx* ctx, const sw_model_mesh* meshes, uint32_t num_mes
 	if((r = ensure(ctx, ctx->model_args, off_at + off_bytes)) ||
 		(scratch_entries && (r = ensure(ctx, ctx->model_scratch, scratch_entries * sizeof(uint64_t)))))
 		return r;
-	// the packed arguments go through pinned staging memory in one copy; the staging is reused once
-	// the previous call's copy has completed (its event)
-	const size_t total = off_at + off_bytes;
-	if(ctx->model_staging_event && (r = hip_check(ctx, hipEventSynchronize(ctx->model_staging_event), "model staging wait")))
-		return r;
-	if(total > ctx->model_staging_cap)
-	{
-		if(ctx->model_staging)
-			(void)hipHostFree(ctx->model_staging);
-		ctx->model_staging = nullptr;
-		ctx->model_staging_cap = 0;
-		const size_t cap = std::max<size_t>(total * 2, 64 << 10);
-		if((r = hip_check(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->model_staging), cap, hipHostMallocDefault),
-				"model staging allocation")))
-			return r;
-		ctx->model_staging_cap = cap;
-	}
-	if(!ctx->model_staging_event &&
-		(r = hip_check(ctx, hipEventCreateWithFlags(&ctx->model_staging_event, hipEventDisableTiming), "model staging event")))
-		return r;
-	std::memcpy(ctx->model_staging, meshes, mesh_bytes);
-	std::memcpy(ctx->model_staging + inst_at, instances, inst_bytes);
-	std::memcpy(ctx->model_staging + off_at, sort_offsets.data(), off_bytes);
 	char* args = static_cast<char*>(ctx->model_args.p);
-	if((r = hip_check(ctx, hipMemcpyAsync(args, ctx->model_staging, total, hipMemcpyHostToDevice, ctx->stream), "model arguments")) ||
-		(r = hip_check(ctx, hipEventRecord(ctx->model_staging_event, ctx->stream), "model staging event")))
+	char* staging = nullptr;
+	if((r = stage_begin(ctx, off_at + off_bytes, &staging)))
+		return r;
+	std::memcpy(staging, meshes, mesh_bytes);
+	std::memcpy(staging + inst_at, instances, inst_bytes);
+	std::memcpy(staging + off_at, sort_offsets.data(), off_bytes);
+	if((r = stage_copy(ctx, args, off_at + off_bytes)))
 		return r;
 	{
 		StageTimer t(ctx, STAGE_MODELS);
@@ -1277,6 +1296,60 @@ int sw_prepare_models(sw_ctx* ctx, const sw_model_mesh* meshes, uint32_t num_mes
 			ctx->counters_host_dev + HOST_MODEL_ERR_OFFSET);
 	}
 	return hip_check(ctx, hipGetLastError(), "model preparation launch");
+}
+
+int sw_apply_texture_ops(
+	sw_ctx* ctx, sw_map* map, const sw_texture_op* ops, uint32_t num_ops, const int32_t* turb_shifts, uint32_t num_turb_shifts)
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	if(!map || (num_ops && !ops) || (num_turb_shifts && !turb_shifts))
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	if(num_ops == 0)
+		return SW_OK;
+	swk::TexOpPlan plan;
+	std::string error;
+	if(swk::pack_texture_ops(ops, num_ops, map->host_levels.data(), uint32_t(map->host_levels.size()), num_turb_shifts, ctx->texop_packed,
+		   plan, error))
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, error);
+	(void)hipSetDevice(ctx->device);
+	const size_t shifts_at = (plan.bytes + 255) & ~size_t(255), total = shifts_at + sizeof(int32_t) * num_turb_shifts;
+	int r;
+	char* staging = nullptr;
+	if((r = ensure(ctx, ctx->texop_args, total)) || (r = stage_begin(ctx, total, &staging)))
+		return r;
+	std::memcpy(staging, ctx->texop_packed.data(), plan.bytes);
+	if(num_turb_shifts)
+		std::memcpy(staging + shifts_at, turb_shifts, sizeof(int32_t) * num_turb_shifts);
+	char* args = static_cast<char*>(ctx->texop_args.p);
+	if((r = stage_copy(ctx, args, total)))
+		return r;
+	{
+		StageTimer t(ctx, STAGE_TEXTURES);
+		swk::launch_texture_ops(ctx->stream, args, plan, reinterpret_cast<const int32_t*>(args + shifts_at),
+			static_cast<sw_texel*>(map->texels.p), static_cast<uint32_t*>(map->lite.p));
+	}
+	return hip_check(ctx, hipGetLastError(), "texture ops launch");
+}
+
+int sw_map_read(sw_ctx* ctx, const sw_map* map, uint32_t which, uint64_t first, uint64_t count, void* dst)
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	if(!map || (count && !dst) || which > 1)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument or unknown buffer");
+	const uint64_t n = which == 0 ? map->ptrs.num_texels : map->ptrs.num_lite;
+	const size_t elem = which == 0 ? sizeof(sw_texel) : sizeof(uint32_t);
+	if(first > n || count > n - first)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "range beyond the map's texels");
+	if(count == 0)
+		return SW_OK;
+	(void)hipSetDevice(ctx->device);
+	const char* src = static_cast<const char*>(which == 0 ? map->texels.p : map->lite.p) + first * elem;
+	int r;
+	if((r = hip_check(ctx, hipMemcpyAsync(dst, src, count * elem, hipMemcpyDeviceToHost, ctx->stream), "map read")))
+		return r;
+	return hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
 }
 
 float sw_update_exposure(const sw_exposure_params* p, float current_exposure, const float average_rgb[3], float frame_duration_s)

@@ -62,12 +62,17 @@ MODEL_INSTANCE_DTYPE = np.dtype([
     ("tc_scale", "<f4", 2), ("tc_shift", "<f4", 2), ("light_cube", "<f4", (6, 3)), ("light_vector_scaled", "<f4", (2, 3)),
     ("light_color", "<f4", (2, 3)), ("first_vertex", "<u4"), ("first_triangle", "<u4")], align=True)
 MODEL_VERTEX_DTYPE = np.dtype([("pos", "<f4", 3), ("tc", "<f4", 2), ("light", "<f4", 3)], align=True)
+# texture animation (sw_apply_texture_ops)
+TEXOP_TURB, TEXOP_LAYER, TEXOP_LAYER_LITE = 0, 1, 2
+TEXTURE_OP_DTYPE = np.dtype([("kind", "<u4"), ("dst_level", "<u4"), ("src_level", "<u4"), ("blend", "<u4"),
+                             ("shift", "<i4", 2), ("light", "<f4", 3), ("turb_shifts", "<u4")], align=True)
 # SkeletonAnimatedVertex as the C ABI lays it out (40 bytes)
 SKELETON_VERTEX_DTYPE = np.dtype([("tex_coord", "<f4", 2), ("position", "<f4", 3), ("normal", "<f4", 3),
                                   ("bones", "u1", (4, 2))], align=True)
 
 EXPECTED_SIZES = {"TEXEL": 8, "LEVEL": 24, "DIRLM": 40, "LIGHT": 32, "SURFACE": 136, "DRAW": 96, "VIEW": 64,
-                  "MODEL_INSTANCE": 288, "MODEL_VERTEX": 32, "SKELETON_VERTEX": 40}
+                  "MODEL_INSTANCE": 288, "MODEL_VERTEX": 32, "SKELETON_VERTEX": 40,
+                  "TEXTURE_OP": 40}
 for _name, _size in EXPECTED_SIZES.items():
     assert globals()[_name + "_DTYPE"].itemsize == _size, (_name, globals()[_name + "_DTYPE"].itemsize)
 
@@ -277,6 +282,8 @@ GPU_SYMBOLS = {
                                         C.POINTER(C.c_float), C.c_void_p, C.c_void_p, C.c_size_t,
                                         C.POINTER(C.c_float)]),
     "sw_update_exposure": (C.c_float, [C.c_void_p, C.c_float, C.POINTER(C.c_float), C.c_float]),
+    "sw_apply_texture_ops": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]),
+    "sw_map_read": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p]),
     "sw_prepare_models": (C.c_int, [C.c_void_p, C.POINTER(ModelMesh), C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
                                     C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p]),
     "sw_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
@@ -431,6 +438,20 @@ class Context:
             dst.free()
         return out, np.array(avg[:], dtype=np.float32)
 
+    # texture animation --------------------------------------------------------------------
+    def apply_texture_ops(self, map_handle, ops, turb_shifts=None):
+        """sw_apply_texture_ops on an uploaded map (TEXTURE_OP_DTYPE array, int32 shift table)."""
+        ops = np.ascontiguousarray(ops, dtype=TEXTURE_OP_DTYPE)
+        sh = np.ascontiguousarray(np.zeros(0, np.int32) if turb_shifts is None else turb_shifts, dtype=np.int32)
+        self.check(self.lib.sw_apply_texture_ops(self.h, map_handle, _ptr(ops), len(ops), _ptr(sh) if len(sh) else None, len(sh)),
+                   "sw_apply_texture_ops")
+
+    def map_read(self, map_handle, which, first, count):
+        """sw_map_read: Texture texels (which 0, TEXEL_DTYPE) or TextureLite texels (1, uint32)."""
+        out = np.zeros(max(int(count), 1), dtype=TEXEL_DTYPE if which == 0 else np.uint32)
+        self.check(self.lib.sw_map_read(self.h, map_handle, which, int(first), int(count), _ptr(out)), "sw_map_read")
+        return out[:count]
+
     # triangle-model preparation ------------------------------------------------------------
     def upload_meshes(self, meshes):
         """Copies ModelMeshData arrays to the device; returns (sw_model_mesh array, buffers)."""
@@ -493,7 +514,8 @@ class Context:
     def profile_enable(self, on=True):
         self.check(self.lib.sw_profile_enable(self.h, 1 if on else 0), "sw_profile_enable")
 
-    PROFILE_STAGES = ("setup_draws", "build_surfaces", "raster_tiles", "tonemap", "setup_and_surfaces", "bloom", "models")
+    PROFILE_STAGES = ("setup_draws", "build_surfaces", "raster_tiles", "tonemap", "setup_and_surfaces", "bloom", "models",
+                      "textures")
 
     def profile_read(self, reset=True):
         ms = (C.c_double * len(self.PROFILE_STAGES))()

@@ -95,6 +95,13 @@ def load():
         lib.swh_decompress_color.restype = None
         lib.swh_compress_vector.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_int8), C.POINTER(C.c_uint8)]
         lib.swh_compress_vector.restype = None
+        lib.swh_turb_shifts.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_uint32, C.c_uint32,
+                                        C.c_void_p, C.c_uint32]
+        lib.swh_turb_shifts.restype = C.c_uint32
+        lib.swh_evaluate_function.argtypes = [C.c_uint32, C.POINTER(C.c_float), C.c_float]
+        lib.swh_evaluate_function.restype = C.c_float
+        lib.swh_layer_shift.argtypes = [C.c_float, C.c_uint32]
+        lib.swh_layer_shift.restype = C.c_int32
         _lib = lib
     return _lib
 
@@ -289,3 +296,27 @@ def large_map(seed=0, grid=(32, 32, 4), **kw):
 def large_map_alpha(seed=0, **kw):
     """The 1080p scene with 10% alpha-blended materials (the HDR + models scene)."""
     return large_map(seed, alpha_blend_fraction=0.1, **kw)
+
+
+class TurbShifts:
+    """make_turb_distortion's shift buffer for one effect instance (swh_turb_shifts), kept between
+    calls like the reference's temp buffer."""
+
+    def __init__(self):
+        self.buf = np.zeros(1 << 16, dtype=np.int32)
+        self.len = 0
+
+    def compute(self, amplitude, wave_length, frequency, time_s, width, height, mip):
+        self.len = int(load().swh_turb_shifts(amplitude, wave_length, frequency, time_s, width, height, mip,
+                                              C.c_void_p(self.buf.ctypes.data), self.len))
+        return self.buf[:self.len].copy()
+
+
+def evaluate_function(kind, params, x):
+    """SingleArgumentFunction::evaluate (swh_evaluate_function)."""
+    p = (C.c_float * 4)(*(list(params) + [0.0] * (4 - len(params))))
+    return float(load().swh_evaluate_function(kind, p, x))
+
+
+def layer_shift(value, mip):
+    return int(load().swh_layer_shift(value, mip))
