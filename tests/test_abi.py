@@ -55,6 +55,7 @@ STRUCTS = {
     "sw_view": sw_abi.VIEW_DTYPE,
     "sw_model_instance": sw_abi.MODEL_INSTANCE_DTYPE,
     "sw_model_vertex": sw_abi.MODEL_VERTEX_DTYPE,
+    "sw_texture_op": sw_abi.TEXTURE_OP_DTYPE,
 }
 
 
