@@ -419,8 +419,18 @@ typedef struct sw_texture_op
 int sw_apply_texture_ops(
 	sw_ctx* ctx, sw_map* map, const sw_texture_op* ops, uint32_t num_ops, const int32_t* turb_shifts, uint32_t num_turb_shifts);
 
+/* make_skybox_side_texture{,64} + make_skybox_side_texture_mips (textures.rs:183-286): a skybox
+ * side from a device image (Image: Color32 pixels for color_bits 32; Image64: Color64 for 64; row
+ * pitch image_width) — the top-left min(w, h) square scaled by trunc(brightness * 256) / 256 per
+ * channel (alpha 0), then its mips (size / 2 ... while at least 1, five levels at most) — written
+ * back to back into the map's skybox buffer of that colour size from element dst_offset.
+ * out_size: the side's size. Queued on the context's stream. */
+int sw_make_skybox_side(sw_ctx* ctx, sw_map* map, uint32_t color_bits, const void* image_device, uint32_t image_width,
+	uint32_t image_height, float brightness, uint64_t dst_offset, uint32_t* out_size);
+
 /* Copies map texels back to host memory (waits): which = 0 the Texture texels (sw_texel), 1 the
- * TextureLite texels (Color32); `count` elements from element `first`. */
+ * TextureLite texels (Color32), 2 the Color32 skybox texels, 3 the Color64 ones; `count` elements
+ * from element `first`. */
 int sw_map_read(sw_ctx* ctx, const sw_map* map, uint32_t which, uint64_t first, uint64_t count, void* dst);
 
 /* Postprocessor::update_exposure (postprocessor.rs:206-223): the next frame's exposure from this

@@ -2255,6 +2255,49 @@ void lite_layer(uint32_t* dst, const sw_texture_level& dl, const uint32_t* src, 
 	}
 }
 
+// make_skybox_side_texture{,64} and make_skybox_side_texture_mips (textures.rs:183-286)
+template <class CT>
+uint32_t skybox_side(const typename CT::T* image, uint32_t image_w, uint32_t image_h, float brightness, typename CT::T* out)
+{
+	const int32_t shift = 8;
+	const float scale = float(1 << shift);
+	const float bf = brightness * scale;
+	const int32_t brightness_i = bf != bf ? 0 : (bf <= -2147483648.0f ? INT32_MIN : (bf >= 2147483648.0f ? INT32_MAX : int32_t(bf)));
+	const int32_t bv[3] = {brightness_i, brightness_i, brightness_i};
+	const __m128i brightness_vec = ci_from_i32x3(bv);
+	const uint32_t size = std::min(image_w, image_h);
+	for(uint32_t y = 0; y < size; ++y)
+		for(uint32_t x = 0; x < size; ++x)
+		{
+			const __m128i c = CT::to_veci(image[x + image_w * y]);
+			out[x + y * size] = CT::from_veci(_mm_srai_epi32(_mm_mullo_epi32(c, brightness_vec), shift));
+		}
+	typename CT::T* prev = out;
+	uint32_t prev_size = size;
+	for(int i = 1; i < 5; ++i)
+	{
+		const uint32_t mip_size = prev_size >> 1;
+		if(mip_size == 0)
+			break;
+		typename CT::T* mip = prev + uint64_t(prev_size) * prev_size;
+		for(uint32_t y = 0; y < mip_size; ++y)
+		{
+			const uint64_t src_offset0 = uint64_t(y * 2) * prev_size, src_offset1 = uint64_t(y * 2 + 1) * prev_size;
+			for(uint32_t x = 0; x < mip_size; ++x)
+			{
+				const uint32_t src_x = x * 2;
+				const __m128i p00 = CT::to_veci(prev[src_x + src_offset0]), p01 = CT::to_veci(prev[src_x + src_offset1]);
+				const __m128i p10 = CT::to_veci(prev[src_x + 1 + src_offset0]), p11 = CT::to_veci(prev[src_x + 1 + src_offset1]);
+				mip[uint64_t(y) * mip_size + x] =
+					CT::from_veci(_mm_srai_epi32(_mm_add_epi32(_mm_add_epi32(p00, p01), _mm_add_epi32(p10, p11)), 2));
+			}
+		}
+		prev = mip;
+		prev_size = mip_size;
+	}
+	return size;
+}
+
 } // namespace
 
 // ---------------------------------------------------------------------------------------------
@@ -2408,6 +2451,14 @@ int oracle_apply_texture_ops(const sw_texture_level* levels, uint32_t num_levels
 			lite_layer(lite + dl.offset, dl, lite + sl.offset, sl, op.shift, op.light, op.blend);
 	}
 	return 0;
+}
+
+// A skybox side and its mips back to back into out (sized by the caller); returns the side's size.
+uint32_t oracle_make_skybox_side(uint32_t color_bits, const void* image, uint32_t image_w, uint32_t image_h, float brightness, void* out)
+{
+	if(color_bits == 64)
+		return skybox_side<C64>(static_cast<const uint64_t*>(image), image_w, image_h, brightness, static_cast<uint64_t*>(out));
+	return skybox_side<C32>(static_cast<const uint32_t*>(image), image_w, image_h, brightness, static_cast<uint32_t*>(out));
 }
 
 // Elementary operations, exposed for the closed-form / exhaustive tests.

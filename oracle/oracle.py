@@ -37,6 +37,8 @@ def load():
         lib.oracle_build_surfaces.restype = C.c_int
         lib.oracle_build_surfaces.argtypes = [C.c_int, C.POINTER(sw_abi.MapDesc), C.POINTER(sw_abi.Batch), C.c_void_p,
                                               C.POINTER(sw_abi.Stats)]
+        lib.oracle_make_skybox_side.restype = C.c_uint32
+        lib.oracle_make_skybox_side.argtypes = [C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_void_p]
         lib.oracle_apply_texture_ops.restype = C.c_int
         lib.oracle_apply_texture_ops.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
         lib.oracle_prepare_models.restype = C.c_int
@@ -168,3 +170,26 @@ def apply_texture_ops(map_data, ops, turb_shifts=None):
     if r != 0:
         raise ValueError("level out of range")
     return texels, lite
+
+
+def skybox_side_texels(size):
+    """Elements of a skybox side with its mips (five levels at most, down to 1x1)."""
+    n, s = 0, size
+    for _ in range(5):
+        if s == 0:
+            break
+        n += s * s
+        s >>= 1
+    return n
+
+
+def make_skybox_side(image, brightness):
+    """make_skybox_side_texture{,64} + mips (textures.rs:183-286) of a uint32 (Color32) or uint64
+    (Color64) image [h, w]. Returns the side and its mips back to back."""
+    lib = load()
+    image = np.ascontiguousarray(image)
+    h, w = image.shape
+    bits = 64 if image.dtype == np.uint64 else 32
+    out = np.zeros(skybox_side_texels(min(w, h)), dtype=image.dtype)
+    lib.oracle_make_skybox_side(bits, C.c_void_p(image.ctypes.data), w, h, C.c_float(brightness), C.c_void_p(out.ctypes.data))
+    return out

@@ -264,4 +264,7 @@ int pack_texture_ops(const sw_texture_op* ops, uint32_t num_ops, const sw_textur
 	uint32_t num_turb_shifts, std::vector<char>& packed, TexOpPlan& plan, std::string& error);
 void launch_texture_ops(hipStream_t st, const char* args_dev, const TexOpPlan& plan, const int32_t* turb_shifts_dev, sw_texel* texels,
 	uint32_t* lite);
+// A skybox side (mip 0 from the image, then up to four halvings) back to back at dst.
+void launch_skybox_side(hipStream_t st, uint32_t color_bits, const void* image, uint32_t image_w, uint32_t size, int32_t brightness,
+	void* dst);
 } // namespace swk

@@ -1332,20 +1332,50 @@ int sw_apply_texture_ops(
 	return hip_check(ctx, hipGetLastError(), "texture ops launch");
 }
 
+int sw_make_skybox_side(sw_ctx* ctx, sw_map* map, uint32_t color_bits, const void* image_device, uint32_t image_width,
+	uint32_t image_height, float brightness, uint64_t dst_offset, uint32_t* out_size)
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	if(!map || !image_device || (color_bits != 32 && color_bits != 64))
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument or colour size not 32 / 64");
+	const uint32_t size = std::min(image_width, image_height);
+	if(size == 0 || size > 65536)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "skybox image size out of range");
+	uint64_t total = 0;
+	for(uint32_t mip = 0, s = size; mip < 5 && s; ++mip, s >>= 1)
+		total += uint64_t(s) * s;
+	const uint64_t cap = color_bits == 64 ? map->num_sky64 : map->num_sky32;
+	if(dst_offset > cap || total > cap - dst_offset)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "skybox side beyond the map's skybox buffer");
+	// (brightness * 256) as i32: truncation, saturating, NaN -> 0
+	const float bf = brightness * 256.0f;
+	const int32_t bi = bf != bf ? 0 : (bf <= -2147483648.0f ? INT32_MIN : (bf >= 2147483648.0f ? INT32_MAX : int32_t(bf)));
+	(void)hipSetDevice(ctx->device);
+	char* dst = static_cast<char*>(color_bits == 64 ? map->sky64.p : map->sky32.p) + dst_offset * (color_bits / 8);
+	swk::launch_skybox_side(ctx->stream, color_bits, image_device, image_width, size, bi, dst);
+	if(out_size)
+		*out_size = size;
+	return hip_check(ctx, hipGetLastError(), "skybox launch");
+}
+
 int sw_map_read(sw_ctx* ctx, const sw_map* map, uint32_t which, uint64_t first, uint64_t count, void* dst)
 {
 	if(!ctx)
 		return SW_ERR_INVALID_ARGUMENT;
-	if(!map || (count && !dst) || which > 1)
+	if(!map || (count && !dst) || which > 3)
 		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument or unknown buffer");
-	const uint64_t n = which == 0 ? map->ptrs.num_texels : map->ptrs.num_lite;
-	const size_t elem = which == 0 ? sizeof(sw_texel) : sizeof(uint32_t);
+	const uint64_t ns[4] = {map->ptrs.num_texels, map->ptrs.num_lite, map->num_sky32, map->num_sky64};
+	const size_t es[4] = {sizeof(sw_texel), sizeof(uint32_t), sizeof(uint32_t), sizeof(uint64_t)};
+	const void* bufs[4] = {map->texels.p, map->lite.p, map->sky32.p, map->sky64.p};
+	const uint64_t n = ns[which];
+	const size_t elem = es[which];
 	if(first > n || count > n - first)
 		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "range beyond the map's texels");
 	if(count == 0)
 		return SW_OK;
 	(void)hipSetDevice(ctx->device);
-	const char* src = static_cast<const char*>(which == 0 ? map->texels.p : map->lite.p) + first * elem;
+	const char* src = static_cast<const char*>(bufs[which]) + first * elem;
 	int r;
 	if((r = hip_check(ctx, hipMemcpyAsync(dst, src, count * elem, hipMemcpyDeviceToHost, ctx->stream), "map read")))
 		return r;

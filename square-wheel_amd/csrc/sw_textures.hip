@@ -200,10 +200,61 @@ __global__ __launch_bounds__(TEXOP_BLOCK) void k_texture_ops(TexOpLaunch L)
 	}
 }
 
+// make_skybox_side_texture / make_skybox_side_texture64 (textures.rs:183-229): the top-left
+// size x size pixels of the image, (c * trunc(brightness * 256)) >> 8 per lane with the alpha lane
+// multiplied by 0, packed with unsigned saturation.
+template <class CT>
+__global__ __launch_bounds__(256) void k_skybox_mip0(
+	const typename CT::T* __restrict__ image, uint32_t image_w, uint32_t size, int32_t brightness, typename CT::T* __restrict__ dst)
+{
+	const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+	if(x >= size)
+		return;
+	const CI c = CT::to_ci(image[uint64_t(y) * image_w + x]);
+	dst[uint64_t(y) * size + x] =
+		CT::from_ci(CI{wmul(c.b, brightness) >> 8, wmul(c.g, brightness) >> 8, wmul(c.r, brightness) >> 8, wmul(c.a, 0) >> 8});
+}
+
+// make_skybox_side_texture_mips (textures.rs:231-286): (p00 + p01 + p10 + p11) >> 2 per lane
+template <class CT>
+__global__ __launch_bounds__(256) void k_skybox_mip(
+	const typename CT::T* __restrict__ prev, uint32_t ps, uint32_t size, typename CT::T* __restrict__ dst)
+{
+	// prev: the previous level, ps x ps (ps = 2 size or 2 size + 1)
+	const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+	if(x >= size)
+		return;
+	const CI p00 = CT::to_ci(prev[uint64_t(2 * y) * ps + 2 * x]), p01 = CT::to_ci(prev[uint64_t(2 * y + 1) * ps + 2 * x]);
+	const CI p10 = CT::to_ci(prev[uint64_t(2 * y) * ps + 2 * x + 1]), p11 = CT::to_ci(prev[uint64_t(2 * y + 1) * ps + 2 * x + 1]);
+	dst[uint64_t(y) * size + x] = CT::from_ci(ci_shr2(ci_add2(ci_add2(p00, p01), ci_add2(p10, p11)), 2));
+}
+
 } // namespace
 
 namespace swk
 {
+// A skybox side and its mips (size, size/2, ... down to 1, at most five levels) back to back at dst.
+void launch_skybox_side(hipStream_t st, uint32_t color_bits, const void* image, uint32_t image_w, uint32_t size, int32_t brightness,
+	void* dst)
+{
+	if(color_bits == 64)
+	{
+		uint64_t* d = static_cast<uint64_t*>(dst);
+		hipLaunchKernelGGL(k_skybox_mip0<Color64T>, dim3((size + 255) / 256, size), dim3(256), 0, st, static_cast<const uint64_t*>(image),
+			image_w, size, brightness, d);
+		for(uint32_t mip = 1, s = size; mip < 5 && (s >> 1); ++mip, d += uint64_t(s) * s, s >>= 1)
+			hipLaunchKernelGGL(k_skybox_mip<Color64T>, dim3(((s >> 1) + 255) / 256, s >> 1), dim3(256), 0, st, d, s, s >> 1, d + uint64_t(s) * s);
+	}
+	else
+	{
+		uint32_t* d = static_cast<uint32_t*>(dst);
+		hipLaunchKernelGGL(k_skybox_mip0<Color32T>, dim3((size + 255) / 256, size), dim3(256), 0, st, static_cast<const uint32_t*>(image),
+			image_w, size, brightness, d);
+		for(uint32_t mip = 1, s = size; mip < 5 && (s >> 1); ++mip, d += uint64_t(s) * s, s >>= 1)
+			hipLaunchKernelGGL(k_skybox_mip<Color32T>, dim3(((s >> 1) + 255) / 256, s >> 1), dim3(256), 0, st, d, s, s >> 1, d + uint64_t(s) * s);
+	}
+}
+
 // Validates the operations against the map's level table and packs the launch arguments (runs,
 // then operations at `ops_at`) into `packed`.
 int pack_texture_ops(const sw_texture_op* ops, uint32_t num_ops, const sw_texture_level* levels, uint32_t num_levels,

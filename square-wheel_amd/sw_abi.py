@@ -284,6 +284,8 @@ GPU_SYMBOLS = {
     "sw_update_exposure": (C.c_float, [C.c_void_p, C.c_float, C.POINTER(C.c_float), C.c_float]),
     "sw_apply_texture_ops": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]),
     "sw_map_read": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p]),
+    "sw_make_skybox_side": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float,
+                                      C.c_uint64, C.POINTER(C.c_uint32)]),
     "sw_prepare_models": (C.c_int, [C.c_void_p, C.POINTER(ModelMesh), C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
                                     C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p]),
     "sw_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
@@ -446,9 +448,24 @@ class Context:
         self.check(self.lib.sw_apply_texture_ops(self.h, map_handle, _ptr(ops), len(ops), _ptr(sh) if len(sh) else None, len(sh)),
                    "sw_apply_texture_ops")
 
+    def make_skybox_side(self, map_handle, image, brightness, dst_offset):
+        """sw_make_skybox_side from a host image (uint32 Color32 or uint64 Color64 [h, w]), uploaded here."""
+        image = np.ascontiguousarray(image)
+        h, w = image.shape
+        buf = self.to_device(image)
+        size = C.c_uint32()
+        try:
+            self.check(self.lib.sw_make_skybox_side(self.h, map_handle, 64 if image.dtype == np.uint64 else 32, buf.ptr, w, h,
+                                                    float(brightness), int(dst_offset), C.byref(size)), "sw_make_skybox_side")
+            self.check(self.lib.sw_sync(self.h), "sw_sync")
+        finally:
+            buf.free()
+        return size.value
+
     def map_read(self, map_handle, which, first, count):
-        """sw_map_read: Texture texels (which 0, TEXEL_DTYPE) or TextureLite texels (1, uint32)."""
-        out = np.zeros(max(int(count), 1), dtype=TEXEL_DTYPE if which == 0 else np.uint32)
+        """sw_map_read: Texture texels (which 0, TEXEL_DTYPE), TextureLite texels (1, uint32), skybox
+        Color32 (2) or Color64 (3) texels."""
+        out = np.zeros(max(int(count), 1), dtype={0: TEXEL_DTYPE, 1: np.uint32, 2: np.uint32, 3: np.uint64}[which])
         self.check(self.lib.sw_map_read(self.h, map_handle, which, int(first), int(count), _ptr(out)), "sw_map_read")
         return out[:count]
 

@@ -133,3 +133,27 @@ def test_texture_op_validation(gpu_ctx, tmap, case):
             gpu_ctx.apply_texture_ops(mh, ops, np.zeros(64, np.int32))
     finally:
         gpu_ctx.release_map(mh)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_skybox_side_bit_exact(gpu_ctx, tmap, bits):
+    """sw_make_skybox_side into the map's skybox buffer: the side and its mips equal the oracle's, and
+    the texels around it are untouched."""
+    rng = np.random.default_rng(40 + bits)
+    dt, lane = (np.uint32, 8) if bits == 32 else (np.uint64, 16)
+    img = rng.integers(0, 2 ** 63 if bits == 64 else 2 ** 32, size=(100, 70), dtype=np.uint64).astype(dt)
+    mh = gpu_ctx.upload_map(tmap.data)
+    try:
+        which = 2 if bits == 32 else 3
+        before = gpu_ctx.map_read(mh, which, 0, len(tmap.data.sky32 if bits == 32 else tmap.data.sky64))
+        off = 1234
+        size = gpu_ctx.make_skybox_side(mh, img, 2.3, off)
+        assert size == 70
+        ref = oracle.make_skybox_side(img, 2.3)
+        after = gpu_ctx.map_read(mh, which, 0, len(before))
+        assert np.array_equal(after[off:off + len(ref)], ref)
+        assert np.array_equal(after[:off], before[:off]) and np.array_equal(after[off + len(ref):], before[off + len(ref):])
+        with pytest.raises(A.SwError):
+            gpu_ctx.make_skybox_side(mh, img, 1.0, len(before) - 10)
+    finally:
+        gpu_ctx.release_map(mh)

@@ -204,3 +204,32 @@ def test_single_argument_functions():
     assert f(4, [2.0, 0.0, 1.0, 0.0], 0.25) == 0.0 and f(4, [1.0, 0.0, 3.0, 1.0], 0.0) == -2.0
     assert f(5, [1.0, 0.0, 2.0, 0.0], 0.3) == 2.0 and f(5, [1.0, 0.0, 2.0, 0.0], 0.7) == -2.0
     assert sw_host.layer_shift(-37.9, 2) == -10 and sw_host.layer_shift(1e10, 0) == 2 ** 31 - 1
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_skybox_side_closed_forms(bits):
+    """make_skybox_side_texture{,64}: the top-left square of the image, (c * trunc(b * 256)) >> 8 per
+    channel, alpha 0, unsigned saturation; four mips average 2x2 blocks ((sum) >> 2)."""
+    rng = np.random.default_rng(30 + bits)
+    lane_max = 255 if bits == 32 else 65535
+    dt, sh = (np.uint32, 8) if bits == 32 else (np.uint64, 16)
+    lanes = rng.integers(0, lane_max + 1, size=(4, 40, 35)).astype(np.uint64)  # odd: mips of 35, 17, 8, 4
+    img = (lanes[0] | (lanes[1] << np.uint64(sh)) | (lanes[2] << np.uint64(2 * sh)) | (lanes[3] << np.uint64(3 * sh))).astype(dt)
+    for brightness in (1.0, 0.5, 1.7, 3.999):
+        out = oracle.make_skybox_side(img, brightness)
+        assert len(out) == 35 * 35 + 17 * 17 + 8 * 8 + 4 * 4 + 2 * 2
+        bi = int(np.float32(brightness) * np.float32(256))
+        mip0 = out[:35 * 35].reshape(35, 35).astype(np.uint64)
+        for k in range(3):
+            ref = np.minimum((lanes[k][:35, :35].astype(np.int64) * bi) >> 8, lane_max)
+            assert np.array_equal((mip0 >> np.uint64(k * sh)) & np.uint64(lane_max), ref.astype(np.uint64))
+        assert np.all(mip0 >> np.uint64(3 * sh) == 0)
+        prev, size, off = mip0, 35, 35 * 35
+        for _ in range(4):  # five levels in all; an odd level's last row and column are dropped
+            size //= 2
+            mip = out[off:off + size * size].reshape(size, size).astype(np.uint64)
+            for k in range(3):
+                p = ((prev >> np.uint64(k * sh)) & np.uint64(lane_max)).astype(np.int64)[:2 * size, :2 * size]
+                ref = (p[0::2, 0::2] + p[1::2, 0::2] + p[0::2, 1::2] + p[1::2, 1::2]) >> 2
+                assert np.array_equal((mip >> np.uint64(k * sh)) & np.uint64(lane_max), ref.astype(np.uint64))
+            prev, off = mip, off + size * size
