@@ -390,6 +390,30 @@ typedef struct sw_exposure_params
 } sw_exposure_params;
 
 /* ---------------------------------------------------------------------------------------------
+ * Shadow maps of dynamic lights (light.rs:4-60, 185-294)
+ * ------------------------------------------------------------------------------------------- */
+#define SW_SHADOW_NONE 0u
+#define SW_SHADOW_CUBE 1u /* six size x size sides back to back, in CubeMapSide order (+x, -x, +y, -y, +z, -z) */
+#define SW_SHADOW_PROJECTOR 2u /* one size x size map seen along basis_z */
+
+/* A light's ShadowMap: depth values (1 / z, larger is nearer) in a device float buffer. */
+typedef struct sw_light_shadow
+{
+	uint32_t kind; /* SW_SHADOW_* */
+	uint32_t size;
+	uint64_t offset; /* element offset of the map (cube: of side 0) in the shadow texel buffer */
+	float basis_x[3], basis_y[3], basis_z[3]; /* projector */
+	uint32_t reserved;
+} sw_light_shadow;
+
+/* The shadow maps the surface builder samples from the next launch on (until called again; num 0
+ * turns them off): shadows[i] belongs to light i of each frame's light list (lights past num cast
+ * no shadow); shadow_texels_device holds num_shadow_texels floats (e.g. filled by sw_render_depth).
+ * get_light_shadow_factor (light.rs:185-294) becomes a 0/1 factor of the light's distance term. */
+int sw_set_light_shadows(
+	sw_ctx* ctx, const sw_light_shadow* shadows, uint32_t num, const float* shadow_texels_device, uint64_t num_shadow_texels);
+
+/* ---------------------------------------------------------------------------------------------
  * Per-frame texture animation (the map materials processor's generative effects) on the map's
  * resident textures: the caller (the materials processor) evaluates the effect's time functions
  * and hands over one operation per mip level.

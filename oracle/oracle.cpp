@@ -99,6 +99,65 @@ inline Vec3 vec3_scalar_mul_add(Vec3 a, float s, Vec3 b)
 }
 inline float vec3_len2(Vec3 v) { return vec3_dot(v, v); }
 
+// Shadow maps (light.rs:185-294), set by oracle_set_light_shadows for the following calls.
+const sw_light_shadow* g_shadows = nullptr;
+uint32_t g_num_shadows = 0;
+const float* g_shadow_texels = nullptr;
+
+// cube_shadow_map_side_fetch (light.rs:234-257)
+float cube_shadow_map_side_fetch(const Numerics& nm, const sw_light_shadow& sm, Vec3 vec, uint32_t side)
+{
+	const float ONE_MINUS_EPS = 1.0f - 1.0f / 65536.0f;
+	const float cubemap_size_f = float(sm.size);
+	const float depth = nm.inv_fast(fmax_r(vec.z, MIN_POSITIVE_VALUE));
+	const float half_depth = 0.5f * depth;
+	const float u_f = fmin_r(fmax_r(f32_mul_add(vec.x, half_depth, 0.5f), 0.0f), ONE_MINUS_EPS) * cubemap_size_f;
+	const float v_f = fmin_r(fmax_r(f32_mul_add(vec.y, half_depth, 0.5f), 0.0f), ONE_MINUS_EPS) * cubemap_size_f;
+	const uint32_t u = uint32_t(u_f), v = uint32_t(v_f);
+	const float value = g_shadow_texels[sm.offset + uint64_t(side) * sm.size * sm.size + u + uint64_t(v) * sm.size];
+	return depth >= value ? 1.0f : 0.0f;
+}
+
+// get_light_shadow_factor (light.rs:185-193) with cube_shadow_map_fetch (:195-231) and
+// projector_shadow_map_fetch (:259-294)
+float get_light_shadow_factor(const Numerics& nm, uint32_t light_index, Vec3 vec)
+{
+	if(light_index >= g_num_shadows)
+		return 1.0f;
+	const sw_light_shadow& sm = g_shadows[light_index];
+	if(sm.kind == SW_SHADOW_CUBE)
+	{
+		const Vec3 a = v3(std::fabs(vec.x), std::fabs(vec.y), std::fabs(vec.z));
+		if(a.x >= a.y && a.x >= a.z)
+			return vec.x >= 0.0f ? cube_shadow_map_side_fetch(nm, sm, v3(-vec.y, vec.z, a.x), 1)
+								 : cube_shadow_map_side_fetch(nm, sm, v3(vec.y, vec.z, a.x), 0);
+		if(a.y >= a.x && a.y >= a.z)
+			return vec.y >= 0.0f ? cube_shadow_map_side_fetch(nm, sm, v3(vec.x, vec.z, a.y), 3)
+								 : cube_shadow_map_side_fetch(nm, sm, v3(-vec.x, vec.z, a.y), 2);
+		return vec.z >= 0.0f ? cube_shadow_map_side_fetch(nm, sm, v3(-vec.x, vec.y, a.z), 5)
+							 : cube_shadow_map_side_fetch(nm, sm, v3(-vec.x, -vec.y, a.z), 4);
+	}
+	if(sm.kind == SW_SHADOW_PROJECTOR)
+	{
+		const Vec3 bx = v3(sm.basis_x[0], sm.basis_x[1], sm.basis_x[2]), by = v3(sm.basis_y[0], sm.basis_y[1], sm.basis_y[2]),
+				   bz = v3(sm.basis_z[0], sm.basis_z[1], sm.basis_z[2]);
+		const float z = vec3_dot(bz, vec);
+		if(z <= 0.0f)
+			return 0.0f;
+		const float depth = nm.inv_fast(z);
+		const float half_depth = 0.5f * depth;
+		const float size_f = float(sm.size);
+		const float u_f = f32_mul_add(vec3_dot(bx, vec), half_depth, 0.5f) * size_f;
+		const float v_f = f32_mul_add(vec3_dot(by, vec), half_depth, 0.5f) * size_f;
+		if(u_f < 0.0f || v_f < 0.0f || u_f >= size_f || v_f >= size_f)
+			return 0.0f;
+		const float value = g_shadow_texels[sm.offset + uint32_t(u_f) + uint64_t(uint32_t(v_f)) * sm.size];
+		return depth >= value ? 1.0f : 0.0f;
+	}
+	return 1.0f;
+}
+
+
 // ColorVec (fast_math.rs:98-248): lanes [B, G, R, A] of an __m128.
 inline __m128 cv_from_color32(uint32_t c) { return _mm_cvtepi32_ps(_mm_cvtepu8_epi32(_mm_cvtsi32_si128(int(c)))); }
 inline __m128 cv_from_color64(uint64_t c)
@@ -602,10 +661,11 @@ void build_surface(
 				for(uint32_t li = 0; li < num_lights; ++li)
 				{
 					// :750-830
-					const sw_light& light = lights[light_indices[s.first_light + li]];
+					const uint32_t light_index = light_indices[s.first_light + li];
+					const sw_light& light = lights[light_index];
 					const Vec3 light_pos = v3(light.position[0], light.position[1], light.position[2]);
 					const Vec3 vec_to_light = light_pos - pos;
-					const float shadow_factor = 1.0f; // ShadowMap::None (light.rs:185-294)
+					const float shadow_factor = get_light_shadow_factor(nm, light_index, vec_to_light);
 					const float vec_to_light_len2 = fmax_r(vec3_len2(vec_to_light), MIN_POSITIVE_VALUE);
 					const float distance_factor = fmax_r(nm.inv_fast(vec_to_light_len2) - light.inv_square_radius, 0.0f);
 					const float shadow_distance_factor = shadow_factor * distance_factor;
@@ -2423,6 +2483,30 @@ int oracle_prepare_models(int mode, const sw_model_mesh* meshes, uint32_t num_me
 		out_counts[i] = reject_and_sort(mesh, verts, out_triangles + uint64_t(in.first_triangle) * 3, ok);
 	}
 	return ok ? 0 : 1;
+}
+
+// Shadow maps for the following oracle calls (arrays stay owned by the caller); num 0: none.
+void oracle_set_light_shadows(const sw_light_shadow* shadows, uint32_t num, const float* texels)
+{
+	g_shadows = shadows;
+	g_num_shadows = num;
+	g_shadow_texels = texels;
+}
+
+// get_light_shadow_factor of one map for a light-relative vector (for the tests).
+float oracle_shadow_factor(int mode, const sw_light_shadow* shadow, const float* texels, const float vec[3])
+{
+	const sw_light_shadow* s0 = g_shadows;
+	const uint32_t n0 = g_num_shadows;
+	const float* t0 = g_shadow_texels;
+	g_shadows = shadow;
+	g_num_shadows = 1;
+	g_shadow_texels = texels;
+	const float r = get_light_shadow_factor(Numerics{mode}, 0, v3(vec[0], vec[1], vec[2]));
+	g_shadows = s0;
+	g_num_shadows = n0;
+	g_shadow_texels = t0;
+	return r;
 }
 
 // Texture operations on host copies of the map's texel buffers, in order (the levels as in

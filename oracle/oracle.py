@@ -37,6 +37,10 @@ def load():
         lib.oracle_build_surfaces.restype = C.c_int
         lib.oracle_build_surfaces.argtypes = [C.c_int, C.POINTER(sw_abi.MapDesc), C.POINTER(sw_abi.Batch), C.c_void_p,
                                               C.POINTER(sw_abi.Stats)]
+        lib.oracle_set_light_shadows.restype = None
+        lib.oracle_set_light_shadows.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+        lib.oracle_shadow_factor.restype = C.c_float
+        lib.oracle_shadow_factor.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]
         lib.oracle_make_skybox_side.restype = C.c_uint32
         lib.oracle_make_skybox_side.argtypes = [C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_void_p]
         lib.oracle_apply_texture_ops.restype = C.c_int
@@ -193,3 +197,29 @@ def make_skybox_side(image, brightness):
     out = np.zeros(skybox_side_texels(min(w, h)), dtype=image.dtype)
     lib.oracle_make_skybox_side(bits, C.c_void_p(image.ctypes.data), w, h, C.c_float(brightness), C.c_void_p(out.ctypes.data))
     return out
+
+
+_shadow_refs = None
+
+
+def set_light_shadows(shadows, texels):
+    """Shadow maps for the following oracle calls (oracle_set_light_shadows); None: none."""
+    global _shadow_refs
+    lib = load()
+    if shadows is None:
+        _shadow_refs = None
+        lib.oracle_set_light_shadows(None, 0, None)
+        return
+    sh = np.ascontiguousarray(shadows, dtype=sw_abi.LIGHT_SHADOW_DTYPE)
+    tx = np.ascontiguousarray(texels, dtype=np.float32)
+    _shadow_refs = (sh, tx)
+    lib.oracle_set_light_shadows(C.c_void_p(sh.ctypes.data), len(sh), C.c_void_p(tx.ctypes.data))
+
+
+def shadow_factor(shadow, texels, vec, mode=MODE_EXACT):
+    """get_light_shadow_factor (light.rs:185-294) for one shadow map and light-relative vector."""
+    lib = load()
+    sh = np.ascontiguousarray(np.atleast_1d(shadow), dtype=sw_abi.LIGHT_SHADOW_DTYPE)
+    tx = np.ascontiguousarray(texels, dtype=np.float32)
+    v = (C.c_float * 3)(*[float(x) for x in vec])
+    return float(lib.oracle_shadow_factor(mode, C.c_void_p(sh.ctypes.data), C.c_void_p(tx.ctypes.data), v))

@@ -190,11 +190,13 @@ struct SurfaceArgs
 	const sw_light* lights;
 	const uint32_t* light_indices;
 	uint32_t num_light_indices;
-	uint32_t pad;
+	uint32_t num_shadows; // lights [0, num_shadows) have an entry in `shadows`
 	void* arena;
 	uint64_t arena_texels;
 	uint32_t* counters; // error flags (validation of descriptors without a prepared table)
 	uint32_t* host_err; // HOST_ERR_WORDS error words in host memory
+	const sw_light_shadow* shadows; // validated on the host: every map lies inside shadow_texels
+	const float* shadow_texels;
 };
 
 // Bloom postprocessing (postprocessor.rs:258-956): the frame's downscaled, blurred copy added to it

@@ -396,12 +396,67 @@ SW_DEV void wrap_texel(bool slow, int32_t tcu, int32_t tcv, uint32_t tw, uint32_
 	}
 }
 
+// cube_shadow_map_fetch / cube_shadow_map_side_fetch (light.rs:195-257) and
+// projector_shadow_map_fetch (:259-294): true when the point at -vec from the light is lit (its
+// 1 / z is at least the map's). The map lies inside the shadow texels (checked on the host).
+SW_DEV bool shadow_lit(const sw_light_shadow& S, const float* __restrict__ texels, V3 vec)
+{
+	if(S.kind == SW_SHADOW_CUBE)
+	{
+		const V3 a = v3(__builtin_fabsf(vec.x), __builtin_fabsf(vec.y), __builtin_fabsf(vec.z));
+		V3 v;
+		uint32_t side;
+		if(a.x >= a.y && a.x >= a.z)
+		{
+			v = vec.x >= 0.0f ? v3(-vec.y, vec.z, a.x) : v3(vec.y, vec.z, a.x);
+			side = vec.x >= 0.0f ? 1u : 0u;
+		}
+		else if(a.y >= a.x && a.y >= a.z)
+		{
+			v = vec.y >= 0.0f ? v3(vec.x, vec.z, a.y) : v3(-vec.x, vec.z, a.y);
+			side = vec.y >= 0.0f ? 3u : 2u;
+		}
+		else
+		{
+			v = vec.z >= 0.0f ? v3(-vec.x, vec.y, a.z) : v3(-vec.x, -vec.y, a.z);
+			side = vec.z >= 0.0f ? 5u : 4u;
+		}
+		const float one_minus_eps = 1.0f - 1.0f / 65536.0f;
+		const float size_f = float(S.size);
+		const float depth = inv_fast(fmax_r(v.z, MIN_POSITIVE_VALUE));
+		const float half_depth = 0.5f * depth;
+		const float u_f = fmin_r(fmax_r(mul_add(v.x, half_depth, 0.5f), 0.0f), one_minus_eps) * size_f;
+		const float v_f = fmin_r(fmax_r(mul_add(v.y, half_depth, 0.5f), 0.0f), one_minus_eps) * size_f;
+		const uint32_t u = uint32_t(u_f), w = uint32_t(v_f);
+		const uint64_t side_texels = uint64_t(S.size) * S.size;
+		return depth >= texels[S.offset + side * side_texels + uint64_t(w) * S.size + u];
+	}
+	if(S.kind == SW_SHADOW_PROJECTOR)
+	{
+		const V3 bx = v3(S.basis_x[0], S.basis_x[1], S.basis_x[2]), by = v3(S.basis_y[0], S.basis_y[1], S.basis_y[2]),
+				 bz = v3(S.basis_z[0], S.basis_z[1], S.basis_z[2]);
+		const float z = dot(bz, vec);
+		if(z <= 0.0f)
+			return false;
+		const float depth = inv_fast(z);
+		const float half_depth = 0.5f * depth;
+		const float size_f = float(S.size);
+		const float u_f = mul_add(dot(bx, vec), half_depth, 0.5f) * size_f;
+		const float v_f = mul_add(dot(by, vec), half_depth, 0.5f) * size_f;
+		if(!(u_f >= 0.0f && v_f >= 0.0f && u_f < size_f && v_f < size_f))
+			return false; // outside (a NaN coordinate too)
+		return depth >= texels[S.offset + uint64_t(uint32_t(v_f)) * S.size + uint32_t(u_f)];
+	}
+	return true;
+}
+
 // One texel of build_surface_impl_6_static_params (surfaces.rs:433-852), specialisations as flags
-// that are uniform per surface.
-template <class CT>
+// that are uniform per surface. SH: lights may have shadow maps (a separate instantiation, so the
+// kernel without them keeps its registers).
+template <class CT, bool SH>
 SW_DEV typename CT::T surface_texel(const MapPtrs& m, const SurfRec& R, const sw_light* __restrict__ lights,
 	const uint32_t* __restrict__ light_indices, uint32_t num_light_indices, uint32_t num_lights_total, uint32_t dst_u,
-	uint32_t dst_v)
+	uint32_t dst_v, const SurfaceArgs& a)
 {
 	const uint32_t fl = R.flags;
 	const bool use_normal_map = (fl & REC_NORMAL_MAP) != 0;
@@ -545,7 +600,10 @@ SW_DEV typename CT::T surface_texel(const MapPtrs& m, const SurfRec& R, const sw
 			const V3 to_light = sub(v3(L.position[0], L.position[1], L.position[2]), pos);
 			const float tl2 = fmax_r(len2(to_light), MIN_POSITIVE_VALUE);
 			const float dist_f = fmax_r(inv_fast(tl2) - L.inv_square_radius, 0.0f);
-			const float sdf = 1.0f * dist_f; // shadow factor 1: ShadowMap::None
+			// shadow factor 1 (ShadowMap::None or lit) or 0; dist_f is finite and >= 0, so 0 * dist_f is +0
+			float sdf = 1.0f * dist_f;
+			if(SH && lidx < a.num_shadows && !shadow_lit(a.shadows[lidx], a.shadow_texels, to_light))
+				sdf = 0.0f;
 			const float di = spec == 2 ? 0.0f : fmax_r(dot(normal, to_light) * inv_sqrt_fast(tl2), 0.0f);
 			float si = 0.0f;
 			if(spec != 0)
@@ -616,7 +674,7 @@ SW_DEV typename CT::T emissive_texel(const MapPtrs& m, const SurfRec& R, typenam
 // the records of up to SURF_CHUNK consecutive surfaces overlapping the block (one lane each), then
 // every thread finds its texels' surfaces among them by a forward scan; a block overlapping more
 // surfaces takes further passes.
-template <class CT>
+template <class CT, bool SH>
 SW_DEV void build_surfaces_block(uint32_t block, const MapPtrs& m, const SurfaceArgs& a, const ViewsArg& views)
 {
 	__shared__ SurfRec recs[SURF_CHUNK];
@@ -729,7 +787,7 @@ SW_DEV void build_surfaces_block(uint32_t block, const MapPtrs& m, const Surface
 			const uint32_t dv = div_rcp(uint32_t(idx - R.start), R.w, R.inv_w, du);
 			typename CT::T out = 0;
 			if(R.flags & REC_DIFFUSE)
-				out = surface_texel<CT>(m, R, a.lights, a.light_indices, a.num_light_indices, a.num_lights, du, dv);
+				out = surface_texel<CT, SH>(m, R, a.lights, a.light_indices, a.num_light_indices, a.num_lights, du, dv, a);
 			else if(((R.flags >> REC_BLEND_SHIFT) & 15u) != SW_BLEND_NONE)
 				out = arena[idx]; // an emissive layer without a diffuse layer blends over what the arena holds
 			if(R.flags & REC_EMISSIVE)
@@ -745,13 +803,13 @@ SW_DEV void build_surfaces_block(uint32_t block, const MapPtrs& m, const Surface
 
 // Held to 80 VGPRs (six waves per SIMD): the kernel is bound by arithmetic and load latency, and
 // the extra waves hide more of the latter than the few spilled values cost.
-template <class CT>
+template <class CT, bool SH>
 __global__ __launch_bounds__(SURF_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_build_surfaces(MapPtrs m, SurfaceArgs a, const ViewsArg views,
 	const uint32_t* __restrict__ counters)
 {
 	if(counters[COUNTER_ERROR] & ERROR_BAD_SURFACE)
 		return; // the host reports the invalid descriptor; nothing is written
-	build_surfaces_block<CT>(blockIdx.x, m, a, views);
+	build_surfaces_block<CT, SH>(blockIdx.x, m, a, views);
 }
 
 // =============================================================================================
@@ -1649,7 +1707,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
 	if(blockIdx.x < a.setup_blocks)
 		setup_draws_block<LANES>(blockIdx.x, a, views);
 	else
-		build_surfaces_block<CT>(blockIdx.x - a.setup_blocks, m, sa, views);
+		build_surfaces_block<CT, false>(blockIdx.x - a.setup_blocks, m, sa, views);
 }
 
 // One block after the raster kernel: sums the per-draw statistics into the result block and clears
@@ -2438,10 +2496,21 @@ void launch_build_surfaces(hipStream_t st, uint32_t color_bits, const MapPtrs& m
 		return;
 	const uint64_t bt = surf_block_texels(sa.arena_texels);
 	const uint64_t blocks = (sa.arena_texels + bt - 1) / bt;
+	const bool sh = sa.num_shadows != 0;
 	if(color_bits == 64)
-		hipLaunchKernelGGL(k_build_surfaces<Color64T>, dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, sa, views, counters);
+	{
+		if(sh)
+			hipLaunchKernelGGL((k_build_surfaces<Color64T, true>), dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, sa, views, counters);
+		else
+			hipLaunchKernelGGL((k_build_surfaces<Color64T, false>), dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, sa, views, counters);
+	}
 	else
-		hipLaunchKernelGGL(k_build_surfaces<Color32T>, dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, sa, views, counters);
+	{
+		if(sh)
+			hipLaunchKernelGGL((k_build_surfaces<Color32T, true>), dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, sa, views, counters);
+		else
+			hipLaunchKernelGGL((k_build_surfaces<Color32T, false>), dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, sa, views, counters);
+	}
 }
 
 template <class CT, bool SSPAN>

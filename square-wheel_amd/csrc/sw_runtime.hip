@@ -66,7 +66,9 @@ struct sw_ctx
 
 	// command buffers / work buffers
 	DevBuf views, surfaces, lights, light_indices, draws, vdata, surf_ext, blk_surf, parts, spans, drec, dbounds, counters, arena, fb,
-		tonemap_out, tonemap_sums, bloom0, bloom1, bloom_lines, model_args, model_scratch, texop_args;
+		tonemap_out, tonemap_sums, bloom0, bloom1, bloom_lines, model_args, model_scratch, texop_args, shadows;
+	uint32_t num_shadows = 0; // sw_set_light_shadows: entries in `shadows`
+	const float* shadow_texels = nullptr;
 	uint32_t tile_rows = 8; // raster tile height (SW_TILE_ROWS_*), set by sw_set_tile_rows / the autotuner
 	bool scalar_spans = true; // raster span records through the scalar unit (chosen by the autotuner)
 	uint32_t* counters_host = nullptr; // pinned, device-mapped: result block (k_finish), then the error words
@@ -450,9 +452,25 @@ int launch_stages(sw_ctx* ctx, const sw_ctx::Launch& L, bool surfaces, bool rast
 	sa.num_light_indices = L.num_light_indices;
 	sa.arena = L.arena;
 	sa.arena_texels = L.arena_texels;
+	sa.num_shadows = ctx->num_shadows;
+	sa.shadows = static_cast<const sw_light_shadow*>(ctx->shadows.p);
+	sa.shadow_texels = ctx->shadow_texels;
 	// a block per draw while the draws alone cannot fill the chip, a wave per draw beyond
 	const uint32_t lanes = nd <= 4096 ? 256u : 64u;
-	if(surfaces && L.surface_mode != SURF_TABLE_DEVICE)
+	if(surfaces && L.surface_mode != SURF_TABLE_DEVICE && ctx->num_shadows)
+	{
+		// shadowed lights: the surface kernel's shadow instantiation runs on its own after the
+		// setup (whose table blocks are not needed: the host prepared the table, or the surface
+		// blocks search)
+		{
+			StageTimer t(ctx, STAGE_SETUP);
+			swk::launch_setup(ctx->stream, lanes, a, ctx->views_arg, L.surfaces, 0, static_cast<SurfExt*>(ctx->surf_ext.p),
+				static_cast<uint32_t*>(ctx->blk_surf.p), map->ptrs);
+		}
+		StageTimer t(ctx, STAGE_SURF);
+		swk::launch_build_surfaces(ctx->stream, L.color_bits, map->ptrs, sa, ctx->views_arg, counters);
+	}
+	else if(surfaces && L.surface_mode != SURF_TABLE_DEVICE)
 	{
 		StageTimer t(ctx, STAGE_SETUP_SURF);
 		swk::launch_setup_and_surfaces(ctx->stream, lanes, L.color_bits, a, ctx->views_arg, sa, map->ptrs);
@@ -779,7 +797,7 @@ void sw_ctx_destroy(sw_ctx* ctx)
 		(void)hipGraphDestroy(ctx->graph);
 	DevBuf* bufs[] = {&ctx->views, &ctx->surfaces, &ctx->lights, &ctx->light_indices, &ctx->draws, &ctx->vdata, &ctx->surf_ext, &ctx->blk_surf,
 		&ctx->parts, &ctx->spans, &ctx->drec, &ctx->dbounds, &ctx->counters, &ctx->arena, &ctx->fb, &ctx->tonemap_out, &ctx->tonemap_sums,
-		&ctx->bloom0, &ctx->bloom1, &ctx->bloom_lines, &ctx->model_args, &ctx->model_scratch, &ctx->texop_args};
+		&ctx->bloom0, &ctx->bloom1, &ctx->bloom_lines, &ctx->model_args, &ctx->model_scratch, &ctx->texop_args, &ctx->shadows};
 	for(DevBuf* b : bufs)
 		release(ctx, *b);
 	if(ctx->counters_host)
@@ -1330,6 +1348,48 @@ int sw_apply_texture_ops(
 			static_cast<sw_texel*>(map->texels.p), static_cast<uint32_t*>(map->lite.p));
 	}
 	return hip_check(ctx, hipGetLastError(), "texture ops launch");
+}
+
+int sw_set_light_shadows(
+	sw_ctx* ctx, const sw_light_shadow* shadows, uint32_t num, const float* shadow_texels_device, uint64_t num_shadow_texels)
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	if(num == 0)
+	{
+		if(ctx->num_shadows)
+			ctx->alloc_epoch++;
+		ctx->num_shadows = 0;
+		ctx->shadow_texels = nullptr;
+		return SW_OK;
+	}
+	if(!shadows || !shadow_texels_device)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	for(uint32_t i = 0; i < num; ++i)
+	{
+		const sw_light_shadow& sh = shadows[i];
+		if(sh.kind > SW_SHADOW_PROJECTOR)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "light shadow " + std::to_string(i) + ": unknown kind");
+		if(sh.kind == SW_SHADOW_NONE)
+			continue;
+		const uint64_t n = uint64_t(sh.size) * sh.size * (sh.kind == SW_SHADOW_CUBE ? 6u : 1u);
+		if(sh.size == 0 || sh.size > 65536 || sh.offset > num_shadow_texels || n > num_shadow_texels - sh.offset)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "light shadow " + std::to_string(i) + ": map outside the shadow texels");
+	}
+	(void)hipSetDevice(ctx->device);
+	int r;
+	if((r = ensure(ctx, ctx->shadows, sizeof(sw_light_shadow) * num)))
+		return r;
+	char* staging = nullptr;
+	if((r = stage_begin(ctx, sizeof(sw_light_shadow) * num, &staging)))
+		return r;
+	std::memcpy(staging, shadows, sizeof(sw_light_shadow) * num);
+	if((r = stage_copy(ctx, ctx->shadows.p, sizeof(sw_light_shadow) * num)))
+		return r;
+	ctx->num_shadows = num;
+	ctx->shadow_texels = shadow_texels_device;
+	ctx->alloc_epoch++; // a captured graph holds the previous shadow arguments
+	return SW_OK;
 }
 
 int sw_make_skybox_side(sw_ctx* ctx, sw_map* map, uint32_t color_bits, const void* image_device, uint32_t image_width,

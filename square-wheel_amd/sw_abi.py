@@ -62,6 +62,10 @@ MODEL_INSTANCE_DTYPE = np.dtype([
     ("tc_scale", "<f4", 2), ("tc_shift", "<f4", 2), ("light_cube", "<f4", (6, 3)), ("light_vector_scaled", "<f4", (2, 3)),
     ("light_color", "<f4", (2, 3)), ("first_vertex", "<u4"), ("first_triangle", "<u4")], align=True)
 MODEL_VERTEX_DTYPE = np.dtype([("pos", "<f4", 3), ("tc", "<f4", 2), ("light", "<f4", 3)], align=True)
+# shadow maps (sw_set_light_shadows)
+SHADOW_NONE, SHADOW_CUBE, SHADOW_PROJECTOR = 0, 1, 2
+LIGHT_SHADOW_DTYPE = np.dtype([("kind", "<u4"), ("size", "<u4"), ("offset", "<u8"), ("basis_x", "<f4", 3), ("basis_y", "<f4", 3),
+                               ("basis_z", "<f4", 3), ("reserved", "<u4")], align=True)
 # texture animation (sw_apply_texture_ops)
 TEXOP_TURB, TEXOP_LAYER, TEXOP_LAYER_LITE = 0, 1, 2
 TEXTURE_OP_DTYPE = np.dtype([("kind", "<u4"), ("dst_level", "<u4"), ("src_level", "<u4"), ("blend", "<u4"),
@@ -72,7 +76,7 @@ SKELETON_VERTEX_DTYPE = np.dtype([("tex_coord", "<f4", 2), ("position", "<f4", 3
 
 EXPECTED_SIZES = {"TEXEL": 8, "LEVEL": 24, "DIRLM": 40, "LIGHT": 32, "SURFACE": 136, "DRAW": 96, "VIEW": 64,
                   "MODEL_INSTANCE": 288, "MODEL_VERTEX": 32, "SKELETON_VERTEX": 40,
-                  "TEXTURE_OP": 40}
+                  "TEXTURE_OP": 40, "LIGHT_SHADOW": 56}
 for _name, _size in EXPECTED_SIZES.items():
     assert globals()[_name + "_DTYPE"].itemsize == _size, (_name, globals()[_name + "_DTYPE"].itemsize)
 
@@ -282,6 +286,7 @@ GPU_SYMBOLS = {
                                         C.POINTER(C.c_float), C.c_void_p, C.c_void_p, C.c_size_t,
                                         C.POINTER(C.c_float)]),
     "sw_update_exposure": (C.c_float, [C.c_void_p, C.c_float, C.POINTER(C.c_float), C.c_float]),
+    "sw_set_light_shadows": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]),
     "sw_apply_texture_ops": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]),
     "sw_map_read": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p]),
     "sw_make_skybox_side": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float,
@@ -439,6 +444,21 @@ class Context:
             src.free()
             dst.free()
         return out, np.array(avg[:], dtype=np.float32)
+
+    # shadow maps ----------------------------------------------------------------------------
+    def set_light_shadows(self, shadows, texels):
+        """sw_set_light_shadows with host arrays (LIGHT_SHADOW_DTYPE, float32 depths uploaded here);
+        None turns the shadows off."""
+        if getattr(self, "_shadow_buf", None) is not None:
+            self._shadow_buf.free()
+            self._shadow_buf = None
+        if shadows is None:
+            self.check(self.lib.sw_set_light_shadows(self.h, None, 0, None, 0), "sw_set_light_shadows")
+            return
+        sh = np.ascontiguousarray(shadows, dtype=LIGHT_SHADOW_DTYPE)
+        tx = np.ascontiguousarray(texels, dtype=np.float32)
+        self._shadow_buf = self.to_device(tx)
+        self.check(self.lib.sw_set_light_shadows(self.h, _ptr(sh), len(sh), self._shadow_buf.ptr, len(tx)), "sw_set_light_shadows")
 
     # texture animation --------------------------------------------------------------------
     def apply_texture_ops(self, map_handle, ops, turb_shifts=None):
