@@ -413,6 +413,33 @@ typedef struct sw_light_shadow
 int sw_set_light_shadows(
 	sw_ctx* ctx, const sw_light_shadow* shadows, uint32_t num, const float* shadow_texels_device, uint64_t num_shadow_texels);
 
+/* A polygon of a depth buffer (DepthRenderer::draw_polygon, depth_renderer.rs:199-287, drawn by
+ * DepthRasterizer::fill_polygon, rasterizer.rs:1295-1474): convex clockwise Fixed16 vertices
+ * (x, y pairs in the vertex data) and the DepthEquation of 1 / z. */
+typedef struct sw_depth_draw
+{
+	uint32_t first_vertex_int; /* index of the first vertex's x in the vertex data */
+	uint32_t num_vertices; /* 3 .. SW_MAX_POLYGON_VERTICES */
+	float depth_equation[3]; /* d_inv_z_dx, d_inv_z_dy, k */
+	uint32_t depth_test; /* 1: keep the larger 1 / z (submodels); 0: overwrite (back-to-front order) */
+} sw_depth_draw;
+
+/* A depth buffer (a cube side or a projector map) and its draws, which follow the previous
+ * target's in the draw array. */
+typedef struct sw_depth_target
+{
+	uint64_t offset; /* of the width x height buffer in the depth texels */
+	uint32_t width, height; /* at most 65535 */
+	uint32_t first_draw, num_draws;
+} sw_depth_target;
+
+/* Draws every target's polygons in order into its buffer (no clear: pixels no polygon covers keep
+ * their values, as the reference's reused shadow-map storage does). draws, vertex_data and targets
+ * are host arrays; depth_texels_device holds num_depth_texels floats (the shadow texels of
+ * sw_set_light_shadows). Queued on the context's stream. */
+int sw_render_depth(sw_ctx* ctx, const sw_depth_target* targets, uint32_t num_targets, const sw_depth_draw* draws, uint32_t num_draws,
+	const int32_t* vertex_data, uint64_t num_vertex_ints, float* depth_texels_device, uint64_t num_depth_texels);
+
 /* ---------------------------------------------------------------------------------------------
  * Per-frame texture animation (the map materials processor's generative effects) on the map's
  * resident textures: the caller (the materials processor) evaluates the effect's time functions
@@ -578,10 +605,10 @@ void sw_host_free(sw_ctx* ctx, void* host_ptr);
  * Stages: 0 setup_draws (edge walk, span records and, for device-resident descriptors, surface
  * extents), 1 build_surfaces, 2 raster_tiles, 3 tonemap, 4 setup_and_surfaces (draw setup and
  * surface texels in one launch, for host-resident descriptors), 5 bloom (downscale, blur and the
- * tonemap with bloom), 6 models (sw_prepare_models), 7 textures (sw_apply_texture_ops).
- * sw_profile_read returns the summed milliseconds and launch counts since the last reset (and
- * resets when reset != 0). Not recorded inside captured graphs. */
-#define SW_PROFILE_STAGES 8
+ * tonemap with bloom), 6 models (sw_prepare_models), 7 textures (sw_apply_texture_ops), 8 depth
+ * (sw_render_depth). sw_profile_read returns the summed milliseconds and launch counts since the
+ * last reset (and resets when reset != 0). Not recorded inside captured graphs. */
+#define SW_PROFILE_STAGES 9
 int sw_profile_enable(sw_ctx* ctx, uint32_t enable);
 int sw_profile_read(sw_ctx* ctx, double out_ms[SW_PROFILE_STAGES], uint64_t out_counts[SW_PROFILE_STAGES], uint32_t reset);
 

@@ -2358,6 +2358,96 @@ uint32_t skybox_side(const typename CT::T* image, uint32_t image_w, uint32_t ima
 	return size;
 }
 
+// ---------------------------------------------------------------------------------------------
+// DepthRasterizer (rasterizer.rs:1295-1474), loop for loop: the edge walk of fill_polygon and the
+// per-pixel inv_z += d_inv_z_dx / per-row line_inv_z += d_inv_z_dy stepping of fill_polygon_part.
+// ---------------------------------------------------------------------------------------------
+struct DepthRasterizer
+{
+	float* depth_buffer;
+	int32_t width, height;
+
+	void fill_polygon_part(bool depth_test, Fixed16 y_start, Fixed16 y_end, Fixed16 left_x_start, Fixed16 left_dx_dy,
+		Fixed16 right_x_start, Fixed16 right_dx_dy, const float eq[3])
+	{
+		const int32_t y_start_int = std::max(fixed16_round_to_int(y_start), 0);
+		const int32_t y_end_int = std::min(fixed16_round_to_int(y_end), height);
+		const Fixed16 y_start_delta = int_to_fixed16(y_start_int) + FIXED16_HALF - y_start;
+		Fixed16 x_left = left_x_start + fixed16_mul(y_start_delta, left_dx_dy) + FIXED16_HALF;
+		Fixed16 x_right = right_x_start + fixed16_mul(y_start_delta, right_dx_dy) + FIXED16_HALF;
+		float line_inv_z = float(y_start_int) * eq[1] + 0.5f * (eq[0] + eq[1]) + eq[2];
+		for(int32_t y_int = y_start_int; y_int < y_end_int; ++y_int)
+		{
+			const int32_t x_start_int = std::max(fixed16_floor_to_int(x_left), 0);
+			const int32_t x_end_int = std::min(fixed16_floor_to_int(x_right), width);
+			if(x_start_int < x_end_int)
+			{
+				float* line = depth_buffer + int64_t(y_int) * width;
+				float inv_z = line_inv_z + float(x_start_int) * eq[0];
+				for(int32_t x = x_start_int; x < x_end_int; ++x)
+				{
+					line[x] = depth_test ? fmax_r(inv_z, line[x]) : inv_z;
+					inv_z += eq[0];
+				}
+			}
+			x_left += left_dx_dy;
+			x_right += right_dx_dy;
+			line_inv_z += eq[1];
+		}
+	}
+
+	void fill_polygon(bool depth_test, const int32_t* v, uint32_t n, const float eq[3])
+	{
+		uint32_t lower = 0;
+		Fixed16 min_y = v[1];
+		for(uint32_t i = 0; i < n; ++i)
+			if(v[i * 2 + 1] < min_y)
+			{
+				min_y = v[i * 2 + 1];
+				lower = i;
+			}
+		uint32_t left = lower, right = lower;
+		Fixed16 cur_y = min_y;
+		for(;;)
+		{
+			uint32_t next_left = left + n - 1;
+			if(next_left >= n)
+				next_left -= n;
+			uint32_t next_right = right + 1;
+			if(next_right >= n)
+				next_right -= n;
+			const int32_t *lv = v + left * 2, *nlv = v + next_left * 2, *rv = v + right * 2, *nrv = v + next_right * 2;
+			const Fixed16 dy_left = nlv[1] - lv[1], dy_right = nrv[1] - rv[1];
+			const Fixed16 next_y = std::min(nlv[1], nrv[1]);
+			if(dy_left > FIXED16_HALF && dy_right > FIXED16_HALF)
+			{
+				const Fixed16 dx_dy_left = fixed16_div(nlv[0] - lv[0], dy_left);
+				const Fixed16 dx_dy_right = fixed16_div(nrv[0] - rv[0], dy_right);
+				fill_polygon_part(depth_test, cur_y, next_y, lv[0] + fixed16_mul(dx_dy_left, cur_y - lv[1]), dx_dy_left,
+					rv[0] + fixed16_mul(dx_dy_right, cur_y - rv[1]), dx_dy_right, eq);
+			}
+			else if(dy_left > 0 && dy_right > 0)
+			{
+				const int32_t cur_y_int = fixed16_round_to_int(cur_y), next_y_int = fixed16_round_to_int(next_y);
+				if(cur_y_int < next_y_int)
+				{
+					const Fixed16 thin_line_y = int_to_fixed16(cur_y_int) + FIXED16_HALF;
+					const Fixed16 x_start_left = lv[0] + fixed16_mul_div(thin_line_y - lv[1], nlv[0] - lv[0], dy_left);
+					const Fixed16 x_start_right = rv[0] + fixed16_mul_div(thin_line_y - rv[1], nrv[0] - rv[0], dy_right);
+					fill_polygon_part(depth_test, cur_y, next_y, x_start_left, 0, x_start_right, 0, eq);
+				}
+			}
+			if(next_left == next_right)
+				break;
+			if(v[next_right * 2 + 1] < v[next_left * 2 + 1])
+				right = next_right;
+			else
+				left = next_left;
+			cur_y = next_y;
+		}
+	}
+};
+
 } // namespace
 
 // ---------------------------------------------------------------------------------------------
@@ -2507,6 +2597,19 @@ float oracle_shadow_factor(int mode, const sw_light_shadow* shadow, const float*
 	g_num_shadows = n0;
 	g_shadow_texels = t0;
 	return r;
+}
+
+// Depth buffers: every target's draws in order (sw_render_depth's inputs, host memory).
+int oracle_render_depth(const sw_depth_target* targets, uint32_t num_targets, const sw_depth_draw* draws, const int32_t* vdata,
+	float* texels)
+{
+	for(uint32_t t = 0; t < num_targets; ++t)
+	{
+		DepthRasterizer r{texels + targets[t].offset, int32_t(targets[t].width), int32_t(targets[t].height)};
+		for(uint32_t k = targets[t].first_draw; k < targets[t].first_draw + targets[t].num_draws; ++k)
+			r.fill_polygon(draws[k].depth_test != 0, vdata + draws[k].first_vertex_int, draws[k].num_vertices, draws[k].depth_equation);
+	}
+	return 0;
 }
 
 // Texture operations on host copies of the map's texel buffers, in order (the levels as in

@@ -37,6 +37,8 @@ def load():
         lib.oracle_build_surfaces.restype = C.c_int
         lib.oracle_build_surfaces.argtypes = [C.c_int, C.POINTER(sw_abi.MapDesc), C.POINTER(sw_abi.Batch), C.c_void_p,
                                               C.POINTER(sw_abi.Stats)]
+        lib.oracle_render_depth.restype = C.c_int
+        lib.oracle_render_depth.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.oracle_set_light_shadows.restype = None
         lib.oracle_set_light_shadows.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
         lib.oracle_shadow_factor.restype = C.c_float
@@ -223,3 +225,16 @@ def shadow_factor(shadow, texels, vec, mode=MODE_EXACT):
     tx = np.ascontiguousarray(texels, dtype=np.float32)
     v = (C.c_float * 3)(*[float(x) for x in vec])
     return float(lib.oracle_shadow_factor(mode, C.c_void_p(sh.ctypes.data), C.c_void_p(tx.ctypes.data), v))
+
+
+def render_depth(targets, draws, vertex_data, texels):
+    """DepthRasterizer (rasterizer.rs:1295-1474) over every target's draws; returns a new texel array."""
+    lib = load()
+    tg = np.ascontiguousarray(targets, dtype=sw_abi.DEPTH_TARGET_DTYPE)
+    dr = np.ascontiguousarray(draws, dtype=sw_abi.DEPTH_DRAW_DTYPE)
+    vd = np.ascontiguousarray(vertex_data if len(vertex_data) else np.zeros(1, np.int32), dtype=np.int32)
+    tx = np.array(texels, dtype=np.float32, copy=True)
+    dr1 = dr if len(dr) else np.zeros(1, sw_abi.DEPTH_DRAW_DTYPE)
+    lib.oracle_render_depth(C.c_void_p(tg.ctypes.data), len(tg), C.c_void_p(dr1.ctypes.data), C.c_void_p(vd.ctypes.data),
+                            C.c_void_p(tx.ctypes.data))
+    return tx

@@ -50,7 +50,7 @@ def _run(cmd):
 
 
 def build_gpu():
-    srcs = [os.path.join(CSRC, "sw_kernels.hip"), os.path.join(CSRC, "sw_runtime.hip"), os.path.join(CSRC, "sw_models.hip"), os.path.join(CSRC, "sw_textures.hip"),
+    srcs = [os.path.join(CSRC, "sw_kernels.hip"), os.path.join(CSRC, "sw_runtime.hip"), os.path.join(CSRC, "sw_models.hip"), os.path.join(CSRC, "sw_textures.hip"), os.path.join(CSRC, "sw_depth.hip"),
             os.path.join(CSRC, "sw_server.cpp")]
     tmp = GPU_LIB + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-o", tmp, *srcs])

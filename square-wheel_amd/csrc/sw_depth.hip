@@ -1,0 +1,300 @@
+// The depth rasteriser of shadow maps (DepthRasterizer, rasterizer.rs:1295-1474): convex polygons
+// with a 1 / z equation drawn into f32 depth buffers, overwriting (back-to-front order) or keeping
+// the larger value (depth test).
+//
+// The reference steps 1 / z by additions along a span (inv_z += d_inv_z_dx) and along the rows of a
+// polygon part (line_inv_z += d_inv_z_dy); float addition rounds at every step, so each value is the
+// end of a chain of additions. The kernels keep those chains: a wave computes 64 consecutive values
+// of a chain by a lane-serial scan (lane j adds to lane j - 1's value), chunk after chunk.
+//
+//   k_depth_setup   one wave per draw: the edge walk (lane 0) into parts, then per row of each part
+//                   the span's x range and its first pixel's 1 / z (the rows' chain by the scan)
+//   k_depth_raster  one wave per depth-buffer row: the target's draws in order, each covering span
+//                   filled chunk by chunk (the pixels' chain by the scan)
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sw_internal.h"
+#include "sw_device.h"
+
+using namespace swd;
+
+namespace
+{
+
+SW_DEV int32_t imax32(int32_t a, int32_t b) { return a > b ? a : b; }
+SW_DEV int32_t imin32(int32_t a, int32_t b) { return a < b ? a : b; }
+
+struct DepthRow
+{
+	uint16_t x0, x1; // [x0, x1): the span's pixels (empty when x0 >= x1)
+	float inv_z; // 1 / z at x0
+};
+
+struct DepthDrawDev
+{
+	uint32_t first_vertex_int, num_vertices;
+	float dx, dy, k; // DepthEquation
+	uint32_t depth_test;
+	uint32_t target;
+	uint32_t pad;
+	uint64_t rec_base; // the draw's row records: rec_base + y, y < the target's height
+};
+
+struct DepthTargetDev
+{
+	uint64_t offset; // of the depth buffer in the texels
+	uint32_t w, h;
+	uint32_t first_draw, num_draws;
+	uint32_t first_row_block; // raster blocks: first_row_block + y
+	uint32_t pad;
+};
+
+struct DepthLaunch
+{
+	const DepthDrawDev* draws;
+	const DepthTargetDev* targets;
+	uint32_t num_targets;
+	uint32_t pad;
+	const int32_t* vdata;
+	DepthRow* rows;
+	int32_t* draw_rows; // per draw: [first, end) rows with records
+	float* texels;
+};
+
+// value of lane j = value of lane j - 1 + d, lane 0 = start (a chain of j float additions)
+SW_DEV float serial_chain(float start, float d)
+{
+	const uint32_t lane = threadIdx.x & 63u;
+	float v = start;
+	for(uint32_t s = 1; s < 64; ++s)
+	{
+		const float p = __shfl(v, int(s - 1));
+		if(lane == s)
+			v = p + d;
+	}
+	return v;
+}
+
+struct Part
+{
+	int32_t y0, y1; // rows [y0, y1), clamped to the buffer
+	int32_t xl, xr; // x_left / x_right of row y0 (with + 0.5)
+	int32_t dxl, dxr;
+	float line_inv_z; // of row y0
+};
+
+#define MAX_DEPTH_PARTS SW_MAX_POLYGON_VERTICES
+
+__global__ __launch_bounds__(64) void k_depth_setup(DepthLaunch L)
+{
+	__shared__ Part parts[MAX_DEPTH_PARTS];
+	__shared__ uint32_t num_parts;
+	const DepthDrawDev d = L.draws[blockIdx.x];
+	const DepthTargetDev t = L.targets[d.target];
+	const uint32_t lane = threadIdx.x;
+	DepthRow* rows = L.rows + d.rec_base;
+	for(uint32_t y = lane; y < t.h; y += 64)
+		rows[y] = DepthRow{0, 0, 0.0f};
+	if(lane == 0)
+	{
+		// fill_polygon (rasterizer.rs:1315-1412): from the lowest vertex up both chains
+		const int32_t* v = L.vdata + d.first_vertex_int;
+		const uint32_t n = d.num_vertices;
+		uint32_t lower = 0;
+		int32_t min_y = v[1];
+		for(uint32_t i = 0; i < n; ++i)
+			if(v[i * 2 + 1] < min_y)
+			{
+				min_y = v[i * 2 + 1];
+				lower = i;
+			}
+		uint32_t left = lower, right = lower, np = 0;
+		int32_t cur_y = min_y;
+		const int32_t H = int32_t(t.h);
+		for(uint32_t guard = 0; guard < 2 * n + 2; ++guard)
+		{
+			const uint32_t next_left = left + n - 1 >= n ? left + n - 1 - n : left + n - 1;
+			const uint32_t next_right = right + 1 >= n ? right + 1 - n : right + 1;
+			const int32_t lx = v[left * 2], ly = v[left * 2 + 1], nlx = v[next_left * 2], nly = v[next_left * 2 + 1];
+			const int32_t rx = v[right * 2], ry = v[right * 2 + 1], nrx = v[next_right * 2], nry = v[next_right * 2 + 1];
+			const int32_t dy_left = nly - ly, dy_right = nry - ry;
+			const int32_t next_y = nly < nry ? nly : nry;
+			bool fill = false;
+			int32_t xs_l = 0, xs_r = 0, dl = 0, dr = 0;
+			if(dy_left > FIXED16_HALF && dy_right > FIXED16_HALF)
+			{
+				dl = fixed16_div(nlx - lx, dy_left);
+				dr = fixed16_div(nrx - rx, dy_right);
+				xs_l = lx + fixed16_mul(dl, cur_y - ly);
+				xs_r = rx + fixed16_mul(dr, cur_y - ry);
+				fill = true;
+			}
+			else if(dy_left > 0 && dy_right > 0)
+			{
+				const int32_t cur_y_int = fixed16_round_to_int(cur_y), next_y_int = fixed16_round_to_int(next_y);
+				if(cur_y_int < next_y_int)
+				{
+					const int32_t thin_line_y = int_to_fixed16(cur_y_int) + FIXED16_HALF;
+					xs_l = lx + fixed16_mul_div(thin_line_y - ly, nlx - lx, dy_left);
+					xs_r = rx + fixed16_mul_div(thin_line_y - ry, nrx - rx, dy_right);
+					fill = true;
+				}
+			}
+			if(fill && np < MAX_DEPTH_PARTS)
+			{
+				// fill_polygon_part (:1414-1465): the part's first row
+				const int32_t y0 = imax32(fixed16_round_to_int(cur_y), 0);
+				const int32_t y1 = imin32(fixed16_round_to_int(next_y), H);
+				const int32_t y_start_delta = int_to_fixed16(y0) + FIXED16_HALF - cur_y;
+				Part& p = parts[np++];
+				p.y0 = y0;
+				p.y1 = y1;
+				p.xl = xs_l + fixed16_mul(y_start_delta, dl) + FIXED16_HALF;
+				p.xr = xs_r + fixed16_mul(y_start_delta, dr) + FIXED16_HALF;
+				p.dxl = dl;
+				p.dxr = dr;
+				p.line_inv_z = float(y0) * d.dy + 0.5f * (d.dx + d.dy) + d.k;
+			}
+			if(next_left == next_right)
+				break;
+			if(nry < nly)
+				right = next_right;
+			else
+				left = next_left;
+			cur_y = next_y;
+		}
+		num_parts = np;
+	}
+	__syncthreads();
+	int32_t first = int32_t(t.h), end = 0;
+	for(uint32_t pi = 0; pi < num_parts; ++pi)
+	{
+		const Part p = parts[pi];
+		if(p.y0 >= p.y1)
+			continue;
+		first = imin32(first, p.y0);
+		end = imax32(end, p.y1);
+		float carry = p.line_inv_z;
+		for(int32_t yb = p.y0; yb < p.y1; yb += 64)
+		{
+			const float line_inv_z = serial_chain(carry, d.dy);
+			const int32_t y = yb + int32_t(lane);
+			if(y < p.y1)
+			{
+				const uint32_t r = uint32_t(y - p.y0);
+				const int32_t xl = int32_t(uint32_t(p.xl) + r * uint32_t(p.dxl)), xr = int32_t(uint32_t(p.xr) + r * uint32_t(p.dxr));
+				const int32_t x0 = imax32(fixed16_floor_to_int(xl), 0), x1 = imin32(fixed16_floor_to_int(xr), int32_t(t.w));
+				DepthRow row{0, 0, 0.0f};
+				if(x0 < x1)
+					row = DepthRow{uint16_t(x0), uint16_t(x1), line_inv_z + float(x0) * d.dx};
+				rows[y] = row;
+			}
+			carry = __shfl(line_inv_z, 63) + d.dy;
+		}
+	}
+	if(lane == 0)
+	{
+		L.draw_rows[blockIdx.x * 2] = first < end ? first : 0;
+		L.draw_rows[blockIdx.x * 2 + 1] = first < end ? end : 0;
+	}
+}
+
+__global__ __launch_bounds__(64) void k_depth_raster(DepthLaunch L)
+{
+	uint32_t lo = 0, hi = L.num_targets;
+	while(hi - lo > 1)
+	{
+		const uint32_t mid = (lo + hi) >> 1;
+		if(L.targets[mid].first_row_block <= blockIdx.x)
+			lo = mid;
+		else
+			hi = mid;
+	}
+	const DepthTargetDev t = L.targets[lo];
+	const int32_t y = int32_t(blockIdx.x - t.first_row_block);
+	const uint32_t lane = threadIdx.x;
+	float* row = L.texels + t.offset + uint64_t(y) * t.w;
+	for(uint32_t di = t.first_draw; di < t.first_draw + t.num_draws; ++di)
+	{
+		const int32_t y_first = L.draw_rows[di * 2], y_end = L.draw_rows[di * 2 + 1];
+		if(y < y_first || y >= y_end)
+			continue;
+		const DepthDrawDev d = L.draws[di];
+		const DepthRow r = L.rows[d.rec_base + uint32_t(y)];
+		float carry = r.inv_z;
+		for(uint32_t xb = r.x0; xb < r.x1; xb += 64)
+		{
+			const float inv_z = serial_chain(carry, d.dx);
+			const uint32_t x = xb + lane;
+			if(x < r.x1)
+				row[x] = d.depth_test ? fmax_r(inv_z, row[x]) : inv_z;
+			carry = __shfl(inv_z, 63) + d.dx;
+		}
+	}
+}
+
+} // namespace
+
+namespace swk
+{
+int depth_plan(const sw_depth_target* targets, uint32_t num_targets, const sw_depth_draw* draws, uint32_t num_draws,
+	uint64_t num_vertex_ints, uint64_t num_texels, std::vector<char>& packed, DepthPlan& plan, std::string& error)
+{
+	std::vector<DepthTargetDev> tg(num_targets);
+	std::vector<DepthDrawDev> dd(num_draws);
+	uint64_t rec = 0;
+	uint32_t row_blocks = 0, next_draw = 0;
+	for(uint32_t i = 0; i < num_targets; ++i)
+	{
+		const sw_depth_target& t = targets[i];
+		const std::string which = "depth target " + std::to_string(i) + ": ";
+		if(t.width == 0 || t.height == 0 || t.width > 65535 || t.height > 65535)
+			return (error = which + "size out of range"), -1;
+		if(t.offset > num_texels || uint64_t(t.width) * t.height > num_texels - t.offset)
+			return (error = which + "buffer outside the depth texels"), -1;
+		if(t.first_draw != next_draw || uint64_t(t.first_draw) + t.num_draws > num_draws)
+			return (error = which + "draw range not following the previous target's"), -1;
+		next_draw = t.first_draw + t.num_draws;
+		if(uint64_t(row_blocks) + t.height > 0x7FFFFFFFu)
+			return (error = "depth targets: too many rows"), -1;
+		tg[i] = DepthTargetDev{t.offset, t.width, t.height, t.first_draw, t.num_draws, row_blocks, 0};
+		row_blocks += t.height;
+		for(uint32_t k = t.first_draw; k < t.first_draw + t.num_draws; ++k)
+		{
+			const sw_depth_draw& s = draws[k];
+			if(s.num_vertices < 3 || s.num_vertices > SW_MAX_POLYGON_VERTICES ||
+				uint64_t(s.first_vertex_int) + 2ull * s.num_vertices > num_vertex_ints)
+				return (error = "depth draw " + std::to_string(k) + ": vertices out of range"), -1;
+			dd[k] = DepthDrawDev{s.first_vertex_int, s.num_vertices, s.depth_equation[0], s.depth_equation[1], s.depth_equation[2],
+				s.depth_test ? 1u : 0u, i, 0, rec};
+			rec += t.height;
+		}
+	}
+	if(next_draw != num_draws)
+		return (error = "depth draws not covered by the targets"), -1;
+	plan.targets_at = (sizeof(DepthDrawDev) * dd.size() + 255) & ~size_t(255);
+	plan.bytes = plan.targets_at + sizeof(DepthTargetDev) * tg.size();
+	plan.row_records = rec;
+	plan.row_blocks = row_blocks;
+	plan.num_draws = num_draws;
+	plan.num_targets = num_targets;
+	packed.assign(plan.bytes, 0);
+	std::memcpy(packed.data(), dd.data(), sizeof(DepthDrawDev) * dd.size());
+	std::memcpy(packed.data() + plan.targets_at, tg.data(), sizeof(DepthTargetDev) * tg.size());
+	return 0;
+}
+
+size_t depth_row_record_bytes() { return sizeof(DepthRow); }
+
+void launch_depth(hipStream_t st, const char* args_dev, const DepthPlan& plan, const int32_t* vdata, void* rows, int32_t* draw_rows, float* texels)
+{
+	if(plan.num_draws == 0 || plan.row_blocks == 0)
+		return;
+	DepthLaunch L{reinterpret_cast<const DepthDrawDev*>(args_dev), reinterpret_cast<const DepthTargetDev*>(args_dev + plan.targets_at),
+		plan.num_targets, 0, vdata, static_cast<DepthRow*>(rows), draw_rows, texels};
+	hipLaunchKernelGGL(k_depth_setup, dim3(plan.num_draws), dim3(64), 0, st, L);
+	hipLaunchKernelGGL(k_depth_raster, dim3(plan.row_blocks), dim3(64), 0, st, L);
+}
+} // namespace swk

@@ -266,6 +266,20 @@ int pack_texture_ops(const sw_texture_op* ops, uint32_t num_ops, const sw_textur
 	uint32_t num_turb_shifts, std::vector<char>& packed, TexOpPlan& plan, std::string& error);
 void launch_texture_ops(hipStream_t st, const char* args_dev, const TexOpPlan& plan, const int32_t* turb_shifts_dev, sw_texel* texels,
 	uint32_t* lite);
+// Depth buffers: depth_plan validates and packs the draws (then targets at targets_at);
+// launch_depth runs the setup (one wave per draw, row records of depth_row_record_bytes() each,
+// the target's height per draw) and the raster (one wave per target row).
+struct DepthPlan
+{
+	size_t bytes = 0, targets_at = 0;
+	uint64_t row_records = 0;
+	uint32_t row_blocks = 0, num_draws = 0, num_targets = 0;
+};
+int depth_plan(const sw_depth_target* targets, uint32_t num_targets, const sw_depth_draw* draws, uint32_t num_draws,
+	uint64_t num_vertex_ints, uint64_t num_texels, std::vector<char>& packed, DepthPlan& plan, std::string& error);
+size_t depth_row_record_bytes();
+void launch_depth(hipStream_t st, const char* args_dev, const DepthPlan& plan, const int32_t* vdata, void* rows, int32_t* draw_rows,
+	float* texels);
 // A skybox side (mip 0 from the image, then up to four halvings) back to back at dst.
 void launch_skybox_side(hipStream_t st, uint32_t color_bits, const void* image, uint32_t image_w, uint32_t size, int32_t brightness,
 	void* dst);

@@ -66,7 +66,8 @@ struct sw_ctx
 
 	// command buffers / work buffers
 	DevBuf views, surfaces, lights, light_indices, draws, vdata, surf_ext, blk_surf, parts, spans, drec, dbounds, counters, arena, fb,
-		tonemap_out, tonemap_sums, bloom0, bloom1, bloom_lines, model_args, model_scratch, texop_args, shadows;
+		tonemap_out, tonemap_sums, bloom0, bloom1, bloom_lines, model_args, model_scratch, texop_args, shadows, depth_args, depth_vdata,
+		depth_rows, depth_draw_rows;
 	uint32_t num_shadows = 0; // sw_set_light_shadows: entries in `shadows`
 	const float* shadow_texels = nullptr;
 	uint32_t tile_rows = 8; // raster tile height (SW_TILE_ROWS_*), set by sw_set_tile_rows / the autotuner
@@ -122,6 +123,7 @@ struct sw_ctx
 	size_t arg_staging_cap = 0;
 	hipEvent_t arg_staging_event = nullptr; // recorded after the last copy out of arg_staging
 	std::vector<char> texop_packed;
+	std::vector<char> depth_packed;
 	std::vector<SurfExt> host_ext;
 	std::vector<uint32_t> host_blk;
 
@@ -231,6 +233,7 @@ enum
 	STAGE_BLOOM = 5,
 	STAGE_MODELS = 6,
 	STAGE_TEXTURES = 7,
+	STAGE_DEPTH = 8,
 };
 
 hipEvent_t take_event(sw_ctx* ctx)
@@ -797,7 +800,8 @@ void sw_ctx_destroy(sw_ctx* ctx)
 		(void)hipGraphDestroy(ctx->graph);
 	DevBuf* bufs[] = {&ctx->views, &ctx->surfaces, &ctx->lights, &ctx->light_indices, &ctx->draws, &ctx->vdata, &ctx->surf_ext, &ctx->blk_surf,
 		&ctx->parts, &ctx->spans, &ctx->drec, &ctx->dbounds, &ctx->counters, &ctx->arena, &ctx->fb, &ctx->tonemap_out, &ctx->tonemap_sums,
-		&ctx->bloom0, &ctx->bloom1, &ctx->bloom_lines, &ctx->model_args, &ctx->model_scratch, &ctx->texop_args, &ctx->shadows};
+		&ctx->bloom0, &ctx->bloom1, &ctx->bloom_lines, &ctx->model_args, &ctx->model_scratch, &ctx->texop_args, &ctx->shadows, &ctx->depth_args, &ctx->depth_vdata,
+		&ctx->depth_rows, &ctx->depth_draw_rows};
 	for(DevBuf* b : bufs)
 		release(ctx, *b);
 	if(ctx->counters_host)
@@ -1348,6 +1352,43 @@ int sw_apply_texture_ops(
 			static_cast<sw_texel*>(map->texels.p), static_cast<uint32_t*>(map->lite.p));
 	}
 	return hip_check(ctx, hipGetLastError(), "texture ops launch");
+}
+
+int sw_render_depth(sw_ctx* ctx, const sw_depth_target* targets, uint32_t num_targets, const sw_depth_draw* draws, uint32_t num_draws,
+	const int32_t* vertex_data, uint64_t num_vertex_ints, float* depth_texels_device, uint64_t num_depth_texels)
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	if((num_targets && !targets) || (num_draws && (!draws || !vertex_data)) || !depth_texels_device)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	if(num_targets == 0)
+		return SW_OK;
+	swk::DepthPlan plan;
+	std::string error;
+	if(swk::depth_plan(targets, num_targets, draws, num_draws, num_vertex_ints, num_depth_texels, ctx->depth_packed, plan, error))
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, error);
+	(void)hipSetDevice(ctx->device);
+	int r;
+	const size_t vbytes = sizeof(int32_t) * num_vertex_ints;
+	if((r = ensure(ctx, ctx->depth_args, plan.bytes)) || (r = ensure(ctx, ctx->depth_vdata, std::max<size_t>(vbytes, 4))) ||
+		(r = ensure(ctx, ctx->depth_rows, std::max<uint64_t>(plan.row_records, 1) * swk::depth_row_record_bytes())) ||
+		(r = ensure(ctx, ctx->depth_draw_rows, sizeof(int32_t) * 2 * std::max<uint32_t>(num_draws, 1))))
+		return r;
+	char* staging = nullptr;
+	if((r = stage_begin(ctx, plan.bytes, &staging)))
+		return r;
+	std::memcpy(staging, ctx->depth_packed.data(), plan.bytes);
+	if((r = stage_copy(ctx, ctx->depth_args.p, plan.bytes)))
+		return r;
+	if(vbytes && (r = hip_check(ctx, hipMemcpyAsync(ctx->depth_vdata.p, vertex_data, vbytes, hipMemcpyHostToDevice, ctx->stream),
+					  "depth vertex data")))
+		return r;
+	{
+		StageTimer t(ctx, STAGE_DEPTH);
+		swk::launch_depth(ctx->stream, static_cast<const char*>(ctx->depth_args.p), plan, static_cast<const int32_t*>(ctx->depth_vdata.p),
+			ctx->depth_rows.p, static_cast<int32_t*>(ctx->depth_draw_rows.p), depth_texels_device);
+	}
+	return hip_check(ctx, hipGetLastError(), "depth launch");
 }
 
 int sw_set_light_shadows(

@@ -66,6 +66,11 @@ MODEL_VERTEX_DTYPE = np.dtype([("pos", "<f4", 3), ("tc", "<f4", 2), ("light", "<
 SHADOW_NONE, SHADOW_CUBE, SHADOW_PROJECTOR = 0, 1, 2
 LIGHT_SHADOW_DTYPE = np.dtype([("kind", "<u4"), ("size", "<u4"), ("offset", "<u8"), ("basis_x", "<f4", 3), ("basis_y", "<f4", 3),
                                ("basis_z", "<f4", 3), ("reserved", "<u4")], align=True)
+# depth buffers (sw_render_depth)
+DEPTH_DRAW_DTYPE = np.dtype([("first_vertex_int", "<u4"), ("num_vertices", "<u4"), ("depth_equation", "<f4", 3),
+                             ("depth_test", "<u4")], align=True)
+DEPTH_TARGET_DTYPE = np.dtype([("offset", "<u8"), ("width", "<u4"), ("height", "<u4"), ("first_draw", "<u4"),
+                               ("num_draws", "<u4")], align=True)
 # texture animation (sw_apply_texture_ops)
 TEXOP_TURB, TEXOP_LAYER, TEXOP_LAYER_LITE = 0, 1, 2
 TEXTURE_OP_DTYPE = np.dtype([("kind", "<u4"), ("dst_level", "<u4"), ("src_level", "<u4"), ("blend", "<u4"),
@@ -76,7 +81,7 @@ SKELETON_VERTEX_DTYPE = np.dtype([("tex_coord", "<f4", 2), ("position", "<f4", 3
 
 EXPECTED_SIZES = {"TEXEL": 8, "LEVEL": 24, "DIRLM": 40, "LIGHT": 32, "SURFACE": 136, "DRAW": 96, "VIEW": 64,
                   "MODEL_INSTANCE": 288, "MODEL_VERTEX": 32, "SKELETON_VERTEX": 40,
-                  "TEXTURE_OP": 40, "LIGHT_SHADOW": 56}
+                  "TEXTURE_OP": 40, "LIGHT_SHADOW": 56, "DEPTH_DRAW": 24, "DEPTH_TARGET": 24}
 for _name, _size in EXPECTED_SIZES.items():
     assert globals()[_name + "_DTYPE"].itemsize == _size, (_name, globals()[_name + "_DTYPE"].itemsize)
 
@@ -286,6 +291,8 @@ GPU_SYMBOLS = {
                                         C.POINTER(C.c_float), C.c_void_p, C.c_void_p, C.c_size_t,
                                         C.POINTER(C.c_float)]),
     "sw_update_exposure": (C.c_float, [C.c_void_p, C.c_float, C.POINTER(C.c_float), C.c_float]),
+    "sw_render_depth": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64,
+                                  C.c_void_p, C.c_uint64]),
     "sw_set_light_shadows": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]),
     "sw_apply_texture_ops": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]),
     "sw_map_read": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p]),
@@ -460,6 +467,21 @@ class Context:
         self._shadow_buf = self.to_device(tx)
         self.check(self.lib.sw_set_light_shadows(self.h, _ptr(sh), len(sh), self._shadow_buf.ptr, len(tx)), "sw_set_light_shadows")
 
+    def render_depth(self, targets, draws, vertex_data, texels):
+        """sw_render_depth into a copy of `texels` (float32) on the device; returns the result."""
+        tg = np.ascontiguousarray(targets, dtype=DEPTH_TARGET_DTYPE)
+        dr = np.ascontiguousarray(draws, dtype=DEPTH_DRAW_DTYPE)
+        vd = np.ascontiguousarray(vertex_data, dtype=np.int32)
+        tx = np.ascontiguousarray(texels, dtype=np.float32)
+        buf = self.to_device(tx)
+        try:
+            self.check(self.lib.sw_render_depth(self.h, _ptr(tg), len(tg), _ptr(dr) if len(dr) else None, len(dr),
+                                                _ptr(vd) if len(vd) else None, len(vd), buf.ptr, len(tx)), "sw_render_depth")
+            self.check(self.lib.sw_sync(self.h), "sw_sync")
+            return buf.read(np.float32, len(tx))
+        finally:
+            buf.free()
+
     # texture animation --------------------------------------------------------------------
     def apply_texture_ops(self, map_handle, ops, turb_shifts=None):
         """sw_apply_texture_ops on an uploaded map (TEXTURE_OP_DTYPE array, int32 shift table)."""
@@ -552,7 +574,7 @@ class Context:
         self.check(self.lib.sw_profile_enable(self.h, 1 if on else 0), "sw_profile_enable")
 
     PROFILE_STAGES = ("setup_draws", "build_surfaces", "raster_tiles", "tonemap", "setup_and_surfaces", "bloom", "models",
-                      "textures")
+                      "textures", "depth")
 
     def profile_read(self, reset=True):
         ms = (C.c_double * len(self.PROFILE_STAGES))()

@@ -1,0 +1,87 @@
+"""GPU parity of sw_render_depth (DepthRasterizer, rasterizer.rs:1295-1474): depth buffers of many
+targets (cube sides and projector maps of several sizes, spans longer than one wave) from random
+convex polygons with rounding-heavy depth equations, overwriting and depth-tested, bit-exact to the
+oracle; then the maps shadow a frame's lights."""
+import numpy as np
+import pytest
+
+import oracle
+import scenes
+import sw_abi as A
+
+pytestmark = pytest.mark.gpu
+
+
+def _depth_scene(rng, sizes, draws_per_target):
+    targets = np.zeros(len(sizes), dtype=A.DEPTH_TARGET_DTYPE)
+    draws, vdata = [], []
+    off = 0
+    for t, (w, h) in enumerate(sizes):
+        targets[t]["offset"], targets[t]["width"], targets[t]["height"] = off, w, h
+        targets[t]["first_draw"], targets[t]["num_draws"] = len(draws), draws_per_target
+        off += w * h
+        for i in range(draws_per_target):
+            fx, de, _ = scenes.make_polygon(rng, w, h, 16, 16, A.TC_PERSPECTIVE, thin=(i % 7 == 3))
+            d = np.zeros(1, dtype=A.DEPTH_DRAW_DTYPE)[0]
+            d["first_vertex_int"], d["num_vertices"] = sum(len(v) for v in vdata), len(fx) // 2
+            d["depth_equation"] = de
+            d["depth_test"] = 1 if i % 3 == 2 else 0
+            draws.append(d)
+            vdata.append(fx)
+    return targets, np.array(draws, dtype=A.DEPTH_DRAW_DTYPE), np.concatenate(vdata).astype(np.int32), off
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_depth_buffers_bit_exact(gpu_ctx, seed):
+    rng = np.random.default_rng(seed)
+    sizes = [(64, 64)] * 6 + [(256, 256), (200, 130), (17, 300), (1, 1)]
+    targets, draws, vdata, n = _depth_scene(rng, sizes, 24)
+    init = rng.uniform(0.0, 0.01, size=n).astype(np.float32)
+    print(f"depth seed {seed}: {len(draws)} draws, {n} texels", flush=True)
+    out = gpu_ctx.render_depth(targets, draws, vdata, init)
+    ref = oracle.render_depth(targets, draws, vdata, init)
+    bad = np.flatnonzero(out.view(np.uint32) != ref.view(np.uint32))
+    assert bad.size == 0, f"{bad.size} texels differ, first {bad[:5].tolist()}"
+    assert np.count_nonzero(out != init) > n // 4  # the polygons cover much of the buffers
+
+
+def test_depth_validation(gpu_ctx):
+    rng = np.random.default_rng(3)
+    targets, draws, vdata, n = _depth_scene(rng, [(32, 32)], 4)
+    with pytest.raises(A.SwError, match="outside the depth texels"):
+        gpu_ctx.render_depth(targets, draws, vdata, np.zeros(n - 1, np.float32))
+    bad = draws.copy()
+    bad[1]["num_vertices"] = 30
+    with pytest.raises(A.SwError, match="vertices out of range"):
+        gpu_ctx.render_depth(targets, bad, vdata, np.zeros(n, np.float32))
+
+
+def test_depth_maps_shadow_a_frame(gpu_ctx):
+    """Cube maps drawn by sw_render_depth into the shadow texels, sampled by the surface builder: the
+    frame equals the oracle's with the oracle's own depth maps."""
+    rng = np.random.default_rng(7)
+    m, b = scenes.make_batch(31, color_bits=32, view_sizes=((160, 120),), num_lights=4, surfaces_per_view=40)
+    size = 32
+    targets, draws, vdata, n = _depth_scene(rng, [(size, size)] * (6 * len(b.lights)), 6)
+    # the polygons' 1/z spans the scene's distances: scale the equations so the maps are decisive
+    draws["depth_equation"] *= np.float32(4.0)
+    init = np.zeros(n, np.float32)
+    depth = gpu_ctx.render_depth(targets, draws, vdata, init)
+    ref_depth = oracle.render_depth(targets, draws, vdata, init)
+    assert np.array_equal(depth.view(np.uint32), ref_depth.view(np.uint32))
+    sh = np.zeros(len(b.lights), dtype=A.LIGHT_SHADOW_DTYPE)
+    sh["kind"], sh["size"] = A.SHADOW_CUBE, size
+    sh["offset"] = np.arange(len(b.lights)) * 6 * size * size
+    try:
+        gpu_ctx.set_light_shadows(sh, depth)
+        oracle.set_light_shadows(sh, ref_depth)
+        mh = gpu_ctx.upload_map(m)
+        try:
+            fb = gpu_ctx.render_plain(mh, b)
+        finally:
+            gpu_ctx.release_map(mh)
+        ref_fb, _, _, _ = oracle.render(m, b)
+        assert np.array_equal(fb.reshape(-1), ref_fb)
+    finally:
+        gpu_ctx.set_light_shadows(None, None)
+        oracle.set_light_shadows(None, None)
