@@ -56,6 +56,9 @@ STRUCTS = {
     "sw_model_instance": sw_abi.MODEL_INSTANCE_DTYPE,
     "sw_model_vertex": sw_abi.MODEL_VERTEX_DTYPE,
     "sw_texture_op": sw_abi.TEXTURE_OP_DTYPE,
+    "sw_light_shadow": sw_abi.LIGHT_SHADOW_DTYPE,
+    "sw_depth_draw": sw_abi.DEPTH_DRAW_DTYPE,
+    "sw_depth_target": sw_abi.DEPTH_TARGET_DTYPE,
 }
 
 
