@@ -191,6 +191,77 @@ def hdr_leg(ctx, args):
     return out
 
 
+def shadows_leg(ctx, m, mh, args):
+    """Shadowed frames: 1920x1080 Color32 with 8 dynamic lights, each with a 256x256 cube shadow map
+    drawn every frame from the front-end's draw lists (swh_shadow_add_cube) by sw_render_depth, then
+    the frame rendered with the maps sampled by the surface builder (sw_set_light_shadows +
+    sw_render_batch). Draw lists and command buffers are built before the timed region (front-end
+    CPU work); the timed step is the depth pass plus the frame."""
+    import ctypes as C
+    W, H, size, nl = args.width, args.height, 256, 8
+    fbld = sw_host.FrameBuilder()
+    poses = m.camera_path(args.seed + 7, 8)
+    steps = []
+    for i, p in enumerate(poses):
+        f = fbld.build(m, p, width=W, height=H, color_bits=32, use_directional_lightmaps=1, num_dynamic_lights=nl,
+                       seed=args.seed + 40 + i, clear_background=1, force_clear=1)
+        b = sw_host.batch_from_frames([f])
+        targets, draws, vdata, n, sh = sw_host.cube_shadows_for_lights(m, b.lights, size)
+        steps.append((b, sw_abi.DeviceCommandBuffers(ctx, b), targets, draws, vdata, n, sh))
+    n_max = max(st[5] for st in steps)
+    depth = ctx.alloc(max(n_max, 1) * 4)
+    out = ctx.alloc(W * H * 4)
+    cbs = [st[0].ctypes_batch(inputs_on_device=True, device_ptrs=st[1].ptrs()) for st in steps]
+    lib = ctx.lib
+
+    def step(i):
+        b, _, tg, dr, vd, n, sh = steps[i % len(steps)]
+        ctx.check(lib.sw_render_depth(ctx.h, C.c_void_p(tg.ctypes.data), len(tg), C.c_void_p(dr.ctypes.data), len(dr),
+                                      C.c_void_p(vd.ctypes.data), len(vd), depth.ptr, n), "sw_render_depth")
+        ctx.check(lib.sw_set_light_shadows(ctx.h, C.c_void_p(sh.ctypes.data), len(sh), depth.ptr, n), "sw_set_light_shadows")
+        ctx.check(lib.sw_render_batch(ctx.h, mh, C.byref(cbs[i % len(cbs)]), out.ptr), "sw_render_batch")
+
+    for i in range(len(steps)):
+        step(i)
+    ctx.check(lib.sw_sync(ctx.h), "sw_sync")
+    reps = 64
+    ctx.profile_read(reset=True)
+    lat = np.zeros(reps)
+    t0 = time.perf_counter()
+    for i in range(reps):
+        sample = i % 8 == 0
+        if sample:
+            ctx.profile_enable(True)
+        ts = time.perf_counter()
+        step(i)
+        lat[i] = time.perf_counter() - ts
+        if sample:
+            ctx.profile_enable(False)
+    ctx.check(lib.sw_sync(ctx.h), "sw_sync")
+    wall = time.perf_counter() - t0
+    prof = ctx.profile_read(reset=True)
+    ctx.check(lib.sw_set_light_shadows(ctx.h, None, 0, None, 0), "sw_set_light_shadows")
+    dus = prof["depth"][0] / max(prof["depth"][1], 1) * 1e3
+    texels = float(np.mean([st[5] for st in steps]))
+    draws = float(np.mean([len(st[3]) for st in steps]))
+    for st in steps:
+        st[1].free()
+    depth.free()
+    out.free()
+    res = {"workload": f"{W}x{H} Color32, {nl} dynamic lights with {size}x{size} cube shadow maps drawn per frame, "
+                       f"{len(steps)}-pose path",
+           "frames_s": round(reps / wall, 2), "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 4),
+           "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 4),
+           "depth_draws_per_frame": round(draws, 1), "depth_texels_per_frame": int(texels),
+           "depth_pass_us": round(dus, 2),
+           # algorithmic bytes of the depth pass: one 4-byte write per depth texel (a closed map covers them all)
+           "depth_gtexel_s": round(texels / (dus * 1e-6) / 1e9, 2),
+           "depth_gb_s": round(texels * 4 / (dus * 1e-6) / 1e9, 1),
+           "kernel_us_per_launch": {k: round(v[0] / max(v[1], 1) * 1e3, 3) for k, v in prof.items() if v[1]}}
+    log(f"shadows: {res}")
+    return res
+
+
 def models_leg(ctx, args):
     """Triangle-model preparation (sw_prepare_models): 64 instances of a 1280-triangle, 642-vertex
     vertex-animated mesh (8 frames), meshes resident in HBM, outputs in HBM; the kernel's duration
@@ -355,6 +426,7 @@ def main():
     ap.add_argument("--no-hdr", action="store_true")
     ap.add_argument("--no-models", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-shadows", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--hdr-poses", type=int, default=16)
     ap.add_argument("--hdr-steps", type=int, default=64)
@@ -540,6 +612,9 @@ def main():
         log(f"batched: {result['batched']}")
         bdev.free()
         bout.free()
+
+    if not args.no_shadows:
+        result["shadows"] = shadows_leg(ctx, m, mh, args)
 
     if not args.no_hdr:
         result["hdr_models"] = hdr_leg(ctx, args)

@@ -30,6 +30,7 @@ extern "C" {
 
 typedef struct swh_map swh_map;
 typedef struct swh_frame swh_frame;
+typedef struct swh_shadow swh_shadow;
 
 typedef struct swh_map_config
 {
@@ -105,6 +106,31 @@ void swh_frame_destroy(swh_frame* frame);
 int swh_frame_build(swh_frame* frame, const swh_map* map, const swh_frame_config* config, const float cam_pos[3],
 	float yaw, float pitch);
 int swh_frame_get(const swh_frame* frame, swh_frame_data* out_data);
+
+/* Shadow-map draw lists (sw_render_depth's input) of the map's static geometry:
+ *   swh_shadow_add_cube <- PartialRenderer::build_shadow_maps' cube branch (partial_renderer.rs:541-566)
+ *                          with calculate_cube_shadow_map_side_matrices (light.rs:73-128) and
+ *                          DepthRenderer::draw_map / draw_tree / draw_polygon (depth_renderer.rs:47-287):
+ *                          per side, visibility from the light, back-to-front leaves, back faces
+ *                          culled, the biased and scaled depth equation, projection and clipping.
+ * Each call appends six size x size targets (sides +x, -x, +y, -y, +z, -z) at the running texel
+ * offset; the draws keep the reference's order. Blended materials cast no shadows. */
+typedef struct swh_shadow_data
+{
+	const sw_depth_target* targets;
+	uint32_t num_targets;
+	const sw_depth_draw* draws;
+	uint32_t num_draws;
+	const int32_t* vertex_data;
+	uint64_t num_vertex_ints;
+	uint64_t num_depth_texels; /* sum of the targets' width x height */
+} swh_shadow_data;
+
+int swh_shadow_create(swh_shadow** out);
+void swh_shadow_destroy(swh_shadow* shadow);
+void swh_shadow_clear(swh_shadow* shadow);
+int swh_shadow_add_cube(swh_shadow* shadow, const swh_map* map, const float position[3], uint32_t size);
+int swh_shadow_get(const swh_shadow* shadow, swh_shadow_data* out_data);
 
 /* Compiled-map files (common/bsp_map_save_load.rs:23-362): the map as a Square-Wheel .sqwm file,
  * format version 11 — the compact BSP map's lumps (nodes, leafs, polygons, portals, leaf portals,

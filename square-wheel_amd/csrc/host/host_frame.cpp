@@ -327,7 +327,24 @@ M4 quat_to_m4(Quat q)
 	return m;
 }
 
-// matrix.rs:13-109
+// matrix.rs:82-109 (complete_view_matrix)
+Camera complete_view(V3 position, const M4& rotation_matrix, V2 fov_scale, float width, float height)
+{
+	const M4 translate = from_translation(v3(-position.x, -position.y, -position.z));
+	const M4 perspective = from_nonuniform_scale(fov_scale.x, fov_scale.y, 1.0f);
+	const M4 resize = from_nonuniform_scale(width * 0.5f, height * 0.5f, 1.0f);
+	M4 shift = identity();
+	shift.c[2].x = width * 0.5f;
+	shift.c[2].y = height * 0.5f;
+	Camera c;
+	c.position = position;
+	c.view_matrix = mul(mul(mul(mul(shift, resize), perspective), rotation_matrix), translate);
+	if(!invert(transpose(c.view_matrix), c.planes_matrix))
+		c.planes_matrix = identity();
+	return c;
+}
+
+// matrix.rs:13-80
 Camera build_camera(V3 position, float azimuth, float elevation, float fov, float width, float height)
 {
 	const Quat rotation = quat_mul(from_angle_z(azimuth + 3.14159265f * 0.5f), from_angle_y(-elevation));
@@ -343,18 +360,25 @@ Camera build_camera(V3 position, float azimuth, float elevation, float fov, floa
 	basis_change.c[2].y = -1.0f;
 	const Quat conj = Quat{rotation.s, v3(-rotation.v.x, -rotation.v.y, -rotation.v.z)};
 	const M4 rotation_matrix = mul(basis_change, quat_to_m4(conj));
-	const M4 translate = from_translation(v3(-position.x, -position.y, -position.z));
-	const M4 perspective = from_nonuniform_scale(fov_scale.x, fov_scale.y, 1.0f);
-	const M4 resize = from_nonuniform_scale(width * 0.5f, height * 0.5f, 1.0f);
-	M4 shift = identity();
-	shift.c[2].x = width * 0.5f;
-	shift.c[2].y = height * 0.5f;
-	Camera c;
-	c.position = position;
-	c.view_matrix = mul(mul(mul(mul(shift, resize), perspective), rotation_matrix), translate);
-	if(!invert(transpose(c.view_matrix), c.planes_matrix))
-		c.planes_matrix = identity();
-	return c;
+	return complete_view(position, rotation_matrix, fov_scale, width, height);
+}
+
+// light.rs:88-128 (get_cube_map_side_matrix): the side's rotation columns; negated unit vectors
+// keep their negative zeros
+M4 cube_side_rotation(uint32_t side)
+{
+	const V4 px = v4(1.0f, 0.0f, 0.0f, 0.0f), nx = v4(-1.0f, -0.0f, -0.0f, -0.0f);
+	const V4 py = v4(0.0f, 1.0f, 0.0f, 0.0f), ny = v4(-0.0f, -1.0f, -0.0f, -0.0f);
+	const V4 pz = v4(0.0f, 0.0f, 1.0f, 0.0f), nz = v4(-0.0f, -0.0f, -1.0f, -0.0f);
+	static const int cols[6][3] = {{3, -1, -2}, {-3, 1, -2}, {1, 3, -2}, {-1, -3, -2}, {1, 2, 3}, {1, -2, -3}};
+	const V4 pos[3] = {px, py, pz}, neg[3] = {nx, ny, nz};
+	M4 m = identity();
+	for(int c = 0; c < 3; ++c)
+	{
+		const int k = cols[side % 6][c]; // +-(axis + 1)
+		m.c[c] = k > 0 ? pos[k - 1] : neg[-k - 1];
+	}
+	return m;
 }
 
 } // namespace swh
@@ -967,6 +991,135 @@ int swh_frame_get(const swh_frame* f, swh_frame_data* out)
 	out->surface_arena_texels = f->arena_texels;
 	out->num_visible_leafs = f->num_visible_leafs;
 	out->num_triangles = f->num_triangles;
+	return 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Shadow-map draw lists: DepthRenderer::draw_map (depth_renderer.rs:47-287) for the six sides of a
+// cube light (light.rs:73-86, partial_renderer.rs:541-566). The map's static geometry, back to front,
+// visible leaves only; the pixel work is sw_render_depth's.
+// ---------------------------------------------------------------------------------------------
+int swh_shadow_create(swh_shadow** out)
+{
+	if(!out)
+		return -1;
+	*out = new(std::nothrow) swh_shadow();
+	return *out ? 0 : -1;
+}
+
+void swh_shadow_destroy(swh_shadow* s) { delete s; }
+
+void swh_shadow_clear(swh_shadow* s)
+{
+	if(!s)
+		return;
+	s->targets.clear();
+	s->draws.clear();
+	s->vdata.clear();
+	s->texels = 0;
+}
+
+int swh_shadow_add_cube(swh_shadow* s, const swh_map* m, const float position[3], uint32_t size)
+{
+	if(!s || !m || !position || size == 0 || size > 65535)
+		return -1;
+	ensure_basis(s->vis, *m);
+	const V3 pos = v3(position[0], position[1], position[2]);
+	const float fs = float(size);
+	const ClippingPolygon frame_bounds = ClippingPolygon::from_box(0.0f, 0.0f, fs, fs);
+	for(uint32_t side = 0; side < 6; ++side)
+	{
+		const Camera cam = complete_view(pos, cube_side_rotation(side), v2(1.0f, 1.0f), fs, fs);
+		swh_frame& vis = s->vis;
+		vis.frame_number++;
+		update_visibility(vis, *m, cam, frame_bounds);
+		sw_depth_target t;
+		t.offset = s->texels;
+		t.width = size;
+		t.height = size;
+		t.first_draw = uint32_t(s->draws.size());
+		// draw_tree (:69-109): back to front, leaves with visibility bounds
+		std::vector<uint32_t> stack;
+		stack.push_back(uint32_t(m->nodes.size() - 1));
+		const V4 w_row = cam.planes_matrix.row(3);
+		while(!stack.empty())
+		{
+			const uint32_t node_index = stack.back();
+			if(node_index >= FIRST_LEAF_INDEX)
+			{
+				stack.pop_back();
+				const uint32_t leaf_index = node_index - FIRST_LEAF_INDEX;
+				if(vis.leaf_visible_frame[leaf_index] != vis.frame_number)
+					continue;
+				V3 cp[8];
+				vis.leaf_bounds[leaf_index].clip_planes(cp);
+				const Leaf& leaf = m->leafs[leaf_index];
+				for(uint32_t k = 0; k < leaf.num_polygons; ++k)
+				{
+					// draw_polygon (:202-254)
+					const Polygon& p = m->polygons[leaf.first_polygon + k];
+					// materials_shadow_table (:23-37): blended materials cast no shadows
+					if(m->materials[p.texture].blending_mode != SW_BLEND_NONE)
+						continue;
+					const V4 pt = mul(cam.planes_matrix, extend(p.plane.vec, -p.plane.dist));
+					if(pt.w <= 0.0f)
+						continue; // back face
+					const float w = -pt.w;
+					const float dx = pt.x / w, dy = pt.y / w;
+					const float bias_const = -1.0f / float(1 << 20), bias_slope = -1.0f, eq_scale = 1.0f - 1.0f / 1024.0f;
+					sw_depth_draw d;
+					d.depth_equation[0] = eq_scale * dx;
+					d.depth_equation[1] = eq_scale * dy;
+					d.depth_equation[2] = eq_scale * (pt.z / w + bias_const + bias_slope * (std::fabs(dx) + std::fabs(dy)));
+					d.depth_test = 0;
+					const uint32_t n = std::min(p.num_vertices, MAX_VERTICES);
+					if(n < 3)
+						continue;
+					V3 vt[MAX_VERTICES];
+					for(uint32_t i = 0; i < n; ++i)
+						vt[i] = view_transform(cam.view_matrix, m->vertices[p.first_vertex + i]);
+					// draw_depth_polygon (:257-287)
+					V2 v2d[MAX_VERTICES];
+					const uint32_t nc = project_and_clip_polygon(cp, vt, n, v2d);
+					if(nc < 3)
+						continue;
+					d.first_vertex_int = uint32_t(s->vdata.size());
+					d.num_vertices = nc;
+					for(uint32_t i = 0; i < nc; ++i)
+					{
+						s->vdata.push_back(saturating_f32_to_i32(v2d[i].x * 65536.0f));
+						s->vdata.push_back(saturating_f32_to_i32(v2d[i].y * 65536.0f));
+					}
+					s->draws.push_back(d);
+				}
+			}
+			else
+			{
+				const Node& node = m->nodes[node_index];
+				const float wn = dot(w_row, extend(node.plane.vec, -node.plane.dist));
+				const uint32_t mask = wn >= 0.0f ? 1u : 0u;
+				stack.back() = node.children[1 ^ mask];
+				stack.push_back(node.children[mask]);
+			}
+		}
+		t.num_draws = uint32_t(s->draws.size()) - t.first_draw;
+		s->targets.push_back(t);
+		s->texels += uint64_t(size) * size;
+	}
+	return 0;
+}
+
+int swh_shadow_get(const swh_shadow* s, swh_shadow_data* out)
+{
+	if(!s || !out)
+		return -1;
+	out->targets = s->targets.data();
+	out->num_targets = uint32_t(s->targets.size());
+	out->draws = s->draws.data();
+	out->num_draws = uint32_t(s->draws.size());
+	out->vertex_data = s->vdata.data();
+	out->num_vertex_ints = s->vdata.size();
+	out->num_depth_texels = s->texels;
 	return 0;
 }
 

@@ -191,3 +191,15 @@ struct swh_frame
 	ModelFrameData models;
 };
 
+// Shadow-map draw lists under construction: the visibility state of the map (leaf bounds) reused
+// side after side, and the depth targets, draws and Fixed16 vertex data of sw_render_depth.
+struct swh_shadow
+{
+	swh_frame vis;
+	std::vector<sw_depth_target> targets;
+	std::vector<sw_depth_draw> draws;
+	std::vector<int32_t> vdata;
+	uint64_t texels = 0;
+};
+
+

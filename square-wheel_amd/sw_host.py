@@ -77,6 +77,16 @@ def load():
         lib.swh_frame_build.restype = C.c_int
         lib.swh_frame_get.argtypes = [C.c_void_p, C.POINTER(FrameData)]
         lib.swh_frame_get.restype = C.c_int
+        lib.swh_shadow_create.argtypes = [C.POINTER(C.c_void_p)]
+        lib.swh_shadow_create.restype = C.c_int
+        lib.swh_shadow_destroy.argtypes = [C.c_void_p]
+        lib.swh_shadow_destroy.restype = None
+        lib.swh_shadow_clear.argtypes = [C.c_void_p]
+        lib.swh_shadow_clear.restype = None
+        lib.swh_shadow_add_cube.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_float), C.c_uint32]
+        lib.swh_shadow_add_cube.restype = C.c_int
+        lib.swh_shadow_get.argtypes = [C.c_void_p, C.POINTER(ShadowData)]
+        lib.swh_shadow_get.restype = C.c_int
         lib.swh_pack_normal_roughness_from_rgb.argtypes = [C.c_uint32, C.c_float]
         lib.swh_pack_normal_roughness_from_rgb.restype = C.c_int32
         lib.swh_build_texture_mips.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
@@ -238,6 +248,66 @@ class FrameBuilder:
             self.close()
         except Exception:
             pass
+
+
+class ShadowData(C.Structure):
+    _fields_ = [("targets", C.c_void_p), ("num_targets", C.c_uint32), ("draws", C.c_void_p), ("num_draws", C.c_uint32),
+                ("vertex_data", C.c_void_p), ("num_vertex_ints", C.c_uint64), ("num_depth_texels", C.c_uint64)]
+
+
+class ShadowBuilder:
+    """Draw lists of cube shadow maps for sw_render_depth (swh_shadow_add_cube; the reference's
+    PartialRenderer::build_shadow_maps cube branch with DepthRenderer::draw_map,
+    partial_renderer.rs:541-566, depth_renderer.rs:47-287): six size x size sides per light, at
+    consecutive texel offsets, in the order +x, -x, +y, -y, +z, -z."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        if load().swh_shadow_create(C.byref(h)) != 0:
+            raise RuntimeError("swh_shadow_create failed")
+        self.h = h
+
+    def clear(self):
+        load().swh_shadow_clear(self.h)
+
+    def add_cube(self, m: Map, position, size):
+        pos = (C.c_float * 3)(*(float(x) for x in position))
+        if load().swh_shadow_add_cube(self.h, m.h, pos, int(size)) != 0:
+            raise ValueError(f"swh_shadow_add_cube failed (size {size})")
+
+    def arrays(self):
+        """(targets, draws, vertex_data, num_depth_texels): copies of the lists built so far."""
+        sd = ShadowData()
+        load().swh_shadow_get(self.h, C.byref(sd))
+        return (_copy(sd.targets, sd.num_targets, A.DEPTH_TARGET_DTYPE), _copy(sd.draws, sd.num_draws, A.DEPTH_DRAW_DTYPE),
+                _copy(sd.vertex_data, sd.num_vertex_ints, np.int32), int(sd.num_depth_texels))
+
+    def close(self):
+        if self.h:
+            load().swh_shadow_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def cube_shadows_for_lights(m: Map, lights, size):
+    """Cube shadow maps for every light of a frame's light list: the draw lists and the matching
+    LIGHT_SHADOW_DTYPE descriptors (light i's six sides at texel offset 6 * size * size * i)."""
+    sb = ShadowBuilder()
+    try:
+        for lt in lights:
+            sb.add_cube(m, lt["position"], size)
+        targets, draws, vdata, n = sb.arrays()
+    finally:
+        sb.close()
+    sh = np.zeros(len(lights), dtype=A.LIGHT_SHADOW_DTYPE)
+    sh["kind"], sh["size"] = A.SHADOW_CUBE, size
+    sh["offset"] = np.arange(len(lights), dtype=np.uint64) * 6 * size * size
+    return targets, draws, vdata, n, sh
 
 
 def batch_from_frames(frames):

@@ -85,3 +85,37 @@ def test_depth_maps_shadow_a_frame(gpu_ctx):
     finally:
         gpu_ctx.set_light_shadows(None, None)
         oracle.set_light_shadows(None, None)
+
+
+@pytest.mark.parametrize("color_bits", [32, 64])
+def test_front_end_cube_shadows_frame(gpu_ctx, color_bits):
+    """The whole shadow path of a frame: the front-end's cube draw lists for the frame's dynamic lights
+    (swh_shadow_add_cube), drawn by sw_render_depth, sampled by the surface builder; depth maps and
+    the frame bit-exact to the oracle's, and the shadows darken the frame."""
+    import sw_host
+    m = sw_host.large_map(5, grid=(12, 12, 4))
+    pose = m.camera_path(9, 1)[0]
+    f = sw_host.FrameBuilder().build(m, pose, width=240, height=160, color_bits=color_bits, num_dynamic_lights=8,
+                                     use_directional_lightmaps=1, seed=4)
+    b = sw_host.batch_from_frames([f])
+    assert len(b.lights) > 0
+    size = 64
+    targets, draws, vdata, n, sh = sw_host.cube_shadows_for_lights(m, b.lights, size)
+    print(f"front-end shadows {color_bits}: {len(b.lights)} lights, {len(draws)} draws", flush=True)
+    init = np.zeros(n, np.float32)
+    depth = gpu_ctx.render_depth(targets, draws, vdata, init)
+    ref_depth = oracle.render_depth(targets, draws, vdata, init)
+    assert np.array_equal(depth.view(np.uint32), ref_depth.view(np.uint32))
+    mh = gpu_ctx.upload_map(m.data)
+    try:
+        unshadowed = gpu_ctx.render_plain(mh, b).reshape(-1)
+        gpu_ctx.set_light_shadows(sh, depth)
+        oracle.set_light_shadows(sh, ref_depth)
+        fb = gpu_ctx.render_plain(mh, b).reshape(-1)
+        ref_fb, _, _, _ = oracle.render(m.data, b)
+        assert np.array_equal(fb, ref_fb)
+        assert not np.array_equal(fb, unshadowed)
+    finally:
+        gpu_ctx.set_light_shadows(None, None)
+        oracle.set_light_shadows(None, None)
+        gpu_ctx.release_map(mh)
