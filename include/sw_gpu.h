@@ -608,6 +608,54 @@ void sw_host_free(sw_ctx* ctx, void* host_ptr);
  * tonemap with bloom), 6 models (sw_prepare_models), 7 textures (sw_apply_texture_ops), 8 depth
  * (sw_render_depth). sw_profile_read returns the summed milliseconds and launch counts since the
  * last reset (and resets when reset != 0). Not recorded inside captured graphs. */
+/* ---------------------------------------------------------------------------------------------
+ * Water: the wave-field simulation and the wavy texture of GenerativeTextureEffectWater
+ * (generative_texture_effect_water.rs). The field (float32, width x height, both powers of two,
+ * at least 16, area below 2^22) lives in two device buffers owned by the caller. The wave sources
+ * (spots, lines, droplets and the random rain, :57-226) stay on the caller's CPU, which hands each
+ * step's points over.
+ * ------------------------------------------------------------------------------------------- */
+
+/* One add_point_value of a step (:68-76): field[x, y] += value, old field[x, y] -= value, with x
+ * and y wrapped (masked) to the field. */
+typedef struct sw_wave_point
+{
+	uint32_t x, y;
+	float value;
+} sw_wave_point;
+
+/* step_update_wave_field (:57-226) after its sources: the points in order, then update_wave_field
+ * (:307-418) on the torus, new = ((left + right + up + down) * 0.5 - old) * attenuation, written over
+ * field_old_device, as the reference writes its old field before swapping the two: the caller swaps
+ * its two pointers after each call. Queued on the context's stream. */
+int sw_water_step(sw_ctx* ctx, float* field_device, float* field_old_device, uint32_t log2_width, uint32_t log2_height,
+	float attenuation, const sw_wave_point* points, uint32_t num_points);
+
+#define SW_WATER_COLOR_KEEP 0u /* SingleColor / SourceTexture: the texels' colours are kept */
+#define SW_WATER_COLOR_DEFORMED 1u /* SourceTextureNormalDeformed */
+#define SW_WATER_COLOR_DEFORMED_X 2u /* SourceTextureNormalDeformedX */
+
+#define SW_WATER_INIT_NONE 0u
+#define SW_WATER_INIT_BASE_COLOR 1u /* the first update of SingleColor: every diffuse = base_color */
+#define SW_WATER_INIT_SOURCE 2u /* the first update of SourceTexture: diffuse from color_level */
+
+typedef struct sw_water_texture_params
+{
+	uint32_t dst_level; /* mip 0 (the field's size, SW_TEX_HAS_NORMAL_MAP); the next 4 levels get its mips */
+	uint32_t color_level; /* the colour image (same size) for DEFORMED, DEFORMED_X and INIT_SOURCE */
+	uint32_t color_mode; /* SW_WATER_COLOR_* */
+	uint32_t init; /* SW_WATER_INIT_* */
+	float roughness; /* the material's; clamped to [1/126, 1] as the reference does */
+	uint32_t base_color; /* INIT_BASE_COLOR's colour: the source texture's last-mip texel */
+} sw_water_texture_params;
+
+/* make_wavy_texture (:420-682) from the field into the map's dst level (normal = normalize(right -
+ * left, down - up, 1), packed with the roughness; colours per color_mode with the deformation scale
+ * 16), then build_texture_mip (textures.rs:309-362) four times into the following levels. */
+int sw_water_make_texture(sw_ctx* ctx, sw_map* map, const float* field_device, uint32_t log2_width, uint32_t log2_height,
+	const sw_water_texture_params* params);
+
+
 #define SW_PROFILE_STAGES 9
 int sw_profile_enable(sw_ctx* ctx, uint32_t enable);
 int sw_profile_read(sw_ctx* ctx, double out_ms[SW_PROFILE_STAGES], uint64_t out_counts[SW_PROFILE_STAGES], uint32_t reset);

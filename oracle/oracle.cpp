@@ -2453,6 +2453,70 @@ struct DepthRasterizer
 // ---------------------------------------------------------------------------------------------
 // C ABI of the oracle (loaded with ctypes by the tests and bench.py)
 // ---------------------------------------------------------------------------------------------
+// ---------------------------------------------------------------------------------------------
+// generative_texture_effect_water.rs: the field step and the wavy texture, following the
+// reference's row cases (first row, inner rows, last row; first and last columns wrapped)
+// ---------------------------------------------------------------------------------------------
+// f(offset, x-1, x+1, y-1, y+1, x, y) over the field in the reference's order (:325-417, :548-682)
+template <class F>
+void water_rows(uint32_t w, uint32_t h, F&& f)
+{
+	auto row = [&](uint32_t y, uint32_t up_row, uint32_t down_row) {
+		const uint32_t line = y * w, end = line + w - 1;
+		f(line, end, line + 1, up_row, down_row, 0u, y);
+		for(uint32_t o = line + 1; o < end; ++o)
+			f(o, o - 1, o + 1, up_row + (o - line), down_row + (o - line), o - line, y);
+		f(end, end - 1, line, up_row + w - 1, down_row + w - 1, w - 1, y);
+	};
+	row(0, (h - 1) * w, w);
+	for(uint32_t y = 1; y < h - 1; ++y)
+		row(y, (y - 1) * w, (y + 1) * w);
+	row(h - 1, (h - 2) * w, 0);
+}
+
+// Color32::get_average_4 (color.rs:51-71)
+uint32_t color32_average_4(const uint32_t c[4])
+{
+	uint32_t sum[4] = {0, 0, 0, 0};
+	for(int k = 0; k < 4; ++k)
+	{
+		sum[0] += c[k] & 0x000000FFu;
+		sum[1] += c[k] & 0x0000FF00u;
+		sum[2] += c[k] & 0x00FF0000u;
+		sum[3] += (c[k] & 0xFF000000u) >> 2;
+	}
+	return (sum[0] >> 2) | ((sum[1] >> 2) & 0x0000FF00u) | ((sum[2] >> 2) & 0x00FF0000u) | (sum[3] & 0xFF000000u);
+}
+
+const float MIN_VALID_ROUGHNESS = 1.0f / (127.0f - 1.0f); // textures.rs:418
+
+// build_texture_mip (textures.rs:309-362)
+void texture_mip(const sw_texel* prev, uint32_t pw, uint32_t ph, sw_texel* out)
+{
+	const uint32_t w = pw >> 1, h = ph >> 1;
+	for(uint32_t y = 0; y < h; ++y)
+		for(uint32_t x = 0; x < w; ++x)
+		{
+			const sw_texel p[4] = {prev[(y * 2) * pw + x * 2], prev[(y * 2 + 1) * pw + x * 2], prev[(y * 2) * pw + x * 2 + 1],
+				prev[(y * 2 + 1) * pw + x * 2 + 1]};
+			const uint32_t c[4] = {p[0].diffuse, p[1].diffuse, p[2].diffuse, p[3].diffuse};
+			Vec3 n[4];
+			float r[4];
+			for(int k = 0; k < 4; ++k)
+				unpack_normal_roughness(p[k].packed_normal_roughness, n[k], r[k]);
+			const Vec3 sum = v3(n[0].x + n[1].x + n[2].x + n[3].x, n[0].y + n[1].y + n[2].y + n[3].y, n[0].z + n[1].z + n[2].z + n[3].z);
+			const float len = std::fmax(std::sqrt(sum.x * sum.x + sum.y * sum.y + sum.z * sum.z), 0.000001f);
+			const float half_cos = len * 0.25f;
+			const float deviation = (1.0f - half_cos) / (1.0f - 0.5f);
+			const float rs = r[0] + r[1] + r[2] + r[3];
+			const float dr = std::fmin(std::fmax(std::fma(rs, 0.25f, deviation), MIN_VALID_ROUGHNESS), 1.0f);
+			sw_texel d;
+			d.diffuse = color32_average_4(c);
+			d.packed_normal_roughness = pack_normal_roughness(v3(sum.x / len, sum.y / len, sum.z / len), dr);
+			out[y * w + x] = d;
+		}
+}
+
 extern "C" {
 
 int oracle_render_batch(
@@ -2637,6 +2701,71 @@ int oracle_apply_texture_ops(const sw_texture_level* levels, uint32_t num_levels
 		else
 			lite_layer(lite + dl.offset, dl, lite + sl.offset, sl, op.shift, op.light, op.blend);
 	}
+	return 0;
+}
+
+// One water step: the points (add_point_value, :68-76), then update_wave_field into field_old
+// (:223-224; the caller swaps the buffers as the reference does).
+int oracle_water_step(float* field, float* field_old, uint32_t log2w, uint32_t log2h, float attenuation, const sw_wave_point* pts,
+	uint32_t num_points)
+{
+	const uint32_t w = 1u << log2w, h = 1u << log2h;
+	for(uint32_t i = 0; i < num_points; ++i)
+	{
+		const uint32_t a = (pts[i].x & (w - 1)) + ((pts[i].y & (h - 1)) << log2w);
+		field[a] += pts[i].value;
+		field_old[a] -= pts[i].value;
+	}
+	float* dst = field_old;
+	const float* src = field;
+	water_rows(w, h, [&](uint32_t o, uint32_t xm, uint32_t xp, uint32_t ym, uint32_t yp, uint32_t, uint32_t) {
+		const float sum = src[xm] + src[xp] + src[ym] + src[yp];
+		const float val = dst[o];
+		dst[o] = (sum * 0.5f - val) * attenuation;
+	});
+	return 0;
+}
+
+// make_wavy_texture (:420-682) into the dst level, then its four mips (:293-299).
+int oracle_water_make_texture(const sw_texture_level* levels, uint32_t num_levels, sw_texel* texels, const float* field, uint32_t log2w,
+	uint32_t log2h, const sw_water_texture_params* p)
+{
+	if(uint64_t(p->dst_level) + 4 >= num_levels || p->color_level >= num_levels)
+		return -1;
+	const uint32_t w = 1u << log2w, h = 1u << log2h;
+	sw_texel* dst = texels + levels[p->dst_level].offset;
+	const sw_texel* color = texels + levels[p->color_level].offset;
+	const float roughness = std::fmin(std::fmax(p->roughness, MIN_VALID_ROUGHNESS), 1.0f);
+	// the first update's colour fill (:437-455)
+	if(p->color_mode == SW_WATER_COLOR_KEEP && p->init == SW_WATER_INIT_BASE_COLOR)
+		for(uint32_t i = 0; i < w * h; ++i)
+			dst[i].diffuse = p->base_color;
+	if(p->color_mode == SW_WATER_COLOR_KEEP && p->init == SW_WATER_INIT_SOURCE)
+		for(uint32_t i = 0; i < w * h; ++i)
+			dst[i].diffuse = color[i].diffuse;
+	const float tc_deform_scale = 16.0f;
+	water_rows(w, h, [&](uint32_t o, uint32_t xm, uint32_t xp, uint32_t ym, uint32_t yp, uint32_t x, uint32_t y) {
+		const float dx = field[xp] - field[xm];
+		const float dy = field[yp] - field[ym];
+		// cgmath normalize: v * (1 / sqrt(x * x + y * y + z * z))
+		const float k = 1.0f / std::sqrt(dx * dx + dy * dy + 1.0f * 1.0f);
+		const Vec3 n = v3(dx * k, dy * k, 1.0f * k);
+		dst[o].packed_normal_roughness = pack_normal_roughness(n, roughness);
+		if(p->color_mode == SW_WATER_COLOR_DEFORMED)
+		{
+			const int32_t du = int32_t(n.x * tc_deform_scale), dv = int32_t(n.y * tc_deform_scale);
+			const uint32_t u = uint32_t(int32_t(x) + du) & (w - 1), v = uint32_t(int32_t(y) + dv) & (h - 1);
+			dst[o].diffuse = color[u + v * w].diffuse;
+		}
+		else if(p->color_mode == SW_WATER_COLOR_DEFORMED_X)
+		{
+			const int32_t du = int32_t(n.x * tc_deform_scale);
+			const uint32_t u = uint32_t(int32_t(x) + du) & (w - 1);
+			dst[o].diffuse = color[u + y * w].diffuse;
+		}
+	});
+	for(uint32_t mip = 1; mip <= 4; ++mip)
+		texture_mip(texels + levels[p->dst_level + mip - 1].offset, w >> (mip - 1), h >> (mip - 1), texels + levels[p->dst_level + mip].offset);
 	return 0;
 }
 

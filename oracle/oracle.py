@@ -45,6 +45,10 @@ def load():
         lib.oracle_shadow_factor.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]
         lib.oracle_make_skybox_side.restype = C.c_uint32
         lib.oracle_make_skybox_side.argtypes = [C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_void_p]
+        lib.oracle_water_step.restype = C.c_int
+        lib.oracle_water_step.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_void_p, C.c_uint32]
+        lib.oracle_water_make_texture.restype = C.c_int
+        lib.oracle_water_make_texture.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
         lib.oracle_apply_texture_ops.restype = C.c_int
         lib.oracle_apply_texture_ops.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
         lib.oracle_prepare_models.restype = C.c_int
@@ -176,6 +180,33 @@ def apply_texture_ops(map_data, ops, turb_shifts=None):
     if r != 0:
         raise ValueError("level out of range")
     return texels, lite
+
+
+def water_step(field, field_old, log2w, log2h, attenuation, points):
+    """One step of the water simulation (generative_texture_effect_water.rs:57-226 after its sources):
+    returns the new (field, field_old) pair, already swapped as the reference swaps them."""
+    lib = load()
+    f = np.ascontiguousarray(field, dtype=np.float32).copy()
+    fo = np.ascontiguousarray(field_old, dtype=np.float32).copy()
+    pts = np.ascontiguousarray(np.zeros(1, sw_abi.WAVE_POINT_DTYPE) if points is None or len(points) == 0 else points,
+                               dtype=sw_abi.WAVE_POINT_DTYPE)
+    n = 0 if points is None else len(points)
+    lib.oracle_water_step(C.c_void_p(f.ctypes.data), C.c_void_p(fo.ctypes.data), log2w, log2h, C.c_float(attenuation),
+                          C.c_void_p(pts.ctypes.data), n)
+    return fo, f
+
+
+def water_make_texture(map_data, field, log2w, log2h, params):
+    """make_wavy_texture + its mips (generative_texture_effect_water.rs:420-682, textures.rs:309-362)
+    on a copy of the map's texels; params is a sw_abi.WaterTextureParams. Returns the texels."""
+    lib = load()
+    texels = map_data.texels.copy()
+    f = np.ascontiguousarray(field, dtype=np.float32)
+    r = lib.oracle_water_make_texture(C.c_void_p(map_data.levels.ctypes.data), len(map_data.levels), C.c_void_p(texels.ctypes.data),
+                                      C.c_void_p(f.ctypes.data), log2w, log2h, C.byref(params))
+    if r != 0:
+        raise ValueError("level out of range")
+    return texels
 
 
 def skybox_side_texels(size):

@@ -63,16 +63,21 @@ struct DepthLaunch
 	float* texels;
 };
 
-// value of lane j = value of lane j - 1 + d, lane 0 = start (a chain of j float additions)
+// value of lane j = value of lane j - 1 + d, lane 0 = start (a chain of j float additions). Each
+// round moves every lane's value one lane up (the DPP wavefront shift wave_shr:1) and adds d; after
+// round s lanes 0 .. s hold their final values. Called with all 64 lanes active.
+SW_DEV float lane_shr1(float v)
+{
+	return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
+}
 SW_DEV float serial_chain(float start, float d)
 {
-	const uint32_t lane = threadIdx.x & 63u;
+	const bool first = (threadIdx.x & 63u) == 0;
 	float v = start;
 	for(uint32_t s = 1; s < 64; ++s)
 	{
-		const float p = __shfl(v, int(s - 1));
-		if(lane == s)
-			v = p + d;
+		const float p = lane_shr1(v) + d;
+		v = first ? start : p;
 	}
 	return v;
 }
@@ -201,6 +206,8 @@ __global__ __launch_bounds__(64) void k_depth_setup(DepthLaunch L)
 	}
 }
 
+#define DEPTH_LDS_ROW 4096u // rows up to this width are drawn in LDS, written back once
+
 __global__ __launch_bounds__(64) void k_depth_raster(DepthLaunch L)
 {
 	uint32_t lo = 0, hi = L.num_targets;
@@ -215,23 +222,63 @@ __global__ __launch_bounds__(64) void k_depth_raster(DepthLaunch L)
 	const DepthTargetDev t = L.targets[lo];
 	const int32_t y = int32_t(blockIdx.x - t.first_row_block);
 	const uint32_t lane = threadIdx.x;
-	float* row = L.texels + t.offset + uint64_t(y) * t.w;
-	for(uint32_t di = t.first_draw; di < t.first_draw + t.num_draws; ++di)
+	float* grow = L.texels + t.offset + uint64_t(y) * t.w;
+	__shared__ float srow[DEPTH_LDS_ROW];
+	const bool in_lds = t.w <= DEPTH_LDS_ROW;
+	float* row = in_lds ? srow : grow;
+	if(in_lds)
 	{
-		const int32_t y_first = L.draw_rows[di * 2], y_end = L.draw_rows[di * 2 + 1];
-		if(y < y_first || y >= y_end)
-			continue;
-		const DepthDrawDev d = L.draws[di];
-		const DepthRow r = L.rows[d.rec_base + uint32_t(y)];
-		float carry = r.inv_z;
-		for(uint32_t xb = r.x0; xb < r.x1; xb += 64)
+		for(uint32_t x = lane; x < t.w; x += 64)
+			srow[x] = grow[x];
+		__syncthreads();
+	}
+	const uint32_t draws_end = t.first_draw + t.num_draws;
+	for(uint32_t base = t.first_draw; base < draws_end; base += 64)
+	{
+		// lane k fetches draw base + k's record of this row; then the covering draws in order
+		const uint32_t di = base + lane;
+		int32_t span = 0, test = 0;
+		float inv_z = 0.0f, dx = 0.0f;
+		bool cov = false;
+		if(di < draws_end)
 		{
-			const float inv_z = serial_chain(carry, d.dx);
-			const uint32_t x = xb + lane;
-			if(x < r.x1)
-				row[x] = d.depth_test ? fmax_r(inv_z, row[x]) : inv_z;
-			carry = __shfl(inv_z, 63) + d.dx;
+			const int32_t y_first = L.draw_rows[di * 2], y_end = L.draw_rows[di * 2 + 1];
+			if(y >= y_first && y < y_end)
+			{
+				const DepthDrawDev d = L.draws[di];
+				const DepthRow r = L.rows[d.rec_base + uint32_t(y)];
+				cov = r.x0 < r.x1;
+				span = int32_t(uint32_t(r.x0) | (uint32_t(r.x1) << 16));
+				inv_z = r.inv_z;
+				dx = d.dx;
+				test = int32_t(d.depth_test);
+			}
 		}
+		uint64_t mask = __ballot(cov);
+		while(mask)
+		{
+			const int k = __builtin_ctzll(mask);
+			mask &= mask - 1;
+			const uint32_t sp = uint32_t(__builtin_amdgcn_readlane(span, k));
+			const uint32_t x0 = sp & 0xFFFFu, x1 = sp >> 16;
+			const float ddx = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, dx), k));
+			const bool dtest = __builtin_amdgcn_readlane(test, k) != 0;
+			float carry = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, inv_z), k));
+			for(uint32_t xb = x0; xb < x1; xb += 64)
+			{
+				const float v = serial_chain(carry, ddx);
+				const uint32_t x = xb + lane;
+				if(x < x1)
+					row[x] = dtest ? fmax_r(v, row[x]) : v;
+				carry = __shfl(v, 63) + ddx;
+			}
+		}
+	}
+	if(in_lds)
+	{
+		__syncthreads();
+		for(uint32_t x = lane; x < t.w; x += 64)
+			grow[x] = srow[x];
 	}
 }
 

@@ -281,6 +281,19 @@ size_t depth_row_record_bytes();
 void launch_depth(hipStream_t st, const char* args_dev, const DepthPlan& plan, const int32_t* vdata, void* rows, int32_t* draw_rows,
 	float* texels);
 // A skybox side (mip 0 from the image, then up to four halvings) back to back at dst.
+// water (generative_texture_effect_water.rs): one step (points, then the stencil into field_old) and
+// the wavy texture + mips (dst levels given by texel offsets)
+void launch_water_step(hipStream_t st, float* field, float* field_old, uint32_t log2w, uint32_t log2h, float attenuation,
+	const sw_wave_point* points_dev, uint32_t num_points);
+struct WaterTexLaunch
+{
+	uint64_t dst_offset[5]; // mip 0 .. 4
+	uint64_t color_offset;
+	uint32_t color_mode, init;
+	float roughness;
+	uint32_t base_color;
+};
+void launch_water_texture(hipStream_t st, const float* field, uint32_t log2w, uint32_t log2h, const WaterTexLaunch& w, sw_texel* texels);
 void launch_skybox_side(hipStream_t st, uint32_t color_bits, const void* image, uint32_t image_w, uint32_t size, int32_t brightness,
 	void* dst);
 } // namespace swk

@@ -67,7 +67,7 @@ struct sw_ctx
 	// command buffers / work buffers
 	DevBuf views, surfaces, lights, light_indices, draws, vdata, surf_ext, blk_surf, parts, spans, drec, dbounds, counters, arena, fb,
 		tonemap_out, tonemap_sums, bloom0, bloom1, bloom_lines, model_args, model_scratch, texop_args, shadows, depth_args, depth_vdata,
-		depth_rows, depth_draw_rows;
+		depth_rows, depth_draw_rows, water_points;
 	uint32_t num_shadows = 0; // sw_set_light_shadows: entries in `shadows`
 	const float* shadow_texels = nullptr;
 	uint32_t tile_rows = 8; // raster tile height (SW_TILE_ROWS_*), set by sw_set_tile_rows / the autotuner
@@ -801,7 +801,7 @@ void sw_ctx_destroy(sw_ctx* ctx)
 	DevBuf* bufs[] = {&ctx->views, &ctx->surfaces, &ctx->lights, &ctx->light_indices, &ctx->draws, &ctx->vdata, &ctx->surf_ext, &ctx->blk_surf,
 		&ctx->parts, &ctx->spans, &ctx->drec, &ctx->dbounds, &ctx->counters, &ctx->arena, &ctx->fb, &ctx->tonemap_out, &ctx->tonemap_sums,
 		&ctx->bloom0, &ctx->bloom1, &ctx->bloom_lines, &ctx->model_args, &ctx->model_scratch, &ctx->texop_args, &ctx->shadows, &ctx->depth_args, &ctx->depth_vdata,
-		&ctx->depth_rows, &ctx->depth_draw_rows};
+		&ctx->depth_rows, &ctx->depth_draw_rows, &ctx->water_points};
 	for(DevBuf* b : bufs)
 		release(ctx, *b);
 	if(ctx->counters_host)
@@ -1352,6 +1352,96 @@ int sw_apply_texture_ops(
 			static_cast<sw_texel*>(map->texels.p), static_cast<uint32_t*>(map->lite.p));
 	}
 	return hip_check(ctx, hipGetLastError(), "texture ops launch");
+}
+
+static bool water_size_ok(uint32_t log2w, uint32_t log2h)
+{
+	// GenerativeTextureEffectWater::new (generative_texture_effect_water.rs:21-34): sides of at least
+	// 2^MAX_MIP, area below 2^22
+	return log2w >= 4 && log2h >= 4 && log2w + log2h < 22;
+}
+
+int sw_water_step(sw_ctx* ctx, float* field_device, float* field_old_device, uint32_t log2_width, uint32_t log2_height,
+	float attenuation, const sw_wave_point* points, uint32_t num_points)
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	if(!field_device || !field_old_device || (num_points && !points))
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	if(!water_size_ok(log2_width, log2_height))
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "water field size out of range (sides >= 16, area < 2^22)");
+	if(field_device == field_old_device)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "the two field buffers must differ");
+	(void)hipSetDevice(ctx->device);
+	int r;
+	const sw_wave_point* pts_dev = nullptr;
+	if(num_points)
+	{
+		const size_t bytes = sizeof(sw_wave_point) * num_points;
+		char* staging = nullptr;
+		if((r = ensure(ctx, ctx->water_points, bytes)) || (r = stage_begin(ctx, bytes, &staging)))
+			return r;
+		std::memcpy(staging, points, bytes);
+		if((r = stage_copy(ctx, ctx->water_points.p, bytes)))
+			return r;
+		pts_dev = static_cast<const sw_wave_point*>(ctx->water_points.p);
+	}
+	{
+		StageTimer t(ctx, STAGE_TEXTURES);
+		swk::launch_water_step(ctx->stream, field_device, field_old_device, log2_width, log2_height, attenuation, pts_dev, num_points);
+	}
+	return hip_check(ctx, hipGetLastError(), "water step launch");
+}
+
+int sw_water_make_texture(sw_ctx* ctx, sw_map* map, const float* field_device, uint32_t log2_width, uint32_t log2_height,
+	const sw_water_texture_params* params)
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	if(!map || !field_device || !params)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	if(!water_size_ok(log2_width, log2_height))
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "water field size out of range (sides >= 16, area < 2^22)");
+	const sw_water_texture_params& p = *params;
+	if(p.color_mode > SW_WATER_COLOR_DEFORMED_X || p.init > SW_WATER_INIT_SOURCE)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "unknown water colour mode or init");
+	const std::vector<sw_texture_level>& lv = map->host_levels;
+	if(uint64_t(p.dst_level) + 4 >= lv.size())
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "water destination level (and its 4 mips) out of the level table");
+	swk::WaterTexLaunch w;
+	for(uint32_t mip = 0; mip <= 4; ++mip)
+	{
+		const sw_texture_level& l = lv[p.dst_level + mip];
+		if(l.width != (1u << (log2_width - mip)) || l.height != (1u << (log2_height - mip)) || (l.flags & SW_TEX_LITE_LEVEL))
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "water destination levels must be the field's size and its mips");
+		w.dst_offset[mip] = l.offset;
+	}
+	if(!(lv[p.dst_level].flags & SW_TEX_HAS_NORMAL_MAP))
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "water destination level lacks SW_TEX_HAS_NORMAL_MAP");
+	const bool needs_color = p.color_mode != SW_WATER_COLOR_KEEP || p.init == SW_WATER_INIT_SOURCE;
+	w.color_offset = 0;
+	if(needs_color)
+	{
+		if(p.color_level >= lv.size())
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "water colour level out of the level table");
+		const sw_texture_level& c = lv[p.color_level];
+		if(c.width != (1u << log2_width) || c.height != (1u << log2_height) || (c.flags & SW_TEX_LITE_LEVEL))
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "water colour level must have the field's size");
+		if(p.color_level >= p.dst_level && p.color_level <= p.dst_level + 4)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "water colour level is one of the destinations");
+		w.color_offset = c.offset;
+	}
+	w.color_mode = p.color_mode;
+	w.init = p.init;
+	// texture_data.material.roughness.max(MIN_VALID_ROUGHNESS).min(1.0) (:283)
+	w.roughness = std::fmin(std::fmax(p.roughness, 1.0f / 126.0f), 1.0f);
+	w.base_color = p.base_color;
+	(void)hipSetDevice(ctx->device);
+	{
+		StageTimer t(ctx, STAGE_TEXTURES);
+		swk::launch_water_texture(ctx->stream, field_device, log2_width, log2_height, w, static_cast<sw_texel*>(map->texels.p));
+	}
+	return hip_check(ctx, hipGetLastError(), "water texture launch");
 }
 
 int sw_render_depth(sw_ctx* ctx, const sw_depth_target* targets, uint32_t num_targets, const sw_depth_draw* draws, uint32_t num_draws,

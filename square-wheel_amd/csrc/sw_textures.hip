@@ -348,3 +348,171 @@ void launch_texture_ops(hipStream_t st, const char* args_dev, const TexOpPlan& p
 	hipLaunchKernelGGL(k_texture_ops, dim3(plan.blocks), dim3(TEXOP_BLOCK), 0, st, L);
 }
 } // namespace swk
+
+namespace
+{
+
+// ---------------------------------------------------------------------------------------------
+// Water (generative_texture_effect_water.rs)
+// ---------------------------------------------------------------------------------------------
+// add_point_value (:68-76) for the step's points, in order by one lane: points that share an
+// address add up in the reference's order.
+__global__ __launch_bounds__(64) void k_water_points(float* field, float* field_old, const sw_wave_point* pts, uint32_t n,
+	uint32_t log2w, uint32_t log2h)
+{
+	if(threadIdx.x != 0)
+		return;
+	const uint32_t mx = (1u << log2w) - 1u, my = (1u << log2h) - 1u;
+	for(uint32_t i = 0; i < n; ++i)
+	{
+		const sw_wave_point p = pts[i];
+		const uint32_t a = (p.x & mx) + ((p.y & my) << log2w);
+		field[a] += p.value;
+		field_old[a] -= p.value;
+	}
+}
+
+// update_wave_field (:307-418): the field is a torus (rows and columns wrap); one lane per element
+__global__ __launch_bounds__(256) void k_water_stencil(float* dst, const float* src, uint32_t log2w, uint32_t log2h, float attenuation)
+{
+	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+	if(i >> (log2w + log2h))
+		return;
+	const uint32_t mx = (1u << log2w) - 1u, my = (1u << log2h) - 1u;
+	const uint32_t x = i & mx, y = i >> log2w;
+	const uint32_t row = y << log2w, up = ((y - 1u) & my) << log2w, down = ((y + 1u) & my) << log2w;
+	const float sum = ((src[row + ((x - 1u) & mx)] + src[row + ((x + 1u) & mx)]) + src[up + x]) + src[down + x];
+	dst[i] = (sum * 0.5f - dst[i]) * attenuation;
+}
+
+// PackedNormalRoughness::pack (textures.rs:33-39, fast_math.rs:64-78): x 127 per lane, cvtps2dq,
+// signed saturation to bytes
+SW_DEV int32_t pack_normal_roughness_dev(float x, float y, float z, float r)
+{
+	const float v[4] = {x * 127.0f, y * 127.0f, z * 127.0f, r * 127.0f};
+	uint32_t out = 0;
+	for(int k = 0; k < 4; ++k)
+	{
+		const int32_t q0 = f32_to_i32_rne_x86(v[k]);
+		const int32_t q = q0 < -128 ? -128 : q0 > 127 ? 127 : q0;
+		out |= (uint32_t(q) & 0xFFu) << (8 * k);
+	}
+	return int32_t(out);
+}
+
+// make_wavy_texture_impl (:497-682) per texel: the normal from the field's central differences,
+// normalised as cgmath's normalize (v * (1 / |v|)), colours by mode
+__global__ __launch_bounds__(256) void k_water_texture(const float* field, uint32_t log2w, uint32_t log2h, swk::WaterTexLaunch w, sw_texel* texels)
+{
+	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+	if(i >> (log2w + log2h))
+		return;
+	const uint32_t mx = (1u << log2w) - 1u, my = (1u << log2h) - 1u;
+	const uint32_t x = i & mx, y = i >> log2w;
+	const uint32_t row = y << log2w, up = ((y - 1u) & my) << log2w, down = ((y + 1u) & my) << log2w;
+	const float dx = field[row + ((x + 1u) & mx)] - field[row + ((x - 1u) & mx)];
+	const float dy = field[down + x] - field[up + x];
+	const float m2 = (dx * dx + dy * dy) + 1.0f;
+	const float s = 1.0f / __builtin_sqrtf(m2);
+	const float nx = dx * s, ny = dy * s, nz = s;
+	sw_texel* out = texels + w.dst_offset[0] + i;
+	const sw_texel* color = texels + w.color_offset;
+	uint32_t diffuse;
+	if(w.color_mode == SW_WATER_COLOR_KEEP)
+		diffuse = w.init == SW_WATER_INIT_BASE_COLOR ? w.base_color : w.init == SW_WATER_INIT_SOURCE ? color[i].diffuse : out->diffuse;
+	else
+	{
+		const int32_t du = int32_t(nx * 16.0f);
+		const uint32_t u = (x + uint32_t(du)) & mx;
+		if(w.color_mode == SW_WATER_COLOR_DEFORMED)
+		{
+			const int32_t dv = int32_t(ny * 16.0f);
+			const uint32_t v = (y + uint32_t(dv)) & my;
+			diffuse = color[u + (v << log2w)].diffuse;
+		}
+		else
+			diffuse = color[u + row].diffuse;
+	}
+	sw_texel t;
+	t.diffuse = diffuse;
+	t.packed_normal_roughness = pack_normal_roughness_dev(nx, ny, nz, w.roughness);
+	*out = t;
+}
+
+// Color32::get_average_4 (color.rs:51-71)
+SW_DEV uint32_t average_4(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3)
+{
+	uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+	const uint32_t c[4] = {c0, c1, c2, c3};
+	for(int k = 0; k < 4; ++k)
+	{
+		s0 += c[k] & 0x000000FFu;
+		s1 += c[k] & 0x0000FF00u;
+		s2 += c[k] & 0x00FF0000u;
+		s3 += (c[k] & 0xFF000000u) >> 2;
+	}
+	return (s0 >> 2) | ((s1 >> 2) & 0x0000FF00u) | ((s2 >> 2) & 0x00FF0000u) | (s3 & 0xFF000000u);
+}
+
+SW_DEV void unpack_normal_roughness_dev(int32_t p, float& x, float& y, float& z, float& r)
+{
+	const float k = 1.0f / 127.0f;
+	x = float(int8_t(p & 0xFF)) * k;
+	y = float(int8_t((p >> 8) & 0xFF)) * k;
+	z = float(int8_t((p >> 16) & 0xFF)) * k;
+	r = float(int8_t((p >> 24) & 0xFF)) * k;
+}
+
+// build_texture_mip (textures.rs:309-362): one lane per texel of the smaller level
+__global__ __launch_bounds__(256) void k_texture_mip(const sw_texel* prev, sw_texel* out, uint32_t log2w, uint32_t log2h)
+{
+	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+	if(i >> (log2w + log2h))
+		return;
+	const uint32_t x = i & ((1u << log2w) - 1u), y = i >> log2w, pw = 2u << log2w;
+	const sw_texel p00 = prev[(2u * y) * pw + 2u * x], p01 = prev[(2u * y + 1u) * pw + 2u * x];
+	const sw_texel p10 = prev[(2u * y) * pw + 2u * x + 1u], p11 = prev[(2u * y + 1u) * pw + 2u * x + 1u];
+	float n[4][3], r[4];
+	unpack_normal_roughness_dev(p00.packed_normal_roughness, n[0][0], n[0][1], n[0][2], r[0]);
+	unpack_normal_roughness_dev(p01.packed_normal_roughness, n[1][0], n[1][1], n[1][2], r[1]);
+	unpack_normal_roughness_dev(p10.packed_normal_roughness, n[2][0], n[2][1], n[2][2], r[2]);
+	unpack_normal_roughness_dev(p11.packed_normal_roughness, n[3][0], n[3][1], n[3][2], r[3]);
+	const float sx = ((n[0][0] + n[1][0]) + n[2][0]) + n[3][0];
+	const float sy = ((n[0][1] + n[1][1]) + n[2][1]) + n[3][1];
+	const float sz = ((n[0][2] + n[1][2]) + n[2][2]) + n[3][2];
+	const float len = __builtin_fmaxf(__builtin_sqrtf((sx * sx + sy * sy) + sz * sz), 0.000001f);
+	const float deviation = (1.0f - len * 0.25f) / 0.5f;
+	const float rs = ((r[0] + r[1]) + r[2]) + r[3];
+	const float dr = __builtin_fminf(__builtin_fmaxf(__builtin_fmaf(rs, 0.25f, deviation), 1.0f / 126.0f), 1.0f);
+	sw_texel t;
+	t.diffuse = average_4(p00.diffuse, p01.diffuse, p10.diffuse, p11.diffuse);
+	t.packed_normal_roughness = pack_normal_roughness_dev(sx / len, sy / len, sz / len, dr);
+	out[i] = t;
+}
+
+} // namespace
+
+namespace swk
+{
+void launch_water_step(hipStream_t st, float* field, float* field_old, uint32_t log2w, uint32_t log2h, float attenuation,
+	const sw_wave_point* points_dev, uint32_t num_points)
+{
+	if(num_points)
+		hipLaunchKernelGGL(k_water_points, dim3(1), dim3(64), 0, st, field, field_old, points_dev, num_points, log2w, log2h);
+	const uint32_t area = 1u << (log2w + log2h);
+	hipLaunchKernelGGL(k_water_stencil, dim3((area + 255u) / 256u), dim3(256), 0, st, field_old, static_cast<const float*>(field), log2w,
+		log2h, attenuation);
+}
+
+void launch_water_texture(hipStream_t st, const float* field, uint32_t log2w, uint32_t log2h, const WaterTexLaunch& w, sw_texel* texels)
+{
+	const uint32_t area = 1u << (log2w + log2h);
+	hipLaunchKernelGGL(k_water_texture, dim3((area + 255u) / 256u), dim3(256), 0, st, field, log2w, log2h, w, texels);
+	for(uint32_t mip = 1; mip <= 4; ++mip)
+	{
+		const uint32_t lw = log2w - mip, lh = log2h - mip, n = 1u << (lw + lh);
+		hipLaunchKernelGGL(k_texture_mip, dim3((n + 255u) / 256u), dim3(256), 0, st, static_cast<const sw_texel*>(texels + w.dst_offset[mip - 1]),
+			texels + w.dst_offset[mip], lw, lh);
+	}
+}
+} // namespace swk

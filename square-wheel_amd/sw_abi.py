@@ -66,6 +66,17 @@ MODEL_VERTEX_DTYPE = np.dtype([("pos", "<f4", 3), ("tc", "<f4", 2), ("light", "<
 SHADOW_NONE, SHADOW_CUBE, SHADOW_PROJECTOR = 0, 1, 2
 LIGHT_SHADOW_DTYPE = np.dtype([("kind", "<u4"), ("size", "<u4"), ("offset", "<u8"), ("basis_x", "<f4", 3), ("basis_y", "<f4", 3),
                                ("basis_z", "<f4", 3), ("reserved", "<u4")], align=True)
+# water (sw_water_step / sw_water_make_texture)
+WAVE_POINT_DTYPE = np.dtype([("x", "<u4"), ("y", "<u4"), ("value", "<f4")])
+WATER_COLOR_KEEP, WATER_COLOR_DEFORMED, WATER_COLOR_DEFORMED_X = 0, 1, 2
+WATER_INIT_NONE, WATER_INIT_BASE_COLOR, WATER_INIT_SOURCE = 0, 1, 2
+
+
+class WaterTextureParams(C.Structure):
+    _fields_ = [("dst_level", C.c_uint32), ("color_level", C.c_uint32), ("color_mode", C.c_uint32), ("init", C.c_uint32),
+                ("roughness", C.c_float), ("base_color", C.c_uint32)]
+
+
 # depth buffers (sw_render_depth)
 DEPTH_DRAW_DTYPE = np.dtype([("first_vertex_int", "<u4"), ("num_vertices", "<u4"), ("depth_equation", "<f4", 3),
                              ("depth_test", "<u4")], align=True)
@@ -294,6 +305,9 @@ GPU_SYMBOLS = {
     "sw_render_depth": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64,
                                   C.c_void_p, C.c_uint64]),
     "sw_set_light_shadows": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]),
+    "sw_water_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_void_p, C.c_uint32]),
+    "sw_water_make_texture": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                                        C.POINTER(WaterTextureParams)]),
     "sw_apply_texture_ops": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]),
     "sw_map_read": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p]),
     "sw_make_skybox_side": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float,
@@ -481,6 +495,34 @@ class Context:
             return buf.read(np.float32, len(tx))
         finally:
             buf.free()
+
+    # water ---------------------------------------------------------------------------------
+    def water_steps(self, field, field_old, log2w, log2h, attenuation, steps_points):
+        """sw_water_step once per entry of steps_points (WAVE_POINT_DTYPE arrays) on device copies of
+        the two fields, swapping them after each step as the reference does; returns (field, field_old)."""
+        a, b = self.to_device(np.ascontiguousarray(field, np.float32)), self.to_device(np.ascontiguousarray(field_old, np.float32))
+        try:
+            for pts in steps_points:
+                pts = np.ascontiguousarray(pts, dtype=WAVE_POINT_DTYPE)
+                self.check(self.lib.sw_water_step(self.h, a.ptr, b.ptr, log2w, log2h, C.c_float(attenuation),
+                                                  _ptr(pts) if len(pts) else None, len(pts)), "sw_water_step")
+                a, b = b, a
+            self.check(self.lib.sw_sync(self.h), "sw_sync")
+            n = 1 << (log2w + log2h)
+            return a.read(np.float32, n), b.read(np.float32, n)
+        finally:
+            a.free()
+            b.free()
+
+    def water_make_texture(self, map_handle, field, log2w, log2h, params):
+        """sw_water_make_texture from a host field (uploaded here) into the uploaded map's levels."""
+        f = self.to_device(np.ascontiguousarray(field, np.float32))
+        try:
+            self.check(self.lib.sw_water_make_texture(self.h, map_handle, f.ptr, log2w, log2h, C.byref(params)),
+                       "sw_water_make_texture")
+            self.check(self.lib.sw_sync(self.h), "sw_sync")
+        finally:
+            f.free()
 
     # texture animation --------------------------------------------------------------------
     def apply_texture_ops(self, map_handle, ops, turb_shifts=None):
