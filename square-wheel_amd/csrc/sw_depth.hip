@@ -9,8 +9,9 @@
 //
 //   k_depth_setup   one wave per draw: the edge walk (lane 0) into parts, then per row of each part
 //                   the span's x range and its first pixel's 1 / z (the rows' chain by the scan)
-//   k_depth_raster  one wave per depth-buffer row: the target's draws in order, each covering span
-//                   filled chunk by chunk (the pixels' chain by the scan)
+//   k_depth_raster  one wave per four depth-buffer rows, a 16-lane row of the wave per buffer row:
+//                   the target's draws in order, each covering span filled 16 pixels at a time
+//                   (the pixels' chain by a 16-lane scan, the four rows' chains side by side)
 #include <cstring>
 #include <string>
 #include <vector>
@@ -47,7 +48,7 @@ struct DepthTargetDev
 	uint64_t offset; // of the depth buffer in the texels
 	uint32_t w, h;
 	uint32_t first_draw, num_draws;
-	uint32_t first_row_block; // raster blocks: first_row_block + y
+	uint32_t first_row_block; // raster blocks: first_row_block + y / DEPTH_ROWS_PER_BLOCK
 	uint32_t pad;
 };
 
@@ -69,6 +70,23 @@ struct DepthLaunch
 SW_DEV float lane_shr1(float v)
 {
 	return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
+}
+// within each 16-lane row of the wave: sublane j gets sublane j - 1's value (DPP row_shr:1), and
+// the chain of four independent 16-value pieces (sublane 0 of each row takes its own start)
+SW_DEV float row_shr1(float v)
+{
+	return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x111, 0xF, 0xF, false));
+}
+SW_DEV float serial_chain16(float start, float d)
+{
+	const bool first = (threadIdx.x & 15u) == 0;
+	float v = start;
+	for(uint32_t s = 1; s < 16; ++s)
+	{
+		const float p = row_shr1(v) + d;
+		v = first ? start : p;
+	}
+	return v;
 }
 SW_DEV float serial_chain(float start, float d)
 {
@@ -206,7 +224,8 @@ __global__ __launch_bounds__(64) void k_depth_setup(DepthLaunch L)
 	}
 }
 
-#define DEPTH_LDS_ROW 4096u // rows up to this width are drawn in LDS, written back once
+#define DEPTH_ROWS_PER_BLOCK 4u // one 16-lane row of the wave per depth-buffer row
+#define DEPTH_LDS_WIDTH 1024u // rows up to this width are drawn in LDS, written back once
 
 __global__ __launch_bounds__(64) void k_depth_raster(DepthLaunch L)
 {
@@ -220,65 +239,77 @@ __global__ __launch_bounds__(64) void k_depth_raster(DepthLaunch L)
 			hi = mid;
 	}
 	const DepthTargetDev t = L.targets[lo];
-	const int32_t y = int32_t(blockIdx.x - t.first_row_block);
-	const uint32_t lane = threadIdx.x;
-	float* grow = L.texels + t.offset + uint64_t(y) * t.w;
-	__shared__ float srow[DEPTH_LDS_ROW];
-	const bool in_lds = t.w <= DEPTH_LDS_ROW;
-	float* row = in_lds ? srow : grow;
+	const uint32_t lane = threadIdx.x, g = lane >> 4, sub = lane & 15u;
+	const int32_t y = int32_t((blockIdx.x - t.first_row_block) * DEPTH_ROWS_PER_BLOCK + g);
+	const bool row_ok = y < int32_t(t.h);
+	__shared__ float srow[DEPTH_ROWS_PER_BLOCK * DEPTH_LDS_WIDTH];
+	const bool in_lds = t.w <= DEPTH_LDS_WIDTH;
+	float* grow = L.texels + t.offset + uint64_t(row_ok ? y : 0) * t.w;
+	float* row = in_lds ? srow + g * DEPTH_LDS_WIDTH : grow;
 	if(in_lds)
 	{
-		for(uint32_t x = lane; x < t.w; x += 64)
-			srow[x] = grow[x];
+		if(row_ok)
+			for(uint32_t x = sub; x < t.w; x += 16)
+				row[x] = grow[x];
 		__syncthreads();
 	}
 	const uint32_t draws_end = t.first_draw + t.num_draws;
-	for(uint32_t base = t.first_draw; base < draws_end; base += 64)
+	for(uint32_t base = t.first_draw; base < draws_end; base += 16)
 	{
-		// lane k fetches draw base + k's record of this row; then the covering draws in order
-		const uint32_t di = base + lane;
-		int32_t span = 0, test = 0;
+		// lane (g, j) fetches draw base + j and its record of row y
+		const uint32_t di = base + sub;
+		uint32_t span = 0;
 		float inv_z = 0.0f, dx = 0.0f;
-		bool cov = false;
+		int32_t test = 0;
 		if(di < draws_end)
 		{
+			const DepthDrawDev d = L.draws[di];
+			dx = d.dx;
+			test = int32_t(d.depth_test);
 			const int32_t y_first = L.draw_rows[di * 2], y_end = L.draw_rows[di * 2 + 1];
-			if(y >= y_first && y < y_end)
+			if(row_ok && y >= y_first && y < y_end)
 			{
-				const DepthDrawDev d = L.draws[di];
 				const DepthRow r = L.rows[d.rec_base + uint32_t(y)];
-				cov = r.x0 < r.x1;
-				span = int32_t(uint32_t(r.x0) | (uint32_t(r.x1) << 16));
-				inv_z = r.inv_z;
-				dx = d.dx;
-				test = int32_t(d.depth_test);
+				if(r.x0 < r.x1)
+				{
+					span = uint32_t(r.x0) | (uint32_t(r.x1) << 16);
+					inv_z = r.inv_z;
+				}
 			}
 		}
-		uint64_t mask = __ballot(cov);
-		while(mask)
+		const uint64_t any = __ballot(span != 0u);
+		uint32_t draw_mask = uint32_t((any | (any >> 16) | (any >> 32) | (any >> 48)) & 0xFFFFu);
+		while(draw_mask)
 		{
-			const int k = __builtin_ctzll(mask);
-			mask &= mask - 1;
-			const uint32_t sp = uint32_t(__builtin_amdgcn_readlane(span, k));
-			const uint32_t x0 = sp & 0xFFFFu, x1 = sp >> 16;
-			const float ddx = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, dx), k));
-			const bool dtest = __builtin_amdgcn_readlane(test, k) != 0;
-			float carry = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, inv_z), k));
-			for(uint32_t xb = x0; xb < x1; xb += 64)
+			// the draws in order; each group draws its own row's span
+			const int j = __builtin_ctz(draw_mask);
+			draw_mask &= draw_mask - 1u;
+			const float ddx = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, dx), j));
+			const bool dtest = __builtin_amdgcn_readlane(test, j) != 0;
+			const int src = int((g << 4) | uint32_t(j));
+			const uint32_t sp = uint32_t(__shfl(int(span), src));
+			float carry = __shfl(inv_z, src);
+			const uint32_t x0 = sp & 0xFFFFu, len = (sp >> 16) - x0; // 0 where the row is not covered (sp = 0)
+			const uint32_t len_all = max(max(uint32_t(__shfl(int(len), 0)), uint32_t(__shfl(int(len), 16))),
+				max(uint32_t(__shfl(int(len), 32)), uint32_t(__shfl(int(len), 48))));
+			for(uint32_t c = 0; c < len_all; c += 16)
 			{
-				const float v = serial_chain(carry, ddx);
-				const uint32_t x = xb + lane;
-				if(x < x1)
+				const float v = serial_chain16(carry, ddx);
+				if(c + sub < len)
+				{
+					const uint32_t x = x0 + c + sub;
 					row[x] = dtest ? fmax_r(v, row[x]) : v;
-				carry = __shfl(v, 63) + ddx;
+				}
+				carry = __shfl(v, int((g << 4) | 15u)) + ddx;
 			}
 		}
 	}
 	if(in_lds)
 	{
 		__syncthreads();
-		for(uint32_t x = lane; x < t.w; x += 64)
-			grow[x] = srow[x];
+		if(row_ok)
+			for(uint32_t x = sub; x < t.w; x += 16)
+				grow[x] = row[x];
 	}
 }
 
@@ -307,7 +338,7 @@ int depth_plan(const sw_depth_target* targets, uint32_t num_targets, const sw_de
 		if(uint64_t(row_blocks) + t.height > 0x7FFFFFFFu)
 			return (error = "depth targets: too many rows"), -1;
 		tg[i] = DepthTargetDev{t.offset, t.width, t.height, t.first_draw, t.num_draws, row_blocks, 0};
-		row_blocks += t.height;
+		row_blocks += (t.height + DEPTH_ROWS_PER_BLOCK - 1) / DEPTH_ROWS_PER_BLOCK;
 		for(uint32_t k = t.first_draw; k < t.first_draw + t.num_draws; ++k)
 		{
 			const sw_depth_draw& s = draws[k];

@@ -55,6 +55,8 @@ struct sw_map
 	std::vector<sw_texture_level> host_levels;
 };
 
+#define ARG_STAGING_SLOTS 4u
+
 struct sw_ctx
 {
 	int device = 0;
@@ -119,9 +121,12 @@ struct sw_ctx
 		// the surface blocks, which then run in the setup launch)
 		uint32_t surface_mode = 0;
 	} last;
-	char* arg_staging = nullptr; // pinned: packed launch arguments (model preparation, texture ops)
-	size_t arg_staging_cap = 0;
-	hipEvent_t arg_staging_event = nullptr; // recorded after the last copy out of arg_staging
+	// pinned: packed launch arguments (model preparation, texture ops, depth draws, shadows), a ring
+	// of slots so that the host fills the next slot while earlier copies wait in the stream
+	char* arg_staging[ARG_STAGING_SLOTS] = {};
+	size_t arg_staging_cap[ARG_STAGING_SLOTS] = {};
+	hipEvent_t arg_staging_event[ARG_STAGING_SLOTS] = {}; // recorded after the last copy out of the slot
+	uint32_t arg_staging_slot = 0; // the slot of the current stage_begin
 	std::vector<char> texop_packed;
 	std::vector<char> depth_packed;
 	std::vector<SurfExt> host_ext;
@@ -806,10 +811,13 @@ void sw_ctx_destroy(sw_ctx* ctx)
 		release(ctx, *b);
 	if(ctx->counters_host)
 		(void)hipHostFree(ctx->counters_host);
-	if(ctx->arg_staging)
-		(void)hipHostFree(ctx->arg_staging);
-	if(ctx->arg_staging_event)
-		(void)hipEventDestroy(ctx->arg_staging_event);
+	for(uint32_t k = 0; k < ARG_STAGING_SLOTS; ++k)
+	{
+		if(ctx->arg_staging[k])
+			(void)hipHostFree(ctx->arg_staging[k]);
+		if(ctx->arg_staging_event[k])
+			(void)hipEventDestroy(ctx->arg_staging_event[k]);
+	}
 	collect_timings(ctx);
 	for(hipEvent_t e : ctx->event_pool)
 		(void)hipEventDestroy(e);
@@ -1204,39 +1212,44 @@ int sw_postprocess_device(sw_ctx* ctx, const void* hdr_device, uint32_t width, u
 	return SW_OK;
 }
 
-// Small launch arguments travel from pinned staging memory in one copy. stage_begin waits until
-// the previous copy out of the staging memory has completed (its event), grows it if needed and
-// returns it; stage_copy queues the copy and records the event.
+// Small launch arguments travel from pinned staging memory. stage_begin takes the next slot of the
+// ring, waits until the copies out of it (ARG_STAGING_SLOTS calls ago) have completed (its event),
+// grows it if needed and returns it; stage_copy queues a copy out of the slot (at src_offset) and
+// records the slot's event after it.
 static int stage_begin(sw_ctx* ctx, size_t bytes, char** out)
 {
 	int r;
-	if(ctx->arg_staging_event && (r = hip_check(ctx, hipEventSynchronize(ctx->arg_staging_event), "argument staging wait")))
+	const uint32_t k = (ctx->arg_staging_slot + 1) % ARG_STAGING_SLOTS;
+	ctx->arg_staging_slot = k;
+	if(ctx->arg_staging_event[k] && (r = hip_check(ctx, hipEventSynchronize(ctx->arg_staging_event[k]), "argument staging wait")))
 		return r;
-	if(bytes > ctx->arg_staging_cap)
+	if(bytes > ctx->arg_staging_cap[k])
 	{
-		if(ctx->arg_staging)
-			(void)hipHostFree(ctx->arg_staging);
-		ctx->arg_staging = nullptr;
-		ctx->arg_staging_cap = 0;
+		if(ctx->arg_staging[k])
+			(void)hipHostFree(ctx->arg_staging[k]);
+		ctx->arg_staging[k] = nullptr;
+		ctx->arg_staging_cap[k] = 0;
 		const size_t cap = std::max<size_t>(bytes * 2, 64 << 10);
-		if((r = hip_check(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->arg_staging), cap, hipHostMallocDefault),
+		if((r = hip_check(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->arg_staging[k]), cap, hipHostMallocDefault),
 				"argument staging allocation")))
 			return r;
-		ctx->arg_staging_cap = cap;
+		ctx->arg_staging_cap[k] = cap;
 	}
-	if(!ctx->arg_staging_event &&
-		(r = hip_check(ctx, hipEventCreateWithFlags(&ctx->arg_staging_event, hipEventDisableTiming), "argument staging event")))
+	if(!ctx->arg_staging_event[k] &&
+		(r = hip_check(ctx, hipEventCreateWithFlags(&ctx->arg_staging_event[k], hipEventDisableTiming), "argument staging event")))
 		return r;
-	*out = ctx->arg_staging;
+	*out = ctx->arg_staging[k];
 	return SW_OK;
 }
 
-static int stage_copy(sw_ctx* ctx, void* dst_device, size_t bytes)
+static int stage_copy(sw_ctx* ctx, void* dst_device, size_t bytes, size_t src_offset = 0)
 {
 	int r;
-	if((r = hip_check(ctx, hipMemcpyAsync(dst_device, ctx->arg_staging, bytes, hipMemcpyHostToDevice, ctx->stream), "argument copy")))
+	const uint32_t k = ctx->arg_staging_slot;
+	if((r = hip_check(ctx, hipMemcpyAsync(dst_device, ctx->arg_staging[k] + src_offset, bytes, hipMemcpyHostToDevice, ctx->stream),
+			"argument copy")))
 		return r;
-	return hip_check(ctx, hipEventRecord(ctx->arg_staging_event, ctx->stream), "argument staging event");
+	return hip_check(ctx, hipEventRecord(ctx->arg_staging_event[k], ctx->stream), "argument staging event");
 }
 
 int sw_prepare_models(sw_ctx* ctx, const sw_model_mesh* meshes, uint32_t num_meshes, const sw_model_instance* instances,
@@ -1464,14 +1477,15 @@ int sw_render_depth(sw_ctx* ctx, const sw_depth_target* targets, uint32_t num_ta
 		(r = ensure(ctx, ctx->depth_rows, std::max<uint64_t>(plan.row_records, 1) * swk::depth_row_record_bytes())) ||
 		(r = ensure(ctx, ctx->depth_draw_rows, sizeof(int32_t) * 2 * std::max<uint32_t>(num_draws, 1))))
 		return r;
+	// the packed draws and the vertex data through one staging slot (no copy from pageable memory)
 	char* staging = nullptr;
-	if((r = stage_begin(ctx, plan.bytes, &staging)))
+	const size_t vat = (plan.bytes + 255) & ~size_t(255);
+	if((r = stage_begin(ctx, vat + vbytes, &staging)))
 		return r;
 	std::memcpy(staging, ctx->depth_packed.data(), plan.bytes);
-	if((r = stage_copy(ctx, ctx->depth_args.p, plan.bytes)))
-		return r;
-	if(vbytes && (r = hip_check(ctx, hipMemcpyAsync(ctx->depth_vdata.p, vertex_data, vbytes, hipMemcpyHostToDevice, ctx->stream),
-					  "depth vertex data")))
+	if(vbytes)
+		std::memcpy(staging + vat, vertex_data, vbytes);
+	if((r = stage_copy(ctx, ctx->depth_args.p, plan.bytes)) || (vbytes && (r = stage_copy(ctx, ctx->depth_vdata.p, vbytes, vat))))
 		return r;
 	{
 		StageTimer t(ctx, STAGE_DEPTH);
