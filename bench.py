@@ -262,6 +262,63 @@ def shadows_leg(ctx, m, mh, args):
     return res
 
 
+def water_leg(ctx, args):
+    """Water textures: a 1024x1024 field (the largest the effect allows is 2^21 elements) stepped
+    four times per frame with 16 wave-source points per step (sw_water_step), then the wavy texture
+    with source-colour deformation and its four mips (sw_water_make_texture), into a map texture."""
+    import ctypes as C
+    lw = lh = 10
+    n = 1 << (lw + lh)
+    m = sw_host.Map(seed=args.seed + 9, grid=(10, 10, 4), texture_size=1024, num_materials=2, num_decal_textures=1,
+                    num_model_textures=1)
+    lv = m.data.levels
+    lv["flags"][0:5] |= 1
+    mh = ctx.upload_map(m.data)
+    rng = np.random.default_rng(args.seed + 11)
+    a = ctx.to_device(rng.normal(size=n).astype(np.float32) * np.float32(0.1))
+    b = ctx.to_device(np.zeros(n, np.float32))
+    pts = np.zeros(16, dtype=sw_abi.WAVE_POINT_DTYPE)
+    pts["x"], pts["y"], pts["value"] = rng.integers(0, 1024, 16), rng.integers(0, 1024, 16), 1.0
+    params = sw_abi.WaterTextureParams(0, 5, sw_abi.WATER_COLOR_DEFORMED, 0, 0.4, 0)
+    lib = ctx.lib
+    att = C.c_float(1.0 - 1.0 / 100.0)
+
+    def frame():
+        nonlocal a, b
+        for _ in range(4):
+            ctx.check(lib.sw_water_step(ctx.h, a.ptr, b.ptr, lw, lh, att, C.c_void_p(pts.ctypes.data), len(pts)), "sw_water_step")
+            a, b = b, a
+        ctx.check(lib.sw_water_make_texture(ctx.h, mh, a.ptr, lw, lh, C.byref(params)), "sw_water_make_texture")
+
+    for _ in range(8):
+        frame()
+    ctx.check(lib.sw_sync(ctx.h), "sw_sync")
+    reps = 100
+    ctx.profile_enable(True)
+    ctx.profile_read(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        frame()
+    ctx.check(lib.sw_sync(ctx.h), "sw_sync")
+    wall = (time.perf_counter() - t0) / reps
+    ctx.profile_enable(False)
+    prof = ctx.profile_read(reset=True)
+    launches = prof["textures"][1]
+    kus = prof["textures"][0] / max(reps, 1) * 1e3  # event-timed device time per frame (4 steps + texture)
+    a.free()
+    b.free()
+    ctx.release_map(mh)
+    # algorithmic bytes per frame: each step reads and writes the two fields (4 + 4 + 4 B per element),
+    # the texture reads the field and the colour level and writes 8 B texels, the mips 1/3 more
+    step_bytes = 12 * n
+    tex_bytes = (4 + 8 + 8) * n + (8 + 8) * n // 3
+    res = {"workload": "1024x1024 water field, 4 steps x 16 source points + wavy texture (deformed colours) + 4 mips per frame",
+           "us_per_frame": round(wall * 1e6, 2), "device_us_per_frame": round(kus, 2), "event_launches": int(launches),
+           "gb_s": round((4 * step_bytes + tex_bytes) / (kus * 1e-6) / 1e9, 1)}
+    log(f"water: {res}")
+    return res
+
+
 def models_leg(ctx, args):
     """Triangle-model preparation (sw_prepare_models): 64 instances of a 1280-triangle, 642-vertex
     vertex-animated mesh (8 frames), meshes resident in HBM, outputs in HBM; the kernel's duration
@@ -427,6 +484,7 @@ def main():
     ap.add_argument("--no-models", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-shadows", action="store_true")
+    ap.add_argument("--no-water", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--hdr-poses", type=int, default=16)
     ap.add_argument("--hdr-steps", type=int, default=64)
@@ -621,6 +679,9 @@ def main():
 
     if not args.no_models:
         result["models"] = models_leg(ctx, args)
+
+    if not args.no_water:
+        result["water"] = water_leg(ctx, args)
 
     if not args.no_serve:
         result["serving"] = serving_leg(args)

@@ -355,20 +355,47 @@ namespace
 // ---------------------------------------------------------------------------------------------
 // Water (generative_texture_effect_water.rs)
 // ---------------------------------------------------------------------------------------------
-// add_point_value (:68-76) for the step's points, in order by one lane: points that share an
-// address add up in the reference's order.
+// add_point_value (:68-76) for the step's points, 64 at a time: the first point of each address in
+// a batch adds the batch's values for that address in order (points at one address add up in the
+// reference's order; distinct addresses go in parallel); batches follow each other.
 __global__ __launch_bounds__(64) void k_water_points(float* field, float* field_old, const sw_wave_point* pts, uint32_t n,
 	uint32_t log2w, uint32_t log2h)
 {
-	if(threadIdx.x != 0)
-		return;
+	__shared__ uint32_t sa[64];
+	__shared__ float sv[64];
+	const uint32_t lane = threadIdx.x;
 	const uint32_t mx = (1u << log2w) - 1u, my = (1u << log2h) - 1u;
-	for(uint32_t i = 0; i < n; ++i)
+	for(uint32_t base = 0; base < n; base += 64)
 	{
-		const sw_wave_point p = pts[i];
-		const uint32_t a = (p.x & mx) + ((p.y & my) << log2w);
-		field[a] += p.value;
-		field_old[a] -= p.value;
+		const uint32_t cnt = n - base < 64u ? n - base : 64u;
+		uint32_t a = 0;
+		if(lane < cnt)
+		{
+			const sw_wave_point p = pts[base + lane];
+			a = (p.x & mx) + ((p.y & my) << log2w);
+			sa[lane] = a;
+			sv[lane] = p.value;
+		}
+		__syncthreads();
+		if(lane < cnt)
+		{
+			bool first = true;
+			for(uint32_t k = 0; k < lane; ++k)
+				first = first && sa[k] != a;
+			if(first)
+			{
+				float f = field[a], fo = field_old[a];
+				for(uint32_t k = lane; k < cnt; ++k)
+					if(sa[k] == a)
+					{
+						f += sv[k];
+						fo -= sv[k];
+					}
+				field[a] = f;
+				field_old[a] = fo;
+			}
+		}
+		__syncthreads();
 	}
 }
 
