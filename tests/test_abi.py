@@ -59,6 +59,8 @@ STRUCTS = {
     "sw_light_shadow": sw_abi.LIGHT_SHADOW_DTYPE,
     "sw_depth_draw": sw_abi.DEPTH_DRAW_DTYPE,
     "sw_depth_target": sw_abi.DEPTH_TARGET_DTYPE,
+    "sw_wave_point": sw_abi.WAVE_POINT_DTYPE,
+    "sw_water_texture_params": np.dtype([(n, "<u4" if t is not ctypes.c_float else "<f4") for n, t in sw_abi.WaterTextureParams._fields_]),
 }
 
 
