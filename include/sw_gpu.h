@@ -431,7 +431,13 @@ typedef struct sw_depth_target
 	uint64_t offset; /* of the width x height buffer in the depth texels */
 	uint32_t width, height; /* at most 65535 */
 	uint32_t first_draw, num_draws;
+	uint32_t flags; /* SW_DEPTH_* */
+	uint32_t reserved;
 } sw_depth_target;
+
+/* After the target's draws, make_shadow_map_circle (light.rs:130-154): the texels outside the
+ * square's inscribed circle become 1e24 (a projector's round beam). Square targets only. */
+#define SW_DEPTH_CIRCLE 1u
 
 /* Draws every target's polygons in order into its buffer (no clear: pixels no polygon covers keep
  * their values, as the reference's reused shadow-map storage does). draws, vertex_data and targets

@@ -130,6 +130,12 @@ int swh_shadow_create(swh_shadow** out);
 void swh_shadow_destroy(swh_shadow* shadow);
 void swh_shadow_clear(swh_shadow* shadow);
 int swh_shadow_add_cube(swh_shadow* shadow, const swh_map* map, const float position[3], uint32_t size);
+/* A projector light (PartialRenderer::build_shadow_maps' projector branch, partial_renderer.rs:567-584):
+ * one size x size target with SW_DEPTH_CIRCLE, the camera from the rotation quaternion (s, x, y, z)
+ * and the field of view (radians, 0 < fov < pi); basis_out (may be null) receives the sw_light_shadow
+ * basis_x, basis_y, basis_z of the map (renderer_structs.rs:288-307). */
+int swh_shadow_add_projector(swh_shadow* shadow, const swh_map* map, const float position[3], const float rotation[4], float fov,
+	uint32_t size, float basis_out[9]);
 int swh_shadow_get(const swh_shadow* shadow, swh_shadow_data* out_data);
 
 /* Compiled-map files (common/bsp_map_save_load.rs:23-362): the map as a Square-Wheel .sqwm file,

@@ -2672,6 +2672,24 @@ int oracle_render_depth(const sw_depth_target* targets, uint32_t num_targets, co
 		DepthRasterizer r{texels + targets[t].offset, int32_t(targets[t].width), int32_t(targets[t].height)};
 		for(uint32_t k = targets[t].first_draw; k < targets[t].first_draw + targets[t].num_draws; ++k)
 			r.fill_polygon(draws[k].depth_test != 0, vdata + draws[k].first_vertex_int, draws[k].num_vertices, draws[k].depth_equation);
+		if(targets[t].flags & SW_DEPTH_CIRCLE)
+		{
+			// make_shadow_map_circle (light.rs:130-154)
+			const float inf = 1.0e24f;
+			const int32_t size = int32_t(targets[t].width), radius = size / 2;
+			float* data = texels + targets[t].offset;
+			for(int32_t y = 0; y < size; ++y)
+			{
+				const int32_t dy = y - radius;
+				const int32_t dx =
+					radius - std::min(int32_t(std::ceil(std::sqrt(float(std::max(radius * radius - dy * dy, 0))))), radius);
+				float* line = data + y * size;
+				for(int32_t x = 0; x < dx; ++x)
+					line[x] = inf;
+				for(int32_t x = size - dx; x < size; ++x)
+					line[x] = inf;
+			}
+		}
 	}
 	return 0;
 }

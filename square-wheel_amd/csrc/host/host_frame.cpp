@@ -1019,6 +1019,91 @@ void swh_shadow_clear(swh_shadow* s)
 	s->texels = 0;
 }
 
+// One shadow map: DepthRenderer::draw_map (depth_renderer.rs:47-109, 202-287) from the camera `cam`,
+// appended as a size x size target at the running texel offset.
+static void add_shadow_target(swh_shadow* s, const swh_map* m, const Camera& cam, uint32_t size, uint32_t flags)
+{
+	const float fs = float(size);
+	const ClippingPolygon frame_bounds = ClippingPolygon::from_box(0.0f, 0.0f, fs, fs);
+	swh_frame& vis = s->vis;
+	vis.frame_number++;
+	update_visibility(vis, *m, cam, frame_bounds);
+	sw_depth_target t;
+	t.offset = s->texels;
+	t.width = size;
+	t.height = size;
+	t.first_draw = uint32_t(s->draws.size());
+	t.flags = flags;
+	t.reserved = 0;
+	// draw_tree (:69-109): back to front, leaves with visibility bounds
+	std::vector<uint32_t> stack;
+	stack.push_back(uint32_t(m->nodes.size() - 1));
+	const V4 w_row = cam.planes_matrix.row(3);
+	while(!stack.empty())
+	{
+		const uint32_t node_index = stack.back();
+		if(node_index >= FIRST_LEAF_INDEX)
+		{
+			stack.pop_back();
+			const uint32_t leaf_index = node_index - FIRST_LEAF_INDEX;
+			if(vis.leaf_visible_frame[leaf_index] != vis.frame_number)
+				continue;
+			V3 cp[8];
+			vis.leaf_bounds[leaf_index].clip_planes(cp);
+			const Leaf& leaf = m->leafs[leaf_index];
+			for(uint32_t k = 0; k < leaf.num_polygons; ++k)
+			{
+				// draw_polygon (:202-254)
+				const Polygon& p = m->polygons[leaf.first_polygon + k];
+				// materials_shadow_table (:23-37): blended materials cast no shadows
+				if(m->materials[p.texture].blending_mode != SW_BLEND_NONE)
+					continue;
+				const V4 pt = mul(cam.planes_matrix, extend(p.plane.vec, -p.plane.dist));
+				if(pt.w <= 0.0f)
+					continue; // back face
+				const float w = -pt.w;
+				const float dx = pt.x / w, dy = pt.y / w;
+				const float bias_const = -1.0f / float(1 << 20), bias_slope = -1.0f, eq_scale = 1.0f - 1.0f / 1024.0f;
+				sw_depth_draw d;
+				d.depth_equation[0] = eq_scale * dx;
+				d.depth_equation[1] = eq_scale * dy;
+				d.depth_equation[2] = eq_scale * (pt.z / w + bias_const + bias_slope * (std::fabs(dx) + std::fabs(dy)));
+				d.depth_test = 0;
+				const uint32_t n = std::min(p.num_vertices, MAX_VERTICES);
+				if(n < 3)
+					continue;
+				V3 vt[MAX_VERTICES];
+				for(uint32_t i = 0; i < n; ++i)
+					vt[i] = view_transform(cam.view_matrix, m->vertices[p.first_vertex + i]);
+				// draw_depth_polygon (:257-287)
+				V2 v2d[MAX_VERTICES];
+				const uint32_t nc = project_and_clip_polygon(cp, vt, n, v2d);
+				if(nc < 3)
+					continue;
+				d.first_vertex_int = uint32_t(s->vdata.size());
+				d.num_vertices = nc;
+				for(uint32_t i = 0; i < nc; ++i)
+				{
+					s->vdata.push_back(saturating_f32_to_i32(v2d[i].x * 65536.0f));
+					s->vdata.push_back(saturating_f32_to_i32(v2d[i].y * 65536.0f));
+				}
+				s->draws.push_back(d);
+			}
+		}
+		else
+		{
+			const Node& node = m->nodes[node_index];
+			const float wn = dot(w_row, extend(node.plane.vec, -node.plane.dist));
+			const uint32_t mask = wn >= 0.0f ? 1u : 0u;
+			stack.back() = node.children[1 ^ mask];
+			stack.push_back(node.children[mask]);
+		}
+	}
+	t.num_draws = uint32_t(s->draws.size()) - t.first_draw;
+	s->targets.push_back(t);
+	s->texels += uint64_t(size) * size;
+}
+
 int swh_shadow_add_cube(swh_shadow* s, const swh_map* m, const float position[3], uint32_t size)
 {
 	if(!s || !m || !position || size == 0 || size > 65535)
@@ -1026,85 +1111,49 @@ int swh_shadow_add_cube(swh_shadow* s, const swh_map* m, const float position[3]
 	ensure_basis(s->vis, *m);
 	const V3 pos = v3(position[0], position[1], position[2]);
 	const float fs = float(size);
-	const ClippingPolygon frame_bounds = ClippingPolygon::from_box(0.0f, 0.0f, fs, fs);
 	for(uint32_t side = 0; side < 6; ++side)
 	{
-		const Camera cam = complete_view(pos, cube_side_rotation(side), v2(1.0f, 1.0f), fs, fs);
-		swh_frame& vis = s->vis;
-		vis.frame_number++;
-		update_visibility(vis, *m, cam, frame_bounds);
-		sw_depth_target t;
-		t.offset = s->texels;
-		t.width = size;
-		t.height = size;
-		t.first_draw = uint32_t(s->draws.size());
-		// draw_tree (:69-109): back to front, leaves with visibility bounds
-		std::vector<uint32_t> stack;
-		stack.push_back(uint32_t(m->nodes.size() - 1));
-		const V4 w_row = cam.planes_matrix.row(3);
-		while(!stack.empty())
-		{
-			const uint32_t node_index = stack.back();
-			if(node_index >= FIRST_LEAF_INDEX)
-			{
-				stack.pop_back();
-				const uint32_t leaf_index = node_index - FIRST_LEAF_INDEX;
-				if(vis.leaf_visible_frame[leaf_index] != vis.frame_number)
-					continue;
-				V3 cp[8];
-				vis.leaf_bounds[leaf_index].clip_planes(cp);
-				const Leaf& leaf = m->leafs[leaf_index];
-				for(uint32_t k = 0; k < leaf.num_polygons; ++k)
-				{
-					// draw_polygon (:202-254)
-					const Polygon& p = m->polygons[leaf.first_polygon + k];
-					// materials_shadow_table (:23-37): blended materials cast no shadows
-					if(m->materials[p.texture].blending_mode != SW_BLEND_NONE)
-						continue;
-					const V4 pt = mul(cam.planes_matrix, extend(p.plane.vec, -p.plane.dist));
-					if(pt.w <= 0.0f)
-						continue; // back face
-					const float w = -pt.w;
-					const float dx = pt.x / w, dy = pt.y / w;
-					const float bias_const = -1.0f / float(1 << 20), bias_slope = -1.0f, eq_scale = 1.0f - 1.0f / 1024.0f;
-					sw_depth_draw d;
-					d.depth_equation[0] = eq_scale * dx;
-					d.depth_equation[1] = eq_scale * dy;
-					d.depth_equation[2] = eq_scale * (pt.z / w + bias_const + bias_slope * (std::fabs(dx) + std::fabs(dy)));
-					d.depth_test = 0;
-					const uint32_t n = std::min(p.num_vertices, MAX_VERTICES);
-					if(n < 3)
-						continue;
-					V3 vt[MAX_VERTICES];
-					for(uint32_t i = 0; i < n; ++i)
-						vt[i] = view_transform(cam.view_matrix, m->vertices[p.first_vertex + i]);
-					// draw_depth_polygon (:257-287)
-					V2 v2d[MAX_VERTICES];
-					const uint32_t nc = project_and_clip_polygon(cp, vt, n, v2d);
-					if(nc < 3)
-						continue;
-					d.first_vertex_int = uint32_t(s->vdata.size());
-					d.num_vertices = nc;
-					for(uint32_t i = 0; i < nc; ++i)
-					{
-						s->vdata.push_back(saturating_f32_to_i32(v2d[i].x * 65536.0f));
-						s->vdata.push_back(saturating_f32_to_i32(v2d[i].y * 65536.0f));
-					}
-					s->draws.push_back(d);
-				}
-			}
-			else
-			{
-				const Node& node = m->nodes[node_index];
-				const float wn = dot(w_row, extend(node.plane.vec, -node.plane.dist));
-				const uint32_t mask = wn >= 0.0f ? 1u : 0u;
-				stack.back() = node.children[1 ^ mask];
-				stack.push_back(node.children[mask]);
-			}
-		}
-		t.num_draws = uint32_t(s->draws.size()) - t.first_draw;
-		s->targets.push_back(t);
-		s->texels += uint64_t(size) * size;
+		add_shadow_target(s, m, complete_view(pos, cube_side_rotation(side), v2(1.0f, 1.0f), fs, fs), size, 0);
+	}
+	return 0;
+}
+
+// PartialRenderer::build_shadow_maps' projector branch (partial_renderer.rs:567-584): the camera of
+// calculate_projector_shadow_map_matrices (light.rs:63-71, matrix.rs:31-80), the map drawn, then
+// make_shadow_map_circle (the target's SW_DEPTH_CIRCLE). basis_out: the sw_light_shadow basis of
+// create_dynamic_light_projector_shadow_map (renderer_structs.rs:288-307), x, y, z.
+int swh_shadow_add_projector(swh_shadow* s, const swh_map* m, const float position[3], const float rotation[4], float fov,
+	uint32_t size, float basis_out[9])
+{
+	if(!s || !m || !position || !rotation || size == 0 || size > 65535 || !(fov > 0.0f && fov < 3.14159265f))
+		return -1;
+	ensure_basis(s->vis, *m);
+	const V3 pos = v3(position[0], position[1], position[2]);
+	const Quat q = Quat{rotation[0], v3(rotation[1], rotation[2], rotation[3])};
+	const float fs = float(size);
+	const float inv_half_fov_tan = 1.0f / std::tan(fov * 0.5f);
+	const V2 fov_scale = v2(inv_half_fov_tan / (fs / fs), inv_half_fov_tan);
+	M4 basis_change = identity();
+	basis_change.c[0].x = 0.0f;
+	basis_change.c[1].y = 0.0f;
+	basis_change.c[2].z = 0.0f;
+	basis_change.c[0].z = 1.0f;
+	basis_change.c[1].x = -1.0f;
+	basis_change.c[2].y = -1.0f;
+	const Quat conj = Quat{q.s, v3(-q.v.x, -q.v.y, -q.v.z)};
+	add_shadow_target(s, m, complete_view(pos, mul(basis_change, quat_to_m4(conj)), fov_scale, fs, fs), size, SW_DEPTH_CIRCLE);
+	if(basis_out)
+	{
+		// cgmath's Quaternion::rotate_vector: v + 2 q.v x (q.v x v + s v)
+		auto rotate = [&](V3 v) {
+			const V3 tmp = cross(q.v, v) + v * q.s;
+			return cross(q.v, tmp) * 2.0f + v;
+		};
+		const V3 bx = rotate(v3(0.0f, 1.0f, 0.0f)) * inv_half_fov_tan;
+		const V3 by = rotate(v3(0.0f, 0.0f, 1.0f)) * inv_half_fov_tan;
+		const V3 bz = rotate(v3(-1.0f, -0.0f, -0.0f));
+		const float b[9] = {bx.x, bx.y, bx.z, by.x, by.y, by.z, bz.x, bz.y, bz.z};
+		std::memcpy(basis_out, b, sizeof(b));
 	}
 	return 0;
 }

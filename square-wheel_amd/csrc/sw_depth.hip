@@ -49,8 +49,18 @@ struct DepthTargetDev
 	uint32_t w, h;
 	uint32_t first_draw, num_draws;
 	uint32_t first_row_block; // raster blocks: first_row_block + y / DEPTH_ROWS_PER_BLOCK
-	uint32_t pad;
+	uint32_t flags; // SW_DEPTH_*
 };
+
+// make_shadow_map_circle (light.rs:130-154): texels [0, dx) and [size - dx, size) of row y are
+// outside the inscribed circle
+SW_DEV uint32_t circle_margin(int32_t size, int32_t y)
+{
+	const int32_t radius = size / 2, dy = y - radius;
+	const int32_t r2 = radius * radius - dy * dy;
+	const int32_t half = int32_t(__builtin_ceilf(__builtin_sqrtf(float(r2 > 0 ? r2 : 0))));
+	return uint32_t(radius - (half < radius ? half : radius));
+}
 
 struct DepthLaunch
 {
@@ -304,13 +314,17 @@ __global__ __launch_bounds__(64) void k_depth_raster(DepthLaunch L)
 			}
 		}
 	}
-	if(in_lds)
+	__syncthreads();
+	if((t.flags & SW_DEPTH_CIRCLE) && row_ok)
 	{
-		__syncthreads();
-		if(row_ok)
-			for(uint32_t x = sub; x < t.w; x += 16)
-				grow[x] = row[x];
+		const uint32_t dx = circle_margin(int32_t(t.w), y);
+		for(uint32_t x = sub; x < t.w; x += 16)
+			if(x < dx || x >= t.w - dx)
+				row[x] = 1.0e24f;
 	}
+	if(in_lds && row_ok)
+		for(uint32_t x = sub; x < t.w; x += 16)
+			grow[x] = row[x];
 }
 
 } // namespace
@@ -335,9 +349,13 @@ int depth_plan(const sw_depth_target* targets, uint32_t num_targets, const sw_de
 		if(t.first_draw != next_draw || uint64_t(t.first_draw) + t.num_draws > num_draws)
 			return (error = which + "draw range not following the previous target's"), -1;
 		next_draw = t.first_draw + t.num_draws;
+		if(t.flags & ~SW_DEPTH_CIRCLE)
+			return (error = which + "unknown flags"), -1;
+		if((t.flags & SW_DEPTH_CIRCLE) && t.width != t.height)
+			return (error = which + "the circle needs a square target"), -1;
 		if(uint64_t(row_blocks) + t.height > 0x7FFFFFFFu)
 			return (error = "depth targets: too many rows"), -1;
-		tg[i] = DepthTargetDev{t.offset, t.width, t.height, t.first_draw, t.num_draws, row_blocks, 0};
+		tg[i] = DepthTargetDev{t.offset, t.width, t.height, t.first_draw, t.num_draws, row_blocks, t.flags};
 		row_blocks += (t.height + DEPTH_ROWS_PER_BLOCK - 1) / DEPTH_ROWS_PER_BLOCK;
 		for(uint32_t k = t.first_draw; k < t.first_draw + t.num_draws; ++k)
 		{

@@ -81,7 +81,8 @@ class WaterTextureParams(C.Structure):
 DEPTH_DRAW_DTYPE = np.dtype([("first_vertex_int", "<u4"), ("num_vertices", "<u4"), ("depth_equation", "<f4", 3),
                              ("depth_test", "<u4")], align=True)
 DEPTH_TARGET_DTYPE = np.dtype([("offset", "<u8"), ("width", "<u4"), ("height", "<u4"), ("first_draw", "<u4"),
-                               ("num_draws", "<u4")], align=True)
+                               ("num_draws", "<u4"), ("flags", "<u4"), ("reserved", "<u4")], align=True)
+DEPTH_CIRCLE = 1
 # texture animation (sw_apply_texture_ops)
 TEXOP_TURB, TEXOP_LAYER, TEXOP_LAYER_LITE = 0, 1, 2
 TEXTURE_OP_DTYPE = np.dtype([("kind", "<u4"), ("dst_level", "<u4"), ("src_level", "<u4"), ("blend", "<u4"),
@@ -92,7 +93,7 @@ SKELETON_VERTEX_DTYPE = np.dtype([("tex_coord", "<f4", 2), ("position", "<f4", 3
 
 EXPECTED_SIZES = {"TEXEL": 8, "LEVEL": 24, "DIRLM": 40, "LIGHT": 32, "SURFACE": 136, "DRAW": 96, "VIEW": 64,
                   "MODEL_INSTANCE": 288, "MODEL_VERTEX": 32, "SKELETON_VERTEX": 40,
-                  "TEXTURE_OP": 40, "LIGHT_SHADOW": 56, "DEPTH_DRAW": 24, "DEPTH_TARGET": 24}
+                  "TEXTURE_OP": 40, "LIGHT_SHADOW": 56, "DEPTH_DRAW": 24, "DEPTH_TARGET": 32}
 for _name, _size in EXPECTED_SIZES.items():
     assert globals()[_name + "_DTYPE"].itemsize == _size, (_name, globals()[_name + "_DTYPE"].itemsize)
 

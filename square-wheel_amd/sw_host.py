@@ -85,6 +85,9 @@ def load():
         lib.swh_shadow_clear.restype = None
         lib.swh_shadow_add_cube.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_float), C.c_uint32]
         lib.swh_shadow_add_cube.restype = C.c_int
+        lib.swh_shadow_add_projector.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float,
+                                                 C.c_uint32, C.POINTER(C.c_float)]
+        lib.swh_shadow_add_projector.restype = C.c_int
         lib.swh_shadow_get.argtypes = [C.c_void_p, C.POINTER(ShadowData)]
         lib.swh_shadow_get.restype = C.c_int
         lib.swh_pack_normal_roughness_from_rgb.argtypes = [C.c_uint32, C.c_float]
@@ -274,6 +277,16 @@ class ShadowBuilder:
         pos = (C.c_float * 3)(*(float(x) for x in position))
         if load().swh_shadow_add_cube(self.h, m.h, pos, int(size)) != 0:
             raise ValueError(f"swh_shadow_add_cube failed (size {size})")
+
+    def add_projector(self, m: Map, position, rotation, fov, size):
+        """One projector map (SW_DEPTH_CIRCLE target); rotation is the quaternion (s, x, y, z). Returns
+        the map's basis (float32 [3, 3]: basis_x, basis_y, basis_z of LIGHT_SHADOW_DTYPE)."""
+        pos = (C.c_float * 3)(*(float(x) for x in position))
+        rot = (C.c_float * 4)(*(float(x) for x in rotation))
+        basis = (C.c_float * 9)()
+        if load().swh_shadow_add_projector(self.h, m.h, pos, rot, float(fov), int(size), basis) != 0:
+            raise ValueError(f"swh_shadow_add_projector failed (size {size}, fov {fov})")
+        return np.array(basis, np.float32).reshape(3, 3)
 
     def arrays(self):
         """(targets, draws, vertex_data, num_depth_texels): copies of the lists built so far."""

@@ -19,6 +19,7 @@ def _depth_scene(rng, sizes, draws_per_target):
     for t, (w, h) in enumerate(sizes):
         targets[t]["offset"], targets[t]["width"], targets[t]["height"] = off, w, h
         targets[t]["first_draw"], targets[t]["num_draws"] = len(draws), draws_per_target
+        targets[t]["flags"] = A.DEPTH_CIRCLE if (w == h and t % 2 == 1) else 0  # projector maps' circle
         off += w * h
         for i in range(draws_per_target):
             fx, de, _ = scenes.make_polygon(rng, w, h, 16, 16, A.TC_PERSPECTIVE, thin=(i % 7 == 3))
@@ -115,6 +116,47 @@ def test_front_end_cube_shadows_frame(gpu_ctx, color_bits):
         ref_fb, _, _, _ = oracle.render(m.data, b)
         assert np.array_equal(fb, ref_fb)
         assert not np.array_equal(fb, unshadowed)
+    finally:
+        gpu_ctx.set_light_shadows(None, None)
+        oracle.set_light_shadows(None, None)
+        gpu_ctx.release_map(mh)
+
+
+def test_front_end_projector_shadows_frame(gpu_ctx):
+    """Projector maps (swh_shadow_add_projector: a rotated camera, the circle mask) for every dynamic
+    light of a frame, drawn by sw_render_depth and sampled through the returned bases: depth maps and
+    the shadowed frame bit-exact to the oracle's."""
+    import sw_host
+    m = sw_host.large_map(5, grid=(12, 12, 4))
+    pose = m.camera_path(11, 1)[0]
+    f = sw_host.FrameBuilder().build(m, pose, width=200, height=150, color_bits=32, num_dynamic_lights=8,
+                                     use_directional_lightmaps=1, seed=6)
+    b = sw_host.batch_from_frames([f])
+    rng = np.random.default_rng(12)
+    size = 64
+    sb = sw_host.ShadowBuilder()
+    sh = np.zeros(len(b.lights), dtype=A.LIGHT_SHADOW_DTYPE)
+    for i, lt in enumerate(b.lights):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        basis = sb.add_projector(m, lt["position"], q, 1.3, size)
+        sh[i]["kind"], sh[i]["size"], sh[i]["offset"] = A.SHADOW_PROJECTOR, size, i * size * size
+        sh[i]["basis_x"], sh[i]["basis_y"], sh[i]["basis_z"] = basis[0], basis[1], basis[2]
+    targets, draws, vdata, n = sb.arrays()
+    sb.close()
+    print(f"front-end projectors: {len(b.lights)} lights, {len(draws)} draws", flush=True)
+    init = np.zeros(n, np.float32)
+    depth = gpu_ctx.render_depth(targets, draws, vdata, init)
+    ref_depth = oracle.render_depth(targets, draws, vdata, init)
+    assert np.array_equal(depth.view(np.uint32), ref_depth.view(np.uint32))
+    assert np.count_nonzero(depth == np.float32(1e24)) > 0
+    mh = gpu_ctx.upload_map(m.data)
+    try:
+        gpu_ctx.set_light_shadows(sh, depth)
+        oracle.set_light_shadows(sh, ref_depth)
+        fb = gpu_ctx.render_plain(mh, b).reshape(-1)
+        ref_fb, _, _, _ = oracle.render(m.data, b)
+        assert np.array_equal(fb, ref_fb)
     finally:
         gpu_ctx.set_light_shadows(None, None)
         oracle.set_light_shadows(None, None)
