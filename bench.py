@@ -308,15 +308,68 @@ def water_leg(ctx, args):
     a.free()
     b.free()
     ctx.release_map(mh)
+    fire = fire_leg(ctx, args)
     # algorithmic bytes per frame: each step reads and writes the two fields (4 + 4 + 4 B per element),
     # the texture reads the field and the colour level and writes 8 B texels, the mips 1/3 more
     step_bytes = 12 * n
     tex_bytes = (4 + 8 + 8) * n + (8 + 8) * n // 3
     res = {"workload": "1024x1024 water field, 4 steps x 16 source points + wavy texture (deformed colours) + 4 mips per frame",
            "us_per_frame": round(wall * 1e6, 2), "device_us_per_frame": round(kus, 2), "event_launches": int(launches),
-           "gb_s": round((4 * step_bytes + tex_bytes) / (kus * 1e-6) / 1e9, 1)}
+           "gb_s": round((4 * step_bytes + tex_bytes) / (kus * 1e-6) / 1e9, 1), "fire": fire}
     log(f"water: {res}")
     return res
+
+
+def fire_leg(ctx, args):
+    """Fire textures: a 64x64 heat map (bytes; the map's emissive textures are 64 wide), 4 steps per
+    frame with 64 set_heat points each, then
+    the palette texture and its four lite mips (sw_fire_step, sw_fire_make_texture), both update
+    variants."""
+    import ctypes as C
+    lw = lh = 6
+    n = 1 << (lw + lh)
+    m = sw_host.Map(seed=args.seed + 13, grid=(10, 10, 4), texture_size=1024, num_materials=2, num_decal_textures=1,
+                    num_model_textures=1)
+    lv = m.data.levels
+    dst = next(i for i in range(len(lv) - 4) if (lv[i]["flags"] & 256) and lv[i]["width"] == 64 and lv[i]["height"] == 64)
+    mh = ctx.upload_map(m.data)
+    rng = np.random.default_rng(args.seed + 15)
+    a = ctx.to_device(rng.integers(0, 256, n).astype(np.uint8))
+    b = ctx.alloc(n)
+    pts = np.zeros(64, dtype=sw_abi.HEAT_POINT_DTYPE)
+    pts["x"], pts["y"], pts["heat"] = rng.integers(0, 64, 64), 62, 255
+    palette = (np.arange(256, dtype=np.uint32) * 0x010101).astype(np.uint32)
+    lib = ctx.lib
+    out = {}
+    for slow in (0, 1):
+        def frame():
+            nonlocal a, b
+            for _ in range(4):
+                ctx.check(lib.sw_fire_step(ctx.h, a.ptr, b.ptr, lw, lh, C.c_float(0.99), slow, C.c_void_p(pts.ctypes.data), len(pts)),
+                          "sw_fire_step")
+                a, b = b, a
+            ctx.check(lib.sw_fire_make_texture(ctx.h, mh, a.ptr, lw, lh, dst, C.c_void_p(palette.ctypes.data)), "sw_fire_make_texture")
+
+        for _ in range(8):
+            frame()
+        ctx.check(lib.sw_sync(ctx.h), "sw_sync")
+        reps = 100
+        ctx.profile_enable(True)
+        ctx.profile_read(reset=True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            frame()
+        ctx.check(lib.sw_sync(ctx.h), "sw_sync")
+        wall = (time.perf_counter() - t0) / reps
+        ctx.profile_enable(False)
+        prof = ctx.profile_read(reset=True)
+        out["slow" if slow else "fast"] = {"us_per_frame": round(wall * 1e6, 2),
+                                           "device_us_per_frame": round(prof["textures"][0] / reps * 1e3, 2)}
+    a.free()
+    b.free()
+    ctx.release_map(mh)
+    out["workload"] = "64x64 heat map, 4 steps x 64 points + palette texture + 4 lite mips per frame"
+    return out
 
 
 def models_leg(ctx, args):

@@ -658,6 +658,34 @@ typedef struct sw_water_texture_params
 /* make_wavy_texture (:420-682) from the field into the map's dst level (normal = normalize(right -
  * left, down - up, 1), packed with the roughness; colours per color_mode with the deformation scale
  * 16), then build_texture_mip (textures.rs:309-362) four times into the following levels. */
+/* ---------------------------------------------------------------------------------------------
+ * Fire: the heat map and the emissive texture of GenerativeTextureEffectFire
+ * (generative_texture_effect_fire.rs). The heat map (bytes, width x height, powers of two, at least
+ * 16, area below 2^22) lives in two device buffers owned by the caller. Heat sources, lines,
+ * lightnings and particles (:96-326, random-driven) stay on the caller's CPU, which hands each
+ * step's set_heat points over.
+ * ------------------------------------------------------------------------------------------- */
+
+/* One set_heat of a step (:80-86): heat[x, y] = max(heat[x, y], heat), x and y wrapped. */
+typedef struct sw_heat_point
+{
+	uint32_t x, y;
+	uint32_t heat; /* 0 .. 255 */
+} sw_heat_point;
+
+/* step_update_heat_map (:62-326): update_heat_map (:357-468; slow != 0: the row-serial variant)
+ * from heat_device into heat_old_device, the last row cleared, then the points; as the reference's
+ * map is updated in place, the caller swaps the two pointers after each call. Queued on the
+ * context's stream. */
+int sw_fire_step(sw_ctx* ctx, uint8_t* heat_device, uint8_t* heat_old_device, uint32_t log2_width, uint32_t log2_height,
+	float attenuation, uint32_t slow, const sw_heat_point* points, uint32_t num_points);
+
+/* generate_emissive_texture_based_on_heat_map (:470-486) into the map's TextureLite level dst_level
+ * (the heat map's size) through the 256-colour palette, then build_texture_lite_mip
+ * (textures.rs:385-415) into the following four levels. */
+int sw_fire_make_texture(sw_ctx* ctx, sw_map* map, const uint8_t* heat_device, uint32_t log2_width, uint32_t log2_height,
+	uint32_t dst_level, const uint32_t palette[256]);
+
 int sw_water_make_texture(sw_ctx* ctx, sw_map* map, const float* field_device, uint32_t log2_width, uint32_t log2_height,
 	const sw_water_texture_params* params);
 

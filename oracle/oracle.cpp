@@ -2744,6 +2744,82 @@ int oracle_water_step(float* field, float* field_old, uint32_t log2w, uint32_t l
 	return 0;
 }
 
+// generative_texture_effect_fire.rs, step_update_heat_map (:62-326) after its sources: update_heat_map
+// in place (:357-468) in the reference's row order, then set_heat for the points (:80-86).
+int oracle_fire_step(uint8_t* heat_map, uint32_t log2w, uint32_t log2h, float attenuation, uint32_t slow, const sw_heat_point* pts,
+	uint32_t num_points)
+{
+	const uint32_t w = 1u << log2w, h = 1u << log2h;
+	const uint32_t SHIFT = 20;
+	const uint32_t attenuation_i = uint32_t(std::min(std::max(attenuation, 0.0f), 1.0f) * float(1u << (SHIFT - 2)));
+	auto f = [&](uint32_t o, uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+		const uint32_t sum = uint32_t(heat_map[a]) + uint32_t(heat_map[b]) + uint32_t(heat_map[c]) + uint32_t(heat_map[d]);
+		heat_map[o] = uint8_t((sum * attenuation_i) >> SHIFT);
+	};
+	if(slow)
+	{
+		for(uint32_t y = 0; y < h - 1; ++y)
+		{
+			const uint32_t ls = y * w, le = ls + w - 1;
+			f(ls, le, ls, ls + 1, ls + w);
+			for(uint32_t x = ls + 1; x < le; ++x)
+				f(x, x - 1, x, x + 1, x + w);
+			f(le, le - 1, le, ls, le + w);
+		}
+	}
+	else
+	{
+		for(uint32_t y = 0; y < h - 2; ++y)
+		{
+			const uint32_t ls = y * w, le = ls + w - 1;
+			f(ls, le + w, ls + w, ls + 1 + w, ls + w * 2);
+			for(uint32_t x = ls + 1; x < le; ++x)
+				f(x, x - 1 + w, x + w, x + 1 + w, x + w * 2);
+			f(le, le - 1 + w, le + w, ls + w, le + w * 2);
+		}
+		const uint32_t ls = (h - 2) * w, le = ls + w - 1;
+		f(ls, le + w, ls + w, ls + 1 + w, ls + w);
+		for(uint32_t x = ls + 1; x < le; ++x)
+			f(x, x - 1 + w, x + w, x + 1 + w, x + w);
+		f(le, le - 1 + w, le + w, ls + w, le + w);
+	}
+	for(uint32_t x = 0; x < w; ++x)
+		heat_map[(h - 1) * w + x] = 0;
+	for(uint32_t i = 0; i < num_points; ++i)
+	{
+		uint8_t& v = heat_map[(pts[i].x & (w - 1)) + ((pts[i].y & (h - 1)) << log2w)];
+		v = std::max(v, uint8_t(pts[i].heat));
+	}
+	return 0;
+}
+
+// generate_emissive_texture_based_on_heat_map (:470-486) into the lite level dst_level, then
+// build_texture_lite_mip (textures.rs:385-415) into the next four levels.
+int oracle_fire_make_texture(const sw_texture_level* levels, uint32_t num_levels, uint32_t* lite, const uint8_t* heat_map, uint32_t log2w,
+	uint32_t log2h, uint32_t dst_level, const uint32_t* palette)
+{
+	if(uint64_t(dst_level) + 4 >= num_levels)
+		return -1;
+	const uint32_t w = 1u << log2w, h = 1u << log2h;
+	uint32_t* dst = lite + levels[dst_level].offset;
+	for(uint32_t i = 0; i < w * h; ++i)
+		dst[i] = palette[heat_map[i]];
+	for(uint32_t mip = 1; mip <= 4; ++mip)
+	{
+		const uint32_t pw = w >> (mip - 1), mw = pw >> 1, mh = (h >> (mip - 1)) >> 1;
+		const uint32_t* prev = lite + levels[dst_level + mip - 1].offset;
+		uint32_t* out = lite + levels[dst_level + mip].offset;
+		for(uint32_t y = 0; y < mh; ++y)
+			for(uint32_t x = 0; x < mw; ++x)
+			{
+				const uint32_t c[4] = {prev[(y * 2) * pw + x * 2], prev[(y * 2 + 1) * pw + x * 2], prev[(y * 2) * pw + x * 2 + 1],
+					prev[(y * 2 + 1) * pw + x * 2 + 1]};
+				out[y * mw + x] = color32_average_4(c);
+			}
+	}
+	return 0;
+}
+
 // make_wavy_texture (:420-682) into the dst level, then its four mips (:293-299).
 int oracle_water_make_texture(const sw_texture_level* levels, uint32_t num_levels, sw_texel* texels, const float* field, uint32_t log2w,
 	uint32_t log2h, const sw_water_texture_params* p)

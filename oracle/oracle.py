@@ -45,6 +45,11 @@ def load():
         lib.oracle_shadow_factor.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]
         lib.oracle_make_skybox_side.restype = C.c_uint32
         lib.oracle_make_skybox_side.argtypes = [C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_void_p]
+        lib.oracle_fire_step.restype = C.c_int
+        lib.oracle_fire_step.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, C.c_void_p, C.c_uint32]
+        lib.oracle_fire_make_texture.restype = C.c_int
+        lib.oracle_fire_make_texture.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                 C.c_void_p]
         lib.oracle_water_step.restype = C.c_int
         lib.oracle_water_step.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_void_p, C.c_uint32]
         lib.oracle_water_make_texture.restype = C.c_int
@@ -194,6 +199,30 @@ def water_step(field, field_old, log2w, log2h, attenuation, points):
     lib.oracle_water_step(C.c_void_p(f.ctypes.data), C.c_void_p(fo.ctypes.data), log2w, log2h, C.c_float(attenuation),
                           C.c_void_p(pts.ctypes.data), n)
     return fo, f
+
+
+def fire_step(heat, log2w, log2h, attenuation, slow, points):
+    """One step of the fire heat map (generative_texture_effect_fire.rs:62-326 after its sources), on a
+    copy; returns the new map."""
+    lib = load()
+    hm = np.ascontiguousarray(heat, dtype=np.uint8).copy()
+    n = 0 if points is None else len(points)
+    pts = np.ascontiguousarray(np.zeros(1, sw_abi.HEAT_POINT_DTYPE) if n == 0 else points, dtype=sw_abi.HEAT_POINT_DTYPE)
+    lib.oracle_fire_step(C.c_void_p(hm.ctypes.data), log2w, log2h, C.c_float(attenuation), 1 if slow else 0, C.c_void_p(pts.ctypes.data), n)
+    return hm
+
+
+def fire_make_texture(map_data, heat, log2w, log2h, dst_level, palette):
+    """The palette texture and its lite mips on a copy of the map's lite texels; returns them."""
+    lib = load()
+    lite = map_data.lite.copy()
+    hm = np.ascontiguousarray(heat, dtype=np.uint8)
+    pal = np.ascontiguousarray(palette, dtype=np.uint32)
+    r = lib.oracle_fire_make_texture(C.c_void_p(map_data.levels.ctypes.data), len(map_data.levels), C.c_void_p(lite.ctypes.data),
+                                     C.c_void_p(hm.ctypes.data), log2w, log2h, dst_level, C.c_void_p(pal.ctypes.data))
+    if r != 0:
+        raise ValueError("level out of range")
+    return lite
 
 
 def water_make_texture(map_data, field, log2w, log2h, params):

@@ -294,6 +294,12 @@ struct WaterTexLaunch
 	uint32_t base_color;
 };
 void launch_water_texture(hipStream_t st, const float* field, uint32_t log2w, uint32_t log2h, const WaterTexLaunch& w, sw_texel* texels);
+// fire (generative_texture_effect_fire.rs): one step (update into heat_old, then the points) and the
+// palette texture + lite mips (dst levels given by texel offsets)
+void launch_fire_step(hipStream_t st, const uint8_t* heat, uint8_t* heat_old, uint32_t log2w, uint32_t log2h, uint32_t attenuation_i,
+	bool slow, const sw_heat_point* points_dev, uint32_t num_points);
+void launch_fire_texture(hipStream_t st, const uint8_t* heat, uint32_t log2w, uint32_t log2h, const uint64_t dst_offset[5],
+	const uint32_t* palette_dev, uint32_t* lite);
 void launch_skybox_side(hipStream_t st, uint32_t color_bits, const void* image, uint32_t image_w, uint32_t size, int32_t brightness,
 	void* dst);
 } // namespace swk

@@ -69,7 +69,7 @@ struct sw_ctx
 	// command buffers / work buffers
 	DevBuf views, surfaces, lights, light_indices, draws, vdata, surf_ext, blk_surf, parts, spans, drec, dbounds, counters, arena, fb,
 		tonemap_out, tonemap_sums, bloom0, bloom1, bloom_lines, model_args, model_scratch, texop_args, shadows, depth_args, depth_vdata,
-		depth_rows, depth_draw_rows, water_points;
+		depth_rows, depth_draw_rows, water_points, fire_args;
 	uint32_t num_shadows = 0; // sw_set_light_shadows: entries in `shadows`
 	const float* shadow_texels = nullptr;
 	uint32_t tile_rows = 8; // raster tile height (SW_TILE_ROWS_*), set by sw_set_tile_rows / the autotuner
@@ -806,7 +806,7 @@ void sw_ctx_destroy(sw_ctx* ctx)
 	DevBuf* bufs[] = {&ctx->views, &ctx->surfaces, &ctx->lights, &ctx->light_indices, &ctx->draws, &ctx->vdata, &ctx->surf_ext, &ctx->blk_surf,
 		&ctx->parts, &ctx->spans, &ctx->drec, &ctx->dbounds, &ctx->counters, &ctx->arena, &ctx->fb, &ctx->tonemap_out, &ctx->tonemap_sums,
 		&ctx->bloom0, &ctx->bloom1, &ctx->bloom_lines, &ctx->model_args, &ctx->model_scratch, &ctx->texop_args, &ctx->shadows, &ctx->depth_args, &ctx->depth_vdata,
-		&ctx->depth_rows, &ctx->depth_draw_rows, &ctx->water_points};
+		&ctx->depth_rows, &ctx->depth_draw_rows, &ctx->water_points, &ctx->fire_args};
 	for(DevBuf* b : bufs)
 		release(ctx, *b);
 	if(ctx->counters_host)
@@ -1404,6 +1404,76 @@ int sw_water_step(sw_ctx* ctx, float* field_device, float* field_old_device, uin
 		swk::launch_water_step(ctx->stream, field_device, field_old_device, log2_width, log2_height, attenuation, pts_dev, num_points);
 	}
 	return hip_check(ctx, hipGetLastError(), "water step launch");
+}
+
+int sw_fire_step(sw_ctx* ctx, uint8_t* heat_device, uint8_t* heat_old_device, uint32_t log2_width, uint32_t log2_height,
+	float attenuation, uint32_t slow, const sw_heat_point* points, uint32_t num_points)
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	if(!heat_device || !heat_old_device || (num_points && !points))
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	if(!water_size_ok(log2_width, log2_height))
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "heat map size out of range (sides >= 16, area < 2^22)");
+	if(heat_device == heat_old_device)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "the two heat buffers must differ");
+	// update_heat_map (generative_texture_effect_fire.rs:359): attenuation in 2^-18 steps
+	const uint32_t attenuation_i = uint32_t(std::fmin(std::fmax(attenuation, 0.0f), 1.0f) * float(1u << 18));
+	(void)hipSetDevice(ctx->device);
+	int r;
+	const sw_heat_point* pts_dev = nullptr;
+	if(num_points)
+	{
+		const size_t bytes = sizeof(sw_heat_point) * num_points;
+		char* staging = nullptr;
+		if((r = ensure(ctx, ctx->fire_args, bytes)) || (r = stage_begin(ctx, bytes, &staging)))
+			return r;
+		std::memcpy(staging, points, bytes);
+		if((r = stage_copy(ctx, ctx->fire_args.p, bytes)))
+			return r;
+		pts_dev = static_cast<const sw_heat_point*>(ctx->fire_args.p);
+	}
+	{
+		StageTimer t(ctx, STAGE_TEXTURES);
+		swk::launch_fire_step(ctx->stream, heat_device, heat_old_device, log2_width, log2_height, attenuation_i, slow != 0, pts_dev, num_points);
+	}
+	return hip_check(ctx, hipGetLastError(), "fire step launch");
+}
+
+int sw_fire_make_texture(sw_ctx* ctx, sw_map* map, const uint8_t* heat_device, uint32_t log2_width, uint32_t log2_height,
+	uint32_t dst_level, const uint32_t palette[256])
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	if(!map || !heat_device || !palette)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	if(!water_size_ok(log2_width, log2_height))
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "heat map size out of range (sides >= 16, area < 2^22)");
+	const std::vector<sw_texture_level>& lv = map->host_levels;
+	if(uint64_t(dst_level) + 4 >= lv.size())
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "fire destination level (and its 4 mips) out of the level table");
+	uint64_t off[5];
+	for(uint32_t mip = 0; mip <= 4; ++mip)
+	{
+		const sw_texture_level& l = lv[dst_level + mip];
+		if(l.width != (1u << (log2_width - mip)) || l.height != (1u << (log2_height - mip)) || !(l.flags & SW_TEX_LITE_LEVEL))
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "fire destination levels must be lite levels of the heat map's size and its mips");
+		off[mip] = l.offset;
+	}
+	(void)hipSetDevice(ctx->device);
+	int r;
+	char* staging = nullptr;
+	if((r = ensure(ctx, ctx->fire_args, 1024)) || (r = stage_begin(ctx, 1024, &staging)))
+		return r;
+	std::memcpy(staging, palette, 1024);
+	if((r = stage_copy(ctx, ctx->fire_args.p, 1024)))
+		return r;
+	{
+		StageTimer t(ctx, STAGE_TEXTURES);
+		swk::launch_fire_texture(ctx->stream, heat_device, log2_width, log2_height, off, static_cast<const uint32_t*>(ctx->fire_args.p),
+			static_cast<uint32_t*>(map->lite.p));
+	}
+	return hip_check(ctx, hipGetLastError(), "fire texture launch");
 }
 
 int sw_water_make_texture(sw_ctx* ctx, sw_map* map, const float* field_device, uint32_t log2_width, uint32_t log2_height,

@@ -400,7 +400,7 @@ __global__ __launch_bounds__(64) void k_water_points(float* field, float* field_
 }
 
 // update_wave_field (:307-418): the field is a torus (rows and columns wrap); one lane per element
-__global__ __launch_bounds__(256) void k_water_stencil(float* dst, const float* src, uint32_t log2w, uint32_t log2h, float attenuation)
+__global__ __launch_bounds__(256) void k_water_stencil(float* __restrict__ dst, const float* __restrict__ src, uint32_t log2w, uint32_t log2h, float attenuation)
 {
 	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
 	if(i >> (log2w + log2h))
@@ -519,6 +519,117 @@ __global__ __launch_bounds__(256) void k_texture_mip(const sw_texel* prev, sw_te
 
 } // namespace
 
+namespace
+{
+// update_heat_map (:357-468), fast variant: row y < h - 2 from rows y + 1 (x - 1, x, x + 1, wrapped)
+// and y + 2; row h - 2 takes row h - 1 twice; the last row is cleared. All reads see the old map
+// (the reference walks rows downwards and reads only rows below), so one lane per element.
+__global__ __launch_bounds__(256) void k_fire_update_fast(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, uint32_t log2w, uint32_t log2h, uint32_t att)
+{
+	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+	if(i >> (log2w + log2h))
+		return;
+	const uint32_t w = 1u << log2w, h = 1u << log2h, mx = w - 1u;
+	const uint32_t x = i & mx, y = i >> log2w;
+	if(y == h - 1u)
+	{
+		dst[i] = 0;
+		return;
+	}
+	const uint32_t r1 = (y + 1u) << log2w, r2 = (y + 2u < h ? y + 2u : y + 1u) << log2w;
+	const uint32_t sum = uint32_t(src[r1 + ((x - 1u) & mx)]) + src[r1 + x] + src[r1 + ((x + 1u) & mx)] + src[r2 + x];
+	dst[i] = uint8_t((sum * att) >> 20);
+}
+
+// the slow variant: each row is a chain (the left neighbour is the value just written; the last
+// texel's right neighbour is the row's new first texel), from the row and the row below; one lane
+// per row
+__global__ __launch_bounds__(64) void k_fire_update_slow(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, uint32_t log2w, uint32_t log2h, uint32_t att)
+{
+	const uint32_t y = blockIdx.x * 64u + threadIdx.x;
+	const uint32_t w = 1u << log2w, h = 1u << log2h;
+	if(y >= h)
+		return;
+	const uint32_t row = y << log2w;
+	if(y == h - 1u)
+	{
+		for(uint32_t x = 0; x < w; ++x)
+			dst[row + x] = 0;
+		return;
+	}
+	const uint32_t below = row + w;
+	const uint32_t first = ((uint32_t(src[row + w - 1u]) + src[row] + src[row + 1u] + src[below]) * att) >> 20;
+	dst[row] = uint8_t(first);
+	uint32_t prev = first & 0xFFu;
+	// the loads do not depend on the chain (src and dst are distinct buffers): unrolled, they overlap
+#pragma unroll 8
+	for(uint32_t x = 1; x < w - 1u; ++x)
+	{
+		const uint32_t v = ((prev + src[row + x] + src[row + x + 1u] + src[below + x]) * att) >> 20;
+		dst[row + x] = uint8_t(v);
+		prev = v & 0xFFu;
+	}
+	const uint32_t last = ((prev + src[row + w - 1u] + (first & 0xFFu) + src[below + w - 1u]) * att) >> 20;
+	dst[row + w - 1u] = uint8_t(last);
+}
+
+// set_heat (:80-86) for the step's points, 64 at a time: the first point of each address takes the
+// maximum of the batch's heats for it (max is order-free)
+__global__ __launch_bounds__(64) void k_fire_points(uint8_t* heat, const sw_heat_point* pts, uint32_t n, uint32_t log2w, uint32_t log2h)
+{
+	__shared__ uint32_t sa[64], sh[64];
+	const uint32_t lane = threadIdx.x;
+	const uint32_t mx = (1u << log2w) - 1u, my = (1u << log2h) - 1u;
+	for(uint32_t base = 0; base < n; base += 64)
+	{
+		const uint32_t cnt = n - base < 64u ? n - base : 64u;
+		uint32_t a = 0;
+		if(lane < cnt)
+		{
+			const sw_heat_point p = pts[base + lane];
+			a = (p.x & mx) + ((p.y & my) << log2w);
+			sa[lane] = a;
+			sh[lane] = p.heat & 0xFFu;
+		}
+		__syncthreads();
+		if(lane < cnt)
+		{
+			bool first = true;
+			for(uint32_t k = 0; k < lane; ++k)
+				first = first && sa[k] != a;
+			if(first)
+			{
+				uint32_t v = heat[a];
+				for(uint32_t k = lane; k < cnt; ++k)
+					if(sa[k] == a && sh[k] > v)
+						v = sh[k];
+				heat[a] = uint8_t(v);
+			}
+		}
+		__syncthreads();
+	}
+}
+
+// generate_emissive_texture_based_on_heat_map (:470-486)
+__global__ __launch_bounds__(256) void k_fire_texture(const uint8_t* heat, uint32_t area, const uint32_t* palette, uint32_t* dst)
+{
+	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+	if(i < area)
+		dst[i] = palette[heat[i]];
+}
+
+// build_texture_lite_mip (textures.rs:385-415)
+__global__ __launch_bounds__(256) void k_lite_mip(const uint32_t* prev, uint32_t* out, uint32_t log2w, uint32_t log2h)
+{
+	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+	if(i >> (log2w + log2h))
+		return;
+	const uint32_t x = i & ((1u << log2w) - 1u), y = i >> log2w, pw = 2u << log2w;
+	out[i] = average_4(prev[(2u * y) * pw + 2u * x], prev[(2u * y + 1u) * pw + 2u * x], prev[(2u * y) * pw + 2u * x + 1u],
+		prev[(2u * y + 1u) * pw + 2u * x + 1u]);
+}
+} // namespace
+
 namespace swk
 {
 void launch_water_step(hipStream_t st, float* field, float* field_old, uint32_t log2w, uint32_t log2h, float attenuation,
@@ -542,4 +653,29 @@ void launch_water_texture(hipStream_t st, const float* field, uint32_t log2w, ui
 			texels + w.dst_offset[mip], lw, lh);
 	}
 }
+void launch_fire_step(hipStream_t st, const uint8_t* heat, uint8_t* heat_old, uint32_t log2w, uint32_t log2h, uint32_t attenuation_i,
+	bool slow, const sw_heat_point* points_dev, uint32_t num_points)
+{
+	const uint32_t area = 1u << (log2w + log2h), h = 1u << log2h;
+	if(slow)
+		hipLaunchKernelGGL(k_fire_update_slow, dim3((h + 63u) / 64u), dim3(64), 0, st, heat, heat_old, log2w, log2h, attenuation_i);
+	else
+		hipLaunchKernelGGL(k_fire_update_fast, dim3((area + 255u) / 256u), dim3(256), 0, st, heat, heat_old, log2w, log2h, attenuation_i);
+	if(num_points)
+		hipLaunchKernelGGL(k_fire_points, dim3(1), dim3(64), 0, st, heat_old, points_dev, num_points, log2w, log2h);
+}
+
+void launch_fire_texture(hipStream_t st, const uint8_t* heat, uint32_t log2w, uint32_t log2h, const uint64_t dst_offset[5],
+	const uint32_t* palette_dev, uint32_t* lite)
+{
+	const uint32_t area = 1u << (log2w + log2h);
+	hipLaunchKernelGGL(k_fire_texture, dim3((area + 255u) / 256u), dim3(256), 0, st, heat, area, palette_dev, lite + dst_offset[0]);
+	for(uint32_t mip = 1; mip <= 4; ++mip)
+	{
+		const uint32_t lw = log2w - mip, lh = log2h - mip, n = 1u << (lw + lh);
+		hipLaunchKernelGGL(k_lite_mip, dim3((n + 255u) / 256u), dim3(256), 0, st, static_cast<const uint32_t*>(lite + dst_offset[mip - 1]),
+			lite + dst_offset[mip], lw, lh);
+	}
+}
+
 } // namespace swk

@@ -69,6 +69,7 @@ LIGHT_SHADOW_DTYPE = np.dtype([("kind", "<u4"), ("size", "<u4"), ("offset", "<u8
 # water (sw_water_step / sw_water_make_texture)
 WAVE_POINT_DTYPE = np.dtype([("x", "<u4"), ("y", "<u4"), ("value", "<f4")])
 WATER_COLOR_KEEP, WATER_COLOR_DEFORMED, WATER_COLOR_DEFORMED_X = 0, 1, 2
+HEAT_POINT_DTYPE = np.dtype([("x", "<u4"), ("y", "<u4"), ("heat", "<u4")])
 WATER_INIT_NONE, WATER_INIT_BASE_COLOR, WATER_INIT_SOURCE = 0, 1, 2
 
 
@@ -306,6 +307,9 @@ GPU_SYMBOLS = {
     "sw_render_depth": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64,
                                   C.c_void_p, C.c_uint64]),
     "sw_set_light_shadows": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]),
+    "sw_fire_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, C.c_void_p,
+                               C.c_uint32]),
+    "sw_fire_make_texture": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]),
     "sw_water_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_void_p, C.c_uint32]),
     "sw_water_make_texture": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                         C.POINTER(WaterTextureParams)]),
@@ -514,6 +518,33 @@ class Context:
         finally:
             a.free()
             b.free()
+
+    def fire_steps(self, heat, log2w, log2h, attenuation, slow, steps_points):
+        """sw_fire_step once per entry of steps_points (HEAT_POINT_DTYPE arrays) on a device copy of the
+        heat map (and a second buffer), swapping after each step; returns the map."""
+        a = self.to_device(np.ascontiguousarray(heat, np.uint8))
+        b = self.alloc(max(len(heat), 16))
+        try:
+            for pts in steps_points:
+                pts = np.ascontiguousarray(pts, dtype=HEAT_POINT_DTYPE)
+                self.check(self.lib.sw_fire_step(self.h, a.ptr, b.ptr, log2w, log2h, C.c_float(attenuation), 1 if slow else 0,
+                                                 _ptr(pts) if len(pts) else None, len(pts)), "sw_fire_step")
+                a, b = b, a
+            self.check(self.lib.sw_sync(self.h), "sw_sync")
+            return a.read(np.uint8, len(heat))
+        finally:
+            a.free()
+            b.free()
+
+    def fire_make_texture(self, map_handle, heat, log2w, log2h, dst_level, palette):
+        """sw_fire_make_texture from a host heat map (uploaded here) into the uploaded map's lite levels."""
+        h = self.to_device(np.ascontiguousarray(heat, np.uint8))
+        pal = np.ascontiguousarray(palette, dtype=np.uint32)
+        try:
+            self.check(self.lib.sw_fire_make_texture(self.h, map_handle, h.ptr, log2w, log2h, dst_level, _ptr(pal)), "sw_fire_make_texture")
+            self.check(self.lib.sw_sync(self.h), "sw_sync")
+        finally:
+            h.free()
 
     def water_make_texture(self, map_handle, field, log2w, log2h, params):
         """sw_water_make_texture from a host field (uploaded here) into the uploaded map's levels."""

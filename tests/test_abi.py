@@ -60,6 +60,7 @@ STRUCTS = {
     "sw_depth_draw": sw_abi.DEPTH_DRAW_DTYPE,
     "sw_depth_target": sw_abi.DEPTH_TARGET_DTYPE,
     "sw_wave_point": sw_abi.WAVE_POINT_DTYPE,
+    "sw_heat_point": sw_abi.HEAT_POINT_DTYPE,
     "sw_water_texture_params": np.dtype([(n, "<u4" if t is not ctypes.c_float else "<f4") for n, t in sw_abi.WaterTextureParams._fields_]),
 }
 
