@@ -429,24 +429,23 @@ SW_DEV int32_t pack_normal_roughness_dev(float x, float y, float z, float r)
 
 // make_wavy_texture_impl (:497-682) per texel: the normal from the field's central differences,
 // normalised as cgmath's normalize (v * (1 / |v|)), colours by mode
-__global__ __launch_bounds__(256) void k_water_texture(const float* field, uint32_t log2w, uint32_t log2h, swk::WaterTexLaunch w, sw_texel* texels)
+SW_DEV sw_texel water_texel(const float* field, uint32_t log2w, uint32_t log2h, const swk::WaterTexLaunch& w, const sw_texel* texels,
+	uint32_t x, uint32_t y)
 {
-	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-	if(i >> (log2w + log2h))
-		return;
 	const uint32_t mx = (1u << log2w) - 1u, my = (1u << log2h) - 1u;
-	const uint32_t x = i & mx, y = i >> log2w;
+	const uint32_t i = x + (y << log2w);
 	const uint32_t row = y << log2w, up = ((y - 1u) & my) << log2w, down = ((y + 1u) & my) << log2w;
 	const float dx = field[row + ((x + 1u) & mx)] - field[row + ((x - 1u) & mx)];
 	const float dy = field[down + x] - field[up + x];
 	const float m2 = (dx * dx + dy * dy) + 1.0f;
 	const float s = 1.0f / __builtin_sqrtf(m2);
 	const float nx = dx * s, ny = dy * s, nz = s;
-	sw_texel* out = texels + w.dst_offset[0] + i;
 	const sw_texel* color = texels + w.color_offset;
 	uint32_t diffuse;
 	if(w.color_mode == SW_WATER_COLOR_KEEP)
-		diffuse = w.init == SW_WATER_INIT_BASE_COLOR ? w.base_color : w.init == SW_WATER_INIT_SOURCE ? color[i].diffuse : out->diffuse;
+		diffuse = w.init == SW_WATER_INIT_BASE_COLOR ? w.base_color
+			: w.init == SW_WATER_INIT_SOURCE      ? color[i].diffuse
+												  : texels[w.dst_offset[0] + i].diffuse;
 	else
 	{
 		const int32_t du = int32_t(nx * 16.0f);
@@ -463,7 +462,7 @@ __global__ __launch_bounds__(256) void k_water_texture(const float* field, uint3
 	sw_texel t;
 	t.diffuse = diffuse;
 	t.packed_normal_roughness = pack_normal_roughness_dev(nx, ny, nz, w.roughness);
-	*out = t;
+	return t;
 }
 
 // Color32::get_average_4 (color.rs:51-71)
@@ -490,15 +489,9 @@ SW_DEV void unpack_normal_roughness_dev(int32_t p, float& x, float& y, float& z,
 	r = float(int8_t((p >> 24) & 0xFF)) * k;
 }
 
-// build_texture_mip (textures.rs:309-362): one lane per texel of the smaller level
-__global__ __launch_bounds__(256) void k_texture_mip(const sw_texel* prev, sw_texel* out, uint32_t log2w, uint32_t log2h)
+// build_texture_mip (textures.rs:309-362) for one texel
+SW_DEV sw_texel texel_mip(sw_texel p00, sw_texel p01, sw_texel p10, sw_texel p11)
 {
-	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-	if(i >> (log2w + log2h))
-		return;
-	const uint32_t x = i & ((1u << log2w) - 1u), y = i >> log2w, pw = 2u << log2w;
-	const sw_texel p00 = prev[(2u * y) * pw + 2u * x], p01 = prev[(2u * y + 1u) * pw + 2u * x];
-	const sw_texel p10 = prev[(2u * y) * pw + 2u * x + 1u], p11 = prev[(2u * y + 1u) * pw + 2u * x + 1u];
 	float n[4][3], r[4];
 	unpack_normal_roughness_dev(p00.packed_normal_roughness, n[0][0], n[0][1], n[0][2], r[0]);
 	unpack_normal_roughness_dev(p01.packed_normal_roughness, n[1][0], n[1][1], n[1][2], r[1]);
@@ -514,7 +507,68 @@ __global__ __launch_bounds__(256) void k_texture_mip(const sw_texel* prev, sw_te
 	sw_texel t;
 	t.diffuse = average_4(p00.diffuse, p01.diffuse, p10.diffuse, p11.diffuse);
 	t.packed_normal_roughness = pack_normal_roughness_dev(sx / len, sy / len, sz / len, dr);
-	out[i] = t;
+	return t;
+}
+
+// The wavy texture of one 16 x 16 tile of mip 0 and the tile's texels of mips 1 .. 4, built in LDS
+// (a mip texel depends only on the square of mip 0 under it): one launch for the texture and its mips.
+__global__ __launch_bounds__(256) void k_water_texture_tiles(const float* field, uint32_t log2w, uint32_t log2h, swk::WaterTexLaunch w,
+	sw_texel* texels)
+{
+	__shared__ sw_texel tile[16 * 16];
+	const uint32_t tx = blockIdx.x & ((1u << (log2w - 4)) - 1u), ty = blockIdx.x >> (log2w - 4);
+	const uint32_t lx = threadIdx.x & 15u, ly = threadIdx.x >> 4;
+	const uint32_t x = tx * 16u + lx, y = ty * 16u + ly;
+	const sw_texel t = water_texel(field, log2w, log2h, w, texels, x, y);
+	tile[threadIdx.x] = t;
+	texels[w.dst_offset[0] + x + (y << log2w)] = t; // each lane reads only its own old texel
+	__syncthreads();
+	for(uint32_t mip = 1, n = 8; mip <= 4; ++mip, n >>= 1)
+	{
+		sw_texel v{};
+		const bool active = lx < n && ly < n;
+		if(active)
+			v = texel_mip(tile[(2u * ly) * 16u + 2u * lx], tile[(2u * ly + 1u) * 16u + 2u * lx], tile[(2u * ly) * 16u + 2u * lx + 1u],
+				tile[(2u * ly + 1u) * 16u + 2u * lx + 1u]);
+		__syncthreads();
+		if(active)
+		{
+			tile[ly * 16u + lx] = v;
+			texels[w.dst_offset[mip] + (tx * n + lx) + ((ty * n + ly) << (log2w - mip))] = v;
+		}
+		__syncthreads();
+	}
+}
+
+// The palette texture (generate_emissive_texture_based_on_heat_map, :470-486) of one 16 x 16 tile and
+// the tile's texels of lite mips 1 .. 4 (build_texture_lite_mip, textures.rs:385-415).
+__global__ __launch_bounds__(256) void k_fire_texture_tiles(const uint8_t* heat, uint32_t log2w, const uint32_t* palette, uint32_t* lite,
+	uint64_t o0, uint64_t o1, uint64_t o2, uint64_t o3, uint64_t o4)
+{
+	__shared__ uint32_t tile[16 * 16];
+	const uint64_t off[5] = {o0, o1, o2, o3, o4};
+	const uint32_t tx = blockIdx.x & ((1u << (log2w - 4)) - 1u), ty = blockIdx.x >> (log2w - 4);
+	const uint32_t lx = threadIdx.x & 15u, ly = threadIdx.x >> 4;
+	const uint32_t x = tx * 16u + lx, y = ty * 16u + ly;
+	const uint32_t c = palette[heat[x + (y << log2w)]];
+	tile[threadIdx.x] = c;
+	lite[off[0] + x + (y << log2w)] = c;
+	__syncthreads();
+	for(uint32_t mip = 1, n = 8; mip <= 4; ++mip, n >>= 1)
+	{
+		uint32_t v = 0;
+		const bool active = lx < n && ly < n;
+		if(active)
+			v = average_4(tile[(2u * ly) * 16u + 2u * lx], tile[(2u * ly + 1u) * 16u + 2u * lx], tile[(2u * ly) * 16u + 2u * lx + 1u],
+				tile[(2u * ly + 1u) * 16u + 2u * lx + 1u]);
+		__syncthreads();
+		if(active)
+		{
+			tile[ly * 16u + lx] = v;
+			lite[off[mip] + (tx * n + lx) + ((ty * n + ly) << (log2w - mip))] = v;
+		}
+		__syncthreads();
+	}
 }
 
 } // namespace
@@ -610,24 +664,6 @@ __global__ __launch_bounds__(64) void k_fire_points(uint8_t* heat, const sw_heat
 	}
 }
 
-// generate_emissive_texture_based_on_heat_map (:470-486)
-__global__ __launch_bounds__(256) void k_fire_texture(const uint8_t* heat, uint32_t area, const uint32_t* palette, uint32_t* dst)
-{
-	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-	if(i < area)
-		dst[i] = palette[heat[i]];
-}
-
-// build_texture_lite_mip (textures.rs:385-415)
-__global__ __launch_bounds__(256) void k_lite_mip(const uint32_t* prev, uint32_t* out, uint32_t log2w, uint32_t log2h)
-{
-	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-	if(i >> (log2w + log2h))
-		return;
-	const uint32_t x = i & ((1u << log2w) - 1u), y = i >> log2w, pw = 2u << log2w;
-	out[i] = average_4(prev[(2u * y) * pw + 2u * x], prev[(2u * y + 1u) * pw + 2u * x], prev[(2u * y) * pw + 2u * x + 1u],
-		prev[(2u * y + 1u) * pw + 2u * x + 1u]);
-}
 } // namespace
 
 namespace swk
@@ -644,14 +680,7 @@ void launch_water_step(hipStream_t st, float* field, float* field_old, uint32_t 
 
 void launch_water_texture(hipStream_t st, const float* field, uint32_t log2w, uint32_t log2h, const WaterTexLaunch& w, sw_texel* texels)
 {
-	const uint32_t area = 1u << (log2w + log2h);
-	hipLaunchKernelGGL(k_water_texture, dim3((area + 255u) / 256u), dim3(256), 0, st, field, log2w, log2h, w, texels);
-	for(uint32_t mip = 1; mip <= 4; ++mip)
-	{
-		const uint32_t lw = log2w - mip, lh = log2h - mip, n = 1u << (lw + lh);
-		hipLaunchKernelGGL(k_texture_mip, dim3((n + 255u) / 256u), dim3(256), 0, st, static_cast<const sw_texel*>(texels + w.dst_offset[mip - 1]),
-			texels + w.dst_offset[mip], lw, lh);
-	}
+	hipLaunchKernelGGL(k_water_texture_tiles, dim3(1u << (log2w + log2h - 8)), dim3(256), 0, st, field, log2w, log2h, w, texels);
 }
 void launch_fire_step(hipStream_t st, const uint8_t* heat, uint8_t* heat_old, uint32_t log2w, uint32_t log2h, uint32_t attenuation_i,
 	bool slow, const sw_heat_point* points_dev, uint32_t num_points)
@@ -668,14 +697,8 @@ void launch_fire_step(hipStream_t st, const uint8_t* heat, uint8_t* heat_old, ui
 void launch_fire_texture(hipStream_t st, const uint8_t* heat, uint32_t log2w, uint32_t log2h, const uint64_t dst_offset[5],
 	const uint32_t* palette_dev, uint32_t* lite)
 {
-	const uint32_t area = 1u << (log2w + log2h);
-	hipLaunchKernelGGL(k_fire_texture, dim3((area + 255u) / 256u), dim3(256), 0, st, heat, area, palette_dev, lite + dst_offset[0]);
-	for(uint32_t mip = 1; mip <= 4; ++mip)
-	{
-		const uint32_t lw = log2w - mip, lh = log2h - mip, n = 1u << (lw + lh);
-		hipLaunchKernelGGL(k_lite_mip, dim3((n + 255u) / 256u), dim3(256), 0, st, static_cast<const uint32_t*>(lite + dst_offset[mip - 1]),
-			lite + dst_offset[mip], lw, lh);
-	}
+	hipLaunchKernelGGL(k_fire_texture_tiles, dim3(1u << (log2w + log2h - 8)), dim3(256), 0, st, heat, log2w, palette_dev, lite, dst_offset[0],
+		dst_offset[1], dst_offset[2], dst_offset[3], dst_offset[4]);
 }
 
 } // namespace swk
