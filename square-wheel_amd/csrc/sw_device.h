@@ -16,6 +16,9 @@
 #include <stdint.h>
 
 #define SW_DEV __device__ __forceinline__
+#define SW_HD __host__ __device__ __forceinline__
+
+#include "sw_divide.h"
 
 namespace swd
 {
@@ -218,24 +221,7 @@ SW_DEV int32_t fixed16_floor_to_int(int32_t x) { return x >> 16; }
 SW_DEV float fixed16_to_f32(int32_t x) { return float(x) / float(FIXED16_ONE); }
 SW_DEV int32_t f32_to_fixed16_unchecked(float x) { return f32_to_i32_trunc_x86(x * float(FIXED16_ONE)); }
 SW_DEV int32_t fixed16_mul(int32_t x, int32_t y) { return int32_t((int64_t(x) * int64_t(y)) >> 16); }
-// ((x as i64) << 16) / y, truncated (fixed_math.rs:56-59). |x << 16| < 2^47 is exact in a double; the
-// correctly rounded double quotient, truncated, is off by at most one, which one integer step corrects.
-// Much cheaper than the GPU's software 64-bit integer division.
-SW_DEV int32_t fixed16_div(int32_t x, int32_t y)
-{
-	if(y == 0)
-		return 0; // unreachable for inputs the reference accepts (it would panic)
-	const int64_t a = int64_t(x) * 65536;
-	const uint64_t ua = a < 0 ? uint64_t(-a) : uint64_t(a);
-	const uint64_t ub = y < 0 ? uint64_t(-int64_t(y)) : uint64_t(y);
-	uint64_t q = uint64_t(double(ua) / double(ub));
-	const int64_t r = int64_t(ua) - int64_t(q * ub);
-	if(r < 0)
-		q -= 1;
-	else if(r >= int64_t(ub))
-		q += 1;
-	return int32_t((a < 0) != (y < 0) ? -int64_t(q) : int64_t(q));
-}
+using swdiv::fixed16_div;
 SW_DEV int32_t fixed16_mul_div(int32_t x, int32_t y, int32_t z)
 {
 	if(z == 0)

@@ -237,45 +237,12 @@ enum : uint32_t
 	REC_BLEND_SHIFT = 12, // 4 bits: the emissive layer's blend
 	REC_LIGHTS_SHIFT = 16, // clamped light count
 };
-#define NO_MASK 0xFFFFFFFFu
 #define SURF_CHUNK 32 // surface records per pass of a block
 
-// n mod d for 1 <= d <= 2^30: a power of two by its mask, otherwise through the double reciprocal,
-// whose truncated quotient is within one of n / d for every 32-bit n (relative error below 2^-51,
-// quotient below 2^32), so one correction either way makes it exact.
-SW_DEV uint32_t mod_rcp(uint32_t n, uint32_t d, uint32_t mask, double inv)
-{
-	if(mask != NO_MASK)
-		return n & mask;
-	const uint32_t q = uint32_t(double(n) * inv);
-	int32_t r = int32_t(n - q * d);
-	if(r < 0)
-		r += int32_t(d);
-	else if(r >= int32_t(d))
-		r -= int32_t(d);
-	return uint32_t(r);
-}
-
-// n / d and n mod d for 1 <= d <= 2^30, exact as mod_rcp.
-SW_DEV uint32_t div_rcp(uint32_t n, uint32_t d, double inv, uint32_t& rem)
-{
-	uint32_t q = uint32_t(double(n) * inv);
-	int32_t r = int32_t(n - q * d);
-	if(r < 0)
-	{
-		r += int32_t(d);
-		--q;
-	}
-	else if(r >= int32_t(d))
-	{
-		r -= int32_t(d);
-		++q;
-	}
-	rem = uint32_t(r);
-	return q;
-}
-
-SW_DEV uint32_t pow2_mask(uint32_t d) { return (d & (d - 1u)) == 0u ? d - 1u : NO_MASK; }
+using swdiv::div_rcp;
+using swdiv::mod_rcp;
+using swdiv::NO_MASK;
+using swdiv::pow2_mask;
 
 // Wrap set-up of one texture level: with tc_start[1] + dst_v free of i32 overflow and sizes below
 // 2^30, rem_euclid(tc_start + d, size) == (rem_euclid(tc_start, size) + d) mod size, which the
@@ -1787,10 +1754,8 @@ SW_DEV int32_t polygon_texel_index(const int32_t* sv, int32_t j, uint32_t tw, ui
 		const int32_t inv_z = wadd(sv[0], wmul(j, sv[1]));
 		uint32_t div = uint32_t(inv_z) >> INV_Z_PRE_SHIFT;
 		div = div ? div : 1u;
-		// (1 << Z_CALC_SHIFT) / div: the correctly rounded double quotient truncates to the integer
-		// quotient exactly (its fraction is 0 or at least 1 / div, far above the double's rounding
-		// error of at most 2^-53 relative), at a fraction of the integer division's cost.
-		const uint32_t z = uint32_t(double(1u << Z_CALC_SHIFT) / double(div));
+		// (1 << Z_CALC_SHIFT) / div, exact through the double quotient
+		const uint32_t z = swdiv::z_from_inv_z(div);
 		const int32_t t0 = wadd(sv[2], wmul(j, sv[3]));
 		const int32_t t1 = wadd(sv[4], wmul(j, sv[5]));
 		u = uint32_t((uint64_t(z) * uint64_t(int64_t(t0))) >> TC_FINAL_SHIFT);
