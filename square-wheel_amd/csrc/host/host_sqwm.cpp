@@ -466,10 +466,34 @@ bool parse_floats(const std::string& s, float* out, int n)
 bool validate(const swh_map& m, std::string& error)
 {
 	const uint32_t nn = uint32_t(m.nodes.size()), nl = uint32_t(m.leafs.size()), np = uint32_t(m.polygons.size());
+	if(nn == 0 || nl == 0)
+		return (error = "map without nodes or leafs"), false;
 	for(const Node& n : m.nodes)
 		for(uint32_t c : n.children)
 			if(c >= FIRST_LEAF_INDEX ? c - FIRST_LEAF_INDEX >= nl : c >= nn)
 				return (error = "node child out of range"), false;
+	// The walks descend from the root (the last node, bsp_map_compact.rs:289-292) without a depth
+	// limit: every node and leaf must be reached exactly once, so a child pointing back at an ancestor
+	// (a cycle) or a shared subtree is an error.
+	{
+		std::vector<uint8_t> seen_node(nn, 0), seen_leaf(nl, 0);
+		std::vector<uint32_t> stack{nn - 1};
+		while(!stack.empty())
+		{
+			const uint32_t index = stack.back();
+			stack.pop_back();
+			if(index >= FIRST_LEAF_INDEX)
+			{
+				if(seen_leaf[index - FIRST_LEAF_INDEX]++)
+					return (error = "BSP leaf reached twice: the nodes do not form a tree"), false;
+				continue;
+			}
+			if(seen_node[index]++)
+				return (error = "BSP node reached twice: the nodes do not form a tree"), false;
+			stack.push_back(m.nodes[index].children[0]);
+			stack.push_back(m.nodes[index].children[1]);
+		}
+	}
 	for(const Leaf& l : m.leafs)
 		if(uint64_t(l.first_polygon) + l.num_polygons > np || uint64_t(l.first_leaf_portal) + l.num_leaf_portals > m.leafs_portals.size())
 			return (error = "leaf polygon or portal range out of range"), false;
@@ -482,8 +506,20 @@ bool validate(const swh_map& m, std::string& error)
 			return (error = "polygon vertex range out of range"), false;
 		if(p.texture >= m.materials.size())
 			return (error = "polygon texture out of range"), false;
-		if(p.lightmap_data_offset >= (m.lm_dir.empty() ? m.lm_simple.size() / 3 : m.lm_dir.size()) && p.lightmap_data_offset != 0)
-			return (error = "polygon lightmap offset out of range"), false;
+		if(p.lightmap_data_offset != 0)
+		{
+			// get_lightmap_size (lightmap.rs:19-27) of the polygon's texture-coordinate extent: the
+			// surface builder and the decal / model light lookups read lw * lh elements from the offset
+			const int64_t lw = ((int64_t(p.tex_coord_max[0]) + LIGHTMAP_SCALE - 1) >> LIGHTMAP_SCALE_LOG2) -
+				(int64_t(p.tex_coord_min[0]) >> LIGHTMAP_SCALE_LOG2) + 1;
+			const int64_t lh = ((int64_t(p.tex_coord_max[1]) + LIGHTMAP_SCALE - 1) >> LIGHTMAP_SCALE_LOG2) -
+				(int64_t(p.tex_coord_min[1]) >> LIGHTMAP_SCALE_LOG2) + 1;
+			const uint64_t count = m.lm_dir.empty() ? m.lm_simple.size() / 3 : m.lm_dir.size();
+			if(lw < 1 || lh < 1 || lw > SW_MAX_LIGHTMAP_SIZE + 2 || lh > SW_MAX_LIGHTMAP_SIZE + 2)
+				return (error = "polygon lightmap size out of range"), false;
+			if(uint64_t(p.lightmap_data_offset) + uint64_t(lw * lh) > count)
+				return (error = "polygon lightmap extends past the lightmap data"), false;
+		}
 	}
 	for(const Portal& p : m.portals)
 		if(p.leafs[0] >= nl || p.leafs[1] >= nl || uint64_t(p.first_vertex) + p.num_vertices > m.vertices.size())

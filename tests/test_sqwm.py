@@ -211,7 +211,8 @@ def _patch(src, dst, offset, value, dtype="<u4"):
     open(dst, "wb").write(bytes(raw))
 
 
-@pytest.mark.parametrize("what", ["truncated", "id", "version", "element_size", "polygon_vertex", "node_child", "texture_name"])
+@pytest.mark.parametrize("what", ["truncated", "id", "version", "element_size", "polygon_vertex", "node_child", "texture_name",
+                                  "lightmap_extent", "node_cycle", "node_shared"])
 def test_bad_files_are_errors(maps, what):
     m, path, d = maps
     bad = os.path.join(d, f"bad_{what}.sqwm")
@@ -229,6 +230,18 @@ def test_bad_files_are_errors(maps, what):
         _patch(path, bad, int(lumps[2, 0]) + 80 * 5, 1 << 30)
     elif what == "node_child":
         _patch(path, bad, int(lumps[0, 0]), 0x7FFFFFFF)
+    elif what == "lightmap_extent":
+        # a lit polygon's lightmap starts at the last element: in range, but its lw * lh extent is not
+        polys = np.frombuffer(raw, DT[2], int(lumps[2, 2]), int(lumps[2, 0]))  # lump: offset, element size, count
+        k = int(np.nonzero(polys["lightmap_data_offset"])[0][0])
+        count = max(int(lumps[12, 2]), int(lumps[13, 2]))
+        _patch(path, bad, int(lumps[2, 0]) + 80 * k + DT[2].fields["lightmap_data_offset"][1], count - 1)
+    elif what in ("node_cycle", "node_shared"):
+        # the root (last node) points back at itself, or both its children name one subtree
+        nn = int(lumps[0, 2])
+        root = int(lumps[0, 0]) + 24 * (nn - 1)
+        other = int(np.frombuffer(raw, "<u4", 1, root + 4)[0])
+        _patch(path, bad, root, nn - 1 if what == "node_cycle" else other)
     else:
         raw2 = bytearray(raw)
         raw2[int(lumps[6, 0]):int(lumps[6, 0]) + 8] = b"unknown_"
