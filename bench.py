@@ -68,32 +68,36 @@ def read_traffic(workload, kernel):
         return None
 
 
-def build_frames(m, poses, width, height, color_bits):
-    fb = sw_host.FrameBuilder()
-    frames = []
-    for p in poses:
-        frames.append(fb.build(m, p, width=width, height=height, color_bits=color_bits, use_directional_lightmaps=1,
-                               clear_background=1, force_clear=1))
-    return frames
+def cpu_model_name():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(m, batches, seconds, threads):
+    """The C++ restatement of the reference's CPU renderer (oracle/), timed frame by frame on this
+    host's cores with the reference's decomposition (surfaces in parallel, the screen split into
+    `threads` rects). Returns (frames/s, frames, seconds, per-frame latencies in s)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # the CPU restatement; timed as the baseline only
 
     oracle.load()
-    n = 0
+    lat = []
     t0 = time.perf_counter()
-    pixels = 0
     while True:
-        b = batches[n % len(batches)]
+        b = batches[len(lat) % len(batches)]
+        ts = time.perf_counter()
         oracle.render(m.data, b, mode=oracle.MODE_SSE, num_threads=threads)
-        n += 1
-        pixels += int(b.views[0]["width"]) * int(b.views[0]["height"])
+        lat.append(time.perf_counter() - ts)
         el = time.perf_counter() - t0
-        if el >= seconds or n >= 4096:
+        if el >= seconds or len(lat) >= 4096:
             break
-    return n / el, n, el
+    return len(lat) / el, len(lat), el, np.array(lat)
 
 
 def hdr_leg(ctx, args):
@@ -105,10 +109,7 @@ def hdr_leg(ctx, args):
     m = sw_host.large_map_alpha(args.seed)
     mh = ctx.upload_map(m.data)
     fbld = sw_host.FrameBuilder()
-    poses = m.camera_path(args.seed + 3, args.hdr_poses)
-    frames = [fbld.build(m, p, width=W, height=H, color_bits=64, use_directional_lightmaps=1, num_dynamic_lights=16,
-                         num_models=64, num_decals=256, seed=args.seed + 5, time=0.1 * i, clear_background=1, force_clear=1)
-              for i, p in enumerate(poses)]
+    frames = sw_host.hdr_models_frames(m, args.seed, args.hdr_poses, W, H, fbld)
     batches = [sw_host.batch_from_frames([f]) for f in frames]
     dev = [sw_abi.DeviceCommandBuffers(ctx, b) for b in batches]
     hdr = ctx.alloc(W * H * 8)
@@ -432,18 +433,10 @@ def serving_leg(args):
     only). The server batches requests on the fly onto three concurrent slots under an HBM budget;
     results are copied by the device into pinned host buffers; latency is arrival -> result in host
     memory."""
-    rng = np.random.default_rng(args.seed + 77)
-    grids = [(16, 16, 4), (24, 24, 4), (32, 32, 4), (48, 48, 4)]
-    maps = [sw_host.large_map(args.seed + i, grid=g) for i, g in enumerate(grids)]
-    fbld = sw_host.FrameBuilder()
-    pool = []
-    for i, m in enumerate(maps):
-        for p in m.camera_path(args.seed + 40 + i, 12):
-            w = int(round(math.exp(rng.uniform(math.log(320), math.log(2560)))))
-            h = max(1, int(round(w * 10 / 16)))
-            bits = 64 if rng.uniform() < 0.5 else 32
-            pool.append((i, fbld.build(m, p, width=w, height=h, color_bits=bits, use_directional_lightmaps=1,
-                                       num_dynamic_lights=4, clear_background=1, force_clear=1)))
+    grids = sw_host.SERVING_GRIDS
+    maps = sw_host.serving_maps(args.seed)
+    pool = sw_host.serving_request_pool(maps, args.seed)
+    rng = np.random.default_rng(args.seed + 78)  # arrivals and picks
     mean_px = float(np.mean([int(np.asarray(f["view"]["width"]).item()) * int(np.asarray(f["view"]["height"]).item()) for _, f in pool]))
     srv = sw_abi.Server(slots=3, hbm_budget=16 << 30, max_views=32)
     ids = [srv.add_map(m.data) for m in maps]
@@ -569,8 +562,7 @@ def main():
     t = time.perf_counter()
     m = sw_host.large_map(args.seed)
     log(f"map built in {time.perf_counter() - t:.1f}s: {m.info}")
-    poses = m.camera_path(args.seed + 1, args.poses)
-    frames = build_frames(m, poses, args.width, args.height, color_bits)
+    frames = sw_host.plain_1080p_frames(m, args.seed, args.poses, args.width, args.height)
     batches = [sw_host.batch_from_frames([f]) for f in frames]
     log(f"{len(frames)} frames built; draws/frame {np.mean([len(f['draws']) for f in frames]):.0f}, "
         f"surface texels/frame {np.mean([f['arena_texels'] for f in frames]):.0f}")
@@ -605,29 +597,32 @@ def main():
     log(f"autotune (plain): {plain_rows} rows; ms per frame by rows {plain_tune}")
     for i in range(args.warmup):
         render(i)
-    ctx.profile_read(reset=True)
-    # Timed region. Every 8th frame carries HIP events around its launches (on the context's
-    # stream), which give the per-kernel durations of the roofline; recording events costs host
-    # time, so the other frames go without.
-    lat = np.zeros(args.steps)
+    # Timed region: K frames of the path, each call returns with its frame finished.
     t0 = time.perf_counter()
     for i in range(args.steps):
-        sample = i % 8 == 0
-        if sample:
-            ctx.profile_enable(True)
+        render(i)
+    wall = time.perf_counter() - t0
+    fps = args.steps / wall
+    # Frame latency over a fixed sample (four passes of the path), whatever --steps is.
+    lat_frames = 4 * len(frames)
+    lat = np.zeros(lat_frames)
+    for i in range(lat_frames):
         ts = time.perf_counter()
         render(i)
         lat[i] = time.perf_counter() - ts
-        if sample:
-            ctx.profile_enable(False)
-    wall = time.perf_counter() - t0
+    # Per-kernel durations for the roofline: HIP events around the launches on the context's stream,
+    # over one pass of the path (recording events costs host time, so the timed frames go without).
+    ctx.profile_read(reset=True)
+    ctx.profile_enable(True)
+    for i in range(len(frames)):
+        render(i)
+    ctx.profile_enable(False)
     prof = ctx.profile_read(reset=True)
-    fps = args.steps / wall
     log(f"plain: {fps:.1f} frames/s, p50 {np.percentile(lat, 50) * 1e3:.3f} ms, p99 {np.percentile(lat, 99) * 1e3:.3f} ms")
     log(f"kernels: { {k: (round(v[0] / max(v[1], 1) * 1e3, 2), v[1]) for k, v in prof.items()} } (us/launch, launches)")
 
-    # roofline of the dominant kernel over the timed region
-    steps_stats = [per_frame_stats[i % len(per_frame_stats)] for i in range(args.steps) if i % 8 == 0]
+    # roofline of the dominant kernel over the event-timed pass
+    steps_stats = [per_frame_stats[i] for i in range(len(frames))]
     bytes_sum = {"build_surfaces": 0, "raster_tiles": 0, "setup_and_surfaces": 0}
     for st in steps_stats:
         for k, v in algorithmic_bytes(st, color_bits, 40).items():
@@ -641,7 +636,10 @@ def main():
                    "build_surfaces": "k_build_surfaces"}.get(dominant)
     traffic = read_traffic("plain1080p", traffic_key) if traffic_key else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dominant,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "traffic_source": "profiles/traffic.json.txt: rocprofv3 FETCH_SIZE + WRITE_SIZE runs of this workload, "
+                                  "scaled by a copy kernel's reading" if traffic is not None else None,
+                "kernel": dominant,
                 "algorithmic_bytes_per_launch": int(bytes_sum[dominant] / launches),
                 "avg_launch_us": round(avg_s * 1e6, 3)}
 
@@ -663,26 +661,17 @@ def main():
                                f"{args.poses}-pose camera path, command buffers resident in HBM"},
         "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 4),
         "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 4),
+        "latency_frames": lat_frames,
         "plain_gpixel_s": round(fps * args.width * args.height / 1e9, 4),
         "kernel_us_per_launch": {k: round(v[0] / max(v[1], 1) * 1e3, 3) for k, v in prof.items()},
         "tile_rows": plain_rows,
-        "event_sampled_frames": (args.steps + 7) // 8,
+        "event_sampled_frames": len(frames),
         "roofline": roofline,
     }
 
     if not args.no_batch:
-        rng = np.random.default_rng(args.seed + 17)
-        bposes = m.camera_path(args.seed + 2, args.batch_views)
-        fbld = sw_host.FrameBuilder()
-        bframes = []
-        total_px = 0
-        for i in range(args.batch_views):
-            wid = int(round(math.exp(rng.uniform(math.log(160), math.log(1920)))))
-            hei = max(1, int(round(wid * (9 / 16 if rng.uniform() < 0.5 else 3 / 4))))
-            wid, hei = min(wid, 1920), min(hei, 1080)
-            bframes.append(fbld.build(m, bposes[i], width=wid, height=hei, color_bits=color_bits, use_directional_lightmaps=1,
-                                      clear_background=1, force_clear=1))
-            total_px += wid * hei
+        bframes = sw_host.batch_view_frames(m, args.seed, args.batch_views, color_bits=color_bits)
+        total_px = sum(int(np.asarray(f["view"]["width"]).item()) * int(np.asarray(f["view"]["height"]).item()) for f in bframes)
         bb = sw_host.batch_from_frames(bframes)
         bdev = sw_abi.DeviceCommandBuffers(ctx, bb)
         bout = ctx.alloc(bb.fb_texels * 4)
@@ -742,11 +731,14 @@ def main():
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
         threads = max(1, min(threads, 64))
-        cfps, n, el = cpu_baseline(m, batches, args.cpu_seconds, threads)
+        cfps, n, el, clat = cpu_baseline(m, batches, args.cpu_seconds, threads)
         result["cpu_baseline"] = {"value": round(cfps, 3), "unit": "frames/s", "cores": threads, "kind": "port",
+                                  "p50_ms": round(float(np.percentile(clat, 50)) * 1e3, 3),
+                                  "p99_ms": round(float(np.percentile(clat, 99)) * 1e3, 3),
+                                  "cpu": cpu_model_name(),
                                   "sample": f"{n} frames of the same {args.poses}-pose 1080p path in {el:.1f}s "
                                             f"(surface build + raster, reference decomposition over {threads} threads, "
-                                            "x86 rsqrtss/rcpss numerics)"}
+                                            "x86 rsqrtss/rcpss numerics, per-thread work counters)"}
         result["speedup_vs_cpu"] = round(fps / cfps, 2)
         log(f"cpu baseline: {cfps:.2f} frames/s on {threads} threads")
 

@@ -330,13 +330,26 @@ inline int32_t rem_euclid(int32_t a, int32_t b)
 // ---------------------------------------------------------------------------------------------
 // Instrumentation (algorithmic work counts)
 // ---------------------------------------------------------------------------------------------
+// Work counters of one thread (the threaded path keeps one per thread and adds them after the join,
+// so counting costs no shared-memory traffic in the timed CPU baseline).
 struct Counters
 {
-	std::atomic<uint64_t> surface_texels{0}, lightmap_elements{0}, emissive_texels{0}, polygon_pixels{0},
-		triangle_pixels{0}, blend_pixels{0};
+	uint64_t surface_texels = 0, lightmap_elements = 0, emissive_texels = 0, polygon_pixels = 0, triangle_pixels = 0,
+		blend_pixels = 0;
 	// Texel fetches outside the texture. The reference debug_assert!s against them
 	// (rasterizer.rs:436-437, :561-564, :709-712, :1231-1234); here they are skipped and counted.
-	std::atomic<uint64_t> out_of_bounds{0};
+	uint64_t out_of_bounds = 0;
+
+	void add(const Counters& o)
+	{
+		surface_texels += o.surface_texels;
+		lightmap_elements += o.lightmap_elements;
+		emissive_texels += o.emissive_texels;
+		polygon_pixels += o.polygon_pixels;
+		triangle_pixels += o.triangle_pixels;
+		blend_pixels += o.blend_pixels;
+		out_of_bounds += o.out_of_bounds;
+	}
 };
 
 inline bool blend_reads_dst(uint32_t blend)
@@ -1550,14 +1563,15 @@ int render_batch(
 			std::atomic<uint32_t> next{view.first_surface};
 			const uint32_t end = view.first_surface + view.num_surfaces;
 			std::vector<std::thread> pool;
+			std::vector<Counters> local(std::max<uint32_t>(num_threads, 64));
 			for(uint32_t t = 0; t < num_threads; ++t)
-				pool.emplace_back([&]() {
+				pool.emplace_back([&, t]() {
 					for(;;)
 					{
 						const uint32_t i = next.fetch_add(4);
 						if(i >= end)
 							break;
-						build_view_surfaces<CT>(nm, m, *b, view, arena, i, std::min(i + 4, end), cptr);
+						build_view_surfaces<CT>(nm, m, *b, view, arena, i, std::min(i + 4, end), &local[t]);
 					}
 				});
 			for(auto& t : pool)
@@ -1567,13 +1581,19 @@ int render_batch(
 				std::fill(vfb, vfb + uint64_t(view.width) * view.height, typename CT::T(view.background_color));
 			std::vector<RectF> rects;
 			split_rect_r(RectF{0.0f, 0.0f, float(view.width), float(view.height)}, std::min<uint32_t>(num_threads, 64), rects);
-			for(const RectF& rf : rects)
-				pool.emplace_back([&, rf]() {
+			std::vector<Counters> rect_local(rects.size());
+			for(size_t k = 0; k < rects.size(); ++k)
+				pool.emplace_back([&, k]() {
+					const RectF& rf = rects[k];
 					const ClipRect c{int32_t(rf.min_x), int32_t(rf.min_y), int32_t(rf.max_x), int32_t(rf.max_y)};
-					raster_view<CT>(m, *b, view, arena, vfb, &c, cptr);
+					raster_view<CT>(m, *b, view, arena, vfb, &c, &rect_local[k]);
 				});
 			for(auto& t : pool)
 				t.join();
+			for(const Counters& c : local)
+				counters.add(c);
+			for(const Counters& c : rect_local)
+				counters.add(c);
 		}
 	}
 	if(stats)

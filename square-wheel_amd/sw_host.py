@@ -381,6 +381,64 @@ def large_map_alpha(seed=0, **kw):
     return large_map(seed, alpha_blend_fraction=0.1, **kw)
 
 
+def plain_1080p_frames(m, seed=0, n=64, width=1920, height=1080, fbld=None):
+    """The plain drop-in workload: an n-pose seeded camera path through map m (camera_path(seed + 1)),
+    each pose a Color32 frame with directional lightmaps that clears its background."""
+    fbld = fbld or FrameBuilder()
+    return [fbld.build(m, p, width=width, height=height, color_bits=32, use_directional_lightmaps=1, clear_background=1,
+                       force_clear=1) for p in m.camera_path(seed + 1, n)]
+
+
+def hdr_models_frames(m, seed=0, n=16, width=2560, height=1440, fbld=None):
+    """The HDR + models workload on the alpha map: Color64 frames with 16 dynamic lights, 64 animated
+    models and 256 decals along an n-pose path (camera_path(seed + 3)), model time 0.1 s per pose."""
+    fbld = fbld or FrameBuilder()
+    return [fbld.build(m, p, width=width, height=height, color_bits=64, use_directional_lightmaps=1, num_dynamic_lights=16,
+                       num_models=64, num_decals=256, seed=seed + 5, time=0.1 * i, clear_background=1, force_clear=1)
+            for i, p in enumerate(m.camera_path(seed + 3, n))]
+
+
+def batch_view_frames(m, seed=0, n_views=256, fbld=None, color_bits=32):
+    """The batched workload: n_views views of map m with independent seeded poses (camera_path(seed + 2)),
+    widths log-uniform in [160, 1920], each view 16:9 or 4:3 by a seeded coin, capped at 1920x1080."""
+    rng = np.random.default_rng(seed + 17)
+    poses = m.camera_path(seed + 2, n_views)
+    fbld = fbld or FrameBuilder()
+    frames = []
+    for i in range(n_views):
+        wid = int(round(math.exp(rng.uniform(math.log(160), math.log(1920)))))
+        hei = max(1, int(round(wid * (9 / 16 if rng.uniform() < 0.5 else 3 / 4))))
+        wid, hei = min(wid, 1920), min(hei, 1080)
+        frames.append(fbld.build(m, poses[i], width=wid, height=hei, color_bits=color_bits, use_directional_lightmaps=1,
+                                 clear_background=1, force_clear=1))
+    return frames
+
+
+SERVING_GRIDS = [(16, 16, 4), (24, 24, 4), (32, 32, 4), (48, 48, 4)]
+
+
+def serving_maps(seed=0):
+    """The serving workload's four resident maps (the 1080p recipe with seeds seed..seed+3)."""
+    return [large_map(seed + i, grid=g) for i, g in enumerate(SERVING_GRIDS)]
+
+
+def serving_request_pool(maps, seed=0, per_map=12, fbld=None):
+    """Requests of the serving workload: per map a seeded path of poses (camera_path(seed + 40 + i)),
+    each a frame of log-uniform width in [320, 2560] at 16:10, Color32 or Color64 by a seeded coin,
+    4 dynamic lights. Returns [(map index, frame)]."""
+    rng = np.random.default_rng(seed + 77)
+    fbld = fbld or FrameBuilder()
+    pool = []
+    for i, m in enumerate(maps):
+        for p in m.camera_path(seed + 40 + i, per_map):
+            w = int(round(math.exp(rng.uniform(math.log(320), math.log(2560)))))
+            h = max(1, int(round(w * 10 / 16)))
+            bits = 64 if rng.uniform() < 0.5 else 32
+            pool.append((i, fbld.build(m, p, width=w, height=h, color_bits=bits, use_directional_lightmaps=1,
+                                       num_dynamic_lights=4, clear_background=1, force_clear=1)))
+    return pool
+
+
 class TurbShifts:
     """make_turb_distortion's shift buffer for one effect instance (swh_turb_shifts), kept between
     calls like the reference's temp buffer."""
