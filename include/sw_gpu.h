@@ -355,6 +355,18 @@ int sw_tonemap_device(
 	sw_ctx* ctx, const void* hdr_device, uint32_t width, uint32_t height, float exposure,
 	const float color_modulate[3], void* dst32_device, size_t dst_pitch_bytes, float out_colors_sum[3]);
 
+/* No counterpart in the reference: for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). sw_tonemap_device with the same output bits,
+ * allocating nothing: the per-block partial colour sums go to a static device array of
+ * 3 * 2048 floats (24 KiB per GPU, held until the process ends) instead of a context buffer, and the
+ * three colour sums (may be NULL) to a device pointer, without a host synchronisation; the work is
+ * queued on the context's stream. Every call on one GPU shares that array: calls that overlap on one
+ * GPU (on several contexts or streams) race. Partial sums beyond the array's capacity are rejected
+ * (the launch never needs more than 2048 blocks' worth). */
+int sw_tonemap_device_noalloc(
+	sw_ctx* ctx, const void* hdr_device, uint32_t width, uint32_t height, float exposure,
+	const float color_modulate[3], void* dst32_device, size_t dst_pitch_bytes, float* out_colors_sum_device);
+
 /* The bloom settings of PostprocessorConfig (postprocessor_config.rs), as the reference clamps them
  * (postprocessor.rs:928-940): sigma in [0, 40] (0 = no bloom), scale_log2 in [1, 3], scale in
  * [1/16, 1/2]. */

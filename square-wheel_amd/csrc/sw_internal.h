@@ -243,8 +243,10 @@ void launch_finish(hipStream_t st, const DrawRec* drec, uint32_t nd, uint32_t* c
 // Bloom: downscale (+ the average colour when `average`), two blur passes, tonemap with the
 // upsampled bloom. Grids take a frame row per block row: height <= 65535.
 void launch_bloom(hipStream_t st, const BloomArgs& a, bool average);
-// Tonemap; `partials` holds 3 * 2048 floats (per-block colour sums); `totals` (3 floats, may be
-// host memory, null: not wanted) receives their sum.
+// Tonemap; `partials` holds 3 * TONEMAP_BLOCKS floats (per-block colour sums, every one the launch
+// uses written before it is read); `totals` (3 floats, may be host memory, null: not wanted)
+// receives their sum. The launch takes min(height, TONEMAP_BLOCKS) blocks.
+const uint32_t TONEMAP_BLOCKS = 2048u;
 void launch_tonemap(hipStream_t st, const uint64_t* hdr, uint32_t w, uint32_t h, float isr, float isg, float isb,
 	uint32_t* dst, uint32_t dst_pitch, float* partials, float* totals);
 // Triangle-model preparation: one block per instance. meshes / instances / sort_offsets are device

@@ -2320,7 +2320,6 @@ __global__ __launch_bounds__(256) void k_bloom_tonemap(BloomArgs a)
 // Tonemap blocks walk the frame's rows (block b: rows b, b + grid, ...; the threads over the row);
 // each block leaves the partial colour sums of its pixels (no atomics on three shared addresses),
 // k_tonemap_sums adds them up.
-#define TONEMAP_BLOCKS 2048u
 
 __global__ __launch_bounds__(256) void k_tonemap(const uint64_t* __restrict__ hdr, uint32_t width, uint32_t height,
 	float inv_scale_r, float inv_scale_g, float inv_scale_b, uint32_t* __restrict__ dst, uint32_t dst_pitch,

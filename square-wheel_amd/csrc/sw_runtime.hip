@@ -11,11 +11,19 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
 
 #include "sw_internal.h"
+
+// Scratch memory of sw_tonemap_device_noalloc: the tonemap's per-block partial colour sums (three
+// floats for each of at most TONEMAP_BLOCKS blocks, 24 KiB), a static device array that every GPU
+// the library runs on holds until the process ends. In host code `scratch` names a host-side stand-in,
+// not a device address: its address on each device comes from hipGetSymbolAddress (scratch_address),
+// and only that address is handed to kernels.
+static __device__ __align__(256) float scratch[3 * swk::TONEMAP_BLOCKS];
 
 namespace
 {
@@ -75,6 +83,7 @@ struct sw_ctx
 	uint32_t tile_rows = 8; // raster tile height (SW_TILE_ROWS_*), set by sw_set_tile_rows / the autotuner
 	bool scalar_spans = true; // raster span records through the scalar unit (chosen by the autotuner)
 	uint32_t* counters_host = nullptr; // pinned, device-mapped: result block (k_finish), then the error words
+	float* noalloc_scratch = nullptr; // this device's address of `scratch`
 	bool finish_launched = false; // the last launch sequence ended with k_finish (span allocation mode)
 	bool stats_pending = false; // the last raster batch's statistics are computed when asked for
 	uint32_t stats_draws = 0; // draws of that batch (its DrawRecs stay in `drec` until the next raster)
@@ -754,6 +763,34 @@ int launch_frame(sw_ctx* ctx, uint32_t num_draws, uint64_t num_vertex_ints, bool
 // =============================================================================================
 // C ABI
 // =============================================================================================
+namespace
+{
+const int MAX_SCRATCH_DEVICES = 64;
+std::mutex g_scratch_mutex;
+float* g_scratch_ptr[MAX_SCRATCH_DEVICES] = {};
+
+// The device address of `scratch` on `device`, which must be current: looked up once per device
+// (each GPU holds its own copy of the array, at its own address).
+int scratch_address(int device, float** out)
+{
+	std::lock_guard<std::mutex> lock(g_scratch_mutex);
+	if(device < 0 || device >= MAX_SCRATCH_DEVICES)
+		return SW_ERR_DEVICE;
+	if(!g_scratch_ptr[device])
+	{
+		void* p = nullptr;
+		if(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+		{
+			(void)hipGetLastError();
+			return SW_ERR_DEVICE;
+		}
+		g_scratch_ptr[device] = static_cast<float*>(p);
+	}
+	*out = g_scratch_ptr[device];
+	return SW_OK;
+}
+} // namespace
+
 extern "C" {
 
 int sw_ctx_create(int device, sw_ctx** out_ctx)
@@ -788,6 +825,11 @@ int sw_ctx_create(int device, sw_ctx** out_ctx)
 		return SW_ERR_DEVICE;
 	}
 	std::memset(ctx->counters_host, 0, sizeof(uint32_t) * HOST_WORDS_TOTAL);
+	if(scratch_address(device, &ctx->noalloc_scratch))
+	{
+		sw_ctx_destroy(ctx);
+		return SW_ERR_DEVICE;
+	}
 	*out_ctx = ctx;
 	return SW_OK;
 }
@@ -1082,7 +1124,7 @@ int sw_tonemap_device(sw_ctx* ctx, const void* hdr_device, uint32_t width, uint3
 		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "bad dst pitch");
 	(void)hipSetDevice(ctx->device);
 	int r;
-	if((r = ensure(ctx, ctx->tonemap_sums, sizeof(float) * 3 * 2048)))
+	if((r = ensure(ctx, ctx->tonemap_sums, sizeof(float) * 3 * swk::TONEMAP_BLOCKS)))
 		return r;
 	float* host_sums = reinterpret_cast<float*>(ctx->counters_host + HOST_POST_OFFSET);
 	float* host_sums_dev = reinterpret_cast<float*>(ctx->counters_host_dev + HOST_POST_OFFSET);
@@ -1107,6 +1149,30 @@ int sw_tonemap_device(sw_ctx* ctx, const void* hdr_device, uint32_t width, uint3
 			out_colors_sum[k] = static_cast<volatile const float*>(host_sums)[k];
 	}
 	return SW_OK;
+}
+
+int sw_tonemap_device_noalloc(sw_ctx* ctx, const void* hdr_device, uint32_t width, uint32_t height, float exposure,
+	const float color_modulate[3], void* dst32_device, size_t dst_pitch_bytes, float* out_colors_sum_device)
+{
+	if(!ctx || !hdr_device || !dst32_device || !color_modulate)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	if(dst_pitch_bytes < size_t(width) * 4 || dst_pitch_bytes % 4 || dst_pitch_bytes / 4 > 0xFFFFFFFFull)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "bad dst pitch");
+	// the launch's blocks each leave three partial sums in the static scratch
+	const uint64_t partial_floats = 3ull * std::max(1u, std::min(height, swk::TONEMAP_BLOCKS));
+	if(partial_floats > sizeof(scratch) / sizeof(float))
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "tonemap partial sums exceed the static scratch (3 * 2048 floats)");
+	if(!ctx->noalloc_scratch)
+		return set_error(ctx, SW_ERR_STATE, "no scratch address for this device");
+	(void)hipSetDevice(ctx->device);
+	// TonemappingFunction::new (postprocessor.rs:1005-1017)
+	const float isr = 1.0f / (exposure * color_modulate[0]);
+	const float isg = 1.0f / (exposure * color_modulate[1]);
+	const float isb = 1.0f / (exposure * color_modulate[2]);
+	// every partial the launch uses is written before k_tonemap_sums reads it: nothing to clear
+	swk::launch_tonemap(ctx->stream, static_cast<const uint64_t*>(hdr_device), width, height, isr, isg, isb,
+		static_cast<uint32_t*>(dst32_device), uint32_t(dst_pitch_bytes / 4), ctx->noalloc_scratch, out_colors_sum_device);
+	return hip_check(ctx, hipGetLastError(), "tonemap launch");
 }
 
 // compute_gaussian_kernel (postprocessor.rs:1026-1045): four samples per tap, f32, libm exp

@@ -300,6 +300,8 @@ GPU_SYMBOLS = {
     "sw_render_batch_graph": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Batch), C.c_void_p]),
     "sw_tonemap_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float,
                                     C.POINTER(C.c_float), C.c_void_p, C.c_size_t, C.POINTER(C.c_float)]),
+    "sw_tonemap_device_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float,
+                                            C.POINTER(C.c_float), C.c_void_p, C.c_size_t, C.c_void_p]),
     "sw_postprocess_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float,
                                         C.POINTER(C.c_float), C.c_void_p, C.c_void_p, C.c_size_t,
                                         C.POINTER(C.c_float)]),
