@@ -100,6 +100,86 @@ def cpu_baseline(m, batches, seconds, threads):
     return len(lat) / el, len(lat), el, np.array(lat)
 
 
+def dropin_leg(ctx, m, mh, frames, args):
+    """The reference's own call shape, Renderer::draw_frame writing into the caller's &mut [ColorT]
+    (renderer.rs:106-124): per frame sw_frame_begin, sw_build_surfaces and sw_raster_draws with the
+    command buffers in host memory (copied to the device by the calls), then sw_frame_end into a
+    pinned host framebuffer. Serial: one context, each frame finished before the next starts.
+    Overlapped: two contexts (two HIP streams) take alternate frames, so frame N's copy back to the
+    host overlaps frame N+1's uploads and kernels; each frame is still complete in host memory when
+    its sw_frame_end returns."""
+    import ctypes as C
+    W, H = args.width, args.height
+    lib = ctx.lib
+    ctx2 = sw_abi.Context(0)
+    mh2 = ctx2.upload_map(m.data)
+    ctxs, mhs = [ctx, ctx2], [mh, mh2]
+    fb_bytes = W * H * 4
+    pinned = []
+    for c in ctxs:
+        p = C.c_void_p()
+        c.check(lib.sw_host_alloc(c.h, fb_bytes, C.byref(p)), "sw_host_alloc")
+        pinned.append(p)
+    prep = []
+    for f in frames:
+        v = f["view"]
+        arrs = [np.ascontiguousarray(f[k]) for k in ("surfaces", "lights", "light_indices", "draws", "vertex_data")]
+        cam = (C.c_float * 3)(*[float(x) for x in np.asarray(v["cam_pos"]).reshape(-1)[:3]])
+        prep.append((cam, int(np.asarray(v["clear_background"]).item()), int(np.asarray(v["background_color"]).item()), arrs))
+
+    def draw(c, i):
+        cam, clear, bgc, (s, li, lx, d, vd) = prep[i % len(prep)]
+        h = ctxs[c].h
+        ctxs[c].check(lib.sw_frame_begin(h, mhs[c], 32, W, H, cam, clear, bgc), "sw_frame_begin")
+        ctxs[c].check(lib.sw_build_surfaces(h, C.c_void_p(s.ctypes.data), len(s), C.c_void_p(li.ctypes.data), len(li),
+                                            C.c_void_p(lx.ctypes.data), len(lx)), "sw_build_surfaces")
+        ctxs[c].check(lib.sw_raster_draws(h, C.c_void_p(d.ctypes.data), len(d), C.c_void_p(vd.ctypes.data), len(vd)),
+                      "sw_raster_draws")
+
+    def end(c):
+        ctxs[c].check(lib.sw_frame_end(ctxs[c].h, pinned[c], W * 4, 0), "sw_frame_end")
+
+    n = max(args.steps, len(frames))
+    out = {}
+    for i in range(len(frames)):
+        draw(0, i)
+        end(0)
+    lat = np.zeros(n)
+    t0 = time.perf_counter()
+    for i in range(n):
+        ts = time.perf_counter()
+        draw(0, i)
+        end(0)
+        lat[i] = time.perf_counter() - ts
+    wall = time.perf_counter() - t0
+    out["serial"] = {"frames_s": round(n / wall, 2), "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 4),
+                     "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 4)}
+    # overlapped: frame i is drawn on context i % 2 while frame i - 1 finishes on the other one
+    lat = np.zeros(n)
+    starts = np.zeros(n)
+    t0 = time.perf_counter()
+    for i in range(n):
+        starts[i] = time.perf_counter()
+        draw(i % 2, i)
+        if i > 0:
+            end((i - 1) % 2)
+            lat[i - 1] = time.perf_counter() - starts[i - 1]
+    end((n - 1) % 2)
+    lat[n - 1] = time.perf_counter() - starts[n - 1]
+    wall = time.perf_counter() - t0
+    out["overlapped"] = {"frames_s": round(n / wall, 2), "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 4),
+                         "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 4), "contexts": 2}
+    out["workload"] = (f"{W}x{H} Color32 frames of the plain path through sw_frame_begin / sw_build_surfaces / "
+                       "sw_raster_draws (host-resident command buffers) / sw_frame_end into pinned host memory")
+    out["frame_bytes_to_host"] = fb_bytes
+    for c, p in zip(ctxs, pinned):
+        lib.sw_host_free(c.h, p)
+    ctx2.release_map(mh2)
+    ctx2.close()
+    log(f"drop-in: {out}")
+    return out
+
+
 def hdr_leg(ctx, args):
     """The HDR + models frame: 2560x1440 Color64 with 16 dynamic lights, 64 animated triangle
     models, 256 decals and 10% alpha-blended materials, rendered from HBM-resident command buffers
@@ -526,6 +606,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-dropin", action="store_true")
     ap.add_argument("--no-hdr", action="store_true")
     ap.add_argument("--no-models", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -669,6 +750,9 @@ def main():
         "roofline": roofline,
     }
 
+    if not args.no_dropin:
+        result["dropin"] = dropin_leg(ctx, m, mh, frames, args)
+
     if not args.no_batch:
         bframes = sw_host.batch_view_frames(m, args.seed, args.batch_views, color_bits=color_bits)
         total_px = sum(int(np.asarray(f["view"]["width"]).item()) * int(np.asarray(f["view"]["height"]).item()) for f in bframes)
@@ -740,6 +824,9 @@ def main():
                                             f"(surface build + raster, reference decomposition over {threads} threads, "
                                             "x86 rsqrtss/rcpss numerics, per-thread work counters)"}
         result["speedup_vs_cpu"] = round(fps / cfps, 2)
+        if "dropin" in result:
+            for k in ("serial", "overlapped"):
+                result["dropin"][k]["speedup_vs_cpu"] = round(result["dropin"][k]["frames_s"] / cfps, 2)
         log(f"cpu baseline: {cfps:.2f} frames/s on {threads} threads")
 
     for d in dev:
