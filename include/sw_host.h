@@ -49,6 +49,7 @@ typedef struct swh_map_config
 	uint32_t directional_lightmaps; /* 1: build DirectionalLightmapElement data too */
 	uint32_t num_decal_textures; /* lite 32x32 decal textures */
 	uint32_t num_model_textures; /* lite 64x64 model skins */
+	uint32_t num_submodels; /* inline brush models (doors and platforms) moved by the frame time */
 } swh_map_config;
 
 typedef struct swh_map_info

@@ -386,6 +386,25 @@ M4 cube_side_rotation(uint32_t side)
 namespace
 {
 
+// PolygonBasisVecs::form_plane_and_tex_coord_equation (surfaces.rs:18-47)
+swh_frame::Basis form_basis(const Plane& plane, const Plane tc[2])
+{
+	M4 b;
+	b.c[0] = extend(tc[0].vec, tc[0].dist);
+	b.c[1] = extend(tc[1].vec, tc[1].dist);
+	b.c[2] = extend(plane.vec, -plane.dist);
+	b.c[3] = v4(0, 0, 0, 1);
+	M4 inv;
+	if(!invert(transpose(b), inv))
+		return swh_frame::Basis{v3(1, 0, 0), v3(0, 1, 0), v3(0, 0, 0), v3(0, 0, 1)};
+	swh_frame::Basis out;
+	out.u = truncate(inv.c[0]);
+	out.v = truncate(inv.c[1]);
+	out.start = truncate(inv.c[3]);
+	out.normal = plane.vec / std::fmax(length(plane.vec), 1.0f / float(1 << 30));
+	return out;
+}
+
 void ensure_basis(swh_frame& f, const swh_map& m)
 {
 	if(f.basis_map == &m && f.basis.size() == m.polygons.size())
@@ -402,24 +421,8 @@ void ensure_basis(swh_frame& f, const swh_map& m)
 	f.frame_number = 0;
 	for(size_t i = 0; i < m.polygons.size(); ++i)
 	{
-		// PolygonBasisVecs::form_plane_and_tex_coord_equation (surfaces.rs:18-47)
 		const Polygon& p = m.polygons[i];
-		M4 b;
-		b.c[0] = extend(p.tex_coord_equation[0].vec, p.tex_coord_equation[0].dist);
-		b.c[1] = extend(p.tex_coord_equation[1].vec, p.tex_coord_equation[1].dist);
-		b.c[2] = extend(p.plane.vec, -p.plane.dist);
-		b.c[3] = v4(0, 0, 0, 1);
-		M4 inv;
-		swh_frame::Basis& out = f.basis[i];
-		if(!invert(transpose(b), inv))
-		{
-			out = swh_frame::Basis{v3(1, 0, 0), v3(0, 1, 0), v3(0, 0, 0), v3(0, 0, 1)};
-			continue;
-		}
-		out.u = truncate(inv.c[0]);
-		out.v = truncate(inv.c[1]);
-		out.start = truncate(inv.c[3]);
-		out.normal = p.plane.vec / std::fmax(length(p.plane.vec), 1.0f / float(1 << 30));
+		f.basis[i] = form_basis(p.plane, p.tex_coord_equation);
 	}
 }
 
@@ -656,6 +659,160 @@ bool polygon_is_affected_by_light(const Polygon& polygon, const swh_frame::Basis
 		tc0 - u_r > float(polygon.tex_coord_max[0]) || tc1 - v_r > float(polygon.tex_coord_max[1]));
 }
 
+// The frame's inline-model entities (frame_info.rs SubmodelEntity, positioned by the game in the
+// reference): doors swing about the vertical axis through their centre, platforms rise and sink,
+// both as functions of the frame time.
+void submodel_entity(uint32_t k, float time, V3 centre, V3& position, Quat& rotation)
+{
+	if(k % 2 == 0)
+	{
+		const float a = 0.7f * std::sin(0.9f * time + 1.3f * float(k));
+		rotation = Quat{std::cos(0.5f * a), v3(0.0f, 0.0f, std::sin(0.5f * a))};
+		position = centre;
+	}
+	else
+	{
+		rotation = Quat{1.0f, v3(0.0f, 0.0f, 0.0f)};
+		position = centre + v3(0.0f, 0.0f, 24.0f * std::sin(1.1f * time + 0.7f * float(k)));
+	}
+}
+
+// InlineModelsIndex::position_model_r (inline_models_index.rs:126-157): the leaves the transformed
+// bbox's corners reach.
+void position_model_r(const swh_map& m, const V3 corners[8], uint32_t node_index, std::vector<uint32_t>& leafs)
+{
+	if(node_index >= FIRST_LEAF_INDEX)
+	{
+		leafs.push_back(node_index - FIRST_LEAF_INDEX);
+		return;
+	}
+	const Node& node = m.nodes[node_index];
+	uint32_t front = 0;
+	for(int i = 0; i < 8; ++i)
+		if(dot(node.plane.vec, corners[i]) > node.plane.dist)
+			++front;
+	if(front > 0)
+		position_model_r(m, corners, node.children[0], leafs);
+	if(front < 8)
+		position_model_r(m, corners, node.children[1], leafs);
+}
+
+void bbox_corners(const BBox& b, V3 out[8])
+{
+	// BBox::get_corners_vertices (bbox.rs:83-95)
+	for(int i = 0; i < 8; ++i)
+		out[i] = v3((i & 4) ? b.max.x : b.min.x, (i & 2) ? b.max.y : b.min.y, (i & 1) ? b.max.z : b.min.z);
+}
+
+// InlineModelsIndex::position_models + get_model_matrix (inline_models_index.rs:31-124, 181-196) and
+// prepare_submodels (partial_renderer.rs:593-664): model matrices, the camera's matrices of each
+// model, its leaves and the dynamic lights whose sphere's box touches its transformed bbox.
+void prepare_submodels(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, const Camera& cam)
+{
+	f.submodels.assign(m.submodels.size(), swh_frame::SubmodelFrame{});
+	f.leaf_submodels.clear();
+	if(m.submodels.empty())
+		return;
+	f.leaf_submodels.assign(m.leafs.size(), std::vector<uint32_t>());
+	for(uint32_t k = 0; k < m.submodels.size(); ++k)
+	{
+		swh_frame::SubmodelFrame& s = f.submodels[k];
+		const Submodel& sm = m.submodels[k];
+		// get_submodel_bbox (bsp_map_compact.rs:236-255) extended by 1/4 (inline_models_index.rs:171-176)
+		BBox b{v3(1e24f, 1e24f, 1e24f), v3(-1e24f, -1e24f, -1e24f)};
+		for(uint32_t pi = sm.first_polygon; pi < sm.first_polygon + sm.num_polygons; ++pi)
+		{
+			const Polygon& p = m.polygons[pi];
+			for(uint32_t v = 0; v < p.num_vertices; ++v)
+			{
+				const V3 q = m.vertices[p.first_vertex + v];
+				b.min = v3(std::min(b.min.x, q.x), std::min(b.min.y, q.y), std::min(b.min.z, q.z));
+				b.max = v3(std::max(b.max.x, q.x), std::max(b.max.y, q.y), std::max(b.max.z, q.z));
+			}
+		}
+		const V3 eps = v3(0.25f, 0.25f, 0.25f);
+		b.min = b.min - eps;
+		b.max = b.max + eps;
+		s.bbox = b;
+		const V3 centre = (b.min + b.max) * 0.5f;
+		V3 position;
+		Quat rotation;
+		submodel_entity(k, cfg.time, centre, position, rotation);
+		s.model = mul(mul(from_translation(position), quat_to_m4(rotation)), from_translation(centre * -1.0f));
+		if(!invert(transpose(s.model), s.world_planes))
+			continue;
+		s.view = mul(cam.view_matrix, s.model);
+		s.planes = mul(cam.planes_matrix, s.world_planes);
+		s.active = true;
+		V3 corners[8];
+		bbox_corners(b, corners);
+		BBox world{v3(1e30f, 1e30f, 1e30f), v3(-1e30f, -1e30f, -1e30f)};
+		for(V3& c : corners)
+		{
+			c = truncate(mul(s.model, extend(c, 1.0f)));
+			world.min = v3(std::min(world.min.x, c.x), std::min(world.min.y, c.y), std::min(world.min.z, c.z));
+			world.max = v3(std::max(world.max.x, c.x), std::max(world.max.y, c.y), std::max(world.max.z, c.z));
+		}
+		position_model_r(m, corners, uint32_t(m.nodes.size() - 1), s.leafs);
+		for(uint32_t l : s.leafs)
+			f.leaf_submodels[l].push_back(k);
+		for(uint32_t li = 0; li < f.lights.size(); ++li)
+		{
+			const sw_light& l = f.lights[li];
+			if(l.position[0] - l.radius > world.max.x || l.position[0] + l.radius < world.min.x ||
+				l.position[1] - l.radius > world.max.y || l.position[1] + l.radius < world.min.y ||
+				l.position[2] - l.radius > world.max.z || l.position[2] + l.radius < world.min.z)
+				continue;
+			s.lights.push_back(li);
+		}
+	}
+}
+
+// The submodel part of prepare_polygons_surfaces (partial_renderer.rs:1362-1447): models in a visible
+// leaf get their polygons' surfaces with the model's camera matrices, clipped by the union of the
+// bounds of their visible leaves, and each visible polygon's basis from its transformed planes.
+void prepare_submodel_surfaces(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, const Camera& cam, uint64_t& offset)
+{
+	for(uint32_t k = 0; k < m.submodels.size(); ++k)
+	{
+		const swh_frame::SubmodelFrame& s = f.submodels[k];
+		if(!s.active)
+			continue;
+		bool have = false;
+		ClippingPolygon bounds{};
+		for(uint32_t l : s.leafs)
+		{
+			if(f.leaf_visible_frame[l] != f.frame_number)
+				continue;
+			if(have)
+				bounds.extend(f.leaf_bounds[l]);
+			else
+				bounds = f.leaf_bounds[l];
+			have = true;
+		}
+		if(!have)
+			continue;
+		V3 cp[8];
+		bounds.clip_planes(cp);
+		Camera sc = cam;
+		sc.view_matrix = s.view;
+		sc.planes_matrix = s.planes;
+		const Submodel& sm = m.submodels[k];
+		for(uint32_t pi = sm.first_polygon; pi < sm.first_polygon + sm.num_polygons; ++pi)
+		{
+			prepare_polygon_surface(f, m, sc, cp, offset, pi, cfg.textures_mip_bias);
+			if(f.poly[pi].visible_frame != f.frame_number)
+				continue;
+			const Polygon& p = m.polygons[pi];
+			const V4 pt = mul(s.world_planes, extend(p.plane.vec, -p.plane.dist));
+			const V4 t0 = mul(s.world_planes, extend(p.tex_coord_equation[0].vec, p.tex_coord_equation[0].dist));
+			const V4 t1 = mul(s.world_planes, extend(p.tex_coord_equation[1].vec, p.tex_coord_equation[1].dist));
+			const Plane tcs[2] = {Plane{truncate(t0), t0.w}, Plane{truncate(t1), t1.w}};
+			f.basis[pi] = form_basis(Plane{truncate(pt), -pt.w}, tcs);
+		}
+	}
+}
+
 // partial_renderer.rs:1682-1799 (per-surface parameters)
 void emit_surface(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, uint32_t pi)
 {
@@ -692,11 +849,19 @@ void emit_surface(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, u
 		s.flags |= SW_SURF_DIRECTIONAL;
 	s.emissive_level = SW_NO_TEXTURE;
 	s.pixels_offset = pd.surface_offset;
-	// dynamic lights of the polygon's leaf, at most 6
+	// dynamic lights of the polygon's leaf, or of its submodel (:1706-1719), at most 6
 	s.first_light = uint32_t(f.light_indices.size());
 	if(!f.leaf_lights.empty())
 	{
-		for(uint32_t li : f.leaf_lights[m.polygon_leaf[pi]])
+		const std::vector<uint32_t>* list = nullptr;
+		if(m.polygon_leaf[pi] != NO_POLYGON_LEAF)
+			list = &f.leaf_lights[m.polygon_leaf[pi]];
+		else
+			for(size_t k = 0; k < m.submodels.size() && k < f.submodels.size(); ++k)
+				if(pi >= m.submodels[k].first_polygon && pi - m.submodels[k].first_polygon < m.submodels[k].num_polygons)
+					list = &f.submodels[k].lights;
+		static const std::vector<uint32_t> none;
+		for(uint32_t li : list ? *list : none)
 		{
 			if(polygon_is_affected_by_light(polygon, basis, f.lights[li]))
 			{
@@ -857,6 +1022,94 @@ void draw_skybox(swh_frame& f, const swh_map& m, const Camera& cam, const Clippi
 // =============================================================================================
 // C API
 // =============================================================================================
+namespace
+{
+
+struct ScreenRect
+{
+	float min_x, min_y, max_x, max_y;
+};
+
+// split_rect (rect_splitting.rs:23-78): halves along the longer side, the split at
+// max * ratio0 + min * ratio1 (f32), the first half getting num_parts / 2 of the parts.
+void split_rect(const ScreenRect& r, uint32_t num_parts, std::vector<ScreenRect>& out)
+{
+	if(num_parts <= 1)
+	{
+		if(out.size() < 64)
+			out.push_back(r);
+		return;
+	}
+	const uint32_t n0 = num_parts / 2, n1 = num_parts - n0;
+	const float ratio0 = float(n0) / float(num_parts), ratio1 = float(n1) / float(num_parts);
+	ScreenRect r0 = r, r1 = r;
+	if(r.max_x - r.min_x >= r.max_y - r.min_y)
+	{
+		const float middle = r.max_x * ratio0 + r.min_x * ratio1;
+		r0.max_x = middle;
+		r1.min_x = middle;
+	}
+	else
+	{
+		const float middle = r.max_y * ratio0 + r.min_y * ratio1;
+		r0.max_y = middle;
+		r1.min_y = middle;
+	}
+	split_rect(r0, n0, out);
+	split_rect(r1, n1, out);
+}
+
+// perform_rasterization_for_viewport_part (partial_renderer.rs:1285-1325) for one viewport part:
+// the skybox, then the BSP tree back to front (draw_tree, :2450-2517) with each leaf's bounds
+// intersected with the part's viewport polygon, its polygons (draw_leaf, :2519-2792) and objects.
+void emit_draw_list(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, const Camera& cam, const ClippingPolygon& viewport,
+	const int32_t clip_rect[4])
+{
+	draw_skybox(f, m, cam, viewport, cfg.textures_mip_bias, clip_rect);
+	std::vector<uint32_t> stack;
+	stack.push_back(uint32_t(m.nodes.size() - 1));
+	const V4 w_row = cam.planes_matrix.row(3);
+	while(!stack.empty())
+	{
+		const uint32_t node_index = stack.back();
+		if(node_index >= FIRST_LEAF_INDEX)
+		{
+			stack.pop_back();
+			const uint32_t leaf_index = node_index - FIRST_LEAF_INDEX;
+			if(f.leaf_visible_frame[leaf_index] != f.frame_number)
+				continue;
+			ClippingPolygon lb = f.leaf_bounds[leaf_index];
+			lb.intersect(viewport);
+			if(!lb.valid())
+				continue;
+			V3 cp[8];
+			lb.clip_planes(cp);
+			const Leaf& leaf = m.leafs[leaf_index];
+			for(uint32_t k = 0; k < leaf.num_polygons; ++k)
+			{
+				const uint32_t pi = leaf.first_polygon + k;
+				const swh_frame::PolyData& pd = f.poly[pi];
+				if(pd.visible_frame != f.frame_number)
+					continue;
+				const Polygon& p = m.polygons[pi];
+				draw_polygon(f, cp, &f.vertices_transformed[p.first_vertex], p.num_vertices, pd.de, pd.te, pd.size, pd.surface_offset,
+					SW_TEXSRC_SURFACE, m.materials[p.texture].blending_mode, clip_rect);
+			}
+			draw_leaf_objects(f, m, cfg, leaf_index, lb, clip_rect, cam.planes_matrix);
+		}
+		else
+		{
+			const Node& node = m.nodes[node_index];
+			const float w = dot(w_row, extend(node.plane.vec, -node.plane.dist));
+			const uint32_t mask = w >= 0.0f ? 1u : 0u;
+			stack.back() = node.children[1 ^ mask];
+			stack.push_back(node.children[mask]);
+		}
+	}
+}
+
+} // namespace
+
 extern "C" {
 
 int swh_frame_create(swh_frame** out)
@@ -905,6 +1158,8 @@ int swh_frame_build(swh_frame* f, const swh_map* m, const swh_frame_config* cfg,
 		for(uint32_t k = 0; k < leaf.num_polygons; ++k)
 			prepare_polygon_surface(*f, *m, cam, cp, offset, leaf.first_polygon + k, cfg->textures_mip_bias);
 	}
+	prepare_submodels(*f, *m, *cfg, cam);
+	prepare_submodel_surfaces(*f, *m, *cfg, cam, offset);
 	f->arena_texels = offset;
 	for(uint32_t pi : f->visible_polygons)
 		emit_surface(*f, *m, *cfg, pi);
@@ -913,48 +1168,27 @@ int swh_frame_build(swh_frame* f, const swh_map* m, const swh_frame_config* cfg,
 	prepare_models(*f, *m, *cfg, cam.view_matrix, cam.planes_matrix);
 	prepare_decals(*f, *m, *cfg, cam.planes_matrix);
 
-	// draw list (partial_renderer.rs:1285-1325 with one full-screen rect)
-	const int32_t clip_rect[4] = {0, 0, int32_t(cfg->width), int32_t(cfg->height)};
-	draw_skybox(*f, *m, cam, frame_bounds, cfg->textures_mip_bias, clip_rect);
-	// draw_tree (:2450-2517): back to front
-	std::vector<uint32_t> stack;
-	stack.push_back(uint32_t(m->nodes.size() - 1));
-	const V4 w_row = cam.planes_matrix.row(3);
-	while(!stack.empty())
+	// draw list: perform_rasterization (partial_renderer.rs:1192-1283). One thread: the whole screen
+	// as clip rect and the screen box as viewport polygon. N threads: split_rect's N rects, each with
+	// its integer clip rect and a viewport polygon widened by half a pixel; every rect gets the whole
+	// traversal, so the draw list holds N runs of draws whose clip rects partition the screen.
+	const uint32_t nthreads = cfg->virtual_threads <= 1 ? 1u : std::min<uint32_t>(cfg->virtual_threads, 64u);
+	if(nthreads == 1)
 	{
-		const uint32_t node_index = stack.back();
-		if(node_index >= FIRST_LEAF_INDEX)
+		const int32_t clip_rect[4] = {0, 0, int32_t(cfg->width), int32_t(cfg->height)};
+		emit_draw_list(*f, *m, *cfg, cam, frame_bounds, clip_rect);
+	}
+	else
+	{
+		std::vector<ScreenRect> rects;
+		split_rect(ScreenRect{0.0f, 0.0f, W, H}, nthreads, rects);
+		for(const ScreenRect& r : rects)
 		{
-			stack.pop_back();
-			const uint32_t leaf_index = node_index - FIRST_LEAF_INDEX;
-			if(f->leaf_visible_frame[leaf_index] != f->frame_number)
-				continue;
-			ClippingPolygon lb = f->leaf_bounds[leaf_index];
-			lb.intersect(frame_bounds);
-			if(!lb.valid())
-				continue;
-			V3 cp[8];
-			lb.clip_planes(cp);
-			const Leaf& leaf = m->leafs[leaf_index];
-			for(uint32_t k = 0; k < leaf.num_polygons; ++k)
-			{
-				const uint32_t pi = leaf.first_polygon + k;
-				const swh_frame::PolyData& pd = f->poly[pi];
-				if(pd.visible_frame != f->frame_number)
-					continue;
-				const Polygon& p = m->polygons[pi];
-				draw_polygon(*f, cp, &f->vertices_transformed[p.first_vertex], p.num_vertices, pd.de, pd.te, pd.size, pd.surface_offset,
-					SW_TEXSRC_SURFACE, m->materials[p.texture].blending_mode, clip_rect);
-			}
-			draw_leaf_objects(*f, *m, *cfg, leaf_index, lb, clip_rect, cam.planes_matrix);
-		}
-		else
-		{
-			const Node& node = m->nodes[node_index];
-			const float w = dot(w_row, extend(node.plane.vec, -node.plane.dist));
-			const uint32_t mask = w >= 0.0f ? 1u : 0u;
-			stack.back() = node.children[1 ^ mask];
-			stack.push_back(node.children[mask]);
+			// `as i32` of the float bounds (:1250-1259), the polygon extended by 0.5 (:1261-1272)
+			const int32_t clip_rect[4] = {int32_t(r.min_x), int32_t(r.min_y), int32_t(r.max_x), int32_t(r.max_y)};
+			const ClippingPolygon viewport =
+				ClippingPolygon::from_box(r.min_x - 0.5f, r.min_y - 0.5f, r.max_x + 0.5f, r.max_y + 0.5f);
+			emit_draw_list(*f, *m, *cfg, cam, viewport, clip_rect);
 		}
 	}
 

@@ -189,6 +189,20 @@ struct swh_frame
 	// dynamic objects
 	std::vector<std::vector<uint32_t>> leaf_lights;
 	ModelFrameData models;
+
+	// inline models of the frame (InlineModelsIndex + SubmodelInfo, partial_renderer.rs:593-664)
+	struct SubmodelFrame
+	{
+		bool active = false; // placed by the frame's entity
+		swh::BBox bbox; // initial bounding box of the polygons, extended by 1/4 (model space)
+		swh::M4 model; // model matrix: translation(position) * rotation * translation(-centre)
+		swh::M4 world_planes; // inverse of the model matrix's transpose: transforms planes
+		swh::M4 view, planes; // the camera's matrices combined with the model's
+		std::vector<uint32_t> leafs; // leaves the transformed bbox touches
+		std::vector<uint32_t> lights; // visible dynamic lights whose sphere's box touches the bbox
+	};
+	std::vector<SubmodelFrame> submodels;
+	std::vector<std::vector<uint32_t>> leaf_submodels; // per leaf: indices into `submodels`
 };
 
 // Shadow-map draw lists under construction: the visibility state of the map (leaf bounds) reused

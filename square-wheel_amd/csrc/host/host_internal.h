@@ -92,6 +92,23 @@ struct Leaf
 	uint32_t first_polygon, num_polygons, first_leaf_portal, num_leaf_portals;
 };
 
+// Inline (brush) models (bsp_map_compact.rs Submodel / SubmodelBSPNode): a submodel's polygons are
+// a range of the polygon array after the leafs' polygons; its own BSP orders them. A child index at
+// or past the node count means "no child".
+struct Submodel
+{
+	uint32_t root_node, first_polygon, num_polygons;
+};
+
+struct SubmodelNode
+{
+	Plane plane;
+	uint32_t first_polygon, num_polygons;
+	uint32_t children[2]; // [0]: front of the plane, [1]: back
+};
+
+const uint32_t NO_POLYGON_LEAF = 0xFFFFFFFFu;
+
 struct Portal
 {
 	uint32_t leafs[2];
@@ -227,7 +244,10 @@ struct swh_map
 	std::vector<uint32_t> leafs_portals;
 	std::vector<swh::V3> vertices;
 	std::vector<swh::Material> materials;
-	std::vector<uint32_t> polygon_leaf;
+	std::vector<uint32_t> polygon_leaf; // NO_POLYGON_LEAF for submodel polygons
+	std::vector<swh::Submodel> submodels;
+	std::vector<swh::SubmodelNode> submodel_nodes;
+	std::vector<uint32_t> submodel_cell; // the empty cell each generated submodel stands in
 
 	std::vector<float> lm_simple; // 3 floats per element
 	std::vector<sw_directional_lightmap_element> lm_dir;

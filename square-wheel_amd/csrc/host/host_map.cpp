@@ -724,6 +724,112 @@ void build_model_mesh(swh_map& m, uint32_t seed, uint32_t freq, uint32_t frames,
 	}
 }
 
+// Inline models: seeded doors (thin upright slabs) and platforms (flat slabs) standing in empty
+// cells on a solid floor, each a box of six outward-facing quads after the leafs' polygons, ordered
+// by a chain of submodel BSP nodes (one face per node; the faces of a convex box never hide one
+// another, so any back-to-front walk of the chain is exact). The lightmapper lights them in their
+// initial position, as the reference's lightmapper does.
+void generate_submodels(swh_map& m, uint32_t count, uint32_t seed, uint32_t nmat, uint32_t texture_size)
+{
+	if(count == 0)
+		return;
+	Rng rng(uint64_t(seed) * 0xD1B54A32D192ED03ull + 77);
+	std::vector<uint32_t> candidates;
+	for(uint32_t c : m.empty_cells)
+	{
+		const uint32_t z = c / (m.gx * m.gy);
+		if(z > 0 && m.cell[c - m.gx * m.gy] == 1)
+			candidates.push_back(c);
+	}
+	for(uint32_t k = 0; k < count && !candidates.empty(); ++k)
+	{
+		const uint32_t pick = rng.below(uint32_t(candidates.size()));
+		const uint32_t c = candidates[pick];
+		candidates.erase(candidates.begin() + pick);
+		const uint32_t x = c % m.gx, y = (c / m.gx) % m.gy, z = c / (m.gx * m.gy);
+		const bool door = k % 2 == 0;
+		const bool along_x = rng.below(2) == 0;
+		const V3 half = door ? (along_x ? v3(40.0f, 8.0f, 44.0f) : v3(8.0f, 40.0f, 44.0f)) : v3(32.0f, 32.0f, 8.0f);
+		const V3 centre = v3(CELL * (float(x) + 0.5f), CELL * (float(y) + 0.5f), CELL * float(z) + (door ? 46.0f : 40.0f));
+		const BBox b{centre - half, centre + half};
+		Submodel sm;
+		sm.first_polygon = uint32_t(m.polygons.size());
+		sm.num_polygons = 6;
+		sm.root_node = uint32_t(m.submodel_nodes.size());
+		const uint32_t texture = rng.below(nmat);
+		for(int dir = 0; dir < 6; ++dir)
+		{
+			const int axis = dir / 2;
+			const bool positive = (dir & 1) != 0;
+			const V3 normal = v3(axis == 0 ? (positive ? 1.0f : -1.0f) : 0.0f, axis == 1 ? (positive ? 1.0f : -1.0f) : 0.0f,
+				axis == 2 ? (positive ? 1.0f : -1.0f) : 0.0f);
+			// the face's four corners: the other two axes at their min / max
+			const int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
+			V3 q[4];
+			for(int k4 = 0; k4 < 4; ++k4)
+			{
+				float p[3];
+				const float mn[3] = {b.min.x, b.min.y, b.min.z}, mx[3] = {b.max.x, b.max.y, b.max.z};
+				p[axis] = positive ? mx[axis] : mn[axis];
+				p[a1] = (k4 == 1 || k4 == 2) ? mx[a1] : mn[a1];
+				p[a2] = (k4 >= 2) ? mx[a2] : mn[a2];
+				q[k4] = v3(p[0], p[1], p[2]);
+			}
+			// clockwise seen from the front: cross(e1, e2) points away from the normal
+			if(dot(cross(q[1] - q[0], q[2] - q[1]), normal) > 0.0f)
+				std::swap(q[1], q[3]);
+			Polygon p;
+			p.first_vertex = uint32_t(m.vertices.size());
+			p.num_vertices = 4;
+			for(int k4 = 0; k4 < 4; ++k4)
+				m.vertices.push_back(q[k4]);
+			p.plane = Plane{normal, dot(normal, q[0])};
+			const float tshift0 = float(rng.below(texture_size)), tshift1 = float(rng.below(texture_size));
+			if(axis == 0)
+			{
+				p.tex_coord_equation[0] = Plane{v3(0, 1, 0), tshift0};
+				p.tex_coord_equation[1] = Plane{v3(0, 0, -1), tshift1};
+			}
+			else if(axis == 1)
+			{
+				p.tex_coord_equation[0] = Plane{v3(1, 0, 0), tshift0};
+				p.tex_coord_equation[1] = Plane{v3(0, 0, -1), tshift1};
+			}
+			else
+			{
+				p.tex_coord_equation[0] = Plane{v3(1, 0, 0), tshift0};
+				p.tex_coord_equation[1] = Plane{v3(0, -1, 0), tshift1};
+			}
+			for(int t = 0; t < 2; ++t)
+			{
+				float tmin = 1e30f, tmax = -1e30f;
+				for(int k4 = 0; k4 < 4; ++k4)
+				{
+					const float tc = dot(p.tex_coord_equation[t].vec, q[k4]) + p.tex_coord_equation[t].dist;
+					tmin = std::min(tmin, tc);
+					tmax = std::max(tmax, tc);
+				}
+				p.tex_coord_min[t] = int32_t(std::floor(tmin));
+				p.tex_coord_max[t] = int32_t(std::ceil(tmax));
+			}
+			p.lightmap_data_offset = 0;
+			p.texture = texture;
+			const uint32_t pi = uint32_t(m.polygons.size());
+			m.polygons.push_back(p);
+			m.polygon_leaf.push_back(NO_POLYGON_LEAF);
+			SubmodelNode node;
+			node.plane = p.plane;
+			node.first_polygon = pi;
+			node.num_polygons = 1;
+			node.children[0] = dir + 1 < 6 ? uint32_t(m.submodel_nodes.size()) + 1u : 0xFFFFFFFFu;
+			node.children[1] = 0xFFFFFFFFu;
+			m.submodel_nodes.push_back(node);
+		}
+		m.submodels.push_back(sm);
+		m.submodel_cell.push_back(c);
+	}
+}
+
 void light_polygons(swh_map& m)
 {
 	const bool directional = m.config.directional_lightmaps != 0;
@@ -1086,6 +1192,7 @@ int swh_map_create(const swh_map_config* cfg, swh_map** out)
 		m->sun_dir = v3(0.35f, 0.25f, -1.0f) / length(v3(0.35f, 0.25f, -1.0f));
 		m->sun_intensity = 0.8f;
 	}
+	generate_submodels(*m, cfg->num_submodels, cfg->seed, nmat, cfg->texture_size);
 	light_polygons(*m);
 	build_light_grid(*m);
 	build_model_mesh(*m, cfg->seed, 7, 8, 28.0f);

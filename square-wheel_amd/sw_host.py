@@ -17,7 +17,7 @@ class MapConfig(C.Structure):
                 ("sun", C.c_uint32), ("sky_fraction", C.c_float), ("texture_size", C.c_uint32),
                 ("num_materials", C.c_uint32), ("normal_maps", C.c_uint32), ("specular_fraction", C.c_float),
                 ("alpha_blend_fraction", C.c_float), ("directional_lightmaps", C.c_uint32),
-                ("num_decal_textures", C.c_uint32), ("num_model_textures", C.c_uint32)]
+                ("num_decal_textures", C.c_uint32), ("num_model_textures", C.c_uint32), ("num_submodels", C.c_uint32)]
 
 
 class MapInfo(C.Structure):
@@ -136,7 +136,7 @@ class Map:
         defaults = dict(seed=0, grid=(8, 8, 2), solid_probability=0.3, num_lights=4, light_intensity_min=0.5,
                         light_intensity_max=2.0, sun=0, sky_fraction=0.1, texture_size=64, num_materials=8,
                         normal_maps=0, specular_fraction=0.0, alpha_blend_fraction=0.0, directional_lightmaps=0,
-                        num_decal_textures=0, num_model_textures=0)
+                        num_decal_textures=0, num_model_textures=0, num_submodels=0)
         defaults.update(kw)
         for k, v in defaults.items():
             if k == "grid":

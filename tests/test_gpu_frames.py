@@ -120,3 +120,30 @@ def test_frames_of_a_loaded_sqwm_map(gpu_ctx, map1, tmp_path):
             assert np.array_equal(img, ref), f"pose {i}: {np.count_nonzero(img != ref)} pixels differ"
     finally:
         gpu_ctx.release_map(mh)
+
+
+@pytest.mark.parametrize("threads", [4, 16])
+def test_virtual_thread_frames_bit_exact(gpu_ctx, map1, threads):
+    """The reference's N-thread image (split_rect's N rects, each pass with its own clip rect and
+    half-pixel-widened viewport): the front-end's N runs of draws render bit-exact against the oracle
+    through the plain and the batched path."""
+    fb = sw_host.FrameBuilder()
+    mh = gpu_ctx.upload_map(map1.data)
+    try:
+        frames = []
+        for i, pose in enumerate(map1.camera_path(9, 3)):
+            _say(f"case virtual threads {threads} pose {i}")
+            f = fb.build(map1, pose, width=640, height=360, use_directional_lightmaps=1, num_dynamic_lights=4,
+                         num_models=6, num_decals=24, seed=i, virtual_threads=threads)
+            frames.append(f)
+            b = sw_host.batch_from_frames([f])
+            img = gpu_ctx.render_plain(mh, b).reshape(-1)
+            ref, _, _, oob = oracle.render(map1.data, b)
+            assert oob == 0
+            assert np.array_equal(img, ref), f"pose {i}: {np.count_nonzero(img != ref)} pixels differ"
+        b = sw_host.batch_from_frames(frames)
+        out = gpu_ctx.render_batch(mh, b)
+        ref, _, _, _ = oracle.render(map1.data, b)
+        assert np.array_equal(out, ref)
+    finally:
+        gpu_ctx.release_map(mh)
