@@ -73,6 +73,7 @@ typedef struct swh_frame_config
 	uint32_t seed; /* dynamic objects seed */
 	float time; /* animation time, seconds */
 	uint32_t virtual_threads; /* 1: full-screen clip rect (the parity configuration) */
+	uint32_t view_model; /* 1: an animated model held in front of the camera (drawn after the tree) */
 } swh_frame_config;
 
 /* A view's command buffers, valid until the frame is rebuilt or destroyed. */
@@ -92,6 +93,7 @@ typedef struct swh_frame_data
 	uint64_t surface_arena_texels;
 	uint32_t num_visible_leafs;
 	uint32_t num_triangles;
+	uint32_t first_submodel_surface; /* surfaces from this index on belong to inline (brush) models */
 } swh_frame_data;
 
 int swh_map_create(const swh_map_config* config, swh_map** out_map);

@@ -1106,6 +1106,7 @@ void emit_draw_list(swh_frame& f, const swh_map& m, const swh_frame_config& cfg,
 			stack.push_back(node.children[mask]);
 		}
 	}
+	draw_view_models(f, m, viewport, clip_rect);
 }
 
 } // namespace
@@ -1159,6 +1160,7 @@ int swh_frame_build(swh_frame* f, const swh_map* m, const swh_frame_config* cfg,
 			prepare_polygon_surface(*f, *m, cam, cp, offset, leaf.first_polygon + k, cfg->textures_mip_bias);
 	}
 	prepare_submodels(*f, *m, *cfg, cam);
+	f->first_submodel_surface = uint32_t(f->visible_polygons.size());
 	prepare_submodel_surfaces(*f, *m, *cfg, cam, offset);
 	f->arena_texels = offset;
 	for(uint32_t pi : f->visible_polygons)
@@ -1225,6 +1227,7 @@ int swh_frame_get(const swh_frame* f, swh_frame_data* out)
 	out->surface_arena_texels = f->arena_texels;
 	out->num_visible_leafs = f->num_visible_leafs;
 	out->num_triangles = f->num_triangles;
+	out->first_submodel_surface = f->first_submodel_surface;
 	return 0;
 }
 

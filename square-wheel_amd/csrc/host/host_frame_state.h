@@ -127,6 +127,7 @@ void draw_polygon(swh_frame& f, const V3 clip_planes[8], const V3* verts, uint32
 // partial_renderer.rs:2519-2792). `bounds` is the leaf's screen ClippingPolygon.
 void draw_leaf_objects(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, uint32_t leaf, const ClippingPolygon& bounds,
 	const int32_t clip_rect[4], const M4& planes);
+void draw_view_models(swh_frame& f, const swh_map& m, const ClippingPolygon& viewport, const int32_t clip_rect[4]);
 
 } // namespace swh
 
@@ -185,6 +186,7 @@ struct swh_frame
 	sw_view view{};
 	uint32_t num_visible_leafs = 0;
 	uint32_t num_triangles = 0;
+	uint32_t first_submodel_surface = 0; // surfaces from here on belong to inline models
 
 	// dynamic objects
 	std::vector<std::vector<uint32_t>> leaf_lights;

@@ -645,10 +645,10 @@ void prepare_models(swh_frame& f, const swh_map& m, const swh_frame_config& cfg,
 	md.vertices.clear();
 	md.triangles.clear();
 	md.leaf_models.clear();
+	md.view_models.clear();
 	const ModelMesh& mesh = m.model_mesh;
-	if(cfg.num_models == 0 || m.model_levels.empty() || m.empty_cells.empty() || mesh.num_vertices == 0)
+	if((cfg.num_models == 0 && cfg.view_model == 0) || m.model_levels.empty() || m.empty_cells.empty() || mesh.num_vertices == 0)
 		return;
-	(void)planes;
 	md.leaf_models.assign(m.leafs.size(), std::vector<uint32_t>());
 	Rng64 rng(uint64_t(cfg.seed) * 0x9E3779B97F4A7C15ull + 101);
 	for(uint32_t i = 0; i < cfg.num_models; ++i)
@@ -667,6 +667,24 @@ void prepare_models(swh_frame& f, const swh_map& m, const swh_frame_config& cfg,
 		e.lerp = 1.0f - (t - ft);
 		e.skin = m.model_levels[i % m.model_levels.size()];
 		e.blend = (i % 8 == 7) ? SW_BLEND_ADDITIVE : SW_BLEND_NONE;
+		md.entities.push_back(e);
+	}
+	M4 inv_view;
+	if(cfg.view_model && invert(view, inv_view))
+	{
+		// held at the lower right of the screen, 48 units in front of the camera
+		const float z = 48.0f;
+		ModelEntity e;
+		e.pos = truncate(mul(inv_view, v4(0.72f * float(cfg.width) * z, 0.8f * float(cfg.height) * z, z, 1.0f)));
+		e.yaw = 0.6f * cfg.time;
+		const float t = cfg.time * 10.0f;
+		const float ft = std::floor(t);
+		e.frames[0] = uint32_t(int64_t(ft) % int64_t(mesh.num_frames));
+		e.frames[1] = (e.frames[0] + 1) % mesh.num_frames;
+		e.lerp = 1.0f - (t - ft);
+		e.skin = m.model_levels[0];
+		e.blend = SW_BLEND_NONE;
+		e.view_model = true;
 		md.entities.push_back(e);
 	}
 
@@ -721,9 +739,11 @@ void prepare_models(swh_frame& f, const swh_map& m, const swh_frame_config& cfg,
 					sp.extend_point(v2(v.x, v.y) / v.z);
 				}
 		}
-		// visible if one of its leaves is visible inside the model's screen box
-		leafs_of_box(m, world_min, world_max, leafs);
-		bool visible = false;
+		// visible if one of its leaves is visible inside the model's screen box (view models always)
+		leafs.clear();
+		if(!e.view_model)
+			leafs_of_box(m, world_min, world_max, leafs);
+		bool visible = e.view_model;
 		for(uint32_t l : leafs)
 		{
 			if(f.leaf_visible_frame[l] != f.frame_number)
@@ -745,6 +765,12 @@ void prepare_models(swh_frame& f, const swh_map& m, const swh_frame_config& cfg,
 		vm.clip[3][0] = sp.xmy.min, vm.clip[3][1] = sp.xmy.max;
 		const V3 center_cam = view_transform(mv, (bbox.min + bbox.max) * 0.5f);
 		vm.depth = center_cam.z;
+		vm.bbox = bbox;
+		vm.view = mv;
+		M4 world_planes;
+		if(!invert(transpose(model_matrix), world_planes))
+			world_planes = identity();
+		vm.planes = mul(planes, world_planes);
 
 		// calculate_triangle_model_texture_mip (triangle_models_rendering.rs:274-322)
 		uint32_t mip = 0;
@@ -860,6 +886,8 @@ void prepare_models(swh_frame& f, const swh_map& m, const swh_frame_config& cfg,
 		vm.num_triangles = uint32_t(order.size());
 		const uint32_t vis = uint32_t(md.visible.size());
 		md.visible.push_back(vm);
+		if(e.view_model)
+			md.view_models.push_back(vis);
 		for(uint32_t l : leafs)
 			md.leaf_models[l].push_back(vis);
 	}
@@ -957,6 +985,208 @@ void prepare_decals(swh_frame& f, const swh_map& m, const swh_frame_config& cfg,
 	}
 }
 
+// ---------------------------------------------------------------------------------------------
+// Order of a leaf's objects (draw_ordering.rs:86-226, LeafObjectsSorter)
+// ---------------------------------------------------------------------------------------------
+// A box seen from the camera: of each pair of opposite sides the one facing the camera (as a
+// camera-space plane), and the eight corners in camera space.
+struct ProjectedBBox
+{
+	Plane planes[3];
+	V3 vertices[8];
+};
+
+Plane project_plane(const Plane& p, const M4& planes_matrix)
+{
+	const V4 t = mul(planes_matrix, extend(p.vec, -p.dist));
+	return Plane{truncate(t), -t.w};
+}
+
+// project_bbox (draw_ordering.rs:74-80, 111-170)
+ProjectedBBox project_bbox(const BBox& b, const M4& view, const M4& planes_matrix)
+{
+	ProjectedBBox r;
+	const Plane pairs[3][2] = {{{v3(1, 0, 0), b.max.x}, {v3(-1, 0, 0), -b.min.x}}, {{v3(0, 1, 0), b.max.y}, {v3(0, -1, 0), -b.min.y}},
+		{{v3(0, 0, 1), b.max.z}, {v3(0, 0, -1), -b.min.z}}};
+	for(int i = 0; i < 3; ++i)
+	{
+		const Plane p0 = project_plane(pairs[i][0], planes_matrix), p1 = project_plane(pairs[i][1], planes_matrix);
+		r.planes[i] = p0.dist < 0.0f ? p0 : p1;
+	}
+	// BBox::get_corners_vertices order (bbox.rs:83-95)
+	for(int i = 0; i < 8; ++i)
+		r.vertices[i] = view_transform(
+			view, v3((i & 4) ? b.max.x : b.min.x, (i & 2) ? b.max.y : b.min.y, (i & 1) ? b.max.z : b.min.z));
+	return r;
+}
+
+// is_bbox_at_front_of_plane (:198-215): a camera-facing plane with every corner at or in front of it
+bool box_in_front_of_plane(const Plane& p, const V3 v[8])
+{
+	if(p.dist >= 0.0f)
+		return false;
+	for(int i = 0; i < 8; ++i)
+		if(!(dot(p.vec, v[i]) >= p.dist))
+			return false;
+	return true;
+}
+
+float box_min_z(const ProjectedBBox& b)
+{
+	float r = b.vertices[0].z;
+	for(int i = 0; i < 8; ++i)
+		r = std::fmin(r, b.vertices[i].z);
+	return r;
+}
+
+// compare_projected_bboxes (:172-196): true when `l` must be drawn after `r`
+bool draw_after(const ProjectedBBox& l, const ProjectedBBox& r)
+{
+	for(const Plane& lp : l.planes)
+		if(box_in_front_of_plane(lp, r.vertices))
+			return false;
+	for(const Plane& rp : r.planes)
+		if(box_in_front_of_plane(rp, l.vertices))
+			return true;
+	return box_min_z(l) < box_min_z(r);
+}
+
+// order_bboxes (:82-109): the reference's pairwise exchange sort (no total order exists), at most
+// 48 objects (MAX_OBJECTS, :48)
+void order_objects(std::vector<std::pair<uint32_t, ProjectedBBox>>& objs)
+{
+	if(objs.size() > 48)
+		objs.resize(48);
+	for(size_t i = 0; i + 1 < objs.size(); ++i)
+		for(size_t j = i + 1; j < objs.size(); ++j)
+			if(draw_after(objs[i].second, objs[j].second))
+				std::swap(objs[i], objs[j]);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Inline models in a leaf (partial_renderer.rs:3098-3288)
+// ---------------------------------------------------------------------------------------------
+// clip_3d_polygon_by_plane (clipping.rs:53-103): the part with dot(vec, v) >= dist
+uint32_t clip_polygon_by_plane(const V3* poly, uint32_t n, const Plane& plane, V3* out, uint32_t cap)
+{
+	if(n == 0)
+		return 0;
+	const V3* prev = &poly[n - 1];
+	float prev_d = dot(plane.vec, *prev);
+	uint32_t count = 0;
+	auto cut = [&](const V3& v0, const V3& v1) {
+		// get_line_plane_intersection (clipping.rs:177-185)
+		const float d0 = dot(v0, plane.vec) - plane.dist, d1 = dot(v1, plane.vec) - plane.dist, sum = d1 - d0;
+		return v0 * (d1 / sum) - v1 * (d0 / sum);
+	};
+	for(uint32_t k = 0; k < n; ++k)
+	{
+		const V3& v = poly[k];
+		const float d = dot(plane.vec, v);
+		if(d > plane.dist)
+		{
+			if(prev_d < plane.dist)
+			{
+				out[count++] = cut(*prev, v);
+				if(count == cap)
+					break;
+			}
+			out[count++] = v;
+			if(count == cap)
+				break;
+		}
+		else if(d == plane.dist)
+		{
+			out[count++] = v;
+			if(count == cap)
+				break;
+		}
+		else if(prev_d > plane.dist)
+		{
+			out[count++] = cut(*prev, v);
+			if(count == cap)
+				break;
+		}
+		prev = &v;
+		prev_d = d;
+	}
+	return count;
+}
+
+// draw_submodel_polygon (:3199-3288): the polygon's camera-space vertices clipped by the leaf's
+// planes, skipping a plane the polygon lies on (and dropping the polygon when it lies on a plane
+// facing the other way: the neighbour leaf draws it), then drawn with the leaf bounds' planes.
+void draw_submodel_polygon(swh_frame& f, const swh_map& m, const V3 clip_planes[8], const Plane* leaf_planes,
+	const Plane* leaf_planes_world, uint32_t num_leaf_planes, uint32_t pi, const int32_t clip_rect[4])
+{
+	const swh_frame::PolyData& pd = f.poly[pi];
+	if(pd.visible_frame != f.frame_number)
+		return;
+	const Polygon& polygon = m.polygons[pi];
+	const swh_frame::Basis& basis = f.basis[pi];
+	V3 a[MAX_VERTICES], b[MAX_VERTICES];
+	uint32_t n = std::min(polygon.num_vertices, MAX_VERTICES);
+	for(uint32_t k = 0; k < n; ++k)
+		a[k] = f.vertices_transformed[polygon.first_vertex + k];
+	V3* src = a;
+	V3* dst = b;
+	const float almost_one = 1.0f - 1.0f / 256.0f, dist_eps = 1.0f / 4.0f;
+	for(uint32_t i = 0; i < num_leaf_planes; ++i)
+	{
+		const Plane& w = leaf_planes_world[i];
+		const float nd = dot(basis.normal, w.vec);
+		if(nd >= almost_one)
+		{
+			if(std::fabs(dot(w.vec, basis.start) - w.dist) <= dist_eps)
+				continue;
+		}
+		else if(nd <= -almost_one)
+		{
+			if(std::fabs(dot(w.vec, basis.start) - w.dist) <= dist_eps)
+				return;
+		}
+		n = clip_polygon_by_plane(src, n, leaf_planes[i], dst, MAX_VERTICES);
+		if(n < 3)
+			return;
+		std::swap(src, dst);
+	}
+	draw_polygon(f, clip_planes, src, n, pd.de, pd.te, pd.size, pd.surface_offset, SW_TEXSRC_SURFACE,
+		m.materials[polygon.texture].blending_mode, clip_rect);
+}
+
+// draw_submodel_bsp_node_r (:3126-3197): the far child, the node's polygons (and the leaf's decals
+// on them), the near child
+void draw_submodel_node(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, const V4& planes_w, const V3 clip_planes[8],
+	const Plane* leaf_planes, const Plane* leaf_planes_world, uint32_t num_leaf_planes, const std::vector<uint32_t>* leaf_decals,
+	uint32_t node_index, const int32_t clip_rect[4], uint32_t depth)
+{
+	const SubmodelNode& node = m.submodel_nodes[node_index];
+	const uint32_t num_nodes = uint32_t(m.submodel_nodes.size());
+	const float w = dot(planes_w, extend(node.plane.vec, -node.plane.dist));
+	const uint32_t mask = w >= 0.0f ? 1u : 0u;
+	const uint32_t c_b = node.children[mask], c_f = node.children[mask ^ 1u];
+	if(c_b < num_nodes && depth < 256)
+		draw_submodel_node(f, m, cfg, planes_w, clip_planes, leaf_planes, leaf_planes_world, num_leaf_planes, leaf_decals, c_b,
+			clip_rect, depth + 1);
+	for(uint32_t pi = node.first_polygon; pi < node.first_polygon + node.num_polygons; ++pi)
+	{
+		draw_submodel_polygon(f, m, clip_planes, leaf_planes, leaf_planes_world, num_leaf_planes, pi, clip_rect);
+		if(leaf_decals && !leaf_decals->empty())
+			draw_polygon_decals(f, m, cfg, clip_planes, pi, *leaf_decals, clip_rect);
+	}
+	if(c_f < num_nodes && depth < 256)
+		draw_submodel_node(f, m, cfg, planes_w, clip_planes, leaf_planes, leaf_planes_world, num_leaf_planes, leaf_decals, c_f,
+			clip_rect, depth + 1);
+}
+
+// draw_view_models (partial_renderer.rs:2284-2309): view models over the viewport part, with no 3D
+// clip planes but the near plane
+void draw_view_models(swh_frame& f, const swh_map& m, const ClippingPolygon& viewport, const int32_t clip_rect[4])
+{
+	for(uint32_t vi : f.models.view_models)
+		draw_mesh(f, m, viewport, nullptr, 0, f.models.visible[vi], clip_rect);
+}
+
 void draw_leaf_objects(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, uint32_t leaf, const ClippingPolygon& bounds,
 	const int32_t clip_rect[4], const M4& planes)
 {
@@ -967,23 +1197,57 @@ void draw_leaf_objects(swh_frame& f, const swh_map& m, const swh_frame_config& c
 	if(!md.leaf_decals.empty() && !md.leaf_decals[leaf].empty())
 		for(uint32_t k = 0; k < lf.num_polygons; ++k)
 			draw_polygon_decals(f, m, cfg, clip_planes, lf.first_polygon + k, md.leaf_decals[leaf], clip_rect);
-	if(md.leaf_models.empty() || md.leaf_models[leaf].empty())
+	static const std::vector<uint32_t> no_objects;
+	const std::vector<uint32_t>& submodels = f.leaf_submodels.empty() ? no_objects : f.leaf_submodels[leaf];
+	const std::vector<uint32_t>& meshes = md.leaf_models.empty() ? no_objects : md.leaf_models[leaf];
+	const size_t total = submodels.size() + meshes.size();
+	if(total == 0)
 		return;
-	// the leaf's volume (a grid cell) as six inward planes, in camera space
+	// the leaf's volume (a grid cell) as six inward planes, in world and camera space
 	const BBox& b = m.leaf_bbox[leaf];
 	const Plane world[6] = {{v3(1, 0, 0), b.min.x}, {v3(-1, 0, 0), -b.max.x}, {v3(0, 1, 0), b.min.y}, {v3(0, -1, 0), -b.max.y},
 		{v3(0, 0, 1), b.min.z}, {v3(0, 0, -1), -b.max.z}};
 	Plane cam[6];
 	for(int i = 0; i < 6; ++i)
+		cam[i] = project_plane(world[i], planes);
+	const std::vector<uint32_t>* leaf_decals = md.leaf_decals.empty() ? nullptr : &md.leaf_decals[leaf];
+	const uint32_t SUBMODEL_ADD = 65536, MESH_ADD = SUBMODEL_ADD + 65536;
+	auto draw_object = [&](uint32_t id) {
+		if(id >= MESH_ADD)
+			draw_mesh(f, m, bounds, cam, 6, md.visible[id - MESH_ADD], clip_rect);
+		else
+		{
+			const uint32_t k = id - SUBMODEL_ADD;
+			const swh_frame::SubmodelFrame& s = f.submodels[k];
+			if(s.active)
+				draw_submodel_node(f, m, cfg, s.planes.row(3), clip_planes, cam, world, 6, leaf_decals, m.submodels[k].root_node,
+					clip_rect, 0);
+		}
+	};
+	if(total == 1)
 	{
-		const V4 t = mul(planes, extend(world[i].vec, -world[i].dist));
-		cam[i] = Plane{truncate(t), -t.w};
+		draw_object(submodels.empty() ? MESH_ADD + meshes[0] : SUBMODEL_ADD + submodels[0]);
+		return;
 	}
-	// objects of a leaf back to front (a simplification of draw_ordering.rs's projected-box order)
-	std::vector<uint32_t> order = md.leaf_models[leaf];
-	std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return md.visible[a].depth > md.visible[c].depth; });
-	for(uint32_t vi : order)
-		draw_mesh(f, m, bounds, cam, 6, md.visible[vi], clip_rect);
+	// several objects: the reference's projected-box order (draw_leaf, :2646-2714)
+	std::vector<std::pair<uint32_t, ProjectedBBox>> objs;
+	for(uint32_t k : submodels)
+	{
+		const swh_frame::SubmodelFrame& s = f.submodels[k];
+		if(!s.active)
+			continue;
+		// get_model_bbox_for_ordering (inline_models_index.rs:66-80): shrunk by 1 towards the centre
+		BBox ob = s.bbox;
+		const V3 c = (ob.min + ob.max) * 0.5f;
+		ob.max = v3(std::fmax(ob.max.x - 1.0f, c.x), std::fmax(ob.max.y - 1.0f, c.y), std::fmax(ob.max.z - 1.0f, c.z));
+		ob.min = v3(std::fmin(ob.min.x + 1.0f, c.x), std::fmin(ob.min.y + 1.0f, c.y), std::fmin(ob.min.z + 1.0f, c.z));
+		objs.push_back({SUBMODEL_ADD + k, project_bbox(ob, s.view, s.planes)});
+	}
+	for(uint32_t vi : meshes)
+		objs.push_back({MESH_ADD + vi, project_bbox(md.visible[vi].bbox, md.visible[vi].view, md.visible[vi].planes)});
+	order_objects(objs);
+	for(const auto& o : objs)
+		draw_object(o.first);
 }
 
 } // namespace swh

@@ -27,6 +27,7 @@ struct ModelEntity
 	float lerp; // weight of frames[0]
 	uint32_t skin; // mip-0 lite level of its texture
 	uint32_t blend;
+	bool view_model = false; // ModelEntityDrawFlags::VIEW_MODEL: drawn after the tree, over the viewport
 };
 
 // One visible model with its transformed vertices and its back-to-front sorted front-facing
@@ -39,7 +40,9 @@ struct VisibleModel
 	uint32_t level; // lite texture level of the selected mip
 	V3 bbox_vertices[8]; // camera space
 	float clip[4][2]; // screen ClippingPolygon axes (x, y, x + y, x - y) as min/max pairs
-	float depth; // for ordering within a leaf
+	float depth; // camera-space depth of the bbox centre
+	BBox bbox; // model space: the union of the two animation frames' boxes
+	M4 view, planes; // the camera's matrices combined with the model matrix (ordering within a leaf)
 };
 
 // Decal (frame_info.rs:113-123): a unit cube (half-size 1) scaled and rotated, projected onto
@@ -62,6 +65,7 @@ struct ModelFrameData
 	std::vector<ModelVertex3d> vertices;
 	std::vector<uint32_t> triangles; // 3 per triangle
 	std::vector<std::vector<uint32_t>> leaf_models; // per leaf: indices into `visible`
+	std::vector<uint32_t> view_models; // indices into `visible`
 	std::vector<DecalEntity> decals;
 	struct DecalInfo
 	{

@@ -231,7 +231,7 @@ std::vector<std::pair<std::string, std::string>> generator_keys(const swh_map_co
 		{"sw_num_materials", u(c.num_materials)}, {"sw_normal_maps", u(c.normal_maps)},
 		{"sw_specular_fraction", fmt_float(c.specular_fraction)}, {"sw_alpha_blend_fraction", fmt_float(c.alpha_blend_fraction)},
 		{"sw_directional_lightmaps", u(c.directional_lightmaps)}, {"sw_num_decal_textures", u(c.num_decal_textures)},
-		{"sw_num_model_textures", u(c.num_model_textures)}};
+		{"sw_num_model_textures", u(c.num_model_textures)}, {"sw_num_submodels", u(c.num_submodels)}};
 }
 
 std::string material_name(const swh_map& m, uint32_t i)
@@ -307,15 +307,26 @@ int build_sqwm(const swh_map& m, std::vector<uint8_t>& out)
 		std::memcpy(textures[i].data(), name.data(), std::min<size_t>(name.size(), TEXTURE_NAME_LEN));
 	}
 	w.lump(LUMP_TEXTURES, textures);
-	w.lump(LUMP_SUBMODELS, static_cast<const Submodel*>(nullptr), 0);
-	w.lump(LUMP_SUBMODELS_BSP_NODES, static_cast<const SubmodelBSPNode*>(nullptr), 0);
+	{
+		std::vector<Submodel> sub(m.submodels.size());
+		for(size_t i = 0; i < sub.size(); ++i)
+			sub[i] = Submodel{m.submodels[i].root_node, m.submodels[i].first_polygon, m.submodels[i].num_polygons};
+		std::vector<SubmodelBSPNode> nodes(m.submodel_nodes.size());
+		for(size_t i = 0; i < nodes.size(); ++i)
+		{
+			const swh::SubmodelNode& n = m.submodel_nodes[i];
+			nodes[i] = SubmodelBSPNode{n.plane, n.first_polygon, n.num_polygons, {n.children[0], n.children[1]}};
+		}
+		w.lump(LUMP_SUBMODELS, sub);
+		w.lump(LUMP_SUBMODELS_BSP_NODES, nodes);
+	}
 
 	// entities: worldspawn (with the generator's keys), then one "light" per static point light
 	Strings strings;
 	std::vector<Entity> entities;
 	std::vector<KeyValuePair> kvs;
-	auto entity = [&](const std::vector<std::pair<std::string, std::string>>& keys) {
-		entities.push_back(Entity{uint32_t(kvs.size()), uint32_t(keys.size()), 0xFFFFFFFFu});
+	auto entity = [&](const std::vector<std::pair<std::string, std::string>>& keys, uint32_t submodel = 0xFFFFFFFFu) {
+		entities.push_back(Entity{uint32_t(kvs.size()), uint32_t(keys.size()), submodel});
 		for(const auto& kv : keys)
 			kvs.push_back(KeyValuePair{strings.add(kv.first), strings.add(kv.second)});
 	};
@@ -325,6 +336,9 @@ int build_sqwm(const swh_map& m, std::vector<uint8_t>& out)
 	for(const PointLight& l : m.lights)
 		entity({{"classname", "light"}, {"origin", fmt_float(l.pos.x) + " " + fmt_float(l.pos.y) + " " + fmt_float(l.pos.z)},
 			{"light", fmt_float(l.intensity / 32.0f)}, {"color", "255 255 255"}});
+	// one brush entity per inline model (doors and platforms alternate), holding its submodel index
+	for(uint32_t k = 0; k < m.submodels.size(); ++k)
+		entity({{"classname", k % 2 == 0 ? "func_door" : "func_plat"}}, k);
 	if(strings.overflow)
 		return -1;
 	w.lump(LUMP_ENTITIES, entities);
@@ -524,6 +538,28 @@ bool validate(const swh_map& m, std::string& error)
 	for(const Portal& p : m.portals)
 		if(p.leafs[0] >= nl || p.leafs[1] >= nl || uint64_t(p.first_vertex) + p.num_vertices > m.vertices.size())
 			return (error = "portal out of range"), false;
+	// inline models: polygon ranges, and node trees walked from each root reaching every node once
+	const uint32_t nsn = uint32_t(m.submodel_nodes.size());
+	for(const swh::SubmodelNode& n : m.submodel_nodes)
+		if(uint64_t(n.first_polygon) + n.num_polygons > np)
+			return (error = "submodel node polygon range out of range"), false;
+	std::vector<uint8_t> seen_sub(nsn, 0);
+	for(const swh::Submodel& sm : m.submodels)
+	{
+		if(uint64_t(sm.first_polygon) + sm.num_polygons > np || sm.root_node >= nsn)
+			return (error = "submodel out of range"), false;
+		std::vector<uint32_t> stack{sm.root_node};
+		while(!stack.empty())
+		{
+			const uint32_t i = stack.back();
+			stack.pop_back();
+			if(seen_sub[i]++)
+				return (error = "submodel node reached twice: the submodel nodes do not form trees"), false;
+			for(uint32_t c : m.submodel_nodes[i].children)
+				if(c < nsn)
+					stack.push_back(c);
+		}
+	}
 	return true;
 }
 
@@ -545,6 +581,8 @@ int load_sqwm(const std::vector<uint8_t>& bytes, swh_map** out, std::string& err
 	std::vector<uint32_t> leafs_portals;
 	std::vector<V3> vertices;
 	std::vector<std::array<char, TEXTURE_NAME_LEN>> textures;
+	std::vector<Submodel> submodels;
+	std::vector<SubmodelBSPNode> submodel_nodes;
 	std::vector<Entity> entities;
 	std::vector<KeyValuePair> kvs;
 	std::vector<uint8_t> strings;
@@ -555,7 +593,7 @@ int load_sqwm(const std::vector<uint8_t>& bytes, swh_map** out, std::string& err
 	std::vector<LightGridElementCompressed> samples;
 	if(!r.lump(LUMP_NODES, nodes) || !r.lump(LUMP_LEAFS, leafs) || !r.lump(LUMP_POLYGONS, polygons) || !r.lump(LUMP_PORTALS, portals) ||
 		!r.lump(LUMP_LEAFS_PORTALS, leafs_portals) || !r.lump(LUMP_VERTICES, vertices) || !r.lump(LUMP_TEXTURES, textures) ||
-		!r.lump(LUMP_ENTITIES, entities) || !r.lump(LUMP_KEY_VALUE_PAIRS, kvs) || !r.lump(LUMP_STRINGS_DATA, strings) ||
+		!r.lump(LUMP_SUBMODELS, submodels) || !r.lump(LUMP_SUBMODELS_BSP_NODES, submodel_nodes) || !r.lump(LUMP_ENTITIES, entities) || !r.lump(LUMP_KEY_VALUE_PAIRS, kvs) || !r.lump(LUMP_STRINGS_DATA, strings) ||
 		!r.lump(LUMP_LIGHTMAPS_DATA, lm) || !r.lump(LUMP_DIRECTIONAL_LIGHTMAPS_DATA, dlm) || !r.lump(LUMP_LIGHT_GRID_HEADER, gh) ||
 		!r.lump(LUMP_LIGHT_GRID_COLUMNS, columns) || !r.lump(LUMP_LIGHT_GRID_SAMPLES, samples))
 		return (error = r.error), -1;
@@ -585,6 +623,8 @@ int load_sqwm(const std::vector<uint8_t>& bytes, swh_map** out, std::string& err
 		!parse_u32s(world["sw_num_decal_textures"], &c.num_decal_textures, 1) ||
 		!parse_u32s(world["sw_num_model_textures"], &c.num_model_textures, 1))
 		return (error = "the worldspawn entity has no generator configuration (sw_* keys): textures cannot be rebuilt"), -1;
+	if(world.count("sw_num_submodels") && !parse_u32s(world["sw_num_submodels"], &c.num_submodels, 1))
+		return (error = "bad sw_num_submodels key"), -1;
 	c.light_intensity_min = intensity[0];
 	c.light_intensity_max = intensity[1];
 
@@ -627,6 +667,20 @@ int load_sqwm(const std::vector<uint8_t>& bytes, swh_map** out, std::string& err
 	m->portals.swap(portals);
 	m->leafs_portals.swap(leafs_portals);
 	m->vertices.swap(vertices);
+	m->submodels.resize(submodels.size());
+	for(size_t i = 0; i < submodels.size(); ++i)
+		m->submodels[i] = swh::Submodel{submodels[i].root_node, submodels[i].first_polygon, submodels[i].num_polygons};
+	m->submodel_nodes.resize(submodel_nodes.size());
+	for(size_t i = 0; i < submodel_nodes.size(); ++i)
+	{
+		const SubmodelBSPNode& n = submodel_nodes[i];
+		m->submodel_nodes[i] = swh::SubmodelNode{n.plane, n.first_polygon, n.num_polygons, {n.children[0], n.children[1]}};
+	}
+	// the polygons' parents from the file: leaf ranges, the rest belongs to inline models
+	m->polygon_leaf.assign(m->polygons.size(), NO_POLYGON_LEAF);
+	for(uint32_t l = 0; l < m->leafs.size(); ++l)
+		for(uint32_t k = 0; k < m->leafs[l].num_polygons && uint64_t(m->leafs[l].first_polygon) + k < m->polygons.size(); ++k)
+			m->polygon_leaf[m->leafs[l].first_polygon + k] = l;
 	for(size_t i = 0; i < lm.size(); ++i)
 		decompress_color(lm[i], &m->lm_simple[i * 3]);
 	for(size_t i = 0; i < dlm.size(); ++i)
@@ -721,6 +775,31 @@ std::string build_quake_map(const swh_map& m)
 	for(const PointLight& l : m.lights)
 		s += "{\n\"classname\" \"light\"\n\"origin\" \"" + fmt_float(l.pos.x) + " " + fmt_float(l.pos.y) + " " + fmt_float(l.pos.z) +
 			"\"\n\"light\" \"" + fmt_float(l.intensity / 32.0f) + "\"\n\"color\" \"255 255 255\"\n}\n";
+	// inline models as brush entities: one box brush each, its faces' textures
+	for(uint32_t k = 0; k < m.submodels.size(); ++k)
+	{
+		const swh::Submodel& sm = m.submodels[k];
+		int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+		for(uint32_t pi = sm.first_polygon; pi < sm.first_polygon + sm.num_polygons; ++pi)
+			for(uint32_t v = 0; v < m.polygons[pi].num_vertices; ++v)
+			{
+				const V3 q = m.vertices[m.polygons[pi].first_vertex + v];
+				const int c3[3] = {int(std::lround(q.x)), int(std::lround(q.y)), int(std::lround(q.z))};
+				for(int a = 0; a < 3; ++a)
+				{
+					lo[a] = std::min(lo[a], c3[a]);
+					hi[a] = std::max(hi[a], c3[a]);
+				}
+			}
+		const std::string tex = material_name(m, m.polygons[sm.first_polygon].texture);
+		s += std::string("{\n\"classname\" \"") + (k % 2 == 0 ? "func_door" : "func_plat") + "\"\n{\n";
+		for(int axis = 0; axis < 3; ++axis)
+		{
+			brush_plane(s, lo, axis, -1, 16, tex);
+			brush_plane(s, hi, axis, +1, 16, tex);
+		}
+		s += "}\n}\n";
+	}
 	if(!m.empty_cells.empty())
 	{
 		const uint32_t c = m.empty_cells[0];

@@ -31,7 +31,7 @@ class FrameConfig(C.Structure):
                 ("textures_mip_bias", C.c_float), ("use_directional_lightmaps", C.c_uint32),
                 ("clear_background", C.c_uint32), ("force_clear", C.c_uint32), ("num_dynamic_lights", C.c_uint32),
                 ("num_models", C.c_uint32), ("num_decals", C.c_uint32), ("seed", C.c_uint32), ("time", C.c_float),
-                ("virtual_threads", C.c_uint32)]
+                ("virtual_threads", C.c_uint32), ("view_model", C.c_uint32)]
 
 
 class ViewC(C.Structure):
@@ -46,7 +46,7 @@ class FrameData(C.Structure):
                 ("num_lights", C.c_uint32), ("light_indices", C.c_void_p), ("num_light_indices", C.c_uint32),
                 ("draws", C.c_void_p), ("num_draws", C.c_uint32), ("vertex_data", C.c_void_p),
                 ("num_vertex_ints", C.c_uint64), ("surface_arena_texels", C.c_uint64),
-                ("num_visible_leafs", C.c_uint32), ("num_triangles", C.c_uint32)]
+                ("num_visible_leafs", C.c_uint32), ("num_triangles", C.c_uint32), ("first_submodel_surface", C.c_uint32)]
 
 
 assert C.sizeof(ViewC) == A.VIEW_DTYPE.itemsize
@@ -217,7 +217,7 @@ class FrameBuilder:
         cfg = FrameConfig()
         defaults = dict(width=320, height=200, color_bits=32, fov=math.pi / 2, textures_mip_bias=0.0,
                         use_directional_lightmaps=0, clear_background=1, force_clear=1, num_dynamic_lights=0,
-                        num_models=0, num_decals=0, seed=0, time=0.0, virtual_threads=1)
+                        num_models=0, num_decals=0, seed=0, time=0.0, virtual_threads=1, view_model=0)
         defaults.update(kw)
         for k, v in defaults.items():
             setattr(cfg, k, v)
@@ -238,6 +238,7 @@ class FrameBuilder:
             "arena_texels": int(fd.surface_arena_texels),
             "num_visible_leafs": int(fd.num_visible_leafs),
             "num_triangles": int(fd.num_triangles),
+            "first_submodel_surface": int(fd.first_submodel_surface),
             "color_bits": int(cfg.color_bits),
         }
 
