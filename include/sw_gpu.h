@@ -73,6 +73,8 @@ enum
 {
 	SW_TEXSRC_SURFACE = 0, /* offset into the frame's surface arena (ColorT texels) */
 	SW_TEXSRC_SKYBOX = 1,  /* offset into the map's skybox texel buffer (ColorT texels) */
+	SW_TEXSRC_PORTAL = 2,  /* offset into the batch's portal textures (ColorT texels): the framebuffers of
+	                          an earlier batch that rendered the views seen through portals and mirrors */
 };
 
 /* Texture flags (textures.rs:11-18). */
@@ -250,6 +252,11 @@ typedef struct sw_batch
 	uint64_t framebuffer_texels; /* total output size in ColorT elements */
 	uint32_t inputs_on_device;
 	uint32_t reserved;
+	/* Textures of SW_TEXSRC_PORTAL draws (build_view_portals_textures, partial_renderer.rs:2008-2282):
+	 * a device buffer of ColorT texels, typically the output of an earlier sw_render_batch over the
+	 * portal views; NULL / 0 when no draw samples it. */
+	const void* portal_texels;
+	uint64_t num_portal_texels;
 } sw_batch;
 
 /* Work counters of the last batch (used for the algorithmic-bytes roofline). */
@@ -298,6 +305,10 @@ int sw_frame_begin(
 int sw_build_surfaces(
 	sw_ctx* ctx, const sw_surface_desc* surfaces, uint32_t num_surfaces, const sw_light* lights, uint32_t num_lights,
 	const uint32_t* light_indices, uint32_t num_light_indices);
+/* The portal textures that SW_TEXSRC_PORTAL draws of the plain path sample (device ColorT texels;
+ * NULL / 0: none), kept for the context's following frames. */
+int sw_set_portal_textures(sw_ctx* ctx, const void* portal_texels_device, uint64_t num_portal_texels);
+
 int sw_raster_draws(
 	sw_ctx* ctx, const sw_draw* draws, uint32_t num_draws, const int32_t* vertex_data, uint64_t num_vertex_ints);
 int sw_frame_end(sw_ctx* ctx, void* dst, size_t dst_pitch_bytes, uint32_t dst_on_device);

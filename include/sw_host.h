@@ -50,6 +50,7 @@ typedef struct swh_map_config
 	uint32_t num_decal_textures; /* lite 32x32 decal textures */
 	uint32_t num_model_textures; /* lite 64x64 model skins */
 	uint32_t num_submodels; /* inline brush models (doors and platforms) moved by the frame time */
+	uint32_t num_view_portals; /* wall portals: mirrors, camera views and parallax views, in turn */
 } swh_map_config;
 
 typedef struct swh_map_info
@@ -74,6 +75,7 @@ typedef struct swh_frame_config
 	float time; /* animation time, seconds */
 	uint32_t virtual_threads; /* 1: full-screen clip rect (the parity configuration) */
 	uint32_t view_model; /* 1: an animated model held in front of the camera (drawn after the tree) */
+	uint32_t portals_depth; /* RendererConfig::portals_depth: levels of views rendered through portals */
 } swh_frame_config;
 
 /* A view's command buffers, valid until the frame is rebuilt or destroyed. */
@@ -94,6 +96,14 @@ typedef struct swh_frame_data
 	uint32_t num_visible_leafs;
 	uint32_t num_triangles;
 	uint32_t first_submodel_surface; /* surfaces from this index on belong to inline (brush) models */
+	/* View portals (partial_renderer.rs:1834-2282): the frame's portal draws (SW_TEXSRC_PORTAL) sample
+	 * num_portal_texels texels of the frame's colour type, the portal textures laid end to end. With
+	 * portals_depth > 0 the textures are the framebuffers of num_portal_views views (one per visible
+	 * portal, in portal order, each a frame of its own: swh_frame_portal_view), laid end to end; at
+	 * the last level they are black (zero). */
+	uint64_t num_portal_texels;
+	uint32_t num_portal_views;
+	uint32_t num_visible_portals;
 } swh_frame_data;
 
 int swh_map_create(const swh_map_config* config, swh_map** out_map);
@@ -101,6 +111,27 @@ void swh_map_destroy(swh_map* map);
 /* Pointers into host arrays owned by the map (upload them with sw_map_upload). */
 int swh_map_desc(const swh_map* map, sw_map_desc* out_desc);
 int swh_map_info_get(const swh_map* map, swh_map_info* out_info);
+/* The map's view portals (frame_info.rs ViewPortal / PortalView, placed by the game in the reference):
+ * quads 1 unit in front of walls, 64 x 80 units, two texels per unit. Returns the number written
+ * (at most capacity; the map's count with out == NULL). */
+enum
+{
+	SWH_PORTAL_MIRROR = 0,
+	SWH_PORTAL_CAMERA = 1, /* CameraAtPosition: a camera at camera_position (yaw, pitch, fov) */
+	SWH_PORTAL_PARALLAX = 2 /* ParallaxPortal: transform maps the viewer's position to the view point */
+};
+typedef struct swh_view_portal
+{
+	uint32_t kind;
+	uint32_t blending_mode;
+	float plane[4]; /* vec, dist: the front side is where dot(vec, p) > dist */
+	float tex_coord_equation[2][4]; /* vec, dist: tc = dot(vec, p) + dist */
+	float vertices[4][3]; /* clockwise on screen seen from the front */
+	float camera_position[3];
+	float camera_yaw, camera_pitch, camera_fov;
+	float transform[16]; /* column-major */
+} swh_view_portal;
+int swh_map_view_portals(const swh_map* map, swh_view_portal* out, uint32_t capacity);
 /* n seeded camera poses at empty-cell centres: out = n x {x, y, z, yaw, pitch}. */
 int swh_camera_path(const swh_map* map, uint32_t seed, uint32_t n, float* out_poses);
 
@@ -109,6 +140,21 @@ void swh_frame_destroy(swh_frame* frame);
 int swh_frame_build(swh_frame* frame, const swh_map* map, const swh_frame_config* config, const float cam_pos[3],
 	float yaw, float pitch);
 int swh_frame_get(const swh_frame* frame, swh_frame_data* out_data);
+/* A frame's portal (PortalInfo, partial_renderer.rs:1834-1996): visible when its texture has pixels;
+ * the texture is resolution[0] x resolution[1] texels of the portal's texture space at `mip`, its
+ * texel (0, 0) at tc_min, at texture_offset in the frame's portal textures. */
+typedef struct swh_portal_info
+{
+	uint32_t visible;
+	uint32_t resolution[2];
+	int32_t tc_min[2];
+	uint32_t mip;
+	uint64_t texture_offset;
+} swh_portal_info;
+int swh_frame_portal_info(const swh_frame* frame, uint32_t portal_index, swh_portal_info* out_info);
+/* The index-th view seen through a visible portal of the frame (index < num_portal_views); valid until
+ * the frame is rebuilt or destroyed. Its own portal views follow the same scheme, one level deeper. */
+int swh_frame_portal_view(const swh_frame* frame, uint32_t index, const swh_frame** out_view);
 
 /* Shadow-map draw lists (sw_render_depth's input) of the map's static geometry:
  *   swh_shadow_add_cube <- PartialRenderer::build_shadow_maps' cube branch (partial_renderer.rs:541-566)

@@ -1487,7 +1487,14 @@ void raster_view(
 				continue;
 			const typename CT::T* tex = d.texture_source == SW_TEXSRC_SKYBOX ?
 				static_cast<const typename CT::T*>(m.skybox) + d.texture_offset :
+				d.texture_source == SW_TEXSRC_PORTAL ? static_cast<const typename CT::T*>(b.portal_texels) + d.texture_offset :
 				arena + view.surface_arena_offset + d.texture_offset;
+			if(d.texture_source == SW_TEXSRC_PORTAL &&
+				(!b.portal_texels ||
+					d.texture_offset + uint64_t(uint32_t(d.texture_size[0])) * uint32_t(d.texture_size[1]) > b.num_portal_texels))
+				continue; // outside the portal textures: skipped, as the kernels skip it
+			if(d.texture_source > SW_TEXSRC_PORTAL)
+				continue;
 			r.fill_polygon(vd, d.num_vertices, d.mode, d.blend, d.depth_equation, d.tc_equation, d.texture_size, tex);
 		}
 		else

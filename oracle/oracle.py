@@ -96,20 +96,45 @@ def load():
     return _lib
 
 
-def render(map_data, batch, mode=MODE_EXACT, num_threads=1, arena=None):
-    """Runs the oracle over a batch; returns (framebuffer, arena, stats dict, out_of_bounds)."""
+def render(map_data, batch, mode=MODE_EXACT, num_threads=1, arena=None, portal=None):
+    """Runs the oracle over a batch; returns (framebuffer, arena, stats dict, out_of_bounds).
+    portal: the portal textures (a ColorT array) that SW_TEXSRC_PORTAL draws sample."""
     lib = load()
     fb = np.zeros(max(batch.fb_texels, 1), dtype=batch.color_dtype)
     if arena is None:
         arena = np.zeros(max(batch.arena_texels, 1), dtype=batch.color_dtype)
     md = map_data.desc()
-    cb = batch.ctypes_batch()
+    if portal is not None:
+        portal = np.ascontiguousarray(portal, dtype=batch.color_dtype)
+        cb = batch.ctypes_batch(portal=(portal.ctypes.data, portal.size))
+    else:
+        cb = batch.ctypes_batch()
     st = sw_abi.Stats()
     oob = C.c_uint64(0)
     r = lib.oracle_render_batch(mode, C.byref(md), C.byref(cb), C.c_void_p(arena.ctypes.data),
                                 C.c_void_p(fb.ctypes.data), num_threads, C.byref(st), C.byref(oob))
     assert r == 0
     return fb[:batch.fb_texels], arena, st.as_dict(), int(oob.value)
+
+
+def render_portal_levels(map_data, batches, mode=MODE_EXACT):
+    """A frame and the views through its portals (one batch per portal level, the frame's own
+    first): the deepest level samples zero portal textures, each level above the framebuffer of the
+    level below (build_view_portals_textures, partial_renderer.rs:2008-2200). Returns every level's
+    framebuffer and the number of out-of-bounds fetches over all levels."""
+    outs = [None] * len(batches)
+    below = None
+    oob_total = 0
+    for k in reversed(range(len(batches))):
+        b = batches[k]
+        if below is None:
+            below = np.zeros(max(b.portal_texels, 1), dtype=b.color_dtype)
+        assert below.size >= b.portal_texels
+        fb, _, _, oob = render(map_data, b, mode=mode, portal=below)
+        oob_total += oob
+        outs[k] = fb
+        below = fb
+    return outs, oob_total
 
 
 def build_surfaces(map_data, batch, mode=MODE_EXACT):

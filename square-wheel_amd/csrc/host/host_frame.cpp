@@ -442,27 +442,13 @@ bool project_portal(const swh_map& m, const Portal& portal, const M4& view, Clip
 	return true;
 }
 
-// map_visibility_calculator.rs:52-284
-void update_visibility(swh_frame& f, const swh_map& m, const Camera& cam, const ClippingPolygon& frame_bounds)
+// mark_reachable_leafs_iterative (map_visibility_calculator.rs:156-267): breadth-first over the
+// leaf graph from the start leaves (each with the frame bounds), through portals facing the camera,
+// each leaf's bounds the union of the projected portals' bounds clipped by the leaf it is seen from.
+void mark_reachable_leafs(swh_frame& f, const swh_map& m, const Camera& cam, const ClippingPolygon& frame_bounds,
+	const uint32_t* start_leafs, uint32_t num_start)
 {
-	const uint32_t current_leaf = find_leaf(m, cam.position);
 	const uint32_t frame = f.frame_number;
-	uint32_t start_leafs[32];
-	uint32_t num_start = 1;
-	start_leafs[0] = current_leaf;
-	const Leaf& leaf_value = m.leafs[current_leaf];
-	for(uint32_t k = 0; k < leaf_value.num_leaf_portals; ++k)
-	{
-		const Portal& p = m.portals[m.leafs_portals[leaf_value.first_leaf_portal + k]];
-		const float scaled_dist = dot(p.plane.vec, cam.position) - p.plane.dist;
-		const float eps = Z_NEAR * 2.0f;
-		if(std::fabs(scaled_dist) <= eps * length(p.plane.vec))
-		{
-			start_leafs[num_start++] = p.leafs[0] == current_leaf ? p.leafs[1] : p.leafs[0];
-			if(num_start == 32)
-				break;
-		}
-	}
 	std::vector<uint32_t> cur, next;
 	for(uint32_t i = 0; i < num_start; ++i)
 	{
@@ -525,6 +511,30 @@ void update_visibility(swh_frame& f, const swh_map& m, const Camera& cam, const 
 		if(++depth > 1024)
 			break;
 	}
+}
+
+// update_visibility (map_visibility_calculator.rs:52-98): the camera's leaf and the leaves behind portals
+// the camera (almost) lies on
+void update_visibility(swh_frame& f, const swh_map& m, const Camera& cam, const ClippingPolygon& frame_bounds)
+{
+	const uint32_t current_leaf = find_leaf(m, cam.position);
+	uint32_t start_leafs[32];
+	uint32_t num_start = 1;
+	start_leafs[0] = current_leaf;
+	const Leaf& leaf_value = m.leafs[current_leaf];
+	for(uint32_t k = 0; k < leaf_value.num_leaf_portals; ++k)
+	{
+		const Portal& p = m.portals[m.leafs_portals[leaf_value.first_leaf_portal + k]];
+		const float scaled_dist = dot(p.plane.vec, cam.position) - p.plane.dist;
+		const float eps = Z_NEAR * 2.0f;
+		if(std::fabs(scaled_dist) <= eps * length(p.plane.vec))
+		{
+			start_leafs[num_start++] = p.leafs[0] == current_leaf ? p.leafs[1] : p.leafs[0];
+			if(num_start == 32)
+				break;
+		}
+	}
+	mark_reachable_leafs(f, m, cam, frame_bounds, start_leafs, num_start);
 	// is_inside_leaf_volume (:269-283)
 	f.inside_leaf_volume = true;
 	for(uint32_t k = 0; k < leaf_value.num_polygons; ++k)
@@ -536,6 +546,15 @@ void update_visibility(swh_frame& f, const swh_map& m, const Camera& cam, const 
 			break;
 		}
 	}
+}
+
+// update_visibility_with_start_leafs (map_visibility_calculator.rs:100-114): portal and mirror views
+// start from the leaves of the portal; whether the camera is inside a leaf's volume is not known
+void update_visibility_with_start_leafs(swh_frame& f, const swh_map& m, const Camera& cam, const ClippingPolygon& frame_bounds,
+	const std::vector<uint32_t>& start_leafs)
+{
+	mark_reachable_leafs(f, m, cam, frame_bounds, start_leafs.data(), uint32_t(start_leafs.size()));
+	f.inside_leaf_volume = true;
 }
 
 // partial_renderer.rs:1464-1638
@@ -677,9 +696,11 @@ void submodel_entity(uint32_t k, float time, V3 centre, V3& position, Quat& rota
 	}
 }
 
-// InlineModelsIndex::position_model_r (inline_models_index.rs:126-157): the leaves the transformed
-// bbox's corners reach.
-void position_model_r(const swh_map& m, const V3 corners[8], uint32_t node_index, std::vector<uint32_t>& leafs)
+// The leaves a convex hull of points reaches: get_convex_hull_bsp_leafs (bsp_map_compact.rs:294-338),
+// which InlineModelsIndex::position_model_r (inline_models_index.rs:126-157) applies to a transformed
+// bbox's corners and DynamicObjectsIndex::position_portals (dynamic_objects_index.rs:144-157) to a
+// portal's vertices.
+void convex_hull_leafs_r(const swh_map& m, const V3* points, uint32_t n, uint32_t node_index, std::vector<uint32_t>& leafs)
 {
 	if(node_index >= FIRST_LEAF_INDEX)
 	{
@@ -688,13 +709,13 @@ void position_model_r(const swh_map& m, const V3 corners[8], uint32_t node_index
 	}
 	const Node& node = m.nodes[node_index];
 	uint32_t front = 0;
-	for(int i = 0; i < 8; ++i)
-		if(dot(node.plane.vec, corners[i]) > node.plane.dist)
+	for(uint32_t i = 0; i < n; ++i)
+		if(dot(node.plane.vec, points[i]) > node.plane.dist)
 			++front;
 	if(front > 0)
-		position_model_r(m, corners, node.children[0], leafs);
-	if(front < 8)
-		position_model_r(m, corners, node.children[1], leafs);
+		convex_hull_leafs_r(m, points, n, node.children[0], leafs);
+	if(front < n)
+		convex_hull_leafs_r(m, points, n, node.children[1], leafs);
 }
 
 void bbox_corners(const BBox& b, V3 out[8])
@@ -753,7 +774,7 @@ void prepare_submodels(swh_frame& f, const swh_map& m, const swh_frame_config& c
 			world.min = v3(std::min(world.min.x, c.x), std::min(world.min.y, c.y), std::min(world.min.z, c.z));
 			world.max = v3(std::max(world.max.x, c.x), std::max(world.max.y, c.y), std::max(world.max.z, c.z));
 		}
-		position_model_r(m, corners, uint32_t(m.nodes.size() - 1), s.leafs);
+		convex_hull_leafs_r(m, corners, 8, uint32_t(m.nodes.size() - 1), s.leafs);
 		for(uint32_t l : s.leafs)
 			f.leaf_submodels[l].push_back(k);
 		for(uint32_t li = 0; li < f.lights.size(); ++li)
@@ -1095,7 +1116,7 @@ void emit_draw_list(swh_frame& f, const swh_map& m, const swh_frame_config& cfg,
 				draw_polygon(f, cp, &f.vertices_transformed[p.first_vertex], p.num_vertices, pd.de, pd.te, pd.size, pd.surface_offset,
 					SW_TEXSRC_SURFACE, m.materials[p.texture].blending_mode, clip_rect);
 			}
-			draw_leaf_objects(f, m, cfg, leaf_index, lb, clip_rect, cam.planes_matrix);
+			draw_leaf_objects(f, m, cfg, leaf_index, lb, clip_rect, cam.view_matrix, cam.planes_matrix);
 		}
 		else
 		{
@@ -1107,6 +1128,294 @@ void emit_draw_list(swh_frame& f, const swh_map& m, const swh_frame_config& cfg,
 		}
 	}
 	draw_view_models(f, m, viewport, clip_rect);
+}
+
+// ---------------------------------------------------------------------------------------------
+// View portals: mirrors, cameras and parallax portals (partial_renderer.rs:1834-2282)
+// ---------------------------------------------------------------------------------------------
+void build_view(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, const Camera& cam,
+	const std::vector<uint32_t>* start_leafs, uint32_t portals_depth);
+
+// prepare_view_portals_textures (partial_renderer.rs:1834-1996): each portal placed into the leaves its
+// polygon reaches (position_portals, dynamic_objects_index.rs:144-157); a portal in a visible leaf that
+// faces the camera and survives clipping by the union of its visible leaves' bounds gets a texture: its
+// texture-space rect (from the world vertices, shrunk by 1/32) at the mip its screen projection asks
+// for, and depth and texture-coordinate equations shifted to that rect. Textures are laid end to end
+// in portal order.
+void prepare_view_portals(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, const Camera& cam)
+{
+	const uint32_t np = uint32_t(m.view_portals.size());
+	f.portal_info.assign(np, swh_frame::PortalInfo{});
+	f.portal_texels = 0;
+	f.leaf_portals.clear();
+	if(np == 0)
+		return;
+	f.leaf_portals.assign(m.leafs.size(), std::vector<uint32_t>());
+	for(uint32_t pi = 0; pi < np; ++pi)
+	{
+		const ViewPortalDef& portal = m.view_portals[pi];
+		swh_frame::PortalInfo& info = f.portal_info[pi];
+		convex_hull_leafs_r(m, portal.vertices, 4, uint32_t(m.nodes.size() - 1), info.leafs);
+		for(uint32_t l : info.leafs)
+			f.leaf_portals[l].push_back(pi);
+	}
+	for(uint32_t pi = 0; pi < np; ++pi)
+	{
+		const ViewPortalDef& portal = m.view_portals[pi];
+		swh_frame::PortalInfo& info = f.portal_info[pi];
+		bool have_bounds = false;
+		ClippingPolygon bounds;
+		for(uint32_t l : info.leafs)
+		{
+			if(f.leaf_visible_frame[l] != f.frame_number)
+				continue;
+			if(have_bounds)
+				bounds.extend(f.leaf_bounds[l]);
+			else
+			{
+				bounds = f.leaf_bounds[l];
+				have_bounds = true;
+			}
+		}
+		if(!have_bounds)
+			continue;
+		V3 clip_planes[8];
+		bounds.clip_planes(clip_planes);
+		const V4 plane_transformed = mul(cam.planes_matrix, extend(portal.plane.vec, -portal.plane.dist));
+		if(plane_transformed.w <= 0.0f)
+			continue;
+		V3 vt[4];
+		for(int k = 0; k < 4; ++k)
+			vt[k] = view_transform(cam.view_matrix, portal.vertices[k]);
+		V2 v2d[MAX_VERTICES];
+		const uint32_t n = project_and_clip_polygon(clip_planes, vt, 4, v2d);
+		if(n < 3)
+			continue;
+		const float inf = float(1 << 29);
+		float tc_min[2] = {inf, inf}, tc_max[2] = {-inf, -inf};
+		for(int k = 0; k < 4; ++k)
+			for(int i = 0; i < 2; ++i)
+			{
+				const float tc = dot(portal.tex_coord_equation[i].vec, portal.vertices[k]) + portal.tex_coord_equation[i].dist;
+				if(tc < tc_min[i])
+					tc_min[i] = tc;
+				if(tc > tc_max[i])
+					tc_max[i] = tc;
+			}
+		for(int i = 0; i < 2; ++i)
+		{
+			tc_min[i] += 1.0f / 32.0f;
+			tc_max[i] -= 1.0f / 32.0f;
+		}
+		const DepthEq de = depth_from_plane(plane_transformed);
+		const V4 tcb[2] = {mul(cam.planes_matrix, extend(portal.tex_coord_equation[0].vec, portal.tex_coord_equation[0].dist)),
+			mul(cam.planes_matrix, extend(portal.tex_coord_equation[1].vec, portal.tex_coord_equation[1].dist))};
+		const TcEq te = tc_from_depth_and_planes(de, tcb);
+		const uint32_t mip = calculate_mip(v2d, n, de, te, cfg.textures_mip_bias);
+		const float mip_scale = 1.0f / float(1 << mip);
+		TcEq tes = scale(te, mip_scale);
+		int32_t tmin[2], tmax[2];
+		for(int i = 0; i < 2; ++i)
+		{
+			tmin[i] = saturating_f32_to_i32(std::floor(tc_min[i]) * mip_scale);
+			tmax[i] = std::max(saturating_f32_to_i32(std::ceil(tc_max[i]) * mip_scale), tmin[i] + 1);
+			const float t = float(tmin[i]);
+			tes.dx[i] -= t * de.dx;
+			tes.dy[i] -= t * de.dy;
+			tes.k[i] -= t * de.k;
+		}
+		info.visible = true;
+		info.resolution[0] = uint32_t(tmax[0] - tmin[0]);
+		info.resolution[1] = uint32_t(tmax[1] - tmin[1]);
+		info.texture_offset = f.portal_texels;
+		info.de = de;
+		info.te = tes;
+		info.tc_min[0] = tmin[0];
+		info.tc_min[1] = tmin[1];
+		info.mip = mip;
+		f.portal_texels += uint64_t(info.resolution[0]) * info.resolution[1];
+	}
+}
+
+// The view matrix of a mirror or a parallax portal (partial_renderer.rs:2071-2176): world -> the
+// portal's texture space at its mip, shifted to its rect, scaled so that the portal plane lies at
+// Z_NEAR (all geometry on the near side is clipped).
+M4 portal_view_matrix(const ViewPortalDef& portal, const swh_frame::PortalInfo& info, V3 eye, float dist_normalized)
+{
+	const float normal_len = length(portal.plane.vec);
+	const float mip_scale = 1.0f / float(1 << info.mip);
+	M4 tc_basis;
+	const Plane* e = portal.tex_coord_equation;
+	tc_basis.c[0] = v4(e[0].vec.x * mip_scale, e[0].vec.y * mip_scale, e[0].vec.z * mip_scale, e[0].dist * mip_scale);
+	tc_basis.c[1] = v4(e[1].vec.x * mip_scale, e[1].vec.y * mip_scale, e[1].vec.z * mip_scale, e[1].dist * mip_scale);
+	const float ns = mip_scale / normal_len;
+	tc_basis.c[2] = v4(portal.plane.vec.x * ns, portal.plane.vec.y * ns, portal.plane.vec.z * ns, -portal.plane.dist * ns);
+	tc_basis.c[3] = v4(0.0f, 0.0f, 0.0f, 1.0f);
+	tc_basis = transpose(tc_basis);
+	const V3 eye_tc = truncate(mul(tc_basis, extend(eye, 1.0f)));
+	const M4 translate = from_translation(v3(-eye_tc.x, -eye_tc.y, -eye_tc.z));
+	M4 shift_to_viewport = identity();
+	shift_to_viewport.c[2].x = eye_tc.x - float(info.tc_min[0]);
+	shift_to_viewport.c[2].y = eye_tc.y - float(info.tc_min[1]);
+	const M4 z_scale = from_nonuniform_scale(Z_NEAR, Z_NEAR, Z_NEAR / (dist_normalized * mip_scale));
+	return mul(mul(mul(shift_to_viewport, z_scale), translate), tc_basis);
+}
+
+Camera camera_from_view_matrix(V3 position, const M4& view)
+{
+	Camera c;
+	c.position = position;
+	c.view_matrix = view;
+	if(!invert(transpose(view), c.planes_matrix))
+		c.planes_matrix = identity();
+	return c;
+}
+
+// build_view_portals_textures (partial_renderer.rs:2008-2200): every visible portal's view, rendered
+// into its texture by the next level's renderer (a third-person view: no view models). Below the
+// last level the textures stay black (zero).
+void build_portal_views(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, const Camera& cam, uint32_t depth)
+{
+	f.num_portal_views = 0;
+	if(depth == 0)
+		return;
+	for(uint32_t pi = 0; pi < m.view_portals.size(); ++pi)
+	{
+		const swh_frame::PortalInfo& info = f.portal_info[pi];
+		if(!info.visible)
+			continue;
+		const ViewPortalDef& portal = m.view_portals[pi];
+		const float rw = float(info.resolution[0]), rh = float(info.resolution[1]);
+		Camera pc;
+		std::vector<uint32_t> transformed_leafs;
+		const std::vector<uint32_t>* start = nullptr;
+		if(portal.kind == PORTAL_CAMERA)
+			pc = build_camera(portal.camera_position, portal.camera_yaw, portal.camera_pitch, portal.camera_fov, rw, rh);
+		else if(portal.kind == PORTAL_MIRROR)
+		{
+			const float normal_square_len = dot(portal.plane.vec, portal.plane.vec);
+			const float dist = dot(portal.plane.vec, cam.position) - portal.plane.dist;
+			const V3 reflected = cam.position - portal.plane.vec * (2.0f * dist / normal_square_len);
+			const float dist_normalized = dist / std::sqrt(normal_square_len);
+			pc = camera_from_view_matrix(reflected, portal_view_matrix(portal, info, reflected, dist_normalized));
+			start = &info.leafs; // the leaves the mirror lies in
+		}
+		else
+		{
+			V3 tv[4];
+			for(int k = 0; k < 4; ++k)
+				tv[k] = truncate(mul(portal.transform, extend(portal.vertices[k], 1.0f)));
+			convex_hull_leafs_r(m, tv, 4, uint32_t(m.nodes.size() - 1), transformed_leafs);
+			const float dist_normalized = (portal.plane.dist - dot(portal.plane.vec, cam.position)) / length(portal.plane.vec);
+			M4 inv_transform;
+			if(!invert(portal.transform, inv_transform))
+				inv_transform = identity();
+			const M4 view = mul(portal_view_matrix(portal, info, cam.position, dist_normalized), inv_transform);
+			pc = camera_from_view_matrix(truncate(mul(portal.transform, extend(cam.position, 1.0f))), view);
+			start = &transformed_leafs;
+		}
+		if(f.portal_views.size() <= f.num_portal_views)
+			f.portal_views.emplace_back(new swh_frame());
+		swh_frame& child = *f.portal_views[f.num_portal_views++];
+		swh_frame_config cc = cfg;
+		cc.width = info.resolution[0];
+		cc.height = info.resolution[1];
+		cc.view_model = 0;
+		build_view(child, m, cc, pc, start, depth - 1);
+	}
+}
+
+// prepare_frame + draw-list emission of one view (partial_renderer.rs:156-233, 1192-1283): visibility
+// from the camera's leaf or from the given start leaves, lights, surfaces, inline models, models and
+// decals, the portals and the views through them, then the draw list.
+void build_view(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, const Camera& cam,
+	const std::vector<uint32_t>* start_leafs, uint32_t portals_depth)
+{
+	ensure_basis(f, m);
+	f.frame_number++;
+	f.visible_polygons.clear();
+	f.surfaces.clear();
+	f.lights.clear();
+	f.light_indices.clear();
+	f.draws.clear();
+	f.vdata.clear();
+	f.have_sky = false;
+	f.num_triangles = 0;
+	const float W = float(cfg.width), H = float(cfg.height);
+	const ClippingPolygon frame_bounds = ClippingPolygon::from_box(0.0f, 0.0f, W, H);
+	if(start_leafs)
+		update_visibility_with_start_leafs(f, m, cam, frame_bounds, *start_leafs);
+	else
+		update_visibility(f, m, cam, frame_bounds);
+
+	// dynamic lights and the leaves they touch
+	build_dynamic_lights(f, m, cfg);
+
+	// surfaces (partial_renderer.rs:1327-1452), leaf order
+	uint64_t offset = 0;
+	f.num_visible_leafs = 0;
+	for(uint32_t li = 0; li < m.leafs.size(); ++li)
+	{
+		if(f.leaf_visible_frame[li] != f.frame_number)
+			continue;
+		f.num_visible_leafs++;
+		V3 cp[8];
+		f.leaf_bounds[li].clip_planes(cp);
+		const Leaf& leaf = m.leafs[li];
+		for(uint32_t k = 0; k < leaf.num_polygons; ++k)
+			prepare_polygon_surface(f, m, cam, cp, offset, leaf.first_polygon + k, cfg.textures_mip_bias);
+	}
+	prepare_submodels(f, m, cfg, cam);
+	f.first_submodel_surface = uint32_t(f.visible_polygons.size());
+	prepare_submodel_surfaces(f, m, cfg, cam, offset);
+	f.arena_texels = offset;
+	for(uint32_t pi : f.visible_polygons)
+		emit_surface(f, m, cfg, pi);
+
+	// models and decals: animation, lighting, projection (per frame, before the tree walk)
+	prepare_models(f, m, cfg, cam.view_matrix, cam.planes_matrix);
+	prepare_decals(f, m, cfg, cam.planes_matrix);
+
+	// portals and the views through them (after the surfaces, :229-233)
+	prepare_view_portals(f, m, cfg, cam);
+	build_portal_views(f, m, cfg, cam, portals_depth);
+
+	// draw list: perform_rasterization (partial_renderer.rs:1192-1283). One thread: the whole screen
+	// as clip rect and the screen box as viewport polygon. N threads: split_rect's N rects, each with
+	// its integer clip rect and a viewport polygon widened by half a pixel; every rect gets the whole
+	// traversal, so the draw list holds N runs of draws whose clip rects partition the screen.
+	const uint32_t nthreads = cfg.virtual_threads <= 1 ? 1u : std::min<uint32_t>(cfg.virtual_threads, 64u);
+	if(nthreads == 1)
+	{
+		const int32_t clip_rect[4] = {0, 0, int32_t(cfg.width), int32_t(cfg.height)};
+		emit_draw_list(f, m, cfg, cam, frame_bounds, clip_rect);
+	}
+	else
+	{
+		std::vector<ScreenRect> rects;
+		split_rect(ScreenRect{0.0f, 0.0f, W, H}, nthreads, rects);
+		for(const ScreenRect& r : rects)
+		{
+			// `as i32` of the float bounds (:1250-1259), the polygon extended by 0.5 (:1261-1272)
+			const int32_t clip_rect[4] = {int32_t(r.min_x), int32_t(r.min_y), int32_t(r.max_x), int32_t(r.max_y)};
+			const ClippingPolygon viewport =
+				ClippingPolygon::from_box(r.min_x - 0.5f, r.min_y - 0.5f, r.max_x + 0.5f, r.max_y + 0.5f);
+			emit_draw_list(f, m, cfg, cam, viewport, clip_rect);
+		}
+	}
+
+	sw_view& v = f.view;
+	std::memset(&v, 0, sizeof(v));
+	v.width = cfg.width;
+	v.height = cfg.height;
+	v.cam_pos[0] = cam.position.x;
+	v.cam_pos[1] = cam.position.y;
+	v.cam_pos[2] = cam.position.z;
+	v.clear_background = (cfg.force_clear || (cfg.clear_background && !f.inside_leaf_volume)) ? 1u : 0u;
+	// ColorVec::from_color_f32x3(&[32.0, 16.0, 8.0]).into() (partial_renderer.rs:255)
+	v.background_color = cfg.color_bits == 64 ? 0x0000002000100008ull : 0x00201008ull;
+	v.num_surfaces = uint32_t(f.surfaces.size());
+	v.num_draws = uint32_t(f.draws.size());
 }
 
 } // namespace
@@ -1127,85 +1436,8 @@ int swh_frame_build(swh_frame* f, const swh_map* m, const swh_frame_config* cfg,
 {
 	if(!f || !m || !cfg || !cam_pos || cfg->width == 0 || cfg->height == 0 || (cfg->color_bits != 32 && cfg->color_bits != 64))
 		return -1;
-	ensure_basis(*f, *m);
-	f->frame_number++;
-	f->visible_polygons.clear();
-	f->surfaces.clear();
-	f->lights.clear();
-	f->light_indices.clear();
-	f->draws.clear();
-	f->vdata.clear();
-	f->have_sky = false;
-	f->num_triangles = 0;
-	const float W = float(cfg->width), H = float(cfg->height);
-	const Camera cam = build_camera(v3(cam_pos[0], cam_pos[1], cam_pos[2]), yaw, pitch, cfg->fov, W, H);
-	const ClippingPolygon frame_bounds = ClippingPolygon::from_box(0.0f, 0.0f, W, H);
-	update_visibility(*f, *m, cam, frame_bounds);
-
-	// dynamic lights and the leaves they touch
-	build_dynamic_lights(*f, *m, *cfg);
-
-	// surfaces (partial_renderer.rs:1327-1452), leaf order
-	uint64_t offset = 0;
-	f->num_visible_leafs = 0;
-	for(uint32_t li = 0; li < m->leafs.size(); ++li)
-	{
-		if(f->leaf_visible_frame[li] != f->frame_number)
-			continue;
-		f->num_visible_leafs++;
-		V3 cp[8];
-		f->leaf_bounds[li].clip_planes(cp);
-		const Leaf& leaf = m->leafs[li];
-		for(uint32_t k = 0; k < leaf.num_polygons; ++k)
-			prepare_polygon_surface(*f, *m, cam, cp, offset, leaf.first_polygon + k, cfg->textures_mip_bias);
-	}
-	prepare_submodels(*f, *m, *cfg, cam);
-	f->first_submodel_surface = uint32_t(f->visible_polygons.size());
-	prepare_submodel_surfaces(*f, *m, *cfg, cam, offset);
-	f->arena_texels = offset;
-	for(uint32_t pi : f->visible_polygons)
-		emit_surface(*f, *m, *cfg, pi);
-
-	// models and decals: animation, lighting, projection (per frame, before the tree walk)
-	prepare_models(*f, *m, *cfg, cam.view_matrix, cam.planes_matrix);
-	prepare_decals(*f, *m, *cfg, cam.planes_matrix);
-
-	// draw list: perform_rasterization (partial_renderer.rs:1192-1283). One thread: the whole screen
-	// as clip rect and the screen box as viewport polygon. N threads: split_rect's N rects, each with
-	// its integer clip rect and a viewport polygon widened by half a pixel; every rect gets the whole
-	// traversal, so the draw list holds N runs of draws whose clip rects partition the screen.
-	const uint32_t nthreads = cfg->virtual_threads <= 1 ? 1u : std::min<uint32_t>(cfg->virtual_threads, 64u);
-	if(nthreads == 1)
-	{
-		const int32_t clip_rect[4] = {0, 0, int32_t(cfg->width), int32_t(cfg->height)};
-		emit_draw_list(*f, *m, *cfg, cam, frame_bounds, clip_rect);
-	}
-	else
-	{
-		std::vector<ScreenRect> rects;
-		split_rect(ScreenRect{0.0f, 0.0f, W, H}, nthreads, rects);
-		for(const ScreenRect& r : rects)
-		{
-			// `as i32` of the float bounds (:1250-1259), the polygon extended by 0.5 (:1261-1272)
-			const int32_t clip_rect[4] = {int32_t(r.min_x), int32_t(r.min_y), int32_t(r.max_x), int32_t(r.max_y)};
-			const ClippingPolygon viewport =
-				ClippingPolygon::from_box(r.min_x - 0.5f, r.min_y - 0.5f, r.max_x + 0.5f, r.max_y + 0.5f);
-			emit_draw_list(*f, *m, *cfg, cam, viewport, clip_rect);
-		}
-	}
-
-	sw_view& v = f->view;
-	std::memset(&v, 0, sizeof(v));
-	v.width = cfg->width;
-	v.height = cfg->height;
-	v.cam_pos[0] = cam_pos[0];
-	v.cam_pos[1] = cam_pos[1];
-	v.cam_pos[2] = cam_pos[2];
-	v.clear_background = (cfg->force_clear || (cfg->clear_background && !f->inside_leaf_volume)) ? 1u : 0u;
-	// ColorVec::from_color_f32x3(&[32.0, 16.0, 8.0]).into() (partial_renderer.rs:255)
-	v.background_color = cfg->color_bits == 64 ? 0x0000002000100008ull : 0x00201008ull;
-	v.num_surfaces = uint32_t(f->surfaces.size());
-	v.num_draws = uint32_t(f->draws.size());
+	const Camera cam = build_camera(v3(cam_pos[0], cam_pos[1], cam_pos[2]), yaw, pitch, cfg->fov, float(cfg->width), float(cfg->height));
+	build_view(*f, *m, *cfg, cam, nullptr, std::min<uint32_t>(cfg->portals_depth, 4u));
 	return 0;
 }
 
@@ -1228,6 +1460,35 @@ int swh_frame_get(const swh_frame* f, swh_frame_data* out)
 	out->num_visible_leafs = f->num_visible_leafs;
 	out->num_triangles = f->num_triangles;
 	out->first_submodel_surface = f->first_submodel_surface;
+	out->num_portal_texels = f->portal_texels;
+	out->num_portal_views = f->num_portal_views;
+	out->num_visible_portals = 0;
+	for(const swh_frame::PortalInfo& p : f->portal_info)
+		out->num_visible_portals += p.visible ? 1u : 0u;
+	return 0;
+}
+
+int swh_frame_portal_info(const swh_frame* f, uint32_t index, swh_portal_info* out)
+{
+	if(!f || !out || index >= f->portal_info.size())
+		return -1;
+	const swh_frame::PortalInfo& p = f->portal_info[index];
+	std::memset(out, 0, sizeof(*out));
+	out->visible = p.visible ? 1u : 0u;
+	out->resolution[0] = p.resolution[0];
+	out->resolution[1] = p.resolution[1];
+	out->tc_min[0] = p.tc_min[0];
+	out->tc_min[1] = p.tc_min[1];
+	out->mip = p.mip;
+	out->texture_offset = p.texture_offset;
+	return 0;
+}
+
+int swh_frame_portal_view(const swh_frame* f, uint32_t index, const swh_frame** out)
+{
+	if(!f || !out || index >= f->num_portal_views)
+		return -1;
+	*out = f->portal_views[index].get();
 	return 0;
 }
 

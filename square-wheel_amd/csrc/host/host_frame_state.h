@@ -1,6 +1,8 @@
 // Frame state shared by the front-end's translation units.
 #pragma once
 
+#include <memory>
+
 #include "host_internal.h"
 #include "host_models.h"
 
@@ -126,7 +128,7 @@ void draw_polygon(swh_frame& f, const V3 clip_planes[8], const V3* verts, uint32
 // Decals of the leaf's polygons, then the leaf's models, after the leaf's polygons (draw_leaf,
 // partial_renderer.rs:2519-2792). `bounds` is the leaf's screen ClippingPolygon.
 void draw_leaf_objects(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, uint32_t leaf, const ClippingPolygon& bounds,
-	const int32_t clip_rect[4], const M4& planes);
+	const int32_t clip_rect[4], const M4& view, const M4& planes);
 void draw_view_models(swh_frame& f, const swh_map& m, const ClippingPolygon& viewport, const int32_t clip_rect[4]);
 
 } // namespace swh
@@ -205,6 +207,26 @@ struct swh_frame
 	};
 	std::vector<SubmodelFrame> submodels;
 	std::vector<std::vector<uint32_t>> leaf_submodels; // per leaf: indices into `submodels`
+
+	// view portals of the frame (PortalInfo, partial_renderer.rs:1834-1997)
+	struct PortalInfo
+	{
+		bool visible = false; // resolution > 0
+		uint32_t resolution[2] = {0, 0};
+		uint64_t texture_offset = 0; // into the frame's portal textures (ColorT texels)
+		DepthEq de;
+		TcEq te;
+		int32_t tc_min[2] = {0, 0};
+		uint32_t mip = 0;
+		std::vector<uint32_t> leafs; // the leaves the portal polygon's convex hull reaches
+	};
+	std::vector<PortalInfo> portal_info;
+	std::vector<std::vector<uint32_t>> leaf_portals; // per leaf: portal indices
+	uint64_t portal_texels = 0; // sum of the portal textures' areas
+	// the views seen through the visible portals, one frame each, in portal order (when the portals
+	// depth allows another level): their framebuffers laid end to end are the portal textures
+	std::vector<std::unique_ptr<swh_frame>> portal_views;
+	uint32_t num_portal_views = 0;
 };
 
 // Shadow-map draw lists under construction: the visibility state of the map (leaf bounds) reused

@@ -109,6 +109,29 @@ struct SubmodelNode
 
 const uint32_t NO_POLYGON_LEAF = 0xFFFFFFFFu;
 
+// A view portal (frame_info.rs:171-196 ViewPortal): a polygon showing another view, placed by the
+// game in the reference and seeded with the map here. Mirrors reflect the camera in the portal's
+// plane; a camera portal shows a camera at a fixed position; a parallax portal moves the camera by
+// a fixed transform.
+enum : uint32_t
+{
+	PORTAL_MIRROR = 0,
+	PORTAL_CAMERA = 1,
+	PORTAL_PARALLAX = 2,
+};
+
+struct ViewPortalDef
+{
+	uint32_t kind;
+	Plane plane; // facing the side the portal is seen from
+	Plane tex_coord_equation[2];
+	V3 vertices[4];
+	uint32_t blending_mode;
+	V3 camera_position; // PORTAL_CAMERA
+	float camera_yaw, camera_pitch, camera_fov;
+	M4 transform; // PORTAL_PARALLAX: camera position -> the portal's view point
+};
+
 struct Portal
 {
 	uint32_t leafs[2];
@@ -248,6 +271,7 @@ struct swh_map
 	std::vector<swh::Submodel> submodels;
 	std::vector<swh::SubmodelNode> submodel_nodes;
 	std::vector<uint32_t> submodel_cell; // the empty cell each generated submodel stands in
+	std::vector<swh::ViewPortalDef> view_portals;
 
 	std::vector<float> lm_simple; // 3 floats per element
 	std::vector<sw_directional_lightmap_element> lm_dir;

@@ -830,6 +830,83 @@ void generate_submodels(swh_map& m, uint32_t count, uint32_t seed, uint32_t nmat
 	}
 }
 
+// View portals on walls: in empty cells with a solid neighbour across x or y, a 64 x 80 quad one unit
+// in front of the wall, facing into the cell, texture coordinates one per unit along the wall and
+// down. Mirrors, camera portals (a camera in another empty cell, looking in a seeded direction) and
+// parallax portals (the view point moved by the offset to another empty cell) alternate.
+void generate_view_portals(swh_map& m, uint32_t count, uint32_t seed)
+{
+	if(count == 0)
+		return;
+	Rng rng(uint64_t(seed) * 0x94D049BB133111EBull + 91);
+	std::vector<std::pair<uint32_t, int>> walls;
+	for(uint32_t c : m.empty_cells)
+	{
+		const uint32_t x = c % m.gx, y = (c / m.gx) % m.gy, z = c / (m.gx * m.gy);
+		for(int dir = 0; dir < 4; ++dir)
+		{
+			const int nx = int(x) + DIRS[dir][0], ny = int(y) + DIRS[dir][1];
+			if(m.cell[cell_index(m, uint32_t(nx), uint32_t(ny), z)] == 1)
+				walls.push_back({c, dir});
+		}
+	}
+	const std::vector<std::pair<uint32_t, int>> all_walls = walls;
+	for(uint32_t k = 0; k < count && !walls.empty(); ++k)
+	{
+		const uint32_t pick = rng.below(uint32_t(walls.size()));
+		const uint32_t c = walls[pick].first;
+		const int dir = walls[pick].second;
+		walls.erase(walls.begin() + pick);
+		const uint32_t x = c % m.gx, y = (c / m.gx) % m.gy, z = c / (m.gx * m.gy);
+		ViewPortalDef p{};
+		p.kind = k % 3;
+		p.blending_mode = SW_BLEND_NONE;
+		const V3 normal = v3(-float(DIRS[dir][0]), -float(DIRS[dir][1]), 0.0f); // into the cell
+		const V3 centre = v3(CELL * (float(x) + 0.5f), CELL * (float(y) + 0.5f), CELL * float(z) + 64.0f);
+		const V3 on_wall = centre - normal * (CELL * 0.5f - 1.0f);
+		// the screen's right for a viewer facing the wall (cameras map world +z to screen up)
+		const V3 right = cross(normal * -1.0f, v3(0.0f, 0.0f, 1.0f));
+		V3 q[4] = {on_wall - right * 32.0f - v3(0, 0, 40.0f), on_wall + right * 32.0f - v3(0, 0, 40.0f),
+			on_wall + right * 32.0f + v3(0, 0, 40.0f), on_wall - right * 32.0f + v3(0, 0, 40.0f)};
+		// clockwise on screen seen from the front (the rasterizer's order): cross(e1, e2) points away
+		// from the normal
+		if(dot(cross(q[1] - q[0], q[2] - q[1]), normal) > 0.0f)
+			std::swap(q[1], q[3]);
+		for(int i = 0; i < 4; ++i)
+			p.vertices[i] = q[i];
+		p.plane = Plane{normal, dot(normal, q[0])};
+		// Two texels per unit, (0, 0) at the top-left corner seen from the front. A mirror's view
+		// matrix maps (u, v, distance along the normal) to the screen, so with u to the right the
+		// mirrored scene comes out counter-clockwise and nothing is rasterized: mirrors take u to
+		// the left, which shows the same reflection. Camera and parallax views keep u to the right.
+		const V3 u = p.kind == PORTAL_MIRROR ? right * -2.0f : right * 2.0f;
+		p.tex_coord_equation[0] = Plane{u, -dot(u, on_wall) + 64.0f};
+		p.tex_coord_equation[1] = Plane{v3(0, 0, -2.0f), 2.0f * (on_wall.z + 40.0f)};
+		const uint32_t other = m.empty_cells[rng.below(uint32_t(m.empty_cells.size()))];
+		const uint32_t ox = other % m.gx, oy = (other / m.gx) % m.gy, oz = other / (m.gx * m.gy);
+		p.camera_position = v3(CELL * (float(ox) + 0.5f), CELL * (float(oy) + 0.5f), CELL * float(oz) + 64.0f);
+		p.camera_yaw = rng.uniform(0.0f, 6.2831853f);
+		p.camera_pitch = rng.uniform(-0.3f, 0.3f);
+		p.camera_fov = 1.5707963f;
+		// parallax: the view from the same spot in front of another wall facing the same way
+		uint32_t target = c;
+		const uint32_t first = rng.below(uint32_t(all_walls.size()));
+		for(size_t i = 0; i < all_walls.size(); ++i)
+		{
+			const auto& w = all_walls[(first + i) % all_walls.size()];
+			if(w.second == dir && w.first != c)
+			{
+				target = w.first;
+				break;
+			}
+		}
+		const uint32_t tx = target % m.gx, ty = (target / m.gx) % m.gy, tz = target / (m.gx * m.gy);
+		p.transform =
+			from_translation(v3(CELL * (float(tx) - float(x)), CELL * (float(ty) - float(y)), CELL * (float(tz) - float(z))));
+		m.view_portals.push_back(p);
+	}
+}
+
 void light_polygons(swh_map& m)
 {
 	const bool directional = m.config.directional_lightmaps != 0;
@@ -1193,6 +1270,7 @@ int swh_map_create(const swh_map_config* cfg, swh_map** out)
 		m->sun_intensity = 0.8f;
 	}
 	generate_submodels(*m, cfg->num_submodels, cfg->seed, nmat, cfg->texture_size);
+	generate_view_portals(*m, cfg->num_view_portals, cfg->seed);
 	light_polygons(*m);
 	build_light_grid(*m);
 	build_model_mesh(*m, cfg->seed, 7, 8, 28.0f);
@@ -1236,6 +1314,42 @@ int swh_map_info_get(const swh_map* m, swh_map_info* info)
 	info->num_texels = m->texels.size();
 	info->num_lightmap_elements = m->lm_simple.size() / 3;
 	return 0;
+}
+
+int swh_map_view_portals(const swh_map* m, swh_view_portal* out, uint32_t capacity)
+{
+	if(!m)
+		return -1;
+	const uint32_t n = uint32_t(m->view_portals.size());
+	if(!out)
+		return int(n);
+	const uint32_t k = std::min(n, capacity);
+	for(uint32_t i = 0; i < k; ++i)
+	{
+		const ViewPortalDef& p = m->view_portals[i];
+		swh_view_portal& o = out[i];
+		std::memset(&o, 0, sizeof(o));
+		o.kind = p.kind;
+		o.blending_mode = p.blending_mode;
+		o.plane[0] = p.plane.vec.x, o.plane[1] = p.plane.vec.y, o.plane[2] = p.plane.vec.z, o.plane[3] = p.plane.dist;
+		for(int e = 0; e < 2; ++e)
+		{
+			const Plane& t = p.tex_coord_equation[e];
+			o.tex_coord_equation[e][0] = t.vec.x, o.tex_coord_equation[e][1] = t.vec.y;
+			o.tex_coord_equation[e][2] = t.vec.z, o.tex_coord_equation[e][3] = t.dist;
+		}
+		for(int v = 0; v < 4; ++v)
+			o.vertices[v][0] = p.vertices[v].x, o.vertices[v][1] = p.vertices[v].y, o.vertices[v][2] = p.vertices[v].z;
+		o.camera_position[0] = p.camera_position.x, o.camera_position[1] = p.camera_position.y;
+		o.camera_position[2] = p.camera_position.z;
+		o.camera_yaw = p.camera_yaw, o.camera_pitch = p.camera_pitch, o.camera_fov = p.camera_fov;
+		for(int c = 0; c < 4; ++c)
+		{
+			o.transform[c * 4 + 0] = p.transform.c[c].x, o.transform[c * 4 + 1] = p.transform.c[c].y;
+			o.transform[c * 4 + 2] = p.transform.c[c].z, o.transform[c * 4 + 3] = p.transform.c[c].w;
+		}
+	}
+	return int(k);
 }
 
 int swh_camera_path(const swh_map* m, uint32_t seed, uint32_t n, float* out)

@@ -1187,8 +1187,34 @@ void draw_view_models(swh_frame& f, const swh_map& m, const ClippingPolygon& vie
 		draw_mesh(f, m, viewport, nullptr, 0, f.models.visible[vi], clip_rect);
 }
 
+// draw_view_portal (partial_renderer.rs:3720-3776): the portal polygon in camera space, clipped by the
+// leaf's planes, drawn with its portal texture (the view seen through it) when it is visible
+void draw_view_portal(swh_frame& f, const swh_map& m, const V3 clip_planes[8], const Plane* leaf_planes, uint32_t num_leaf_planes,
+	const M4& view, uint32_t pi, const int32_t clip_rect[4])
+{
+	const swh_frame::PortalInfo& info = f.portal_info[pi];
+	if(!info.visible)
+		return;
+	const ViewPortalDef& portal = m.view_portals[pi];
+	V3 a[MAX_VERTICES], b[MAX_VERTICES];
+	uint32_t n = 4;
+	for(uint32_t k = 0; k < n; ++k)
+		a[k] = view_transform(view, portal.vertices[k]);
+	V3* src = a;
+	V3* dst = b;
+	for(uint32_t i = 0; i < num_leaf_planes; ++i)
+	{
+		n = clip_polygon_by_plane(src, n, leaf_planes[i], dst, MAX_VERTICES);
+		if(n < 3)
+			return;
+		std::swap(src, dst);
+	}
+	draw_polygon(f, clip_planes, src, n, info.de, info.te, info.resolution, info.texture_offset, SW_TEXSRC_PORTAL,
+		portal.blending_mode, clip_rect);
+}
+
 void draw_leaf_objects(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, uint32_t leaf, const ClippingPolygon& bounds,
-	const int32_t clip_rect[4], const M4& planes)
+	const int32_t clip_rect[4], const M4& view, const M4& planes)
 {
 	ModelFrameData& md = f.models;
 	V3 clip_planes[8];
@@ -1200,7 +1226,8 @@ void draw_leaf_objects(swh_frame& f, const swh_map& m, const swh_frame_config& c
 	static const std::vector<uint32_t> no_objects;
 	const std::vector<uint32_t>& submodels = f.leaf_submodels.empty() ? no_objects : f.leaf_submodels[leaf];
 	const std::vector<uint32_t>& meshes = md.leaf_models.empty() ? no_objects : md.leaf_models[leaf];
-	const size_t total = submodels.size() + meshes.size();
+	const std::vector<uint32_t>& portals = f.leaf_portals.empty() ? no_objects : f.leaf_portals[leaf];
+	const size_t total = submodels.size() + meshes.size() + portals.size();
 	if(total == 0)
 		return;
 	// the leaf's volume (a grid cell) as six inward planes, in world and camera space
@@ -1211,9 +1238,12 @@ void draw_leaf_objects(swh_frame& f, const swh_map& m, const swh_frame_config& c
 	for(int i = 0; i < 6; ++i)
 		cam[i] = project_plane(world[i], planes);
 	const std::vector<uint32_t>* leaf_decals = md.leaf_decals.empty() ? nullptr : &md.leaf_decals[leaf];
-	const uint32_t SUBMODEL_ADD = 65536, MESH_ADD = SUBMODEL_ADD + 65536;
+	// the reference's object ids (:2663-2667); sprites are not part of this build
+	const uint32_t SUBMODEL_ADD = 65536, MESH_ADD = SUBMODEL_ADD + 65536, PORTAL_ADD = MESH_ADD + 2 * 65536;
 	auto draw_object = [&](uint32_t id) {
-		if(id >= MESH_ADD)
+		if(id >= PORTAL_ADD)
+			draw_view_portal(f, m, clip_planes, cam, 6, view, id - PORTAL_ADD, clip_rect);
+		else if(id >= MESH_ADD)
 			draw_mesh(f, m, bounds, cam, 6, md.visible[id - MESH_ADD], clip_rect);
 		else
 		{
@@ -1226,7 +1256,7 @@ void draw_leaf_objects(swh_frame& f, const swh_map& m, const swh_frame_config& c
 	};
 	if(total == 1)
 	{
-		draw_object(submodels.empty() ? MESH_ADD + meshes[0] : SUBMODEL_ADD + submodels[0]);
+		draw_object(!submodels.empty() ? SUBMODEL_ADD + submodels[0] : !meshes.empty() ? MESH_ADD + meshes[0] : PORTAL_ADD + portals[0]);
 		return;
 	}
 	// several objects: the reference's projected-box order (draw_leaf, :2646-2714)
@@ -1245,6 +1275,19 @@ void draw_leaf_objects(swh_frame& f, const swh_map& m, const swh_frame_config& c
 	}
 	for(uint32_t vi : meshes)
 		objs.push_back({MESH_ADD + vi, project_bbox(md.visible[vi].bbox, md.visible[vi].view, md.visible[vi].planes)});
+	for(uint32_t pi : portals)
+	{
+		// every portal placed in the leaf takes part in the ordering, visible or not (:2724-2737)
+		const ViewPortalDef& portal = m.view_portals[pi];
+		BBox pb{portal.vertices[0], portal.vertices[0]};
+		for(int k = 1; k < 4; ++k)
+		{
+			const V3 v = portal.vertices[k];
+			pb.min = v3(std::fmin(pb.min.x, v.x), std::fmin(pb.min.y, v.y), std::fmin(pb.min.z, v.z));
+			pb.max = v3(std::fmax(pb.max.x, v.x), std::fmax(pb.max.y, v.y), std::fmax(pb.max.z, v.z));
+		}
+		objs.push_back({PORTAL_ADD + pi, project_bbox(pb, view, planes)});
+	}
 	order_objects(objs);
 	for(const auto& o : objs)
 		draw_object(o.first);

@@ -143,7 +143,8 @@ enum
 {
 	DRAW_STAT_TRIANGLE = 1u << 0,
 	DRAW_STAT_READS_DST = 1u << 1, // the blending mode reads the destination
-	DRAW_STAT_SKYBOX = 1u << 2, // a polygon textured from the skybox (else the surface arena)
+	DRAW_STAT_SKYBOX = 1u << 2, // a polygon textured from the skybox
+	DRAW_STAT_PORTAL = 1u << 3, // a polygon textured from the portal textures (else the surface arena)
 	DRAW_STAT_KIND_SHIFT = 4, // 2 bits: 0 perspective, 1 line z, 2 affine, 3 triangle
 	DRAW_STAT_BLEND_SHIFT = 8, // 4 bits: SW_BLEND_*
 };
@@ -158,6 +159,8 @@ struct MapPtrs
 	uint32_t num_levels;
 	uint32_t pad;
 	uint64_t num_texels, num_lite, num_lm_simple, num_lm_dir;
+	const void* portal; // ColorT texels of SW_TEXSRC_PORTAL draws (set per launch)
+	uint64_t num_portal;
 };
 
 // Arguments of the draw setup blocks.
@@ -176,7 +179,7 @@ struct SetupArgs
 	uint32_t* host_err; // HOST_ERR_WORDS error words in host memory
 	uint64_t span_capacity;
 	uint64_t span_stride; // > 0: draw i owns span records [i * stride, (i + 1) * stride), no allocation
-	uint64_t arena_texels, skybox_texels, lite_texels;
+	uint64_t arena_texels, skybox_texels, lite_texels, portal_texels;
 };
 
 // Arguments of the surface texel blocks.

@@ -1473,7 +1473,7 @@ SW_DEV void setup_draws_block(uint32_t block, const SetupArgs& a, const ViewsArg
 	DrawRec* __restrict__ drec = a.drec;
 	uint32_t* __restrict__ counters = a.counters;
 	const uint64_t span_capacity = a.span_capacity, arena_texels = a.arena_texels, skybox_texels = a.skybox_texels,
-				   lite_texels = a.lite_texels;
+				   lite_texels = a.lite_texels, portal_texels = a.portal_texels;
 	constexpr uint32_t GROUPS = 256 / LANES; // draws per block
 	__shared__ uint32_t group_rows[GROUPS];
 	__shared__ unsigned long long block_base;
@@ -1514,9 +1514,11 @@ SW_DEV void setup_draws_block(uint32_t block, const SetupArgs& a, const ViewsArg
 			tex_texels < (1ull << 31) && d.blend <= SW_BLEND_ALPHA_BLEND;
 		if(d.kind == SW_DRAW_POLYGON)
 		{
-			const uint64_t limit = d.texture_source == SW_TEXSRC_SKYBOX ? skybox_texels : views[lo].arena_end;
-			const uint64_t tbase = d.texture_source == SW_TEXSRC_SKYBOX ? 0 : view.surface_arena_offset;
-			valid = valid && d.num_vertices >= 3 && d.num_vertices <= SW_MAX_POLYGON_VERTICES &&
+			const uint64_t limit = d.texture_source == SW_TEXSRC_SKYBOX ? skybox_texels
+				: d.texture_source == SW_TEXSRC_PORTAL                 ? portal_texels
+																	   : views[lo].arena_end;
+			const uint64_t tbase = d.texture_source == SW_TEXSRC_SURFACE ? view.surface_arena_offset : 0;
+			valid = valid && d.texture_source <= SW_TEXSRC_PORTAL && d.num_vertices >= 3 && d.num_vertices <= SW_MAX_POLYGON_VERTICES &&
 				uint64_t(d.first_vertex_int) + 2ull * d.num_vertices <= num_vertex_ints && tbase + d.texture_offset + tex_texels <= limit;
 		}
 		else
@@ -1530,7 +1532,7 @@ SW_DEV void setup_draws_block(uint32_t block, const SetupArgs& a, const ViewsArg
 			w.tex_w = uint32_t(d.texture_size[0]);
 			w.tex_h = uint32_t(d.texture_size[1]);
 			w.pad = 0;
-			w.tex_offset = (d.kind == SW_DRAW_POLYGON && d.texture_source != SW_TEXSRC_SKYBOX ? view.surface_arena_offset : 0) +
+			w.tex_offset = (d.kind == SW_DRAW_POLYGON && d.texture_source == SW_TEXSRC_SURFACE ? view.surface_arena_offset : 0) +
 				d.texture_offset;
 		}
 		v = vdata + d.first_vertex_int;
@@ -1642,6 +1644,7 @@ SW_DEV void setup_draws_block(uint32_t block, const SetupArgs& a, const ViewsArg
 		r.pixels = uint32_t(pixels);
 		r.stat = (d.kind == SW_DRAW_TRIANGLE ? DRAW_STAT_TRIANGLE : 0u) | (reads ? DRAW_STAT_READS_DST : 0u) |
 			(d.kind == SW_DRAW_POLYGON && d.texture_source == SW_TEXSRC_SKYBOX ? DRAW_STAT_SKYBOX : 0u) |
+			(d.kind == SW_DRAW_POLYGON && d.texture_source == SW_TEXSRC_PORTAL ? DRAW_STAT_PORTAL : 0u) |
 			(kind << DRAW_STAT_KIND_SHIFT) | ((d.blend & 15u) << DRAW_STAT_BLEND_SHIFT);
 		r.rows = a.span_stride ? uint32_t(st.next_span - base) : rows;
 		const bool empty = r.num_parts == 0 || row_min >= row_max || x_min >= x_max;
@@ -1873,6 +1876,7 @@ void k_raster_tiles(MapPtrs m, const DrawBounds* __restrict__ bounds,
 		const int32_t h_th = int32_t(r.tex_h);
 		const uint64_t tex = (r.stat & DRAW_STAT_TRIANGLE) ? reinterpret_cast<uint64_t>(m.lite + r.tex_offset)
 			: (r.stat & DRAW_STAT_SKYBOX)                  ? reinterpret_cast<uint64_t>(skybox + r.tex_offset)
+			: (r.stat & DRAW_STAT_PORTAL)                  ? reinterpret_cast<uint64_t>(static_cast<const C*>(m.portal) + r.tex_offset)
 														   : reinterpret_cast<uint64_t>(arena + r.tex_offset);
 		const int32_t h_tex_lo = int32_t(uint32_t(tex));
 		const int32_t h_tex_hi = int32_t(uint32_t(tex >> 32));

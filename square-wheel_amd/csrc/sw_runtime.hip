@@ -84,6 +84,8 @@ struct sw_ctx
 	bool scalar_spans = true; // raster span records through the scalar unit (chosen by the autotuner)
 	uint32_t* counters_host = nullptr; // pinned, device-mapped: result block (k_finish), then the error words
 	float* noalloc_scratch = nullptr; // this device's address of `scratch`
+	const void* portal_texels = nullptr; // sw_set_portal_textures (plain path)
+	uint64_t num_portal_texels = 0;
 	bool finish_launched = false; // the last launch sequence ended with k_finish (span allocation mode)
 	bool stats_pending = false; // the last raster batch's statistics are computed when asked for
 	uint32_t stats_draws = 0; // draws of that batch (its DrawRecs stay in `drec` until the next raster)
@@ -125,6 +127,8 @@ struct sw_ctx
 		void* fb = nullptr;
 		uint32_t total_tiles = 0;
 		uint32_t max_view_height = 0;
+		const void* portal = nullptr; // textures of SW_TEXSRC_PORTAL draws
+		uint64_t num_portal = 0;
 		// how the surface kernel finds each texel's surface: SURF_TABLE_DEVICE (the setup launch
 		// prepares the table), SURF_TABLE_HOST (the host did), SURF_SEARCH (one view, searched in
 		// the surface blocks, which then run in the setup launch)
@@ -440,6 +444,9 @@ int launch_stages(sw_ctx* ctx, const sw_ctx::Launch& L, bool surfaces, bool rast
 			return r;
 	}
 	uint32_t* counters = static_cast<uint32_t*>(ctx->counters.p);
+	MapPtrs mp = map->ptrs;
+	mp.portal = L.portal;
+	mp.num_portal = L.num_portal;
 	SetupArgs a{};
 	a.draws = L.draws;
 	a.num_draws = nd;
@@ -456,6 +463,7 @@ int launch_stages(sw_ctx* ctx, const sw_ctx::Launch& L, bool surfaces, bool rast
 	a.arena_texels = L.arena_texels;
 	a.skybox_texels = L.color_bits == 64 ? map->num_sky64 : map->num_sky32;
 	a.lite_texels = map->ptrs.num_lite;
+	a.portal_texels = L.num_portal;
 	SurfaceArgs sa{};
 	sa.surfaces = L.surfaces;
 	sa.num_surfaces = ns;
@@ -482,34 +490,34 @@ int launch_stages(sw_ctx* ctx, const sw_ctx::Launch& L, bool surfaces, bool rast
 		{
 			StageTimer t(ctx, STAGE_SETUP);
 			swk::launch_setup(ctx->stream, lanes, a, ctx->views_arg, L.surfaces, 0, static_cast<SurfExt*>(ctx->surf_ext.p),
-				static_cast<uint32_t*>(ctx->blk_surf.p), map->ptrs);
+				static_cast<uint32_t*>(ctx->blk_surf.p), mp);
 		}
 		StageTimer t(ctx, STAGE_SURF);
-		swk::launch_build_surfaces(ctx->stream, L.color_bits, map->ptrs, sa, ctx->views_arg, counters);
+		swk::launch_build_surfaces(ctx->stream, L.color_bits, mp, sa, ctx->views_arg, counters);
 	}
 	else if(surfaces && L.surface_mode != SURF_TABLE_DEVICE)
 	{
 		StageTimer t(ctx, STAGE_SETUP_SURF);
-		swk::launch_setup_and_surfaces(ctx->stream, lanes, L.color_bits, a, ctx->views_arg, sa, map->ptrs);
+		swk::launch_setup_and_surfaces(ctx->stream, lanes, L.color_bits, a, ctx->views_arg, sa, mp);
 	}
 	else
 	{
 		{
 			StageTimer t(ctx, STAGE_SETUP);
 			swk::launch_setup(ctx->stream, lanes, a, ctx->views_arg, L.surfaces, ns, static_cast<SurfExt*>(ctx->surf_ext.p),
-				static_cast<uint32_t*>(ctx->blk_surf.p), map->ptrs);
+				static_cast<uint32_t*>(ctx->blk_surf.p), mp);
 		}
 		if(surfaces)
 		{
 			StageTimer t(ctx, STAGE_SURF);
-			swk::launch_build_surfaces(ctx->stream, L.color_bits, map->ptrs, sa, ctx->views_arg, counters);
+			swk::launch_build_surfaces(ctx->stream, L.color_bits, mp, sa, ctx->views_arg, counters);
 		}
 	}
 	if(raster)
 	{
 		{
 			StageTimer t(ctx, STAGE_RASTER);
-			swk::launch_raster_tiles(ctx->stream, L.color_bits, ctx->tile_rows, ctx->scalar_spans, map->ptrs,
+			swk::launch_raster_tiles(ctx->stream, L.color_bits, ctx->tile_rows, ctx->scalar_spans, mp,
 				static_cast<DrawBounds*>(ctx->dbounds.p),
 				static_cast<DrawRec*>(ctx->drec.p), static_cast<PartRec*>(ctx->parts.p), static_cast<SpanRec*>(ctx->spans.p),
 				ctx->views_arg, L.arena, L.color_bits == 64 ? map->sky64.p : map->sky32.p, L.fb, L.total_tiles, counters);
@@ -749,6 +757,8 @@ int launch_frame(sw_ctx* ctx, uint32_t num_draws, uint64_t num_vertex_ints, bool
 	L.arena_texels = ctx->frame_arena_texels;
 	L.arena = ctx->arena.p;
 	L.fb = ctx->fb.p;
+	L.portal = ctx->portal_texels;
+	L.num_portal = ctx->portal_texels ? ctx->num_portal_texels : 0;
 	const bool surfaces = ctx->frame_surfaces_pending;
 	ctx->frame_surfaces_pending = false;
 	if(!raster && !surfaces)
@@ -1019,6 +1029,18 @@ int sw_build_surfaces(sw_ctx* ctx, const sw_surface_desc* surfaces, uint32_t num
 	fill_surface_stats(ctx, surfaces, num_surfaces, true);
 	// built by the launch sequence of sw_raster_draws / sw_frame_end (or sw_read_surfaces)
 	ctx->frame_surfaces_pending = true;
+	return SW_OK;
+}
+
+int sw_set_portal_textures(sw_ctx* ctx, const void* portal_texels_device, uint64_t num_portal_texels)
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	if(!portal_texels_device && num_portal_texels)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null portal textures");
+	ctx->portal_texels = portal_texels_device;
+	ctx->num_portal_texels = portal_texels_device ? num_portal_texels : 0;
+	ctx->graph_key = 0;
 	return SW_OK;
 }
 
@@ -1917,6 +1939,8 @@ static int prepare_batch(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_
 	L.num_draws = b->num_draws;
 	L.num_vertex_ints = b->num_vertex_ints;
 	L.arena_texels = b->surface_arena_texels;
+	L.portal = b->portal_texels;
+	L.num_portal = b->portal_texels ? b->num_portal_texels : 0;
 	L.arena = ctx->arena.p;
 	L.fb = out_fb;
 	if(b->inputs_on_device)
@@ -2042,6 +2066,8 @@ int sw_render_batch_graph(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out
 	mix(b->num_draws);
 	mix(b->color_bits);
 	mix(b->surface_arena_texels);
+	mix(uint64_t(uintptr_t(b->portal_texels)));
+	mix(b->num_portal_texels);
 	mix(ctx->tile_rows);
 	mix(ctx->scalar_spans ? 1 : 0);
 	for(uint32_t i = 0; i < b->num_views; ++i)

@@ -21,7 +21,7 @@ SW_ERR_INVALID_ARGUMENT, SW_ERR_OUT_OF_MEMORY, SW_ERR_DEVICE, SW_ERR_STATE, SW_E
 BLEND_NONE, BLEND_AVERAGE, BLEND_ADDITIVE, BLEND_ALPHA_TEST, BLEND_ALPHA_BLEND = range(5)
 TC_PERSPECTIVE, TC_LINE_Z, TC_AFFINE = range(3)
 DRAW_POLYGON, DRAW_TRIANGLE = 0, 1
-TEXSRC_SURFACE, TEXSRC_SKYBOX = 0, 1
+TEXSRC_SURFACE, TEXSRC_SKYBOX, TEXSRC_PORTAL = 0, 1, 2
 TEX_HAS_NORMAL_MAP, TEX_HAS_NON_ONE_ROUGHNESS, TEX_IS_METAL = 1, 2, 4
 TEX_LITE_LEVEL = 1 << 8
 SURF_DIFFUSE_LAYER, SURF_DIRECTIONAL, SURF_EMISSIVE = 1, 2, 4
@@ -175,6 +175,7 @@ class Batch(C.Structure):
         ("num_draws", C.c_uint32), ("draws", C.c_void_p), ("vertex_data", C.c_void_p),
         ("num_vertex_ints", C.c_uint64), ("surface_arena_texels", C.c_uint64), ("framebuffer_texels", C.c_uint64),
         ("inputs_on_device", C.c_uint32), ("reserved", C.c_uint32),
+        ("portal_texels", C.c_void_p), ("num_portal_texels", C.c_uint64),
     ]
 
 
@@ -228,7 +229,8 @@ class MapData:
 class BatchData:
     """Command buffers of a batch of views (host numpy arrays)."""
 
-    def __init__(self, views, surfaces, lights, light_indices, draws, vertex_data, color_bits, arena_texels=None):
+    def __init__(self, views, surfaces, lights, light_indices, draws, vertex_data, color_bits, arena_texels=None,
+                 portal_texels=0):
         self.views = np.ascontiguousarray(views, dtype=VIEW_DTYPE)
         self.surfaces = np.ascontiguousarray(surfaces, dtype=SURFACE_DTYPE)
         self.lights = np.ascontiguousarray(lights, dtype=LIGHT_DTYPE)
@@ -236,6 +238,7 @@ class BatchData:
         self.draws = np.ascontiguousarray(draws, dtype=DRAW_DTYPE)
         self.vertex_data = np.ascontiguousarray(vertex_data, dtype=np.int32)
         self.color_bits = int(color_bits)
+        self.portal_texels = int(portal_texels)  # texels the SW_TEXSRC_PORTAL draws sample
         if arena_texels is None:
             arena_texels = 0
             for v in self.views:
@@ -251,8 +254,12 @@ class BatchData:
     def color_dtype(self):
         return np.uint64 if self.color_bits == 64 else np.uint32
 
-    def ctypes_batch(self, inputs_on_device=False, device_ptrs=None):
+    def ctypes_batch(self, inputs_on_device=False, device_ptrs=None, portal=None):
+        """portal: (pointer, texels) of the portal textures SW_TEXSRC_PORTAL draws sample: a device
+        pointer for the GPU, a host array's pointer for the oracle."""
         b = Batch()
+        if portal is not None:
+            b.portal_texels, b.num_portal_texels = portal
         b.views, b.num_views = _ptr(self.views), len(self.views)
         b.color_bits = self.color_bits
         b.num_surfaces = len(self.surfaces)
@@ -293,6 +300,7 @@ GPU_SYMBOLS = {
     "sw_build_surfaces": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
                                     C.c_uint32]),
     "sw_raster_draws": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]),
+    "sw_set_portal_textures": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     "sw_frame_end": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint32]),
     "sw_tonemap": (C.c_int, [C.c_void_p, C.c_float, C.POINTER(C.c_float), C.c_void_p, C.c_size_t, C.c_uint32,
                              C.POINTER(C.c_float)]),
@@ -417,11 +425,20 @@ class Context:
         return s.as_dict()
 
     # plain path -------------------------------------------------------------------------
-    def render_plain(self, map_h, b: BatchData, view_index=0, tonemap=None, peek_surfaces=False):
+    def render_plain(self, map_h, b: BatchData, view_index=0, tonemap=None, peek_surfaces=False, portal=None):
         """Renders one view through sw_frame_begin / sw_build_surfaces / sw_raster_draws / sw_frame_end.
         Returns the framebuffer (H, W) as uint32 / uint64 (and the tonemapped uint32 image if asked).
         peek_surfaces: read the surface arena between sw_build_surfaces and sw_raster_draws and
-        return (framebuffer, arena)."""
+        return (framebuffer, arena). portal: host array of the portal textures the view's portal
+        draws sample (copied to the device and set with sw_set_portal_textures for this frame)."""
+        if portal is not None:
+            buf = self.to_device(np.ascontiguousarray(portal, dtype=b.color_dtype))
+            self.check(self.lib.sw_set_portal_textures(self.h, buf.ptr, int(portal.size)), "sw_set_portal_textures")
+            try:
+                return self.render_plain(map_h, b, view_index, tonemap, peek_surfaces)
+            finally:
+                self.lib.sw_set_portal_textures(self.h, None, 0)
+                buf.free()
         v = b.views[view_index]
         w, h = int(v["width"]), int(v["height"])
         cam = (C.c_float * 3)(*[float(x) for x in v["cam_pos"]])
@@ -680,18 +697,19 @@ class Context:
         self.check(self.lib.sw_autotune_batch(self.h, map_h, C.byref(cb), out.ptr, C.byref(rows), ms), "sw_autotune_batch")
         return int(rows.value), {r: float(ms[i]) for i, r in enumerate(TILE_ROWS)}
 
-    def render_batch(self, map_h, b: BatchData, graph=False, device_inputs=None, out=None, copy_back=True):
+    def render_batch(self, map_h, b: BatchData, graph=False, device_inputs=None, out=None, copy_back=True, portal=None):
         """Renders all views of `b` in one set of launches; returns the flat framebuffer (uint32/64).
-        device_inputs: DeviceCommandBuffers already in HBM (inputs stay resident across calls)."""
+        device_inputs: DeviceCommandBuffers already in HBM (inputs stay resident across calls).
+        portal: (device pointer, texels) of the portal textures the batch's portal draws sample."""
         nbytes = max(b.fb_texels, 1) * (8 if b.color_bits == 64 else 4)
         own = out is None
         if own:
             out = self.alloc(nbytes)
         try:
             if device_inputs is not None:
-                cb = b.ctypes_batch(inputs_on_device=True, device_ptrs=device_inputs.ptrs())
+                cb = b.ctypes_batch(inputs_on_device=True, device_ptrs=device_inputs.ptrs(), portal=portal)
             else:
-                cb = b.ctypes_batch()
+                cb = b.ctypes_batch(portal=portal)
             fn = self.lib.sw_render_batch_graph if graph else self.lib.sw_render_batch
             self.check(fn(self.h, map_h, C.byref(cb), out.ptr), fn.__name__)
             if not copy_back:
@@ -700,6 +718,33 @@ class Context:
         finally:
             if own:
                 out.free()
+
+
+    def render_portal_levels(self, map_h, batches):
+        """Renders a frame and the views through its portals (batches: one BatchData per portal level,
+        the frame's own first). The deepest level samples zero portal textures; each level above
+        samples the level below's framebuffer, kept in HBM. Returns every level's flat framebuffer."""
+        outs = [None] * len(batches)
+        below = None
+        try:
+            for k in reversed(range(len(batches))):
+                b = batches[k]
+                if below is None:
+                    below = self.to_device(np.zeros(max(b.portal_texels, 1), dtype=b.color_dtype))
+                elif b.portal_texels != batches[k + 1].fb_texels:
+                    raise ValueError("portal textures of a level do not match the framebuffers of the level below")
+                out = self.alloc(max(b.fb_texels, 1) * (8 if b.color_bits == 64 else 4))
+                try:
+                    outs[k] = self.render_batch(map_h, b, out=out, portal=(below.ptr, max(b.portal_texels, 1)))
+                except Exception:
+                    out.free()
+                    raise
+                below.free()
+                below = out
+        finally:
+            if below is not None:
+                below.free()
+        return outs
 
 
 class DeviceBuffer:

@@ -17,7 +17,8 @@ class MapConfig(C.Structure):
                 ("sun", C.c_uint32), ("sky_fraction", C.c_float), ("texture_size", C.c_uint32),
                 ("num_materials", C.c_uint32), ("normal_maps", C.c_uint32), ("specular_fraction", C.c_float),
                 ("alpha_blend_fraction", C.c_float), ("directional_lightmaps", C.c_uint32),
-                ("num_decal_textures", C.c_uint32), ("num_model_textures", C.c_uint32), ("num_submodels", C.c_uint32)]
+                ("num_decal_textures", C.c_uint32), ("num_model_textures", C.c_uint32), ("num_submodels", C.c_uint32),
+                ("num_view_portals", C.c_uint32)]
 
 
 class MapInfo(C.Structure):
@@ -31,7 +32,8 @@ class FrameConfig(C.Structure):
                 ("textures_mip_bias", C.c_float), ("use_directional_lightmaps", C.c_uint32),
                 ("clear_background", C.c_uint32), ("force_clear", C.c_uint32), ("num_dynamic_lights", C.c_uint32),
                 ("num_models", C.c_uint32), ("num_decals", C.c_uint32), ("seed", C.c_uint32), ("time", C.c_float),
-                ("virtual_threads", C.c_uint32), ("view_model", C.c_uint32)]
+                ("virtual_threads", C.c_uint32), ("view_model", C.c_uint32),
+                ("portals_depth", C.c_uint32)]
 
 
 class ViewC(C.Structure):
@@ -46,10 +48,19 @@ class FrameData(C.Structure):
                 ("num_lights", C.c_uint32), ("light_indices", C.c_void_p), ("num_light_indices", C.c_uint32),
                 ("draws", C.c_void_p), ("num_draws", C.c_uint32), ("vertex_data", C.c_void_p),
                 ("num_vertex_ints", C.c_uint64), ("surface_arena_texels", C.c_uint64),
-                ("num_visible_leafs", C.c_uint32), ("num_triangles", C.c_uint32), ("first_submodel_surface", C.c_uint32)]
+                ("num_visible_leafs", C.c_uint32), ("num_triangles", C.c_uint32), ("first_submodel_surface", C.c_uint32),
+                ("num_portal_texels", C.c_uint64), ("num_portal_views", C.c_uint32), ("num_visible_portals", C.c_uint32)]
 
 
 assert C.sizeof(ViewC) == A.VIEW_DTYPE.itemsize
+
+PORTAL_MIRROR, PORTAL_CAMERA, PORTAL_PARALLAX = 0, 1, 2
+VIEW_PORTAL_DTYPE = np.dtype([("kind", "<u4"), ("blending_mode", "<u4"), ("plane", "<f4", 4),
+                              ("tex_coord_equation", "<f4", (2, 4)), ("vertices", "<f4", (4, 3)),
+                              ("camera_position", "<f4", 3), ("camera_yaw", "<f4"), ("camera_pitch", "<f4"),
+                              ("camera_fov", "<f4"), ("transform", "<f4", 16)])
+PORTAL_INFO_DTYPE = np.dtype([("visible", "<u4"), ("resolution", "<u4", 2), ("tc_min", "<i4", 2), ("mip", "<u4"),
+                              ("texture_offset", "<u8")], align=True)
 
 
 def load():
@@ -77,6 +88,12 @@ def load():
         lib.swh_frame_build.restype = C.c_int
         lib.swh_frame_get.argtypes = [C.c_void_p, C.POINTER(FrameData)]
         lib.swh_frame_get.restype = C.c_int
+        lib.swh_frame_portal_view.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]
+        lib.swh_frame_portal_view.restype = C.c_int
+        lib.swh_frame_portal_info.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+        lib.swh_frame_portal_info.restype = C.c_int
+        lib.swh_map_view_portals.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
+        lib.swh_map_view_portals.restype = C.c_int
         lib.swh_shadow_create.argtypes = [C.POINTER(C.c_void_p)]
         lib.swh_shadow_create.restype = C.c_int
         lib.swh_shadow_destroy.argtypes = [C.c_void_p]
@@ -136,7 +153,7 @@ class Map:
         defaults = dict(seed=0, grid=(8, 8, 2), solid_probability=0.3, num_lights=4, light_intensity_min=0.5,
                         light_intensity_max=2.0, sun=0, sky_fraction=0.1, texture_size=64, num_materials=8,
                         normal_maps=0, specular_fraction=0.0, alpha_blend_fraction=0.0, directional_lightmaps=0,
-                        num_decal_textures=0, num_model_textures=0, num_submodels=0)
+                        num_decal_textures=0, num_model_textures=0, num_submodels=0, num_view_portals=0)
         defaults.update(kw)
         for k, v in defaults.items():
             if k == "grid":
@@ -188,6 +205,15 @@ class Map:
             _copy(d.levels, d.num_levels, A.LEVEL_DTYPE), _copy(d.skybox_texels32, d.num_skybox_texels32, np.uint32),
             _copy(d.skybox_texels64, d.num_skybox_texels64, np.uint64))
 
+    def view_portals(self):
+        """The map's view portals (VIEW_PORTAL_DTYPE; transform column-major)."""
+        lib = load()
+        n = lib.swh_map_view_portals(self.h, None, 0)
+        out = np.zeros(max(n, 0), dtype=VIEW_PORTAL_DTYPE)
+        if n > 0:
+            assert lib.swh_map_view_portals(self.h, C.c_void_p(out.ctypes.data), n) == n
+        return out
+
     def camera_path(self, seed, n):
         out = np.zeros((n, 5), dtype=np.float32)
         if load().swh_camera_path(self.h, seed, n, C.c_void_p(out.ctypes.data)) != 0:
@@ -206,6 +232,42 @@ class Map:
             pass
 
 
+def _frame_dict(h, color_bits):
+    """The command buffers of a built frame (copies), with the views seen through its portals."""
+    lib = load()
+    fd = FrameData()
+    lib.swh_frame_get(h, C.byref(fd))
+    portals = []
+    i = 0
+    info = np.zeros(1, dtype=PORTAL_INFO_DTYPE)
+    while lib.swh_frame_portal_info(h, i, C.c_void_p(info.ctypes.data)) == 0:
+        portals.append(info[0].copy())
+        i += 1
+    views = []
+    for i in range(fd.num_portal_views):
+        ch = C.c_void_p()
+        if lib.swh_frame_portal_view(h, i, C.byref(ch)) != 0:
+            raise RuntimeError("swh_frame_portal_view failed")
+        views.append(_frame_dict(ch, color_bits))
+    return {
+        "view": np.frombuffer(bytes(fd.view), dtype=A.VIEW_DTYPE).copy(),
+        "surfaces": _copy(fd.surfaces, fd.num_surfaces, A.SURFACE_DTYPE),
+        "lights": _copy(fd.lights, fd.num_lights, A.LIGHT_DTYPE),
+        "light_indices": _copy(fd.light_indices, fd.num_light_indices, np.uint32),
+        "draws": _copy(fd.draws, fd.num_draws, A.DRAW_DTYPE),
+        "vertex_data": _copy(fd.vertex_data, fd.num_vertex_ints, np.int32),
+        "arena_texels": int(fd.surface_arena_texels),
+        "num_visible_leafs": int(fd.num_visible_leafs),
+        "num_triangles": int(fd.num_triangles),
+        "first_submodel_surface": int(fd.first_submodel_surface),
+        "color_bits": color_bits,
+        "portal_texels": int(fd.num_portal_texels),
+        "num_visible_portals": int(fd.num_visible_portals),
+        "portal_views": views,
+        "portals": np.array(portals, dtype=PORTAL_INFO_DTYPE),
+    }
+
+
 class FrameBuilder:
     def __init__(self):
         h = C.c_void_p()
@@ -217,7 +279,8 @@ class FrameBuilder:
         cfg = FrameConfig()
         defaults = dict(width=320, height=200, color_bits=32, fov=math.pi / 2, textures_mip_bias=0.0,
                         use_directional_lightmaps=0, clear_background=1, force_clear=1, num_dynamic_lights=0,
-                        num_models=0, num_decals=0, seed=0, time=0.0, virtual_threads=1, view_model=0)
+                        num_models=0, num_decals=0, seed=0, time=0.0, virtual_threads=1, view_model=0,
+                        portals_depth=2)
         defaults.update(kw)
         for k, v in defaults.items():
             setattr(cfg, k, v)
@@ -225,22 +288,7 @@ class FrameBuilder:
         lib = load()
         if lib.swh_frame_build(self.h, m.h, C.byref(cfg), cam, float(pose[3]), float(pose[4])) != 0:
             raise RuntimeError("swh_frame_build failed")
-        fd = FrameData()
-        lib.swh_frame_get(self.h, C.byref(fd))
-        view = np.frombuffer(bytes(fd.view), dtype=A.VIEW_DTYPE).copy()
-        return {
-            "view": view,
-            "surfaces": _copy(fd.surfaces, fd.num_surfaces, A.SURFACE_DTYPE),
-            "lights": _copy(fd.lights, fd.num_lights, A.LIGHT_DTYPE),
-            "light_indices": _copy(fd.light_indices, fd.num_light_indices, np.uint32),
-            "draws": _copy(fd.draws, fd.num_draws, A.DRAW_DTYPE),
-            "vertex_data": _copy(fd.vertex_data, fd.num_vertex_ints, np.int32),
-            "arena_texels": int(fd.surface_arena_texels),
-            "num_visible_leafs": int(fd.num_visible_leafs),
-            "num_triangles": int(fd.num_triangles),
-            "first_submodel_surface": int(fd.first_submodel_surface),
-            "color_bits": int(cfg.color_bits),
-        }
+        return _frame_dict(self.h, int(cfg.color_bits))
 
     def close(self):
         if self.h:
@@ -329,7 +377,7 @@ def batch_from_frames(frames):
     views = np.concatenate([f["view"] for f in frames]).copy()
     surf, lights, lidx, draws, vdata = [], [], [], [], []
     s_off = d_off = l_off = li_off = v_off = 0
-    arena_off = fb_off = 0
+    arena_off = fb_off = portal_off = 0
     for k, f in enumerate(frames):
         v = views[k]
         v["first_surface"], v["num_surfaces"] = s_off, len(f["surfaces"])
@@ -340,6 +388,7 @@ def batch_from_frames(frames):
         li = f["light_indices"] + l_off
         d = f["draws"].copy()
         d["first_vertex_int"] += v_off
+        d["texture_offset"][d["texture_source"] == A.TEXSRC_PORTAL] += np.uint64(portal_off)
         surf.append(s)
         lights.append(f["lights"])
         lidx.append(li.astype(np.uint32))
@@ -352,9 +401,31 @@ def batch_from_frames(frames):
         v_off += len(f["vertex_data"])
         arena_off += f["arena_texels"]
         fb_off += int(v["width"]) * int(v["height"])
+        portal_off += f.get("portal_texels", 0)
     cat = lambda xs, dt: np.concatenate(xs) if xs else np.zeros(0, dtype=dt)  # noqa: E731
     return A.BatchData(views, cat(surf, A.SURFACE_DTYPE), cat(lights, A.LIGHT_DTYPE), cat(lidx, np.uint32),
-                       cat(draws, A.DRAW_DTYPE), cat(vdata, np.int32), frames[0]["color_bits"], arena_texels=arena_off)
+                       cat(draws, A.DRAW_DTYPE), cat(vdata, np.int32), frames[0]["color_bits"], arena_texels=arena_off,
+                       portal_texels=portal_off)
+
+
+def portal_levels(frame):
+    """The frame and the views seen through its portals, level by level: level k + 1 lists the portal
+    views of level k's frames in order, so that level k + 1's framebuffers laid end to end (one batch)
+    are level k's portal textures laid end to end. The last level's portal draws sample zeros."""
+    levels = [[frame]]
+    while True:
+        nxt = [c for f in levels[-1] for c in f.get("portal_views", [])]
+        if not nxt:
+            return levels
+        tex = sum(f["portal_texels"] for f in levels[-1])
+        px = sum(int(c["view"]["width"][0]) * int(c["view"]["height"][0]) for c in nxt)
+        assert tex == px, (tex, px)
+        levels.append(nxt)
+
+
+def portal_level_batches(frame):
+    """One BatchData per portal level (portal_levels), level 0 (the frame itself) first."""
+    return [batch_from_frames(level) for level in portal_levels(frame)]
 
 
 # ---------------------------------------------------------------------------------------------

@@ -176,3 +176,33 @@ def test_inline_models_and_view_model_frames(gpu_ctx, color_bits):
         assert np.array_equal(out, ref)
     finally:
         gpu_ctx.release_map(mh)
+
+
+@pytest.mark.parametrize("color_bits", [32, 64])
+def test_portal_frames_bit_exact(gpu_ctx, color_bits):
+    """Mirrors, camera and parallax portals two levels deep: every level's views on the GPU (each
+    level one batch sampling the level below's framebuffer in HBM) against the oracle's levels; the
+    main view again through the plain path with the portal textures set by sw_set_portal_textures."""
+    m = sw_host.large_map(4, grid=(10, 10, 4), num_view_portals=12)
+    fb = sw_host.FrameBuilder()
+    mh = gpu_ctx.upload_map(m.data)
+    try:
+        done = 0
+        poses = m.camera_path(8, 48)
+        for i in (13, 7, 17, 23):  # poses that see portal views two levels deep
+            f = fb.build(m, poses[i], width=480, height=300, color_bits=color_bits, use_directional_lightmaps=1,
+                         num_dynamic_lights=4, num_models=8, time=0.3 * i, portals_depth=2)
+            levels = sw_host.portal_level_batches(f)
+            assert len(levels) == 3
+            _say(f"case portals pose {i}: {[len(b.views) for b in levels]} views per level, color {color_bits}")
+            outs = gpu_ctx.render_portal_levels(mh, levels)
+            refs, oob = oracle.render_portal_levels(m.data, levels)
+            assert oob == 0
+            for k, (o, r) in enumerate(zip(outs, refs)):
+                assert np.array_equal(o, r), f"pose {i} level {k}: {np.count_nonzero(o != r)} pixels differ"
+            plain = gpu_ctx.render_plain(mh, levels[0], portal=refs[1]).reshape(-1)
+            assert np.array_equal(plain, refs[0])
+            done += 1
+        assert done == 4
+    finally:
+        gpu_ctx.release_map(mh)
