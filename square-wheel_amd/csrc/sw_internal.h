@@ -157,7 +157,7 @@ struct MapPtrs
 	const sw_directional_lightmap_element* lm_dir;
 	const sw_texture_level* levels;
 	uint32_t num_levels;
-	uint32_t pad;
+	uint32_t lm_simple_offset; // byte offset of lm_simple from lm_dir (one buffer below 4 GiB)
 	uint64_t num_texels, num_lite, num_lm_simple, num_lm_dir;
 	const void* portal; // ColorT texels of SW_TEXSRC_PORTAL draws (set per launch)
 	uint64_t num_portal;

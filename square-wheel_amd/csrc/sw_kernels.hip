@@ -123,26 +123,41 @@ struct LmE
 	float col[3];
 };
 
-SW_DEV LmE lm_load(const MapPtrs& m, bool directional, uint64_t i)
+// Element i of a global buffer whose byte size is below 2^32 (checked when the map is uploaded, or
+// by the draw validation): the 32-bit byte offset lets the load take the buffer's base from SGPRs
+// and the offset from one VGPR, with no 64-bit address arithmetic.
+template <class T>
+SW_DEV T elem32(const T* base, uint32_t i)
 {
+	typedef __attribute__((address_space(1))) const T* GlobalPtr;
+	typedef __attribute__((address_space(1))) const char* GlobalBytes;
+	return *(GlobalPtr)((GlobalBytes)base + i * uint32_t(sizeof(T)));
+}
+
+// Element i of the lightmaps: both kinds start with the three ambient floats, and lie in one buffer
+// (simple elements at lm_simple_offset), so a 32-bit byte offset from the buffer's base selects the
+// element whatever the texel's kind.
+SW_DEV LmE lm_load(const MapPtrs& m, bool directional, uint32_t i)
+{
+	typedef __attribute__((address_space(1))) const float* GlobalFloats;
+	typedef __attribute__((address_space(1))) const char* GlobalBytes;
+	const uint32_t off = directional ? i * uint32_t(sizeof(sw_directional_lightmap_element)) : m.lm_simple_offset + i * 12u;
+	const GlobalFloats f = (GlobalFloats)((GlobalBytes)m.lm_dir + off);
 	LmE e;
+	e.amb[0] = f[0];
+	e.amb[1] = f[1];
+	e.amb[2] = f[2];
 	if(directional)
 	{
-		const sw_directional_lightmap_element& d = m.lm_dir[i];
-		e.amb[0] = d.ambient_light[0];
-		e.amb[1] = d.ambient_light[1];
-		e.amb[2] = d.ambient_light[2];
-		e.dir = v3(d.light_direction_vector_scaled[0], d.light_direction_vector_scaled[1], d.light_direction_vector_scaled[2]);
-		e.dev = d.directional_light_deviation;
-		e.col[0] = d.directional_light_color[0];
-		e.col[1] = d.directional_light_color[1];
-		e.col[2] = d.directional_light_color[2];
+		// sw_directional_lightmap_element: ambient, direction, deviation, colour
+		e.dir = v3(f[3], f[4], f[5]);
+		e.dev = f[6];
+		e.col[0] = f[7];
+		e.col[1] = f[8];
+		e.col[2] = f[9];
 	}
 	else
 	{
-		e.amb[0] = m.lm_simple[i * 3 + 0];
-		e.amb[1] = m.lm_simple[i * 3 + 1];
-		e.amb[2] = m.lm_simple[i * 3 + 2];
 		e.dir = v3(0.0f, 0.0f, 0.0f);
 		e.dev = 0.0f;
 		e.col[0] = e.col[1] = e.col[2] = 0.0f;
@@ -421,11 +436,10 @@ SW_DEV bool shadow_lit(const sw_light_shadow& S, const float* __restrict__ texel
 // that are uniform per surface. SH: lights may have shadow maps (a separate instantiation, so the
 // kernel without them keeps its registers).
 template <class CT, bool SH>
-SW_DEV typename CT::T surface_texel(const MapPtrs& m, const SurfRec& R, const sw_light* __restrict__ lights,
+SW_DEV typename CT::T surface_texel(const MapPtrs& m, const SurfRec& R, uint32_t fl, const sw_light* __restrict__ lights,
 	const uint32_t* __restrict__ light_indices, uint32_t num_light_indices, uint32_t num_lights_total, uint32_t dst_u,
 	uint32_t dst_v, const SurfaceArgs& a)
 {
-	const uint32_t fl = R.flags;
 	const bool use_normal_map = (fl & REC_NORMAL_MAP) != 0;
 	const uint32_t spec = (fl >> REC_SPEC_SHIFT) & 3u;
 	const bool directional = (fl & REC_DIRECTIONAL) != 0;
@@ -435,7 +449,13 @@ SW_DEV typename CT::T surface_texel(const MapPtrs& m, const SurfRec& R, const sw
 	uint32_t src_u, src_v;
 	wrap_texel((fl & REC_SLOW_WRAP) != 0, R.tcu, R.tcv, R.tw, R.th, R.tw_mask, R.th_mask, R.inv_tw, R.inv_th, dst_u, dst_v,
 		src_u, src_v);
-	const sw_texel texel = m.texels[R.tex_base + uint64_t(src_v) * R.tw + src_u];
+	sw_texel texel;
+	{
+		// the 8-byte texel as one 64-bit load (diffuse in the low half)
+		const uint64_t t = elem32(reinterpret_cast<const uint64_t*>(m.texels), uint32_t(R.tex_base) + src_v * R.tw + src_u);
+		texel.diffuse = uint32_t(t);
+		texel.packed_normal_roughness = int32_t(uint32_t(t >> 32));
+	}
 
 	const V3 u_vec = R.u_vec, v_vec = R.v_vec;
 	const V3 start_pos_v = scalar_mul_add(v_vec, float(dst_v), R.start_pos);
@@ -472,8 +492,8 @@ SW_DEV typename CT::T surface_texel(const MapPtrs& m, const SurfRec& R, const sw
 		const uint32_t base_u = dst_u + R.lm_su;
 		const uint32_t lu = base_u >> sl;
 		const float ku = mul_add(float(base_u & mask), inv_scale, k_shift);
-		const uint64_t row0 = uint64_t(R.lm_offset) + uint64_t(lv) * R.lm_stride;
-		const uint64_t row1 = row0 + R.lm_stride;
+		const uint32_t row0 = R.lm_offset + lv * R.lm_stride;
+		const uint32_t row1 = row0 + R.lm_stride;
 		const LmE l00 = lm_load(m, directional, row0 + lu);
 		const LmE l01 = lm_load(m, directional, row0 + lu + 1);
 		const LmE l10 = lm_load(m, directional, row1 + lu);
@@ -601,16 +621,17 @@ SW_DEV typename CT::T surface_texel(const MapPtrs& m, const SurfRec& R, const sw
 
 // surfaces.rs:903-971, one texel
 template <class CT>
-SW_DEV typename CT::T emissive_texel(const MapPtrs& m, const SurfRec& R, typename CT::T dst, uint32_t dst_u, uint32_t dst_v)
+SW_DEV typename CT::T emissive_texel(const MapPtrs& m, const SurfRec& R, uint32_t fl, typename CT::T dst, uint32_t dst_u,
+	uint32_t dst_v)
 {
 	uint32_t src_u, src_v;
-	wrap_texel((R.flags & REC_EMISSIVE_SLOW_WRAP) != 0, R.etcu, R.etcv, R.ew, R.eh, R.ew_mask, R.eh_mask, R.inv_ew, R.inv_eh,
+	wrap_texel((fl & REC_EMISSIVE_SLOW_WRAP) != 0, R.etcu, R.etcv, R.ew, R.eh, R.ew_mask, R.eh_mask, R.inv_ew, R.inv_eh,
 		dst_u, dst_v, src_u, src_v);
-	const uint32_t t = m.lite[R.etex_base + uint64_t(src_v) * R.ew + src_u];
+	const uint32_t t = elem32(m.lite, uint32_t(R.etex_base) + src_v * R.ew + src_u);
 	const int32_t lb = R.elight[0], lg = R.elight[1], lr = R.elight[2];
 	const CI tc = Color32T::to_ci(t);
 	const CI mod = CI{wmul(tc.b, lb) >> 8, wmul(tc.g, lg) >> 8, wmul(tc.r, lr) >> 8, wmul(tc.a, 0) >> 8};
-	switch((R.flags >> REC_BLEND_SHIFT) & 15u)
+	switch((fl >> REC_BLEND_SHIFT) & 15u)
 	{
 	case SW_BLEND_NONE: return CT::from_ci(mod);
 	case SW_BLEND_AVERAGE:
@@ -748,17 +769,18 @@ SW_DEV void build_surfaces_block(uint32_t block, const MapPtrs& m, const Surface
 			if(r == count)
 				break; // this and later texels lie past the pass's surfaces
 			const SurfRec& R = recs[r];
-			if(idx < R.start || (R.flags & REC_SKIP))
+			const uint32_t fl = R.flags;
+			if(idx < R.start || (fl & REC_SKIP))
 				continue; // gap between surfaces, or a surface left untouched
 			uint32_t du;
 			const uint32_t dv = div_rcp(uint32_t(idx - R.start), R.w, R.inv_w, du);
 			typename CT::T out = 0;
-			if(R.flags & REC_DIFFUSE)
-				out = surface_texel<CT, SH>(m, R, a.lights, a.light_indices, a.num_light_indices, a.num_lights, du, dv, a);
-			else if(((R.flags >> REC_BLEND_SHIFT) & 15u) != SW_BLEND_NONE)
+			if(fl & REC_DIFFUSE)
+				out = surface_texel<CT, SH>(m, R, fl, a.lights, a.light_indices, a.num_light_indices, a.num_lights, du, dv, a);
+			else if(((fl >> REC_BLEND_SHIFT) & 15u) != SW_BLEND_NONE)
 				out = arena[idx]; // an emissive layer without a diffuse layer blends over what the arena holds
-			if(R.flags & REC_EMISSIVE)
-				out = emissive_texel<CT>(m, R, out, du, dv);
+			if(fl & REC_EMISSIVE)
+				out = emissive_texel<CT>(m, R, fl, out, du, dv);
 			arena[idx] = out;
 		}
 		if(count < SURF_CHUNK)
@@ -1511,7 +1533,7 @@ SW_DEV void setup_draws_block(uint32_t block, const SetupArgs& a, const ViewsArg
 		// unchecked in release builds, fast_math.rs:750-777; here an invalid draw is skipped).
 		const uint64_t tex_texels = uint64_t(uint32_t(d.texture_size[0])) * uint32_t(d.texture_size[1]);
 		valid = d.texture_size[0] > 0 && d.texture_size[1] > 0 && d.texture_size[0] <= 65536 && d.texture_size[1] <= 65536 &&
-			tex_texels < (1ull << 31) && d.blend <= SW_BLEND_ALPHA_BLEND;
+			tex_texels <= (1ull << 29) && d.blend <= SW_BLEND_ALPHA_BLEND;
 		if(d.kind == SW_DRAW_POLYGON)
 		{
 			const uint64_t limit = d.texture_source == SW_TEXSRC_SKYBOX ? skybox_texels
@@ -1779,13 +1801,53 @@ template <class T>
 SW_DEV T global_at(uint64_t addr, int32_t i)
 {
 	typedef __attribute__((address_space(1))) const T* GlobalPtr;
-	return ((GlobalPtr)addr)[i];
+	typedef __attribute__((address_space(1))) const char* GlobalBytes;
+	// i * sizeof(T) < 2^32: textures hold at most 2^29 texels (the draw validation)
+	return *(GlobalPtr)((GlobalBytes)addr + uint32_t(i) * uint32_t(sizeof(T)));
 }
 
 // Wave lane exchange: pull `v` from lane `src` / push `v` to lane `dst` (ds_bpermute / ds_permute).
 SW_DEV int32_t lane_pull(int32_t v, uint32_t src) { return __builtin_amdgcn_ds_bpermute(int32_t(src << 2), v); }
 SW_DEV int32_t lane_push(int32_t v, uint32_t dst) { return __builtin_amdgcn_ds_permute(int32_t(dst << 2), v); }
 SW_DEV int32_t rl(int32_t v, uint32_t lane) { return __builtin_amdgcn_readlane(v, lane); }
+
+// Texel indices of a draw's active rows in the tile (bit rr of gmask: row rr's span meets the
+// tile), the span values read from lane g * ROWS + rr: -1 outside the span or the texture. KIND 0:
+// perspective, 1: linear tc, 3: triangle (upper0 = y0 - first row - rows of the lower part: row rr
+// is in the upper part when upper0 + rr >= 0; its u / v steps d0 / d1 per part).
+template <int KIND, int ROWS>
+SW_DEV void span_texel_indices(int32_t (&tix)[ROWS], uint32_t gmask, int g, const SpanRec& mine, int32_t x, uint32_t tw,
+	uint32_t th, int32_t upper0, int32_t d0u, int32_t d0l, int32_t d1u, int32_t d1l)
+{
+#pragma unroll
+	for(int rr = 0; rr < ROWS; ++rr)
+	{
+		if(!(gmask & (1u << rr)))
+			continue;
+		const int sl = g * ROWS + rr;
+		const int32_t sx0 = __builtin_amdgcn_readlane(mine.x0, sl), sx1 = __builtin_amdgcn_readlane(mine.x1, sl);
+		const int32_t j = x - sx0;
+		const bool in = x >= sx0 && x < sx1;
+		int32_t t;
+		if constexpr(KIND == 3)
+		{
+			const bool upper = upper0 + rr >= 0;
+			const int32_t s0 = __builtin_amdgcn_readlane(mine.v[0], sl), s1 = __builtin_amdgcn_readlane(mine.v[1], sl);
+			const uint32_t u = uint32_t(fixed16_floor_to_int(wadd(s0, wmul(j, upper ? d0u : d0l))));
+			const uint32_t v = uint32_t(fixed16_floor_to_int(wadd(s1, wmul(j, upper ? d1u : d1l))));
+			t = u < tw && v < th ? int32_t(u + v * tw) : -1;
+		}
+		else
+		{
+			int32_t sv[6];
+#pragma unroll
+			for(int i = 0; i < (KIND == 0 ? 6 : 4); ++i)
+				sv[i] = __builtin_amdgcn_readlane(mine.v[i], sl);
+			t = polygon_texel_index<KIND>(sv, j, tw, th);
+		}
+		tix[rr] = in ? t : -1;
+	}
+}
 
 // The tile's draws are taken 64 at a time in draw order: a ballot bins them, the hits are
 // compacted into the low lanes, which load their draw's parameters at once. Then GD draws at a
@@ -1897,6 +1959,145 @@ void k_raster_tiles(MapPtrs m, const DrawBounds* __restrict__ bounds,
 			return; // the host runs the batch again with room for every span record
 		for(uint32_t kg = 0; kg < nh; kg += GD)
 		{
+			if constexpr(!SSPAN)
+			{
+				// Lane L < GD * ROWS loads the span record of draw kg + L / ROWS at tile row
+				// L % ROWS and tests whether its span meets the tile; the ballot of the tests
+				// names the active items, and only those are walked (scalar bit tests), their
+				// values read from the item's lane. Texels of the lanes outside a span are fetched
+				// at index 0 of the same texture (in bounds) and discarded by a select, so the
+				// fetches and blends change no execution mask.
+				SpanRec mine = SpanRec{0, 0, {0, 0, 0, 0, 0, 0}};
+				bool act = false;
+				{
+					const uint32_t kq = (kg + lane / ROWS) & 63u;
+					const int32_t q_row = y0 + int32_t(lane % ROWS) - lane_pull(h_row_first, kq);
+					const int32_t q_rows = lane_pull(h_rows, kq);
+					const int32_t q_base = lane_pull(h_span_base, kq);
+					if(lane < uint32_t(GD * ROWS) && kg + lane / ROWS < nh && q_row >= 0 && q_row < q_rows)
+					{
+						mine = spans[uint32_t(q_base) + uint32_t(q_row)];
+						act = mine.x0 < mine.x1 && mine.x1 > x0 && mine.x0 < x0 + 64;
+					}
+				}
+				const uint64_t amask = __ballot(act);
+				if(amask == 0)
+					continue;
+				int32_t tix[GD][ROWS];
+				C texel[GD][ROWS];
+#pragma unroll
+				for(int g = 0; g < GD; ++g)
+				{
+					const uint32_t gmask = uint32_t(amask >> (g * ROWS)) & ((1u << ROWS) - 1u);
+#pragma unroll
+					for(int rr = 0; rr < ROWS; ++rr)
+					{
+						tix[g][rr] = -1;
+						texel[g][rr] = 0;
+					}
+					if(gmask == 0)
+						continue;
+					const uint32_t k = kg + uint32_t(g); // below nh: the draw has an active item
+					const int32_t kind = rl(h_kind, k);
+					const uint32_t tw = uint32_t(rl(h_tw, k)), th = uint32_t(rl(h_th, k));
+					// the draw's kind is tested once, the rows below are straight-line code
+					if(kind == 0)
+						span_texel_indices<0, ROWS>(tix[g], gmask, g, mine, x, tw, th, 0, 0, 0, 0, 0);
+					else if(kind == 3)
+						span_texel_indices<3, ROWS>(tix[g], gmask, g, mine, x, tw, th, y0 - rl(h_row_first, k) - rl(h_rows0, k),
+							rl(pd[5], k), rl(pd[0], k), rl(pd[6], k), rl(pd[1], k));
+					else
+						span_texel_indices<1, ROWS>(tix[g], gmask, g, mine, x, tw, th, 0, 0, 0, 0, 0);
+				}
+				// every span value is read before the fetches are issued, so all of the group's
+				// fetches are in flight together
+#pragma unroll
+				for(int g = 0; g < GD; ++g)
+				{
+					const uint32_t gmask = uint32_t(amask >> (g * ROWS)) & ((1u << ROWS) - 1u);
+					if(gmask == 0)
+						continue;
+					const uint32_t k = kg + uint32_t(g);
+					const uint64_t tex = uint64_t(uint32_t(rl(h_tex_lo, k))) | (uint64_t(uint32_t(rl(h_tex_hi, k))) << 32);
+					if(rl(h_kind, k) == 3)
+					{
+#pragma unroll
+						for(int rr = 0; rr < ROWS; ++rr)
+							if(gmask & (1u << rr))
+								texel[g][rr] = C(global_at<uint32_t>(tex, tix[g][rr] < 0 ? 0 : tix[g][rr]));
+					}
+					else
+					{
+#pragma unroll
+						for(int rr = 0; rr < ROWS; ++rr)
+							if(gmask & (1u << rr))
+								texel[g][rr] = global_at<C>(tex, tix[g][rr] < 0 ? 0 : tix[g][rr]);
+					}
+				}
+#pragma unroll
+				for(int g = 0; g < GD; ++g)
+				{
+					const uint32_t gmask = uint32_t(amask >> (g * ROWS)) & ((1u << ROWS) - 1u);
+					if(gmask == 0)
+						continue;
+					const uint32_t k = kg + uint32_t(g);
+					const uint32_t blend = uint32_t(rl(h_blend, k));
+					if(rl(h_kind, k) != 3)
+					{
+						if(blend == SW_BLEND_NONE)
+						{
+#pragma unroll
+							for(int rr = 0; rr < ROWS; ++rr)
+								if(gmask & (1u << rr))
+									acc[rr] = tix[g][rr] >= 0 ? texel[g][rr] : acc[rr];
+						}
+						else
+						{
+#pragma unroll
+							for(int rr = 0; rr < ROWS; ++rr)
+							{
+								if(!(gmask & (1u << rr)))
+									continue;
+								const C nc = write_into_framebuffer<CT>(blend, acc[rr], texel[g][rr]);
+								acc[rr] = tix[g][rr] >= 0 ? nc : acc[rr];
+							}
+						}
+						continue;
+					}
+					const int32_t part_row0 = rl(h_row_first, k), rows0 = rl(h_rows0, k);
+					const int32_t dr_u = rl(pd[7], k), dr_l = rl(pd[2], k);
+					const int32_t dg_u = rl(pd[8], k), dg_l = rl(pd[3], k);
+					const int32_t db_u = rl(pd[9], k), db_l = rl(pd[4], k);
+#pragma unroll
+					for(int rr = 0; rr < ROWS; ++rr)
+					{
+						if(!(gmask & (1u << rr)))
+							continue;
+						const int sl = g * ROWS + rr;
+						const bool upper = y0 + rr - part_row0 >= rows0;
+						const int32_t j = x - rl(mine.x0, sl);
+						const int32_t lr = wadd(rl(mine.v[2], sl), wmul(j, upper ? dr_u : dr_l));
+						const int32_t lg = wadd(rl(mine.v[3], sl), wmul(j, upper ? dg_u : dg_l));
+						const int32_t lb = wadd(rl(mine.v[4], sl), wmul(j, upper ? db_u : db_l));
+						const uint32_t tv = uint32_t(texel[g][rr]);
+						const CI tc = Color32T::to_ci(tv);
+						const CI lit = CI{wmul(tc.b, lb) >> 16, wmul(tc.g, lg) >> 16, wmul(tc.r, lr) >> 16, 0};
+						const C dst_c = acc[rr];
+						C nc = dst_c;
+						switch(blend)
+						{
+						case SW_BLEND_NONE: nc = CT::from_ci(lit); break;
+						case SW_BLEND_AVERAGE: nc = CT::average(dst_c, CT::from_ci(lit)); break;
+						case SW_BLEND_ADDITIVE: nc = CT::saturated_sum(dst_c, CT::from_ci(lit)); break;
+						case SW_BLEND_ALPHA_TEST: nc = Color32T::test_alpha(tv) ? CT::from_ci(lit) : dst_c; break;
+						case SW_BLEND_ALPHA_BLEND: nc = alpha_blend_ci<CT>(dst_c, lit, Color32T::get_alpha(tv)); break;
+						default: break;
+						}
+						acc[rr] = tix[g][rr] >= 0 ? nc : dst_c;
+					}
+				}
+				continue;
+			}
 			// Span records of the group's 16 items. SSPAN: each item's record is loaded through the
 			// scalar unit into SGPRs when the item is processed. Otherwise lane L < GD * ROWS loads
 			// the record of draw kg + L / ROWS at tile row L % ROWS and items read them with

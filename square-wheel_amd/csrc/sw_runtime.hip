@@ -907,6 +907,12 @@ int sw_map_upload(sw_ctx* ctx, const sw_map_desc* d, sw_map** out_map)
 		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
 	*out_map = nullptr;
 	(void)hipSetDevice(ctx->device);
+	// The kernels address each buffer with 32-bit byte offsets from its base.
+	const uint64_t lim = 1ull << 32;
+	if(d->num_texels >= lim / sizeof(sw_texel) || d->num_lite_texels >= lim / sizeof(uint32_t) ||
+		d->num_lightmap_elements >= lim / (3 * sizeof(float)) ||
+		d->num_directional_lightmap_elements >= lim / sizeof(sw_directional_lightmap_element))
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "map buffers above 4 GiB each (texels, lite texels, lightmaps)");
 	// Validate the texture table against its buffers.
 	for(uint32_t i = 0; i < d->num_levels; ++i)
 	{
@@ -921,8 +927,22 @@ int sw_map_upload(sw_ctx* ctx, const sw_map_desc* d, sw_map** out_map)
 	int r = SW_OK;
 	r = r ? r : upload(ctx, m->texels, d->texels, sizeof(sw_texel) * d->num_texels);
 	r = r ? r : upload(ctx, m->lite, d->lite_texels, sizeof(uint32_t) * d->num_lite_texels);
-	r = r ? r : upload(ctx, m->lm_simple, d->lightmaps_simple, sizeof(float) * 3 * d->num_lightmap_elements);
-	r = r ? r : upload(ctx, m->lm_dir, d->lightmaps_directional, sizeof(sw_directional_lightmap_element) * d->num_directional_lightmap_elements);
+	// Both lightmap kinds in one buffer (directional elements first, the simple ones at a 256-byte
+	// aligned offset), so that the kernels address either with one base and a 32-bit offset.
+	const uint64_t dir_bytes = sizeof(sw_directional_lightmap_element) * d->num_directional_lightmap_elements;
+	const uint64_t simple_off = (dir_bytes + 255) & ~uint64_t(255);
+	const uint64_t simple_bytes = sizeof(float) * 3 * d->num_lightmap_elements;
+	if(simple_off + simple_bytes >= (1ull << 32))
+		r = set_error(ctx, SW_ERR_INVALID_ARGUMENT, "lightmaps above 4 GiB");
+	r = r ? r : ensure(ctx, m->lm_dir, simple_off + simple_bytes);
+	if(!r && dir_bytes)
+		r = hip_check(ctx, hipMemcpyAsync(m->lm_dir.p, d->lightmaps_directional, dir_bytes, hipMemcpyHostToDevice, ctx->stream),
+			"hipMemcpyAsync H2D");
+	if(!r && simple_bytes)
+		r = hip_check(ctx,
+			hipMemcpyAsync(static_cast<char*>(m->lm_dir.p) + simple_off, d->lightmaps_simple, simple_bytes, hipMemcpyHostToDevice,
+				ctx->stream),
+			"hipMemcpyAsync H2D");
 	r = r ? r : upload(ctx, m->levels, d->levels, sizeof(sw_texture_level) * d->num_levels);
 	r = r ? r : upload(ctx, m->sky32, d->skybox_texels32, sizeof(uint32_t) * (d->skybox_texels32 ? d->num_skybox_texels32 : 0));
 	r = r ? r : upload(ctx, m->sky64, d->skybox_texels64, sizeof(uint64_t) * (d->skybox_texels64 ? d->num_skybox_texels64 : 0));
@@ -938,8 +958,9 @@ int sw_map_upload(sw_ctx* ctx, const sw_map_desc* d, sw_map** out_map)
 	m->host_levels.assign(d->levels, d->levels + d->num_levels);
 	m->ptrs.texels = static_cast<const sw_texel*>(m->texels.p);
 	m->ptrs.lite = static_cast<const uint32_t*>(m->lite.p);
-	m->ptrs.lm_simple = static_cast<const float*>(m->lm_simple.p);
 	m->ptrs.lm_dir = static_cast<const sw_directional_lightmap_element*>(m->lm_dir.p);
+	m->ptrs.lm_simple = reinterpret_cast<const float*>(static_cast<const char*>(m->lm_dir.p) + simple_off);
+	m->ptrs.lm_simple_offset = uint32_t(simple_off);
 	m->ptrs.levels = static_cast<const sw_texture_level*>(m->levels.p);
 	m->ptrs.num_levels = d->num_levels;
 	m->ptrs.num_texels = d->num_texels;

@@ -75,6 +75,28 @@ uint64_t sweep_fixed16_div(int32_t x_lo, int32_t x_hi, const int32_t* ys, uint32
 	return bad;
 }
 
+// The fast branch of z_from_inv_z for every divisor in [lo, hi] (within [1024, 2^24)) with its
+// reciprocal off by up to `ulps` units in the last place either way (the device's v_rcp_f32 is
+// within one): every result must still be 2^31 / div.
+uint64_t sweep_z_rcp_perturbed(uint32_t lo, uint32_t hi, int32_t ulps)
+{
+	uint64_t bad = 0;
+	for(uint64_t d = lo; d <= hi; ++d)
+	{
+		const float r = 1.0f / float(uint32_t(d));
+		uint32_t bits;
+		__builtin_memcpy(&bits, &r, 4);
+		for(int32_t k = -ulps; k <= ulps; ++k)
+		{
+			const uint32_t b = bits + uint32_t(k);
+			float rk;
+			__builtin_memcpy(&rk, &b, 4);
+			bad += swdiv::z_from_rcp(uint32_t(d), rk) != uint32_t((1ull << 31) / d);
+		}
+	}
+	return bad;
+}
+
 // z_from_inv_z for every divisor in [lo, hi].
 uint64_t sweep_z(uint32_t lo, uint32_t hi)
 {

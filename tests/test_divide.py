@@ -32,6 +32,8 @@ def lib():
     lib.sweep_fixed16_div.argtypes = [C.c_int32, C.c_int32, C.c_void_p, C.c_uint32]
     lib.sweep_z.restype = u64
     lib.sweep_z.argtypes = [C.c_uint32, C.c_uint32]
+    lib.sweep_z_rcp_perturbed.restype = u64
+    lib.sweep_z_rcp_perturbed.argtypes = [C.c_uint32, C.c_uint32, C.c_int32]
     for n in ("check_fixed16_div", "check_div_rcp", "check_mod_rcp", "check_z_from_inv_z"):
         getattr(lib, n).restype = None
     return lib
@@ -98,3 +100,10 @@ def test_z_from_inv_z_every_divisor(lib):
     z = np.zeros_like(d)
     lib.check_z_from_inv_z(_p(d), _p(z), d.size)
     assert np.array_equal(z, ((1 << 31) // d.astype(np.uint64)).astype(np.uint32))
+
+
+def test_z_fast_branch_tolerates_an_inexact_reciprocal(lib):
+    """The fast branch (1024 <= div < 2^24) starts from an approximate f32 reciprocal (the device's
+    v_rcp_f32, within 1 ulp); with the reciprocal off by up to 2 ulp either way its one correction
+    step still lands on 2^31 / div for every divisor."""
+    assert lib.sweep_z_rcp_perturbed(1024, (1 << 24) - 1, 2) == 0
