@@ -37,6 +37,11 @@ SW_DEV void raise_error(uint32_t* counters, uint32_t* host_err, uint32_t flag)
 	host_err[__builtin_ctz(flag)] = 1u;
 }
 
+// Wave lane exchange: pull `v` from lane `src` / push `v` to lane `dst` (ds_bpermute / ds_permute).
+SW_DEV int32_t lane_pull(int32_t v, uint32_t src) { return __builtin_amdgcn_ds_bpermute(int32_t(src << 2), v); }
+SW_DEV int32_t lane_push(int32_t v, uint32_t dst) { return __builtin_amdgcn_ds_permute(int32_t(dst << 2), v); }
+SW_DEV int32_t rl(int32_t v, uint32_t lane) { return __builtin_amdgcn_readlane(v, lane); }
+
 // Arena extent and validity of surface s (run by the extra blocks of k_setup_draws).
 SW_DEV uint32_t view_of_surface(const ViewsArg& views, uint32_t s)
 {
@@ -1068,6 +1073,65 @@ SW_DEV uint32_t part_rows(const int32_t clip[4], int32_t y_start, int32_t y_end)
 	return a < b ? uint32_t(b - a) : 0u;
 }
 
+// Span record of row k of a part (rasterizer.rs:316-422, :500-556, :673-703, :1000-1214); updates the
+// lane's bounds and pixel count.
+SW_DEV void emit_span_row(const sw_draw& d, const PolyEq& e, uint32_t kind, const int32_t clip[4], int32_t y_start_int,
+	uint32_t k, int32_t x_left0, int32_t ldx, int32_t x_right0, int32_t rdx, const AffinePart* ap, const TriPart* tp,
+	SpanRec* __restrict__ spans, uint64_t base, SetupState& st)
+{
+	const int32_t y = y_start_int + int32_t(k);
+	const int32_t x_left = wadd(x_left0, wmul(int32_t(k), ldx));
+	const int32_t x_right = wadd(x_right0, wmul(int32_t(k), rdx));
+	const int32_t x_start_int = imax32(fixed16_floor_to_int(x_left), clip[0]);
+	const int32_t x_end_int = imin32(fixed16_floor_to_int(x_right), clip[2]);
+	SpanRec r;
+	r.x0 = 0;
+	r.x1 = 0;
+	for(int i = 0; i < 6; ++i)
+		r.v[i] = 0;
+	if(x_start_int < x_end_int)
+	{
+		r.x0 = x_start_int;
+		r.x1 = x_end_int;
+		if(kind == 0)
+			span_perspective(e, d.texture_size, y, x_start_int, x_end_int, r);
+		else if(kind == 1)
+			span_line_z(e, d.texture_size, y, x_start_int, x_end_int, r);
+		else if(kind == 2)
+		{
+			const int32_t tl0 = wadd(ap->tc_left[0], wmul(int32_t(k), ap->d_tc_left[0]));
+			const int32_t tl1 = wadd(ap->tc_left[1], wmul(int32_t(k), ap->d_tc_left[1]));
+			const int32_t tr0 = wadd(ap->tc_right[0], wmul(int32_t(k), ap->d_tc_right[0]));
+			const int32_t tr1 = wadd(ap->tc_right[1], wmul(int32_t(k), ap->d_tc_right[1]));
+			const int32_t x_delta = imax32(x_right - x_left, FIXED16_ONE);
+			r.v[0] = tl0;
+			r.v[1] = fixed16_div(tr0 - tl0, x_delta);
+			r.v[2] = tl1;
+			r.v[3] = fixed16_div(tr1 - tl1, x_delta);
+		}
+		else
+		{
+			const int32_t x_start_delta = int_to_fixed16(x_start_int) + FIXED16_ONE - x_left;
+			const int32_t tl0 = wadd(tp->tc_left[0], wmul(int32_t(k), tp->d_tc_dy_left[0]));
+			const int32_t tl1 = wadd(tp->tc_left[1], wmul(int32_t(k), tp->d_tc_dy_left[1]));
+			r.v[0] = tl0 + fixed16_mul(x_start_delta, tp->d_tc_dx[0]);
+			r.v[1] = tl1 + fixed16_mul(x_start_delta, tp->d_tc_dx[1]);
+#pragma unroll
+			for(int i = 0; i < 3; ++i)
+			{
+				const int32_t ll = wadd(tp->light_left[i], wmul(int32_t(k), tp->d_light_left[i]));
+				r.v[2 + i] = ll + fixed16_mul(x_start_delta, tp->d_light_dx[i]);
+			}
+		}
+		st.x_min = imin32(st.x_min, x_start_int);
+		st.x_max = imax32(st.x_max, x_end_int);
+		st.row_min = imin32(st.row_min, y);
+		st.row_max = imax32(st.row_max, y + 1);
+		st.pixels += uint64_t(x_end_int - x_start_int);
+	}
+	spans[base + k] = r;
+}
+
 // Emits one part: rows [y_start_int, y_end_int) spread over the draw's lanes. Bounds and pixel
 // counts accumulate per thread (reduced once per draw).
 // kind: 0 perspective, 1 line z, 2 affine, 3 triangle.
@@ -1116,73 +1180,52 @@ SW_DEV void emit_part(const sw_draw& d, const PolyEq& e, uint32_t kind, const in
 	}
 	if(st.lane == 0)
 		parts[part_slot_base + st.num_parts] = pr;
-	st.num_parts++;
+	const uint32_t part_index = st.num_parts++;
 	if(!fits)
 	{
 		st.overflow = true;
 		return;
 	}
 
-	for(uint32_t k = st.lane; k < rows; k += st.lanes)
+	// Rows of the part over the draw's lanes. With several waves per draw, a part of fewer rows than
+	// lanes goes to one wave (parts in turn), so that the waves work on different parts at once.
+	uint32_t k0 = st.lane, kstep = st.lanes;
+	if(st.lanes > 64 && rows < st.lanes)
 	{
-		const int32_t y = y_start_int + int32_t(k);
-		const int32_t x_left = wadd(x_left0, wmul(int32_t(k), ls.dx_dy));
-		const int32_t x_right = wadd(x_right0, wmul(int32_t(k), rs.dx_dy));
-		const int32_t x_start_int = imax32(fixed16_floor_to_int(x_left), clip[0]);
-		const int32_t x_end_int = imin32(fixed16_floor_to_int(x_right), clip[2]);
-		SpanRec r;
-		r.x0 = 0;
-		r.x1 = 0;
-		for(int i = 0; i < 6; ++i)
-			r.v[i] = 0;
-		if(x_start_int < x_end_int)
-		{
-			r.x0 = x_start_int;
-			r.x1 = x_end_int;
-			if(kind == 0)
-				span_perspective(e, d.texture_size, y, x_start_int, x_end_int, r);
-			else if(kind == 1)
-				span_line_z(e, d.texture_size, y, x_start_int, x_end_int, r);
-			else if(kind == 2)
-			{
-				const int32_t tl0 = wadd(ap->tc_left[0], wmul(int32_t(k), ap->d_tc_left[0]));
-				const int32_t tl1 = wadd(ap->tc_left[1], wmul(int32_t(k), ap->d_tc_left[1]));
-				const int32_t tr0 = wadd(ap->tc_right[0], wmul(int32_t(k), ap->d_tc_right[0]));
-				const int32_t tr1 = wadd(ap->tc_right[1], wmul(int32_t(k), ap->d_tc_right[1]));
-				const int32_t x_delta = imax32(x_right - x_left, FIXED16_ONE);
-				r.v[0] = tl0;
-				r.v[1] = fixed16_div(tr0 - tl0, x_delta);
-				r.v[2] = tl1;
-				r.v[3] = fixed16_div(tr1 - tl1, x_delta);
-			}
-			else
-			{
-				const int32_t x_start_delta = int_to_fixed16(x_start_int) + FIXED16_ONE - x_left;
-				const int32_t tl0 = wadd(tp->tc_left[0], wmul(int32_t(k), tp->d_tc_dy_left[0]));
-				const int32_t tl1 = wadd(tp->tc_left[1], wmul(int32_t(k), tp->d_tc_dy_left[1]));
-				r.v[0] = tl0 + fixed16_mul(x_start_delta, tp->d_tc_dx[0]);
-				r.v[1] = tl1 + fixed16_mul(x_start_delta, tp->d_tc_dx[1]);
-#pragma unroll
-				for(int i = 0; i < 3; ++i)
-				{
-					const int32_t ll = wadd(tp->light_left[i], wmul(int32_t(k), tp->d_light_left[i]));
-					r.v[2 + i] = ll + fixed16_mul(x_start_delta, tp->d_light_dx[i]);
-				}
-			}
-			st.x_min = imin32(st.x_min, x_start_int);
-			st.x_max = imax32(st.x_max, x_end_int);
-			st.row_min = imin32(st.row_min, y);
-			st.row_max = imax32(st.row_max, y + 1);
-			st.pixels += uint64_t(x_end_int - x_start_int);
-		}
-		spans[base + k] = r;
+		if((part_index & (st.lanes / 64 - 1)) != (st.lane >> 6))
+			return;
+		k0 = st.lane & 63u;
+		kstep = 64;
 	}
+	for(uint32_t k = k0; k < rows; k += kstep)
+		emit_span_row(d, e, kind, clip, y_start_int, k, x_left0, ls.dx_dy, x_right0, rs.dx_dy, ap, tp, spans, base, st);
+}
+
+// The polygon's vertices, one per lane of the wave (lane i holds vertex i, i < num_vertices <= 24):
+// the edge walk reads them with readlane instead of a chain of dependent loads.
+struct LaneVerts
+{
+	int32_t x, y;
+	SW_DEV int32_t vx(uint32_t i) const { return __builtin_amdgcn_readlane(x, i); }
+	SW_DEV int32_t vy(uint32_t i) const { return __builtin_amdgcn_readlane(y, i); }
+};
+
+SW_DEV LaneVerts load_lane_verts(const int32_t* __restrict__ v, uint32_t n)
+{
+	const uint32_t lane = threadIdx.x & 63u;
+	LaneVerts lv{0, 0};
+	if(lane < n)
+	{
+		lv.x = v[lane * 2];
+		lv.y = v[lane * 2 + 1];
+	}
+	return lv;
 }
 
 // The polygon edge walk of rasterizer.rs:160-270, executed identically by every lane of the wave.
 // EMIT = false only counts the rows of the parts (no divisions); EMIT = true writes parts / spans.
 template <bool EMIT>
-SW_DEV uint32_t walk_polygon(const sw_draw& d, const int32_t* __restrict__ v, const int32_t clip[4],
+SW_DEV uint32_t walk_polygon(const sw_draw& d, const LaneVerts& v, const int32_t clip[4],
 	PartRec* __restrict__ parts, SpanRec* __restrict__ spans, uint32_t part_slot_base, SetupState& st)
 {
 	const uint32_t n = d.num_vertices;
@@ -1191,11 +1234,11 @@ SW_DEV uint32_t walk_polygon(const sw_draw& d, const int32_t* __restrict__ v, co
 	if(EMIT)
 		e = make_poly_eq(d);
 	uint32_t lower = 0;
-	int32_t min_y = v[1];
+	int32_t min_y = v.vy(0);
 	for(uint32_t i = 0; i < n; ++i)
-		if(v[i * 2 + 1] < min_y)
+		if(v.vy(i) < min_y)
 		{
-			min_y = v[i * 2 + 1];
+			min_y = v.vy(i);
 			lower = i;
 		}
 	uint32_t left = lower, right = lower;
@@ -1209,8 +1252,8 @@ SW_DEV uint32_t walk_polygon(const sw_draw& d, const int32_t* __restrict__ v, co
 		uint32_t nr = right + 1;
 		if(nr >= n)
 			nr -= n;
-		const int32_t lx = v[left * 2], ly = v[left * 2 + 1], nlx = v[nl * 2], nly = v[nl * 2 + 1];
-		const int32_t rx = v[right * 2], ry = v[right * 2 + 1], nrx = v[nr * 2], nry = v[nr * 2 + 1];
+		const int32_t lx = v.vx(left), ly = v.vy(left), nlx = v.vx(nl), nly = v.vy(nl);
+		const int32_t rx = v.vx(right), ry = v.vy(right), nrx = v.vx(nr), nry = v.vy(nr);
 		const int32_t dy_left = nly - ly;
 		const int32_t dy_right = nry - ry;
 		const int32_t next_y = imin32(nly, nry);
@@ -1256,6 +1299,162 @@ SW_DEV uint32_t walk_polygon(const sw_draw& d, const int32_t* __restrict__ v, co
 		cur_y = next_y;
 	}
 	return rows;
+}
+
+// Parts and spans of a polygon with the per-part arithmetic spread over lanes. The edge walk of
+// walk_polygon runs first with comparisons only (identical on every lane) and keeps, in lane p, part
+// p's edges, rows and span base with emit_part's bookkeeping; lane p then computes the part's sides
+// (the fixed16 divisions), start positions and affine set-up and writes its PartRec; last, every
+// part's rows are spread over the draw's lanes, row r of the draw's run on lane r mod lanes.
+struct LanePart
+{
+	int32_t cur_y, next_y, y_start_int;
+	uint32_t l, nl, r, nr, wide, rows, row_begin;
+	uint64_t base;
+	int32_t row1;
+};
+
+SW_DEV void emit_polygon(const sw_draw& d, const LaneVerts& v, const int32_t clip[4], PartRec* __restrict__ parts,
+	SpanRec* __restrict__ spans, uint32_t part_slot_base, SetupState& st)
+{
+	const uint32_t n = d.num_vertices;
+	const uint32_t kind = d.mode == SW_TC_LINE_Z ? 1u : (d.mode == SW_TC_AFFINE ? 2u : 0u);
+	const uint32_t lane = threadIdx.x & 63u;
+	uint32_t lower = 0;
+	int32_t min_y = v.vy(0);
+	for(uint32_t i = 0; i < n; ++i)
+		if(v.vy(i) < min_y)
+		{
+			min_y = v.vy(i);
+			lower = i;
+		}
+	uint32_t left = lower, right = lower;
+	int32_t cur_y = min_y;
+	LanePart mp{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+	uint32_t total_rows = 0;
+	for(uint32_t iter = 0; iter < n; ++iter)
+	{
+		uint32_t nl = left + n - 1;
+		if(nl >= n)
+			nl -= n;
+		uint32_t nr = right + 1;
+		if(nr >= n)
+			nr -= n;
+		const int32_t ly = v.vy(left), nly = v.vy(nl), ry = v.vy(right), nry = v.vy(nr);
+		const int32_t dy_left = nly - ly;
+		const int32_t dy_right = nry - ry;
+		const int32_t next_y = imin32(nly, nry);
+		const bool wide = dy_left > FIXED16_HALF && dy_right > FIXED16_HALF;
+		const bool thin = !wide && dy_left > 0 && dy_right > 0 && fixed16_round_to_int(cur_y) < fixed16_round_to_int(next_y);
+		if(wide || thin)
+		{
+			const int32_t y_start_int = imax32(fixed16_round_to_int(cur_y), clip[1]);
+			const int32_t y_end_int = imin32(fixed16_round_to_int(next_y), clip[3]);
+			if(y_start_int < y_end_int && st.num_parts < MAX_PARTS_PER_DRAW)
+			{
+				if(st.num_parts == 0)
+				{
+					st.row_first = y_start_int;
+					st.rows0 = uint32_t(y_end_int - y_start_int);
+				}
+				else if(y_start_int != st.row_next)
+					st.gap = true;
+				st.row_next = y_end_int;
+				const uint32_t rows = uint32_t(y_end_int - y_start_int);
+				const uint64_t base = st.next_span;
+				const bool fits = !st.overflow && base + rows <= st.span_end;
+				st.next_span += rows;
+				if(lane == st.num_parts)
+					mp = LanePart{cur_y, next_y, y_start_int, left, nl, right, nr, wide ? 1u : 0u, fits ? rows : 0u, total_rows,
+						base, fits ? y_end_int : y_start_int};
+				st.num_parts++;
+				if(fits)
+					total_rows += rows;
+				else
+					st.overflow = true;
+			}
+		}
+		if(nl == nr)
+			break;
+		if(nry < nly)
+			right = nr;
+		else
+			left = nl;
+		cur_y = next_y;
+	}
+	const uint32_t np = st.num_parts;
+	if(np == 0)
+		return;
+
+	// per-part set-up, lane p for part p (every lane takes part in the lane exchanges)
+	const int32_t lx = lane_pull(v.x, mp.l), lyv = lane_pull(v.y, mp.l), nlx = lane_pull(v.x, mp.nl), nlyv = lane_pull(v.y, mp.nl);
+	const int32_t rx = lane_pull(v.x, mp.r), ryv = lane_pull(v.y, mp.r), nrx = lane_pull(v.x, mp.nr), nryv = lane_pull(v.y, mp.nr);
+	int32_t x_left0 = 0, x_right0 = 0, ldx = 0, rdx = 0;
+	AffinePart ap{{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+	if(lane < np)
+	{
+		Side ls, rs;
+		if(mp.wide)
+		{
+			const int32_t dxl = fixed16_div(nlx - lx, nlyv - lyv);
+			const int32_t dxr = fixed16_div(nrx - rx, nryv - ryv);
+			ls = Side{lx + fixed16_mul(dxl, mp.cur_y - lyv), dxl};
+			rs = Side{rx + fixed16_mul(dxr, mp.cur_y - ryv), dxr};
+		}
+		else
+		{
+			const int32_t thin_y = int_to_fixed16(fixed16_round_to_int(mp.cur_y)) + FIXED16_HALF;
+			ls = Side{lx + fixed16_mul_div(thin_y - lyv, nlx - lx, nlyv - lyv), 0};
+			rs = Side{rx + fixed16_mul_div(thin_y - ryv, nrx - rx, nryv - ryv), 0};
+		}
+		const int32_t y_start_delta = int_to_fixed16(mp.y_start_int) + FIXED16_HALF - mp.cur_y;
+		x_left0 = ls.x_start + fixed16_mul(y_start_delta, ls.dx_dy) + FIXED16_HALF;
+		x_right0 = rs.x_start + fixed16_mul(y_start_delta, rs.dx_dy) + FIXED16_HALF;
+		ldx = ls.dx_dy;
+		rdx = rs.dx_dy;
+		if(kind == 2)
+			ap = affine_part_setup(d, mp.cur_y, mp.next_y, ls, rs, y_start_delta);
+		if(st.lane < 64) // the first wave of the draw
+		{
+			PartRec pr;
+			pr.row0 = mp.y_start_int;
+			pr.row1 = mp.row1;
+			pr.span_base = uint32_t(mp.base);
+			pr.kind = kind;
+			for(int i = 0; i < 5; ++i)
+				pr.d[i] = 0;
+			parts[part_slot_base + lane] = pr;
+		}
+	}
+
+	// Rows: part q's row k is row row_begin(q) + k of the draw's run. Every readlane of part q's
+	// values happens while all lanes are active (the compiler may reuse an inactive lane's copy of
+	// a value inside a divergent region).
+	const PolyEq e = make_poly_eq(d);
+	const uint32_t lanes = st.lanes;
+	for(uint32_t q = 0; q < np; ++q)
+	{
+		const uint32_t rows = uint32_t(__builtin_amdgcn_readlane(int32_t(mp.rows), q));
+		if(rows == 0)
+			continue; // the part overflowed (and so did every later one); uniform
+		const uint32_t begin = uint32_t(__builtin_amdgcn_readlane(int32_t(mp.row_begin), q));
+		const int32_t y_start_int = __builtin_amdgcn_readlane(mp.y_start_int, q);
+		const uint64_t base = uint64_t(uint32_t(__builtin_amdgcn_readlane(int32_t(uint32_t(mp.base)), q))) |
+			(uint64_t(uint32_t(__builtin_amdgcn_readlane(int32_t(uint32_t(mp.base >> 32)), q))) << 32);
+		const int32_t xl0 = __builtin_amdgcn_readlane(x_left0, q), xr0 = __builtin_amdgcn_readlane(x_right0, q);
+		const int32_t qldx = __builtin_amdgcn_readlane(ldx, q), qrdx = __builtin_amdgcn_readlane(rdx, q);
+		AffinePart qap;
+		if(kind == 2)
+			for(int i = 0; i < 2; ++i)
+			{
+				qap.tc_left[i] = __builtin_amdgcn_readlane(ap.tc_left[i], q);
+				qap.tc_right[i] = __builtin_amdgcn_readlane(ap.tc_right[i], q);
+				qap.d_tc_left[i] = __builtin_amdgcn_readlane(ap.d_tc_left[i], q);
+				qap.d_tc_right[i] = __builtin_amdgcn_readlane(ap.d_tc_right[i], q);
+			}
+		for(uint32_t k = (st.lane - begin) & (lanes - 1u); k < rows; k += lanes)
+			emit_span_row(d, e, kind, clip, y_start_int, k, xl0, qldx, xr0, qrdx, &qap, nullptr, spans, base, st);
+	}
 }
 
 // The triangle split of rasterizer.rs:765-982 (sorting, long edge, middle vertex).
@@ -1560,6 +1759,11 @@ SW_DEV void setup_draws_block(uint32_t block, const SetupArgs& a, const ViewsArg
 		v = vdata + d.first_vertex_int;
 	}
 
+	// a polygon's vertices, one per lane (a triangle's 21 values load at fixed offsets)
+	LaneVerts lverts{0, 0};
+	if(active && valid && d.kind == SW_DRAW_POLYGON)
+		lverts = load_lane_verts(v, d.num_vertices);
+
 	SetupState st;
 	st.lane = lane;
 	st.lanes = LANES;
@@ -1590,7 +1794,7 @@ SW_DEV void setup_draws_block(uint32_t block, const SetupArgs& a, const ViewsArg
 	{
 		// pass 1: rows of every part (no divisions), one span allocation per block
 		if(active && valid)
-			rows = d.kind == SW_DRAW_POLYGON ? walk_polygon<false>(d, v, clip, parts, spans, part_slot_base, st)
+			rows = d.kind == SW_DRAW_POLYGON ? walk_polygon<false>(d, lverts, clip, parts, spans, part_slot_base, st)
 											 : triangle_rows(v, clip);
 		if(lane == 0)
 			group_rows[group] = rows;
@@ -1617,7 +1821,7 @@ SW_DEV void setup_draws_block(uint32_t block, const SetupArgs& a, const ViewsArg
 	if(active && valid && !st.overflow)
 	{
 		if(d.kind == SW_DRAW_POLYGON)
-			walk_polygon<true>(d, v, clip, parts, spans, part_slot_base, st);
+			emit_polygon(d, lverts, clip, parts, spans, part_slot_base, st);
 		else
 			emit_triangle(d, v, clip, parts, spans, part_slot_base, st);
 	}
@@ -1806,10 +2010,6 @@ SW_DEV T global_at(uint64_t addr, int32_t i)
 	return *(GlobalPtr)((GlobalBytes)addr + uint32_t(i) * uint32_t(sizeof(T)));
 }
 
-// Wave lane exchange: pull `v` from lane `src` / push `v` to lane `dst` (ds_bpermute / ds_permute).
-SW_DEV int32_t lane_pull(int32_t v, uint32_t src) { return __builtin_amdgcn_ds_bpermute(int32_t(src << 2), v); }
-SW_DEV int32_t lane_push(int32_t v, uint32_t dst) { return __builtin_amdgcn_ds_permute(int32_t(dst << 2), v); }
-SW_DEV int32_t rl(int32_t v, uint32_t lane) { return __builtin_amdgcn_readlane(v, lane); }
 
 // Texel indices of a draw's active rows in the tile (bit rr of gmask: row rr's span meets the
 // tile), the span values read from lane g * ROWS + rr: -1 outside the span or the texture. KIND 0:
