@@ -56,14 +56,15 @@ enum
 
 // Surface texels are built by blocks of SURF_BLOCK threads over the flattened arena, each block
 // surf_block_texels() consecutive texels: up to SURF_MAX_TEXELS_PER_THREAD per thread (the block's
-// per-surface set-up is shared by more texels), fewer when the arena is too small to fill the
-// device with several rounds of blocks.
+// per-surface set-up is shared by more texels). A small arena (one frame) takes as few texels per
+// thread as keep it within about 1024 blocks, which the device runs at once (five blocks per CU
+// besides the draw blocks): a second round of blocks would add a whole block's latency.
 #define SURF_BLOCK 256
 #define SURF_MAX_TEXELS_PER_THREAD 4
 __host__ __device__ inline uint32_t surf_block_texels(uint64_t arena_texels)
 {
-	const uint64_t rounds = arena_texels / (uint64_t(SURF_BLOCK) * 5120u);
-	return SURF_BLOCK * (rounds >= 4 ? 4u : rounds >= 2 ? 2u : 1u);
+	const uint64_t per_thread = (arena_texels + uint64_t(SURF_BLOCK) * 1024u - 1) / (uint64_t(SURF_BLOCK) * 1024u);
+	return SURF_BLOCK * uint32_t(per_thread < 1 ? 1 : per_thread > SURF_MAX_TEXELS_PER_THREAD ? SURF_MAX_TEXELS_PER_THREAD : per_thread);
 }
 
 // Arena extent of one surface and the view it belongs to.
