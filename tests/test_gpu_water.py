@@ -60,6 +60,53 @@ def test_water_texture_bit_exact(gpu_ctx, mode, init):
     assert bad.size == 0, f"{bad.size} texels differ, first {bad[:5].tolist()}"
 
 
+def _water_levels(m, lw, lh, rng):
+    """Levels 0-4 of the map rewritten as a 2^lw x 2^lh texture and its four mips (with a normal map)
+    in texels appended to the map, level 5 as a colour texture of the same size."""
+    d = m.data
+    w, h = 1 << lw, 1 << lh
+    sizes = [(max(w >> k, 1), max(h >> k, 1)) for k in range(5)]
+    extra = sum(a * b for a, b in sizes) + w * h
+    base = len(d.texels)
+    add = np.zeros(extra, dtype=d.texels.dtype)
+    add.view(np.uint32).reshape(-1, 2)[:, 0] = rng.integers(0, 2 ** 32, extra, dtype=np.uint64).astype(np.uint32)
+    d.texels = np.concatenate([d.texels, add])
+    off = base
+    for k, (a, b) in enumerate(sizes):
+        d.levels[k]["offset"], d.levels[k]["width"], d.levels[k]["height"] = off, a, b
+        d.levels[k]["flags"] |= 1
+        off += a * b
+    d.levels[5]["offset"], d.levels[5]["width"], d.levels[5]["height"] = off, w, h
+    d.levels[5]["flags"] &= ~np.uint32(A.TEX_LITE_LEVEL if hasattr(A, "TEX_LITE_LEVEL") else 0)
+
+
+@pytest.mark.parametrize("log2", [(4, 4), (7, 5), (5, 8), (10, 10)])
+@pytest.mark.parametrize("mode", [A.WATER_COLOR_KEEP, A.WATER_COLOR_DEFORMED, A.WATER_COLOR_DEFORMED_X])
+def test_water_texture_sizes_bit_exact(gpu_ctx, log2, mode):
+    """Non-square and single-tile sizes and the bench's 1024 x 1024: the tile split and every mip's
+    write stride, all five levels against the oracle."""
+    lw, lh = log2
+    m = sw_host.Map(seed=3, grid=(10, 10, 4), texture_size=64, num_materials=3, num_decal_textures=1, num_model_textures=1)
+    rng = np.random.default_rng(lw * 31 + lh * 7 + mode)
+    _water_levels(m, lw, lh, rng)
+    field = (rng.normal(size=1 << (lw + lh)) * 0.8).astype(np.float32)
+    params = A.WaterTextureParams(0, 5, mode, A.WATER_INIT_NONE, 0.3, 0x80604020)
+    print(f"water texture {lw}x{lh} mode {mode}", flush=True)
+    mh = gpu_ctx.upload_map(m.data)
+    try:
+        gpu_ctx.water_make_texture(mh, field, lw, lh, params)
+        got = gpu_ctx.map_read(mh, 0, 0, len(m.data.texels))
+    finally:
+        gpu_ctx.release_map(mh)
+    ref = oracle.water_make_texture(m.data, field, lw, lh, params)
+    lv = m.data.levels
+    for k in range(5):
+        o, n = int(lv[k]["offset"]), int(lv[k]["width"]) * int(lv[k]["height"])
+        bad = np.flatnonzero(got[o:o + n].view(np.uint64) != ref[o:o + n].view(np.uint64))
+        assert bad.size == 0, f"mip {k}: {bad.size} texels differ, first {bad[:5].tolist()}"
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
 def test_water_validation(gpu_ctx):
     m = sw_host.Map(seed=3, grid=(10, 10, 4), texture_size=64, num_materials=3, num_decal_textures=1, num_model_textures=1)
     m.data.levels["flags"][0:5] &= ~np.uint32(1)
