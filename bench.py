@@ -675,7 +675,7 @@ def main():
 
     # tile height for this frame size, chosen by the autotuner on the first pose
     plain_rows, plain_tune = ctx.autotune(mh, batches[0], dev[0], out)
-    log(f"autotune (plain): {plain_rows} rows; ms per frame by rows {plain_tune}")
+    log(f"autotune (plain): {plain_rows} rows, scalar spans {ctx.scalar_spans()}; ms per frame by rows {plain_tune}")
     for i in range(args.warmup):
         render(i)
     # Timed region: K frames of the path, each call returns with its frame finished.
@@ -762,7 +762,8 @@ def main():
     launches = max(prof[dominant][1], 1)
     avg_s = prof[dominant][0] / 1e3 / launches
     achieved = bytes_sum[dominant] / launches / avg_s / 1e9
-    traffic_key = {"raster_tiles": f"k_raster_tiles_rows{plain_rows}", "setup_and_surfaces": "k_setup_and_surfaces_256",
+    raster_key = f"k_raster_tiles_rows{plain_rows}" + ("_sspan" if ctx.scalar_spans() else "")
+    traffic_key = {"raster_tiles": raster_key, "setup_and_surfaces": "k_setup_and_surfaces_256",
                    "build_surfaces": "k_build_surfaces"}.get(dominant)
     traffic = read_traffic("plain1080p", traffic_key) if traffic_key else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
