@@ -898,7 +898,7 @@ class Server:
         takes the result buffers from a pool of pinned host memory the device copies into directly
         (hand them back with release())."""
         v = frame["view"]
-        w, h = int(v["width"]), int(v["height"])
+        w, h = int(np.asarray(v["width"]).item()), int(np.asarray(v["height"]).item())
         bits = int(frame["color_bits"])
         want_tm = tonemap_exposure is not None and bits == 64
         fb = tm = None
