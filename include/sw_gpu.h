@@ -309,6 +309,26 @@ int sw_build_surfaces(
  * NULL / 0: none), kept for the context's following frames. */
 int sw_set_portal_textures(sw_ctx* ctx, const void* portal_texels_device, uint64_t num_portal_texels);
 
+/* A portal texture's overlay: mix_surface_with_texture of a lite texture level over the portal's
+ * rect of the portal textures, after its view has been drawn there and before any draw samples it
+ * (the portal.texture case of build_view_portals_textures, partial_renderer.rs:2205-2276).
+ * Texel (x, y) of the rect takes the level's texel ((tc_start[0] + x) rem_euclid w, (tc_start[1] + y)
+ * rem_euclid h), modulated by cvtps2dq(light * 256) per channel, blended by `blend`. */
+typedef struct sw_portal_mix
+{
+	uint64_t dst_offset; /* the rect's first texel in the portal textures; row pitch = size[0] */
+	uint32_t size[2];
+	int32_t tc_start[2]; /* the portal's tc_min at its mip */
+	uint32_t level;      /* a lite level of the map (the portal texture at the portal's mip) */
+	uint32_t blend;      /* SW_BLEND_* */
+	float light[3];      /* R, G, B: light_scale x the light grid's light at the portal centre + light_add */
+	uint32_t reserved;
+} sw_portal_mix;
+
+/* Applies the overlays in order (rects of one call must not overlap) to device ColorT texels. */
+int sw_mix_portal_textures(sw_ctx* ctx, sw_map* map, uint32_t color_bits, const sw_portal_mix* mixes, uint32_t num_mixes,
+	void* portal_texels_device, uint64_t num_portal_texels);
+
 int sw_raster_draws(
 	sw_ctx* ctx, const sw_draw* draws, uint32_t num_draws, const int32_t* vertex_data, uint64_t num_vertex_ints);
 int sw_frame_end(sw_ctx* ctx, void* dst, size_t dst_pitch_bytes, uint32_t dst_on_device);

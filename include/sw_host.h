@@ -130,6 +130,12 @@ typedef struct swh_view_portal
 	float camera_position[3];
 	float camera_yaw, camera_pitch, camera_fov;
 	float transform[16]; /* column-major */
+	/* ViewPortalTexture (frame_info.rs:198-205): texture_level is the lite level of the overlay's
+	 * mip 0 (its mips follow), SW_NO_TEXTURE for none */
+	uint32_t texture_level;
+	uint32_t texture_blending_mode;
+	float texture_light_scale;
+	float texture_light_add[3];
 } swh_view_portal;
 int swh_map_view_portals(const swh_map* map, swh_view_portal* out, uint32_t capacity);
 /* n seeded camera poses at empty-cell centres: out = n x {x, y, z, yaw, pitch}. */
@@ -150,6 +156,13 @@ typedef struct swh_portal_info
 	int32_t tc_min[2];
 	uint32_t mip;
 	uint64_t texture_offset;
+	/* the portal's texture overlay (partial_renderer.rs:2205-2276), for sw_mix_portal_textures:
+	 * mix_level = the overlay's lite level at `mip` (SW_NO_TEXTURE: none), its blending mode and the
+	 * light at the portal's centre (R, G, B) */
+	uint32_t mix_level;
+	uint32_t mix_blend;
+	float mix_light[3];
+	uint32_t reserved;
 } swh_portal_info;
 int swh_frame_portal_info(const swh_frame* frame, uint32_t portal_index, swh_portal_info* out_info);
 /* The index-th view seen through a visible portal of the frame (index < num_portal_views); valid until

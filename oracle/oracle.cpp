@@ -5,7 +5,8 @@
 //
 // It restates, loop for loop, the reference's Rust code:
 //   - surface builder      square_wheel/surfaces.rs:433-852 (build_surface_impl_6_static_params),
-//                          :854-971 (mix_surface_with_texture), :973-1086 (lightmap element ops),
+//                          :854-971 (mix_surface_with_texture, also the portal textures' overlay of
+//                          partial_renderer.rs:2205-2276), :973-1086 (lightmap element ops),
 //                          :1088-1154 (specular / Fresnel helpers)
 //   - span rasteriser      square_wheel/rasterizer.rs:137-271 (edge walk), :273-454 (perspective),
 //                          :456-581 (line z corrected), :583-729 (affine), :765-1292 (triangles),
@@ -2745,6 +2746,40 @@ int oracle_apply_texture_ops(const sw_texture_level* levels, uint32_t num_levels
 			texture_layer(texels + dl.offset, dl, texels + sl.offset, sl, op.shift, op.light, op.blend);
 		else
 			lite_layer(lite + dl.offset, dl, lite + sl.offset, sl, op.shift, op.light, op.blend);
+	}
+	return 0;
+}
+
+// The portal textures' overlays (build_view_portals_textures, partial_renderer.rs:2205-2276): for
+// each portal with a texture, mix_surface_with_texture (surfaces.rs:903-971) of the texture's level
+// at the portal's mip over the portal's rect, in order.
+int oracle_mix_portal_textures(const sw_map_desc* map, uint32_t color_bits, const sw_portal_mix* mixes, uint32_t num_mixes,
+	void* texels, uint64_t num_texels)
+{
+	if(!map || (color_bits != 32 && color_bits != 64))
+		return -1;
+	MapData m{};
+	m.lite_texels = map->lite_texels;
+	m.levels = map->levels;
+	for(uint32_t i = 0; i < num_mixes; ++i)
+	{
+		const sw_portal_mix& x = mixes[i];
+		if(x.level >= map->num_levels || !(map->levels[x.level].flags & SW_TEX_LITE_LEVEL))
+			return -1;
+		if(x.dst_offset + uint64_t(x.size[0]) * x.size[1] > num_texels)
+			return -1;
+		sw_surface_desc s{};
+		s.size[0] = x.size[0];
+		s.size[1] = x.size[1];
+		s.tc_start[0] = x.tc_start[0];
+		s.tc_start[1] = x.tc_start[1];
+		s.emissive_level = x.level;
+		for(int c = 0; c < 3; ++c)
+			s.emissive_light[c] = x.light[c];
+		if(color_bits == 32)
+			mix_surface_with_texture<C32>(m, s, x.blend, static_cast<uint32_t*>(texels) + x.dst_offset, nullptr);
+		else
+			mix_surface_with_texture<C64>(m, s, x.blend, static_cast<uint64_t*>(texels) + x.dst_offset, nullptr);
 	}
 	return 0;
 }

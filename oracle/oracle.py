@@ -37,6 +37,9 @@ def load():
         lib.oracle_build_surfaces.restype = C.c_int
         lib.oracle_build_surfaces.argtypes = [C.c_int, C.POINTER(sw_abi.MapDesc), C.POINTER(sw_abi.Batch), C.c_void_p,
                                               C.POINTER(sw_abi.Stats)]
+        lib.oracle_mix_portal_textures.restype = C.c_int
+        lib.oracle_mix_portal_textures.argtypes = [C.POINTER(sw_abi.MapDesc), C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                                   C.c_uint64]
         lib.oracle_render_depth.restype = C.c_int
         lib.oracle_render_depth.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.oracle_set_light_shadows.restype = None
@@ -117,11 +120,22 @@ def render(map_data, batch, mode=MODE_EXACT, num_threads=1, arena=None, portal=N
     return fb[:batch.fb_texels], arena, st.as_dict(), int(oob.value)
 
 
-def render_portal_levels(map_data, batches, mode=MODE_EXACT):
+def mix_portal_textures(map_data, color_bits, mixes, texels):
+    """The portal textures' overlays (partial_renderer.rs:2205-2276) over `texels` in place."""
+    lib = load()
+    mixes = np.ascontiguousarray(mixes, dtype=sw_abi.PORTAL_MIX_DTYPE)
+    md = map_data.desc()
+    r = lib.oracle_mix_portal_textures(C.byref(md), color_bits, C.c_void_p(mixes.ctypes.data) if mixes.size else None,
+                                       int(mixes.size), C.c_void_p(texels.ctypes.data), int(texels.size))
+    assert r == 0
+
+
+def render_portal_levels(map_data, batches, mode=MODE_EXACT, mixes=None):
     """A frame and the views through its portals (one batch per portal level, the frame's own
     first): the deepest level samples zero portal textures, each level above the framebuffer of the
-    level below (build_view_portals_textures, partial_renderer.rs:2008-2200). Returns every level's
-    framebuffer and the number of out-of-bounds fetches over all levels."""
+    level below (build_view_portals_textures, partial_renderer.rs:2008-2200); with `mixes`
+    (portal_level_mixes) each level's portal textures get their overlays first (:2205-2276).
+    Returns every level's framebuffer and the number of out-of-bounds fetches over all levels."""
     outs = [None] * len(batches)
     below = None
     oob_total = 0
@@ -130,6 +144,9 @@ def render_portal_levels(map_data, batches, mode=MODE_EXACT):
         if below is None:
             below = np.zeros(max(b.portal_texels, 1), dtype=b.color_dtype)
         assert below.size >= b.portal_texels
+        if mixes is not None and len(mixes[k + 1]):
+            below = below.copy()
+            mix_portal_textures(map_data, b.color_bits, mixes[k + 1], below)
         fb, _, _, oob = render(map_data, b, mode=mode, portal=below)
         oob_total += oob
         outs[k] = fb

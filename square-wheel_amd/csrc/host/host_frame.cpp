@@ -1234,6 +1234,27 @@ void prepare_view_portals(swh_frame& f, const swh_map& m, const swh_frame_config
 		info.tc_min[1] = tmin[1];
 		info.mip = mip;
 		f.portal_texels += uint64_t(info.resolution[0]) * info.resolution[1];
+		if(portal.texture_level != SW_NO_TEXTURE)
+		{
+			// partial_renderer.rs:2228-2258: the light at the centre of the portal's rect on its
+			// plane (mip basis), scaled and offset; the texture's level at the portal's mip
+			info.mix_level = portal.texture_level + mip;
+			info.mix_blend = portal.texture_blending_mode;
+			float light[3] = {0.0f, 0.0f, 0.0f};
+			if(portal.texture_light_scale > 0.0f)
+			{
+				swh_frame::Basis b = form_basis(portal.plane, portal.tex_coord_equation);
+				const float scale = float(1u << mip);
+				b.u = b.u * scale;
+				b.v = b.v * scale;
+				const V3 centre = b.start + b.u * (float(tmin[0]) + float(info.resolution[0]) * 0.5f) +
+					b.v * (float(tmin[1]) + float(info.resolution[1]) * 0.5f);
+				grid_surface_light(m, centre, b.normal, light);
+			}
+			const float add[3] = {portal.texture_light_add.x, portal.texture_light_add.y, portal.texture_light_add.z};
+			for(int i = 0; i < 3; ++i)
+				info.mix_light[i] = light[i] * portal.texture_light_scale + add[i];
+		}
 	}
 }
 
@@ -1481,6 +1502,10 @@ int swh_frame_portal_info(const swh_frame* f, uint32_t index, swh_portal_info* o
 	out->tc_min[1] = p.tc_min[1];
 	out->mip = p.mip;
 	out->texture_offset = p.texture_offset;
+	out->mix_level = p.visible ? p.mix_level : SW_NO_TEXTURE;
+	out->mix_blend = p.mix_blend;
+	for(int i = 0; i < 3; ++i)
+		out->mix_light[i] = p.mix_light[i];
 	return 0;
 }
 

@@ -219,6 +219,11 @@ struct swh_frame
 		int32_t tc_min[2] = {0, 0};
 		uint32_t mip = 0;
 		std::vector<uint32_t> leafs; // the leaves the portal polygon's convex hull reaches
+		// the overlay of a portal with a texture (partial_renderer.rs:2205-2276): the texture's
+		// level at the portal's mip, and the light at the portal's centre
+		uint32_t mix_level = SW_NO_TEXTURE;
+		uint32_t mix_blend = SW_BLEND_NONE;
+		float mix_light[3] = {0.0f, 0.0f, 0.0f};
 	};
 	std::vector<PortalInfo> portal_info;
 	std::vector<std::vector<uint32_t>> leaf_portals; // per leaf: portal indices

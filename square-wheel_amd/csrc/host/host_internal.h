@@ -130,6 +130,11 @@ struct ViewPortalDef
 	V3 camera_position; // PORTAL_CAMERA
 	float camera_yaw, camera_pitch, camera_fov;
 	M4 transform; // PORTAL_PARALLAX: camera position -> the portal's view point
+	// ViewPortalTexture (frame_info.rs:198-205): an overlay mixed over the portal texture, or none
+	uint32_t texture_level = SW_NO_TEXTURE; // lite level of the texture's mip 0 (its mips follow)
+	uint32_t texture_blending_mode = SW_BLEND_AVERAGE;
+	float texture_light_scale = 1.0f;
+	V3 texture_light_add{0.0f, 0.0f, 0.0f};
 };
 
 struct Portal

@@ -903,6 +903,19 @@ void generate_view_portals(swh_map& m, uint32_t count, uint32_t seed)
 		const uint32_t tx = target % m.gx, ty = (target / m.gx) % m.gy, tz = target / (m.gx * m.gy);
 		p.transform =
 			from_translation(v3(CELL * (float(tx) - float(x)), CELL * (float(ty) - float(y)), CELL * (float(tz) - float(z))));
+		// every second portal carries a texture overlay (get_portal_texture, test_game/world_spawn.rs:1009-1041:
+		// light scale 1, no added light): an opaque one averaged in (the default blending), or a
+		// round alpha-masked one blended by its alpha
+		if(k % 4 == 1 && !m.model_levels.empty())
+		{
+			p.texture_level = m.model_levels[(k / 4) % m.model_levels.size()];
+			p.texture_blending_mode = SW_BLEND_AVERAGE;
+		}
+		else if(k % 4 == 3 && !m.decal_levels.empty())
+		{
+			p.texture_level = m.decal_levels[(k / 4) % m.decal_levels.size()];
+			p.texture_blending_mode = SW_BLEND_ALPHA_BLEND;
+		}
 		m.view_portals.push_back(p);
 	}
 }
@@ -1348,6 +1361,11 @@ int swh_map_view_portals(const swh_map* m, swh_view_portal* out, uint32_t capaci
 			o.transform[c * 4 + 0] = p.transform.c[c].x, o.transform[c * 4 + 1] = p.transform.c[c].y;
 			o.transform[c * 4 + 2] = p.transform.c[c].z, o.transform[c * 4 + 3] = p.transform.c[c].w;
 		}
+		o.texture_level = p.texture_level;
+		o.texture_blending_mode = p.texture_blending_mode;
+		o.texture_light_scale = p.texture_light_scale;
+		o.texture_light_add[0] = p.texture_light_add.x, o.texture_light_add[1] = p.texture_light_add.y;
+		o.texture_light_add[2] = p.texture_light_add.z;
 	}
 	return int(k);
 }

@@ -635,6 +635,15 @@ void draw_polygon_decals(swh_frame& f, const swh_map& m, const swh_frame_config&
 
 } // namespace
 
+void grid_surface_light(const swh_map& m, V3 pos, V3 n, float out[3])
+{
+	const LightGridElement g = fetch_light_from_grid(m, pos);
+	light_cube_light(g.light_cube, n, out);
+	const float d = std::max(dot(n, g.light_direction_vector_scaled), 0.0f);
+	for(int i = 0; i < 3; ++i)
+		out[i] += g.directional_light_color[i] * d;
+}
+
 // Model entities of the frame: seeded placement (stable along a camera path), animation and spin
 // from the frame time.
 void prepare_models(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, const M4& view, const M4& planes)

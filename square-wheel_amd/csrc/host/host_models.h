@@ -79,5 +79,8 @@ struct ModelFrameData
 void build_dynamic_lights(swh_frame& f, const swh_map& m, const swh_frame_config& cfg);
 void prepare_models(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, const M4& view, const M4& planes);
 void prepare_decals(swh_frame& f, const swh_map& m, const swh_frame_config& cfg, const M4& planes);
+// The light-grid light of a surface point with normal n (partial_renderer.rs:2232-2247):
+// get_light_cube_light of the grid's cube plus the directional light by max(dot(n, dir), 0).
+void grid_surface_light(const swh_map& m, V3 pos, V3 n, float out[3]);
 
 } // namespace swh

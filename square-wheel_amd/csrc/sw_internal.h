@@ -308,4 +308,20 @@ void launch_fire_texture(hipStream_t st, const uint8_t* heat, uint32_t log2w, ui
 	const uint32_t* palette_dev, uint32_t* lite);
 void launch_skybox_side(hipStream_t st, uint32_t color_bits, const void* image, uint32_t image_w, uint32_t size, int32_t brightness,
 	void* dst);
+
+// One portal texture overlay, resolved against the map's level table on the host.
+struct PortalMixRec
+{
+	uint64_t dst;        // first texel of the rect (ColorT texels)
+	uint32_t w, h;       // the rect
+	int32_t tcu, tcv;    // tc_start
+	uint32_t tex_offset; // the lite level's first texel
+	uint32_t tw, th;     // the lite level's size
+	uint32_t blend;
+	float light[3];      // R, G, B
+	uint32_t reserved;
+};
+// mix_surface_with_texture over each record's rect (grid: texel chunks x records)
+void launch_portal_mix(hipStream_t st, uint32_t color_bits, const uint32_t* lite, const PortalMixRec* recs, uint32_t num,
+	uint64_t max_texels, void* texels);
 } // namespace swk

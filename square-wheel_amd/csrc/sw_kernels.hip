@@ -624,6 +624,9 @@ SW_DEV typename CT::T surface_texel(const MapPtrs& m, const SurfRec& R, uint32_t
 	return CT::from_cv(result);
 }
 
+template <class CT>
+SW_DEV typename CT::T emissive_blend(uint32_t t, int32_t lb, int32_t lg, int32_t lr, uint32_t blend, typename CT::T dst);
+
 // surfaces.rs:903-971, one texel
 template <class CT>
 SW_DEV typename CT::T emissive_texel(const MapPtrs& m, const SurfRec& R, uint32_t fl, typename CT::T dst, uint32_t dst_u,
@@ -633,10 +636,17 @@ SW_DEV typename CT::T emissive_texel(const MapPtrs& m, const SurfRec& R, uint32_
 	wrap_texel((fl & REC_EMISSIVE_SLOW_WRAP) != 0, R.etcu, R.etcv, R.ew, R.eh, R.ew_mask, R.eh_mask, R.inv_ew, R.inv_eh,
 		dst_u, dst_v, src_u, src_v);
 	const uint32_t t = elem32(m.lite, uint32_t(R.etex_base) + src_v * R.ew + src_u);
-	const int32_t lb = R.elight[0], lg = R.elight[1], lr = R.elight[2];
+	return emissive_blend<CT>(t, R.elight[0], R.elight[1], R.elight[2], (fl >> REC_BLEND_SHIFT) & 15u, dst);
+}
+
+// surfaces.rs:920-968: the lite texel t modulated by the integer light (lanes B, G, R; alpha by 0)
+// and blended into dst
+template <class CT>
+SW_DEV typename CT::T emissive_blend(uint32_t t, int32_t lb, int32_t lg, int32_t lr, uint32_t blend, typename CT::T dst)
+{
 	const CI tc = Color32T::to_ci(t);
 	const CI mod = CI{wmul(tc.b, lb) >> 8, wmul(tc.g, lg) >> 8, wmul(tc.r, lr) >> 8, wmul(tc.a, 0) >> 8};
-	switch((fl >> REC_BLEND_SHIFT) & 15u)
+	switch(blend)
 	{
 	case SW_BLEND_NONE: return CT::from_ci(mod);
 	case SW_BLEND_AVERAGE:
@@ -2797,6 +2807,41 @@ __global__ __launch_bounds__(256) void k_tonemap_sums(const float* __restrict__ 
 		sums[threadIdx.x] = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
 }
 
+// A portal texture's overlay (partial_renderer.rs:2205-2276 -> mix_surface_with_texture,
+// surfaces.rs:903-971): block (x, r) takes texels [x * 1024, x * 1024 + 1024) of record r's rect,
+// four per thread, each fetching its wrapped lite texel and blending it into the texel in place.
+// The rects of one launch do not overlap (the host checks), so every texel has one writer.
+constexpr uint32_t PORTAL_MIX_BLOCK_TEXELS = 1024;
+
+template <class CT>
+__global__ __launch_bounds__(256) void k_portal_mix(const uint32_t* __restrict__ lite, const swk::PortalMixRec* __restrict__ recs,
+	typename CT::T* __restrict__ texels)
+{
+	const swk::PortalMixRec r = recs[blockIdx.y];
+	const uint64_t n = uint64_t(r.w) * r.h;
+	// cvtps2dq of light * 256, lanes B, G, R (surfaces.rs:911-914)
+	const int32_t lb = f32_to_i32_rne_x86(r.light[2] * 256.0f);
+	const int32_t lg = f32_to_i32_rne_x86(r.light[1] * 256.0f);
+	const int32_t lr = f32_to_i32_rne_x86(r.light[0] * 256.0f);
+#pragma unroll
+	for(uint32_t k = 0; k < PORTAL_MIX_BLOCK_TEXELS / 256; ++k)
+	{
+		const uint64_t i = uint64_t(blockIdx.x) * PORTAL_MIX_BLOCK_TEXELS + k * 256 + threadIdx.x;
+		if(i >= n)
+			break;
+		const uint32_t y = uint32_t(i / r.w), x = uint32_t(i - uint64_t(y) * r.w);
+		// u steps from rem_euclid(tc_min[0]) with a wrap at the width: (tc_min[0] + x) mod w, exact;
+		// v is the wrapping i32 sum tc_min[1] + y taken rem_euclid (surfaces.rs:920-923)
+		int64_t su = (int64_t(r.tcu) + x) % int64_t(r.tw);
+		int32_t sv = int32_t(uint32_t(r.tcv) + y) % int32_t(r.th);
+		su += su < 0 ? r.tw : 0;
+		sv += sv < 0 ? int32_t(r.th) : 0;
+		const uint32_t t = lite[uint64_t(r.tex_offset) + uint64_t(sv) * r.tw + uint64_t(su)];
+		typename CT::T& dst = texels[r.dst + i];
+		dst = emissive_blend<CT>(t, lb, lg, lr, r.blend, dst);
+	}
+}
+
 __global__ void k_zero_floats(float* p, uint32_t n)
 {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2952,6 +2997,18 @@ void launch_bloom(hipStream_t st, const BloomArgs& a, bool average)
 		hipLaunchKernelGGL(k_bloom_tonemap<true>, grid, dim3(256), 0, st, a);
 	else
 		hipLaunchKernelGGL(k_bloom_tonemap<false>, grid, dim3(256), 0, st, a);
+}
+
+void launch_portal_mix(hipStream_t st, uint32_t color_bits, const uint32_t* lite, const PortalMixRec* recs, uint32_t num,
+	uint64_t max_texels, void* texels)
+{
+	if(num == 0 || max_texels == 0)
+		return;
+	const dim3 grid(uint32_t((max_texels + PORTAL_MIX_BLOCK_TEXELS - 1) / PORTAL_MIX_BLOCK_TEXELS), num);
+	if(color_bits == 64)
+		hipLaunchKernelGGL(k_portal_mix<Color64T>, grid, dim3(256), 0, st, lite, recs, static_cast<uint64_t*>(texels));
+	else
+		hipLaunchKernelGGL(k_portal_mix<Color32T>, grid, dim3(256), 0, st, lite, recs, static_cast<uint32_t*>(texels));
 }
 
 void launch_tonemap(hipStream_t st, const uint64_t* hdr, uint32_t w, uint32_t h, float isr, float isg, float isb,

@@ -77,7 +77,7 @@ struct sw_ctx
 	// command buffers / work buffers
 	DevBuf views, surfaces, lights, light_indices, draws, vdata, surf_ext, blk_surf, parts, spans, drec, dbounds, counters, arena, fb,
 		tonemap_out, tonemap_sums, bloom0, bloom1, bloom_lines, model_args, model_scratch, texop_args, shadows, depth_args, depth_vdata,
-		depth_rows, depth_draw_rows, water_points, fire_args;
+		depth_rows, depth_draw_rows, water_points, fire_args, portal_mix;
 	uint32_t num_shadows = 0; // sw_set_light_shadows: entries in `shadows`
 	const float* shadow_texels = nullptr;
 	uint32_t tile_rows = 8; // raster tile height (SW_TILE_ROWS_*), set by sw_set_tile_rows / the autotuner
@@ -1359,6 +1359,75 @@ static int stage_copy(sw_ctx* ctx, void* dst_device, size_t bytes, size_t src_of
 			"argument copy")))
 		return r;
 	return hip_check(ctx, hipEventRecord(ctx->arg_staging_event[k], ctx->stream), "argument staging event");
+}
+
+int sw_mix_portal_textures(sw_ctx* ctx, sw_map* map, uint32_t color_bits, const sw_portal_mix* mixes, uint32_t num_mixes,
+	void* portal_texels_device, uint64_t num_portal_texels)
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	if(!map || (num_mixes && (!mixes || !portal_texels_device)))
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	if(color_bits != 32 && color_bits != 64)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "color_bits must be 32 or 64");
+	if(num_mixes == 0)
+		return SW_OK;
+	std::vector<swk::PortalMixRec> recs(num_mixes);
+	std::vector<std::pair<uint64_t, uint64_t>> ranges(num_mixes);
+	uint64_t max_texels = 0;
+	for(uint32_t i = 0; i < num_mixes; ++i)
+	{
+		const sw_portal_mix& x = mixes[i];
+		const std::string which = "portal mix " + std::to_string(i) + ": ";
+		if(x.level >= map->host_levels.size() || !(map->host_levels[x.level].flags & SW_TEX_LITE_LEVEL))
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, which + "level is not a lite level of the map");
+		const sw_texture_level& lv = map->host_levels[x.level];
+		if(lv.width == 0 || lv.height == 0 || lv.offset + uint64_t(lv.width) * lv.height > map->ptrs.num_lite)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, which + "empty or out-of-range level");
+		if(x.blend > SW_BLEND_ALPHA_BLEND)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, which + "unknown blending mode");
+		const uint64_t n = uint64_t(x.size[0]) * x.size[1];
+		if(x.dst_offset > num_portal_texels || n > num_portal_texels - x.dst_offset)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, which + "rect outside the portal textures");
+		ranges[i] = {x.dst_offset, x.dst_offset + n};
+		max_texels = std::max(max_texels, n);
+		swk::PortalMixRec& r = recs[i];
+		r.dst = x.dst_offset;
+		r.w = x.size[0];
+		r.h = x.size[1];
+		r.tcu = x.tc_start[0];
+		r.tcv = x.tc_start[1];
+		r.tex_offset = uint32_t(lv.offset);
+		r.tw = lv.width;
+		r.th = lv.height;
+		r.blend = x.blend;
+		r.light[0] = x.light[0];
+		r.light[1] = x.light[1];
+		r.light[2] = x.light[2];
+		r.reserved = 0;
+	}
+	// one writer per texel: the rects of one call must not overlap
+	std::sort(ranges.begin(), ranges.end());
+	for(uint32_t i = 1; i < num_mixes; ++i)
+		if(ranges[i].first < ranges[i - 1].second && ranges[i].first != ranges[i].second &&
+			ranges[i - 1].first != ranges[i - 1].second)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "portal mixes overlap");
+	if(max_texels == 0)
+		return SW_OK;
+	if(max_texels > uint64_t(0xFFFFFFFFu) * 1024u)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "portal rect too large");
+	(void)hipSetDevice(ctx->device);
+	int r;
+	const size_t bytes = sizeof(swk::PortalMixRec) * num_mixes;
+	char* staging = nullptr;
+	if((r = ensure(ctx, ctx->portal_mix, bytes)) || (r = stage_begin(ctx, bytes, &staging)))
+		return r;
+	std::memcpy(staging, recs.data(), bytes);
+	if((r = stage_copy(ctx, ctx->portal_mix.p, bytes)))
+		return r;
+	swk::launch_portal_mix(ctx->stream, color_bits, static_cast<const uint32_t*>(map->lite.p),
+		static_cast<const swk::PortalMixRec*>(ctx->portal_mix.p), num_mixes, max_texels, portal_texels_device);
+	return hip_check(ctx, hipGetLastError(), "portal mix launch");
 }
 
 int sw_prepare_models(sw_ctx* ctx, const sw_model_mesh* meshes, uint32_t num_meshes, const sw_model_instance* instances,

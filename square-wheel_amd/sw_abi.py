@@ -50,6 +50,9 @@ DRAW_DTYPE = np.dtype([
     ("kind", "<u4"), ("blend", "<u4"), ("mode", "<u4"), ("texture_source", "<u4"), ("first_vertex_int", "<u4"),
     ("num_vertices", "<u4"), ("depth_equation", "<f4", 3), ("tc_equation", "<f4", 6), ("texture_offset", "<u8"),
     ("texture_size", "<i4", 2), ("clip_rect", "<i4", 4)], align=True)
+# portal texture overlays (sw_mix_portal_textures)
+PORTAL_MIX_DTYPE = np.dtype([("dst_offset", "<u8"), ("size", "<u4", 2), ("tc_start", "<i4", 2), ("level", "<u4"),
+                             ("blend", "<u4"), ("light", "<f4", 3), ("reserved", "<u4")], align=True)
 VIEW_DTYPE = np.dtype([
     ("width", "<u4"), ("height", "<u4"), ("cam_pos", "<f4", 3), ("clear_background", "<u4"),
     ("background_color", "<u8"), ("first_surface", "<u4"), ("num_surfaces", "<u4"), ("first_draw", "<u4"),
@@ -301,6 +304,7 @@ GPU_SYMBOLS = {
                                     C.c_uint32]),
     "sw_raster_draws": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]),
     "sw_set_portal_textures": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "sw_mix_portal_textures": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]),
     "sw_frame_end": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint32]),
     "sw_tonemap": (C.c_int, [C.c_void_p, C.c_float, C.POINTER(C.c_float), C.c_void_p, C.c_size_t, C.c_uint32,
                              C.POINTER(C.c_float)]),
@@ -720,10 +724,18 @@ class Context:
                 out.free()
 
 
-    def render_portal_levels(self, map_h, batches):
+    def mix_portal_textures(self, map_h, color_bits, mixes, buf, texels):
+        """sw_mix_portal_textures: the overlays `mixes` (PORTAL_MIX_DTYPE) over device ColorT texels."""
+        mixes = np.ascontiguousarray(mixes, dtype=PORTAL_MIX_DTYPE)
+        self.check(self.lib.sw_mix_portal_textures(self.h, map_h, color_bits, _ptr(mixes) if mixes.size else None,
+                                                   int(mixes.size), buf.ptr, int(texels)), "sw_mix_portal_textures")
+
+    def render_portal_levels(self, map_h, batches, mixes=None):
         """Renders a frame and the views through its portals (batches: one BatchData per portal level,
         the frame's own first). The deepest level samples zero portal textures; each level above
-        samples the level below's framebuffer, kept in HBM. Returns every level's flat framebuffer."""
+        samples the level below's framebuffer, kept in HBM. mixes (sw_host.portal_level_mixes): the
+        portal textures' overlays, mixed into the textures level k samples before it is drawn.
+        Returns every level's flat framebuffer."""
         outs = [None] * len(batches)
         below = None
         try:
@@ -733,6 +745,8 @@ class Context:
                     below = self.to_device(np.zeros(max(b.portal_texels, 1), dtype=b.color_dtype))
                 elif b.portal_texels != batches[k + 1].fb_texels:
                     raise ValueError("portal textures of a level do not match the framebuffers of the level below")
+                if mixes is not None and len(mixes[k + 1]):
+                    self.mix_portal_textures(map_h, b.color_bits, mixes[k + 1], below, max(b.portal_texels, 1))
                 out = self.alloc(max(b.fb_texels, 1) * (8 if b.color_bits == 64 else 4))
                 try:
                     outs[k] = self.render_batch(map_h, b, out=out, portal=(below.ptr, max(b.portal_texels, 1)))

@@ -58,9 +58,12 @@ PORTAL_MIRROR, PORTAL_CAMERA, PORTAL_PARALLAX = 0, 1, 2
 VIEW_PORTAL_DTYPE = np.dtype([("kind", "<u4"), ("blending_mode", "<u4"), ("plane", "<f4", 4),
                               ("tex_coord_equation", "<f4", (2, 4)), ("vertices", "<f4", (4, 3)),
                               ("camera_position", "<f4", 3), ("camera_yaw", "<f4"), ("camera_pitch", "<f4"),
-                              ("camera_fov", "<f4"), ("transform", "<f4", 16)])
+                              ("camera_fov", "<f4"), ("transform", "<f4", 16), ("texture_level", "<u4"),
+                              ("texture_blending_mode", "<u4"), ("texture_light_scale", "<f4"),
+                              ("texture_light_add", "<f4", 3)])
 PORTAL_INFO_DTYPE = np.dtype([("visible", "<u4"), ("resolution", "<u4", 2), ("tc_min", "<i4", 2), ("mip", "<u4"),
-                              ("texture_offset", "<u8")], align=True)
+                              ("texture_offset", "<u8"), ("mix_level", "<u4"), ("mix_blend", "<u4"),
+                              ("mix_light", "<f4", 3), ("reserved", "<u4")], align=True)
 
 
 def load():
@@ -426,6 +429,33 @@ def portal_levels(frame):
 def portal_level_batches(frame):
     """One BatchData per portal level (portal_levels), level 0 (the frame itself) first."""
     return [batch_from_frames(level) for level in portal_levels(frame)]
+
+
+def portal_level_mixes(frame):
+    """The portal textures' overlays per portal level (sw_mix_portal_textures' input): entry k (k >= 1)
+    lists, as PORTAL_MIX_DTYPE, the overlays of the portals of level k - 1's frames, with their rects
+    placed in the portal textures level k - 1 samples (level k's framebuffers laid end to end; for
+    the last level, whose portals have no views, the black textures below it); they are mixed after
+    level k is drawn and before level k - 1 is. Entry 0 is empty; there are len(levels) + 1 entries."""
+    levels = portal_levels(frame)
+    out = [np.zeros(0, dtype=A.PORTAL_MIX_DTYPE)]
+    for k in range(1, len(levels) + 1):
+        rows = []
+        base = 0
+        for f in levels[k - 1]:
+            for p in f["portals"]:
+                if p["visible"] and int(p["mix_level"]) != A.NO_TEXTURE:
+                    m = np.zeros(1, dtype=A.PORTAL_MIX_DTYPE)[0]
+                    m["dst_offset"] = base + int(p["texture_offset"])
+                    m["size"] = p["resolution"]
+                    m["tc_start"] = p["tc_min"]
+                    m["level"] = p["mix_level"]
+                    m["blend"] = p["mix_blend"]
+                    m["light"] = p["mix_light"]
+                    rows.append(m)
+            base += int(f["portal_texels"])
+        out.append(np.array(rows, dtype=A.PORTAL_MIX_DTYPE))
+    return out
 
 
 # ---------------------------------------------------------------------------------------------

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import sw_abi
+import sw_host
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 INCLUDE = os.path.join(ROOT, "include")
@@ -62,6 +63,9 @@ STRUCTS = {
     "sw_wave_point": sw_abi.WAVE_POINT_DTYPE,
     "sw_heat_point": sw_abi.HEAT_POINT_DTYPE,
     "sw_water_texture_params": np.dtype([(n, "<u4" if t is not ctypes.c_float else "<f4") for n, t in sw_abi.WaterTextureParams._fields_]),
+    "sw_portal_mix": sw_abi.PORTAL_MIX_DTYPE,
+    "swh_portal_info": sw_host.PORTAL_INFO_DTYPE,
+    "swh_view_portal": sw_host.VIEW_PORTAL_DTYPE,
 }
 
 
@@ -74,7 +78,7 @@ CTYPES_STRUCTS = [(sw_abi.MapDesc, "sw_map_desc"), (sw_abi.Batch, "sw_batch"), (
 @pytest.fixture(scope="module")
 def c_layout():
     """sizeof / offsetof of every struct field, printed by a C program compiled from the header."""
-    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "sw_server.h"', "int main(void) {"]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "sw_server.h"', '#include "sw_host.h"', "int main(void) {"]
     for name, dt in STRUCTS.items():
         lines.append(f'printf("{name} @sizeof %zu\\n", sizeof({name}));')
         for field in dt.names:

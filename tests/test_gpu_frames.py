@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import oracle
+import sw_abi
 import sw_host
 
 pytestmark = pytest.mark.gpu
@@ -181,28 +182,79 @@ def test_inline_models_and_view_model_frames(gpu_ctx, color_bits):
 @pytest.mark.parametrize("color_bits", [32, 64])
 def test_portal_frames_bit_exact(gpu_ctx, color_bits):
     """Mirrors, camera and parallax portals two levels deep: every level's views on the GPU (each
-    level one batch sampling the level below's framebuffer in HBM) against the oracle's levels; the
-    main view again through the plain path with the portal textures set by sw_set_portal_textures."""
+    level one batch sampling the level below's framebuffer in HBM, with the portal textures'
+    overlays mixed in by sw_mix_portal_textures) against the oracle's levels; the main view again
+    through the plain path with the portal textures set by sw_set_portal_textures."""
     m = sw_host.large_map(4, grid=(10, 10, 4), num_view_portals=12)
     fb = sw_host.FrameBuilder()
     mh = gpu_ctx.upload_map(m.data)
     try:
-        done = 0
+        done = overlays = 0
         poses = m.camera_path(8, 48)
         for i in (13, 7, 17, 23):  # poses that see portal views two levels deep
             f = fb.build(m, poses[i], width=480, height=300, color_bits=color_bits, use_directional_lightmaps=1,
                          num_dynamic_lights=4, num_models=8, time=0.3 * i, portals_depth=2)
             levels = sw_host.portal_level_batches(f)
             assert len(levels) == 3
-            _say(f"case portals pose {i}: {[len(b.views) for b in levels]} views per level, color {color_bits}")
-            outs = gpu_ctx.render_portal_levels(mh, levels)
-            refs, oob = oracle.render_portal_levels(m.data, levels)
+            mixes = sw_host.portal_level_mixes(f)
+            overlays += sum(len(x) for x in mixes)
+            _say(f"case portals pose {i}: {[len(b.views) for b in levels]} views per level, "
+                 f"{[len(x) for x in mixes]} overlays per level, color {color_bits}")
+            outs = gpu_ctx.render_portal_levels(mh, levels, mixes=mixes)
+            refs, oob = oracle.render_portal_levels(m.data, levels, mixes=mixes)
             assert oob == 0
             for k, (o, r) in enumerate(zip(outs, refs)):
                 assert np.array_equal(o, r), f"pose {i} level {k}: {np.count_nonzero(o != r)} pixels differ"
-            plain = gpu_ctx.render_plain(mh, levels[0], portal=refs[1]).reshape(-1)
+            # the main view through the plain path, sampling level 1 with its overlays
+            tex = refs[1].copy()
+            oracle.mix_portal_textures(m.data, color_bits, mixes[1], tex)
+            plain = gpu_ctx.render_plain(mh, levels[0], portal=tex).reshape(-1)
             assert np.array_equal(plain, refs[0])
             done += 1
-        assert done == 4
+        assert done == 4 and overlays > 0
     finally:
+        gpu_ctx.release_map(mh)
+
+
+@pytest.mark.parametrize("color_bits", [32, 64])
+def test_portal_texture_mix_bit_exact(gpu_ctx, color_bits):
+    """sw_mix_portal_textures on random rects of a texture buffer (every blend mode, negative and
+    wrapping texture coordinates, a non-power-of-two level, light values that overflow the pack)
+    against the oracle; overlapping rects and a non-lite level are errors that leave the buffer."""
+    m = sw_host.large_map(2, grid=(6, 6, 4))
+    rng = np.random.default_rng(31 + color_bits)
+    lite_levels = [i for i, lv in enumerate(m.data.levels) if lv["flags"] & sw_abi.TEX_LITE_LEVEL]
+    assert lite_levels
+    n = 1 << 16
+    dtype = np.uint32 if color_bits == 32 else np.uint64
+    mixes = np.zeros(24, sw_abi.PORTAL_MIX_DTYPE)
+    base = 0
+    for i in range(len(mixes)):
+        w, h = int(rng.integers(1, 90)), int(rng.integers(1, 40))
+        mixes[i] = (base, (w, h), (int(rng.integers(-300, 300)), int(rng.integers(-300, 300))),
+                    lite_levels[int(rng.integers(0, len(lite_levels)))], i % 5, rng.uniform(0.0, 2.5, 3), 0)
+        base += w * h + int(rng.integers(0, 9))
+    mixes[5]["light"] = (150.0, 0.75, 0.0)
+    assert base <= n
+    _say(f"case portal mix: {len(mixes)} rects, {base} texels, color {color_bits}")
+    texels = rng.integers(0, 2 ** 63, size=n, dtype=np.uint64).astype(dtype)
+    mh = gpu_ctx.upload_map(m.data)
+    buf = gpu_ctx.to_device(texels)
+    try:
+        gpu_ctx.mix_portal_textures(mh, color_bits, mixes, buf, n)
+        got = buf.read(dtype, n)
+        want = texels.copy()
+        oracle.mix_portal_textures(m.data, color_bits, mixes, want)
+        assert np.array_equal(got, want), f"{np.count_nonzero(got != want)} texels differ"
+        bad = mixes[:2].copy()
+        bad[1]["dst_offset"] = bad[0]["dst_offset"] + 1
+        with pytest.raises(RuntimeError):
+            gpu_ctx.mix_portal_textures(mh, color_bits, bad, buf, n)
+        bad = mixes[:1].copy()
+        bad[0]["level"] = [i for i in range(len(m.data.levels)) if i not in lite_levels][0]
+        with pytest.raises(RuntimeError):
+            gpu_ctx.mix_portal_textures(mh, color_bits, bad, buf, n)
+        assert np.array_equal(buf.read(dtype, n), want)
+    finally:
+        buf.free()
         gpu_ctx.release_map(mh)
