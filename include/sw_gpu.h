@@ -617,6 +617,42 @@ int sw_prepare_models(
 	sw_model_vertex* out_vertices, uint64_t vertex_capacity, uint16_t* out_triangles, uint64_t triangle_capacity,
 	uint32_t* out_counts);
 
+/* One mesh drawn in one leaf from prepared models (draw_mesh, partial_renderer.rs:3525-3718): the
+ * caller keeps the per-mesh decisions (the mesh box against the leaf's bounds, which 3D planes —
+ * the near plane and the leaf's planes — and which 2D box planes cut it, its texture mip and
+ * blending) and the run's place in the draw list; each prepared triangle is clipped by the 3D
+ * planes, projected, clipped by the 2D planes and drawn as a fan of triangles. */
+#define SW_MESH_MAX_PLANES3D 7u
+typedef struct sw_mesh_run
+{
+	uint32_t count_index;    /* the instance whose out_counts entry (visible triangles) the run draws */
+	uint32_t first_vertex;   /* its prepared vertices */
+	uint32_t num_vertices;   /* the mesh's vertices (triangle indices are below it) */
+	uint32_t first_triangle; /* its prepared, back-to-front sorted triangles (3 u16 each) */
+	uint32_t max_triangles;  /* the mesh's triangle count: the run's triangle slots */
+	uint32_t num_planes3d;   /* <= SW_MESH_MAX_PLANES3D; keeps dot(vec, pos) - dist >= 0 */
+	uint32_t num_planes2d;   /* <= 4; keeps x * l[0] + y * l[1] - l[2] >= 0 on the projected plane */
+	uint32_t blend;
+	float planes3d[SW_MESH_MAX_PLANES3D][4];
+	float planes2d[4][3];
+	uint64_t texture_offset; /* the model's lite texture level at its mip */
+	int32_t texture_size[2];
+	int32_t clip_rect[4];
+} sw_mesh_run;
+
+/* The fans of the runs' triangles as triangle-draw vertex data: slot j of triangle t of run r (t below
+ * max_triangles, j below fan_r = 1 + num_planes3d + num_planes2d) is 21 i32 (three vertices of
+ * fixed16 x, y, tc[2], light[3]) at out_vertex_ints_device + int_base_r + 21 * (t * fan_r + j), where
+ * int_base_r sums 21 * max_triangles * fan of the runs before r; out_fan_counts (host, one byte per
+ * triangle slot, runs back to back) receives each triangle's fan size (0: clipped away, or beyond the
+ * instance's visible triangles). The caller then draws, in its draw list at the run's place, slot
+ * (t, j) for j below the fan size, t in order. Inputs are sw_prepare_models' device outputs;
+ * synchronous (returns with the fan sizes in host memory). */
+int sw_expand_mesh_runs(sw_ctx* ctx, const sw_mesh_run* runs, uint32_t num_runs, const sw_model_vertex* prepared_vertices_device,
+	uint64_t vertex_capacity, const uint16_t* prepared_triangles_device, uint64_t triangle_capacity,
+	const uint32_t* counts_device, uint32_t num_counts, int32_t* out_vertex_ints_device, uint64_t vertex_int_capacity,
+	uint8_t* out_fan_counts);
+
 /* Work counters of the last batch. The pixel counters of a batch are summed on the first call
  * after it (one small kernel and a wait), so batches that nobody asks about pay nothing for them.
  * No counterpart in the reference (its renderer keeps no such counters). */

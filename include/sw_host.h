@@ -76,6 +76,11 @@ typedef struct swh_frame_config
 	uint32_t virtual_threads; /* 1: full-screen clip rect (the parity configuration) */
 	uint32_t view_model; /* 1: an animated model held in front of the camera (drawn after the tree) */
 	uint32_t portals_depth; /* RendererConfig::portals_depth: levels of views rendered through portals */
+	/* 1: the frame's triangle models are prepared on the device. The front-end then animates no
+	 * vertices and draws no model triangles: it lists the visible models as sw_model_instance
+	 * (sw_prepare_models' input, the map's mesh being swh_map_model_mesh) and each mesh drawn in a
+	 * leaf as an sw_mesh_run at its place in the draw list (swh_frame_model_data) */
+	uint32_t device_models;
 } swh_frame_config;
 
 /* A view's command buffers, valid until the frame is rebuilt or destroyed. */
@@ -168,6 +173,45 @@ int swh_frame_portal_info(const swh_frame* frame, uint32_t portal_index, swh_por
 /* The index-th view seen through a visible portal of the frame (index < num_portal_views); valid until
  * the frame is rebuilt or destroyed. Its own portal views follow the same scheme, one level deeper. */
 int swh_frame_portal_view(const swh_frame* frame, uint32_t index, const swh_frame** out_view);
+
+/* The frame's triangle models (prepare_models / draw_mesh, partial_renderer.rs:995-1190, 3525-3718).
+ * runs[i] is a mesh drawn in one leaf; its triangles belong in the draw list just before draw
+ * run_positions[i] (runs at one position keep their order). With device_models the instances are
+ * sw_prepare_models' input (mesh 0 = swh_map_model_mesh, outputs at first_vertex / first_triangle,
+ * counts indexed by the runs' count_index) and the draw list holds no model triangles. Without it
+ * the front-end prepared the models itself: prepared_* hold its vertices, sorted triangles and
+ * visible-triangle counts, the runs refer to them, and run_num_draws[i] triangle draws of the list
+ * from run_positions[i] on are that run's (sw_expand_mesh_runs' fans of the same triangles). */
+typedef struct swh_model_data
+{
+	const sw_model_instance* instances;
+	uint32_t num_instances;
+	const sw_mesh_run* runs;
+	uint32_t num_runs;
+	const uint32_t* run_positions;
+	const uint32_t* run_num_draws; /* host preparation only */
+	const sw_model_vertex* prepared_vertices; /* host preparation only */
+	uint64_t num_prepared_vertices;
+	const uint16_t* prepared_triangles; /* host preparation only, 3 per triangle */
+	uint64_t num_prepared_triangles;
+	const uint32_t* prepared_counts; /* host preparation only, per run count_index */
+	uint32_t num_prepared_counts;
+	uint64_t vertex_capacity; /* vertices and triangles the instances' outputs span */
+	uint64_t triangle_capacity;
+} swh_model_data;
+int swh_frame_model_data(const swh_frame* frame, swh_model_data* out);
+
+/* The map's animated model mesh as sw_model_mesh's VERTEX_ANIMATED arrays (host memory owned by
+ * the map): tc[2] per vertex, frame_vertices = {position[3], normal[3]} per vertex per frame, and
+ * triangles (3 u16 each, in the winding the camera sees front faces clockwise). */
+typedef struct swh_model_mesh_data
+{
+	uint32_t num_vertices, num_triangles, num_frames;
+	const float* tc;
+	const float* frame_vertices;
+	const uint16_t* triangles;
+} swh_model_mesh_data;
+int swh_map_model_mesh(const swh_map* map, swh_model_mesh_data* out);
 
 /* Shadow-map draw lists (sw_render_depth's input) of the map's static geometry:
  *   swh_shadow_add_cube <- PartialRenderer::build_shadow_maps' cube branch (partial_renderer.rs:541-566)

@@ -2784,6 +2784,177 @@ int oracle_mix_portal_textures(const sw_map_desc* map, uint32_t color_bits, cons
 	return 0;
 }
 
+// draw_mesh's triangle loop (partial_renderer.rs:3612-3716) over prepared models, with the
+// reference's model polygon clipping (clipping.rs:203-385), in sw_expand_mesh_runs' output layout.
+struct OracleMV
+{
+	float pos[3];
+	float tc[2];
+	float light[3];
+};
+
+// get_model_line_plane_intersection / get_model_line_line_intersection (clipping.rs:315-352)
+static OracleMV oracle_mv_lerp(const OracleMV& v0, const OracleMV& v1, float dist0, float dist1, int dims)
+{
+	const float dist_sum = dist1 - dist0;
+	const float k0 = dist0 / dist_sum;
+	const float k1 = dist1 / dist_sum;
+	OracleMV r{};
+	for(int i = 0; i < dims; ++i)
+		r.pos[i] = v0.pos[i] * k1 - v1.pos[i] * k0;
+	for(int i = 0; i < 2; ++i)
+		r.tc[i] = v0.tc[i] * k1 - v1.tc[i] * k0;
+	for(int i = 0; i < 3; ++i)
+		r.light[i] = v0.light[i] * k1 - v1.light[i] * k0;
+	return r;
+}
+
+// clip_3d_model_polygon_by_plane (clipping.rs:259-313): keeps dot(plane.vec, pos) >= plane.dist
+static size_t oracle_clip3(const std::vector<OracleMV>& in, const float plane[4], std::vector<OracleMV>& out)
+{
+	out.clear();
+	auto dotv = [&](const OracleMV& v) { return (plane[0] * v.pos[0] + plane[1] * v.pos[1]) + plane[2] * v.pos[2]; };
+	const OracleMV* prev = &in.back();
+	float prev_dist = dotv(*prev);
+	for(const OracleMV& v : in)
+	{
+		const float dist = dotv(v);
+		if(dist > plane[3])
+		{
+			if(prev_dist < plane[3])
+				out.push_back(oracle_mv_lerp(*prev, v, dotv(*prev) - plane[3], dotv(v) - plane[3], 3));
+			out.push_back(v);
+		}
+		else if(dist == plane[3])
+			out.push_back(v);
+		else if(prev_dist > plane[3])
+			out.push_back(oracle_mv_lerp(*prev, v, dotv(*prev) - plane[3], dotv(v) - plane[3], 3));
+		prev = &v;
+		prev_dist = dist;
+	}
+	return out.size();
+}
+
+// clip_2d_model_polygon (clipping.rs:203-257): keeps pos . (l.x, l.y) - l.z >= 0
+static size_t oracle_clip2(const std::vector<OracleMV>& in, const float line[3], std::vector<OracleMV>& out)
+{
+	out.clear();
+	auto dist_of = [&](const OracleMV& v) { return (v.pos[0] * line[0] + v.pos[1] * line[1]) - line[2]; };
+	const OracleMV* prev = &in.back();
+	float prev_dist = dist_of(*prev);
+	for(const OracleMV& v : in)
+	{
+		const float dist = dist_of(v);
+		if(dist > 0.0f)
+		{
+			if(prev_dist < 0.0f)
+				out.push_back(oracle_mv_lerp(*prev, v, prev_dist, dist, 2));
+			out.push_back(v);
+		}
+		else if(dist == 0.0f)
+			out.push_back(v);
+		else if(prev_dist > 0.0f)
+			out.push_back(oracle_mv_lerp(*prev, v, prev_dist, dist, 2));
+		prev = &v;
+		prev_dist = dist;
+	}
+	return out.size();
+}
+
+// f32_to_fixed16 (fixed_math.rs): a saturating cast of x * 2^16, NaN -> 0
+static int32_t oracle_f32_to_fixed16(float x)
+{
+	const float y = x * 65536.0f;
+	if(std::isnan(y))
+		return 0;
+	if(y >= 2147483647.0f)
+		return INT32_MAX;
+	if(y <= -2147483648.0f)
+		return INT32_MIN;
+	return int32_t(y);
+}
+
+int oracle_expand_mesh_runs(const sw_mesh_run* runs, uint32_t num_runs, const sw_model_vertex* vertices, uint64_t num_vertices,
+	const uint16_t* triangles, uint64_t num_triangles, const uint32_t* counts, uint32_t num_counts, int32_t* out_ints,
+	uint64_t num_ints, uint8_t* out_fans)
+{
+	uint64_t int_base = 0, fan_base = 0;
+	for(uint32_t ri = 0; ri < num_runs; ++ri)
+	{
+		const sw_mesh_run& R = runs[ri];
+		if(R.count_index >= num_counts || R.num_planes3d > SW_MESH_MAX_PLANES3D || R.num_planes2d > 4 ||
+			uint64_t(R.first_vertex) + R.num_vertices > num_vertices || uint64_t(R.first_triangle) + R.max_triangles > num_triangles)
+			return -1;
+		const uint32_t fan_max = 1u + R.num_planes3d + R.num_planes2d;
+		if(int_base + uint64_t(R.max_triangles) * fan_max * 21u > num_ints)
+			return -1;
+		const uint32_t visible = std::min(counts[R.count_index], R.max_triangles);
+		std::vector<OracleMV> a, b;
+		for(uint32_t t = 0; t < R.max_triangles; ++t)
+		{
+			uint32_t fan = 0;
+			if(t < visible)
+			{
+				a.clear();
+				for(int k = 0; k < 3; ++k)
+				{
+					const uint32_t vi = triangles[(uint64_t(R.first_triangle) + t) * 3 + k];
+					if(vi >= R.num_vertices)
+						return -1;
+					const sw_model_vertex& v = vertices[R.first_vertex + vi];
+					a.push_back(OracleMV{{v.pos[0], v.pos[1], v.pos[2]}, {v.tc[0], v.tc[1]}, {v.light[0], v.light[1], v.light[2]}});
+				}
+				bool culled = false;
+				for(uint32_t p = 0; p < R.num_planes3d && !culled; ++p)
+				{
+					culled = oracle_clip3(a, R.planes3d[p], b) < 3;
+					std::swap(a, b);
+				}
+				if(!culled)
+				{
+					for(OracleMV& v : a)
+					{
+						// pos.truncate() / pos.z
+						const float z = v.pos[2];
+						v.pos[0] = v.pos[0] / z;
+						v.pos[1] = v.pos[1] / z;
+						v.pos[2] = 0.0f;
+					}
+					for(uint32_t p = 0; p < R.num_planes2d && !culled; ++p)
+					{
+						culled = oracle_clip2(a, R.planes2d[p], b) < 3;
+						std::swap(a, b);
+					}
+				}
+				if(!culled)
+				{
+					fan = uint32_t(a.size()) - 2u;
+					if(fan > fan_max)
+						return -1;
+					for(uint32_t j = 0; j < fan; ++j)
+					{
+						const OracleMV* tri[3] = {&a[0], &a[j + 1], &a[j + 2]};
+						int32_t* o = out_ints + int_base + (uint64_t(t) * fan_max + j) * 21u;
+						for(int v = 0; v < 3; ++v)
+						{
+							o[v * 7 + 0] = oracle_f32_to_fixed16(tri[v]->pos[0]);
+							o[v * 7 + 1] = oracle_f32_to_fixed16(tri[v]->pos[1]);
+							o[v * 7 + 2] = oracle_f32_to_fixed16(tri[v]->tc[0]);
+							o[v * 7 + 3] = oracle_f32_to_fixed16(tri[v]->tc[1]);
+							for(int c = 0; c < 3; ++c)
+								o[v * 7 + 4 + c] = oracle_f32_to_fixed16(tri[v]->light[c]);
+						}
+					}
+				}
+			}
+			out_fans[fan_base + t] = uint8_t(fan);
+		}
+		int_base += uint64_t(R.max_triangles) * fan_max * 21u;
+		fan_base += R.max_triangles;
+	}
+	return 0;
+}
+
 // One water step: the points (add_point_value, :68-76), then update_wave_field into field_old
 // (:223-224; the caller swaps the buffers as the reference does).
 int oracle_water_step(float* field, float* field_old, uint32_t log2w, uint32_t log2h, float attenuation, const sw_wave_point* pts,

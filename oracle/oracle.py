@@ -37,6 +37,9 @@ def load():
         lib.oracle_build_surfaces.restype = C.c_int
         lib.oracle_build_surfaces.argtypes = [C.c_int, C.POINTER(sw_abi.MapDesc), C.POINTER(sw_abi.Batch), C.c_void_p,
                                               C.POINTER(sw_abi.Stats)]
+        lib.oracle_expand_mesh_runs.restype = C.c_int
+        lib.oracle_expand_mesh_runs.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
+                                                C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.c_void_p]
         lib.oracle_mix_portal_textures.restype = C.c_int
         lib.oracle_mix_portal_textures.argtypes = [C.POINTER(sw_abi.MapDesc), C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
                                                    C.c_uint64]
@@ -118,6 +121,24 @@ def render(map_data, batch, mode=MODE_EXACT, num_threads=1, arena=None, portal=N
                                 C.c_void_p(fb.ctypes.data), num_threads, C.byref(st), C.byref(oob))
     assert r == 0
     return fb[:batch.fb_texels], arena, st.as_dict(), int(oob.value)
+
+
+def expand_mesh_runs(runs, vertices, triangles, counts):
+    """draw_mesh's triangle loop over prepared models (partial_renderer.rs:3612-3716) in
+    sw_expand_mesh_runs' layout: returns (vertex ints, fan sizes)."""
+    lib = load()
+    runs = np.ascontiguousarray(runs, dtype=sw_abi.MESH_RUN_DTYPE)
+    vertices = np.ascontiguousarray(vertices, dtype=sw_abi.MODEL_VERTEX_DTYPE)
+    triangles = np.ascontiguousarray(triangles, dtype=np.uint16).reshape(-1)
+    counts = np.ascontiguousarray(counts, dtype=np.uint32)
+    _, ints, slots = sw_abi.mesh_run_layout(runs)
+    out = np.zeros(max(ints, 1), dtype=np.int32)
+    fans = np.zeros(max(slots, 1), dtype=np.uint8)
+    r = lib.oracle_expand_mesh_runs(runs.ctypes.data if runs.size else None, len(runs), vertices.ctypes.data, len(vertices),
+                                    triangles.ctypes.data, len(triangles) // 3, counts.ctypes.data, len(counts),
+                                    out.ctypes.data, out.size, fans.ctypes.data)
+    assert r == 0
+    return out[:ints], fans[:slots]
 
 
 def mix_portal_textures(map_data, color_bits, mixes, texels):

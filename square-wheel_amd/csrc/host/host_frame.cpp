@@ -1509,6 +1509,29 @@ int swh_frame_portal_info(const swh_frame* f, uint32_t index, swh_portal_info* o
 	return 0;
 }
 
+int swh_frame_model_data(const swh_frame* f, swh_model_data* out)
+{
+	if(!f || !out)
+		return -1;
+	const swh::ModelFrameData& md = f->models;
+	std::memset(out, 0, sizeof(*out));
+	out->instances = md.instances.data();
+	out->num_instances = uint32_t(md.instances.size());
+	out->runs = md.runs.data();
+	out->num_runs = uint32_t(md.runs.size());
+	out->run_positions = md.run_positions.data();
+	out->run_num_draws = md.run_num_draws.data();
+	out->prepared_vertices = md.prepared_vertices.data();
+	out->num_prepared_vertices = md.prepared_vertices.size();
+	out->prepared_triangles = md.prepared_triangles.data();
+	out->num_prepared_triangles = md.prepared_triangles.size() / 3;
+	out->prepared_counts = md.prepared_counts.data();
+	out->num_prepared_counts = uint32_t(md.prepared_counts.size());
+	out->vertex_capacity = md.vertex_capacity;
+	out->triangle_capacity = md.triangle_capacity;
+	return 0;
+}
+
 int swh_frame_portal_view(const swh_frame* f, uint32_t index, const swh_frame** out)
 {
 	if(!f || !out || index >= f->num_portal_views)

@@ -252,6 +252,10 @@ struct ModelMesh
 	std::vector<float> tc; // 2 per vertex, in [0, 1]
 	std::vector<uint32_t> tris; // 3 per triangle
 	std::vector<BBox> frame_bbox;
+	// sw_model_mesh's VERTEX_ANIMATED layout: {position, normal} per vertex per frame, and the
+	// triangles in the winding the camera sees front faces clockwise
+	std::vector<float> abi_frame_vertices;
+	std::vector<uint16_t> abi_triangles;
 };
 
 } // namespace swh

@@ -1287,6 +1287,24 @@ int swh_map_create(const swh_map_config* cfg, swh_map** out)
 	light_polygons(*m);
 	build_light_grid(*m);
 	build_model_mesh(*m, cfg->seed, 7, 8, 28.0f);
+	{
+		ModelMesh& mm = m->model_mesh;
+		mm.abi_frame_vertices.resize(size_t(mm.num_frames) * mm.num_vertices * 6);
+		for(size_t i = 0; i < size_t(mm.num_frames) * mm.num_vertices; ++i)
+		{
+			float* o = &mm.abi_frame_vertices[i * 6];
+			o[0] = mm.pos[i].x, o[1] = mm.pos[i].y, o[2] = mm.pos[i].z;
+			o[3] = mm.normal[i].x, o[4] = mm.normal[i].y, o[5] = mm.normal[i].z;
+		}
+		// wound outward counter-clockwise: swap the last two, as the front-end's own preparation does
+		mm.abi_triangles.resize(mm.tris.size());
+		for(size_t k = 0; k + 2 < mm.tris.size(); k += 3)
+		{
+			mm.abi_triangles[k] = uint16_t(mm.tris[k]);
+			mm.abi_triangles[k + 1] = uint16_t(mm.tris[k + 2]);
+			mm.abi_triangles[k + 2] = uint16_t(mm.tris[k + 1]);
+		}
+	}
 	*out = m;
 	return 0;
 }
@@ -1368,6 +1386,20 @@ int swh_map_view_portals(const swh_map* m, swh_view_portal* out, uint32_t capaci
 		o.texture_light_add[2] = p.texture_light_add.z;
 	}
 	return int(k);
+}
+
+int swh_map_model_mesh(const swh_map* m, swh_model_mesh_data* out)
+{
+	if(!m || !out)
+		return -1;
+	const ModelMesh& mm = m->model_mesh;
+	out->num_vertices = mm.num_vertices;
+	out->num_triangles = uint32_t(mm.tris.size() / 3);
+	out->num_frames = mm.num_frames;
+	out->tc = mm.tc.data();
+	out->frame_vertices = mm.abi_frame_vertices.data();
+	out->triangles = mm.abi_triangles.data();
+	return 0;
 }
 
 int swh_camera_path(const swh_map* m, uint32_t seed, uint32_t n, float* out)

@@ -360,6 +360,40 @@ void draw_mesh(swh_frame& f, const swh_map& m, const ClippingPolygon& bounds, co
 		if(cur_box[i].z > mesh_box[i].z)
 			planes2d[n2++] = cur_box[i];
 
+	// the run: the mesh's per-leaf decisions, at its place in the draw list
+	ModelFrameData& md = f.models;
+	{
+		sw_mesh_run run;
+		std::memset(&run, 0, sizeof(run));
+		run.count_index = uint32_t(&vm - md.visible.data());
+		run.first_vertex = vm.first_vertex;
+		run.num_vertices = vm.num_vertices;
+		run.first_triangle = vm.first_triangle;
+		run.max_triangles = vm.num_triangles;
+		run.num_planes3d = n3;
+		run.num_planes2d = n2;
+		run.blend = e.blend;
+		for(uint32_t p = 0; p < n3; ++p)
+		{
+			run.planes3d[p][0] = planes3d[p].vec.x, run.planes3d[p][1] = planes3d[p].vec.y;
+			run.planes3d[p][2] = planes3d[p].vec.z, run.planes3d[p][3] = planes3d[p].dist;
+		}
+		for(uint32_t p = 0; p < n2; ++p)
+			run.planes2d[p][0] = planes2d[p].x, run.planes2d[p][1] = planes2d[p].y, run.planes2d[p][2] = planes2d[p].z;
+		const sw_texture_level& tl = m.levels[vm.level];
+		run.texture_offset = tl.offset;
+		run.texture_size[0] = int32_t(tl.width);
+		run.texture_size[1] = int32_t(tl.height);
+		for(int i = 0; i < 4; ++i)
+			run.clip_rect[i] = clip_rect[i];
+		md.runs.push_back(run);
+		md.run_positions.push_back(uint32_t(f.draws.size()));
+		md.run_num_draws.push_back(0);
+	}
+	if(md.device)
+		return;
+	const size_t draws_before = f.draws.size();
+
 	const ModelVertex3d* verts = &f.models.vertices[vm.first_vertex];
 	const uint32_t* tris = &f.models.triangles[vm.first_triangle * 3];
 	for(uint32_t t = 0; t < vm.num_triangles; ++t)
@@ -418,6 +452,7 @@ void draw_mesh(swh_frame& f, const swh_map& m, const ClippingPolygon& bounds, co
 			emit_triangle(f, m, pts, vm.level, e.blend, clip_rect);
 		}
 	}
+	md.run_num_draws.back() = uint32_t(f.draws.size() - draws_before);
 }
 
 // get_polygon_lightmap_light (partial_renderer.rs:3807-3875): bilinear fetch of the polygon's
@@ -655,6 +690,15 @@ void prepare_models(swh_frame& f, const swh_map& m, const swh_frame_config& cfg,
 	md.triangles.clear();
 	md.leaf_models.clear();
 	md.view_models.clear();
+	md.device = cfg.device_models != 0;
+	md.instances.clear();
+	md.runs.clear();
+	md.run_positions.clear();
+	md.run_num_draws.clear();
+	md.prepared_vertices.clear();
+	md.prepared_triangles.clear();
+	md.prepared_counts.clear();
+	md.vertex_capacity = md.triangle_capacity = 0;
 	const ModelMesh& mesh = m.model_mesh;
 	if((cfg.num_models == 0 && cfg.view_model == 0) || m.model_levels.empty() || m.empty_cells.empty() || mesh.num_vertices == 0)
 		return;
@@ -840,6 +884,56 @@ void prepare_models(swh_frame& f, const swh_map& m, const swh_frame_config& cfg,
 			ml.color[1][i] = dy.directional_light_color[i];
 		}
 
+		// the model as an instance of the device preparation (sw_prepare_models' arguments:
+		// animation point, matrices, tc scale, ModelLightData; outputs in capacity order)
+		{
+			sw_model_instance in;
+			std::memset(&in, 0, sizeof(in));
+			in.mesh = 0;
+			in.frames[0] = e.frames[0];
+			in.frames[1] = e.frames[1];
+			in.lerp = e.lerp;
+			for(int c = 0; c < 4; ++c)
+				for(int r = 0; r < 4; ++r)
+				{
+					in.model_matrix[c * 4 + r] = model_matrix.at(r, c);
+					in.model_view_matrix[c * 4 + r] = mv.at(r, c);
+				}
+			in.tc_scale[0] = tsize[0];
+			in.tc_scale[1] = tsize[1];
+			for(int side = 0; side < 6; ++side)
+				for(int i = 0; i < 3; ++i)
+					in.light_cube[side][i] = ml.cube[side][i];
+			for(int d = 0; d < 2; ++d)
+			{
+				in.light_vector_scaled[d][0] = ml.dir[d].x;
+				in.light_vector_scaled[d][1] = ml.dir[d].y;
+				in.light_vector_scaled[d][2] = ml.dir[d].z;
+				for(int i = 0; i < 3; ++i)
+					in.light_color[d][i] = ml.color[d][i];
+			}
+			in.first_vertex = uint32_t(md.vertex_capacity);
+			in.first_triangle = uint32_t(md.triangle_capacity);
+			md.vertex_capacity += mesh.num_vertices;
+			md.triangle_capacity += mesh.tris.size() / 3;
+			md.instances.push_back(in);
+			if(md.device)
+			{
+				// the device animates, lights, culls and sorts: the runs draw up to every triangle
+				vm.first_vertex = in.first_vertex;
+				vm.num_vertices = mesh.num_vertices;
+				vm.first_triangle = in.first_triangle;
+				vm.num_triangles = uint32_t(mesh.tris.size() / 3);
+				const uint32_t vis = uint32_t(md.visible.size());
+				md.visible.push_back(vm);
+				if(e.view_model)
+					md.view_models.push_back(vis);
+				for(uint32_t l : leafs)
+					md.leaf_models[l].push_back(vis);
+				continue;
+			}
+		}
+
 		// animate_and_transform_triangle_mesh_vertices (triangle_models_rendering.rs:7-200):
 		// the normals matrix of a rotation is the rotation itself
 		vm.first_vertex = uint32_t(md.vertices.size());
@@ -899,6 +993,22 @@ void prepare_models(swh_frame& f, const swh_map& m, const swh_frame_config& cfg,
 			md.view_models.push_back(vis);
 		for(uint32_t l : leafs)
 			md.leaf_models[l].push_back(vis);
+	}
+	if(!md.device)
+	{
+		// the host-prepared models in the device preparation's output layout
+		md.prepared_vertices.resize(md.vertices.size());
+		for(size_t i = 0; i < md.vertices.size(); ++i)
+		{
+			const ModelVertex3d& v = md.vertices[i];
+			sw_model_vertex& o = md.prepared_vertices[i];
+			o.pos[0] = v.pos.x, o.pos[1] = v.pos.y, o.pos[2] = v.pos.z;
+			o.tc[0] = v.tc[0], o.tc[1] = v.tc[1];
+			o.light[0] = v.light[0], o.light[1] = v.light[1], o.light[2] = v.light[2];
+		}
+		md.prepared_triangles.assign(md.triangles.begin(), md.triangles.end());
+		for(const VisibleModel& vm : md.visible)
+			md.prepared_counts.push_back(vm.num_triangles);
 	}
 }
 

@@ -74,6 +74,15 @@ struct ModelFrameData
 	};
 	std::vector<DecalInfo> decal_info;
 	std::vector<std::vector<uint32_t>> leaf_decals; // per leaf: decal indices
+	// the models as instances of sw_prepare_models and the meshes drawn in leaves as runs
+	bool device = false; // swh_frame_config::device_models: no host animation, runs instead of triangles
+	std::vector<sw_model_instance> instances; // per visible model
+	std::vector<sw_mesh_run> runs;
+	std::vector<uint32_t> run_positions, run_num_draws;
+	std::vector<sw_model_vertex> prepared_vertices; // host preparation: `vertices` in the ABI layout
+	std::vector<uint16_t> prepared_triangles;
+	std::vector<uint32_t> prepared_counts;
+	uint64_t vertex_capacity = 0, triangle_capacity = 0;
 };
 
 void build_dynamic_lights(swh_frame& f, const swh_map& m, const swh_frame_config& cfg);

@@ -131,13 +131,16 @@ struct DrawRec
 	uint64_t tex_offset; // texture's first element: in the arena (view offset included), skybox or lite texels
 };
 
-// A draw's screen bounds for the raster kernel's binning (8 bytes, so that testing 64 draws
-// against a tile moves little data; the full record is loaded for the draws that hit). An empty
-// draw has x_min > x_max.
+// A draw's screen bounds for the raster kernel's binning (16 bytes, so that testing 64 draws
+// against a tile moves little data; the full record is loaded for the draws that hit), with where
+// its span records lie, so that a tile can load the span records of its draws without waiting for
+// their full records. An empty draw has x_min > x_max.
 struct DrawBounds
 {
 	int16_t x_min, x_max; // columns [x_min, x_max)
-	int16_t row_min, row_max; // rows [row_min, row_max)
+	int16_t row_min, row_max; // rows [row_min, row_max): the rows whose spans are not empty
+	uint32_t span_base; // = DrawRec::span_base
+	int32_t row_first; // = DrawRec::row_first
 };
 
 enum
@@ -253,6 +256,10 @@ void launch_bloom(hipStream_t st, const BloomArgs& a, bool average);
 const uint32_t TONEMAP_BLOCKS = 2048u;
 void launch_tonemap(hipStream_t st, const uint64_t* hdr, uint32_t w, uint32_t h, float isr, float isg, float isb,
 	uint32_t* dst, uint32_t dst_pitch, float* partials, float* totals);
+// draw_mesh over prepared models: grid (triangle slots / 256, runs); bases = per run (first
+// vertex int, first fan-count byte), device copies of runs and bases; the host checked every range.
+void launch_expand_mesh_runs(hipStream_t st, const sw_mesh_run* runs, uint32_t num_runs, uint32_t max_triangles, const uint64_t* bases,
+	const sw_model_vertex* vertices, const uint16_t* triangles, const uint32_t* counts, int32_t* out_ints, uint8_t* out_fans);
 // Triangle-model preparation: one block per instance. meshes / instances / sort_offsets are device
 // copies; scratch holds the sort keys of instances above model_sort_lds_limit() triangles
 // (sort_offsets[i] entries into it, a power of two of them each).

@@ -1884,8 +1884,9 @@ SW_DEV void setup_draws_block(uint32_t block, const SetupArgs& a, const ViewsArg
 			(kind << DRAW_STAT_KIND_SHIFT) | ((d.blend & 15u) << DRAW_STAT_BLEND_SHIFT);
 		r.rows = a.span_stride ? uint32_t(st.next_span - base) : rows;
 		const bool empty = r.num_parts == 0 || row_min >= row_max || x_min >= x_max;
-		a.bounds[di] = empty ? DrawBounds{1, 0, 1, 0}
-							 : DrawBounds{int16_t(x_min), int16_t(x_max), int16_t(row_min), int16_t(row_max)};
+		a.bounds[di] = empty ? DrawBounds{1, 0, 1, 0, 0, 0}
+							 : DrawBounds{int16_t(x_min), int16_t(x_max), int16_t(row_min), int16_t(row_max), uint32_t(base),
+								   st.row_first};
 	}
 }
 
@@ -2120,9 +2121,10 @@ void k_raster_tiles(MapPtrs m, const DrawBounds* __restrict__ bounds,
 		// ordered binning: lane i tests draw base + i against this tile
 		const uint32_t di = first_draw + base + lane;
 		bool hit = false;
+		DrawBounds b = DrawBounds{1, 0, 1, 0, 0, 0};
 		if(base + lane < num_draws)
 		{
-			const DrawBounds b = bounds[di];
+			b = bounds[di];
 			hit = b.row_min < y0 + ROWS && b.row_max > y0 && b.x_min < x0 + 64 && b.x_max > x0;
 		}
 		const uint64_t mask = __ballot(hit);
@@ -2133,6 +2135,37 @@ void k_raster_tiles(MapPtrs m, const DrawBounds* __restrict__ bounds,
 		// hits' full records are loaded by those lanes
 		const uint32_t dst = hit ? uint32_t(__popcll(mask & lanes_below)) : nh + uint32_t(__popcll(~mask & lanes_below));
 		const uint32_t h_di = uint32_t(lane_push(int32_t(di), dst));
+		// With 4-row tiles in the vector span mode, the span records of the first 16 / ROWS hits are
+		// loaded now, from the bounds, while the hits' full records are loaded below: lane L takes
+		// draw L / ROWS at tile row L % ROWS, for a row whose span is not empty (inside the records).
+		constexpr bool PREFETCH = !SSPAN && ROWS == 4;
+		SpanRec next_span = SpanRec{0, 0, {0, 0, 0, 0, 0, 0}};
+		bool next_ok = false;
+		int32_t b_span_base = 0, b_row_first = 0, b_rmin = 1, b_rmax = 0;
+		if constexpr(PREFETCH)
+		{
+			b_span_base = lane_push(int32_t(b.span_base), dst);
+			b_row_first = lane_push(b.row_first, dst);
+			b_rmin = lane_push(int32_t(b.row_min), dst);
+			b_rmax = lane_push(int32_t(b.row_max), dst);
+			if(err & ERROR_SPAN_OVERFLOW)
+				return; // the host runs the batch again with room for every span record
+		}
+		auto load_group_spans_b = [&](uint32_t kgp, SpanRec& out) -> bool {
+			const uint32_t kq = (kgp + lane / ROWS) & 63u;
+			const int32_t y = y0 + int32_t(lane % ROWS);
+			const int32_t q_base = lane_pull(b_span_base, kq);
+			const int32_t q_first = lane_pull(b_row_first, kq);
+			const int32_t q_min = lane_pull(b_rmin, kq), q_max = lane_pull(b_rmax, kq);
+			if(lane < uint32_t(GD * ROWS) && kgp + lane / ROWS < nh && y >= q_min && y < q_max)
+			{
+				out = spans[uint32_t(q_base) + uint32_t(y - q_first)];
+				return true;
+			}
+			return false;
+		};
+		if constexpr(PREFETCH)
+			next_ok = load_group_spans_b(0, next_span);
 		DrawRec r{};
 		if(lane < nh)
 			r = drec[h_di];
@@ -2167,29 +2200,45 @@ void k_raster_tiles(MapPtrs m, const DrawBounds* __restrict__ bounds,
 
 		if(err & ERROR_SPAN_OVERFLOW)
 			return; // the host runs the batch again with room for every span record
+		// Lane L < GD * ROWS loads the span record of draw kg + L / ROWS at tile row L % ROWS
+		// (non-SSPAN mode). With 4-row tiles the next group's records are loaded while the current
+		// group's texels are fetched, so each group after the first waits for one memory round
+		// trip, not two (taller tiles have no registers to spare for the second set of records).
+		auto load_group_spans = [&](uint32_t kgp, SpanRec& out) -> bool {
+			const uint32_t kq = (kgp + lane / ROWS) & 63u;
+			const int32_t q_row = y0 + int32_t(lane % ROWS) - lane_pull(h_row_first, kq);
+			const int32_t q_rows = lane_pull(h_rows, kq);
+			const int32_t q_base = lane_pull(h_span_base, kq);
+			if(lane < uint32_t(GD * ROWS) && kgp + lane / ROWS < nh && q_row >= 0 && q_row < q_rows)
+			{
+				out = spans[uint32_t(q_base) + uint32_t(q_row)];
+				return true;
+			}
+			return false;
+		};
 		for(uint32_t kg = 0; kg < nh; kg += GD)
 		{
 			if constexpr(!SSPAN)
 			{
-				// Lane L < GD * ROWS loads the span record of draw kg + L / ROWS at tile row
-				// L % ROWS and tests whether its span meets the tile; the ballot of the tests
+				// Each lane tests whether its record's span meets the tile; the ballot of the tests
 				// names the active items, and only those are walked (scalar bit tests), their
 				// values read from the item's lane. Texels of the lanes outside a span are fetched
 				// at index 0 of the same texture (in bounds) and discarded by a select, so the
 				// fetches and blends change no execution mask.
 				SpanRec mine = SpanRec{0, 0, {0, 0, 0, 0, 0, 0}};
 				bool act = false;
+				if constexpr(PREFETCH)
 				{
-					const uint32_t kq = (kg + lane / ROWS) & 63u;
-					const int32_t q_row = y0 + int32_t(lane % ROWS) - lane_pull(h_row_first, kq);
-					const int32_t q_rows = lane_pull(h_rows, kq);
-					const int32_t q_base = lane_pull(h_span_base, kq);
-					if(lane < uint32_t(GD * ROWS) && kg + lane / ROWS < nh && q_row >= 0 && q_row < q_rows)
-					{
-						mine = spans[uint32_t(q_base) + uint32_t(q_row)];
-						act = mine.x0 < mine.x1 && mine.x1 > x0 && mine.x0 < x0 + 64;
-					}
+					if(next_ok)
+						mine = next_span;
+					act = next_ok;
+					next_ok = false;
+					if(kg + GD < nh)
+						next_ok = load_group_spans_b(kg + GD, next_span);
 				}
+				else
+					act = load_group_spans(kg, mine);
+				act = act && mine.x0 < mine.x1 && mine.x1 > x0 && mine.x0 < x0 + 64;
 				const uint64_t amask = __ballot(act);
 				if(amask == 0)
 					continue;

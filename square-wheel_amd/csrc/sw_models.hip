@@ -413,10 +413,181 @@ __global__ __launch_bounds__(MODEL_BLOCK) void k_prepare_models(ModelLaunch L)
 		*L.host_err = 1u;
 }
 
+// ---------------------------------------------------------------------------------------------
+// draw_mesh's per-triangle part (partial_renderer.rs:3612-3716) over prepared models: clip by the
+// run's 3D planes (clip_3d_model_polygon_by_plane, clipping.rs:259-313), project, clip by its 2D
+// planes (clip_2d_model_polygon, :203-257), then the fan's triangles as fixed16 vertex data.
+// ---------------------------------------------------------------------------------------------
+#define MESH_MAX_POLY (3u + SW_MESH_MAX_PLANES3D + 4u)
+
+struct MV
+{
+	float p[3]; // 3D position, or 2D position in p[0..1]
+	float tc[2];
+	float l[3];
+};
+
+// v0 * k1 - v1 * k0 with k0 = d0 / (d1 - d0), k1 = d1 / (d1 - d0) (get_model_line_*_intersection)
+SW_DEV MV mv_intersect(const MV& v0, const MV& v1, float d0, float d1, int np)
+{
+	const float sum = d1 - d0;
+	const float k0 = d0 / sum, k1 = d1 / sum;
+	MV r;
+	for(int i = 0; i < 3; ++i)
+		r.p[i] = i < np ? v0.p[i] * k1 - v1.p[i] * k0 : 0.0f;
+	for(int i = 0; i < 2; ++i)
+		r.tc[i] = v0.tc[i] * k1 - v1.tc[i] * k0;
+	for(int i = 0; i < 3; ++i)
+		r.l[i] = v0.l[i] * k1 - v1.l[i] * k0;
+	return r;
+}
+
+// One clip pass; `dist` of a vertex is dot(pos, plane) - offset, kept where >= 0 (the reference's
+// dot > offset / == offset tests, equivalent for finite values).
+template <int NP>
+SW_DEV uint32_t mv_clip(const MV* in, uint32_t n, const float* plane, float offset, MV* out)
+{
+	auto dist_of = [&](const MV& v) {
+		if constexpr(NP == 3)
+			return (v.p[0] * plane[0] + v.p[1] * plane[1]) + v.p[2] * plane[2] - offset;
+		else
+			return v.p[0] * plane[0] + v.p[1] * plane[1] - offset;
+	};
+	uint32_t count = 0;
+	const MV* prev = &in[n - 1];
+	float prev_d = dist_of(*prev);
+	for(uint32_t i = 0; i < n && count < MESH_MAX_POLY; ++i)
+	{
+		const MV& v = in[i];
+		const float d = dist_of(v);
+		if(d > 0.0f)
+		{
+			if(prev_d < 0.0f)
+			{
+				out[count++] = mv_intersect(*prev, v, prev_d, d, NP);
+				if(count == MESH_MAX_POLY)
+					break;
+			}
+			out[count++] = v;
+		}
+		else if(d == 0.0f)
+			out[count++] = v;
+		else if(prev_d > 0.0f)
+			out[count++] = mv_intersect(*prev, v, prev_d, d, NP);
+		prev = &v;
+		prev_d = d;
+	}
+	return count;
+}
+
+// f32_to_fixed16 (fixed_math.rs): (x * 65536) as i32, saturating, NaN -> 0
+SW_DEV int32_t mesh_fixed16(float x)
+{
+	const float y = x * 65536.0f;
+	if(y != y)
+		return 0;
+	if(y >= 2147483647.0f)
+		return INT32_MAX;
+	if(y <= -2147483648.0f)
+		return INT32_MIN;
+	return int32_t(y);
+}
+
+struct ExpandLaunch
+{
+	const sw_mesh_run* runs;
+	const uint64_t* bases; // per run: first vertex int, first fan-count byte
+	const sw_model_vertex* vertices;
+	const uint16_t* triangles;
+	const uint32_t* counts;
+	int32_t* out_ints;
+	uint8_t* out_fans;
+};
+
+// Block (x, r): triangle slots [x * 256, x * 256 + 256) of run r, one per thread.
+__global__ __launch_bounds__(256) void k_expand_mesh_runs(ExpandLaunch a)
+{
+	const sw_mesh_run& R = a.runs[blockIdx.y];
+	const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+	if(t >= R.max_triangles)
+		return;
+	const uint64_t int_base = a.bases[blockIdx.y * 2], fan_base = a.bases[blockIdx.y * 2 + 1];
+	const uint32_t visible = min(a.counts[R.count_index], R.max_triangles);
+	uint32_t fan = 0;
+	if(t < visible)
+	{
+		MV b0[MESH_MAX_POLY], b1[MESH_MAX_POLY];
+		for(int k = 0; k < 3; ++k)
+		{
+			const uint32_t vi = min(uint32_t(a.triangles[(uint64_t(R.first_triangle) + t) * 3 + k]), R.num_vertices - 1u);
+			const sw_model_vertex v = a.vertices[uint64_t(R.first_vertex) + vi];
+			b0[k] = MV{{v.pos[0], v.pos[1], v.pos[2]}, {v.tc[0], v.tc[1]}, {v.light[0], v.light[1], v.light[2]}};
+		}
+		uint32_t n = 3;
+		MV* src = b0;
+		MV* dst = b1;
+		for(uint32_t p = 0; p < R.num_planes3d && n >= 3; ++p)
+		{
+			n = mv_clip<3>(src, n, R.planes3d[p], R.planes3d[p][3], dst);
+			MV* tmp = src;
+			src = dst;
+			dst = tmp;
+		}
+		if(n >= 3)
+		{
+			// pos.truncate() / pos.z
+			for(uint32_t k = 0; k < n; ++k)
+			{
+				const float z = src[k].p[2];
+				src[k].p[0] = src[k].p[0] / z;
+				src[k].p[1] = src[k].p[1] / z;
+				src[k].p[2] = 0.0f;
+			}
+			for(uint32_t p = 0; p < R.num_planes2d && n >= 3; ++p)
+			{
+				n = mv_clip<2>(src, n, R.planes2d[p], R.planes2d[p][2], dst);
+				MV* tmp = src;
+				src = dst;
+				dst = tmp;
+			}
+		}
+		if(n >= 3)
+		{
+			const uint32_t fan_max = 1u + R.num_planes3d + R.num_planes2d;
+			fan = min(n - 2u, fan_max);
+			int32_t* out = a.out_ints + int_base + uint64_t(t) * fan_max * 21u;
+			for(uint32_t j = 0; j < fan; ++j)
+			{
+				const MV* tri[3] = {&src[0], &src[j + 1], &src[j + 2]};
+				for(int v = 0; v < 3; ++v)
+				{
+					int32_t* o = out + j * 21u + v * 7;
+					o[0] = mesh_fixed16(tri[v]->p[0]);
+					o[1] = mesh_fixed16(tri[v]->p[1]);
+					o[2] = mesh_fixed16(tri[v]->tc[0]);
+					o[3] = mesh_fixed16(tri[v]->tc[1]);
+					o[4] = mesh_fixed16(tri[v]->l[0]);
+					o[5] = mesh_fixed16(tri[v]->l[1]);
+					o[6] = mesh_fixed16(tri[v]->l[2]);
+				}
+			}
+		}
+	}
+	a.out_fans[fan_base + t] = uint8_t(fan);
+}
+
 } // namespace
 
 namespace swk
 {
+void launch_expand_mesh_runs(hipStream_t st, const sw_mesh_run* runs, uint32_t num_runs, uint32_t max_triangles, const uint64_t* bases,
+	const sw_model_vertex* vertices, const uint16_t* triangles, const uint32_t* counts, int32_t* out_ints, uint8_t* out_fans)
+{
+	if(num_runs == 0 || max_triangles == 0)
+		return;
+	ExpandLaunch a{runs, bases, vertices, triangles, counts, out_ints, out_fans};
+	hipLaunchKernelGGL(k_expand_mesh_runs, dim3((max_triangles + 255u) / 256u, num_runs), dim3(256), 0, st, a);
+}
 void launch_prepare_models(hipStream_t st, uint32_t num_instances, const sw_model_mesh* meshes, const sw_model_instance* instances,
 	const uint64_t* sort_offsets, uint64_t* scratch, sw_model_vertex* out_vertices, uint16_t* out_triangles, uint32_t* out_counts,
 	uint32_t* host_err)

@@ -77,7 +77,7 @@ struct sw_ctx
 	// command buffers / work buffers
 	DevBuf views, surfaces, lights, light_indices, draws, vdata, surf_ext, blk_surf, parts, spans, drec, dbounds, counters, arena, fb,
 		tonemap_out, tonemap_sums, bloom0, bloom1, bloom_lines, model_args, model_scratch, texop_args, shadows, depth_args, depth_vdata,
-		depth_rows, depth_draw_rows, water_points, fire_args, portal_mix;
+		depth_rows, depth_draw_rows, water_points, fire_args, portal_mix, mesh_args, mesh_fans;
 	uint32_t num_shadows = 0; // sw_set_light_shadows: entries in `shadows`
 	const float* shadow_texels = nullptr;
 	uint32_t tile_rows = 8; // raster tile height (SW_TILE_ROWS_*), set by sw_set_tile_rows / the autotuner
@@ -1428,6 +1428,74 @@ int sw_mix_portal_textures(sw_ctx* ctx, sw_map* map, uint32_t color_bits, const 
 	swk::launch_portal_mix(ctx->stream, color_bits, static_cast<const uint32_t*>(map->lite.p),
 		static_cast<const swk::PortalMixRec*>(ctx->portal_mix.p), num_mixes, max_texels, portal_texels_device);
 	return hip_check(ctx, hipGetLastError(), "portal mix launch");
+}
+
+int sw_expand_mesh_runs(sw_ctx* ctx, const sw_mesh_run* runs, uint32_t num_runs, const sw_model_vertex* prepared_vertices_device,
+	uint64_t vertex_capacity, const uint16_t* prepared_triangles_device, uint64_t triangle_capacity, const uint32_t* counts_device,
+	uint32_t num_counts, int32_t* out_vertex_ints_device, uint64_t vertex_int_capacity, uint8_t* out_fan_counts)
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	if(num_runs == 0)
+		return SW_OK;
+	if(!runs || !prepared_vertices_device || !prepared_triangles_device || !counts_device || !out_vertex_ints_device ||
+		!out_fan_counts)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	// per run: its vertex ints and fan-count bytes, back to back; every index range checked here
+	std::vector<uint64_t> bases(size_t(num_runs) * 2);
+	uint64_t ints = 0, slots = 0;
+	uint32_t max_triangles = 0;
+	for(uint32_t i = 0; i < num_runs; ++i)
+	{
+		const sw_mesh_run& r = runs[i];
+		const std::string which = "mesh run " + std::to_string(i) + ": ";
+		if(r.count_index >= num_counts)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, which + "count index out of range");
+		if(r.num_planes3d > SW_MESH_MAX_PLANES3D || r.num_planes2d > 4)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, which + "too many clip planes");
+		if(r.max_triangles > SW_MODEL_MAX_TRIANGLES || r.num_vertices == 0 || r.num_vertices > 65536)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, which + "mesh size out of range");
+		if(uint64_t(r.first_vertex) + r.num_vertices > vertex_capacity ||
+			uint64_t(r.first_triangle) + r.max_triangles > triangle_capacity)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, which + "prepared vertices or triangles beyond the buffers");
+		if(r.blend > SW_BLEND_ALPHA_BLEND)
+			return set_error(ctx, SW_ERR_INVALID_ARGUMENT, which + "unknown blending mode");
+		const uint64_t fan = 1u + r.num_planes3d + r.num_planes2d;
+		bases[size_t(i) * 2] = ints;
+		bases[size_t(i) * 2 + 1] = slots;
+		ints += uint64_t(r.max_triangles) * fan * 21u;
+		slots += r.max_triangles;
+		max_triangles = std::max(max_triangles, r.max_triangles);
+	}
+	if(ints > vertex_int_capacity)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "vertex int capacity below the runs' fan slots");
+	if(slots == 0)
+		return SW_OK;
+	(void)hipSetDevice(ctx->device);
+	const size_t run_bytes = sizeof(sw_mesh_run) * num_runs, base_at = (run_bytes + 255) & ~size_t(255),
+				 total = base_at + sizeof(uint64_t) * bases.size();
+	int r;
+	if((r = ensure(ctx, ctx->mesh_args, total)) || (r = ensure(ctx, ctx->mesh_fans, slots)))
+		return r;
+	char* staging = nullptr;
+	if((r = stage_begin(ctx, total, &staging)))
+		return r;
+	std::memcpy(staging, runs, run_bytes);
+	std::memcpy(staging + base_at, bases.data(), sizeof(uint64_t) * bases.size());
+	if((r = stage_copy(ctx, ctx->mesh_args.p, total)))
+		return r;
+	{
+		StageTimer t(ctx, STAGE_MODELS);
+		swk::launch_expand_mesh_runs(ctx->stream, static_cast<const sw_mesh_run*>(ctx->mesh_args.p), num_runs, max_triangles,
+			reinterpret_cast<const uint64_t*>(static_cast<const char*>(ctx->mesh_args.p) + base_at), prepared_vertices_device,
+			prepared_triangles_device, counts_device, out_vertex_ints_device, static_cast<uint8_t*>(ctx->mesh_fans.p));
+	}
+	if((r = hip_check(ctx, hipGetLastError(), "mesh expansion launch")))
+		return r;
+	if((r = hip_check(ctx, hipMemcpyAsync(out_fan_counts, ctx->mesh_fans.p, slots, hipMemcpyDeviceToHost, ctx->stream),
+			"fan counts copy")))
+		return r;
+	return hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
 }
 
 int sw_prepare_models(sw_ctx* ctx, const sw_model_mesh* meshes, uint32_t num_meshes, const sw_model_instance* instances,

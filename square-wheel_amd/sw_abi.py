@@ -65,6 +65,26 @@ MODEL_INSTANCE_DTYPE = np.dtype([
     ("tc_scale", "<f4", 2), ("tc_shift", "<f4", 2), ("light_cube", "<f4", (6, 3)), ("light_vector_scaled", "<f4", (2, 3)),
     ("light_color", "<f4", (2, 3)), ("first_vertex", "<u4"), ("first_triangle", "<u4")], align=True)
 MODEL_VERTEX_DTYPE = np.dtype([("pos", "<f4", 3), ("tc", "<f4", 2), ("light", "<f4", 3)], align=True)
+# meshes drawn from prepared models (sw_expand_mesh_runs)
+MESH_MAX_PLANES3D = 7
+MESH_RUN_DTYPE = np.dtype([
+    ("count_index", "<u4"), ("first_vertex", "<u4"), ("num_vertices", "<u4"), ("first_triangle", "<u4"),
+    ("max_triangles", "<u4"), ("num_planes3d", "<u4"), ("num_planes2d", "<u4"), ("blend", "<u4"),
+    ("planes3d", "<f4", (MESH_MAX_PLANES3D, 4)), ("planes2d", "<f4", (4, 3)), ("texture_offset", "<u8"),
+    ("texture_size", "<i4", 2), ("clip_rect", "<i4", 4)], align=True)
+
+
+def mesh_run_layout(runs):
+    """sw_expand_mesh_runs' output layout: per run (first vertex int, fan size bound, first fan-count
+    byte), and the total ints and bytes."""
+    ints = slots = 0
+    out = []
+    for r in runs:
+        fan = 1 + int(r["num_planes3d"]) + int(r["num_planes2d"])
+        out.append((ints, fan, slots))
+        ints += int(r["max_triangles"]) * fan * 21
+        slots += int(r["max_triangles"])
+    return out, ints, slots
 # shadow maps (sw_set_light_shadows)
 SHADOW_NONE, SHADOW_CUBE, SHADOW_PROJECTOR = 0, 1, 2
 LIGHT_SHADOW_DTYPE = np.dtype([("kind", "<u4"), ("size", "<u4"), ("offset", "<u8"), ("basis_x", "<f4", 3), ("basis_y", "<f4", 3),
@@ -305,6 +325,8 @@ GPU_SYMBOLS = {
     "sw_raster_draws": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]),
     "sw_set_portal_textures": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     "sw_mix_portal_textures": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]),
+    "sw_expand_mesh_runs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
+                                      C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.c_void_p]),
     "sw_frame_end": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint32]),
     "sw_tonemap": (C.c_int, [C.c_void_p, C.c_float, C.POINTER(C.c_float), C.c_void_p, C.c_size_t, C.c_uint32,
                              C.POINTER(C.c_float)]),
@@ -649,6 +671,20 @@ class Context:
             for b in bufs + [vbuf, tbuf, cbuf]:
                 b.free()
         return verts, tris, counts
+
+    def expand_mesh_runs(self, runs, vertices, vertex_capacity, triangles, triangle_capacity, counts, num_counts, out_ints_ptr,
+                         int_capacity):
+        """sw_expand_mesh_runs over device buffers (vertices / triangles / counts: DeviceBuffer of
+        sw_prepare_models' outputs; out_ints_ptr: a device address). Returns the fan sizes (u8 per
+        triangle slot)."""
+        runs = np.ascontiguousarray(runs, dtype=MESH_RUN_DTYPE)
+        _, _, slots = mesh_run_layout(runs)
+        fans = np.zeros(max(slots, 1), dtype=np.uint8)
+        self.check(self.lib.sw_expand_mesh_runs(self.h, _ptr(runs) if runs.size else None, len(runs), vertices.ptr,
+                                                int(vertex_capacity), triangles.ptr, int(triangle_capacity), counts.ptr,
+                                                int(num_counts), C.c_void_p(out_ints_ptr), int(int_capacity), _ptr(fans)),
+                   "sw_expand_mesh_runs")
+        return fans[:slots]
 
     # device memory -----------------------------------------------------------------------
     def alloc(self, nbytes):

@@ -33,7 +33,20 @@ class FrameConfig(C.Structure):
                 ("clear_background", C.c_uint32), ("force_clear", C.c_uint32), ("num_dynamic_lights", C.c_uint32),
                 ("num_models", C.c_uint32), ("num_decals", C.c_uint32), ("seed", C.c_uint32), ("time", C.c_float),
                 ("virtual_threads", C.c_uint32), ("view_model", C.c_uint32),
-                ("portals_depth", C.c_uint32)]
+                ("portals_depth", C.c_uint32), ("device_models", C.c_uint32)]
+
+
+class ModelData(C.Structure):
+    _fields_ = [("instances", C.c_void_p), ("num_instances", C.c_uint32), ("runs", C.c_void_p), ("num_runs", C.c_uint32),
+                ("run_positions", C.c_void_p), ("run_num_draws", C.c_void_p), ("prepared_vertices", C.c_void_p),
+                ("num_prepared_vertices", C.c_uint64), ("prepared_triangles", C.c_void_p),
+                ("num_prepared_triangles", C.c_uint64), ("prepared_counts", C.c_void_p), ("num_prepared_counts", C.c_uint32),
+                ("vertex_capacity", C.c_uint64), ("triangle_capacity", C.c_uint64)]
+
+
+class MeshDataC(C.Structure):
+    _fields_ = [("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32), ("num_frames", C.c_uint32),
+                ("tc", C.c_void_p), ("frame_vertices", C.c_void_p), ("triangles", C.c_void_p)]
 
 
 class ViewC(C.Structure):
@@ -97,6 +110,10 @@ def load():
         lib.swh_frame_portal_info.restype = C.c_int
         lib.swh_map_view_portals.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
         lib.swh_map_view_portals.restype = C.c_int
+        lib.swh_frame_model_data.argtypes = [C.c_void_p, C.POINTER(ModelData)]
+        lib.swh_frame_model_data.restype = C.c_int
+        lib.swh_map_model_mesh.argtypes = [C.c_void_p, C.POINTER(MeshDataC)]
+        lib.swh_map_model_mesh.restype = C.c_int
         lib.swh_shadow_create.argtypes = [C.POINTER(C.c_void_p)]
         lib.swh_shadow_create.restype = C.c_int
         lib.swh_shadow_destroy.argtypes = [C.c_void_p]
@@ -217,6 +234,17 @@ class Map:
             assert lib.swh_map_view_portals(self.h, C.c_void_p(out.ctypes.data), n) == n
         return out
 
+    def model_mesh(self):
+        """The map's animated model mesh (swh_map_model_mesh) as sw_model_mesh's VERTEX_ANIMATED data."""
+        md = MeshDataC()
+        if load().swh_map_model_mesh(self.h, C.byref(md)) != 0:
+            raise RuntimeError("swh_map_model_mesh failed")
+        nv, nt, nf = int(md.num_vertices), int(md.num_triangles), int(md.num_frames)
+        tc = _copy(md.tc, nv * 2, np.float32).reshape(nv, 2)
+        fv = _copy(md.frame_vertices, nf * nv * 6, np.float32).reshape(nf, nv, 6)
+        tris = _copy(md.triangles, nt * 3, np.uint16).reshape(nt, 3)
+        return A.ModelMeshData(A.MODEL_VERTEX_ANIMATED, tc, tris, frame_vertices=fv)
+
     def camera_path(self, seed, n):
         out = np.zeros((n, 5), dtype=np.float32)
         if load().swh_camera_path(self.h, seed, n, C.c_void_p(out.ctypes.data)) != 0:
@@ -252,7 +280,22 @@ def _frame_dict(h, color_bits):
         if lib.swh_frame_portal_view(h, i, C.byref(ch)) != 0:
             raise RuntimeError("swh_frame_portal_view failed")
         views.append(_frame_dict(ch, color_bits))
+    md = ModelData()
+    if lib.swh_frame_model_data(h, C.byref(md)) != 0:
+        raise RuntimeError("swh_frame_model_data failed")
+    models = {
+        "instances": _copy(md.instances, md.num_instances, A.MODEL_INSTANCE_DTYPE),
+        "runs": _copy(md.runs, md.num_runs, A.MESH_RUN_DTYPE),
+        "run_positions": _copy(md.run_positions, md.num_runs, np.uint32),
+        "run_num_draws": _copy(md.run_num_draws, md.num_runs, np.uint32),
+        "prepared_vertices": _copy(md.prepared_vertices, md.num_prepared_vertices, A.MODEL_VERTEX_DTYPE),
+        "prepared_triangles": _copy(md.prepared_triangles, md.num_prepared_triangles * 3, np.uint16).reshape(-1, 3),
+        "prepared_counts": _copy(md.prepared_counts, md.num_prepared_counts, np.uint32),
+        "vertex_capacity": int(md.vertex_capacity),
+        "triangle_capacity": int(md.triangle_capacity),
+    }
     return {
+        "models": models,
         "view": np.frombuffer(bytes(fd.view), dtype=A.VIEW_DTYPE).copy(),
         "surfaces": _copy(fd.surfaces, fd.num_surfaces, A.SURFACE_DTYPE),
         "lights": _copy(fd.lights, fd.num_lights, A.LIGHT_DTYPE),
@@ -283,7 +326,7 @@ class FrameBuilder:
         defaults = dict(width=320, height=200, color_bits=32, fov=math.pi / 2, textures_mip_bias=0.0,
                         use_directional_lightmaps=0, clear_background=1, force_clear=1, num_dynamic_lights=0,
                         num_models=0, num_decals=0, seed=0, time=0.0, virtual_threads=1, view_model=0,
-                        portals_depth=2)
+                        portals_depth=2, device_models=0)
         defaults.update(kw)
         for k, v in defaults.items():
             setattr(cfg, k, v)
@@ -424,6 +467,105 @@ def portal_levels(frame):
         px = sum(int(c["view"]["width"][0]) * int(c["view"]["height"][0]) for c in nxt)
         assert tex == px, (tex, px)
         levels.append(nxt)
+
+
+def assemble_model_draws(frame, fans, int_base):
+    """The draw list of a frame built with device_models: the frame's draws with, at each mesh run's
+    position, the triangles of its fans (sw_expand_mesh_runs' fan sizes `fans`), triangle t's fan
+    in order, t in order; their vertex data sits at vertex int int_base + the run's layout."""
+    md = frame["models"]
+    runs, pos = md["runs"], md["run_positions"]
+    layout, _, _ = A.mesh_run_layout(runs)
+    draws = frame["draws"]
+    pieces = []
+    start = 0
+    for r, run in enumerate(runs):
+        p = int(pos[r])
+        pieces.append(draws[start:p])
+        start = p
+        ib, fan_max, fb = layout[r]
+        nt = int(run["max_triangles"])
+        f = np.asarray(fans[fb:fb + nt], dtype=np.int64)
+        t_idx = np.repeat(np.arange(nt), f)
+        j_idx = np.arange(int(f.sum())) - np.repeat(np.cumsum(f) - f, f)
+        d = np.zeros(len(t_idx), dtype=A.DRAW_DTYPE)
+        d["kind"] = A.DRAW_TRIANGLE
+        d["blend"] = run["blend"]
+        d["first_vertex_int"] = int_base + ib + (t_idx * fan_max + j_idx) * 21
+        d["num_vertices"] = 3
+        d["texture_offset"] = run["texture_offset"]
+        d["texture_size"] = run["texture_size"]
+        d["clip_rect"] = run["clip_rect"]
+        pieces.append(d)
+    pieces.append(draws[start:])
+    return np.concatenate(pieces) if pieces else draws
+
+
+class DeviceModels:
+    """Frames built with device_models drawn with their models prepared on the device: the map's
+    mesh stays in HBM; per frame sw_prepare_models (animation, lighting, back faces, sort), then
+    sw_expand_mesh_runs (draw_mesh's clipping and fans) into the frame's vertex data in HBM, the
+    runs' triangles put into the draw list at their places, and one sw_render_batch."""
+
+    def __init__(self, ctx, m: Map):
+        self.ctx = ctx
+        self.mesh = m.model_mesh()
+        self.mesh_desc, self.mesh_bufs = ctx.upload_meshes([self.mesh])
+
+    def render(self, map_h, frame, out=None):
+        """Returns (framebuffer, fans); `out`: a device framebuffer to draw into (kept), else one
+        allocated and read back."""
+        ctx = self.ctx
+        md = frame["models"]
+        inst = np.ascontiguousarray(md["instances"])
+        vcap, tcap = max(md["vertex_capacity"], 1), max(md["triangle_capacity"], 1)
+        base = batch_from_frames([frame])
+        host_ints = len(base.vertex_data)
+        _, run_ints, _ = A.mesh_run_layout(md["runs"])
+        bufs = []
+        try:
+            vbuf = ctx.alloc(vcap * A.MODEL_VERTEX_DTYPE.itemsize)
+            tbuf = ctx.alloc(tcap * 6 + 16)
+            cbuf = ctx.alloc(max(len(inst), 1) * 4)
+            ints = ctx.alloc(max(host_ints + run_ints, 1) * 4)
+            bufs += [vbuf, tbuf, cbuf, ints]
+            if len(inst):
+                ctx.check(ctx.lib.sw_prepare_models(ctx.h, self.mesh_desc, 1, A._ptr(inst), len(inst), vbuf.ptr, vcap, tbuf.ptr,
+                                                    tcap, cbuf.ptr), "sw_prepare_models")
+            if host_ints:
+                ctx.check(ctx.lib.sw_copy_to_device(ctx.h, ints.ptr, A._ptr(base.vertex_data), host_ints * 4), "sw_copy_to_device")
+            fans = ctx.expand_mesh_runs(md["runs"], vbuf, vcap, tbuf, tcap, cbuf, len(inst),
+                                        (ints.ptr.value or 0) + host_ints * 4, run_ints)
+            draws = assemble_model_draws(frame, fans, host_ints)
+            f2 = dict(frame, draws=draws, vertex_data=np.zeros(host_ints + run_ints, np.int32))
+            b = batch_from_frames([f2])
+            dev = {"surfaces": ctx.to_device(b.surfaces), "lights": ctx.to_device(b.lights),
+                   "light_indices": ctx.to_device(b.light_indices), "draws": ctx.to_device(b.draws)}
+            bufs += list(dev.values())
+            ptrs = {k: v.ptr for k, v in dev.items()}
+            ptrs["vertex_data"] = ints.ptr
+            if out is not None:
+                cb = b.ctypes_batch(inputs_on_device=True, device_ptrs=ptrs)
+                ctx.check(ctx.lib.sw_render_batch(ctx.h, map_h, C.byref(cb), out.ptr), "sw_render_batch")
+                return None, fans
+            img = ctx.render_batch(map_h, b, device_inputs=_Ptrs(ptrs))
+            return img, fans
+        finally:
+            for x in bufs:
+                x.free()
+
+    def close(self):
+        for b in self.mesh_bufs:
+            b.free()
+        self.mesh_bufs = []
+
+
+class _Ptrs:
+    def __init__(self, p):
+        self.p = p
+
+    def ptrs(self):
+        return self.p
 
 
 def portal_level_batches(frame):

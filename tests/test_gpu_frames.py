@@ -258,3 +258,49 @@ def test_portal_texture_mix_bit_exact(gpu_ctx, color_bits):
     finally:
         buf.free()
         gpu_ctx.release_map(mh)
+
+
+@pytest.mark.parametrize("color_bits", [32, 64])
+def test_device_model_frames_bit_exact(gpu_ctx, color_bits):
+    """Frames whose models are prepared on the device: sw_prepare_models, sw_expand_mesh_runs (the
+    meshes clipped by near, leaf and 2D box planes, fans in place) and one sw_render_batch, against
+    the oracle's preparation, expansion and frame; the fan sizes equal the oracle's."""
+    m = sw_host.large_map(0, grid=(12, 12, 4))
+    poses = m.camera_path(3, 40)
+    mesh = m.model_mesh()
+    fbld = sw_host.FrameBuilder()
+    mh = gpu_ctx.upload_map(m.data)
+    dm = sw_host.DeviceModels(gpu_ctx, m)
+    try:
+        clipped = 0
+        for i in (7, 21, 31, 0):
+            _say(f"case device models pose {i} bits {color_bits}")
+            f = fbld.build(m, poses[i], width=640, height=360, color_bits=color_bits, use_directional_lightmaps=1,
+                           num_models=24, num_decals=10, seed=i, time=0.3 * i, view_model=1, device_models=1)
+            md = f["models"]
+            clipped += int(np.sum(md["runs"]["num_planes3d"] > 0))
+            img, fans = dm.render(mh, f)
+            verts, tris, counts, ok = oracle.prepare_models([mesh], md["instances"])
+            assert ok
+            ints, ofans = oracle.expand_mesh_runs(md["runs"], verts, tris, counts)
+            assert np.array_equal(fans, ofans), f"pose {i}: fan sizes differ"
+            draws = sw_host.assemble_model_draws(f, ofans, len(f["vertex_data"]))
+            f2 = dict(f, draws=draws, vertex_data=np.concatenate([f["vertex_data"], ints]).astype(np.int32))
+            ref, _, st, oob = oracle.render(m.data, sw_host.batch_from_frames([f2]))
+            assert oob == 0 and st["triangle_pixels"] > 0
+            assert np.array_equal(img, ref), f"pose {i}: {np.count_nonzero(img != ref)} pixels differ"
+        assert clipped >= 3
+        # a run whose count index is out of range is an error
+        bad = f["models"]["runs"][:1].copy()
+        bad["count_index"] = 1000
+        vb, tb, cb = gpu_ctx.alloc(1 << 16), gpu_ctx.alloc(1 << 16), gpu_ctx.alloc(64)
+        ib = gpu_ctx.alloc(1 << 20)
+        try:
+            with pytest.raises(RuntimeError):
+                gpu_ctx.expand_mesh_runs(bad, vb, 1, tb, 1, cb, 1, ib.ptr.value, (1 << 20) // 4)
+        finally:
+            for b in (vb, tb, cb, ib):
+                b.free()
+    finally:
+        dm.close()
+        gpu_ctx.release_map(mh)
