@@ -508,6 +508,62 @@ def models_leg(ctx, args):
     return out
 
 
+def device_models_leg(ctx, args):
+    """The HDR + models frames with their models prepared on the device (swh_frame_config.device_models):
+    the front-end's CPU time per frame with and without its own model preparation, and the frame
+    through sw_prepare_models + sw_expand_mesh_runs (which waits for the fan sizes) + the draw list
+    completed on the host + sw_render_batch, against the same frames with host-prepared models
+    (both with host-resident command buffers, uploaded per frame)."""
+    import ctypes as C
+    W, H = 2560, 1440
+    m = sw_host.large_map_alpha(args.seed)
+    mh = ctx.upload_map(m.data)
+    fbld = sw_host.FrameBuilder()
+    poses = m.camera_path(args.seed + 3, 16)
+    kw = dict(width=W, height=H, color_bits=64, use_directional_lightmaps=1, num_dynamic_lights=16, num_models=64,
+              num_decals=256, seed=args.seed + 5, clear_background=1, force_clear=1)
+    fe = {}
+    frames = {}
+    for dev in (0, 1):
+        t0 = time.perf_counter()
+        frames[dev] = [fbld.build(m, p, time=0.1 * i, device_models=dev, **kw) for i, p in enumerate(poses)]
+        fe[dev] = (time.perf_counter() - t0) / len(poses) * 1e3
+    dm = sw_host.DeviceModels(ctx, m)
+    hdr = ctx.alloc(W * H * 8)
+    host_batches = [sw_host.batch_from_frames([f]) for f in frames[0]]
+    lib = ctx.lib
+    cbs = [b.ctypes_batch() for b in host_batches]
+    for i in range(len(poses)):
+        dm.render(mh, frames[1][i], out=hdr)
+        ctx.check(lib.sw_render_batch(ctx.h, mh, C.byref(cbs[i]), hdr.ptr), "sw_render_batch")
+    reps = 4 * len(poses)
+    ctx.profile_read(reset=True)
+    ctx.profile_enable(True)
+    t0 = time.perf_counter()
+    for i in range(reps):
+        dm.render(mh, frames[1][i % len(poses)], out=hdr)
+    dev_ms = (time.perf_counter() - t0) / reps * 1e3
+    ctx.profile_enable(False)
+    prof = ctx.profile_read(reset=True)
+    t0 = time.perf_counter()
+    for i in range(reps):
+        ctx.check(lib.sw_render_batch(ctx.h, mh, C.byref(cbs[i % len(poses)]), hdr.ptr), "sw_render_batch")
+    host_ms = (time.perf_counter() - t0) / reps * 1e3
+    runs = float(np.mean([len(f["models"]["runs"]) for f in frames[1]]))
+    tris = float(np.mean([int(np.sum(f["draws"]["kind"] == sw_abi.DRAW_TRIANGLE)) for f in frames[0]]))
+    dm.close()
+    hdr.free()
+    ctx.release_map(mh)
+    out = {"workload": f"{W}x{H} Color64, 64 models (1280-triangle vertex-animated mesh), 256 decals, 16 lights, "
+                       f"{len(poses)}-pose path; command buffers uploaded per frame",
+           "front_end_ms_host_models": round(fe[0], 3), "front_end_ms_device_models": round(fe[1], 3),
+           "frame_ms_device_models": round(dev_ms, 4), "frame_ms_host_models": round(host_ms, 4),
+           "prepare_and_expand_us_per_frame": round(prof["models"][0] / reps * 1e3, 2),
+           "mesh_runs_per_frame": round(runs, 1), "model_and_decal_triangle_draws_per_frame": round(tris, 1)}
+    log(f"device models: {out}")
+    return out
+
+
 def serving_leg(args):
     """Open-loop serving: four resident maps (16x16x4 .. 48x48x4 grids), Poisson arrivals at fixed
     rates, each request a random map, pose, resolution (log-uniform 320x200 .. 2560x1440, 16:10) and
@@ -862,6 +918,7 @@ def main():
 
     if not args.no_models:
         result["models"] = models_leg(ctx, args)
+        result["device_models"] = device_models_leg(ctx, args)
 
     if not args.no_water:
         result["water"] = water_leg(ctx, args)

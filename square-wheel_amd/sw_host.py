@@ -505,67 +505,69 @@ class DeviceModels:
     """Frames built with device_models drawn with their models prepared on the device: the map's
     mesh stays in HBM; per frame sw_prepare_models (animation, lighting, back faces, sort), then
     sw_expand_mesh_runs (draw_mesh's clipping and fans) into the frame's vertex data in HBM, the
-    runs' triangles put into the draw list at their places, and one sw_render_batch."""
+    runs' triangles put into the draw list at their places, and one sw_render_batch. The device
+    buffers grow as needed and are kept between frames."""
 
     def __init__(self, ctx, m: Map):
         self.ctx = ctx
         self.mesh = m.model_mesh()
         self.mesh_desc, self.mesh_bufs = ctx.upload_meshes([self.mesh])
+        self.bufs = {}
+
+    def _buf(self, name, nbytes):
+        b = self.bufs.get(name)
+        if b is None or b.nbytes < nbytes:
+            if b is not None:
+                b.free()
+            b = self.bufs[name] = self.ctx.alloc(max(nbytes, 256))
+        return b
+
+    def _upload(self, name, a):
+        b = self._buf(name, a.nbytes)
+        if a.nbytes:
+            self.ctx.check(self.ctx.lib.sw_copy_to_device(self.ctx.h, b.ptr, A._ptr(a), a.nbytes), "sw_copy_to_device")
+        return b
 
     def render(self, map_h, frame, out=None):
-        """Returns (framebuffer, fans); `out`: a device framebuffer to draw into (kept), else one
-        allocated and read back."""
+        """Returns (framebuffer, fans); `out`: a device framebuffer to draw into (kept, nothing read
+        back), else the framebuffer is read back."""
         ctx = self.ctx
         md = frame["models"]
         inst = np.ascontiguousarray(md["instances"])
         vcap, tcap = max(md["vertex_capacity"], 1), max(md["triangle_capacity"], 1)
-        base = batch_from_frames([frame])
-        host_ints = len(base.vertex_data)
+        host_ints = len(frame["vertex_data"])
         _, run_ints, _ = A.mesh_run_layout(md["runs"])
-        bufs = []
-        try:
-            vbuf = ctx.alloc(vcap * A.MODEL_VERTEX_DTYPE.itemsize)
-            tbuf = ctx.alloc(tcap * 6 + 16)
-            cbuf = ctx.alloc(max(len(inst), 1) * 4)
-            ints = ctx.alloc(max(host_ints + run_ints, 1) * 4)
-            bufs += [vbuf, tbuf, cbuf, ints]
-            if len(inst):
-                ctx.check(ctx.lib.sw_prepare_models(ctx.h, self.mesh_desc, 1, A._ptr(inst), len(inst), vbuf.ptr, vcap, tbuf.ptr,
-                                                    tcap, cbuf.ptr), "sw_prepare_models")
-            if host_ints:
-                ctx.check(ctx.lib.sw_copy_to_device(ctx.h, ints.ptr, A._ptr(base.vertex_data), host_ints * 4), "sw_copy_to_device")
-            fans = ctx.expand_mesh_runs(md["runs"], vbuf, vcap, tbuf, tcap, cbuf, len(inst),
-                                        (ints.ptr.value or 0) + host_ints * 4, run_ints)
-            draws = assemble_model_draws(frame, fans, host_ints)
-            f2 = dict(frame, draws=draws, vertex_data=np.zeros(host_ints + run_ints, np.int32))
-            b = batch_from_frames([f2])
-            dev = {"surfaces": ctx.to_device(b.surfaces), "lights": ctx.to_device(b.lights),
-                   "light_indices": ctx.to_device(b.light_indices), "draws": ctx.to_device(b.draws)}
-            bufs += list(dev.values())
-            ptrs = {k: v.ptr for k, v in dev.items()}
-            ptrs["vertex_data"] = ints.ptr
-            if out is not None:
-                cb = b.ctypes_batch(inputs_on_device=True, device_ptrs=ptrs)
-                ctx.check(ctx.lib.sw_render_batch(ctx.h, map_h, C.byref(cb), out.ptr), "sw_render_batch")
-                return None, fans
-            img = ctx.render_batch(map_h, b, device_inputs=_Ptrs(ptrs))
-            return img, fans
-        finally:
-            for x in bufs:
-                x.free()
+        vbuf = self._buf("vertices", vcap * A.MODEL_VERTEX_DTYPE.itemsize)
+        tbuf = self._buf("triangles", tcap * 6)
+        cbuf = self._buf("counts", max(len(inst), 1) * 4)
+        ints = self._buf("vertex_data", (host_ints + run_ints) * 4)
+        if len(inst):
+            ctx.check(ctx.lib.sw_prepare_models(ctx.h, self.mesh_desc, 1, A._ptr(inst), len(inst), vbuf.ptr, vcap, tbuf.ptr,
+                                                tcap, cbuf.ptr), "sw_prepare_models")
+        if host_ints:
+            vd = np.ascontiguousarray(frame["vertex_data"], dtype=np.int32)
+            ctx.check(ctx.lib.sw_copy_to_device(ctx.h, ints.ptr, A._ptr(vd), host_ints * 4), "sw_copy_to_device")
+        fans = ctx.expand_mesh_runs(md["runs"], vbuf, vcap, tbuf, tcap, cbuf, len(inst),
+                                    (ints.ptr.value or 0) + host_ints * 4, run_ints)
+        draws = assemble_model_draws(frame, fans, host_ints)
+        b = batch_from_frames([dict(frame, draws=draws, vertex_data=np.zeros(0, np.int32))])
+        ptrs = {k: self._upload(k, getattr(b, k)).ptr for k in ("surfaces", "lights", "light_indices", "draws")}
+        ptrs["vertex_data"] = ints.ptr
+        cb = b.ctypes_batch(inputs_on_device=True, device_ptrs=ptrs)
+        cb.num_vertex_ints = host_ints + run_ints
+        if out is not None:
+            ctx.check(ctx.lib.sw_render_batch(ctx.h, map_h, C.byref(cb), out.ptr), "sw_render_batch")
+            return None, fans
+        nbytes = max(b.fb_texels, 1) * (8 if b.color_bits == 64 else 4)
+        fb = self._buf("framebuffer", nbytes)
+        ctx.check(ctx.lib.sw_render_batch(ctx.h, map_h, C.byref(cb), fb.ptr), "sw_render_batch")
+        return fb.read(b.color_dtype, b.fb_texels), fans
 
     def close(self):
-        for b in self.mesh_bufs:
+        for b in list(self.mesh_bufs) + list(self.bufs.values()):
             b.free()
         self.mesh_bufs = []
-
-
-class _Ptrs:
-    def __init__(self, p):
-        self.p = p
-
-    def ptrs(self):
-        return self.p
+        self.bufs = {}
 
 
 def portal_level_batches(frame):
