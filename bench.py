@@ -204,7 +204,7 @@ def hdr_leg(ctx, args):
         ctx.check(lib.sw_tonemap_device(ctx.h, hdr.ptr, W, H, C.c_float(1.0), mod, ldr.ptr, W * 4, sums), "sw_tonemap_device")
 
     tris = [int(np.sum(f["draws"]["kind"] == sw_abi.DRAW_TRIANGLE)) for f in frames]
-    rows, tune = ctx.autotune(mh, batches[0], dev[0], hdr)
+    rows, tune = ctx.autotune_many(mh, batches[::4], dev[::4], hdr)
     log(f"autotune (hdr): {rows} rows, scalar spans {ctx.scalar_spans()}; ms per frame by rows {tune}")
     for i in range(4 * len(cbs)):
         step(i)
@@ -731,8 +731,8 @@ def main():
         st["emissive_texels"] = 0
         per_frame_stats.append(st)
 
-    # tile height for this frame size, chosen by the autotuner on the first pose
-    plain_rows, plain_tune = ctx.autotune(mh, batches[0], dev[0], out)
+    # tile height and span mode for this frame size, chosen by the autotuner on every fourth pose
+    plain_rows, plain_tune = ctx.autotune_many(mh, batches[::4], dev[::4], out)
     log(f"autotune (plain): {plain_rows} rows, scalar spans {ctx.scalar_spans()}; ms per frame by rows {plain_tune}")
     for i in range(args.warmup):
         render(i)
@@ -758,7 +758,7 @@ def main():
     dev2 = [sw_abi.DeviceCommandBuffers(ctx2, b) for b in batches]
     out2 = ctx2.alloc(args.width * args.height * 4)
     cbs2 = [b.ctypes_batch(inputs_on_device=True, device_ptrs=d.ptrs()) for b, d in zip(batches, dev2)]
-    ctx2.autotune(mh2, batches[0], dev2[0], out2)
+    ctx2.autotune_many(mh2, batches[::4], dev2[::4], out2)
     pipe = [(ctx, mh, cbs, out), (ctx2, mh2, cbs2, out2)]
 
     def launch(i):

@@ -380,6 +380,12 @@ uint32_t sw_get_scalar_spans(const sw_ctx* ctx);
 int sw_autotune_batch(
 	sw_ctx* ctx, sw_map* map, const sw_batch* batch, void* out_framebuffer_device, uint32_t* out_rows,
 	double out_ms[SW_TILE_ROWS_COUNT]);
+/* The same over a sample of batches (each candidate timed over all of them, median of five
+ * rounds; out_ms per batch): for a path whose frames differ, such as a camera path. out_framebuffer
+ * holds the largest batch's framebuffer. */
+int sw_autotune_batches(
+	sw_ctx* ctx, sw_map* map, const sw_batch* batches, uint32_t num_batches, void* out_framebuffer_device,
+	uint32_t* out_rows, double out_ms[SW_TILE_ROWS_COUNT]);
 
 /* Tonemap a device Color64 buffer (pitch = width) into a device Color32 buffer. */
 int sw_tonemap_device(

@@ -150,6 +150,7 @@ struct sw_ctx
 	hipGraph_t graph = nullptr;
 	uint64_t graph_key = 0;
 	uint64_t graph_epoch = 0;
+	uint64_t graph_views_hash = 0; // content of the views array the captured graph last read
 
 	sw_stats stats{};
 
@@ -2235,6 +2236,17 @@ int sw_render_batch_graph(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out
 		mix(b->views[i].num_draws);
 		mix(b->views[i].num_surfaces);
 	}
+	// content of the views (camera positions, framebuffer offsets, clear flags may change between
+	// replays without a new capture): a replay with the same views skips rebuilding and uploading them
+	static_assert(sizeof(sw_view) % 8 == 0, "sw_view hashed as 64-bit words");
+	uint64_t views_hash = 1469598103934665603ull ^ b->framebuffer_texels;
+	for(uint32_t i = 0; i < b->num_views; ++i)
+	{
+		uint64_t w[sizeof(sw_view) / 8];
+		std::memcpy(w, &b->views[i], sizeof(w));
+		for(uint64_t x : w)
+			views_hash = (views_hash ^ x) * 1099511628211ull;
+	}
 	int r;
 	if(!ctx->graph_exec || key != ctx->graph_key || ctx->graph_epoch != ctx->alloc_epoch)
 	{
@@ -2280,8 +2292,9 @@ int sw_render_batch_graph(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out
 			return r;
 		ctx->graph_key = key;
 		ctx->graph_epoch = ctx->alloc_epoch;
+		ctx->graph_views_hash = views_hash;
 	}
-	else
+	else if(views_hash != ctx->graph_views_hash)
 	{
 		// the captured kernels read the views from the device array: refresh it (camera
 		// positions, framebuffer offsets and clear flags may change between replays)
@@ -2291,6 +2304,7 @@ int sw_render_batch_graph(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out
 			(r = validate_views_fit(ctx, ve, b->framebuffer_texels, b->num_surfaces, b->num_draws)) ||
 			(r = set_views(ctx, ve, false)))
 			return r;
+		ctx->graph_views_hash = views_hash;
 	}
 	if((r = hip_check(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream), "graph launch")))
 		return r;
@@ -2327,13 +2341,16 @@ int sw_set_scalar_spans(sw_ctx* ctx, uint32_t enable)
 
 uint32_t sw_get_scalar_spans(const sw_ctx* ctx) { return ctx && ctx->scalar_spans ? 1u : 0u; }
 
-int sw_autotune_batch(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_fb, uint32_t* out_rows,
+int sw_autotune_batches(sw_ctx* ctx, sw_map* map, const sw_batch* batches, uint32_t num_batches, void* out_fb, uint32_t* out_rows,
 	double out_ms[SW_TILE_ROWS_COUNT])
 {
 	if(!ctx)
 		return SW_ERR_INVALID_ARGUMENT;
+	if(!batches || num_batches == 0)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "no batch to tune on");
 	(void)hipSetDevice(ctx->device);
 	static const uint32_t candidates[SW_TILE_ROWS_COUNT] = {4, 8, 16};
+	const int ROUNDS = 5;
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 	int r = hip_check(ctx, hipEventCreate(&e0), "hipEventCreate");
 	if(!r)
@@ -2343,22 +2360,23 @@ int sw_autotune_batch(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_fb,
 	bool best_scalar = ctx->scalar_spans;
 	const bool profiling = ctx->profiling;
 	ctx->profiling = false;
-	// every tile height with both ways of loading span records; out_ms keeps each height's better
+	// every tile height with both ways of loading span records, each timed over all the batches
+	// (median of five rounds); out_ms keeps each height's better mode, per batch
 	for(uint32_t c = 0; c < 2 * SW_TILE_ROWS_COUNT && !r; ++c)
 	{
 		const uint32_t rows = candidates[c / 2];
 		if((r = sw_set_tile_rows(ctx, rows)))
 			break;
 		ctx->scalar_spans = (c & 1) != 0;
-		// one untimed run sizes the buffers, then the median of three timed runs
-		if((r = sw_render_batch(ctx, map, b, out_fb)))
-			break;
-		float t[3] = {0.0f, 0.0f, 0.0f};
-		for(int k = 0; k < 3 && !r; ++k)
+		// one untimed pass sizes the buffers
+		for(uint32_t i = 0; i < num_batches && !r; ++i)
+			r = sw_render_batch(ctx, map, &batches[i], out_fb);
+		float t[ROUNDS] = {};
+		for(int k = 0; k < ROUNDS && !r; ++k)
 		{
 			r = hip_check(ctx, hipEventRecord(e0, ctx->stream), "hipEventRecord");
-			if(!r)
-				r = sw_render_batch(ctx, map, b, out_fb);
+			for(uint32_t i = 0; i < num_batches && !r; ++i)
+				r = sw_render_batch(ctx, map, &batches[i], out_fb);
 			if(!r)
 				r = hip_check(ctx, hipEventRecord(e1, ctx->stream), "hipEventRecord");
 			if(!r)
@@ -2368,12 +2386,13 @@ int sw_autotune_batch(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_fb,
 		}
 		if(r)
 			break;
-		std::sort(t, t + 3);
-		if(out_ms && ((c & 1) == 0 || t[1] < out_ms[c / 2]))
-			out_ms[c / 2] = t[1];
-		if(c == 0 || t[1] < best_ms)
+		std::sort(t, t + ROUNDS);
+		const double ms = double(t[ROUNDS / 2]) / num_batches;
+		if(out_ms && ((c & 1) == 0 || ms < out_ms[c / 2]))
+			out_ms[c / 2] = ms;
+		if(c == 0 || ms < best_ms)
 		{
-			best_ms = t[1];
+			best_ms = ms;
 			best = rows;
 			best_scalar = ctx->scalar_spans;
 		}
@@ -2392,6 +2411,16 @@ int sw_autotune_batch(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_fb,
 	if(out_rows)
 		*out_rows = best;
 	return r ? r : r2;
+}
+
+int sw_autotune_batch(sw_ctx* ctx, sw_map* map, const sw_batch* b, void* out_fb, uint32_t* out_rows,
+	double out_ms[SW_TILE_ROWS_COUNT])
+{
+	if(!ctx)
+		return SW_ERR_INVALID_ARGUMENT;
+	if(!b)
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null batch");
+	return sw_autotune_batches(ctx, map, b, 1, out_fb, out_rows, out_ms);
 }
 
 } // extern "C"

@@ -379,6 +379,8 @@ GPU_SYMBOLS = {
     "sw_get_tile_rows": (C.c_uint32, [C.c_void_p]),
     "sw_autotune_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Batch), C.c_void_p, C.POINTER(C.c_uint32),
                                     C.POINTER(C.c_double)]),
+    "sw_autotune_batches": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_uint32),
+                                      C.POINTER(C.c_double)]),
 }
 
 TILE_ROWS = (4, 8, 16)
@@ -727,6 +729,18 @@ class Context:
 
     def scalar_spans(self):
         return bool(self.lib.sw_get_scalar_spans(self.h))
+
+    def autotune_many(self, map_h, batches, device_inputs, out):
+        """sw_autotune_batches over several batches (device-resident inputs); returns (rows, {rows: ms
+        per batch})."""
+        arr = (Batch * len(batches))()
+        for i, (b, d) in enumerate(zip(batches, device_inputs)):
+            arr[i] = b.ctypes_batch(inputs_on_device=True, device_ptrs=d.ptrs())
+        rows = C.c_uint32()
+        ms = (C.c_double * len(TILE_ROWS))()
+        self.check(self.lib.sw_autotune_batches(self.h, map_h, arr, len(batches), out.ptr, C.byref(rows), ms),
+                   "sw_autotune_batches")
+        return int(rows.value), {r: float(ms[i]) for i, r in enumerate(TILE_ROWS)}
 
     def autotune(self, map_h, b: BatchData, device_inputs, out):
         """Times every raster tile height on batch `b` (device-resident inputs) and keeps the fastest.

@@ -179,6 +179,36 @@ def test_autotune_keeps_output(gpu_ctx):
     assert np.array_equal(g, ofb)
 
 
+def test_autotune_over_several_batches_keeps_output(gpu_ctx):
+    """sw_autotune_batches times every tile height over a sample of batches (frames of one map at
+    different sizes, into one framebuffer), keeps the fastest, and every batch stays exact."""
+    import sw_abi
+    _say("case autotune over batches")
+    # the same seed gives the same map; the view sizes differ
+    pairs = [scenes.make_batch(43, color_bits=32, view_sizes=(sz,)) for sz in ((320, 200), (160, 120), (256, 256))]
+    m = pairs[0][0]
+    bs = [b for _, b in pairs]
+    prev, prev_sspan = gpu_ctx.tile_rows(), gpu_ctx.scalar_spans()
+    mh = gpu_ctx.upload_map(m)
+    devs = [sw_abi.DeviceCommandBuffers(gpu_ctx, b) for b in bs]
+    out = gpu_ctx.alloc(max(b.fb_texels for b in bs) * 4)
+    try:
+        rows, ms = gpu_ctx.autotune_many(mh, bs, devs, out)
+        assert rows in sw_abi.TILE_ROWS and gpu_ctx.tile_rows() == rows
+        assert all(v > 0 for v in ms.values()) and ms[rows] == min(ms.values())
+        fbs = [gpu_ctx.render_batch(mh, b, device_inputs=d, out=out) for b, d in zip(bs, devs)]
+    finally:
+        gpu_ctx.set_tile_rows(prev)
+        gpu_ctx.set_scalar_spans(prev_sspan)
+        out.free()
+        for d in devs:
+            d.free()
+        gpu_ctx.release_map(mh)
+    for b, fb in zip(bs, fbs):
+        ofb, _, _, _ = oracle.render(m, b)
+        assert np.array_equal(fb, ofb)
+
+
 @pytest.mark.parametrize("color_bits", [32, 64])
 def test_surfaces_read_before_raster(gpu_ctx, color_bits):
     """sw_read_surfaces between sw_build_surfaces and sw_raster_draws builds the surfaces on the
