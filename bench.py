@@ -693,7 +693,7 @@ def main():
         return
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): this build runs on a single GPU; "
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the renderer runs on a single GPU; "
               "views and requests would shard across GPUs with no collective", flush=True)
         sys.exit(2)
 
