@@ -1,7 +1,7 @@
 // ORACLE -- CPU restatement of Square-Wheel's per-frame pixel path. TEST INFRASTRUCTURE ONLY.
 //
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this library, and
-// only as the checker (or as the timed CPU baseline). The product path never calls it.
+// only to check results against (or as the timed CPU baseline). The product path never calls it.
 //
 // It restates, loop for loop, the reference's Rust code:
 //   - surface builder      square_wheel/surfaces.rs:433-852 (build_surface_impl_6_static_params),

@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/liboracle.so). TEST INFRASTRUCTURE ONLY.
 
-Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, as the checker or as the
-timed CPU baseline. The product path never imports this module.
+Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, to check results against
+or as the timed CPU baseline. The product path never imports this module.
 """
 import ctypes as C
 import os

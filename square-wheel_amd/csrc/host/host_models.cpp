@@ -1357,7 +1357,7 @@ void draw_leaf_objects(swh_frame& f, const swh_map& m, const swh_frame_config& c
 	for(int i = 0; i < 6; ++i)
 		cam[i] = project_plane(world[i], planes);
 	const std::vector<uint32_t>* leaf_decals = md.leaf_decals.empty() ? nullptr : &md.leaf_decals[leaf];
-	// the reference's object ids (:2663-2667); sprites are not part of this build
+	// the reference's object ids (:2663-2667); the front-end places no sprites
 	const uint32_t SUBMODEL_ADD = 65536, MESH_ADD = SUBMODEL_ADD + 65536, PORTAL_ADD = MESH_ADD + 2 * 65536;
 	auto draw_object = [&](uint32_t id) {
 		if(id >= PORTAL_ADD)
