@@ -688,6 +688,8 @@ int sw_copy_from_device(sw_ctx* ctx, void* host_dst, const void* device_src, uin
 /* Queues a copy on the context's stream and returns (complete at the next wait / sync; truly
  * asynchronous for pinned host memory from sw_host_alloc). */
 int sw_copy_from_device_async(sw_ctx* ctx, void* host_dst, const void* device_src, uint64_t bytes);
+/* The other direction: the host source must stay valid and unchanged until the next wait / sync. */
+int sw_copy_to_device_async(sw_ctx* ctx, void* device_dst, const void* host_src, uint64_t bytes);
 /* Pinned host memory (for asynchronous copies). */
 int sw_host_alloc(sw_ctx* ctx, uint64_t bytes, void** out_host_ptr);
 void sw_host_free(sw_ctx* ctx, void* host_ptr);

@@ -200,6 +200,12 @@ typedef struct swh_model_data
 	uint64_t triangle_capacity;
 } swh_model_data;
 int swh_frame_model_data(const swh_frame* frame, swh_model_data* out);
+/* The draw list of a device_models frame completed with its runs' triangles: draws[0 .. num_draws)
+ * with, before draw run_positions[r], run r's fan triangles (sw_expand_mesh_runs' slots with a
+ * non-zero fan size, triangle t's fan in order, t in order; their vertex data at vertex int
+ * int_base + the run's slot). Returns the number of draws written to out (at most capacity), or -1. */
+int64_t swh_assemble_model_draws(const sw_draw* draws, uint32_t num_draws, const sw_mesh_run* runs, const uint32_t* run_positions,
+	uint32_t num_runs, const uint8_t* fan_counts, uint64_t int_base, sw_draw* out, uint64_t capacity);
 
 /* The map's animated model mesh as sw_model_mesh's VERTEX_ANIMATED arrays (host memory owned by
  * the map): tc[2] per vertex, frame_vertices = {position[3], normal[3]} per vertex per frame, and

@@ -1532,6 +1532,57 @@ int swh_frame_model_data(const swh_frame* f, swh_model_data* out)
 	return 0;
 }
 
+int64_t swh_assemble_model_draws(const sw_draw* draws, uint32_t num_draws, const sw_mesh_run* runs, const uint32_t* run_positions,
+	uint32_t num_runs, const uint8_t* fans, uint64_t int_base, sw_draw* out, uint64_t capacity)
+{
+	if((num_draws && !draws) || (num_runs && (!runs || !run_positions || !fans)) || !out)
+		return -1;
+	uint64_t n = 0, ib = 0, fb = 0;
+	uint32_t next = 0;
+	auto put = [&](const sw_draw& d) {
+		if(n >= capacity)
+			return false;
+		out[n++] = d;
+		return true;
+	};
+	for(uint32_t r = 0; r < num_runs; ++r)
+	{
+		const sw_mesh_run& run = runs[r];
+		if(run_positions[r] < next || run_positions[r] > num_draws)
+			return -1;
+		for(; next < run_positions[r]; ++next)
+			if(!put(draws[next]))
+				return -1;
+		const uint64_t fan_max = 1u + run.num_planes3d + run.num_planes2d;
+		sw_draw d;
+		std::memset(&d, 0, sizeof(d));
+		d.kind = SW_DRAW_TRIANGLE;
+		d.blend = run.blend;
+		d.num_vertices = 3;
+		d.texture_offset = run.texture_offset;
+		d.texture_size[0] = run.texture_size[0];
+		d.texture_size[1] = run.texture_size[1];
+		for(int i = 0; i < 4; ++i)
+			d.clip_rect[i] = run.clip_rect[i];
+		for(uint32_t t = 0; t < run.max_triangles; ++t)
+			for(uint32_t j = 0; j < fans[fb + t]; ++j)
+			{
+				const uint64_t first = int_base + ib + (uint64_t(t) * fan_max + j) * 21u;
+				if(first > 0xFFFFFFFFull)
+					return -1;
+				d.first_vertex_int = uint32_t(first);
+				if(!put(d))
+					return -1;
+			}
+		ib += uint64_t(run.max_triangles) * fan_max * 21u;
+		fb += run.max_triangles;
+	}
+	for(; next < num_draws; ++next)
+		if(!put(draws[next]))
+			return -1;
+	return int64_t(n);
+}
+
 int swh_frame_portal_view(const swh_frame* f, uint32_t index, const swh_frame** out)
 {
 	if(!f || !out || index >= f->num_portal_views)

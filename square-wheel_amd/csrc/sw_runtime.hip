@@ -1972,6 +1972,16 @@ int sw_copy_from_device(sw_ctx* ctx, void* dst, const void* src, uint64_t bytes)
 	return r ? r : hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
 }
 
+int sw_copy_to_device_async(sw_ctx* ctx, void* dst, const void* src, uint64_t bytes)
+{
+	if(!ctx || ((!dst || !src) && bytes))
+		return set_error(ctx, SW_ERR_INVALID_ARGUMENT, "null argument");
+	(void)hipSetDevice(ctx->device);
+	if(!bytes)
+		return SW_OK;
+	return hip_check(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, ctx->stream), "copy to device");
+}
+
 int sw_copy_from_device_async(sw_ctx* ctx, void* dst, const void* src, uint64_t bytes)
 {
 	if(!ctx || ((!dst || !src) && bytes))

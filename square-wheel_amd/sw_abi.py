@@ -371,6 +371,7 @@ GPU_SYMBOLS = {
     "sw_ctx_device_bytes": (C.c_uint64, [C.c_void_p]),
     "sw_render_batch_async": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Batch), C.c_void_p]),
     "sw_copy_from_device_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    "sw_copy_to_device_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     "sw_host_alloc": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
     "sw_host_free": (None, [C.c_void_p, C.c_void_p]),
     "sw_batch_poll": (C.c_int, [C.c_void_p]),

@@ -114,6 +114,9 @@ def load():
         lib.swh_frame_model_data.restype = C.c_int
         lib.swh_map_model_mesh.argtypes = [C.c_void_p, C.POINTER(MeshDataC)]
         lib.swh_map_model_mesh.restype = C.c_int
+        lib.swh_assemble_model_draws.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                                                 C.c_uint64, C.c_void_p, C.c_uint64]
+        lib.swh_assemble_model_draws.restype = C.c_int64
         lib.swh_shadow_create.argtypes = [C.POINTER(C.c_void_p)]
         lib.swh_shadow_create.restype = C.c_int
         lib.swh_shadow_destroy.argtypes = [C.c_void_p]
@@ -470,35 +473,23 @@ def portal_levels(frame):
 
 
 def assemble_model_draws(frame, fans, int_base):
-    """The draw list of a frame built with device_models: the frame's draws with, at each mesh run's
-    position, the triangles of its fans (sw_expand_mesh_runs' fan sizes `fans`), triangle t's fan
-    in order, t in order; their vertex data sits at vertex int int_base + the run's layout."""
+    """The draw list of a frame built with device_models (swh_assemble_model_draws): the frame's draws
+    with, at each mesh run's position, the triangles of its fans (sw_expand_mesh_runs' fan sizes
+    `fans`), triangle t's fan in order, t in order; their vertex data sits at vertex int int_base +
+    the run's layout."""
     md = frame["models"]
-    runs, pos = md["runs"], md["run_positions"]
-    layout, _, _ = A.mesh_run_layout(runs)
-    draws = frame["draws"]
-    pieces = []
-    start = 0
-    for r, run in enumerate(runs):
-        p = int(pos[r])
-        pieces.append(draws[start:p])
-        start = p
-        ib, fan_max, fb = layout[r]
-        nt = int(run["max_triangles"])
-        f = np.asarray(fans[fb:fb + nt], dtype=np.int64)
-        t_idx = np.repeat(np.arange(nt), f)
-        j_idx = np.arange(int(f.sum())) - np.repeat(np.cumsum(f) - f, f)
-        d = np.zeros(len(t_idx), dtype=A.DRAW_DTYPE)
-        d["kind"] = A.DRAW_TRIANGLE
-        d["blend"] = run["blend"]
-        d["first_vertex_int"] = int_base + ib + (t_idx * fan_max + j_idx) * 21
-        d["num_vertices"] = 3
-        d["texture_offset"] = run["texture_offset"]
-        d["texture_size"] = run["texture_size"]
-        d["clip_rect"] = run["clip_rect"]
-        pieces.append(d)
-    pieces.append(draws[start:])
-    return np.concatenate(pieces) if pieces else draws
+    runs = np.ascontiguousarray(md["runs"], dtype=A.MESH_RUN_DTYPE)
+    pos = np.ascontiguousarray(md["run_positions"], dtype=np.uint32)
+    draws = np.ascontiguousarray(frame["draws"], dtype=A.DRAW_DTYPE)
+    fans = np.ascontiguousarray(fans, dtype=np.uint8)
+    cap = len(draws) + int(fans.astype(np.int64).sum())
+    out = np.zeros(max(cap, 1), dtype=A.DRAW_DTYPE)
+    n = load().swh_assemble_model_draws(A._ptr(draws), len(draws), A._ptr(runs) if runs.size else None,
+                                        A._ptr(pos) if pos.size else None, len(runs), A._ptr(fans) if fans.size else None,
+                                        int(int_base), A._ptr(out), len(out))
+    if n < 0:
+        raise ValueError("swh_assemble_model_draws: run positions or fan sizes out of range")
+    return out[:n]
 
 
 class DeviceModels:
@@ -544,24 +535,49 @@ class DeviceModels:
         if len(inst):
             ctx.check(ctx.lib.sw_prepare_models(ctx.h, self.mesh_desc, 1, A._ptr(inst), len(inst), vbuf.ptr, vcap, tbuf.ptr,
                                                 tcap, cbuf.ptr), "sw_prepare_models")
+        vd = np.ascontiguousarray(frame["vertex_data"], dtype=np.int32)
         if host_ints:
-            vd = np.ascontiguousarray(frame["vertex_data"], dtype=np.int32)
-            ctx.check(ctx.lib.sw_copy_to_device(ctx.h, ints.ptr, A._ptr(vd), host_ints * 4), "sw_copy_to_device")
+            # queued: sw_expand_mesh_runs below waits for it
+            ctx.check(ctx.lib.sw_copy_to_device_async(ctx.h, ints.ptr, A._ptr(vd), host_ints * 4), "sw_copy_to_device_async")
         fans = ctx.expand_mesh_runs(md["runs"], vbuf, vcap, tbuf, tcap, cbuf, len(inst),
                                     (ints.ptr.value or 0) + host_ints * 4, run_ints)
         draws = assemble_model_draws(frame, fans, host_ints)
-        b = batch_from_frames([dict(frame, draws=draws, vertex_data=np.zeros(0, np.int32))])
-        ptrs = {k: self._upload(k, getattr(b, k)).ptr for k in ("surfaces", "lights", "light_indices", "draws")}
-        ptrs["vertex_data"] = ints.ptr
-        cb = b.ctypes_batch(inputs_on_device=True, device_ptrs=ptrs)
-        cb.num_vertex_ints = host_ints + run_ints
+        # the frame's command buffers, queued behind the expansion (kept alive until the render
+        # call below has waited for them)
+        keep = {"surfaces": np.ascontiguousarray(frame["surfaces"]), "lights": np.ascontiguousarray(frame["lights"]),
+                "light_indices": np.ascontiguousarray(frame["light_indices"], dtype=np.uint32), "draws": draws}
+        ptrs = {}
+        for k, a in keep.items():
+            buf = self._buf(k, a.nbytes)
+            if a.nbytes:
+                ctx.check(ctx.lib.sw_copy_to_device_async(ctx.h, buf.ptr, A._ptr(a), a.nbytes), "sw_copy_to_device_async")
+            ptrs[k] = buf.ptr
+        view = np.array(frame["view"], dtype=A.VIEW_DTYPE).reshape(1)
+        view["first_surface"] = 0
+        view["num_surfaces"] = len(keep["surfaces"])
+        view["first_draw"] = 0
+        view["num_draws"] = len(draws)
+        view["fb_offset"] = 0
+        view["surface_arena_offset"] = 0
+        bits = int(frame["color_bits"])
+        fb_texels = int(view["width"][0]) * int(view["height"][0])
+        cb = A.Batch()
+        cb.views, cb.num_views, cb.color_bits = A._ptr(view), 1, bits
+        cb.surfaces, cb.num_surfaces = ptrs["surfaces"], len(keep["surfaces"])
+        cb.lights, cb.num_lights = ptrs["lights"], len(keep["lights"])
+        cb.light_indices, cb.num_light_indices = ptrs["light_indices"], len(keep["light_indices"])
+        cb.draws, cb.num_draws = ptrs["draws"], len(draws)
+        cb.vertex_data, cb.num_vertex_ints = ints.ptr, host_ints + run_ints
+        cb.surface_arena_texels = int(frame["arena_texels"])
+        cb.framebuffer_texels = fb_texels
+        cb.inputs_on_device = 1
         if out is not None:
             ctx.check(ctx.lib.sw_render_batch(ctx.h, map_h, C.byref(cb), out.ptr), "sw_render_batch")
             return None, fans
-        nbytes = max(b.fb_texels, 1) * (8 if b.color_bits == 64 else 4)
-        fb = self._buf("framebuffer", nbytes)
+        dtype = np.uint64 if bits == 64 else np.uint32
+        fb = self._buf("framebuffer", max(fb_texels, 1) * np.dtype(dtype).itemsize)
         ctx.check(ctx.lib.sw_render_batch(ctx.h, map_h, C.byref(cb), fb.ptr), "sw_render_batch")
-        return fb.read(b.color_dtype, b.fb_texels), fans
+        return fb.read(dtype, fb_texels), fans
 
     def close(self):
         for b in list(self.mesh_bufs) + list(self.bufs.values()):

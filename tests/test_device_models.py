@@ -12,7 +12,9 @@ Pins on the CPU:
     at the same places;
   - the oracle's device-model frame (its model preparation, its expansion, the triangles put in
     place) draws its models: a frame with models differs from the same frame without them exactly
-    where the runs' triangles cover pixels.
+    where the runs' triangles cover pixels;
+  - the front-end's native draw-list assembly equals an independent numpy restatement, and rejects
+    run positions out of order.
 """
 import numpy as np
 import pytest
@@ -117,3 +119,51 @@ def test_oracle_device_frame_draws_its_models(scene):
         assert oob == 0 and st["triangle_pixels"] > 0
         without, _, _, _ = oracle.render(m.data, sw_host.batch_from_frames([d]))
         assert np.count_nonzero(with_models != without) > 0
+
+
+def _assemble_reference(frame, fans, int_base):
+    """numpy restatement of the assembly: the draws split at the run positions, each run's fan
+    triangles (slot t * fan_max + j for j below fan[t]) inserted there."""
+    md = frame["models"]
+    runs, pos = md["runs"], md["run_positions"]
+    layout, _, _ = A.mesh_run_layout(runs)
+    draws = frame["draws"]
+    pieces, start = [], 0
+    for r, run in enumerate(runs):
+        p = int(pos[r])
+        pieces.append(draws[start:p])
+        start = p
+        ib, fan_max, fb = layout[r]
+        nt = int(run["max_triangles"])
+        f = np.asarray(fans[fb:fb + nt], dtype=np.int64)
+        t_idx = np.repeat(np.arange(nt), f)
+        j_idx = np.arange(int(f.sum())) - np.repeat(np.cumsum(f) - f, f)
+        d = np.zeros(len(t_idx), dtype=A.DRAW_DTYPE)
+        d["kind"] = A.DRAW_TRIANGLE
+        d["blend"] = run["blend"]
+        d["first_vertex_int"] = int_base + ib + (t_idx * fan_max + j_idx) * 21
+        d["num_vertices"] = 3
+        d["texture_offset"] = run["texture_offset"]
+        d["texture_size"] = run["texture_size"]
+        d["clip_rect"] = run["clip_rect"]
+        pieces.append(d)
+    pieces.append(draws[start:])
+    return np.concatenate(pieces)
+
+
+def test_native_assembly_equals_a_numpy_restatement(scene):
+    m, poses = scene
+    mesh = m.model_mesh()
+    rng = np.random.default_rng(2)
+    for i in (7, 21, 31):
+        d = _build(m, poses[i], i, 1)
+        md = d["models"]
+        _, _, slots = A.mesh_run_layout(md["runs"])
+        fans = rng.integers(0, 3, size=slots).astype(np.uint8)
+        got = sw_host.assemble_model_draws(d, fans, 1000)
+        want = _assemble_reference(d, fans, 1000)
+        assert np.array_equal(got, want)
+    bad = dict(d, models=dict(d["models"], run_positions=d["models"]["run_positions"][::-1].copy()))
+    if len(bad["models"]["run_positions"]) > 1 and bad["models"]["run_positions"][0] > bad["models"]["run_positions"][-1]:
+        with pytest.raises(ValueError):
+            sw_host.assemble_model_draws(bad, fans, 1000)
