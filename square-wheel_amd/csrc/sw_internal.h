@@ -229,6 +229,15 @@ struct BloomArgs
 
 namespace swk
 {
+// Events a timed stage offers to its kernel launch (profiling only): the launch records them at the
+// kernel's start and end and sets `used`; per host thread.
+struct LaunchEvents
+{
+	hipEvent_t start = nullptr, stop = nullptr;
+	bool used = false;
+};
+LaunchEvents& launch_events();
+
 // Kernel 1: draw setup (edge walk -> parts -> span records) with `lanes` (64 or 256) threads per
 // draw and, in extra blocks, the surfaces' arena extents, validation and block -> surface table
 // (device-resident descriptors).

@@ -21,6 +21,8 @@
 #include <algorithm>
 
 #include "sw_device.h"
+#include <hip/hip_ext.h>
+
 #include "sw_internal.h"
 
 using namespace swd;
@@ -2910,6 +2912,30 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t value)
 // =============================================================================================
 namespace swk
 {
+LaunchEvents& launch_events()
+{
+	static thread_local LaunchEvents e;
+	return e;
+}
+} // namespace swk
+
+// A kernel launch that, when a timed stage offers events (profiling), records them at the kernel's
+// own start and end (hipExtLaunchKernelGGL), so the stage's time is the kernel's duration.
+template <typename F, typename... Args>
+static void timed_launch(F kernel, dim3 grid, dim3 block, uint32_t shm, hipStream_t st, Args... args)
+{
+	swk::LaunchEvents& e = swk::launch_events();
+	if(e.start && !e.used)
+	{
+		e.used = true;
+		hipExtLaunchKernelGGL(kernel, grid, block, shm, st, e.start, e.stop, 0u, args...);
+	}
+	else
+		hipLaunchKernelGGL(kernel, grid, block, shm, st, args...);
+}
+
+namespace swk
+{
 
 void launch_setup(hipStream_t st, uint32_t lanes, SetupArgs a, const ViewsArg& views, const sw_surface_desc* surfaces,
 	uint32_t ns, SurfExt* surf_ext, uint32_t* blk_surf, const MapPtrs& m)
@@ -2921,9 +2947,9 @@ void launch_setup(hipStream_t st, uint32_t lanes, SetupArgs a, const ViewsArg& v
 	if(blocks == 0)
 		return;
 	if(lanes == 256)
-		hipLaunchKernelGGL(k_setup_draws<256>, dim3(blocks), dim3(256), 0, st, a, views, surfaces, ns, surf_ext, blk_surf, m);
+		timed_launch(k_setup_draws<256>, dim3(blocks), dim3(256), 0, st, a, views, surfaces, ns, surf_ext, blk_surf, m);
 	else
-		hipLaunchKernelGGL(k_setup_draws<64>, dim3(blocks), dim3(256), 0, st, a, views, surfaces, ns, surf_ext, blk_surf, m);
+		timed_launch(k_setup_draws<64>, dim3(blocks), dim3(256), 0, st, a, views, surfaces, ns, surf_ext, blk_surf, m);
 }
 
 template <int LANES>
@@ -2931,9 +2957,9 @@ static void launch_fused_lanes(hipStream_t st, uint32_t color_bits, uint32_t blo
 	const SurfaceArgs& sa, const MapPtrs& m)
 {
 	if(color_bits == 64)
-		hipLaunchKernelGGL((k_setup_and_surfaces<LANES, Color64T>), dim3(blocks), dim3(256), 0, st, a, views, sa, m);
+		timed_launch(k_setup_and_surfaces<LANES, Color64T>, dim3(blocks), dim3(256), 0, st, a, views, sa, m);
 	else
-		hipLaunchKernelGGL((k_setup_and_surfaces<LANES, Color32T>), dim3(blocks), dim3(256), 0, st, a, views, sa, m);
+		timed_launch(k_setup_and_surfaces<LANES, Color32T>, dim3(blocks), dim3(256), 0, st, a, views, sa, m);
 }
 
 void launch_setup_and_surfaces(hipStream_t st, uint32_t lanes, uint32_t color_bits, SetupArgs a, const ViewsArg& views,
@@ -2963,16 +2989,16 @@ void launch_build_surfaces(hipStream_t st, uint32_t color_bits, const MapPtrs& m
 	if(color_bits == 64)
 	{
 		if(sh)
-			hipLaunchKernelGGL((k_build_surfaces<Color64T, true>), dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, sa, views, counters);
+			timed_launch(k_build_surfaces<Color64T, true>, dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, sa, views, counters);
 		else
-			hipLaunchKernelGGL((k_build_surfaces<Color64T, false>), dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, sa, views, counters);
+			timed_launch(k_build_surfaces<Color64T, false>, dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, sa, views, counters);
 	}
 	else
 	{
 		if(sh)
-			hipLaunchKernelGGL((k_build_surfaces<Color32T, true>), dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, sa, views, counters);
+			timed_launch(k_build_surfaces<Color32T, true>, dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, sa, views, counters);
 		else
-			hipLaunchKernelGGL((k_build_surfaces<Color32T, false>), dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, sa, views, counters);
+			timed_launch(k_build_surfaces<Color32T, false>, dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, sa, views, counters);
 	}
 }
 
@@ -2989,13 +3015,13 @@ static void launch_raster_ct(hipStream_t st, uint32_t tile_rows, const MapPtrs& 
 	const C* sky = static_cast<const C*>(skybox);
 	C* out = static_cast<C*>(fb);
 	if(tile_rows == 4)
-		hipLaunchKernelGGL((k_raster_tiles<CT, 4, SSPAN>), dim3(blocks), dim3(256), 0, st, m, bounds, drec, parts, spans, views,
+		timed_launch(k_raster_tiles<CT, 4, SSPAN>, dim3(blocks), dim3(256), 0, st, m, bounds, drec, parts, spans, views,
 			a, sky, out, total_tiles, counters);
 	else if(tile_rows == 8)
-		hipLaunchKernelGGL((k_raster_tiles<CT, 8, SSPAN>), dim3(blocks), dim3(256), 0, st, m, bounds, drec, parts, spans, views,
+		timed_launch(k_raster_tiles<CT, 8, SSPAN>, dim3(blocks), dim3(256), 0, st, m, bounds, drec, parts, spans, views,
 			a, sky, out, total_tiles, counters);
 	else
-		hipLaunchKernelGGL((k_raster_tiles<CT, 16, SSPAN>), dim3(blocks), dim3(256), 0, st, m, bounds, drec, parts, spans, views,
+		timed_launch(k_raster_tiles<CT, 16, SSPAN>, dim3(blocks), dim3(256), 0, st, m, bounds, drec, parts, spans, views,
 			a, sky, out, total_tiles, counters);
 }
 

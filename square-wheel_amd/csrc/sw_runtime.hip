@@ -273,31 +273,43 @@ hipEvent_t take_event(sw_ctx* ctx)
 }
 
 // Brackets one launch with events on the context's stream when profiling is on.
+// Times a stage with two events. The stage's first kernel launched through timed_launch records
+// them at the kernel's own start and end; a stage without such a launch is timed by the events
+// recorded on the stream around it (which adds a few microseconds of marker overhead).
 struct StageTimer
 {
 	sw_ctx* ctx;
 	int stage;
-	hipEvent_t a = nullptr;
+	hipEvent_t a = nullptr, b = nullptr;
+	swk::LaunchEvents saved;
 	StageTimer(sw_ctx* c, int s) : ctx(c), stage(s)
 	{
 		if(ctx->profiling && !ctx->capturing)
 		{
 			a = take_event(ctx);
+			b = a ? take_event(ctx) : nullptr;
+			if(a && !b)
+			{
+				ctx->event_pool.push_back(a);
+				a = nullptr;
+			}
 			if(a)
+			{
+				// the start marker, overwritten by a timed launch's own start record
 				(void)hipEventRecord(a, ctx->stream);
+				saved = swk::launch_events();
+				swk::launch_events() = swk::LaunchEvents{a, b, false};
+			}
 		}
 	}
 	~StageTimer()
 	{
 		if(!a)
 			return;
-		hipEvent_t b = take_event(ctx);
-		if(!b)
-		{
-			ctx->event_pool.push_back(a);
-			return;
-		}
-		(void)hipEventRecord(b, ctx->stream);
+		const bool attached = swk::launch_events().used;
+		swk::launch_events() = saved;
+		if(!attached)
+			(void)hipEventRecord(b, ctx->stream); // no timed launch in the stage: the end marker
 		ctx->pending.push_back(sw_ctx::Pending{stage, a, b});
 	}
 };
