@@ -198,6 +198,53 @@ SW_DEV LmE lm_mix(const LmE& a, const LmE& b, float ratio, bool directional)
 	return r;
 }
 
+// A lightmap line element in LDS: 12 floats for a directional element (ambient, direction,
+// deviation, colour, two unused), 4 for a simple one (ambient, one unused), so every element
+// starts on 16 bytes.
+SW_DEV uint32_t line_stride(bool directional) { return directional ? 12u : 4u; }
+
+SW_DEV void line_store(float* p, const LmE& e, bool directional)
+{
+	if(directional)
+	{
+		*reinterpret_cast<float4*>(p) = make_float4(e.amb[0], e.amb[1], e.amb[2], e.dir.x);
+		*reinterpret_cast<float4*>(p + 4) = make_float4(e.dir.y, e.dir.z, e.dev, e.col[0]);
+		*reinterpret_cast<float2*>(p + 8) = make_float2(e.col[1], e.col[2]);
+	}
+	else
+		*reinterpret_cast<float4*>(p) = make_float4(e.amb[0], e.amb[1], e.amb[2], 0.0f);
+}
+
+SW_DEV LmE line_load(const float* p, bool directional)
+{
+	LmE e;
+	if(directional)
+	{
+		const float4 a = *reinterpret_cast<const float4*>(p);
+		const float4 b = *reinterpret_cast<const float4*>(p + 4);
+		const float2 c = *reinterpret_cast<const float2*>(p + 8);
+		e.amb[0] = a.x;
+		e.amb[1] = a.y;
+		e.amb[2] = a.z;
+		e.dir = v3(a.w, b.x, b.y);
+		e.dev = b.z;
+		e.col[0] = b.w;
+		e.col[1] = c.x;
+		e.col[2] = c.y;
+	}
+	else
+	{
+		const float4 a = *reinterpret_cast<const float4*>(p);
+		e.amb[0] = a.x;
+		e.amb[1] = a.y;
+		e.amb[2] = a.z;
+		e.dir = v3(0.0f, 0.0f, 0.0f);
+		e.dev = 0.0f;
+		e.col[0] = e.col[1] = e.col[2] = 0.0f;
+	}
+	return e;
+}
+
 // surfaces.rs:1088-1150
 SW_DEV float specular_intensity(float c, float inv_roughness)
 {
@@ -243,6 +290,11 @@ struct SurfRec
 	int32_t elight[3]; // cvtps2dq(emissive_light * 256), lanes B, G, R
 	float inv_scale, k_shift;
 	V3 start_pos, u_vec, v_vec, u_n, v_n, normal, cam;
+	// The surface's lightmap lines in the block's LDS (blocks that stage them): line (row, column)
+	// = the vertical mix of lightmap rows lv and lv + 1 for surface row line_dv0 + row, at lightmap
+	// column line_lu0 + column; line_off (in floats) is NO_LINES when the surface's texels fetch
+	// their four elements from global memory instead.
+	uint32_t line_off, line_e0, line_ne, line_dv0, line_lu0, line_ncols;
 };
 
 enum : uint32_t
@@ -260,6 +312,9 @@ enum : uint32_t
 	REC_LIGHTS_SHIFT = 16, // clamped light count
 };
 #define SURF_CHUNK 32 // surface records per pass of a block
+#define LINE_FLOATS 4096 // LDS floats of a block's staged lightmap lines (16 KiB)
+#define NO_LINES 0xFFFFFFFFu
+#define SURF_LINES_MIN_TEXELS 512 // blocks of at least this many texels stage lightmap lines
 
 using swdiv::div_rcp;
 using swdiv::mod_rcp;
@@ -297,6 +352,8 @@ SW_DEV void make_surf_rec(SurfRec& r, const sw_surface_desc& d, uint64_t start, 
 {
 	r.start = start;
 	r.end = end;
+	r.line_off = NO_LINES;
+	r.line_e0 = r.line_ne = 0;
 	r.w = d.size[0];
 	r.inv_w = 1.0 / double(d.size[0] ? d.size[0] : 1u);
 	const bool diffuse = (d.flags & SW_SURF_DIFFUSE_LAYER) != 0;
@@ -442,10 +499,10 @@ SW_DEV bool shadow_lit(const sw_light_shadow& S, const float* __restrict__ texel
 // One texel of build_surface_impl_6_static_params (surfaces.rs:433-852), specialisations as flags
 // that are uniform per surface. SH: lights may have shadow maps (a separate instantiation, so the
 // kernel without them keeps its registers).
-template <class CT, bool SH>
+template <class CT, bool SH, bool LINES>
 SW_DEV typename CT::T surface_texel(const MapPtrs& m, const SurfRec& R, uint32_t fl, const sw_light* __restrict__ lights,
 	const uint32_t* __restrict__ light_indices, uint32_t num_light_indices, uint32_t num_lights_total, uint32_t dst_u,
-	uint32_t dst_v, const SurfaceArgs& a)
+	uint32_t dst_v, const SurfaceArgs& a, const float* lines)
 {
 	const bool use_normal_map = (fl & REC_NORMAL_MAP) != 0;
 	const uint32_t spec = (fl >> REC_SPEC_SHIFT) & 3u;
@@ -492,21 +549,33 @@ SW_DEV typename CT::T surface_texel(const MapPtrs& m, const SurfRec& R, uint32_t
 		const float inv_scale = R.inv_scale;
 		const float k_shift = R.k_shift;
 		const uint32_t mask = (1u << sl) - 1u;
-		// vertical mix of the two lightmap rows (:477-502) for the two columns this texel needs
-		const uint32_t base_v = dst_v + R.lm_sv;
-		const uint32_t lv = base_v >> sl;
-		const float kv = mul_add(float(base_v & mask), inv_scale, k_shift);
 		const uint32_t base_u = dst_u + R.lm_su;
 		const uint32_t lu = base_u >> sl;
 		const float ku = mul_add(float(base_u & mask), inv_scale, k_shift);
-		const uint32_t row0 = R.lm_offset + lv * R.lm_stride;
-		const uint32_t row1 = row0 + R.lm_stride;
-		const LmE l00 = lm_load(m, directional, row0 + lu);
-		const LmE l01 = lm_load(m, directional, row0 + lu + 1);
-		const LmE l10 = lm_load(m, directional, row1 + lu);
-		const LmE l11 = lm_load(m, directional, row1 + lu + 1);
-		const LmE line0 = lm_mix(l10, l00, kv, directional);
-		const LmE line1 = lm_mix(l11, l01, kv, directional);
+		LmE line0, line1;
+		if(LINES && R.line_off != NO_LINES)
+		{
+			// the row's lightmap line (:473-502), mixed once per block and column
+			const uint32_t stride = line_stride(directional);
+			const float* p = lines + R.line_off + ((dst_v - R.line_dv0) * R.line_ncols + (lu - R.line_lu0)) * stride;
+			line0 = line_load(p, directional);
+			line1 = line_load(p + stride, directional);
+		}
+		else
+		{
+			// vertical mix of the two lightmap rows (:477-502) for the two columns this texel needs
+			const uint32_t base_v = dst_v + R.lm_sv;
+			const uint32_t lv = base_v >> sl;
+			const float kv = mul_add(float(base_v & mask), inv_scale, k_shift);
+			const uint32_t row0 = R.lm_offset + lv * R.lm_stride;
+			const uint32_t row1 = row0 + R.lm_stride;
+			const LmE l00 = lm_load(m, directional, row0 + lu);
+			const LmE l01 = lm_load(m, directional, row0 + lu + 1);
+			const LmE l10 = lm_load(m, directional, row1 + lu);
+			const LmE l11 = lm_load(m, directional, row1 + lu + 1);
+			line0 = lm_mix(l10, l00, kv, directional);
+			line1 = lm_mix(l11, l01, kv, directional);
+		}
 		const LmE lm = lm_mix(line1, line0, ku, directional); // (:544-549)
 
 		if(spec == 0)
@@ -678,12 +747,16 @@ SW_DEV typename CT::T emissive_blend(uint32_t t, int32_t lb, int32_t lg, int32_t
 // SURF_MAX_TEXELS_PER_THREAD per thread (strided by the block size, so each pass is coalesced). The block's first wave builds
 // the records of up to SURF_CHUNK consecutive surfaces overlapping the block (one lane each), then
 // every thread finds its texels' surfaces among them by a forward scan; a block overlapping more
-// surfaces takes further passes.
-template <class CT, bool SH>
+// surfaces takes further passes. LINES: before the texels, the block mixes each lightmapped
+// surface's lightmap lines for its rows (as the reference does per row, surfaces.rs:473-502) into
+// LDS, up to LINE_FLOATS; the texels then read two line elements from LDS instead of four
+// lightmap elements from global memory (same operations in the same order, so the same bits).
+template <class CT, bool SH, bool LINES>
 SW_DEV void build_surfaces_block(uint32_t block, const MapPtrs& m, const SurfaceArgs& a, const ViewsArg& views)
 {
 	__shared__ SurfRec recs[SURF_CHUNK];
-	__shared__ uint32_t sh_base, sh_count;
+	__shared__ uint32_t sh_base, sh_count, sh_lines;
+	__shared__ __align__(16) float lines[LINES ? LINE_FLOATS : 4];
 	const sw_surface_desc* __restrict__ surfaces = a.surfaces;
 	const uint32_t num_surfaces = a.num_surfaces;
 	const SurfExt* __restrict__ surf_ext = a.surf_ext;
@@ -774,6 +847,83 @@ SW_DEV void build_surfaces_block(uint32_t block, const MapPtrs& m, const Surface
 		}
 		__syncthreads();
 		const uint32_t count = sh_count;
+		if(LINES)
+		{
+			if(tid < 64)
+			{
+				// the lines each surface's rows in [b0, b1) need: every column a full row reads
+				uint32_t ne = 0, nf = 0, dv0 = 0, lu0 = 0, ncols = 0;
+				if(tid < count)
+				{
+					const SurfRec& R = recs[tid];
+					const uint64_t lo = std::max<uint64_t>(R.start, b0), hi = std::min<uint64_t>(R.end, b1);
+					if((R.flags & (REC_SKIP | REC_DIFFUSE | REC_LIGHTMAP)) == (REC_DIFFUSE | REC_LIGHTMAP) && lo < hi)
+					{
+						uint32_t rem;
+						dv0 = div_rcp(uint32_t(lo - R.start), R.w, R.inv_w, rem);
+						const uint32_t dv1 = div_rcp(uint32_t(hi - 1 - R.start), R.w, R.inv_w, rem);
+						lu0 = R.lm_su >> R.lm_sl;
+						ncols = ((R.w - 1u + R.lm_su) >> R.lm_sl) - lu0 + 2u;
+						// rows <= bt and columns <= 2^16 + 1: no overflow below
+						ne = (dv1 - dv0 + 1u) * ncols;
+						nf = ne * line_stride((R.flags & REC_DIRECTIONAL) != 0);
+					}
+				}
+				uint32_t sf = nf, se = ne; // inclusive prefix sums over the records
+#pragma unroll
+				for(uint32_t dlt = 1; dlt < 64; dlt <<= 1)
+				{
+					const uint32_t tf = __shfl_up(sf, dlt), te = __shfl_up(se, dlt);
+					if(tid >= dlt)
+					{
+						sf += tf;
+						se += te;
+					}
+				}
+				// the staged records are a prefix (skipping those without lines): sf never decreases
+				const bool fits = ne > 0 && sf <= LINE_FLOATS;
+				if(tid < count)
+				{
+					SurfRec& R = recs[tid];
+					R.line_off = fits ? sf - nf : NO_LINES;
+					R.line_e0 = se - ne;
+					R.line_ne = fits ? ne : 0u;
+					R.line_dv0 = dv0;
+					R.line_lu0 = lu0;
+					R.line_ncols = ncols;
+				}
+				const uint64_t fm = __ballot(fits);
+				const uint32_t total = __shfl(se, fm ? 63 - __builtin_clzll(fm) : 0);
+				if(tid == 0)
+					sh_lines = fm ? total : 0u;
+			}
+			__syncthreads();
+			const uint32_t total = sh_lines;
+			uint32_t lr = 0;
+#pragma unroll 1
+			for(uint32_t j = tid; j < total; j += SURF_BLOCK)
+			{
+				// j < total lies in a staged record; those without lines have line_ne 0
+				while(j >= recs[lr].line_e0 + recs[lr].line_ne)
+					++lr;
+				const SurfRec& R = recs[lr];
+				const uint32_t q = j - R.line_e0;
+				const uint32_t row = q / R.line_ncols;
+				const uint32_t col = q - row * R.line_ncols;
+				const bool directional = (R.flags & REC_DIRECTIONAL) != 0;
+				const uint32_t sl = R.lm_sl;
+				// :477-502 for surface row line_dv0 + row and lightmap column line_lu0 + col
+				const uint32_t base_v = R.line_dv0 + row + R.lm_sv;
+				const uint32_t lv = base_v >> sl;
+				const float kv = mul_add(float(base_v & ((1u << sl) - 1u)), R.inv_scale, R.k_shift);
+				const uint32_t row0 = R.lm_offset + lv * R.lm_stride;
+				const uint32_t lu = R.line_lu0 + col;
+				const LmE l0 = lm_load(m, directional, row0 + lu);
+				const LmE l1 = lm_load(m, directional, row0 + R.lm_stride + lu);
+				line_store(lines + R.line_off + q * line_stride(directional), lm_mix(l1, l0, kv, directional), directional);
+			}
+			__syncthreads();
+		}
 		uint32_t r = 0;
 #pragma unroll 1
 		for(uint32_t k = 0; k < bt / SURF_BLOCK; ++k)
@@ -793,7 +943,8 @@ SW_DEV void build_surfaces_block(uint32_t block, const MapPtrs& m, const Surface
 			const uint32_t dv = div_rcp(uint32_t(idx - R.start), R.w, R.inv_w, du);
 			typename CT::T out = 0;
 			if(fl & REC_DIFFUSE)
-				out = surface_texel<CT, SH>(m, R, fl, a.lights, a.light_indices, a.num_light_indices, a.num_lights, du, dv, a);
+				out = surface_texel<CT, SH, LINES>(
+					m, R, fl, a.lights, a.light_indices, a.num_light_indices, a.num_lights, du, dv, a, lines);
 			else if(((fl >> REC_BLEND_SHIFT) & 15u) != SW_BLEND_NONE)
 				out = arena[idx]; // an emissive layer without a diffuse layer blends over what the arena holds
 			if(fl & REC_EMISSIVE)
@@ -809,13 +960,13 @@ SW_DEV void build_surfaces_block(uint32_t block, const MapPtrs& m, const Surface
 
 // Held to 80 VGPRs (six waves per SIMD): the kernel is bound by arithmetic and load latency, and
 // the extra waves hide more of the latter than the few spilled values cost.
-template <class CT, bool SH>
+template <class CT, bool SH, bool LINES>
 __global__ __launch_bounds__(SURF_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_build_surfaces(MapPtrs m, SurfaceArgs a, const ViewsArg views,
 	const uint32_t* __restrict__ counters)
 {
 	if(counters[COUNTER_ERROR] & ERROR_BAD_SURFACE)
 		return; // the host reports the invalid descriptor; nothing is written
-	build_surfaces_block<CT, SH>(blockIdx.x, m, a, views);
+	build_surfaces_block<CT, SH, LINES>(blockIdx.x, m, a, views);
 }
 
 // =============================================================================================
@@ -1916,7 +2067,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
 	if(blockIdx.x < a.setup_blocks)
 		setup_draws_block<LANES>(blockIdx.x, a, views);
 	else
-		build_surfaces_block<CT, false>(blockIdx.x - a.setup_blocks, m, sa, views);
+		build_surfaces_block<CT, false, false>(blockIdx.x - a.setup_blocks, m, sa, views);
 }
 
 // One block after the raster kernel: sums the per-draw statistics into the result block and clears
@@ -2986,19 +3137,28 @@ void launch_build_surfaces(hipStream_t st, uint32_t color_bits, const MapPtrs& m
 	const uint64_t bt = surf_block_texels(sa.arena_texels);
 	const uint64_t blocks = (sa.arena_texels + bt - 1) / bt;
 	const bool sh = sa.num_shadows != 0;
+	// blocks of several texels per thread stage their lightmap lines in LDS: a line serves the
+	// texels of a lightmap cell's width in its row, which the staging's extra step and barrier
+	// then repay
+	const bool lines = bt >= SURF_LINES_MIN_TEXELS;
+	const dim3 g{uint32_t(blocks)}, b{SURF_BLOCK};
 	if(color_bits == 64)
 	{
 		if(sh)
-			timed_launch(k_build_surfaces<Color64T, true>, dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, sa, views, counters);
+			lines ? timed_launch(k_build_surfaces<Color64T, true, true>, g, b, 0, st, m, sa, views, counters)
+				  : timed_launch(k_build_surfaces<Color64T, true, false>, g, b, 0, st, m, sa, views, counters);
 		else
-			timed_launch(k_build_surfaces<Color64T, false>, dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, sa, views, counters);
+			lines ? timed_launch(k_build_surfaces<Color64T, false, true>, g, b, 0, st, m, sa, views, counters)
+				  : timed_launch(k_build_surfaces<Color64T, false, false>, g, b, 0, st, m, sa, views, counters);
 	}
 	else
 	{
 		if(sh)
-			timed_launch(k_build_surfaces<Color32T, true>, dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, sa, views, counters);
+			lines ? timed_launch(k_build_surfaces<Color32T, true, true>, g, b, 0, st, m, sa, views, counters)
+				  : timed_launch(k_build_surfaces<Color32T, true, false>, g, b, 0, st, m, sa, views, counters);
 		else
-			timed_launch(k_build_surfaces<Color32T, false>, dim3(uint32_t(blocks)), dim3(SURF_BLOCK), 0, st, m, sa, views, counters);
+			lines ? timed_launch(k_build_surfaces<Color32T, false, true>, g, b, 0, st, m, sa, views, counters)
+				  : timed_launch(k_build_surfaces<Color32T, false, false>, g, b, 0, st, m, sa, views, counters);
 	}
 }
 

@@ -357,6 +357,41 @@ def test_many_views_large_arena(gpu_ctx, device_inputs):
         assert np.array_equal(g, ofb)
 
 
+@pytest.mark.parametrize("color_bits", [32, 64])
+@pytest.mark.parametrize("arena_scale", ["three_per_thread", "four_per_thread"])
+def test_lightmap_lines_staged(gpu_ctx, color_bits, arena_scale):
+    """Surface blocks of several texels per thread stage each surface's lightmap lines (the
+    vertical mix of two lightmap rows per surface row and column) in LDS; surfaces whose lines do
+    not fit the block's share fetch their elements directly. Lightmap scales 1..16, simple and
+    directional, surfaces up to 150 texels wide (many blocks hold both kinds): arena and frame
+    equal the oracle's, eager and graph-replayed."""
+    import sw_abi
+    _say(f"case lightmap lines bits={color_bits} {arena_scale}")
+    per_view = 30 if arena_scale == "three_per_thread" else 60
+    sizes = ((96, 64), (80, 60), (128, 72), (64, 64), (100, 50), (72, 96))
+    m, b = scenes.make_batch(81 + color_bits, color_bits=color_bits, view_sizes=sizes, surfaces_per_view=per_view,
+                             draws_per_view=30, max_surface_size=150, num_lm=8000)
+    lo, hi = (524_288, 786_432) if arena_scale == "three_per_thread" else (786_432, 1 << 30)
+    assert lo < b.arena_texels <= hi, b.arena_texels
+    mh = gpu_ctx.upload_map(m)
+    dev = sw_abi.DeviceCommandBuffers(gpu_ctx, b)
+    out = gpu_ctx.alloc(b.fb_texels * color_bits // 8)
+    try:
+        fb = gpu_ctx.render_batch(mh, b, device_inputs=dev, out=out)
+        arena = gpu_ctx.read_surfaces(b.arena_texels, color_bits)
+        graphs = [gpu_ctx.render_batch(mh, b, graph=True, device_inputs=dev, out=out) for _ in range(2)]
+    finally:
+        out.free()
+        dev.free()
+        gpu_ctx.release_map(mh)
+    ofb, oarena, _, _ = oracle.render(m, b, mode=oracle.MODE_EXACT)
+    assert np.array_equal(arena, oarena[:b.arena_texels]), \
+        f"surface arena differs in {np.count_nonzero(arena != oarena[:b.arena_texels])} texels"
+    assert np.array_equal(fb, ofb)
+    for g in graphs:
+        assert np.array_equal(g, ofb)
+
+
 def test_decreasing_arena_offsets_are_an_error(gpu_ctx):
     import sw_abi
     _say("case decreasing arena offsets")

@@ -79,6 +79,30 @@ def test_shadowed_plain_frame_bit_exact(shadow_ctx):
         gpu_ctx.release_map(mh)
 
 
+@pytest.mark.parametrize("color_bits", [32, 64])
+def test_shadowed_large_arena_bit_exact(shadow_ctx, color_bits):
+    """Shadowed surfaces over an arena of several texels per thread (blocks that stage their
+    lightmap lines in LDS): arena and frame equal the oracle's."""
+    gpu_ctx = shadow_ctx
+    rng = np.random.default_rng(40 + color_bits)
+    m, b = scenes.make_batch(23, color_bits=color_bits, view_sizes=((160, 120), (128, 96), (97, 61)), num_lights=9,
+                             surfaces_per_view=45, max_surface_size=150, num_lm=8000)
+    assert b.arena_texels > 262_144, b.arena_texels
+    sh, tex = _random_shadows(rng, len(b.lights))
+    gpu_ctx.set_light_shadows(sh, tex)
+    oracle.set_light_shadows(sh, tex)
+    mh = gpu_ctx.upload_map(m)
+    try:
+        print(f"shadowed large arena bits {color_bits}", flush=True)
+        fb = gpu_ctx.render_batch(mh, b)
+        arena = gpu_ctx.read_surfaces(b.arena_texels, color_bits)
+        ref_fb, ref_arena, _, _ = oracle.render(m, b)
+        assert np.array_equal(arena, ref_arena[:b.arena_texels]), np.count_nonzero(arena != ref_arena[:b.arena_texels])
+        assert np.array_equal(fb, ref_fb)
+    finally:
+        gpu_ctx.release_map(mh)
+
+
 def test_shadow_validation(shadow_ctx):
     sh = np.zeros(2, dtype=A.LIGHT_SHADOW_DTYPE)
     sh[1]["kind"], sh[1]["size"], sh[1]["offset"] = A.SHADOW_CUBE, 8, 10
