@@ -81,13 +81,13 @@ def test_shadowed_plain_frame_bit_exact(shadow_ctx):
 
 @pytest.mark.parametrize("color_bits", [32, 64])
 def test_shadowed_large_arena_bit_exact(shadow_ctx, color_bits):
-    """Shadowed surfaces over an arena of several texels per thread (blocks that stage their
-    lightmap lines in LDS): arena and frame equal the oracle's."""
+    """Shadowed surfaces over an arena of four texels per thread (blocks that stage their lightmap
+    lines in LDS): arena and frame equal the oracle's."""
     gpu_ctx = shadow_ctx
     rng = np.random.default_rng(40 + color_bits)
-    m, b = scenes.make_batch(23, color_bits=color_bits, view_sizes=((160, 120), (128, 96), (97, 61)), num_lights=9,
-                             surfaces_per_view=45, max_surface_size=150, num_lm=8000)
-    assert b.arena_texels > 262_144, b.arena_texels
+    m, b = scenes.make_batch(23, color_bits=color_bits, view_sizes=((160, 120), (128, 96), (97, 61), (120, 80)),
+                             num_lights=9, surfaces_per_view=60, max_surface_size=150, num_lm=8000)
+    assert b.arena_texels > 786_432, b.arena_texels
     sh, tex = _random_shadows(rng, len(b.lights))
     gpu_ctx.set_light_shadows(sh, tex)
     oracle.set_light_shadows(sh, tex)
