@@ -113,9 +113,12 @@ def test_graph_replay_matches_eager(gpu_ctx):
 
 
 @pytest.mark.parametrize("exposure", [0.25, 1.0, 3.0])
-def test_tonemap_bit_exact(gpu_ctx, exposure):
-    _say(f"case tonemap exposure={exposure}")
-    m, b = scenes.make_batch(9, color_bits=64, view_sizes=((256, 160),))
+@pytest.mark.parametrize("size", [(256, 160), (255, 161), (2560, 1440)])
+def test_tonemap_bit_exact(gpu_ctx, exposure, size):
+    """Tonemapped frames of an even, an odd and the HDR workload's 2560x1440 size equal the
+    oracle's bit for bit (alpha 255), colour sums within float tolerance."""
+    _say(f"case tonemap exposure={exposure} size={size}")
+    m, b = scenes.make_batch(9, color_bits=64, view_sizes=(size,))
     mh = gpu_ctx.upload_map(m)
     try:
         fb, out, sums = gpu_ctx.render_plain(mh, b, tonemap=(exposure, (1.0, 0.9, 1.1)))
