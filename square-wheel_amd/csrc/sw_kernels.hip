@@ -864,7 +864,8 @@ SW_DEV void build_surfaces_block(uint32_t block, const MapPtrs& m, const Surface
 						const uint32_t dv1 = div_rcp(uint32_t(hi - 1 - R.start), R.w, R.inv_w, rem);
 						lu0 = R.lm_su >> R.lm_sl;
 						ncols = ((R.w - 1u + R.lm_su) >> R.lm_sl) - lu0 + 2u;
-						// rows <= bt and columns <= 2^16 + 1: no overflow below
+						// rows <= bt / w + 2 and columns <= w + 2: at most bt + 2w + 2bt/w + 4 < 2^18
+						// elements per record, so the sums below stay far below 2^32
 						ne = (dv1 - dv0 + 1u) * ncols;
 						nf = ne * line_stride((R.flags & REC_DIRECTIONAL) != 0);
 					}
