@@ -98,8 +98,16 @@ SW_DEV CV cv_from_color64(uint64_t c)
 		float(uint32_t(c >> 48))};
 }
 // fast_math.rs:131-144
-SW_DEV uint32_t lane_to_u8(float x) { return uint32_t(f32_to_i32_rne_x86(fmin_r(x, 255.0f))) & 0xFFu; }
-SW_DEV uint32_t lane_to_u16(float x) { return uint32_t(f32_to_i32_rne_x86(fmin_r(x, 65535.0f))) & 0xFFFFu; }
+// After the min the value is at most 255 (65535) and never NaN (the min returns 255 for a NaN, as
+// minps does), so cvtps2dq's out-of-range result 0x80000000 can only come from values below -2^31:
+// clamping those to -2^31 first gives the same low bits from an in-range conversion, without
+// f32_to_i32_rne_x86's range test (equal for every float but signalling NaNs, which arithmetic
+// never produces; tests/test_lane_conversion.py).
+SW_DEV uint32_t lane_to_u8(float x) { return uint32_t(int32_t(__builtin_rintf(fmax_r(fmin_r(x, 255.0f), -2147483648.0f)))) & 0xFFu; }
+SW_DEV uint32_t lane_to_u16(float x)
+{
+	return uint32_t(int32_t(__builtin_rintf(fmax_r(fmin_r(x, 65535.0f), -2147483648.0f)))) & 0xFFFFu;
+}
 
 // ---------------------------------------------------------------------------------------------
 // ColorVecI lanes (i32) [B, G, R, A] and the two colour types
