@@ -2928,7 +2928,7 @@ __global__ __launch_bounds__(256) void k_bloom_tonemap(BloomArgs a)
 	{
 		const uint32_t x = x0 + uint32_t(j) * 256u;
 		if(x < a.w)
-			a.dst[uint64_t(y) * a.dst_pitch + x] = bloom_pixel<LINEAR>(a, x, y, cv_from_color64(v[j]));
+			__builtin_nontemporal_store(bloom_pixel<LINEAR>(a, x, y, cv_from_color64(v[j])), &a.dst[uint64_t(y) * a.dst_pitch + x]);
 	}
 }
 
@@ -2968,7 +2968,7 @@ __global__ __launch_bounds__(256) void k_tonemap(const uint64_t* __restrict__ hd
 				sg += c.g;
 				sr += c.r;
 				if(x < width)
-					out[x] = tonemap_cv(c, inv_scale);
+					__builtin_nontemporal_store(tonemap_cv(c, inv_scale), &out[x]); // the final image: not read again here
 			}
 		}
 	}
