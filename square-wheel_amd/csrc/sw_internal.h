@@ -60,7 +60,7 @@ enum
 // thread as keep it within about 1024 blocks, which the device runs at once (five blocks per CU
 // besides the draw blocks): a second round of blocks would add a whole block's latency.
 #define SURF_BLOCK 256
-#define SURF_MAX_TEXELS_PER_THREAD 4
+#define SURF_MAX_TEXELS_PER_THREAD 8
 __host__ __device__ inline uint32_t surf_block_texels(uint64_t arena_texels)
 {
 	const uint64_t per_thread = (arena_texels + uint64_t(SURF_BLOCK) * 1024u - 1) / (uint64_t(SURF_BLOCK) * 1024u);

@@ -3138,7 +3138,7 @@ void launch_build_surfaces(hipStream_t st, uint32_t color_bits, const MapPtrs& m
 	const uint64_t bt = surf_block_texels(sa.arena_texels);
 	const uint64_t blocks = (sa.arena_texels + bt - 1) / bt;
 	const bool sh = sa.num_shadows != 0;
-	// blocks of four texels per thread stage their lightmap lines in LDS: a line serves the texels
+	// blocks of four or more texels per thread stage their lightmap lines in LDS: a line serves the texels
 	// of a lightmap cell's width in its row, which then repays the staging's extra step and
 	// barrier (with two texels per thread, 1080p shadowed frames, it does not: 9.1 us against 8.9)
 	const bool lines = bt >= SURF_LINES_MIN_TEXELS;

@@ -363,7 +363,7 @@ def test_many_views_large_arena(gpu_ctx, device_inputs):
 @pytest.mark.parametrize("color_bits", [32, 64])
 @pytest.mark.parametrize("seed", [81, 82])
 def test_lightmap_lines_staged(gpu_ctx, color_bits, seed):
-    """Surface blocks of four texels per thread (arenas above 786,432 texels) stage each surface's
+    """Surface blocks of four or more texels per thread (arenas above 786,432 texels) stage each surface's
     lightmap lines (the vertical mix of two lightmap rows per surface row and column) in LDS;
     surfaces whose lines do not fit the block's share fetch their elements directly. Lightmap
     scales 1..16, simple and directional, surfaces up to 150 texels wide (many blocks hold both

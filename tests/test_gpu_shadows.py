@@ -81,7 +81,7 @@ def test_shadowed_plain_frame_bit_exact(shadow_ctx):
 
 @pytest.mark.parametrize("color_bits", [32, 64])
 def test_shadowed_large_arena_bit_exact(shadow_ctx, color_bits):
-    """Shadowed surfaces over an arena of four texels per thread (blocks that stage their lightmap
+    """Shadowed surfaces over an arena of four or more texels per thread (blocks that stage their lightmap
     lines in LDS): arena and frame equal the oracle's."""
     gpu_ctx = shadow_ctx
     rng = np.random.default_rng(40 + color_bits)
